@@ -1,0 +1,1353 @@
+/* CPU restatement of the reference solver's per-iteration path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product, never linked by
+ * it.  Pinned against the compiled reference (oracle/_ref) by
+ * tests/test_oracle_vs_ref.py and against closed forms by tests/test_oracle_laws.py.
+ *
+ * Storage mirrors the reference's extents (source/FluidProps.cpp:650-760,
+ * source/Geometry.cpp:109-155): cell arrays [0..IM][0..JM][0..KM], face arrays
+ * over the same box, Dummy slabs [1..ND] per face.  Arithmetic follows the
+ * reference's expression order so that, built with -ffp-contract=off against
+ * the same libm, results are bit-identical to the reference at one thread.
+ */
+#include "fv3d_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ND 2
+#define F32(x) ((double)(x##f))
+
+typedef struct { double r, ru, rv, rw, re; } cv_t;
+typedef struct { double d, u, v, w, p, t; } pv_t;
+typedef struct { double x, y, z; } v3_t;
+typedef struct { v3_t S; double s; v3_t c; double r, th; } face_t;
+typedef struct { double vol; v3_t c; double r, th; } cell_t;
+
+typedef struct {
+    int kind, face, is, ie, js, je, ks, ke, rotate;
+    double rot;
+} patch_t;
+
+typedef struct {
+    int IM, JM, KM, nblade;
+    double w_rot, cellvol;
+    v3_t *node;                  /* [1..IM][1..JM][1..KM] in a (IM+1)(JM+1)(KM+1) box */
+    face_t *fi, *fj, *fk;        /* face boxes (IM+1)(JM+1)(KM+1) */
+    cell_t *cell;
+    cv_t *cv, *cv_old, *dq, *res, *fc, *face_fc[3];
+    pv_t *pv;
+    double *lamc[3], *lamv[3], *dt, *mut;
+    cv_t *dcv[6];                /* IS IP JS JP KS KP Dummy slabs */
+    pv_t *dpv[6];
+    int flag_low_dt;
+    patch_t *patch;
+    int npatch;
+} blk_t;
+
+struct orc_ctx {
+    orc_config cfg;
+    orc_nondim nd;
+    double cfl, cfl_res;
+    double total_vol;
+    double res[6];
+    int iter;
+    int nb;
+    blk_t blk[64];
+};
+
+/* ------------------------------------------------------------------ index */
+static inline long C3(const blk_t *b, int i, int j, int k) { return ((long)i * (b->JM + 1) + j) * (b->KM + 1) + k; }
+static inline long CI(const blk_t *b, int i, int j, int k) { return (((long)(i - 1)) * (b->JM - 1) + (j - 1)) * (b->KM - 1) + (k - 1); }
+/* Dummy slab index: d 0..5 = IS IP JS JP KS KP; (a,b) tangential, n layer */
+static inline long DI(const blk_t *b, int d, int i, int j, int k)
+{
+    /* IS/IP: [nd][JM][KM], JS/JP: [IM][nd][KM], KS/KP: [IM][JM][nd] (nd 0..ND) */
+    if (d < 2) return ((long)i * b->JM + j) * b->KM + k;
+    if (d < 4) return ((long)i * (ND + 1) + j) * b->KM + k;
+    return ((long)i * b->JM + j) * (ND + 1) + k;
+}
+static long dsize(const blk_t *b, int d)
+{
+    if (d < 2) return (long)(ND + 1) * b->JM * b->KM;
+    if (d < 4) return (long)b->IM * (ND + 1) * b->KM;
+    return (long)b->IM * b->JM * (ND + 1);
+}
+
+/* ------------------------------------------------------------ small algebra */
+static inline v3_t vsub(v3_t a, v3_t b) { v3_t c = {a.x - b.x, a.y - b.y, a.z - b.z}; return c; }
+static inline v3_t vadd(v3_t a, v3_t b) { v3_t c = {a.x + b.x, a.y + b.y, a.z + b.z}; return c; }
+static inline v3_t vscale(v3_t a, double s) { v3_t c = {a.x * s, a.y * s, a.z * s}; return c; }
+static inline v3_t vdiv(v3_t a, double s) { v3_t c = {a.x / s, a.y / s, a.z / s}; return c; }
+static inline double vdot(v3_t a, v3_t b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+static inline cv_t cadd(cv_t a, cv_t b) { cv_t c = {a.r + b.r, a.ru + b.ru, a.rv + b.rv, a.rw + b.rw, a.re + b.re}; return c; }
+static inline cv_t csub(cv_t a, cv_t b) { cv_t c = {a.r - b.r, a.ru - b.ru, a.rv - b.rv, a.rw - b.rw, a.re - b.re}; return c; }
+static inline cv_t cmul(cv_t a, cv_t b) { cv_t c = {a.r * b.r, a.ru * b.ru, a.rv * b.rv, a.rw * b.rw, a.re * b.re}; return c; }
+static inline cv_t cdivv(cv_t a, cv_t b) { cv_t c = {a.r / b.r, a.ru / b.ru, a.rv / b.rv, a.rw / b.rw, a.re / b.re}; return c; }
+static inline cv_t cscale(cv_t a, double s) { cv_t c = {a.r * s, a.ru * s, a.rv * s, a.rw * s, a.re * s}; return c; }
+static inline cv_t cdiv(cv_t a, double s) { cv_t c = {a.r / s, a.ru / s, a.rv / s, a.rw / s, a.re / s}; return c; }
+static inline cv_t caddk(cv_t a, double s) { cv_t c = {a.r + s, a.ru + s, a.rv + s, a.rw + s, a.re + s}; return c; }
+static inline cv_t ksubc(double s, cv_t a) { cv_t c = {s - a.r, s - a.ru, s - a.rv, s - a.rw, s - a.re}; return c; }
+/* POW(x, 2.0): the reference's component-wise POW (source/defs.cpp:163-172,
+ * 295-305) receives the exponent at run time and calls libm pow, whose result
+ * differs from x*x in the last bit for some x.  The exponent is kept opaque here
+ * so the compiler cannot fold the call into x*x. */
+static volatile double pow_two = 2.0;
+static inline cv_t cvsq(cv_t a) { double e = pow_two; cv_t c = {pow(a.r, e), pow(a.ru, e), pow(a.rv, e), pow(a.rw, e), pow(a.re, e)}; return c; }
+
+static inline pv_t padd(pv_t a, pv_t b) { pv_t c = {a.d + b.d, a.u + b.u, a.v + b.v, a.w + b.w, a.p + b.p, a.t + b.t}; return c; }
+static inline pv_t psub(pv_t a, pv_t b) { pv_t c = {a.d - b.d, a.u - b.u, a.v - b.v, a.w - b.w, a.p - b.p, a.t - b.t}; return c; }
+static inline pv_t pmul(pv_t a, pv_t b) { pv_t c = {a.d * b.d, a.u * b.u, a.v * b.v, a.w * b.w, a.p * b.p, a.t * b.t}; return c; }
+static inline pv_t pdivv(pv_t a, pv_t b) { pv_t c = {a.d / b.d, a.u / b.u, a.v / b.v, a.w / b.w, a.p / b.p, a.t / b.t}; return c; }
+static inline pv_t kmulp(double s, pv_t a) { pv_t c = {s * a.d, s * a.u, s * a.v, s * a.w, s * a.p, s * a.t}; return c; }
+static inline pv_t pdiv(pv_t a, double s) { pv_t c = {a.d / s, a.u / s, a.v / s, a.w / s, a.p / s, a.t / s}; return c; }
+static inline pv_t paddk(pv_t a, double s) { pv_t c = {a.d + s, a.u + s, a.v + s, a.w + s, a.p + s, a.t + s}; return c; }
+static inline pv_t ksubp(double s, pv_t a) { pv_t c = {s - a.d, s - a.u, s - a.v, s - a.w, s - a.p, s - a.t}; return c; }
+static inline pv_t pvsq(pv_t a) { double e = pow_two; pv_t c = {pow(a.d, e), pow(a.u, e), pow(a.v, e), pow(a.w, e), pow(a.p, e), pow(a.t, e)}; return c; }
+
+static inline double MAXd(double a, double b) { return a > b ? a : b; }
+static inline double MINd(double a, double b) { return a < b ? a : b; }
+static inline double MIN3d(double a1, double a2, double a3) { double m = a1; if (m > a2) m = a2; if (m > a3) m = a3; return m; }
+
+/* source/Reconstruction.cpp:472-559 */
+static inline double mm1(double a, double b)
+{
+    if (a * b < 0.0) return 0.0;
+    if (fabs(a) <= fabs(b)) return a;
+    return b;
+}
+static cv_t cminmod(cv_t a, cv_t b) { cv_t c = {mm1(a.r, b.r), mm1(a.ru, b.ru), mm1(a.rv, b.rv), mm1(a.rw, b.rw), mm1(a.re, b.re)}; return c; }
+static pv_t pminmod(pv_t a, pv_t b) { pv_t c = {mm1(a.d, b.d), mm1(a.u, b.u), mm1(a.v, b.v), mm1(a.w, b.w), mm1(a.p, b.p), mm1(a.t, b.t)}; return c; }
+
+/* ------------------------------------------------------------- set-up */
+void orc_nondimensionalize(const orc_config *cfg, orc_nondim *nd)
+{
+    /* config values are parsed with stof (source/UserInput.cpp:60-76) */
+    double gamma = (float)cfg->gamma, rgas = (float)cfg->rgas;
+    double ttin = (float)cfg->tt_in, ptin = (float)cfg->pt_in, pout = (float)cfg->p_out;
+    double len_ref = (float)cfg->len_ref;
+    /* source/UserInput.cpp:1073-1084 */
+    nd->temp_ref = ttin;
+    nd->press_ref = ptin;
+    nd->vel_ref = sqrt(gamma * rgas * nd->temp_ref);
+    nd->mue_ref = F32(1.716e-5) * pow(nd->temp_ref / F32(288.15), F32(3.0) / F32(2.0)) * (double)(288.15f + 110.4f) /
+                  (nd->temp_ref + F32(110.4));
+    nd->dens_ref = nd->press_ref / rgas / nd->temp_ref;
+    nd->re_ref = nd->dens_ref * nd->vel_ref * len_ref / nd->mue_ref;
+    nd->p_out = pout / (nd->dens_ref * nd->vel_ref * nd->vel_ref);
+    nd->tt_in = F32(1.0);
+    nd->pt_in = F32(1.0) / gamma;
+    nd->rgas = F32(1.0) / (F32(1.0) * F32(1.0) * gamma);
+    nd->gamma = gamma;
+    nd->pr_lam = (float)cfg->pr_lam;
+    nd->pr_tur = (float)cfg->pr_tur;
+    nd->ev_ratio = (float)cfg->ev_ratio;
+    for (int c = 0; c < 3; ++c) nd->inlet_dir[c] = cfg->inlet_dir[c];
+}
+
+orc_ctx *orc_create(const orc_config *cfg)
+{
+    orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+    c->cfg = *cfg;
+    orc_nondimensionalize(cfg, &c->nd);
+    c->cfl = (float)cfg->cfl;
+    c->cfl_res = F32(1.0E-3);     /* source/Solver.cpp:21 */
+    return c;
+}
+
+static void blk_free(blk_t *b)
+{
+    free(b->node); free(b->fi); free(b->fj); free(b->fk); free(b->cell);
+    free(b->cv); free(b->cv_old); free(b->dq); free(b->res); free(b->fc); free(b->pv);
+    for (int d = 0; d < 3; ++d) { free(b->face_fc[d]); free(b->lamc[d]); free(b->lamv[d]); }
+    free(b->dt); free(b->mut);
+    for (int d = 0; d < 6; ++d) { free(b->dcv[d]); free(b->dpv[d]); }
+    free(b->patch);
+}
+
+void orc_destroy(orc_ctx *c)
+{
+    if (!c) return;
+    for (int n = 0; n < c->nb; ++n) blk_free(&c->blk[n]);
+    free(c);
+}
+
+int orc_block_add(orc_ctx *c, int IM, int JM, int KM, const double *nodes, int nblade, double rpm)
+{
+    if (c->nb >= 64) return -1;
+    blk_t *b = &c->blk[c->nb++];
+    memset(b, 0, sizeof *b);
+    b->IM = IM; b->JM = JM; b->KM = KM; b->nblade = nblade;
+    /* source/UserInput.cpp:444 and 1088 */
+    b->w_rot = (double)(2.0f * 3.1415926535897932f) * rpm / F32(60.0);
+    b->w_rot = b->w_rot / (c->nd.vel_ref / (double)(float)c->cfg.len_ref);
+    long box = (long)(IM + 1) * (JM + 1) * (KM + 1);
+    b->node = (v3_t *)calloc(box, sizeof(v3_t));
+    for (int i = 1; i <= IM; ++i)
+        for (int j = 1; j <= JM; ++j)
+            for (int k = 1; k <= KM; ++k) {
+                const double *p = nodes + (((long)(i - 1) * JM + (j - 1)) * KM + (k - 1)) * 3;
+                v3_t v = {p[0], p[1], p[2]};
+                b->node[C3(b, i, j, k)] = v;
+            }
+    b->fi = (face_t *)calloc(box, sizeof(face_t));
+    b->fj = (face_t *)calloc(box, sizeof(face_t));
+    b->fk = (face_t *)calloc(box, sizeof(face_t));
+    b->cell = (cell_t *)calloc(box, sizeof(cell_t));
+    b->cv = (cv_t *)calloc(box, sizeof(cv_t));
+    b->cv_old = (cv_t *)calloc(box, sizeof(cv_t));
+    b->dq = (cv_t *)calloc(box, sizeof(cv_t));
+    b->res = (cv_t *)calloc(box, sizeof(cv_t));
+    b->fc = (cv_t *)calloc(box, sizeof(cv_t));
+    b->pv = (pv_t *)calloc(box, sizeof(pv_t));
+    for (int d = 0; d < 3; ++d) {
+        b->face_fc[d] = (cv_t *)calloc(box, sizeof(cv_t));
+        b->lamc[d] = (double *)calloc(box, sizeof(double));
+        b->lamv[d] = (double *)calloc(box, sizeof(double));
+    }
+    b->dt = (double *)calloc(box, sizeof(double));
+    b->mut = (double *)calloc(box, sizeof(double));
+    for (int d = 0; d < 6; ++d) {
+        b->dcv[d] = (cv_t *)calloc(dsize(b, d), sizeof(cv_t));
+        b->dpv[d] = (pv_t *)calloc(dsize(b, d), sizeof(pv_t));
+    }
+    return c->nb - 1;
+}
+
+int orc_patch_add(orc_ctx *c, int kind, int block, int face, const int *rng, int rotate)
+{
+    if (block < 1 || block > c->nb) return -1;
+    blk_t *b = &c->blk[block - 1];
+    b->patch = (patch_t *)realloc(b->patch, sizeof(patch_t) * (b->npatch + 1));
+    patch_t *p = &b->patch[b->npatch++];
+    p->kind = kind; p->face = face; p->rotate = rotate;
+    p->is = rng[0]; p->ie = rng[1]; p->js = rng[2]; p->je = rng[3]; p->ks = rng[4]; p->ke = rng[5];
+    p->rot = 0.0;
+    return 0;
+}
+
+/* source/GeometryVariables.cpp:10-105 */
+static void face_vector_volume(blk_t *b)
+{
+    int IM = b->IM, JM = b->JM, KM = b->KM;
+    v3_t *N = b->node;
+#define ND3(i, j, k) N[C3(b, i, j, k)]
+    for (int i = 1; i <= IM; ++i)
+        for (int j = 1; j <= JM - 1; ++j)
+            for (int k = 1; k <= KM - 1; ++k) {
+                face_t *f = &b->fi[C3(b, i, j, k)];
+                v3_t A = vsub(ND3(i, j + 1, k), ND3(i, j, k + 1)), B = vsub(ND3(i, j + 1, k + 1), ND3(i, j, k));
+                f->S.x = F32(0.5) * (A.y * B.z - B.y * A.z);
+                f->S.y = F32(0.5) * (A.z * B.x - A.x * B.z);
+                f->S.z = F32(0.5) * (A.x * B.y - B.x * A.y);
+                f->c = vscale(vadd(vadd(vadd(ND3(i, j, k), ND3(i, j + 1, k)), ND3(i, j + 1, k + 1)), ND3(i, j, k + 1)), 0.25);
+                f->r = sqrt(f->c.x * f->c.x + f->c.y * f->c.y);
+                f->th = atan(f->c.y / f->c.x);
+                f->s = sqrt(f->S.x * f->S.x + f->S.y * f->S.y + f->S.z * f->S.z);
+            }
+    for (int i = 1; i <= IM - 1; ++i)
+        for (int j = 1; j <= JM; ++j)
+            for (int k = 1; k <= KM - 1; ++k) {
+                face_t *f = &b->fj[C3(b, i, j, k)];
+                v3_t A = vsub(ND3(i, j, k + 1), ND3(i + 1, j, k)), B = vsub(ND3(i + 1, j, k + 1), ND3(i, j, k));
+                f->S.x = F32(0.5) * (A.y * B.z - B.y * A.z);
+                f->S.y = F32(0.5) * (A.z * B.x - A.x * B.z);
+                f->S.z = F32(0.5) * (A.x * B.y - B.x * A.y);
+                f->c = vscale(vadd(vadd(vadd(ND3(i, j, k), ND3(i + 1, j, k)), ND3(i + 1, j, k + 1)), ND3(i, j, k + 1)), 0.25);
+                f->r = sqrt(f->c.x * f->c.x + f->c.y * f->c.y);
+                f->th = atan(f->c.y / f->c.x);
+                f->s = sqrt(f->S.x * f->S.x + f->S.y * f->S.y + f->S.z * f->S.z);
+            }
+    for (int i = 1; i <= IM - 1; ++i)
+        for (int j = 1; j <= JM - 1; ++j)
+            for (int k = 1; k <= KM; ++k) {
+                face_t *f = &b->fk[C3(b, i, j, k)];
+                v3_t A = vsub(ND3(i + 1, j + 1, k), ND3(i, j, k)), B = vsub(ND3(i, j + 1, k), ND3(i + 1, j, k));
+                f->S.x = F32(0.5) * (A.y * B.z - B.y * A.z);
+                f->S.y = F32(0.5) * (A.z * B.x - A.x * B.z);
+                f->S.z = F32(0.5) * (A.x * B.y - B.x * A.y);
+                f->c = vscale(vadd(vadd(vadd(ND3(i, j, k), ND3(i + 1, j, k)), ND3(i + 1, j + 1, k)), ND3(i, j + 1, k)), 0.25);
+                f->r = sqrt(f->c.x * f->c.x + f->c.y * f->c.y);
+                f->th = atan(f->c.y / f->c.x);
+                f->s = sqrt(f->S.x * f->S.x + f->S.y * f->S.y + f->S.z * f->S.z);
+            }
+    b->cellvol = 0.0;
+    for (int i = 1; i <= IM - 1; ++i)
+        for (int j = 1; j <= JM - 1; ++j)
+            for (int k = 1; k <= KM - 1; ++k) {
+                cell_t *cl = &b->cell[C3(b, i, j, k)];
+                const face_t *a0 = &b->fi[C3(b, i, j, k)], *a1 = &b->fi[C3(b, i + 1, j, k)];
+                const face_t *b0 = &b->fj[C3(b, i, j, k)], *b1 = &b->fj[C3(b, i, j + 1, k)];
+                const face_t *c0 = &b->fk[C3(b, i, j, k)], *c1 = &b->fk[C3(b, i, j, k + 1)];
+                double vol = vdot(a1->c, a1->S) - vdot(a0->c, a0->S) + vdot(b1->c, b1->S) - vdot(b0->c, b0->S) +
+                             vdot(c1->c, c1->S) - vdot(c0->c, c0->S);
+                cl->vol = vol / F32(3.0);
+                v3_t s = vadd(ND3(i, j, k), ND3(i + 1, j, k));
+                s = vadd(s, ND3(i, j + 1, k)); s = vadd(s, ND3(i + 1, j + 1, k));
+                s = vadd(s, ND3(i, j, k + 1)); s = vadd(s, ND3(i + 1, j, k + 1));
+                s = vadd(s, ND3(i, j + 1, k + 1)); s = vadd(s, ND3(i + 1, j + 1, k + 1));
+                cl->c = vscale(s, 0.125);
+                cl->r = sqrt(cl->c.x * cl->c.x + cl->c.y * cl->c.y);
+                cl->th = atan(cl->c.y / cl->c.x);
+                b->cellvol = b->cellvol + cl->vol;
+            }
+#undef ND3
+}
+
+int orc_geometry(orc_ctx *c)
+{
+    c->total_vol = 0.0;
+    for (int n = 0; n < c->nb; ++n) {
+        face_vector_volume(&c->blk[n]);
+        c->total_vol = c->total_vol + c->blk[n].cellvol;   /* source/ComputeAllBlockVars.cpp:25-36 */
+    }
+    return 0;
+}
+
+int orc_set_state(orc_ctx *c, int block, const double *pv, const double *ener)
+{
+    if (block < 1 || block > c->nb) return -1;
+    blk_t *b = &c->blk[block - 1];
+    int ni = b->IM - 1, nj = b->JM - 1, nk = b->KM - 1;
+    for (int i = 1; i <= ni; ++i)
+        for (int j = 1; j <= nj; ++j)
+            for (int k = 1; k <= nk; ++k) {
+                long s = ((long)(i - 1) * nj + (j - 1)) * nk + (k - 1);
+                const double *w = pv + s * 6;
+                long x = C3(b, i, j, k);
+                /* source/UserInput.cpp:1139-1150 */
+                b->cv[x].r = w[0];
+                b->cv[x].ru = w[0] * w[1];
+                b->cv[x].rv = w[0] * w[2];
+                b->cv[x].rw = w[0] * w[3];
+                b->cv[x].re = ener[s];
+                pv_t p = {w[0], w[1], w[2], w[3], w[4], w[5]};
+                b->pv[x] = p;
+            }
+    return 0;
+}
+
+/* ---------------------------------------------------------- boundary fill */
+static void set_dummies(blk_t *b, int d, int a, int bb, cv_t cv, pv_t pv)
+{
+    for (int nd = 1; nd <= ND; ++nd) {
+        long x;
+        if (d < 2) x = DI(b, d, nd, a, bb);
+        else if (d < 4) x = DI(b, d, a, nd, bb);
+        else x = DI(b, d, a, bb, nd);
+        b->dcv[d][x] = cv;
+        b->dpv[d][x] = pv;
+    }
+}
+
+/* source/BndCondInlet.cpp:7-127 */
+static void bc_inlet(orc_ctx *c, blk_t *b, const patch_t *P)
+{
+    const orc_nondim *nd = &c->nd;
+    double Gamma = nd->gamma, Rgas = nd->rgas, Ttin = nd->tt_in, Ptin = nd->pt_in;
+    const double *A = nd->inlet_dir;
+    int KM = b->KM;
+    double Vn_m = sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
+    if (P->ks != P->ke) return;
+    for (int i = P->is; i <= P->ie; ++i)
+        for (int j = P->js; j <= P->je; ++j) {
+            int k, kv;
+            double uvel, vvel, wvel, vdn, n_m;
+            v3_t S;
+            if (P->ks == 1) {
+                k = 1; kv = 0;
+                S = b->fk[C3(b, i, j, k)].S;
+                n_m = sqrt(S.x * S.x + S.y * S.y + S.z * S.z);
+                pv_t *w = &b->pv[C3(b, i, j, k)];
+                uvel = w->u; vvel = w->v; wvel = w->w;
+                vdn = F32(-1.0) * (uvel * S.x / n_m + vvel * S.y / n_m + wvel * S.z / n_m);
+            } else if (P->ks == KM) {
+                k = KM - 1; kv = KM;
+                S = b->fk[C3(b, i, j, kv)].S;
+                n_m = sqrt(S.x * S.x + S.y * S.y + S.z * S.z);
+                pv_t *w = &b->pv[C3(b, i, j, k)];
+                uvel = w->u; vvel = w->v; wvel = w->w;
+                vdn = (uvel * S.x / n_m + vvel * S.y / n_m + wvel * S.z / n_m);
+            } else {
+                continue;
+            }
+            double c_d = sqrt(Gamma * Rgas * b->pv[C3(b, i, j, k)].t);
+            double V2_d = uvel * uvel + vvel * vvel + wvel * wvel;
+            double costh = F32(-1.0) * vdn / sqrt(V2_d);
+            double c_02 = c_d * c_d + (Gamma - F32(1.0)) * V2_d / F32(2.0);
+            double R_minus = vdn - F32(2.0) * c_d / (Gamma - F32(1.0));
+            double meduA = (-R_minus * (Gamma - F32(1.0))) / ((Gamma - F32(1.0)) * costh * costh + F32(2.0));
+            double meduB = sqrt(c_02 * ((Gamma - F32(1.0)) * costh * costh + F32(2.0)) / (Gamma - F32(1.0)) / (R_minus * R_minus) -
+                                (Gamma - F32(1.0)) / F32(2.0));
+            double c_b = meduA * (F32(1.0) + costh * meduB);
+            double TempBC = Ttin * (c_b * c_b / c_02);
+            if (TempBC > Ttin) TempBC = F32(0.99) * Ttin;
+            double PressBC = Ptin * pow(TempBC / Ttin, Gamma / (Gamma - F32(1.0)));
+            double DensBC = PressBC / Rgas / TempBC;
+            double VBC = sqrt(F32(2.0) * Gamma / (Gamma - 1) * Rgas * (Ttin - TempBC));
+            long g = C3(b, i, j, kv);
+            pv_t *pg = &b->pv[g];
+            pg->u = VBC * A[0] / Vn_m;
+            pg->v = VBC * A[1] / Vn_m;
+            pg->w = VBC * A[2] / Vn_m;
+            uvel = pg->u; vvel = pg->v; wvel = pg->w;
+            V2_d = uvel * uvel + vvel * vvel + wvel * wvel;
+            double ener = PressBC / DensBC / (Gamma - F32(1.0)) + V2_d / F32(2.0);
+            pg->p = PressBC; pg->t = TempBC; pg->d = DensBC;
+            cv_t *cg = &b->cv[g];
+            cg->r = DensBC; cg->ru = DensBC * uvel; cg->rv = DensBC * vvel; cg->rw = DensBC * wvel; cg->re = DensBC * ener;
+            set_dummies(b, P->ks == 1 ? 4 : 5, i, j, *cg, *pg);
+        }
+}
+
+/* source/BndCondOutlet.cpp:116-305 (centrifugal = general without the supersonic branch) */
+static void bc_outlet_general(orc_ctx *c, blk_t *b, const patch_t *P, int centrifugal)
+{
+    const orc_nondim *nd = &c->nd;
+    double Gamma = nd->gamma, Rgas = nd->rgas, Pout = nd->p_out;
+    int KM = b->KM;
+    if (P->ks != P->ke) return;
+    for (int i = P->is; i <= P->ie; ++i)
+        for (int j = P->js; j <= P->je; ++j) {
+            int k, kv;
+            v3_t norm;
+            if (P->ks == 1) { k = 1; kv = 0; const face_t *f = &b->fk[C3(b, i, j, k)]; norm = vdiv(f->S, f->s); }
+            else if (P->ks == KM) { k = KM - 1; kv = KM; const face_t *f = &b->fk[C3(b, i, j, kv)]; norm = vdiv(f->S, f->s); }
+            else continue;
+            pv_t *pi = &b->pv[C3(b, i, j, k)];
+            pv_t *pg = &b->pv[C3(b, i, j, kv)];
+            double dens0 = pi->d;
+            double c0 = sqrt(Gamma * Rgas * pi->t);
+            double Vn = pi->u * norm.x + pi->v * norm.y + pi->w * norm.z;
+            double Ma_n = Vn / c0;
+            if (centrifugal || Ma_n <= 1.0) {
+                pg->p = Pout;
+                pg->d = pi->d + (Pout - pi->p) / c0 / c0;
+                pg->u = pi->u + norm.x * (pi->p - Pout) / dens0 / c0;
+                pg->v = pi->v + norm.y * (pi->p - Pout) / dens0 / c0;
+                pg->w = pi->w + norm.z * (pi->p - Pout) / dens0 / c0;
+            } else {
+                pg->p = pi->p; pg->d = pi->d; pg->u = pi->u; pg->v = pi->v; pg->w = pi->w;
+            }
+            pg->t = pg->p / pg->d / Rgas;
+            double V2_d = pg->u * pg->u + pg->v * pg->v + pg->w * pg->w;
+            double ener = pg->p / pg->d / (Gamma - F32(1.0)) + V2_d / F32(2.0);
+            cv_t *cg = &b->cv[C3(b, i, j, kv)];
+            cg->r = pg->d; cg->ru = pg->d * pg->u; cg->rv = pg->d * pg->v; cg->rw = pg->d * pg->w; cg->re = pg->d * ener;
+            set_dummies(b, P->ks == 1 ? 4 : 5, i, j, *cg, *pg);
+        }
+}
+
+/* source/BndCondOutlet.cpp:13-113 (radial equilibrium, K-max faces only) */
+static void bc_outlet_axial(orc_ctx *c, blk_t *b, const patch_t *P)
+{
+    const orc_nondim *nd = &c->nd;
+    double Gamma = nd->gamma, Rgas = nd->rgas, Pout = nd->p_out;
+    int IM = b->IM, JM = b->JM, KM = b->KM;
+    if (P->ks != P->ke || P->ke != KM) return;
+    int k = KM - 1;
+    double cu[1024];
+    if (IM > 1000) return;
+    for (int i = 1; i <= IM - 1; ++i) {
+        cu[i] = 0.0;
+        double S_temp = 0.0;
+        for (int j = 1; j <= JM - 1; ++j) {
+            double uvel = b->pv[C3(b, i, j, k)].u, vvel = b->pv[C3(b, i, j, k)].v;
+            const face_t *f = &b->fk[C3(b, i, j, KM)];
+            double cu_temp = F32(-1.0) * uvel * sin(f->th) + vvel * cos(f->th);
+            cu[i] = cu[i] + cu_temp * f->s;
+            S_temp = S_temp + f->s;
+        }
+        cu[i] = cu[i] / S_temp;
+    }
+    int iref = (int)(0.5 * IM);
+    for (int j = 1; j <= JM - 1; ++j) {
+        b->pv[C3(b, iref, j, KM)].p = Pout;
+        for (int i = iref - 1; i >= 1; --i) {
+            double rn = b->fk[C3(b, i, j, KM)].r, rn_1 = b->fk[C3(b, i + 1, j, KM)].r;
+            double dp_dr = b->cv[C3(b, i, j, KM)].r * cu[i] * cu[i] / rn;
+            b->pv[C3(b, i, j, KM)].p = b->pv[C3(b, i + 1, j, KM)].p - dp_dr * (rn_1 - rn);
+        }
+        for (int i = iref + 1; i <= IM - 1; ++i) {
+            double rn_1 = b->fk[C3(b, i, j, k)].r, rn = b->fk[C3(b, i - 1, j, k)].r;
+            double dp_dr = b->cv[C3(b, i, j, KM)].r * cu[i] * cu[i] / rn;
+            b->pv[C3(b, i, j, KM)].p = b->pv[C3(b, i - 1, j, KM)].p + dp_dr * (rn_1 - rn);
+        }
+    }
+    for (int i = 1; i <= IM - 1; ++i)
+        for (int j = 1; j <= JM - 1; ++j) {
+            cv_t *cg = &b->cv[C3(b, i, j, KM)];
+            const cv_t *ci = &b->cv[C3(b, i, j, KM - 1)];
+            pv_t *pg = &b->pv[C3(b, i, j, KM)];
+            const pv_t *pi = &b->pv[C3(b, i, j, KM - 1)];
+            cg->r = ci->r; cg->ru = ci->ru; cg->rv = ci->rv; cg->rw = ci->rw;
+            pg->d = ci->r; pg->u = pi->u; pg->v = pi->v; pg->w = pi->w;
+            pg->t = pg->p / Rgas / cg->r;
+            double V2 = pg->u * pg->u + pg->v * pg->v + pg->w * pg->w;
+            cg->re = pg->p / (Gamma - F32(1.0)) + pg->d * V2 / F32(2.0);
+            set_dummies(b, 5, i, j, *cg, *pg);
+        }
+}
+
+/* source/BndCondWall.cpp:20-284 */
+static void bc_wall(blk_t *b, const patch_t *P)
+{
+    int IM = b->IM, JM = b->JM, KM = b->KM;
+    double rot = P->rot;
+    int dir, lo;
+    if (P->is == P->ie) { dir = 0; lo = (P->is == 1); if (!lo && P->is != IM) return; }
+    else if (P->js == P->je) { dir = 1; lo = (P->js == 1); if (!lo && P->js != JM) return; }
+    else if (P->ks == P->ke) { dir = 2; lo = (P->ks == 1); if (!lo && P->ks != KM) return; }
+    else return;
+    int aS = dir == 0 ? P->js : P->is, aE = dir == 0 ? P->je : P->ie;
+    int bS = dir == 2 ? P->js : P->ks, bE = dir == 2 ? P->je : P->ke;
+    for (int a = aS; a <= aE; ++a)
+        for (int bb = bS; bb <= bE; ++bb) {
+            int i, j, k, iv, jv, kv;
+            double x, y;
+            const face_t *f;
+            if (dir == 0) { j = a; k = bb; i = lo ? 1 : IM - 1; iv = lo ? 0 : IM; jv = j; kv = k; f = &b->fi[C3(b, lo ? 1 : IM, j, k)]; }
+            else if (dir == 1) { i = a; k = bb; j = lo ? 1 : JM - 1; jv = lo ? 0 : JM; iv = i; kv = k; f = &b->fj[C3(b, i, lo ? 1 : JM, k)]; }
+            else { i = a; j = bb; k = lo ? 1 : KM - 1; kv = lo ? 0 : KM; iv = i; jv = j; f = &b->fk[C3(b, i, j, lo ? 1 : KM)]; }
+            x = f->c.x; y = f->c.y;
+            pv_t *pi = &b->pv[C3(b, i, j, k)];
+            pv_t *pg = &b->pv[C3(b, iv, jv, kv)];
+            cv_t *ci = &b->cv[C3(b, i, j, k)];
+            cv_t *cg = &b->cv[C3(b, iv, jv, kv)];
+            pg->p = pi->p; pg->t = pi->t; pg->d = pi->d;
+            double Vwall_x = -rot * y, Vwall_y = rot * x;
+            v3_t Vin = {pi->u - Vwall_x, pi->v - Vwall_y, pi->w - F32(0.0)};
+            v3_t Vout = {F32(0.0) - Vin.x, F32(0.0) - Vin.y, F32(0.0) - Vin.z};
+            pg->u = Vout.x + Vwall_x;
+            pg->v = Vout.y + Vwall_y;
+            pg->w = Vout.z + F32(0.0);
+            cg->r = ci->r;
+            cg->ru = pg->u * cg->r;
+            cg->rv = pg->v * cg->r;
+            cg->rw = pg->w * cg->r;
+            cg->re = ci->re;
+            int d = 2 * dir + (lo ? 0 : 1);
+            pv_t pd = kmulp(0.5, padd(*pg, *pi));
+            cv_t cd = cscale(cadd(*cg, *ci), 0.5);
+            set_dummies(b, d, a, bb, cd, pd);
+        }
+}
+
+/* source/BoundaryCondition.cpp:9-63 */
+int orc_bc(orc_ctx *c)
+{
+    for (int pass = 0; pass < 3; ++pass)
+        for (int n = 0; n < c->nb; ++n) {
+            blk_t *b = &c->blk[n];
+            for (int q = 0; q < b->npatch; ++q) {
+                patch_t *P = &b->patch[q];
+                if (P->kind != pass) continue;
+                if (pass == 0) bc_inlet(c, b, P);
+                else if (pass == 1) {
+                    if (c->cfg.turbo_type == 0) bc_outlet_axial(c, b, P);
+                    else bc_outlet_general(c, b, P, c->cfg.turbo_type == 1);
+                } else {
+                    P->rot = P->rotate ? b->w_rot : 0.0;
+                    bc_wall(b, P);
+                }
+            }
+        }
+    return 0;
+}
+
+/* ------------------------------------------------------- spectral radii */
+/* source/Spectral_curl_grad.cpp:9-42 */
+int orc_spectral(orc_ctx *c)
+{
+    double Gamma = c->nd.gamma, Rgas = c->nd.rgas;
+    for (int n = 0; n < c->nb; ++n) {
+        blk_t *b = &c->blk[n];
+        double w_rot = b->w_rot;
+        for (int i = 1; i <= b->IM - 1; ++i)
+            for (int j = 1; j <= b->JM - 1; ++j)
+                for (int k = 1; k <= b->KM - 1; ++k) {
+                    long x = C3(b, i, j, k);
+                    const pv_t *w = &b->pv[x];
+                    const cell_t *cl = &b->cell[x];
+                    double uvelr = w->u + w_rot * cl->c.y;
+                    double vvelr = w->v - w_rot * cl->c.x;
+                    double wvelr = w->w;
+                    double csoun = sqrt(Gamma * Rgas * w->t);
+                    const face_t *fa[3][2] = {{&b->fi[x], &b->fi[C3(b, i + 1, j, k)]},
+                                              {&b->fj[x], &b->fj[C3(b, i, j + 1, k)]},
+                                              {&b->fk[x], &b->fk[C3(b, i, j, k + 1)]}};
+                    for (int d = 0; d < 3; ++d) {
+                        v3_t S = vscale(vadd(fa[d][0]->S, fa[d][1]->S), F32(0.5));
+                        double CS = csoun * sqrt(S.x * S.x + S.y * S.y + S.z * S.z);
+                        b->lamc[d][x] = fabs(uvelr * S.x + vvelr * S.y + wvelr * S.z) + CS;
+                    }
+                }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------ time step */
+/* source/TimeStep.cpp:7-92 */
+int orc_timestep(orc_ctx *c)
+{
+    for (int n = 0; n < c->nb; ++n) {
+        blk_t *b = &c->blk[n];
+        double dt_factor = b->flag_low_dt == 0 ? F32(1.0) : (double)(1.0f / (1.0f + b->flag_low_dt));
+        for (int i = 1; i <= b->IM - 1; ++i)
+            for (int j = 1; j <= b->JM - 1; ++j)
+                for (int k = 1; k <= b->KM - 1; ++k) {
+                    long x = C3(b, i, j, k);
+                    double lc = b->lamc[0][x] + b->lamc[1][x] + b->lamc[2][x];
+                    double lv = 0.0;
+                    b->dt[x] = dt_factor * c->cfl * b->cell[x].vol / (lc + (double)(float)c->cfg.c_timestep * lv);
+                }
+        if (c->cfg.global_dt) {
+            double m = 1E10;
+            for (int i = 1; i <= b->IM - 1; ++i)
+                for (int j = 1; j <= b->JM - 1; ++j)
+                    for (int k = 1; k <= b->KM - 1; ++k) m = MINd(m, b->dt[C3(b, i, j, k)]);
+            for (int i = 1; i <= b->IM - 1; ++i)
+                for (int j = 1; j <= b->JM - 1; ++j)
+                    for (int k = 1; k <= b->KM - 1; ++k) b->dt[C3(b, i, j, k)] = m;
+        }
+        if (b->flag_low_dt > 0) b->flag_low_dt = b->flag_low_dt - 1;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------- reconstruction */
+/* source/Reconstruction.cpp:70-265 */
+static void recon_prim(int scheme, double Gamma, pv_t FF, pv_t F, pv_t B, pv_t BB, cv_t *cvL, cv_t *cvR)
+{
+    const double eps = F32(1E-6), eta = F32(0.2), k2c = 1.0, k2u = -1.0, lim_zero = F32(1E-20);
+    pv_t L = {0}, R = {0};
+    if (scheme == 100) {
+        L = padd(F, kmulp(0.5, pminmod(psub(B, F), psub(F, FF))));
+        R = psub(B, kmulp(0.5, pminmod(psub(B, F), psub(BB, B))));
+    } else if (scheme == 200) {
+        L = padd(F, kmulp(F32(0.25), padd(kmulp(F32(1.0) - k2u, pminmod(kmulp(F32(2.0), psub(B, F)), psub(F, FF))),
+                                         kmulp(F32(1.0) + k2u, pminmod(kmulp(F32(2.0), psub(F, FF)), psub(B, F))))));
+        R = psub(B, kmulp(F32(0.25), padd(kmulp(F32(1.0) - k2u, pminmod(kmulp(F32(2.0), psub(B, F)), psub(BB, B))),
+                                         kmulp(F32(1.0) + k2u, pminmod(kmulp(F32(2.0), psub(BB, B)), psub(B, F))))));
+    } else if (scheme == 300) {
+        L = padd(F, kmulp(F32(0.25), padd(kmulp(F32(1.0) - k2c, pminmod(kmulp(F32(2.0), psub(B, F)), psub(F, FF))),
+                                         kmulp(F32(1.0) + k2c, pminmod(psub(B, F), kmulp(2.0, psub(F, FF)))))));
+        R = psub(B, kmulp(F32(0.25), padd(kmulp(F32(1.0) - k2c, pminmod(kmulp(F32(2.0), psub(B, F)), psub(BB, B))),
+                                         kmulp(F32(1.0) + k2c, pminmod(psub(B, F), kmulp(2.0, psub(BB, B)))))));
+    } else if (scheme == 400) {
+        pv_t r1 = pdivv(paddk(psub(F, FF), eps), paddk(psub(B, F), eps));
+        pv_t r2 = pdivv(paddk(psub(B, F), eps), paddk(psub(BB, B), eps));
+        /* f1 = 1.0f-eta + (eta-0.55f)/2.0f/r2 + (eta+0.55f)/2.0f*r1 */
+        double a0 = F32(1.0) - eta, a1 = (eta - F32(0.55)) / F32(2.0), a2 = (eta + F32(0.55)) / F32(2.0);
+        pv_t f1 = padd(paddk(kmulp(1.0, pdivv((pv_t){a1, a1, a1, a1, a1, a1}, r2)), a0), kmulp(a2, r1));
+        pv_t f;
+        f.d = MAXd(F32(0.0), MIN3d(F32(2.0), f1.d, F32(2.0) * r1.d));
+        f.u = MAXd(F32(0.0), MIN3d(F32(2.0), f1.u, F32(2.0) * r1.u));
+        f.v = MAXd(F32(0.0), MIN3d(F32(2.0), f1.v, F32(2.0) * r1.v));
+        f.w = MAXd(F32(0.0), MIN3d(F32(2.0), f1.w, F32(2.0) * r1.w));
+        f.p = MAXd(F32(0.0), MIN3d(F32(2.0), f1.p, F32(2.0) * r1.p));
+        f.t = 0.0;
+        L = padd(F, pmul(kmulp(F32(0.5), f), psub(B, F)));
+        r1 = pdivv(paddk(psub(BB, B), eps), paddk(psub(B, F), eps));
+        r2 = pdivv(paddk(psub(B, F), eps), paddk(psub(F, FF), eps));
+        f1 = padd(paddk(kmulp(1.0, pdivv((pv_t){a1, a1, a1, a1, a1, a1}, r2)), a0), kmulp(a2, r1));
+        f.d = MAXd(F32(0.0), MIN3d(F32(2.0), f1.d, F32(2.0) * r1.d));
+        f.u = MAXd(F32(0.0), MIN3d(F32(2.0), f1.u, F32(2.0) * r1.u));
+        f.v = MAXd(F32(0.0), MIN3d(F32(2.0), f1.v, F32(2.0) * r1.v));
+        f.w = MAXd(F32(0.0), MIN3d(F32(2.0), f1.w, F32(2.0) * r1.w));
+        f.p = MAXd(F32(0.0), MIN3d(F32(2.0), f1.p, F32(2.0) * r1.p));
+        R = psub(B, pmul(kmulp(F32(0.5), f), psub(B, F)));
+    } else if (scheme == 500) {
+        pv_t dF = psub(F, FF), dB = psub(B, F), dR = psub(BB, B);
+        pv_t SL = pdivv(paddk(pmul(kmulp(2.0, dF), dB), eps), paddk(padd(pvsq(dF), pvsq(dB)), eps));
+        pv_t SR = pdivv(paddk(pmul(kmulp(2.0, dR), dB), eps), paddk(padd(pvsq(dR), pvsq(dB)), eps));
+        L = padd(F, pmul(pdiv(SL, 4.0), padd(pmul(ksubp(1.0, pdiv(SL, 3.0)), dF), pmul(paddk(pdiv(SL, 3.0), 1.0), dB))));
+        R = psub(B, pmul(pdiv(SR, 4.0), padd(pmul(ksubp(1.0, pdiv(SR, 3.0)), dR), pmul(paddk(pdiv(SR, 3.0), 1.0), dB))));
+    }
+    if (L.d <= lim_zero || L.p <= lim_zero) L = F;
+    if (R.d <= lim_zero || R.p <= lim_zero) R = B;
+    cvL->r = L.d; cvL->ru = L.d * L.u; cvL->rv = L.d * L.v; cvL->rw = L.d * L.w;
+    cvL->re = L.p / (Gamma - F32(1.0)) + L.d * (L.u * L.u + L.v * L.v + L.w * L.w) / F32(2.0);
+    cvR->r = R.d; cvR->ru = R.d * R.u; cvR->rv = R.d * R.v; cvR->rw = R.d * R.w;
+    cvR->re = R.p / (Gamma - F32(1.0)) + R.d * (R.u * R.u + R.v * R.v + R.w * R.w) / F32(2.0);
+}
+
+/* source/Reconstruction.cpp:7-68 */
+static void recon_cons(int scheme, double Gamma, cv_t FF, cv_t F, cv_t B, cv_t BB, cv_t *cvL, cv_t *cvR)
+{
+    const double eps = F32(1E-6), eta = F32(0.05), k2c = F32(1.0), k2u = F32(-1.0), lim_zero = F32(1E-20);
+    cv_t L = {0}, R = {0};
+    if (scheme == 100) {
+        L = cadd(F, cscale(cminmod(csub(B, F), csub(F, FF)), F32(0.5)));
+        R = csub(B, cscale(cminmod(csub(B, F), csub(BB, B)), F32(0.5)));
+    } else if (scheme == 200) {
+        L = cadd(F, cscale(cadd(cscale(cminmod(cscale(csub(B, F), F32(2.0)), csub(F, FF)), F32(1.0) - k2u),
+                                cscale(cminmod(cscale(csub(F, FF), F32(2.0)), csub(B, F)), F32(1.0) + k2u)), 0.25));
+        R = csub(B, cscale(cadd(cscale(cminmod(cscale(csub(B, F), F32(2.0)), csub(BB, B)), F32(1.0) - k2u),
+                                cscale(cminmod(cscale(csub(BB, B), F32(2.0)), csub(B, F)), F32(1.0) + k2u)), 0.25));
+    } else if (scheme == 300) {
+        L = cadd(F, cscale(cadd(cscale(cminmod(cscale(csub(B, F), F32(2.0)), csub(F, FF)), F32(1.0) - k2c),
+                                cscale(cminmod(csub(B, F), cscale(csub(F, FF), F32(2.0))), F32(1.0) + k2c)), 0.25));
+        R = csub(B, cscale(cadd(cscale(cminmod(cscale(csub(B, F), F32(2.0)), csub(BB, B)), F32(1.0) - k2c),
+                                cscale(cminmod(csub(B, F), cscale(csub(BB, B), F32(2.0))), F32(1.0) + k2c)), 0.25));
+    } else if (scheme == 400) {
+        double a0 = F32(1.0) - eta, a1 = (eta - F32(0.55)) / F32(2.0), a2 = (eta + F32(0.55)) / F32(2.0);
+        cv_t r1 = cdivv(caddk(csub(F, FF), eps), caddk(csub(B, F), eps));
+        cv_t r2 = cdivv(caddk(csub(B, F), eps), caddk(csub(BB, B), eps));
+        cv_t f1 = cadd(caddk(cdivv((cv_t){a1, a1, a1, a1, a1}, r2), a0), cscale(r1, a2));
+        cv_t f;
+        f.r = MAXd(F32(0.0), MIN3d(F32(2.0), f1.r, F32(2.0) * r1.r));
+        f.ru = MAXd(F32(0.0), MIN3d(F32(2.0), f1.ru, F32(2.0) * r1.ru));
+        f.rv = MAXd(F32(0.0), MIN3d(F32(2.0), f1.rv, F32(2.0) * r1.rv));
+        f.rw = MAXd(F32(0.0), MIN3d(F32(2.0), f1.rw, F32(2.0) * r1.rw));
+        f.re = MAXd(F32(0.0), MIN3d(F32(2.0), f1.re, F32(2.0) * r1.re));
+        L = cadd(F, cmul(cscale(f, F32(0.5)), csub(B, F)));
+        r1 = cdivv(caddk(csub(BB, B), eps), caddk(csub(B, F), eps));
+        r2 = cdivv(caddk(csub(B, F), eps), caddk(csub(F, FF), eps));
+        f1 = cadd(caddk(cdivv((cv_t){a1, a1, a1, a1, a1}, r2), a0), cscale(r1, a2));
+        f.r = MAXd(F32(0.0), MIN3d(F32(2.0), f1.r, F32(2.0) * r1.r));
+        f.ru = MAXd(F32(0.0), MIN3d(F32(2.0), f1.ru, F32(2.0) * r1.ru));
+        f.rv = MAXd(F32(0.0), MIN3d(F32(2.0), f1.rv, F32(2.0) * r1.rv));
+        f.rw = MAXd(F32(0.0), MIN3d(F32(2.0), f1.rw, F32(2.0) * r1.rw));
+        f.re = MAXd(F32(0.0), MIN3d(F32(2.0), f1.re, F32(2.0) * r1.re));
+        R = csub(B, cmul(cscale(f, F32(0.5)), csub(B, F)));
+    } else if (scheme == 500) {
+        cv_t dF = csub(F, FF), dB = csub(B, F), dR = csub(BB, B);
+        cv_t SL = cdivv(caddk(cmul(cscale(dF, F32(2.0)), dB), eps), caddk(cadd(cvsq(dF), cvsq(dB)), eps));
+        cv_t SR = cdivv(caddk(cmul(cscale(dR, F32(2.0)), dB), eps), caddk(cadd(cvsq(dR), cvsq(dB)), eps));
+        L = cadd(F, cmul(cdiv(SL, F32(4.0)), cadd(cmul(ksubc(F32(1.0), cdiv(SL, F32(3.0))), dF), cmul(caddk(cdiv(SL, F32(3.0)), F32(1.0)), dB))));
+        R = csub(B, cmul(cdiv(SR, F32(4.0)), cadd(cmul(ksubc(F32(1.0), cdiv(SR, F32(3.0))), dR), cmul(caddk(cdiv(SR, F32(3.0)), F32(1.0)), dB))));
+    }
+    double pL = (Gamma - F32(1.0)) * (L.re - F32(0.5) * (L.ru * L.ru + L.rv * L.rv + L.rw * L.rw) / L.r);
+    double pR = (Gamma - F32(1.0)) * (R.re - F32(0.5) * (R.ru * R.ru + R.rv * R.rv + R.rw * R.rw) / R.r);
+    if (L.r <= lim_zero || pL <= lim_zero) L = F;
+    if (R.r <= lim_zero || pR <= lim_zero) R = B;
+    *cvL = L; *cvR = R;
+}
+
+/* --------------------------------------------------------- face fluxes */
+typedef struct { v3_t n, t1, t2; double que; } frame_t;
+
+/* source/Flux_Split_Scheme.cpp:728-776 */
+static frame_t coord_rotate(const face_t *f, v3_t diag, double w_rot)
+{
+    frame_t F;
+    F.n = vdiv(f->S, f->s);
+    double fue = F32(-1.0) * f->c.y * w_rot, fve = f->c.x * w_rot;
+    F.t1 = vdiv(diag, sqrt(diag.x * diag.x + diag.y * diag.y + diag.z * diag.z));
+    v3_t r = {F.n.y * F.t1.z - F.n.z * F.t1.y, F.n.z * F.t1.x - F.n.x * F.t1.z, F.n.x * F.t1.y - F.n.y * F.t1.x};
+    F.t2 = vdiv(r, sqrt(r.x * r.x + r.y * r.y + r.z * r.z));
+    F.que = fue * F.n.x + fve * F.n.y + F32(0.0) * F.n.z;
+    return F;
+}
+
+static cv_t rotate_in(cv_t q, const frame_t *F)
+{
+    cv_t Q;
+    Q.r = q.r;
+    Q.ru = q.ru * F->n.x + q.rv * F->n.y + q.rw * F->n.z;
+    Q.rv = q.ru * F->t1.x + q.rv * F->t1.y + q.rw * F->t1.z;
+    Q.rw = q.ru * F->t2.x + q.rv * F->t2.y + q.rw * F->t2.z;
+    Q.re = q.re;
+    return Q;
+}
+
+static inline double entropy_corr(double z, double d) { return z > d ? z : F32(0.5) * (z * z + d * d) / d; }
+
+/* source/Flux_Split_Scheme.cpp:568-635 */
+static cv_t roe(double Gamma, cv_t QL, cv_t QR, double que)
+{
+    double densL = QL.r, uvelL = QL.ru / QL.r, vvelL = QL.rv / QL.r, wvelL = QL.rw / QL.r, EL = QL.re / QL.r;
+    double pL = (Gamma - F32(1.0)) * densL * (EL - F32(0.5) * (uvelL * uvelL + vvelL * vvelL + wvelL * wvelL));
+    double HL = EL + pL / densL;
+    double densR = QR.r, uvelR = QR.ru / QR.r, vvelR = QR.rv / QR.r, wvelR = QR.rw / QR.r, ER = QR.re / QR.r;
+    double pR = (Gamma - F32(1.0)) * densR * (ER - F32(0.5) * (uvelR * uvelR + vvelR * vvelR + wvelR * wvelR));
+    double HR = ER + pR / densR;
+    cv_t FL, FR;
+    FL.r = QL.r * (uvelL - que); FL.ru = QL.ru * (uvelL - que) + pL; FL.rv = QL.rv * (uvelL - que);
+    FL.rw = QL.rw * (uvelL - que); FL.re = (QL.re + pL) * (uvelL - que) + que * pL;
+    FR.r = QR.r * (uvelR - que); FR.ru = QR.ru * (uvelR - que) + pR; FR.rv = QR.rv * (uvelR - que);
+    FR.rw = QR.rw * (uvelR - que); FR.re = (QR.re + pR) * (uvelR - que) + que * pR;
+    double tmpp = sqrt(densR / densL);
+    double uav = (uvelL + tmpp * uvelR) / (F32(1.0) + tmpp);
+    double vav = (vvelL + tmpp * vvelR) / (F32(1.0) + tmpp);
+    double wav = (wvelL + tmpp * wvelR) / (F32(1.0) + tmpp);
+    double Hav = (HL + tmpp * HR) / (F32(1.0) + tmpp);
+    double cav = sqrt((Gamma - F32(1.0)) * (Hav - F32(0.5) * (uav * uav + vav * vav + wav * wav)));
+    double l1 = fabs((uav - que) - cav), l2 = fabs((uav - que)), l5 = fabs((uav - que) + cav);
+    double delta = F32(0.5);
+    l1 = entropy_corr(l1, delta); l2 = entropy_corr(l2, delta); l5 = entropy_corr(l5, delta);
+    double a2 = (QR.rv - QL.rv) / (vav + F32(1E-8)) - (densR - densL);
+    double a3 = (QR.rw - QL.rw) / (wav + F32(1E-8)) - (densR - densL);
+    double a4 = (Gamma - F32(1.0)) * ((densR - densL) * (Hav - uav * uav - vav * vav - wav * wav) + uav * (QR.ru - QL.ru) +
+                                      vav * (QR.rv - QL.rv) + wav * (QR.rw - QL.rw) - (QR.re - QL.re)) / (cav * cav);
+    double a1 = ((uav + cav) * (densR - densL) - (QR.ru - QL.ru) - cav * a4) / (F32(2.0) * cav);
+    double a5 = densR - densL - (a1 + a4);
+    cv_t Fc;
+    Fc.r = F32(0.5) * (FL.r + FR.r) - F32(0.5) * (l1 * a1 + l2 * a4 + l5 * a5);
+    Fc.ru = F32(0.5) * (FL.ru + FR.ru) - F32(0.5) * (l1 * a1 * (uav - cav) + l2 * a4 * uav + l5 * a5 * (uav + cav));
+    Fc.rv = F32(0.5) * (FL.rv + FR.rv) - F32(0.5) * (l1 * a1 * vav + l2 * a2 * vav + l2 * a4 * vav + l5 * a5 * vav);
+    Fc.rw = F32(0.5) * (FL.rw + FR.rw) - F32(0.5) * (l1 * a1 * wav + l2 * a3 * wav + l2 * a4 * wav + l5 * a5 * wav);
+    Fc.re = F32(0.5) * (FL.re + FR.re) - F32(0.5) * (l1 * a1 * (Hav - uav * cav) + l2 * a2 * vav * vav + l2 * a3 * wav * wav +
+                                                     l2 * a4 * F32(0.5) * (uav * uav + vav * vav + wav * wav) + l5 * a5 * (Hav + uav * cav));
+    return Fc;
+}
+
+/* source/Flux_Split_Scheme.cpp:82-149 */
+static cv_t steger_warming(double Gamma, cv_t QL, cv_t QR, double que)
+{
+    const double eps = F32(1E-6);
+    double lp[6], lm[6], fp[6], fm[6];
+    double densL = QL.r, uL = QL.ru / QL.r, vL = QL.rv / QL.r, wL = QL.rw / QL.r, eL = QL.re / QL.r;
+    double t = Gamma * (Gamma - F32(1.0)) * (eL - F32(0.5) * (uL * uL + vL * vL + wL * wL));
+    t = sqrt(t * t);
+    double cL = sqrt(t);
+    lp[1] = F32(0.5) * ((uL - que) + sqrt((uL - que) * (uL - que) + eps * eps));
+    lp[2] = lp[1]; lp[3] = lp[1];
+    lp[4] = F32(0.5) * (((uL - que) - cL) + sqrt(((uL - que) - cL) * ((uL - que) - cL) + eps * eps));
+    lp[5] = F32(0.5) * (((uL - que) + cL) + sqrt(((uL - que) + cL) * ((uL - que) + cL) + eps * eps));
+    double W2 = (F32(3.0) - Gamma) * (lp[4] + lp[5]) * cL * cL / F32(2.0) / (Gamma - F32(1.0));
+    double A = (uL - cL) * (uL - cL) + vL * vL + wL * wL;
+    double B = (uL + cL) * (uL + cL) + vL * vL + wL * wL;
+    fp[1] = densL / F32(2.0) / Gamma * (F32(2.0) * (Gamma - F32(1.0)) * lp[1] + lp[4] + lp[5]);
+    fp[2] = densL / F32(2.0) / Gamma * (F32(2.0) * (Gamma - F32(1.0)) * lp[1] * uL + lp[4] * (uL - cL) + lp[5] * (uL + cL));
+    fp[3] = densL / F32(2.0) / Gamma * (F32(2.0) * (Gamma - F32(1.0)) * lp[1] * vL + lp[4] * vL + lp[5] * vL);
+    fp[4] = densL / F32(2.0) / Gamma * (F32(2.0) * (Gamma - F32(1.0)) * lp[1] * wL + lp[4] * wL + lp[5] * wL);
+    fp[5] = densL / F32(2.0) / Gamma * ((Gamma - F32(1.0)) * lp[1] * (uL * uL + vL * vL + wL * wL) + F32(0.5) * lp[4] * A +
+                                        F32(0.5) * lp[5] * B + W2);
+    double densR = QR.r, uR = QR.ru / QR.r, vR = QR.rv / QR.r, wR = QR.rw / QR.r, eR = QR.re / QR.r;
+    t = Gamma * (Gamma - F32(1.0)) * (eR - F32(0.5) * (uR * uR + vR * vR + wR * wR));
+    t = sqrt(t * t);
+    double cR = sqrt(t);
+    lm[1] = F32(0.5) * ((uR - que) - sqrt((uR - que) * (uR - que) + eps * eps));
+    lm[2] = lm[1]; lm[3] = lm[1];
+    lm[4] = F32(0.5) * (((uR - que) - cR) - sqrt(((uR - que) - cR) * ((uR - que) - cR) + eps * eps));
+    lm[5] = F32(0.5) * (((uR - que) + cR) - sqrt(((uR - que) + cR) * ((uR - que) + cR) + eps * eps));
+    W2 = (F32(3.0) - Gamma) * (lm[4] + lm[5]) * cR * cR / F32(2.0) / (Gamma - F32(1.0));
+    A = (uR - cR) * (uR - cR) + vR * vR + wR * wR;
+    B = (uR + cR) * (uR + cR) + vR * vR + wR * wR;
+    fm[1] = densR / F32(2.0) / Gamma * (F32(2.0) * (Gamma - F32(1.0)) * lm[1] + lm[4] + lm[5]);
+    fm[2] = densR / F32(2.0) / Gamma * (F32(2.0) * (Gamma - F32(1.0)) * lm[1] * uR + lm[4] * (uR - cR) + lm[5] * (uR + cR));
+    fm[3] = densR / F32(2.0) / Gamma * (F32(2.0) * (Gamma - F32(1.0)) * lm[1] * vR + lm[4] * vR + lm[5] * vR);
+    fm[4] = densR / F32(2.0) / Gamma * (F32(2.0) * (Gamma - F32(1.0)) * lm[1] * wR + lm[4] * wR + lm[5] * wR);
+    fm[5] = densR / F32(2.0) / Gamma * ((Gamma - F32(1.0)) * lm[1] * (uR * uR + vR * vR + wR * wR) + F32(0.5) * lm[4] * A +
+                                        F32(0.5) * lm[5] * B + W2);
+    cv_t Fc = {fp[1] + fm[1], fp[2] + fm[2], fp[3] + fm[3], fp[4] + fm[4], fp[5] + fm[5]};
+    return Fc;
+}
+
+/* stencil element: cell array inside [1, n-1], Dummy layers outside */
+static void stencil(const blk_t *b, int dir, int i, int j, int k, int off, cv_t *q, pv_t *w)
+{
+    int IM = b->IM, JM = b->JM, KM = b->KM;
+    int ii = i, jj = j, kk = k, *m = dir == 0 ? &ii : dir == 1 ? &jj : &kk;
+    int N = dir == 0 ? IM : dir == 1 ? JM : KM;
+    *m += off;
+    int p = *m;
+    if (p >= 1 && p <= N - 1) { long x = C3(b, ii, jj, kk); *q = b->cv[x]; *w = b->pv[x]; return; }
+    int d, layer;
+    if (p < 1) { d = 2 * dir; layer = 1 - p; }
+    else { d = 2 * dir + 1; layer = p - (N - 1); }
+    long x;
+    if (dir == 0) x = DI(b, d, layer, jj, kk);
+    else if (dir == 1) x = DI(b, d, ii, layer, kk);
+    else x = DI(b, d, ii, jj, layer);
+    *q = b->dcv[d][x];
+    *w = b->dpv[d][x];
+}
+
+/* source/Upwind_Convective.cpp:5-360 */
+static void face_flux(orc_ctx *c, blk_t *b, int dir, int i, int j, int k)
+{
+    double Gamma = c->nd.gamma;
+    cv_t qFF, qF, qB, qBB, cvL, cvR;
+    pv_t wFF, wF, wB, wBB;
+    stencil(b, dir, i, j, k, -2, &qFF, &wFF);
+    stencil(b, dir, i, j, k, -1, &qF, &wF);
+    stencil(b, dir, i, j, k, 0, &qB, &wB);
+    stencil(b, dir, i, j, k, 1, &qBB, &wBB);
+    if (c->cfg.recon_var == 100) recon_cons(c->cfg.recon_scheme, Gamma, qFF, qF, qB, qBB, &cvL, &cvR);
+    else recon_prim(c->cfg.recon_scheme, Gamma, wFF, wF, wB, wBB, &cvL, &cvR);
+    const face_t *f;
+    v3_t diag;
+    v3_t *N = b->node;
+    if (dir == 0) { f = &b->fi[C3(b, i, j, k)]; diag = vsub(N[C3(b, i, j + 1, k + 1)], N[C3(b, i, j, k)]); }
+    else if (dir == 1) { f = &b->fj[C3(b, i, j, k)]; diag = vsub(N[C3(b, i + 1, j, k + 1)], N[C3(b, i, j, k)]); }
+    else { f = &b->fk[C3(b, i, j, k)]; diag = vsub(N[C3(b, i + 1, j + 1, k)], N[C3(b, i, j, k)]); }
+    frame_t F = coord_rotate(f, diag, b->w_rot);
+    cv_t QL = rotate_in(cvL, &F), QR = rotate_in(cvR, &F);
+    cv_t Fc = c->cfg.flux_scheme == 200 ? steger_warming(Gamma, QL, QR, F.que) : roe(Gamma, QL, QR, F.que);
+    cv_t *o = &b->face_fc[dir][C3(b, i, j, k)];
+    o->r = Fc.r * f->s;
+    o->ru = (Fc.ru * F.n.x + Fc.rv * F.t1.x + Fc.rw * F.t2.x) * f->s;
+    o->rv = (Fc.ru * F.n.y + Fc.rv * F.t1.y + Fc.rw * F.t2.y) * f->s;
+    o->rw = (Fc.ru * F.n.z + Fc.rv * F.t1.z + Fc.rw * F.t2.z) * f->s;
+    o->re = Fc.re * f->s;
+}
+
+/* source/WallFlux.cpp:70-175 */
+static void wall_flux(orc_ctx *c, blk_t *b, const patch_t *P)
+{
+    double Gamma = c->nd.gamma;
+    for (int i = P->is; i <= P->ie; ++i)
+        for (int j = P->js; j <= P->je; ++j)
+            for (int k = P->ks; k <= P->ke; ++k) {
+                int dir = (P->face - 1) / 2;
+                const face_t *f;
+                long xa = C3(b, i, j, k), xb;
+                if (dir == 0) { f = &b->fi[xa]; xb = C3(b, i - 1, j, k); }
+                else if (dir == 1) { f = &b->fj[xa]; xb = C3(b, i, j - 1, k); }
+                else { f = &b->fk[xa]; xb = C3(b, i, j, k - 1); }
+                v3_t S = f->S;
+                pv_t fp = kmulp(F32(0.5), padd(b->pv[xa], b->pv[xb]));
+                double fue = F32(-1.0) * f->c.y * P->rot, fve = f->c.x * P->rot;
+                cv_t fc;
+                fc.r = fp.d; fc.ru = fp.d * fp.u; fc.rv = fp.d * fp.v; fc.rw = fp.d * fp.w;
+                double Vn_a = fp.u * S.x + fp.v * S.y + fp.w * S.z;
+                double Vn_e = fue * S.x + fve * S.y + F32(0.0) * S.z;
+                fc.re = fp.p / (Gamma - F32(1.0)) + fp.d * (fp.u * fp.u + fp.v * fp.v + fp.w * fp.w) / F32(2.0);
+                cv_t *o = &b->face_fc[dir][xa];
+                o->r = fc.r * (Vn_a - Vn_e);
+                o->ru = fc.ru * (Vn_a - Vn_e) + fp.p * S.x;
+                o->rv = fc.rv * (Vn_a - Vn_e) + fp.p * S.y;
+                o->rw = fc.rw * (Vn_a - Vn_e) + fp.p * S.z;
+                o->re = (fc.re + fp.p) * (Vn_a - Vn_e) + fp.p * Vn_e;
+                if (P->face % 2 == 1) b->mut[xb] = -b->mut[xa];
+                else b->mut[xa] = -b->mut[xb];
+            }
+}
+
+/* source/ComputeAllBlockVars.cpp:73-119 */
+int orc_convective(orc_ctx *c)
+{
+    for (int n = 0; n < c->nb; ++n) {
+        blk_t *b = &c->blk[n];
+        int IM = b->IM, JM = b->JM, KM = b->KM;
+        for (int i = 1; i <= IM; ++i)
+            for (int j = 1; j <= JM - 1; ++j)
+                for (int k = 1; k <= KM - 1; ++k) face_flux(c, b, 0, i, j, k);
+        for (int i = 1; i <= IM - 1; ++i)
+            for (int j = 1; j <= JM; ++j)
+                for (int k = 1; k <= KM - 1; ++k) face_flux(c, b, 1, i, j, k);
+        for (int i = 1; i <= IM - 1; ++i)
+            for (int j = 1; j <= JM - 1; ++j)
+                for (int k = 1; k <= KM; ++k) face_flux(c, b, 2, i, j, k);
+    }
+    for (int n = 0; n < c->nb; ++n) {
+        blk_t *b = &c->blk[n];
+        for (int q = 0; q < b->npatch; ++q)
+            if (b->patch[q].kind == 2) wall_flux(c, b, &b->patch[q]);
+    }
+    for (int n = 0; n < c->nb; ++n) {
+        blk_t *b = &c->blk[n];
+        double w_rot = b->w_rot;
+        for (int i = 1; i <= b->IM - 1; ++i)
+            for (int j = 1; j <= b->JM - 1; ++j)
+                for (int k = 1; k <= b->KM - 1; ++k) {
+                    long x = C3(b, i, j, k);
+                    const pv_t *w = &b->pv[x];
+                    double vol = b->cell[x].vol;
+                    cv_t Qc = {0.0, w->d * (w_rot * w->v) * vol, F32(-1.0) * w->d * (w_rot * w->u) * vol, 0.0, 0.0};
+                    cv_t s = csub(b->face_fc[0][C3(b, i + 1, j, k)], b->face_fc[0][x]);
+                    s = cadd(s, b->face_fc[1][C3(b, i, j + 1, k)]);
+                    s = csub(s, b->face_fc[1][x]);
+                    s = cadd(s, b->face_fc[2][C3(b, i, j, k + 1)]);
+                    s = csub(s, b->face_fc[2][x]);
+                    b->fc[x] = csub(s, Qc);
+                }
+    }
+    return 0;
+}
+
+/* source/ComputeAllBlockVars.cpp:223-255 (Euler path: Fv = Fd = 0) */
+int orc_resflux(orc_ctx *c)
+{
+    orc_bc(c);
+    orc_spectral(c);
+    orc_timestep(c);
+    orc_convective(c);
+    const cv_t zero = {0, 0, 0, 0, 0};
+    for (int n = 0; n < c->nb; ++n) {
+        blk_t *b = &c->blk[n];
+        for (int i = 1; i <= b->IM - 1; ++i)
+            for (int j = 1; j <= b->JM - 1; ++j)
+                for (int k = 1; k <= b->KM - 1; ++k) {
+                    long x = C3(b, i, j, k);
+                    b->res[x] = csub(csub(b->fc[x], zero), zero);
+                }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------- LU-SGS */
+/* source/LU_SGS_Marching.cpp:293-419; nb = neighbour cell, fa/fb its two faces
+ * (higher index first), as delt_Fn_up/down read them. */
+static cv_t delt_fn(orc_ctx *c, blk_t *b, long nb, const face_t *fa, const face_t *fb)
+{
+    double Gamma = c->nd.gamma;
+    v3_t norm;
+    norm.x = (fa->S.x / fa->s + fb->S.x / fb->s) / F32(2.0);
+    norm.y = (fa->S.y / fa->s + fb->S.y / fb->s) / F32(2.0);
+    norm.z = (fa->S.z / fa->s + fb->S.z / fb->s) / F32(2.0);
+    double uvel_e = F32(-1.0) * b->cell[nb].c.y * b->w_rot;
+    double vvel_e = b->cell[nb].c.x * b->w_rot;
+    cv_t old = b->cv[nb];
+    cv_t nw = cadd(old, b->dq[nb]);
+    double dens = nw.r, uvel = nw.ru / nw.r, vvel = nw.rv / nw.r, wvel = nw.rw / nw.r, E = nw.re / nw.r;
+    double press_new = (Gamma - F32(1.0)) * dens * (E - F32(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel));
+    double press_old = b->pv[nb].p;
+    double Vn_old = old.ru / old.r * norm.x + old.rv / old.r * norm.y + old.rw / old.r * norm.z;
+    double Vn_new = uvel * norm.x + vvel * norm.y + wvel * norm.z;
+    double Vn_e = uvel_e * norm.x + vvel_e * norm.y + F32(0.0) * norm.z;
+    cv_t d;
+    d.r = nw.r * (Vn_new - Vn_e) - old.r * (Vn_old - Vn_e);
+    d.ru = (nw.ru * (Vn_new - Vn_e) + press_new * norm.x) - (old.ru * (Vn_old - Vn_e) + press_old * norm.x);
+    d.rv = (nw.rv * (Vn_new - Vn_e) + press_new * norm.y) - (old.rv * (Vn_old - Vn_e) + press_old * norm.y);
+    d.rw = (nw.rw * (Vn_new - Vn_e) + press_new * norm.z) - (old.rw * (Vn_old - Vn_e) + press_old * norm.z);
+    d.re = ((nw.re + press_new) * (Vn_new - Vn_e) + press_new * Vn_e) - ((old.re + press_old) * (Vn_old - Vn_e) + press_old * Vn_e);
+    return d;
+}
+
+static void lusgs_sweeps(orc_ctx *c, blk_t *b)
+{
+    int IM = b->IM, JM = b->JM, KM = b->KM;
+    double relax = (float)c->cfg.relax;
+    const cv_t zero = {0, 0, 0, 0, 0};
+    /* source/LU_SGS_Marching.cpp:61-160 */
+    for (int plane = 3; plane <= IM + JM + KM - 3; ++plane)
+        for (int i = 1; i <= IM - 1; ++i)
+            for (int j = 1; j <= JM - 1; ++j) {
+                int k = plane - i - j;
+                if (k < 1 || k > KM - 1) continue;
+                long x = C3(b, i, j, k);
+                double alpha = b->cell[x].vol / b->dt[x] + relax * (b->lamc[0][x] + b->lamc[1][x] + b->lamc[2][x]) +
+                               F32(2.0) * (b->lamv[0][x] + b->lamv[1][x] + b->lamv[2][x]);
+                cv_t A[3], V[3];
+                for (int d = 0; d < 3; ++d) {
+                    int ok = d == 0 ? i != 1 : d == 1 ? j != 1 : k != 1;
+                    if (!ok) { A[d] = zero; V[d] = zero; continue; }
+                    long nb = d == 0 ? C3(b, i - 1, j, k) : d == 1 ? C3(b, i, j - 1, k) : C3(b, i, j, k - 1);
+                    const face_t *fa = d == 0 ? &b->fi[x] : d == 1 ? &b->fj[x] : &b->fk[x];
+                    const face_t *fb = d == 0 ? &b->fi[nb] : d == 1 ? &b->fj[nb] : &b->fk[nb];
+                    cv_t dF = delt_fn(c, b, nb, fa, fb);
+                    double s = F32(0.5) * (fb->s + fa->s);
+                    A[d] = cscale(cadd(cscale(dF, s), cscale(b->dq[nb], relax * b->lamc[d][nb])), F32(0.5));
+                    V[d] = cscale(b->dq[nb], b->lamv[d][nb]);
+                }
+                cv_t s = cadd(cscale(b->res[x], F32(-1.0)), A[0]);
+                s = cadd(s, A[1]); s = cadd(s, A[2]); s = cadd(s, V[0]); s = cadd(s, V[1]); s = cadd(s, V[2]);
+                b->dq[x] = cdiv(s, alpha);
+            }
+    /* source/LU_SGS_Marching.cpp:162-260 */
+    for (int plane = IM + JM + KM - 3; plane >= 3; --plane)
+        for (int i = IM - 1; i >= 1; --i)
+            for (int j = JM - 1; j >= 1; --j) {
+                int k = plane - i - j;
+                if (k < 1 || k > KM - 1) continue;
+                long x = C3(b, i, j, k);
+                double alpha = b->cell[x].vol / b->dt[x] + relax * (b->lamc[0][x] + b->lamc[1][x] + b->lamc[2][x]) +
+                               F32(2.0) * (b->lamv[0][x] + b->lamv[1][x] + b->lamv[2][x]);
+                cv_t A[3], V[3];
+                for (int d = 0; d < 3; ++d) {
+                    int ok = d == 0 ? i != IM - 1 : d == 1 ? j != JM - 1 : k != KM - 1;
+                    if (!ok) { A[d] = zero; V[d] = zero; continue; }
+                    long nb = d == 0 ? C3(b, i + 1, j, k) : d == 1 ? C3(b, i, j + 1, k) : C3(b, i, j, k + 1);
+                    long n2 = d == 0 ? C3(b, i + 2, j, k) : d == 1 ? C3(b, i, j + 2, k) : C3(b, i, j, k + 2);
+                    const face_t *fa = d == 0 ? &b->fi[n2] : d == 1 ? &b->fj[n2] : &b->fk[n2];
+                    const face_t *fb = d == 0 ? &b->fi[nb] : d == 1 ? &b->fj[nb] : &b->fk[nb];
+                    cv_t dF = delt_fn(c, b, nb, fa, fb);
+                    double s = F32(0.5) * (fb->s + fa->s);
+                    A[d] = cscale(csub(cscale(dF, s), cscale(b->dq[nb], relax * b->lamc[d][nb])), F32(0.5));
+                    V[d] = cscale(b->dq[nb], b->lamv[d][nb]);
+                }
+                cv_t s = csub(cscale(b->dq[x], alpha), A[0]);
+                s = csub(s, A[1]); s = csub(s, A[2]); s = cadd(s, V[0]); s = cadd(s, V[1]); s = cadd(s, V[2]);
+                b->dq[x] = cdiv(s, alpha);
+            }
+}
+
+/* source/PrimitiveVars.cpp:7-29 */
+static void primitive_vars(orc_ctx *c, blk_t *b)
+{
+    double Gamma = c->nd.gamma, Rgas = c->nd.rgas;
+    for (int i = 1; i < b->IM; ++i)
+        for (int j = 1; j < b->JM; ++j)
+            for (int k = 1; k < b->KM; ++k) {
+                long x = C3(b, i, j, k);
+                cv_t *q = &b->cv[x];
+                pv_t *w = &b->pv[x];
+                w->d = q->r;
+                w->u = q->ru / q->r;
+                w->v = q->rv / q->r;
+                w->w = q->rw / q->r;
+                double V2 = w->u * w->u + w->v * w->v + w->w * w->w;
+                w->p = (Gamma - F32(1.0)) * (q->re - q->r * V2 / F32(2.0));
+                w->t = w->p / w->d / Rgas;
+            }
+}
+
+static inline int bad_cell(const pv_t *w)
+{
+    const double dmin = F32(1.0E-6), dmax = F32(1000.0), pmin = F32(1.0E-6), pmax = F32(1000.0), vmax = F32(1000.0);
+    return w->d < dmin || w->d > dmax || w->p < pmin || w->p > pmax || w->u > vmax || w->v > vmax || w->w > vmax;
+}
+
+/* source/Limit_FlowVars.cpp:8-119 */
+static void limit_vars(orc_ctx *c, blk_t *b)
+{
+    double Gamma = c->nd.gamma, Rgas = c->nd.rgas;
+    const double dmin = F32(1.0E-6), dmax = F32(1000.0), pmin = F32(1.0E-6), pmax = F32(1000.0), vmax = F32(1000.0);
+    int IM = b->IM, JM = b->JM, KM = b->KM, nneg = 0;
+    for (int i = 1; i <= IM - 1; ++i)
+        for (int j = 1; j <= JM - 1; ++j)
+            for (int k = 1; k <= KM - 1; ++k) {
+                pv_t *w = &b->pv[C3(b, i, j, k)];
+                if (!bad_cell(w)) continue;
+                nneg++;
+                int kn = 0, i1 = 0, j1 = 0, k1 = 0;
+                double d = 0.0, u = 0.0, v = 0.0, ww = 0.0, p = 0.0;
+                for (int ia = -1; ia <= 1; ++ia)
+                    for (int ja = -1; ja <= 1; ++ja)
+                        for (int ka = -1; ka <= 1; ++ka) {
+                            k1 = k + ka; j1 = j + ja; i1 = i + ia;
+                            if (i1 > 0 && i1 < IM && j1 > 0 && j1 < JM && k1 > 0 && k1 < KM) {
+                                const pv_t *q = &b->pv[C3(b, i1, j1, k1)];
+                                if (!bad_cell(q)) {
+                                    d = d + q->d; u = u + q->u; v = v + q->v; ww = ww + q->w; p = p + q->p;
+                                    kn = kn + 1;
+                                }
+                            }
+                        }
+                if (kn == 0) {
+                    const pv_t *q = &b->pv[C3(b, i1, j1, k1)];
+                    d = q->d; u = q->u; v = q->v; ww = q->w; p = q->p;
+                    if (d < dmin) d = dmin;
+                    if (d > dmax) d = dmax;
+                    if (p < pmin) p = pmin;
+                    if (p > pmax) p = pmax;
+                    if (fabs(u) > vmax) u = u < 0 ? -fabs(vmax) : fabs(vmax);
+                    if (fabs(v) > vmax) v = v < 0 ? -fabs(vmax) : fabs(vmax);
+                    if (fabs(ww) > vmax) ww = ww < 0 ? -fabs(vmax) : fabs(vmax);
+                } else {
+                    double sn1 = (double)(1.0f / kn);
+                    d = d * sn1; u = u * sn1; v = v * sn1; ww = ww * sn1; p = p * sn1;
+                }
+                w->d = d; w->u = u; w->v = v; w->w = ww; w->p = p;
+                w->t = p / d / Rgas;
+                cv_t *q = &b->cv[C3(b, i, j, k)];
+                q->r = d; q->ru = d * u; q->rv = d * v; q->rw = d * ww;
+                q->re = p / (Gamma - F32(1.0)) + d * (u * u + v * v + ww * ww) / F32(2.0);
+            }
+    if (nneg > 0) b->flag_low_dt = 10;
+}
+
+/* source/LU_SGS_Marching.cpp:10-291 (single grid, no SA) */
+int orc_lusgs(orc_ctx *c)
+{
+    for (int n = 0; n < c->nb; ++n) {
+        blk_t *b = &c->blk[n];
+        for (int i = 1; i <= b->IM - 1; ++i)
+            for (int j = 1; j <= b->JM - 1; ++j)
+                for (int k = 1; k <= b->KM - 1; ++k) b->cv_old[C3(b, i, j, k)] = b->cv[C3(b, i, j, k)];
+    }
+    orc_resflux(c);
+    for (int n = 0; n < c->nb; ++n) {
+        blk_t *b = &c->blk[n];
+        lusgs_sweeps(c, b);
+        for (int i = 1; i <= b->IM - 1; ++i)
+            for (int j = 1; j <= b->JM - 1; ++j)
+                for (int k = 1; k <= b->KM - 1; ++k) {
+                    long x = C3(b, i, j, k);
+                    b->cv[x] = cadd(b->cv[x], b->dq[x]);
+                }
+    }
+    for (int n = 0; n < c->nb; ++n) primitive_vars(c, &c->blk[n]);
+    if (c->cfg.limiter)
+        for (int n = 0; n < c->nb; ++n) limit_vars(c, &c->blk[n]);
+    return 0;
+}
+
+/* source/ComputeAllBlockVars.cpp:318-372 */
+int orc_res_calculate(orc_ctx *c, double *res6)
+{
+    double r[5] = {0, 0, 0, 0, 0};
+    for (int n = 0; n < c->nb; ++n) {
+        blk_t *b = &c->blk[n];
+        for (int i = 1; i <= b->IM - 1; ++i)
+            for (int j = 1; j <= b->JM - 1; ++j)
+                for (int k = 1; k <= b->KM - 1; ++k) {
+                    long x = C3(b, i, j, k);
+                    const cv_t *o = &b->cv_old[x], *q = &b->cv[x];
+                    cv_t m;
+                    m.r = fabs((o->r - q->r) / o->r);
+                    m.ru = fabs((o->ru - q->ru) / o->ru);
+                    m.rv = fabs((o->rv - q->rv) / o->rv);
+                    m.rw = fabs((o->rw - q->rw) / o->rw);
+                    m.re = fabs((o->re - q->re) / o->re);
+                    m = cscale(m, b->cell[x].vol);
+                    r[0] = r[0] + m.r; r[1] = r[1] + m.ru; r[2] = r[2] + m.rv; r[3] = r[3] + m.rw; r[4] = r[4] + m.re;
+                }
+    }
+    for (int q = 0; q < 5; ++q) c->res[q] = r[q] / c->total_vol;
+    c->res[5] = 0.0;
+    if (res6) for (int q = 0; q < 6; ++q) res6[q] = c->res[q];
+    return 0;
+}
+
+/* ----------------------------------------------------- performance data */
+/* source/Ouput.cpp:9-95 */
+static void section(orc_ctx *c, blk_t *b, int K, double *mass, double *pt, double *tt)
+{
+    double Gamma = c->nd.gamma, Rgas = c->nd.rgas;
+    double m = 0.0, mp = 0.0, ptm = 0.0, mt = 0.0, ttm = 0.0;
+    for (int i = 1; i <= b->IM - 1; ++i)
+        for (int j = 1; j <= b->JM - 1; ++j) {
+            const pv_t *a = &b->pv[C3(b, i, j, K)], *z = &b->pv[C3(b, i, j, K - 1)];
+            const face_t *f = &b->fk[C3(b, i, j, K)];
+            double dens = F32(0.5) * (a->d + z->d), u = F32(0.5) * (a->u + z->u), v = F32(0.5) * (a->v + z->v),
+                   w = F32(0.5) * (a->w + z->w);
+            double mc = dens * (u * f->S.x + v * f->S.y + w * f->S.z);
+            m = m + mc;
+            double temp = F32(0.5) * (a->t + z->t);
+            double Ma = sqrt(u * u + v * v + w * w) / sqrt(Gamma * Rgas * temp);
+            double Pt_cell = a->p * pow(F32(1.0) + (Gamma - F32(1.0)) / F32(2.0) * Ma * Ma, Gamma / (Gamma - F32(1.0)));
+            ptm = ptm + pow(Pt_cell, (Gamma - F32(1.0)) / Gamma) * mc;
+            mp = mp + mc;
+            double Tt_cell = a->t * pow(1 + (Gamma - F32(1.0)) / F32(2.0) * Ma * Ma, 1);
+            ttm = ttm + Tt_cell * mc;
+            mt = mt + mc;
+        }
+    *mass = m * c->nd.dens_ref * c->nd.vel_ref;
+    *pt = pow(ptm / mp, Gamma / (Gamma - F32(1.0))) * c->nd.dens_ref * c->nd.vel_ref * c->nd.vel_ref;
+    *tt = ttm / mt * c->nd.temp_ref;
+}
+
+/* source/Ouput.cpp:97-107,282-300 (first inlet and first outlet patch) */
+int orc_perf(orc_ctx *c, double *out5)
+{
+    blk_t *bi = 0, *bo = 0;
+    int ki = 0, ko = 0;
+    for (int n = 0; n < c->nb; ++n)
+        for (int q = 0; q < c->blk[n].npatch; ++q) {
+            patch_t *P = &c->blk[n].patch[q];
+            if (P->kind == 0 && !bi) { bi = &c->blk[n]; ki = P->ks; }
+            if (P->kind == 1 && !bo) { bo = &c->blk[n]; ko = P->ks; }
+        }
+    if (!bi || !bo) return -1;
+    double mi, pti, tti, mo, pto, tto;
+    section(c, bi, ki, &mi, &pti, &tti);
+    section(c, bo, ko, &mo, &pto, &tto);
+    mi = mi * bi->nblade;
+    mo = mo * bo->nblade;
+    double Gamma = c->nd.gamma, Rgas = c->nd.rgas;
+    double PR = pto / pti;
+    double L_adk = Gamma / (Gamma - F32(1.0)) * Rgas * tti * (pow(PR, (Gamma - F32(1.0)) / Gamma) - F32(1.0));
+    double L_k = Gamma / (Gamma - F32(1.0)) * Rgas * tti * (tto / tti - F32(1.0));
+    out5[0] = mi;
+    out5[1] = mo;
+    out5[2] = pto / pti;
+    out5[3] = L_adk / L_k;
+    out5[4] = fabs((mi - mo) / mi);
+    return 0;
+}
+
+double orc_cfl(orc_ctx *c) { return c->cfl; }
+
+/* source/Solver.cpp:47-66 and source/Structured_FV3D.cpp:142-158 */
+int orc_iterate(orc_ctx *c, int n, double *hist)
+{
+    const double amp = F32(1.2), cmax = F32(10.0);
+    for (int it = 0; it < n; ++it) {
+        c->iter++;
+        if (c->iter > 1) {
+            if (c->res[0] <= c->cfl_res) {
+                c->cfl = c->cfl * amp;
+                c->cfl_res = c->cfl_res / F32(10.0);
+            }
+            if (c->cfl > cmax) c->cfl = cmax;
+        }
+        orc_lusgs(c);
+        orc_res_calculate(c, 0);
+        double p[5];
+        orc_perf(c, p);
+        if (hist) {
+            double *h = hist + (long)it * 7;
+            h[0] = c->iter; h[1] = c->res[0]; h[2] = p[0]; h[3] = p[1]; h[4] = p[2]; h[5] = p[3]; h[6] = c->cfl;
+        }
+    }
+    return 0;
+}
+
+/* -------------------------------------------------------------- output */
+long orc_get(orc_ctx *c, int block, const char *name, double *out)
+{
+    if (block < 1 || block > c->nb) return -1;
+    blk_t *b = &c->blk[block - 1];
+    int IM = b->IM, JM = b->JM, KM = b->KM;
+    long n = 0;
+    if (!strcmp(name, "cv") || !strcmp(name, "pv")) {
+        int pvq = name[0] == 'p';
+        for (int i = 0; i <= IM; ++i)
+            for (int j = 0; j <= JM; ++j)
+                for (int k = 0; k <= KM; ++k) {
+                    long x = C3(b, i, j, k);
+                    if (pvq) { const pv_t *w = &b->pv[x]; double t[6] = {w->d, w->u, w->v, w->w, w->p, w->t}; memcpy(out + n, t, sizeof t); n += 6; }
+                    else { const cv_t *q = &b->cv[x]; double t[5] = {q->r, q->ru, q->rv, q->rw, q->re}; memcpy(out + n, t, sizeof t); n += 5; }
+                }
+        return n;
+    }
+    const cv_t *c5 = 0;
+    const double *s1 = 0;
+    if (!strcmp(name, "res")) c5 = b->res;
+    else if (!strcmp(name, "dq")) c5 = b->dq;
+    else if (!strcmp(name, "fc")) c5 = b->fc;
+    else if (!strcmp(name, "dt")) s1 = b->dt;
+    else if (!strcmp(name, "lamcI")) s1 = b->lamc[0];
+    else if (!strcmp(name, "lamcJ")) s1 = b->lamc[1];
+    else if (!strcmp(name, "lamcK")) s1 = b->lamc[2];
+    else if (!strcmp(name, "fcI") || !strcmp(name, "fcJ") || !strcmp(name, "fcK")) {
+        int d = name[2] - 'I';
+        for (int i = 1; i <= IM - (d != 0); ++i)
+            for (int j = 1; j <= JM - (d != 1); ++j)
+                for (int k = 1; k <= KM - (d != 2); ++k) {
+                    const cv_t *q = &b->face_fc[d][C3(b, i, j, k)];
+                    double t[5] = {q->r, q->ru, q->rv, q->rw, q->re};
+                    memcpy(out + n, t, sizeof t);
+                    n += 5;
+                }
+        return n;
+    }
+    else if (!strcmp(name, "vol")) {
+        for (int i = 1; i <= IM - 1; ++i)
+            for (int j = 1; j <= JM - 1; ++j)
+                for (int k = 1; k <= KM - 1; ++k) out[n++] = b->cell[C3(b, i, j, k)].vol;
+        return n;
+    }
+    if (c5 || s1) {
+        for (int i = 1; i <= IM - 1; ++i)
+            for (int j = 1; j <= JM - 1; ++j)
+                for (int k = 1; k <= KM - 1; ++k) {
+                    long x = C3(b, i, j, k);
+                    if (c5) { const cv_t *q = &c5[x]; double t[5] = {q->r, q->ru, q->rv, q->rw, q->re}; memcpy(out + n, t, sizeof t); n += 5; }
+                    else out[n++] = s1[x];
+                }
+        return n;
+    }
+    static const char *dn[6] = {"dIS", "dIP", "dJS", "dJP", "dKS", "dKP"};
+    for (int d = 0; d < 6; ++d) {
+        if (strncmp(name, dn[d], 3) != 0) continue;
+        int pvq = strstr(name, "_pv") != 0;
+        int ra = d < 2 ? ND : IM - 1, rb = d < 2 ? JM - 1 : d < 4 ? ND : JM - 1, rc = d < 4 ? KM - 1 : ND;
+        for (int a = 1; a <= ra; ++a)
+            for (int bb = 1; bb <= rb; ++bb)
+                for (int cc = 1; cc <= rc; ++cc) {
+                    long x = DI(b, d, a, bb, cc);
+                    if (pvq) { const pv_t *w = &b->dpv[d][x]; double t[6] = {w->d, w->u, w->v, w->w, w->p, w->t}; memcpy(out + n, t, sizeof t); n += 6; }
+                    else { const cv_t *q = &b->dcv[d][x]; double t[5] = {q->r, q->ru, q->rv, q->rw, q->re}; memcpy(out + n, t, sizeof t); n += 5; }
+                }
+        return n;
+    }
+    return -2;
+}

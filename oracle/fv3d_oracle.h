@@ -1,0 +1,92 @@
+/* CPU restatement of the reference solver's per-iteration path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker used by tests/,
+ * __graft_entry__.smoke() and bench.py's CPU-baseline leg.  The product never
+ * links, loads or calls it.
+ *
+ * It restates, in plain C with the reference's array extents and expression
+ * order, the functions of the reference's LU-SGS iteration: ghost fill, spectral
+ * radii, local time step, MUSCL/NND reconstruction with the Roe and
+ * Steger-Warming fluxes, wall flux, residual assembly, LU-SGS sweeps, update,
+ * primitive recovery, variable limiting, residual norm and the section
+ * performance integrals.  Each function cites the reference file:line it
+ * follows.  Indices are the reference's: cells 1..IM-1, ghost cells 0 and IM,
+ * faces 1..IM, Dummy layers 1..NDummy.
+ */
+#ifndef FV3D_ORACLE_H
+#define FV3D_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    /* raw config.dat values (parsed as float by the reference) */
+    double rgas, gamma, pr_lam, pr_tur;
+    double pt_in, tt_in, p_out, ma0, len_ref, ev_ratio;
+    double inlet_dir[3];
+    double cfl, c_timestep, relax;
+    int turbo_type;      /* 0 axial, 1 centrifugal, 2 general */
+    int viscous;         /* 0 Euler, 1 Navier-Stokes */
+    int recon_var;       /* 100 conservative, 200 primitive */
+    int recon_scheme;    /* 100 NND, 200 MUSCL2u, 300 MUSCL2c, 400 MUSCL2o, 500 MUSCL3 */
+    int flux_scheme;     /* 200 Steger-Warming, 300 Roe */
+    int turbulence;      /* 0 none (100 BL, 200 SA: not in this library) */
+    int limiter;         /* limit flow variables */
+    int global_dt;       /* 0 local time step */
+    int nthreads;
+} orc_config;
+
+typedef struct {
+    double gamma, rgas, pr_lam, pr_tur;
+    double pt_in, tt_in, p_out;      /* non-dimensional */
+    double temp_ref, press_ref, vel_ref, mue_ref, dens_ref, re_ref;
+    double ev_ratio;
+    double inlet_dir[3];
+} orc_nondim;
+
+typedef struct orc_ctx orc_ctx;
+
+/* Non-dimensionalisation of source/UserInput.cpp:1070-1092. */
+void orc_nondimensionalize(const orc_config *cfg, orc_nondim *nd);
+
+orc_ctx *orc_create(const orc_config *cfg);
+void orc_destroy(orc_ctx *c);
+
+/* nodes: IM*JM*KM*3 doubles, node (i,j,k) (1-based) at ((i-1)*JM+(j-1))*KM+(k-1).
+ * rpm as written in the block table; w_rot = 2*PI*rpm/60 non-dimensionalised. */
+int orc_block_add(orc_ctx *c, int IM, int JM, int KM, const double *nodes, int nblade, double rpm);
+
+/* kind: 0 inlet, 1 outlet, 2 wall.  rng = is ie js je ks ke as in the
+ * boundary-definition file. */
+int orc_patch_add(orc_ctx *c, int kind, int block, int face, const int *rng, int rotate);
+
+int orc_geometry(orc_ctx *c);
+
+/* Restart semantics of source/UserInput.cpp:1118-1159: pv is (ni,nj,nk,6)
+ * rho,u,v,w,p,T and ener (ni,nj,nk), cell (i,j,k) at ((i-1)*nj+(j-1))*nk+(k-1). */
+int orc_set_state(orc_ctx *c, int block, const double *pv, const double *ener);
+
+int orc_bc(orc_ctx *c);
+int orc_spectral(orc_ctx *c);
+int orc_timestep(orc_ctx *c);
+int orc_convective(orc_ctx *c);
+int orc_resflux(orc_ctx *c);
+int orc_lusgs(orc_ctx *c);
+int orc_res_calculate(orc_ctx *c, double *res6);
+int orc_perf(orc_ctx *c, double *out5);   /* mass_in, mass_out, PR, eff, mass_ratio */
+double orc_cfl(orc_ctx *c);
+
+/* n iterations of CSolver::Solver (single grid) + performance parameters.
+ * hist receives n rows of: iter res_dens mass_in mass_out PR eff CFL. */
+int orc_iterate(orc_ctx *c, int n, double *hist);
+
+/* Copies a field out.  Names: cv pv (full (IM+1)(JM+1)(KM+1)xC incl. ghost
+ * cells), res dq dt lamcI lamcJ lamcK (interior (ni,nj,nk)xC), dIS_cv dIP_cv
+ * dJS_cv dJP_cv dKS_cv dKP_cv and the _pv variants (layers 1..NDummy). */
+long orc_get(orc_ctx *c, int block, const char *name, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
