@@ -1,0 +1,940 @@
+// Context, memory management and C ABI of libfv3d.
+//
+// The context owns every device allocation.  Metrics are computed once on the
+// host in FP64 with the reference's expression order
+// (source/GeometryVariables.cpp:10-105) and uploaded; all per-iteration work
+// runs on the context's stream without host round trips.
+#include "fv3d.h"
+#include "fv3d_kernels.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+using namespace fv3d;
+
+namespace {
+
+struct HostPatch {
+    int kind, block, face, rotate;
+    int rng[6];
+};
+
+struct HostBlock {
+    int IM, JM, KM, nblade;
+    double rpm, w_rot, cellvol;
+    std::vector<double> nodes;   // (IM, JM, KM, 3) as given
+    BlockDev dev;                // device pointers + dims
+    std::vector<void *> allocs;
+};
+
+}  // namespace
+
+struct fv3d_ctx {
+    fv3d_config cfg;
+    Gas gas;
+    double temp_ref, press_ref, vel_ref, mue_ref, dens_ref, re_ref, len_ref;
+    int device;
+    hipStream_t stream = nullptr;
+    std::vector<HostBlock> blocks;
+    std::vector<HostPatch> patches;
+    bool ready = false;
+    std::string err;
+    // device work lists
+    BlockDev *d_blocks = nullptr;
+    PatchDev *d_patches = nullptr;
+    long *d_pstart_bc = nullptr, *d_cstart = nullptr, *d_pstart_plane = nullptr;
+    int4 *d_tiles = nullptr;
+    int2 *d_secs = nullptr;
+    int npatch_dev = 0, ntiles = 0, nplanes = 0;
+    long nbc = 0, ncell = 0, nplane_threads = 0;
+    double *d_partial = nullptr;
+    int npart = 0;
+    double *d_sums = nullptr;
+    int *d_bad = nullptr;
+    int *d_flags = nullptr;
+    ScalarsDev *d_sc = nullptr;
+    double *d_hist = nullptr;
+    long hist_cap = 0;
+    double *d_stage_row = nullptr;   // history row written by the captured graph
+    double *d_faces[3] = {nullptr, nullptr, nullptr};
+    int store_faces = 0;
+    double total_vol = 0.0;
+    // persistent wavefront sweep: tile lists (forward, backward) and the
+    // sync words [err, ticket_f, ticket_b, done_f[ntiles], done_b[ntiles]]
+    KbaTile *d_kba[2] = {nullptr, nullptr};
+    int nkba = 0;
+    int *d_kba_sync = nullptr;
+    int kba_chunk = 1;
+    int sweep_mode = 2;   // 0 one launch per hyperplane, 1 wavefront, 2 prefetching wavefront
+    int kba_grid = 0;
+    // graph
+    hipGraphExec_t graph_exec = nullptr;
+    // timing
+    int timing = 0;
+    hipEvent_t ev[8];
+    double t_ms[5] = {0, 0, 0, 0, 0};
+    long t_n[5] = {0, 0, 0, 0, 0};
+};
+
+static int fail(fv3d_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCK(c, x)                                                                                  \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return fail((c), FV3D_EHIP, "%s: %s", #x, hipGetErrorString(e_));       \
+    } while (0)
+
+template <class T>
+static int dalloc(fv3d_ctx *c, T **p, size_t n)
+{
+    HIPCK(c, hipMalloc((void **)p, n * sizeof(T) + 64));
+    HIPCK(c, hipMemsetAsync(*p, 0, n * sizeof(T) + 64, c->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------- set-up
+// Non-dimensionalisation (source/UserInput.cpp:60-76, 1070-1092).  Config
+// values are parsed by the reference with stof, hence the float rounding.
+static void nondimensionalize(fv3d_ctx *c)
+{
+    const fv3d_config &f = c->cfg;
+    double gamma = (float)f.gamma, rgas = (float)f.rgas;
+    double ttin = (float)f.tt_in, ptin = (float)f.pt_in, pout = (float)f.p_out;
+    c->len_ref = (float)f.len_ref;
+    c->temp_ref = ttin;
+    c->press_ref = ptin;
+    c->vel_ref = sqrt(gamma * rgas * c->temp_ref);
+    c->mue_ref = F32C(1.716e-5) * pow(c->temp_ref / F32C(288.15), F32C(3.0) / F32C(2.0)) * (double)(288.15f + 110.4f) /
+                 (c->temp_ref + F32C(110.4));
+    c->dens_ref = c->press_ref / rgas / c->temp_ref;
+    c->re_ref = c->dens_ref * c->vel_ref * c->len_ref / c->mue_ref;
+    Gas &g = c->gas;
+    g.gamma = gamma;
+    g.rgas = F32C(1.0) / (F32C(1.0) * F32C(1.0) * gamma);
+    g.pt_in = F32C(1.0) / gamma;
+    g.tt_in = F32C(1.0);
+    g.p_out = pout / (c->dens_ref * c->vel_ref * c->vel_ref);
+    for (int d = 0; d < 3; ++d) g.inlet_dir[d] = f.inlet_dir[d];
+    g.c_timestep = (float)f.c_timestep;
+    g.relax = (float)f.relax;
+    g.recon_var = f.recon_var;
+    g.recon_scheme = f.recon_scheme;
+    g.flux_scheme = f.flux_scheme;
+}
+
+// source/GeometryVariables.cpp:10-105
+static void face_vector_volume(HostBlock &b, std::vector<double> &fS0, std::vector<double> &fS1, std::vector<double> &fS2,
+                               std::vector<double> &fs0, std::vector<double> &fs1, std::vector<double> &fs2,
+                               std::vector<double> &fc0, std::vector<double> &fc1, std::vector<double> &fc2,
+                               std::vector<double> &vol, std::vector<double> &cc, std::vector<double> &node)
+{
+    const int IM = b.IM, JM = b.JM, KM = b.KM;
+    const long box = (long)(IM + 1) * (JM + 1) * (KM + 1);
+    const int sj = KM + 1, si = (JM + 1) * (KM + 1);
+    auto X = [&](int i, int j, int k) { return (long)i * si + (long)j * sj + k; };
+    auto N = [&](int i, int j, int k, int c) { return b.nodes[(((long)(i - 1) * JM + (j - 1)) * KM + (k - 1)) * 3 + c]; };
+    for (auto *v : {&fS0, &fS1, &fS2, &fc0, &fc1, &fc2, &cc, &node}) v->assign(3 * box, 0.0);
+    for (auto *v : {&fs0, &fs1, &fs2, &vol}) v->assign(box, 0.0);
+    for (int i = 1; i <= IM; ++i)
+        for (int j = 1; j <= JM; ++j)
+            for (int k = 1; k <= KM; ++k)
+                for (int c = 0; c < 3; ++c) node[c * box + X(i, j, k)] = N(i, j, k, c);
+    struct V { double x, y, z; };
+    auto nd = [&](int i, int j, int k) { return V{N(i, j, k, 0), N(i, j, k, 1), N(i, j, k, 2)}; };
+    auto sub = [](V a, V b) { return V{a.x - b.x, a.y - b.y, a.z - b.z}; };
+    auto add = [](V a, V b) { return V{a.x + b.x, a.y + b.y, a.z + b.z}; };
+    auto face = [&](std::vector<double> &S, std::vector<double> &s, std::vector<double> &cen, long x, V A, V B, V n1, V n2,
+                    V n3, V n4) {
+        double Sx = F32C(0.5) * (A.y * B.z - B.y * A.z);
+        double Sy = F32C(0.5) * (A.z * B.x - A.x * B.z);
+        double Sz = F32C(0.5) * (A.x * B.y - B.x * A.y);
+        V cs = add(add(add(n1, n2), n3), n4);
+        S[x] = Sx; S[box + x] = Sy; S[2 * box + x] = Sz;
+        cen[x] = cs.x * 0.25; cen[box + x] = cs.y * 0.25; cen[2 * box + x] = cs.z * 0.25;
+        s[x] = sqrt(Sx * Sx + Sy * Sy + Sz * Sz);
+    };
+    for (int i = 1; i <= IM; ++i)
+        for (int j = 1; j <= JM - 1; ++j)
+            for (int k = 1; k <= KM - 1; ++k)
+                face(fS0, fs0, fc0, X(i, j, k), sub(nd(i, j + 1, k), nd(i, j, k + 1)), sub(nd(i, j + 1, k + 1), nd(i, j, k)),
+                     nd(i, j, k), nd(i, j + 1, k), nd(i, j + 1, k + 1), nd(i, j, k + 1));
+    for (int i = 1; i <= IM - 1; ++i)
+        for (int j = 1; j <= JM; ++j)
+            for (int k = 1; k <= KM - 1; ++k)
+                face(fS1, fs1, fc1, X(i, j, k), sub(nd(i, j, k + 1), nd(i + 1, j, k)), sub(nd(i + 1, j, k + 1), nd(i, j, k)),
+                     nd(i, j, k), nd(i + 1, j, k), nd(i + 1, j, k + 1), nd(i, j, k + 1));
+    for (int i = 1; i <= IM - 1; ++i)
+        for (int j = 1; j <= JM - 1; ++j)
+            for (int k = 1; k <= KM; ++k)
+                face(fS2, fs2, fc2, X(i, j, k), sub(nd(i + 1, j + 1, k), nd(i, j, k)), sub(nd(i, j + 1, k), nd(i + 1, j, k)),
+                     nd(i, j, k), nd(i + 1, j, k), nd(i + 1, j + 1, k), nd(i, j + 1, k));
+    b.cellvol = 0.0;
+    auto dotf = [&](std::vector<double> &cen, std::vector<double> &S, long x) {
+        return cen[x] * S[x] + cen[box + x] * S[box + x] + cen[2 * box + x] * S[2 * box + x];
+    };
+    for (int i = 1; i <= IM - 1; ++i)
+        for (int j = 1; j <= JM - 1; ++j)
+            for (int k = 1; k <= KM - 1; ++k) {
+                long x = X(i, j, k);
+                double v = dotf(fc0, fS0, X(i + 1, j, k)) - dotf(fc0, fS0, x) + dotf(fc1, fS1, X(i, j + 1, k)) - dotf(fc1, fS1, x) +
+                           dotf(fc2, fS2, X(i, j, k + 1)) - dotf(fc2, fS2, x);
+                vol[x] = v / F32C(3.0);
+                V s = add(nd(i, j, k), nd(i + 1, j, k));
+                s = add(s, nd(i, j + 1, k)); s = add(s, nd(i + 1, j + 1, k));
+                s = add(s, nd(i, j, k + 1)); s = add(s, nd(i + 1, j, k + 1));
+                s = add(s, nd(i, j + 1, k + 1)); s = add(s, nd(i + 1, j + 1, k + 1));
+                cc[x] = s.x * 0.125; cc[box + x] = s.y * 0.125; cc[2 * box + x] = s.z * 0.125;
+                b.cellvol = b.cellvol + vol[x];
+            }
+}
+
+template <class T>
+static int upload(fv3d_ctx *c, HostBlock &b, T **dst, const std::vector<T> &src)
+{
+    int rc = dalloc(c, dst, src.size());
+    if (rc) return rc;
+    b.allocs.push_back(*dst);
+    HIPCK(c, hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+static int setup_block(fv3d_ctx *c, HostBlock &b, int index)
+{
+    BlockDev &B = b.dev;
+    memset(&B, 0, sizeof B);
+    B.IM = b.IM; B.JM = b.JM; B.KM = b.KM;
+    B.ni = b.IM - 1; B.nj = b.JM - 1; B.nk = b.KM - 1;
+    B.sj = b.KM + 1; B.si = (b.JM + 1) * (b.KM + 1);
+    B.box = (long)(b.IM + 1) * (b.JM + 1) * (b.KM + 1);
+    B.nblade = b.nblade;
+    B.w_rot = b.w_rot;
+    std::vector<double> fS[3], fs[3], fc[3], vol, cc, node;
+    face_vector_volume(b, fS[0], fS[1], fS[2], fs[0], fs[1], fs[2], fc[0], fc[1], fc[2], vol, cc, node);
+    int rc;
+    double *p;
+    for (int d = 0; d < 3; ++d) {
+        if ((rc = upload(c, b, &p, fS[d]))) return rc; B.fS[d] = p;
+        if ((rc = upload(c, b, &p, fs[d]))) return rc; B.fs[d] = p;
+        if ((rc = upload(c, b, &p, fc[d]))) return rc; B.fc[d] = p;
+    }
+    if ((rc = upload(c, b, &p, vol))) return rc; B.vol = p;
+    if ((rc = upload(c, b, &p, cc))) return rc; B.cc = p;
+    if ((rc = upload(c, b, &p, node))) return rc; B.node = p;
+    auto alloc = [&](double **q, long n) { int r = dalloc(c, q, n); if (!r) b.allocs.push_back(*q); return r; };
+    if ((rc = alloc(&B.Q, 5 * B.box))) return rc;
+    if ((rc = alloc(&B.W, 6 * B.box))) return rc;
+    if ((rc = alloc(&B.lamc, 3 * B.box))) return rc;
+    if ((rc = alloc(&B.dt, B.box))) return rc;
+    if ((rc = alloc(&B.alpha, B.box))) return rc;
+    {
+        // static sweep metrics of each cell: averaged unit normal of its two
+        // faces along d (upper face first), averaged area and the normal grid
+        // velocity at its centroid (source/LU_SGS_Marching.cpp:301-341)
+        std::vector<double> swg(15 * B.box, 0.0);
+        const long st[3] = {B.si, B.sj, 1};
+        for (int i = 1; i <= B.ni; ++i)
+            for (int j = 1; j <= B.nj; ++j)
+                for (int k = 1; k <= B.nk; ++k) {
+                    long x = cidx(B, i, j, k);
+                    for (int d = 0; d < 3; ++d) {
+                        long fa = x + st[d], fb = x;
+                        const std::vector<double> &S = fS[d], &s = fs[d];
+                        double nx = (S[fa] / s[fa] + S[fb] / s[fb]) / F32C(2.0);
+                        double ny = (S[B.box + fa] / s[fa] + S[B.box + fb] / s[fb]) / F32C(2.0);
+                        double nz = (S[2 * B.box + fa] / s[fa] + S[2 * B.box + fb] / s[fb]) / F32C(2.0);
+                        double uvel_e = F32C(-1.0) * cc[B.box + x] * B.w_rot, vvel_e = cc[x] * B.w_rot;
+                        swg[(d * 5 + 0) * B.box + x] = nx;
+                        swg[(d * 5 + 1) * B.box + x] = ny;
+                        swg[(d * 5 + 2) * B.box + x] = nz;
+                        swg[(d * 5 + 3) * B.box + x] = F32C(0.5) * (s[fb] + s[fa]);
+                        swg[(d * 5 + 4) * B.box + x] = uvel_e * nx + vvel_e * ny + F32C(0.0) * nz;
+                    }
+                }
+        if ((rc = upload(c, b, &p, swg))) return rc;
+        B.swg = p;
+    }
+    if ((rc = alloc(&B.R, 5 * B.box))) return rc;
+    if ((rc = alloc(&B.dQ, 5 * B.box))) return rc;
+    // Dummy slabs: tangential boxes (JM+1)(KM+1), (IM+1)(KM+1), (IM+1)(JM+1)
+    const int ta[3] = {b.JM, b.IM, b.IM}, tb[3] = {b.KM, b.KM, b.JM};
+    for (int s = 0; s < 6; ++s) {
+        int d = s / 2;
+        B.Dt[s] = (long)(ta[d] + 1) * (tb[d] + 1);
+        B.Db[s] = tb[d] + 1;
+        if ((rc = alloc(&B.D[s], 2 * 11 * B.Dt[s]))) return rc;
+    }
+    // wall masks per side
+    std::vector<unsigned char> mask[6];
+    std::vector<double> rot[6];
+    bool any[6] = {false, false, false, false, false, false};
+    for (int s = 0; s < 6; ++s) { mask[s].assign(B.Dt[s], 0); rot[s].assign(B.Dt[s], 0.0); }
+    for (const HostPatch &P : c->patches) {
+        if (P.kind != 2 || P.block != index + 1) continue;
+        int d = (P.face - 1) / 2, hi = (P.face - 1) % 2;
+        int s = 2 * d + hi;
+        const int *r = P.rng;
+        int aS = d == 0 ? r[2] : r[0], aE = d == 0 ? r[3] : r[1];
+        int bS = d == 2 ? r[2] : r[4], bE = d == 2 ? r[3] : r[5];
+        double wr = P.rotate ? b.w_rot : 0.0;
+        for (int a = aS; a <= aE; ++a)
+            for (int bb = bS; bb <= bE; ++bb) {
+                mask[s][(long)a * B.Db[s] + bb] = 1;
+                rot[s][(long)a * B.Db[s] + bb] = wr;
+            }
+        any[s] = true;
+    }
+    for (int s = 0; s < 6; ++s) {
+        if (!any[s]) { B.wmask[s] = nullptr; B.wrot[s] = nullptr; continue; }
+        unsigned char *m;
+        double *rr;
+        if ((rc = upload(c, b, &m, mask[s]))) return rc;
+        if ((rc = upload(c, b, &rr, rot[s]))) return rc;
+        B.wmask[s] = m;
+        B.wrot[s] = rr;
+    }
+    B.flag_low_dt = c->d_flags + index;
+    return 0;
+}
+
+// ------------------------------------------------------------------ API
+extern "C" {
+
+int fv3d_ctx_create(const fv3d_config *cfg, fv3d_ctx **out)
+{
+    if (!cfg || !out) return FV3D_EINVAL;
+    *out = nullptr;
+    fv3d_ctx *c = new fv3d_ctx();
+    c->cfg = *cfg;
+    if (cfg->recon_var != 100 && cfg->recon_var != 200) { *out = c; return fail(c, FV3D_EINVAL, "recon_var %d unsupported", cfg->recon_var); }
+    if (cfg->recon_scheme < 100 || cfg->recon_scheme > 500 || cfg->recon_scheme % 100) { *out = c; return fail(c, FV3D_EINVAL, "recon_scheme %d unsupported", cfg->recon_scheme); }
+    if (cfg->flux_scheme != 200 && cfg->flux_scheme != 300) { *out = c; return fail(c, FV3D_EINVAL, "flux_scheme %d unsupported", cfg->flux_scheme); }
+    if (cfg->viscous || cfg->turbulence) { *out = c; return fail(c, FV3D_EINVAL, "viscous/turbulent path not in this build"); }
+    if (cfg->global_dt) { *out = c; return fail(c, FV3D_EINVAL, "global time step not in this build"); }
+    if (cfg->turbo_type == 0) { *out = c; return fail(c, FV3D_EINVAL, "axial (radial-equilibrium) outlet not in this build"); }
+    if (cfg->precision != FV3D_PLAIN_FP64) { *out = c; return fail(c, FV3D_EINVAL, "precision mode %d not in this build", cfg->precision); }
+    c->device = cfg->device;
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) { *out = c; return fail(c, FV3D_EHIP, "hipSetDevice(%d): %s", c->device, hipGetErrorString(e)); }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { *out = c; return fail(c, FV3D_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+    for (int n = 0; n < 8; ++n) (void)hipEventCreate(&c->ev[n]);
+    nondimensionalize(c);
+    *out = c;
+    return FV3D_OK;
+}
+
+int fv3d_ctx_destroy(fv3d_ctx *c)
+{
+    if (!c) return FV3D_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    for (HostBlock &b : c->blocks)
+        for (void *p : b.allocs) (void)hipFree(p);
+    void *ps[] = {c->d_blocks, c->d_patches, c->d_pstart_bc, c->d_cstart, c->d_pstart_plane, c->d_tiles, c->d_secs,
+                  c->d_partial, c->d_sums, c->d_bad, c->d_flags, c->d_sc, c->d_hist, c->d_stage_row, c->d_kba[0], c->d_kba[1], c->d_kba_sync, c->d_faces[0], c->d_faces[1], c->d_faces[2]};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    if (c->stream) {
+        for (int n = 0; n < 8; ++n) (void)hipEventDestroy(c->ev[n]);
+        (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+    return FV3D_OK;
+}
+
+const char *fv3d_last_error(const fv3d_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int fv3d_block_add(fv3d_ctx *c, int IM, int JM, int KM, const double *nodes, int nblade, double rpm)
+{
+    if (!c || !nodes) return FV3D_EINVAL;
+    if (c->ready) return fail(c, FV3D_ESTATE, "block_add after setup");
+    if (IM < 4 || JM < 4 || KM < 4) return fail(c, FV3D_EINVAL, "block %dx%dx%d: need at least 3 cells per direction", IM, JM, KM);
+    if ((int)c->blocks.size() >= kMaxBlocks) return fail(c, FV3D_EINVAL, "more than %d blocks", kMaxBlocks);
+    HostBlock b;
+    b.IM = IM; b.JM = JM; b.KM = KM; b.nblade = nblade; b.rpm = rpm;
+    b.nodes.assign(nodes, nodes + (size_t)IM * JM * KM * 3);
+    // source/UserInput.cpp:444 and 1088
+    b.w_rot = (double)(2.0f * 3.1415926535897932f) * rpm / F32C(60.0);
+    b.w_rot = b.w_rot / (c->vel_ref / c->len_ref);
+    c->blocks.push_back(std::move(b));
+    return (int)c->blocks.size();
+}
+
+int fv3d_patch_add(fv3d_ctx *c, int kind, int block, int face, const int *rng, int rotate)
+{
+    if (!c || !rng) return FV3D_EINVAL;
+    if (c->ready) return fail(c, FV3D_ESTATE, "patch_add after setup");
+    if (kind < 0 || kind > 2) return fail(c, FV3D_EINVAL, "patch kind %d not in this build", kind);
+    if (block < 1 || block > (int)c->blocks.size()) return fail(c, FV3D_EINVAL, "patch block %d", block);
+    if (face < 1 || face > 6) return fail(c, FV3D_EINVAL, "patch face %d", face);
+    const HostBlock &b = c->blocks[block - 1];
+    int d = (face - 1) / 2, hi = (face - 1) % 2;
+    int N[3] = {b.IM, b.JM, b.KM};
+    if (rng[2 * d] != rng[2 * d + 1] || rng[2 * d] != (hi ? N[d] : 1))
+        return fail(c, FV3D_EINVAL, "patch normal range %d..%d does not match face %d", rng[2 * d], rng[2 * d + 1], face);
+    for (int e = 0; e < 3; ++e)
+        if (e != d && (rng[2 * e] < 1 || rng[2 * e + 1] > N[e] - 1 || rng[2 * e] > rng[2 * e + 1]))
+            return fail(c, FV3D_EINVAL, "patch tangential range %d..%d outside 1..%d", rng[2 * e], rng[2 * e + 1], N[e] - 1);
+    if ((kind == 0 || kind == 1) && d != 2) return fail(c, FV3D_EINVAL, "inlet/outlet patches must be K faces");
+    HostPatch p;
+    p.kind = kind; p.block = block; p.face = face; p.rotate = rotate;
+    memcpy(p.rng, rng, sizeof p.rng);
+    c->patches.push_back(p);
+    return FV3D_OK;
+}
+
+int fv3d_setup(fv3d_ctx *c)
+{
+    if (!c) return FV3D_EINVAL;
+    if (c->ready) return fail(c, FV3D_ESTATE, "setup called twice");
+    if (c->blocks.empty()) return fail(c, FV3D_ESTATE, "no blocks");
+    int nb = (int)c->blocks.size();
+    int rc;
+    if ((rc = dalloc(c, &c->d_flags, nb))) return rc;
+    if ((rc = dalloc(c, &c->d_bad, nb))) return rc;
+    c->total_vol = 0.0;
+    for (int n = 0; n < nb; ++n) {
+        if ((rc = setup_block(c, c->blocks[n], n))) return rc;
+        c->total_vol = c->total_vol + c->blocks[n].cellvol;   // source/ComputeAllBlockVars.cpp:25-36
+    }
+    std::vector<BlockDev> bd;
+    for (auto &b : c->blocks) bd.push_back(b.dev);
+    if ((rc = dalloc(c, &c->d_blocks, nb))) return rc;
+    HIPCK(c, hipMemcpyAsync(c->d_blocks, bd.data(), nb * sizeof(BlockDev), hipMemcpyHostToDevice, c->stream));
+    // patch work list
+    std::vector<PatchDev> pd;
+    std::vector<long> pst{0};
+    int nin = 0, nout = 0;
+    int2 secs[2] = {{0, 0}, {0, 0}};
+    for (const HostPatch &P : c->patches) {
+        PatchDev q;
+        q.kind = P.kind; q.block = P.block - 1;
+        q.dir = (P.face - 1) / 2; q.hi = (P.face - 1) % 2;
+        int d = q.dir;
+        const int *r = P.rng;
+        q.a0 = d == 0 ? r[2] : r[0];
+        int aE = d == 0 ? r[3] : r[1];
+        q.b0 = d == 2 ? r[2] : r[4];
+        int bE = d == 2 ? r[3] : r[5];
+        q.na = aE - q.a0 + 1; q.nb = bE - q.b0 + 1;
+        q.centrifugal = c->cfg.turbo_type == 1;
+        q.rot = P.rotate ? c->blocks[P.block - 1].w_rot : 0.0;
+        pd.push_back(q);
+        pst.push_back(pst.back() + (long)q.na * q.nb);
+        if (P.kind == 0 && nin++ == 0) secs[0] = make_int2(P.block - 1, r[4]);
+        if (P.kind == 1 && nout++ == 0) secs[1] = make_int2(P.block - 1, r[4]);
+    }
+    if (nin == 0 || nout == 0) return fail(c, FV3D_EINVAL, "need an inlet and an outlet patch");
+    c->npatch_dev = (int)pd.size();
+    c->nbc = pst.back();
+    if ((rc = dalloc(c, &c->d_patches, pd.size()))) return rc;
+    HIPCK(c, hipMemcpyAsync(c->d_patches, pd.data(), pd.size() * sizeof(PatchDev), hipMemcpyHostToDevice, c->stream));
+    if ((rc = dalloc(c, &c->d_pstart_bc, pst.size()))) return rc;
+    HIPCK(c, hipMemcpyAsync(c->d_pstart_bc, pst.data(), pst.size() * sizeof(long), hipMemcpyHostToDevice, c->stream));
+    if ((rc = dalloc(c, &c->d_secs, 2))) return rc;
+    HIPCK(c, hipMemcpyAsync(c->d_secs, secs, sizeof secs, hipMemcpyHostToDevice, c->stream));
+    // cell and plane work lists
+    std::vector<long> cst{0}, plst{0};
+    std::vector<int4> tiles;
+    c->nplanes = 0;
+    for (int n = 0; n < nb; ++n) {
+        const BlockDev &B = c->blocks[n].dev;
+        cst.push_back(cst.back() + (long)B.ni * B.nj * B.nk);
+        plst.push_back(plst.back() + (long)B.ni * B.nj);
+        c->nplanes = std::max(c->nplanes, B.IM + B.JM + B.KM - 5);
+        for (int i0 = 0; i0 < B.ni; i0 += kTI)
+            for (int j0 = 0; j0 < B.nj; j0 += kTJ)
+                for (int k0 = 0; k0 < B.nk; k0 += kTK) tiles.push_back(make_int4(n, i0, j0, k0));
+    }
+    c->ncell = cst.back();
+    c->nplane_threads = plst.back();
+    c->ntiles = (int)tiles.size();
+    if ((rc = dalloc(c, &c->d_cstart, cst.size()))) return rc;
+    HIPCK(c, hipMemcpyAsync(c->d_cstart, cst.data(), cst.size() * sizeof(long), hipMemcpyHostToDevice, c->stream));
+    if ((rc = dalloc(c, &c->d_pstart_plane, plst.size()))) return rc;
+    HIPCK(c, hipMemcpyAsync(c->d_pstart_plane, plst.data(), plst.size() * sizeof(long), hipMemcpyHostToDevice, c->stream));
+    if ((rc = dalloc(c, &c->d_tiles, tiles.size()))) return rc;
+    HIPCK(c, hipMemcpyAsync(c->d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    // wavefront tiles: forward in increasing, backward in decreasing tile
+    // diagonal, so every upstream tile holds a smaller ticket
+    {
+        struct T { int n, J, K, diag; };
+        std::vector<T> lst;
+        std::vector<int> nTJ(nb), nTK(nb);
+        for (int n = 0; n < nb; ++n) {
+            const BlockDev &B = c->blocks[n].dev;
+            nTJ[n] = (B.nj + kKbaTJ - 1) / kKbaTJ;
+            nTK[n] = (B.nk + kKbaTK - 1) / kKbaTK;
+            for (int J = 0; J < nTJ[n]; ++J)
+                for (int K = 0; K < nTK[n]; ++K) lst.push_back({n, J, K, J + K});
+        }
+        for (int dir = 0; dir < 2; ++dir) {
+            std::vector<T> o = lst;
+            if (dir == 1)
+                for (T &t : o) t.diag = (nTJ[t.n] - 1 - t.J) + (nTK[t.n] - 1 - t.K);
+            std::stable_sort(o.begin(), o.end(), [](const T &a, const T &b) { return a.diag < b.diag; });
+            std::vector<std::vector<int>> tix(nb);
+            for (int n = 0; n < nb; ++n) tix[n].assign((size_t)nTJ[n] * nTK[n], -1);
+            for (size_t q = 0; q < o.size(); ++q) tix[o[q].n][(size_t)o[q].J * nTK[o[q].n] + o[q].K] = (int)q;
+            std::vector<KbaTile> kt;
+            for (const T &t : o) {
+                int dj = dir == 0 ? t.J - 1 : t.J + 1, dk = dir == 0 ? t.K - 1 : t.K + 1;
+                KbaTile x;
+                x.block = t.n; x.j0 = t.J * kKbaTJ; x.k0 = t.K * kKbaTK;
+                x.dep_j = (dj >= 0 && dj < nTJ[t.n]) ? tix[t.n][(size_t)dj * nTK[t.n] + t.K] : -1;
+                x.dep_k = (dk >= 0 && dk < nTK[t.n]) ? tix[t.n][(size_t)t.J * nTK[t.n] + dk] : -1;
+                kt.push_back(x);
+            }
+            if ((rc = dalloc(c, &c->d_kba[dir], kt.size()))) return rc;
+            HIPCK(c, hipMemcpyAsync(c->d_kba[dir], kt.data(), kt.size() * sizeof(KbaTile), hipMemcpyHostToDevice, c->stream));
+        }
+        c->nkba = (int)lst.size();
+        if ((rc = dalloc(c, &c->d_kba_sync, 3 + 2 * (size_t)c->nkba))) return rc;
+        c->kba_grid = std::min(c->nkba, 4096);
+    }
+    c->npart = (int)((c->ncell + 255) / 256);
+    if ((rc = dalloc(c, &c->d_partial, 5 * (size_t)c->npart))) return rc;
+    if ((rc = dalloc(c, &c->d_sums, 6))) return rc;
+    if ((rc = dalloc(c, &c->d_stage_row, 7))) return rc;
+    // device scalars
+    ScalarsDev sc;
+    memset(&sc, 0, sizeof sc);
+    sc.cfl = (float)c->cfg.cfl;
+    sc.cfl_res = F32C(1.0E-3);   // source/Solver.cpp:21
+    sc.total_vol = c->total_vol;
+    sc.dens_ref = c->dens_ref; sc.vel_ref = c->vel_ref; sc.temp_ref = c->temp_ref;
+    sc.nblade_in = c->blocks[secs[0].x].nblade;
+    sc.nblade_out = c->blocks[secs[1].x].nblade;
+    sc.p_out = c->gas.p_out;
+    if ((rc = dalloc(c, &c->d_sc, 1))) return rc;
+    HIPCK(c, hipMemcpyAsync(c->d_sc, &sc, sizeof sc, hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    c->ready = true;
+    return FV3D_OK;
+}
+
+int fv3d_state_upload(fv3d_ctx *c, int block, const double *pv, const double *ener)
+{
+    if (!c || !pv || !ener) return FV3D_EINVAL;
+    if (!c->ready) return fail(c, FV3D_ESTATE, "state_upload before setup");
+    if (block < 1 || block > (int)c->blocks.size()) return fail(c, FV3D_EINVAL, "block %d", block);
+    const BlockDev &B = c->blocks[block - 1].dev;
+    std::vector<double> Q(5 * B.box, 0.0), W(6 * B.box, 0.0);
+    HIPCK(c, hipMemcpy(Q.data(), B.Q, Q.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCK(c, hipMemcpy(W.data(), B.W, W.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 1; i <= B.ni; ++i)
+        for (int j = 1; j <= B.nj; ++j)
+            for (int k = 1; k <= B.nk; ++k) {
+                long s = ((long)(i - 1) * B.nj + (j - 1)) * B.nk + (k - 1);
+                long x = cidx(B, i, j, k);
+                const double *w = pv + 6 * s;
+                // source/UserInput.cpp:1139-1150
+                Q[x] = w[0]; Q[B.box + x] = w[0] * w[1]; Q[2 * B.box + x] = w[0] * w[2]; Q[3 * B.box + x] = w[0] * w[3];
+                Q[4 * B.box + x] = ener[s];
+                for (int q = 0; q < 6; ++q) W[q * B.box + x] = w[q];
+            }
+    HIPCK(c, hipMemcpyAsync(B.Q, Q.data(), Q.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(B.W, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return FV3D_OK;
+}
+
+// ------------------------------------------------------------- stages
+static int launch_bc(fv3d_ctx *c)
+{
+    if (c->nbc == 0) return FV3D_OK;
+    hipLaunchKernelGGL(k_bc, dim3((unsigned)((c->nbc + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks, c->d_patches,
+                       c->npatch_dev, c->d_pstart_bc, c->gas, &c->d_sc->p_out);
+    HIPCK(c, hipGetLastError());
+    return FV3D_OK;
+}
+
+static int launch_spectral_dt(fv3d_ctx *c)
+{
+    hipLaunchKernelGGL(k_spectral_dt, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
+                       c->d_cstart, (int)c->blocks.size(), c->gas, &c->d_sc->cfl);
+    HIPCK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_dt_flag_decrement, dim3(1), dim3(64), 0, c->stream, c->d_blocks, (int)c->blocks.size());
+    HIPCK(c, hipGetLastError());
+    return FV3D_OK;
+}
+
+static int launch_convective(fv3d_ctx *c)
+{
+    hipLaunchKernelGGL((k_convective<kTI, kTJ, kTK>), dim3((unsigned)c->ntiles), dim3(kTI * kTJ * kTK), 0, c->stream,
+                       c->d_blocks, c->d_tiles, c->gas, c->store_faces, c->d_faces[0], c->d_faces[1], c->d_faces[2]);
+    HIPCK(c, hipGetLastError());
+    return FV3D_OK;
+}
+
+static int launch_sweeps_kba(fv3d_ctx *c)
+{
+    long nsync = 2 + 2L * c->nkba;
+    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((nsync + 255) / 256)), dim3(256), 0, c->stream, c->d_kba_sync + 1, nsync, 0);
+    int *err = c->d_kba_sync, *tf = c->d_kba_sync + 1, *tb = c->d_kba_sync + 2;
+    int *df = c->d_kba_sync + 3, *db = df + c->nkba;
+    if (c->sweep_mode == 2) {
+        hipLaunchKernelGGL((k_lusgs_kba2<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(kKbaTJ * kKbaTK), 0,
+                           c->stream, c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas);
+        hipLaunchKernelGGL((k_lusgs_kba2<false, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(kKbaTJ * kKbaTK), 0,
+                           c->stream, c->d_blocks, c->d_kba[1], c->nkba, tb, db, err, c->gas.relax, c->gas);
+    } else {
+        hipLaunchKernelGGL((k_lusgs_kba<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(kKbaTJ * kKbaTK), 0,
+                           c->stream, c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_chunk);
+        hipLaunchKernelGGL((k_lusgs_kba<false, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(kKbaTJ * kKbaTK), 0,
+                           c->stream, c->d_blocks, c->d_kba[1], c->nkba, tb, db, err, c->gas.relax, c->gas, c->kba_chunk);
+    }
+    HIPCK(c, hipGetLastError());
+    return FV3D_OK;
+}
+
+static int launch_sweeps(fv3d_ctx *c)
+{
+    if (c->sweep_mode >= 1) return launch_sweeps_kba(c);
+    unsigned g = (unsigned)((c->nplane_threads + 255) / 256);
+    int nb = (int)c->blocks.size();
+    for (int s = 0; s < c->nplanes; ++s)
+        hipLaunchKernelGGL(k_lusgs_plane<true>, dim3(g), dim3(256), 0, c->stream, c->d_blocks, c->d_pstart_plane, nb, s,
+                           c->gas.relax, c->gas);
+    for (int s = 0; s < c->nplanes; ++s)
+        hipLaunchKernelGGL(k_lusgs_plane<false>, dim3(g), dim3(256), 0, c->stream, c->d_blocks, c->d_pstart_plane, nb, s,
+                           c->gas.relax, c->gas);
+    HIPCK(c, hipGetLastError());
+    return FV3D_OK;
+}
+
+static int launch_update(fv3d_ctx *c)
+{
+    int nb = (int)c->blocks.size();
+    hipLaunchKernelGGL(k_update, dim3((unsigned)c->npart), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb, c->gas,
+                       c->d_partial, c->d_bad);
+    HIPCK(c, hipGetLastError());
+    if (c->cfg.limiter) {
+        hipLaunchKernelGGL(k_limiter_serial, dim3(1), dim3(64), 0, c->stream, c->d_blocks, nb, c->gas, c->d_bad);
+        HIPCK(c, hipGetLastError());
+    }
+    return FV3D_OK;
+}
+
+static int launch_finalize(fv3d_ctx *c, double *hist_row)
+{
+    hipLaunchKernelGGL(k_sections, dim3(2), dim3(256), 0, c->stream, c->d_blocks, c->d_secs, c->gas, c->d_sums);
+    HIPCK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->d_partial, c->npart, c->d_sums, c->d_sc, c->gas,
+                       hist_row);
+    HIPCK(c, hipGetLastError());
+    return FV3D_OK;
+}
+
+#define READY(c)                                                       \
+    do {                                                               \
+        if (!(c)) return FV3D_EINVAL;                                  \
+        if (!(c)->ready) return fail((c), FV3D_ESTATE, "not set up"); \
+    } while (0)
+
+int fv3d_bc_apply(fv3d_ctx *c) { READY(c); return launch_bc(c); }
+int fv3d_spectral(fv3d_ctx *c) { READY(c); return launch_spectral_dt(c); }
+int fv3d_timestep(fv3d_ctx *c) { READY(c); return FV3D_OK; }   // fused into fv3d_spectral
+int fv3d_convective(fv3d_ctx *c) { READY(c); return launch_convective(c); }
+
+int fv3d_resflux(fv3d_ctx *c)
+{
+    READY(c);
+    int rc;
+    if ((rc = launch_bc(c))) return rc;
+    if ((rc = launch_spectral_dt(c))) return rc;
+    return launch_convective(c);
+}
+
+// The wavefront sweep bounds every wait; a wait that ran out sets the error
+// word instead of hanging the device.
+static int check_sweep_error(fv3d_ctx *c)
+{
+    int e = 0;
+    HIPCK(c, hipMemcpy(&e, c->d_kba_sync, sizeof(int), hipMemcpyDeviceToHost));
+    if (e) {
+        int z = 0;
+        HIPCK(c, hipMemcpy(c->d_kba_sync, &z, sizeof(int), hipMemcpyHostToDevice));
+        return fail(c, FV3D_ESTATE, "wavefront sweep: a progress wait exceeded its bound");
+    }
+    return FV3D_OK;
+}
+
+static int lusgs_timed(fv3d_ctx *c)
+{
+    int rc;
+    if (c->timing) HIPCK(c, hipEventRecord(c->ev[0], c->stream));
+    if ((rc = launch_bc(c))) return rc;
+    if ((rc = launch_spectral_dt(c))) return rc;
+    if (c->timing) HIPCK(c, hipEventRecord(c->ev[1], c->stream));
+    if ((rc = launch_convective(c))) return rc;
+    if (c->timing) HIPCK(c, hipEventRecord(c->ev[2], c->stream));
+    if ((rc = launch_sweeps(c))) return rc;
+    if (c->timing) HIPCK(c, hipEventRecord(c->ev[3], c->stream));
+    if ((rc = launch_update(c))) return rc;
+    if (c->timing) HIPCK(c, hipEventRecord(c->ev[4], c->stream));
+    return FV3D_OK;
+}
+
+int fv3d_lusgs(fv3d_ctx *c)
+{
+    READY(c);
+    int rc = lusgs_timed(c);
+    if (rc) return rc;
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return check_sweep_error(c);
+}
+
+int fv3d_res_calculate(fv3d_ctx *c, double *res6)
+{
+    READY(c);
+    std::vector<double> part(5 * (size_t)c->npart);
+    HIPCK(c, hipMemcpyAsync(part.data(), c->d_partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    for (int q = 0; q < 5; ++q) {
+        double s = 0.0;
+        for (int p = 0; p < c->npart; ++p) s += part[(size_t)q * c->npart + p];
+        if (res6) res6[q] = s / c->total_vol;
+    }
+    if (res6) res6[5] = 0.0;
+    return FV3D_OK;
+}
+
+int fv3d_perf(fv3d_ctx *c, double *out5)
+{
+    READY(c);
+    hipLaunchKernelGGL(k_sections, dim3(2), dim3(256), 0, c->stream, c->d_blocks, c->d_secs, c->gas, c->d_sums);
+    HIPCK(c, hipGetLastError());
+    double s[6];
+    HIPCK(c, hipMemcpyAsync(s, c->d_sums, sizeof s, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    ScalarsDev sc;
+    HIPCK(c, hipMemcpy(&sc, c->d_sc, sizeof sc, hipMemcpyDeviceToHost));
+    const double Gamma = c->gas.gamma, Rgas = c->gas.rgas;
+    double mi = s[0] * sc.dens_ref * sc.vel_ref * sc.nblade_in;
+    double mo = s[3] * sc.dens_ref * sc.vel_ref * sc.nblade_out;
+    double pti = pow(s[1] / s[0], Gamma / (Gamma - F32C(1.0))) * sc.dens_ref * sc.vel_ref * sc.vel_ref;
+    double pto = pow(s[4] / s[3], Gamma / (Gamma - F32C(1.0))) * sc.dens_ref * sc.vel_ref * sc.vel_ref;
+    double tti = s[2] / s[0] * sc.temp_ref, tto = s[5] / s[3] * sc.temp_ref;
+    double PR = pto / pti;
+    double L_adk = Gamma / (Gamma - F32C(1.0)) * Rgas * tti * (pow(PR, (Gamma - F32C(1.0)) / Gamma) - F32C(1.0));
+    double L_k = Gamma / (Gamma - F32C(1.0)) * Rgas * tti * (tto / tti - F32C(1.0));
+    if (out5) {
+        out5[0] = mi; out5[1] = mo; out5[2] = PR; out5[3] = L_adk / L_k; out5[4] = fabs((mi - mo) / mi);
+    }
+    return FV3D_OK;
+}
+
+static int ensure_hist(fv3d_ctx *c, long n)
+{
+    if (n <= c->hist_cap) return FV3D_OK;
+    if (c->d_hist) HIPCK(c, hipFree(c->d_hist));
+    c->d_hist = nullptr;
+    c->hist_cap = 0;
+    int rc = dalloc(c, &c->d_hist, 7 * (size_t)n);
+    if (rc) return rc;
+    c->hist_cap = n;
+    return FV3D_OK;
+}
+
+static int accumulate_timing(fv3d_ctx *c)
+{
+    HIPCK(c, hipEventSynchronize(c->ev[5]));
+    float ms;
+    const int pairs[4][2] = {{2, 3}, {3, 4}, {0, 1}, {4, 5}};   // convective, sweeps, update, bc+spectral... finalize
+    // class 0 residual, 1 sweeps, 2 update, 3 bc+spectral+dt, 4 finalize
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[2])); c->t_ms[0] += ms; c->t_n[0]++;
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3])); c->t_ms[1] += ms; c->t_n[1]++;
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev[3], c->ev[4])); c->t_ms[2] += ms; c->t_n[2]++;
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); c->t_ms[3] += ms; c->t_n[3]++;
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev[4], c->ev[5])); c->t_ms[4] += ms; c->t_n[4]++;
+    (void)pairs;
+    return FV3D_OK;
+}
+
+int fv3d_iterate(fv3d_ctx *c, int n, double *hist)
+{
+    READY(c);
+    if (n < 0) return fail(c, FV3D_EINVAL, "n=%d", n);
+    int rc;
+    if ((rc = ensure_hist(c, std::max(n, 1)))) return rc;
+    for (int it = 0; it < n; ++it) {
+        if ((rc = lusgs_timed(c))) return rc;
+        if ((rc = launch_finalize(c, c->d_hist + 7L * it))) return rc;
+        if (c->timing) {
+            HIPCK(c, hipEventRecord(c->ev[5], c->stream));
+            if ((rc = accumulate_timing(c))) return rc;
+        }
+    }
+    if (hist && n > 0)
+        HIPCK(c, hipMemcpyAsync(hist, c->d_hist, 7L * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    if ((rc = check_sweep_error(c))) return rc;
+    if (hist && n > 0 && !isfinite(hist[7L * (n - 1) + 1])) return fail(c, FV3D_ENUMERIC, "residual is not finite");
+    return FV3D_OK;
+}
+
+int fv3d_iterate_graph(fv3d_ctx *c, int n, double *hist)
+{
+    READY(c);
+    if (n < 0) return fail(c, FV3D_EINVAL, "n=%d", n);
+    int rc;
+    if ((rc = ensure_hist(c, std::max(n, 1)))) return rc;
+    // the graph writes its history row into a fixed staging slot that is
+    // copied into the history buffer after each replay
+    double *stage_row = c->d_stage_row;
+    if (!c->graph_exec) {
+        hipGraph_t g;
+        int saved = c->timing;
+        c->timing = 0;
+        HIPCK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = lusgs_timed(c);
+        if (!rc) rc = launch_finalize(c, stage_row);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        c->timing = saved;
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(c, FV3D_EHIP, "graph capture: %s", hipGetErrorString(e));
+        HIPCK(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+        HIPCK(c, hipGraphDestroy(g));
+    }
+    for (int it = 0; it < n; ++it) {
+        HIPCK(c, hipGraphLaunch(c->graph_exec, c->stream));
+        HIPCK(c, hipMemcpyAsync(c->d_hist + 7L * it, stage_row, 7 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (hist && n > 0)
+        HIPCK(c, hipMemcpyAsync(hist, c->d_hist, 7L * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    if ((rc = check_sweep_error(c))) return rc;
+    if (hist && n > 0 && !isfinite(hist[7L * (n - 1) + 1])) return fail(c, FV3D_ENUMERIC, "residual is not finite");
+    return FV3D_OK;
+}
+
+int fv3d_sync(fv3d_ctx *c)
+{
+    if (!c) return FV3D_EINVAL;
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return FV3D_OK;
+}
+
+int fv3d_set_timing(fv3d_ctx *c, int on)
+{
+    if (!c) return FV3D_EINVAL;
+    c->timing = on;
+    for (int q = 0; q < 5; ++q) { c->t_ms[q] = 0.0; c->t_n[q] = 0; }
+    return FV3D_OK;
+}
+
+int fv3d_timing(fv3d_ctx *c, double *ms5, long *counts5)
+{
+    if (!c) return FV3D_EINVAL;
+    for (int q = 0; q < 5; ++q) {
+        if (ms5) ms5[q] = c->t_n[q] ? c->t_ms[q] / c->t_n[q] : 0.0;
+        if (counts5) counts5[q] = c->t_n[q];
+    }
+    return FV3D_OK;
+}
+
+long fv3d_cell_count(const fv3d_ctx *c) { return c ? c->ncell : 0; }
+
+int fv3d_set_debug(fv3d_ctx *c, int flags)
+{
+    READY(c);
+    c->store_faces = flags & 1;
+    c->sweep_mode = (flags & 2) ? 0 : (flags & 4) ? 1 : 2;
+    if ((flags >> 8) & 0xff) c->kba_chunk = (flags >> 8) & 0xff;
+    if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
+    if (c->store_faces && !c->d_faces[0]) {
+        long mx = 0;
+        for (auto &b : c->blocks) mx = std::max(mx, b.dev.box);
+        if (c->blocks.size() != 1) return fail(c, FV3D_EINVAL, "face-flux capture supports one block");
+        for (int d = 0; d < 3; ++d) {
+            int rc = dalloc(c, &c->d_faces[d], 5 * (size_t)mx);
+            if (rc) return rc;
+        }
+    }
+    return FV3D_OK;
+}
+
+long fv3d_field_download(fv3d_ctx *c, int block, const char *name, double *out)
+{
+    READY(c);
+    if (!name || !out) return FV3D_EINVAL;
+    if (block < 1 || block > (int)c->blocks.size()) return fail(c, FV3D_EINVAL, "block %d", block);
+    const BlockDev &B = c->blocks[block - 1].dev;
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    auto fetch = [&](const double *src, long n, std::vector<double> &dst) -> int {
+        dst.resize(n);
+        HIPCK(c, hipMemcpy(dst.data(), src, n * sizeof(double), hipMemcpyDeviceToHost));
+        return 0;
+    };
+    std::vector<double> h;
+    long cnt = 0;
+    std::string nm(name);
+    if (nm == "cv" || nm == "pv") {
+        int C = nm == "cv" ? 5 : 6;
+        if (fetch(nm == "cv" ? B.Q : B.W, C * B.box, h)) return FV3D_EHIP;
+        for (int i = 0; i <= B.IM; ++i)
+            for (int j = 0; j <= B.JM; ++j)
+                for (int k = 0; k <= B.KM; ++k)
+                    for (int q = 0; q < C; ++q) out[cnt++] = h[q * B.box + cidx(B, i, j, k)];
+        return cnt;
+    }
+    const double *src = nullptr;
+    int C = 0, hiI = 0, hiJ = 0, hiK = 0;
+    if (nm == "res") { src = B.R; C = 5; }
+    else if (nm == "dq") { src = B.dQ; C = 5; }
+    else if (nm == "dt") { src = B.dt; C = 1; }
+    else if (nm == "vol") { src = B.vol; C = 1; }
+    else if (nm == "lamcI") { src = B.lamc; C = 1; }
+    else if (nm == "lamcJ") { src = B.lamc + B.box; C = 1; }
+    else if (nm == "lamcK") { src = B.lamc + 2 * B.box; C = 1; }
+    else if (nm == "fcI" || nm == "fcJ" || nm == "fcK") {
+        int d = nm[2] - 'I';
+        if (!c->d_faces[d]) return fail(c, FV3D_ESTATE, "face fluxes were not captured (fv3d_set_debug)");
+        src = c->d_faces[d]; C = 5; hiI = d == 0; hiJ = d == 1; hiK = d == 2;
+    }
+    if (src) {
+        if (fetch(src, (long)C * B.box, h)) return FV3D_EHIP;
+        for (int i = 1; i <= B.ni + hiI; ++i)
+            for (int j = 1; j <= B.nj + hiJ; ++j)
+                for (int k = 1; k <= B.nk + hiK; ++k)
+                    for (int q = 0; q < C; ++q) out[cnt++] = h[q * B.box + cidx(B, i, j, k)];
+        return cnt;
+    }
+    static const char *dn[6] = {"dIS", "dIP", "dJS", "dJP", "dKS", "dKP"};
+    for (int s = 0; s < 6; ++s) {
+        if (nm.compare(0, 3, dn[s]) != 0) continue;
+        bool pvq = nm.size() > 3 && nm.substr(3) == "_pv";
+        int C2 = pvq ? 6 : 5, c0 = pvq ? 5 : 0;
+        if (fetch(B.D[s], 22 * B.Dt[s], h)) return FV3D_EHIP;
+        int d = s / 2;
+        int ra = d == 0 ? 2 : B.ni, rb = d == 0 ? B.nj : d == 1 ? 2 : B.nj, rc = d == 2 ? 2 : B.nk;
+        for (int a = 1; a <= ra; ++a)
+            for (int b2 = 1; b2 <= rb; ++b2)
+                for (int c2 = 1; c2 <= rc; ++c2) {
+                    int layer = d == 0 ? a : d == 1 ? b2 : c2;
+                    int ta = d == 0 ? b2 : a, tb = d == 2 ? b2 : c2;
+                    for (int q = 0; q < C2; ++q)
+                        out[cnt++] = h[((long)(layer - 1) * 11 + c0 + q) * B.Dt[s] + (long)ta * B.Db[s] + tb];
+                }
+        return cnt;
+    }
+    return fail(c, FV3D_EINVAL, "unknown field %s", name);
+}
+
+}  // extern "C"

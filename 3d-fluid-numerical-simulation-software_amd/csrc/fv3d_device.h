@@ -1,0 +1,242 @@
+// Device-side data layout and per-face / per-cell physics of the LU-SGS
+// iteration, written for gfx950 (CDNA4).
+//
+// Layout (per block, structure of arrays, k fastest):
+//   cell-centred arrays cover the reference's index box [0..IM]x[0..JM]x[0..KM]
+//   (ghost cells 0 and IM included), component c of field F at F[c*box + idx];
+//   face-centred arrays use the same box (face i of the I direction at index i);
+//   the NDummy=2 layers of each of the six sides live in their own slabs,
+//   11 components (5 conservative then 6 primitive) per layer.
+//
+// All arithmetic follows the reference's expression order; the library is
+// compiled with -ffp-contract=off so FP64 results match the CPU oracle except
+// where libm pow (MUSCL3 limiter) rounds differently from a*a.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fv3d {
+
+constexpr int kND = 2;        // Dummy layers for the upwind schemes (source/UserInput.cpp:132-155)
+constexpr int kMaxBlocks = 64;
+
+// float literals of the reference widened to double (include/defs.h keeps
+// many constants as float literals even when REAL is double)
+#define F32C(x) ((double)(x##f))
+
+struct BlockDev {
+    int IM, JM, KM;           // node counts
+    int ni, nj, nk;           // cell counts = IM-1, JM-1, KM-1
+    long box;                 // (IM+1)(JM+1)(KM+1)
+    int sj, si;               // index strides: k 1, j sj=(KM+1), i si=(JM+1)(KM+1)
+    int nblade;
+    double w_rot;             // non-dimensional rotation rate
+    double *Q;                // 5*box conservative
+    double *W;                // 6*box primitive rho,u,v,w,p,T
+    double *D[6];             // Dummy slabs IS IP JS JP KS KP
+    long Dt[6];               // tangential box size of each slab
+    int Db[6];                // stride of the first tangential index
+    const double *fS[3];      // face vector, 3*box per direction
+    const double *fs[3];      // face area |S|
+    const double *fc[3];      // face centroid, 3*box
+    const double *vol;        // cell volume
+    const double *cc;         // cell centroid, 3*box
+    const double *node;       // node coordinates, 3*box
+    const unsigned char *wmask[6];  // wall mask per side (tangential box)
+    const double *wrot[6];          // wall rotation rate per side
+    double *lamc;             // 3*box convective spectral radii
+    double *dt;               // local time step
+    double *alpha;            // LU-SGS diagonal Vol/dt + relax*sum(lamc) + 2*sum(lamv)
+    const double *swg;        // 15*box static sweep metrics per cell: for d = 0..2,
+                              // averaged unit normal (3), averaged area, grid normal velocity
+    double *R;                // 5*box residual
+    double *dQ;               // 5*box LU-SGS increment
+    int *flag_low_dt;         // time-step damping counter (source/Limit_FlowVars.cpp:117)
+};
+
+__host__ __device__ inline long cidx(const BlockDev &B, int i, int j, int k)
+{
+    return (long)i * B.si + (long)j * B.sj + k;
+}
+
+// Dummy slab address of component c (0-4 cv, 5-10 pv) of layer l (1..2) on side s
+// at tangential indices (a, b): I sides (j,k), J sides (i,k), K sides (i,j).
+__device__ inline double *dummy_ptr(const BlockDev &B, int s, int l, int c, int a, int b)
+{
+    return B.D[s] + ((long)(l - 1) * 11 + c) * B.Dt[s] + (long)a * B.Db[s] + b;
+}
+
+struct Gas {
+    double gamma, rgas;
+    double pt_in, tt_in, p_out;
+    double inlet_dir[3];
+    double c_timestep, relax;
+    int recon_var, recon_scheme, flux_scheme;
+};
+
+// ---------------------------------------------------------------- minmod
+__device__ inline double mm1(double a, double b)
+{
+    if (a * b < 0.0) return 0.0;
+    if (fabs(a) <= fabs(b)) return a;
+    return b;
+}
+
+// -------------------------------------------------------- reconstruction
+// One component of the primitive-variable reconstruction
+// (source/Reconstruction.cpp:70-245).  Returns left and right face values.
+__device__ inline void recon1_prim(int scheme, double FF, double F, double B, double BB, double &L, double &R)
+{
+    const double eps = F32C(1E-6);
+    if (scheme == 500) {
+        double dF = F - FF, dB = B - F, dR = BB - B;
+        double SL = ((2.0 * dF) * dB + eps) / ((dF * dF + dB * dB) + eps);
+        double SR = ((2.0 * dR) * dB + eps) / ((dR * dR + dB * dB) + eps);
+        L = F + (SL / 4.0) * ((1.0 - SL / 3.0) * dF + (SL / 3.0 + 1.0) * dB);
+        R = B - (SR / 4.0) * ((1.0 - SR / 3.0) * dR + (SR / 3.0 + 1.0) * dB);
+    } else if (scheme == 100) {
+        L = F + 0.5 * mm1(B - F, F - FF);
+        R = B - 0.5 * mm1(B - F, BB - B);
+    } else if (scheme == 200) {
+        const double k2u = -1.0;
+        L = F + F32C(0.25) * ((F32C(1.0) - k2u) * mm1(F32C(2.0) * (B - F), F - FF) +
+                              (F32C(1.0) + k2u) * mm1(F32C(2.0) * (F - FF), B - F));
+        R = B - F32C(0.25) * ((F32C(1.0) - k2u) * mm1(F32C(2.0) * (B - F), BB - B) +
+                              (F32C(1.0) + k2u) * mm1(F32C(2.0) * (BB - B), B - F));
+    } else if (scheme == 300) {
+        const double k2c = 1.0;
+        L = F + F32C(0.25) * ((F32C(1.0) - k2c) * mm1(F32C(2.0) * (B - F), F - FF) +
+                              (F32C(1.0) + k2c) * mm1(B - F, 2.0 * (F - FF)));
+        R = B - F32C(0.25) * ((F32C(1.0) - k2c) * mm1(F32C(2.0) * (B - F), BB - B) +
+                              (F32C(1.0) + k2c) * mm1(B - F, 2.0 * (BB - B)));
+    } else {  // 400 MUSCL2o
+        const double eta = F32C(0.2);
+        const double a0 = F32C(1.0) - eta, a1 = (eta - F32C(0.55)) / F32C(2.0), a2 = (eta + F32C(0.55)) / F32C(2.0);
+        double r1 = ((F - FF) + eps) / ((B - F) + eps);
+        double r2 = ((B - F) + eps) / ((BB - B) + eps);
+        double f1 = (1.0 * (a1 / r2) + a0) + a2 * r1;
+        double m = fmin(fmin(F32C(2.0), f1), F32C(2.0) * r1);
+        double f = F32C(0.0) > m ? F32C(0.0) : m;
+        L = F + (F32C(0.5) * f) * (B - F);
+        r1 = ((BB - B) + eps) / ((B - F) + eps);
+        r2 = ((B - F) + eps) / ((F - FF) + eps);
+        f1 = (1.0 * (a1 / r2) + a0) + a2 * r1;
+        m = fmin(fmin(F32C(2.0), f1), F32C(2.0) * r1);
+        f = F32C(0.0) > m ? F32C(0.0) : m;
+        R = B - (F32C(0.5) * f) * (B - F);
+    }
+}
+
+// One component of the conservative-variable reconstruction
+// (source/Reconstruction.cpp:7-60).
+__device__ inline void recon1_cons(int scheme, double FF, double F, double B, double BB, double &L, double &R)
+{
+    const double eps = F32C(1E-6);
+    if (scheme == 500) {
+        double dF = F - FF, dB = B - F, dR = BB - B;
+        double SL = ((dF * F32C(2.0)) * dB + eps) / ((dF * dF + dB * dB) + eps);
+        double SR = ((dR * F32C(2.0)) * dB + eps) / ((dR * dR + dB * dB) + eps);
+        L = F + (SL / F32C(4.0)) * ((F32C(1.0) - SL / F32C(3.0)) * dF + (SL / F32C(3.0) + F32C(1.0)) * dB);
+        R = B - (SR / F32C(4.0)) * ((F32C(1.0) - SR / F32C(3.0)) * dR + (SR / F32C(3.0) + F32C(1.0)) * dB);
+    } else if (scheme == 100) {
+        L = F + mm1(B - F, F - FF) * F32C(0.5);
+        R = B - mm1(B - F, BB - B) * F32C(0.5);
+    } else if (scheme == 200) {
+        const double k2u = F32C(-1.0);
+        L = F + (mm1((B - F) * F32C(2.0), F - FF) * (F32C(1.0) - k2u) +
+                 mm1((F - FF) * F32C(2.0), B - F) * (F32C(1.0) + k2u)) * 0.25;
+        R = B - (mm1((B - F) * F32C(2.0), BB - B) * (F32C(1.0) - k2u) +
+                 mm1((BB - B) * F32C(2.0), B - F) * (F32C(1.0) + k2u)) * 0.25;
+    } else if (scheme == 300) {
+        const double k2c = F32C(1.0);
+        L = F + (mm1((B - F) * F32C(2.0), F - FF) * (F32C(1.0) - k2c) +
+                 mm1(B - F, (F - FF) * F32C(2.0)) * (F32C(1.0) + k2c)) * 0.25;
+        R = B - (mm1((B - F) * F32C(2.0), BB - B) * (F32C(1.0) - k2c) +
+                 mm1(B - F, (BB - B) * F32C(2.0)) * (F32C(1.0) + k2c)) * 0.25;
+    } else {
+        const double eta = F32C(0.05);
+        const double a0 = F32C(1.0) - eta, a1 = (eta - F32C(0.55)) / F32C(2.0), a2 = (eta + F32C(0.55)) / F32C(2.0);
+        double r1 = ((F - FF) + eps) / ((B - F) + eps);
+        double r2 = ((B - F) + eps) / ((BB - B) + eps);
+        double f1 = (a1 / r2 + a0) + r1 * a2;
+        double m = fmin(fmin(F32C(2.0), f1), F32C(2.0) * r1);
+        double f = F32C(0.0) > m ? F32C(0.0) : m;
+        L = F + (f * F32C(0.5)) * (B - F);
+        r1 = ((BB - B) + eps) / ((B - F) + eps);
+        r2 = ((B - F) + eps) / ((F - FF) + eps);
+        f1 = (a1 / r2 + a0) + r1 * a2;
+        m = fmin(fmin(F32C(2.0), f1), F32C(2.0) * r1);
+        f = F32C(0.0) > m ? F32C(0.0) : m;
+        R = B - (f * F32C(0.5)) * (B - F);
+    }
+}
+
+// MIN3 of source/defs.cpp:401-408 keeps the first argument on ties and NaN;
+// fmin differs only for NaN operands, which the positivity fallback catches.
+
+// ---------------------------------------------------------- face fluxes
+// Roe flux in the face frame (source/Flux_Split_Scheme.cpp:568-635).
+__device__ inline void roe_flux(double Gamma, const double QL[5], const double QR[5], double que, double Fc[5])
+{
+    double densL = QL[0], uvelL = QL[1] / QL[0], vvelL = QL[2] / QL[0], wvelL = QL[3] / QL[0], EL = QL[4] / QL[0];
+    double pL = (Gamma - F32C(1.0)) * densL * (EL - F32C(0.5) * (uvelL * uvelL + vvelL * vvelL + wvelL * wvelL));
+    double HL = EL + pL / densL;
+    double densR = QR[0], uvelR = QR[1] / QR[0], vvelR = QR[2] / QR[0], wvelR = QR[3] / QR[0], ER = QR[4] / QR[0];
+    double pR = (Gamma - F32C(1.0)) * densR * (ER - F32C(0.5) * (uvelR * uvelR + vvelR * vvelR + wvelR * wvelR));
+    double HR = ER + pR / densR;
+    double FL0 = QL[0] * (uvelL - que), FL1 = QL[1] * (uvelL - que) + pL, FL2 = QL[2] * (uvelL - que),
+           FL3 = QL[3] * (uvelL - que), FL4 = (QL[4] + pL) * (uvelL - que) + que * pL;
+    double FR0 = QR[0] * (uvelR - que), FR1 = QR[1] * (uvelR - que) + pR, FR2 = QR[2] * (uvelR - que),
+           FR3 = QR[3] * (uvelR - que), FR4 = (QR[4] + pR) * (uvelR - que) + que * pR;
+    double tmpp = sqrt(densR / densL);
+    double uav = (uvelL + tmpp * uvelR) / (F32C(1.0) + tmpp);
+    double vav = (vvelL + tmpp * vvelR) / (F32C(1.0) + tmpp);
+    double wav = (wvelL + tmpp * wvelR) / (F32C(1.0) + tmpp);
+    double Hav = (HL + tmpp * HR) / (F32C(1.0) + tmpp);
+    double cav = sqrt((Gamma - F32C(1.0)) * (Hav - F32C(0.5) * (uav * uav + vav * vav + wav * wav)));
+    const double delta = F32C(0.5);
+    double l1 = fabs((uav - que) - cav), l2 = fabs(uav - que), l5 = fabs((uav - que) + cav);
+    l1 = l1 > delta ? l1 : F32C(0.5) * (l1 * l1 + delta * delta) / delta;
+    l2 = l2 > delta ? l2 : F32C(0.5) * (l2 * l2 + delta * delta) / delta;
+    l5 = l5 > delta ? l5 : F32C(0.5) * (l5 * l5 + delta * delta) / delta;
+    double a2 = (QR[2] - QL[2]) / (vav + F32C(1E-8)) - (densR - densL);
+    double a3 = (QR[3] - QL[3]) / (wav + F32C(1E-8)) - (densR - densL);
+    double a4 = (Gamma - F32C(1.0)) *
+                ((densR - densL) * (Hav - uav * uav - vav * vav - wav * wav) + uav * (QR[1] - QL[1]) +
+                 vav * (QR[2] - QL[2]) + wav * (QR[3] - QL[3]) - (QR[4] - QL[4])) / (cav * cav);
+    double a1 = ((uav + cav) * (densR - densL) - (QR[1] - QL[1]) - cav * a4) / (F32C(2.0) * cav);
+    double a5 = densR - densL - (a1 + a4);
+    Fc[0] = F32C(0.5) * (FL0 + FR0) - F32C(0.5) * (l1 * a1 + l2 * a4 + l5 * a5);
+    Fc[1] = F32C(0.5) * (FL1 + FR1) - F32C(0.5) * (l1 * a1 * (uav - cav) + l2 * a4 * uav + l5 * a5 * (uav + cav));
+    Fc[2] = F32C(0.5) * (FL2 + FR2) - F32C(0.5) * (l1 * a1 * vav + l2 * a2 * vav + l2 * a4 * vav + l5 * a5 * vav);
+    Fc[3] = F32C(0.5) * (FL3 + FR3) - F32C(0.5) * (l1 * a1 * wav + l2 * a3 * wav + l2 * a4 * wav + l5 * a5 * wav);
+    Fc[4] = F32C(0.5) * (FL4 + FR4) -
+            F32C(0.5) * (l1 * a1 * (Hav - uav * cav) + l2 * a2 * vav * vav + l2 * a3 * wav * wav +
+                         l2 * a4 * F32C(0.5) * (uav * uav + vav * vav + wav * wav) + l5 * a5 * (Hav + uav * cav));
+}
+
+// One side of the Steger-Warming splitting (source/Flux_Split_Scheme.cpp:82-149);
+// sgn = +1 for F+(Q_L), -1 for F-(Q_R).
+__device__ inline void sw_half(double Gamma, const double Q[5], double que, double sgn, double f[5])
+{
+    const double eps = F32C(1E-6);
+    double dens = Q[0], u = Q[1] / Q[0], v = Q[2] / Q[0], w = Q[3] / Q[0], e = Q[4] / Q[0];
+    double t = Gamma * (Gamma - F32C(1.0)) * (e - F32C(0.5) * (u * u + v * v + w * w));
+    t = sqrt(t * t);
+    double c = sqrt(t);
+    double l1 = F32C(0.5) * ((u - que) + sgn * sqrt((u - que) * (u - que) + eps * eps));
+    double l4 = F32C(0.5) * (((u - que) - c) + sgn * sqrt(((u - que) - c) * ((u - que) - c) + eps * eps));
+    double l5 = F32C(0.5) * (((u - que) + c) + sgn * sqrt(((u - que) + c) * ((u - que) + c) + eps * eps));
+    double W2 = (F32C(3.0) - Gamma) * (l4 + l5) * c * c / F32C(2.0) / (Gamma - F32C(1.0));
+    double A = (u - c) * (u - c) + v * v + w * w;
+    double Bq = (u + c) * (u + c) + v * v + w * w;
+    double pre = dens / F32C(2.0) / Gamma;
+    f[0] = pre * (F32C(2.0) * (Gamma - F32C(1.0)) * l1 + l4 + l5);
+    f[1] = pre * (F32C(2.0) * (Gamma - F32C(1.0)) * l1 * u + l4 * (u - c) + l5 * (u + c));
+    f[2] = pre * (F32C(2.0) * (Gamma - F32C(1.0)) * l1 * v + l4 * v + l5 * v);
+    f[3] = pre * (F32C(2.0) * (Gamma - F32C(1.0)) * l1 * w + l4 * w + l5 * w);
+    f[4] = pre * ((Gamma - F32C(1.0)) * l1 * (u * u + v * v + w * w) + F32C(0.5) * l4 * A + F32C(0.5) * l5 * Bq + W2);
+}
+
+}  // namespace fv3d

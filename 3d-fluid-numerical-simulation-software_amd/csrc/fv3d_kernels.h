@@ -1,0 +1,61 @@
+// Kernel declarations and the device-side descriptors they share with the
+// context code.
+#pragma once
+
+#include "fv3d_device.h"
+
+namespace fv3d {
+
+// convective-residual tile (cells per workgroup = kTI*kTJ*kTK)
+constexpr int kTI = 4, kTJ = 4, kTK = 16;
+
+struct PatchDev {
+    int kind;          // 0 inlet, 1 outlet, 2 wall
+    int block;         // 0-based block index
+    int dir, hi;       // side: direction 0/1/2 and high (1) or low (0) face
+    int a0, b0;        // first tangential cell indices
+    int na, nb;        // tangential extents
+    int centrifugal;   // outlet variant
+    double rot;        // wall rotation rate
+};
+
+// Per-run scalars kept on the device so iterations need no host round trip.
+struct ScalarsDev {
+    double cfl, cfl_res;
+    double res[5];
+    double total_vol;
+    double dens_ref, vel_ref, temp_ref;
+    double nblade_in, nblade_out;
+    double mass_in, mass_out, pr, eff;
+    double p_out;
+    double iter;
+};
+
+__global__ void k_bc(const BlockDev *blocks, const PatchDev *patches, int npatch, const long *pstart, Gas g,
+                     const double *pout_dev);
+__global__ void k_spectral_dt(const BlockDev *blocks, const long *cstart, int nblock, Gas g, const double *cfl_dev);
+__global__ void k_dt_flag_decrement(const BlockDev *blocks, int nblock);
+template <int TI, int TJ, int TK>
+__global__ void k_convective(const BlockDev *blocks, const int4 *tiles, Gas g, int store_faces, double *fI, double *fJ,
+                             double *fK);
+template <bool FORWARD>
+__global__ void k_lusgs_plane(const BlockDev *blocks, const long *pstart, int nblock, int step, double relax, Gas g);
+// one tile of the persistent wavefront sweep; dep_*: ticket of the upstream
+// tile along j / k, or -1 at the block edge
+struct KbaTile {
+    int block, j0, k0, dep_j, dep_k;
+};
+constexpr int kKbaTJ = 8, kKbaTK = 8;
+template <bool FWD, int TJ, int TK>
+__global__ void k_lusgs_kba(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *done, int *err,
+                            double relax, Gas g, int chunk);
+template <bool FWD, int TJ, int TK>
+__global__ void k_lusgs_kba2(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *done, int *err,
+                             double relax, Gas g);
+__global__ void k_fill_int(int *p, long n, int v);
+__global__ void k_update(const BlockDev *blocks, const long *cstart, int nblock, Gas g, double *partial, int *bad_flag);
+__global__ void k_limiter_serial(const BlockDev *blocks, int nblock, Gas g, int *bad_flag);
+__global__ void k_sections(const BlockDev *blocks, const int2 *secs, Gas g, double *sums);
+__global__ void k_finalize(const double *partial, int npart, const double *sums, ScalarsDev *sc, Gas g, double *hist_row);
+
+}  // namespace fv3d

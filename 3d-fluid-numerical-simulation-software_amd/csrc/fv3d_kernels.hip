@@ -1,0 +1,968 @@
+// HIP kernels of the LU-SGS iteration for gfx950.
+//
+// One launch per stage covers every block of the mesh through a work list, so
+// multi-block meshes cost no more launches than a single block.  Reductions
+// (residual norm, section integrals) are two-pass with a fixed summation order
+// and therefore deterministic run to run.
+#include "fv3d_kernels.h"
+
+#include <math.h>
+
+namespace fv3d {
+
+// ===================================================================== BC
+// One thread per patch face.  Inlet (source/BndCondInlet.cpp:7-127), general
+// and centrifugal outlet (source/BndCondOutlet.cpp:116-305), no-slip wall
+// (source/BndCondWall.cpp:20-284).  Writes ghost cells of Q/W and the two
+// Dummy layers of the patch side.
+__device__ static void set_dummies(const BlockDev &B, int s, int a, int b, const double cv[5], const double pv[6])
+{
+    for (int l = 1; l <= kND; ++l) {
+        for (int c = 0; c < 5; ++c) *dummy_ptr(B, s, l, c, a, b) = cv[c];
+        for (int c = 0; c < 6; ++c) *dummy_ptr(B, s, l, 5 + c, a, b) = pv[c];
+    }
+}
+
+__global__ void k_bc(const BlockDev *__restrict__ blocks, const PatchDev *__restrict__ patches, int npatch,
+                     const long *__restrict__ pstart, Gas g, const double *__restrict__ pout_dev)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= pstart[npatch]) return;
+    int q = 0;
+    while (t >= pstart[q + 1]) ++q;
+    const PatchDev P = patches[q];
+    const BlockDev &B = blocks[P.block];
+    long r = t - pstart[q];
+    int a = P.a0 + (int)(r / P.nb), b = P.b0 + (int)(r % P.nb);   // tangential cell indices
+    const int d = P.dir, hi = P.hi;
+    const int N = d == 0 ? B.IM : d == 1 ? B.JM : B.KM;
+    // interior cell next to the face and the ghost cell behind it
+    int ii, jj, kk, gi, gj, gk;
+    int pi = hi ? N - 1 : 1, pg = hi ? N : 0;
+    if (d == 0) { ii = pi; gi = pg; jj = gj = a; kk = gk = b; }
+    else if (d == 1) { jj = pi; gj = pg; ii = gi = a; kk = gk = b; }
+    else { kk = pi; gk = pg; ii = gi = a; jj = gj = b; }
+    const long xi = cidx(B, ii, jj, kk), xg = cidx(B, gi, gj, gk);
+    const long box = B.box;
+    const int s = 2 * d + hi;
+    const double Gamma = g.gamma, Rgas = g.rgas;
+    double cv[5], pv[6];
+    if (P.kind == 0) {
+        // inlet: K faces only
+        if (d != 2) return;
+        const int kf = hi ? N : 1;
+        const long xf = cidx(B, ii, jj, kf);
+        double Sx = B.fS[2][xf], Sy = B.fS[2][box + xf], Sz = B.fS[2][2 * box + xf];
+        double n_m = sqrt(Sx * Sx + Sy * Sy + Sz * Sz);
+        double uvel = B.W[1 * box + xi], vvel = B.W[2 * box + xi], wvel = B.W[3 * box + xi];
+        double vdn = uvel * Sx / n_m + vvel * Sy / n_m + wvel * Sz / n_m;
+        if (!hi) vdn = F32C(-1.0) * vdn;
+        const double *A = g.inlet_dir;
+        double Vn_m = sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
+        double c_d = sqrt(Gamma * Rgas * B.W[5 * box + xi]);
+        double V2_d = uvel * uvel + vvel * vvel + wvel * wvel;
+        double costh = F32C(-1.0) * vdn / sqrt(V2_d);
+        double c_02 = c_d * c_d + (Gamma - F32C(1.0)) * V2_d / F32C(2.0);
+        double R_minus = vdn - F32C(2.0) * c_d / (Gamma - F32C(1.0));
+        double meduA = (-R_minus * (Gamma - F32C(1.0))) / ((Gamma - F32C(1.0)) * costh * costh + F32C(2.0));
+        double meduB = sqrt(c_02 * ((Gamma - F32C(1.0)) * costh * costh + F32C(2.0)) / (Gamma - F32C(1.0)) /
+                                (R_minus * R_minus) - (Gamma - F32C(1.0)) / F32C(2.0));
+        double c_b = meduA * (F32C(1.0) + costh * meduB);
+        double Ttin = g.tt_in, Ptin = g.pt_in;
+        double TempBC = Ttin * (c_b * c_b / c_02);
+        if (TempBC > Ttin) TempBC = F32C(0.99) * Ttin;
+        double PressBC = Ptin * pow(TempBC / Ttin, Gamma / (Gamma - F32C(1.0)));
+        double DensBC = PressBC / Rgas / TempBC;
+        double VBC = sqrt(F32C(2.0) * Gamma / (Gamma - 1) * Rgas * (Ttin - TempBC));
+        double u = VBC * A[0] / Vn_m, v = VBC * A[1] / Vn_m, w = VBC * A[2] / Vn_m;
+        double V2 = u * u + v * v + w * w;
+        double ener = PressBC / DensBC / (Gamma - F32C(1.0)) + V2 / F32C(2.0);
+        pv[0] = DensBC; pv[1] = u; pv[2] = v; pv[3] = w; pv[4] = PressBC; pv[5] = TempBC;
+        cv[0] = DensBC; cv[1] = DensBC * u; cv[2] = DensBC * v; cv[3] = DensBC * w; cv[4] = DensBC * ener;
+    } else if (P.kind == 1) {
+        // general / centrifugal outlet, K faces only
+        if (d != 2) return;
+        const int kf = hi ? N : 1;
+        const long xf = cidx(B, ii, jj, kf);
+        double s_ = B.fs[2][xf];
+        double nx = B.fS[2][xf] / s_, ny = B.fS[2][box + xf] / s_, nz = B.fS[2][2 * box + xf] / s_;
+        double d0 = B.W[xi], u0 = B.W[box + xi], v0 = B.W[2 * box + xi], w0 = B.W[3 * box + xi];
+        double p0 = B.W[4 * box + xi], T0 = B.W[5 * box + xi];
+        double Pout = *pout_dev;
+        double dens0 = d0;
+        double c0 = sqrt(Gamma * Rgas * T0);
+        double Vn = u0 * nx + v0 * ny + w0 * nz;
+        double Ma_n = Vn / c0;
+        if (P.centrifugal || Ma_n <= 1.0) {
+            pv[4] = Pout;
+            pv[0] = d0 + (Pout - p0) / c0 / c0;
+            pv[1] = u0 + nx * (p0 - Pout) / dens0 / c0;
+            pv[2] = v0 + ny * (p0 - Pout) / dens0 / c0;
+            pv[3] = w0 + nz * (p0 - Pout) / dens0 / c0;
+        } else {
+            pv[4] = p0; pv[0] = d0; pv[1] = u0; pv[2] = v0; pv[3] = w0;
+        }
+        pv[5] = pv[4] / pv[0] / Rgas;
+        double V2 = pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3];
+        double ener = pv[4] / pv[0] / (Gamma - F32C(1.0)) + V2 / F32C(2.0);
+        cv[0] = pv[0]; cv[1] = pv[0] * pv[1]; cv[2] = pv[0] * pv[2]; cv[3] = pv[0] * pv[3]; cv[4] = pv[0] * ener;
+    } else {
+        // no-slip wall: mirror the relative velocity about the wall velocity
+        const int f = hi ? N : 1;
+        const long xf = d == 0 ? cidx(B, f, jj, kk) : d == 1 ? cidx(B, ii, f, kk) : cidx(B, ii, jj, f);
+        double x = B.fc[d][xf], y = B.fc[d][box + xf];
+        double rot = P.rot;
+        double Vwall_x = -rot * y, Vwall_y = rot * x;
+        double Vin_x = B.W[box + xi] - Vwall_x, Vin_y = B.W[2 * box + xi] - Vwall_y, Vin_z = B.W[3 * box + xi] - F32C(0.0);
+        double Vout_x = F32C(0.0) - Vin_x, Vout_y = F32C(0.0) - Vin_y, Vout_z = F32C(0.0) - Vin_z;
+        pv[0] = B.W[xi];
+        pv[1] = Vout_x + Vwall_x;
+        pv[2] = Vout_y + Vwall_y;
+        pv[3] = Vout_z + F32C(0.0);
+        pv[4] = B.W[4 * box + xi];
+        pv[5] = B.W[5 * box + xi];
+        cv[0] = B.Q[xi];
+        cv[1] = pv[1] * cv[0];
+        cv[2] = pv[2] * cv[0];
+        cv[3] = pv[3] * cv[0];
+        cv[4] = B.Q[4 * box + xi];
+        for (int c = 0; c < 5; ++c) B.Q[c * box + xg] = cv[c];
+        for (int c = 0; c < 6; ++c) B.W[c * box + xg] = pv[c];
+        double cd[5], pd[6];
+        for (int c = 0; c < 5; ++c) cd[c] = (cv[c] + B.Q[c * box + xi]) * 0.5;
+        for (int c = 0; c < 6; ++c) pd[c] = 0.5 * (pv[c] + B.W[c * box + xi]);
+        set_dummies(B, s, a, b, cd, pd);
+        return;
+    }
+    for (int c = 0; c < 5; ++c) B.Q[c * box + xg] = cv[c];
+    for (int c = 0; c < 6; ++c) B.W[c * box + xg] = pv[c];
+    set_dummies(B, s, a, b, cv, pv);
+}
+
+// ======================================================= spectral + dt
+// source/Spectral_curl_grad.cpp:9-42 and source/TimeStep.cpp:7-62 (Euler).
+__global__ void k_spectral_dt(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart, int nblock, Gas g,
+                              const double *__restrict__ cfl_dev)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cstart[nblock]) return;
+    int n = 0;
+    while (t >= cstart[n + 1]) ++n;
+    const BlockDev &B = blocks[n];
+    long r = t - cstart[n];
+    int k = (int)(r % B.nk) + 1;
+    r /= B.nk;
+    int j = (int)(r % B.nj) + 1;
+    int i = (int)(r / B.nj) + 1;
+    const long box = B.box, x = cidx(B, i, j, k);
+    const double Gamma = g.gamma, Rgas = g.rgas;
+    double uvelr = B.W[box + x] + B.w_rot * B.cc[box + x];
+    double vvelr = B.W[2 * box + x] - B.w_rot * B.cc[x];
+    double wvelr = B.W[3 * box + x];
+    double csoun = sqrt(Gamma * Rgas * B.W[5 * box + x]);
+    double lam[3];
+    const long xn[3] = {x + B.si, x + B.sj, x + 1};
+    for (int d = 0; d < 3; ++d) {
+        const double *S = B.fS[d];
+        double Sx = (S[x] + S[xn[d]]) * F32C(0.5);
+        double Sy = (S[box + x] + S[box + xn[d]]) * F32C(0.5);
+        double Sz = (S[2 * box + x] + S[2 * box + xn[d]]) * F32C(0.5);
+        double CS = csoun * sqrt(Sx * Sx + Sy * Sy + Sz * Sz);
+        lam[d] = fabs(uvelr * Sx + vvelr * Sy + wvelr * Sz) + CS;
+        B.lamc[d * box + x] = lam[d];
+    }
+    int flag = *B.flag_low_dt;
+    double dt_factor = flag == 0 ? F32C(1.0) : (double)(1.0f / (1.0f + (float)flag));
+    double lc = lam[0] + lam[1] + lam[2];
+    double lv = 0.0;
+    double dt = dt_factor * (*cfl_dev) * B.vol[x] / (lc + g.c_timestep * lv);
+    B.dt[x] = dt;
+    // diagonal of the LU-SGS operator (source/LU_SGS_Marching.cpp:76-77)
+    B.alpha[x] = B.vol[x] / dt + g.relax * (lam[0] + lam[1] + lam[2]) + F32C(2.0) * (0.0 + 0.0 + 0.0);
+}
+
+// Decrement of the time-step damping counter at the end of CTimeDiscr::TimeStep
+// (source/TimeStep.cpp:89-92), after every cell has read it.
+__global__ void k_dt_flag_decrement(const BlockDev *__restrict__ blocks, int nblock)
+{
+    int n = threadIdx.x;
+    if (n < nblock) {
+        int *f = blocks[n].flag_low_dt;
+        if (*f > 0) *f = *f - 1;
+    }
+}
+
+// =================================================== convective residual
+// Face flux of CSpaceDiscr::Upwind_Convective (source/Upwind_Convective.cpp:5-360)
+// with the WallFlux override (source/WallFlux.cpp:70-175).
+__device__ static void face_flux(const BlockDev &B, const Gas &g, int d, int i, int j, int k, double out[5])
+{
+    const long box = B.box;
+    const double Gamma = g.gamma;
+    const int N = d == 0 ? B.IM : d == 1 ? B.JM : B.KM;
+    const int p = d == 0 ? i : d == 1 ? j : k;
+    const long xf = cidx(B, i, j, k);
+    if (p == 1 || p == N) {
+        const int s = 2 * d + (p == N);
+        const int ta = d == 0 ? j : i, tb = d == 2 ? j : k;
+        const long tix = (long)ta * B.Db[s] + tb;
+        if (B.wmask[s] != nullptr && B.wmask[s][tix]) {
+            const double rot = B.wrot[s][tix];
+            const long xb = xf - (d == 0 ? B.si : d == 1 ? B.sj : 1);
+            double Sx = B.fS[d][xf], Sy = B.fS[d][box + xf], Sz = B.fS[d][2 * box + xf];
+            double fd = F32C(0.5) * (B.W[xf] + B.W[xb]);
+            double fu = F32C(0.5) * (B.W[box + xf] + B.W[box + xb]);
+            double fv = F32C(0.5) * (B.W[2 * box + xf] + B.W[2 * box + xb]);
+            double fw = F32C(0.5) * (B.W[3 * box + xf] + B.W[3 * box + xb]);
+            double fp = F32C(0.5) * (B.W[4 * box + xf] + B.W[4 * box + xb]);
+            double fue = F32C(-1.0) * B.fc[d][box + xf] * rot, fve = B.fc[d][xf] * rot;
+            double c1 = fd * fu, c2 = fd * fv, c3 = fd * fw;
+            double Vn_a = fu * Sx + fv * Sy + fw * Sz;
+            double Vn_e = fue * Sx + fve * Sy + F32C(0.0) * Sz;
+            double c4 = fp / (Gamma - F32C(1.0)) + fd * (fu * fu + fv * fv + fw * fw) / F32C(2.0);
+            out[0] = fd * (Vn_a - Vn_e);
+            out[1] = c1 * (Vn_a - Vn_e) + fp * Sx;
+            out[2] = c2 * (Vn_a - Vn_e) + fp * Sy;
+            out[3] = c3 * (Vn_a - Vn_e) + fp * Sz;
+            out[4] = (c4 + fp) * (Vn_a - Vn_e) + fp * Vn_e;
+            return;
+        }
+    }
+    // four-cell stencil along d; cells outside [1, N-1] come from the Dummy layers
+    const bool prim = g.recon_var == 200;
+    double st[4][5];
+    const long dstride = d == 0 ? B.si : d == 1 ? B.sj : 1;
+    const double *src = prim ? B.W : B.Q;
+    for (int o = 0; o < 4; ++o) {
+        int pc = p - 2 + o;
+        if (pc >= 1 && pc <= N - 1) {
+            long xc = xf + (long)(o - 2) * dstride;
+            for (int c = 0; c < 5; ++c) st[o][c] = src[c * box + xc];
+        } else {
+            int s = 2 * d + (pc >= N ? 1 : 0);
+            int layer = pc < 1 ? 1 - pc : pc - (N - 1);
+            int ta = d == 0 ? j : i, tb = d == 2 ? j : k;
+            int c0 = prim ? 5 : 0;
+            for (int c = 0; c < 5; ++c) st[o][c] = *dummy_ptr(B, s, layer, c0 + c, ta, tb);
+        }
+    }
+    double L[5], R[5], cvL[5], cvR[5];
+    const double lim_zero = F32C(1E-20);
+    if (prim) {
+        for (int c = 0; c < 5; ++c) recon1_prim(g.recon_scheme, st[0][c], st[1][c], st[2][c], st[3][c], L[c], R[c]);
+        if (L[0] <= lim_zero || L[4] <= lim_zero) for (int c = 0; c < 5; ++c) L[c] = st[1][c];
+        if (R[0] <= lim_zero || R[4] <= lim_zero) for (int c = 0; c < 5; ++c) R[c] = st[2][c];
+        cvL[0] = L[0]; cvL[1] = L[0] * L[1]; cvL[2] = L[0] * L[2]; cvL[3] = L[0] * L[3];
+        cvL[4] = L[4] / (Gamma - F32C(1.0)) + L[0] * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]) / F32C(2.0);
+        cvR[0] = R[0]; cvR[1] = R[0] * R[1]; cvR[2] = R[0] * R[2]; cvR[3] = R[0] * R[3];
+        cvR[4] = R[4] / (Gamma - F32C(1.0)) + R[0] * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3]) / F32C(2.0);
+    } else {
+        for (int c = 0; c < 5; ++c) recon1_cons(g.recon_scheme, st[0][c], st[1][c], st[2][c], st[3][c], cvL[c], cvR[c]);
+        double pL = (Gamma - F32C(1.0)) * (cvL[4] - F32C(0.5) * (cvL[1] * cvL[1] + cvL[2] * cvL[2] + cvL[3] * cvL[3]) / cvL[0]);
+        double pR = (Gamma - F32C(1.0)) * (cvR[4] - F32C(0.5) * (cvR[1] * cvR[1] + cvR[2] * cvR[2] + cvR[3] * cvR[3]) / cvR[0]);
+        if (cvL[0] <= lim_zero || pL <= lim_zero) for (int c = 0; c < 5; ++c) cvL[c] = st[1][c];
+        if (cvR[0] <= lim_zero || pR <= lim_zero) for (int c = 0; c < 5; ++c) cvR[c] = st[2][c];
+    }
+    // face frame (source/Flux_Split_Scheme.cpp:728-776)
+    const double s_ = B.fs[d][xf];
+    double nx = B.fS[d][xf] / s_, ny = B.fS[d][box + xf] / s_, nz = B.fS[d][2 * box + xf] / s_;
+    const long xd = d == 0 ? cidx(B, i, j + 1, k + 1) : d == 1 ? cidx(B, i + 1, j, k + 1) : cidx(B, i + 1, j + 1, k);
+    double rx = B.node[xd] - B.node[xf], ry = B.node[box + xd] - B.node[box + xf], rz = B.node[2 * box + xd] - B.node[2 * box + xf];
+    double rl = sqrt(rx * rx + ry * ry + rz * rz);
+    double t1x = rx / rl, t1y = ry / rl, t1z = rz / rl;
+    double qx = ny * t1z - nz * t1y, qy = nz * t1x - nx * t1z, qz = nx * t1y - ny * t1x;
+    double ql = sqrt(qx * qx + qy * qy + qz * qz);
+    double t2x = qx / ql, t2y = qy / ql, t2z = qz / ql;
+    double fue = F32C(-1.0) * B.fc[d][box + xf] * B.w_rot, fve = B.fc[d][xf] * B.w_rot;
+    double que = fue * nx + fve * ny + F32C(0.0) * nz;
+    double QL[5], QR[5];
+    QL[0] = cvL[0]; QL[4] = cvL[4];
+    QL[1] = cvL[1] * nx + cvL[2] * ny + cvL[3] * nz;
+    QL[2] = cvL[1] * t1x + cvL[2] * t1y + cvL[3] * t1z;
+    QL[3] = cvL[1] * t2x + cvL[2] * t2y + cvL[3] * t2z;
+    QR[0] = cvR[0]; QR[4] = cvR[4];
+    QR[1] = cvR[1] * nx + cvR[2] * ny + cvR[3] * nz;
+    QR[2] = cvR[1] * t1x + cvR[2] * t1y + cvR[3] * t1z;
+    QR[3] = cvR[1] * t2x + cvR[2] * t2y + cvR[3] * t2z;
+    double Fc[5];
+    if (g.flux_scheme == 200) {
+        double fp_[5], fm_[5];
+        sw_half(Gamma, QL, que, 1.0, fp_);
+        sw_half(Gamma, QR, que, -1.0, fm_);
+        for (int c = 0; c < 5; ++c) Fc[c] = fp_[c] + fm_[c];
+    } else {
+        roe_flux(Gamma, QL, QR, que, Fc);
+    }
+    out[0] = Fc[0] * s_;
+    out[1] = (Fc[1] * nx + Fc[2] * t1x + Fc[3] * t2x) * s_;
+    out[2] = (Fc[1] * ny + Fc[2] * t1y + Fc[3] * t2y) * s_;
+    out[3] = (Fc[1] * nz + Fc[2] * t1z + Fc[3] * t2z) * s_;
+    out[4] = Fc[4] * s_;
+}
+
+// Tile of TI x TJ x TK cells per workgroup; every face of the tile is computed
+// once into LDS, then each cell sums its six faces and subtracts the rotation
+// source (source/ComputeAllBlockVars.cpp:85-118, 237-254).  Optionally stores
+// the face fluxes for inspection.
+template <int TI, int TJ, int TK>
+__global__ void __launch_bounds__(TI *TJ *TK) k_convective(const BlockDev *__restrict__ blocks, const int4 *__restrict__ tiles,
+                                                           Gas g, int store_faces, double *fI0, double *fJ0, double *fK0)
+{
+    __shared__ double sI[5][TI + 1][TJ][TK];
+    __shared__ double sJ[5][TI][TJ + 1][TK];
+    __shared__ double sK[5][TI][TJ][TK + 1];
+    const int4 tl = tiles[blockIdx.x];
+    const BlockDev &B = blocks[tl.x];
+    const int t = threadIdx.x;
+    const int tk = t % TK, tj = (t / TK) % TJ, ti = t / (TK * TJ);
+    const int i = tl.y + ti + 1, j = tl.z + tj + 1, k = tl.w + tk + 1;
+    const bool valid = i <= B.ni && j <= B.nj && k <= B.nk;
+    double f[5];
+    if (valid) {
+        face_flux(B, g, 0, i, j, k, f);
+        for (int c = 0; c < 5; ++c) sI[c][ti][tj][tk] = f[c];
+        face_flux(B, g, 1, i, j, k, f);
+        for (int c = 0; c < 5; ++c) sJ[c][ti][tj][tk] = f[c];
+        face_flux(B, g, 2, i, j, k, f);
+        for (int c = 0; c < 5; ++c) sK[c][ti][tj][tk] = f[c];
+        if (ti == TI - 1 || i == B.ni) {
+            face_flux(B, g, 0, i + 1, j, k, f);
+            for (int c = 0; c < 5; ++c) sI[c][ti + 1][tj][tk] = f[c];
+        }
+        if (tj == TJ - 1 || j == B.nj) {
+            face_flux(B, g, 1, i, j + 1, k, f);
+            for (int c = 0; c < 5; ++c) sJ[c][ti][tj + 1][tk] = f[c];
+        }
+        if (tk == TK - 1 || k == B.nk) {
+            face_flux(B, g, 2, i, j, k + 1, f);
+            for (int c = 0; c < 5; ++c) sK[c][ti][tj][tk + 1] = f[c];
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+    const long box = B.box, x = cidx(B, i, j, k);
+    const double w_rot = B.w_rot;
+    const double vol = B.vol[x];
+    const double dens = B.W[x], uvel = B.W[box + x], vvel = B.W[2 * box + x];
+    double Qc[5] = {0.0, dens * (w_rot * vvel) * vol, F32C(-1.0) * dens * (w_rot * uvel) * vol, 0.0, 0.0};
+    for (int c = 0; c < 5; ++c) {
+        double s = sI[c][ti + 1][tj][tk] - sI[c][ti][tj][tk];
+        s = s + sJ[c][ti][tj + 1][tk];
+        s = s - sJ[c][ti][tj][tk];
+        s = s + sK[c][ti][tj][tk + 1];
+        s = s - sK[c][ti][tj][tk];
+        s = s - Qc[c];
+        // Res = Fc - Fv - Fd with Fv = Fd = 0 for the inviscid upwind path
+        B.R[c * box + x] = (s - 0.0) - 0.0;
+    }
+    if (store_faces) {
+        for (int c = 0; c < 5; ++c) {
+            fI0[c * box + x] = sI[c][ti][tj][tk];
+            fJ0[c * box + x] = sJ[c][ti][tj][tk];
+            fK0[c * box + x] = sK[c][ti][tj][tk];
+            if (ti == TI - 1 || i == B.ni) fI0[c * box + x + B.si] = sI[c][ti + 1][tj][tk];
+            if (tj == TJ - 1 || j == B.nj) fJ0[c * box + x + B.sj] = sJ[c][ti][tj + 1][tk];
+            if (tk == TK - 1 || k == B.nk) fK0[c * box + x + 1] = sK[c][ti][tj][tk + 1];
+        }
+    }
+}
+
+template __global__ void k_convective<kTI, kTJ, kTK>(const BlockDev *, const int4 *, Gas, int, double *, double *, double *);
+
+// ================================================================ LU-SGS
+// Flux difference of CTimeDiscr::delt_Fn_up / delt_Fn_down
+// (source/LU_SGS_Marching.cpp:293-419) at neighbour cell nb whose faces along
+// d are nb (lower) and nb + stride (upper).
+__device__ static void delt_fn(const BlockDev &B, double Gamma, int d, long nb, double df[5])
+{
+    const long box = B.box;
+    const long st = d == 0 ? B.si : d == 1 ? B.sj : 1;
+    const long fa = nb + st, fb = nb;   // upper face first, as the reference sums them
+    const double *S = B.fS[d], *s = B.fs[d];
+    double nx = (S[fa] / s[fa] + S[fb] / s[fb]) / F32C(2.0);
+    double ny = (S[box + fa] / s[fa] + S[box + fb] / s[fb]) / F32C(2.0);
+    double nz = (S[2 * box + fa] / s[fa] + S[2 * box + fb] / s[fb]) / F32C(2.0);
+    double uvel_e = F32C(-1.0) * B.cc[box + nb] * B.w_rot;
+    double vvel_e = B.cc[nb] * B.w_rot;
+    double o[5], nw[5];
+    for (int c = 0; c < 5; ++c) { o[c] = B.Q[c * box + nb]; nw[c] = o[c] + B.dQ[c * box + nb]; }
+    double dens = nw[0], uvel = nw[1] / nw[0], vvel = nw[2] / nw[0], wvel = nw[3] / nw[0], E = nw[4] / nw[0];
+    double press_new = (Gamma - F32C(1.0)) * dens * (E - F32C(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel));
+    double press_old = B.W[4 * box + nb];
+    double Vn_old = o[1] / o[0] * nx + o[2] / o[0] * ny + o[3] / o[0] * nz;
+    double Vn_new = uvel * nx + vvel * ny + wvel * nz;
+    double Vn_e = uvel_e * nx + vvel_e * ny + F32C(0.0) * nz;
+    df[0] = nw[0] * (Vn_new - Vn_e) - o[0] * (Vn_old - Vn_e);
+    df[1] = (nw[1] * (Vn_new - Vn_e) + press_new * nx) - (o[1] * (Vn_old - Vn_e) + press_old * nx);
+    df[2] = (nw[2] * (Vn_new - Vn_e) + press_new * ny) - (o[2] * (Vn_old - Vn_e) + press_old * ny);
+    df[3] = (nw[3] * (Vn_new - Vn_e) + press_new * nz) - (o[3] * (Vn_old - Vn_e) + press_old * nz);
+    df[4] = ((nw[4] + press_new) * (Vn_new - Vn_e) + press_new * Vn_e) - ((o[4] + press_old) * (Vn_old - Vn_e) + press_old * Vn_e);
+}
+
+// One hyperplane i+j+k = plane of every block.  Thread (n, i, j) handles cell
+// (i, j, plane-i-j) of block n.  Cells of one plane depend only on cells of the
+// previous plane, so the result equals the reference's serial order exactly.
+template <bool FORWARD>
+__global__ void k_lusgs_plane(const BlockDev *__restrict__ blocks, const long *__restrict__ pstart, int nblock, int step,
+                              double relax, Gas g)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= pstart[nblock]) return;
+    int n = 0;
+    while (t >= pstart[n + 1]) ++n;
+    const BlockDev &B = blocks[n];
+    const long r = t - pstart[n];
+    const int i = (int)(r / B.nj) + 1, j = (int)(r % B.nj) + 1;
+    const int last = B.IM + B.JM + B.KM - 3;
+    const int plane = FORWARD ? 3 + step : last - step;
+    if (plane < 3 || plane > last) return;
+    const int k = plane - i - j;
+    if (k < 1 || k > B.nk) return;
+    const long box = B.box, x = cidx(B, i, j, k);
+    const double Gamma = g.gamma;
+    double alpha = B.vol[x] / B.dt[x] + relax * (B.lamc[x] + B.lamc[box + x] + B.lamc[2 * box + x]) +
+                   F32C(2.0) * (0.0 + 0.0 + 0.0);
+    double A[3][5], V[3][5];
+    const long st[3] = {B.si, B.sj, 1};
+    const int pos[3] = {i, j, k};
+    const int lim[3] = {B.ni, B.nj, B.nk};
+    for (int d = 0; d < 3; ++d) {
+        bool ok = FORWARD ? pos[d] != 1 : pos[d] != lim[d];
+        if (!ok) {
+            for (int c = 0; c < 5; ++c) { A[d][c] = 0.0; V[d][c] = 0.0; }
+            continue;
+        }
+        const long nb = FORWARD ? x - st[d] : x + st[d];
+        double df[5];
+        delt_fn(B, Gamma, d, nb, df);
+        const double *s = B.fs[d];
+        double sd = F32C(0.5) * (s[nb] + s[nb + st[d]]);
+        double rl = relax * B.lamc[d * box + nb];
+        for (int c = 0; c < 5; ++c) {
+            double dq = B.dQ[c * box + nb];
+            A[d][c] = FORWARD ? (df[c] * sd + dq * rl) * F32C(0.5) : (df[c] * sd - dq * rl) * F32C(0.5);
+            V[d][c] = dq * 0.0;
+        }
+    }
+    for (int c = 0; c < 5; ++c) {
+        double s;
+        if (FORWARD) {
+            s = B.R[c * box + x] * F32C(-1.0) + A[0][c];
+            s = s + A[1][c]; s = s + A[2][c];
+        } else {
+            s = B.dQ[c * box + x] * alpha - A[0][c];
+            s = s - A[1][c]; s = s - A[2][c];
+        }
+        s = s + V[0][c]; s = s + V[1][c]; s = s + V[2][c];
+        B.dQ[c * box + x] = s / alpha;
+    }
+}
+
+template __global__ void k_lusgs_plane<true>(const BlockDev *, const long *, int, int, double, Gas);
+template __global__ void k_lusgs_plane<false>(const BlockDev *, const long *, int, int, double, Gas);
+
+// Persistent wavefront form of the same sweeps.  The march axis is i; the
+// (j,k) cross-section is cut into TJ x TK tiles, one wave per tile.  Inside a
+// tile thread (tj,tk) handles cell i = s - tj - tk + 1 at step s (mirrored for
+// the backward sweep), so every step depends only on the previous one: the
+// i-neighbour is the thread's own last result, the j/k-neighbours come from LDS
+// or, at the tile edge, from the upstream tile through global memory.  Tiles
+// are handed out by a ticket counter in topological order, so a tile waits
+// only on tiles already held by running waves; progress counters are published
+// with agent-scope release stores and polled with bounded spins.
+template <bool FWD, int TJ, int TK>
+__global__ void __launch_bounds__(TJ *TK) k_lusgs_kba(const BlockDev *__restrict__ blocks, const KbaTile *__restrict__ tiles,
+                                                      int ntiles, int *__restrict__ ticket, int *__restrict__ done,
+                                                      int *__restrict__ err, double relax, Gas g, int chunk)
+{
+    static_assert(TJ * TK == 64, "one wave per tile");
+    __shared__ double sdq[2][5][TJ][TK];
+    const int t = threadIdx.x;
+    const int tj = t / TK, tk = t % TK;
+    const double Gamma = g.gamma;
+    for (;;) {
+        int q = 0;
+        if (t == 0) q = atomicAdd(ticket, 1);
+        q = __shfl(q, 0);
+        if (q >= ntiles) return;
+        const KbaTile tl = tiles[q];
+        const BlockDev &B = blocks[tl.block];
+        const long box = B.box;
+        // mirrored lane position for the backward sweep
+        const int lj = FWD ? tj : TJ - 1 - tj, lk = FWD ? tk : TK - 1 - tk;
+        const int j = tl.j0 + tj + 1, k = tl.k0 + tk + 1;
+        const bool col = j <= B.nj && k <= B.nk;
+        const int nsteps = B.ni + TJ + TK - 2;
+        int seen_j = tl.dep_j < 0 ? 1 << 30 : 0, seen_k = tl.dep_k < 0 ? 1 << 30 : 0;
+        double prev[5] = {0, 0, 0, 0, 0};
+        for (int s = 0; s < nsteps; ++s) {
+            const int need_j = min(s + TJ, nsteps), need_k = min(s + TK, nsteps);
+            if (seen_j < need_j || seen_k < need_k) {
+                long spins = 0;
+                while (seen_j < need_j) {
+                    seen_j = __hip_atomic_load(done + tl.dep_j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (seen_j >= need_j) break;
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > (1L << 22)) { if (t == 0) atomicExch(err, 1); seen_j = 1 << 30; }
+                }
+                while (seen_k < need_k) {
+                    seen_k = __hip_atomic_load(done + tl.dep_k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (seen_k >= need_k) break;
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > (1L << 22)) { if (t == 0) atomicExch(err, 1); seen_k = 1 << 30; }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            const int li = s - lj - lk;            // 0-based position along the march axis
+            const int i = FWD ? li + 1 : B.ni - li;
+            const bool valid = col && li >= 0 && li < B.ni;
+            double out[5];
+            if (valid) {
+                const long x = cidx(B, i, j, k);
+                double alpha = B.vol[x] / B.dt[x] + relax * (B.lamc[x] + B.lamc[box + x] + B.lamc[2 * box + x]) +
+                               F32C(2.0) * (0.0 + 0.0 + 0.0);
+                const int pos[3] = {i, j, k};
+                const int lim[3] = {B.ni, B.nj, B.nk};
+                const long st[3] = {B.si, B.sj, 1};
+                double A[3][5], V[3][5];
+                for (int d = 0; d < 3; ++d) {
+                    const bool ok = FWD ? pos[d] != 1 : pos[d] != lim[d];
+                    if (!ok) {
+                        for (int c = 0; c < 5; ++c) { A[d][c] = 0.0; V[d][c] = 0.0; }
+                        continue;
+                    }
+                    const long nb = FWD ? x - st[d] : x + st[d];
+                    double dqn[5];
+                    if (d == 0) {
+                        for (int c = 0; c < 5; ++c) dqn[c] = prev[c];
+                    } else if (d == 1 ? tj != (FWD ? 0 : TJ - 1) : tk != (FWD ? 0 : TK - 1)) {
+                        const int pj = d == 1 ? tj + (FWD ? -1 : 1) : tj, pk = d == 2 ? tk + (FWD ? -1 : 1) : tk;
+                        for (int c = 0; c < 5; ++c) dqn[c] = sdq[(s - 1) & 1][c][pj][pk];
+                    } else {
+                        for (int c = 0; c < 5; ++c) dqn[c] = B.dQ[c * box + nb];
+                    }
+                    // delt_Fn at the neighbour with its increment dqn
+                    const long fa = nb + st[d], fb = nb;
+                    const double *S = B.fS[d], *sa = B.fs[d];
+                    double nx = (S[fa] / sa[fa] + S[fb] / sa[fb]) / F32C(2.0);
+                    double ny = (S[box + fa] / sa[fa] + S[box + fb] / sa[fb]) / F32C(2.0);
+                    double nz = (S[2 * box + fa] / sa[fa] + S[2 * box + fb] / sa[fb]) / F32C(2.0);
+                    double uvel_e = F32C(-1.0) * B.cc[box + nb] * B.w_rot, vvel_e = B.cc[nb] * B.w_rot;
+                    double o[5], nw[5];
+                    for (int c = 0; c < 5; ++c) { o[c] = B.Q[c * box + nb]; nw[c] = o[c] + dqn[c]; }
+                    double dens = nw[0], uvel = nw[1] / nw[0], vvel = nw[2] / nw[0], wvel = nw[3] / nw[0], E = nw[4] / nw[0];
+                    double press_new = (Gamma - F32C(1.0)) * dens * (E - F32C(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel));
+                    double press_old = B.W[4 * box + nb];
+                    double Vn_old = o[1] / o[0] * nx + o[2] / o[0] * ny + o[3] / o[0] * nz;
+                    double Vn_new = uvel * nx + vvel * ny + wvel * nz;
+                    double Vn_e = uvel_e * nx + vvel_e * ny + F32C(0.0) * nz;
+                    double df[5];
+                    df[0] = nw[0] * (Vn_new - Vn_e) - o[0] * (Vn_old - Vn_e);
+                    df[1] = (nw[1] * (Vn_new - Vn_e) + press_new * nx) - (o[1] * (Vn_old - Vn_e) + press_old * nx);
+                    df[2] = (nw[2] * (Vn_new - Vn_e) + press_new * ny) - (o[2] * (Vn_old - Vn_e) + press_old * ny);
+                    df[3] = (nw[3] * (Vn_new - Vn_e) + press_new * nz) - (o[3] * (Vn_old - Vn_e) + press_old * nz);
+                    df[4] = ((nw[4] + press_new) * (Vn_new - Vn_e) + press_new * Vn_e) -
+                            ((o[4] + press_old) * (Vn_old - Vn_e) + press_old * Vn_e);
+                    double sd = F32C(0.5) * (sa[fb] + sa[fa]);
+                    double rl = relax * B.lamc[d * box + nb];
+                    for (int c = 0; c < 5; ++c) {
+                        A[d][c] = FWD ? (df[c] * sd + dqn[c] * rl) * F32C(0.5) : (df[c] * sd - dqn[c] * rl) * F32C(0.5);
+                        V[d][c] = dqn[c] * 0.0;
+                    }
+                }
+                for (int c = 0; c < 5; ++c) {
+                    double v;
+                    if (FWD) {
+                        v = B.R[c * box + x] * F32C(-1.0) + A[0][c];
+                        v = v + A[1][c]; v = v + A[2][c];
+                    } else {
+                        v = B.dQ[c * box + x] * alpha - A[0][c];
+                        v = v - A[1][c]; v = v - A[2][c];
+                    }
+                    v = v + V[0][c]; v = v + V[1][c]; v = v + V[2][c];
+                    out[c] = v / alpha;
+                    B.dQ[c * box + x] = out[c];
+                    sdq[s & 1][c][tj][tk] = out[c];
+                    prev[c] = out[c];
+                }
+            }
+            __syncthreads();
+            if ((s + 1) % chunk == 0 || s + 1 == nsteps) {
+                if (t == 0) __hip_atomic_store(done + q, s + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+}
+
+template __global__ void k_lusgs_kba<true, 8, 8>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
+template __global__ void k_lusgs_kba<false, 8, 8>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
+
+// Clears the ticket, error flag and progress counters of one sweep.
+__global__ void k_fill_int(int *p, long n, int v)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+// Second form of the wavefront sweep: static sweep metrics and the diagonal
+// come precomputed, the next step's inputs are prefetched into registers while
+// the current step computes, and tile-edge increments travel through
+// agent-coherent atomic stores/loads followed by a progress counter, so no
+// cache-wide write-back or invalidate is needed per step.
+struct SweepIn {
+    double own[6];       // R (forward) or dQ* (backward) x5, alpha
+    double q[3][5];      // neighbour Q per direction
+    double pl[3][2];     // neighbour p, relax*lamc
+    double gm[3][5];     // neighbour averaged normal (3), area, grid normal velocity
+};
+
+template <bool FWD>
+__device__ inline void sweep_load(const BlockDev &B, long x, int i, int j, int k, double relax, SweepIn &in)
+{
+    const long box = B.box;
+    for (int c = 0; c < 5; ++c) in.own[c] = FWD ? B.R[c * box + x] : B.dQ[c * box + x];
+    in.own[5] = B.alpha[x];
+    const int pos[3] = {i, j, k};
+    const int lim[3] = {B.ni, B.nj, B.nk};
+    const long st[3] = {B.si, B.sj, 1};
+    for (int d = 0; d < 3; ++d) {
+        const bool ok = FWD ? pos[d] != 1 : pos[d] != lim[d];
+        const long nb = ok ? (FWD ? x - st[d] : x + st[d]) : x;
+        for (int c = 0; c < 5; ++c) in.q[d][c] = B.Q[c * box + nb];
+        in.pl[d][0] = B.W[4 * box + nb];
+        in.pl[d][1] = relax * B.lamc[d * box + nb];
+        for (int c = 0; c < 5; ++c) in.gm[d][c] = B.swg[(d * 5 + c) * box + nb];
+    }
+}
+
+template <bool FWD, int TJ, int TK>
+__global__ void __launch_bounds__(TJ *TK) k_lusgs_kba2(const BlockDev *__restrict__ blocks, const KbaTile *__restrict__ tiles,
+                                                       int ntiles, int *__restrict__ ticket, int *__restrict__ done,
+                                                       int *__restrict__ err, double relax, Gas g)
+{
+    static_assert(TJ * TK == 64, "one wave per tile");
+    __shared__ double sdq[2][5][TJ][TK];
+    const int t = threadIdx.x;
+    const int tj = t / TK, tk = t % TK;
+    const double Gamma = g.gamma;
+    for (;;) {
+        int q = 0;
+        if (t == 0) q = atomicAdd(ticket, 1);
+        q = __shfl(q, 0);
+        if (q >= ntiles) return;
+        const KbaTile tl = tiles[q];
+        const BlockDev &B = blocks[tl.block];
+        const long box = B.box;
+        const int lj = FWD ? tj : TJ - 1 - tj, lk = FWD ? tk : TK - 1 - tk;
+        const int j = tl.j0 + tj + 1, k = tl.k0 + tk + 1;
+        const bool col = j <= B.nj && k <= B.nk;
+        const int nsteps = B.ni + TJ + TK - 2;
+        const bool edge_j = tj == (FWD ? 0 : TJ - 1), edge_k = tk == (FWD ? 0 : TK - 1);
+        // downstream tile edges publish through coherent stores
+        const bool pub = (tj == (FWD ? TJ - 1 : 0)) || (tk == (FWD ? TK - 1 : 0));
+        int seen_j = tl.dep_j < 0 ? 1 << 30 : 0, seen_k = tl.dep_k < 0 ? 1 << 30 : 0;
+        double prev[5] = {0, 0, 0, 0, 0};
+        SweepIn nxt;
+        int li = -lj - lk;
+        {
+            const int i = FWD ? li + 1 : B.ni - li;
+            if (col && li >= 0 && li < B.ni) sweep_load<FWD>(B, cidx(B, i, j, k), i, j, k, relax, nxt);
+        }
+        for (int s = 0; s < nsteps; ++s, ++li) {
+            const int i = FWD ? li + 1 : B.ni - li;
+            const bool valid = col && li >= 0 && li < B.ni;
+            SweepIn cur = nxt;
+            // prefetch the next step of this column
+            {
+                const int li2 = li + 1, i2 = FWD ? li2 + 1 : B.ni - li2;
+                if (col && li2 >= 0 && li2 < B.ni) sweep_load<FWD>(B, cidx(B, i2, j, k), i2, j, k, relax, nxt);
+            }
+            const int need_j = min(s + TJ, nsteps), need_k = min(s + TK, nsteps);
+            if (seen_j < need_j || seen_k < need_k) {
+                long spins = 0;
+                while (seen_j < need_j) {
+                    seen_j = __hip_atomic_load(done + tl.dep_j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (seen_j >= need_j) break;
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > (1L << 22)) { if (t == 0) atomicExch(err, 1); seen_j = 1 << 30; }
+                }
+                while (seen_k < need_k) {
+                    seen_k = __hip_atomic_load(done + tl.dep_k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (seen_k >= need_k) break;
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > (1L << 22)) { if (t == 0) atomicExch(err, 1); seen_k = 1 << 30; }
+                }
+            }
+            if (valid) {
+                const long x = cidx(B, i, j, k);
+                const int pos[3] = {i, j, k};
+                const int lim[3] = {B.ni, B.nj, B.nk};
+                const long st[3] = {B.si, B.sj, 1};
+                const double alpha = cur.own[5];
+                double A[3][5], V[3][5];
+                for (int d = 0; d < 3; ++d) {
+                    const bool act = FWD ? pos[d] != 1 : pos[d] != lim[d];
+                    if (!act) {
+                        for (int c = 0; c < 5; ++c) { A[d][c] = 0.0; V[d][c] = 0.0; }
+                        continue;
+                    }
+                    double dqn[5];
+                    if (d == 0) {
+                        for (int c = 0; c < 5; ++c) dqn[c] = prev[c];
+                    } else if (d == 1 ? !edge_j : !edge_k) {
+                        const int pj = d == 1 ? tj + (FWD ? -1 : 1) : tj, pk = d == 2 ? tk + (FWD ? -1 : 1) : tk;
+                        for (int c = 0; c < 5; ++c) dqn[c] = sdq[(s - 1) & 1][c][pj][pk];
+                    } else {
+                        const long nb = FWD ? x - st[d] : x + st[d];
+                        for (int c = 0; c < 5; ++c)
+                            dqn[c] = __hip_atomic_load(B.dQ + c * box + nb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    const double *o = cur.q[d];
+                    const double nx = cur.gm[d][0], ny = cur.gm[d][1], nz = cur.gm[d][2], sd = cur.gm[d][3];
+                    const double Vn_e = cur.gm[d][4];
+                    double nw[5];
+                    for (int c = 0; c < 5; ++c) nw[c] = o[c] + dqn[c];
+                    double dens = nw[0], uvel = nw[1] / nw[0], vvel = nw[2] / nw[0], wvel = nw[3] / nw[0], E = nw[4] / nw[0];
+                    double press_new = (Gamma - F32C(1.0)) * dens * (E - F32C(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel));
+                    double press_old = cur.pl[d][0];
+                    double Vn_old = o[1] / o[0] * nx + o[2] / o[0] * ny + o[3] / o[0] * nz;
+                    double Vn_new = uvel * nx + vvel * ny + wvel * nz;
+                    double df[5];
+                    df[0] = nw[0] * (Vn_new - Vn_e) - o[0] * (Vn_old - Vn_e);
+                    df[1] = (nw[1] * (Vn_new - Vn_e) + press_new * nx) - (o[1] * (Vn_old - Vn_e) + press_old * nx);
+                    df[2] = (nw[2] * (Vn_new - Vn_e) + press_new * ny) - (o[2] * (Vn_old - Vn_e) + press_old * ny);
+                    df[3] = (nw[3] * (Vn_new - Vn_e) + press_new * nz) - (o[3] * (Vn_old - Vn_e) + press_old * nz);
+                    df[4] = ((nw[4] + press_new) * (Vn_new - Vn_e) + press_new * Vn_e) -
+                            ((o[4] + press_old) * (Vn_old - Vn_e) + press_old * Vn_e);
+                    const double rl = cur.pl[d][1];
+                    for (int c = 0; c < 5; ++c) {
+                        A[d][c] = FWD ? (df[c] * sd + dqn[c] * rl) * F32C(0.5) : (df[c] * sd - dqn[c] * rl) * F32C(0.5);
+                        V[d][c] = dqn[c] * 0.0;   // lamv * dQ, lamv = 0 on the inviscid path
+                    }
+                }
+                for (int c = 0; c < 5; ++c) {
+                    double v;
+                    if (FWD) {
+                        v = cur.own[c] * F32C(-1.0) + A[0][c];
+                        v = v + A[1][c]; v = v + A[2][c];
+                    } else {
+                        v = cur.own[c] * alpha - A[0][c];
+                        v = v - A[1][c]; v = v - A[2][c];
+                    }
+                    v = v + V[0][c]; v = v + V[1][c]; v = v + V[2][c];
+                    const double out = v / alpha;
+                    if (pub) __hip_atomic_store(B.dQ + c * box + x, out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    else B.dQ[c * box + x] = out;
+                    sdq[s & 1][c][tj][tk] = out;
+                    prev[c] = out;
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (t == 0) __hip_atomic_store(done + q, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+template __global__ void k_lusgs_kba2<true, 8, 8>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas);
+template __global__ void k_lusgs_kba2<false, 8, 8>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas);
+
+// ===================================================== update + norms
+// Q += dQ (source/LU_SGS_Marching.cpp:262-287), Res_Calculate terms
+// (source/ComputeAllBlockVars.cpp:340-357), PrimitiveVars
+// (source/PrimitiveVars.cpp:7-29) and the Limit_Vars trigger test
+// (source/Limit_FlowVars.cpp:28-29).  One partial sum per workgroup.
+__global__ void __launch_bounds__(256) k_update(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart, int nblock,
+                                                Gas g, double *__restrict__ partial, int *__restrict__ bad_flag)
+{
+    __shared__ double red[5][256];
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double rm[5] = {0, 0, 0, 0, 0};
+    if (t < cstart[nblock]) {
+        int n = 0;
+        while (t >= cstart[n + 1]) ++n;
+        const BlockDev &B = blocks[n];
+        long r = t - cstart[n];
+        int k = (int)(r % B.nk) + 1;
+        r /= B.nk;
+        int j = (int)(r % B.nj) + 1;
+        int i = (int)(r / B.nj) + 1;
+        const long box = B.box, x = cidx(B, i, j, k);
+        double q[5];
+        const double vol = B.vol[x];
+        for (int c = 0; c < 5; ++c) {
+            double o = B.Q[c * box + x];
+            q[c] = o + B.dQ[c * box + x];
+            rm[c] = fabs((o - q[c]) / o) * vol;
+            B.Q[c * box + x] = q[c];
+        }
+        const double Gamma = g.gamma, Rgas = g.rgas;
+        double d = q[0], u = q[1] / q[0], v = q[2] / q[0], w = q[3] / q[0];
+        double V2 = u * u + v * v + w * w;
+        double p = (Gamma - F32C(1.0)) * (q[4] - q[0] * V2 / F32C(2.0));
+        double T = p / d / Rgas;
+        B.W[x] = d; B.W[box + x] = u; B.W[2 * box + x] = v; B.W[3 * box + x] = w; B.W[4 * box + x] = p; B.W[5 * box + x] = T;
+        const double dmin = F32C(1.0E-6), dmax = F32C(1000.0), vmax = F32C(1000.0);
+        if (d < dmin || d > dmax || p < dmin || p > dmax || u > vmax || v > vmax || w > vmax) atomicOr(bad_flag + n, 1);
+    }
+    for (int c = 0; c < 5; ++c) red[c][threadIdx.x] = rm[c];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int c = 0; c < 5; ++c) red[c][threadIdx.x] = red[c][threadIdx.x] + red[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 5; ++c) partial[(long)c * gridDim.x + blockIdx.x] = red[c][0];
+}
+
+// Limit_Vars (source/Limit_FlowVars.cpp:8-119) in the reference's serial order:
+// cells are repaired one after the other, so later repairs see earlier ones.
+// Runs as one thread and returns at once unless k_update flagged a bad cell.
+__device__ static bool bad_cell(const BlockDev &B, long x)
+{
+    const long box = B.box;
+    const double dmin = F32C(1.0E-6), dmax = F32C(1000.0), vmax = F32C(1000.0);
+    double d = B.W[x], p = B.W[4 * box + x];
+    return d < dmin || d > dmax || p < dmin || p > dmax || B.W[box + x] > vmax || B.W[2 * box + x] > vmax ||
+           B.W[3 * box + x] > vmax;
+}
+
+__global__ void k_limiter_serial(const BlockDev *__restrict__ blocks, int nblock, Gas g, int *__restrict__ bad_flag)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double Gamma = g.gamma, Rgas = g.rgas;
+    const double dmin = F32C(1.0E-6), dmax = F32C(1000.0), vmax = F32C(1000.0);
+    for (int n = 0; n < nblock; ++n) {
+        if (!bad_flag[n]) continue;
+        bad_flag[n] = 0;
+        const BlockDev &B = blocks[n];
+        const long box = B.box;
+        int nneg = 0;
+        for (int i = 1; i <= B.ni; ++i)
+            for (int j = 1; j <= B.nj; ++j)
+                for (int k = 1; k <= B.nk; ++k) {
+                    const long x = cidx(B, i, j, k);
+                    if (!bad_cell(B, x)) continue;
+                    nneg++;
+                    int kn = 0, i1 = 0, j1 = 0, k1 = 0;
+                    double d = 0.0, u = 0.0, v = 0.0, w = 0.0, p = 0.0;
+                    for (int ia = -1; ia <= 1; ++ia)
+                        for (int ja = -1; ja <= 1; ++ja)
+                            for (int ka = -1; ka <= 1; ++ka) {
+                                k1 = k + ka; j1 = j + ja; i1 = i + ia;
+                                if (i1 > 0 && i1 < B.IM && j1 > 0 && j1 < B.JM && k1 > 0 && k1 < B.KM) {
+                                    const long y = cidx(B, i1, j1, k1);
+                                    if (!bad_cell(B, y)) {
+                                        d = d + B.W[y]; u = u + B.W[box + y]; v = v + B.W[2 * box + y];
+                                        w = w + B.W[3 * box + y]; p = p + B.W[4 * box + y];
+                                        kn = kn + 1;
+                                    }
+                                }
+                            }
+                    if (kn == 0) {
+                        const long y = cidx(B, i1, j1, k1);
+                        d = B.W[y]; u = B.W[box + y]; v = B.W[2 * box + y]; w = B.W[3 * box + y]; p = B.W[4 * box + y];
+                        if (d < dmin) d = dmin;
+                        if (d > dmax) d = dmax;
+                        if (p < dmin) p = dmin;
+                        if (p > dmax) p = dmax;
+                        if (fabs(u) > vmax) u = u < 0 ? -fabs(vmax) : fabs(vmax);
+                        if (fabs(v) > vmax) v = v < 0 ? -fabs(vmax) : fabs(vmax);
+                        if (fabs(w) > vmax) w = w < 0 ? -fabs(vmax) : fabs(vmax);
+                    } else {
+                        double sn1 = (double)(1.0f / (float)kn);
+                        d = d * sn1; u = u * sn1; v = v * sn1; w = w * sn1; p = p * sn1;
+                    }
+                    B.W[x] = d; B.W[box + x] = u; B.W[2 * box + x] = v; B.W[3 * box + x] = w; B.W[4 * box + x] = p;
+                    B.W[5 * box + x] = p / d / Rgas;
+                    B.Q[x] = d; B.Q[box + x] = d * u; B.Q[2 * box + x] = d * v; B.Q[3 * box + x] = d * w;
+                    B.Q[4 * box + x] = p / (Gamma - F32C(1.0)) + d * (u * u + v * v + w * w) / F32C(2.0);
+                }
+        if (nneg > 0) *B.flag_low_dt = 10;
+    }
+}
+
+// ============================================= section integrals, norms
+// Mass flow, mass-averaged total pressure and total temperature over one K
+// plane (source/Ouput.cpp:9-95).  One workgroup per section, fixed-order tree
+// reduction; sums[sec*5 + {mass, ptm, ttm}].
+__global__ void __launch_bounds__(256) k_sections(const BlockDev *__restrict__ blocks, const int2 *__restrict__ secs, Gas g,
+                                                  double *__restrict__ sums)
+{
+    __shared__ double red[3][256];
+    const int2 sc = secs[blockIdx.x];
+    const BlockDev &B = blocks[sc.x];
+    const int K = sc.y;
+    const long box = B.box;
+    const double Gamma = g.gamma, Rgas = g.rgas;
+    double m = 0.0, pt = 0.0, tt = 0.0;
+    for (int f = threadIdx.x; f < B.ni * B.nj; f += blockDim.x) {
+        int i = f / B.nj + 1, j = f % B.nj + 1;
+        long a = cidx(B, i, j, K), z = cidx(B, i, j, K - 1);
+        double dens = F32C(0.5) * (B.W[a] + B.W[z]), u = F32C(0.5) * (B.W[box + a] + B.W[box + z]);
+        double v = F32C(0.5) * (B.W[2 * box + a] + B.W[2 * box + z]), w = F32C(0.5) * (B.W[3 * box + a] + B.W[3 * box + z]);
+        const double *S = B.fS[2];
+        double mc = dens * (u * S[a] + v * S[box + a] + w * S[2 * box + a]);
+        double temp = F32C(0.5) * (B.W[5 * box + a] + B.W[5 * box + z]);
+        double Ma = sqrt(u * u + v * v + w * w) / sqrt(Gamma * Rgas * temp);
+        double Pt_cell = B.W[4 * box + a] * pow(F32C(1.0) + (Gamma - F32C(1.0)) / F32C(2.0) * Ma * Ma, Gamma / (Gamma - F32C(1.0)));
+        m += mc;
+        pt += pow(Pt_cell, (Gamma - F32C(1.0)) / Gamma) * mc;
+        tt += B.W[5 * box + a] * (1 + (Gamma - F32C(1.0)) / F32C(2.0) * Ma * Ma) * mc;
+    }
+    red[0][threadIdx.x] = m; red[1][threadIdx.x] = pt; red[2][threadIdx.x] = tt;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; ++c) sums[blockIdx.x * 3 + c] = red[c][0];
+}
+
+// Residual norm (fixed-order sum of the per-workgroup partials), performance
+// parameters (source/Ouput.cpp:97-107,282-300), CFL ramp for the next iteration
+// (source/Solver.cpp:49-60) and the history row.  One workgroup.
+__global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ partial, int npart, const double *__restrict__ sums,
+                                                  ScalarsDev *__restrict__ sc, Gas g, double *__restrict__ hist_row)
+{
+    __shared__ double red[5][256];
+    double r[5] = {0, 0, 0, 0, 0};
+    for (int p = threadIdx.x; p < npart; p += blockDim.x)
+        for (int c = 0; c < 5; ++c) r[c] += partial[(long)c * npart + p];
+    for (int c = 0; c < 5; ++c) red[c][threadIdx.x] = r[c];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int c = 0; c < 5; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    for (int c = 0; c < 5; ++c) sc->res[c] = red[c][0] / sc->total_vol;
+    const double Gamma = g.gamma, Rgas = g.rgas;
+    // sums: section 0 inlet, section 1 outlet: mass, ptm, ttm
+    double mi = sums[0] * sc->dens_ref * sc->vel_ref * sc->nblade_in;
+    double mo = sums[3] * sc->dens_ref * sc->vel_ref * sc->nblade_out;
+    double pti = pow(sums[1] / sums[0], Gamma / (Gamma - F32C(1.0))) * sc->dens_ref * sc->vel_ref * sc->vel_ref;
+    double pto = pow(sums[4] / sums[3], Gamma / (Gamma - F32C(1.0))) * sc->dens_ref * sc->vel_ref * sc->vel_ref;
+    double tti = sums[2] / sums[0] * sc->temp_ref, tto = sums[5] / sums[3] * sc->temp_ref;
+    double PR = pto / pti;
+    double L_adk = Gamma / (Gamma - F32C(1.0)) * Rgas * tti * (pow(PR, (Gamma - F32C(1.0)) / Gamma) - F32C(1.0));
+    double L_k = Gamma / (Gamma - F32C(1.0)) * Rgas * tti * (tto / tti - F32C(1.0));
+    sc->iter += 1;
+    if (hist_row) {
+        hist_row[0] = sc->iter; hist_row[1] = sc->res[0]; hist_row[2] = mi; hist_row[3] = mo;
+        hist_row[4] = PR; hist_row[5] = L_adk / L_k; hist_row[6] = sc->cfl;
+    }
+    sc->mass_in = mi; sc->mass_out = mo; sc->pr = PR; sc->eff = L_adk / L_k;
+    // CFL ramp applied for the next iteration
+    if (sc->res[0] <= sc->cfl_res) {
+        sc->cfl = sc->cfl * F32C(1.2);
+        sc->cfl_res = sc->cfl_res / F32C(10.0);
+    }
+    if (sc->cfl > F32C(10.0)) sc->cfl = F32C(10.0);
+}
+
+}  // namespace fv3d

@@ -1,0 +1,75 @@
+"""ctypes binding of libfv3d.so (the HIP library built in-tree for gfx950).
+
+The library is loaded from ``lib/libfv3d.so`` next to this package; if it is
+missing the import fails loudly -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libfv3d.so")
+
+# every symbol declared in include/fv3d.h
+EXPORTS = [
+    "fv3d_ctx_create", "fv3d_ctx_destroy", "fv3d_last_error", "fv3d_block_add", "fv3d_patch_add", "fv3d_setup",
+    "fv3d_state_upload", "fv3d_field_download", "fv3d_bc_apply", "fv3d_spectral", "fv3d_timestep", "fv3d_convective",
+    "fv3d_resflux", "fv3d_lusgs", "fv3d_res_calculate", "fv3d_perf", "fv3d_iterate", "fv3d_iterate_graph", "fv3d_sync",
+    "fv3d_set_timing", "fv3d_timing", "fv3d_set_debug", "fv3d_cell_count",
+]
+
+
+class Config(C.Structure):
+    _fields_ = [("rgas", C.c_double), ("gamma", C.c_double), ("pr_lam", C.c_double), ("pr_tur", C.c_double),
+                ("pt_in", C.c_double), ("tt_in", C.c_double), ("p_out", C.c_double), ("ma0", C.c_double),
+                ("len_ref", C.c_double), ("ev_ratio", C.c_double), ("inlet_dir", C.c_double * 3),
+                ("cfl", C.c_double), ("c_timestep", C.c_double), ("relax", C.c_double),
+                ("turbo_type", C.c_int), ("viscous", C.c_int), ("recon_var", C.c_int), ("recon_scheme", C.c_int),
+                ("flux_scheme", C.c_int), ("turbulence", C.c_int), ("limiter", C.c_int), ("global_dt", C.c_int),
+                ("device", C.c_int), ("precision", C.c_int)]
+
+
+class Fv3dError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Loads libfv3d.so and declares the C ABI; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise Fv3dError(f"libfv3d.so not built at {LIB_PATH}; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, i, d, l, p = C.c_void_p, C.c_int, C.c_double, C.c_long, C.c_void_p
+    sig = {
+        "fv3d_ctx_create": (i, [C.POINTER(Config), C.POINTER(vp)]),
+        "fv3d_ctx_destroy": (i, [vp]),
+        "fv3d_last_error": (C.c_char_p, [vp]),
+        "fv3d_block_add": (i, [vp, i, i, i, p, i, d]),
+        "fv3d_patch_add": (i, [vp, i, i, i, p, i]),
+        "fv3d_setup": (i, [vp]),
+        "fv3d_state_upload": (i, [vp, i, p, p]),
+        "fv3d_field_download": (l, [vp, i, C.c_char_p, p]),
+        "fv3d_res_calculate": (i, [vp, p]),
+        "fv3d_perf": (i, [vp, p]),
+        "fv3d_iterate": (i, [vp, i, p]),
+        "fv3d_iterate_graph": (i, [vp, i, p]),
+        "fv3d_set_timing": (i, [vp, i]),
+        "fv3d_timing": (i, [vp, p, p]),
+        "fv3d_set_debug": (i, [vp, i]),
+        "fv3d_cell_count": (l, [vp]),
+    }
+    for name in ("fv3d_bc_apply", "fv3d_spectral", "fv3d_timestep", "fv3d_convective", "fv3d_resflux", "fv3d_lusgs",
+                 "fv3d_sync"):
+        sig[name] = (i, [vp])
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L

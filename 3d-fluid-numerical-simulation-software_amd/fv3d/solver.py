@@ -1,0 +1,143 @@
+"""Host-side mirror of the reference's CMesh / CSolver surface over libfv3d.
+
+``GpuSolver`` loads a case (fv3d.cases.Case, i.e. the reference's config /
+grid / boundary-definition / restart content) into a libfv3d context and
+exposes the stages of CMesh::ResFlux and CMesh::LU_SGS_Marching
+(source/ComputeAllBlockVars.cpp:223-255, source/LU_SGS_Marching.cpp:10-291)
+and the iteration of CSolver::Solver (source/Solver.cpp:47-66).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .refio import Config as CaseConfig
+
+_TURBO = {"A": 0, "C": 1}
+_KIND = {"inlet": 0, "outlet": 1, "wall": 2}
+
+
+def make_config(cfg: CaseConfig, device: int = 0, precision: int = 0) -> _lib.Config:
+    s = _lib.Config()
+    s.rgas, s.gamma, s.pr_lam, s.pr_tur = cfg.rgas, cfg.gamma, cfg.pr_lam, cfg.pr_tur
+    s.pt_in, s.tt_in, s.p_out, s.ma0 = cfg.pt_in, cfg.tt_in, cfg.p_out, cfg.ma0
+    s.len_ref, s.ev_ratio = cfg.len_ref, cfg.ev_ratio
+    for c in range(3):
+        s.inlet_dir[c] = cfg.inlet_angle[c]
+    s.cfl, s.c_timestep, s.relax = cfg.cfl, cfg.c_timestep, cfg.relax_factor
+    s.turbo_type = _TURBO.get(cfg.turbo_type.upper(), 2)
+    s.viscous = 0 if cfg.equation.upper() == "E" else 1
+    s.recon_var, s.recon_scheme, s.flux_scheme = cfg.recon_var, cfg.recon_scheme, cfg.flux_scheme
+    s.turbulence = cfg.turbulence
+    s.limiter = 1 if cfg.limiter.upper() == "Y" else 0
+    s.global_dt = 0 if cfg.timestep.upper() == "L" else 1
+    s.device = device
+    s.precision = precision
+    return s
+
+
+class GpuSolver:
+    """One libfv3d context holding one case."""
+
+    def __init__(self, case, device: int = 0, precision: int = 0, debug: int = 0):
+        self.L = _lib.load()
+        self.case = case
+        self._cfg = make_config(case.cfg, device, precision)
+        h = C.c_void_p()
+        rc = self.L.fv3d_ctx_create(C.byref(self._cfg), C.byref(h))
+        self.h = h
+        self._check(rc, "ctx_create")
+        for b, xyz in enumerate(case.nodes):
+            bi = case.blocks[b]
+            a = np.ascontiguousarray(xyz, dtype=np.float64)
+            rc = self.L.fv3d_block_add(self.h, bi.im, bi.jm, bi.km, a.ctypes.data, bi.nblade, bi.rpm)
+            self._check(rc, "block_add")
+        for p in case.patches:
+            if p.kind not in _KIND:
+                raise NotImplementedError(f"patch kind {p.kind} not in this build")
+            r = np.ascontiguousarray(p.rng, dtype=np.int32)
+            self._check(self.L.fv3d_patch_add(self.h, _KIND[p.kind], p.block, p.face, r.ctypes.data, p.rotate),
+                        "patch_add")
+        self._check(self.L.fv3d_setup(self.h), "setup")
+        if debug:
+            self._check(self.L.fv3d_set_debug(self.h, debug), "set_debug")
+        for b, st in enumerate(case.states):
+            pv = np.ascontiguousarray(np.stack([st["rho"], st["u"], st["v"], st["w"], st["p"], st["T"]], -1))
+            e = np.ascontiguousarray(st["rhoE"])
+            self._check(self.L.fv3d_state_upload(self.h, b + 1, pv.ctypes.data, e.ctypes.data), "state_upload")
+
+    def _check(self, rc, what):
+        if rc < 0:
+            msg = self.L.fv3d_last_error(self.h).decode() if self.h else ""
+            raise _lib.Fv3dError(f"fv3d {what} failed ({rc}): {msg}")
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fv3d_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # -- stages
+    def call(self, stage: str) -> None:
+        self._check(getattr(self.L, "fv3d_" + stage)(self.h), stage)
+
+    def iterate(self, n: int, graph: bool = False) -> np.ndarray:
+        h = np.zeros((max(n, 1), 7))
+        fn = self.L.fv3d_iterate_graph if graph else self.L.fv3d_iterate
+        self._check(fn(self.h, n, h.ctypes.data), "iterate")
+        return h[:n]
+
+    def res_calculate(self) -> np.ndarray:
+        r = np.zeros(6)
+        self._check(self.L.fv3d_res_calculate(self.h, r.ctypes.data), "res_calculate")
+        return r
+
+    def perf(self) -> np.ndarray:
+        r = np.zeros(5)
+        self._check(self.L.fv3d_perf(self.h, r.ctypes.data), "perf")
+        return r
+
+    def sync(self):
+        self._check(self.L.fv3d_sync(self.h), "sync")
+
+    def set_timing(self, on: bool):
+        self._check(self.L.fv3d_set_timing(self.h, int(on)), "set_timing")
+
+    def timing(self):
+        ms = np.zeros(5)
+        n = np.zeros(5, dtype=np.int64)
+        self._check(self.L.fv3d_timing(self.h, ms.ctypes.data, n.ctypes.data), "timing")
+        return ms, n
+
+    @property
+    def n_cells(self) -> int:
+        return int(self.L.fv3d_cell_count(self.h))
+
+    def get(self, block: int, name: str) -> np.ndarray:
+        bi = self.case.blocks[block - 1]
+        im, jm, km = bi.im, bi.jm, bi.km
+        ni, nj, nk = im - 1, jm - 1, km - 1
+        if name in ("cv", "pv"):
+            shape = (im + 1, jm + 1, km + 1, 5 if name == "cv" else 6)
+        elif name in ("res", "dq"):
+            shape = (ni, nj, nk, 5)
+        elif name in ("fcI", "fcJ", "fcK"):
+            shape = {"fcI": (im, nj, nk, 5), "fcJ": (ni, jm, nk, 5), "fcK": (ni, nj, km, 5)}[name]
+        elif name in ("dt", "lamcI", "lamcJ", "lamcK", "vol"):
+            shape = (ni, nj, nk)
+        elif name.startswith("d") and name[1] in "IJK":
+            c = 6 if name.endswith("_pv") else 5
+            shape = {"I": (2, nj, nk, c), "J": (ni, 2, nk, c), "K": (ni, nj, 2, c)}[name[1]]
+        else:
+            raise KeyError(name)
+        out = np.zeros(shape)
+        n = self.L.fv3d_field_download(self.h, block, name.encode(), out.ctypes.data)
+        self._check(n, "field_download " + name)
+        if n != out.size:
+            raise _lib.Fv3dError(f"field {name}: got {n} values, expected {out.size}")
+        return out

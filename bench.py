@@ -1,0 +1,130 @@
+"""Throughput of the LU-SGS iteration on one MI355X.
+
+A step is one full LU-SGS iteration (ghost fill, spectral radii, local time
+step, MUSCL + Roe residual, forward and backward hyperplane sweeps, update,
+residual norm, section integrals and the CFL ramp) over the whole synthetic
+grid, with the state resident in HBM.  Prints one JSON line.
+
+The workload is named in ``config.workload``.  ``cpu_baseline`` times the
+reference solver itself (compiled from its sources into oracle/_ref) on the
+host, single-threaded: with more than one OpenMP thread the reference's face
+loops race on a shared variable (source/Upwind_Convective.cpp:7,29) and give
+wrong fluxes.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "3d-fluid-numerical-simulation-software_amd"))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+
+# Algorithmic HBM bytes per cell and launch (FP64), each array a kernel
+# semantically reads or writes counted once; see DESIGN.md.
+BYTES_PER_CELL = {
+    "residual": 8 * (5 + 9 + 3 + 3 + 1 + 5),   # W, face S, |S| x3, nodes, vol; write R
+    "lusgs": 2 * 8 * (5 + 5 + 1 + 3 + 1 + 1 + 9 + 3 + 5),  # two sweeps: R|dQ*, Q, p, lamc, dt, vol, S, |S|; write dQ
+    "update": 8 * (5 + 5 + 1 + 5 + 6),         # Q, dQ, vol; write Q, W
+    "bc_spectral_dt": 8 * (4 + 3 + 18 + 1 + 3 + 1),  # W(u,v,w,T), centroid, S x6, vol; write lamc, dt
+}
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--ni", type=int, default=64)
+    ap.add_argument("--nj", type=int, default=64)
+    ap.add_argument("--nk", type=int, default=256)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--sweep", choices=["kba2", "kba", "plane"], default="kba2")
+    ap.add_argument("--chunk", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=40)
+    return ap.parse_args()
+
+
+def cpu_baseline(iters: int):
+    """Reference solver on the C0 bump channel (96x32x32 cells), one thread."""
+    import oracle as O
+    from fv3d import cases
+    if not os.path.exists(O.REF_BIN):
+        return None
+    case = cases.bump_channel(ni=32, nj=32, nk=96, seed=1, max_iter=iters)
+    with tempfile.TemporaryDirectory() as td:
+        cdir, odir = os.path.join(td, "case"), os.path.join(td, "out")
+        case.write(cdir)
+        r = O.run_reference(cdir, iters, odir, threads=1, dump=False, timeout=600)
+    t = r["timing"]
+    return {"value": t["cells"] * t["iterations"] / t["loop_seconds"], "unit": "cell-updates/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"reference solver (oracle/_ref, -O2, 1 thread) on the 96x32x32 bump channel, "
+                      f"{t['iterations']} LU-SGS iterations, {t['loop_seconds']:.2f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    from fv3d import cases
+    from fv3d.solver import GpuSolver
+
+    case = cases.bump_channel(ni=a.ni, nj=a.nj, nk=a.nk, seed=1, max_iter=a.steps)
+    g = GpuSolver(case, device=0, debug={"plane": 2, "kba": 4, "kba2": 0}[a.sweep] | (a.chunk << 8))
+    ncell = g.n_cells
+    graph = not a.no_graph
+    g.iterate(a.warmup, graph=graph)
+    g.sync()
+    t0 = time.perf_counter()
+    hist = g.iterate(a.steps, graph=graph)
+    g.sync()
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    value = ncell * a.steps / secs
+    # per-kernel device times from HIP events on the library's stream
+    g.set_timing(True)
+    g.iterate(max(5, min(a.steps, 20)), graph=False)
+    ms, _ = g.timing()
+    g.set_timing(False)
+    classes = ["residual", "lusgs", "update", "bc_spectral_dt"]
+    times = {cl: ms[i] for i, cl in enumerate(classes)}
+    dom = max(times, key=times.get)
+    achieved = BYTES_PER_CELL[dom] * ncell / (times[dom] * 1e-3) / 1e9
+    cpu = None if a.no_cpu_baseline else cpu_baseline(a.cpu_iters)
+    out = {
+        "metric": "cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * secs / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"inviscid bump channel {a.nk}x{a.nj}x{a.ni} cells (single block), Roe + MUSCL3 "
+                               f"primitive, LU-SGS local dt CFL 5, plain FP64",
+                   "cells": ncell, "hip_graph": graph, "sweep": a.sweep, "chunk": a.chunk},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel_ms": times, "bytes_per_cell": BYTES_PER_CELL[dom]},
+        "cpu_baseline": cpu,
+        "final_res_dens": float(hist[-1, 1]),
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

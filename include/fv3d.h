@@ -1,0 +1,137 @@
+/* libfv3d -- C ABI of the MI355X per-iteration path of a structured
+ * multi-block finite-volume turbomachinery solver.
+ *
+ * Every entry point returns 0 on success and a negative fv3d_status code on
+ * failure; fv3d_last_error() then holds a message.  The library never exits
+ * the process.  Calls on one context are stream-ordered on that context's HIP
+ * stream; only the functions that return host data synchronise.  One context
+ * per host thread.  Host pointers are borrowed for the duration of the call.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   the CMesh / CSpaceDiscr / CTimeDiscr / CSolver method surface of
+ *   include/Mesh.h:31-90, include/SpaceDiscr.h:62-69, include/TimeDiscr.h:43-49,
+ *   include/Solver.h:46-52.  The reference has no FFI; its classes call each
+ *   other directly.  The C++ classes in
+ *   3d-fluid-numerical-simulation-software_amd/csrc/fv3d_classes.h keep those
+ *   names and forward to the entry points below.
+ */
+#ifndef FV3D_H
+#define FV3D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fv3d_ctx fv3d_ctx;
+
+enum fv3d_status {
+    FV3D_OK = 0,
+    FV3D_EINVAL = -1,      /* bad argument or unsupported option */
+    FV3D_ESTATE = -2,      /* call out of order (e.g. before fv3d_setup) */
+    FV3D_EHIP = -3,        /* HIP runtime error */
+    FV3D_ENOMEM = -4,
+    FV3D_ENUMERIC = -5     /* non-finite residual (the reference's divergence print) */
+};
+
+enum fv3d_precision {
+    FV3D_PLAIN_FP64 = 0,   /* every kernel in FP64 */
+    FV3D_DEFECT_FP32 = 1   /* FP64 residual/update/norms, FP32 LU-SGS operator */
+};
+
+/* Raw config.dat values, as parsed by CUserInput::ReadConfig
+ * (source/UserInput.cpp:17-239); the library applies the reference's float
+ * parsing and non-dimensionalisation (source/UserInput.cpp:1070-1092). */
+typedef struct {
+    double rgas, gamma, pr_lam, pr_tur;
+    double pt_in, tt_in, p_out, ma0, len_ref, ev_ratio;
+    double inlet_dir[3];
+    double cfl, c_timestep, relax;
+    int turbo_type;     /* 0 axial, 1 centrifugal, 2 general outlet */
+    int viscous;        /* 0 Euler, 1 Navier-Stokes */
+    int recon_var;      /* 100 conservative, 200 primitive */
+    int recon_scheme;   /* 100 NND, 200 MUSCL2u, 300 MUSCL2c, 400 MUSCL2o, 500 MUSCL3 */
+    int flux_scheme;    /* 200 Steger-Warming, 300 Roe */
+    int turbulence;     /* 0 none */
+    int limiter;        /* limit flow variables (source/Limit_FlowVars.cpp:8-119) */
+    int global_dt;      /* 0 local time step */
+    int device;         /* HIP device ordinal */
+    int precision;      /* enum fv3d_precision */
+} fv3d_config;
+
+/* Context life cycle. */
+int fv3d_ctx_create(const fv3d_config *cfg, fv3d_ctx **out);
+int fv3d_ctx_destroy(fv3d_ctx *ctx);
+const char *fv3d_last_error(const fv3d_ctx *ctx);
+
+/* Adds a block (CUserInput::ReadGrids, source/UserInput.cpp:241-286 and the
+ * block table, 439-445).  nodes: IM*JM*KM*3 doubles, node (i,j,k) 1-based at
+ * ((i-1)*JM+(j-1))*KM+(k-1).  Returns the 1-based block id (>0) or an error. */
+int fv3d_block_add(fv3d_ctx *ctx, int IM, int JM, int KM, const double *nodes, int nblade, double rpm);
+
+/* Adds a patch (CUserInput::ReadBndGrid, source/UserInput.cpp:336-432).
+ * kind 0 inlet, 1 outlet, 2 wall; rng = is ie js je ks ke. */
+int fv3d_patch_add(fv3d_ctx *ctx, int kind, int block, int face, const int *rng, int rotate);
+
+/* Metrics (CGeometry::FaceVectorVolume, source/GeometryVariables.cpp:10-105),
+ * device allocation and launch plans.  Must follow the last block/patch add. */
+int fv3d_setup(fv3d_ctx *ctx);
+
+/* Restart-file state (CUserInput::ReadSolution, source/UserInput.cpp:1118-1159):
+ * pv (ni,nj,nk,6) = rho,u,v,w,p,T and ener (ni,nj,nk), i slowest. */
+int fv3d_state_upload(fv3d_ctx *ctx, int block, const double *pv, const double *ener);
+
+/* Copies a field to the host (synchronises).  Names and shapes:
+ *   cv (IM+1,JM+1,KM+1,5), pv (IM+1,JM+1,KM+1,6)  -- cell arrays incl. ghost cells
+ *   res dq (ni,nj,nk,5); dt lamcI lamcJ lamcK vol (ni,nj,nk)
+ *   fcI (IM,nj,nk,5) fcJ (ni,JM,nk,5) fcK (ni,nj,KM,5)  -- face fluxes
+ *   dIS_cv dIP_cv (2,nj,nk,5), dJS_cv dJP_cv (ni,2,nk,5), dKS_cv dKP_cv (ni,nj,2,5)
+ *   and the same with _pv and 6 components -- Dummy layers 1..2.
+ * Returns the number of doubles written. */
+long fv3d_field_download(fv3d_ctx *ctx, int block, const char *name, double *out);
+
+/* Stages of CMesh::ResFlux (source/ComputeAllBlockVars.cpp:223-255). */
+int fv3d_bc_apply(fv3d_ctx *ctx);          /* CMesh::BoundaryCondition */
+int fv3d_spectral(fv3d_ctx *ctx);          /* CFluidProps::SpectralRadii */
+int fv3d_timestep(fv3d_ctx *ctx);          /* CTimeDiscr::TimeStep */
+int fv3d_convective(fv3d_ctx *ctx);        /* CMesh::ConvectiveFlux incl. WallFlux, source term and Res */
+int fv3d_resflux(fv3d_ctx *ctx);           /* all of CMesh::ResFlux */
+
+/* CMesh::LU_SGS_Marching (source/LU_SGS_Marching.cpp:10-291): save, ResFlux,
+ * forward and backward hyperplane sweeps, update, PrimitiveVars, Limit_Vars. */
+int fv3d_lusgs(fv3d_ctx *ctx);
+
+/* CMesh::Res_Calculate (source/ComputeAllBlockVars.cpp:318-372); synchronises. */
+int fv3d_res_calculate(fv3d_ctx *ctx, double *res6);
+
+/* COutput::OutputGeneralParameters (source/Ouput.cpp:282-300); synchronises.
+ * out5 = mass_in, mass_out, PR, efficiency, mass-flow ratio. */
+int fv3d_perf(fv3d_ctx *ctx, double *out5);
+
+/* n iterations of CSolver::Solver (source/Solver.cpp:47-66) with the CFL ramp
+ * kept on the device, each followed by the performance integrals.  hist (may be
+ * NULL) receives n rows of: iter res_dens mass_in mass_out PR eff CFL.
+ * Synchronises once at the end. */
+int fv3d_iterate(fv3d_ctx *ctx, int n, double *hist);
+
+/* Same as fv3d_iterate, but replays one captured HIP graph per iteration. */
+int fv3d_iterate_graph(fv3d_ctx *ctx, int n, double *hist);
+
+int fv3d_sync(fv3d_ctx *ctx);
+
+/* Average device time in milliseconds of the last timed iterations, per
+ * kernel class (0 residual, 1 LU-SGS sweeps, 2 update, 3 ghost fill + spectral
+ * + time step, 4 norms/integrals).  Filled by fv3d_iterate when timing is on. */
+int fv3d_set_timing(fv3d_ctx *ctx, int on);
+int fv3d_timing(fv3d_ctx *ctx, double *ms5, long *counts5);
+
+/* Debug flags: bit 0 keeps the face fluxes of the last residual evaluation
+ * so fv3d_field_download can return fcI/fcJ/fcK (single-block meshes). */
+int fv3d_set_debug(fv3d_ctx *ctx, int flags);
+
+/* Number of interior cells over all blocks. */
+long fv3d_cell_count(const fv3d_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
