@@ -71,7 +71,8 @@ struct fv3d_ctx {
     int nkba = 0;
     int *d_kba_sync = nullptr;
     int kba_chunk = 1;
-    int sweep_mode = 2;   // 0 one launch per hyperplane, 1 wavefront, 2 prefetching wavefront
+    int kba_lag = 0;
+    int sweep_mode = 3;   // 0 one launch per hyperplane, 1..3 wavefront forms (3: four waves per tile)
     int kba_grid = 0;
     // graph
     hipGraphExec_t graph_exec = nullptr;
@@ -504,7 +505,7 @@ int fv3d_setup(fv3d_ctx *c)
             HIPCK(c, hipMemcpyAsync(c->d_kba[dir], kt.data(), kt.size() * sizeof(KbaTile), hipMemcpyHostToDevice, c->stream));
         }
         c->nkba = (int)lst.size();
-        if ((rc = dalloc(c, &c->d_kba_sync, 3 + 2 * (size_t)c->nkba))) return rc;
+        if ((rc = dalloc(c, &c->d_kba_sync, 64 + 2 * (size_t)c->nkba * kDonePad))) return rc;
         c->kba_grid = std::min(c->nkba, 4096);
     }
     c->npart = (int)((c->ncell + 255) / 256);
@@ -584,15 +585,23 @@ static int launch_convective(fv3d_ctx *c)
 
 static int launch_sweeps_kba(fv3d_ctx *c)
 {
-    long nsync = 2 + 2L * c->nkba;
-    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((nsync + 255) / 256)), dim3(256), 0, c->stream, c->d_kba_sync + 1, nsync, 0);
-    int *err = c->d_kba_sync, *tf = c->d_kba_sync + 1, *tb = c->d_kba_sync + 2;
-    int *df = c->d_kba_sync + 3, *db = df + c->nkba;
-    if (c->sweep_mode == 2) {
+    // layout: err at 0, tickets at 32 and 48, padded counters from 64
+    long nsync = 64 + 2L * c->nkba * kDonePad - 32;
+    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((nsync + 255) / 256)), dim3(256), 0, c->stream, c->d_kba_sync + 32, nsync, 0);
+    int *err = c->d_kba_sync, *tf = c->d_kba_sync + 32, *tb = c->d_kba_sync + 48;
+    int *df = c->d_kba_sync + 64, *db = df + (long)c->nkba * kDonePad;
+    if (c->sweep_mode == 3) {
+        hipLaunchKernelGGL((k_lusgs_kba3<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
+                           c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_lag);
+        hipLaunchKernelGGL((k_lusgs_kba3<false, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
+                           c->d_blocks, c->d_kba[1], c->nkba, tb, db, err, c->gas.relax, c->gas, c->kba_lag);
+    } else if (c->sweep_mode == 2) {
         hipLaunchKernelGGL((k_lusgs_kba2<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(kKbaTJ * kKbaTK), 0,
-                           c->stream, c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas);
+                           c->stream, c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_chunk,
+                           c->kba_lag);
         hipLaunchKernelGGL((k_lusgs_kba2<false, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(kKbaTJ * kKbaTK), 0,
-                           c->stream, c->d_blocks, c->d_kba[1], c->nkba, tb, db, err, c->gas.relax, c->gas);
+                           c->stream, c->d_blocks, c->d_kba[1], c->nkba, tb, db, err, c->gas.relax, c->gas, c->kba_chunk,
+                           c->kba_lag);
     } else {
         hipLaunchKernelGGL((k_lusgs_kba<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(kKbaTJ * kKbaTK), 0,
                            c->stream, c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_chunk);
@@ -855,8 +864,9 @@ int fv3d_set_debug(fv3d_ctx *c, int flags)
 {
     READY(c);
     c->store_faces = flags & 1;
-    c->sweep_mode = (flags & 2) ? 0 : (flags & 4) ? 1 : 2;
+    c->sweep_mode = (flags & 2) ? 0 : (flags & 4) ? 1 : (flags & 8) ? 2 : 3;
     if ((flags >> 8) & 0xff) c->kba_chunk = (flags >> 8) & 0xff;
+    c->kba_lag = (flags >> 16) & 0xff;
     if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
     if (c->store_faces && !c->d_faces[0]) {
         long mx = 0;

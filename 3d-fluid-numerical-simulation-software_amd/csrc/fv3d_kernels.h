@@ -45,13 +45,17 @@ __global__ void k_lusgs_plane(const BlockDev *blocks, const long *pstart, int nb
 struct KbaTile {
     int block, j0, k0, dep_j, dep_k;
 };
-constexpr int kKbaTJ = 8, kKbaTK = 8;
+constexpr int kKbaTJ = 4, kKbaTK = 16;
+constexpr int kDonePad = 32;   // progress counters one per 128-byte line
 template <bool FWD, int TJ, int TK>
 __global__ void k_lusgs_kba(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *done, int *err,
                             double relax, Gas g, int chunk);
 template <bool FWD, int TJ, int TK>
 __global__ void k_lusgs_kba2(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *done, int *err,
-                             double relax, Gas g);
+                             double relax, Gas g, int chunk, int lag);
+template <bool FWD, int TJ, int TK>
+__global__ void k_lusgs_kba3(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *done, int *err,
+                             double relax, Gas g, int lag);
 __global__ void k_fill_int(int *p, long n, int v);
 __global__ void k_update(const BlockDev *blocks, const long *cstart, int nblock, Gas g, double *partial, int *bad_flag);
 __global__ void k_limiter_serial(const BlockDev *blocks, int nblock, Gas g, int *bad_flag);

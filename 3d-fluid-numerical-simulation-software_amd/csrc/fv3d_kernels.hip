@@ -500,13 +500,13 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_kba(const BlockDev *__restrict
             if (seen_j < need_j || seen_k < need_k) {
                 long spins = 0;
                 while (seen_j < need_j) {
-                    seen_j = __hip_atomic_load(done + tl.dep_j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    seen_j = __hip_atomic_load(done + tl.dep_j * kDonePad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (seen_j >= need_j) break;
                     __builtin_amdgcn_s_sleep(1);
                     if (++spins > (1L << 22)) { if (t == 0) atomicExch(err, 1); seen_j = 1 << 30; }
                 }
                 while (seen_k < need_k) {
-                    seen_k = __hip_atomic_load(done + tl.dep_k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    seen_k = __hip_atomic_load(done + tl.dep_k * kDonePad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (seen_k >= need_k) break;
                     __builtin_amdgcn_s_sleep(1);
                     if (++spins > (1L << 22)) { if (t == 0) atomicExch(err, 1); seen_k = 1 << 30; }
@@ -588,14 +588,239 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_kba(const BlockDev *__restrict
             }
             __syncthreads();
             if ((s + 1) % chunk == 0 || s + 1 == nsteps) {
-                if (t == 0) __hip_atomic_store(done + q, s + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                if (t == 0) __hip_atomic_store(done + q * kDonePad, s + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
     }
 }
 
-template __global__ void k_lusgs_kba<true, 8, 8>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
-template __global__ void k_lusgs_kba<false, 8, 8>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
+template __global__ void k_lusgs_kba<true, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
+template __global__ void k_lusgs_kba<false, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
+
+// Third form of the wavefront sweep: four waves per tile of TJ x TK = 64
+// cells.  Waves 0, 1, 2 each own one neighbour direction (i, j, k): one step
+// ahead they load the neighbour's state and metrics (wave 0 also the cell's own
+// right-hand side and diagonal), then compute the flux difference and the
+// off-diagonal terms of the step's 64 cells and leave them in LDS.  Wave 3
+// combines them into dQ, keeps dQ in LDS for the next step and stores it.
+// Wave 3 issues no loads, so before it stores step s it waits only for the
+// stores of step s-1 and then publishes s completed steps.
+//
+// Each wave kind runs its own straight-line loop (the direction loops are
+// specialised per direction at compile time) and every load is issued
+// unconditionally with indices clamped into the block: the compiler's
+// memory-counter waits are then exact, so a load issued one step ahead is
+// waited for only when its value is used.  The loop is unrolled by two over
+// two prefetch buffers so that no prefetched value is copied in flight.
+typedef const double __attribute__((address_space(1))) *gcd_t;
+typedef double __attribute__((address_space(1))) *gd_t;
+typedef int __attribute__((address_space(1))) *gi_t;
+#define GLD(p, idx) (((gcd_t)(p))[idx])
+
+namespace {
+struct SweepPf {
+    double q[5], p, l, m[5], x[5], r[5], a;
+    int pend;
+};
+template <int D>
+struct Dir {
+    static constexpr int value = D;
+};
+}  // namespace
+
+template <bool FWD, int TJ, int TK>
+__global__ void __launch_bounds__(256) k_lusgs_kba3(const BlockDev *__restrict__ blocks, const KbaTile *__restrict__ tiles,
+                                                    int ntiles, int *__restrict__ ticket, int *__restrict__ done,
+                                                    int *__restrict__ err, double relax, Gas g, int lag)
+{
+    static_assert(TJ * TK == 64, "64 cells per tile");
+    __shared__ double sdq[2][5][64];
+    __shared__ double sA[3][5][64];
+    __shared__ double sV[3][5][64];
+    __shared__ double sR[6][64];
+    __shared__ int s_q;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    const int tj = lane / TK, tk = lane % TK;
+    const double Gamma = g.gamma;
+    for (;;) {
+        if (threadIdx.x == 0) s_q = atomicAdd(ticket, 1);
+        __syncthreads();
+        const int q = __builtin_amdgcn_readfirstlane(s_q);
+        __syncthreads();
+        if (q >= ntiles) return;
+        const KbaTile tl = tiles[q];
+        const BlockDev &B = blocks[tl.block];
+        const int ni = B.ni, nj = B.nj, nk = B.nk, si = B.si, sj = B.sj, box = (int)B.box;
+        const int lj = FWD ? tj : TJ - 1 - tj, lk = FWD ? tk : TK - 1 - tk;
+        const int j = tl.j0 + tj + 1, k = tl.k0 + tk + 1;
+        const bool col = j <= nj && k <= nk;
+        const int jk = min(j, nj) * sj + min(k, nk);
+        const int nsteps = ni + TJ + TK - 2;
+        // cell of this lane at a step, clamped into the block when the lane is idle
+        auto cell_at = [&](int step, int &i, int &x) -> bool {
+            const int l2 = step - lj - lk;
+            const int l2c = min(max(l2, 0), ni - 1);
+            i = FWD ? l2c + 1 : ni - l2c;
+            x = i * si + jk;
+            return col && l2 >= 0 && l2 < ni;
+        };
+        auto bar = []() {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        };
+        auto dir_loop = [&](auto dc) {
+            constexpr int d = decltype(dc)::value;
+            constexpr int T = d == 1 ? TJ : TK;
+            const gcd_t Qg = (gcd_t)B.Q, Wg = (gcd_t)B.W, Lg = (gcd_t)B.lamc, Mg = (gcd_t)B.swg, Ag = (gcd_t)B.alpha;
+            const gcd_t Rg = (gcd_t)(FWD ? (const double *)B.R : (const double *)B.dQ);
+            const gd_t Dg = (gd_t)B.dQ;
+            const int st = d == 0 ? si : d == 1 ? sj : 1;
+            const int dep = d == 1 ? tl.dep_j : d == 2 ? tl.dep_k : -1;
+            const bool edge = d == 1 ? tj == (FWD ? 0 : TJ - 1) : d == 2 ? tk == (FWD ? 0 : TK - 1) : false;
+            const gi_t dep_ctr = (gi_t)(done + max(dep, 0) * kDonePad);
+            int seen = dep < 0 ? 1 << 30 : 0;
+            // wait for the upstream tile, then load the data of `step`
+            auto issue = [&](SweepPf &bn, int step, int pend) {
+                if (d != 0 && dep >= 0) {
+                    seen = max(seen, pend);
+                    const int need = min(step + T + lag, nsteps);
+                    long spins = 0;
+                    while (seen < need) {
+                        seen = __hip_atomic_load(dep_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (seen >= need) break;
+                        __builtin_amdgcn_s_sleep(1);
+                        if (++spins > (1L << 22)) { atomicExch(err, 1); seen = 1 << 30; }
+                    }
+                }
+                int i, x;
+                cell_at(step, i, x);
+                const int nb = FWD ? x - st : x + st;
+                for (int c = 0; c < 5; ++c) bn.q[c] = Qg[c * box + nb];
+                bn.p = Wg[4 * box + nb];
+                bn.l = Lg[d * box + nb];
+                for (int c = 0; c < 5; ++c) bn.m[c] = Mg[(d * 5 + c) * box + nb];
+                if (d == 0) {
+                    for (int c = 0; c < 5; ++c) bn.r[c] = Rg[c * box + x];
+                    bn.a = Ag[x];
+                    bn.pend = 0;
+                } else {
+                    // upstream dQ for the edge lanes (the other lanes' values go unused)
+                    for (int c = 0; c < 5; ++c)
+                        bn.x[c] = __hip_atomic_load(Dg + c * box + nb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    // early look at the upstream counter, consumed by the next wait
+                    bn.pend = __hip_atomic_load(dep_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            };
+            auto work = [&](const SweepPf &b, int s) {
+                int i, x;
+                const bool valid = cell_at(s, i, x);
+                const int p = d == 0 ? i : d == 1 ? j : k, lim = d == 0 ? ni : d == 1 ? nj : nk;
+                const bool act = FWD ? p != 1 : p != lim;
+                double A[5], V[5];
+                if (valid && act) {
+                    double dqn[5];
+                    if (d == 0) {
+                        for (int c = 0; c < 5; ++c) dqn[c] = sdq[(s - 1) & 1][c][lane];
+                    } else if (!edge) {
+                        const int pl = d == 1 ? lane + (FWD ? -TK : TK) : lane + (FWD ? -1 : 1);
+                        for (int c = 0; c < 5; ++c) dqn[c] = sdq[(s - 1) & 1][c][pl];
+                    } else {
+                        for (int c = 0; c < 5; ++c) dqn[c] = b.x[c];
+                    }
+                    const double nx = b.m[0], ny = b.m[1], nz = b.m[2], sd = b.m[3], Vn_e = b.m[4];
+                    const double *o = b.q;
+                    const double cL = relax * b.l;
+                    double nw[5];
+                    for (int c = 0; c < 5; ++c) nw[c] = o[c] + dqn[c];
+                    double dens = nw[0], uvel = nw[1] / nw[0], vvel = nw[2] / nw[0], wvel = nw[3] / nw[0], E = nw[4] / nw[0];
+                    double press_new = (Gamma - F32C(1.0)) * dens * (E - F32C(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel));
+                    double press_old = b.p;
+                    double Vn_old = o[1] / o[0] * nx + o[2] / o[0] * ny + o[3] / o[0] * nz;
+                    double Vn_new = uvel * nx + vvel * ny + wvel * nz;
+                    double df[5];
+                    df[0] = nw[0] * (Vn_new - Vn_e) - o[0] * (Vn_old - Vn_e);
+                    df[1] = (nw[1] * (Vn_new - Vn_e) + press_new * nx) - (o[1] * (Vn_old - Vn_e) + press_old * nx);
+                    df[2] = (nw[2] * (Vn_new - Vn_e) + press_new * ny) - (o[2] * (Vn_old - Vn_e) + press_old * ny);
+                    df[3] = (nw[3] * (Vn_new - Vn_e) + press_new * nz) - (o[3] * (Vn_old - Vn_e) + press_old * nz);
+                    df[4] = ((nw[4] + press_new) * (Vn_new - Vn_e) + press_new * Vn_e) -
+                            ((o[4] + press_old) * (Vn_old - Vn_e) + press_old * Vn_e);
+                    for (int c = 0; c < 5; ++c) {
+                        A[c] = FWD ? (df[c] * sd + dqn[c] * cL) * F32C(0.5) : (df[c] * sd - dqn[c] * cL) * F32C(0.5);
+                        V[c] = dqn[c] * 0.0;   // lamv * dQ, lamv = 0 on the inviscid path
+                    }
+                } else {
+                    for (int c = 0; c < 5; ++c) { A[c] = 0.0; V[c] = 0.0; }
+                }
+                for (int c = 0; c < 5; ++c) { sA[d][c][lane] = A[c]; sV[d][c][lane] = V[c]; }
+                if (d == 0) {
+                    for (int c = 0; c < 5; ++c) sR[c][lane] = b.r[c];
+                    sR[5][lane] = b.a;
+                }
+                bar();   // terms of step s are in LDS
+                bar();   // dQ of step s is in LDS
+            };
+            SweepPf b0, b1;
+            issue(b0, 0, 0);
+            for (int s = 0; s < nsteps; s += 2) {
+                issue(b1, s + 1, b0.pend);
+                work(b0, s);
+                if (s + 1 >= nsteps) break;
+                issue(b0, s + 2, b1.pend);
+                work(b1, s + 1);
+            }
+        };
+        if (wave == 0) {
+            dir_loop(Dir<0>());
+        } else if (wave == 1) {
+            dir_loop(Dir<1>());
+        } else if (wave == 2) {
+            dir_loop(Dir<2>());
+        } else {
+            // ---- combine wave (source/LU_SGS_Marching.cpp:155, 255)
+            const gd_t Dg = (gd_t)B.dQ;
+            const bool pub = (tj == (FWD ? TJ - 1 : 0)) || (tk == (FWD ? TK - 1 : 0));
+            const gi_t my_ctr = (gi_t)(done + q * kDonePad);
+            for (int s = 0; s < nsteps; ++s) {
+                int i, x;
+                const bool valid = cell_at(s, i, x);
+                double outv[5];
+                bar();
+                if (valid) {
+                    const double alpha = sR[5][lane];
+                    for (int c = 0; c < 5; ++c) {
+                        double v;
+                        if (FWD) {
+                            v = sR[c][lane] * F32C(-1.0) + sA[0][c][lane];
+                            v = v + sA[1][c][lane]; v = v + sA[2][c][lane];
+                        } else {
+                            v = sR[c][lane] * alpha - sA[0][c][lane];
+                            v = v - sA[1][c][lane]; v = v - sA[2][c][lane];
+                        }
+                        v = v + sV[0][c][lane]; v = v + sV[1][c][lane]; v = v + sV[2][c][lane];
+                        outv[c] = v / alpha;
+                        sdq[s & 1][c][lane] = outv[c];
+                    }
+                }
+                bar();
+                // the stores of step s-1 have completed: publish s finished steps
+                __builtin_amdgcn_s_waitcnt(0);
+                if (s > 0 && lane == 0) __hip_atomic_store(my_ctr, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (valid) {
+                    for (int c = 0; c < 5; ++c) {
+                        if (pub) __hip_atomic_store(Dg + c * box + x, outv[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        else Dg[c * box + x] = outv[c];
+                    }
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            if (lane == 0) __hip_atomic_store(my_ctr, nsteps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+template __global__ void k_lusgs_kba3<true, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
+template __global__ void k_lusgs_kba3<false, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
 
 // Clears the ticket, error flag and progress counters of one sweep.
 __global__ void k_fill_int(int *p, long n, int v)
@@ -638,7 +863,7 @@ __device__ inline void sweep_load(const BlockDev &B, long x, int i, int j, int k
 template <bool FWD, int TJ, int TK>
 __global__ void __launch_bounds__(TJ *TK) k_lusgs_kba2(const BlockDev *__restrict__ blocks, const KbaTile *__restrict__ tiles,
                                                        int ntiles, int *__restrict__ ticket, int *__restrict__ done,
-                                                       int *__restrict__ err, double relax, Gas g)
+                                                       int *__restrict__ err, double relax, Gas g, int chunk, int lag)
 {
     static_assert(TJ * TK == 64, "one wave per tile");
     __shared__ double sdq[2][5][TJ][TK];
@@ -658,46 +883,75 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_kba2(const BlockDev *__restric
         const bool col = j <= B.nj && k <= B.nk;
         const int nsteps = B.ni + TJ + TK - 2;
         const bool edge_j = tj == (FWD ? 0 : TJ - 1), edge_k = tk == (FWD ? 0 : TK - 1);
-        // downstream tile edges publish through coherent stores
+        const bool xj = edge_j && tl.dep_j >= 0, xk = edge_k && tl.dep_k >= 0;
         const bool pub = (tj == (FWD ? TJ - 1 : 0)) || (tk == (FWD ? TK - 1 : 0));
+        const long sj = B.sj;
         int seen_j = tl.dep_j < 0 ? 1 << 30 : 0, seen_k = tl.dep_k < 0 ? 1 << 30 : 0;
+        int pend_j = seen_j, pend_k = seen_k;
         double prev[5] = {0, 0, 0, 0, 0};
+        double xnj[5] = {0, 0, 0, 0, 0}, xnk[5] = {0, 0, 0, 0, 0};
         SweepIn nxt;
+        // waits until the upstream tiles have published the steps that step
+        // `st` of this tile reads, plus `lag` further steps of slack
+        auto ensure = [&](int st) {
+            const int need_j = min(st + TJ + lag, nsteps), need_k = min(st + TK + lag, nsteps);
+            seen_j = max(seen_j, pend_j);
+            seen_k = max(seen_k, pend_k);
+            long spins = 0;
+            while (seen_j < need_j) {
+                seen_j = __hip_atomic_load(done + tl.dep_j * kDonePad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (seen_j >= need_j) break;
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > (1L << 22)) { if (t == 0) atomicExch(err, 1); seen_j = 1 << 30; }
+            }
+            while (seen_k < need_k) {
+                seen_k = __hip_atomic_load(done + tl.dep_k * kDonePad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (seen_k >= need_k) break;
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > (1L << 22)) { if (t == 0) atomicExch(err, 1); seen_k = 1 << 30; }
+            }
+        };
+        // tile-edge increments of the upstream tiles read by step `st`
+        auto load_cross = [&](int st) {
+            const int l2 = st - lj - lk, i2 = FWD ? l2 + 1 : B.ni - l2;
+            if (col && l2 >= 0 && l2 < B.ni) {
+                const long x2 = cidx(B, i2, j, k);
+                if (xj)
+                    for (int c = 0; c < 5; ++c)
+                        xnj[c] = __hip_atomic_load(B.dQ + c * box + (FWD ? x2 - sj : x2 + sj), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                if (xk)
+                    for (int c = 0; c < 5; ++c)
+                        xnk[c] = __hip_atomic_load(B.dQ + c * box + (FWD ? x2 - 1 : x2 + 1), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        };
         int li = -lj - lk;
         {
             const int i = FWD ? li + 1 : B.ni - li;
             if (col && li >= 0 && li < B.ni) sweep_load<FWD>(B, cidx(B, i, j, k), i, j, k, relax, nxt);
         }
+        ensure(0);
+        load_cross(0);
         for (int s = 0; s < nsteps; ++s, ++li) {
             const int i = FWD ? li + 1 : B.ni - li;
             const bool valid = col && li >= 0 && li < B.ni;
-            SweepIn cur = nxt;
-            // prefetch the next step of this column
-            {
+            const SweepIn cur = nxt;
+            double xcj[5], xck[5];
+            for (int c = 0; c < 5; ++c) { xcj[c] = xnj[c]; xck[c] = xnk[c]; }
+            // next step: availability, tile-edge increments, static inputs, and a non-blocking poll
+            if (s + 1 < nsteps) {
+                ensure(s + 1);
+                load_cross(s + 1);
                 const int li2 = li + 1, i2 = FWD ? li2 + 1 : B.ni - li2;
                 if (col && li2 >= 0 && li2 < B.ni) sweep_load<FWD>(B, cidx(B, i2, j, k), i2, j, k, relax, nxt);
-            }
-            const int need_j = min(s + TJ, nsteps), need_k = min(s + TK, nsteps);
-            if (seen_j < need_j || seen_k < need_k) {
-                long spins = 0;
-                while (seen_j < need_j) {
-                    seen_j = __hip_atomic_load(done + tl.dep_j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (seen_j >= need_j) break;
-                    __builtin_amdgcn_s_sleep(1);
-                    if (++spins > (1L << 22)) { if (t == 0) atomicExch(err, 1); seen_j = 1 << 30; }
-                }
-                while (seen_k < need_k) {
-                    seen_k = __hip_atomic_load(done + tl.dep_k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (seen_k >= need_k) break;
-                    __builtin_amdgcn_s_sleep(1);
-                    if (++spins > (1L << 22)) { if (t == 0) atomicExch(err, 1); seen_k = 1 << 30; }
-                }
+                if (seen_j < nsteps) pend_j = __hip_atomic_load(done + tl.dep_j * kDonePad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (seen_k < nsteps) pend_k = __hip_atomic_load(done + tl.dep_k * kDonePad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             if (valid) {
                 const long x = cidx(B, i, j, k);
                 const int pos[3] = {i, j, k};
                 const int lim[3] = {B.ni, B.nj, B.nk};
-                const long st[3] = {B.si, B.sj, 1};
                 const double alpha = cur.own[5];
                 double A[3][5], V[3][5];
                 for (int d = 0; d < 3; ++d) {
@@ -713,9 +967,7 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_kba2(const BlockDev *__restric
                         const int pj = d == 1 ? tj + (FWD ? -1 : 1) : tj, pk = d == 2 ? tk + (FWD ? -1 : 1) : tk;
                         for (int c = 0; c < 5; ++c) dqn[c] = sdq[(s - 1) & 1][c][pj][pk];
                     } else {
-                        const long nb = FWD ? x - st[d] : x + st[d];
-                        for (int c = 0; c < 5; ++c)
-                            dqn[c] = __hip_atomic_load(B.dQ + c * box + nb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        for (int c = 0; c < 5; ++c) dqn[c] = d == 1 ? xcj[c] : xck[c];
                     }
                     const double *o = cur.q[d];
                     const double nx = cur.gm[d][0], ny = cur.gm[d][1], nz = cur.gm[d][2], sd = cur.gm[d][3];
@@ -757,15 +1009,21 @@ __global__ void __launch_bounds__(TJ *TK) k_lusgs_kba2(const BlockDev *__restric
                     prev[c] = out;
                 }
             }
-            __builtin_amdgcn_s_waitcnt(0);
-            __syncthreads();
-            if (t == 0) __hip_atomic_store(done + q, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // neighbours in this wave read the increment from LDS next step
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+            if ((s + 1) % chunk == 0 || s + 1 == nsteps) {
+                // every edge store of the published steps has completed
+                __builtin_amdgcn_s_waitcnt(0);
+                if (t == 0) __hip_atomic_store(done + q * kDonePad, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
     }
 }
 
-template __global__ void k_lusgs_kba2<true, 8, 8>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas);
-template __global__ void k_lusgs_kba2<false, 8, 8>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas);
+template __global__ void k_lusgs_kba2<true, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int, int);
+template __global__ void k_lusgs_kba2<false, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int, int);
 
 // ===================================================== update + norms
 // Q += dQ (source/LU_SGS_Marching.cpp:262-287), Res_Calculate terms

@@ -45,8 +45,9 @@ def parse():
     ap.add_argument("--nj", type=int, default=64)
     ap.add_argument("--nk", type=int, default=256)
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--sweep", choices=["kba2", "kba", "plane"], default="kba2")
+    ap.add_argument("--sweep", choices=["kba3", "kba2", "kba", "plane"], default="kba3")
     ap.add_argument("--chunk", type=int, default=1)
+    ap.add_argument("--lag", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=40)
     return ap.parse_args()
@@ -80,7 +81,7 @@ def main():
     from fv3d.solver import GpuSolver
 
     case = cases.bump_channel(ni=a.ni, nj=a.nj, nk=a.nk, seed=1, max_iter=a.steps)
-    g = GpuSolver(case, device=0, debug={"plane": 2, "kba": 4, "kba2": 0}[a.sweep] | (a.chunk << 8))
+    g = GpuSolver(case, device=0, debug={"plane": 2, "kba": 4, "kba2": 8, "kba3": 0}[a.sweep] | (a.chunk << 8) | (a.lag << 16))
     ncell = g.n_cells
     graph = not a.no_graph
     g.iterate(a.warmup, graph=graph)
@@ -116,7 +117,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"inviscid bump channel {a.nk}x{a.nj}x{a.ni} cells (single block), Roe + MUSCL3 "
                                f"primitive, LU-SGS local dt CFL 5, plain FP64",
-                   "cells": ncell, "hip_graph": graph, "sweep": a.sweep, "chunk": a.chunk},
+                   "cells": ncell, "hip_graph": graph, "sweep": a.sweep, "chunk": a.chunk, "lag": a.lag},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel_ms": times, "bytes_per_cell": BYTES_PER_CELL[dom]},

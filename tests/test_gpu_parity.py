@@ -102,14 +102,17 @@ def test_wavefront_sweep_equals_plane_sweep():
     Gauss-Seidel result exactly.  Grid sizes leave partial 8x8 tiles."""
     case = cases.bump_channel(ni=19, nj=13, nk=29, seed=7, max_iter=3)
     print("case wavefront vs plane 19x13x29", flush=True)
-    a = GpuSolver(case)            # prefetching wavefront (default)
+    a = GpuSolver(case)            # four-wave wavefront (default)
     b = GpuSolver(case, debug=2)   # per-plane launches
     c = GpuSolver(case, debug=4)   # first wavefront form
+    e = GpuSolver(case, debug=8)   # prefetching one-wave wavefront
     c.call("lusgs")
+    e.call("lusgs")
     a.call("lusgs")
     b.call("lusgs")
     np.testing.assert_array_equal(a.get(1, "dq"), b.get(1, "dq"))
     np.testing.assert_array_equal(c.get(1, "dq"), b.get(1, "dq"))
+    np.testing.assert_array_equal(e.get(1, "dq"), b.get(1, "dq"))
     ha, hb = a.iterate(3), b.iterate(3)
     np.testing.assert_array_equal(ha, hb)
     o = O.Oracle(case)
@@ -117,11 +120,14 @@ def test_wavefront_sweep_equals_plane_sweep():
     assert relerr(a.get(1, "cv"), o.get(1, "cv")) <= 1e-11
 
 
-@pytest.mark.parametrize("chunk", [1, 4, 16])
-def test_wavefront_chunking(chunk):
+@pytest.mark.parametrize("form,chunk,lag", [(1, 1, 0), (1, 4, 0), (1, 16, 0), (2, 1, 0), (2, 4, 4), (2, 2, 7),
+                                            (3, 1, 0), (3, 1, 3)])
+def test_wavefront_chunking(form, chunk, lag):
+    """Chunked progress publishing and extra consumer lag change only the
+    timing of the wavefront sweep, never its result."""
     case = cases.bump_channel(ni=10, nj=17, nk=33, seed=8, max_iter=2)
-    print(f"case wavefront chunk {chunk}", flush=True)
-    a = GpuSolver(case, debug=4 | (chunk << 8))   # first wavefront form, chunked publishing
+    print(f"case wavefront form {form} chunk {chunk} lag {lag}", flush=True)
+    a = GpuSolver(case, debug={1: 4, 2: 8, 3: 0}[form] | (chunk << 8) | (lag << 16))
     b = GpuSolver(case, debug=2)
     np.testing.assert_array_equal(a.iterate(2), b.iterate(2))
     np.testing.assert_array_equal(a.get(1, "cv"), b.get(1, "cv"))
