@@ -626,6 +626,7 @@ template <int D>
 struct Dir {
     static constexpr int value = D;
 };
+constexpr int kSweepAhead = 1;   // prefetch distance of the direction waves, in steps
 }  // namespace
 
 template <bool FWD, int TJ, int TK>
@@ -656,6 +657,8 @@ __global__ void __launch_bounds__(256) k_lusgs_kba3(const BlockDev *__restrict__
         const bool col = j <= nj && k <= nk;
         const int jk = min(j, nj) * sj + min(k, nk);
         const int nsteps = ni + TJ + TK - 2;
+        constexpr int NB = kSweepAhead + 1;   // prefetch buffers = unrolling of the step loop
+        const int send = (nsteps + kSweepAhead + NB - 1) / NB * NB - kSweepAhead;
         // cell of this lane at a step, clamped into the block when the lane is idle
         auto cell_at = [&](int step, int &i, int &x) -> bool {
             const int l2 = step - lj - lk;
@@ -680,22 +683,38 @@ __global__ void __launch_bounds__(256) k_lusgs_kba3(const BlockDev *__restrict__
             const bool edge = d == 1 ? tj == (FWD ? 0 : TJ - 1) : d == 2 ? tk == (FWD ? 0 : TK - 1) : false;
             const gi_t dep_ctr = (gi_t)(done + max(dep, 0) * kDonePad);
             int seen = dep < 0 ? 1 << 30 : 0;
-            // wait for the upstream tile, then load the data of `step`
+            // wait for the upstream tile, then load everything `step` needs: the
+            // counter first, so that waiting for it later never waits for the
+            // data loads issued after it
             auto issue = [&](SweepPf &bn, int step, int pend) {
-                if (d != 0 && dep >= 0) {
+                if (d != 0) {
+                    // every path consumes the early look, and not before this point
+                    // (the empty asm keeps the compiler from hoisting the read)
+                    asm volatile("" : "+v"(pend));
                     seen = max(seen, pend);
-                    const int need = min(step + T + lag, nsteps);
-                    long spins = 0;
-                    while (seen < need) {
-                        seen = __hip_atomic_load(dep_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        if (seen >= need) break;
-                        __builtin_amdgcn_s_sleep(1);
-                        if (++spins > (1L << 22)) { atomicExch(err, 1); seen = 1 << 30; }
+                    if (dep >= 0) {
+                        const int need = min(step + T + lag, nsteps);
+                        long spins = 0;
+                        while (seen < need) {
+                            seen = __hip_atomic_load(dep_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            if (seen >= need) break;
+                            __builtin_amdgcn_s_sleep(1);
+                            if (++spins > (1L << 22)) { atomicExch(err, 1); seen = 1 << 30; }
+                        }
                     }
                 }
                 int i, x;
                 cell_at(step, i, x);
                 const int nb = FWD ? x - st : x + st;
+                if (d != 0) {
+                    // early look at the upstream counter, consumed kSweepAhead steps later
+                    bn.pend = __hip_atomic_load(dep_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    // upstream dQ for the edge lanes (the other lanes' values go unused)
+                    for (int c = 0; c < 5; ++c)
+                        bn.x[c] = __hip_atomic_load(Dg + c * box + nb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                } else {
+                    bn.pend = 0;
+                }
                 for (int c = 0; c < 5; ++c) bn.q[c] = Qg[c * box + nb];
                 bn.p = Wg[4 * box + nb];
                 bn.l = Lg[d * box + nb];
@@ -703,54 +722,46 @@ __global__ void __launch_bounds__(256) k_lusgs_kba3(const BlockDev *__restrict__
                 if (d == 0) {
                     for (int c = 0; c < 5; ++c) bn.r[c] = Rg[c * box + x];
                     bn.a = Ag[x];
-                    bn.pend = 0;
-                } else {
-                    // upstream dQ for the edge lanes (the other lanes' values go unused)
-                    for (int c = 0; c < 5; ++c)
-                        bn.x[c] = __hip_atomic_load(Dg + c * box + nb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    // early look at the upstream counter, consumed by the next wait
-                    bn.pend = __hip_atomic_load(dep_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             };
             auto work = [&](const SweepPf &b, int s) {
                 int i, x;
                 const bool valid = cell_at(s, i, x);
                 const int p = d == 0 ? i : d == 1 ? j : k, lim = d == 0 ? ni : d == 1 ? nj : nk;
-                const bool act = FWD ? p != 1 : p != lim;
+                const bool use = valid && (FWD ? p != 1 : p != lim);
+                // Branch-free: every lane evaluates the flux difference on its
+                // (clamped) data and idle lanes select zero, so each prefetched
+                // register is read on every path.
+                const int pl = (d == 0 || edge) ? lane
+                                                : d == 1 ? lane + (FWD ? -TK : TK) : lane + (FWD ? -1 : 1);
+                double dqn[5];
+                for (int c = 0; c < 5; ++c) {
+                    const double in_tile = sdq[(s - 1) & 1][c][pl];
+                    dqn[c] = (d != 0 && edge) ? b.x[c] : in_tile;
+                }
+                const double nx = b.m[0], ny = b.m[1], nz = b.m[2], sd = b.m[3], Vn_e = b.m[4];
+                const double *o = b.q;
+                const double cL = relax * b.l;
+                double nw[5];
+                for (int c = 0; c < 5; ++c) nw[c] = o[c] + dqn[c];
+                double dens = nw[0], uvel = nw[1] / nw[0], vvel = nw[2] / nw[0], wvel = nw[3] / nw[0], E = nw[4] / nw[0];
+                double press_new = (Gamma - F32C(1.0)) * dens * (E - F32C(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel));
+                double press_old = b.p;
+                double Vn_old = o[1] / o[0] * nx + o[2] / o[0] * ny + o[3] / o[0] * nz;
+                double Vn_new = uvel * nx + vvel * ny + wvel * nz;
+                double df[5];
+                df[0] = nw[0] * (Vn_new - Vn_e) - o[0] * (Vn_old - Vn_e);
+                df[1] = (nw[1] * (Vn_new - Vn_e) + press_new * nx) - (o[1] * (Vn_old - Vn_e) + press_old * nx);
+                df[2] = (nw[2] * (Vn_new - Vn_e) + press_new * ny) - (o[2] * (Vn_old - Vn_e) + press_old * ny);
+                df[3] = (nw[3] * (Vn_new - Vn_e) + press_new * nz) - (o[3] * (Vn_old - Vn_e) + press_old * nz);
+                df[4] = ((nw[4] + press_new) * (Vn_new - Vn_e) + press_new * Vn_e) -
+                        ((o[4] + press_old) * (Vn_old - Vn_e) + press_old * Vn_e);
                 double A[5], V[5];
-                if (valid && act) {
-                    double dqn[5];
-                    if (d == 0) {
-                        for (int c = 0; c < 5; ++c) dqn[c] = sdq[(s - 1) & 1][c][lane];
-                    } else if (!edge) {
-                        const int pl = d == 1 ? lane + (FWD ? -TK : TK) : lane + (FWD ? -1 : 1);
-                        for (int c = 0; c < 5; ++c) dqn[c] = sdq[(s - 1) & 1][c][pl];
-                    } else {
-                        for (int c = 0; c < 5; ++c) dqn[c] = b.x[c];
-                    }
-                    const double nx = b.m[0], ny = b.m[1], nz = b.m[2], sd = b.m[3], Vn_e = b.m[4];
-                    const double *o = b.q;
-                    const double cL = relax * b.l;
-                    double nw[5];
-                    for (int c = 0; c < 5; ++c) nw[c] = o[c] + dqn[c];
-                    double dens = nw[0], uvel = nw[1] / nw[0], vvel = nw[2] / nw[0], wvel = nw[3] / nw[0], E = nw[4] / nw[0];
-                    double press_new = (Gamma - F32C(1.0)) * dens * (E - F32C(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel));
-                    double press_old = b.p;
-                    double Vn_old = o[1] / o[0] * nx + o[2] / o[0] * ny + o[3] / o[0] * nz;
-                    double Vn_new = uvel * nx + vvel * ny + wvel * nz;
-                    double df[5];
-                    df[0] = nw[0] * (Vn_new - Vn_e) - o[0] * (Vn_old - Vn_e);
-                    df[1] = (nw[1] * (Vn_new - Vn_e) + press_new * nx) - (o[1] * (Vn_old - Vn_e) + press_old * nx);
-                    df[2] = (nw[2] * (Vn_new - Vn_e) + press_new * ny) - (o[2] * (Vn_old - Vn_e) + press_old * ny);
-                    df[3] = (nw[3] * (Vn_new - Vn_e) + press_new * nz) - (o[3] * (Vn_old - Vn_e) + press_old * nz);
-                    df[4] = ((nw[4] + press_new) * (Vn_new - Vn_e) + press_new * Vn_e) -
-                            ((o[4] + press_old) * (Vn_old - Vn_e) + press_old * Vn_e);
-                    for (int c = 0; c < 5; ++c) {
-                        A[c] = FWD ? (df[c] * sd + dqn[c] * cL) * F32C(0.5) : (df[c] * sd - dqn[c] * cL) * F32C(0.5);
-                        V[c] = dqn[c] * 0.0;   // lamv * dQ, lamv = 0 on the inviscid path
-                    }
-                } else {
-                    for (int c = 0; c < 5; ++c) { A[c] = 0.0; V[c] = 0.0; }
+                for (int c = 0; c < 5; ++c) {
+                    const double a = FWD ? (df[c] * sd + dqn[c] * cL) * F32C(0.5) : (df[c] * sd - dqn[c] * cL) * F32C(0.5);
+                    const double v = dqn[c] * 0.0;   // lamv * dQ, lamv = 0 on the inviscid path
+                    A[c] = use ? a : 0.0;
+                    V[c] = use ? v : 0.0;
                 }
                 for (int c = 0; c < 5; ++c) { sA[d][c][lane] = A[c]; sV[d][c][lane] = V[c]; }
                 if (d == 0) {
@@ -760,14 +771,25 @@ __global__ void __launch_bounds__(256) k_lusgs_kba3(const BlockDev *__restrict__
                 bar();   // terms of step s are in LDS
                 bar();   // dQ of step s is in LDS
             };
-            SweepPf b0, b1;
-            issue(b0, 0, 0);
-            for (int s = 0; s < nsteps; s += 2) {
-                issue(b1, s + 1, b0.pend);
-                work(b0, s);
-                if (s + 1 >= nsteps) break;
-                issue(b0, s + 2, b1.pend);
-                work(b1, s + 1);
+            // kSweepAhead steps in flight: step t's data is loaded while step
+            // t - kSweepAhead is computed; the loop is unrolled over the buffers
+            // The loop runs the steps [-kSweepAhead, send): the idle steps at
+            // either end (every lane invalid, the combine wave runs them too)
+            // let no load precede the loop and make the step count a multiple
+            // of the unrolling, so every step runs the same code.
+            SweepPf b[NB] = {};
+            // nothing in flight on entry (loads left over from the previous
+            // tile would otherwise make the compiler wait inside the loop)
+            __builtin_amdgcn_s_waitcnt(0);
+            for (int s0 = -kSweepAhead; s0 < send; s0 += NB) {
+#pragma unroll
+                for (int u = 0; u < NB; ++u) {
+                    const int s = s0 + u;
+                    // the counter look taken kSweepAhead steps ago: waiting for it
+                    // never waits for a load issued after it
+                    issue(b[(u + kSweepAhead) % NB], s + kSweepAhead, b[u].pend);
+                    work(b[u], s);
+                }
             }
         };
         if (wave == 0) {
@@ -781,7 +803,7 @@ __global__ void __launch_bounds__(256) k_lusgs_kba3(const BlockDev *__restrict__
             const gd_t Dg = (gd_t)B.dQ;
             const bool pub = (tj == (FWD ? TJ - 1 : 0)) || (tk == (FWD ? TK - 1 : 0));
             const gi_t my_ctr = (gi_t)(done + q * kDonePad);
-            for (int s = 0; s < nsteps; ++s) {
+            for (int s = -kSweepAhead; s < send; ++s) {
                 int i, x;
                 const bool valid = cell_at(s, i, x);
                 double outv[5];
@@ -805,7 +827,7 @@ __global__ void __launch_bounds__(256) k_lusgs_kba3(const BlockDev *__restrict__
                 bar();
                 // the stores of step s-1 have completed: publish s finished steps
                 __builtin_amdgcn_s_waitcnt(0);
-                if (s > 0 && lane == 0) __hip_atomic_store(my_ctr, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (s > 0 && lane == 0) __hip_atomic_store(my_ctr, min(s, nsteps), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (valid) {
                     for (int c = 0; c < 5; ++c) {
                         if (pub) __hip_atomic_store(Dg + c * box + x, outv[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -50,7 +50,24 @@ def parse():
     ap.add_argument("--lag", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run three plain 1 GiB device copies and exit (scales the HBM byte counters)")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """Three device-to-device copies of 1 GiB each.  Under rocprofv3 --pmc the
+    FETCH_SIZE / WRITE_SIZE readings of this copy kernel, against the 1 GiB it
+    must read and write, give the factor that turns those counters into bytes."""
+    import torch
+    n = 1 << 27
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": n * 8, "launches": 3}))
 
 
 def cpu_baseline(iters: int):
@@ -73,6 +90,9 @@ def cpu_baseline(iters: int):
 
 def main():
     a = parse()
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
                           "n_gpus": a.gpus}))
