@@ -1319,6 +1319,17 @@ long orc_get(orc_ctx *c, int block, const char *name, double *out)
                 }
         return n;
     }
+    else if (!strcmp(name, "SI") || !strcmp(name, "SJ") || !strcmp(name, "SK")) {
+        int d = name[1] - 'I';
+        const face_t *F = d == 0 ? b->fi : d == 1 ? b->fj : b->fk;
+        for (int i = 1; i <= IM - (d != 0); ++i)
+            for (int j = 1; j <= JM - (d != 1); ++j)
+                for (int k = 1; k <= KM - (d != 2); ++k) {
+                    const face_t *f = &F[C3(b, i, j, k)];
+                    out[n++] = f->S.x; out[n++] = f->S.y; out[n++] = f->S.z;
+                }
+        return n;
+    }
     else if (!strcmp(name, "vol")) {
         for (int i = 1; i <= IM - 1; ++i)
             for (int j = 1; j <= JM - 1; ++j)
@@ -1350,4 +1361,40 @@ long orc_get(orc_ctx *c, int block, const char *name, double *out)
         return n;
     }
     return -2;
+}
+
+/* ------------------------------------------------------------ test hooks */
+/* Point evaluations of the face flux and the reconstruction, so that tests can
+ * check them against closed forms.  States are in the face frame for the flux
+ * (q = rho, rho*u_n, rho*u_t1, rho*u_t2, rho*E); the reconstruction takes
+ * primitive records (rho,u,v,w,p,T) for var 200 and conservative ones for var
+ * 100, and returns conservative left/right states. */
+int orc_test_flux(int scheme, double gamma, const double *ql, const double *qr, double que, double *out)
+{
+    cv_t L = {ql[0], ql[1], ql[2], ql[3], ql[4]}, R = {qr[0], qr[1], qr[2], qr[3], qr[4]}, F;
+    if (scheme == 300) F = roe(gamma, L, R, que);
+    else if (scheme == 200) F = steger_warming(gamma, L, R, que);
+    else return -1;
+    out[0] = F.r; out[1] = F.ru; out[2] = F.rv; out[3] = F.rw; out[4] = F.re;
+    return 0;
+}
+
+int orc_test_recon(int var, int scheme, double gamma, const double *ff, const double *f, const double *b,
+                   const double *bb, double *l, double *r)
+{
+    cv_t L, R;
+    if (var == 200) {
+        pv_t a = {ff[0], ff[1], ff[2], ff[3], ff[4], ff[5]}, c = {f[0], f[1], f[2], f[3], f[4], f[5]};
+        pv_t d = {b[0], b[1], b[2], b[3], b[4], b[5]}, e = {bb[0], bb[1], bb[2], bb[3], bb[4], bb[5]};
+        recon_prim(scheme, gamma, a, c, d, e, &L, &R);
+    } else if (var == 100) {
+        cv_t a = {ff[0], ff[1], ff[2], ff[3], ff[4]}, c = {f[0], f[1], f[2], f[3], f[4]};
+        cv_t d = {b[0], b[1], b[2], b[3], b[4]}, e = {bb[0], bb[1], bb[2], bb[3], bb[4]};
+        recon_cons(scheme, gamma, a, c, d, e, &L, &R);
+    } else {
+        return -1;
+    }
+    l[0] = L.r; l[1] = L.ru; l[2] = L.rv; l[3] = L.rw; l[4] = L.re;
+    r[0] = R.r; r[1] = R.ru; r[2] = R.rv; r[3] = R.rw; r[4] = R.re;
+    return 0;
 }

@@ -83,8 +83,17 @@ int orc_iterate(orc_ctx *c, int n, double *hist);
 
 /* Copies a field out.  Names: cv pv (full (IM+1)(JM+1)(KM+1)xC incl. ghost
  * cells), res dq dt lamcI lamcJ lamcK (interior (ni,nj,nk)xC), dIS_cv dIP_cv
- * dJS_cv dJP_cv dKS_cv dKP_cv and the _pv variants (layers 1..NDummy). */
+ * dJS_cv dJP_cv dKS_cv dKP_cv and the _pv variants (layers 1..NDummy), vol,
+ * fcI fcJ fcK (face fluxes), SI SJ SK (face vectors). */
 long orc_get(orc_ctx *c, int block, const char *name, double *out);
+
+/* Test hooks: the face flux (scheme 200 Steger-Warming, 300 Roe) on face-frame
+ * states, and the reconstruction (var 100 conservative / 200 primitive) of one
+ * face from its four-cell stencil FF, F | B, BB.  Also orc_get "SI" "SJ" "SK":
+ * face vectors (faces x 3). */
+int orc_test_flux(int scheme, double gamma, const double *ql, const double *qr, double que, double *out);
+int orc_test_recon(int var, int scheme, double gamma, const double *ff, const double *f, const double *b,
+                   const double *bb, double *l, double *r);
 
 #ifdef __cplusplus
 }

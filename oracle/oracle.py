@@ -53,6 +53,8 @@ def lib():
         L.orc_iterate.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_get.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_void_p]
         L.orc_get.restype = C.c_long
+        L.orc_test_flux.argtypes = [C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
+        L.orc_test_recon.argtypes = [C.c_int, C.c_int, C.c_double] + [C.c_void_p] * 6
         _lib = L
     return _lib
 
@@ -135,6 +137,8 @@ class Oracle:
             shape = (ni, nj, nk, 5)
         elif name in ("fcI", "fcJ", "fcK"):
             shape = {"fcI": (im, nj, nk, 5), "fcJ": (ni, jm, nk, 5), "fcK": (ni, nj, km, 5)}[name]
+        elif name in ("SI", "SJ", "SK"):
+            shape = {"SI": (im, nj, nk, 3), "SJ": (ni, jm, nk, 3), "SK": (ni, nj, km, 3)}[name]
         elif name in ("dt", "lamcI", "lamcJ", "lamcK", "vol"):
             shape = (ni, nj, nk)
         elif name.startswith("d") and name[1] in "IJK":
@@ -165,3 +169,22 @@ def run_reference(case_dir: str, n_iter: int, out_dir: str, threads: int = 1, du
     with open(os.path.join(out_dir, "timing.json")) as f:
         timing = json.load(f)
     return {"history": hist, "timing": timing}
+
+
+def point_flux(scheme: int, gamma: float, ql, qr, que: float = 0.0) -> np.ndarray:
+    """Face flux of the oracle (200 Steger-Warming, 300 Roe) on face-frame states."""
+    a = np.ascontiguousarray(ql, dtype=np.float64)
+    b = np.ascontiguousarray(qr, dtype=np.float64)
+    out = np.zeros(5)
+    if lib().orc_test_flux(scheme, gamma, a.ctypes.data, b.ctypes.data, que, out.ctypes.data) != 0:
+        raise ValueError(f"flux scheme {scheme}")
+    return out
+
+
+def point_recon(var: int, scheme: int, gamma: float, ff, f, b, bb):
+    """Left/right conservative states of one face from its stencil FF, F | B, BB."""
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (ff, f, b, bb)]
+    l, r = np.zeros(5), np.zeros(5)
+    if lib().orc_test_recon(var, scheme, gamma, *[x.ctypes.data for x in arrs], l.ctypes.data, r.ctypes.data) != 0:
+        raise ValueError(f"reconstruction var {var}")
+    return l, r
