@@ -1,0 +1,132 @@
+// fv3d_run: runs a reference case directory on the GPU through the C++ class
+// layer, in place of the reference's executable (source/Structured_FV3D.cpp:19-214).
+//
+//   fv3d_run <case_dir> [iterations] [--history FILE] [--eager] [--device N]
+//   fv3d_run <case_dir> --parse-only
+//
+// The case directory holds config.dat, the grid and boundary-definition files
+// it names, and RestartSolution.dat (reuse_solution = Y).  Without an iteration
+// count the config's max_iter is used.  The history ends at the first
+// iteration whose density residual is at or below the config's convergence
+// tolerance with inlet and outlet mass flows within 0.3 %, as the reference's
+// main loop does (the device may have run up to the end of that chunk).  --parse-only reads the case files and
+// prints a JSON summary without touching the GPU.
+#include <cinttypes>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "fv3d_classes.h"
+
+namespace {
+
+double checksum(const std::vector<double> &a)
+{
+    double s = 0.0;
+    for (size_t n = 0; n < a.size(); ++n) s += a[n] * (double)(1 + n % 7);
+    return s;
+}
+
+void print_summary(const fv3d::CaseConfig &cc, const fv3d::BndTable &bnd, const std::vector<std::array<int, 3>> &dims,
+                   const std::vector<std::vector<double>> &nodes, const std::vector<fv3d::BlockState> &st)
+{
+    const fv3d_config &r = cc.raw;
+    std::printf("{\"config\": {\"rgas\": %.17g, \"gamma\": %.17g, \"pr_lam\": %.17g, \"pr_tur\": %.17g, "
+                "\"pt_in\": %.17g, \"tt_in\": %.17g, \"p_out\": %.17g, \"ma0\": %.17g, \"len_ref\": %.17g, "
+                "\"ev_ratio\": %.17g, \"inlet_dir\": [%.17g, %.17g, %.17g], \"cfl\": %.17g, \"c_timestep\": %.17g, "
+                "\"relax\": %.17g, \"turbo_type\": %d, \"viscous\": %d, \"recon_var\": %d, \"recon_scheme\": %d, "
+                "\"flux_scheme\": %d, \"turbulence\": %d, \"limiter\": %d, \"global_dt\": %d, \"max_iter\": %d, "
+                "\"conv_tol\": %.17g, \"reuse_solution\": %d, \"time_scheme\": %d},\n",
+                r.rgas, r.gamma, r.pr_lam, r.pr_tur, r.pt_in, r.tt_in, r.p_out, r.ma0, r.len_ref, r.ev_ratio,
+                r.inlet_dir[0], r.inlet_dir[1], r.inlet_dir[2], r.cfl, r.c_timestep, r.relax, r.turbo_type,
+                r.viscous, r.recon_var, r.recon_scheme, r.flux_scheme, r.turbulence, r.limiter, r.global_dt,
+                cc.max_iter, cc.conv_tol, (int)cc.reuse_solution, cc.time_scheme);
+    std::printf(" \"patches\": [");
+    for (size_t n = 0; n < bnd.patches.size(); ++n) {
+        const fv3d::PatchRow &p = bnd.patches[n];
+        std::printf("%s[%d, %d, %d, %d, %d, %d, %d, %d, %d, %d]", n ? ", " : "", p.kind, p.block, p.face, p.rotate,
+                    p.rng[0], p.rng[1], p.rng[2], p.rng[3], p.rng[4], p.rng[5]);
+    }
+    std::printf("],\n \"blocks\": [");
+    for (size_t b = 0; b < bnd.blocks.size(); ++b) {
+        const fv3d::BlockRow &q = bnd.blocks[b];
+        std::printf("%s[%d, %d, %d, %d, %d, %.17g]", b ? ", " : "", q.bid, q.IM, q.JM, q.KM, q.nblade, q.rpm);
+    }
+    std::printf("],\n \"grids\": [");
+    for (size_t b = 0; b < dims.size(); ++b)
+        std::printf("%s[%d, %d, %d, %.17g]", b ? ", " : "", dims[b][0], dims[b][1], dims[b][2], checksum(nodes[b]));
+    std::printf("],\n \"states\": [");
+    for (size_t b = 0; b < st.size(); ++b)
+        std::printf("%s[%.17g, %.17g]", b ? ", " : "", checksum(st[b].pv), checksum(st[b].ener));
+    std::printf("]}\n");
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: %s <case_dir> [iterations] [--history FILE] [--eager] [--device N] [--parse-only]\n",
+                     argv[0]);
+        return 2;
+    }
+    std::string dir = argv[1], history;
+    int n_iter = -1, device = 0;
+    bool eager = false, parse_only = false;
+    for (int a = 2; a < argc; ++a) {
+        if (!std::strcmp(argv[a], "--history") && a + 1 < argc) history = argv[++a];
+        else if (!std::strcmp(argv[a], "--eager")) eager = true;
+        else if (!std::strcmp(argv[a], "--parse-only")) parse_only = true;
+        else if (!std::strcmp(argv[a], "--device") && a + 1 < argc) device = std::atoi(argv[++a]);
+        else n_iter = std::atoi(argv[a]);
+    }
+    try {
+        fv3d::CaseConfig cc = fv3d::ReadConfig(dir + "/config.dat");
+        fv3d::BndTable bnd = fv3d::ReadBndGrid(dir + "/" + cc.bc_file);
+        std::vector<std::array<int, 3>> dims;
+        std::vector<std::vector<double>> nodes = fv3d::ReadGrids(dir + "/" + cc.grid_file, &dims);
+        if (!cc.reuse_solution)
+            throw fv3d::Error(FV3D_EINVAL, "this build starts from RestartSolution.dat (reuse_solution = Y)");
+        std::vector<fv3d::BlockState> st = fv3d::ReadSolution(dir + "/RestartSolution.dat", dims);
+        if (parse_only) {
+            print_summary(cc, bnd, dims, nodes, st);
+            return 0;
+        }
+        cc.raw.device = device;
+        fv3d::CMesh mesh(cc.raw);
+        fv3d::LoadCase(mesh, dir, bnd, nodes, dims, st);
+        fv3d::CSolver solver(mesh, !eager);
+        if (n_iter < 0) n_iter = cc.max_iter;
+        FILE *hf = history.empty() ? nullptr : std::fopen(history.c_str(), "w");
+        if (!history.empty() && !hf) throw fv3d::Error(FV3D_EINVAL, "cannot write " + history);
+        std::printf("cells %ld, iterations %d\n", mesh.CellCount(), n_iter);
+        // iterations in chunks, one host synchronisation per chunk
+        const int chunk = 50;
+        int done = 0;
+        bool converged = false;
+        while (done < n_iter && !converged) {
+            int m = n_iter - done < chunk ? n_iter - done : chunk;
+            std::vector<fv3d::HistoryRow> h = solver.Run(m);
+            size_t last = 0;
+            for (size_t q = 0; q < h.size() && !converged; ++q) {
+                const fv3d::HistoryRow &r = h[q];
+                if (hf)
+                    std::fprintf(hf, "%.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", r[0], r[1], r[2], r[3], r[4], r[5],
+                                 r[6]);
+                last = q;
+                // source/Structured_FV3D.cpp:214: residual and mass-flow balance
+                if (r[1] <= cc.conv_tol && std::fabs((r[2] - r[3]) / r[2]) <= 0.003) converged = true;
+            }
+            done += m;
+            const fv3d::HistoryRow &r = h[last];
+            std::printf("iter %6d  res %.6e  mass in %.6e out %.6e  PR %.6f  eff %.6f  CFL %.3f\n", (int)r[0], r[1],
+                        r[2], r[3], r[4], r[5], r[6]);
+        }
+        if (hf) std::fclose(hf);
+        return 0;
+    } catch (const fv3d::Error &e) {
+        std::fprintf(stderr, "fv3d_run: %s (status %d)\n", e.what(), e.status);
+        return 1;
+    }
+}

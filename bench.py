@@ -25,14 +25,19 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "3d-fluid-numerical-simulation-software_amd"))
 sys.path.insert(0, os.path.join(REPO, "oracle"))
 
-# Algorithmic HBM bytes per cell and launch (FP64), each array a kernel
-# semantically reads or writes counted once; see DESIGN.md.
+# Algorithmic HBM bytes per cell (FP64): every array a kernel semantically reads
+# or writes counted once, derived quantities (W, |S|, unit normals) as
+# recomputed, metrics as stored.
 BYTES_PER_CELL = {
-    "residual": 8 * (5 + 9 + 3 + 3 + 1 + 5),   # W, face S, |S| x3, nodes, vol; write R
-    "lusgs": 2 * 8 * (5 + 5 + 1 + 3 + 1 + 1 + 9 + 3 + 5),  # two sweeps: R|dQ*, Q, p, lamc, dt, vol, S, |S|; write dQ
-    "update": 8 * (5 + 5 + 1 + 5 + 6),         # Q, dQ, vol; write Q, W
-    "bc_spectral_dt": 8 * (4 + 3 + 18 + 1 + 3 + 1),  # W(u,v,w,T), centroid, S x6, vol; write lamc, dt
+    "residual": 176,          # Q 5, S 9, nodes 3 -> R 5
+    "lusgs": 2 * 232,         # per sweep: R|dQ* 5, Q 5, lamc 3, dt 1, vol 1, S 9 -> dQ 5
+    "update": 128,            # Q 5, dQ 5, vol 1 -> Q 5
+    "bc_spectral_dt": 152,    # Q 5, S 9, vol 1 -> lamc 3, dt 1
 }
+# HBM bytes one iteration's two sweeps actually move on the default workload,
+# from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (copy-calibrated), see
+# profiles/lusgs_iteration_bump256x64x64.md.  Keyed by (ni, nj, nk, sweep form).
+MEASURED_SWEEP_TRAFFIC = {(64, 64, 256, "kba3"): (1311.7e6 + 1009.9e6)}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -138,8 +143,10 @@ def main():
         "config": {"workload": f"inviscid bump channel {a.nk}x{a.nj}x{a.ni} cells (single block), Roe + MUSCL3 "
                                f"primitive, LU-SGS local dt CFL 5, plain FP64",
                    "cells": ncell, "hip_graph": graph, "sweep": a.sweep, "chunk": a.chunk, "lag": a.lag},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": "lusgs forward+backward sweep" if dom == "lusgs" else dom,
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_SWEEP_TRAFFIC.get((a.ni, a.nj, a.nk, a.sweep)) if dom == "lusgs" else None,
                      "kernel_ms": times, "bytes_per_cell": BYTES_PER_CELL[dom]},
         "cpu_baseline": cpu,
         "final_res_dens": float(hist[-1, 1]),

@@ -11,8 +11,7 @@
  *   the CMesh / CSpaceDiscr / CTimeDiscr / CSolver method surface of
  *   include/Mesh.h:31-90, include/SpaceDiscr.h:62-69, include/TimeDiscr.h:43-49,
  *   include/Solver.h:46-52.  The reference has no FFI; its classes call each
- *   other directly.  The C++ classes in
- *   3d-fluid-numerical-simulation-software_amd/csrc/fv3d_classes.h keep those
+ *   other directly.  The C++ classes in include/fv3d_classes.h keep those
  *   names and forward to the entry points below.
  */
 #ifndef FV3D_H
