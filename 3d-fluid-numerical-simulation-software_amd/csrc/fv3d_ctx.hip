@@ -31,6 +31,7 @@ struct HostBlock {
     std::vector<double> nodes;   // (IM, JM, KM, 3) as given
     BlockDev dev;                // device pointers + dims
     std::vector<void *> allocs;
+    long sweep_rows = 0;         // rows of the tile-skewed sweep layout
 };
 
 }  // namespace
@@ -72,15 +73,16 @@ struct fv3d_ctx {
     int *d_kba_sync = nullptr;
     int kba_chunk = 1;
     int kba_lag = 0;
-    int sweep_mode = 3;   // 0 one launch per hyperplane, 1..3 wavefront forms (3: four waves per tile)
+    int sweep_mode = 4;   // 0 one launch per hyperplane, 1..4 wavefront forms (3: four waves per tile,
+                          // 4: the same on the tile-skewed row layout)
     int kba_grid = 0;
     // graph
     hipGraphExec_t graph_exec = nullptr;
     // timing
     int timing = 0;
-    hipEvent_t ev[8];
-    double t_ms[5] = {0, 0, 0, 0, 0};
-    long t_n[5] = {0, 0, 0, 0, 0};
+    hipEvent_t ev[10];
+    double t_ms[FV3D_TIMING_SLOTS] = {};
+    long t_n[FV3D_TIMING_SLOTS] = {};
 };
 
 static int fail(fv3d_ctx *c, int code, const char *fmt, ...)
@@ -271,6 +273,15 @@ static int setup_block(fv3d_ctx *c, HostBlock &b, int index)
     }
     if ((rc = alloc(&B.R, 5 * B.box))) return rc;
     if ((rc = alloc(&B.dQ, 5 * B.box))) return rc;
+    {
+        // tile-skewed sweep rows (filled by k_sweep_gather)
+        B.ntk = (B.nk + kKbaTK - 1) / kKbaTK;
+        B.nr = B.ni + kKbaTJ + kKbaTK - 2;
+        const long ntile = (long)((B.nj + kKbaTJ - 1) / kKbaTJ) * B.ntk;
+        b.sweep_rows = ntile * B.nr;
+        if ((rc = alloc(&B.srow, b.sweep_rows * kSweepQ * 64))) return rc;
+        if ((rc = alloc(&B.sdq, b.sweep_rows * 5 * 64))) return rc;
+    }
     // Dummy slabs: tangential boxes (JM+1)(KM+1), (IM+1)(KM+1), (IM+1)(JM+1)
     const int ta[3] = {b.JM, b.IM, b.IM}, tb[3] = {b.KM, b.KM, b.JM};
     for (int s = 0; s < 6; ++s) {
@@ -333,7 +344,7 @@ int fv3d_ctx_create(const fv3d_config *cfg, fv3d_ctx **out)
     if (e != hipSuccess) { *out = c; return fail(c, FV3D_EHIP, "hipSetDevice(%d): %s", c->device, hipGetErrorString(e)); }
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { *out = c; return fail(c, FV3D_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
-    for (int n = 0; n < 8; ++n) (void)hipEventCreate(&c->ev[n]);
+    for (int n = 0; n < 10; ++n) (void)hipEventCreate(&c->ev[n]);
     nondimensionalize(c);
     *out = c;
     return FV3D_OK;
@@ -351,7 +362,7 @@ int fv3d_ctx_destroy(fv3d_ctx *c)
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (c->stream) {
-        for (int n = 0; n < 8; ++n) (void)hipEventDestroy(c->ev[n]);
+        for (int n = 0; n < 10; ++n) (void)hipEventDestroy(c->ev[n]);
         (void)hipStreamDestroy(c->stream);
     }
     delete c;
@@ -417,6 +428,12 @@ int fv3d_setup(fv3d_ctx *c)
     for (auto &b : c->blocks) bd.push_back(b.dev);
     if ((rc = dalloc(c, &c->d_blocks, nb))) return rc;
     HIPCK(c, hipMemcpyAsync(c->d_blocks, bd.data(), nb * sizeof(BlockDev), hipMemcpyHostToDevice, c->stream));
+    for (int n = 0; n < nb; ++n) {
+        const long rows = c->blocks[n].sweep_rows;
+        hipLaunchKernelGGL(k_sweep_gather, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, c->stream, c->d_blocks, n, rows,
+                           kGatherMetrics);
+        HIPCK(c, hipGetLastError());
+    }
     // patch work list
     std::vector<PatchDev> pd;
     std::vector<long> pst{0};
@@ -590,7 +607,23 @@ static int launch_sweeps_kba(fv3d_ctx *c)
     hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((nsync + 255) / 256)), dim3(256), 0, c->stream, c->d_kba_sync + 32, nsync, 0);
     int *err = c->d_kba_sync, *tf = c->d_kba_sync + 32, *tb = c->d_kba_sync + 48;
     int *df = c->d_kba_sync + 64, *db = df + (long)c->nkba * kDonePad;
-    if (c->sweep_mode == 3) {
+    if (c->sweep_mode == 4) {
+        const int nb = (int)c->blocks.size();
+        for (int n = 0; n < nb; ++n) {
+            const long rows = c->blocks[n].sweep_rows;
+            hipLaunchKernelGGL(k_sweep_gather, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, c->stream, c->d_blocks, n,
+                               rows, kGatherState);
+        }
+        if (c->timing) HIPCK(c, hipEventRecord(c->ev[6], c->stream));
+        hipLaunchKernelGGL((k_lusgs_sl<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
+                           c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_lag);
+        if (c->timing) HIPCK(c, hipEventRecord(c->ev[7], c->stream));
+        hipLaunchKernelGGL((k_lusgs_sl<false, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
+                           c->d_blocks, c->d_kba[1], c->nkba, tb, db, err, c->gas.relax, c->gas, c->kba_lag);
+        if (c->timing) HIPCK(c, hipEventRecord(c->ev[8], c->stream));
+        hipLaunchKernelGGL(k_sweep_scatter, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream,
+                           c->d_blocks, c->d_cstart, nb);
+    } else if (c->sweep_mode == 3) {
         hipLaunchKernelGGL((k_lusgs_kba3<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
                            c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_lag);
         hipLaunchKernelGGL((k_lusgs_kba3<false, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
@@ -772,6 +805,11 @@ static int accumulate_timing(fv3d_ctx *c)
     HIPCK(c, hipEventElapsedTime(&ms, c->ev[3], c->ev[4])); c->t_ms[2] += ms; c->t_n[2]++;
     HIPCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); c->t_ms[3] += ms; c->t_n[3]++;
     HIPCK(c, hipEventElapsedTime(&ms, c->ev[4], c->ev[5])); c->t_ms[4] += ms; c->t_n[4]++;
+    if (c->sweep_mode == 4) {
+        // the forward and backward row-layout sweep kernels alone
+        HIPCK(c, hipEventElapsedTime(&ms, c->ev[6], c->ev[7])); c->t_ms[5] += ms; c->t_n[5]++;
+        HIPCK(c, hipEventElapsedTime(&ms, c->ev[7], c->ev[8])); c->t_ms[6] += ms; c->t_n[6]++;
+    }
     (void)pairs;
     return FV3D_OK;
 }
@@ -844,16 +882,16 @@ int fv3d_set_timing(fv3d_ctx *c, int on)
 {
     if (!c) return FV3D_EINVAL;
     c->timing = on;
-    for (int q = 0; q < 5; ++q) { c->t_ms[q] = 0.0; c->t_n[q] = 0; }
+    for (int q = 0; q < FV3D_TIMING_SLOTS; ++q) { c->t_ms[q] = 0.0; c->t_n[q] = 0; }
     return FV3D_OK;
 }
 
-int fv3d_timing(fv3d_ctx *c, double *ms5, long *counts5)
+int fv3d_timing(fv3d_ctx *c, double *ms, long *counts)
 {
     if (!c) return FV3D_EINVAL;
-    for (int q = 0; q < 5; ++q) {
-        if (ms5) ms5[q] = c->t_n[q] ? c->t_ms[q] / c->t_n[q] : 0.0;
-        if (counts5) counts5[q] = c->t_n[q];
+    for (int q = 0; q < FV3D_TIMING_SLOTS; ++q) {
+        if (ms) ms[q] = c->t_n[q] ? c->t_ms[q] / c->t_n[q] : 0.0;
+        if (counts) counts[q] = c->t_n[q];
     }
     return FV3D_OK;
 }
@@ -864,7 +902,7 @@ int fv3d_set_debug(fv3d_ctx *c, int flags)
 {
     READY(c);
     c->store_faces = flags & 1;
-    c->sweep_mode = (flags & 2) ? 0 : (flags & 4) ? 1 : (flags & 8) ? 2 : 3;
+    c->sweep_mode = (flags & 2) ? 0 : (flags & 4) ? 1 : (flags & 8) ? 2 : (flags & 16) ? 3 : 4;
     if ((flags >> 8) & 0xff) c->kba_chunk = (flags >> 8) & 0xff;
     c->kba_lag = (flags >> 16) & 0xff;
     if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }

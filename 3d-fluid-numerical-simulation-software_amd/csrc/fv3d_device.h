@@ -53,6 +53,15 @@ struct BlockDev {
     double *R;                // 5*box residual
     double *dQ;               // 5*box LU-SGS increment
     int *flag_low_dt;         // time-step damping counter (source/Limit_FlowVars.cpp:117)
+    // Tile-skewed sweep layout.  The (j,k) plane is cut into tiles of
+    // kKbaTJ x kKbaTK columns; tile (J,K) is slot J*ntk+K and owns nr rows.
+    // Row r of a tile holds the 64 cells with i + tj + tk = r + 1 (tj, tk the
+    // cell's place in the tile), so every cell a wavefront step touches lies
+    // in one row and its three upstream neighbours in the row before.
+    double *srow;             // per row: kSweepQ quantities x 64 lanes
+    double *sdq;              // per row: dQ, 5 components x 64 lanes
+    int ntk;                  // tiles along k
+    int nr;                   // rows per tile = ni + kKbaTJ + kKbaTK - 2
 };
 
 __host__ __device__ inline long cidx(const BlockDev &B, int i, int j, int k)

@@ -845,6 +845,270 @@ template __global__ void k_lusgs_kba3<true, kKbaTJ, kKbaTK>(const BlockDev *, co
 template __global__ void k_lusgs_kba3<false, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
 
 // Clears the ticket, error flag and progress counters of one sweep.
+// ---------------------------------------------------------------------------
+// Tile-skewed sweep layout (BlockDev::srow / sdq).
+//
+// k_sweep_gather copies cell data into the rows, one wave per row: lane
+// (tj,tk) of row r of tile (J,K) is cell (r+1-tj-tk, J*TJ+tj+1, K*TK+tk+1).
+// Lanes outside the block get a harmless unit state.  The sweeps then read
+// whole 512-byte rows; k_sweep_scatter copies dQ back to the cell box.
+__global__ void __launch_bounds__(256) k_sweep_gather(const BlockDev *__restrict__ blocks, int block, long nrows, int what)
+{
+    constexpr int TJ = kKbaTJ, TK = kKbaTK;
+    const BlockDev &B = blocks[block];
+    const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= nrows) return;
+    const int lane = threadIdx.x & 63, tj = lane / TK, tk = lane % TK;
+    const int slot = (int)(row / B.nr), r = (int)(row % B.nr);
+    const int J = slot / B.ntk, K = slot % B.ntk;
+    const int i = r + 1 - tj - tk, j = J * TJ + tj + 1, k = K * TK + tk + 1;
+    const bool valid = i >= 1 && i <= B.ni && j <= B.nj && k <= B.nk;
+    const long box = B.box, x = valid ? cidx(B, i, j, k) : 0;
+    double *dst = B.srow + row * (kSweepQ * 64) + lane;
+    if (what & kGatherState) {
+        for (int c = 0; c < 5; ++c) dst[(kSQ_Q + c) * 64] = valid ? B.Q[c * box + x] : (c == 0 ? 1.0 : 0.0);
+        dst[kSQ_P * 64] = valid ? B.W[4 * box + x] : 1.0;
+        for (int d = 0; d < 3; ++d) dst[(kSQ_LAM + d) * 64] = valid ? B.lamc[d * box + x] : 0.0;
+        for (int c = 0; c < 5; ++c) dst[(kSQ_R + c) * 64] = valid ? B.R[c * box + x] : 0.0;
+        dst[kSQ_A * 64] = valid ? B.alpha[x] : 1.0;
+    }
+    if (what & kGatherMetrics)
+        for (int q = 0; q < 15; ++q) dst[(kSQ_SWG + q) * 64] = valid ? B.swg[q * box + x] : 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_sweep_scatter(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
+                                                       int nblock)
+{
+    constexpr int TJ = kKbaTJ, TK = kKbaTK;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cstart[nblock]) return;
+    int n = 0;
+    while (t >= cstart[n + 1]) ++n;
+    const BlockDev &B = blocks[n];
+    long r = t - cstart[n];
+    const int k = (int)(r % B.nk) + 1;
+    r /= B.nk;
+    const int j = (int)(r % B.nj) + 1;
+    const int i = (int)(r / B.nj) + 1;
+    const int tj = (j - 1) % TJ, tk = (k - 1) % TK;
+    const int slot = ((j - 1) / TJ) * B.ntk + (k - 1) / TK, row = i + tj + tk - 1;
+    const double *src = B.sdq + ((long)slot * B.nr + row) * (5 * 64) + tj * TK + tk;
+    const long box = B.box, x = cidx(B, i, j, k);
+    for (int c = 0; c < 5; ++c) B.dQ[c * box + x] = src[c * 64];
+}
+
+// Fourth form of the wavefront sweep: the four-wave schedule of k_lusgs_kba3
+// on the tile-skewed layout.  A step touches one row of its tile: the three
+// direction waves read their neighbours' state and metrics from the previous
+// row (a lane shift) or, on the tile's upstream edge, from the upstream tile's
+// row, so every load is a contiguous run of one row.  dQ goes to the tile's
+// dQ rows.  Progress, waits and publication are those of k_lusgs_kba3.
+template <bool FWD, int TJ, int TK>
+__global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ blocks, const KbaTile *__restrict__ tiles,
+                                                  int ntiles, int *__restrict__ ticket, int *__restrict__ done,
+                                                  int *__restrict__ err, double relax, Gas g, int lag)
+{
+    static_assert(TJ * TK == 64, "64 cells per tile");
+    __shared__ double sdq[2][5][64];
+    __shared__ double sA[3][5][64];
+    __shared__ double sV[3][5][64];
+    __shared__ double sR[6][64];
+    __shared__ int s_q;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    const int tj = lane / TK, tk = lane % TK;
+    const double Gamma = g.gamma;
+    constexpr int RQ = kSweepQ * 64, RD = 5 * 64;   // doubles per row
+    for (;;) {
+        if (threadIdx.x == 0) s_q = atomicAdd(ticket, 1);
+        __syncthreads();
+        const int q = __builtin_amdgcn_readfirstlane(s_q);
+        __syncthreads();
+        if (q >= ntiles) return;
+        const KbaTile tl = tiles[q];
+        const BlockDev &B = blocks[tl.block];
+        const int ni = B.ni, nj = B.nj, nk = B.nk, nr = B.nr, ntk = B.ntk;
+        const int slot = (tl.j0 / TJ) * ntk + tl.k0 / TK;
+        const gcd_t rows = (gcd_t)(B.srow + (long)slot * nr * RQ);
+        const gd_t drows = (gd_t)(B.sdq + (long)slot * nr * RD);
+        const int lj = FWD ? tj : TJ - 1 - tj, lk = FWD ? tk : TK - 1 - tk;
+        const int j = tl.j0 + tj + 1, k = tl.k0 + tk + 1;
+        const bool col = j <= nj && k <= nk;
+        const int nsteps = nr;
+        constexpr int NB = kSweepAhead + 1;   // prefetch buffers = unrolling of the step loop
+        const int send = (nsteps + kSweepAhead + NB - 1) / NB * NB - kSweepAhead;
+        auto clampr = [&](int r) { return min(max(r, 0), nr - 1); };
+        // own row of a step, and whether this lane's cell is in the block
+        auto row_of = [&](int step) { return FWD ? step : nr - 1 - step; };
+        auto cell_at = [&](int step, int &i) -> bool {
+            const int l2 = step - lj - lk;
+            i = FWD ? l2 + 1 : ni - l2;
+            return col && l2 >= 0 && l2 < ni;
+        };
+        auto bar = []() {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        };
+        auto dir_loop = [&](auto dc) {
+            constexpr int d = decltype(dc)::value;
+            constexpr int T = d == 1 ? TJ : TK;
+            const int dep = d == 1 ? tl.dep_j : d == 2 ? tl.dep_k : -1;
+            const bool edge = d == 1 ? tj == (FWD ? 0 : TJ - 1) : d == 2 ? tk == (FWD ? 0 : TK - 1) : false;
+            // upstream tile's rows (this tile's own when there is none: the
+            // values are then never used)
+            const int uslot = dep < 0 ? slot : d == 1 ? slot + (FWD ? -ntk : ntk) : slot + (FWD ? -1 : 1);
+            const gcd_t urows = (gcd_t)(B.srow + (long)uslot * nr * RQ);
+            const gcd_t udrows = (gcd_t)(B.sdq + (long)uslot * nr * RD);
+            // the upstream lane facing this lane across the tile edge
+            const int ulane = d == 1 ? (FWD ? (TJ - 1) * TK : 0) + tk : tj * TK + (FWD ? TK - 1 : 0);
+            // in-tile neighbour lane (lanes on the edge read the upstream row instead)
+            const int nlane = (d == 0 || edge) ? lane : d == 1 ? lane + (FWD ? -TK : TK) : lane + (FWD ? -1 : 1);
+            const bool from_up = d != 0 && edge && dep >= 0;
+            const gi_t dep_ctr = (gi_t)(done + max(dep, 0) * kDonePad);
+            int seen = dep < 0 ? 1 << 30 : 0;
+            auto issue = [&](SweepPf &bn, int step, int pend) {
+                if (d != 0) {
+                    // every path consumes the early look, and not before this point
+                    asm volatile("" : "+v"(pend));
+                    seen = max(seen, pend);
+                    if (dep >= 0) {
+                        const int need = min(step + T + lag, nsteps);
+                        long spins = 0;
+                        while (seen < need) {
+                            seen = __hip_atomic_load(dep_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            if (seen >= need) break;
+                            __builtin_amdgcn_s_sleep(1);
+                            if (++spins > (1L << 22)) { atomicExch(err, 1); seen = 1 << 30; }
+                        }
+                    }
+                }
+                const int rt = row_of(step);
+                const int urow = clampr(FWD ? rt + T - 1 : rt - (T - 1));
+                const int nrow = clampr(FWD ? rt - 1 : rt + 1);
+                const gcd_t src = (from_up ? urows + urow * RQ + ulane : rows + nrow * RQ + nlane);
+                if (d != 0) {
+                    // early look at the upstream counter, consumed kSweepAhead steps later
+                    bn.pend = __hip_atomic_load(dep_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    // upstream dQ for the edge lanes (the other lanes read the same words)
+                    for (int c = 0; c < 5; ++c)
+                        bn.x[c] = __hip_atomic_load(udrows + urow * RD + c * 64 + ulane, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+                } else {
+                    bn.pend = 0;
+                }
+                for (int c = 0; c < 5; ++c) bn.q[c] = src[(kSQ_Q + c) * 64];
+                bn.p = src[kSQ_P * 64];
+                bn.l = src[(kSQ_LAM + d) * 64];
+                for (int c = 0; c < 5; ++c) bn.m[c] = src[(kSQ_SWG + d * 5 + c) * 64];
+                if (d == 0) {
+                    const int ro = clampr(rt);
+                    for (int c = 0; c < 5; ++c)
+                        bn.r[c] = FWD ? rows[ro * RQ + (kSQ_R + c) * 64 + lane] : ((gcd_t)drows)[ro * RD + c * 64 + lane];
+                    bn.a = rows[ro * RQ + kSQ_A * 64 + lane];
+                }
+            };
+            auto work = [&](const SweepPf &b, int s) {
+                int i;
+                const bool valid = cell_at(s, i);
+                const int p = d == 0 ? i : d == 1 ? j : k, lim = d == 0 ? ni : d == 1 ? nj : nk;
+                const bool use = valid && (FWD ? p != 1 : p != lim);
+                double dqn[5];
+                for (int c = 0; c < 5; ++c) {
+                    const double in_tile = sdq[(s - 1) & 1][c][nlane];
+                    dqn[c] = (d != 0 && edge) ? b.x[c] : in_tile;
+                }
+                const double nx = b.m[0], ny = b.m[1], nz = b.m[2], sd = b.m[3], Vn_e = b.m[4];
+                const double *o = b.q;
+                const double cL = relax * b.l;
+                double nw[5];
+                for (int c = 0; c < 5; ++c) nw[c] = o[c] + dqn[c];
+                double dens = nw[0], uvel = nw[1] / nw[0], vvel = nw[2] / nw[0], wvel = nw[3] / nw[0], E = nw[4] / nw[0];
+                double press_new = (Gamma - F32C(1.0)) * dens * (E - F32C(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel));
+                double press_old = b.p;
+                double Vn_old = o[1] / o[0] * nx + o[2] / o[0] * ny + o[3] / o[0] * nz;
+                double Vn_new = uvel * nx + vvel * ny + wvel * nz;
+                double df[5];
+                df[0] = nw[0] * (Vn_new - Vn_e) - o[0] * (Vn_old - Vn_e);
+                df[1] = (nw[1] * (Vn_new - Vn_e) + press_new * nx) - (o[1] * (Vn_old - Vn_e) + press_old * nx);
+                df[2] = (nw[2] * (Vn_new - Vn_e) + press_new * ny) - (o[2] * (Vn_old - Vn_e) + press_old * ny);
+                df[3] = (nw[3] * (Vn_new - Vn_e) + press_new * nz) - (o[3] * (Vn_old - Vn_e) + press_old * nz);
+                df[4] = ((nw[4] + press_new) * (Vn_new - Vn_e) + press_new * Vn_e) -
+                        ((o[4] + press_old) * (Vn_old - Vn_e) + press_old * Vn_e);
+                double A[5], V[5];
+                for (int c = 0; c < 5; ++c) {
+                    const double a = FWD ? (df[c] * sd + dqn[c] * cL) * F32C(0.5) : (df[c] * sd - dqn[c] * cL) * F32C(0.5);
+                    const double v = dqn[c] * 0.0;   // lamv * dQ, lamv = 0 on the inviscid path
+                    A[c] = use ? a : 0.0;
+                    V[c] = use ? v : 0.0;
+                }
+                for (int c = 0; c < 5; ++c) { sA[d][c][lane] = A[c]; sV[d][c][lane] = V[c]; }
+                if (d == 0) {
+                    for (int c = 0; c < 5; ++c) sR[c][lane] = b.r[c];
+                    sR[5][lane] = b.a;
+                }
+                bar();   // terms of step s are in LDS
+                bar();   // dQ of step s is in LDS
+            };
+            SweepPf b[NB] = {};
+            __builtin_amdgcn_s_waitcnt(0);
+            for (int s0 = -kSweepAhead; s0 < send; s0 += NB) {
+#pragma unroll
+                for (int u = 0; u < NB; ++u) {
+                    const int s = s0 + u;
+                    issue(b[(u + kSweepAhead) % NB], s + kSweepAhead, b[u].pend);
+                    work(b[u], s);
+                }
+            }
+        };
+        if (wave == 0) {
+            dir_loop(Dir<0>());
+        } else if (wave == 1) {
+            dir_loop(Dir<1>());
+        } else if (wave == 2) {
+            dir_loop(Dir<2>());
+        } else {
+            // ---- combine wave (source/LU_SGS_Marching.cpp:155, 255)
+            const gi_t my_ctr = (gi_t)(done + q * kDonePad);
+            for (int s = -kSweepAhead; s < send; ++s) {
+                int i;
+                const bool valid = cell_at(s, i);
+                const int ro = clampr(row_of(s));
+                double outv[5];
+                bar();
+                if (valid) {
+                    const double alpha = sR[5][lane];
+                    for (int c = 0; c < 5; ++c) {
+                        double v;
+                        if (FWD) {
+                            v = sR[c][lane] * F32C(-1.0) + sA[0][c][lane];
+                            v = v + sA[1][c][lane]; v = v + sA[2][c][lane];
+                        } else {
+                            v = sR[c][lane] * alpha - sA[0][c][lane];
+                            v = v - sA[1][c][lane]; v = v - sA[2][c][lane];
+                        }
+                        v = v + sV[0][c][lane]; v = v + sV[1][c][lane]; v = v + sV[2][c][lane];
+                        outv[c] = v / alpha;
+                        sdq[s & 1][c][lane] = outv[c];
+                    }
+                }
+                bar();
+                // the stores of step s-1 have completed: publish s finished steps
+                __builtin_amdgcn_s_waitcnt(0);
+                if (s > 0 && lane == 0) __hip_atomic_store(my_ctr, min(s, nsteps), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (valid)
+                    for (int c = 0; c < 5; ++c)
+                        __hip_atomic_store(drows + ro * RD + c * 64 + lane, outv[c], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            if (lane == 0) __hip_atomic_store(my_ctr, nsteps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+template __global__ void k_lusgs_sl<true, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
+template __global__ void k_lusgs_sl<false, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
+
 __global__ void k_fill_int(int *p, long n, int v)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;

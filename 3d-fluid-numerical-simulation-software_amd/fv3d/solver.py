@@ -108,9 +108,11 @@ class GpuSolver:
     def set_timing(self, on: bool):
         self._check(self.L.fv3d_set_timing(self.h, int(on)), "set_timing")
 
+    TIMING_SLOTS = 7   # FV3D_TIMING_SLOTS
+
     def timing(self):
-        ms = np.zeros(5)
-        n = np.zeros(5, dtype=np.int64)
+        ms = np.zeros(self.TIMING_SLOTS)
+        n = np.zeros(self.TIMING_SLOTS, dtype=np.int64)
         self._check(self.L.fv3d_timing(self.h, ms.ctypes.data, n.ctypes.data), "timing")
         return ms, n
 

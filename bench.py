@@ -30,14 +30,14 @@ sys.path.insert(0, os.path.join(REPO, "oracle"))
 # recomputed, metrics as stored.
 BYTES_PER_CELL = {
     "residual": 176,          # Q 5, S 9, nodes 3 -> R 5
-    "lusgs": 2 * 232,         # per sweep: R|dQ* 5, Q 5, lamc 3, dt 1, vol 1, S 9 -> dQ 5
+    "sweep": 232,             # one sweep: R|dQ* 5, Q 5, lamc 3, dt 1, vol 1, S 9 -> dQ 5
     "update": 128,            # Q 5, dQ 5, vol 1 -> Q 5
     "bc_spectral_dt": 152,    # Q 5, S 9, vol 1 -> lamc 3, dt 1
 }
-# HBM bytes one iteration's two sweeps actually move on the default workload,
-# from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (copy-calibrated), see
-# profiles/lusgs_iteration_bump256x64x64.md.  Keyed by (ni, nj, nk, sweep form).
-MEASURED_SWEEP_TRAFFIC = {(64, 64, 256, "kba3"): (1311.7e6 + 1009.9e6)}
+# HBM bytes one forward-sweep launch actually moves on a workload, from
+# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (copy-calibrated); the
+# summaries are under profiles/.  Keyed by (ni, nj, nk, sweep form).
+MEASURED_FWD_SWEEP_TRAFFIC = {}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--nj", type=int, default=64)
     ap.add_argument("--nk", type=int, default=256)
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--sweep", choices=["kba3", "kba2", "kba", "plane"], default="kba3")
+    ap.add_argument("--sweep", choices=["rows", "kba3", "kba2", "kba", "plane"], default="rows")
     ap.add_argument("--chunk", type=int, default=1)
     ap.add_argument("--lag", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -106,7 +106,8 @@ def main():
     from fv3d.solver import GpuSolver
 
     case = cases.bump_channel(ni=a.ni, nj=a.nj, nk=a.nk, seed=1, max_iter=a.steps)
-    g = GpuSolver(case, device=0, debug={"plane": 2, "kba": 4, "kba2": 8, "kba3": 0}[a.sweep] | (a.chunk << 8) | (a.lag << 16))
+    g = GpuSolver(case, device=0,
+                  debug={"plane": 2, "kba": 4, "kba2": 8, "kba3": 16, "rows": 0}[a.sweep] | (a.chunk << 8) | (a.lag << 16))
     ncell = g.n_cells
     graph = not a.no_graph
     g.iterate(a.warmup, graph=graph)
@@ -122,10 +123,18 @@ def main():
     g.iterate(max(5, min(a.steps, 20)), graph=False)
     ms, _ = g.timing()
     g.set_timing(False)
-    classes = ["residual", "lusgs", "update", "bc_spectral_dt"]
+    classes = ["residual", "lusgs", "update", "bc_spectral_dt", "finalize"]
     times = {cl: ms[i] for i, cl in enumerate(classes)}
-    dom = max(times, key=times.get)
-    achieved = BYTES_PER_CELL[dom] * ncell / (times[dom] * 1e-3) / 1e9
+    if ms[5] > 0:
+        # dominant kernel: the forward sweep, timed alone with HIP events on the
+        # library's stream
+        kname, kms, kbytes = "k_lusgs_sl<forward> (LU-SGS forward sweep)", ms[5], BYTES_PER_CELL["sweep"]
+        times["forward_sweep"], times["backward_sweep"] = ms[5], ms[6]
+        traffic = MEASURED_FWD_SWEEP_TRAFFIC.get((a.ni, a.nj, a.nk, a.sweep))
+    else:
+        kname, kms, kbytes = "LU-SGS forward+backward sweeps", ms[1], 2 * BYTES_PER_CELL["sweep"]
+        traffic = None
+    achieved = kbytes * ncell / (kms * 1e-3) / 1e9
     cpu = None if a.no_cpu_baseline else cpu_baseline(a.cpu_iters)
     out = {
         "metric": "cell-updates/s",
@@ -143,11 +152,9 @@ def main():
         "config": {"workload": f"inviscid bump channel {a.nk}x{a.nj}x{a.ni} cells (single block), Roe + MUSCL3 "
                                f"primitive, LU-SGS local dt CFL 5, plain FP64",
                    "cells": ncell, "hip_graph": graph, "sweep": a.sweep, "chunk": a.chunk, "lag": a.lag},
-        "roofline": {"bound": "hbm", "kernel": "lusgs forward+backward sweep" if dom == "lusgs" else dom,
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": MEASURED_SWEEP_TRAFFIC.get((a.ni, a.nj, a.nk, a.sweep)) if dom == "lusgs" else None,
-                     "kernel_ms": times, "bytes_per_cell": BYTES_PER_CELL[dom]},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel_ms_avg": kms,
+                     "bytes_per_cell": kbytes, "class_ms": times},
         "cpu_baseline": cpu,
         "final_res_dens": float(hist[-1, 1]),
     }

@@ -118,13 +118,20 @@ int fv3d_iterate_graph(fv3d_ctx *ctx, int n, double *hist);
 int fv3d_sync(fv3d_ctx *ctx);
 
 /* Average device time in milliseconds of the last timed iterations, per
- * kernel class (0 residual, 1 LU-SGS sweeps, 2 update, 3 ghost fill + spectral
- * + time step, 4 norms/integrals).  Filled by fv3d_iterate when timing is on. */
+ * kernel class: 0 residual, 1 LU-SGS sweeps (with the row gather and dQ
+ * scatter of the default sweep form), 2 update, 3 ghost fill + spectral + time
+ * step, 4 norms/integrals, 5 forward sweep kernel, 6 backward sweep kernel
+ * (5 and 6 for the default form only).  ms and counts hold FV3D_TIMING_SLOTS
+ * entries.  Filled by fv3d_iterate when timing is on. */
+#define FV3D_TIMING_SLOTS 7
 int fv3d_set_timing(fv3d_ctx *ctx, int on);
-int fv3d_timing(fv3d_ctx *ctx, double *ms5, long *counts5);
+int fv3d_timing(fv3d_ctx *ctx, double *ms, long *counts);
 
 /* Debug flags: bit 0 keeps the face fluxes of the last residual evaluation
- * so fv3d_field_download can return fcI/fcJ/fcK (single-block meshes). */
+ * so fv3d_field_download can return fcI/fcJ/fcK (single-block meshes).
+ * Bits 1-4 select another LU-SGS sweep form for comparison (1: one launch per
+ * hyperplane, 2-4: earlier wavefront forms); bits 8-15 and 16-23 set the
+ * chunk and extra lag of the wavefront forms. */
 int fv3d_set_debug(fv3d_ctx *ctx, int flags);
 
 /* Number of interior cells over all blocks. */

@@ -98,22 +98,20 @@ def test_c0_against_reference_history(golden_dir):
 
 
 def test_wavefront_sweep_equals_plane_sweep():
-    """The persistent wavefront sweep and the one-launch-per-hyperplane sweep
-    visit cells in different orders; both must give the reference's
-    Gauss-Seidel result exactly.  Grid sizes leave partial 8x8 tiles."""
+    """The persistent wavefront sweeps and the one-launch-per-hyperplane sweep
+    visit cells in different orders; all must give the reference's
+    Gauss-Seidel result exactly.  Grid sizes leave partial 4x16 tiles."""
     case = cases.bump_channel(ni=19, nj=13, nk=29, seed=7, max_iter=3)
     print("case wavefront vs plane 19x13x29", flush=True)
-    a = GpuSolver(case)            # four-wave wavefront (default)
-    b = GpuSolver(case, debug=2)   # per-plane launches
-    c = GpuSolver(case, debug=4)   # first wavefront form
-    e = GpuSolver(case, debug=8)   # prefetching one-wave wavefront
-    c.call("lusgs")
-    e.call("lusgs")
-    a.call("lusgs")
-    b.call("lusgs")
-    np.testing.assert_array_equal(a.get(1, "dq"), b.get(1, "dq"))
-    np.testing.assert_array_equal(c.get(1, "dq"), b.get(1, "dq"))
-    np.testing.assert_array_equal(e.get(1, "dq"), b.get(1, "dq"))
+    a = GpuSolver(case)             # four waves on the tile-skewed rows (default)
+    b = GpuSolver(case, debug=2)    # per-plane launches
+    c = GpuSolver(case, debug=4)    # first wavefront form
+    e = GpuSolver(case, debug=8)    # prefetching one-wave wavefront
+    f = GpuSolver(case, debug=16)   # four waves on the cell box
+    for g in (c, e, f, a, b):
+        g.call("lusgs")
+    for g in (a, c, e, f):
+        np.testing.assert_array_equal(g.get(1, "dq"), b.get(1, "dq"))
     ha, hb = a.iterate(3), b.iterate(3)
     np.testing.assert_array_equal(ha, hb)
     o = O.Oracle(case)
@@ -122,13 +120,13 @@ def test_wavefront_sweep_equals_plane_sweep():
 
 
 @pytest.mark.parametrize("form,chunk,lag", [(1, 1, 0), (1, 4, 0), (1, 16, 0), (2, 1, 0), (2, 4, 4), (2, 2, 7),
-                                            (3, 1, 0), (3, 1, 3)])
+                                            (3, 1, 0), (3, 1, 3), (4, 1, 0), (4, 1, 3)])
 def test_wavefront_chunking(form, chunk, lag):
     """Chunked progress publishing and extra consumer lag change only the
     timing of the wavefront sweep, never its result."""
     case = cases.bump_channel(ni=10, nj=17, nk=33, seed=8, max_iter=2)
     print(f"case wavefront form {form} chunk {chunk} lag {lag}", flush=True)
-    a = GpuSolver(case, debug={1: 4, 2: 8, 3: 0}[form] | (chunk << 8) | (lag << 16))
+    a = GpuSolver(case, debug={1: 4, 2: 8, 3: 16, 4: 0}[form] | (chunk << 8) | (lag << 16))
     b = GpuSolver(case, debug=2)
     np.testing.assert_array_equal(a.iterate(2), b.iterate(2))
     np.testing.assert_array_equal(a.get(1, "cv"), b.get(1, "cv"))
@@ -151,3 +149,33 @@ def test_cpp_runner_matches_python(tmp_path):
     h_py = g.iterate(12, graph=True)
     g.close()
     np.testing.assert_array_equal(h_cpp, h_py)
+
+
+def two_block_case():
+    """Two independent bump channels of different sizes in one mesh: every
+    per-block table (sweep rows, tile lists, cell ranges) is exercised."""
+    a = cases.bump_channel(ni=7, nj=9, nk=21, seed=11, max_iter=4)
+    b = cases.bump_channel(ni=12, nj=5, nk=17, seed=12, max_iter=4)
+    for p in b.patches:
+        p.block = 2
+    b.blocks[0].bid = 2
+    a.nodes.append(b.nodes[0] + np.array([2.0, 0.0, 0.0]))
+    a.patches += b.patches
+    a.blocks.append(b.blocks[0])
+    a.states.append(b.states[0])
+    return a
+
+
+def test_two_blocks_against_oracle_and_plane_sweep():
+    case = two_block_case()
+    print("case two blocks 7x9x21 + 12x5x17", flush=True)
+    a = GpuSolver(case)
+    b = GpuSolver(case, debug=2)
+    ha, hb = a.iterate(4), b.iterate(4)
+    np.testing.assert_array_equal(ha, hb)
+    o = O.Oracle(case)
+    ho = o.iterate(4)
+    assert relerr(ha[:, 1], ho[:, 1]) <= 1e-10
+    for blk in (1, 2):
+        np.testing.assert_array_equal(a.get(blk, "cv"), b.get(blk, "cv"))
+        assert relerr(a.get(blk, "cv"), o.get(blk, "cv")) <= 1e-11
