@@ -37,7 +37,7 @@ BYTES_PER_CELL = {
 # HBM bytes one forward-sweep launch actually moves on a workload, from
 # rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (copy-calibrated); the
 # summaries are under profiles/.  Keyed by (ni, nj, nk, sweep form).
-MEASURED_FWD_SWEEP_TRAFFIC = {}
+MEASURED_FWD_SWEEP_TRAFFIC = {(64, 64, 256, "rows"): 487.2e6}
 HBM_PEAK_GBS = 8000.0
 
 
