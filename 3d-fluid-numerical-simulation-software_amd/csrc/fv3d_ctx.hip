@@ -32,6 +32,7 @@ struct HostBlock {
     BlockDev dev;                // device pointers + dims
     std::vector<void *> allocs;
     long sweep_rows = 0;         // rows of the tile-skewed sweep layout
+    long pack_groups = 0;        // workgroups of k_sweep_pack / k_sweep_unpack
 };
 
 }  // namespace
@@ -274,11 +275,12 @@ static int setup_block(fv3d_ctx *c, HostBlock &b, int index)
     if ((rc = alloc(&B.R, 5 * B.box))) return rc;
     if ((rc = alloc(&B.dQ, 5 * B.box))) return rc;
     {
-        // tile-skewed sweep rows (filled by k_sweep_gather)
+        // tile-skewed sweep rows (k_sweep_fill at set-up, k_sweep_pack per iteration)
         B.ntk = (B.nk + kKbaTK - 1) / kKbaTK;
         B.nr = B.ni + kKbaTJ + kKbaTK - 2;
         const long ntile = (long)((B.nj + kKbaTJ - 1) / kKbaTJ) * B.ntk;
         b.sweep_rows = ntile * B.nr;
+        b.pack_groups = ntile * ((B.ni + kPackChunk - 1) / kPackChunk);
         if ((rc = alloc(&B.srow, b.sweep_rows * kSweepQ * 64))) return rc;
         if ((rc = alloc(&B.sdq, b.sweep_rows * 5 * 64))) return rc;
     }
@@ -430,8 +432,8 @@ int fv3d_setup(fv3d_ctx *c)
     HIPCK(c, hipMemcpyAsync(c->d_blocks, bd.data(), nb * sizeof(BlockDev), hipMemcpyHostToDevice, c->stream));
     for (int n = 0; n < nb; ++n) {
         const long rows = c->blocks[n].sweep_rows;
-        hipLaunchKernelGGL(k_sweep_gather, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, c->stream, c->d_blocks, n, rows,
-                           kGatherMetrics);
+        hipLaunchKernelGGL(k_sweep_fill, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, c->stream, c->d_blocks, n, rows,
+                           kGatherState | kGatherMetrics);
         HIPCK(c, hipGetLastError());
     }
     // patch work list
@@ -527,7 +529,7 @@ int fv3d_setup(fv3d_ctx *c)
     }
     c->npart = (int)((c->ncell + 255) / 256);
     if ((rc = dalloc(c, &c->d_partial, 5 * (size_t)c->npart))) return rc;
-    if ((rc = dalloc(c, &c->d_sums, 6))) return rc;
+    if ((rc = dalloc(c, &c->d_sums, 2 * 3 * kSecParts))) return rc;
     if ((rc = dalloc(c, &c->d_stage_row, 7))) return rc;
     // device scalars
     ScalarsDev sc;
@@ -609,11 +611,8 @@ static int launch_sweeps_kba(fv3d_ctx *c)
     int *df = c->d_kba_sync + 64, *db = df + (long)c->nkba * kDonePad;
     if (c->sweep_mode == 4) {
         const int nb = (int)c->blocks.size();
-        for (int n = 0; n < nb; ++n) {
-            const long rows = c->blocks[n].sweep_rows;
-            hipLaunchKernelGGL(k_sweep_gather, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, c->stream, c->d_blocks, n,
-                               rows, kGatherState);
-        }
+        for (int n = 0; n < nb; ++n)
+            hipLaunchKernelGGL(k_sweep_pack, dim3((unsigned)c->blocks[n].pack_groups), dim3(256), 0, c->stream, c->d_blocks, n);
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[6], c->stream));
         hipLaunchKernelGGL((k_lusgs_sl<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
                            c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_lag);
@@ -621,8 +620,9 @@ static int launch_sweeps_kba(fv3d_ctx *c)
         hipLaunchKernelGGL((k_lusgs_sl<false, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
                            c->d_blocks, c->d_kba[1], c->nkba, tb, db, err, c->gas.relax, c->gas, c->kba_lag);
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[8], c->stream));
-        hipLaunchKernelGGL(k_sweep_scatter, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream,
-                           c->d_blocks, c->d_cstart, nb);
+        for (int n = 0; n < nb; ++n)
+            hipLaunchKernelGGL(k_sweep_unpack, dim3((unsigned)c->blocks[n].pack_groups), dim3(256), 0, c->stream, c->d_blocks,
+                               n);
     } else if (c->sweep_mode == 3) {
         hipLaunchKernelGGL((k_lusgs_kba3<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
                            c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_lag);
@@ -675,7 +675,7 @@ static int launch_update(fv3d_ctx *c)
 
 static int launch_finalize(fv3d_ctx *c, double *hist_row)
 {
-    hipLaunchKernelGGL(k_sections, dim3(2), dim3(256), 0, c->stream, c->d_blocks, c->d_secs, c->gas, c->d_sums);
+    hipLaunchKernelGGL(k_sections, dim3(2 * kSecParts), dim3(256), 0, c->stream, c->d_blocks, c->d_secs, c->gas, c->d_sums);
     HIPCK(c, hipGetLastError());
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->d_partial, c->npart, c->d_sums, c->d_sc, c->gas,
                        hist_row);
@@ -760,11 +760,16 @@ int fv3d_res_calculate(fv3d_ctx *c, double *res6)
 int fv3d_perf(fv3d_ctx *c, double *out5)
 {
     READY(c);
-    hipLaunchKernelGGL(k_sections, dim3(2), dim3(256), 0, c->stream, c->d_blocks, c->d_secs, c->gas, c->d_sums);
+    hipLaunchKernelGGL(k_sections, dim3(2 * kSecParts), dim3(256), 0, c->stream, c->d_blocks, c->d_secs, c->gas, c->d_sums);
     HIPCK(c, hipGetLastError());
-    double s[6];
-    HIPCK(c, hipMemcpyAsync(s, c->d_sums, sizeof s, hipMemcpyDeviceToHost, c->stream));
+    double part[2 * 3 * kSecParts];
+    HIPCK(c, hipMemcpyAsync(part, c->d_sums, sizeof part, hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
+    // the partials in the order k_finalize adds them
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int sec = 0; sec < 2; ++sec)
+        for (int p = 0; p < kSecParts; ++p)
+            for (int q = 0; q < 3; ++q) s[sec * 3 + q] += part[(sec * kSecParts + p) * 3 + q];
     ScalarsDev sc;
     HIPCK(c, hipMemcpy(&sc, c->d_sc, sizeof sc, hipMemcpyDeviceToHost));
     const double Gamma = c->gas.gamma, Rgas = c->gas.rgas;

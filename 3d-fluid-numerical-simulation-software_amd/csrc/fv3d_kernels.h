@@ -50,7 +50,7 @@ constexpr int kDonePad = 32;   // progress counters one per 128-byte line
 // quantities of a sweep-layout row (BlockDev::srow): Q (5), p, lamc (3),
 // swg (15), R (5), alpha
 constexpr int kSQ_Q = 0, kSQ_P = 5, kSQ_LAM = 6, kSQ_SWG = 9, kSQ_R = 24, kSQ_A = 29, kSweepQ = 30;
-// what k_sweep_gather copies: the per-iteration state or the static metrics
+// what k_sweep_fill writes: the state quantities and/or the static metrics
 constexpr int kGatherState = 1, kGatherMetrics = 2;
 template <bool FWD, int TJ, int TK>
 __global__ void k_lusgs_kba(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *done, int *err,
@@ -64,8 +64,11 @@ __global__ void k_lusgs_kba3(const BlockDev *blocks, const KbaTile *tiles, int n
 template <bool FWD, int TJ, int TK>
 __global__ void k_lusgs_sl(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *done, int *err,
                            double relax, Gas g, int lag);
-__global__ void k_sweep_gather(const BlockDev *blocks, int block, long nrows, int what);
-__global__ void k_sweep_scatter(const BlockDev *blocks, const long *cstart, int nblock);
+__global__ void k_sweep_fill(const BlockDev *blocks, int block, long nrows, int what);
+__global__ void k_sweep_pack(const BlockDev *blocks, int block);
+__global__ void k_sweep_unpack(const BlockDev *blocks, int block);
+constexpr int kPackChunk = 16;   // i-planes per k_sweep_pack / k_sweep_unpack workgroup
+constexpr int kSecParts = 32;    // workgroups per section in k_sections
 __global__ void k_fill_int(int *p, long n, int v);
 __global__ void k_update(const BlockDev *blocks, const long *cstart, int nblock, Gas g, double *partial, int *bad_flag);
 __global__ void k_limiter_serial(const BlockDev *blocks, int nblock, Gas g, int *bad_flag);

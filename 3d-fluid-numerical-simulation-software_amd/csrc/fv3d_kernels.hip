@@ -848,11 +848,12 @@ template __global__ void k_lusgs_kba3<false, kKbaTJ, kKbaTK>(const BlockDev *, c
 // ---------------------------------------------------------------------------
 // Tile-skewed sweep layout (BlockDev::srow / sdq).
 //
-// k_sweep_gather copies cell data into the rows, one wave per row: lane
-// (tj,tk) of row r of tile (J,K) is cell (r+1-tj-tk, J*TJ+tj+1, K*TK+tk+1).
-// Lanes outside the block get a harmless unit state.  The sweeps then read
-// whole 512-byte rows; k_sweep_scatter copies dQ back to the cell box.
-__global__ void __launch_bounds__(256) k_sweep_gather(const BlockDev *__restrict__ blocks, int block, long nrows, int what)
+// Lane (tj,tk) of row r of tile (J,K) is cell (r+1-tj-tk, J*TJ+tj+1, K*TK+tk+1).
+// k_sweep_fill writes whole rows, one wave per row, giving lanes outside the
+// block a harmless unit state; it runs at set-up (metrics and those lanes).
+// Each iteration k_sweep_pack copies the state of the cells in, and after the
+// sweeps k_sweep_unpack copies dQ back to the cell box.
+__global__ void __launch_bounds__(256) k_sweep_fill(const BlockDev *__restrict__ blocks, int block, long nrows, int what)
 {
     constexpr int TJ = kKbaTJ, TK = kKbaTK;
     const BlockDev &B = blocks[block];
@@ -876,25 +877,71 @@ __global__ void __launch_bounds__(256) k_sweep_gather(const BlockDev *__restrict
         for (int q = 0; q < 15; ++q) dst[(kSQ_SWG + q) * 64] = valid ? B.swg[q * box + x] : 0.0;
 }
 
-__global__ void __launch_bounds__(256) k_sweep_scatter(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
-                                                       int nblock)
+// One workgroup per (tile, chunk of kPackChunk i-planes).  The chunk's cells are
+// read plane by plane along k (coalesced) into LDS, then written out row by
+// row: the rows a chunk touches are partly written (the lanes whose cell lies
+// in the chunk), each as one contiguous run.
+__global__ void __launch_bounds__(256) k_sweep_pack(const BlockDev *__restrict__ blocks, int block)
+{
+    constexpr int TJ = kKbaTJ, TK = kKbaTK, NQ = 15;
+    __shared__ double buf[kPackChunk][64];
+    const BlockDev &B = blocks[block];
+    const int nchunk = (B.ni + kPackChunk - 1) / kPackChunk;
+    const int slot = blockIdx.x / nchunk, i0 = (blockIdx.x % nchunk) * kPackChunk + 1;
+    const int J = slot / B.ntk, K = slot % B.ntk;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int tj = lane / TK, tk = lane % TK, j = J * TJ + tj + 1, k = K * TK + tk + 1;
+    const bool col = j <= B.nj && k <= B.nk;
+    const long box = B.box;
+    // the cell-box array and row quantity of each packed quantity
+    for (int qi = 0; qi < NQ; ++qi) {
+        const double *src;
+        int rq;
+        if (qi < 5) { src = B.Q + qi * box; rq = kSQ_Q + qi; }
+        else if (qi == 5) { src = B.W + 4 * box; rq = kSQ_P; }
+        else if (qi < 9) { src = B.lamc + (qi - 6) * box; rq = kSQ_LAM + qi - 6; }
+        else if (qi < 14) { src = B.R + (qi - 9) * box; rq = kSQ_R + qi - 9; }
+        else { src = B.alpha; rq = kSQ_A; }
+        for (int ci = w; ci < kPackChunk; ci += 4) {
+            const int i = i0 + ci;
+            if (col && i <= B.ni) buf[ci][lane] = src[cidx(B, i, j, k)];
+        }
+        __syncthreads();
+        // rows r = i + tj + tk - 1 for i in the chunk
+        for (int rr = w; rr < kPackChunk + TJ + TK - 2; rr += 4) {
+            const int r = i0 - 1 + rr, i = r + 1 - tj - tk;
+            if (col && i >= i0 && i < i0 + kPackChunk && i <= B.ni)
+                B.srow[((long)slot * B.nr + r) * (kSweepQ * 64) + rq * 64 + lane] = buf[i - i0][lane];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict__ blocks, int block)
 {
     constexpr int TJ = kKbaTJ, TK = kKbaTK;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cstart[nblock]) return;
-    int n = 0;
-    while (t >= cstart[n + 1]) ++n;
-    const BlockDev &B = blocks[n];
-    long r = t - cstart[n];
-    const int k = (int)(r % B.nk) + 1;
-    r /= B.nk;
-    const int j = (int)(r % B.nj) + 1;
-    const int i = (int)(r / B.nj) + 1;
-    const int tj = (j - 1) % TJ, tk = (k - 1) % TK;
-    const int slot = ((j - 1) / TJ) * B.ntk + (k - 1) / TK, row = i + tj + tk - 1;
-    const double *src = B.sdq + ((long)slot * B.nr + row) * (5 * 64) + tj * TK + tk;
-    const long box = B.box, x = cidx(B, i, j, k);
-    for (int c = 0; c < 5; ++c) B.dQ[c * box + x] = src[c * 64];
+    __shared__ double buf[kPackChunk][64];
+    const BlockDev &B = blocks[block];
+    const int nchunk = (B.ni + kPackChunk - 1) / kPackChunk;
+    const int slot = blockIdx.x / nchunk, i0 = (blockIdx.x % nchunk) * kPackChunk + 1;
+    const int J = slot / B.ntk, K = slot % B.ntk;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int tj = lane / TK, tk = lane % TK, j = J * TJ + tj + 1, k = K * TK + tk + 1;
+    const bool col = j <= B.nj && k <= B.nk;
+    const long box = B.box;
+    for (int c = 0; c < 5; ++c) {
+        for (int rr = w; rr < kPackChunk + TJ + TK - 2; rr += 4) {
+            const int r = i0 - 1 + rr, i = r + 1 - tj - tk;
+            if (col && i >= i0 && i < i0 + kPackChunk && i <= B.ni)
+                buf[i - i0][lane] = B.sdq[((long)slot * B.nr + r) * (5 * 64) + c * 64 + lane];
+        }
+        __syncthreads();
+        for (int ci = w; ci < kPackChunk; ci += 4) {
+            const int i = i0 + ci;
+            if (col && i <= B.ni) B.dQ[c * box + cidx(B, i, j, k)] = buf[ci][lane];
+        }
+        __syncthreads();
+    }
 }
 
 // Fourth form of the wavefront sweep: the four-wave schedule of k_lusgs_kba3
@@ -1431,17 +1478,20 @@ __global__ void k_limiter_serial(const BlockDev *__restrict__ blocks, int nblock
 // Mass flow, mass-averaged total pressure and total temperature over one K
 // plane (source/Ouput.cpp:9-95).  One workgroup per section, fixed-order tree
 // reduction; sums[sec*5 + {mass, ptm, ttm}].
+// grid: 2 x kSecParts workgroups; workgroup p of a section sums the cells
+// p*256 + t, (p + kSecParts)*256 + t, ... into sums[(sec*kSecParts + p)*3 + c]
 __global__ void __launch_bounds__(256) k_sections(const BlockDev *__restrict__ blocks, const int2 *__restrict__ secs, Gas g,
                                                   double *__restrict__ sums)
 {
     __shared__ double red[3][256];
-    const int2 sc = secs[blockIdx.x];
+    const int sec = blockIdx.x / kSecParts, part = blockIdx.x % kSecParts;
+    const int2 sc = secs[sec];
     const BlockDev &B = blocks[sc.x];
     const int K = sc.y;
     const long box = B.box;
     const double Gamma = g.gamma, Rgas = g.rgas;
     double m = 0.0, pt = 0.0, tt = 0.0;
-    for (int f = threadIdx.x; f < B.ni * B.nj; f += blockDim.x) {
+    for (int f = part * blockDim.x + threadIdx.x; f < B.ni * B.nj; f += kSecParts * blockDim.x) {
         int i = f / B.nj + 1, j = f % B.nj + 1;
         long a = cidx(B, i, j, K), z = cidx(B, i, j, K - 1);
         double dens = F32C(0.5) * (B.W[a] + B.W[z]), u = F32C(0.5) * (B.W[box + a] + B.W[box + z]);
@@ -1469,7 +1519,7 @@ __global__ void __launch_bounds__(256) k_sections(const BlockDev *__restrict__ b
 // Residual norm (fixed-order sum of the per-workgroup partials), performance
 // parameters (source/Ouput.cpp:97-107,282-300), CFL ramp for the next iteration
 // (source/Solver.cpp:49-60) and the history row.  One workgroup.
-__global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ partial, int npart, const double *__restrict__ sums,
+__global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ partial, int npart, const double *__restrict__ secsum,
                                                   ScalarsDev *__restrict__ sc, Gas g, double *__restrict__ hist_row)
 {
     __shared__ double red[5][256];
@@ -1486,6 +1536,11 @@ __global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ par
     if (threadIdx.x != 0) return;
     for (int c = 0; c < 5; ++c) sc->res[c] = red[c][0] / sc->total_vol;
     const double Gamma = g.gamma, Rgas = g.rgas;
+    // section partials in a fixed order: section 0 inlet, 1 outlet; mass, ptm, ttm
+    double sums[6] = {0, 0, 0, 0, 0, 0};
+    for (int sec = 0; sec < 2; ++sec)
+        for (int p = 0; p < kSecParts; ++p)
+            for (int c = 0; c < 3; ++c) sums[sec * 3 + c] += secsum[(sec * kSecParts + p) * 3 + c];
     // sums: section 0 inlet, section 1 outlet: mass, ptm, ttm
     double mi = sums[0] * sc->dens_ref * sc->vel_ref * sc->nblade_in;
     double mo = sums[3] * sc->dens_ref * sc->vel_ref * sc->nblade_out;
