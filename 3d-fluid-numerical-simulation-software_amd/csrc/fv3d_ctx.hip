@@ -283,6 +283,9 @@ static int setup_block(fv3d_ctx *c, HostBlock &b, int index)
         b.pack_groups = ntile * ((B.ni + kPackChunk - 1) / kPackChunk);
         if ((rc = alloc(&B.srow, b.sweep_rows * kSweepQ * 64))) return rc;
         if ((rc = alloc(&B.sdq, b.sweep_rows * 5 * 64))) return rc;
+        // both mailboxes in one allocation, reset together each iteration
+        if ((rc = alloc(&B.mbox[0], 2 * b.sweep_rows * kMboxRow))) return rc;
+        B.mbox[1] = B.mbox[0] + b.sweep_rows * kMboxRow;
     }
     // Dummy slabs: tangential boxes (JM+1)(KM+1), (IM+1)(KM+1), (IM+1)(JM+1)
     const int ta[3] = {b.JM, b.IM, b.IM}, tb[3] = {b.KM, b.KM, b.JM};
@@ -611,14 +614,19 @@ static int launch_sweeps_kba(fv3d_ctx *c)
     int *df = c->d_kba_sync + 64, *db = df + (long)c->nkba * kDonePad;
     if (c->sweep_mode == 4) {
         const int nb = (int)c->blocks.size();
-        for (int n = 0; n < nb; ++n)
-            hipLaunchKernelGGL(k_sweep_pack, dim3((unsigned)c->blocks[n].pack_groups), dim3(256), 0, c->stream, c->d_blocks, n);
+        for (int n = 0; n < nb; ++n) {
+            const HostBlock &hb = c->blocks[n];
+            const long nw = 2 * hb.sweep_rows * kMboxRow;
+            hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, c->stream,
+                               (unsigned long long *)hb.dev.mbox[0], nw, kMboxEmpty);
+            hipLaunchKernelGGL(k_sweep_pack, dim3((unsigned)hb.pack_groups), dim3(256), 0, c->stream, c->d_blocks, n);
+        }
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[6], c->stream));
         hipLaunchKernelGGL((k_lusgs_sl<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
-                           c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_lag);
+                           c->d_blocks, c->d_kba[0], c->nkba, tf, err, c->gas.relax, c->gas);
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[7], c->stream));
         hipLaunchKernelGGL((k_lusgs_sl<false, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
-                           c->d_blocks, c->d_kba[1], c->nkba, tb, db, err, c->gas.relax, c->gas, c->kba_lag);
+                           c->d_blocks, c->d_kba[1], c->nkba, tb, err, c->gas.relax, c->gas);
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[8], c->stream));
         for (int n = 0; n < nb; ++n)
             hipLaunchKernelGGL(k_sweep_unpack, dim3((unsigned)c->blocks[n].pack_groups), dim3(256), 0, c->stream, c->d_blocks,

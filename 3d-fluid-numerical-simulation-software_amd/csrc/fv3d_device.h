@@ -62,6 +62,11 @@ struct BlockDev {
     double *sdq;              // per row: dQ, 5 components x 64 lanes
     int ntk;                  // tiles along k
     int nr;                   // rows per tile = ni + kKbaTJ + kKbaTK - 2
+    // Edge mailboxes of the row-layout sweeps (forward, backward): per tile
+    // row, the dQ of the lanes a downstream tile reads across the j edge
+    // (5 x kKbaTK) and across the k edge (5 x kKbaTJ).  Reset to a sentinel
+    // before each iteration; a consumer waits until its words are not it.
+    double *mbox[2];
 };
 
 __host__ __device__ inline long cidx(const BlockDev &B, int i, int j, int k)

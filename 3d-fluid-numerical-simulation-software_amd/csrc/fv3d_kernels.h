@@ -62,8 +62,13 @@ template <bool FWD, int TJ, int TK>
 __global__ void k_lusgs_kba3(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *done, int *err,
                              double relax, Gas g, int lag);
 template <bool FWD, int TJ, int TK>
-__global__ void k_lusgs_sl(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *done, int *err,
-                           double relax, Gas g, int lag);
+__global__ void k_lusgs_sl(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *err, double relax,
+                           Gas g);
+// mailbox words per tile row, and the bit pattern of an empty word (a NaN
+// payload no arithmetic produces)
+constexpr int kMboxJ = 5 * kKbaTK, kMboxRow = 5 * (kKbaTK + kKbaTJ);
+constexpr unsigned long long kMboxEmpty = 0x7ff4dead5eed0001ull;
+__global__ void k_fill_u64(unsigned long long *p, long n, unsigned long long v);
 __global__ void k_sweep_fill(const BlockDev *blocks, int block, long nrows, int what);
 __global__ void k_sweep_pack(const BlockDev *blocks, int block);
 __global__ void k_sweep_unpack(const BlockDev *blocks, int block);

@@ -622,11 +622,21 @@ struct SweepPf {
     double q[5], p, l, m[5], x[5], r[5], a;
     int pend;
 };
+// row-layout sweep: per step, the neighbour's static data with the terms of
+// the flux difference that do not depend on dQ, the upstream dQ, and (wave 0)
+// the cell's right-hand side and diagonal
+struct RowPf {
+    double q[5], m[5], of[5], vn_old, cl;   // state, metrics, old-state flux terms
+    double p, l;
+    double x[5];
+    double r[5], a;
+};
 template <int D>
 struct Dir {
     static constexpr int value = D;
 };
 constexpr int kSweepAhead = 1;   // prefetch distance of the direction waves, in steps
+constexpr int kRowS0 = -2;       // first (idle) step of the row-layout sweep loops
 }  // namespace
 
 template <bool FWD, int TJ, int TK>
@@ -949,11 +959,17 @@ __global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict
 // direction waves read their neighbours' state and metrics from the previous
 // row (a lane shift) or, on the tile's upstream edge, from the upstream tile's
 // row, so every load is a contiguous run of one row.  dQ goes to the tile's
-// dQ rows.  Progress, waits and publication are those of k_lusgs_kba3.
+// dQ rows.
+//
+// Across a tile edge dQ travels through the upstream tile's mailbox: its
+// combine wave stores the dQ of its edge lanes there as soon as it has them,
+// and the consumer reads the words a step ahead and, before using them, waits
+// (bounded) while any word it needs still holds the empty pattern.  Tiles
+// start in ticket order, so every upstream tile is already running.
 template <bool FWD, int TJ, int TK>
 __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ blocks, const KbaTile *__restrict__ tiles,
-                                                  int ntiles, int *__restrict__ ticket, int *__restrict__ done,
-                                                  int *__restrict__ err, double relax, Gas g, int lag)
+                                                  int ntiles, int *__restrict__ ticket, int *__restrict__ err,
+                                                  double relax, Gas g)
 {
     static_assert(TJ * TK == 64, "64 cells per tile");
     __shared__ double sdq[2][5][64];
@@ -981,8 +997,8 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
         const int j = tl.j0 + tj + 1, k = tl.k0 + tk + 1;
         const bool col = j <= nj && k <= nk;
         const int nsteps = nr;
-        constexpr int NB = kSweepAhead + 1;   // prefetch buffers = unrolling of the step loop
-        const int send = (nsteps + kSweepAhead + NB - 1) / NB * NB - kSweepAhead;
+        // steps run: [kRowS0, send), a whole number of three-step unrollings
+        const int send = (nsteps - kRowS0 + 2) / 3 * 3 + kRowS0;
         auto clampr = [&](int r) { return min(max(r, 0), nr - 1); };
         // own row of a step, and whether this lane's cell is in the block
         auto row_of = [&](int step) { return FWD ? step : nr - 1 - step; };
@@ -1011,38 +1027,46 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
             // in-tile neighbour lane (lanes on the edge read the upstream row instead)
             const int nlane = (d == 0 || edge) ? lane : d == 1 ? lane + (FWD ? -TK : TK) : lane + (FWD ? -1 : 1);
             const bool from_up = d != 0 && edge && dep >= 0;
-            const gi_t dep_ctr = (gi_t)(done + max(dep, 0) * kDonePad);
-            int seen = dep < 0 ? 1 << 30 : 0;
-            auto issue = [&](SweepPf &bn, int step, int pend) {
-                if (d != 0) {
-                    // every path consumes the early look, and not before this point
-                    asm volatile("" : "+v"(pend));
-                    seen = max(seen, pend);
-                    if (dep >= 0) {
-                        const int need = min(step + T + lag, nsteps);
-                        long spins = 0;
-                        while (seen < need) {
-                            seen = __hip_atomic_load(dep_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            if (seen >= need) break;
-                            __builtin_amdgcn_s_sleep(1);
-                            if (++spins > (1L << 22)) { atomicExch(err, 1); seen = 1 << 30; }
-                        }
-                    }
+            // the upstream tile's mailbox words this lane reads (the other lanes
+            // read the same words as the edge lanes)
+            const gd_t umbox = (gd_t)(B.mbox[FWD ? 0 : 1] + (long)uslot * nr * kMboxRow);
+            const int mword = d == 1 ? tk : kMboxJ + tj;
+            const int mstride = d == 1 ? TK : TJ;
+            // upstream dQ of a step, read one step ahead without waiting
+            auto issue_cross = [&](RowPf &bn, int step) {
+                if (d == 0) return;
+                const int urow = clampr(FWD ? row_of(step) + T - 1 : row_of(step) - (T - 1));
+                for (int c = 0; c < 5; ++c)
+                    bn.x[c] = __hip_atomic_load(umbox + urow * kMboxRow + mword + c * mstride, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+            };
+            // before using the upstream dQ of step s: re-read while a word the
+            // step needs is still empty (bounded; a run-out sets the error word)
+            auto settle_cross = [&](RowPf &b, int s, bool need) {
+                if (d == 0) return;
+                auto empty = [&](const double *x) {
+                    bool e = false;
+                    for (int c = 0; c < 5; ++c) e = e || (__builtin_bit_cast(unsigned long long, x[c]) == kMboxEmpty);
+                    return e;
+                };
+                if (!__builtin_amdgcn_ballot_w64(need && empty(b.x))) return;
+                const int urow = clampr(FWD ? row_of(s) + T - 1 : row_of(s) - (T - 1));
+                long spins = 0;
+                while (__builtin_amdgcn_ballot_w64(need && empty(b.x))) {
+                    __builtin_amdgcn_s_sleep(1);
+                    for (int c = 0; c < 5; ++c)
+                        b.x[c] = __hip_atomic_load(umbox + urow * kMboxRow + mword + c * mstride, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                    if (++spins > (1L << 22)) { atomicExch(err, 1); break; }
                 }
+            };
+            // the neighbour's state and metrics of a step (static during the
+            // sweep, so no wait), and for wave 0 the cell's right-hand side
+            auto issue_static = [&](RowPf &bn, int step) {
                 const int rt = row_of(step);
                 const int urow = clampr(FWD ? rt + T - 1 : rt - (T - 1));
                 const int nrow = clampr(FWD ? rt - 1 : rt + 1);
                 const gcd_t src = (from_up ? urows + urow * RQ + ulane : rows + nrow * RQ + nlane);
-                if (d != 0) {
-                    // early look at the upstream counter, consumed kSweepAhead steps later
-                    bn.pend = __hip_atomic_load(dep_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    // upstream dQ for the edge lanes (the other lanes read the same words)
-                    for (int c = 0; c < 5; ++c)
-                        bn.x[c] = __hip_atomic_load(udrows + urow * RD + c * 64 + ulane, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
-                } else {
-                    bn.pend = 0;
-                }
                 for (int c = 0; c < 5; ++c) bn.q[c] = src[(kSQ_Q + c) * 64];
                 bn.p = src[kSQ_P * 64];
                 bn.l = src[(kSQ_LAM + d) * 64];
@@ -1054,36 +1078,50 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                     bn.a = rows[ro * RQ + kSQ_A * 64 + lane];
                 }
             };
-            auto work = [&](const SweepPf &b, int s) {
+            // the terms of the flux difference that use only the old state
+            // (source/LU_SGS_Marching.cpp:293-419; same operations, evaluated early)
+            auto precompute = [&](RowPf &b) {
+                const double *o = b.q;
+                const double nx = b.m[0], ny = b.m[1], nz = b.m[2], Vn_e = b.m[4];
+                const double press_old = b.p;
+                const double Vn_old = o[1] / o[0] * nx + o[2] / o[0] * ny + o[3] / o[0] * nz;
+                b.vn_old = Vn_old;
+                b.of[0] = o[0] * (Vn_old - Vn_e);
+                b.of[1] = o[1] * (Vn_old - Vn_e) + press_old * nx;
+                b.of[2] = o[2] * (Vn_old - Vn_e) + press_old * ny;
+                b.of[3] = o[3] * (Vn_old - Vn_e) + press_old * nz;
+                b.of[4] = (o[4] + press_old) * (Vn_old - Vn_e) + press_old * Vn_e;
+                b.cl = relax * b.l;
+            };
+            // step s: the dQ-dependent part of the flux difference (critical
+            // path), then, while the combine wave works, the loads and early
+            // terms of the next steps
+            auto work = [&](RowPf &b, RowPf &b1, RowPf &b2, int s) {
                 int i;
                 const bool valid = cell_at(s, i);
                 const int p = d == 0 ? i : d == 1 ? j : k, lim = d == 0 ? ni : d == 1 ? nj : nk;
                 const bool use = valid && (FWD ? p != 1 : p != lim);
+                settle_cross(b, s, use && from_up);
                 double dqn[5];
                 for (int c = 0; c < 5; ++c) {
                     const double in_tile = sdq[(s - 1) & 1][c][nlane];
                     dqn[c] = (d != 0 && edge) ? b.x[c] : in_tile;
                 }
                 const double nx = b.m[0], ny = b.m[1], nz = b.m[2], sd = b.m[3], Vn_e = b.m[4];
-                const double *o = b.q;
-                const double cL = relax * b.l;
                 double nw[5];
-                for (int c = 0; c < 5; ++c) nw[c] = o[c] + dqn[c];
+                for (int c = 0; c < 5; ++c) nw[c] = b.q[c] + dqn[c];
                 double dens = nw[0], uvel = nw[1] / nw[0], vvel = nw[2] / nw[0], wvel = nw[3] / nw[0], E = nw[4] / nw[0];
                 double press_new = (Gamma - F32C(1.0)) * dens * (E - F32C(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel));
-                double press_old = b.p;
-                double Vn_old = o[1] / o[0] * nx + o[2] / o[0] * ny + o[3] / o[0] * nz;
                 double Vn_new = uvel * nx + vvel * ny + wvel * nz;
                 double df[5];
-                df[0] = nw[0] * (Vn_new - Vn_e) - o[0] * (Vn_old - Vn_e);
-                df[1] = (nw[1] * (Vn_new - Vn_e) + press_new * nx) - (o[1] * (Vn_old - Vn_e) + press_old * nx);
-                df[2] = (nw[2] * (Vn_new - Vn_e) + press_new * ny) - (o[2] * (Vn_old - Vn_e) + press_old * ny);
-                df[3] = (nw[3] * (Vn_new - Vn_e) + press_new * nz) - (o[3] * (Vn_old - Vn_e) + press_old * nz);
-                df[4] = ((nw[4] + press_new) * (Vn_new - Vn_e) + press_new * Vn_e) -
-                        ((o[4] + press_old) * (Vn_old - Vn_e) + press_old * Vn_e);
+                df[0] = nw[0] * (Vn_new - Vn_e) - b.of[0];
+                df[1] = (nw[1] * (Vn_new - Vn_e) + press_new * nx) - b.of[1];
+                df[2] = (nw[2] * (Vn_new - Vn_e) + press_new * ny) - b.of[2];
+                df[3] = (nw[3] * (Vn_new - Vn_e) + press_new * nz) - b.of[3];
+                df[4] = ((nw[4] + press_new) * (Vn_new - Vn_e) + press_new * Vn_e) - b.of[4];
                 double A[5], V[5];
                 for (int c = 0; c < 5; ++c) {
-                    const double a = FWD ? (df[c] * sd + dqn[c] * cL) * F32C(0.5) : (df[c] * sd - dqn[c] * cL) * F32C(0.5);
+                    const double a = FWD ? (df[c] * sd + dqn[c] * b.cl) * F32C(0.5) : (df[c] * sd - dqn[c] * b.cl) * F32C(0.5);
                     const double v = dqn[c] * 0.0;   // lamv * dQ, lamv = 0 on the inviscid path
                     A[c] = use ? a : 0.0;
                     V[c] = use ? v : 0.0;
@@ -1094,17 +1132,20 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                     sR[5][lane] = b.a;
                 }
                 bar();   // terms of step s are in LDS
+                issue_cross(b1, s + 1);
+                issue_static(b2, s + 2);
+                precompute(b1);
                 bar();   // dQ of step s is in LDS
             };
-            SweepPf b[NB] = {};
+            // b[u] serves step s0+u: static data issued two steps ahead, the
+            // upstream dQ one step ahead.  The loop starts two idle steps early
+            // (every lane invalid) so that no load precedes it and every step
+            // runs the same code; nothing is in flight on entry.
+            RowPf b[3] = {};
             __builtin_amdgcn_s_waitcnt(0);
-            for (int s0 = -kSweepAhead; s0 < send; s0 += NB) {
+            for (int s0 = kRowS0; s0 < send; s0 += 3) {
 #pragma unroll
-                for (int u = 0; u < NB; ++u) {
-                    const int s = s0 + u;
-                    issue(b[(u + kSweepAhead) % NB], s + kSweepAhead, b[u].pend);
-                    work(b[u], s);
-                }
+                for (int u = 0; u < 3; ++u) work(b[u], b[(u + 1) % 3], b[(u + 2) % 3], s0 + u);
             }
         };
         if (wave == 0) {
@@ -1115,8 +1156,11 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
             dir_loop(Dir<2>());
         } else {
             // ---- combine wave (source/LU_SGS_Marching.cpp:155, 255)
-            const gi_t my_ctr = (gi_t)(done + q * kDonePad);
-            for (int s = -kSweepAhead; s < send; ++s) {
+            const gd_t mbox = (gd_t)(B.mbox[FWD ? 0 : 1] + (long)slot * nr * kMboxRow);
+            // downstream edges of this tile: the j edge (lanes tj = jl, word tk)
+            // and the k edge (lanes tk = kl, word tj)
+            const bool jedge = tj == (FWD ? TJ - 1 : 0), kedge = tk == (FWD ? TK - 1 : 0);
+            for (int s = kRowS0; s < send; ++s) {
                 int i;
                 const bool valid = cell_at(s, i);
                 const int ro = clampr(row_of(s));
@@ -1137,24 +1181,32 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                         outv[c] = v / alpha;
                         sdq[s & 1][c][lane] = outv[c];
                     }
+                    // hand the edge values to the downstream tiles at once
+                    if (jedge)
+                        for (int c = 0; c < 5; ++c)
+                            __hip_atomic_store(mbox + ro * kMboxRow + tk + c * TK, outv[c], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                    if (kedge)
+                        for (int c = 0; c < 5; ++c)
+                            __hip_atomic_store(mbox + ro * kMboxRow + kMboxJ + tj + c * TJ, outv[c], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
                 }
                 bar();
-                // the stores of step s-1 have completed: publish s finished steps
-                __builtin_amdgcn_s_waitcnt(0);
-                if (s > 0 && lane == 0) __hip_atomic_store(my_ctr, min(s, nsteps), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (valid)
-                    for (int c = 0; c < 5; ++c)
-                        __hip_atomic_store(drows + ro * RD + c * 64 + lane, outv[c], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
+                    for (int c = 0; c < 5; ++c) drows[ro * RD + c * 64 + lane] = outv[c];
             }
-            __builtin_amdgcn_s_waitcnt(0);
-            if (lane == 0) __hip_atomic_store(my_ctr, nsteps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
 
-template __global__ void k_lusgs_sl<true, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
-template __global__ void k_lusgs_sl<false, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
+template __global__ void k_lusgs_sl<true, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, double, Gas);
+template __global__ void k_lusgs_sl<false, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, double, Gas);
+
+__global__ void k_fill_u64(unsigned long long *p, long n, unsigned long long v)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
 
 __global__ void k_fill_int(int *p, long n, int v)
 {
