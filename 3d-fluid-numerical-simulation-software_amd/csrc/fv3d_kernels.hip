@@ -1181,7 +1181,10 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                         outv[c] = v / alpha;
                         sdq[s & 1][c][lane] = outv[c];
                     }
-                    // hand the edge values to the downstream tiles at once
+                }
+                bar();
+                if (valid) {
+                    // hand the edge values to the downstream tiles, then keep dQ
                     if (jedge)
                         for (int c = 0; c < 5; ++c)
                             __hip_atomic_store(mbox + ro * kMboxRow + tk + c * TK, outv[c], __ATOMIC_RELAXED,
@@ -1190,10 +1193,8 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                         for (int c = 0; c < 5; ++c)
                             __hip_atomic_store(mbox + ro * kMboxRow + kMboxJ + tj + c * TJ, outv[c], __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_AGENT);
-                }
-                bar();
-                if (valid)
                     for (int c = 0; c < 5; ++c) drows[ro * RD + c * 64 + lane] = outv[c];
+                }
             }
         }
     }
