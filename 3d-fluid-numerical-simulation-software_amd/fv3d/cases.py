@@ -84,14 +84,17 @@ def _tanh_cluster(n: int, beta: float) -> np.ndarray:
 def bump_channel(ni: int = 32, nj: int = 32, nk: int = 96, seed: int = 1, beta: float = 2.0,
                  bump_height: float = 0.1, cfl: float = 5.0, max_iter: int = 500,
                  flux: int = 300, recon_scheme: int = 500, recon_var: int = 200,
-                 p_ratio: float = 0.9, n_threads: int = 1) -> Case:
+                 p_ratio: float = 0.9, n_threads: int = 1, viscous: bool = False,
+                 len_ref: float = 1.0) -> Case:
     """Channel with a circular-arc bump on the lower wall, spanwise uniform.
 
     Streamwise z in [-1, 2] chords (index k), wall-normal y in [y_bump(z), 1]
     (index j, tanh-clustered at both walls), span x in [0, 0.5] (index i).
     Inlet k=1 (total conditions), outlet k=KM at ``p_ratio * Pt``, slip/no-slip
-    walls on the other four faces.  Inviscid Euler, Roe + MUSCL3 on primitive
-    variables, LU-SGS with local time step.
+    walls on the other four faces.  Inviscid Euler by default (``viscous``
+    selects laminar Navier-Stokes; the grid is in units of ``len_ref`` metres,
+    which sets the Reynolds number), Roe + MUSCL3 on primitive variables,
+    LU-SGS with local time step.
     """
     im, jm, km = ni + 1, nj + 1, nk + 1
     z = np.linspace(-1.0, 2.0, km)
@@ -106,7 +109,8 @@ def bump_channel(ni: int = 32, nj: int = 32, nk: int = 96, seed: int = 1, beta: 
     xyz[..., 1] = ylow[None, None, :] + (1.0 - ylow[None, None, :]) * eta[None, :, None]
     xyz[..., 2] = z[None, None, :]
 
-    cfg = Config(grid_file="grid.dat", turbo_type="G", equation="E", cfl=cfl, max_iter=max_iter,
+    cfg = Config(grid_file="grid.dat", turbo_type="G", equation="N" if viscous else "E", len_ref=len_ref,
+                 cfl=cfl, max_iter=max_iter,
                  flux_scheme=flux, recon_scheme=recon_scheme, recon_var=recon_var,
                  p_out=p_ratio * 101325.0, n_threads=n_threads)
     patches = [

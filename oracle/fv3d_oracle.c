@@ -22,8 +22,11 @@
 typedef struct { double r, ru, rv, rw, re; } cv_t;
 typedef struct { double d, u, v, w, p, t; } pv_t;
 typedef struct { double x, y, z; } v3_t;
-typedef struct { v3_t S; double s; v3_t c; double r, th; } face_t;
-typedef struct { double vol; v3_t c; double r, th; } cell_t;
+/* g: the nine coordinate-transformation coefficients of JacobiConvertCoeff in
+ * the reference's order gradx_kesi grady_kesi gradz_kesi, gradx_yita ...,
+ * gradx_zita ... (include/defs.h TFace / TCell) */
+typedef struct { v3_t S; double s; v3_t c; double r, th; double g[9]; } face_t;
+typedef struct { double vol; v3_t c; double r, th; double g[9]; } cell_t;
 
 typedef struct {
     int kind, face, is, ie, js, je, ks, ke, rotate;
@@ -39,6 +42,10 @@ typedef struct {
     cv_t *cv, *cv_old, *dq, *res, *fc, *face_fc[3];
     pv_t *pv;
     double *lamc[3], *lamv[3], *dt, *mut;
+    /* Navier-Stokes: vorticity magnitude per cell, face gradients (12 per face:
+     * d/dx of u v w T, then d/dy, then d/dz), face and cell viscous fluxes */
+    double *omega, *grad[3];
+    cv_t *fv, *face_fv[3];
     cv_t *dcv[6];                /* IS IP JS JP KS KP Dummy slabs */
     pv_t *dpv[6];
     int flag_low_dt;
@@ -56,6 +63,9 @@ struct orc_ctx {
     int nb;
     blk_t blk[64];
 };
+
+static void curl(blk_t *b);
+static void gradients(blk_t *b);
 
 /* ------------------------------------------------------------------ index */
 static inline long C3(const blk_t *b, int i, int j, int k) { return ((long)i * (b->JM + 1) + j) * (b->KM + 1) + k; }
@@ -161,8 +171,10 @@ static void blk_free(blk_t *b)
 {
     free(b->node); free(b->fi); free(b->fj); free(b->fk); free(b->cell);
     free(b->cv); free(b->cv_old); free(b->dq); free(b->res); free(b->fc); free(b->pv);
-    for (int d = 0; d < 3; ++d) { free(b->face_fc[d]); free(b->lamc[d]); free(b->lamv[d]); }
-    free(b->dt); free(b->mut);
+    for (int d = 0; d < 3; ++d) {
+        free(b->face_fc[d]); free(b->lamc[d]); free(b->lamv[d]); free(b->grad[d]); free(b->face_fv[d]);
+    }
+    free(b->dt); free(b->mut); free(b->omega); free(b->fv);
     for (int d = 0; d < 6; ++d) { free(b->dcv[d]); free(b->dpv[d]); }
     free(b->patch);
 }
@@ -206,9 +218,13 @@ int orc_block_add(orc_ctx *c, int IM, int JM, int KM, const double *nodes, int n
         b->face_fc[d] = (cv_t *)calloc(box, sizeof(cv_t));
         b->lamc[d] = (double *)calloc(box, sizeof(double));
         b->lamv[d] = (double *)calloc(box, sizeof(double));
+        b->grad[d] = (double *)calloc(box * 12, sizeof(double));
+        b->face_fv[d] = (cv_t *)calloc(box, sizeof(cv_t));
     }
     b->dt = (double *)calloc(box, sizeof(double));
     b->mut = (double *)calloc(box, sizeof(double));
+    b->omega = (double *)calloc(box, sizeof(double));
+    b->fv = (cv_t *)calloc(box, sizeof(cv_t));
     for (int d = 0; d < 6; ++d) {
         b->dcv[d] = (cv_t *)calloc(dsize(b, d), sizeof(cv_t));
         b->dpv[d] = (pv_t *)calloc(dsize(b, d), sizeof(pv_t));
@@ -296,6 +312,90 @@ static void face_vector_volume(blk_t *b)
 #undef ND3
 }
 
+/* The nine coefficients from the three coordinate derivatives along kesi,
+ * yita, zita (source/GeometryVariables.cpp:191-210, repeated at 252-267,
+ * 310-326, 367-387).  The reference's gradz_kesi repeats the gradx_kesi
+ * expression; that is kept. */
+static void metric9(v3_t e1, v3_t e2, v3_t e3, double g[9])
+{
+    const double lim_zero = F32(1E-20);
+    double partA = (e1.x * (e2.y * e3.z - e3.y * e2.z));
+    double partB = (e2.x * (e1.y * e3.z - e3.y * e1.z));
+    double partC = (e3.x * (e1.y * e2.z - e2.y * e1.z));
+    double J1 = (partA - partB + partC);
+    if (fabs(J1) <= lim_zero) J1 = lim_zero;
+    double J = F32(1.0) / J1;
+    g[0] = J * (e2.y * e3.z - e3.y * e2.z);
+    g[1] = J * (e2.z * e3.x - e3.z * e2.x);
+    g[2] = J * (e2.y * e3.z - e3.y * e2.z);
+    g[3] = J * (e3.y * e1.z - e1.y * e3.z);
+    g[4] = J * (e3.z * e1.x - e1.z * e3.x);
+    g[5] = J * (e3.x * e1.y - e1.x * e3.y);
+    g[6] = J * (e1.y * e2.z - e2.y * e1.z);
+    g[7] = J * (e1.z * e2.x - e2.z * e1.x);
+    g[8] = J * (e1.x * e2.y - e2.x * e1.y);
+}
+
+/* one-sided at the first and last index, central between */
+static v3_t diff3(v3_t lo, v3_t mid, v3_t hi, int first, int last)
+{
+    if (first) return vsub(hi, mid);
+    if (last) return vsub(mid, lo);
+    return vscale(vsub(hi, lo), F32(0.5));
+}
+
+/* 0.5f*(a + b) - 0.5f*(c + d) */
+static v3_t half_pair_diff(v3_t a, v3_t b, v3_t c, v3_t d)
+{
+    return vsub(vscale(vadd(a, b), F32(0.5)), vscale(vadd(c, d), F32(0.5)));
+}
+
+/* source/GeometryVariables.cpp:108-391 */
+static void jacobi_convert_coeff(blk_t *b)
+{
+    int IM = b->IM, JM = b->JM, KM = b->KM;
+    v3_t *N = b->node;
+#define ND3(i, j, k) N[C3(b, i, j, k)]
+#define CC(i, j, k) b->cell[C3(b, i, j, k)].c
+    for (int i = 1; i <= IM - 1; ++i)
+        for (int j = 1; j <= JM - 1; ++j)
+            for (int k = 1; k <= KM - 1; ++k) {
+                v3_t e1 = diff3(CC(i - 1, j, k), CC(i, j, k), CC(i + 1, j, k), i == 1, i == IM - 1);
+                v3_t e2 = diff3(CC(i, j - 1, k), CC(i, j, k), CC(i, j + 1, k), j == 1, j == JM - 1);
+                v3_t e3 = diff3(CC(i, j, k - 1), CC(i, j, k), CC(i, j, k + 1), k == 1, k == KM - 1);
+                metric9(e1, e2, e3, b->cell[C3(b, i, j, k)].g);
+            }
+#define FC(F, i, j, k) b->F[C3(b, i, j, k)].c
+    for (int i = 1; i <= IM; ++i)
+        for (int j = 1; j <= JM - 1; ++j)
+            for (int k = 1; k <= KM - 1; ++k) {
+                v3_t e1 = diff3(FC(fi, i - 1, j, k), FC(fi, i, j, k), FC(fi, i < IM ? i + 1 : i, j, k), i == 1, i == IM);
+                v3_t e2 = half_pair_diff(ND3(i, j + 1, k), ND3(i, j + 1, k + 1), ND3(i, j, k), ND3(i, j, k + 1));
+                /* the reference averages Node[i][j][k] with itself here (line 247) */
+                v3_t e3 = half_pair_diff(ND3(i, j, k + 1), ND3(i, j + 1, k + 1), ND3(i, j, k), ND3(i, j, k));
+                metric9(e1, e2, e3, b->fi[C3(b, i, j, k)].g);
+            }
+    for (int i = 1; i <= IM - 1; ++i)
+        for (int j = 1; j <= JM; ++j)
+            for (int k = 1; k <= KM - 1; ++k) {
+                v3_t e1 = half_pair_diff(ND3(i + 1, j, k), ND3(i + 1, j, k + 1), ND3(i, j, k), ND3(i, j, k + 1));
+                v3_t e2 = diff3(FC(fj, i, j - 1, k), FC(fj, i, j, k), FC(fj, i, j < JM ? j + 1 : j, k), j == 1, j == JM);
+                v3_t e3 = half_pair_diff(ND3(i, j, k + 1), ND3(i + 1, j, k + 1), ND3(i, j, k), ND3(i + 1, j, k));
+                metric9(e1, e2, e3, b->fj[C3(b, i, j, k)].g);
+            }
+    for (int i = 1; i <= IM - 1; ++i)
+        for (int j = 1; j <= JM - 1; ++j)
+            for (int k = 1; k <= KM; ++k) {
+                v3_t e1 = half_pair_diff(ND3(i + 1, j + 1, k), ND3(i + 1, j, k), ND3(i, j, k), ND3(i, j + 1, k));
+                v3_t e2 = half_pair_diff(ND3(i + 1, j + 1, k), ND3(i, j + 1, k), ND3(i + 1, j, k), ND3(i, j, k));
+                v3_t e3 = diff3(FC(fk, i, j, k - 1), FC(fk, i, j, k), FC(fk, i, j, k < KM ? k + 1 : k), k == 1, k == KM);
+                metric9(e1, e2, e3, b->fk[C3(b, i, j, k)].g);
+            }
+#undef FC
+#undef CC
+#undef ND3
+}
+
 int orc_geometry(orc_ctx *c)
 {
     c->total_vol = 0.0;
@@ -303,6 +403,7 @@ int orc_geometry(orc_ctx *c)
         face_vector_volume(&c->blk[n]);
         c->total_vol = c->total_vol + c->blk[n].cellvol;   /* source/ComputeAllBlockVars.cpp:25-36 */
     }
+    for (int n = 0; n < c->nb; ++n) jacobi_convert_coeff(&c->blk[n]);
     return 0;
 }
 
@@ -585,13 +686,181 @@ int orc_spectral(orc_ctx *c)
                     }
                 }
     }
+    /* source/ComputeAllBlockVars.cpp:56-63 */
+    if (c->cfg.viscous)
+        for (int n = 0; n < c->nb; ++n) {
+            curl(&c->blk[n]);
+            gradients(&c->blk[n]);
+        }
     return 0;
+}
+
+/* ------------------------------------------------- viscous quantities */
+/* CFluidProps::SutherLand (include/FluidProps.h:107-112) */
+static double sutherland(const orc_nondim *nd, double temp)
+{
+    return F32(1.0) / nd->re_ref * pow(temp, F32(3.0) / F32(2.0)) * (double)(1.0f + 110.4f / 288.15f) /
+           (temp + (double)(110.4f / 288.15f));
+}
+
+static inline void uvwt(const blk_t *b, int i, int j, int k, double o[4])
+{
+    const pv_t *w = &b->pv[C3(b, i, j, k)];
+    o[0] = w->u; o[1] = w->v; o[2] = w->w; o[3] = w->t;
+}
+
+/* Difference of u v w T along tangential direction t at the face of
+ * direction D whose higher-index cell is x (source/Spectral_curl_grad.cpp:
+ * 85-493).  P averages the cells x, x+t, x+t-D, x-D and M the cells x, x-f,
+ * x-f-s, x-s, where f and s are the lower and the higher of the directions D
+ * and t, summed in that order.  A cell outside the block along both D and t
+ * (a ghost-box corner, never filled) is left out and the remaining three are
+ * averaged.  At the J face j == JM, k == 1 the reference scales M's
+ * three-cell sum by 0.25 as well (line 331); that is kept. */
+static void tangential_diff(const blk_t *b, int D, int t, const int x[3], double out[4])
+{
+    const int N[3] = {b->IM, b->JM, b->KM};
+    const int f = D < t ? D : t, s = D < t ? t : D;
+    int off[2][4][3];
+    memset(off, 0, sizeof off);
+    off[0][1][t] = 1;
+    off[0][2][t] = 1; off[0][2][D] = -1;
+    off[0][3][D] = -1;
+    off[1][1][f] = -1;
+    off[1][2][f] = -1; off[1][2][s] = -1;
+    off[1][3][s] = -1;
+    double avg[2][4];
+    for (int h = 0; h < 2; ++h) {
+        double sum[4] = {0, 0, 0, 0};
+        int n = 0;
+        for (int m = 0; m < 4; ++m) {
+            int y[3] = {x[0] + off[h][m][0], x[1] + off[h][m][1], x[2] + off[h][m][2]};
+            int outD = y[D] < 1 || y[D] > N[D] - 1, outT = y[t] < 1 || y[t] > N[t] - 1;
+            if (outD && outT) continue;
+            double v[4];
+            uvwt(b, y[0], y[1], y[2], v);
+            for (int q = 0; q < 4; ++q) sum[q] = n == 0 ? v[q] : sum[q] + v[q];
+            ++n;
+        }
+        int quirk = h == 1 && D == 1 && t == 2 && x[1] == b->JM && x[2] == 1;
+        for (int q = 0; q < 4; ++q)
+            avg[h][q] = n == 4 ? sum[q] / F32(4.0) : quirk ? F32(0.25) * sum[q] / F32(3.0) : sum[q] / F32(3.0);
+    }
+    for (int q = 0; q < 4; ++q) out[q] = avg[0][q] - avg[1][q];
+}
+
+/* CFluidProps::Gradient (source/Spectral_curl_grad.cpp:85-493): gradients of
+ * u v w T at the faces from index-space differences and the face coefficients */
+static void gradients(blk_t *b)
+{
+    int IM = b->IM, JM = b->JM, KM = b->KM;
+    for (int D = 0; D < 3; ++D) {
+        const face_t *F = D == 0 ? b->fi : D == 1 ? b->fj : b->fk;
+        for (int i = 1; i <= IM - (D != 0); ++i)
+            for (int j = 1; j <= JM - (D != 1); ++j)
+                for (int k = 1; k <= KM - (D != 2); ++k) {
+                    int x[3] = {i, j, k};
+                    double dv[3][4];
+                    for (int t = 0; t < 3; ++t) {
+                        if (t == D) {
+                            double a[4], m[4];
+                            uvwt(b, i, j, k, a);
+                            uvwt(b, i - (D == 0), j - (D == 1), k - (D == 2), m);
+                            for (int q = 0; q < 4; ++q) dv[t][q] = a[q] - m[q];
+                        } else {
+                            tangential_diff(b, D, t, x, dv[t]);
+                        }
+                    }
+                    long xx = C3(b, i, j, k);
+                    const double *g = F[xx].g;
+                    double *o = b->grad[D] + xx * 12;
+                    for (int c = 0; c < 3; ++c)
+                        for (int q = 0; q < 4; ++q)
+                            o[c * 4 + q] = dv[0][q] * g[c] + dv[1][q] * g[3 + c] + dv[2][q] * g[6 + c];
+                }
+    }
+}
+
+/* CFluidProps::Curl (source/Spectral_curl_grad.cpp:44-83) */
+static void curl(blk_t *b)
+{
+    for (int k = 1; k <= b->KM - 1; ++k)
+        for (int j = 1; j <= b->JM - 1; ++j)
+            for (int i = 1; i <= b->IM - 1; ++i) {
+                double d[3][4], p[4], m[4];
+                for (int t = 0; t < 3; ++t) {
+                    uvwt(b, i + (t == 0), j + (t == 1), k + (t == 2), p);
+                    uvwt(b, i - (t == 0), j - (t == 1), k - (t == 2), m);
+                    for (int q = 0; q < 3; ++q) d[t][q] = F32(0.5) * (p[q] - m[q]);
+                }
+                long x = C3(b, i, j, k);
+                const double *g = b->cell[x].g;
+                double v_x = d[0][1] * g[0] + d[1][1] * g[3] + d[2][1] * g[6];
+                double w_x = d[0][2] * g[0] + d[1][2] * g[3] + d[2][2] * g[6];
+                double u_y = d[0][0] * g[1] + d[1][0] * g[4] + d[2][0] * g[7];
+                double w_y = d[0][2] * g[1] + d[1][2] * g[4] + d[2][2] * g[7];
+                double u_z = d[0][0] * g[2] + d[1][0] * g[5] + d[2][0] * g[8];
+                double v_z = d[0][1] * g[2] + d[1][1] * g[5] + d[2][1] * g[8];
+                b->omega[x] = sqrt(F32(0.5) * ((w_y - v_z) * (w_y - v_z) + (u_z - w_x) * (u_z - w_x) +
+                                               (v_x - u_y) * (v_x - u_y)));
+            }
+}
+
+/* CSpaceDiscr::ViscousFlux (source/ViscousFlux.cpp:9-283); without a
+ * turbulence model mue_tur = cond_tur = 0 (lines 68-72) */
+static void viscous_flux(orc_ctx *c, blk_t *b)
+{
+    int IM = b->IM, JM = b->JM, KM = b->KM;
+    double Gamma = c->nd.gamma, Rgas = c->nd.rgas, Pr_lam = c->nd.pr_lam, Pr_tur = c->nd.pr_tur;
+    const double c43 = (double)(4.0f / 3.0f), c23 = (double)(2.0f / 3.0f);
+    for (int D = 0; D < 3; ++D) {
+        const face_t *F = D == 0 ? b->fi : D == 1 ? b->fj : b->fk;
+        for (int i = 1; i <= IM - (D != 0); ++i)
+            for (int j = 1; j <= JM - (D != 1); ++j)
+                for (int k = 1; k <= KM - (D != 2); ++k) {
+                    long x = C3(b, i, j, k), xm = C3(b, i - (D == 0), j - (D == 1), k - (D == 2));
+                    const pv_t *a = &b->pv[x], *m = &b->pv[xm];
+                    double uf = F32(0.5) * (a->u + m->u), vf = F32(0.5) * (a->v + m->v);
+                    double wf = F32(0.5) * (a->w + m->w), tf = F32(0.5) * (a->t + m->t);
+                    double mue_lam = sutherland(&c->nd, tf);
+                    double cond_lam = mue_lam * Gamma * Rgas / (Gamma - F32(1.0)) / Pr_lam;
+                    double mue_tur = 0.0, cond_tur = 0.0;
+                    if (c->cfg.turbulence) {
+                        mue_tur = (b->mut[xm] + b->mut[x]) / F32(2.0);
+                        cond_tur = mue_tur * Gamma * Rgas / (Gamma - F32(1.0)) / Pr_tur;
+                    }
+                    double mue = mue_lam + mue_tur, cond = cond_lam + cond_tur;
+                    const double *gx = b->grad[D] + x * 12, *gy = gx + 4, *gz = gx + 8;
+                    double txx = mue * (c43 * gx[0] - c23 * (gy[1] + gz[2]));
+                    double txy = mue * (gy[0] + gx[1]);
+                    double txz = mue * (gz[0] + gx[2]);
+                    double tyy = mue * (c43 * gy[1] - c23 * (gx[0] + gz[2]));
+                    double tyz = mue * (gz[1] + gy[2]);
+                    double tzz = mue * (c43 * gz[2] - c23 * (gx[0] + gy[1]));
+                    v3_t F1 = {0.0, 0.0, 0.0}, F2 = {txx, txy, txz}, F3 = {txy, tyy, tyz}, F4 = {txz, tyz, tzz};
+                    v3_t F5 = {uf * txx + vf * txy + wf * txz + cond * gx[3], uf * txy + vf * tyy + wf * tyz + cond * gy[3],
+                               uf * txz + vf * tyz + wf * tzz + cond * gz[3]};
+                    v3_t S = F[x].S;
+                    cv_t fv = {vdot(F1, S), vdot(F2, S), vdot(F3, S), vdot(F4, S), vdot(F5, S)};
+                    b->face_fv[D][x] = fv;
+                }
+    }
+    for (int i = 1; i <= IM - 1; ++i)
+        for (int j = 1; j <= JM - 1; ++j)
+            for (int k = 1; k <= KM - 1; ++k) {
+                long x = C3(b, i, j, k);
+                cv_t s = csub(b->face_fv[0][C3(b, i + 1, j, k)], b->face_fv[0][x]);
+                s = cadd(s, b->face_fv[1][C3(b, i, j + 1, k)]); s = csub(s, b->face_fv[1][x]);
+                s = cadd(s, b->face_fv[2][C3(b, i, j, k + 1)]); s = csub(s, b->face_fv[2][x]);
+                b->fv[x] = s;
+            }
 }
 
 /* ------------------------------------------------------------ time step */
 /* source/TimeStep.cpp:7-92 */
 int orc_timestep(orc_ctx *c)
 {
+    double Gamma = c->nd.gamma, Pr_lam = c->nd.pr_lam, Pr_tur = c->nd.pr_tur;
     for (int n = 0; n < c->nb; ++n) {
         blk_t *b = &c->blk[n];
         double dt_factor = b->flag_low_dt == 0 ? F32(1.0) : (double)(1.0f / (1.0f + b->flag_low_dt));
@@ -601,6 +870,19 @@ int orc_timestep(orc_ctx *c)
                     long x = C3(b, i, j, k);
                     double lc = b->lamc[0][x] + b->lamc[1][x] + b->lamc[2][x];
                     double lv = 0.0;
+                    if (c->cfg.viscous) {
+                        const face_t *fa[3][2] = {{&b->fi[x], &b->fi[C3(b, i + 1, j, k)]},
+                                                  {&b->fj[x], &b->fj[C3(b, i, j + 1, k)]},
+                                                  {&b->fk[x], &b->fk[C3(b, i, j, k + 1)]}};
+                        double mue_lam = sutherland(&c->nd, b->pv[x].t), mue_tur = b->mut[x], dens = b->pv[x].d;
+                        double a = (double)(4.0f / 3.0f) / dens, g = Gamma / dens;
+                        for (int d = 0; d < 3; ++d) {
+                            v3_t S = vscale(vadd(fa[d][0]->S, fa[d][1]->S), 0.5);
+                            double eq = S.x * S.x + S.y * S.y + S.z * S.z;
+                            b->lamv[d][x] = (a > g ? a : g) * (mue_lam / Pr_lam + mue_tur / Pr_tur) * eq / b->cell[x].vol;
+                        }
+                        lv = b->lamv[0][x] + b->lamv[1][x] + b->lamv[2][x];
+                    }
                     b->dt[x] = dt_factor * c->cfl * b->cell[x].vol / (lc + (double)(float)c->cfg.c_timestep * lv);
                 }
         if (c->cfg.global_dt) {
@@ -961,13 +1243,16 @@ int orc_convective(orc_ctx *c)
     return 0;
 }
 
-/* source/ComputeAllBlockVars.cpp:223-255 (Euler path: Fv = Fd = 0) */
+/* source/ComputeAllBlockVars.cpp:223-255; Fd = 0 for the upwind schemes and
+ * Fv = 0 for Euler (131-160) */
 int orc_resflux(orc_ctx *c)
 {
     orc_bc(c);
     orc_spectral(c);
     orc_timestep(c);
     orc_convective(c);
+    if (c->cfg.viscous)
+        for (int n = 0; n < c->nb; ++n) viscous_flux(c, &c->blk[n]);
     const cv_t zero = {0, 0, 0, 0, 0};
     for (int n = 0; n < c->nb; ++n) {
         blk_t *b = &c->blk[n];
@@ -975,7 +1260,7 @@ int orc_resflux(orc_ctx *c)
             for (int j = 1; j <= b->JM - 1; ++j)
                 for (int k = 1; k <= b->KM - 1; ++k) {
                     long x = C3(b, i, j, k);
-                    b->res[x] = csub(csub(b->fc[x], zero), zero);
+                    b->res[x] = csub(csub(b->fc[x], c->cfg.viscous ? b->fv[x] : zero), zero);
                 }
     }
     return 0;
@@ -1307,15 +1592,50 @@ long orc_get(orc_ctx *c, int block, const char *name, double *out)
     else if (!strcmp(name, "lamcI")) s1 = b->lamc[0];
     else if (!strcmp(name, "lamcJ")) s1 = b->lamc[1];
     else if (!strcmp(name, "lamcK")) s1 = b->lamc[2];
-    else if (!strcmp(name, "fcI") || !strcmp(name, "fcJ") || !strcmp(name, "fcK")) {
+    else if (!strcmp(name, "lamvI")) s1 = b->lamv[0];
+    else if (!strcmp(name, "lamvJ")) s1 = b->lamv[1];
+    else if (!strcmp(name, "lamvK")) s1 = b->lamv[2];
+    else if (!strcmp(name, "omega")) s1 = b->omega;
+    else if (!strcmp(name, "fv")) c5 = b->fv;
+    else if (!strcmp(name, "mut")) {
+        for (int i = 0; i <= IM; ++i)
+            for (int j = 0; j <= JM; ++j)
+                for (int k = 0; k <= KM; ++k) out[n++] = b->mut[C3(b, i, j, k)];
+        return n;
+    }
+    else if (!strcmp(name, "fcI") || !strcmp(name, "fcJ") || !strcmp(name, "fcK") || !strcmp(name, "fvI") ||
+             !strcmp(name, "fvJ") || !strcmp(name, "fvK")) {
         int d = name[2] - 'I';
+        const cv_t *src = name[1] == 'c' ? b->face_fc[d] : b->face_fv[d];
         for (int i = 1; i <= IM - (d != 0); ++i)
             for (int j = 1; j <= JM - (d != 1); ++j)
                 for (int k = 1; k <= KM - (d != 2); ++k) {
-                    const cv_t *q = &b->face_fc[d][C3(b, i, j, k)];
+                    const cv_t *q = &src[C3(b, i, j, k)];
                     double t[5] = {q->r, q->ru, q->rv, q->rw, q->re};
                     memcpy(out + n, t, sizeof t);
                     n += 5;
+                }
+        return n;
+    }
+    else if (!strcmp(name, "gradI") || !strcmp(name, "gradJ") || !strcmp(name, "gradK")) {
+        int d = name[4] - 'I';
+        for (int i = 1; i <= IM - (d != 0); ++i)
+            for (int j = 1; j <= JM - (d != 1); ++j)
+                for (int k = 1; k <= KM - (d != 2); ++k) {
+                    memcpy(out + n, b->grad[d] + C3(b, i, j, k) * 12, 12 * sizeof(double));
+                    n += 12;
+                }
+        return n;
+    }
+    else if (!strcmp(name, "metI") || !strcmp(name, "metJ") || !strcmp(name, "metK") || !strcmp(name, "metC")) {
+        int d = name[3] == 'C' ? 3 : name[3] - 'I';
+        const face_t *F = d == 0 ? b->fi : d == 1 ? b->fj : b->fk;
+        for (int i = 1; i <= IM - (d != 0); ++i)
+            for (int j = 1; j <= JM - (d != 1); ++j)
+                for (int k = 1; k <= KM - (d != 2); ++k) {
+                    long x = C3(b, i, j, k);
+                    memcpy(out + n, d == 3 ? b->cell[x].g : F[x].g, 9 * sizeof(double));
+                    n += 9;
                 }
         return n;
     }
@@ -1378,6 +1698,27 @@ int orc_test_flux(int scheme, double gamma, const double *ql, const double *qr, 
     out[0] = F.r; out[1] = F.ru; out[2] = F.rv; out[3] = F.rw; out[4] = F.re;
     return 0;
 }
+
+/* Sets the whole primitive box (ghost cells included, (IM+1)(JM+1)(KM+1)x6,
+ * i slowest) of a block, then evaluates the viscous stages on it without
+ * applying boundary conditions: curl, face gradients and viscous fluxes. */
+int orc_test_viscous(orc_ctx *c, int block, const double *pv_box)
+{
+    if (block < 1 || block > c->nb) return -1;
+    blk_t *b = &c->blk[block - 1];
+    long box = (long)(b->IM + 1) * (b->JM + 1) * (b->KM + 1);
+    for (long x = 0; x < box; ++x) {
+        const double *p = pv_box + x * 6;
+        pv_t w = {p[0], p[1], p[2], p[3], p[4], p[5]};
+        b->pv[x] = w;
+    }
+    curl(b);
+    gradients(b);
+    viscous_flux(c, b);
+    return 0;
+}
+
+double orc_test_sutherland(orc_ctx *c, double temp) { return sutherland(&c->nd, temp); }
 
 int orc_test_recon(int var, int scheme, double gamma, const double *ff, const double *f, const double *b,
                    const double *bb, double *l, double *r)

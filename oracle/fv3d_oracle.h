@@ -7,7 +7,9 @@
  * It restates, in plain C with the reference's array extents and expression
  * order, the functions of the reference's LU-SGS iteration: ghost fill, spectral
  * radii, local time step, MUSCL/NND reconstruction with the Roe and
- * Steger-Warming fluxes, wall flux, residual assembly, LU-SGS sweeps, update,
+ * Steger-Warming fluxes, wall flux, the laminar Navier-Stokes terms (metric
+ * coefficients, face gradients, curl, Sutherland viscosity, viscous flux and
+ * the viscous spectral radii), residual assembly, LU-SGS sweeps, update,
  * primitive recovery, variable limiting, residual norm and the section
  * performance integrals.  Each function cites the reference file:line it
  * follows.  Indices are the reference's: cells 1..IM-1, ghost cells 0 and IM,
@@ -81,10 +83,13 @@ double orc_cfl(orc_ctx *c);
  * hist receives n rows of: iter res_dens mass_in mass_out PR eff CFL. */
 int orc_iterate(orc_ctx *c, int n, double *hist);
 
-/* Copies a field out.  Names: cv pv (full (IM+1)(JM+1)(KM+1)xC incl. ghost
- * cells), res dq dt lamcI lamcJ lamcK (interior (ni,nj,nk)xC), dIS_cv dIP_cv
- * dJS_cv dJP_cv dKS_cv dKP_cv and the _pv variants (layers 1..NDummy), vol,
- * fcI fcJ fcK (face fluxes), SI SJ SK (face vectors). */
+/* Copies a field out.  Names: cv pv mut (full (IM+1)(JM+1)(KM+1)xC incl.
+ * ghost cells), res dq dt lamcI lamcJ lamcK lamvI lamvJ lamvK omega fv
+ * (interior (ni,nj,nk)xC), dIS_cv dIP_cv dJS_cv dJP_cv dKS_cv dKP_cv and the
+ * _pv variants (layers 1..NDummy), vol, fcI fcJ fcK (convective face fluxes),
+ * fvI fvJ fvK (viscous face fluxes), gradI gradJ gradK (face gradients: d/dx,
+ * d/dy, d/dz of u v w T), metI metJ metK metC (face and cell transformation
+ * coefficients, 9 each), SI SJ SK (face vectors). */
 long orc_get(orc_ctx *c, int block, const char *name, double *out);
 
 /* Test hooks: the face flux (scheme 200 Steger-Warming, 300 Roe) on face-frame
@@ -94,6 +99,12 @@ long orc_get(orc_ctx *c, int block, const char *name, double *out);
 int orc_test_flux(int scheme, double gamma, const double *ql, const double *qr, double que, double *out);
 int orc_test_recon(int var, int scheme, double gamma, const double *ff, const double *f, const double *b,
                    const double *bb, double *l, double *r);
+
+/* Viscous stages on a given primitive box (ghost cells included), without
+ * boundary conditions; then orc_get "omega" "gradI/J/K" "fvI/J/K" "fv".  And
+ * the Sutherland viscosity of a non-dimensional temperature. */
+int orc_test_viscous(orc_ctx *c, int block, const double *pv_box);
+double orc_test_sutherland(orc_ctx *c, double temp);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,9 @@ def lib():
         L.orc_get.restype = C.c_long
         L.orc_test_flux.argtypes = [C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
         L.orc_test_recon.argtypes = [C.c_int, C.c_int, C.c_double] + [C.c_void_p] * 6
+        L.orc_test_viscous.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_test_sutherland.argtypes = [C.c_void_p, C.c_double]
+        L.orc_test_sutherland.restype = C.c_double
         _lib = L
     return _lib
 
@@ -127,19 +130,34 @@ class Oracle:
         lib().orc_perf(self.h, r.ctypes.data)
         return r
 
+    def viscous_on(self, block: int, pv_box: np.ndarray) -> None:
+        """Curl, face gradients and viscous fluxes of a given primitive box
+        (ghost cells included, shape (IM+1, JM+1, KM+1, 6)), no boundary conditions."""
+        a = np.ascontiguousarray(pv_box, dtype=np.float64)
+        lib().orc_test_viscous(self.h, block, a.ctypes.data)
+
+    def sutherland(self, temp: float) -> float:
+        return lib().orc_test_sutherland(self.h, temp)
+
     def get(self, block: int, name: str) -> np.ndarray:
         bi = self.case.blocks[block - 1]
         im, jm, km = bi.im, bi.jm, bi.km
         ni, nj, nk = im - 1, jm - 1, km - 1
         if name in ("cv", "pv"):
             shape = (im + 1, jm + 1, km + 1, 5 if name == "cv" else 6)
-        elif name in ("res", "dq", "fc"):
+        elif name in ("res", "dq", "fc", "fv"):
             shape = (ni, nj, nk, 5)
-        elif name in ("fcI", "fcJ", "fcK"):
-            shape = {"fcI": (im, nj, nk, 5), "fcJ": (ni, jm, nk, 5), "fcK": (ni, nj, km, 5)}[name]
+        elif name in ("fcI", "fcJ", "fcK", "fvI", "fvJ", "fvK"):
+            shape = {"I": (im, nj, nk, 5), "J": (ni, jm, nk, 5), "K": (ni, nj, km, 5)}[name[2]]
+        elif name in ("gradI", "gradJ", "gradK"):
+            shape = {"I": (im, nj, nk, 12), "J": (ni, jm, nk, 12), "K": (ni, nj, km, 12)}[name[4]]
+        elif name in ("metI", "metJ", "metK", "metC"):
+            shape = {"I": (im, nj, nk, 9), "J": (ni, jm, nk, 9), "K": (ni, nj, km, 9), "C": (ni, nj, nk, 9)}[name[3]]
+        elif name == "mut":
+            shape = (im + 1, jm + 1, km + 1)
         elif name in ("SI", "SJ", "SK"):
             shape = {"SI": (im, nj, nk, 3), "SJ": (ni, jm, nk, 3), "SK": (ni, nj, km, 3)}[name]
-        elif name in ("dt", "lamcI", "lamcJ", "lamcK", "vol"):
+        elif name in ("dt", "lamcI", "lamcJ", "lamcK", "lamvI", "lamvJ", "lamvK", "omega", "vol"):
             shape = (ni, nj, nk)
         elif name.startswith("d") and name[1] in "IJK":
             c = 6 if name.endswith("_pv") else 5

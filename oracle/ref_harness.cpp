@@ -114,6 +114,24 @@ void dump_block(const std::string &dir, const std::string &tag, CBlocks &B)
                            [](const TVisVars &t, double *o) { o[0] = t.mue; });
         dump3<REAL, 1>(p + "_omega.npy", F.Omega, 1, IM - 1, 1, JM - 1, 1, KM - 1, r1);
         dump3<TConsVars, 5>(p + "_fv.npy", B.SpaceDiscr.Cell_Fv, 1, IM - 1, 1, JM - 1, 1, KM - 1, cv5);
+        dump3<TConsVars, 5>(p + "_fvI.npy", B.SpaceDiscr.FaceI_Fv, 1, IM, 1, JM - 1, 1, KM - 1, cv5);
+        dump3<TConsVars, 5>(p + "_fvJ.npy", B.SpaceDiscr.FaceJ_Fv, 1, IM - 1, 1, JM, 1, KM - 1, cv5);
+        dump3<TConsVars, 5>(p + "_fvK.npy", B.SpaceDiscr.FaceK_Fv, 1, IM - 1, 1, JM - 1, 1, KM, cv5);
+        // face gradients: d/dx, d/dy, d/dz of u v w T
+        TPrimVars ***gx[3] = {F.gradx_I, F.gradx_J, F.gradx_K}, ***gy[3] = {F.grady_I, F.grady_J, F.grady_K},
+                  ***gz[3] = {F.gradz_I, F.gradz_J, F.gradz_K};
+        for (int d = 0; d < 3; ++d) {
+            std::vector<double> v;
+            for (int i = 1; i <= IM - (d != 0); ++i)
+                for (int j = 1; j <= JM - (d != 1); ++j)
+                    for (int k = 1; k <= KM - (d != 2); ++k)
+                        for (TPrimVars ***g : {gx[d], gy[d], gz[d]}) {
+                            const TPrimVars &w = g[i][j][k];
+                            v.insert(v.end(), {w.uvel, w.vvel, w.wvel, w.temp});
+                        }
+            write_npy(p + "_grad" + std::string(1, (char)('I' + d)) + ".npy", v,
+                      {IM - (d != 0), JM - (d != 1), KM - (d != 2), 12});
+        }
     }
     if (CFluidProps::turbulenceType == TurbulenceTypes::SA) {
         dump3<TEddyVars, 1>(p + "_ev.npy", F.Cell_ev, 0, IM, 0, JM, 0, KM,

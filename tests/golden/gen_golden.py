@@ -10,7 +10,7 @@ The reference runs single-threaded: with more than one OpenMP thread its
 convective face loops share the ``flag`` variable across threads
 (source/Upwind_Convective.cpp:7,29,109,219,328) and produce wrong faces.
 
-Usage: python tests/golden/gen_golden.py [--c0]
+Usage: python tests/golden/gen_golden.py [--c0] [--only NAME ...]
 """
 from __future__ import annotations
 
@@ -39,7 +39,10 @@ TINY = {
     "tiny_roe_muscl3_cons": (dict(ni=6, nj=5, nk=10, seed=3, recon_var=100), 20),
     "tiny_roe_muscl2o_prim": (dict(ni=6, nj=5, nk=10, seed=4, recon_scheme=400), 10),
     "tiny_sw_muscl2u_cons": (dict(ni=6, nj=5, nk=10, seed=5, flux=200, recon_scheme=200, recon_var=100), 10),
+    # laminar Navier-Stokes at a Reynolds number of a few hundred per unit length
+    "tiny_ns_roe_muscl3_prim": (dict(ni=6, nj=5, nk=10, seed=6, viscous=True, len_ref=2e-5), 20),
 }
+NS_FIELDS_IT1 = ["omega", "gradI", "gradJ", "gradK", "fvI", "fvJ", "fvK", "fv", "lamvI", "lamvJ", "lamvK"]
 
 FIELDS_IT1 = ["res", "dq", "dt", "lamcI", "lamcJ", "lamcK", "cv", "pv", "fcI", "fcJ", "fcK",
               "dIS_cv", "dIP_cv", "dJS_cv", "dJP_cv", "dKS_cv", "dKP_cv",
@@ -53,7 +56,7 @@ def gen_tiny(name, kw, n_iter):
         case.write(cdir)
         res = O.run_reference(cdir, n_iter, odir, threads=1, dump_first=True)
         arrs = {"history": res["history"]}
-        for f in FIELDS_IT1:
+        for f in FIELDS_IT1 + (NS_FIELDS_IT1 if kw.get("viscous") else []):
             arrs["it1_" + f] = np.load(os.path.join(odir, f"it1_b1_{f}.npy"))
         for f in ("cv", "pv", "res", "dq"):
             arrs["final_" + f] = np.load(os.path.join(odir, f"final_b1_{f}.npy"))
@@ -90,11 +93,13 @@ def gen_c0(n_iter=500):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--c0", action="store_true", help="also run the 500-iteration C0 case (minutes)")
+    ap.add_argument("--only", nargs="*", help="regenerate only these tiny fixtures")
     a = ap.parse_args()
     subprocess.run(["make", "-C", os.path.join(REPO, "oracle"), "ref", "restatement"], check=True,
                    stdout=subprocess.DEVNULL)
     for name, (kw, n) in TINY.items():
-        gen_tiny(name, kw, n)
+        if a.only is None or name in a.only:
+            gen_tiny(name, kw, n)
     if a.c0:
         gen_c0()
 

@@ -3,8 +3,8 @@
 The fixtures (tests/golden/*.npz) hold the reference's own outputs for seeded
 synthetic cases; the restatement must reproduce them bit for bit at one
 thread.  Cases cover Roe and Steger-Warming, MUSCL3 / NND / MUSCL2u / MUSCL2o
-reconstruction on primitive and conservative variables, and the inlet,
-general-outlet and wall ghost fills.
+reconstruction on primitive and conservative variables, the inlet,
+general-outlet and wall ghost fills, and the laminar Navier-Stokes terms.
 """
 import json
 import os
@@ -16,7 +16,8 @@ from fv3d import cases
 import oracle as O
 
 TINY = ["tiny_roe_muscl3_prim", "tiny_sw_nnd_prim", "tiny_roe_muscl3_cons", "tiny_roe_muscl2o_prim",
-        "tiny_sw_muscl2u_cons"]
+        "tiny_sw_muscl2u_cons", "tiny_ns_roe_muscl3_prim"]
+NS_FIELDS = ("omega", "gradI", "gradJ", "gradK", "fvI", "fvJ", "fvK", "fv", "lamvI", "lamvJ", "lamvK")
 
 
 def _load(golden_dir, name):
@@ -31,8 +32,11 @@ def test_first_iteration_bitexact(golden_dir, name):
     case = cases.bump_channel(max_iter=meta["iterations"], **meta["kwargs"])
     o = O.Oracle(case)
     o.call("lusgs")
-    for f in ("res", "dq", "dt", "lamcI", "lamcJ", "lamcK", "fcI", "fcJ", "fcK", "cv", "pv",
-              "dIS_cv", "dIP_cv", "dJS_cv", "dJP_cv", "dKS_cv", "dKP_cv", "dKS_pv", "dKP_pv", "dIS_pv"):
+    fields = ("res", "dq", "dt", "lamcI", "lamcJ", "lamcK", "fcI", "fcJ", "fcK", "cv", "pv",
+              "dIS_cv", "dIP_cv", "dJS_cv", "dJP_cv", "dKS_cv", "dKP_cv", "dKS_pv", "dKP_pv", "dIS_pv")
+    if meta["kwargs"].get("viscous"):
+        fields += NS_FIELDS
+    for f in fields:
         ref = z["it1_" + f]
         got = o.get(1, f).reshape(ref.shape)
         assert np.array_equal(got, ref), f"{name}: {f} differs, max {np.max(np.abs(got - ref))}"

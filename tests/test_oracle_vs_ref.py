@@ -4,8 +4,9 @@ Runs only where the reference harness has been built (make -C oracle ref, in
 a container that has the reference sources); elsewhere these tests skip and
 the committed fixtures (test_oracle_golden.py) carry the comparison.  The
 cases cover the paths the fixtures leave out: MUSCL2c reconstruction, the
-axial (radial-equilibrium) and centrifugal outlets, and a rotating block with
-rotating walls.  The reference runs on one thread (its face loops race on a
+axial (radial-equilibrium) and centrifugal outlets, a rotating block with
+rotating walls, and laminar Navier-Stokes with Steger-Warming/NND and with a
+rotating block.  The reference runs on one thread (its face loops race on a
 shared variable with more, source/Upwind_Convective.cpp:7,29).
 """
 import os
@@ -21,6 +22,9 @@ pytestmark = pytest.mark.skipif(not os.path.exists(O.REF_BIN), reason="reference
 
 FIELDS = ["res", "dq", "dt", "lamcI", "lamcJ", "lamcK", "cv", "pv", "fcI", "fcJ", "fcK",
           "dIS_cv", "dIP_cv", "dJS_cv", "dJP_cv", "dKS_cv", "dKP_cv"]
+# laminar Navier-Stokes: vorticity, face gradients, viscous face and cell
+# fluxes, viscous spectral radii
+NS_FIELDS = ["omega", "gradI", "gradJ", "gradK", "fvI", "fvJ", "fvK", "fv", "lamvI", "lamvJ", "lamvK"]
 
 
 def _shifted(case, dx=1.0, dy=0.3):
@@ -45,22 +49,35 @@ def variant(name):
         for p in c.patches:
             if p.kind == "wall" and p.face in (3, 1):
                 p.rotate = 1
+    elif name == "ns_sw_nnd_cons":
+        c = cases.bump_channel(ni=7, nj=6, nk=9, seed=24, flux=200, recon_scheme=100, recon_var=100, max_iter=12,
+                               viscous=True, len_ref=1e-4)
+    elif name == "ns_rotating_roe_muscl2o":
+        c = _shifted(cases.bump_channel(ni=6, nj=5, nk=10, seed=25, recon_scheme=400, max_iter=12, viscous=True,
+                                        len_ref=5e-5))
+        c.blocks[0].rpm = 3000.0
+        c.blocks[0].nblade = 22
+        for p in c.patches:
+            if p.kind == "wall" and p.face in (3, 1):
+                p.rotate = 1
     else:
         raise KeyError(name)
     return c
 
 
-@pytest.mark.parametrize("name", ["muscl2c_axial", "nnd_cons_sw_centrifugal", "rotating_roe_muscl3"])
+@pytest.mark.parametrize("name", ["muscl2c_axial", "nnd_cons_sw_centrifugal", "rotating_roe_muscl3",
+                                  "ns_sw_nnd_cons", "ns_rotating_roe_muscl2o"])
 def test_restatement_matches_reference(name):
     case = variant(name)
     n = case.cfg.max_iter
+    fields = FIELDS + (NS_FIELDS if case.cfg.equation == "N" else [])
     with tempfile.TemporaryDirectory() as td:
         cdir, odir = os.path.join(td, "case"), os.path.join(td, "out")
         case.write(cdir)
         ref = O.run_reference(cdir, n, odir, threads=1, dump_first=True, timeout=300)
         o = O.Oracle(case)
         o.call("lusgs")
-        for f in FIELDS:
+        for f in fields:
             want = np.load(os.path.join(odir, f"it1_b1_{f}.npy"))
             got = o.get(1, f).reshape(want.shape)
             assert np.array_equal(got, want), f"{name}: {f}, max diff {np.max(np.abs(got - want))}"
