@@ -65,6 +65,7 @@ struct fv3d_ctx {
     long hist_cap = 0;
     double *d_stage_row = nullptr;   // history row written by the captured graph
     double *d_faces[3] = {nullptr, nullptr, nullptr};
+    ViscDebug vdbg = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}, nullptr};
     int store_faces = 0;
     double total_vol = 0.0;
     // persistent wavefront sweep: tile lists (forward, backward) and the
@@ -139,6 +140,10 @@ static void nondimensionalize(fv3d_ctx *c)
     g.recon_var = f.recon_var;
     g.recon_scheme = f.recon_scheme;
     g.flux_scheme = f.flux_scheme;
+    g.viscous = f.viscous ? 1 : 0;
+    g.re_ref = c->re_ref;
+    g.pr_lam = (float)f.pr_lam;
+    g.pr_tur = (float)f.pr_tur;
 }
 
 // source/GeometryVariables.cpp:10-105
@@ -244,6 +249,17 @@ static int setup_block(fv3d_ctx *c, HostBlock &b, int index)
     if ((rc = alloc(&B.W, 6 * B.box))) return rc;
     if ((rc = alloc(&B.lamc, 3 * B.box))) return rc;
     if ((rc = alloc(&B.dt, B.box))) return rc;
+    if (c->cfg.viscous) {
+        // filled by k_metrics at the end of set-up
+        for (int d = 0; d < 3; ++d) {
+            if ((rc = alloc(&p, 9 * B.box))) return rc;
+            B.fg[d] = p;
+        }
+        if ((rc = alloc(&p, 9 * B.box))) return rc;
+        B.cg = p;
+        if ((rc = alloc(&B.lamv, 3 * B.box))) return rc;
+        if ((rc = alloc(&B.omega, B.box))) return rc;
+    }
     if ((rc = alloc(&B.alpha, B.box))) return rc;
     {
         // static sweep metrics of each cell: averaged unit normal of its two
@@ -340,7 +356,7 @@ int fv3d_ctx_create(const fv3d_config *cfg, fv3d_ctx **out)
     if (cfg->recon_var != 100 && cfg->recon_var != 200) { *out = c; return fail(c, FV3D_EINVAL, "recon_var %d unsupported", cfg->recon_var); }
     if (cfg->recon_scheme < 100 || cfg->recon_scheme > 500 || cfg->recon_scheme % 100) { *out = c; return fail(c, FV3D_EINVAL, "recon_scheme %d unsupported", cfg->recon_scheme); }
     if (cfg->flux_scheme != 200 && cfg->flux_scheme != 300) { *out = c; return fail(c, FV3D_EINVAL, "flux_scheme %d unsupported", cfg->flux_scheme); }
-    if (cfg->viscous || cfg->turbulence) { *out = c; return fail(c, FV3D_EINVAL, "viscous/turbulent path not in this build"); }
+    if (cfg->turbulence) { *out = c; return fail(c, FV3D_EINVAL, "turbulence model %d not in this build", cfg->turbulence); }
     if (cfg->global_dt) { *out = c; return fail(c, FV3D_EINVAL, "global time step not in this build"); }
     if (cfg->turbo_type == 0) { *out = c; return fail(c, FV3D_EINVAL, "axial (radial-equilibrium) outlet not in this build"); }
     if (cfg->precision != FV3D_PLAIN_FP64) { *out = c; return fail(c, FV3D_EINVAL, "precision mode %d not in this build", cfg->precision); }
@@ -434,6 +450,13 @@ int fv3d_setup(fv3d_ctx *c)
     if ((rc = dalloc(c, &c->d_blocks, nb))) return rc;
     HIPCK(c, hipMemcpyAsync(c->d_blocks, bd.data(), nb * sizeof(BlockDev), hipMemcpyHostToDevice, c->stream));
     for (int n = 0; n < nb; ++n) {
+        const BlockDev &B = c->blocks[n].dev;
+        if (c->cfg.viscous) {
+            // coordinate-transformation coefficients (source/ComputeAllBlockVars.cpp:34-35)
+            hipLaunchKernelGGL(k_metrics, dim3((unsigned)((4 * B.box + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
+                               n, (double *)B.fg[0], (double *)B.fg[1], (double *)B.fg[2], (double *)B.cg);
+            HIPCK(c, hipGetLastError());
+        }
         const long rows = c->blocks[n].sweep_rows;
         hipLaunchKernelGGL(k_sweep_fill, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, c->stream, c->d_blocks, n, rows,
                            kGatherState | kGatherMetrics);
@@ -605,6 +628,28 @@ static int launch_convective(fv3d_ctx *c)
     return FV3D_OK;
 }
 
+// Viscous flux subtracted from the residual (CMesh::ViscousFlux,
+// source/ComputeAllBlockVars.cpp:131-160, and Res = Fc - Fv - Fd, 245).
+static int launch_viscous(fv3d_ctx *c)
+{
+    if (!c->cfg.viscous) return FV3D_OK;
+    hipLaunchKernelGGL((k_viscous<kTI, kTJ, kTK>), dim3((unsigned)c->ntiles), dim3(kTI * kTJ * kTK), 0, c->stream,
+                       c->d_blocks, c->d_tiles, c->gas, c->store_faces, c->vdbg);
+    HIPCK(c, hipGetLastError());
+    return FV3D_OK;
+}
+
+// Vorticity magnitude (CFluidProps::Curl).  Only the turbulence models read it,
+// so the laminar iteration does not evaluate it; fv3d_curl does on request.
+static int launch_curl(fv3d_ctx *c)
+{
+    if (!c->cfg.viscous) return fail(c, FV3D_EINVAL, "the vorticity is evaluated for Navier-Stokes only");
+    hipLaunchKernelGGL(k_curl, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart,
+                       (int)c->blocks.size());
+    HIPCK(c, hipGetLastError());
+    return FV3D_OK;
+}
+
 static int launch_sweeps_kba(fv3d_ctx *c)
 {
     // layout: err at 0, tickets at 32 and 48, padded counters from 64
@@ -622,11 +667,19 @@ static int launch_sweeps_kba(fv3d_ctx *c)
             hipLaunchKernelGGL(k_sweep_pack, dim3((unsigned)hb.pack_groups), dim3(256), 0, c->stream, c->d_blocks, n);
         }
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[6], c->stream));
-        hipLaunchKernelGGL((k_lusgs_sl<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
-                           c->d_blocks, c->d_kba[0], c->nkba, tf, err, c->gas.relax, c->gas);
+        if (c->cfg.viscous)
+            hipLaunchKernelGGL((k_lusgs_sl<true, kKbaTJ, kKbaTK, true>), dim3((unsigned)c->kba_grid), dim3(256), 0,
+                               c->stream, c->d_blocks, c->d_kba[0], c->nkba, tf, err, c->gas.relax, c->gas);
+        else
+            hipLaunchKernelGGL((k_lusgs_sl<true, kKbaTJ, kKbaTK, false>), dim3((unsigned)c->kba_grid), dim3(256), 0,
+                               c->stream, c->d_blocks, c->d_kba[0], c->nkba, tf, err, c->gas.relax, c->gas);
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[7], c->stream));
-        hipLaunchKernelGGL((k_lusgs_sl<false, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
-                           c->d_blocks, c->d_kba[1], c->nkba, tb, err, c->gas.relax, c->gas);
+        if (c->cfg.viscous)
+            hipLaunchKernelGGL((k_lusgs_sl<false, kKbaTJ, kKbaTK, true>), dim3((unsigned)c->kba_grid), dim3(256), 0,
+                               c->stream, c->d_blocks, c->d_kba[1], c->nkba, tb, err, c->gas.relax, c->gas);
+        else
+            hipLaunchKernelGGL((k_lusgs_sl<false, kKbaTJ, kKbaTK, false>), dim3((unsigned)c->kba_grid), dim3(256), 0,
+                               c->stream, c->d_blocks, c->d_kba[1], c->nkba, tb, err, c->gas.relax, c->gas);
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[8], c->stream));
         for (int n = 0; n < nb; ++n)
             hipLaunchKernelGGL(k_sweep_unpack, dim3((unsigned)c->blocks[n].pack_groups), dim3(256), 0, c->stream, c->d_blocks,
@@ -701,6 +754,8 @@ int fv3d_bc_apply(fv3d_ctx *c) { READY(c); return launch_bc(c); }
 int fv3d_spectral(fv3d_ctx *c) { READY(c); return launch_spectral_dt(c); }
 int fv3d_timestep(fv3d_ctx *c) { READY(c); return FV3D_OK; }   // fused into fv3d_spectral
 int fv3d_convective(fv3d_ctx *c) { READY(c); return launch_convective(c); }
+int fv3d_viscous(fv3d_ctx *c) { READY(c); return launch_viscous(c); }
+int fv3d_curl(fv3d_ctx *c) { READY(c); return launch_curl(c); }
 
 int fv3d_resflux(fv3d_ctx *c)
 {
@@ -708,7 +763,8 @@ int fv3d_resflux(fv3d_ctx *c)
     int rc;
     if ((rc = launch_bc(c))) return rc;
     if ((rc = launch_spectral_dt(c))) return rc;
-    return launch_convective(c);
+    if ((rc = launch_convective(c))) return rc;
+    return launch_viscous(c);
 }
 
 // The wavefront sweep bounds every wait; a wait that ran out sets the error
@@ -733,6 +789,7 @@ static int lusgs_timed(fv3d_ctx *c)
     if ((rc = launch_spectral_dt(c))) return rc;
     if (c->timing) HIPCK(c, hipEventRecord(c->ev[1], c->stream));
     if ((rc = launch_convective(c))) return rc;
+    if ((rc = launch_viscous(c))) return rc;
     if (c->timing) HIPCK(c, hipEventRecord(c->ev[2], c->stream));
     if ((rc = launch_sweeps(c))) return rc;
     if (c->timing) HIPCK(c, hipEventRecord(c->ev[3], c->stream));
@@ -916,6 +973,10 @@ int fv3d_set_debug(fv3d_ctx *c, int flags)
     READY(c);
     c->store_faces = flags & 1;
     c->sweep_mode = (flags & 2) ? 0 : (flags & 4) ? 1 : (flags & 8) ? 2 : (flags & 16) ? 3 : 4;
+    if (c->cfg.viscous && c->sweep_mode >= 1 && c->sweep_mode <= 3) {
+        c->sweep_mode = 4;
+        return fail(c, FV3D_EINVAL, "the viscous terms run with the default sweep form or the per-hyperplane form");
+    }
     if ((flags >> 8) & 0xff) c->kba_chunk = (flags >> 8) & 0xff;
     c->kba_lag = (flags >> 16) & 0xff;
     if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
@@ -925,6 +986,14 @@ int fv3d_set_debug(fv3d_ctx *c, int flags)
         if (c->blocks.size() != 1) return fail(c, FV3D_EINVAL, "face-flux capture supports one block");
         for (int d = 0; d < 3; ++d) {
             int rc = dalloc(c, &c->d_faces[d], 5 * (size_t)mx);
+            if (rc) return rc;
+            if (c->cfg.viscous) {
+                if ((rc = dalloc(c, &c->vdbg.fv[d], 5 * (size_t)mx))) return rc;
+                if ((rc = dalloc(c, &c->vdbg.grad[d], 12 * (size_t)mx))) return rc;
+            }
+        }
+        if (c->cfg.viscous) {
+            int rc = dalloc(c, &c->vdbg.fvc, 5 * (size_t)mx);
             if (rc) return rc;
         }
     }
@@ -964,10 +1033,21 @@ long fv3d_field_download(fv3d_ctx *c, int block, const char *name, double *out)
     else if (nm == "lamcI") { src = B.lamc; C = 1; }
     else if (nm == "lamcJ") { src = B.lamc + B.box; C = 1; }
     else if (nm == "lamcK") { src = B.lamc + 2 * B.box; C = 1; }
+    else if (nm == "lamvI" || nm == "lamvJ" || nm == "lamvK" || nm == "omega") {
+        if (!B.lamv) return fail(c, FV3D_EINVAL, "%s exists for Navier-Stokes only", name);
+        src = nm == "omega" ? B.omega : B.lamv + (nm[4] - 'I') * B.box;
+        C = 1;
+    }
     else if (nm == "fcI" || nm == "fcJ" || nm == "fcK") {
         int d = nm[2] - 'I';
         if (!c->d_faces[d]) return fail(c, FV3D_ESTATE, "face fluxes were not captured (fv3d_set_debug)");
         src = c->d_faces[d]; C = 5; hiI = d == 0; hiJ = d == 1; hiK = d == 2;
+    }
+    else if (nm == "fvI" || nm == "fvJ" || nm == "fvK" || nm == "gradI" || nm == "gradJ" || nm == "gradK" || nm == "fv") {
+        int d = nm == "fv" ? -1 : nm[nm.size() - 1] - 'I';
+        const double *p = d < 0 ? c->vdbg.fvc : nm[0] == 'f' ? c->vdbg.fv[d] : c->vdbg.grad[d];
+        if (!p) return fail(c, FV3D_ESTATE, "viscous fluxes were not captured (Navier-Stokes and fv3d_set_debug)");
+        src = p; C = nm[0] == 'g' ? 12 : 5; hiI = d == 0; hiJ = d == 1; hiK = d == 2;
     }
     if (src) {
         if (fetch(src, (long)C * B.box, h)) return FV3D_EHIP;

@@ -46,6 +46,14 @@ struct BlockDev {
     const unsigned char *wmask[6];  // wall mask per side (tangential box)
     const double *wrot[6];          // wall rotation rate per side
     double *lamc;             // 3*box convective spectral radii
+    // laminar Navier-Stokes (null for Euler): coordinate-transformation
+    // coefficients of the faces of each direction and of the cells, 9*box each
+    // (component m at m*box: gradx/grady/gradz of kesi, then yita, then zita),
+    // viscous spectral radii 3*box, vorticity magnitude box
+    const double *fg[3];
+    const double *cg;
+    double *lamv;
+    double *omega;
     double *dt;               // local time step
     double *alpha;            // LU-SGS diagonal Vol/dt + relax*sum(lamc) + 2*sum(lamv)
     const double *swg;        // 15*box static sweep metrics per cell: for d = 0..2,
@@ -87,7 +95,16 @@ struct Gas {
     double inlet_dir[3];
     double c_timestep, relax;
     int recon_var, recon_scheme, flux_scheme;
+    int viscous;                  // laminar Navier-Stokes terms on
+    double re_ref, pr_lam, pr_tur;
 };
+
+// CFluidProps::SutherLand (include/FluidProps.h:107-112), non-dimensional
+__device__ inline double sutherland(const Gas &g, double temp)
+{
+    return F32C(1.0) / g.re_ref * pow(temp, F32C(3.0) / F32C(2.0)) * (double)(1.0f + 110.4f / 288.15f) /
+           (temp + (double)(110.4f / 288.15f));
+}
 
 // ---------------------------------------------------------------- minmod
 __device__ inline double mm1(double a, double b)

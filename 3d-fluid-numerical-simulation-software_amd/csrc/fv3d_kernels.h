@@ -48,8 +48,8 @@ struct KbaTile {
 constexpr int kKbaTJ = 4, kKbaTK = 16;
 constexpr int kDonePad = 32;   // progress counters one per 128-byte line
 // quantities of a sweep-layout row (BlockDev::srow): Q (5), p, lamc (3),
-// swg (15), R (5), alpha
-constexpr int kSQ_Q = 0, kSQ_P = 5, kSQ_LAM = 6, kSQ_SWG = 9, kSQ_R = 24, kSQ_A = 29, kSweepQ = 30;
+// swg (15), R (5), alpha, lamv (3, Navier-Stokes only)
+constexpr int kSQ_Q = 0, kSQ_P = 5, kSQ_LAM = 6, kSQ_SWG = 9, kSQ_R = 24, kSQ_A = 29, kSQ_LV = 30, kSweepQ = 33;
 // what k_sweep_fill writes: the state quantities and/or the static metrics
 constexpr int kGatherState = 1, kGatherMetrics = 2;
 template <bool FWD, int TJ, int TK>
@@ -61,7 +61,7 @@ __global__ void k_lusgs_kba2(const BlockDev *blocks, const KbaTile *tiles, int n
 template <bool FWD, int TJ, int TK>
 __global__ void k_lusgs_kba3(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *done, int *err,
                              double relax, Gas g, int lag);
-template <bool FWD, int TJ, int TK>
+template <bool FWD, int TJ, int TK, bool VISC>
 __global__ void k_lusgs_sl(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *err, double relax,
                            Gas g);
 // mailbox words per tile row, and the bit pattern of an empty word (a NaN
@@ -79,5 +79,20 @@ __global__ void k_update(const BlockDev *blocks, const long *cstart, int nblock,
 __global__ void k_limiter_serial(const BlockDev *blocks, int nblock, Gas g, int *bad_flag);
 __global__ void k_sections(const BlockDev *blocks, const int2 *secs, Gas g, double *sums);
 __global__ void k_finalize(const double *partial, int npart, const double *sums, ScalarsDev *sc, Gas g, double *hist_row);
+
+// laminar Navier-Stokes terms (fv3d_viscous.hip)
+// coordinate-transformation coefficients of every face and cell of one block (set-up)
+__global__ void k_metrics(const BlockDev *blocks, int block, double *fg0, double *fg1, double *fg2, double *cg);
+// viscous face fluxes of a tile of cells, summed into R (R = Fc - Fv); with
+// store set also the face fluxes, face gradients and cell sums
+struct ViscDebug {
+    double *fv[3];     // 5*box per direction
+    double *grad[3];   // 12*box per direction: d/dx, d/dy, d/dz of u v w T
+    double *fvc;       // 5*box
+};
+template <int TI, int TJ, int TK>
+__global__ void k_viscous(const BlockDev *blocks, const int4 *tiles, Gas g, int store, ViscDebug dbg);
+// vorticity magnitude per cell
+__global__ void k_curl(const BlockDev *blocks, const long *cstart, int nblock);
 
 }  // namespace fv3d

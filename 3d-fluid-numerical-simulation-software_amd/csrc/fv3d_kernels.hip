@@ -140,7 +140,8 @@ __global__ void k_bc(const BlockDev *__restrict__ blocks, const PatchDev *__rest
 }
 
 // ======================================================= spectral + dt
-// source/Spectral_curl_grad.cpp:9-42 and source/TimeStep.cpp:7-62 (Euler).
+// source/Spectral_curl_grad.cpp:9-42 and source/TimeStep.cpp:7-62, with the
+// viscous spectral radii for Navier-Stokes (TimeStep.cpp:38-49).
 __global__ void k_spectral_dt(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart, int nblock, Gas g,
                               const double *__restrict__ cfl_dev)
 {
@@ -160,25 +161,36 @@ __global__ void k_spectral_dt(const BlockDev *__restrict__ blocks, const long *_
     double vvelr = B.W[2 * box + x] - B.w_rot * B.cc[x];
     double wvelr = B.W[3 * box + x];
     double csoun = sqrt(Gamma * Rgas * B.W[5 * box + x]);
-    double lam[3];
+    double lam[3], eq[3];
     const long xn[3] = {x + B.si, x + B.sj, x + 1};
     for (int d = 0; d < 3; ++d) {
         const double *S = B.fS[d];
         double Sx = (S[x] + S[xn[d]]) * F32C(0.5);
         double Sy = (S[box + x] + S[box + xn[d]]) * F32C(0.5);
         double Sz = (S[2 * box + x] + S[2 * box + xn[d]]) * F32C(0.5);
-        double CS = csoun * sqrt(Sx * Sx + Sy * Sy + Sz * Sz);
+        eq[d] = Sx * Sx + Sy * Sy + Sz * Sz;
+        double CS = csoun * sqrt(eq[d]);
         lam[d] = fabs(uvelr * Sx + vvelr * Sy + wvelr * Sz) + CS;
         B.lamc[d * box + x] = lam[d];
     }
     int flag = *B.flag_low_dt;
     double dt_factor = flag == 0 ? F32C(1.0) : (double)(1.0f / (1.0f + (float)flag));
     double lc = lam[0] + lam[1] + lam[2];
-    double lv = 0.0;
+    double lv = 0.0, lvd[3] = {0.0, 0.0, 0.0};
+    if (g.viscous) {
+        const double mue_lam = sutherland(g, B.W[5 * box + x]), mue_tur = 0.0, dens = B.W[x];
+        const double a = (double)(4.0f / 3.0f) / dens, b = Gamma / dens;
+        const double m = a > b ? a : b;
+        for (int d = 0; d < 3; ++d) {
+            lvd[d] = m * (mue_lam / g.pr_lam + mue_tur / g.pr_tur) * eq[d] / B.vol[x];
+            B.lamv[d * box + x] = lvd[d];
+        }
+        lv = lvd[0] + lvd[1] + lvd[2];
+    }
     double dt = dt_factor * (*cfl_dev) * B.vol[x] / (lc + g.c_timestep * lv);
     B.dt[x] = dt;
     // diagonal of the LU-SGS operator (source/LU_SGS_Marching.cpp:76-77)
-    B.alpha[x] = B.vol[x] / dt + g.relax * (lam[0] + lam[1] + lam[2]) + F32C(2.0) * (0.0 + 0.0 + 0.0);
+    B.alpha[x] = B.vol[x] / dt + g.relax * (lam[0] + lam[1] + lam[2]) + F32C(2.0) * (lvd[0] + lvd[1] + lvd[2]);
 }
 
 // Decrement of the time-step damping counter at the end of CTimeDiscr::TimeStep
@@ -420,8 +432,7 @@ __global__ void k_lusgs_plane(const BlockDev *__restrict__ blocks, const long *_
     if (k < 1 || k > B.nk) return;
     const long box = B.box, x = cidx(B, i, j, k);
     const double Gamma = g.gamma;
-    double alpha = B.vol[x] / B.dt[x] + relax * (B.lamc[x] + B.lamc[box + x] + B.lamc[2 * box + x]) +
-                   F32C(2.0) * (0.0 + 0.0 + 0.0);
+    double alpha = B.alpha[x];   // Vol/dt + relax*sum(lamc) + 2*sum(lamv), from k_spectral_dt
     double A[3][5], V[3][5];
     const long st[3] = {B.si, B.sj, 1};
     const int pos[3] = {i, j, k};
@@ -438,10 +449,12 @@ __global__ void k_lusgs_plane(const BlockDev *__restrict__ blocks, const long *_
         const double *s = B.fs[d];
         double sd = F32C(0.5) * (s[nb] + s[nb + st[d]]);
         double rl = relax * B.lamc[d * box + nb];
+        // lamv * dQ (lamv = 0 on the inviscid path)
+        double lv = B.lamv ? B.lamv[d * box + nb] : 0.0;
         for (int c = 0; c < 5; ++c) {
             double dq = B.dQ[c * box + nb];
             A[d][c] = FORWARD ? (df[c] * sd + dq * rl) * F32C(0.5) : (df[c] * sd - dq * rl) * F32C(0.5);
-            V[d][c] = dq * 0.0;
+            V[d][c] = dq * lv;
         }
     }
     for (int c = 0; c < 5; ++c) {
@@ -627,7 +640,7 @@ struct SweepPf {
 // the cell's right-hand side and diagonal
 struct RowPf {
     double q[5], m[5], of[5], vn_old, cl;   // state, metrics, old-state flux terms
-    double p, l;
+    double p, l, lv;
     double x[5];
     double r[5], a;
 };
@@ -882,6 +895,7 @@ __global__ void __launch_bounds__(256) k_sweep_fill(const BlockDev *__restrict__
         for (int d = 0; d < 3; ++d) dst[(kSQ_LAM + d) * 64] = valid ? B.lamc[d * box + x] : 0.0;
         for (int c = 0; c < 5; ++c) dst[(kSQ_R + c) * 64] = valid ? B.R[c * box + x] : 0.0;
         dst[kSQ_A * 64] = valid ? B.alpha[x] : 1.0;
+        for (int d = 0; d < 3; ++d) dst[(kSQ_LV + d) * 64] = (valid && B.lamv) ? B.lamv[d * box + x] : 0.0;
     }
     if (what & kGatherMetrics)
         for (int q = 0; q < 15; ++q) dst[(kSQ_SWG + q) * 64] = valid ? B.swg[q * box + x] : 0.0;
@@ -893,8 +907,9 @@ __global__ void __launch_bounds__(256) k_sweep_fill(const BlockDev *__restrict__
 // in the chunk), each as one contiguous run.
 __global__ void __launch_bounds__(256) k_sweep_pack(const BlockDev *__restrict__ blocks, int block)
 {
-    constexpr int TJ = kKbaTJ, TK = kKbaTK, NQ = 15;
+    constexpr int TJ = kKbaTJ, TK = kKbaTK;
     __shared__ double buf[kPackChunk][64];
+    const int NQ = blocks[block].lamv ? 18 : 15;   // lamv rows for Navier-Stokes
     const BlockDev &B = blocks[block];
     const int nchunk = (B.ni + kPackChunk - 1) / kPackChunk;
     const int slot = blockIdx.x / nchunk, i0 = (blockIdx.x % nchunk) * kPackChunk + 1;
@@ -911,7 +926,8 @@ __global__ void __launch_bounds__(256) k_sweep_pack(const BlockDev *__restrict__
         else if (qi == 5) { src = B.W + 4 * box; rq = kSQ_P; }
         else if (qi < 9) { src = B.lamc + (qi - 6) * box; rq = kSQ_LAM + qi - 6; }
         else if (qi < 14) { src = B.R + (qi - 9) * box; rq = kSQ_R + qi - 9; }
-        else { src = B.alpha; rq = kSQ_A; }
+        else if (qi == 14) { src = B.alpha; rq = kSQ_A; }
+        else { src = B.lamv + (qi - 15) * box; rq = kSQ_LV + qi - 15; }
         for (int ci = w; ci < kPackChunk; ci += 4) {
             const int i = i0 + ci;
             if (col && i <= B.ni) buf[ci][lane] = src[cidx(B, i, j, k)];
@@ -966,7 +982,7 @@ __global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict
 // and the consumer reads the words a step ahead and, before using them, waits
 // (bounded) while any word it needs still holds the empty pattern.  Tiles
 // start in ticket order, so every upstream tile is already running.
-template <bool FWD, int TJ, int TK>
+template <bool FWD, int TJ, int TK, bool VISC>
 __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ blocks, const KbaTile *__restrict__ tiles,
                                                   int ntiles, int *__restrict__ ticket, int *__restrict__ err,
                                                   double relax, Gas g)
@@ -1070,6 +1086,7 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                 for (int c = 0; c < 5; ++c) bn.q[c] = src[(kSQ_Q + c) * 64];
                 bn.p = src[kSQ_P * 64];
                 bn.l = src[(kSQ_LAM + d) * 64];
+                if (VISC) bn.lv = src[(kSQ_LV + d) * 64];
                 for (int c = 0; c < 5; ++c) bn.m[c] = src[(kSQ_SWG + d * 5 + c) * 64];
                 if (d == 0) {
                     const int ro = clampr(rt);
@@ -1122,7 +1139,7 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                 double A[5], V[5];
                 for (int c = 0; c < 5; ++c) {
                     const double a = FWD ? (df[c] * sd + dqn[c] * b.cl) * F32C(0.5) : (df[c] * sd - dqn[c] * b.cl) * F32C(0.5);
-                    const double v = dqn[c] * 0.0;   // lamv * dQ, lamv = 0 on the inviscid path
+                    const double v = dqn[c] * (VISC ? b.lv : 0.0);   // lamv * dQ, lamv = 0 on the inviscid path
                     A[c] = use ? a : 0.0;
                     V[c] = use ? v : 0.0;
                 }
@@ -1200,8 +1217,10 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
     }
 }
 
-template __global__ void k_lusgs_sl<true, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, double, Gas);
-template __global__ void k_lusgs_sl<false, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, double, Gas);
+template __global__ void k_lusgs_sl<true, kKbaTJ, kKbaTK, false>(const BlockDev *, const KbaTile *, int, int *, int *, double, Gas);
+template __global__ void k_lusgs_sl<false, kKbaTJ, kKbaTK, false>(const BlockDev *, const KbaTile *, int, int *, int *, double, Gas);
+template __global__ void k_lusgs_sl<true, kKbaTJ, kKbaTK, true>(const BlockDev *, const KbaTile *, int, int *, int *, double, Gas);
+template __global__ void k_lusgs_sl<false, kKbaTJ, kKbaTK, true>(const BlockDev *, const KbaTile *, int, int *, int *, double, Gas);
 
 __global__ void k_fill_u64(unsigned long long *p, long n, unsigned long long v)
 {

@@ -126,11 +126,13 @@ class GpuSolver:
         ni, nj, nk = im - 1, jm - 1, km - 1
         if name in ("cv", "pv"):
             shape = (im + 1, jm + 1, km + 1, 5 if name == "cv" else 6)
-        elif name in ("res", "dq"):
+        elif name in ("res", "dq", "fv"):
             shape = (ni, nj, nk, 5)
-        elif name in ("fcI", "fcJ", "fcK"):
-            shape = {"fcI": (im, nj, nk, 5), "fcJ": (ni, jm, nk, 5), "fcK": (ni, nj, km, 5)}[name]
-        elif name in ("dt", "lamcI", "lamcJ", "lamcK", "vol"):
+        elif name in ("fcI", "fcJ", "fcK", "fvI", "fvJ", "fvK"):
+            shape = {"I": (im, nj, nk, 5), "J": (ni, jm, nk, 5), "K": (ni, nj, km, 5)}[name[2]]
+        elif name in ("gradI", "gradJ", "gradK"):
+            shape = {"I": (im, nj, nk, 12), "J": (ni, jm, nk, 12), "K": (ni, nj, km, 12)}[name[4]]
+        elif name in ("dt", "lamcI", "lamcJ", "lamcK", "lamvI", "lamvJ", "lamvK", "omega", "vol"):
             shape = (ni, nj, nk)
         elif name.startswith("d") and name[1] in "IJK":
             c = 6 if name.endswith("_pv") else 5

@@ -46,11 +46,11 @@ typedef struct {
     double inlet_dir[3];
     double cfl, c_timestep, relax;
     int turbo_type;     /* 0 axial, 1 centrifugal, 2 general outlet */
-    int viscous;        /* 0 Euler, 1 Navier-Stokes */
+    int viscous;        /* 0 Euler, 1 Navier-Stokes (laminar) */
     int recon_var;      /* 100 conservative, 200 primitive */
     int recon_scheme;   /* 100 NND, 200 MUSCL2u, 300 MUSCL2c, 400 MUSCL2o, 500 MUSCL3 */
     int flux_scheme;    /* 200 Steger-Warming, 300 Roe */
-    int turbulence;     /* 0 none */
+    int turbulence;     /* 0 none (the only model in this build) */
     int limiter;        /* limit flow variables (source/Limit_FlowVars.cpp:8-119) */
     int global_dt;      /* 0 local time step */
     int device;         /* HIP device ordinal */
@@ -83,6 +83,9 @@ int fv3d_state_upload(fv3d_ctx *ctx, int block, const double *pv, const double *
  *   cv (IM+1,JM+1,KM+1,5), pv (IM+1,JM+1,KM+1,6)  -- cell arrays incl. ghost cells
  *   res dq (ni,nj,nk,5); dt lamcI lamcJ lamcK vol (ni,nj,nk)
  *   fcI (IM,nj,nk,5) fcJ (ni,JM,nk,5) fcK (ni,nj,KM,5)  -- face fluxes
+ *   Navier-Stokes: lamvI lamvJ lamvK omega (ni,nj,nk); with debug bit 0 also
+ *   fvI fvJ fvK (viscous face fluxes, shapes as fcI..), gradI gradJ gradK
+ *   (face gradients, 12: d/dx, d/dy, d/dz of u v w T) and fv (ni,nj,nk,5)
  *   dIS_cv dIP_cv (2,nj,nk,5), dJS_cv dJP_cv (ni,2,nk,5), dKS_cv dKP_cv (ni,nj,2,5)
  *   and the same with _pv and 6 components -- Dummy layers 1..2.
  * Returns the number of doubles written. */
@@ -93,6 +96,15 @@ int fv3d_bc_apply(fv3d_ctx *ctx);          /* CMesh::BoundaryCondition */
 int fv3d_spectral(fv3d_ctx *ctx);          /* CFluidProps::SpectralRadii */
 int fv3d_timestep(fv3d_ctx *ctx);          /* CTimeDiscr::TimeStep */
 int fv3d_convective(fv3d_ctx *ctx);        /* CMesh::ConvectiveFlux incl. WallFlux, source term and Res */
+/* CMesh::ViscousFlux (source/ComputeAllBlockVars.cpp:131-160) with
+ * CFluidProps::Gradient (source/Spectral_curl_grad.cpp:85-493): subtracts the
+ * viscous flux from the residual that fv3d_convective wrote (Navier-Stokes;
+ * a no-op for Euler). */
+int fv3d_viscous(fv3d_ctx *ctx);
+/* CFluidProps::Curl (source/Spectral_curl_grad.cpp:44-83): the vorticity
+ * magnitude "omega".  Only turbulence models read it, so fv3d_resflux does not
+ * evaluate it for laminar flow. */
+int fv3d_curl(fv3d_ctx *ctx);
 int fv3d_resflux(fv3d_ctx *ctx);           /* all of CMesh::ResFlux */
 
 /* CMesh::LU_SGS_Marching (source/LU_SGS_Marching.cpp:10-291): save, ResFlux,
@@ -128,10 +140,11 @@ int fv3d_set_timing(fv3d_ctx *ctx, int on);
 int fv3d_timing(fv3d_ctx *ctx, double *ms, long *counts);
 
 /* Debug flags: bit 0 keeps the face fluxes of the last residual evaluation
- * so fv3d_field_download can return fcI/fcJ/fcK (single-block meshes).
+ * so fv3d_field_download can return fcI/fcJ/fcK, and for Navier-Stokes the
+ * viscous fvI/fvJ/fvK, gradI/gradJ/gradK and fv (single-block meshes).
  * Bits 1-4 select another LU-SGS sweep form for comparison (1: one launch per
- * hyperplane, 2-4: earlier wavefront forms); bits 8-15 and 16-23 set the
- * chunk and extra lag of the wavefront forms. */
+ * hyperplane, 2-4: earlier wavefront forms, Euler only); bits 8-15 and 16-23
+ * set the chunk and extra lag of the wavefront forms. */
 int fv3d_set_debug(fv3d_ctx *ctx, int flags);
 
 /* Number of interior cells over all blocks. */
