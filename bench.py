@@ -31,13 +31,14 @@ sys.path.insert(0, os.path.join(REPO, "oracle"))
 BYTES_PER_CELL = {
     "residual": 176,          # Q 5, S 9, nodes 3 -> R 5
     "sweep": 232,             # one sweep: R|dQ* 5, Q 5, lamc 3, dt 1, vol 1, S 9 -> dQ 5
+    "sweep_ns": 256,          # the same plus lamv 3 (Navier-Stokes)
     "update": 128,            # Q 5, dQ 5, vol 1 -> Q 5
     "bc_spectral_dt": 152,    # Q 5, S 9, vol 1 -> lamc 3, dt 1
 }
 # HBM bytes one forward-sweep launch actually moves on a workload, from
 # rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (copy-calibrated); the
-# summaries are under profiles/.  Keyed by (ni, nj, nk, sweep form).
-MEASURED_FWD_SWEEP_TRAFFIC = {(64, 64, 256, "rows"): 487.2e6}
+# summaries are under profiles/.  Keyed by (ni, nj, nk, sweep form, viscous).
+MEASURED_FWD_SWEEP_TRAFFIC = {(64, 64, 256, "rows", False): 472.5e6, (64, 64, 256, "rows", True): 506.0e6}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -50,6 +51,7 @@ def parse():
     ap.add_argument("--nj", type=int, default=64)
     ap.add_argument("--nk", type=int, default=256)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--viscous", action="store_true", help="laminar Navier-Stokes (Reynolds number ~2e3 per unit length)")
     ap.add_argument("--sweep", choices=["rows", "kba3", "kba2", "kba", "plane"], default="rows")
     ap.add_argument("--chunk", type=int, default=1)
     ap.add_argument("--lag", type=int, default=0)
@@ -105,7 +107,8 @@ def main():
     from fv3d import cases
     from fv3d.solver import GpuSolver
 
-    case = cases.bump_channel(ni=a.ni, nj=a.nj, nk=a.nk, seed=1, max_iter=a.steps)
+    case = cases.bump_channel(ni=a.ni, nj=a.nj, nk=a.nk, seed=1, max_iter=a.steps, viscous=a.viscous,
+                              len_ref=1e-4 if a.viscous else 1.0)
     g = GpuSolver(case, device=0,
                   debug={"plane": 2, "kba": 4, "kba2": 8, "kba3": 16, "rows": 0}[a.sweep] | (a.chunk << 8) | (a.lag << 16))
     ncell = g.n_cells
@@ -128,11 +131,13 @@ def main():
     if ms[5] > 0:
         # dominant kernel: the forward sweep, timed alone with HIP events on the
         # library's stream
-        kname, kms, kbytes = "k_lusgs_sl<forward> (LU-SGS forward sweep)", ms[5], BYTES_PER_CELL["sweep"]
+        kname, kms = "k_lusgs_sl<forward> (LU-SGS forward sweep)", ms[5]
+        kbytes = BYTES_PER_CELL["sweep_ns" if a.viscous else "sweep"]
         times["forward_sweep"], times["backward_sweep"] = ms[5], ms[6]
-        traffic = MEASURED_FWD_SWEEP_TRAFFIC.get((a.ni, a.nj, a.nk, a.sweep))
+        traffic = MEASURED_FWD_SWEEP_TRAFFIC.get((a.ni, a.nj, a.nk, a.sweep, a.viscous))
     else:
-        kname, kms, kbytes = "LU-SGS forward+backward sweeps", ms[1], 2 * BYTES_PER_CELL["sweep"]
+        kname, kms = "LU-SGS forward+backward sweeps", ms[1]
+        kbytes = 2 * BYTES_PER_CELL["sweep_ns" if a.viscous else "sweep"]
         traffic = None
     achieved = kbytes * ncell / (kms * 1e-3) / 1e9
     cpu = None if a.no_cpu_baseline else cpu_baseline(a.cpu_iters)
@@ -149,7 +154,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"inviscid bump channel {a.nk}x{a.nj}x{a.ni} cells (single block), Roe + MUSCL3 "
+        "config": {"workload": f"{'laminar Navier-Stokes' if a.viscous else 'inviscid'} bump channel "
+                               f"{a.nk}x{a.nj}x{a.ni} cells (single block), Roe + MUSCL3 "
                                f"primitive, LU-SGS local dt CFL 5, plain FP64",
                    "cells": ncell, "hip_graph": graph, "sweep": a.sweep, "chunk": a.chunk, "lag": a.lag},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -6,7 +6,8 @@
 // Reference surfaces mirrored (paths relative to the reference tree):
 //   CMesh        include/Mesh.h:31-90       (Geometry, BoundaryCondition, ResFlux,
 //                                             Time_Marching, Res_Calculate)
-//   CSpaceDiscr  include/SpaceDiscr.h:62-69 (ConvectiveFlux)
+//   CSpaceDiscr  include/SpaceDiscr.h:62-69 (ConvectiveFlux, ViscousFlux)
+//   CFluidProps  include/FluidProps.h (Curl)
 //   CTimeDiscr   include/TimeDiscr.h:43-49  (TimeStep)
 //   CSolver      include/Solver.h:46-52     (Solver: one iteration with the CFL ramp)
 //   CUserInput   source/UserInput.cpp:17-239 (config.dat), 241-286 (grid),
@@ -322,6 +323,19 @@ public:
     explicit CSpaceDiscr(CMesh &m) : m_(m) {}
     // CMesh::ConvectiveFlux incl. WallFlux, the source term and Res = Fc.
     void ConvectiveFlux() { check(m_.ctx(), fv3d_convective(m_.ctx()), "fv3d_convective"); }
+    // CMesh::ViscousFlux with the face gradients: Res -= Fv (after ConvectiveFlux;
+    // nothing for Euler).
+    void ViscousFlux() { check(m_.ctx(), fv3d_viscous(m_.ctx()), "fv3d_viscous"); }
+
+private:
+    CMesh &m_;
+};
+
+class CFluidProps {
+public:
+    explicit CFluidProps(CMesh &m) : m_(m) {}
+    // CFluidProps::Curl: vorticity magnitude, field "omega" (Navier-Stokes).
+    void Curl() { check(m_.ctx(), fv3d_curl(m_.ctx()), "fv3d_curl"); }
 
 private:
     CMesh &m_;
