@@ -124,3 +124,35 @@ def bump_channel(ni: int = 32, nj: int = 32, nk: int = 96, seed: int = 1, beta: 
     blocks = [BlockInfo(1, im, jm, km, 1, 0.0)]
     states = [uniform_state(cfg, (ni, nj, nk), seed=seed)]
     return Case(cfg, [xyz], patches, blocks, states)
+
+
+def with_wall_shear(case: Case, delta: float = 0.3) -> Case:
+    """Scales the velocity of every block's restart state by a wall profile
+    tanh(d_i/delta) * tanh(d_j/delta), with d the normalised index distance to
+    the nearer I and J faces, so a viscous or turbulence-model case starts
+    with shear layers at the walls.  Pressure and density are kept; T and rhoE
+    follow."""
+    c = nondim_constants(case.cfg)
+    g, r = c["gamma"], c["rgas"]
+    for st in case.states:
+        ni, nj, nk = st["rho"].shape
+        di = np.minimum(np.arange(ni) + 0.5, ni - np.arange(ni) - 0.5) / ni
+        dj = np.minimum(np.arange(nj) + 0.5, nj - np.arange(nj) - 0.5) / nj
+        f = (np.tanh(di / delta)[:, None] * np.tanh(dj / delta)[None, :])[..., None]
+        for q in ("u", "v", "w"):
+            st[q] = st[q] * f
+        st["T"] = st["p"] / (st["rho"] * r)
+        st["rhoE"] = st["p"] / (g - 1.0) + st["rho"] * (st["u"] ** 2 + st["v"] ** 2 + st["w"] ** 2) / 2.0
+    return case
+
+
+def from_meta(meta: Dict) -> Case:
+    """The case a fixture's metadata describes: ``kwargs`` for bump_channel and
+    the iteration count, optionally ``wall_shear`` (with_wall_shear) and
+    ``turbulence`` (the config's turbulence model code)."""
+    case = bump_channel(max_iter=meta["iterations"], **meta["kwargs"])
+    if meta.get("wall_shear"):
+        case = with_wall_shear(case)
+    if meta.get("turbulence"):
+        case.cfg.turbulence = meta["turbulence"]
+    return case

@@ -856,6 +856,103 @@ static void viscous_flux(orc_ctx *c, blk_t *b)
             }
 }
 
+/* -------------------------------------------------------- Baldwin-Lomax */
+/* CBlocks::BL_model_1d (source/BL_Model.cpp:283-329): arrays indexed 1..M-1
+ * from the wall outwards */
+static void bl_model_1d(int M, const double *dens, const double *Vvel, const double *dis, const double *Omega,
+                        const double *mue_l, double *mue_t)
+{
+    const double A_plus = F32(26.0), Cwk = F32(1.0), k = F32(0.4), Ckleb = F32(0.3), alpha = F32(0.0168), Ccp = F32(1.6);
+    double Tw = fabs(Omega[1] * mue_l[1]);
+    for (int i = 1; i < M; ++i)
+        if (fabs(Omega[i] * mue_l[i]) > Tw) Tw = fabs(Omega[i] * mue_l[i]);
+    double Ret = sqrt(dens[1] * Tw) / mue_l[1];
+    double Fmax = 0.0, y_max = 0.0, Udif = 0.0;
+    for (int i = 1; i < M; ++i) {
+        if (Vvel[i] > Udif) Udif = Vvel[i];
+        double y_plus = dis[i] * Ret;
+        double F = dis[i] * fabs(Omega[i]) * (F32(1.0) - exp(-y_plus / A_plus));
+        if (F > Fmax) { Fmax = F; y_max = dis[i]; }
+    }
+    double wa = y_max * Fmax, wb = Cwk * y_max * Udif * Udif / Fmax;
+    double Fwake = wa < wb ? wa : wb;
+    for (int i = 1; i < M; ++i) {
+        double y_plus = Ret * dis[i];
+        double bL = k * dis[i] * (F32(1.0) - exp(-y_plus / A_plus));
+        double mue_in = dens[i] * bL * bL * Omega[i];
+        double Fkleb = F32(1.0) / (F32(1.0) + F32(5.5) * pow((Ckleb * dis[i] / y_max), 6.0));
+        double mue_out = alpha * Ccp * dens[i] * Fwake * Fkleb;
+        mue_t[i] = fabs(mue_in) > fabs(mue_out) ? mue_out : mue_in;
+    }
+}
+
+/* CBlocks::BL_Model (source/BL_Model.cpp:8-279): for every wall patch of the
+ * block, in table order, the 1-D model along each grid line normal to it; a
+ * cell's first visit sets mue_t, later visits keep the minimum; the first cell
+ * and the ghost cell of each line are then set to zero.  The reference passes
+ * JM as the line length for K-face walls (line 225), which is only meaningful
+ * when JM == KM; orc_bl rejects K-face walls otherwise. */
+/* cell (i,j,k) at position n along a line normal to a wall of direction D
+ * whose tangential indices are a (j for I walls, i otherwise) and cc (j for K
+ * walls, k otherwise) */
+static inline long bl_cell(const blk_t *b, int D, int n, int a, int cc)
+{
+    return D == 0 ? C3(b, n, a, cc) : D == 1 ? C3(b, a, n, cc) : C3(b, a, cc, n);
+}
+
+static int bl_model(orc_ctx *c, blk_t *b)
+{
+    int IM = b->IM, JM = b->JM, KM = b->KM;
+    int MM = IM > JM ? IM : JM;
+    if (KM > MM) MM = KM;
+    double *dens = calloc(MM, sizeof(double)), *Vvel = calloc(MM, sizeof(double)), *dis = calloc(MM, sizeof(double));
+    double *Om = calloc(MM, sizeof(double)), *mue_l = calloc(MM, sizeof(double)), *mue_t = calloc(MM, sizeof(double));
+    long box = (long)(IM + 1) * (JM + 1) * (KM + 1);
+    unsigned char *flag = calloc(box, 1);
+    int rc = 0;
+    for (int q = 0; q < b->npatch && !rc; ++q) {
+        const patch_t *P = &b->patch[q];
+        if (P->kind != 2) continue;
+        int D = (P->face - 1) / 2, hi = (P->face - 1) % 2;
+        if (D == 2 && JM != KM) { rc = -1; break; }
+        int N = D == 0 ? IM : D == 1 ? JM : KM;
+        int M = D == 0 ? IM : JM;
+        int a0 = D == 0 ? P->js : P->is, a1 = D == 0 ? P->je : P->ie;
+        int c0 = D == 2 ? P->js : P->ks, c1 = D == 2 ? P->je : P->ke;
+        for (int a = a0; a <= a1; ++a)
+            for (int cc = c0; cc <= c1; ++cc) {
+                const face_t *F = D == 0 ? b->fi : D == 1 ? b->fj : b->fk;
+                const v3_t cw = F[bl_cell(b, D, hi ? N : 1, a, cc)].c;   /* wall face of the line */
+                for (int n = 1; n <= N - 1; ++n) {
+                    int n0 = hi ? N - n : n;
+                    long x = bl_cell(b, D, n, a, cc);
+                    v3_t cl = b->cell[x].c;
+                    dis[n0] = sqrt(pow((cl.x - cw.x), 2.0) + pow((cl.y - cw.y), 2.0) + pow((cl.z - cw.z), 2.0));
+                    const pv_t *w = &b->pv[x];
+                    dens[n0] = w->d;
+                    Vvel[n0] = sqrt(w->u * w->u + w->v * w->v + w->w * w->w);
+                    Om[n0] = b->omega[x];
+                    mue_l[n0] = sutherland(&c->nd, w->t);
+                }
+                bl_model_1d(M, dens, Vvel, dis, Om, mue_l, mue_t);
+                for (int n = 1; n <= N - 1; ++n) {
+                    int n0 = hi ? N - n : n;
+                    long x = bl_cell(b, D, n, a, cc);
+                    if (!flag[x]) { flag[x] = 1; b->mut[x] = mue_t[n0]; }
+                    else b->mut[x] = b->mut[x] < mue_t[n0] ? b->mut[x] : mue_t[n0];
+                }
+            }
+        /* first cell and ghost cell of every line of this wall */
+        for (int a = a0; a <= a1; ++a)
+            for (int cc = c0; cc <= c1; ++cc) {
+                b->mut[bl_cell(b, D, hi ? N - 1 : 1, a, cc)] = 0.0;
+                b->mut[bl_cell(b, D, hi ? N : 0, a, cc)] = 0.0;
+            }
+    }
+    free(dens); free(Vvel); free(dis); free(Om); free(mue_l); free(mue_t); free(flag);
+    return rc;
+}
+
 /* ------------------------------------------------------------ time step */
 /* source/TimeStep.cpp:7-92 */
 int orc_timestep(orc_ctx *c)
@@ -1249,6 +1346,10 @@ int orc_resflux(orc_ctx *c)
 {
     orc_bc(c);
     orc_spectral(c);
+    /* source/ComputeAllBlockVars.cpp:200-221 (fine grid) */
+    if (c->cfg.viscous && c->cfg.turbulence == 100)
+        for (int n = 0; n < c->nb; ++n)
+            if (bl_model(c, &c->blk[n])) return -1;
     orc_timestep(c);
     orc_convective(c);
     if (c->cfg.viscous)
@@ -1437,7 +1538,7 @@ int orc_lusgs(orc_ctx *c)
             for (int j = 1; j <= b->JM - 1; ++j)
                 for (int k = 1; k <= b->KM - 1; ++k) b->cv_old[C3(b, i, j, k)] = b->cv[C3(b, i, j, k)];
     }
-    orc_resflux(c);
+    if (orc_resflux(c)) return -1;
     for (int n = 0; n < c->nb; ++n) {
         blk_t *b = &c->blk[n];
         lusgs_sweeps(c, b);

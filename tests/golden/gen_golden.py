@@ -41,6 +41,9 @@ TINY = {
     "tiny_sw_muscl2u_cons": (dict(ni=6, nj=5, nk=10, seed=5, flux=200, recon_scheme=200, recon_var=100), 10),
     # laminar Navier-Stokes at a Reynolds number of a few hundred per unit length
     "tiny_ns_roe_muscl3_prim": (dict(ni=6, nj=5, nk=10, seed=6, viscous=True, len_ref=2e-5), 20),
+    # Baldwin-Lomax from a state with wall shear layers
+    "tiny_bl_roe_muscl3_prim": (dict(ni=7, nj=6, nk=10, seed=7, viscous=True, len_ref=1e-3), 20,
+                                {"turbulence": 100, "wall_shear": True}),
 }
 NS_FIELDS_IT1 = ["omega", "gradI", "gradJ", "gradK", "fvI", "fvJ", "fvK", "fv", "lamvI", "lamvJ", "lamvK"]
 
@@ -49,18 +52,20 @@ FIELDS_IT1 = ["res", "dq", "dt", "lamcI", "lamcJ", "lamcK", "cv", "pv", "fcI", "
               "dIS_pv", "dIP_pv", "dJS_pv", "dJP_pv", "dKS_pv", "dKP_pv"]
 
 
-def gen_tiny(name, kw, n_iter):
-    case = cases.bump_channel(max_iter=n_iter, **kw)
+def gen_tiny(name, kw, n_iter, extra=None):
+    meta = {"kwargs": kw, "iterations": n_iter}
+    meta.update(extra or {})
+    case = cases.from_meta(meta)
     with tempfile.TemporaryDirectory() as td:
         cdir, odir = os.path.join(td, "case"), os.path.join(td, "out")
         case.write(cdir)
         res = O.run_reference(cdir, n_iter, odir, threads=1, dump_first=True)
         arrs = {"history": res["history"]}
-        for f in FIELDS_IT1 + (NS_FIELDS_IT1 if kw.get("viscous") else []):
+        for f in FIELDS_IT1 + (NS_FIELDS_IT1 if kw.get("viscous") else []) + (["mut"] if meta.get("turbulence") else []):
             arrs["it1_" + f] = np.load(os.path.join(odir, f"it1_b1_{f}.npy"))
-        for f in ("cv", "pv", "res", "dq"):
+        for f in ("cv", "pv", "res", "dq") + (("mut",) if meta.get("turbulence") else ()):
             arrs["final_" + f] = np.load(os.path.join(odir, f"final_b1_{f}.npy"))
-    arrs["meta"] = np.frombuffer(json.dumps({"kwargs": kw, "iterations": n_iter}).encode(), dtype=np.uint8)
+    arrs["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
     np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)
     print("wrote", name)
 
@@ -97,9 +102,9 @@ def main():
     a = ap.parse_args()
     subprocess.run(["make", "-C", os.path.join(REPO, "oracle"), "ref", "restatement"], check=True,
                    stdout=subprocess.DEVNULL)
-    for name, (kw, n) in TINY.items():
+    for name, spec in TINY.items():
         if a.only is None or name in a.only:
-            gen_tiny(name, kw, n)
+            gen_tiny(name, *spec)
     if a.c0:
         gen_c0()
 

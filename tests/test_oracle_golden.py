@@ -4,7 +4,8 @@ The fixtures (tests/golden/*.npz) hold the reference's own outputs for seeded
 synthetic cases; the restatement must reproduce them bit for bit at one
 thread.  Cases cover Roe and Steger-Warming, MUSCL3 / NND / MUSCL2u / MUSCL2o
 reconstruction on primitive and conservative variables, the inlet,
-general-outlet and wall ghost fills, and the laminar Navier-Stokes terms.
+general-outlet and wall ghost fills, the laminar Navier-Stokes terms and the
+Baldwin-Lomax eddy viscosity.
 """
 import json
 import os
@@ -16,7 +17,7 @@ from fv3d import cases
 import oracle as O
 
 TINY = ["tiny_roe_muscl3_prim", "tiny_sw_nnd_prim", "tiny_roe_muscl3_cons", "tiny_roe_muscl2o_prim",
-        "tiny_sw_muscl2u_cons", "tiny_ns_roe_muscl3_prim"]
+        "tiny_sw_muscl2u_cons", "tiny_ns_roe_muscl3_prim", "tiny_bl_roe_muscl3_prim"]
 NS_FIELDS = ("omega", "gradI", "gradJ", "gradK", "fvI", "fvJ", "fvK", "fv", "lamvI", "lamvJ", "lamvK")
 
 
@@ -29,13 +30,15 @@ def _load(golden_dir, name):
 @pytest.mark.parametrize("name", TINY)
 def test_first_iteration_bitexact(golden_dir, name):
     z, meta = _load(golden_dir, name)
-    case = cases.bump_channel(max_iter=meta["iterations"], **meta["kwargs"])
+    case = cases.from_meta(meta)
     o = O.Oracle(case)
     o.call("lusgs")
     fields = ("res", "dq", "dt", "lamcI", "lamcJ", "lamcK", "fcI", "fcJ", "fcK", "cv", "pv",
               "dIS_cv", "dIP_cv", "dJS_cv", "dJP_cv", "dKS_cv", "dKP_cv", "dKS_pv", "dKP_pv", "dIS_pv")
     if meta["kwargs"].get("viscous"):
         fields += NS_FIELDS
+    if meta.get("turbulence"):
+        fields += ("mut",)
     for f in fields:
         ref = z["it1_" + f]
         got = o.get(1, f).reshape(ref.shape)
@@ -45,7 +48,7 @@ def test_first_iteration_bitexact(golden_dir, name):
 @pytest.mark.parametrize("name", TINY)
 def test_history_bitexact(golden_dir, name):
     z, meta = _load(golden_dir, name)
-    case = cases.bump_channel(max_iter=meta["iterations"], **meta["kwargs"])
+    case = cases.from_meta(meta)
     o = O.Oracle(case)
     h = o.iterate(meta["iterations"])
     ref = z["history"]
@@ -53,6 +56,8 @@ def test_history_bitexact(golden_dir, name):
     assert np.array_equal(h[:, :7], ref[:, :7])
     assert np.array_equal(o.get(1, "cv"), z["final_cv"])
     assert np.array_equal(o.get(1, "pv"), z["final_pv"])
+    if meta.get("turbulence"):
+        assert np.array_equal(o.get(1, "mut").ravel(), z["final_mut"].ravel())
 
 
 def test_c0_first_iterations(golden_dir):

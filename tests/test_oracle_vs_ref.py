@@ -5,8 +5,8 @@ a container that has the reference sources); elsewhere these tests skip and
 the committed fixtures (test_oracle_golden.py) carry the comparison.  The
 cases cover the paths the fixtures leave out: MUSCL2c reconstruction, the
 axial (radial-equilibrium) and centrifugal outlets, a rotating block with
-rotating walls, and laminar Navier-Stokes with Steger-Warming/NND and with a
-rotating block.  The reference runs on one thread (its face loops race on a
+rotating walls, laminar Navier-Stokes with Steger-Warming/NND and with a
+rotating block, and Baldwin-Lomax with a K-face wall and with a rotating block.  The reference runs on one thread (its face loops race on a
 shared variable with more, source/Upwind_Convective.cpp:7,29).
 """
 import os
@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 from fv3d import cases
+from fv3d.refio import Patch
 import oracle as O
 
 pytestmark = pytest.mark.skipif(not os.path.exists(O.REF_BIN), reason="reference harness not built here")
@@ -60,17 +61,42 @@ def variant(name):
         for p in c.patches:
             if p.kind == "wall" and p.face in (3, 1):
                 p.rotate = 1
+    elif name == "bl_sw_muscl2u_kwall":
+        # Baldwin-Lomax with walls on I, J and part of the K = 1 face (the
+        # reference's K-wall line length is JM, so JM == KM here)
+        ni, nj, nk = 6, 8, 8
+        c = cases.with_wall_shear(cases.bump_channel(ni=ni, nj=nj, nk=nk, seed=26, flux=200, recon_scheme=200,
+                                                     max_iter=10, viscous=True, len_ref=1e-3))
+        c.cfg.turbulence = 100
+        pt = []
+        for p in c.patches:
+            if p.kind == "inlet":
+                pt.append(Patch("inlet", 1, 5, (1, ni, 1, nj - 2, 1, 1)))
+                pt.append(Patch("wall", 1, 5, (1, ni, nj - 1, nj, 1, 1)))
+            else:
+                pt.append(p)
+        c.patches = pt
+    elif name == "bl_rotating_roe_muscl3":
+        c = _shifted(cases.with_wall_shear(cases.bump_channel(ni=7, nj=6, nk=11, seed=27, max_iter=10, viscous=True,
+                                                              len_ref=1e-3)))
+        c.cfg.turbulence = 100
+        c.blocks[0].rpm = 3000.0
+        c.blocks[0].nblade = 22
+        for p in c.patches:
+            if p.kind == "wall" and p.face in (3, 1):
+                p.rotate = 1
     else:
         raise KeyError(name)
     return c
 
 
 @pytest.mark.parametrize("name", ["muscl2c_axial", "nnd_cons_sw_centrifugal", "rotating_roe_muscl3",
-                                  "ns_sw_nnd_cons", "ns_rotating_roe_muscl2o"])
+                                  "ns_sw_nnd_cons", "ns_rotating_roe_muscl2o", "bl_sw_muscl2u_kwall",
+                                  "bl_rotating_roe_muscl3"])
 def test_restatement_matches_reference(name):
     case = variant(name)
     n = case.cfg.max_iter
-    fields = FIELDS + (NS_FIELDS if case.cfg.equation == "N" else [])
+    fields = FIELDS + (NS_FIELDS if case.cfg.equation == "N" else []) + (["mut"] if case.cfg.turbulence else [])
     with tempfile.TemporaryDirectory() as td:
         cdir, odir = os.path.join(td, "case"), os.path.join(td, "out")
         case.write(cdir)
@@ -88,3 +114,6 @@ def test_restatement_matches_reference(name):
         assert np.array_equal(h[:, :7], ref["history"][:, :7], equal_nan=True), name
         want = np.load(os.path.join(odir, "final_b1_cv.npy"))
         assert np.array_equal(o2.get(1, "cv"), want)
+        if case.cfg.turbulence:
+            want = np.load(os.path.join(odir, "final_b1_mut.npy"))
+            assert np.array_equal(o2.get(1, "mut").ravel(), want.ravel())
