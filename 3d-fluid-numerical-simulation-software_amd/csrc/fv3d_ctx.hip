@@ -50,6 +50,7 @@ struct fv3d_ctx {
     // device work lists
     BlockDev *d_blocks = nullptr;
     PatchDev *d_patches = nullptr;
+    std::vector<PatchDev> pd_host;   // the device patch list, host copy
     long *d_pstart_bc = nullptr, *d_cstart = nullptr, *d_pstart_plane = nullptr;
     int4 *d_tiles = nullptr;
     int2 *d_secs = nullptr;
@@ -141,6 +142,7 @@ static void nondimensionalize(fv3d_ctx *c)
     g.recon_scheme = f.recon_scheme;
     g.flux_scheme = f.flux_scheme;
     g.viscous = f.viscous ? 1 : 0;
+    g.turbulence = f.turbulence;
     g.re_ref = c->re_ref;
     g.pr_lam = (float)f.pr_lam;
     g.pr_tur = (float)f.pr_tur;
@@ -260,6 +262,13 @@ static int setup_block(fv3d_ctx *c, HostBlock &b, int index)
         if ((rc = alloc(&B.lamv, 3 * B.box))) return rc;
         if ((rc = alloc(&B.omega, B.box))) return rc;
     }
+    if (c->cfg.turbulence) {
+        if ((rc = alloc(&B.mut, B.box))) return rc;
+        int *f;
+        if ((rc = dalloc(c, &f, B.box))) return rc;
+        b.allocs.push_back(f);
+        B.blflag = f;
+    }
     if ((rc = alloc(&B.alpha, B.box))) return rc;
     {
         // static sweep metrics of each cell: averaged unit normal of its two
@@ -356,7 +365,8 @@ int fv3d_ctx_create(const fv3d_config *cfg, fv3d_ctx **out)
     if (cfg->recon_var != 100 && cfg->recon_var != 200) { *out = c; return fail(c, FV3D_EINVAL, "recon_var %d unsupported", cfg->recon_var); }
     if (cfg->recon_scheme < 100 || cfg->recon_scheme > 500 || cfg->recon_scheme % 100) { *out = c; return fail(c, FV3D_EINVAL, "recon_scheme %d unsupported", cfg->recon_scheme); }
     if (cfg->flux_scheme != 200 && cfg->flux_scheme != 300) { *out = c; return fail(c, FV3D_EINVAL, "flux_scheme %d unsupported", cfg->flux_scheme); }
-    if (cfg->turbulence) { *out = c; return fail(c, FV3D_EINVAL, "turbulence model %d not in this build", cfg->turbulence); }
+    if (cfg->turbulence != 0 && cfg->turbulence != 100) { *out = c; return fail(c, FV3D_EINVAL, "turbulence model %d not in this build", cfg->turbulence); }
+    if (cfg->turbulence && !cfg->viscous) { *out = c; return fail(c, FV3D_EINVAL, "a turbulence model needs the Navier-Stokes equations"); }
     if (cfg->global_dt) { *out = c; return fail(c, FV3D_EINVAL, "global time step not in this build"); }
     if (cfg->turbo_type == 0) { *out = c; return fail(c, FV3D_EINVAL, "axial (radial-equilibrium) outlet not in this build"); }
     if (cfg->precision != FV3D_PLAIN_FP64) { *out = c; return fail(c, FV3D_EINVAL, "precision mode %d not in this build", cfg->precision); }
@@ -486,7 +496,21 @@ int fv3d_setup(fv3d_ctx *c)
         if (P.kind == 1 && nout++ == 0) secs[1] = make_int2(P.block - 1, r[4]);
     }
     if (nin == 0 || nout == 0) return fail(c, FV3D_EINVAL, "need an inlet and an outlet patch");
+    if (c->cfg.turbulence == 100)
+        for (const HostPatch &P : c->patches) {
+            if (P.kind != 2) continue;
+            const HostBlock &hb = c->blocks[P.block - 1];
+            const int d = (P.face - 1) / 2, N = d == 0 ? hb.IM : d == 1 ? hb.JM : hb.KM;
+            // the reference's 1-D model runs over JM cells for K-face walls
+            // (source/BL_Model.cpp:225), which is defined only when JM == KM
+            if (d == 2 && hb.JM != hb.KM)
+                return fail(c, FV3D_EINVAL, "Baldwin-Lomax on a K-face wall needs JM == KM (block %d)", P.block);
+            if (N - 1 > 64 * kBlChunks)
+                return fail(c, FV3D_EINVAL, "Baldwin-Lomax lines are limited to %d cells (block %d)", 64 * kBlChunks,
+                            P.block);
+        }
     c->npatch_dev = (int)pd.size();
+    c->pd_host = pd;
     c->nbc = pst.back();
     if ((rc = dalloc(c, &c->d_patches, pd.size()))) return rc;
     HIPCK(c, hipMemcpyAsync(c->d_patches, pd.data(), pd.size() * sizeof(PatchDev), hipMemcpyHostToDevice, c->stream));
@@ -639,6 +663,28 @@ static int launch_viscous(fv3d_ctx *c)
     return FV3D_OK;
 }
 
+// Turbulence model before the time step (source/ComputeAllBlockVars.cpp:
+// 228-232): the vorticity, then Baldwin-Lomax wall by wall in table order.
+static int launch_turbulence(fv3d_ctx *c)
+{
+    if (c->cfg.turbulence != 100) return FV3D_OK;
+    const int nb = (int)c->blocks.size();
+    for (int n = 0; n < nb; ++n) {
+        const BlockDev &B = c->blocks[n].dev;
+        hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((B.box + 255) / 256)), dim3(256), 0, c->stream, B.blflag, B.box, 0);
+    }
+    hipLaunchKernelGGL(k_curl, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb);
+    for (int q = 0; q < (int)c->patches.size(); ++q) {
+        const HostPatch &P = c->patches[q];
+        if (P.kind != 2) continue;
+        const long lines = (long)c->pd_host[q].na * c->pd_host[q].nb;
+        hipLaunchKernelGGL(k_bl_wall, dim3((unsigned)((lines + 3) / 4)), dim3(256), 0, c->stream, c->d_blocks, c->d_patches,
+                           q, c->gas);
+    }
+    HIPCK(c, hipGetLastError());
+    return FV3D_OK;
+}
+
 // Vorticity magnitude (CFluidProps::Curl).  Only the turbulence models read it,
 // so the laminar iteration does not evaluate it; fv3d_curl does on request.
 static int launch_curl(fv3d_ctx *c)
@@ -762,6 +808,7 @@ int fv3d_resflux(fv3d_ctx *c)
     READY(c);
     int rc;
     if ((rc = launch_bc(c))) return rc;
+    if ((rc = launch_turbulence(c))) return rc;
     if ((rc = launch_spectral_dt(c))) return rc;
     if ((rc = launch_convective(c))) return rc;
     return launch_viscous(c);
@@ -786,6 +833,7 @@ static int lusgs_timed(fv3d_ctx *c)
     int rc;
     if (c->timing) HIPCK(c, hipEventRecord(c->ev[0], c->stream));
     if ((rc = launch_bc(c))) return rc;
+    if ((rc = launch_turbulence(c))) return rc;
     if ((rc = launch_spectral_dt(c))) return rc;
     if (c->timing) HIPCK(c, hipEventRecord(c->ev[1], c->stream));
     if ((rc = launch_convective(c))) return rc;
@@ -1015,6 +1063,14 @@ long fv3d_field_download(fv3d_ctx *c, int block, const char *name, double *out)
     std::vector<double> h;
     long cnt = 0;
     std::string nm(name);
+    if (nm == "mut") {
+        if (!B.mut) return fail(c, FV3D_EINVAL, "mut exists with a turbulence model only");
+        if (fetch(B.mut, B.box, h)) return FV3D_EHIP;
+        for (int i = 0; i <= B.IM; ++i)
+            for (int j = 0; j <= B.JM; ++j)
+                for (int k = 0; k <= B.KM; ++k) out[cnt++] = h[cidx(B, i, j, k)];
+        return cnt;
+    }
     if (nm == "cv" || nm == "pv") {
         int C = nm == "cv" ? 5 : 6;
         if (fetch(nm == "cv" ? B.Q : B.W, C * B.box, h)) return FV3D_EHIP;

@@ -54,6 +54,10 @@ struct BlockDev {
     const double *cg;
     double *lamv;
     double *omega;
+    // Baldwin-Lomax (null without a turbulence model): eddy viscosity over the
+    // cell box (ghost cells included) and the per-iteration visited flags
+    double *mut;
+    int *blflag;
     double *dt;               // local time step
     double *alpha;            // LU-SGS diagonal Vol/dt + relax*sum(lamc) + 2*sum(lamv)
     const double *swg;        // 15*box static sweep metrics per cell: for d = 0..2,
@@ -95,7 +99,8 @@ struct Gas {
     double inlet_dir[3];
     double c_timestep, relax;
     int recon_var, recon_scheme, flux_scheme;
-    int viscous;                  // laminar Navier-Stokes terms on
+    int viscous;                  // Navier-Stokes terms on
+    int turbulence;               // 0 laminar, 100 Baldwin-Lomax
     double re_ref, pr_lam, pr_tur;
 };
 

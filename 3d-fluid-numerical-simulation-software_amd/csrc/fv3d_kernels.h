@@ -94,5 +94,9 @@ template <int TI, int TJ, int TK>
 __global__ void k_viscous(const BlockDev *blocks, const int4 *tiles, Gas g, int store, ViscDebug dbg);
 // vorticity magnitude per cell
 __global__ void k_curl(const BlockDev *blocks, const long *cstart, int nblock);
+// Baldwin-Lomax along the grid lines normal to one wall patch, one wave per
+// line of up to 64*kBlChunks cells
+constexpr int kBlChunks = 8;
+__global__ void k_bl_wall(const BlockDev *blocks, const PatchDev *patches, int patch, Gas g);
 
 }  // namespace fv3d

@@ -178,7 +178,7 @@ __global__ void k_spectral_dt(const BlockDev *__restrict__ blocks, const long *_
     double lc = lam[0] + lam[1] + lam[2];
     double lv = 0.0, lvd[3] = {0.0, 0.0, 0.0};
     if (g.viscous) {
-        const double mue_lam = sutherland(g, B.W[5 * box + x]), mue_tur = 0.0, dens = B.W[x];
+        const double mue_lam = sutherland(g, B.W[5 * box + x]), mue_tur = B.mut ? B.mut[x] : 0.0, dens = B.W[x];
         const double a = (double)(4.0f / 3.0f) / dens, b = Gamma / dens;
         const double m = a > b ? a : b;
         for (int d = 0; d < 3; ++d) {

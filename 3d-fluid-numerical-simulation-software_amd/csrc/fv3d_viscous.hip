@@ -1,6 +1,7 @@
-// Laminar Navier-Stokes terms of the LU-SGS iteration for gfx950: the
+// Navier-Stokes terms of the LU-SGS iteration for gfx950: the
 // coordinate-transformation coefficients (set-up), the face gradients and
-// viscous face fluxes summed into the residual, and the vorticity magnitude.
+// viscous face fluxes summed into the residual, the vorticity magnitude and
+// the Baldwin-Lomax eddy viscosity.
 //
 // Arithmetic follows the reference's expression order (the library is built
 // with FP contraction off), so results match the CPU oracle to the last bits
@@ -206,7 +207,13 @@ __device__ static void face_fv(const double *sw, int l, int i, int j, int k, con
     const double tf = F32C(0.5) * (sw[3 * NH + l] + sw[3 * NH + l - st]);
     const double mue_lam = sutherland(g, tf);
     const double cond_lam = mue_lam * Gamma * Rgas / (Gamma - F32C(1.0)) / g.pr_lam;
-    const double mue = mue_lam + 0.0, cond = cond_lam + 0.0;
+    double mue_tur = 0.0, cond_tur = 0.0;
+    if (g.turbulence) {
+        // face average of the eddy viscosity (source/ViscousFlux.cpp:59-62)
+        mue_tur = (B.mut[x - (D == 0 ? B.si : D == 1 ? B.sj : 1)] + B.mut[x]) / F32C(2.0);
+        cond_tur = mue_tur * Gamma * Rgas / (Gamma - F32C(1.0)) / g.pr_tur;
+    }
+    const double mue = mue_lam + mue_tur, cond = cond_lam + cond_tur;
     const double c43 = (double)(4.0f / 3.0f), c23 = (double)(2.0f / 3.0f);
     const double *gx = gr[0], *gy = gr[1], *gz = gr[2];
     const double txx = mue * (c43 * gx[0] - c23 * (gy[1] + gz[2]));
@@ -331,6 +338,134 @@ __global__ void k_curl(const BlockDev *__restrict__ blocks, const long *__restri
     const double u_z = d[0][0] * gm[2] + d[1][0] * gm[5] + d[2][0] * gm[8];
     const double v_z = d[0][1] * gm[2] + d[1][1] * gm[5] + d[2][1] * gm[8];
     B.omega[x] = sqrt(F32C(0.5) * ((w_y - v_z) * (w_y - v_z) + (u_z - w_x) * (u_z - w_x) + (v_x - u_y) * (v_x - u_y)));
+}
+
+// ------------------------------------------------------------- Baldwin-Lomax
+// Wave reductions over the 64 lanes.
+__device__ static inline double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// CBlocks::BL_Model for one wall patch (source/BL_Model.cpp:8-279) with
+// CBlocks::BL_model_1d (283-329).  One wave per grid line normal to the wall;
+// lane l holds the cells n = 1 + l + 64 r along the line, n0 their distance
+// index from the wall (1 = the first cell).  The sequential maxima of the 1-D
+// model become wave reductions: tau_w and U_dif are plain maxima (NaN entries
+// never win, as with the reference's strict comparisons), and (F_max, y_max)
+// is the largest positive F with ties going to the smallest n0, which is the
+// first one the reference's loop meets.  A cell's first visit in this
+// iteration sets mue_t, later walls keep the minimum (walls run one launch
+// each, in table order).  Then the first and ghost cell of the line are set to
+// zero and the ghost gets the negated first-cell value, as WallFlux does
+// (source/WallFlux.cpp:106-110).
+__global__ void __launch_bounds__(256) k_bl_wall(const BlockDev *__restrict__ blocks, const PatchDev *__restrict__ patches,
+                                                 int patch, Gas g)
+{
+    constexpr int R = kBlChunks;
+    const PatchDev P = patches[patch];
+    const BlockDev &B = blocks[P.block];
+    const long line = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (line >= (long)P.na * P.nb) return;   // whole waves
+    const int lane = threadIdx.x & 63;
+    const int a = P.a0 + (int)(line / P.nb), cc = P.b0 + (int)(line % P.nb);
+    const int D = P.dir, hi = P.hi;
+    const int N = D == 0 ? B.IM : D == 1 ? B.JM : B.KM;   // cells 1..N-1 along the line
+    const long box = B.box;
+    auto cell = [&](int n) -> long { return D == 0 ? cidx(B, n, a, cc) : D == 1 ? cidx(B, a, n, cc) : cidx(B, a, cc, n); };
+    // wall face of the line: face index 1 or N
+    const long xw = cell(hi ? N : 1);
+    const double *FC = B.fc[D];
+    const double wx = FC[xw], wy = FC[box + xw], wz = FC[2 * box + xw];
+    double dis[R], dens[R], vv[R], om[R], ml[R];
+    int n0[R];
+    bool ok[R];
+    double tw = -1.0, ud = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int n = 1 + lane + 64 * r;
+        ok[r] = n <= N - 1;
+        n0[r] = hi ? N - n : n;
+        const long x = cell(ok[r] ? n : 1);
+        const double dx = B.cc[x] - wx, dy = B.cc[box + x] - wy, dz = B.cc[2 * box + x] - wz;
+        dis[r] = sqrt(pow(dx, 2.0) + pow(dy, 2.0) + pow(dz, 2.0));
+        dens[r] = B.W[x];
+        const double u = B.W[box + x], v = B.W[2 * box + x], w = B.W[3 * box + x];
+        vv[r] = sqrt(u * u + v * v + w * w);
+        om[r] = B.omega[x];
+        ml[r] = sutherland(g, B.W[5 * box + x]);
+        const double t = fabs(om[r] * ml[r]);
+        if (ok[r] && t > tw) tw = t;
+        if (ok[r] && vv[r] > ud) ud = vv[r];
+    }
+    // the first cell (n0 = 1) sits at n = 1 (low wall) or n = N-1 (high wall)
+    const int n1 = hi ? N - 1 : 1, l1 = (n1 - 1) & 63, r1 = (n1 - 1) >> 6;
+    double d1 = 0.0, m1 = 0.0, t1 = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (r == r1) {
+            d1 = __shfl(dens[r], l1);
+            m1 = __shfl(ml[r], l1);
+            t1 = __shfl(fabs(om[r] * ml[r]), l1);
+        }
+    double Tw = wave_max(tw);
+    if (t1 != t1) Tw = t1;   // a NaN first entry is kept by the reference's loop
+    const double Udif = wave_max(ud);
+    const double Ret = sqrt(d1 * Tw) / m1;
+    const double A_plus = F32C(26.0), Cwk = F32C(1.0), kap = F32C(0.4), Ckleb = F32C(0.3), alpha = F32C(0.0168),
+                 Ccp = F32C(1.6);
+    // largest positive F, smallest n0 on ties
+    double bF = 0.0, bY = 0.0;
+    int bN = 1 << 30;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const double y_plus = dis[r] * Ret;
+        const double F = dis[r] * fabs(om[r]) * (F32C(1.0) - exp(-y_plus / A_plus));
+        if (ok[r] && F > 0.0 && (F > bF || (F == bF && n0[r] < bN))) {
+            bF = F;
+            bY = dis[r];
+            bN = n0[r];
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double oF = __shfl_xor(bF, o), oY = __shfl_xor(bY, o);
+        const int oN = __shfl_xor(bN, o);
+        if (oF > bF || (oF == bF && oN < bN)) {
+            bF = oF;
+            bY = oY;
+            bN = oN;
+        }
+    }
+    const double Fmax = bF, y_max = bY;
+    const double wa = y_max * Fmax, wb = Cwk * y_max * Udif * Udif / Fmax;
+    const double Fwake = wa < wb ? wa : wb;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (!ok[r]) continue;
+        const double y_plus = Ret * dis[r];
+        const double bL = kap * dis[r] * (F32C(1.0) - exp(-y_plus / A_plus));
+        const double mue_in = dens[r] * bL * bL * om[r];
+        const double Fkleb = F32C(1.0) / (F32C(1.0) + F32C(5.5) * pow((Ckleb * dis[r] / y_max), 6.0));
+        const double mue_out = alpha * Ccp * dens[r] * Fwake * Fkleb;
+        const double mt = fabs(mue_in) > fabs(mue_out) ? mue_out : mue_in;
+        const long x = cell(1 + lane + 64 * r);
+        double nv;
+        if (B.blflag[x] == 0) {
+            B.blflag[x] = 1;
+            nv = mt;
+        } else {
+            const double cur = B.mut[x];
+            nv = cur < mt ? cur : mt;
+        }
+        // the first cell ends at zero (source/BL_Model.cpp:101-118)
+        B.mut[x] = n0[r] == 1 ? 0.0 : nv;
+    }
+    // ghost cell: zero, then the negated first-cell value WallFlux writes
+    if (lane == 0) B.mut[cell(hi ? N : 0)] = -0.0;
 }
 
 }  // namespace fv3d

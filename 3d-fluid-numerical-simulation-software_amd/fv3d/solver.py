@@ -126,6 +126,8 @@ class GpuSolver:
         ni, nj, nk = im - 1, jm - 1, km - 1
         if name in ("cv", "pv"):
             shape = (im + 1, jm + 1, km + 1, 5 if name == "cv" else 6)
+        elif name == "mut":
+            shape = (im + 1, jm + 1, km + 1)
         elif name in ("res", "dq", "fv"):
             shape = (ni, nj, nk, 5)
         elif name in ("fcI", "fcJ", "fcK", "fvI", "fvJ", "fvK"):

@@ -46,11 +46,11 @@ typedef struct {
     double inlet_dir[3];
     double cfl, c_timestep, relax;
     int turbo_type;     /* 0 axial, 1 centrifugal, 2 general outlet */
-    int viscous;        /* 0 Euler, 1 Navier-Stokes (laminar) */
+    int viscous;        /* 0 Euler, 1 Navier-Stokes */
     int recon_var;      /* 100 conservative, 200 primitive */
     int recon_scheme;   /* 100 NND, 200 MUSCL2u, 300 MUSCL2c, 400 MUSCL2o, 500 MUSCL3 */
     int flux_scheme;    /* 200 Steger-Warming, 300 Roe */
-    int turbulence;     /* 0 none (the only model in this build) */
+    int turbulence;     /* 0 none (laminar), 100 Baldwin-Lomax (source/BL_Model.cpp) */
     int limiter;        /* limit flow variables (source/Limit_FlowVars.cpp:8-119) */
     int global_dt;      /* 0 local time step */
     int device;         /* HIP device ordinal */
@@ -83,7 +83,8 @@ int fv3d_state_upload(fv3d_ctx *ctx, int block, const double *pv, const double *
  *   cv (IM+1,JM+1,KM+1,5), pv (IM+1,JM+1,KM+1,6)  -- cell arrays incl. ghost cells
  *   res dq (ni,nj,nk,5); dt lamcI lamcJ lamcK vol (ni,nj,nk)
  *   fcI (IM,nj,nk,5) fcJ (ni,JM,nk,5) fcK (ni,nj,KM,5)  -- face fluxes
- *   Navier-Stokes: lamvI lamvJ lamvK omega (ni,nj,nk); with debug bit 0 also
+ *   Navier-Stokes: lamvI lamvJ lamvK omega (ni,nj,nk); Baldwin-Lomax: mut
+ *   (IM+1,JM+1,KM+1) incl. ghost cells; with debug bit 0 also
  *   fvI fvJ fvK (viscous face fluxes, shapes as fcI..), gradI gradJ gradK
  *   (face gradients, 12: d/dx, d/dy, d/dz of u v w T) and fv (ni,nj,nk,5)
  *   dIS_cv dIP_cv (2,nj,nk,5), dJS_cv dJP_cv (ni,2,nk,5), dKS_cv dKP_cv (ni,nj,2,5)

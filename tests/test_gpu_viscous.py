@@ -1,14 +1,16 @@
-"""Laminar Navier-Stokes path of the HIP library against the CPU oracle.
+"""Navier-Stokes path (laminar and Baldwin-Lomax) of the HIP library against
+the CPU oracle.
 
-The oracle's viscous terms are pinned bit for bit to the compiled reference
-(test_oracle_golden.py, test_oracle_vs_ref.py) and to closed forms
-(test_oracle_viscous_laws.py).  Here the HIP kernels (metric coefficients,
-face gradients, viscous flux, viscous spectral radii, vorticity, and the
-viscous terms of the LU-SGS sweeps) are compared with it.  Tolerance: FP
-contraction is off in both, so values agree to the last bits except where the
-device pow (Sutherland's law, the MUSCL3 limiter) rounds differently from
-libm's; the bound is 1e-12 of each field's magnitude for one iteration and
-1e-9 for twenty.
+The oracle's viscous and turbulence terms are pinned bit for bit to the
+compiled reference (test_oracle_golden.py, test_oracle_vs_ref.py) and to closed
+forms (test_oracle_viscous_laws.py).  Here the HIP kernels (metric
+coefficients, face gradients, viscous flux, viscous spectral radii, vorticity,
+the viscous terms of the LU-SGS sweeps and the Baldwin-Lomax eddy viscosity)
+are compared with it.  Tolerance: FP contraction is off in both, so values
+agree to the last bits except where the device pow and exp (Sutherland's law,
+the wall distance, the Baldwin-Lomax damping, the MUSCL3 limiter) round
+differently from libm's; the bound is 1e-12 of each field's magnitude for one
+iteration and 1e-9 for twenty.
 """
 import json
 import os
@@ -124,3 +126,69 @@ def test_graph_replay_matches_eager():
     hb = np.concatenate([b.iterate(2, graph=True), b.iterate(4, graph=True)])
     np.testing.assert_array_equal(ha, hb)
     np.testing.assert_array_equal(a.get(1, "cv"), b.get(1, "cv"))
+
+
+# ------------------------------------------------------------- Baldwin-Lomax
+def bl_case(**kw):
+    args = dict(ni=7, nj=6, nk=10, seed=46, viscous=True, len_ref=1e-3, max_iter=20)
+    args.update(kw)
+    c = cases.with_wall_shear(cases.bump_channel(**args))
+    c.cfg.turbulence = 100
+    return c
+
+
+def bl_kwall_case():
+    from fv3d.refio import Patch
+    ni, nj, nk = 6, 8, 8
+    c = bl_case(ni=ni, nj=nj, nk=nk, seed=47, flux=200, recon_scheme=200)
+    pt = []
+    for p in c.patches:
+        if p.kind == "inlet":
+            pt.append(Patch("inlet", 1, 5, (1, ni, 1, nj - 2, 1, 1)))
+            pt.append(Patch("wall", 1, 5, (1, ni, nj - 1, nj, 1, 1)))
+        else:
+            pt.append(p)
+    c.patches = pt
+    return c
+
+
+@pytest.mark.parametrize("which", ["ij_walls", "k_wall"])
+def test_bl_first_iteration(which):
+    case = bl_case() if which == "ij_walls" else bl_kwall_case()
+    print(f"case BL first iteration {which}", flush=True)
+    o = O.Oracle(case)
+    o.call("lusgs")
+    g = GpuSolver(case)
+    g.call("lusgs")
+    assert np.max(np.abs(o.get(1, "mut"))) > 0.0
+    for f in ("mut", "lamvI", "lamvJ", "lamvK", "dt", "res", "dq", "cv", "pv"):
+        a, b = g.get(1, f), o.get(1, f)
+        assert relerr(a, b) <= 1e-12, f"{which} {f}: rel err {relerr(a, b):.3e}"
+
+
+def test_bl_history_against_reference_fixture(golden_dir):
+    z = np.load(os.path.join(golden_dir, "tiny_bl_roe_muscl3_prim.npz"))
+    meta = json.loads(bytes(z["meta"]).decode())
+    n = meta["iterations"]
+    case = cases.from_meta(meta)
+    print(f"case BL fixture n={n}", flush=True)
+    g = GpuSolver(case)
+    h = g.iterate(n)
+    ref = z["history"]
+    for col in range(1, 7):
+        assert relerr(h[:, col], ref[:, col]) <= 1e-9, f"history col {col}"
+    assert relerr(g.get(1, "cv")[1:-1, 1:-1, 1:-1], z["final_cv"][1:-1, 1:-1, 1:-1]) <= 1e-9
+    assert relerr(g.get(1, "mut").ravel(), z["final_mut"].ravel()) <= 1e-9
+
+
+def test_bl_row_sweep_equals_plane_sweep():
+    case = bl_case(ni=19, nj=13, nk=29, seed=48, max_iter=3)
+    print("case BL row vs plane sweep 19x13x29", flush=True)
+    a = GpuSolver(case)
+    b = GpuSolver(case, debug=2)
+    np.testing.assert_array_equal(a.iterate(3), b.iterate(3))
+    np.testing.assert_array_equal(a.get(1, "mut"), b.get(1, "mut"))
+    o = O.Oracle(case)
+    o.iterate(3)
+    assert relerr(a.get(1, "cv"), o.get(1, "cv")) <= 1e-11
+    assert relerr(a.get(1, "mut"), o.get(1, "mut")) <= 1e-10
