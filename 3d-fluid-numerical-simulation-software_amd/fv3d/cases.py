@@ -12,7 +12,7 @@ source/Flux_Split_Scheme.cpp:737-738).
 from __future__ import annotations
 
 import dataclasses
-from typing import Dict, List
+from typing import Dict, List, Sequence
 
 import numpy as np
 
@@ -147,12 +147,216 @@ def with_wall_shear(case: Case, delta: float = 0.3) -> Case:
 
 
 def from_meta(meta: Dict) -> Case:
-    """The case a fixture's metadata describes: ``kwargs`` for bump_channel and
-    the iteration count, optionally ``wall_shear`` (with_wall_shear) and
-    ``turbulence`` (the config's turbulence model code)."""
-    case = bump_channel(max_iter=meta["iterations"], **meta["kwargs"])
+    """The case a fixture's metadata describes: ``kwargs`` for bump_channel (or
+    for annular_passage when ``generator`` names it, with ``turn`` a list of
+    (block, quarter) re-indexings) and the iteration count, optionally
+    ``wall_shear`` (with_wall_shear) and ``turbulence`` (the config's
+    turbulence model code)."""
+    if meta.get("generator") == "annular_passage":
+        case = annular_passage(max_iter=meta["iterations"], **meta["kwargs"])
+        for blk, quarter in meta.get("turn", []):
+            turn_block(case, blk, quarter)
+    else:
+        case = bump_channel(max_iter=meta["iterations"], **meta["kwargs"])
     if meta.get("wall_shear"):
         case = with_wall_shear(case)
     if meta.get("turbulence"):
         case.cfg.turbulence = meta["turbulence"]
     return case
+
+
+def annular_passage(ni: int = 8, nj: int = 8, nk_blocks: Sequence[int] = (16,), seed: int = 1,
+                    hub: float = 0.2, tip: float = 0.25, nblade: int = 22, rpm: float = 0.0,
+                    z_extent: Sequence[float] = (), camber: float = 0.0, blade_blocks: Sequence[int] = (),
+                    beta: float = 1.5, cfl: float = 5.0, max_iter: int = 500, flux: int = 300,
+                    recon_scheme: int = 500, recon_var: int = 200, p_ratio: float = 0.95,
+                    viscous: bool = False, len_ref: float = 1.0, rotate_walls: bool = True,
+                    periodic_from_high: bool = False) -> Case:
+    """Annular-sector blade passage: i radial (hub I- to tip I+, tanh-clustered),
+    j pitchwise over one blade pitch 2*pi/nblade, k axial.  The passage is cut
+    along k into blocks of ``nk_blocks`` cells joined by interior patches
+    (K+ of block b to K- of block b+1, orientation 2).  In the blocks listed in
+    ``blade_blocks`` (0-based) the J faces are the two sides of a zero-thickness
+    circular-arc blade of pitchwise ``camber`` (as a fraction of the pitch) and
+    are walls; elsewhere J- and J+ of a block are periodic (orientation 2).
+    Hub and tip are walls (rotating with the block when ``rotate_walls``).
+    Inlet K- of the first block, outlet K+ of the last.  All blocks share one
+    rotation speed ``rpm`` and blade count.  ``periodic_from_high`` lists the
+    periodic pair from J+ to J- (its second side then lies at the smaller
+    angle, which makes the reference flip the block's pitch angle every time
+    it applies the condition, source/BndCondInteriorPeriodic.cpp:54-59)."""
+    nb = len(nk_blocks)
+    if not z_extent:
+        z_extent = [float(b) for b in range(nb + 1)]
+        z_extent = [0.1 * (z - nb / 2.0) for z in z_extent]
+    pitch = 2.0 * np.pi / nblade
+    r = hub + (tip - hub) * _tanh_cluster(ni, beta)
+    cfg = Config(grid_file="grid.dat", turbo_type="G", equation="N" if viscous else "E", len_ref=len_ref,
+                 cfl=cfl, max_iter=max_iter, flux_scheme=flux, recon_scheme=recon_scheme, recon_var=recon_var,
+                 p_out=p_ratio * 101325.0, n_threads=1)
+    nodes, patches, blocks, states = [], [], [], []
+    for b, nk in enumerate(nk_blocks):
+        im, jm, km = ni + 1, nj + 1, nk + 1
+        z = np.linspace(z_extent[b], z_extent[b + 1], km)
+        t = (z - z_extent[b]) / (z_extent[b + 1] - z_extent[b])
+        if b in blade_blocks and camber != 0.0:
+            # circular arc through (0, 0) and (1, 0) with mid-height camber
+            h = camber
+            rad = (0.25 + h * h) / (2.0 * h)
+            th0 = pitch * ((h - rad) + np.sqrt(np.maximum(rad * rad - (t - 0.5) ** 2, 0.0)))
+        else:
+            th0 = np.zeros(km)
+        th = th0[None, None, :] + pitch * np.linspace(0.0, 1.0, jm)[None, :, None]
+        xyz = np.empty((im, jm, km, 3))
+        rr = r[:, None, None]
+        xyz[..., 0] = rr * np.cos(th)
+        xyz[..., 1] = rr * np.sin(th)
+        xyz[..., 2] = z[None, None, :]
+        nodes.append(xyz)
+        bid = b + 1
+        blocks.append(BlockInfo(bid, im, jm, km, nblade, rpm))
+        rw = 1 if (rotate_walls and rpm != 0.0) else 0
+        patches.append(Patch("wall", bid, 1, (1, 1, 1, nj, 1, nk), rotate=rw))
+        patches.append(Patch("wall", bid, 2, (im, im, 1, nj, 1, nk), rotate=0))
+        if b in blade_blocks:
+            patches.append(Patch("wall", bid, 3, (1, ni, 1, 1, 1, nk), rotate=rw))
+            patches.append(Patch("wall", bid, 4, (1, ni, jm, jm, 1, nk), rotate=rw))
+        else:
+            lo, hi_ = (3, (1, ni, 1, 1, 1, nk)), (4, (1, ni, jm, jm, 1, nk))
+            a, c = (hi_, lo) if periodic_from_high else (lo, hi_)
+            patches.append(Patch("periodic", bid, a[0], a[1], orient=2, block2=bid, face2=c[0], rng2=c[1]))
+        if b == 0:
+            patches.append(Patch("inlet", bid, 5, (1, ni, 1, nj, 1, 1)))
+        else:
+            patches.append(Patch("interior", bid - 1, 6, (1, ni, 1, nj, nodes[b - 1].shape[2], nodes[b - 1].shape[2]),
+                                 orient=2, block2=bid, face2=5, rng2=(1, ni, 1, nj, 1, 1)))
+        if b == nb - 1:
+            patches.append(Patch("outlet", bid, 6, (1, ni, 1, nj, km, km)))
+        states.append(uniform_state(cfg, (ni, nj, nk), seed=seed + b))
+    # table order: inlets, outlets, walls, periodic, interior
+    order = {"inlet": 0, "outlet": 1, "wall": 2, "periodic": 3, "interior": 4}
+    patches.sort(key=lambda p: order[p.kind])
+    return Case(cfg, nodes, patches, blocks, states)
+
+
+def turn_block(case: Case, block: int, quarter: int) -> Case:
+    """Re-indexes block ``block`` (1-based) by a rotation of its (i, j) index
+    plane by ``quarter`` x 90 degrees, keeping the grid, the state and every
+    patch on it the same physical objects; handedness is preserved.  With
+    quarter 1 the new (i, j) = (j, NI-1-i), with 2 (NI-1-i, NJ-1-j), with 3
+    (NJ-1-j, i) (0-based cell indices).  Patch orientations on interfaces with
+    other blocks are recomputed for the reference's CMesh::Trans convention
+    (source/ShadowOutIn.cpp:166-325)."""
+    b = block - 1
+    q = quarter % 4
+    if q == 0:
+        return case
+    xyz = case.nodes[b]
+    im, jm = xyz.shape[:2]
+    ni, nj = im - 1, jm - 1
+
+    def rot_arr(a):
+        # a[i, j, ...] -> new array indexed by the new (i, j)
+        if q == 1:
+            return np.ascontiguousarray(np.transpose(a, (1, 0) + tuple(range(2, a.ndim)))[:, ::-1])
+        if q == 2:
+            return np.ascontiguousarray(a[::-1, ::-1])
+        return np.ascontiguousarray(np.transpose(a, (1, 0) + tuple(range(2, a.ndim)))[::-1, :])
+
+    case.nodes[b] = rot_arr(xyz)
+    case.states[b] = {k: rot_arr(v) for k, v in case.states[b].items()}
+    bi = case.blocks[b]
+    if q in (1, 3):
+        bi.im, bi.jm = bi.jm, bi.im
+    # old cell (i, j) (1-based) -> new; node index n (1..N) likewise as a face index
+    def cell(i, j):
+        if q == 1:
+            return j, ni + 1 - i
+        if q == 2:
+            return ni + 1 - i, nj + 1 - j
+        return nj + 1 - j, i
+
+    def face_map(face, rng):
+        d, hi = (face - 1) // 2, (face - 1) % 2
+        i0, i1, j0, j1, k0, k1 = rng
+        if d == 2:
+            a, c = cell(i0, j0), cell(i1, j1)
+            ri, rj = sorted((a[0], c[0])), sorted((a[1], c[1]))
+            return face, (ri[0], ri[1], rj[0], rj[1], k0, k1)
+        # the cells next to an I or J face, mapped; their new normal index is
+        # 1 (low side) or the new cell count (high side)
+        if d == 0:
+            ci = 1 if hi == 0 else ni
+            a, c = cell(ci, j0), cell(ci, j1)
+        else:
+            cj = 1 if hi == 0 else nj
+            a, c = cell(i0, cj), cell(i1, cj)
+        nim, njm = (nj, ni) if q in (1, 3) else (ni, nj)
+        nd = 1 - d if q in (1, 3) else d
+        if nd == 0:
+            newhi = 0 if a[0] == 1 else 1
+            rj = sorted((a[1], c[1]))
+            return 1 + newhi, (1 if newhi == 0 else nim + 1,) * 2 + (rj[0], rj[1], k0, k1)
+        newhi = 0 if a[1] == 1 else 1
+        ri = sorted((a[0], c[0]))
+        return 3 + newhi, (ri[0], ri[1]) + (1 if newhi == 0 else njm + 1,) * 2 + (k0, k1)
+
+    for p in case.patches:
+        if p.block == block:
+            p.face, p.rng = face_map(p.face, p.rng)
+        if p.kind in ("periodic", "interior", "mixing") and p.block2 == block:
+            p.face2, p.rng2 = face_map(p.face2, p.rng2)
+    for p in case.patches:
+        if p.kind in ("periodic", "interior") and block in (p.block, p.block2):
+            p.orient = _orientation(case, p)
+    return case
+
+
+def _shadow_axes(face: int):
+    """(first, second) tangential axes of the reference's shadow array of a face
+    (source/ShadowOutIn.cpp:12-164)."""
+    return {1: (2, 1), 2: (1, 2), 3: (0, 2), 4: (2, 0), 5: (1, 0), 6: (0, 1)}[face]
+
+
+def _orientation(case: Case, p: Patch) -> int:
+    """The CMesh::Trans code that maps side 1's shadow array onto side 2's for
+    a conformal interface, found by matching face-centre coordinates."""
+    def face_centres(block, face, rng):
+        xyz = case.nodes[block - 1]
+        d = (face - 1) // 2
+        al, am = _shadow_axes(face)
+        L = rng[2 * al + 1] - rng[2 * al] + 1
+        M = rng[2 * am + 1] - rng[2 * am] + 1
+        out = np.zeros((L, M, 3))
+        for l in range(L):
+            for m in range(M):
+                c = [0, 0, 0]
+                c[al] = rng[2 * al] + l
+                c[am] = rng[2 * am] + m
+                c[d] = rng[2 * d]
+                # face centre: mean of its four nodes (node indices, 1-based)
+                pts = []
+                for s in (0, 1):
+                    for t in (0, 1):
+                        n = list(c)
+                        n[al] += s
+                        n[am] += t
+                        pts.append(xyz[n[0] - 1, n[1] - 1, n[2] - 1])
+                out[l, m] = np.mean(pts, axis=0)
+        return out
+
+    a = face_centres(p.block, p.face, p.rng)
+    b = face_centres(p.block2, p.face2, p.rng2)
+    if p.kind == "periodic":
+        # compare in (r, z) and pitchwise order only
+        def rz(x):
+            return np.stack([np.hypot(x[..., 0], x[..., 1]), x[..., 2]], -1)
+        a, b = rz(a), rz(b)
+    L1, M1 = a.shape[:2]
+    cands = {1: lambda A: A[:, ::-1], 2: lambda A: np.transpose(A, (1, 0, 2)), 3: lambda A: A[::-1, :],
+             4: lambda A: np.transpose(A, (1, 0, 2))[::-1, ::-1]}
+    for o, f in cands.items():
+        t = f(a)
+        if t.shape == b.shape and np.allclose(t, b, rtol=0, atol=1e-9):
+            return o
+    raise ValueError(f"no orientation maps {p}")

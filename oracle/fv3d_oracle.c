@@ -33,9 +33,18 @@ typedef struct {
     double rot;
 } patch_t;
 
+/* two-sided patch of the boundary table: 0 interior, 1 periodic
+ * (source/UserInput.cpp:386-417); side 0 is block 1 of the row */
+typedef struct {
+    int kind, orient;
+    int blk[2], face[2], rng[2][6];
+    int flip;   /* periodic: theta of side 2 below theta of side 1 (source/BndCondInteriorPeriodic.cpp:54-59) */
+} conn_t;
+
 typedef struct {
     int IM, JM, KM, nblade;
     double w_rot, cellvol;
+    double delta_th;             /* blade pitch 2*PI/NBlade, sign flipped by the periodic exchange */
     v3_t *node;                  /* [1..IM][1..JM][1..KM] in a (IM+1)(JM+1)(KM+1) box */
     face_t *fi, *fj, *fk;        /* face boxes (IM+1)(JM+1)(KM+1) */
     cell_t *cell;
@@ -62,6 +71,8 @@ struct orc_ctx {
     int iter;
     int nb;
     blk_t blk[64];
+    conn_t conn[256];
+    int nconn;
 };
 
 static void curl(blk_t *b);
@@ -195,6 +206,7 @@ int orc_block_add(orc_ctx *c, int IM, int JM, int KM, const double *nodes, int n
     /* source/UserInput.cpp:444 and 1088 */
     b->w_rot = (double)(2.0f * 3.1415926535897932f) * rpm / F32(60.0);
     b->w_rot = b->w_rot / (c->nd.vel_ref / (double)(float)c->cfg.len_ref);
+    b->delta_th = (double)(2.0f * 3.1415926535897932f / (float)nblade);   /* source/UserInput.cpp:443 */
     long box = (long)(IM + 1) * (JM + 1) * (KM + 1);
     b->node = (v3_t *)calloc(box, sizeof(v3_t));
     for (int i = 1; i <= IM; ++i)
@@ -230,6 +242,29 @@ int orc_block_add(orc_ctx *c, int IM, int JM, int KM, const double *nodes, int n
         b->dpv[d] = (pv_t *)calloc(dsize(b, d), sizeof(pv_t));
     }
     return c->nb - 1;
+}
+
+int orc_connection_add(orc_ctx *c, int kind, int orient, int block1, int face1, const int *rng1, int block2, int face2,
+                       const int *rng2)
+{
+    if (c->nconn >= 256 || block1 < 1 || block1 > c->nb || block2 < 1 || block2 > c->nb) return -1;
+    if (orient < 1 || orient > 4 || face1 < 1 || face1 > 6 || face2 < 1 || face2 > 6) return -1;
+    for (int q = 0; q < 3; ++q)
+        if (rng1[2 * q] > rng1[2 * q + 1] || rng2[2 * q] > rng2[2 * q + 1]) return -1;
+    conn_t *t = &c->conn[c->nconn++];
+    t->kind = kind; t->orient = orient;
+    t->blk[0] = block1 - 1; t->blk[1] = block2 - 1;
+    t->face[0] = face1; t->face[1] = face2;
+    memcpy(t->rng[0], rng1, 6 * sizeof(int));
+    memcpy(t->rng[1], rng2, 6 * sizeof(int));
+    t->flip = 0;
+    if (kind == 1) {
+        const blk_t *A = &c->blk[block1 - 1], *B = &c->blk[block2 - 1];
+        v3_t na = A->node[C3(A, rng1[0], rng1[2], rng1[4])], nb = B->node[C3(B, rng2[0], rng2[2], rng2[4])];
+        double th_1 = atan(na.y / na.x), th_2 = atan(nb.y / nb.x);
+        t->flip = (th_2 - th_1) < 0.0;
+    }
+    return 0;
 }
 
 int orc_patch_add(orc_ctx *c, int kind, int block, int face, const int *rng, int rotate)
@@ -637,6 +672,106 @@ static void bc_wall(blk_t *b, const patch_t *P)
 }
 
 /* source/BoundaryCondition.cpp:9-63 */
+/* ------------------------------------------------ interior / periodic */
+/* Position (l, m) of the 2-D shadow array of face fc: the first and second
+ * tangential index it runs over (source/ShadowOutIn.cpp:12-164): i- (k, j),
+ * i+ (j, k), j- (i, k), j+ (k, i), k- (j, i), k+ (i, j). */
+static void shadow_axes(int fc, int *al, int *am)
+{
+    static const int L[6] = {2, 1, 0, 2, 1, 0}, Mx[6] = {1, 2, 2, 0, 0, 1};
+    *al = L[fc - 1];
+    *am = Mx[fc - 1];
+}
+
+/* cell index of layer nd next to face fc at shadow position (l, m) (1-based);
+ * nd = 0 is the ghost cell */
+static long shadow_cell(const blk_t *b, int fc, int nd, const int *r, int l, int m)
+{
+    int al, am, x[3];
+    shadow_axes(fc, &al, &am);
+    x[al] = r[2 * al] + l - 1;
+    x[am] = r[2 * am] + m - 1;
+    int d = (fc - 1) / 2, N = d == 0 ? b->IM : d == 1 ? b->JM : b->KM;
+    x[d] = fc % 2 == 1 ? nd : N - nd;
+    return C3(b, x[0], x[1], x[2]);
+}
+
+static void shadow_extent(int fc, const int *r, int *L, int *M)
+{
+    int al, am;
+    shadow_axes(fc, &al, &am);
+    *L = r[2 * al + 1] - r[2 * al] + 1;
+    *M = r[2 * am + 1] - r[2 * am] + 1;
+}
+
+/* CMesh::ShadowOut, Trans, ShadowIn (source/ShadowOutIn.cpp): layer nd of the
+ * cells of side s next to its face goes, re-indexed by the orientation and
+ * rotated by dr about z, into layer nd of the other side's Dummy slab (and its
+ * ghost cell for nd = 1). */
+static void shadow_copy(orc_ctx *c, const conn_t *t, int s, int nd, double dr)
+{
+    const blk_t *A = &c->blk[t->blk[s]];
+    blk_t *B = &c->blk[t->blk[1 - s]];
+    int fa = t->face[s], fb = t->face[1 - s];
+    int L1, M1;
+    shadow_extent(fa, t->rng[s], &L1, &M1);
+    int L2 = (t->orient == 2 || t->orient == 4) ? M1 : L1, M2 = (t->orient == 2 || t->orient == 4) ? L1 : M1;
+    for (int l1 = 1; l1 <= L1; ++l1)
+        for (int m1 = 1; m1 <= M1; ++m1) {
+            int l2, m2;
+            switch (t->orient) {
+            case 1: l2 = l1; m2 = M1 + 1 - m1; break;
+            case 2: l2 = m1; m2 = l1; break;
+            case 3: l2 = L1 + 1 - l1; m2 = m1; break;
+            default: l2 = M1 + 1 - m1; m2 = L1 + 1 - l1; break;
+            }
+            (void)L2; (void)M2;
+            long xs = shadow_cell(A, fa, nd, t->rng[s], l1, m1);
+            cv_t q = A->cv[xs];
+            pv_t w = A->pv[xs];
+            if (dr != 0.0) {
+                cv_t q0 = q;
+                pv_t w0 = w;
+                q.ru = q0.ru * cos(dr) - q0.rv * sin(dr);
+                q.rv = q0.rv * cos(dr) + q0.ru * sin(dr);
+                w.u = w0.u * cos(dr) - w0.v * sin(dr);
+                w.v = w0.v * cos(dr) + w0.u * sin(dr);
+            }
+            /* the target's own (l, m) axes */
+            int al, am, x[3];
+            shadow_axes(fb, &al, &am);
+            x[al] = t->rng[1 - s][2 * al] + l2 - 1;
+            x[am] = t->rng[1 - s][2 * am] + m2 - 1;
+            int d = (fb - 1) / 2;
+            x[d] = nd;
+            long xd = DI(B, fb - 1, x[0], x[1], x[2]);
+            B->dcv[fb - 1][xd] = q;
+            B->dpv[fb - 1][xd] = w;
+            if (nd == 1) {
+                long xg = shadow_cell(B, fb, 0, t->rng[1 - s], l2, m2);
+                B->cv[xg] = q;
+                B->pv[xg] = w;
+            }
+        }
+}
+
+/* CMesh::BndCondInterior / BndCondPeriodic (source/BndCondInteriorPeriodic.cpp) */
+static void bc_connections(orc_ctx *c)
+{
+    for (int kind = 0; kind < 2; ++kind)
+        for (int q = 0; q < c->nconn; ++q) {
+            const conn_t *t = &c->conn[q];
+            if (t->kind != kind) continue;
+            blk_t *A = &c->blk[t->blk[0]];
+            if (kind == 1 && t->flip) A->delta_th = F32(0.0) - A->delta_th;
+            double d1 = kind == 1 ? A->delta_th : 0.0, d2 = kind == 1 ? F32(0.0) - A->delta_th : 0.0;
+            for (int nd = 1; nd <= ND; ++nd) {
+                shadow_copy(c, t, 0, nd, d1);
+                shadow_copy(c, t, 1, nd, d2);
+            }
+        }
+}
+
 int orc_bc(orc_ctx *c)
 {
     for (int pass = 0; pass < 3; ++pass)
@@ -655,6 +790,7 @@ int orc_bc(orc_ctx *c)
                 }
             }
         }
+    bc_connections(c);
     return 0;
 }
 
@@ -1561,6 +1697,8 @@ int orc_res_calculate(orc_ctx *c, double *res6)
     double r[5] = {0, 0, 0, 0, 0};
     for (int n = 0; n < c->nb; ++n) {
         blk_t *b = &c->blk[n];
+        /* the OpenMP reduction sums each block from zero, then adds it to the total */
+        double t[5] = {0, 0, 0, 0, 0};
         for (int i = 1; i <= b->IM - 1; ++i)
             for (int j = 1; j <= b->JM - 1; ++j)
                 for (int k = 1; k <= b->KM - 1; ++k) {
@@ -1573,8 +1711,9 @@ int orc_res_calculate(orc_ctx *c, double *res6)
                     m.rw = fabs((o->rw - q->rw) / o->rw);
                     m.re = fabs((o->re - q->re) / o->re);
                     m = cscale(m, b->cell[x].vol);
-                    r[0] = r[0] + m.r; r[1] = r[1] + m.ru; r[2] = r[2] + m.rv; r[3] = r[3] + m.rw; r[4] = r[4] + m.re;
+                    t[0] = t[0] + m.r; t[1] = t[1] + m.ru; t[2] = t[2] + m.rv; t[3] = t[3] + m.rw; t[4] = t[4] + m.re;
                 }
+        for (int q = 0; q < 5; ++q) r[q] = r[q] + t[q];
     }
     for (int q = 0; q < 5; ++q) c->res[q] = r[q] / c->total_vol;
     c->res[5] = 0.0;

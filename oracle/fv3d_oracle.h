@@ -63,6 +63,11 @@ int orc_block_add(orc_ctx *c, int IM, int JM, int KM, const double *nodes, int n
  * boundary-definition file. */
 int orc_patch_add(orc_ctx *c, int kind, int block, int face, const int *rng, int rotate);
 
+/* Two-sided patch (source/UserInput.cpp:386-417): kind 0 interior, 1 periodic;
+ * orient 1..4 as in CMesh::Trans; ranges as in orc_patch_add, start <= end. */
+int orc_connection_add(orc_ctx *c, int kind, int orient, int block1, int face1, const int *rng1, int block2, int face2,
+                       const int *rng2);
+
 int orc_geometry(orc_ctx *c);
 
 /* Restart semantics of source/UserInput.cpp:1118-1159: pv is (ni,nj,nk,6)

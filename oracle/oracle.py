@@ -42,6 +42,8 @@ def lib():
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_block_add.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_double]
         L.orc_patch_add.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int]
+        L.orc_connection_add.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                         C.c_void_p]
         L.orc_set_state.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         for fn in ("orc_geometry", "orc_bc", "orc_spectral", "orc_timestep", "orc_convective", "orc_resflux",
                    "orc_lusgs"):
@@ -97,6 +99,13 @@ class Oracle:
             a = np.ascontiguousarray(xyz, dtype=np.float64)
             L.orc_block_add(self.h, bi.im, bi.jm, bi.km, a.ctypes.data, bi.nblade, bi.rpm)
         for p in case.patches:
+            if p.kind in ("interior", "periodic"):
+                r1 = np.ascontiguousarray(p.rng, dtype=np.int32)
+                r2 = np.ascontiguousarray(p.rng2, dtype=np.int32)
+                if L.orc_connection_add(self.h, 0 if p.kind == "interior" else 1, p.orient, p.block, p.face,
+                                        r1.ctypes.data, p.block2, p.face2, r2.ctypes.data) != 0:
+                    raise ValueError(f"oracle rejected {p}")
+                continue
             if p.kind not in _KIND:
                 raise NotImplementedError(f"oracle patch kind {p.kind}")
             r = np.ascontiguousarray(p.rng, dtype=np.int32)

@@ -44,6 +44,12 @@ TINY = {
     # Baldwin-Lomax from a state with wall shear layers
     "tiny_bl_roe_muscl3_prim": (dict(ni=7, nj=6, nk=10, seed=7, viscous=True, len_ref=1e-3), 20,
                                 {"turbulence": 100, "wall_shear": True}),
+    # three-block rotating blade passage: periodic and interior patches (the
+    # middle block turned so that the interfaces use orientations 1 and 3),
+    # cambered blade walls, Baldwin-Lomax
+    "tiny_passage3_bl": (dict(ni=6, nj=6, nk_blocks=(5, 8, 4), seed=8, rpm=6000.0, blade_blocks=(1,),
+                              camber=0.1, viscous=True, len_ref=1e-3), 12,
+                         {"generator": "annular_passage", "turn": [(2, 1)], "turbulence": 100, "wall_shear": True}),
 }
 NS_FIELDS_IT1 = ["omega", "gradI", "gradJ", "gradK", "fvI", "fvJ", "fvK", "fv", "lamvI", "lamvJ", "lamvK"]
 
@@ -61,10 +67,14 @@ def gen_tiny(name, kw, n_iter, extra=None):
         case.write(cdir)
         res = O.run_reference(cdir, n_iter, odir, threads=1, dump_first=True)
         arrs = {"history": res["history"]}
-        for f in FIELDS_IT1 + (NS_FIELDS_IT1 if kw.get("viscous") else []) + (["mut"] if meta.get("turbulence") else []):
-            arrs["it1_" + f] = np.load(os.path.join(odir, f"it1_b1_{f}.npy"))
-        for f in ("cv", "pv", "res", "dq") + (("mut",) if meta.get("turbulence") else ()):
-            arrs["final_" + f] = np.load(os.path.join(odir, f"final_b1_{f}.npy"))
+        for b in range(1, len(case.blocks) + 1):
+            tag = "" if b == 1 else f"b{b}_"
+            for f in FIELDS_IT1 + (NS_FIELDS_IT1 if kw.get("viscous") else []) + (["mut"] if meta.get("turbulence") else []):
+                arrs["it1_" + tag + f] = np.load(os.path.join(odir, f"it1_b{b}_{f}.npy"))
+        for b in range(1, len(case.blocks) + 1):
+            tag = "" if b == 1 else f"b{b}_"
+            for f in ("cv", "pv", "res", "dq") + (("mut",) if meta.get("turbulence") else ()):
+                arrs["final_" + tag + f] = np.load(os.path.join(odir, f"final_b{b}_{f}.npy"))
     arrs["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
     np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)
     print("wrote", name)

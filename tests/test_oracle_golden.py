@@ -4,8 +4,9 @@ The fixtures (tests/golden/*.npz) hold the reference's own outputs for seeded
 synthetic cases; the restatement must reproduce them bit for bit at one
 thread.  Cases cover Roe and Steger-Warming, MUSCL3 / NND / MUSCL2u / MUSCL2o
 reconstruction on primitive and conservative variables, the inlet,
-general-outlet and wall ghost fills, the laminar Navier-Stokes terms and the
-Baldwin-Lomax eddy viscosity.
+general-outlet and wall ghost fills, the laminar Navier-Stokes terms, the
+Baldwin-Lomax eddy viscosity, and a three-block rotating passage with periodic
+and interior patches.
 """
 import json
 import os
@@ -17,7 +18,7 @@ from fv3d import cases
 import oracle as O
 
 TINY = ["tiny_roe_muscl3_prim", "tiny_sw_nnd_prim", "tiny_roe_muscl3_cons", "tiny_roe_muscl2o_prim",
-        "tiny_sw_muscl2u_cons", "tiny_ns_roe_muscl3_prim", "tiny_bl_roe_muscl3_prim"]
+        "tiny_sw_muscl2u_cons", "tiny_ns_roe_muscl3_prim", "tiny_bl_roe_muscl3_prim", "tiny_passage3_bl"]
 NS_FIELDS = ("omega", "gradI", "gradJ", "gradK", "fvI", "fvJ", "fvK", "fv", "lamvI", "lamvJ", "lamvK")
 
 
@@ -39,10 +40,12 @@ def test_first_iteration_bitexact(golden_dir, name):
         fields += NS_FIELDS
     if meta.get("turbulence"):
         fields += ("mut",)
-    for f in fields:
-        ref = z["it1_" + f]
-        got = o.get(1, f).reshape(ref.shape)
-        assert np.array_equal(got, ref), f"{name}: {f} differs, max {np.max(np.abs(got - ref))}"
+    for b in range(1, len(case.blocks) + 1):
+        tag = "" if b == 1 else f"b{b}_"
+        for f in fields:
+            ref = z["it1_" + tag + f]
+            got = o.get(b, f).reshape(ref.shape)
+            assert np.array_equal(got, ref), f"{name} block {b}: {f} differs, max {np.max(np.abs(got - ref))}"
 
 
 @pytest.mark.parametrize("name", TINY)
@@ -54,10 +57,12 @@ def test_history_bitexact(golden_dir, name):
     ref = z["history"]
     # columns: iter res_dens mass_in mass_out PR eff CFL
     assert np.array_equal(h[:, :7], ref[:, :7])
-    assert np.array_equal(o.get(1, "cv"), z["final_cv"])
-    assert np.array_equal(o.get(1, "pv"), z["final_pv"])
-    if meta.get("turbulence"):
-        assert np.array_equal(o.get(1, "mut").ravel(), z["final_mut"].ravel())
+    for b in range(1, len(case.blocks) + 1):
+        tag = "" if b == 1 else f"b{b}_"
+        assert np.array_equal(o.get(b, "cv"), z["final_" + tag + "cv"])
+        assert np.array_equal(o.get(b, "pv"), z["final_" + tag + "pv"])
+        if meta.get("turbulence"):
+            assert np.array_equal(o.get(b, "mut").ravel(), z["final_" + tag + "mut"].ravel())
 
 
 def test_c0_first_iterations(golden_dir):

@@ -6,7 +6,10 @@ the committed fixtures (test_oracle_golden.py) carry the comparison.  The
 cases cover the paths the fixtures leave out: MUSCL2c reconstruction, the
 axial (radial-equilibrium) and centrifugal outlets, a rotating block with
 rotating walls, laminar Navier-Stokes with Steger-Warming/NND and with a
-rotating block, and Baldwin-Lomax with a K-face wall and with a rotating block.  The reference runs on one thread (its face loops race on a
+rotating block, Baldwin-Lomax with a K-face wall and with a rotating block,
+and three-block passages whose interior and periodic patches use all four
+orientations of the reference's shadow-array transfer, the flipped periodic
+pitch angle, and cambered blade walls.  The reference runs on one thread (its face loops race on a
 shared variable with more, source/Upwind_Convective.cpp:7,29).
 """
 import os
@@ -85,6 +88,18 @@ def variant(name):
         for p in c.patches:
             if p.kind == "wall" and p.face in (3, 1):
                 p.rotate = 1
+    elif name.startswith("passage_turn"):
+        # three-block passage, middle block re-indexed: the interfaces use
+        # orientations (3, 1), (4, 4) or (1, 3); turn 2 also makes the middle
+        # block's periodic pair run from the higher angle (the flipped pitch)
+        q = int(name[-1])
+        c = cases.annular_passage(ni=6, nj=8, nk_blocks=(5, 6, 4), max_iter=6, seed=30 + q, rpm=5000.0)
+        cases.turn_block(c, 2, q)
+    elif name == "passage_bl_blade":
+        c = cases.with_wall_shear(cases.annular_passage(ni=7, nj=6, nk_blocks=(4, 7, 5), max_iter=6, seed=34,
+                                                        rpm=8000.0, blade_blocks=(1,), camber=0.15, viscous=True,
+                                                        len_ref=1e-3, periodic_from_high=True))
+        c.cfg.turbulence = 100
     else:
         raise KeyError(name)
     return c
@@ -92,7 +107,8 @@ def variant(name):
 
 @pytest.mark.parametrize("name", ["muscl2c_axial", "nnd_cons_sw_centrifugal", "rotating_roe_muscl3",
                                   "ns_sw_nnd_cons", "ns_rotating_roe_muscl2o", "bl_sw_muscl2u_kwall",
-                                  "bl_rotating_roe_muscl3"])
+                                  "bl_rotating_roe_muscl3", "passage_turn1", "passage_turn2", "passage_turn3",
+                                  "passage_bl_blade"])
 def test_restatement_matches_reference(name):
     case = variant(name)
     n = case.cfg.max_iter
@@ -103,17 +119,19 @@ def test_restatement_matches_reference(name):
         ref = O.run_reference(cdir, n, odir, threads=1, dump_first=True, timeout=300)
         o = O.Oracle(case)
         o.call("lusgs")
-        for f in fields:
-            want = np.load(os.path.join(odir, f"it1_b1_{f}.npy"))
-            got = o.get(1, f).reshape(want.shape)
-            assert np.array_equal(got, want), f"{name}: {f}, max diff {np.max(np.abs(got - want))}"
+        for b in range(1, len(case.blocks) + 1):
+            for f in fields:
+                want = np.load(os.path.join(odir, f"it1_b{b}_{f}.npy"))
+                got = o.get(b, f).reshape(want.shape)
+                assert np.array_equal(got, want), f"{name} block {b}: {f}, max diff {np.max(np.abs(got - want))}"
         o2 = O.Oracle(case)
         h = o2.iterate(n)
         # (the rotating case reverses the outlet mass flow, which makes the
         # mass-averaged pressure ratio NaN on both sides)
         assert np.array_equal(h[:, :7], ref["history"][:, :7], equal_nan=True), name
-        want = np.load(os.path.join(odir, "final_b1_cv.npy"))
-        assert np.array_equal(o2.get(1, "cv"), want)
-        if case.cfg.turbulence:
-            want = np.load(os.path.join(odir, "final_b1_mut.npy"))
-            assert np.array_equal(o2.get(1, "mut").ravel(), want.ravel())
+        for b in range(1, len(case.blocks) + 1):
+            want = np.load(os.path.join(odir, f"final_b{b}_cv.npy"))
+            assert np.array_equal(o2.get(b, "cv"), want)
+            if case.cfg.turbulence:
+                want = np.load(os.path.join(odir, f"final_b{b}_mut.npy"))
+                assert np.array_equal(o2.get(b, "mut").ravel(), want.ravel())
