@@ -25,6 +25,12 @@ struct HostPatch {
     int rng[6];
 };
 
+// two-sided patch: 0 interior, 1 periodic (side 0 = block 1 of the table row)
+struct HostConn {
+    int kind, orient;
+    int block[2], face[2], rng[2][6];
+};
+
 struct HostBlock {
     int IM, JM, KM, nblade;
     double rpm, w_rot, cellvol;
@@ -45,11 +51,19 @@ struct fv3d_ctx {
     hipStream_t stream = nullptr;
     std::vector<HostBlock> blocks;
     std::vector<HostPatch> patches;
+    std::vector<HostConn> conns;
     bool ready = false;
     std::string err;
     // device work lists
     BlockDev *d_blocks = nullptr;
     PatchDev *d_patches = nullptr;
+    // interior / periodic exchange: element list, periodic flip table and the
+    // per-block pitch-angle sign (device state that persists over iterations)
+    ExchDev *d_exch = nullptr;
+    long nexch = 0;
+    PeriodicDev *d_per = nullptr;
+    int nper = 0;
+    int *d_pitch_sign = nullptr;
     std::vector<PatchDev> pd_host;   // the device patch list, host copy
     long *d_pstart_bc = nullptr, *d_cstart = nullptr, *d_pstart_plane = nullptr;
     int4 *d_tiles = nullptr;
@@ -353,6 +367,113 @@ static int setup_block(fv3d_ctx *c, HostBlock &b, int index)
     return 0;
 }
 
+// Interior and periodic patches (CMesh::BndCondInterior / BndCondPeriodic,
+// source/BndCondInteriorPeriodic.cpp, with ShadowOut / Trans / ShadowIn,
+// source/ShadowOutIn.cpp).  The reference copies layer nd of one side's cells
+// into a 2-D shadow array whose axes depend on the face type, re-indexes it by
+// the orientation and writes it into layer nd of the other side's Dummy slab
+// (and, for nd = 1, its ghost cells).  Here that index chain is resolved once
+// into a list of (source cell, target slot) elements.
+static void shadow_axes(int fc, int *al, int *am)
+{
+    static const int L[6] = {2, 1, 0, 2, 1, 0}, M[6] = {1, 2, 2, 0, 0, 1};
+    *al = L[fc - 1];
+    *am = M[fc - 1];
+}
+
+static int build_exchange(fv3d_ctx *c)
+{
+    std::vector<ExchDev> el;
+    std::vector<PeriodicDev> per;
+    // interiors first, then periodics (source/BoundaryCondition.cpp:46-58)
+    for (int kind = 0; kind < 2; ++kind)
+        for (const HostConn &t : c->conns) {
+            if (t.kind != kind) continue;
+            int pidx = -1;
+            if (kind == 1) {
+                // the pitch-angle flip test of source/BndCondInteriorPeriodic.cpp:54-59
+                const HostBlock &A = c->blocks[t.block[0] - 1], &B = c->blocks[t.block[1] - 1];
+                auto node = [](const HostBlock &b, int i, int j, int k, int q) {
+                    return b.nodes[(((long)(i - 1) * b.JM + (j - 1)) * b.KM + (k - 1)) * 3 + q];
+                };
+                double th_1 = atan(node(A, t.rng[0][0], t.rng[0][2], t.rng[0][4], 1) / node(A, t.rng[0][0], t.rng[0][2], t.rng[0][4], 0));
+                double th_2 = atan(node(B, t.rng[1][0], t.rng[1][2], t.rng[1][4], 1) / node(B, t.rng[1][0], t.rng[1][2], t.rng[1][4], 0));
+                PeriodicDev q;
+                q.block = t.block[0] - 1;
+                q.flip = (th_2 - th_1) < 0.0;
+                // delta_th = 2*PI/NBlade in float (source/UserInput.cpp:443)
+                const double d = (double)(2.0f * 3.1415926535897932f / (float)A.nblade);
+                q.cs[0] = cos(d); q.cs[1] = sin(d);
+                const double md = F32C(0.0) - d;
+                q.cs[2] = cos(md); q.cs[3] = sin(md);
+                pidx = (int)per.size();
+                per.push_back(q);
+            }
+            for (int s = 0; s < 2; ++s) {
+                const HostBlock &A = c->blocks[t.block[s] - 1], &B = c->blocks[t.block[1 - s] - 1];
+                const BlockDev &DA = A.dev, &DB = B.dev;
+                const int fa = t.face[s], fb = t.face[1 - s];
+                int al, am, bl, bm;
+                shadow_axes(fa, &al, &am);
+                shadow_axes(fb, &bl, &bm);
+                const int L1 = t.rng[s][2 * al + 1] - t.rng[s][2 * al] + 1, M1 = t.rng[s][2 * am + 1] - t.rng[s][2 * am] + 1;
+                const int L2 = (t.orient == 2 || t.orient == 4) ? M1 : L1, M2 = (t.orient == 2 || t.orient == 4) ? L1 : M1;
+                const int bL = t.rng[1 - s][2 * bl + 1] - t.rng[1 - s][2 * bl] + 1;
+                const int bM = t.rng[1 - s][2 * bm + 1] - t.rng[1 - s][2 * bm] + 1;
+                if (L2 != bL || M2 != bM)
+                    return fail(c, FV3D_EINVAL, "connection sides do not match under orientation %d", t.orient);
+                const int da = (fa - 1) / 2, db = (fb - 1) / 2;
+                const int NA = da == 0 ? A.IM : da == 1 ? A.JM : A.KM, NB = db == 0 ? B.IM : db == 1 ? B.JM : B.KM;
+                for (int nd = 1; nd <= kND; ++nd)
+                    for (int l1 = 1; l1 <= L1; ++l1)
+                        for (int m1 = 1; m1 <= M1; ++m1) {
+                            int l2, m2;
+                            switch (t.orient) {
+                            case 1: l2 = l1; m2 = M1 + 1 - m1; break;
+                            case 2: l2 = m1; m2 = l1; break;
+                            case 3: l2 = L1 + 1 - l1; m2 = m1; break;
+                            default: l2 = M1 + 1 - m1; m2 = L1 + 1 - l1; break;
+                            }
+                            int xs[3], xt[3];
+                            xs[al] = t.rng[s][2 * al] + l1 - 1;
+                            xs[am] = t.rng[s][2 * am] + m1 - 1;
+                            xs[da] = fa % 2 == 1 ? nd : NA - nd;
+                            xt[bl] = t.rng[1 - s][2 * bl] + l2 - 1;
+                            xt[bm] = t.rng[1 - s][2 * bm] + m2 - 1;
+                            ExchDev e;
+                            e.src_block = t.block[s] - 1;
+                            e.dst_block = t.block[1 - s] - 1;
+                            e.src = cidx(DA, xs[0], xs[1], xs[2]);
+                            e.side = fb - 1;
+                            e.layer = nd;
+                            // Dummy tangential position: I sides (j,k), J sides (i,k), K sides (i,j)
+                            const int ta = db == 0 ? xt[1] : xt[0], tb = db == 2 ? xt[1] : xt[2];
+                            e.slot = (long)ta * DB.Db[fb - 1] + tb;
+                            xt[db] = fb % 2 == 1 ? 0 : NB;
+                            e.ghost = nd == 1 ? cidx(DB, xt[0], xt[1], xt[2]) : -1;
+                            e.rot = kind == 1 ? (s == 0 ? 1 : 2) : 0;
+                            e.per = pidx;
+                            el.push_back(e);
+                        }
+            }
+        }
+    c->nexch = (long)el.size();
+    c->nper = (int)per.size();
+    int rc;
+    if (!el.empty()) {
+        if ((rc = dalloc(c, &c->d_exch, el.size()))) return rc;
+        HIPCK(c, hipMemcpyAsync(c->d_exch, el.data(), el.size() * sizeof(ExchDev), hipMemcpyHostToDevice, c->stream));
+    }
+    if (!per.empty()) {
+        if ((rc = dalloc(c, &c->d_per, per.size()))) return rc;
+        HIPCK(c, hipMemcpyAsync(c->d_per, per.data(), per.size() * sizeof(PeriodicDev), hipMemcpyHostToDevice, c->stream));
+        std::vector<int> sg(c->blocks.size(), 1);
+        if ((rc = dalloc(c, &c->d_pitch_sign, sg.size()))) return rc;
+        HIPCK(c, hipMemcpyAsync(c->d_pitch_sign, sg.data(), sg.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    }
+    return FV3D_OK;
+}
+
 // ------------------------------------------------------------------ API
 extern "C" {
 
@@ -416,6 +537,36 @@ int fv3d_block_add(fv3d_ctx *c, int IM, int JM, int KM, const double *nodes, int
     b.w_rot = b.w_rot / (c->vel_ref / c->len_ref);
     c->blocks.push_back(std::move(b));
     return (int)c->blocks.size();
+}
+
+int fv3d_connection_add(fv3d_ctx *c, int kind, int orient, int block1, int face1, const int *rng1, int block2,
+                        int face2, const int *rng2)
+{
+    if (!c || !rng1 || !rng2) return FV3D_EINVAL;
+    if (c->ready) return fail(c, FV3D_ESTATE, "connection_add after setup");
+    if (kind != 0 && kind != 1) return fail(c, FV3D_EINVAL, "connection kind %d (0 interior, 1 periodic)", kind);
+    if (orient < 1 || orient > 4) return fail(c, FV3D_EINVAL, "orientation %d (1..4)", orient);
+    HostConn t;
+    t.kind = kind; t.orient = orient;
+    const int blk[2] = {block1, block2}, fc[2] = {face1, face2};
+    const int *rg[2] = {rng1, rng2};
+    for (int s = 0; s < 2; ++s) {
+        if (blk[s] < 1 || blk[s] > (int)c->blocks.size()) return fail(c, FV3D_EINVAL, "connection block %d", blk[s]);
+        if (fc[s] < 1 || fc[s] > 6) return fail(c, FV3D_EINVAL, "connection face %d", fc[s]);
+        const HostBlock &b = c->blocks[blk[s] - 1];
+        int d = (fc[s] - 1) / 2, hi = (fc[s] - 1) % 2;
+        int N[3] = {b.IM, b.JM, b.KM};
+        if (rg[s][2 * d] != rg[s][2 * d + 1] || rg[s][2 * d] != (hi ? N[d] : 1))
+            return fail(c, FV3D_EINVAL, "connection normal range does not match face %d", fc[s]);
+        for (int e = 0; e < 3; ++e)
+            if (e != d && (rg[s][2 * e] < 1 || rg[s][2 * e + 1] > N[e] - 1 || rg[s][2 * e] > rg[s][2 * e + 1]))
+                return fail(c, FV3D_EINVAL, "connection tangential range %d..%d outside 1..%d", rg[s][2 * e],
+                            rg[s][2 * e + 1], N[e] - 1);
+        t.block[s] = blk[s]; t.face[s] = fc[s];
+        memcpy(t.rng[s], rg[s], sizeof t.rng[s]);
+    }
+    c->conns.push_back(t);
+    return FV3D_OK;
 }
 
 int fv3d_patch_add(fv3d_ctx *c, int kind, int block, int face, const int *rng, int rotate)
@@ -511,6 +662,7 @@ int fv3d_setup(fv3d_ctx *c)
         }
     c->npatch_dev = (int)pd.size();
     c->pd_host = pd;
+    if ((rc = build_exchange(c))) return rc;
     c->nbc = pst.back();
     if ((rc = dalloc(c, &c->d_patches, pd.size()))) return rc;
     HIPCK(c, hipMemcpyAsync(c->d_patches, pd.data(), pd.size() * sizeof(PatchDev), hipMemcpyHostToDevice, c->stream));
@@ -627,9 +779,14 @@ int fv3d_state_upload(fv3d_ctx *c, int block, const double *pv, const double *en
 // ------------------------------------------------------------- stages
 static int launch_bc(fv3d_ctx *c)
 {
-    if (c->nbc == 0) return FV3D_OK;
-    hipLaunchKernelGGL(k_bc, dim3((unsigned)((c->nbc + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks, c->d_patches,
-                       c->npatch_dev, c->d_pstart_bc, c->gas, &c->d_sc->p_out);
+    if (c->nbc > 0)
+        hipLaunchKernelGGL(k_bc, dim3((unsigned)((c->nbc + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks, c->d_patches,
+                           c->npatch_dev, c->d_pstart_bc, c->gas, &c->d_sc->p_out);
+    if (c->nper > 0)
+        hipLaunchKernelGGL(k_periodic_flip, dim3(1), dim3(64), 0, c->stream, c->d_per, c->nper, c->d_pitch_sign);
+    if (c->nexch > 0)
+        hipLaunchKernelGGL(k_exchange, dim3((unsigned)((c->nexch + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
+                           c->d_exch, c->nexch, c->d_per);
     HIPCK(c, hipGetLastError());
     return FV3D_OK;
 }

@@ -33,6 +33,23 @@ struct ScalarsDev {
 
 __global__ void k_bc(const BlockDev *blocks, const PatchDev *patches, int npatch, const long *pstart, Gas g,
                      const double *pout_dev);
+// one interior/periodic transfer: the state of cell src of block src_block goes
+// to layer `layer` of Dummy slab `side` of block dst_block at tangential slot
+// `slot` (and to ghost cell `ghost` of that block, -1 for none); rot 0 copies,
+// 1 and 2 rotate by the pitch angle of periodic patch `per` with sign +/-
+struct ExchDev {
+    int src_block, dst_block, side, layer, rot, per;
+    long src, slot, ghost;
+};
+// periodic patch: the block whose pitch angle it uses, whether applying it
+// flips that angle's sign (source/BndCondInteriorPeriodic.cpp:54-59), the sign
+// in force for this application, and cos/sin of +delta and -delta
+struct PeriodicDev {
+    int block, flip, sign;
+    double cs[4];
+};
+__global__ void k_periodic_flip(PeriodicDev *per, int nper, int *pitch_sign);
+__global__ void k_exchange(const BlockDev *blocks, const ExchDev *el, long n, const PeriodicDev *per);
 __global__ void k_spectral_dt(const BlockDev *blocks, const long *cstart, int nblock, Gas g, const double *cfl_dev);
 __global__ void k_dt_flag_decrement(const BlockDev *blocks, int nblock);
 template <int TI, int TJ, int TK>

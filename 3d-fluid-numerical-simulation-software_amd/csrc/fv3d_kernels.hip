@@ -139,6 +139,50 @@ __global__ void k_bc(const BlockDev *__restrict__ blocks, const PatchDev *__rest
     set_dummies(B, s, a, b, cv, pv);
 }
 
+// ================================================== interior / periodic
+// The pitch-angle sign of each periodic patch's block, updated in table order
+// (source/BndCondInteriorPeriodic.cpp:54-59); one thread, so patches that share
+// a block see each other's flips in order.
+__global__ void k_periodic_flip(PeriodicDev *__restrict__ per, int nper, int *__restrict__ pitch_sign)
+{
+    if (threadIdx.x != 0) return;
+    for (int p = 0; p < nper; ++p) {
+        if (per[p].flip) pitch_sign[per[p].block] = -pitch_sign[per[p].block];
+        per[p].sign = pitch_sign[per[p].block];
+    }
+}
+
+// One transfer of source/ShadowOutIn.cpp: Trans rotates the momentum and the
+// velocity about z by dr when dr != 0 (lines 188-199).
+__global__ void k_exchange(const BlockDev *__restrict__ blocks, const ExchDev *__restrict__ el, long n,
+                           const PeriodicDev *__restrict__ per)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const ExchDev e = el[t];
+    const BlockDev &A = blocks[e.src_block], &B = blocks[e.dst_block];
+    double q[5], w[6];
+    for (int c = 0; c < 5; ++c) q[c] = A.Q[c * A.box + e.src];
+    for (int c = 0; c < 6; ++c) w[c] = A.W[c * A.box + e.src];
+    if (e.rot) {
+        const PeriodicDev &P = per[e.per];
+        const bool pos = (e.rot == 1) == (P.sign > 0);   // dr = +delta or -delta
+        const double cr = pos ? P.cs[0] : P.cs[2], sr = pos ? P.cs[1] : P.cs[3];
+        const double qx = q[1], qy = q[2], ux = w[1], uy = w[2];
+        q[1] = qx * cr - qy * sr;
+        q[2] = qy * cr + qx * sr;
+        w[1] = ux * cr - uy * sr;
+        w[2] = uy * cr + ux * sr;
+    }
+    double *D = B.D[e.side] + (long)(e.layer - 1) * 11 * B.Dt[e.side] + e.slot;
+    for (int c = 0; c < 5; ++c) D[c * B.Dt[e.side]] = q[c];
+    for (int c = 0; c < 6; ++c) D[(5 + c) * B.Dt[e.side]] = w[c];
+    if (e.ghost >= 0) {
+        for (int c = 0; c < 5; ++c) B.Q[c * B.box + e.ghost] = q[c];
+        for (int c = 0; c < 6; ++c) B.W[c * B.box + e.ghost] = w[c];
+    }
+}
+
 // ======================================================= spectral + dt
 // source/Spectral_curl_grad.cpp:9-42 and source/TimeStep.cpp:7-62, with the
 // viscous spectral radii for Navier-Stokes (TimeStep.cpp:38-49).

@@ -55,6 +55,13 @@ class GpuSolver:
             rc = self.L.fv3d_block_add(self.h, bi.im, bi.jm, bi.km, a.ctypes.data, bi.nblade, bi.rpm)
             self._check(rc, "block_add")
         for p in case.patches:
+            if p.kind in ("interior", "periodic"):
+                r1 = np.ascontiguousarray(p.rng, dtype=np.int32)
+                r2 = np.ascontiguousarray(p.rng2, dtype=np.int32)
+                self._check(self.L.fv3d_connection_add(self.h, 0 if p.kind == "interior" else 1, p.orient, p.block,
+                                                       p.face, r1.ctypes.data, p.block2, p.face2, r2.ctypes.data),
+                            "connection_add")
+                continue
             if p.kind not in _KIND:
                 raise NotImplementedError(f"patch kind {p.kind} not in this build")
             r = np.ascontiguousarray(p.rng, dtype=np.int32)

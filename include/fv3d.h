@@ -71,6 +71,15 @@ int fv3d_block_add(fv3d_ctx *ctx, int IM, int JM, int KM, const double *nodes, i
  * kind 0 inlet, 1 outlet, 2 wall; rng = is ie js je ks ke. */
 int fv3d_patch_add(fv3d_ctx *ctx, int kind, int block, int face, const int *rng, int rotate);
 
+/* Adds an interior (kind 0) or periodic (kind 1) patch between face1 of block1
+ * and face2 of block2 (CUserInput::ReadBndGrid, source/UserInput.cpp:386-417;
+ * applied by CMesh::BndCondInterior / BndCondPeriodic,
+ * source/BndCondInteriorPeriodic.cpp).  orient 1..4 is the reference's
+ * CMesh::Trans code (source/ShadowOutIn.cpp:166-325); ranges as in
+ * fv3d_patch_add.  A periodic patch rotates by the pitch 2*PI/NBlade of block1. */
+int fv3d_connection_add(fv3d_ctx *ctx, int kind, int orient, int block1, int face1, const int *rng1, int block2,
+                        int face2, const int *rng2);
+
 /* Metrics (CGeometry::FaceVectorVolume, source/GeometryVariables.cpp:10-105),
  * device allocation and launch plans.  Must follow the last block/patch add. */
 int fv3d_setup(fv3d_ctx *ctx);

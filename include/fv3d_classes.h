@@ -283,6 +283,13 @@ public:
     {
         check(ctx_, fv3d_patch_add(ctx_, kind, block, face, rng, rotate), "fv3d_patch_add");
     }
+    // Interior (kind 0) or periodic (kind 1) patch (CUserInput::ReadBndGrid).
+    void AddConnection(int kind, int orient, int block1, int face1, const int rng1[6], int block2, int face2,
+                       const int rng2[6])
+    {
+        check(ctx_, fv3d_connection_add(ctx_, kind, orient, block1, face1, rng1, block2, face2, rng2),
+              "fv3d_connection_add");
+    }
     // CMesh::Geometry: metrics and device set-up.
     void Geometry() { check(ctx_, fv3d_setup(ctx_), "fv3d_setup"); }
     // CMesh::Initial from a restart solution.
@@ -396,8 +403,11 @@ inline void LoadCase(CMesh &mesh, const std::string &dir, const BndTable &bnd,
         mesh.AddBlock(dims[b][0], dims[b][1], dims[b][2], nodes[b], nblade, rpm);
     }
     for (const PatchRow &p : bnd.patches) {
-        if (p.kind > 2) throw Error(FV3D_EINVAL, "two-sided patches (periodic/interior/mixing) not in this build");
-        mesh.AddPatch(p.kind, p.block, p.face, p.rng, p.rotate);
+        if (p.kind == 5) throw Error(FV3D_EINVAL, "mixing-plane patches not in this build");
+        if (p.kind == 3 || p.kind == 4)
+            mesh.AddConnection(p.kind == 3 ? 1 : 0, p.orient, p.block, p.face, p.rng, p.block2, p.face2, p.rng2);
+        else
+            mesh.AddPatch(p.kind, p.block, p.face, p.rng, p.rotate);
     }
     mesh.Geometry();
     for (size_t b = 0; b < states.size(); ++b) mesh.Initial((int)b + 1, states[b]);
