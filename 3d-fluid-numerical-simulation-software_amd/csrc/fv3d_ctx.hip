@@ -490,7 +490,7 @@ int fv3d_ctx_create(const fv3d_config *cfg, fv3d_ctx **out)
     if (cfg->turbulence && !cfg->viscous) { *out = c; return fail(c, FV3D_EINVAL, "a turbulence model needs the Navier-Stokes equations"); }
     if (cfg->global_dt) { *out = c; return fail(c, FV3D_EINVAL, "global time step not in this build"); }
     if (cfg->turbo_type == 0) { *out = c; return fail(c, FV3D_EINVAL, "axial (radial-equilibrium) outlet not in this build"); }
-    if (cfg->precision != FV3D_PLAIN_FP64) { *out = c; return fail(c, FV3D_EINVAL, "precision mode %d not in this build", cfg->precision); }
+    if (cfg->precision != FV3D_PLAIN_FP64 && cfg->precision != FV3D_DEFECT_FP32) { *out = c; return fail(c, FV3D_EINVAL, "precision mode %d", cfg->precision); }
     c->device = cfg->device;
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) { *out = c; return fail(c, FV3D_EHIP, "hipSetDevice(%d): %s", c->device, hipGetErrorString(e)); }
@@ -619,8 +619,12 @@ int fv3d_setup(fv3d_ctx *c)
             HIPCK(c, hipGetLastError());
         }
         const long rows = c->blocks[n].sweep_rows;
-        hipLaunchKernelGGL(k_sweep_fill, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, c->stream, c->d_blocks, n, rows,
-                           kGatherState | kGatherMetrics);
+        if (c->cfg.precision == FV3D_DEFECT_FP32)
+            hipLaunchKernelGGL(k_sweep_fill<float>, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, c->stream, c->d_blocks,
+                               n, rows, kGatherState | kGatherMetrics);
+        else
+            hipLaunchKernelGGL(k_sweep_fill<double>, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, c->stream, c->d_blocks,
+                               n, rows, kGatherState | kGatherMetrics);
         HIPCK(c, hipGetLastError());
     }
     // patch work list
@@ -861,32 +865,54 @@ static int launch_sweeps_kba(fv3d_ctx *c)
     int *err = c->d_kba_sync, *tf = c->d_kba_sync + 32, *tb = c->d_kba_sync + 48;
     int *df = c->d_kba_sync + 64, *db = df + (long)c->nkba * kDonePad;
     if (c->sweep_mode == 4) {
+        const bool f32 = c->cfg.precision == FV3D_DEFECT_FP32;
+        const bool visc = c->cfg.viscous;
         const int nb = (int)c->blocks.size();
         for (int n = 0; n < nb; ++n) {
             const HostBlock &hb = c->blocks[n];
             const long nw = 2 * hb.sweep_rows * kMboxRow;
-            hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, c->stream,
-                               (unsigned long long *)hb.dev.mbox[0], nw, kMboxEmpty);
-            hipLaunchKernelGGL(k_sweep_pack, dim3((unsigned)hb.pack_groups), dim3(256), 0, c->stream, c->d_blocks, n);
+            if (f32)
+                hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, c->stream,
+                                   (int *)hb.dev.mbox[0], nw, (int)kMboxEmpty32);
+            else
+                hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, c->stream,
+                                   (unsigned long long *)hb.dev.mbox[0], nw, kMboxEmpty);
+            if (f32)
+                hipLaunchKernelGGL(k_sweep_pack<float>, dim3((unsigned)hb.pack_groups), dim3(256), 0, c->stream, c->d_blocks, n);
+            else
+                hipLaunchKernelGGL(k_sweep_pack<double>, dim3((unsigned)hb.pack_groups), dim3(256), 0, c->stream, c->d_blocks, n);
         }
+        // forward and backward sweeps in the operator's precision, with or
+        // without the viscous terms
+        auto sweep = [&](bool fwd) {
+            int *tk = fwd ? tf : tb;
+            const KbaTile *tl = c->d_kba[fwd ? 0 : 1];
+            dim3 g((unsigned)c->kba_grid), b(256);
+#define FV3D_SL_LAUNCH(F, V, T) \
+    hipLaunchKernelGGL((k_lusgs_sl<F, kKbaTJ, kKbaTK, V, T>), g, b, 0, c->stream, c->d_blocks, tl, c->nkba, tk, err, \
+                       c->gas.relax, c->gas)
+            if (fwd) {
+                if (f32) { if (visc) FV3D_SL_LAUNCH(true, true, float); else FV3D_SL_LAUNCH(true, false, float); }
+                else { if (visc) FV3D_SL_LAUNCH(true, true, double); else FV3D_SL_LAUNCH(true, false, double); }
+            } else {
+                if (f32) { if (visc) FV3D_SL_LAUNCH(false, true, float); else FV3D_SL_LAUNCH(false, false, float); }
+                else { if (visc) FV3D_SL_LAUNCH(false, true, double); else FV3D_SL_LAUNCH(false, false, double); }
+            }
+#undef FV3D_SL_LAUNCH
+        };
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[6], c->stream));
-        if (c->cfg.viscous)
-            hipLaunchKernelGGL((k_lusgs_sl<true, kKbaTJ, kKbaTK, true>), dim3((unsigned)c->kba_grid), dim3(256), 0,
-                               c->stream, c->d_blocks, c->d_kba[0], c->nkba, tf, err, c->gas.relax, c->gas);
-        else
-            hipLaunchKernelGGL((k_lusgs_sl<true, kKbaTJ, kKbaTK, false>), dim3((unsigned)c->kba_grid), dim3(256), 0,
-                               c->stream, c->d_blocks, c->d_kba[0], c->nkba, tf, err, c->gas.relax, c->gas);
+        sweep(true);
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[7], c->stream));
-        if (c->cfg.viscous)
-            hipLaunchKernelGGL((k_lusgs_sl<false, kKbaTJ, kKbaTK, true>), dim3((unsigned)c->kba_grid), dim3(256), 0,
-                               c->stream, c->d_blocks, c->d_kba[1], c->nkba, tb, err, c->gas.relax, c->gas);
-        else
-            hipLaunchKernelGGL((k_lusgs_sl<false, kKbaTJ, kKbaTK, false>), dim3((unsigned)c->kba_grid), dim3(256), 0,
-                               c->stream, c->d_blocks, c->d_kba[1], c->nkba, tb, err, c->gas.relax, c->gas);
+        sweep(false);
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[8], c->stream));
-        for (int n = 0; n < nb; ++n)
-            hipLaunchKernelGGL(k_sweep_unpack, dim3((unsigned)c->blocks[n].pack_groups), dim3(256), 0, c->stream, c->d_blocks,
-                               n);
+        for (int n = 0; n < nb; ++n) {
+            if (f32)
+                hipLaunchKernelGGL(k_sweep_unpack<float>, dim3((unsigned)c->blocks[n].pack_groups), dim3(256), 0, c->stream,
+                                   c->d_blocks, n);
+            else
+                hipLaunchKernelGGL(k_sweep_unpack<double>, dim3((unsigned)c->blocks[n].pack_groups), dim3(256), 0, c->stream,
+                                   c->d_blocks, n);
+        }
     } else if (c->sweep_mode == 3) {
         hipLaunchKernelGGL((k_lusgs_kba3<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
                            c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_lag);
@@ -1181,6 +1207,10 @@ int fv3d_set_debug(fv3d_ctx *c, int flags)
     if (c->cfg.viscous && c->sweep_mode >= 1 && c->sweep_mode <= 3) {
         c->sweep_mode = 4;
         return fail(c, FV3D_EINVAL, "the viscous terms run with the default sweep form or the per-hyperplane form");
+    }
+    if (c->cfg.precision == FV3D_DEFECT_FP32 && c->sweep_mode != 4) {
+        c->sweep_mode = 4;
+        return fail(c, FV3D_EINVAL, "the FP32 operator runs with the default sweep form only");
     }
     if ((flags >> 8) & 0xff) c->kba_chunk = (flags >> 8) & 0xff;
     c->kba_lag = (flags >> 16) & 0xff;

@@ -78,16 +78,22 @@ __global__ void k_lusgs_kba2(const BlockDev *blocks, const KbaTile *tiles, int n
 template <bool FWD, int TJ, int TK>
 __global__ void k_lusgs_kba3(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *done, int *err,
                              double relax, Gas g, int lag);
-template <bool FWD, int TJ, int TK, bool VISC>
+// T: double (plain FP64) or float (the FP32 operator of defect correction)
+template <bool FWD, int TJ, int TK, bool VISC, class T>
 __global__ void k_lusgs_sl(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *err, double relax,
                            Gas g);
 // mailbox words per tile row, and the bit pattern of an empty word (a NaN
 // payload no arithmetic produces)
 constexpr int kMboxJ = 5 * kKbaTK, kMboxRow = 5 * (kKbaTK + kKbaTJ);
 constexpr unsigned long long kMboxEmpty = 0x7ff4dead5eed0001ull;
+constexpr unsigned int kMboxEmpty32 = 0x7fa5eed1u;   // the same for FP32 words
 __global__ void k_fill_u64(unsigned long long *p, long n, unsigned long long v);
+// row storage in T (double, or float for the FP32 operator); unpack widens dQ to double
+template <class T>
 __global__ void k_sweep_fill(const BlockDev *blocks, int block, long nrows, int what);
+template <class T>
 __global__ void k_sweep_pack(const BlockDev *blocks, int block);
+template <class T>
 __global__ void k_sweep_unpack(const BlockDev *blocks, int block);
 constexpr int kPackChunk = 16;   // i-planes per k_sweep_pack / k_sweep_unpack workgroup
 constexpr int kSecParts = 32;    // workgroups per section in k_sections

@@ -672,6 +672,12 @@ template __global__ void k_lusgs_kba<false, kKbaTJ, kKbaTK>(const BlockDev *, co
 typedef const double __attribute__((address_space(1))) *gcd_t;
 typedef double __attribute__((address_space(1))) *gd_t;
 typedef int __attribute__((address_space(1))) *gi_t;
+// the row-layout sweep in the arithmetic type T
+template <class T>
+using gcT = const T __attribute__((address_space(1))) *;
+template <class T>
+using gT = T __attribute__((address_space(1))) *;
+#define TC(x) ((T)(x##f))
 #define GLD(p, idx) (((gcd_t)(p))[idx])
 
 namespace {
@@ -682,11 +688,12 @@ struct SweepPf {
 // row-layout sweep: per step, the neighbour's static data with the terms of
 // the flux difference that do not depend on dQ, the upstream dQ, and (wave 0)
 // the cell's right-hand side and diagonal
+template <class T>
 struct RowPf {
-    double q[5], m[5], of[5], vn_old, cl;   // state, metrics, old-state flux terms
-    double p, l, lv;
-    double x[5];
-    double r[5], a;
+    T q[5], m[5], of[5], vn_old, cl;   // state, metrics, old-state flux terms
+    T p, l, lv;
+    T x[5];
+    T r[5], a;
 };
 template <int D>
 struct Dir {
@@ -694,6 +701,16 @@ struct Dir {
 };
 constexpr int kSweepAhead = 1;   // prefetch distance of the direction waves, in steps
 constexpr int kRowS0 = -2;       // first (idle) step of the row-layout sweep loops
+// whether a mailbox word still holds the empty pattern
+__device__ inline bool mbox_empty(double x) { return __builtin_bit_cast(unsigned long long, x) == kMboxEmpty; }
+__device__ inline bool mbox_empty(float x) { return __builtin_bit_cast(unsigned int, x) == kMboxEmpty32; }
+// the mailbox set of a sweep direction, in words of T (both sets share one
+// allocation whose second half starts at mbox[1])
+template <class T>
+__device__ inline T *mbox_base(const BlockDev &B, bool fwd)
+{
+    return (T *)B.mbox[0] + (fwd ? 0 : (B.mbox[1] - B.mbox[0]));
+}
 }  // namespace
 
 template <bool FWD, int TJ, int TK>
@@ -920,6 +937,7 @@ template __global__ void k_lusgs_kba3<false, kKbaTJ, kKbaTK>(const BlockDev *, c
 // block a harmless unit state; it runs at set-up (metrics and those lanes).
 // Each iteration k_sweep_pack copies the state of the cells in, and after the
 // sweeps k_sweep_unpack copies dQ back to the cell box.
+template <class T>
 __global__ void __launch_bounds__(256) k_sweep_fill(const BlockDev *__restrict__ blocks, int block, long nrows, int what)
 {
     constexpr int TJ = kKbaTJ, TK = kKbaTK;
@@ -932,7 +950,7 @@ __global__ void __launch_bounds__(256) k_sweep_fill(const BlockDev *__restrict__
     const int i = r + 1 - tj - tk, j = J * TJ + tj + 1, k = K * TK + tk + 1;
     const bool valid = i >= 1 && i <= B.ni && j <= B.nj && k <= B.nk;
     const long box = B.box, x = valid ? cidx(B, i, j, k) : 0;
-    double *dst = B.srow + row * (kSweepQ * 64) + lane;
+    T *dst = (T *)B.srow + row * (kSweepQ * 64) + lane;
     if (what & kGatherState) {
         for (int c = 0; c < 5; ++c) dst[(kSQ_Q + c) * 64] = valid ? B.Q[c * box + x] : (c == 0 ? 1.0 : 0.0);
         dst[kSQ_P * 64] = valid ? B.W[4 * box + x] : 1.0;
@@ -949,6 +967,7 @@ __global__ void __launch_bounds__(256) k_sweep_fill(const BlockDev *__restrict__
 // read plane by plane along k (coalesced) into LDS, then written out row by
 // row: the rows a chunk touches are partly written (the lanes whose cell lies
 // in the chunk), each as one contiguous run.
+template <class T>
 __global__ void __launch_bounds__(256) k_sweep_pack(const BlockDev *__restrict__ blocks, int block)
 {
     constexpr int TJ = kKbaTJ, TK = kKbaTK;
@@ -981,12 +1000,13 @@ __global__ void __launch_bounds__(256) k_sweep_pack(const BlockDev *__restrict__
         for (int rr = w; rr < kPackChunk + TJ + TK - 2; rr += 4) {
             const int r = i0 - 1 + rr, i = r + 1 - tj - tk;
             if (col && i >= i0 && i < i0 + kPackChunk && i <= B.ni)
-                B.srow[((long)slot * B.nr + r) * (kSweepQ * 64) + rq * 64 + lane] = buf[i - i0][lane];
+                ((T *)B.srow)[((long)slot * B.nr + r) * (kSweepQ * 64) + rq * 64 + lane] = (T)buf[i - i0][lane];
         }
         __syncthreads();
     }
 }
 
+template <class T>
 __global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict__ blocks, int block)
 {
     constexpr int TJ = kKbaTJ, TK = kKbaTK;
@@ -1003,7 +1023,7 @@ __global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict
         for (int rr = w; rr < kPackChunk + TJ + TK - 2; rr += 4) {
             const int r = i0 - 1 + rr, i = r + 1 - tj - tk;
             if (col && i >= i0 && i < i0 + kPackChunk && i <= B.ni)
-                buf[i - i0][lane] = B.sdq[((long)slot * B.nr + r) * (5 * 64) + c * 64 + lane];
+                buf[i - i0][lane] = (double)((const T *)B.sdq)[((long)slot * B.nr + r) * (5 * 64) + c * 64 + lane];
         }
         __syncthreads();
         for (int ci = w; ci < kPackChunk; ci += 4) {
@@ -1013,6 +1033,13 @@ __global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict
         __syncthreads();
     }
 }
+
+template __global__ void k_sweep_fill<double>(const BlockDev *, int, long, int);
+template __global__ void k_sweep_fill<float>(const BlockDev *, int, long, int);
+template __global__ void k_sweep_pack<double>(const BlockDev *, int);
+template __global__ void k_sweep_pack<float>(const BlockDev *, int);
+template __global__ void k_sweep_unpack<double>(const BlockDev *, int);
+template __global__ void k_sweep_unpack<float>(const BlockDev *, int);
 
 // Fourth form of the wavefront sweep: the four-wave schedule of k_lusgs_kba3
 // on the tile-skewed layout.  A step touches one row of its tile: the three
@@ -1026,21 +1053,24 @@ __global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict
 // and the consumer reads the words a step ahead and, before using them, waits
 // (bounded) while any word it needs still holds the empty pattern.  Tiles
 // start in ticket order, so every upstream tile is already running.
-template <bool FWD, int TJ, int TK, bool VISC>
+template <bool FWD, int TJ, int TK, bool VISC, class T>
 __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ blocks, const KbaTile *__restrict__ tiles,
                                                   int ntiles, int *__restrict__ ticket, int *__restrict__ err,
                                                   double relax, Gas g)
 {
     static_assert(TJ * TK == 64, "64 cells per tile");
-    __shared__ double sdq[2][5][64];
-    __shared__ double sA[3][5][64];
-    __shared__ double sV[3][5][64];
-    __shared__ double sR[6][64];
+    // The FP32 operator forms the flux difference without cancellation; the
+    // FP64 one follows the reference's expression order.
+    constexpr bool kDeltaForm = sizeof(T) == 4;
+    __shared__ T sdq[2][5][64];
+    __shared__ T sA[3][5][64];
+    __shared__ T sV[3][5][64];
+    __shared__ T sR[6][64];
     __shared__ int s_q;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     const int tj = lane / TK, tk = lane % TK;
-    const double Gamma = g.gamma;
-    constexpr int RQ = kSweepQ * 64, RD = 5 * 64;   // doubles per row
+    const T Gamma = (T)g.gamma;
+    constexpr int RQ = kSweepQ * 64, RD = 5 * 64;   // words per row
     for (;;) {
         if (threadIdx.x == 0) s_q = atomicAdd(ticket, 1);
         __syncthreads();
@@ -1051,8 +1081,8 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
         const BlockDev &B = blocks[tl.block];
         const int ni = B.ni, nj = B.nj, nk = B.nk, nr = B.nr, ntk = B.ntk;
         const int slot = (tl.j0 / TJ) * ntk + tl.k0 / TK;
-        const gcd_t rows = (gcd_t)(B.srow + (long)slot * nr * RQ);
-        const gd_t drows = (gd_t)(B.sdq + (long)slot * nr * RD);
+        const gcT<T> rows = (gcT<T>)((const T *)B.srow + (long)slot * nr * RQ);
+        const gT<T> drows = (gT<T>)((T *)B.sdq + (long)slot * nr * RD);
         const int lj = FWD ? tj : TJ - 1 - tj, lk = FWD ? tk : TK - 1 - tk;
         const int j = tl.j0 + tj + 1, k = tl.k0 + tk + 1;
         const bool col = j <= nj && k <= nk;
@@ -1074,14 +1104,14 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
         };
         auto dir_loop = [&](auto dc) {
             constexpr int d = decltype(dc)::value;
-            constexpr int T = d == 1 ? TJ : TK;
+            constexpr int TE = d == 1 ? TJ : TK;   // tile extent along d
             const int dep = d == 1 ? tl.dep_j : d == 2 ? tl.dep_k : -1;
             const bool edge = d == 1 ? tj == (FWD ? 0 : TJ - 1) : d == 2 ? tk == (FWD ? 0 : TK - 1) : false;
             // upstream tile's rows (this tile's own when there is none: the
             // values are then never used)
             const int uslot = dep < 0 ? slot : d == 1 ? slot + (FWD ? -ntk : ntk) : slot + (FWD ? -1 : 1);
-            const gcd_t urows = (gcd_t)(B.srow + (long)uslot * nr * RQ);
-            const gcd_t udrows = (gcd_t)(B.sdq + (long)uslot * nr * RD);
+            const gcT<T> urows = (gcT<T>)((const T *)B.srow + (long)uslot * nr * RQ);
+            const gcT<T> udrows = (gcT<T>)((const T *)B.sdq + (long)uslot * nr * RD);
             // the upstream lane facing this lane across the tile edge
             const int ulane = d == 1 ? (FWD ? (TJ - 1) * TK : 0) + tk : tj * TK + (FWD ? TK - 1 : 0);
             // in-tile neighbour lane (lanes on the edge read the upstream row instead)
@@ -1089,28 +1119,28 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
             const bool from_up = d != 0 && edge && dep >= 0;
             // the upstream tile's mailbox words this lane reads (the other lanes
             // read the same words as the edge lanes)
-            const gd_t umbox = (gd_t)(B.mbox[FWD ? 0 : 1] + (long)uslot * nr * kMboxRow);
+            const gT<T> umbox = (gT<T>)(mbox_base<T>(B, FWD) + (long)uslot * nr * kMboxRow);
             const int mword = d == 1 ? tk : kMboxJ + tj;
             const int mstride = d == 1 ? TK : TJ;
             // upstream dQ of a step, read one step ahead without waiting
-            auto issue_cross = [&](RowPf &bn, int step) {
+            auto issue_cross = [&](RowPf<T> &bn, int step) {
                 if (d == 0) return;
-                const int urow = clampr(FWD ? row_of(step) + T - 1 : row_of(step) - (T - 1));
+                const int urow = clampr(FWD ? row_of(step) + TE - 1 : row_of(step) - (TE - 1));
                 for (int c = 0; c < 5; ++c)
                     bn.x[c] = __hip_atomic_load(umbox + urow * kMboxRow + mword + c * mstride, __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_AGENT);
             };
             // before using the upstream dQ of step s: re-read while a word the
             // step needs is still empty (bounded; a run-out sets the error word)
-            auto settle_cross = [&](RowPf &b, int s, bool need) {
+            auto settle_cross = [&](RowPf<T> &b, int s, bool need) {
                 if (d == 0) return;
-                auto empty = [&](const double *x) {
+                auto empty = [&](const T *x) {
                     bool e = false;
-                    for (int c = 0; c < 5; ++c) e = e || (__builtin_bit_cast(unsigned long long, x[c]) == kMboxEmpty);
+                    for (int c = 0; c < 5; ++c) e = e || (mbox_empty(x[c]));
                     return e;
                 };
                 if (!__builtin_amdgcn_ballot_w64(need && empty(b.x))) return;
-                const int urow = clampr(FWD ? row_of(s) + T - 1 : row_of(s) - (T - 1));
+                const int urow = clampr(FWD ? row_of(s) + TE - 1 : row_of(s) - (TE - 1));
                 long spins = 0;
                 while (__builtin_amdgcn_ballot_w64(need && empty(b.x))) {
                     __builtin_amdgcn_s_sleep(1);
@@ -1122,11 +1152,11 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
             };
             // the neighbour's state and metrics of a step (static during the
             // sweep, so no wait), and for wave 0 the cell's right-hand side
-            auto issue_static = [&](RowPf &bn, int step) {
+            auto issue_static = [&](RowPf<T> &bn, int step) {
                 const int rt = row_of(step);
-                const int urow = clampr(FWD ? rt + T - 1 : rt - (T - 1));
+                const int urow = clampr(FWD ? rt + TE - 1 : rt - (TE - 1));
                 const int nrow = clampr(FWD ? rt - 1 : rt + 1);
-                const gcd_t src = (from_up ? urows + urow * RQ + ulane : rows + nrow * RQ + nlane);
+                const gcT<T> src = (from_up ? urows + urow * RQ + ulane : rows + nrow * RQ + nlane);
                 for (int c = 0; c < 5; ++c) bn.q[c] = src[(kSQ_Q + c) * 64];
                 bn.p = src[kSQ_P * 64];
                 bn.l = src[(kSQ_LAM + d) * 64];
@@ -1135,57 +1165,90 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                 if (d == 0) {
                     const int ro = clampr(rt);
                     for (int c = 0; c < 5; ++c)
-                        bn.r[c] = FWD ? rows[ro * RQ + (kSQ_R + c) * 64 + lane] : ((gcd_t)drows)[ro * RD + c * 64 + lane];
+                        bn.r[c] = FWD ? rows[ro * RQ + (kSQ_R + c) * 64 + lane] : ((gcT<T>)drows)[ro * RD + c * 64 + lane];
                     bn.a = rows[ro * RQ + kSQ_A * 64 + lane];
                 }
             };
             // the terms of the flux difference that use only the old state
             // (source/LU_SGS_Marching.cpp:293-419; same operations, evaluated early)
-            auto precompute = [&](RowPf &b) {
-                const double *o = b.q;
-                const double nx = b.m[0], ny = b.m[1], nz = b.m[2], Vn_e = b.m[4];
-                const double press_old = b.p;
-                const double Vn_old = o[1] / o[0] * nx + o[2] / o[0] * ny + o[3] / o[0] * nz;
+            auto precompute = [&](RowPf<T> &b) {
+                const T *o = b.q;
+                const T nx = b.m[0], ny = b.m[1], nz = b.m[2], Vn_e = b.m[4];
+                const T press_old = b.p;
+                const T Vn_old = o[1] / o[0] * nx + o[2] / o[0] * ny + o[3] / o[0] * nz;
                 b.vn_old = Vn_old;
-                b.of[0] = o[0] * (Vn_old - Vn_e);
-                b.of[1] = o[1] * (Vn_old - Vn_e) + press_old * nx;
-                b.of[2] = o[2] * (Vn_old - Vn_e) + press_old * ny;
-                b.of[3] = o[3] * (Vn_old - Vn_e) + press_old * nz;
-                b.of[4] = (o[4] + press_old) * (Vn_old - Vn_e) + press_old * Vn_e;
-                b.cl = relax * b.l;
+                if constexpr (kDeltaForm) {
+                    // old velocity and total enthalpy per volume for the
+                    // cancellation-free difference below
+                    b.of[0] = o[1] / o[0];
+                    b.of[1] = o[2] / o[0];
+                    b.of[2] = o[3] / o[0];
+                    b.of[3] = o[4] + press_old;
+                } else {
+                    b.of[0] = o[0] * (Vn_old - Vn_e);
+                    b.of[1] = o[1] * (Vn_old - Vn_e) + press_old * nx;
+                    b.of[2] = o[2] * (Vn_old - Vn_e) + press_old * ny;
+                    b.of[3] = o[3] * (Vn_old - Vn_e) + press_old * nz;
+                    b.of[4] = (o[4] + press_old) * (Vn_old - Vn_e) + press_old * Vn_e;
+                }
+                b.cl = (T)relax * b.l;
             };
             // step s: the dQ-dependent part of the flux difference (critical
             // path), then, while the combine wave works, the loads and early
             // terms of the next steps
-            auto work = [&](RowPf &b, RowPf &b1, RowPf &b2, int s) {
+            auto work = [&](RowPf<T> &b, RowPf<T> &b1, RowPf<T> &b2, int s) {
                 int i;
                 const bool valid = cell_at(s, i);
                 const int p = d == 0 ? i : d == 1 ? j : k, lim = d == 0 ? ni : d == 1 ? nj : nk;
                 const bool use = valid && (FWD ? p != 1 : p != lim);
                 settle_cross(b, s, use && from_up);
-                double dqn[5];
+                T dqn[5];
                 for (int c = 0; c < 5; ++c) {
-                    const double in_tile = sdq[(s - 1) & 1][c][nlane];
+                    const T in_tile = sdq[(s - 1) & 1][c][nlane];
                     dqn[c] = (d != 0 && edge) ? b.x[c] : in_tile;
                 }
-                const double nx = b.m[0], ny = b.m[1], nz = b.m[2], sd = b.m[3], Vn_e = b.m[4];
-                double nw[5];
-                for (int c = 0; c < 5; ++c) nw[c] = b.q[c] + dqn[c];
-                double dens = nw[0], uvel = nw[1] / nw[0], vvel = nw[2] / nw[0], wvel = nw[3] / nw[0], E = nw[4] / nw[0];
-                double press_new = (Gamma - F32C(1.0)) * dens * (E - F32C(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel));
-                double Vn_new = uvel * nx + vvel * ny + wvel * nz;
-                double df[5];
-                df[0] = nw[0] * (Vn_new - Vn_e) - b.of[0];
-                df[1] = (nw[1] * (Vn_new - Vn_e) + press_new * nx) - b.of[1];
-                df[2] = (nw[2] * (Vn_new - Vn_e) + press_new * ny) - b.of[2];
-                df[3] = (nw[3] * (Vn_new - Vn_e) + press_new * nz) - b.of[3];
-                df[4] = ((nw[4] + press_new) * (Vn_new - Vn_e) + press_new * Vn_e) - b.of[4];
-                double A[5], V[5];
+                const T nx = b.m[0], ny = b.m[1], nz = b.m[2], sd = b.m[3], Vn_e = b.m[4];
+                T df[5];
+                if constexpr (kDeltaForm) {
+                    // F_n(Q+dQ) - F_n(Q) written as sums of products with a
+                    // difference factor, so its rounding error scales with dQ:
+                    //   du = (dm - u d rho)/(rho + d rho),  dVn = du.n,
+                    //   dp = (gamma-1)(dE - (m.du + dm.u')/2),  w' = Vn' - Vn_e
+                    //   d(rho w) = d rho w' + rho dVn,  d(m w + p n) = dm w' + m dVn + dp n,
+                    //   d((E+p) w + p Vn_e) = (dE + dp) w' + (E+p) dVn + dp Vn_e
+                    const T *q = b.q;
+                    const T rho1 = q[0] + dqn[0];
+                    const T du = (dqn[1] - b.of[0] * dqn[0]) / rho1, dv = (dqn[2] - b.of[1] * dqn[0]) / rho1,
+                            dw = (dqn[3] - b.of[2] * dqn[0]) / rho1;
+                    const T u1 = b.of[0] + du, v1 = b.of[1] + dv, w1 = b.of[2] + dw;
+                    const T dVn = du * nx + dv * ny + dw * nz;
+                    const T wn = (b.vn_old + dVn) - Vn_e;
+                    const T dp = (Gamma - TC(1.0)) *
+                                 (dqn[4] - TC(0.5) * ((q[1] * du + q[2] * dv + q[3] * dw) +
+                                                      (dqn[1] * u1 + dqn[2] * v1 + dqn[3] * w1)));
+                    df[0] = dqn[0] * wn + q[0] * dVn;
+                    df[1] = dqn[1] * wn + q[1] * dVn + dp * nx;
+                    df[2] = dqn[2] * wn + q[2] * dVn + dp * ny;
+                    df[3] = dqn[3] * wn + q[3] * dVn + dp * nz;
+                    df[4] = (dqn[4] + dp) * wn + b.of[3] * dVn + dp * Vn_e;
+                } else {
+                    T nw[5];
+                    for (int c = 0; c < 5; ++c) nw[c] = b.q[c] + dqn[c];
+                    T dens = nw[0], uvel = nw[1] / nw[0], vvel = nw[2] / nw[0], wvel = nw[3] / nw[0], E = nw[4] / nw[0];
+                    T press_new = (Gamma - TC(1.0)) * dens * (E - TC(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel));
+                    T Vn_new = uvel * nx + vvel * ny + wvel * nz;
+                    df[0] = nw[0] * (Vn_new - Vn_e) - b.of[0];
+                    df[1] = (nw[1] * (Vn_new - Vn_e) + press_new * nx) - b.of[1];
+                    df[2] = (nw[2] * (Vn_new - Vn_e) + press_new * ny) - b.of[2];
+                    df[3] = (nw[3] * (Vn_new - Vn_e) + press_new * nz) - b.of[3];
+                    df[4] = ((nw[4] + press_new) * (Vn_new - Vn_e) + press_new * Vn_e) - b.of[4];
+                }
+                T A[5], V[5];
                 for (int c = 0; c < 5; ++c) {
-                    const double a = FWD ? (df[c] * sd + dqn[c] * b.cl) * F32C(0.5) : (df[c] * sd - dqn[c] * b.cl) * F32C(0.5);
-                    const double v = dqn[c] * (VISC ? b.lv : 0.0);   // lamv * dQ, lamv = 0 on the inviscid path
-                    A[c] = use ? a : 0.0;
-                    V[c] = use ? v : 0.0;
+                    const T a = FWD ? (df[c] * sd + dqn[c] * b.cl) * TC(0.5) : (df[c] * sd - dqn[c] * b.cl) * TC(0.5);
+                    const T v = dqn[c] * (VISC ? b.lv : T(0));   // lamv * dQ, lamv = 0 on the inviscid path
+                    A[c] = use ? a : T(0);
+                    V[c] = use ? v : T(0);
                 }
                 for (int c = 0; c < 5; ++c) { sA[d][c][lane] = A[c]; sV[d][c][lane] = V[c]; }
                 if (d == 0) {
@@ -1202,7 +1265,7 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
             // upstream dQ one step ahead.  The loop starts two idle steps early
             // (every lane invalid) so that no load precedes it and every step
             // runs the same code; nothing is in flight on entry.
-            RowPf b[3] = {};
+            RowPf<T> b[3] = {};
             __builtin_amdgcn_s_waitcnt(0);
             for (int s0 = kRowS0; s0 < send; s0 += 3) {
 #pragma unroll
@@ -1217,7 +1280,7 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
             dir_loop(Dir<2>());
         } else {
             // ---- combine wave (source/LU_SGS_Marching.cpp:155, 255)
-            const gd_t mbox = (gd_t)(B.mbox[FWD ? 0 : 1] + (long)slot * nr * kMboxRow);
+            const gT<T> mbox = (gT<T>)(mbox_base<T>(B, FWD) + (long)slot * nr * kMboxRow);
             // downstream edges of this tile: the j edge (lanes tj = jl, word tk)
             // and the k edge (lanes tk = kl, word tj)
             const bool jedge = tj == (FWD ? TJ - 1 : 0), kedge = tk == (FWD ? TK - 1 : 0);
@@ -1225,14 +1288,14 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                 int i;
                 const bool valid = cell_at(s, i);
                 const int ro = clampr(row_of(s));
-                double outv[5];
+                T outv[5];
                 bar();
                 if (valid) {
-                    const double alpha = sR[5][lane];
+                    const T alpha = sR[5][lane];
                     for (int c = 0; c < 5; ++c) {
-                        double v;
+                        T v;
                         if (FWD) {
-                            v = sR[c][lane] * F32C(-1.0) + sA[0][c][lane];
+                            v = sR[c][lane] * TC(-1.0) + sA[0][c][lane];
                             v = v + sA[1][c][lane]; v = v + sA[2][c][lane];
                         } else {
                             v = sR[c][lane] * alpha - sA[0][c][lane];
@@ -1261,10 +1324,17 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
     }
 }
 
-template __global__ void k_lusgs_sl<true, kKbaTJ, kKbaTK, false>(const BlockDev *, const KbaTile *, int, int *, int *, double, Gas);
-template __global__ void k_lusgs_sl<false, kKbaTJ, kKbaTK, false>(const BlockDev *, const KbaTile *, int, int *, int *, double, Gas);
-template __global__ void k_lusgs_sl<true, kKbaTJ, kKbaTK, true>(const BlockDev *, const KbaTile *, int, int *, int *, double, Gas);
-template __global__ void k_lusgs_sl<false, kKbaTJ, kKbaTK, true>(const BlockDev *, const KbaTile *, int, int *, int *, double, Gas);
+#define FV3D_SL(F, V, T) \
+    template __global__ void k_lusgs_sl<F, kKbaTJ, kKbaTK, V, T>(const BlockDev *, const KbaTile *, int, int *, int *, double, Gas);
+FV3D_SL(true, false, double)
+FV3D_SL(false, false, double)
+FV3D_SL(true, true, double)
+FV3D_SL(false, true, double)
+FV3D_SL(true, false, float)
+FV3D_SL(false, false, float)
+FV3D_SL(true, true, float)
+FV3D_SL(false, true, float)
+#undef FV3D_SL
 
 __global__ void k_fill_u64(unsigned long long *p, long n, unsigned long long v)
 {

@@ -34,7 +34,9 @@ enum fv3d_status {
 
 enum fv3d_precision {
     FV3D_PLAIN_FP64 = 0,   /* every kernel in FP64 */
-    FV3D_DEFECT_FP32 = 1   /* FP64 residual/update/norms, FP32 LU-SGS operator */
+    FV3D_DEFECT_FP32 = 1   /* FP64 residual/update/norms, FP32 LU-SGS operator (both
+                              sweeps, default sweep form); converges to the FP64
+                              fixed point */
 };
 
 /* Raw config.dat values, as parsed by CUserInput::ReadConfig
