@@ -65,6 +65,17 @@ struct fv3d_ctx {
     int nper = 0;
     int *d_pitch_sign = nullptr;
     std::vector<PatchDev> pd_host;   // the device patch list, host copy
+    double *d_mbox = nullptr;         // sweep mailboxes of all blocks
+    long mbox_words = 0;
+    long *d_pack_start = nullptr;     // running pack workgroup counts per block
+    long pack_groups = 0;
+    // Baldwin-Lomax launches: round r takes the r-th wall patch of every block;
+    // d_bl_walls holds (patch index, line count) pairs, round r at offset
+    // bl_rounds[r].first with bl_round_n[r] entries and bl_rounds[r].second lines
+    int2 *d_bl_walls = nullptr;
+    std::vector<std::pair<int, long>> bl_rounds;
+    std::vector<int> bl_round_n;
+    std::vector<int> bl_groups;   // cell groups per line of each round (16, 32 or 64)
     long *d_pstart_bc = nullptr, *d_cstart = nullptr, *d_pstart_plane = nullptr;
     int4 *d_tiles = nullptr;
     int2 *d_secs = nullptr;
@@ -322,9 +333,8 @@ static int setup_block(fv3d_ctx *c, HostBlock &b, int index)
         b.pack_groups = ntile * ((B.ni + kPackChunk - 1) / kPackChunk);
         if ((rc = alloc(&B.srow, b.sweep_rows * kSweepQ * 64))) return rc;
         if ((rc = alloc(&B.sdq, b.sweep_rows * 5 * 64))) return rc;
-        // both mailboxes in one allocation, reset together each iteration
-        if ((rc = alloc(&B.mbox[0], 2 * b.sweep_rows * kMboxRow))) return rc;
-        B.mbox[1] = B.mbox[0] + b.sweep_rows * kMboxRow;
+        // the mailboxes of all blocks share one allocation (fv3d_setup), so
+        // one fill resets them each iteration
     }
     // Dummy slabs: tangential boxes (JM+1)(KM+1), (IM+1)(KM+1), (IM+1)(JM+1)
     const int ta[3] = {b.JM, b.IM, b.IM}, tb[3] = {b.KM, b.KM, b.JM};
@@ -606,6 +616,26 @@ int fv3d_setup(fv3d_ctx *c)
         if ((rc = setup_block(c, c->blocks[n], n))) return rc;
         c->total_vol = c->total_vol + c->blocks[n].cellvol;   // source/ComputeAllBlockVars.cpp:25-36
     }
+    {
+        // mailboxes (forward and backward sets per block) and the pack work list
+        long words = 0;
+        std::vector<long> gst{0};
+        for (auto &b : c->blocks) {
+            words += 2 * b.sweep_rows * kMboxRow;
+            gst.push_back(gst.back() + b.pack_groups);
+        }
+        if ((rc = dalloc(c, &c->d_mbox, words))) return rc;
+        c->mbox_words = words;
+        long off = 0;
+        for (auto &b : c->blocks) {
+            b.dev.mbox[0] = c->d_mbox + off;
+            b.dev.mbox[1] = b.dev.mbox[0] + b.sweep_rows * kMboxRow;
+            off += 2 * b.sweep_rows * kMboxRow;
+        }
+        c->pack_groups = gst.back();
+        if ((rc = dalloc(c, &c->d_pack_start, gst.size()))) return rc;
+        HIPCK(c, hipMemcpyAsync(c->d_pack_start, gst.data(), gst.size() * sizeof(long), hipMemcpyHostToDevice, c->stream));
+    }
     std::vector<BlockDev> bd;
     for (auto &b : c->blocks) bd.push_back(b.dev);
     if ((rc = dalloc(c, &c->d_blocks, nb))) return rc;
@@ -655,15 +685,46 @@ int fv3d_setup(fv3d_ctx *c)
         for (const HostPatch &P : c->patches) {
             if (P.kind != 2) continue;
             const HostBlock &hb = c->blocks[P.block - 1];
-            const int d = (P.face - 1) / 2, N = d == 0 ? hb.IM : d == 1 ? hb.JM : hb.KM;
+            const int d = (P.face - 1) / 2;
             // the reference's 1-D model runs over JM cells for K-face walls
             // (source/BL_Model.cpp:225), which is defined only when JM == KM
             if (d == 2 && hb.JM != hb.KM)
                 return fail(c, FV3D_EINVAL, "Baldwin-Lomax on a K-face wall needs JM == KM (block %d)", P.block);
-            if (N - 1 > 64 * kBlChunks)
-                return fail(c, FV3D_EINVAL, "Baldwin-Lomax lines are limited to %d cells (block %d)", 64 * kBlChunks,
-                            P.block);
         }
+    if (c->cfg.turbulence == 100) {
+        std::vector<std::vector<int>> per_block(c->blocks.size());
+        for (int q = 0; q < (int)c->patches.size(); ++q)
+            if (c->patches[q].kind == 2) per_block[c->patches[q].block - 1].push_back(q);
+        std::vector<int2> wl;
+        for (size_t r = 0;; ++r) {
+            long lines = 0;
+            int n = 0, first = (int)wl.size(), maxlen = 0;
+            for (auto &v : per_block)
+                if (r < v.size()) {
+                    const PatchDev &q = pd[v[r]];
+                    // lines rounded up to whole workgroups, so a workgroup never spans two walls
+                    const long nl = ((long)q.na * q.nb + kBlLines - 1) / kBlLines * kBlLines;
+                    wl.push_back(make_int2(v[r], (int)nl));
+                    lines += nl;
+                    ++n;
+                    const HostBlock &hb = c->blocks[q.block];
+                    const int N = q.dir == 0 ? hb.IM : q.dir == 1 ? hb.JM : hb.KM;
+                    maxlen = std::max(maxlen, N - 1);
+                }
+            if (!n) break;
+            int G = 16;
+            while (G * kBlCpt < maxlen && G < 64) G *= 2;
+            if (G * kBlCpt < maxlen)
+                return fail(c, FV3D_EINVAL, "Baldwin-Lomax lines are limited to %d cells", 64 * kBlCpt);
+            c->bl_rounds.push_back({first, lines});
+            c->bl_round_n.push_back(n);
+            c->bl_groups.push_back(G);
+        }
+        if (!wl.empty()) {
+            if ((rc = dalloc(c, &c->d_bl_walls, wl.size()))) return rc;
+            HIPCK(c, hipMemcpyAsync(c->d_bl_walls, wl.data(), wl.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+        }
+    }
     c->npatch_dev = (int)pd.size();
     c->pd_host = pd;
     if ((rc = build_exchange(c))) return rc;
@@ -835,12 +896,16 @@ static int launch_turbulence(fv3d_ctx *c)
         hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((B.box + 255) / 256)), dim3(256), 0, c->stream, B.blflag, B.box, 0);
     }
     hipLaunchKernelGGL(k_curl, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb);
-    for (int q = 0; q < (int)c->patches.size(); ++q) {
-        const HostPatch &P = c->patches[q];
-        if (P.kind != 2) continue;
-        const long lines = (long)c->pd_host[q].na * c->pd_host[q].nb;
-        hipLaunchKernelGGL(k_bl_wall, dim3((unsigned)((lines + 3) / 4)), dim3(256), 0, c->stream, c->d_blocks, c->d_patches,
-                           q, c->gas);
+    for (size_t r = 0; r < c->bl_rounds.size(); ++r) {
+        const unsigned wgs = (unsigned)(c->bl_rounds[r].second / kBlLines);
+        const int2 *wl = c->d_bl_walls + c->bl_rounds[r].first;
+        const int G = c->bl_groups[r], n = c->bl_round_n[r];
+        if (G == 16)
+            hipLaunchKernelGGL(k_bl_lines<16>, dim3(wgs), dim3(kBlLines * 16), 0, c->stream, c->d_blocks, c->d_patches, wl, n, c->gas);
+        else if (G == 32)
+            hipLaunchKernelGGL(k_bl_lines<32>, dim3(wgs), dim3(kBlLines * 32), 0, c->stream, c->d_blocks, c->d_patches, wl, n, c->gas);
+        else
+            hipLaunchKernelGGL(k_bl_lines<64>, dim3(wgs), dim3(kBlLines * 64), 0, c->stream, c->d_blocks, c->d_patches, wl, n, c->gas);
     }
     HIPCK(c, hipGetLastError());
     return FV3D_OK;
@@ -868,19 +933,18 @@ static int launch_sweeps_kba(fv3d_ctx *c)
         const bool f32 = c->cfg.precision == FV3D_DEFECT_FP32;
         const bool visc = c->cfg.viscous;
         const int nb = (int)c->blocks.size();
-        for (int n = 0; n < nb; ++n) {
-            const HostBlock &hb = c->blocks[n];
-            const long nw = 2 * hb.sweep_rows * kMboxRow;
-            if (f32)
-                hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, c->stream,
-                                   (int *)hb.dev.mbox[0], nw, (int)kMboxEmpty32);
-            else
-                hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, c->stream,
-                                   (unsigned long long *)hb.dev.mbox[0], nw, kMboxEmpty);
-            if (f32)
-                hipLaunchKernelGGL(k_sweep_pack<float>, dim3((unsigned)hb.pack_groups), dim3(256), 0, c->stream, c->d_blocks, n);
-            else
-                hipLaunchKernelGGL(k_sweep_pack<double>, dim3((unsigned)hb.pack_groups), dim3(256), 0, c->stream, c->d_blocks, n);
+        const unsigned pg = (unsigned)c->pack_groups;
+        if (f32) {
+            // FP32 words: fill every 32-bit word of the shared allocation
+            const long nw = 2 * c->mbox_words;
+            hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, c->stream, (int *)c->d_mbox, nw,
+                               (int)kMboxEmpty32);
+            hipLaunchKernelGGL(k_sweep_pack<float>, dim3(pg), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb);
+        } else {
+            const long nw = c->mbox_words;
+            hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, c->stream,
+                               (unsigned long long *)c->d_mbox, nw, kMboxEmpty);
+            hipLaunchKernelGGL(k_sweep_pack<double>, dim3(pg), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb);
         }
         // forward and backward sweeps in the operator's precision, with or
         // without the viscous terms
@@ -905,14 +969,10 @@ static int launch_sweeps_kba(fv3d_ctx *c)
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[7], c->stream));
         sweep(false);
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[8], c->stream));
-        for (int n = 0; n < nb; ++n) {
-            if (f32)
-                hipLaunchKernelGGL(k_sweep_unpack<float>, dim3((unsigned)c->blocks[n].pack_groups), dim3(256), 0, c->stream,
-                                   c->d_blocks, n);
-            else
-                hipLaunchKernelGGL(k_sweep_unpack<double>, dim3((unsigned)c->blocks[n].pack_groups), dim3(256), 0, c->stream,
-                                   c->d_blocks, n);
-        }
+        if (f32)
+            hipLaunchKernelGGL(k_sweep_unpack<float>, dim3(pg), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb);
+        else
+            hipLaunchKernelGGL(k_sweep_unpack<double>, dim3(pg), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb);
     } else if (c->sweep_mode == 3) {
         hipLaunchKernelGGL((k_lusgs_kba3<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
                            c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_lag);

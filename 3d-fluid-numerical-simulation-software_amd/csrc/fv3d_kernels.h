@@ -91,10 +91,11 @@ __global__ void k_fill_u64(unsigned long long *p, long n, unsigned long long v);
 // row storage in T (double, or float for the FP32 operator); unpack widens dQ to double
 template <class T>
 __global__ void k_sweep_fill(const BlockDev *blocks, int block, long nrows, int what);
+// one launch over all blocks: workgroups gstart[n] .. gstart[n+1]-1 serve block n
 template <class T>
-__global__ void k_sweep_pack(const BlockDev *blocks, int block);
+__global__ void k_sweep_pack(const BlockDev *blocks, const long *gstart, int nblock);
 template <class T>
-__global__ void k_sweep_unpack(const BlockDev *blocks, int block);
+__global__ void k_sweep_unpack(const BlockDev *blocks, const long *gstart, int nblock);
 constexpr int kPackChunk = 16;   // i-planes per k_sweep_pack / k_sweep_unpack workgroup
 constexpr int kSecParts = 32;    // workgroups per section in k_sections
 __global__ void k_fill_int(int *p, long n, int v);
@@ -117,9 +118,12 @@ template <int TI, int TJ, int TK>
 __global__ void k_viscous(const BlockDev *blocks, const int4 *tiles, Gas g, int store, ViscDebug dbg);
 // vorticity magnitude per cell
 __global__ void k_curl(const BlockDev *blocks, const long *cstart, int nblock);
-// Baldwin-Lomax along the grid lines normal to one wall patch, one wave per
-// line of up to 64*kBlChunks cells
-constexpr int kBlChunks = 8;
-__global__ void k_bl_wall(const BlockDev *blocks, const PatchDev *patches, int patch, Gas g);
+// Baldwin-Lomax along the grid lines normal to a set of wall patches (at most
+// one per block): kBlLines lines x G groups of kBlCpt cells per workgroup (so
+// lines of up to G*kBlCpt cells); walls[w] = (patch index, line count rounded
+// up to a multiple of kBlLines)
+constexpr int kBlLines = 16, kBlCpt = 8;
+template <int G>
+__global__ void k_bl_lines(const BlockDev *blocks, const PatchDev *patches, const int2 *walls, int nwalls, Gas g);
 
 }  // namespace fv3d

@@ -968,19 +968,24 @@ __global__ void __launch_bounds__(256) k_sweep_fill(const BlockDev *__restrict__
 // row: the rows a chunk touches are partly written (the lanes whose cell lies
 // in the chunk), each as one contiguous run.
 template <class T>
-__global__ void __launch_bounds__(256) k_sweep_pack(const BlockDev *__restrict__ blocks, int block)
+__global__ void __launch_bounds__(256) k_sweep_pack(const BlockDev *__restrict__ blocks, const long *__restrict__ gstart,
+                                                    int nblock)
 {
     constexpr int TJ = kKbaTJ, TK = kKbaTK;
     __shared__ double buf[kPackChunk][64];
-    const int NQ = blocks[block].lamv ? 18 : 15;   // lamv rows for Navier-Stokes
+    // the block of this workgroup: gstart holds the running workgroup counts
+    int block = 0;
+    while (block + 1 < nblock && (long)blockIdx.x >= gstart[block + 1]) ++block;
     const BlockDev &B = blocks[block];
+    const long wg = (long)blockIdx.x - gstart[block];
     const int nchunk = (B.ni + kPackChunk - 1) / kPackChunk;
-    const int slot = blockIdx.x / nchunk, i0 = (blockIdx.x % nchunk) * kPackChunk + 1;
+    const int slot = (int)(wg / nchunk), i0 = (int)(wg % nchunk) * kPackChunk + 1;
     const int J = slot / B.ntk, K = slot % B.ntk;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int tj = lane / TK, tk = lane % TK, j = J * TJ + tj + 1, k = K * TK + tk + 1;
     const bool col = j <= B.nj && k <= B.nk;
     const long box = B.box;
+    const int NQ = B.lamv ? 18 : 15;   // lamv rows for Navier-Stokes
     // the cell-box array and row quantity of each packed quantity
     for (int qi = 0; qi < NQ; ++qi) {
         const double *src;
@@ -1007,13 +1012,18 @@ __global__ void __launch_bounds__(256) k_sweep_pack(const BlockDev *__restrict__
 }
 
 template <class T>
-__global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict__ blocks, int block)
+__global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict__ blocks, const long *__restrict__ gstart,
+                                                      int nblock)
 {
     constexpr int TJ = kKbaTJ, TK = kKbaTK;
     __shared__ double buf[kPackChunk][64];
+    // the block of this workgroup: gstart holds the running workgroup counts
+    int block = 0;
+    while (block + 1 < nblock && (long)blockIdx.x >= gstart[block + 1]) ++block;
     const BlockDev &B = blocks[block];
+    const long wg = (long)blockIdx.x - gstart[block];
     const int nchunk = (B.ni + kPackChunk - 1) / kPackChunk;
-    const int slot = blockIdx.x / nchunk, i0 = (blockIdx.x % nchunk) * kPackChunk + 1;
+    const int slot = (int)(wg / nchunk), i0 = (int)(wg % nchunk) * kPackChunk + 1;
     const int J = slot / B.ntk, K = slot % B.ntk;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int tj = lane / TK, tk = lane % TK, j = J * TJ + tj + 1, k = K * TK + tk + 1;
@@ -1036,10 +1046,10 @@ __global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict
 
 template __global__ void k_sweep_fill<double>(const BlockDev *, int, long, int);
 template __global__ void k_sweep_fill<float>(const BlockDev *, int, long, int);
-template __global__ void k_sweep_pack<double>(const BlockDev *, int);
-template __global__ void k_sweep_pack<float>(const BlockDev *, int);
-template __global__ void k_sweep_unpack<double>(const BlockDev *, int);
-template __global__ void k_sweep_unpack<float>(const BlockDev *, int);
+template __global__ void k_sweep_pack<double>(const BlockDev *, const long *, int);
+template __global__ void k_sweep_pack<float>(const BlockDev *, const long *, int);
+template __global__ void k_sweep_unpack<double>(const BlockDev *, const long *, int);
+template __global__ void k_sweep_unpack<float>(const BlockDev *, const long *, int);
 
 // Fourth form of the wavefront sweep: the four-wave schedule of k_lusgs_kba3
 // on the tile-skewed layout.  A step touches one row of its tile: the three

@@ -341,118 +341,137 @@ __global__ void k_curl(const BlockDev *__restrict__ blocks, const long *__restri
 }
 
 // ------------------------------------------------------------- Baldwin-Lomax
-// Wave reductions over the 64 lanes.
-__device__ static inline double wave_max(double v)
+// CBlocks::BL_Model (source/BL_Model.cpp:8-279) with CBlocks::BL_model_1d
+// (283-329).  A launch takes one wall of each block (the r-th wall patch of
+// every block in table order), so walls of one block still run in table order
+// and a cell's first visit in the iteration sets mue_t while later walls keep
+// the minimum.  A workgroup serves kBlLines grid lines normal to the wall that
+// are neighbours along their fastest tangential index (so a load of one cell
+// position is contiguous over the lines) and splits each line into G groups;
+// thread (group g, line l) holds the cells n0 = 1 + g + G c, c < kBlCpt, of its
+// line in registers.  The reference's sequential maxima become reductions
+// over the groups in LDS: tau_w and U_dif are plain maxima (NaN entries never
+// win, as with its strict comparisons; a NaN first entry is kept), and
+// (F_max, y_max) is the largest positive F with ties going to the smallest n0,
+// which is the first one its loop meets.  The first cell ends at zero; the
+// ghost cell gets zero and then the negated first-cell value, as WallFlux
+// writes it (source/WallFlux.cpp:106-110).
+template <int G>
+__global__ void __launch_bounds__(kBlLines *G) k_bl_lines(const BlockDev *__restrict__ blocks,
+                                                          const PatchDev *__restrict__ patches,
+                                                          const int2 *__restrict__ walls, int nwalls, Gas g)
 {
-    for (int o = 32; o > 0; o >>= 1) {
-        const double w = __shfl_xor(v, o);
-        v = w > v ? w : v;
-    }
-    return v;
-}
-
-// CBlocks::BL_Model for one wall patch (source/BL_Model.cpp:8-279) with
-// CBlocks::BL_model_1d (283-329).  One wave per grid line normal to the wall;
-// lane l holds the cells n = 1 + l + 64 r along the line, n0 their distance
-// index from the wall (1 = the first cell).  The sequential maxima of the 1-D
-// model become wave reductions: tau_w and U_dif are plain maxima (NaN entries
-// never win, as with the reference's strict comparisons), and (F_max, y_max)
-// is the largest positive F with ties going to the smallest n0, which is the
-// first one the reference's loop meets.  A cell's first visit in this
-// iteration sets mue_t, later walls keep the minimum (walls run one launch
-// each, in table order).  Then the first and ghost cell of the line are set to
-// zero and the ghost gets the negated first-cell value, as WallFlux does
-// (source/WallFlux.cpp:106-110).
-__global__ void __launch_bounds__(256) k_bl_wall(const BlockDev *__restrict__ blocks, const PatchDev *__restrict__ patches,
-                                                 int patch, Gas g)
-{
-    constexpr int R = kBlChunks;
-    const PatchDev P = patches[patch];
+    constexpr int L = kBlLines, C = kBlCpt;
+    __shared__ double s_a[G][L], s_b[G][L];
+    __shared__ int s_n[G][L];
+    __shared__ double s_line[6][L];   // Tw, Udif, Ret, Fmax, y_max, Fwake
+    const int lane = threadIdx.x % L, grp = threadIdx.x / L;
+    // the wall and the lines of this workgroup (walls[w].y is its line count,
+    // rounded up to whole groups of L lines by the host)
+    long t = (long)blockIdx.x * L;
+    int w = 0;
+    while (w < nwalls && t >= walls[w].y) { t -= walls[w].y; ++w; }
+    const PatchDev P = patches[walls[min(w, nwalls - 1)].x];
     const BlockDev &B = blocks[P.block];
-    const long line = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (line >= (long)P.na * P.nb) return;   // whole waves
-    const int lane = threadIdx.x & 63;
-    const int a = P.a0 + (int)(line / P.nb), cc = P.b0 + (int)(line % P.nb);
+    const long line = t + lane;
+    const bool active = w < nwalls && line < (long)P.na * P.nb;
+    const int a = P.a0 + (int)(active ? line / P.nb : 0), cc = P.b0 + (int)(active ? line % P.nb : 0);
     const int D = P.dir, hi = P.hi;
-    const int N = D == 0 ? B.IM : D == 1 ? B.JM : B.KM;   // cells 1..N-1 along the line
+    const int N = D == 0 ? B.IM : D == 1 ? B.JM : B.KM;
+    const int M = D == 0 ? B.IM : B.JM;   // line length of the reference's call (K walls need JM == KM)
     const long box = B.box;
-    auto cell = [&](int n) -> long { return D == 0 ? cidx(B, n, a, cc) : D == 1 ? cidx(B, a, n, cc) : cidx(B, a, cc, n); };
-    // wall face of the line: face index 1 or N
-    const long xw = cell(hi ? N : 1);
+    auto cell = [&](int n0) -> long {
+        const int n = hi ? N - n0 : n0;
+        return D == 0 ? cidx(B, n, a, cc) : D == 1 ? cidx(B, a, n, cc) : cidx(B, a, cc, n);
+    };
+    const long xw = D == 0 ? cidx(B, hi ? N : 1, a, cc) : D == 1 ? cidx(B, a, hi ? N : 1, cc) : cidx(B, a, cc, hi ? N : 1);
     const double *FC = B.fc[D];
     const double wx = FC[xw], wy = FC[box + xw], wz = FC[2 * box + xw];
-    double dis[R], dens[R], vv[R], om[R], ml[R];
-    int n0[R];
-    bool ok[R];
-    double tw = -1.0, ud = 0.0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int n = 1 + lane + 64 * r;
-        ok[r] = n <= N - 1;
-        n0[r] = hi ? N - n : n;
-        const long x = cell(ok[r] ? n : 1);
-        const double dx = B.cc[x] - wx, dy = B.cc[box + x] - wy, dz = B.cc[2 * box + x] - wz;
-        dis[r] = sqrt(pow(dx, 2.0) + pow(dy, 2.0) + pow(dz, 2.0));
-        dens[r] = B.W[x];
-        const double u = B.W[box + x], v = B.W[2 * box + x], w = B.W[3 * box + x];
-        vv[r] = sqrt(u * u + v * v + w * w);
-        om[r] = B.omega[x];
-        ml[r] = sutherland(g, B.W[5 * box + x]);
-        const double t = fabs(om[r] * ml[r]);
-        if (ok[r] && t > tw) tw = t;
-        if (ok[r] && vv[r] > ud) ud = vv[r];
-    }
-    // the first cell (n0 = 1) sits at n = 1 (low wall) or n = N-1 (high wall)
-    const int n1 = hi ? N - 1 : 1, l1 = (n1 - 1) & 63, r1 = (n1 - 1) >> 6;
-    double d1 = 0.0, m1 = 0.0, t1 = 0.0;
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-        if (r == r1) {
-            d1 = __shfl(dens[r], l1);
-            m1 = __shfl(ml[r], l1);
-            t1 = __shfl(fabs(om[r] * ml[r]), l1);
-        }
-    double Tw = wave_max(tw);
-    if (t1 != t1) Tw = t1;   // a NaN first entry is kept by the reference's loop
-    const double Udif = wave_max(ud);
-    const double Ret = sqrt(d1 * Tw) / m1;
     const double A_plus = F32C(26.0), Cwk = F32C(1.0), kap = F32C(0.4), Ckleb = F32C(0.3), alpha = F32C(0.0168),
                  Ccp = F32C(1.6);
+    double dis[C], om[C], dens[C];
+    bool ok[C];
+    double tw = -1.0, ud = 0.0, t1 = 0.0, d1 = 0.0, m1 = 0.0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int n0 = 1 + grp + G * c;
+        ok[c] = active && n0 < M;
+        const long x = cell(ok[c] ? n0 : 1);
+        const double dx = B.cc[x] - wx, dy = B.cc[box + x] - wy, dz = B.cc[2 * box + x] - wz;
+        dis[c] = sqrt(pow(dx, 2.0) + pow(dy, 2.0) + pow(dz, 2.0));
+        om[c] = B.omega[x];
+        dens[c] = B.W[x];
+        const double ml = sutherland(g, B.W[5 * box + x]);
+        const double tv = fabs(om[c] * ml);
+        const double u = B.W[box + x], v = B.W[2 * box + x], ww = B.W[3 * box + x];
+        const double vv = sqrt(u * u + v * v + ww * ww);
+        if (ok[c] && tv > tw) tw = tv;
+        if (ok[c] && vv > ud) ud = vv;
+        if (c == 0 && grp == 0) {
+            t1 = tv;
+            d1 = dens[0];
+            m1 = ml;
+        }
+    }
+    s_a[grp][lane] = tw;
+    s_b[grp][lane] = ud;
+    __syncthreads();
+    if (grp == 0) {
+        double Tw = t1, Udif = 0.0;
+        for (int q = 0; q < G; ++q) {
+            if (s_a[q][lane] > Tw) Tw = s_a[q][lane];
+            if (s_b[q][lane] > Udif) Udif = s_b[q][lane];
+        }
+        s_line[0][lane] = Tw;
+        s_line[1][lane] = Udif;
+        s_line[2][lane] = sqrt(d1 * Tw) / m1;
+    }
+    __syncthreads();
+    const double Ret = s_line[2][lane];
     // largest positive F, smallest n0 on ties
     double bF = 0.0, bY = 0.0;
     int bN = 1 << 30;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const double y_plus = dis[r] * Ret;
-        const double F = dis[r] * fabs(om[r]) * (F32C(1.0) - exp(-y_plus / A_plus));
-        if (ok[r] && F > 0.0 && (F > bF || (F == bF && n0[r] < bN))) {
+    for (int c = 0; c < C; ++c) {
+        const double y_plus = dis[c] * Ret;
+        const double F = dis[c] * fabs(om[c]) * (F32C(1.0) - exp(-y_plus / A_plus));
+        const int n0 = 1 + grp + G * c;
+        if (ok[c] && F > 0.0 && (F > bF || (F == bF && n0 < bN))) {
             bF = F;
-            bY = dis[r];
-            bN = n0[r];
+            bY = dis[c];
+            bN = n0;
         }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        const double oF = __shfl_xor(bF, o), oY = __shfl_xor(bY, o);
-        const int oN = __shfl_xor(bN, o);
-        if (oF > bF || (oF == bF && oN < bN)) {
-            bF = oF;
-            bY = oY;
-            bN = oN;
-        }
+    s_a[grp][lane] = bF;
+    s_b[grp][lane] = bY;
+    s_n[grp][lane] = bN;
+    __syncthreads();
+    if (grp == 0) {
+        for (int q = 1; q < G; ++q)
+            if (s_a[q][lane] > bF || (s_a[q][lane] == bF && s_n[q][lane] < bN)) {
+                bF = s_a[q][lane];
+                bY = s_b[q][lane];
+                bN = s_n[q][lane];
+            }
+        const double Udif = s_line[1][lane];
+        const double wa = bY * bF, wb = Cwk * bY * Udif * Udif / bF;
+        s_line[3][lane] = bF;
+        s_line[4][lane] = bY;
+        s_line[5][lane] = wa < wb ? wa : wb;
     }
-    const double Fmax = bF, y_max = bY;
-    const double wa = y_max * Fmax, wb = Cwk * y_max * Udif * Udif / Fmax;
-    const double Fwake = wa < wb ? wa : wb;
+    __syncthreads();
+    const double y_max = s_line[4][lane], Fwake = s_line[5][lane];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        if (!ok[r]) continue;
-        const double y_plus = Ret * dis[r];
-        const double bL = kap * dis[r] * (F32C(1.0) - exp(-y_plus / A_plus));
-        const double mue_in = dens[r] * bL * bL * om[r];
-        const double Fkleb = F32C(1.0) / (F32C(1.0) + F32C(5.5) * pow((Ckleb * dis[r] / y_max), 6.0));
-        const double mue_out = alpha * Ccp * dens[r] * Fwake * Fkleb;
+    for (int c = 0; c < C; ++c) {
+        if (!ok[c]) continue;
+        const int n0 = 1 + grp + G * c;
+        const double y_plus = Ret * dis[c];
+        const double bL = kap * dis[c] * (F32C(1.0) - exp(-y_plus / A_plus));
+        const double mue_in = dens[c] * bL * bL * om[c];
+        const double Fkleb = F32C(1.0) / (F32C(1.0) + F32C(5.5) * pow((Ckleb * dis[c] / y_max), 6.0));
+        const double mue_out = alpha * Ccp * dens[c] * Fwake * Fkleb;
         const double mt = fabs(mue_in) > fabs(mue_out) ? mue_out : mue_in;
-        const long x = cell(1 + lane + 64 * r);
+        const long x = cell(n0);
         double nv;
         if (B.blflag[x] == 0) {
             B.blflag[x] = 1;
@@ -461,11 +480,16 @@ __global__ void __launch_bounds__(256) k_bl_wall(const BlockDev *__restrict__ bl
             const double cur = B.mut[x];
             nv = cur < mt ? cur : mt;
         }
-        // the first cell ends at zero (source/BL_Model.cpp:101-118)
-        B.mut[x] = n0[r] == 1 ? 0.0 : nv;
+        B.mut[x] = n0 == 1 ? 0.0 : nv;   // the first cell ends at zero (source/BL_Model.cpp:101-118)
     }
-    // ghost cell: zero, then the negated first-cell value WallFlux writes
-    if (lane == 0) B.mut[cell(hi ? N : 0)] = -0.0;
+    if (active && grp == 0) {
+        const long xg = D == 0 ? cidx(B, hi ? N : 0, a, cc) : D == 1 ? cidx(B, a, hi ? N : 0, cc) : cidx(B, a, cc, hi ? N : 0);
+        B.mut[xg] = -0.0;
+    }
 }
+
+template __global__ void k_bl_lines<16>(const BlockDev *, const PatchDev *, const int2 *, int, Gas);
+template __global__ void k_bl_lines<32>(const BlockDev *, const PatchDev *, const int2 *, int, Gas);
+template __global__ void k_bl_lines<64>(const BlockDev *, const PatchDev *, const int2 *, int, Gas);
 
 }  // namespace fv3d

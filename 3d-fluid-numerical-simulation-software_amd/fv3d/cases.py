@@ -171,7 +171,7 @@ def annular_passage(ni: int = 8, nj: int = 8, nk_blocks: Sequence[int] = (16,), 
                     beta: float = 1.5, cfl: float = 5.0, max_iter: int = 500, flux: int = 300,
                     recon_scheme: int = 500, recon_var: int = 200, p_ratio: float = 0.95,
                     viscous: bool = False, len_ref: float = 1.0, rotate_walls: bool = True,
-                    periodic_from_high: bool = False) -> Case:
+                    periodic_from_high: bool = False, stagger: float = 0.0) -> Case:
     """Annular-sector blade passage: i radial (hub I- to tip I+, tanh-clustered),
     j pitchwise over one blade pitch 2*pi/nblade, k axial.  The passage is cut
     along k into blocks of ``nk_blocks`` cells joined by interior patches
@@ -181,7 +181,10 @@ def annular_passage(ni: int = 8, nj: int = 8, nk_blocks: Sequence[int] = (16,), 
     are walls; elsewhere J- and J+ of a block are periodic (orientation 2).
     Hub and tip are walls (rotating with the block when ``rotate_walls``).
     Inlet K- of the first block, outlet K+ of the last.  All blocks share one
-    rotation speed ``rpm`` and blade count.  ``periodic_from_high`` lists the
+    rotation speed ``rpm`` and blade count.  ``stagger`` (degrees) shears the
+    whole passage helically: at radius r the pitchwise position advances by
+    tan(stagger) * (z - z_mid) / r, so the blade and the periodic surfaces meet
+    a relative inflow at that angle.  ``periodic_from_high`` lists the
     periodic pair from J+ to J- (its second side then lies at the smaller
     angle, which makes the reference flip the block's pitch angle every time
     it applies the condition, source/BndCondInteriorPeriodic.cpp:54-59)."""
@@ -206,7 +209,9 @@ def annular_passage(ni: int = 8, nj: int = 8, nk_blocks: Sequence[int] = (16,), 
             th0 = pitch * ((h - rad) + np.sqrt(np.maximum(rad * rad - (t - 0.5) ** 2, 0.0)))
         else:
             th0 = np.zeros(km)
-        th = th0[None, None, :] + pitch * np.linspace(0.0, 1.0, jm)[None, :, None]
+        zmid = 0.5 * (z_extent[0] + z_extent[-1])
+        shear = np.tan(np.radians(stagger)) * (z[None, None, :] - zmid) / r[:, None, None]
+        th = th0[None, None, :] + pitch * np.linspace(0.0, 1.0, jm)[None, :, None] + shear
         xyz = np.empty((im, jm, km, 3))
         rr = r[:, None, None]
         xyz[..., 0] = rr * np.cos(th)

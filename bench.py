@@ -1,22 +1,31 @@
-"""Throughput of the LU-SGS iteration on one MI355X.
+"""Throughput and time to a 1e-6 residual drop of the LU-SGS iteration on one
+MI355X, the chosen form (FP32 defect correction: FP32 LU-SGS operator over an
+FP64 residual) against plain FP64.
 
-A step is one full LU-SGS iteration (ghost fill, spectral radii, local time
-step, MUSCL + Roe residual, forward and backward hyperplane sweeps, update,
-residual norm, section integrals and the CFL ramp) over the whole synthetic
-grid, with the state resident in HBM.  Prints one JSON line.
+A step is one full LU-SGS iteration (ghost fill incl. interior/periodic
+exchange, vorticity and Baldwin-Lomax, spectral radii, local time step,
+MUSCL + Roe residual, viscous flux, forward and backward hyperplane sweeps,
+update, residual norm, section integrals and the CFL ramp) over the whole
+synthetic grid, with the state resident in HBM.  Prints one JSON line.
 
-The workload is named in ``config.workload``.  ``cpu_baseline`` times the
-reference solver itself (compiled from its sources into oracle/_ref) on the
-host, single-threaded: with more than one OpenMP thread the reference's face
-loops race on a shared variable (source/Upwind_Convective.cpp:7,29) and give
-wrong fluxes.
+Default workload: a five-block annular blade passage (64 x 64 cross-section,
+k-blocks of 12/40/128/60/24 cells, 1,081,344 cells), cambered blade walls,
+hub and tip walls, periodic pitch, interior patches between the blocks,
+Navier-Stokes with Baldwin-Lomax, Roe + MUSCL3 on primitive variables, local
+time step CFL 5.  ``value`` is the chosen form's cell-updates/s on it.  The
+time to a 1e-6 residual drop is measured on the 96x32x32 bump channel, which
+converges in both forms within seconds.
+
+``cpu_baseline`` times the reference solver itself (compiled from its sources
+into oracle/_ref) on the host, single-threaded: with more than one OpenMP
+thread the reference's face loops race on a shared variable
+(source/Upwind_Convective.cpp:7,29) and give wrong fluxes.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
-import subprocess
 import sys
 import tempfile
 import time
@@ -25,21 +34,19 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "3d-fluid-numerical-simulation-software_amd"))
 sys.path.insert(0, os.path.join(REPO, "oracle"))
 
-# Algorithmic HBM bytes per cell (FP64): every array a kernel semantically reads
-# or writes counted once, derived quantities (W, |S|, unit normals) as
-# recomputed, metrics as stored.
-BYTES_PER_CELL = {
-    "residual": 176,          # Q 5, S 9, nodes 3 -> R 5
-    "sweep": 232,             # one sweep: R|dQ* 5, Q 5, lamc 3, dt 1, vol 1, S 9 -> dQ 5
-    "sweep_ns": 256,          # the same plus lamv 3 (Navier-Stokes)
-    "update": 128,            # Q 5, dQ 5, vol 1 -> Q 5
-    "bc_spectral_dt": 152,    # Q 5, S 9, vol 1 -> lamc 3, dt 1
-}
-# HBM bytes one forward-sweep launch actually moves on a workload, from
-# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (copy-calibrated); the
-# summaries are under profiles/.  Keyed by (ni, nj, nk, sweep form, viscous).
-MEASURED_FWD_SWEEP_TRAFFIC = {(64, 64, 256, "rows", False): 472.5e6, (64, 64, 256, "rows", True): 506.0e6}
+METRIC = "cell-updates/s + time to 1e-6 residual drop, chosen vs plain FP64, 1 GPU"
+
+# Algorithmic HBM bytes per cell of one sweep launch: every array the sweep
+# semantically reads or writes counted once (R|dQ* 5, Q 5, lamc 3, dt 1,
+# vol 1, S 9 -> dQ 5; lamv 3 more for Navier-Stokes), in the operator's word
+# size (8 B plain FP64, 4 B for the FP32 operator).
+SWEEP_WORDS = {False: 29, True: 32}
+# HBM bytes one forward-sweep launch actually moves, from rocprofv3
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (copy-calibrated); the summaries are
+# under profiles/.  Keyed by (workload, precision).
+MEASURED_FWD_SWEEP_TRAFFIC = {("bump256x64x64", "fp64"): 472.5e6, ("bump256x64x64ns", "fp64"): 506.0e6}
 HBM_PEAK_GBS = 8000.0
+PREC = {"fp64": 0, "fp32": 1}
 
 
 def parse():
@@ -47,14 +54,17 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--ni", type=int, default=64)
-    ap.add_argument("--nj", type=int, default=64)
-    ap.add_argument("--nk", type=int, default=256)
+    ap.add_argument("--workload", choices=["passage5", "bump"], default="passage5")
+    ap.add_argument("--ni", type=int, default=64, help="bump workload: cells along i (span)")
+    ap.add_argument("--nj", type=int, default=64, help="bump workload: cells along j")
+    ap.add_argument("--nk", type=int, default=256, help="bump workload: cells along k (streamwise)")
+    ap.add_argument("--viscous", action="store_true", help="bump workload: laminar Navier-Stokes")
+    ap.add_argument("--precision", choices=["fp32", "fp64", "both"], default="both",
+                    help="which forms to time (the chosen form is fp32)")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--viscous", action="store_true", help="laminar Navier-Stokes (Reynolds number ~2e3 per unit length)")
     ap.add_argument("--sweep", choices=["rows", "kba3", "kba2", "kba", "plane"], default="rows")
-    ap.add_argument("--chunk", type=int, default=1)
-    ap.add_argument("--lag", type=int, default=0)
+    ap.add_argument("--no-drop", action="store_true", help="skip the time-to-1e-6-drop runs")
+    ap.add_argument("--drop-max-iter", type=int, default=40000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=40)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -95,6 +105,77 @@ def cpu_baseline(iters: int):
                       f"{t['iterations']} LU-SGS iterations, {t['loop_seconds']:.2f} s"}
 
 
+def workload(a):
+    from fv3d import cases
+    if a.workload == "bump":
+        case = cases.bump_channel(ni=a.ni, nj=a.nj, nk=a.nk, seed=1, max_iter=a.steps, viscous=a.viscous,
+                                  len_ref=1e-4 if a.viscous else 1.0)
+        name = f"{'laminar Navier-Stokes' if a.viscous else 'inviscid'} bump channel {a.nk}x{a.nj}x{a.ni} cells " \
+               f"(single block), Roe + MUSCL3 primitive, LU-SGS local dt CFL 5"
+        key = f"bump{a.nk}x{a.nj}x{a.ni}" + ("ns" if a.viscous else "")
+        return case, name, key, a.viscous
+    case = cases.annular_passage(ni=64, nj=64, nk_blocks=(12, 40, 128, 60, 24), seed=1, rpm=0.0, blade_blocks=(2,),
+                                 camber=0.1, viscous=True, len_ref=1.0, max_iter=a.steps,
+                                 z_extent=(-0.06, -0.035, -0.015, 0.035, 0.055, 0.08))
+    case.cfg.turbulence = 100
+    name = ("five-block annular blade passage, 64x64 cross-section, k-blocks 12/40/128/60/24 (1,081,344 cells; "
+            "block sizes 49k-524k), hub 0.2 m tip 0.25 m, 22 blades, cambered zero-thickness blade walls, "
+            "periodic pitch and interior patches, Navier-Stokes + Baldwin-Lomax, Roe + MUSCL3 primitive, LU-SGS "
+            "local dt CFL 5, single grid level, stationary")
+    return case, name, "passage5", True
+
+
+def time_form(case, precision, a):
+    """Whole-job throughput of one form and its per-kernel-class device times."""
+    from fv3d.solver import GpuSolver
+    dbg = {"plane": 2, "kba": 4, "kba2": 8, "kba3": 16, "rows": 0}[a.sweep]
+    g = GpuSolver(case, device=0, precision=PREC[precision], debug=dbg)
+    graph = not a.no_graph
+    g.iterate(a.warmup, graph=graph)
+    g.sync()
+    t0 = time.perf_counter()
+    hist = g.iterate(a.steps, graph=graph)
+    g.sync()
+    secs = time.perf_counter() - t0
+    ncell = g.n_cells
+    # per-kernel device times from HIP events on the library's stream
+    g.set_timing(True)
+    g.iterate(max(5, min(a.steps, 20)), graph=False)
+    ms, _ = g.timing()
+    g.set_timing(False)
+    g.close()
+    classes = ["residual", "lusgs", "update", "bc_spectral_dt", "finalize", "forward_sweep", "backward_sweep"]
+    return {"value": ncell * a.steps / secs, "ms_per_step": 1e3 * secs / a.steps, "cells": ncell,
+            "class_ms": {cl: ms[i] for i, cl in enumerate(classes)}, "final_res_dens": float(hist[-1, 1])}
+
+
+def time_to_drop(precision, max_iter, chunk=500):
+    """Wall time and iterations until the density residual is 1e-6 of its
+    first value, C0 bump channel 96x32x32, iterations replayed from one HIP
+    graph; the host reads the history once per chunk."""
+    import numpy as np
+    from fv3d import cases
+    from fv3d.solver import GpuSolver
+    case = cases.bump_channel(ni=32, nj=32, nk=96, seed=1, max_iter=max_iter)
+    g = GpuSolver(case, device=0, precision=PREC[precision])
+    h = g.iterate(1, graph=True)   # captures the graph; the first residual is the reference level
+    r0 = float(h[0, 1])
+    g.sync()
+    t0 = time.perf_counter()
+    it, hit = 1, None
+    while it < max_iter and hit is None:
+        h = g.iterate(chunk, graph=True)
+        k = np.nonzero(h[:, 1] <= 1e-6 * r0)[0]
+        if len(k):
+            hit = it + int(k[0]) + 1
+        it += chunk
+    g.sync()
+    secs = time.perf_counter() - t0
+    g.close()
+    return {"iterations": hit, "seconds": secs if hit else None, "res0": r0,
+            "note": None if hit else f"not reached in {max_iter} iterations"}
+
+
 def main():
     a = parse()
     if a.calibrate_copy:
@@ -104,65 +185,48 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
                           "n_gpus": a.gpus}))
         sys.exit(2)
-    from fv3d import cases
-    from fv3d.solver import GpuSolver
-
-    case = cases.bump_channel(ni=a.ni, nj=a.nj, nk=a.nk, seed=1, max_iter=a.steps, viscous=a.viscous,
-                              len_ref=1e-4 if a.viscous else 1.0)
-    g = GpuSolver(case, device=0,
-                  debug={"plane": 2, "kba": 4, "kba2": 8, "kba3": 16, "rows": 0}[a.sweep] | (a.chunk << 8) | (a.lag << 16))
-    ncell = g.n_cells
-    graph = not a.no_graph
-    g.iterate(a.warmup, graph=graph)
-    g.sync()
-    t0 = time.perf_counter()
-    hist = g.iterate(a.steps, graph=graph)
-    g.sync()
-    t1 = time.perf_counter()
-    secs = t1 - t0
-    value = ncell * a.steps / secs
-    # per-kernel device times from HIP events on the library's stream
-    g.set_timing(True)
-    g.iterate(max(5, min(a.steps, 20)), graph=False)
-    ms, _ = g.timing()
-    g.set_timing(False)
-    classes = ["residual", "lusgs", "update", "bc_spectral_dt", "finalize"]
-    times = {cl: ms[i] for i, cl in enumerate(classes)}
-    if ms[5] > 0:
-        # dominant kernel: the forward sweep, timed alone with HIP events on the
-        # library's stream
-        kname, kms = "k_lusgs_sl<forward> (LU-SGS forward sweep)", ms[5]
-        kbytes = BYTES_PER_CELL["sweep_ns" if a.viscous else "sweep"]
-        times["forward_sweep"], times["backward_sweep"] = ms[5], ms[6]
-        traffic = MEASURED_FWD_SWEEP_TRAFFIC.get((a.ni, a.nj, a.nk, a.sweep, a.viscous))
-    else:
-        kname, kms = "LU-SGS forward+backward sweeps", ms[1]
-        kbytes = 2 * BYTES_PER_CELL["sweep_ns" if a.viscous else "sweep"]
-        traffic = None
-    achieved = kbytes * ncell / (kms * 1e-3) / 1e9
+    case, wname, wkey, viscous = workload(a)
+    forms = ["fp32", "fp64"] if a.precision == "both" else [a.precision]
+    res = {p: time_form(case, p, a) for p in forms}
+    chosen = "fp32" if "fp32" in res else "fp64"
+    r = res[chosen]
+    # dominant kernel: the forward sweep of the chosen form, timed alone with HIP
+    # events on the library's stream (the only stream its kernels run on)
+    words = SWEEP_WORDS[viscous]
+    kbytes = words * (4 if chosen == "fp32" else 8)
+    kms = r["class_ms"]["forward_sweep"]
+    achieved = kbytes * r["cells"] / (kms * 1e-3) / 1e9
+    drop = None
+    if not a.no_drop:
+        drop = {p: time_to_drop(p, a.drop_max_iter) for p in forms}
+        drop["workload"] = "C0 bump channel 96x32x32 (98,304 cells), inviscid, Roe + MUSCL3, CFL 5 ramp"
+        if "fp32" in drop and "fp64" in drop and drop["fp32"]["seconds"] and drop["fp64"]["seconds"]:
+            drop["speedup_chosen_vs_plain"] = drop["fp64"]["seconds"] / drop["fp32"]["seconds"]
     cpu = None if a.no_cpu_baseline else cpu_baseline(a.cpu_iters)
     out = {
-        "metric": "cell-updates/s",
-        "value": value,
+        "metric": METRIC,
+        "value": r["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * secs / a.steps,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64 residual/update, f32 LU-SGS operator" if chosen == "fp32" else "f64",
         "data": "synthetic",
-        "config": {"workload": f"{'laminar Navier-Stokes' if a.viscous else 'inviscid'} bump channel "
-                               f"{a.nk}x{a.nj}x{a.ni} cells (single block), Roe + MUSCL3 "
-                               f"primitive, LU-SGS local dt CFL 5, plain FP64",
-                   "cells": ncell, "hip_graph": graph, "sweep": a.sweep, "chunk": a.chunk, "lag": a.lag},
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel_ms_avg": kms,
-                     "bytes_per_cell": kbytes, "class_ms": times},
+        "config": {"workload": wname, "cells": r["cells"], "hip_graph": not a.no_graph, "sweep": a.sweep,
+                   "chosen_form": "FP32 defect correction" if chosen == "fp32" else "plain FP64"},
+        "plain_fp64": ({"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
+                        "class_ms": res["fp64"]["class_ms"]} if "fp64" in res else None),
+        "time_to_1e-6_drop": drop,
+        "roofline": {"bound": "hbm", "kernel": f"k_lusgs_sl<forward, {chosen}> (LU-SGS forward sweep)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_FWD_SWEEP_TRAFFIC.get((wkey, chosen)), "kernel_ms_avg": kms,
+                     "bytes_per_cell": kbytes, "class_ms": r["class_ms"]},
         "cpu_baseline": cpu,
-        "final_res_dens": float(hist[-1, 1]),
+        "final_res_dens": r["final_res_dens"],
     }
     print(json.dumps(out))
 
