@@ -44,7 +44,8 @@ SWEEP_WORDS = {False: 29, True: 32}
 # HBM bytes one forward-sweep launch actually moves, from rocprofv3
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (copy-calibrated); the summaries are
 # under profiles/.  Keyed by (workload, precision).
-MEASURED_FWD_SWEEP_TRAFFIC = {("bump256x64x64", "fp64"): 472.5e6, ("bump256x64x64ns", "fp64"): 506.0e6}
+MEASURED_FWD_SWEEP_TRAFFIC = {("bump256x64x64", "fp64"): 472.5e6, ("bump256x64x64ns", "fp64"): 506.0e6,
+                              ("passage5", "fp32"): 312.2e6, ("passage5", "fp64"): 568.9e6}
 HBM_PEAK_GBS = 8000.0
 PREC = {"fp64": 0, "fp32": 1}
 
