@@ -1,7 +1,7 @@
 // fv3d_run: runs a reference case directory on the GPU through the C++ class
 // layer, in place of the reference's executable (source/Structured_FV3D.cpp:19-214).
 //
-//   fv3d_run <case_dir> [iterations] [--history FILE] [--eager] [--device N]
+//   fv3d_run <case_dir> [iterations] [--history FILE] [--eager] [--device N] [--fp32]
 //   fv3d_run <case_dir> --parse-only
 //
 // The case directory holds config.dat, the grid and boundary-definition files
@@ -10,7 +10,8 @@
 // iteration whose density residual is at or below the config's convergence
 // tolerance with inlet and outlet mass flows within 0.3 %, as the reference's
 // main loop does (the device may have run up to the end of that chunk).  --parse-only reads the case files and
-// prints a JSON summary without touching the GPU.
+// prints a JSON summary without touching the GPU.  --fp32 runs the LU-SGS
+// operator in FP32 over the FP64 residual (defect correction; same fixed point).
 #include <cinttypes>
 #include <cmath>
 #include <cstdio>
@@ -67,17 +68,18 @@ void print_summary(const fv3d::CaseConfig &cc, const fv3d::BndTable &bnd, const 
 int main(int argc, char **argv)
 {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: %s <case_dir> [iterations] [--history FILE] [--eager] [--device N] [--parse-only]\n",
+        std::fprintf(stderr, "usage: %s <case_dir> [iterations] [--history FILE] [--eager] [--device N] [--fp32] [--parse-only]\n",
                      argv[0]);
         return 2;
     }
     std::string dir = argv[1], history;
     int n_iter = -1, device = 0;
-    bool eager = false, parse_only = false;
+    bool eager = false, parse_only = false, fp32 = false;
     for (int a = 2; a < argc; ++a) {
         if (!std::strcmp(argv[a], "--history") && a + 1 < argc) history = argv[++a];
         else if (!std::strcmp(argv[a], "--eager")) eager = true;
         else if (!std::strcmp(argv[a], "--parse-only")) parse_only = true;
+        else if (!std::strcmp(argv[a], "--fp32")) fp32 = true;
         else if (!std::strcmp(argv[a], "--device") && a + 1 < argc) device = std::atoi(argv[++a]);
         else n_iter = std::atoi(argv[a]);
     }
@@ -94,6 +96,7 @@ int main(int argc, char **argv)
             return 0;
         }
         cc.raw.device = device;
+        if (fp32) cc.raw.precision = FV3D_DEFECT_FP32;
         fv3d::CMesh mesh(cc.raw);
         fv3d::LoadCase(mesh, dir, bnd, nodes, dims, st);
         fv3d::CSolver solver(mesh, !eager);
