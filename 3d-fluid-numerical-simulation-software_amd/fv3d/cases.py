@@ -150,8 +150,9 @@ def from_meta(meta: Dict) -> Case:
     """The case a fixture's metadata describes: ``kwargs`` for bump_channel (or
     for annular_passage when ``generator`` names it, with ``turn`` a list of
     (block, quarter) re-indexings) and the iteration count, optionally
-    ``wall_shear`` (with_wall_shear) and ``turbulence`` (the config's
-    turbulence model code)."""
+    ``wall_shear`` (with_wall_shear), ``turbulence`` (the config's
+    turbulence model code) and ``cfg`` (other config fields by name, e.g. the
+    two-grid settings)."""
     if meta.get("generator") == "annular_passage":
         case = annular_passage(max_iter=meta["iterations"], **meta["kwargs"])
         for blk, quarter in meta.get("turn", []):
@@ -162,6 +163,10 @@ def from_meta(meta: Dict) -> Case:
         case = with_wall_shear(case)
     if meta.get("turbulence"):
         case.cfg.turbulence = meta["turbulence"]
+    for k, v in meta.get("cfg", {}).items():
+        if not hasattr(case.cfg, k):
+            raise KeyError(k)
+        setattr(case.cfg, k, tuple(v) if isinstance(v, list) else v)
     return case
 
 

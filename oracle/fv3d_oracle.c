@@ -60,6 +60,9 @@ typedef struct {
     int flag_low_dt;
     patch_t *patch;
     int npatch;
+    /* coarse level of the two-grid cycle (source/Mesh.cpp:109-152): restricted
+     * state cv_2h, restricted residual Res_2h and the correction delt_cv */
+    cv_t *cv2h, *res2h, *delt;
 } blk_t;
 
 struct orc_ctx {
@@ -73,6 +76,10 @@ struct orc_ctx {
     blk_t blk[64];
     conn_t conn[256];
     int nconn;
+    int first_in[2], first_out[2];   /* block, patch index of BndGeom.Inlet[1] / Outlet[1] */
+    int ng;                          /* grid levels in use */
+    int force;                       /* coarse level: UseForceFunction (source/LU_SGS_Marching.cpp:49-57) */
+    struct orc_ctx *coarse;
 };
 
 static void curl(blk_t *b);
@@ -175,6 +182,8 @@ orc_ctx *orc_create(const orc_config *cfg)
     orc_nondimensionalize(cfg, &c->nd);
     c->cfl = (float)cfg->cfl;
     c->cfl_res = F32(1.0E-3);     /* source/Solver.cpp:21 */
+    c->first_in[0] = c->first_out[0] = -1;
+    c->ng = 1;
     return c;
 }
 
@@ -188,11 +197,13 @@ static void blk_free(blk_t *b)
     free(b->dt); free(b->mut); free(b->omega); free(b->fv);
     for (int d = 0; d < 6; ++d) { free(b->dcv[d]); free(b->dpv[d]); }
     free(b->patch);
+    free(b->cv2h); free(b->res2h); free(b->delt);
 }
 
 void orc_destroy(orc_ctx *c)
 {
     if (!c) return;
+    orc_destroy(c->coarse);
     for (int n = 0; n < c->nb; ++n) blk_free(&c->blk[n]);
     free(c);
 }
@@ -276,6 +287,8 @@ int orc_patch_add(orc_ctx *c, int kind, int block, int face, const int *rng, int
     p->kind = kind; p->face = face; p->rotate = rotate;
     p->is = rng[0]; p->ie = rng[1]; p->js = rng[2]; p->je = rng[3]; p->ks = rng[4]; p->ke = rng[5];
     p->rot = 0.0;
+    if (kind == 0 && c->first_in[0] < 0) { c->first_in[0] = block - 1; c->first_in[1] = b->npatch - 1; }
+    if (kind == 1 && c->first_out[0] < 0) { c->first_out[0] = block - 1; c->first_out[1] = b->npatch - 1; }
     return 0;
 }
 
@@ -431,6 +444,85 @@ static void jacobi_convert_coeff(blk_t *b)
 #undef ND3
 }
 
+/* ------------------------------------------------------- grid levels */
+/* One patch range of CUserInput::NumberOfMultigrid (source/UserInput.cpp:
+ * 816-844): tangential cell ranges become node ranges (end + 1, unless the range
+ * is a single index), and the level halves only if every start and end node is
+ * odd.  The block loop of the same function (1020-1040) never lowers the count
+ * below two, so with the two-level cap only the patches decide. */
+static int range_coarsens(const int *r)
+{
+    int q[6];
+    memcpy(q, r, sizeof q);
+    for (int a = 0; a < 3; ++a)
+        if (q[2 * a] != q[2 * a + 1]) q[2 * a + 1] = q[2 * a + 1] + 1;
+    for (int e = 0; e < 6; ++e)
+        if ((q[e] - 1) % 2 != 0) return 0;
+    return 1;
+}
+
+/* Patch ranges of the coarse level (source/UserInput.cpp:446-797): a single
+ * index n becomes (n+1)/2, a cell range s..e becomes (s+1)/2 .. e/2. */
+static void coarse_range(const int *r, int *o)
+{
+    for (int a = 0; a < 3; ++a) {
+        int s = r[2 * a], e = r[2 * a + 1];
+        o[2 * a] = (s + 1) / 2;
+        o[2 * a + 1] = s == e ? (e + 1) / 2 : e / 2;
+    }
+}
+
+static int build_coarse(orc_ctx *c)
+{
+    orc_config cc = c->cfg;
+    cc.multigrid = 0;
+    cc.turbulence = 0;   /* the turbulence model runs on the fine level only (source/ComputeAllBlockVars.cpp:230) */
+    orc_ctx *m = orc_create(&cc);
+    for (int n = 0; n < c->nb; ++n) {
+        const blk_t *b = &c->blk[n];
+        /* source/UserInput.cpp:296-331 */
+        int IM = (b->IM + 1) / 2, JM = (b->JM + 1) / 2, KM = (b->KM + 1) / 2;
+        double *nodes = (double *)malloc(sizeof(double) * 3 * (size_t)IM * JM * KM);
+        for (int i = 1; i <= IM; ++i)
+            for (int j = 1; j <= JM; ++j)
+                for (int k = 1; k <= KM; ++k) {
+                    v3_t v = b->node[C3(b, 2 * i - 1, 2 * j - 1, 2 * k - 1)];
+                    double *p = nodes + (((long)(i - 1) * JM + (j - 1)) * KM + (k - 1)) * 3;
+                    p[0] = v.x; p[1] = v.y; p[2] = v.z;
+                }
+        orc_block_add(m, IM, JM, KM, nodes, b->nblade, 0.0);
+        free(nodes);
+        blk_t *mb = &m->blk[n];
+        /* source/UserInput.cpp:449-454 copies the pitch and the rotation speed */
+        mb->w_rot = b->w_rot;
+        mb->delta_th = b->delta_th;
+        long box = (long)(IM + 1) * (JM + 1) * (KM + 1);
+        mb->cv2h = (cv_t *)calloc(box, sizeof(cv_t));
+        mb->res2h = (cv_t *)calloc(box, sizeof(cv_t));
+        mb->delt = (cv_t *)calloc(box, sizeof(cv_t));
+    }
+    for (int n = 0; n < c->nb; ++n)
+        for (int q = 0; q < c->blk[n].npatch; ++q) {
+            const patch_t *P = &c->blk[n].patch[q];
+            int r[6] = {P->is, P->ie, P->js, P->je, P->ks, P->ke}, o[6];
+            coarse_range(r, o);
+            orc_patch_add(m, P->kind, n + 1, P->face, o, P->rotate);
+        }
+    m->first_in[0] = c->first_in[0]; m->first_in[1] = c->first_in[1];
+    m->first_out[0] = c->first_out[0]; m->first_out[1] = c->first_out[1];
+    for (int t = 0; t < c->nconn; ++t) {
+        const conn_t *T = &c->conn[t];
+        int o1[6], o2[6];
+        coarse_range(T->rng[0], o1);
+        coarse_range(T->rng[1], o2);
+        if (orc_connection_add(m, T->kind, T->orient, T->blk[0] + 1, T->face[0], o1, T->blk[1] + 1, T->face[1], o2))
+            return -1;
+    }
+    orc_geometry(m);
+    c->coarse = m;
+    return 0;
+}
+
 int orc_geometry(orc_ctx *c)
 {
     c->total_vol = 0.0;
@@ -439,6 +531,116 @@ int orc_geometry(orc_ctx *c)
         c->total_vol = c->total_vol + c->blk[n].cellvol;   /* source/ComputeAllBlockVars.cpp:25-36 */
     }
     for (int n = 0; n < c->nb; ++n) jacobi_convert_coeff(&c->blk[n]);
+    c->ng = 1;
+    if (c->cfg.multigrid) {
+        int ok = 1;
+        for (int n = 0; n < c->nb; ++n)
+            for (int q = 0; q < c->blk[n].npatch; ++q) {
+                const patch_t *P = &c->blk[n].patch[q];
+                int r[6] = {P->is, P->ie, P->js, P->je, P->ks, P->ke};
+                ok = ok && range_coarsens(r);
+            }
+        /* the two-sided patches are read with their first side only (source/UserInput.cpp:932, 962) */
+        for (int t = 0; t < c->nconn; ++t) ok = ok && range_coarsens(c->conn[t].rng[0]);
+        if (ok) {
+            if (build_coarse(c)) return -1;
+            c->ng = 2;
+        }
+    }
+    return 0;
+}
+
+int orc_levels(orc_ctx *c) { return c->ng; }
+orc_ctx *orc_coarse(orc_ctx *c) { return c->coarse; }
+
+/* ------------------------------------------------------- start field */
+/* CBlocks::Initial_Uniform (source/Initialization.cpp:17-72) */
+static void initial_uniform(orc_ctx *c, blk_t *b)
+{
+    double Gamma = c->nd.gamma, Rgas = c->nd.rgas, Ptin = c->nd.pt_in, Ttin = c->nd.tt_in;
+    double Ma0 = (float)c->cfg.ma0;
+    const double *dir = c->cfg.inlet_dir;
+    double m2 = F32(1.0) + (Gamma - F32(1.0)) / F32(2.0) * Ma0 * Ma0;
+    double press = Ptin * pow(m2, F32(0.0) - Gamma / (Gamma - F32(1.0)));
+    double temp = Ttin * pow(m2, (double)(0.0f - 1.0f));
+    double dens = press / (Rgas * temp);
+    double Vel = pow(Gamma * Rgas * temp, F32(0.5)) * Ma0;
+    double n_m = pow(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2], 0.5);
+    for (int i = 1; i <= b->IM - 1; ++i)
+        for (int j = 1; j <= b->JM - 1; ++j)
+            for (int k = 1; k <= b->KM - 1; ++k) {
+                long x = C3(b, i, j, k);
+                double u = Vel * dir[0] / n_m, v = Vel * dir[1] / n_m, w = Vel * dir[2] / n_m;
+                pv_t p = {dens, u, v, w, press, temp};
+                b->pv[x] = p;
+                double V2 = u * u + v * v + w * w;
+                double ener = press / (dens * (Gamma - F32(1.0))) + V2 / F32(2.0);
+                cv_t q = {dens, dens * u, dens * v, dens * w, dens * ener};
+                b->cv[x] = q;
+            }
+}
+
+/* CMesh::Initial_Interpolate (source/Initialization.cpp:74-165): pressure and
+ * flow direction linear in z between the first inlet and the first outlet. */
+static void initial_interpolate(orc_ctx *c)
+{
+    double Gamma = c->nd.gamma, Rgas = c->nd.rgas, Ptin = c->nd.pt_in, Ttin = c->nd.tt_in, Pout = c->nd.p_out;
+    double P21ratio = (float)c->cfg.p21ratio;
+    const double *din = c->cfg.inlet_dir, *dout = c->cfg.outlet_dir;
+    const blk_t *bi = &c->blk[c->first_in[0]], *bo = &c->blk[c->first_out[0]];
+    int ks = bi->patch[c->first_in[1]].ks, ke = bo->patch[c->first_out[1]].ke;
+    double zmin = F32(1E+32), zmax = F32(-1E+32);
+    for (int i = 1; i <= bi->IM - 1; ++i)
+        for (int j = 1; j <= bi->JM - 1; ++j) {
+            double z = bi->fk[C3(bi, i, j, ks)].c.z;
+            zmin = z < zmin ? z : zmin;
+        }
+    for (int i = 1; i <= bo->IM - 1; ++i)
+        for (int j = 1; j <= bo->JM - 1; ++j) {
+            double z = bo->fk[C3(bo, i, j, ke)].c.z;
+            zmax = z > zmax ? z : zmax;
+        }
+    double dz = zmax - zmin;
+    double Pin = Pout / P21ratio;
+    if (Pin >= Ptin) Pin = F32(0.99) * Ptin;
+    double dp = Pout - Pin;
+    double dB[3] = {dout[0] - din[0], dout[1] - din[1], dout[2] - din[2]};
+    double temp = Ttin * pow(Pin / Ptin, (Gamma - F32(1.0)) / Gamma);
+    double dens0 = Pin / (Rgas * temp);
+    double cs = sqrt(Gamma * Pin / dens0);
+    double Ma = sqrt(F32(2.0) * ((Ttin / temp) - F32(1.0)) / (Gamma - F32(1.0)));
+    double Vel = Ma * cs;
+    for (int n = 0; n < c->nb; ++n) {
+        blk_t *b = &c->blk[n];
+        for (int i = 1; i <= b->IM - 1; ++i)
+            for (int j = 1; j <= b->JM - 1; ++j)
+                for (int k = 1; k <= b->KM - 1; ++k) {
+                    long x = C3(b, i, j, k);
+                    double z = b->cell[x].c.z;
+                    double B[3];
+                    for (int a = 0; a < 3; ++a) B[a] = din[a] + dB[a] * (z - zmin) / dz;
+                    double press = Pin + dp * (z - zmin) / dz;
+                    double dens = press / Rgas / temp;
+                    double n_m = sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+                    double u = Vel * B[0] / n_m, v = Vel * B[1] / n_m, w = Vel * B[2] / n_m;
+                    pv_t p = {dens, u, v, w, press, temp};
+                    b->pv[x] = p;
+                    double V2 = u * u + v * v + w * w;
+                    double ener = press / (Gamma - F32(1.0)) + dens * V2 / F32(2.0);
+                    cv_t q = {dens, dens * u, dens * v, dens * w, ener};
+                    b->cv[x] = q;
+                }
+    }
+}
+
+int orc_initial(orc_ctx *c)
+{
+    orc_ctx *m = c->ng == 2 ? c->coarse : c;
+    if (m->first_in[0] < 0 || m->first_out[0] < 0) return -1;
+    if (c->cfg.initial_type == 0)
+        for (int n = 0; n < m->nb; ++n) initial_uniform(m, &m->blk[n]);
+    else
+        initial_interpolate(m);
     return 0;
 }
 
@@ -1674,7 +1876,24 @@ int orc_lusgs(orc_ctx *c)
             for (int j = 1; j <= b->JM - 1; ++j)
                 for (int k = 1; k <= b->KM - 1; ++k) b->cv_old[C3(b, i, j, k)] = b->cv[C3(b, i, j, k)];
     }
-    if (orc_resflux(c)) return -1;
+    if (!c->force) {
+        if (orc_resflux(c)) return -1;
+    } else {
+        /* CMesh::ComputeForceFunction / AddForceFunction (source/ComputeAllBlockVars.cpp:
+         * 269-309): the coarse residual becomes the restricted fine one, through
+         * Res + (Res_2h - Res); lambda, dt and the Dummy layers stay as the last
+         * coarse ResFlux left them */
+        for (int n = 0; n < c->nb; ++n) {
+            blk_t *b = &c->blk[n];
+            for (int i = 1; i <= b->IM - 1; ++i)
+                for (int j = 1; j <= b->JM - 1; ++j)
+                    for (int k = 1; k <= b->KM - 1; ++k) {
+                        long x = C3(b, i, j, k);
+                        cv_t qf = csub(b->res2h[x], b->res[x]);
+                        b->res[x] = cadd(b->res[x], qf);
+                    }
+        }
+    }
     for (int n = 0; n < c->nb; ++n) {
         blk_t *b = &c->blk[n];
         lusgs_sweeps(c, b);
@@ -1780,6 +1999,169 @@ int orc_perf(orc_ctx *c, double *out5)
 
 double orc_cfl(orc_ctx *c) { return c->cfl; }
 
+/* ------------------------------------------------------ two-grid cycle */
+/* CSolver::Prolong, fine to coarse (source/Prolong_Interpolate.cpp:7-50): the
+ * volume-weighted state and the summed residual of the eight fine cells. */
+static void mg_restrict(orc_ctx *f, orc_ctx *m)
+{
+    for (int n = 0; n < m->nb; ++n) {
+        blk_t *b = &f->blk[n], *B = &m->blk[n];
+        for (int i = 1; i <= B->IM - 1; ++i)
+            for (int j = 1; j <= B->JM - 1; ++j)
+                for (int k = 1; k <= B->KM - 1; ++k) {
+                    const int I[2] = {2 * i - 1, 2 * i}, J[2] = {2 * j - 1, 2 * j}, K[2] = {2 * k - 1, 2 * k};
+                    cv_t sq = {0, 0, 0, 0, 0}, sr = {0, 0, 0, 0, 0};
+                    int first = 1;
+                    for (int a = 0; a < 2; ++a)
+                        for (int bb = 0; bb < 2; ++bb)
+                            for (int cc = 0; cc < 2; ++cc) {
+                                long y = C3(b, I[a], J[bb], K[cc]);
+                                cv_t q = cscale(b->cv[y], b->cell[y].vol);
+                                sq = first ? q : cadd(sq, q);
+                                sr = first ? b->res[y] : cadd(sr, b->res[y]);
+                                first = 0;
+                            }
+                    long x = C3(B, i, j, k);
+                    B->cv2h[x] = cdiv(sq, B->cell[x].vol);
+                    B->res2h[x] = sr;
+                    B->cv[x] = B->cv2h[x];
+                }
+    }
+}
+
+/* CSolver::ComputeDeltConsVars (source/Prolong_Interpolate.cpp:159-176) */
+static void mg_delta(orc_ctx *m)
+{
+    for (int n = 0; n < m->nb; ++n) {
+        blk_t *B = &m->blk[n];
+        for (int i = 1; i <= B->IM - 1; ++i)
+            for (int j = 1; j <= B->JM - 1; ++j)
+                for (int k = 1; k <= B->KM - 1; ++k) {
+                    long x = C3(B, i, j, k);
+                    B->delt[x] = csub(B->cv[x], B->cv2h[x]);
+                }
+    }
+}
+
+/* CMesh::CornerTreatment (source/ComputeAllBlockVars.cpp:374-439) on one
+ * array: the twelve edges of the ghost box from the two neighbouring face
+ * ghosts, then the eight corners from three edges and three face ghosts. */
+static void corner_one(const blk_t *b, cv_t *a)
+{
+    int IM = b->IM, JM = b->JM, KM = b->KM;
+#define A(i, j, k) a[C3(b, i, j, k)]
+#define HALF(p, q) cscale(cadd(p, q), F32(0.5))
+    for (int i = 1; i <= IM - 1; ++i) {
+        A(i, 0, 0) = HALF(A(i, 1, 0), A(i, 0, 1));
+        A(i, JM, 0) = HALF(A(i, JM - 1, 0), A(i, JM, 1));
+        A(i, 0, KM) = HALF(A(i, 0, KM - 1), A(i, 1, KM));
+        A(i, JM, KM) = HALF(A(i, JM - 1, KM), A(i, JM, KM - 1));
+    }
+    for (int j = 1; j <= JM - 1; ++j) {
+        A(0, j, 0) = HALF(A(1, j, 0), A(0, j, 1));
+        A(0, j, KM) = HALF(A(1, j, KM), A(0, j, KM - 1));
+        A(IM, j, KM) = HALF(A(IM - 1, j, KM), A(IM, j, KM - 1));
+        A(IM, j, 0) = HALF(A(IM - 1, j, 0), A(IM, j, 1));
+    }
+    for (int k = 1; k <= KM - 1; ++k) {
+        A(0, 0, k) = HALF(A(1, 0, k), A(0, 1, k));
+        A(IM, 0, k) = HALF(A(IM, 1, k), A(IM - 1, 0, k));
+        A(0, JM, k) = HALF(A(1, JM, k), A(0, JM - 1, k));
+        A(IM, JM, k) = HALF(A(IM - 1, JM, k), A(IM, JM - 1, k));
+    }
+#define CORNER(i, j, k, i1, j1, k1) \
+    A(i, j, k) = cdiv(csub(cscale(cadd(cadd(A(i1, j, k), A(i, j1, k)), A(i, j, k1)), F32(2.0)), \
+                           cadd(cadd(A(i1, j1, k), A(i1, j, k1)), A(i, j1, k1))), F32(3.0))
+    CORNER(0, 0, 0, 1, 1, 1);
+    CORNER(IM, 0, 0, IM - 1, 1, 1);
+    CORNER(0, JM, 0, 1, JM - 1, 1);
+    CORNER(IM, JM, 0, IM - 1, JM - 1, 1);
+    CORNER(IM, 0, KM, IM - 1, 1, KM - 1);
+    CORNER(0, 0, KM, 1, 1, KM - 1);
+    CORNER(0, JM, KM, 1, JM - 1, KM - 1);
+    CORNER(IM, JM, KM, IM - 1, JM - 1, KM - 1);
+#undef CORNER
+#undef HALF
+#undef A
+}
+
+static void mg_corner(orc_ctx *m)
+{
+    for (int n = 0; n < m->nb; ++n) {
+        corner_one(&m->blk[n], m->blk[n].cv);
+        corner_one(&m->blk[n], m->blk[n].delt);
+    }
+}
+
+/* CSolver::Interpolate, coarse to fine (source/Prolong_Interpolate.cpp:52-156):
+ * weights 27, 9, 3, 1 (/64) on the nearest coarse cell, the three that share a
+ * face with it towards the fine cell, the three that share an edge and the one
+ * that shares a corner.  state = 1 sets the fine state from the coarse one
+ * (the end of the coarse phase), else the coarse correction is added. */
+static void mg_interpolate(orc_ctx *f, orc_ctx *m, int state)
+{
+    const double a1 = (double)(27.0f / 64.0f), a2 = (double)(9.0f / 64.0f), a3 = (double)(3.0f / 64.0f),
+                 a4 = (double)(1.0f / 64.0f);
+    for (int n = 0; n < f->nb; ++n) {
+        blk_t *b = &f->blk[n];
+        const blk_t *B = &m->blk[n];
+        const cv_t *src = state ? B->cv : B->delt;
+        for (int i = 1; i <= b->IM - 1; ++i)
+            for (int j = 1; j <= b->JM - 1; ++j)
+                for (int k = 1; k <= b->KM - 1; ++k) {
+                    int i1 = i % 2 == 0 ? i / 2 : i / 2 + 1, i2 = i % 2 == 0 ? i / 2 + 1 : i / 2;
+                    int j1 = j % 2 == 0 ? j / 2 : j / 2 + 1, j2 = j % 2 == 0 ? j / 2 + 1 : j / 2;
+                    int k1 = k % 2 == 0 ? k / 2 : k / 2 + 1, k2 = k % 2 == 0 ? k / 2 + 1 : k / 2;
+#define S(p, q, r) src[C3(B, p, q, r)]
+                    cv_t t1 = cscale(S(i1, j1, k1), a1);
+                    cv_t t2 = cscale(cadd(cadd(S(i1, j1, k2), S(i1, j2, k1)), S(i2, j1, k1)), a2);
+                    cv_t t3 = cscale(cadd(cadd(S(i2, j2, k1), S(i1, j2, k2)), S(i2, j1, k2)), a3);
+                    cv_t t4 = cscale(S(i2, j2, k2), a4);
+#undef S
+                    cv_t v = cadd(cadd(cadd(t1, t2), t3), t4);
+                    long x = C3(b, i, j, k);
+                    b->cv[x] = state ? v : cadd(b->cv[x], v);
+                }
+    }
+}
+
+static void prim_and_limit(orc_ctx *c)
+{
+    for (int n = 0; n < c->nb; ++n) primitive_vars(c, &c->blk[n]);
+    if (c->cfg.limiter)
+        for (int n = 0; n < c->nb; ++n) limit_vars(c, &c->blk[n]);
+}
+
+/* One iteration of source/Solver.cpp:61-103 (reuse_solution = N): the coarse
+ * phase up to Coarse_iter, then fine LU-SGS + coarse correction. */
+static int two_grid(orc_ctx *c)
+{
+    orc_ctx *m = c->coarse;
+    m->cfl = c->cfl;   /* CTimeDiscr::CFL is shared by the levels */
+    if (c->iter <= c->cfg.coarse_iter) {
+        if (orc_lusgs(m)) return -1;
+        orc_res_calculate(m, c->res);
+        if (c->iter == c->cfg.coarse_iter) {
+            mg_corner(m);
+            mg_interpolate(c, m, 1);
+            prim_and_limit(c);
+        }
+        return 0;
+    }
+    m->force = 1;
+    if (orc_lusgs(c)) return -1;
+    orc_res_calculate(c, 0);
+    if (orc_resflux(c)) return -1;
+    mg_restrict(c, m);
+    prim_and_limit(m);
+    if (orc_lusgs(m)) return -1;
+    mg_delta(m);
+    mg_corner(m);
+    mg_interpolate(c, m, 0);
+    prim_and_limit(c);
+    return 0;
+}
+
 /* source/Solver.cpp:47-66 and source/Structured_FV3D.cpp:142-158 */
 int orc_iterate(orc_ctx *c, int n, double *hist)
 {
@@ -1793,10 +2175,15 @@ int orc_iterate(orc_ctx *c, int n, double *hist)
             }
             if (c->cfl > cmax) c->cfl = cmax;
         }
-        orc_lusgs(c);
-        orc_res_calculate(c, 0);
         double p[5];
-        orc_perf(c, p);
+        if (c->ng == 2) {
+            if (two_grid(c)) return -1;
+            orc_perf(c->iter <= c->cfg.coarse_iter ? c->coarse : c, p);
+        } else {
+            if (orc_lusgs(c)) return -1;
+            orc_res_calculate(c, 0);
+            orc_perf(c, p);
+        }
         if (hist) {
             double *h = hist + (long)it * 7;
             h[0] = c->iter; h[1] = c->res[0]; h[2] = p[0]; h[3] = p[1]; h[4] = p[2]; h[5] = p[3]; h[6] = c->cfl;

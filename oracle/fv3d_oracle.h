@@ -37,6 +37,12 @@ typedef struct {
     int limiter;         /* limit flow variables */
     int global_dt;       /* 0 local time step */
     int nthreads;
+    /* two-grid cycle and start field (source/UserInput.cpp:50-58, 86-92) */
+    int multigrid;       /* UseMultigrid */
+    int coarse_iter;     /* Coarse_iter */
+    int initial_type;    /* 0 uniform, 1 interpolate (CMesh::Initial) */
+    double p21ratio;     /* outlet/inlet static pressure ratio of the interpolated start */
+    double outlet_dir[3];
 } orc_config;
 
 typedef struct {
@@ -68,7 +74,20 @@ int orc_patch_add(orc_ctx *c, int kind, int block, int face, const int *rng, int
 int orc_connection_add(orc_ctx *c, int kind, int orient, int block1, int face1, const int *rng1, int block2, int face2,
                        const int *rng2);
 
+/* Metrics of every block.  With cfg.multigrid it also decides the number of
+ * grid levels as CUserInput::NumberOfMultigrid does (source/UserInput.cpp:
+ * 799-1067, at most two) and builds the coarse level: nodes (2i-1, 2j-1,
+ * 2k-1) (241-334), patch ranges halved (446-797). */
 int orc_geometry(orc_ctx *c);
+
+/* Grid levels in use (1 or 2) and the coarse level's context (NULL for one). */
+int orc_levels(orc_ctx *c);
+orc_ctx *orc_coarse(orc_ctx *c);
+
+/* CMesh::Initial (source/ComputeAllBlockVars.cpp:38-48, source/Initialization.cpp)
+ * on the level the reference starts from without a restart file: the coarse
+ * level when two levels run, else the only one (source/Structured_FV3D.cpp:86-96). */
+int orc_initial(orc_ctx *c);
 
 /* Restart semantics of source/UserInput.cpp:1118-1159: pv is (ni,nj,nk,6)
  * rho,u,v,w,p,T and ener (ni,nj,nk), cell (i,j,k) at ((i-1)*nj+(j-1))*nk+(k-1). */
@@ -84,8 +103,11 @@ int orc_res_calculate(orc_ctx *c, double *res6);
 int orc_perf(orc_ctx *c, double *out5);   /* mass_in, mass_out, PR, eff, mass_ratio */
 double orc_cfl(orc_ctx *c);
 
-/* n iterations of CSolver::Solver (single grid) + performance parameters.
- * hist receives n rows of: iter res_dens mass_in mass_out PR eff CFL. */
+/* n iterations of CSolver::Solver (source/Solver.cpp:47-135: single grid, or
+ * the coarse start-up phase and two-grid cycle of reuse_solution = N) and the
+ * performance parameters of the level the reference reports
+ * (source/Structured_FV3D.cpp:148-158).  hist receives n rows of: iter
+ * res_dens mass_in mass_out PR eff CFL. */
 int orc_iterate(orc_ctx *c, int n, double *hist);
 
 /* Copies a field out.  Names: cv pv mut (full (IM+1)(JM+1)(KM+1)xC incl.

@@ -25,7 +25,9 @@ class _Cfg(C.Structure):
                 ("cfl", C.c_double), ("c_timestep", C.c_double), ("relax", C.c_double),
                 ("turbo_type", C.c_int), ("viscous", C.c_int), ("recon_var", C.c_int),
                 ("recon_scheme", C.c_int), ("flux_scheme", C.c_int), ("turbulence", C.c_int),
-                ("limiter", C.c_int), ("global_dt", C.c_int), ("nthreads", C.c_int)]
+                ("limiter", C.c_int), ("global_dt", C.c_int), ("nthreads", C.c_int),
+                ("multigrid", C.c_int), ("coarse_iter", C.c_int), ("initial_type", C.c_int),
+                ("p21ratio", C.c_double), ("outlet_dir", C.c_double * 3)]
 
 
 _lib = None
@@ -60,6 +62,10 @@ def lib():
         L.orc_test_viscous.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_test_sutherland.argtypes = [C.c_void_p, C.c_double]
         L.orc_test_sutherland.restype = C.c_double
+        L.orc_levels.argtypes = [C.c_void_p]
+        L.orc_coarse.argtypes = [C.c_void_p]
+        L.orc_coarse.restype = C.c_void_p
+        L.orc_initial.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -83,6 +89,12 @@ def config_struct(cfg) -> _Cfg:
     s.limiter = 1 if cfg.limiter.upper() == "Y" else 0
     s.global_dt = 0 if cfg.timestep.upper() == "L" else 1
     s.nthreads = 1
+    s.multigrid = 1 if cfg.use_multigrid.upper() == "Y" else 0
+    s.coarse_iter = cfg.coarse_iter
+    s.initial_type = 0 if cfg.initial.upper() == "U" else 1
+    s.p21ratio = cfg.p21_ratio
+    for c in range(3):
+        s.outlet_dir[c] = cfg.outlet_angle[c]
     return s
 
 
@@ -110,11 +122,17 @@ class Oracle:
                 raise NotImplementedError(f"oracle patch kind {p.kind}")
             r = np.ascontiguousarray(p.rng, dtype=np.int32)
             L.orc_patch_add(self.h, _KIND[p.kind], p.block, p.face, r.ctypes.data, p.rotate)
-        L.orc_geometry(self.h)
-        for b, st in enumerate(case.states):
-            pv = np.ascontiguousarray(np.stack([st["rho"], st["u"], st["v"], st["w"], st["p"], st["T"]], -1))
-            e = np.ascontiguousarray(st["rhoE"])
-            L.orc_set_state(self.h, b + 1, pv.ctypes.data, e.ctypes.data)
+        if L.orc_geometry(self.h) != 0:
+            raise ValueError("oracle rejected the grid levels")
+        if case.cfg.reuse_solution.upper() == "Y":
+            if L.orc_levels(self.h) == 2:
+                raise ValueError("two grid levels need the coarse start-up phase (reuse_solution = N)")
+            for b, st in enumerate(case.states):
+                pv = np.ascontiguousarray(np.stack([st["rho"], st["u"], st["v"], st["w"], st["p"], st["T"]], -1))
+                e = np.ascontiguousarray(st["rhoE"])
+                L.orc_set_state(self.h, b + 1, pv.ctypes.data, e.ctypes.data)
+        elif L.orc_initial(self.h) != 0:
+            raise ValueError("start field needs an inlet and an outlet patch")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -148,9 +166,19 @@ class Oracle:
     def sutherland(self, temp: float) -> float:
         return lib().orc_test_sutherland(self.h, temp)
 
-    def get(self, block: int, name: str) -> np.ndarray:
+    def levels(self) -> int:
+        return lib().orc_levels(self.h)
+
+    def get(self, block: int, name: str, level: int = 1) -> np.ndarray:
+        """A field of one block; level 2 reads the coarse level of the two-grid cycle."""
         bi = self.case.blocks[block - 1]
         im, jm, km = bi.im, bi.jm, bi.km
+        h = self.h
+        if level == 2:
+            h = lib().orc_coarse(self.h)
+            if not h:
+                raise ValueError("no coarse level")
+            im, jm, km = (im + 1) // 2, (jm + 1) // 2, (km + 1) // 2
         ni, nj, nk = im - 1, jm - 1, km - 1
         if name in ("cv", "pv"):
             shape = (im + 1, jm + 1, km + 1, 5 if name == "cv" else 6)
@@ -174,14 +202,14 @@ class Oracle:
         else:
             raise KeyError(name)
         out = np.zeros(shape)
-        n = lib().orc_get(self.h, block, name.encode(), out.ctypes.data)
+        n = lib().orc_get(h, block, name.encode(), out.ctypes.data)
         if n != out.size:
             raise RuntimeError(f"orc_get({name}) returned {n}, expected {out.size}")
         return out
 
 
 def run_reference(case_dir: str, n_iter: int, out_dir: str, threads: int = 1, dump_first: bool = False,
-                  dump: bool = True, timeout: Optional[float] = None) -> Dict:
+                  dump: bool = True, timeout: Optional[float] = None, dump_coarse: bool = False) -> Dict:
     """Runs the compiled reference (oracle/_ref/fv3d_ref) on a case directory."""
     if not os.path.exists(REF_BIN):
         raise RuntimeError(f"reference harness not built: {REF_BIN}")
@@ -191,11 +219,15 @@ def run_reference(case_dir: str, n_iter: int, out_dir: str, threads: int = 1, du
         args.append("--dump-first")
     if not dump:
         args.append("--no-dump")
+    if dump_coarse:
+        args.append("--dump-coarse")
     subprocess.run(args, check=True, timeout=timeout)
     hist = np.loadtxt(os.path.join(out_dir, "history.txt"), ndmin=2)
     with open(os.path.join(out_dir, "timing.json")) as f:
         timing = json.load(f)
-    return {"history": hist, "timing": timing}
+    with open(os.path.join(out_dir, "levels.txt")) as f:
+        levels = int(f.read())
+    return {"history": hist, "timing": timing, "levels": levels}
 
 
 def point_flux(scheme: int, gamma: float, ql, qr, que: float = 0.0) -> np.ndarray:

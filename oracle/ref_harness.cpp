@@ -18,7 +18,9 @@
 // (source/TimeStep.cpp:54-58).
 //
 // Usage: fv3d_ref CASE_DIR N_ITER OUT_DIR [--threads N] [--dump-first]
-//                 [--no-dump] [--quiet]
+//                 [--no-dump] [--dump-coarse] [--quiet]
+// --dump-coarse also writes the coarse level's final fields (final_c<block>_*)
+// when the two-grid cycle runs.
 #include "Solver.h"
 #include "Output.h"
 #include "UserInput.h"
@@ -171,12 +173,13 @@ int main(int argc, char **argv)
     int n_iter = std::atoi(argv[2]);
     std::string out_dir = argv[3];
     int threads = -1;
-    bool dump_first = false, dump = true, quiet = false;
+    bool dump_first = false, dump = true, quiet = false, dump_coarse = false;
     for (int a = 4; a < argc; ++a) {
         if (!std::strcmp(argv[a], "--threads") && a + 1 < argc) threads = std::atoi(argv[++a]);
         else if (!std::strcmp(argv[a], "--dump-first")) dump_first = true;
         else if (!std::strcmp(argv[a], "--no-dump")) dump = false;
         else if (!std::strcmp(argv[a], "--quiet")) quiet = true;
+        else if (!std::strcmp(argv[a], "--dump-coarse")) dump_coarse = true;
     }
     char cwd[4096];
     if (!getcwd(cwd, sizeof cwd)) return 3;
@@ -239,6 +242,11 @@ int main(int argc, char **argv)
     if (dump)
         for (int nb = 1; nb <= CMesh::NB; ++nb)
             dump_block(out_dir, "final_b" + std::to_string(nb), Solver.Mesh[1].Block[nb]);
+    if (dump && dump_coarse && Solver.NG == 2)
+        for (int nb = 1; nb <= CMesh::NB; ++nb)
+            dump_block(out_dir, "final_c" + std::to_string(nb), Solver.Mesh[2].Block[nb]);
+    std::FILE *lv = std::fopen((out_dir + "/levels.txt").c_str(), "w");
+    if (lv) { std::fprintf(lv, "%d\n", Solver.NG); std::fclose(lv); }
 
     long cells = 0;
     for (int nb = 1; nb <= CMesh::NB; ++nb) {

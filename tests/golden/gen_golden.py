@@ -51,6 +51,22 @@ TINY = {
                               camber=0.1, viscous=True, len_ref=1e-3), 12,
                          {"generator": "annular_passage", "turn": [(2, 1)], "turbulence": 100, "wall_shear": True}),
 }
+# two-grid cycle from CMesh::Initial on the coarse level: name -> (kwargs,
+# iterations, meta); the fixture holds the history and the final fields of
+# both levels
+MG = {
+    "tiny_mg_bump": (dict(ni=8, nj=8, nk=16, seed=9), 12,
+                     {"cfg": {"use_multigrid": "Y", "reuse_solution": "N", "coarse_iter": 5, "initial": "U"}}),
+    "tiny_mg_passage3_bl": (dict(ni=8, nj=8, nk_blocks=(6, 10, 6), seed=10, rpm=3000.0, blade_blocks=(1,),
+                                 camber=0.1, viscous=True, len_ref=1e-3), 10,
+                            {"generator": "annular_passage", "turn": [(2, 1)], "turbulence": 100,
+                             "wall_shear": True,
+                             "cfg": {"use_multigrid": "Y", "reuse_solution": "N", "coarse_iter": 3, "initial": "I",
+                                     "outlet_angle": [0.0, 0.2, 1.0], "flux_scheme": 200}}),
+}
+MG_FINAL = ["cv", "pv", "res", "dq", "dt"]
+MG_COARSE_FINAL = ["cv", "res", "dq", "dt"]
+
 NS_FIELDS_IT1 = ["omega", "gradI", "gradJ", "gradK", "fvI", "fvJ", "fvK", "fv", "lamvI", "lamvJ", "lamvK"]
 
 FIELDS_IT1 = ["res", "dq", "dt", "lamcI", "lamcJ", "lamcK", "cv", "pv", "fcI", "fcJ", "fcK",
@@ -75,6 +91,26 @@ def gen_tiny(name, kw, n_iter, extra=None):
             tag = "" if b == 1 else f"b{b}_"
             for f in ("cv", "pv", "res", "dq") + (("mut",) if meta.get("turbulence") else ()):
                 arrs["final_" + tag + f] = np.load(os.path.join(odir, f"final_b{b}_{f}.npy"))
+    arrs["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)
+    print("wrote", name)
+
+
+def gen_mg(name, kw, n_iter, extra):
+    meta = {"kwargs": kw, "iterations": n_iter}
+    meta.update(extra)
+    case = cases.from_meta(meta)
+    with tempfile.TemporaryDirectory() as td:
+        cdir, odir = os.path.join(td, "case"), os.path.join(td, "out")
+        case.write(cdir)
+        res = O.run_reference(cdir, n_iter, odir, threads=1, dump_coarse=True)
+        assert res["levels"] == 2
+        arrs = {"history": res["history"]}
+        for b in range(1, len(case.blocks) + 1):
+            for f in MG_FINAL + (["mut"] if meta.get("turbulence") else []):
+                arrs[f"final_b{b}_{f}"] = np.load(os.path.join(odir, f"final_b{b}_{f}.npy"))
+            for f in MG_COARSE_FINAL:
+                arrs[f"final_c{b}_{f}"] = np.load(os.path.join(odir, f"final_c{b}_{f}.npy"))
     arrs["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
     np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)
     print("wrote", name)
@@ -115,6 +151,9 @@ def main():
     for name, spec in TINY.items():
         if a.only is None or name in a.only:
             gen_tiny(name, *spec)
+    for name, spec in MG.items():
+        if a.only is None or name in a.only:
+            gen_mg(name, *spec)
     if a.c0:
         gen_c0()
 

@@ -135,3 +135,70 @@ def test_restatement_matches_reference(name):
             if case.cfg.turbulence:
                 want = np.load(os.path.join(odir, f"final_b{b}_mut.npy"))
                 assert np.array_equal(o2.get(b, "mut").ravel(), want.ravel())
+
+
+# ------------------------------------------------------------ two-grid cycle
+def mg_variant(name):
+    """Two-grid cases (reuse_solution = N: CMesh::Initial on the coarse level,
+    Coarse_iter coarse iterations, then the cycle of source/Solver.cpp:86-103)."""
+    if name == "bump_uniform":
+        c = cases.bump_channel(ni=8, nj=8, nk=16, seed=31, max_iter=12)
+        c.cfg.coarse_iter, c.cfg.initial = 5, "U"
+    elif name == "bump_interpolate_sw_limiter":
+        c = cases.bump_channel(ni=8, nj=6, nk=16, seed=32, flux=200, recon_scheme=200, max_iter=12)
+        c.cfg.coarse_iter, c.cfg.initial, c.cfg.outlet_angle = 4, "I", (0.1, 0.0, 1.0)
+        c.cfg.limiter = "Y"
+    elif name == "bl_bump":
+        c = cases.with_wall_shear(cases.bump_channel(ni=8, nj=8, nk=16, seed=33, viscous=True, len_ref=1e-3,
+                                                     max_iter=12))
+        c.cfg.turbulence = 100
+        c.cfg.coarse_iter, c.cfg.initial = 4, "U"
+    elif name in ("bl_passage", "bl_passage_turn1"):
+        c = cases.with_wall_shear(cases.annular_passage(ni=8, nj=8, nk_blocks=(6, 10, 6), seed=34, rpm=3000.0,
+                                                        blade_blocks=(1,), camber=0.1, viscous=True, len_ref=1e-3,
+                                                        max_iter=10))
+        c.cfg.turbulence = 100
+        if name.endswith("turn1"):
+            c = cases.turn_block(c, 1, 1)
+        c.cfg.coarse_iter, c.cfg.initial, c.cfg.outlet_angle = 3, "I", (0.0, 0.2, 1.0)
+        c.cfg.flux_scheme = 200
+    elif name == "not_coarsenable":
+        # 7 cells along i: the patch ranges end on an even node, one level only
+        c = cases.bump_channel(ni=7, nj=8, nk=16, seed=35, max_iter=8)
+        c.cfg.coarse_iter, c.cfg.initial = 3, "U"
+    else:
+        raise KeyError(name)
+    c.cfg.use_multigrid, c.cfg.reuse_solution = "Y", "N"
+    return c
+
+
+MG_FINE = ["cv", "pv", "res", "dq", "dt", "lamcI", "lamcJ", "lamcK", "dIS_cv", "dIP_cv", "dJS_cv", "dJP_cv",
+           "dKS_cv", "dKP_cv"]
+MG_COARSE = ["cv", "pv", "res", "dq", "dt", "lamcI", "lamcJ", "lamcK"]
+
+
+@pytest.mark.parametrize("name", ["bump_uniform", "bump_interpolate_sw_limiter", "bl_bump", "bl_passage",
+                                  "bl_passage_turn1", "not_coarsenable"])
+def test_two_grid_matches_reference(name):
+    case = mg_variant(name)
+    n = case.cfg.max_iter
+    with tempfile.TemporaryDirectory() as td:
+        cdir, odir = os.path.join(td, "case"), os.path.join(td, "out")
+        case.write(cdir)
+        ref = O.run_reference(cdir, n, odir, threads=1, dump_coarse=True, timeout=300)
+        o = O.Oracle(case)
+        assert o.levels() == ref["levels"] == (1 if name == "not_coarsenable" else 2)
+        h = o.iterate(n)
+        assert np.all(np.isfinite(h[:, :7]))
+        assert np.array_equal(h[:, :7], ref["history"][:, :7]), name
+        visc = case.cfg.equation == "N"
+        for b in range(1, len(case.blocks) + 1):
+            for f in MG_FINE + (["lamvI", "lamvJ", "lamvK"] if visc else []) + (["mut"] if case.cfg.turbulence else []):
+                want = np.load(os.path.join(odir, f"final_b{b}_{f}.npy"))
+                got = o.get(b, f).reshape(want.shape)
+                assert np.array_equal(got, want), f"{name} fine block {b}: {f}"
+            if ref["levels"] == 2:
+                for f in MG_COARSE + (["lamvI", "lamvJ", "lamvK"] if visc else []):
+                    want = np.load(os.path.join(odir, f"final_c{b}_{f}.npy"))
+                    got = o.get(b, f, level=2).reshape(want.shape)
+                    assert np.array_equal(got, want), f"{name} coarse block {b}: {f}"
