@@ -104,8 +104,22 @@ struct fv3d_ctx {
     int sweep_mode = 4;   // 0 one launch per hyperplane, 1..4 wavefront forms (3: four waves per tile,
                           // 4: the same on the tile-skewed row layout)
     int kba_grid = 0;
+    // two-grid cycle: the coarse level is a context of its own on the same
+    // stream (coarse is set on the fine level, parent on the coarse one).  The
+    // CFL number, its ramp threshold and the iteration count are shared: they
+    // live in the fine level's scalars (ramp_sc); d_cfl points at its CFL.
+    fv3d_ctx *coarse = nullptr;
+    fv3d_ctx *parent = nullptr;
+    bool owns_stream = true;
+    bool mg_coarse = false;          // allocate cv_2h / Res_2h (coarse level)
+    int ng = 1;
+    long iter_host = 0;              // iterations run (host copy, picks the cycle phase)
+    bool initial_done = false;       // fv3d_initial ran (the two-grid cycle needs it)
+    ScalarsDev *ramp_sc = nullptr;
+    double *d_cfl = nullptr;
     // graph
     hipGraphExec_t graph_exec = nullptr;
+    hipGraphExec_t graph_coarse = nullptr;   // coarse start-up phase of the two-grid cycle
     // timing
     int timing = 0;
     hipEvent_t ev[10];
@@ -324,6 +338,10 @@ static int setup_block(fv3d_ctx *c, HostBlock &b, int index)
     }
     if ((rc = alloc(&B.R, 5 * B.box))) return rc;
     if ((rc = alloc(&B.dQ, 5 * B.box))) return rc;
+    if (c->mg_coarse) {
+        if ((rc = alloc(&B.cv2h, 5 * B.box))) return rc;
+        if ((rc = alloc(&B.r2h, 5 * B.box))) return rc;
+    }
     {
         // tile-skewed sweep rows (k_sweep_fill at set-up, k_sweep_pack per iteration)
         B.ntk = (B.nk + kKbaTK - 1) / kKbaTK;
@@ -516,7 +534,23 @@ int fv3d_ctx_destroy(fv3d_ctx *c)
 {
     if (!c) return FV3D_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->coarse) {
+        fv3d_ctx_destroy(c->coarse);
+        c->coarse = nullptr;
+    }
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph_coarse) (void)hipGraphExecDestroy(c->graph_coarse);
+    if (c->d_bl_walls) (void)hipFree(c->d_bl_walls);
+    if (c->d_exch) (void)hipFree(c->d_exch);
+    if (c->d_per) (void)hipFree(c->d_per);
+    if (c->d_pitch_sign) (void)hipFree(c->d_pitch_sign);
+    if (c->d_mbox) (void)hipFree(c->d_mbox);
+    if (c->d_pack_start) (void)hipFree(c->d_pack_start);
+    for (int d = 0; d < 3; ++d) {
+        if (c->vdbg.fv[d]) (void)hipFree(c->vdbg.fv[d]);
+        if (c->vdbg.grad[d]) (void)hipFree(c->vdbg.grad[d]);
+    }
+    if (c->vdbg.fvc) (void)hipFree(c->vdbg.fvc);
     for (HostBlock &b : c->blocks)
         for (void *p : b.allocs) (void)hipFree(p);
     void *ps[] = {c->d_blocks, c->d_patches, c->d_pstart_bc, c->d_cstart, c->d_pstart_plane, c->d_tiles, c->d_secs,
@@ -525,7 +559,7 @@ int fv3d_ctx_destroy(fv3d_ctx *c)
         if (p) (void)hipFree(p);
     if (c->stream) {
         for (int n = 0; n < 10; ++n) (void)hipEventDestroy(c->ev[n]);
-        (void)hipStreamDestroy(c->stream);
+        if (c->owns_stream) (void)hipStreamDestroy(c->stream);
     }
     delete c;
     return FV3D_OK;
@@ -599,6 +633,105 @@ int fv3d_patch_add(fv3d_ctx *c, int kind, int block, int face, const int *rng, i
     p.kind = kind; p.block = block; p.face = face; p.rotate = rotate;
     memcpy(p.rng, rng, sizeof p.rng);
     c->patches.push_back(p);
+    return FV3D_OK;
+}
+
+// One patch range of CUserInput::NumberOfMultigrid (source/UserInput.cpp:
+// 816-844): a tangential cell range s..e becomes the node range s..e+1 (a
+// single index stays), and the grid coarsens only if every start and end node
+// is odd.  The block loop of the same function (1020-1040) never lowers the
+// level count below two, so with the reference's cap of two levels
+// (1057-1060) only the patches decide; two-sided patches are read with their
+// first side (932, 962).
+static bool range_coarsens(const int *r)
+{
+    int q[6];
+    memcpy(q, r, sizeof q);
+    for (int a = 0; a < 3; ++a)
+        if (q[2 * a] != q[2 * a + 1]) q[2 * a + 1] += 1;
+    for (int e = 0; e < 6; ++e)
+        if ((q[e] - 1) % 2 != 0) return false;
+    return true;
+}
+
+// Coarse patch ranges (source/UserInput.cpp:446-797): a single index n becomes
+// (n+1)/2, a cell range s..e becomes (s+1)/2 .. e/2.
+static void coarse_range(const int *r, int *o)
+{
+    for (int a = 0; a < 3; ++a) {
+        const int s = r[2 * a], e = r[2 * a + 1];
+        o[2 * a] = (s + 1) / 2;
+        o[2 * a + 1] = s == e ? (e + 1) / 2 : e / 2;
+    }
+}
+
+// The coarse level of the two-grid cycle (source/UserInput.cpp:241-334 and
+// 446-797): nodes (2i-1, 2j-1, 2k-1) of every block, the same patches with
+// halved ranges, no turbulence model (source/ComputeAllBlockVars.cpp:230), the
+// pitch and rotation rate of the fine blocks.  It shares the fine level's
+// stream and CFL state.
+static int setup_coarse(fv3d_ctx *c)
+{
+    bool ok = true;
+    for (const HostPatch &P : c->patches) ok = ok && range_coarsens(P.rng);
+    for (const HostConn &T : c->conns) ok = ok && range_coarsens(T.rng[0]);
+    if (!ok) {
+        c->ng = 1;
+        return FV3D_OK;
+    }
+    // coarse cell (I,J,K) covers fine cells 2I-1..2I: every block needs an even
+    // number of cells along each direction (patches covering its faces ensure
+    // it; a block whose faces they do not cover is rejected here)
+    for (size_t n = 0; n < c->blocks.size(); ++n) {
+        const HostBlock &b = c->blocks[n];
+        if ((b.IM - 1) % 2 || (b.JM - 1) % 2 || (b.KM - 1) % 2)
+            return fail(c, FV3D_EINVAL, "two-grid cycle: block %d has %dx%dx%d cells, not even along every direction",
+                        (int)n + 1, b.IM - 1, b.JM - 1, b.KM - 1);
+    }
+    fv3d_ctx *m = new fv3d_ctx();
+    m->cfg = c->cfg;
+    m->cfg.multigrid = 0;
+    m->cfg.turbulence = 0;
+    m->device = c->device;
+    m->stream = c->stream;
+    m->owns_stream = false;
+    m->parent = c;
+    m->mg_coarse = true;
+    for (int n = 0; n < 10; ++n) (void)hipEventCreate(&m->ev[n]);
+    nondimensionalize(m);
+    auto bail = [&](int rc) {
+        int code = fail(c, rc, "coarse level: %s", m->err.c_str());
+        fv3d_ctx_destroy(m);
+        return code;
+    };
+    int rc;
+    for (const HostBlock &b : c->blocks) {
+        const int IM = (b.IM + 1) / 2, JM = (b.JM + 1) / 2, KM = (b.KM + 1) / 2;
+        std::vector<double> nodes(3 * (size_t)IM * JM * KM);
+        for (int i = 1; i <= IM; ++i)
+            for (int j = 1; j <= JM; ++j)
+                for (int k = 1; k <= KM; ++k)
+                    for (int q = 0; q < 3; ++q)
+                        nodes[(((size_t)(i - 1) * JM + (j - 1)) * KM + (k - 1)) * 3 + q] =
+                            b.nodes[(((size_t)(2 * i - 2) * b.JM + (2 * j - 2)) * b.KM + (2 * k - 2)) * 3 + q];
+        if ((rc = fv3d_block_add(m, IM, JM, KM, nodes.data(), b.nblade, b.rpm)) < 0) return bail(rc);
+        m->blocks.back().w_rot = b.w_rot;   // source/UserInput.cpp:449-454
+    }
+    for (const HostPatch &P : c->patches) {
+        int o[6];
+        coarse_range(P.rng, o);
+        if ((rc = fv3d_patch_add(m, P.kind, P.block, P.face, o, P.rotate))) return bail(rc);
+    }
+    for (const HostConn &T : c->conns) {
+        int o1[6], o2[6];
+        coarse_range(T.rng[0], o1);
+        coarse_range(T.rng[1], o2);
+        if ((rc = fv3d_connection_add(m, T.kind, T.orient, T.block[0], T.face[0], o1, T.block[1], T.face[1], o2)))
+            return bail(rc);
+    }
+    if ((rc = fv3d_setup(m))) return bail(rc);
+    c->coarse = m;
+    c->ng = 2;
     return FV3D_OK;
 }
 
@@ -811,7 +944,13 @@ int fv3d_setup(fv3d_ctx *c)
     if ((rc = dalloc(c, &c->d_sc, 1))) return rc;
     HIPCK(c, hipMemcpyAsync(c->d_sc, &sc, sizeof sc, hipMemcpyHostToDevice, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
+    c->ramp_sc = c->parent ? c->parent->ramp_sc : c->d_sc;
+    c->d_cfl = &c->ramp_sc->cfl;
     c->ready = true;
+    if (c->cfg.multigrid && (rc = setup_coarse(c))) {
+        c->ready = false;
+        return rc;
+    }
     return FV3D_OK;
 }
 
@@ -841,6 +980,122 @@ int fv3d_state_upload(fv3d_ctx *c, int block, const double *pv, const double *en
     return FV3D_OK;
 }
 
+int fv3d_levels(const fv3d_ctx *c) { return c ? c->ng : FV3D_EINVAL; }
+
+// CMesh::Initial (source/ComputeAllBlockVars.cpp:38-48) on one level: the
+// uniform field of CBlocks::Initial_Uniform (source/Initialization.cpp:17-72)
+// or CMesh::Initial_Interpolate (74-165), pressure and flow direction linear in
+// z between the first inlet's K face and the first outlet's.  Host code: it
+// runs once, before the first iteration.
+static int initial_level(fv3d_ctx *m, int type)
+{
+    const Gas &g = m->gas;
+    const double Gamma = g.gamma, Rgas = g.rgas, Ptin = g.pt_in, Ttin = g.tt_in, Pout = g.p_out;
+    const double *din = m->cfg.inlet_dir, *dout = m->cfg.outlet_dir;
+    const int nb = (int)m->blocks.size();
+    // cell centroids (and K-face centroids) in the reference's operation order
+    std::vector<std::vector<double>> cc(nb), fck(nb);
+    for (int n = 0; n < nb; ++n) {
+        std::vector<double> fS[3], fs[3], fc[3], vol, node;
+        face_vector_volume(m->blocks[n], fS[0], fS[1], fS[2], fs[0], fs[1], fs[2], fc[0], fc[1], fc[2], vol, cc[n], node);
+        fck[n] = std::move(fc[2]);
+    }
+    double press0 = 0, temp = 0, dens0 = 0, Vel = 0, n_m0 = 0, zmin = 0, dz = 1, Pin = 0, dp = 0, dB[3] = {0, 0, 0};
+    if (type == 0) {
+        const double Ma0 = (float)m->cfg.ma0;
+        const double m2 = F32C(1.0) + (Gamma - F32C(1.0)) / F32C(2.0) * Ma0 * Ma0;
+        press0 = Ptin * pow(m2, F32C(0.0) - Gamma / (Gamma - F32C(1.0)));
+        temp = Ttin * pow(m2, (double)(0.0f - 1.0f));
+        dens0 = press0 / (Rgas * temp);
+        Vel = pow(Gamma * Rgas * temp, F32C(0.5)) * Ma0;
+        n_m0 = pow(din[0] * din[0] + din[1] * din[1] + din[2] * din[2], 0.5);
+    } else {
+        const HostPatch *pin = nullptr, *pout = nullptr;
+        for (const HostPatch &P : m->patches) {
+            if (P.kind == 0 && !pin) pin = &P;
+            if (P.kind == 1 && !pout) pout = &P;
+        }
+        if (!pin || !pout) return fail(m, FV3D_EINVAL, "the interpolated start needs an inlet and an outlet patch");
+        zmin = F32C(1E+32);
+        double zmax = F32C(-1E+32);
+        const HostBlock &bi = m->blocks[pin->block - 1], &bo = m->blocks[pout->block - 1];
+        auto fz = [&](const HostBlock &b, const std::vector<double> &f, int i, int j, int k) {
+            const long box = (long)(b.IM + 1) * (b.JM + 1) * (b.KM + 1);
+            return f[2 * box + ((long)i * (b.JM + 1) + j) * (b.KM + 1) + k];
+        };
+        for (int i = 1; i <= bi.IM - 1; ++i)
+            for (int j = 1; j <= bi.JM - 1; ++j) {
+                const double z = fz(bi, fck[pin->block - 1], i, j, pin->rng[4]);
+                zmin = z < zmin ? z : zmin;
+            }
+        for (int i = 1; i <= bo.IM - 1; ++i)
+            for (int j = 1; j <= bo.JM - 1; ++j) {
+                const double z = fz(bo, fck[pout->block - 1], i, j, pout->rng[5]);
+                zmax = z > zmax ? z : zmax;
+            }
+        dz = zmax - zmin;
+        Pin = Pout / (double)(float)m->cfg.p21ratio;
+        if (Pin >= Ptin) Pin = F32C(0.99) * Ptin;
+        dp = Pout - Pin;
+        for (int a = 0; a < 3; ++a) dB[a] = dout[a] - din[a];
+        temp = Ttin * pow(Pin / Ptin, (Gamma - F32C(1.0)) / Gamma);
+        dens0 = Pin / (Rgas * temp);
+        const double cs = sqrt(Gamma * Pin / dens0);
+        const double Ma = sqrt(F32C(2.0) * ((Ttin / temp) - F32C(1.0)) / (Gamma - F32C(1.0)));
+        Vel = Ma * cs;
+    }
+    for (int n = 0; n < nb; ++n) {
+        const BlockDev &B = m->blocks[n].dev;
+        std::vector<double> Q(5 * B.box, 0.0), W(6 * B.box, 0.0);
+        HIPCK(m, hipMemcpy(Q.data(), B.Q, Q.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCK(m, hipMemcpy(W.data(), B.W, W.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 1; i <= B.ni; ++i)
+            for (int j = 1; j <= B.nj; ++j)
+                for (int k = 1; k <= B.nk; ++k) {
+                    const long x = cidx(B, i, j, k);
+                    double w[6], ener;
+                    if (type == 0) {
+                        const double u = Vel * din[0] / n_m0, v = Vel * din[1] / n_m0, ww = Vel * din[2] / n_m0;
+                        w[0] = dens0; w[1] = u; w[2] = v; w[3] = ww; w[4] = press0; w[5] = temp;
+                        const double V2 = u * u + v * v + ww * ww;
+                        ener = dens0 * (press0 / (dens0 * (Gamma - F32C(1.0))) + V2 / F32C(2.0));
+                    } else {
+                        const double z = cc[n][2 * B.box + x];
+                        double Bt[3];
+                        for (int a = 0; a < 3; ++a) Bt[a] = din[a] + dB[a] * (z - zmin) / dz;
+                        const double press = Pin + dp * (z - zmin) / dz;
+                        const double dens = press / Rgas / temp;
+                        const double n_m = sqrt(Bt[0] * Bt[0] + Bt[1] * Bt[1] + Bt[2] * Bt[2]);
+                        const double u = Vel * Bt[0] / n_m, v = Vel * Bt[1] / n_m, ww = Vel * Bt[2] / n_m;
+                        w[0] = dens; w[1] = u; w[2] = v; w[3] = ww; w[4] = press; w[5] = temp;
+                        const double V2 = u * u + v * v + ww * ww;
+                        ener = press / (Gamma - F32C(1.0)) + dens * V2 / F32C(2.0);
+                    }
+                    Q[x] = w[0]; Q[B.box + x] = w[0] * w[1]; Q[2 * B.box + x] = w[0] * w[2];
+                    Q[3 * B.box + x] = w[0] * w[3]; Q[4 * B.box + x] = ener;
+                    for (int q = 0; q < 6; ++q) W[q * B.box + x] = w[q];
+                }
+        HIPCK(m, hipMemcpy(B.Q, Q.data(), Q.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCK(m, hipMemcpy(B.W, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return FV3D_OK;
+}
+
+int fv3d_initial(fv3d_ctx *c)
+{
+    if (!c) return FV3D_EINVAL;
+    if (!c->ready) return fail(c, FV3D_ESTATE, "initial before setup");
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    fv3d_ctx *m = c->ng == 2 ? c->coarse : c;
+    int rc = initial_level(m, c->cfg.initial_type);
+    if (rc) {
+        if (m != c) fail(c, rc, "coarse level: %s", m->err.c_str());
+        return rc;
+    }
+    c->initial_done = true;
+    return FV3D_OK;
+}
+
 // ------------------------------------------------------------- stages
 static int launch_bc(fv3d_ctx *c)
 {
@@ -859,7 +1114,7 @@ static int launch_bc(fv3d_ctx *c)
 static int launch_spectral_dt(fv3d_ctx *c)
 {
     hipLaunchKernelGGL(k_spectral_dt, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
-                       c->d_cstart, (int)c->blocks.size(), c->gas, &c->d_sc->cfl);
+                       c->d_cstart, (int)c->blocks.size(), c->gas, c->d_cfl);
     HIPCK(c, hipGetLastError());
     hipLaunchKernelGGL(k_dt_flag_decrement, dim3(1), dim3(64), 0, c->stream, c->d_blocks, (int)c->blocks.size());
     HIPCK(c, hipGetLastError());
@@ -1027,8 +1282,8 @@ static int launch_finalize(fv3d_ctx *c, double *hist_row)
 {
     hipLaunchKernelGGL(k_sections, dim3(2 * kSecParts), dim3(256), 0, c->stream, c->d_blocks, c->d_secs, c->gas, c->d_sums);
     HIPCK(c, hipGetLastError());
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->d_partial, c->npart, c->d_sums, c->d_sc, c->gas,
-                       hist_row);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->d_partial, c->npart, c->d_sums, c->d_sc,
+                       c->ramp_sc, c->gas, hist_row);
     HIPCK(c, hipGetLastError());
     return FV3D_OK;
 }
@@ -1143,6 +1398,66 @@ int fv3d_perf(fv3d_ctx *c, double *out5)
     return FV3D_OK;
 }
 
+static int launch_limiter(fv3d_ctx *c)
+{
+    if (!c->cfg.limiter) return FV3D_OK;
+    hipLaunchKernelGGL(k_limiter_serial, dim3(1), dim3(64), 0, c->stream, c->d_blocks, (int)c->blocks.size(), c->gas,
+                       c->d_bad);
+    HIPCK(c, hipGetLastError());
+    return FV3D_OK;
+}
+
+// One iteration of the two-grid CSolver::Solver (source/Solver.cpp:61-103,
+// reuse_solution = N).  Phases: coarse start-up iterations (LU-SGS on the
+// coarse level, which reports), the last of them at Coarse_iter (followed by
+// the coarse corner treatment and the interpolation of the coarse state onto
+// the fine level), and the cycle: fine LU-SGS and its residual norm, the fine
+// residual at the new state, restriction with the forced coarse residual,
+// coarse LU-SGS on the lambda, dt and diagonal of the last coarse ResFlux, the
+// prolonged correction; the fine level reports.
+enum { kMgCoarse = 0, kMgSwitch = 1, kMgCycle = 2 };
+
+static int mg_iteration(fv3d_ctx *c, int phase, double *row)
+{
+    fv3d_ctx *m = c->coarse;
+    const int nb = (int)c->blocks.size();
+    int rc;
+    if (phase != kMgCycle) {
+        if ((rc = lusgs_timed(m))) return rc;
+        if (phase == kMgSwitch) {
+            hipLaunchKernelGGL(k_mg_corner, dim3((unsigned)nb), dim3(256), 0, c->stream, m->d_blocks);
+            hipLaunchKernelGGL(k_mg_prolong<true>, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream,
+                               c->d_blocks, m->d_blocks, c->d_cstart, nb, c->gas, c->d_bad);
+            HIPCK(c, hipGetLastError());
+            if ((rc = launch_limiter(c))) return rc;
+        }
+        return launch_finalize(m, row);
+    }
+    if ((rc = lusgs_timed(c))) return rc;
+    if ((rc = launch_bc(c))) return rc;
+    if ((rc = launch_turbulence(c))) return rc;
+    if ((rc = launch_spectral_dt(c))) return rc;
+    if ((rc = launch_convective(c))) return rc;
+    if ((rc = launch_viscous(c))) return rc;
+    hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)((m->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
+                       m->d_blocks, m->d_cstart, nb, m->gas, m->d_bad);
+    HIPCK(c, hipGetLastError());
+    if ((rc = launch_limiter(m))) return rc;
+    if ((rc = launch_sweeps(m))) return rc;
+    if ((rc = launch_update(m))) return rc;
+    hipLaunchKernelGGL(k_mg_prolong<false>, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream,
+                       c->d_blocks, m->d_blocks, c->d_cstart, nb, c->gas, c->d_bad);
+    HIPCK(c, hipGetLastError());
+    if ((rc = launch_limiter(c))) return rc;
+    if (c->timing) HIPCK(c, hipEventRecord(c->ev[9], c->stream));
+    return launch_finalize(c, row);
+}
+
+static int mg_phase(const fv3d_ctx *c, long it)
+{
+    return it < c->cfg.coarse_iter ? kMgCoarse : it == c->cfg.coarse_iter ? kMgSwitch : kMgCycle;
+}
+
 static int ensure_hist(fv3d_ctx *c, long n)
 {
     if (n <= c->hist_cap) return FV3D_OK;
@@ -1165,7 +1480,8 @@ static int accumulate_timing(fv3d_ctx *c)
     HIPCK(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3])); c->t_ms[1] += ms; c->t_n[1]++;
     HIPCK(c, hipEventElapsedTime(&ms, c->ev[3], c->ev[4])); c->t_ms[2] += ms; c->t_n[2]++;
     HIPCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); c->t_ms[3] += ms; c->t_n[3]++;
-    HIPCK(c, hipEventElapsedTime(&ms, c->ev[4], c->ev[5])); c->t_ms[4] += ms; c->t_n[4]++;
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev[9], c->ev[5])); c->t_ms[4] += ms; c->t_n[4]++;
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev[4], c->ev[9])); c->t_ms[7] += ms; c->t_n[7]++;
     if (c->sweep_mode == 4) {
         // the forward and backward row-layout sweep kernels alone
         HIPCK(c, hipEventElapsedTime(&ms, c->ev[6], c->ev[7])); c->t_ms[5] += ms; c->t_n[5]++;
@@ -1181,10 +1497,24 @@ int fv3d_iterate(fv3d_ctx *c, int n, double *hist)
     if (n < 0) return fail(c, FV3D_EINVAL, "n=%d", n);
     int rc;
     if ((rc = ensure_hist(c, std::max(n, 1)))) return rc;
+    if (c->ng == 2 && !c->initial_done)
+        return fail(c, FV3D_ESTATE, "the two-grid cycle starts from fv3d_initial (reuse_solution = N): from a restart "
+                                    "file the reference's cycle divides by coarse time steps it never computed");
     for (int it = 0; it < n; ++it) {
-        if ((rc = lusgs_timed(c))) return rc;
-        if ((rc = launch_finalize(c, c->d_hist + 7L * it))) return rc;
-        if (c->timing) {
+        const long itn = c->iter_host + 1;
+        double *row = c->d_hist + 7L * it;
+        bool timed = true;
+        if (c->ng == 2) {
+            const int phase = mg_phase(c, itn);
+            timed = phase == kMgCycle;
+            if ((rc = mg_iteration(c, phase, row))) return rc;
+        } else {
+            if ((rc = lusgs_timed(c))) return rc;
+            if (c->timing) HIPCK(c, hipEventRecord(c->ev[9], c->stream));
+            if ((rc = launch_finalize(c, row))) return rc;
+        }
+        c->iter_host = itn;
+        if (c->timing && timed) {
             HIPCK(c, hipEventRecord(c->ev[5], c->stream));
             if ((rc = accumulate_timing(c))) return rc;
         }
@@ -1193,6 +1523,7 @@ int fv3d_iterate(fv3d_ctx *c, int n, double *hist)
         HIPCK(c, hipMemcpyAsync(hist, c->d_hist, 7L * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
     if ((rc = check_sweep_error(c))) return rc;
+    if (c->coarse && (rc = check_sweep_error(c->coarse))) return fail(c, rc, "coarse level: %s", c->coarse->err.c_str());
     if (hist && n > 0 && !isfinite(hist[7L * (n - 1) + 1])) return fail(c, FV3D_ENUMERIC, "residual is not finite");
     return FV3D_OK;
 }
@@ -1203,31 +1534,53 @@ int fv3d_iterate_graph(fv3d_ctx *c, int n, double *hist)
     if (n < 0) return fail(c, FV3D_EINVAL, "n=%d", n);
     int rc;
     if ((rc = ensure_hist(c, std::max(n, 1)))) return rc;
-    // the graph writes its history row into a fixed staging slot that is
-    // copied into the history buffer after each replay
+    if (c->ng == 2 && !c->initial_done)
+        return fail(c, FV3D_ESTATE, "the two-grid cycle starts from fv3d_initial (reuse_solution = N): from a restart "
+                                    "file the reference's cycle divides by coarse time steps it never computed");
+    // each graph writes its history row into a fixed staging slot that is
+    // copied into the history buffer after each replay; with two levels the
+    // coarse start-up phase and the cycle have a graph each, and the one
+    // iteration between them (Coarse_iter) runs eagerly
     double *stage_row = c->d_stage_row;
-    if (!c->graph_exec) {
+    auto capture = [&](hipGraphExec_t *exec, int phase) -> int {
+        if (*exec) return FV3D_OK;
         hipGraph_t g;
         int saved = c->timing;
         c->timing = 0;
         HIPCK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = lusgs_timed(c);
-        if (!rc) rc = launch_finalize(c, stage_row);
+        int r;
+        if (c->ng == 2) {
+            r = mg_iteration(c, phase, stage_row);
+        } else {
+            r = lusgs_timed(c);
+            if (!r) r = launch_finalize(c, stage_row);
+        }
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         c->timing = saved;
-        if (rc) return rc;
+        if (r) return r;
         if (e != hipSuccess) return fail(c, FV3D_EHIP, "graph capture: %s", hipGetErrorString(e));
-        HIPCK(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+        HIPCK(c, hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
         HIPCK(c, hipGraphDestroy(g));
-    }
+        return FV3D_OK;
+    };
     for (int it = 0; it < n; ++it) {
-        HIPCK(c, hipGraphLaunch(c->graph_exec, c->stream));
+        const long itn = c->iter_host + 1;
+        const int phase = c->ng == 2 ? mg_phase(c, itn) : kMgCycle;
+        if (phase == kMgSwitch) {
+            if ((rc = mg_iteration(c, phase, stage_row))) return rc;
+        } else {
+            hipGraphExec_t *exec = phase == kMgCoarse ? &c->graph_coarse : &c->graph_exec;
+            if ((rc = capture(exec, phase))) return rc;
+            HIPCK(c, hipGraphLaunch(*exec, c->stream));
+        }
         HIPCK(c, hipMemcpyAsync(c->d_hist + 7L * it, stage_row, 7 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        c->iter_host = itn;
     }
     if (hist && n > 0)
         HIPCK(c, hipMemcpyAsync(hist, c->d_hist, 7L * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
     if ((rc = check_sweep_error(c))) return rc;
+    if (c->coarse && (rc = check_sweep_error(c->coarse))) return fail(c, rc, "coarse level: %s", c->coarse->err.c_str());
     if (hist && n > 0 && !isfinite(hist[7L * (n - 1) + 1])) return fail(c, FV3D_ENUMERIC, "residual is not finite");
     return FV3D_OK;
 }
@@ -1293,6 +1646,16 @@ int fv3d_set_debug(fv3d_ctx *c, int flags)
         }
     }
     return FV3D_OK;
+}
+
+long fv3d_level_field_download(fv3d_ctx *c, int level, int block, const char *name, double *out)
+{
+    if (!c) return FV3D_EINVAL;
+    if (level == 1) return fv3d_field_download(c, block, name, out);
+    if (level != 2 || !c->coarse) return fail(c, FV3D_EINVAL, "grid level %d not in use", level);
+    long r = fv3d_field_download(c->coarse, block, name, out);
+    if (r < 0) fail(c, (int)r, "coarse level: %s", c->coarse->err.c_str());
+    return r;
 }
 
 long fv3d_field_download(fv3d_ctx *c, int block, const char *name, double *out)

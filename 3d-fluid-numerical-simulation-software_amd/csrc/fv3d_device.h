@@ -79,6 +79,10 @@ struct BlockDev {
     // (5 x kKbaTK) and across the k edge (5 x kKbaTJ).  Reset to a sentinel
     // before each iteration; a consumer waits until its words are not it.
     double *mbox[2];
+    // coarse level of the two-grid cycle (null elsewhere): the restricted state
+    // cv_2h and the restricted residual Res_2h, 5*box each
+    double *cv2h;
+    double *r2h;
 };
 
 __host__ __device__ inline long cidx(const BlockDev &B, int i, int j, int k)

@@ -102,7 +102,18 @@ __global__ void k_fill_int(int *p, long n, int v);
 __global__ void k_update(const BlockDev *blocks, const long *cstart, int nblock, Gas g, double *partial, int *bad_flag);
 __global__ void k_limiter_serial(const BlockDev *blocks, int nblock, Gas g, int *bad_flag);
 __global__ void k_sections(const BlockDev *blocks, const int2 *secs, Gas g, double *sums);
-__global__ void k_finalize(const double *partial, int npart, const double *sums, ScalarsDev *sc, Gas g, double *hist_row);
+// norms and section integrals of one level (lv) with the CFL ramp and the
+// iteration count kept in ramp (the fine level's scalars, shared by the levels)
+__global__ void k_finalize(const double *partial, int npart, const double *sums, ScalarsDev *lv, ScalarsDev *ramp, Gas g,
+                           double *hist_row);
+// two-grid cycle (CSolver::Prolong / Interpolate, CMesh::CornerTreatment,
+// ComputeForceFunction / AddForceFunction)
+__global__ void k_mg_restrict(const BlockDev *fine, const BlockDev *coarse, const long *cstart, int nblock, Gas g,
+                              int *bad_flag);
+__global__ void k_mg_corner(const BlockDev *coarse);
+template <bool STATE>
+__global__ void k_mg_prolong(const BlockDev *fine, const BlockDev *coarse, const long *cstart, int nblock, Gas g,
+                             int *bad_flag);
 
 // laminar Navier-Stokes terms (fv3d_viscous.hip)
 // coordinate-transformation coefficients of every face and cell of one block (set-up)

@@ -1716,8 +1716,9 @@ __global__ void __launch_bounds__(256) k_sections(const BlockDev *__restrict__ b
 // parameters (source/Ouput.cpp:97-107,282-300), CFL ramp for the next iteration
 // (source/Solver.cpp:49-60) and the history row.  One workgroup.
 __global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ partial, int npart, const double *__restrict__ secsum,
-                                                  ScalarsDev *__restrict__ sc, Gas g, double *__restrict__ hist_row)
+                                                  ScalarsDev *lv, ScalarsDev *ramp, Gas g, double *__restrict__ hist_row)
 {
+    ScalarsDev *sc = lv;
     __shared__ double red[5][256];
     double r[5] = {0, 0, 0, 0, 0};
     for (int p = threadIdx.x; p < npart; p += blockDim.x)
@@ -1746,18 +1747,19 @@ __global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ par
     double PR = pto / pti;
     double L_adk = Gamma / (Gamma - F32C(1.0)) * Rgas * tti * (pow(PR, (Gamma - F32C(1.0)) / Gamma) - F32C(1.0));
     double L_k = Gamma / (Gamma - F32C(1.0)) * Rgas * tti * (tto / tti - F32C(1.0));
-    sc->iter += 1;
+    ramp->iter += 1;
     if (hist_row) {
-        hist_row[0] = sc->iter; hist_row[1] = sc->res[0]; hist_row[2] = mi; hist_row[3] = mo;
-        hist_row[4] = PR; hist_row[5] = L_adk / L_k; hist_row[6] = sc->cfl;
+        hist_row[0] = ramp->iter; hist_row[1] = sc->res[0]; hist_row[2] = mi; hist_row[3] = mo;
+        hist_row[4] = PR; hist_row[5] = L_adk / L_k; hist_row[6] = ramp->cfl;
     }
     sc->mass_in = mi; sc->mass_out = mo; sc->pr = PR; sc->eff = L_adk / L_k;
-    // CFL ramp applied for the next iteration
-    if (sc->res[0] <= sc->cfl_res) {
-        sc->cfl = sc->cfl * F32C(1.2);
-        sc->cfl_res = sc->cfl_res / F32C(10.0);
+    // CFL ramp applied for the next iteration (source/Solver.cpp:49-59), on the
+    // density residual of the level that ran
+    if (sc->res[0] <= ramp->cfl_res) {
+        ramp->cfl = ramp->cfl * F32C(1.2);
+        ramp->cfl_res = ramp->cfl_res / F32C(10.0);
     }
-    if (sc->cfl > F32C(10.0)) sc->cfl = F32C(10.0);
+    if (ramp->cfl > F32C(10.0)) ramp->cfl = F32C(10.0);
 }
 
 }  // namespace fv3d

@@ -5,7 +5,8 @@
 //   fv3d_run <case_dir> --parse-only
 //
 // The case directory holds config.dat, the grid and boundary-definition files
-// it names, and RestartSolution.dat (reuse_solution = Y).  Without an iteration
+// it names, and RestartSolution.dat when reuse_solution = Y (else the start
+// field is CMesh::Initial's, on the coarse level with UseMultigrid).  Without an iteration
 // count the config's max_iter is used.  The history ends at the first
 // iteration whose density residual is at or below the config's convergence
 // tolerance with inlet and outlet mass flows within 0.3 %, as the reference's
@@ -38,11 +39,13 @@ void print_summary(const fv3d::CaseConfig &cc, const fv3d::BndTable &bnd, const 
                 "\"ev_ratio\": %.17g, \"inlet_dir\": [%.17g, %.17g, %.17g], \"cfl\": %.17g, \"c_timestep\": %.17g, "
                 "\"relax\": %.17g, \"turbo_type\": %d, \"viscous\": %d, \"recon_var\": %d, \"recon_scheme\": %d, "
                 "\"flux_scheme\": %d, \"turbulence\": %d, \"limiter\": %d, \"global_dt\": %d, \"max_iter\": %d, "
-                "\"conv_tol\": %.17g, \"reuse_solution\": %d, \"time_scheme\": %d},\n",
+                "\"conv_tol\": %.17g, \"reuse_solution\": %d, \"time_scheme\": %d, \"multigrid\": %d, "
+                "\"coarse_iter\": %d, \"initial_type\": %d, \"p21ratio\": %.17g, \"outlet_dir\": [%.17g, %.17g, %.17g]},\n",
                 r.rgas, r.gamma, r.pr_lam, r.pr_tur, r.pt_in, r.tt_in, r.p_out, r.ma0, r.len_ref, r.ev_ratio,
                 r.inlet_dir[0], r.inlet_dir[1], r.inlet_dir[2], r.cfl, r.c_timestep, r.relax, r.turbo_type,
                 r.viscous, r.recon_var, r.recon_scheme, r.flux_scheme, r.turbulence, r.limiter, r.global_dt,
-                cc.max_iter, cc.conv_tol, (int)cc.reuse_solution, cc.time_scheme);
+                cc.max_iter, cc.conv_tol, (int)cc.reuse_solution, cc.time_scheme, r.multigrid, r.coarse_iter,
+                r.initial_type, r.p21ratio, r.outlet_dir[0], r.outlet_dir[1], r.outlet_dir[2]);
     std::printf(" \"patches\": [");
     for (size_t n = 0; n < bnd.patches.size(); ++n) {
         const fv3d::PatchRow &p = bnd.patches[n];
@@ -88,9 +91,8 @@ int main(int argc, char **argv)
         fv3d::BndTable bnd = fv3d::ReadBndGrid(dir + "/" + cc.bc_file);
         std::vector<std::array<int, 3>> dims;
         std::vector<std::vector<double>> nodes = fv3d::ReadGrids(dir + "/" + cc.grid_file, &dims);
-        if (!cc.reuse_solution)
-            throw fv3d::Error(FV3D_EINVAL, "this build starts from RestartSolution.dat (reuse_solution = Y)");
-        std::vector<fv3d::BlockState> st = fv3d::ReadSolution(dir + "/RestartSolution.dat", dims);
+        std::vector<fv3d::BlockState> st;
+        if (cc.reuse_solution) st = fv3d::ReadSolution(dir + "/RestartSolution.dat", dims);
         if (parse_only) {
             print_summary(cc, bnd, dims, nodes, st);
             return 0;

@@ -14,8 +14,8 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libfv3d.so")
 # every symbol declared in include/fv3d.h
 EXPORTS = [
     "fv3d_ctx_create", "fv3d_ctx_destroy", "fv3d_last_error", "fv3d_block_add", "fv3d_patch_add",
-    "fv3d_connection_add", "fv3d_setup",
-    "fv3d_state_upload", "fv3d_field_download", "fv3d_bc_apply", "fv3d_spectral", "fv3d_timestep", "fv3d_convective",
+    "fv3d_connection_add", "fv3d_setup", "fv3d_levels", "fv3d_initial",
+    "fv3d_state_upload", "fv3d_field_download", "fv3d_level_field_download", "fv3d_bc_apply", "fv3d_spectral", "fv3d_timestep", "fv3d_convective",
     "fv3d_viscous", "fv3d_curl", "fv3d_resflux", "fv3d_lusgs", "fv3d_res_calculate", "fv3d_perf", "fv3d_iterate", "fv3d_iterate_graph", "fv3d_sync",
     "fv3d_set_timing", "fv3d_timing", "fv3d_set_debug", "fv3d_cell_count",
 ]
@@ -28,7 +28,9 @@ class Config(C.Structure):
                 ("cfl", C.c_double), ("c_timestep", C.c_double), ("relax", C.c_double),
                 ("turbo_type", C.c_int), ("viscous", C.c_int), ("recon_var", C.c_int), ("recon_scheme", C.c_int),
                 ("flux_scheme", C.c_int), ("turbulence", C.c_int), ("limiter", C.c_int), ("global_dt", C.c_int),
-                ("device", C.c_int), ("precision", C.c_int)]
+                ("device", C.c_int), ("precision", C.c_int),
+                ("multigrid", C.c_int), ("coarse_iter", C.c_int), ("initial_type", C.c_int),
+                ("p21ratio", C.c_double), ("outlet_dir", C.c_double * 3)]
 
 
 class Fv3dError(RuntimeError):
@@ -57,6 +59,8 @@ def load():
         "fv3d_setup": (i, [vp]),
         "fv3d_state_upload": (i, [vp, i, p, p]),
         "fv3d_field_download": (l, [vp, i, C.c_char_p, p]),
+        "fv3d_level_field_download": (l, [vp, i, i, C.c_char_p, p]),
+        "fv3d_levels": (i, [vp]),
         "fv3d_res_calculate": (i, [vp, p]),
         "fv3d_perf": (i, [vp, p]),
         "fv3d_iterate": (i, [vp, i, p]),
@@ -67,7 +71,7 @@ def load():
         "fv3d_cell_count": (l, [vp]),
     }
     for name in ("fv3d_bc_apply", "fv3d_spectral", "fv3d_timestep", "fv3d_convective", "fv3d_viscous", "fv3d_curl",
-                 "fv3d_resflux", "fv3d_lusgs", "fv3d_sync"):
+                 "fv3d_resflux", "fv3d_lusgs", "fv3d_sync", "fv3d_initial"):
         sig[name] = (i, [vp])
     for name, (res, args) in sig.items():
         f = getattr(L, name)

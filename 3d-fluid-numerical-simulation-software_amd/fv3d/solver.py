@@ -4,7 +4,9 @@
 grid / boundary-definition / restart content) into a libfv3d context and
 exposes the stages of CMesh::ResFlux and CMesh::LU_SGS_Marching
 (source/ComputeAllBlockVars.cpp:223-255, source/LU_SGS_Marching.cpp:10-291)
-and the iteration of CSolver::Solver (source/Solver.cpp:47-66).
+and the iteration of CSolver::Solver (source/Solver.cpp:47-135), single grid
+or the two-grid cycle.  Without a restart (``reuse_solution = N``) the start
+field comes from CMesh::Initial on the level the reference starts from.
 """
 from __future__ import annotations
 
@@ -35,6 +37,12 @@ def make_config(cfg: CaseConfig, device: int = 0, precision: int = 0) -> _lib.Co
     s.global_dt = 0 if cfg.timestep.upper() == "L" else 1
     s.device = device
     s.precision = precision
+    s.multigrid = 1 if cfg.use_multigrid.upper() == "Y" else 0
+    s.coarse_iter = cfg.coarse_iter
+    s.initial_type = 0 if cfg.initial.upper() == "U" else 1
+    s.p21ratio = cfg.p21_ratio
+    for c in range(3):
+        s.outlet_dir[c] = cfg.outlet_angle[c]
     return s
 
 
@@ -70,10 +78,13 @@ class GpuSolver:
         self._check(self.L.fv3d_setup(self.h), "setup")
         if debug:
             self._check(self.L.fv3d_set_debug(self.h, debug), "set_debug")
-        for b, st in enumerate(case.states):
-            pv = np.ascontiguousarray(np.stack([st["rho"], st["u"], st["v"], st["w"], st["p"], st["T"]], -1))
-            e = np.ascontiguousarray(st["rhoE"])
-            self._check(self.L.fv3d_state_upload(self.h, b + 1, pv.ctypes.data, e.ctypes.data), "state_upload")
+        if case.cfg.reuse_solution.upper() == "Y":
+            for b, st in enumerate(case.states):
+                pv = np.ascontiguousarray(np.stack([st["rho"], st["u"], st["v"], st["w"], st["p"], st["T"]], -1))
+                e = np.ascontiguousarray(st["rhoE"])
+                self._check(self.L.fv3d_state_upload(self.h, b + 1, pv.ctypes.data, e.ctypes.data), "state_upload")
+        else:
+            self._check(self.L.fv3d_initial(self.h), "initial")
 
     def _check(self, rc, what):
         if rc < 0:
@@ -115,7 +126,11 @@ class GpuSolver:
     def set_timing(self, on: bool):
         self._check(self.L.fv3d_set_timing(self.h, int(on)), "set_timing")
 
-    TIMING_SLOTS = 7   # FV3D_TIMING_SLOTS
+    TIMING_SLOTS = 8   # FV3D_TIMING_SLOTS
+
+    @property
+    def levels(self) -> int:
+        return self._check(self.L.fv3d_levels(self.h), "levels")
 
     def timing(self):
         ms = np.zeros(self.TIMING_SLOTS)
@@ -127,9 +142,12 @@ class GpuSolver:
     def n_cells(self) -> int:
         return int(self.L.fv3d_cell_count(self.h))
 
-    def get(self, block: int, name: str) -> np.ndarray:
+    def get(self, block: int, name: str, level: int = 1) -> np.ndarray:
+        """A field of one block; level 2 reads the coarse level of the two-grid cycle."""
         bi = self.case.blocks[block - 1]
         im, jm, km = bi.im, bi.jm, bi.km
+        if level == 2:
+            im, jm, km = (im + 1) // 2, (jm + 1) // 2, (km + 1) // 2
         ni, nj, nk = im - 1, jm - 1, km - 1
         if name in ("cv", "pv"):
             shape = (im + 1, jm + 1, km + 1, 5 if name == "cv" else 6)
@@ -149,7 +167,7 @@ class GpuSolver:
         else:
             raise KeyError(name)
         out = np.zeros(shape)
-        n = self.L.fv3d_field_download(self.h, block, name.encode(), out.ctypes.data)
+        n = self.L.fv3d_level_field_download(self.h, level, block, name.encode(), out.ctypes.data)
         self._check(n, "field_download " + name)
         if n != out.size:
             raise _lib.Fv3dError(f"field {name}: got {n} values, expected {out.size}")

@@ -56,7 +56,13 @@ typedef struct {
     int limiter;        /* limit flow variables (source/Limit_FlowVars.cpp:8-119) */
     int global_dt;      /* 0 local time step */
     int device;         /* HIP device ordinal */
-    int precision;      /* enum fv3d_precision */
+    int precision;      /* enum fv3d_precision, for both grid levels */
+    /* two-grid cycle and start field (source/UserInput.cpp:50-58, 72-92) */
+    int multigrid;      /* UseMultigrid: a second, coarse level when every patch range allows it */
+    int coarse_iter;    /* Coarse_iter: coarse-level iterations before the cycle */
+    int initial_type;   /* start field of fv3d_initial: 0 uniform, 1 interpolated along z */
+    double p21ratio;    /* P21ratio: outlet/inlet static pressure of the interpolated start */
+    double outlet_dir[3];   /* OutletAirAngle: flow direction at the outlet of the interpolated start */
 } fv3d_config;
 
 /* Context life cycle. */
@@ -83,8 +89,24 @@ int fv3d_connection_add(fv3d_ctx *ctx, int kind, int orient, int block1, int fac
                         int face2, const int *rng2);
 
 /* Metrics (CGeometry::FaceVectorVolume, source/GeometryVariables.cpp:10-105),
- * device allocation and launch plans.  Must follow the last block/patch add. */
+ * device allocation and launch plans.  Must follow the last block/patch add.
+ * With cfg.multigrid it decides the number of grid levels as
+ * CUserInput::NumberOfMultigrid does (source/UserInput.cpp:799-1067: two when
+ * every patch range starts and ends on an odd node, else one) and sets up the
+ * coarse level (nodes 2i-1, 2j-1, 2k-1 and halved patch ranges,
+ * source/UserInput.cpp:241-334 and 446-797). */
 int fv3d_setup(fv3d_ctx *ctx);
+
+/* Grid levels in use after fv3d_setup: 1, or 2 for the two-grid cycle. */
+int fv3d_levels(const fv3d_ctx *ctx);
+
+/* CMesh::Initial (source/ComputeAllBlockVars.cpp:38-48; Initial_Uniform and
+ * Initial_Interpolate, source/Initialization.cpp:17-165): the start field when
+ * there is no restart file (reuse_solution = N), on the level the reference
+ * starts from: the coarse one with two levels, else the only one.  The
+ * two-grid cycle must start here: from a restart file the reference's cycle
+ * divides by coarse time steps it never computed. */
+int fv3d_initial(fv3d_ctx *ctx);
 
 /* Restart-file state (CUserInput::ReadSolution, source/UserInput.cpp:1118-1159):
  * pv (ni,nj,nk,6) = rho,u,v,w,p,T and ener (ni,nj,nk), i slowest. */
@@ -102,6 +124,9 @@ int fv3d_state_upload(fv3d_ctx *ctx, int block, const double *pv, const double *
  *   and the same with _pv and 6 components -- Dummy layers 1..2.
  * Returns the number of doubles written. */
 long fv3d_field_download(fv3d_ctx *ctx, int block, const char *name, double *out);
+
+/* The same for a grid level: 1 fine, 2 coarse (two-grid cycle). */
+long fv3d_level_field_download(fv3d_ctx *ctx, int level, int block, const char *name, double *out);
 
 /* Stages of CMesh::ResFlux (source/ComputeAllBlockVars.cpp:223-255). */
 int fv3d_bc_apply(fv3d_ctx *ctx);          /* CMesh::BoundaryCondition */
@@ -130,9 +155,13 @@ int fv3d_res_calculate(fv3d_ctx *ctx, double *res6);
  * out5 = mass_in, mass_out, PR, efficiency, mass-flow ratio. */
 int fv3d_perf(fv3d_ctx *ctx, double *out5);
 
-/* n iterations of CSolver::Solver (source/Solver.cpp:47-66) with the CFL ramp
- * kept on the device, each followed by the performance integrals.  hist (may be
- * NULL) receives n rows of: iter res_dens mass_in mass_out PR eff CFL.
+/* n iterations of CSolver::Solver (source/Solver.cpp:47-135) with the CFL ramp
+ * kept on the device, each followed by the performance integrals of the level
+ * the reference reports (source/Structured_FV3D.cpp:148-158).  With two levels:
+ * coarse LU-SGS up to Coarse_iter, the coarse-to-fine state interpolation at
+ * Coarse_iter, then per iteration fine LU-SGS, fine residual, restriction,
+ * forced coarse LU-SGS and the prolonged correction.  hist (may be NULL)
+ * receives n rows of: iter res_dens mass_in mass_out PR eff CFL.
  * Synchronises once at the end. */
 int fv3d_iterate(fv3d_ctx *ctx, int n, double *hist);
 
@@ -145,9 +174,11 @@ int fv3d_sync(fv3d_ctx *ctx);
  * kernel class: 0 residual, 1 LU-SGS sweeps (with the row gather and dQ
  * scatter of the default sweep form), 2 update, 3 ghost fill + spectral + time
  * step, 4 norms/integrals, 5 forward sweep kernel, 6 backward sweep kernel
- * (5 and 6 for the default form only).  ms and counts hold FV3D_TIMING_SLOTS
- * entries.  Filled by fv3d_iterate when timing is on. */
-#define FV3D_TIMING_SLOTS 7
+ * (5 and 6 for the default form only), 7 coarse-grid correction (second fine
+ * residual, restriction, coarse LU-SGS, prolongation; two-grid cycle only).
+ * ms and counts hold FV3D_TIMING_SLOTS entries.  Filled by fv3d_iterate when
+ * timing is on (fine-level iterations only). */
+#define FV3D_TIMING_SLOTS 8
 int fv3d_set_timing(fv3d_ctx *ctx, int on);
 int fv3d_timing(fv3d_ctx *ctx, double *ms, long *counts);
 

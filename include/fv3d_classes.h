@@ -93,16 +93,24 @@ inline CaseConfig ReadConfig(const std::string &path)
     const char tt = v(9).empty() ? 'G' : v(9)[0];
     r.turbo_type = (tt == 'A' || tt == 'a') ? 0 : (tt == 'C' || tt == 'c') ? 1 : 2;
     r.viscous = !(v(10).size() && (v(10)[0] == 'E' || v(10)[0] == 'e'));
+    r.initial_type = (v(11).size() && (v(11)[0] == 'U' || v(11)[0] == 'u')) ? 0 : 1;
     r.rgas = num(12); r.gamma = num(13); r.pr_lam = num(14); r.pr_tur = num(15);
     r.pt_in = num(19); r.tt_in = num(20); r.p_out = num(21);
+    r.p21ratio = num(22);
     r.ma0 = num(23); r.len_ref = num(24); r.ev_ratio = num(25);
     {
         std::istringstream s(v(26));
         s >> r.inlet_dir[0] >> r.inlet_dir[1] >> r.inlet_dir[2];
     }
+    {
+        std::istringstream s(v(27));
+        s >> r.outlet_dir[0] >> r.outlet_dir[1] >> r.outlet_dir[2];
+    }
     c.max_iter = inum(33); c.coarse_iter = inum(34);
     c.conv_tol = num(36); c.n_threads = inum(37);
     c.multigrid = flag(38); c.reuse_solution = flag(39);
+    r.multigrid = c.multigrid;
+    r.coarse_iter = c.coarse_iter;
     r.cfl = num(44); r.c_timestep = num(45); r.relax = num(46);
     r.limiter = flag(51);
     r.global_dt = !(v(52).size() && (v(52)[0] == 'L' || v(52)[0] == 'l'));
@@ -290,13 +298,18 @@ public:
         check(ctx_, fv3d_connection_add(ctx_, kind, orient, block1, face1, rng1, block2, face2, rng2),
               "fv3d_connection_add");
     }
-    // CMesh::Geometry: metrics and device set-up.
+    // CMesh::Geometry: metrics and device set-up, and with UseMultigrid the
+    // coarse level when the patch ranges allow it.
     void Geometry() { check(ctx_, fv3d_setup(ctx_), "fv3d_setup"); }
-    // CMesh::Initial from a restart solution.
+    int Levels() const { return fv3d_levels(ctx_); }
+    // CUserInput::ReadSolution: one block's restart state.
     void Initial(int block, const BlockState &st)
     {
         check(ctx_, fv3d_state_upload(ctx_, block, st.pv.data(), st.ener.data()), "fv3d_state_upload");
     }
+    // CMesh::Initial: the uniform or interpolated start field (on the coarse
+    // level when two levels run).
+    void Initial() { check(ctx_, fv3d_initial(ctx_), "fv3d_initial"); }
 
     void BoundaryCondition() { check(ctx_, fv3d_bc_apply(ctx_), "fv3d_bc_apply"); }
     void ResFlux() { check(ctx_, fv3d_resflux(ctx_), "fv3d_resflux"); }
@@ -368,7 +381,8 @@ using HistoryRow = std::array<double, 7>;
 class CSolver {
 public:
     explicit CSolver(CMesh &m, bool graph = true) : m_(m), graph_(graph) {}
-    // CSolver::Solver: one single-grid iteration with the CFL ramp.
+    // CSolver::Solver: one iteration with the CFL ramp (single grid, or the
+    // coarse start-up phase / two-grid cycle).
     HistoryRow Solver()
     {
         HistoryRow r{};
@@ -391,7 +405,8 @@ private:
 };
 
 // Loads a reference case directory (config.dat, grid, boundary table and,
-// when reuse_solution is set, RestartSolution.dat) into a mesh.
+// when reuse_solution is set, RestartSolution.dat) into a mesh; without
+// restart states the start field is CMesh::Initial's.
 inline void LoadCase(CMesh &mesh, const std::string &dir, const BndTable &bnd,
                      const std::vector<std::vector<double>> &nodes, const std::vector<std::array<int, 3>> &dims,
                      const std::vector<BlockState> &states)
@@ -410,6 +425,8 @@ inline void LoadCase(CMesh &mesh, const std::string &dir, const BndTable &bnd,
             mesh.AddPatch(p.kind, p.block, p.face, p.rng, p.rotate);
     }
     mesh.Geometry();
+    if (states.empty())
+        mesh.Initial();
     for (size_t b = 0; b < states.size(); ++b) mesh.Initial((int)b + 1, states[b]);
 }
 

@@ -31,6 +31,8 @@ def case_dir(tmp_path):
     case.blocks[0].nblade = 17
     case.patches[2].rotate = 1
     case.cfg.turbo_type = "C"
+    case.cfg.use_multigrid, case.cfg.coarse_iter, case.cfg.initial = "Y", 37, "I"
+    case.cfg.p21_ratio, case.cfg.outlet_angle = 1.125, (0.25, -0.5, 1.0)
     d = str(tmp_path / "case")
     case.write(d)
     return case, d
@@ -51,6 +53,8 @@ def test_parse_only_matches_written_case(case_dir):
     assert c["inlet_dir"] == list(cfg.inlet_angle)
     assert (c["turbo_type"], c["viscous"], c["recon_var"], c["recon_scheme"], c["flux_scheme"]) == (1, 0, 100, 400, 200)
     assert c["max_iter"] == cfg.max_iter and c["reuse_solution"] == 1 and c["time_scheme"] == cfg.time_scheme
+    assert (c["multigrid"], c["coarse_iter"], c["initial_type"], c["p21ratio"]) == (1, 37, 1, 1.125)
+    assert c["outlet_dir"] == [0.25, -0.5, 1.0]
     kinds = {"inlet": 0, "outlet": 1, "wall": 2}
     want_p = sorted([[kinds[p.kind], p.block, p.face, p.rotate] + list(p.rng) for p in case.patches])
     assert sorted(s["patches"]) == want_p
