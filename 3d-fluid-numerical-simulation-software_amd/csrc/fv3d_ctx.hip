@@ -69,13 +69,14 @@ struct fv3d_ctx {
     long mbox_words = 0;
     long *d_pack_start = nullptr;     // running pack workgroup counts per block
     long pack_groups = 0;
-    // Baldwin-Lomax launches: round r takes the r-th wall patch of every block;
-    // d_bl_walls holds (patch index, line count) pairs, round r at offset
-    // bl_rounds[r].first with bl_round_n[r] entries and bl_rounds[r].second lines
+    // Baldwin-Lomax: one launch per group size G (16, 32 or 64 cell groups per
+    // line, by line length) over every wall that needs it; d_bl_walls holds
+    // (patch index, line count) pairs, launch q at offset bl_launch[q].first
+    // with bl_launch_n[q] walls and bl_launch[q].second lines
     int2 *d_bl_walls = nullptr;
-    std::vector<std::pair<int, long>> bl_rounds;
-    std::vector<int> bl_round_n;
-    std::vector<int> bl_groups;   // cell groups per line of each round (16, 32 or 64)
+    std::vector<std::pair<int, long>> bl_launch;
+    std::vector<int> bl_launch_n;
+    std::vector<int> bl_groups;
     long *d_pstart_bc = nullptr, *d_cstart = nullptr, *d_pstart_plane = nullptr;
     int4 *d_tiles = nullptr;
     int2 *d_secs = nullptr;
@@ -110,6 +111,8 @@ struct fv3d_ctx {
     // live in the fine level's scalars (ramp_sc); d_cfl points at its CFL.
     fv3d_ctx *coarse = nullptr;
     fv3d_ctx *parent = nullptr;
+    int4 *d_mg_tiles = nullptr;      // prolongation tiles over the coarse blocks
+    int n_mg_tiles = 0;
     bool owns_stream = true;
     bool mg_coarse = false;          // allocate cv_2h / Res_2h (coarse level)
     int ng = 1;
@@ -303,10 +306,13 @@ static int setup_block(fv3d_ctx *c, HostBlock &b, int index)
     }
     if (c->cfg.turbulence) {
         if ((rc = alloc(&B.mut, B.box))) return rc;
-        int *f;
+        unsigned long long *f;
         if ((rc = dalloc(c, &f, B.box))) return rc;
         b.allocs.push_back(f);
-        B.blflag = f;
+        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((B.box + 255) / 256)), dim3(256), 0, c->stream, f, B.box,
+                           kBlUnvisited);
+        HIPCK(c, hipGetLastError());
+        B.blmin = f;
     }
     if ((rc = alloc(&B.alpha, B.box))) return rc;
     {
@@ -545,6 +551,7 @@ int fv3d_ctx_destroy(fv3d_ctx *c)
     if (c->d_per) (void)hipFree(c->d_per);
     if (c->d_pitch_sign) (void)hipFree(c->d_pitch_sign);
     if (c->d_mbox) (void)hipFree(c->d_mbox);
+    if (c->d_mg_tiles) (void)hipFree(c->d_mg_tiles);
     if (c->d_pack_start) (void)hipFree(c->d_pack_start);
     for (int d = 0; d < 3; ++d) {
         if (c->vdbg.fv[d]) (void)hipFree(c->vdbg.fv[d]);
@@ -730,6 +737,19 @@ static int setup_coarse(fv3d_ctx *c)
             return bail(rc);
     }
     if ((rc = fv3d_setup(m))) return bail(rc);
+    std::vector<int4> tl;
+    for (int n = 0; n < (int)m->blocks.size(); ++n) {
+        const BlockDev &B = m->blocks[n].dev;
+        for (int I = 1; I <= B.ni; I += kMgTI)
+            for (int J = 1; J <= B.nj; J += kMgTJ)
+                for (int K = 1; K <= B.nk; K += kMgTK) tl.push_back(make_int4(n, I, J, K));
+    }
+    if ((rc = dalloc(c, &c->d_mg_tiles, tl.size()))) {
+        fv3d_ctx_destroy(m);
+        return rc;
+    }
+    HIPCK(c, hipMemcpyAsync(c->d_mg_tiles, tl.data(), tl.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    c->n_mg_tiles = (int)tl.size();
     c->coarse = m;
     c->ng = 2;
     return FV3D_OK;
@@ -825,32 +845,29 @@ int fv3d_setup(fv3d_ctx *c)
                 return fail(c, FV3D_EINVAL, "Baldwin-Lomax on a K-face wall needs JM == KM (block %d)", P.block);
         }
     if (c->cfg.turbulence == 100) {
-        std::vector<std::vector<int>> per_block(c->blocks.size());
-        for (int q = 0; q < (int)c->patches.size(); ++q)
-            if (c->patches[q].kind == 2) per_block[c->patches[q].block - 1].push_back(q);
         std::vector<int2> wl;
-        for (size_t r = 0;; ++r) {
+        for (int G = 16; G <= 64; G *= 2) {
             long lines = 0;
-            int n = 0, first = (int)wl.size(), maxlen = 0;
-            for (auto &v : per_block)
-                if (r < v.size()) {
-                    const PatchDev &q = pd[v[r]];
-                    // lines rounded up to whole workgroups, so a workgroup never spans two walls
-                    const long nl = ((long)q.na * q.nb + kBlLines - 1) / kBlLines * kBlLines;
-                    wl.push_back(make_int2(v[r], (int)nl));
-                    lines += nl;
-                    ++n;
-                    const HostBlock &hb = c->blocks[q.block];
-                    const int N = q.dir == 0 ? hb.IM : q.dir == 1 ? hb.JM : hb.KM;
-                    maxlen = std::max(maxlen, N - 1);
-                }
-            if (!n) break;
-            int G = 16;
-            while (G * kBlCpt < maxlen && G < 64) G *= 2;
-            if (G * kBlCpt < maxlen)
-                return fail(c, FV3D_EINVAL, "Baldwin-Lomax lines are limited to %d cells", 64 * kBlCpt);
-            c->bl_rounds.push_back({first, lines});
-            c->bl_round_n.push_back(n);
+            int n = 0, first = (int)wl.size();
+            for (int q = 0; q < (int)pd.size(); ++q) {
+                if (c->patches[q].kind != 2) continue;
+                const PatchDev &P = pd[q];
+                const HostBlock &hb = c->blocks[P.block];
+                const int len = (P.dir == 0 ? hb.IM : P.dir == 1 ? hb.JM : hb.KM) - 1;
+                int need = 16;
+                while (need * kBlCpt < len && need < 64) need *= 2;
+                if (need * kBlCpt < len)
+                    return fail(c, FV3D_EINVAL, "Baldwin-Lomax lines are limited to %d cells", 64 * kBlCpt);
+                if (need != G) continue;
+                // lines rounded up to whole workgroups, so a workgroup never spans two walls
+                const long nl = ((long)P.na * P.nb + kBlLines - 1) / kBlLines * kBlLines;
+                wl.push_back(make_int2(q, (int)nl));
+                lines += nl;
+                ++n;
+            }
+            if (!n) continue;
+            c->bl_launch.push_back({first, lines});
+            c->bl_launch_n.push_back(n);
             c->bl_groups.push_back(G);
         }
         if (!wl.empty()) {
@@ -1141,20 +1158,17 @@ static int launch_viscous(fv3d_ctx *c)
 }
 
 // Turbulence model before the time step (source/ComputeAllBlockVars.cpp:
-// 228-232): the vorticity, then Baldwin-Lomax wall by wall in table order.
+// 228-232): the vorticity, then Baldwin-Lomax on all walls at once (one launch
+// per line group size) and the merge of the per-cell minima.
 static int launch_turbulence(fv3d_ctx *c)
 {
     if (c->cfg.turbulence != 100) return FV3D_OK;
     const int nb = (int)c->blocks.size();
-    for (int n = 0; n < nb; ++n) {
-        const BlockDev &B = c->blocks[n].dev;
-        hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((B.box + 255) / 256)), dim3(256), 0, c->stream, B.blflag, B.box, 0);
-    }
     hipLaunchKernelGGL(k_curl, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb);
-    for (size_t r = 0; r < c->bl_rounds.size(); ++r) {
-        const unsigned wgs = (unsigned)(c->bl_rounds[r].second / kBlLines);
-        const int2 *wl = c->d_bl_walls + c->bl_rounds[r].first;
-        const int G = c->bl_groups[r], n = c->bl_round_n[r];
+    for (size_t r = 0; r < c->bl_launch.size(); ++r) {
+        const unsigned wgs = (unsigned)(c->bl_launch[r].second / kBlLines);
+        const int2 *wl = c->d_bl_walls + c->bl_launch[r].first;
+        const int G = c->bl_groups[r], n = c->bl_launch_n[r];
         if (G == 16)
             hipLaunchKernelGGL(k_bl_lines<16>, dim3(wgs), dim3(kBlLines * 16), 0, c->stream, c->d_blocks, c->d_patches, wl, n, c->gas);
         else if (G == 32)
@@ -1162,6 +1176,9 @@ static int launch_turbulence(fv3d_ctx *c)
         else
             hipLaunchKernelGGL(k_bl_lines<64>, dim3(wgs), dim3(kBlLines * 64), 0, c->stream, c->d_blocks, c->d_patches, wl, n, c->gas);
     }
+    if (!c->bl_launch.empty())
+        hipLaunchKernelGGL(k_bl_merge, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
+                           c->d_cstart, nb);
     HIPCK(c, hipGetLastError());
     return FV3D_OK;
 }
@@ -1426,8 +1443,8 @@ static int mg_iteration(fv3d_ctx *c, int phase, double *row)
         if ((rc = lusgs_timed(m))) return rc;
         if (phase == kMgSwitch) {
             hipLaunchKernelGGL(k_mg_corner, dim3((unsigned)nb), dim3(256), 0, c->stream, m->d_blocks);
-            hipLaunchKernelGGL(k_mg_prolong<true>, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream,
-                               c->d_blocks, m->d_blocks, c->d_cstart, nb, c->gas, c->d_bad);
+            hipLaunchKernelGGL(k_mg_prolong<true>, dim3((unsigned)c->n_mg_tiles), dim3(256), 0, c->stream, c->d_blocks,
+                               m->d_blocks, c->d_mg_tiles, c->gas, c->d_bad);
             HIPCK(c, hipGetLastError());
             if ((rc = launch_limiter(c))) return rc;
         }
@@ -1445,8 +1462,8 @@ static int mg_iteration(fv3d_ctx *c, int phase, double *row)
     if ((rc = launch_limiter(m))) return rc;
     if ((rc = launch_sweeps(m))) return rc;
     if ((rc = launch_update(m))) return rc;
-    hipLaunchKernelGGL(k_mg_prolong<false>, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream,
-                       c->d_blocks, m->d_blocks, c->d_cstart, nb, c->gas, c->d_bad);
+    hipLaunchKernelGGL(k_mg_prolong<false>, dim3((unsigned)c->n_mg_tiles), dim3(256), 0, c->stream, c->d_blocks,
+                       m->d_blocks, c->d_mg_tiles, c->gas, c->d_bad);
     HIPCK(c, hipGetLastError());
     if ((rc = launch_limiter(c))) return rc;
     if (c->timing) HIPCK(c, hipEventRecord(c->ev[9], c->stream));

@@ -55,9 +55,11 @@ struct BlockDev {
     double *lamv;
     double *omega;
     // Baldwin-Lomax (null without a turbulence model): eddy viscosity over the
-    // cell box (ghost cells included) and the per-iteration visited flags
+    // cell box (ghost cells included) and the minimum over the walls whose lines
+    // visit a cell in this evaluation, as the bits of a non-negative double
+    // (all ones = not visited; the merge resets it)
     double *mut;
-    int *blflag;
+    unsigned long long *blmin;
     double *dt;               // local time step
     double *alpha;            // LU-SGS diagonal Vol/dt + relax*sum(lamc) + 2*sum(lamv)
     const double *swg;        // 15*box static sweep metrics per cell: for d = 0..2,

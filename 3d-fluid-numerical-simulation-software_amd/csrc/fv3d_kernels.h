@@ -111,9 +111,10 @@ __global__ void k_finalize(const double *partial, int npart, const double *sums,
 __global__ void k_mg_restrict(const BlockDev *fine, const BlockDev *coarse, const long *cstart, int nblock, Gas g,
                               int *bad_flag);
 __global__ void k_mg_corner(const BlockDev *coarse);
+// coarse tile of the prolongation (tiles[w] = block, first coarse I, J, K)
+constexpr int kMgTI = 2, kMgTJ = 4, kMgTK = 8;
 template <bool STATE>
-__global__ void k_mg_prolong(const BlockDev *fine, const BlockDev *coarse, const long *cstart, int nblock, Gas g,
-                             int *bad_flag);
+__global__ void k_mg_prolong(const BlockDev *fine, const BlockDev *coarse, const int4 *tiles, Gas g, int *bad_flag);
 
 // laminar Navier-Stokes terms (fv3d_viscous.hip)
 // coordinate-transformation coefficients of every face and cell of one block (set-up)
@@ -129,12 +130,15 @@ template <int TI, int TJ, int TK>
 __global__ void k_viscous(const BlockDev *blocks, const int4 *tiles, Gas g, int store, ViscDebug dbg);
 // vorticity magnitude per cell
 __global__ void k_curl(const BlockDev *blocks, const long *cstart, int nblock);
-// Baldwin-Lomax along the grid lines normal to a set of wall patches (at most
-// one per block): kBlLines lines x G groups of kBlCpt cells per workgroup (so
-// lines of up to G*kBlCpt cells); walls[w] = (patch index, line count rounded
-// up to a multiple of kBlLines)
+// Baldwin-Lomax along the grid lines normal to a set of wall patches:
+// kBlLines lines x G groups of kBlCpt cells per workgroup (so lines of up to
+// G*kBlCpt cells); walls[w] = (patch index, line count rounded up to a
+// multiple of kBlLines).  Each cell's value is folded into blmin with an
+// atomic minimum; k_bl_merge then moves the minima into mut.
 constexpr int kBlLines = 16, kBlCpt = 8;
+constexpr unsigned long long kBlUnvisited = ~0ull;
 template <int G>
 __global__ void k_bl_lines(const BlockDev *blocks, const PatchDev *patches, const int2 *walls, int nwalls, Gas g);
+__global__ void k_bl_merge(const BlockDev *blocks, const long *cstart, int nblock);
 
 }  // namespace fv3d

@@ -141,48 +141,68 @@ __global__ void __launch_bounds__(256) k_mg_corner(const BlockDev *__restrict__ 
 // face with it on the fine cell's side, the three sharing an edge and the one
 // sharing a corner; STATE sets the fine state from the coarse one (end of the
 // coarse phase), else the coarse correction Q_H - cv_2h is added (zero in the
-// coarse ghost cells, which the reference never writes).  Fused: the fine
+// coarse ghost cells, which the reference never writes).  A workgroup takes a
+// tile of kMgTI x kMgTJ x kMgTK coarse cells: it stages their values and a
+// one-cell halo in LDS (each coarse value read once), then forms the eight
+// fine cells under every coarse cell of the tile.  Fused: the fine
 // PrimitiveVars and the bad-cell flags for the limiter.
 template <bool STATE>
 __global__ void __launch_bounds__(256) k_mg_prolong(const BlockDev *__restrict__ fine, const BlockDev *__restrict__ coarse,
-                                                    const long *__restrict__ cstart, int nblock, Gas g,
-                                                    int *__restrict__ bad_flag)
+                                                    const int4 *__restrict__ tiles, Gas g, int *__restrict__ bad_flag)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cstart[nblock]) return;
-    const int n = find_block(cstart, nblock, t);
+    constexpr int HI = kMgTI + 2, HJ = kMgTJ + 2, HK = kMgTK + 2, NH = HI * HJ * HK;
+    __shared__ double sd[5][NH];
+    const int4 tl = tiles[blockIdx.x];
+    const int n = tl.x, I0 = tl.y, J0 = tl.z, K0 = tl.w;
     const BlockDev &F = fine[n], &B = coarse[n];
-    long r = t - cstart[n];
-    const int k = (int)(r % F.nk) + 1;
-    r /= F.nk;
-    const int j = (int)(r % F.nj) + 1;
-    const int i = (int)(r / F.nj) + 1;
-    const int i1 = (i & 1) ? i / 2 + 1 : i / 2, i2 = (i & 1) ? i / 2 : i / 2 + 1;
-    const int j1 = (j & 1) ? j / 2 + 1 : j / 2, j2 = (j & 1) ? j / 2 : j / 2 + 1;
-    const int k1 = (k & 1) ? k / 2 + 1 : k / 2, k2 = (k & 1) ? k / 2 : k / 2 + 1;
+    const long cbox = B.box;
+    for (int e = threadIdx.x; e < NH; e += blockDim.x) {
+        const int c3 = e % HK, b3 = (e / HK) % HJ, a3 = e / (HK * HJ);
+        const int I = I0 - 1 + a3, J = J0 - 1 + b3, K = K0 - 1 + c3;
+        const bool inbox = I <= B.ni + 1 && J <= B.nj + 1 && K <= B.nk + 1;
+        const bool interior = I >= 1 && I <= B.ni && J >= 1 && J <= B.nj && K >= 1 && K <= B.nk;
+        const long y = inbox ? cidx(B, I, J, K) : 0;
+        for (int c = 0; c < 5; ++c) {
+            double v = 0.0;
+            if (STATE) {
+                if (inbox) v = B.Q[c * cbox + y];
+            } else if (interior) {
+                v = B.Q[c * cbox + y] - B.cv2h[c * cbox + y];
+            }
+            sd[c][e] = v;
+        }
+    }
+    __syncthreads();
     const double a1 = (double)(27.0f / 64.0f), a2 = (double)(9.0f / 64.0f), a3 = (double)(3.0f / 64.0f),
                  a4 = (double)(1.0f / 64.0f);
-    const long box = B.box, x = cidx(F, i, j, k);
-    auto val = [&](int c, int p, int q, int s) -> double {
-        const long y = cidx(B, p, q, s);
-        if (STATE) return B.Q[c * box + y];
-        const bool in = p >= 1 && p <= B.ni && q >= 1 && q <= B.nj && s >= 1 && s <= B.nk;
-        return in ? B.Q[c * box + y] - B.cv2h[c * box + y] : 0.0;
-    };
-    double qn[5];
-    for (int c = 0; c < 5; ++c) {
-        const double t1 = val(c, i1, j1, k1) * a1;
-        const double t2 = ((val(c, i1, j1, k2) + val(c, i1, j2, k1)) + val(c, i2, j1, k1)) * a2;
-        const double t3 = ((val(c, i2, j2, k1) + val(c, i1, j2, k2)) + val(c, i2, j1, k2)) * a3;
-        const double t4 = val(c, i2, j2, k2) * a4;
-        const double v = ((t1 + t2) + t3) + t4;
-        qn[c] = STATE ? v : F.Q[c * F.box + x] + v;
-        F.Q[c * F.box + x] = qn[c];
+    constexpr int FI = 2 * kMgTI, FJ = 2 * kMgTJ, FK = 2 * kMgTK;
+    for (int e = threadIdx.x; e < FI * FJ * FK; e += blockDim.x) {
+        const int fk = e % FK, fj = (e / FK) % FJ, fi = e / (FK * FJ);
+        const int i = 2 * I0 - 1 + fi, j = 2 * J0 - 1 + fj, k = 2 * K0 - 1 + fk;
+        if (i > F.ni || j > F.nj || k > F.nk) continue;
+        // nearest coarse cell and the neighbour towards the fine cell, in halo coordinates
+        const int i1 = (i + 1) / 2 - (I0 - 1), j1 = (j + 1) / 2 - (J0 - 1), k1 = (k + 1) / 2 - (K0 - 1);
+        const int i2 = (i & 1) ? i1 - 1 : i1 + 1, j2 = (j & 1) ? j1 - 1 : j1 + 1, k2 = (k & 1) ? k1 - 1 : k1 + 1;
+        auto at = [&](int p, int q, int r) { return (p * HJ + q) * HK + r; };
+        const int h111 = at(i1, j1, k1), h112 = at(i1, j1, k2), h121 = at(i1, j2, k1), h211 = at(i2, j1, k1);
+        const int h221 = at(i2, j2, k1), h122 = at(i1, j2, k2), h212 = at(i2, j1, k2), h222 = at(i2, j2, k2);
+        const long x = cidx(F, i, j, k);
+        double qn[5];
+        for (int c = 0; c < 5; ++c) {
+            const double *S = sd[c];
+            const double t1 = S[h111] * a1;
+            const double t2 = ((S[h112] + S[h121]) + S[h211]) * a2;
+            const double t3 = ((S[h221] + S[h122]) + S[h212]) * a3;
+            const double t4 = S[h222] * a4;
+            const double v = ((t1 + t2) + t3) + t4;
+            qn[c] = STATE ? v : F.Q[c * F.box + x] + v;
+            F.Q[c * F.box + x] = qn[c];
+        }
+        prim_vars(F, x, qn, g, bad_flag + n);
     }
-    prim_vars(F, x, qn, g, bad_flag + n);
 }
 
-template __global__ void k_mg_prolong<true>(const BlockDev *, const BlockDev *, const long *, int, Gas, int *);
-template __global__ void k_mg_prolong<false>(const BlockDev *, const BlockDev *, const long *, int, Gas, int *);
+template __global__ void k_mg_prolong<true>(const BlockDev *, const BlockDev *, const int4 *, Gas, int *);
+template __global__ void k_mg_prolong<false>(const BlockDev *, const BlockDev *, const int4 *, Gas, int *);
 
 }  // namespace fv3d

@@ -342,10 +342,15 @@ __global__ void k_curl(const BlockDev *__restrict__ blocks, const long *__restri
 
 // ------------------------------------------------------------- Baldwin-Lomax
 // CBlocks::BL_Model (source/BL_Model.cpp:8-279) with CBlocks::BL_model_1d
-// (283-329).  A launch takes one wall of each block (the r-th wall patch of
-// every block in table order), so walls of one block still run in table order
-// and a cell's first visit in the iteration sets mue_t while later walls keep
-// the minimum.  A workgroup serves kBlLines grid lines normal to the wall that
+// (283-329).  The reference visits the walls of a block in table order: a
+// cell's first visit sets mue_t, later walls keep the minimum, and each wall
+// zeroes its first cell.  Every value is non-negative (mue_in and mue_out are
+// products of non-negative factors), so that is the minimum over the visiting
+// walls whatever the order, and the first cell's zero wins it; all walls of
+// all blocks therefore run in one launch and fold their values into blmin with
+// an atomic minimum on the bits (ordered as the values for non-negative
+// doubles).  Cells that no wall line visits keep mue_t (k_bl_merge).  A
+// workgroup serves kBlLines grid lines normal to the wall that
 // are neighbours along their fastest tangential index (so a load of one cell
 // position is contiguous over the lines) and splits each line into G groups;
 // thread (group g, line l) holds the cells n0 = 1 + g + G c, c < kBlCpt, of its
@@ -389,7 +394,7 @@ __global__ void __launch_bounds__(kBlLines *G) k_bl_lines(const BlockDev *__rest
     const double wx = FC[xw], wy = FC[box + xw], wz = FC[2 * box + xw];
     const double A_plus = F32C(26.0), Cwk = F32C(1.0), kap = F32C(0.4), Ckleb = F32C(0.3), alpha = F32C(0.0168),
                  Ccp = F32C(1.6);
-    double dis[C], om[C], dens[C];
+    double dis[C], om[C], dens[C], damp[C];
     bool ok[C];
     double tw = -1.0, ud = 0.0, t1 = 0.0, d1 = 0.0, m1 = 0.0;
 #pragma unroll
@@ -398,7 +403,8 @@ __global__ void __launch_bounds__(kBlLines *G) k_bl_lines(const BlockDev *__rest
         ok[c] = active && n0 < M;
         const long x = cell(ok[c] ? n0 : 1);
         const double dx = B.cc[x] - wx, dy = B.cc[box + x] - wy, dz = B.cc[2 * box + x] - wz;
-        dis[c] = sqrt(pow(dx, 2.0) + pow(dy, 2.0) + pow(dz, 2.0));
+        // POW(x, 2.0) of the reference compiles to x*x
+        dis[c] = sqrt(dx * dx + dy * dy + dz * dz);
         om[c] = B.omega[x];
         dens[c] = B.W[x];
         const double ml = sutherland(g, B.W[5 * box + x]);
@@ -434,7 +440,8 @@ __global__ void __launch_bounds__(kBlLines *G) k_bl_lines(const BlockDev *__rest
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const double y_plus = dis[c] * Ret;
-        const double F = dis[c] * fabs(om[c]) * (F32C(1.0) - exp(-y_plus / A_plus));
+        damp[c] = F32C(1.0) - exp(-y_plus / A_plus);
+        const double F = dis[c] * fabs(om[c]) * damp[c];
         const int n0 = 1 + grp + G * c;
         if (ok[c] && F > 0.0 && (F > bF || (F == bF && n0 < bN))) {
             bF = F;
@@ -465,26 +472,42 @@ __global__ void __launch_bounds__(kBlLines *G) k_bl_lines(const BlockDev *__rest
     for (int c = 0; c < C; ++c) {
         if (!ok[c]) continue;
         const int n0 = 1 + grp + G * c;
-        const double y_plus = Ret * dis[c];
-        const double bL = kap * dis[c] * (F32C(1.0) - exp(-y_plus / A_plus));
+        const double bL = kap * dis[c] * damp[c];
         const double mue_in = dens[c] * bL * bL * om[c];
         const double Fkleb = F32C(1.0) / (F32C(1.0) + F32C(5.5) * pow((Ckleb * dis[c] / y_max), 6.0));
         const double mue_out = alpha * Ccp * dens[c] * Fwake * Fkleb;
         const double mt = fabs(mue_in) > fabs(mue_out) ? mue_out : mue_in;
         const long x = cell(n0);
-        double nv;
-        if (B.blflag[x] == 0) {
-            B.blflag[x] = 1;
-            nv = mt;
-        } else {
-            const double cur = B.mut[x];
-            nv = cur < mt ? cur : mt;
-        }
-        B.mut[x] = n0 == 1 ? 0.0 : nv;   // the first cell ends at zero (source/BL_Model.cpp:101-118)
+        // the first cell ends at zero (source/BL_Model.cpp:101-118)
+        const double v = n0 == 1 ? 0.0 : mt;
+        atomicMin(B.blmin + x, (unsigned long long)__double_as_longlong(v));
     }
     if (active && grp == 0) {
         const long xg = D == 0 ? cidx(B, hi ? N : 0, a, cc) : D == 1 ? cidx(B, a, hi ? N : 0, cc) : cidx(B, a, cc, hi ? N : 0);
         B.mut[xg] = -0.0;
+    }
+}
+
+// mue_t of every visited interior cell from the minimum over its walls; the
+// minimum resets to "not visited" for the next evaluation.
+__global__ void __launch_bounds__(256) k_bl_merge(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
+                                                  int nblock)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cstart[nblock]) return;
+    int n = 0;
+    while (t >= cstart[n + 1]) ++n;
+    const BlockDev &B = blocks[n];
+    long r = t - cstart[n];
+    const int k = (int)(r % B.nk) + 1;
+    r /= B.nk;
+    const int j = (int)(r % B.nj) + 1;
+    const int i = (int)(r / B.nj) + 1;
+    const long x = cidx(B, i, j, k);
+    const unsigned long long v = B.blmin[x];
+    if (v != kBlUnvisited) {
+        B.mut[x] = __longlong_as_double((long long)v);
+        B.blmin[x] = kBlUnvisited;
     }
 }
 
