@@ -113,6 +113,9 @@ struct fv3d_ctx {
     fv3d_ctx *parent = nullptr;
     int4 *d_mg_tiles = nullptr;      // prolongation tiles over the coarse blocks
     int n_mg_tiles = 0;
+    int *d_axial = nullptr;          // radial-equilibrium outlet patches (indices into d_patches)
+    int n_axial = 0;
+    unsigned long long *d_dtmin = nullptr;   // per-block minimum time step (global time step)
     bool owns_stream = true;
     bool mg_coarse = false;          // allocate cv_2h / Res_2h (coarse level)
     int ng = 1;
@@ -522,8 +525,6 @@ int fv3d_ctx_create(const fv3d_config *cfg, fv3d_ctx **out)
     if (cfg->flux_scheme != 200 && cfg->flux_scheme != 300) { *out = c; return fail(c, FV3D_EINVAL, "flux_scheme %d unsupported", cfg->flux_scheme); }
     if (cfg->turbulence != 0 && cfg->turbulence != 100) { *out = c; return fail(c, FV3D_EINVAL, "turbulence model %d not in this build", cfg->turbulence); }
     if (cfg->turbulence && !cfg->viscous) { *out = c; return fail(c, FV3D_EINVAL, "a turbulence model needs the Navier-Stokes equations"); }
-    if (cfg->global_dt) { *out = c; return fail(c, FV3D_EINVAL, "global time step not in this build"); }
-    if (cfg->turbo_type == 0) { *out = c; return fail(c, FV3D_EINVAL, "axial (radial-equilibrium) outlet not in this build"); }
     if (cfg->precision != FV3D_PLAIN_FP64 && cfg->precision != FV3D_DEFECT_FP32) { *out = c; return fail(c, FV3D_EINVAL, "precision mode %d", cfg->precision); }
     c->device = cfg->device;
     hipError_t e = hipSetDevice(c->device);
@@ -552,6 +553,8 @@ int fv3d_ctx_destroy(fv3d_ctx *c)
     if (c->d_pitch_sign) (void)hipFree(c->d_pitch_sign);
     if (c->d_mbox) (void)hipFree(c->d_mbox);
     if (c->d_mg_tiles) (void)hipFree(c->d_mg_tiles);
+    if (c->d_axial) (void)hipFree(c->d_axial);
+    if (c->d_dtmin) (void)hipFree(c->d_dtmin);
     if (c->d_pack_start) (void)hipFree(c->d_pack_start);
     for (int d = 0; d < 3; ++d) {
         if (c->vdbg.fv[d]) (void)hipFree(c->vdbg.fv[d]);
@@ -827,6 +830,7 @@ int fv3d_setup(fv3d_ctx *c)
         int bE = d == 2 ? r[3] : r[5];
         q.na = aE - q.a0 + 1; q.nb = bE - q.b0 + 1;
         q.centrifugal = c->cfg.turbo_type == 1;
+        q.axial = P.kind == 1 && c->cfg.turbo_type == 0;
         q.rot = P.rotate ? c->blocks[P.block - 1].w_rot : 0.0;
         pd.push_back(q);
         pst.push_back(pst.back() + (long)q.na * q.nb);
@@ -834,6 +838,25 @@ int fv3d_setup(fv3d_ctx *c)
         if (P.kind == 1 && nout++ == 0) secs[1] = make_int2(P.block - 1, r[4]);
     }
     if (nin == 0 || nout == 0) return fail(c, FV3D_EINVAL, "need an inlet and an outlet patch");
+    {
+        // radial-equilibrium outlets: each covers its block's whole K+ face
+        std::vector<int> ax;
+        for (int q = 0; q < (int)pd.size(); ++q) {
+            if (!pd[q].axial || pd[q].dir != 2 || !pd[q].hi) continue;
+            for (int r : ax)
+                if (pd[r].block == pd[q].block)
+                    return fail(c, FV3D_EINVAL, "two radial-equilibrium outlet patches on block %d", pd[q].block + 1);
+            if (c->blocks[pd[q].block].IM - 1 > kAxialMaxI)
+                return fail(c, FV3D_EINVAL, "radial-equilibrium outlet: more than %d cells along i", kAxialMaxI);
+            ax.push_back(q);
+        }
+        if (!ax.empty()) {
+            if ((rc = dalloc(c, &c->d_axial, ax.size()))) return rc;
+            HIPCK(c, hipMemcpyAsync(c->d_axial, ax.data(), ax.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+            c->n_axial = (int)ax.size();
+        }
+        if (c->cfg.global_dt && (rc = dalloc(c, &c->d_dtmin, nb))) return rc;
+    }
     if (c->cfg.turbulence == 100)
         for (const HostPatch &P : c->patches) {
             if (P.kind != 2) continue;
@@ -1119,6 +1142,9 @@ static int launch_bc(fv3d_ctx *c)
     if (c->nbc > 0)
         hipLaunchKernelGGL(k_bc, dim3((unsigned)((c->nbc + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks, c->d_patches,
                            c->npatch_dev, c->d_pstart_bc, c->gas, &c->d_sc->p_out);
+    if (c->n_axial > 0)
+        hipLaunchKernelGGL(k_bc_axial, dim3((unsigned)c->n_axial), dim3(256), 0, c->stream, c->d_blocks, c->d_patches,
+                           c->d_axial, c->gas, &c->d_sc->p_out);
     if (c->nper > 0)
         hipLaunchKernelGGL(k_periodic_flip, dim3(1), dim3(64), 0, c->stream, c->d_per, c->nper, c->d_pitch_sign);
     if (c->nexch > 0)
@@ -1133,6 +1159,15 @@ static int launch_spectral_dt(fv3d_ctx *c)
     hipLaunchKernelGGL(k_spectral_dt, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
                        c->d_cstart, (int)c->blocks.size(), c->gas, c->d_cfl);
     HIPCK(c, hipGetLastError());
+    if (c->cfg.global_dt) {
+        const int nb = (int)c->blocks.size();
+        const unsigned g = (unsigned)((c->ncell + 255) / 256);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, c->stream, c->d_dtmin, (long)nb,
+                           (unsigned long long)0x4202A05F20000000ull);   // 1E10
+        hipLaunchKernelGGL(k_dt_block_min, dim3(g), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb, c->d_dtmin);
+        hipLaunchKernelGGL(k_dt_global, dim3(g), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb, c->gas, c->d_dtmin);
+        HIPCK(c, hipGetLastError());
+    }
     hipLaunchKernelGGL(k_dt_flag_decrement, dim3(1), dim3(64), 0, c->stream, c->d_blocks, (int)c->blocks.size());
     HIPCK(c, hipGetLastError());
     return FV3D_OK;

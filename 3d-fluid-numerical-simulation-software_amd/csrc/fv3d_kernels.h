@@ -16,6 +16,7 @@ struct PatchDev {
     int a0, b0;        // first tangential cell indices
     int na, nb;        // tangential extents
     int centrifugal;   // outlet variant
+    int axial;         // radial-equilibrium outlet: applied by k_bc_axial, skipped by k_bc
     double rot;        // wall rotation rate
 };
 
@@ -33,6 +34,16 @@ struct ScalarsDev {
 
 __global__ void k_bc(const BlockDev *blocks, const PatchDev *patches, int npatch, const long *pstart, Gas g,
                      const double *pout_dev);
+// radial-equilibrium outlet, one workgroup per patch (list: patch indices);
+// the radial lines of a face are at most kAxialMaxI cells long
+constexpr int kAxialMaxI = 2048;
+__global__ void k_bc_axial(const BlockDev *blocks, const PatchDev *patches, const int *list, Gas g,
+                           const double *pout_dev);
+// global time step (timestep type "G"): per-block minimum of the local steps
+// (dtmin as bits of positive doubles), then dt and the LU-SGS diagonal
+__global__ void k_dt_block_min(const BlockDev *blocks, const long *cstart, int nblock, unsigned long long *dtmin);
+__global__ void k_dt_global(const BlockDev *blocks, const long *cstart, int nblock, Gas g,
+                            const unsigned long long *dtmin);
 // one interior/periodic transfer: the state of cell src of block src_block goes
 // to layer `layer` of Dummy slab `side` of block dst_block at tangential slot
 // `slot` (and to ghost cell `ghost` of that block, -1 for none); rot 0 copies,

@@ -80,8 +80,9 @@ __global__ void k_bc(const BlockDev *__restrict__ blocks, const PatchDev *__rest
         pv[0] = DensBC; pv[1] = u; pv[2] = v; pv[3] = w; pv[4] = PressBC; pv[5] = TempBC;
         cv[0] = DensBC; cv[1] = DensBC * u; cv[2] = DensBC * v; cv[3] = DensBC * w; cv[4] = DensBC * ener;
     } else if (P.kind == 1) {
-        // general / centrifugal outlet, K faces only
-        if (d != 2) return;
+        // general / centrifugal outlet, K faces only (the radial-equilibrium
+        // outlet is k_bc_axial's)
+        if (d != 2 || P.axial) return;
         const int kf = hi ? N : 1;
         const long xf = cidx(B, ii, jj, kf);
         double s_ = B.fs[2][xf];
@@ -137,6 +138,134 @@ __global__ void k_bc(const BlockDev *__restrict__ blocks, const PatchDev *__rest
     for (int c = 0; c < 5; ++c) B.Q[c * box + xg] = cv[c];
     for (int c = 0; c < 6; ++c) B.W[c * box + xg] = pv[c];
     set_dummies(B, s, a, b, cv, pv);
+}
+
+// CBndConds::BndCondOutlet_Axial (source/BndCondOutlet.cpp:7-114): static
+// pressure in radial equilibrium, dp/dr = rho c_u^2 / r, fixed to Pout at
+// i = (int)(0.5 IM) and integrated outwards along i on every j line; c_u is the
+// area-weighted circumferential velocity of the last cell layer per i.  As in
+// the reference it covers the whole K+ face of the block, reads the density of
+// the ghost layer left by the previous application, and integrates towards
+// the tip with the radii of the faces at KM-1.  One workgroup per patch: the
+// per-i averages (serial over j, the reference's order), then one thread per
+// j line, then the ghost cells and Dummy layers.
+__global__ void __launch_bounds__(256) k_bc_axial(const BlockDev *__restrict__ blocks,
+                                                  const PatchDev *__restrict__ patches, const int *__restrict__ list,
+                                                  Gas g, const double *__restrict__ pout_dev)
+{
+    __shared__ double cu[kAxialMaxI];
+    const PatchDev P = patches[list[blockIdx.x]];
+    if (P.dir != 2 || !P.hi) return;   // K+ faces only (source/BndCondOutlet.cpp:33-41)
+    const BlockDev &B = blocks[P.block];
+    const int IM = B.IM, JM = B.JM, KM = B.KM, k = KM - 1;
+    const long box = B.box;
+    const double *fc = B.fc[2], *fs = B.fs[2];
+    const double Gamma = g.gamma, Rgas = g.rgas;
+    for (int i = 1 + (int)threadIdx.x; i <= IM - 1; i += blockDim.x) {
+        double c_u = 0.0, S_temp = 0.0;
+        for (int j = 1; j <= JM - 1; ++j) {
+            const long xc = cidx(B, i, j, k), xf = cidx(B, i, j, KM);
+            const double th = atan(fc[box + xf] / fc[xf]);
+            const double cu_temp = F32C(-1.0) * B.W[box + xc] * sin(th) + B.W[2 * box + xc] * cos(th);
+            c_u = c_u + cu_temp * fs[xf];
+            S_temp = S_temp + fs[xf];
+        }
+        cu[i] = c_u / S_temp;
+    }
+    __syncthreads();
+    const int iref = (int)(0.5 * IM);
+    const double Pout = *pout_dev;
+    auto rad = [&](int i, int j, int kk) {
+        const long xf = cidx(B, i, j, kk);
+        return sqrt(fc[xf] * fc[xf] + fc[box + xf] * fc[box + xf]);
+    };
+    double *p = B.W + 4 * box;
+    for (int j = 1 + (int)threadIdx.x; j <= JM - 1; j += blockDim.x) {
+        p[cidx(B, iref, j, KM)] = Pout;
+        for (int i = iref - 1; i >= 1; --i) {
+            const double rn = rad(i, j, KM), rn_1 = rad(i + 1, j, KM);
+            const double dp_dr = B.Q[cidx(B, i, j, KM)] * cu[i] * cu[i] / rn;
+            p[cidx(B, i, j, KM)] = p[cidx(B, i + 1, j, KM)] - dp_dr * (rn_1 - rn);
+        }
+        for (int i = iref + 1; i <= IM - 1; ++i) {
+            const double rn_1 = rad(i, j, k), rn = rad(i - 1, j, k);
+            const double dp_dr = B.Q[cidx(B, i, j, KM)] * cu[i] * cu[i] / rn;
+            p[cidx(B, i, j, KM)] = p[cidx(B, i - 1, j, KM)] + dp_dr * (rn_1 - rn);
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < (IM - 1) * (JM - 1); e += blockDim.x) {
+        const int i = 1 + e / (JM - 1), j = 1 + e % (JM - 1);
+        const long xg = cidx(B, i, j, KM), xi = cidx(B, i, j, k);
+        double cv[5], pv[6];
+        for (int c = 0; c < 4; ++c) cv[c] = B.Q[c * box + xi];
+        pv[0] = cv[0];
+        pv[1] = B.W[box + xi]; pv[2] = B.W[2 * box + xi]; pv[3] = B.W[3 * box + xi];
+        pv[4] = p[xg];
+        pv[5] = pv[4] / Rgas / cv[0];
+        const double V2 = pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3];
+        cv[4] = pv[4] / (Gamma - F32C(1.0)) + pv[0] * V2 / F32C(2.0);
+        for (int c = 0; c < 5; ++c) B.Q[c * box + xg] = cv[c];
+        for (int c = 0; c < 6; ++c) B.W[c * box + xg] = pv[c];
+        set_dummies(B, 5, i, j, cv, pv);
+    }
+}
+
+// Global time step of CTimeDiscr::TimeStep (source/TimeStep.cpp:64-86): the
+// minimum over each block's cells, starting from 1E10 (the caller fills dtmin
+// with its bits), then every cell of the block takes it and the LU-SGS
+// diagonal is formed again from it.  Time steps are positive, so the bits
+// order as the values.
+__global__ void __launch_bounds__(256) k_dt_block_min(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
+                                                      int nblock, unsigned long long *__restrict__ dtmin)
+{
+    constexpr int kSlots = 16;
+    __shared__ unsigned long long smin[kSlots];
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t0 = (long)blockIdx.x * blockDim.x;
+    int n0 = 0;
+    while (n0 + 1 < nblock && t0 >= cstart[n0 + 1]) ++n0;
+    if (threadIdx.x < kSlots) smin[threadIdx.x] = ~0ull;
+    __syncthreads();
+    if (t < cstart[nblock]) {
+        int n = n0;
+        while (t >= cstart[n + 1]) ++n;
+        const BlockDev &B = blocks[n];
+        long r = t - cstart[n];
+        const int k = (int)(r % B.nk) + 1;
+        r /= B.nk;
+        const int j = (int)(r % B.nj) + 1;
+        const int i = (int)(r / B.nj) + 1;
+        const unsigned long long v = (unsigned long long)__double_as_longlong(B.dt[cidx(B, i, j, k)]);
+        if (n - n0 < kSlots) atomicMin(&smin[n - n0], v);
+        else atomicMin(dtmin + n, v);
+    }
+    __syncthreads();
+    if (threadIdx.x < kSlots && n0 + (int)threadIdx.x < nblock && smin[threadIdx.x] != ~0ull)
+        atomicMin(dtmin + n0 + threadIdx.x, smin[threadIdx.x]);
+}
+
+__global__ void __launch_bounds__(256) k_dt_global(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
+                                                   int nblock, Gas g, const unsigned long long *__restrict__ dtmin)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cstart[nblock]) return;
+    int n = 0;
+    while (t >= cstart[n + 1]) ++n;
+    const BlockDev &B = blocks[n];
+    long r = t - cstart[n];
+    const int k = (int)(r % B.nk) + 1;
+    r /= B.nk;
+    const int j = (int)(r % B.nj) + 1;
+    const int i = (int)(r / B.nj) + 1;
+    const long box = B.box, x = cidx(B, i, j, k);
+    const double dt = __longlong_as_double((long long)dtmin[n]);
+    B.dt[x] = dt;
+    double lvd[3] = {0.0, 0.0, 0.0};
+    if (g.viscous)
+        for (int d = 0; d < 3; ++d) lvd[d] = B.lamv[d * box + x];
+    B.alpha[x] = B.vol[x] / dt + g.relax * (B.lamc[x] + B.lamc[box + x] + B.lamc[2 * box + x]) +
+                 F32C(2.0) * (lvd[0] + lvd[1] + lvd[2]);
 }
 
 // ================================================== interior / periodic

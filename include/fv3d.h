@@ -54,7 +54,7 @@ typedef struct {
     int flux_scheme;    /* 200 Steger-Warming, 300 Roe */
     int turbulence;     /* 0 none (laminar), 100 Baldwin-Lomax (source/BL_Model.cpp) */
     int limiter;        /* limit flow variables (source/Limit_FlowVars.cpp:8-119) */
-    int global_dt;      /* 0 local time step */
+    int global_dt;      /* 0 local time step, 1 global: the minimum over each block (source/TimeStep.cpp:64-86) */
     int device;         /* HIP device ordinal */
     int precision;      /* enum fv3d_precision, for both grid levels */
     /* two-grid cycle and start field (source/UserInput.cpp:50-58, 72-92) */

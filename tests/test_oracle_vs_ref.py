@@ -162,6 +162,12 @@ def mg_variant(name):
             c = cases.turn_block(c, 1, 1)
         c.cfg.coarse_iter, c.cfg.initial, c.cfg.outlet_angle = 3, "I", (0.0, 0.2, 1.0)
         c.cfg.flux_scheme = 200
+    elif name == "global_dt_axial_passage":
+        # global time step (per-block minimum) and the radial-equilibrium outlet
+        # on a three-block passage
+        c = cases.annular_passage(ni=8, nj=6, nk_blocks=(4, 8, 4), seed=36, rpm=4000.0, max_iter=8)
+        c.cfg.timestep, c.cfg.turbo_type = "G", "A"
+        c.cfg.coarse_iter, c.cfg.initial = 2, "U"
     elif name == "not_coarsenable":
         # 7 cells along i: the patch ranges end on an even node, one level only
         c = cases.bump_channel(ni=7, nj=8, nk=16, seed=35, max_iter=8)
@@ -178,7 +184,7 @@ MG_COARSE = ["cv", "pv", "res", "dq", "dt", "lamcI", "lamcJ", "lamcK"]
 
 
 @pytest.mark.parametrize("name", ["bump_uniform", "bump_interpolate_sw_limiter", "bl_bump", "bl_passage",
-                                  "bl_passage_turn1", "not_coarsenable"])
+                                  "bl_passage_turn1", "global_dt_axial_passage", "not_coarsenable"])
 def test_two_grid_matches_reference(name):
     case = mg_variant(name)
     n = case.cfg.max_iter
