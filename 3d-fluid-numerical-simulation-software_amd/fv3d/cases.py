@@ -146,6 +146,19 @@ def with_wall_shear(case: Case, delta: float = 0.3) -> Case:
     return case
 
 
+def with_sa(case: Case, amp: float = 0.5) -> Case:
+    """Spalart-Allmaras on a case: turbulence model 200 and a restart nu~ of
+    ev_ratio * (1 + amp * sin(2 pi i/ni) cos(2 pi k/nk)) per block, so the
+    first iteration sees non-uniform gradients of nu~."""
+    case.cfg.turbulence = 200
+    for st in case.states:
+        ni, nj, nk = st["rho"].shape
+        a = np.sin(2.0 * np.pi * (np.arange(ni) + 0.5) / ni)[:, None, None]
+        b = np.cos(2.0 * np.pi * (np.arange(nk) + 0.5) / nk)[None, None, :]
+        st["nut"] = cfg_float(case.cfg.ev_ratio) * (1.0 + amp * a * b) * np.ones((ni, nj, nk))
+    return case
+
+
 def from_meta(meta: Dict) -> Case:
     """The case a fixture's metadata describes: ``kwargs`` for bump_channel (or
     for annular_passage when ``generator`` names it, with ``turn`` a list of
@@ -161,7 +174,9 @@ def from_meta(meta: Dict) -> Case:
         case = bump_channel(max_iter=meta["iterations"], **meta["kwargs"])
     if meta.get("wall_shear"):
         case = with_wall_shear(case)
-    if meta.get("turbulence"):
+    if meta.get("turbulence") == 200:
+        case = with_sa(case)
+    elif meta.get("turbulence"):
         case.cfg.turbulence = meta["turbulence"]
     for k, v in meta.get("cfg", {}).items():
         if not hasattr(case.cfg, k):

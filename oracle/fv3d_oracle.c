@@ -63,6 +63,13 @@ typedef struct {
     /* coarse level of the two-grid cycle (source/Mesh.cpp:109-152): restricted
      * state cv_2h, restricted residual Res_2h and the correction delt_cv */
     cv_t *cv2h, *res2h, *delt;
+    /* Spalart-Allmaras: nu~ over the cell box, its old value, increment and
+     * residual, the wall distance, the face gradients of nu~ (x, y, z per face),
+     * the face fluxes of SA_Model (convective, viscous, and the plain gradient
+     * flux used by the c_b2 term) and the cell sums of ConvectiveFlux/SA_Model */
+    double *ev, *ev_old, *dev, *res_ev, *dwall;
+    double *gev[3], *fc_tur[3], *fv_tur[3], *fv1_tur[3];
+    double *cfc_tur, *cfv_tur;
 } blk_t;
 
 struct orc_ctx {
@@ -84,6 +91,7 @@ struct orc_ctx {
 
 static void curl(blk_t *b);
 static void gradients(blk_t *b);
+static void wall_distance(orc_ctx *c);
 
 /* ------------------------------------------------------------------ index */
 static inline long C3(const blk_t *b, int i, int j, int k) { return ((long)i * (b->JM + 1) + j) * (b->KM + 1) + k; }
@@ -198,6 +206,8 @@ static void blk_free(blk_t *b)
     for (int d = 0; d < 6; ++d) { free(b->dcv[d]); free(b->dpv[d]); }
     free(b->patch);
     free(b->cv2h); free(b->res2h); free(b->delt);
+    free(b->ev); free(b->ev_old); free(b->dev); free(b->res_ev); free(b->dwall); free(b->cfc_tur); free(b->cfv_tur);
+    for (int d = 0; d < 3; ++d) { free(b->gev[d]); free(b->fc_tur[d]); free(b->fv_tur[d]); free(b->fv1_tur[d]); }
 }
 
 void orc_destroy(orc_ctx *c)
@@ -251,6 +261,14 @@ int orc_block_add(orc_ctx *c, int IM, int JM, int KM, const double *nodes, int n
     for (int d = 0; d < 6; ++d) {
         b->dcv[d] = (cv_t *)calloc(dsize(b, d), sizeof(cv_t));
         b->dpv[d] = (pv_t *)calloc(dsize(b, d), sizeof(pv_t));
+    }
+    double **sa[] = {&b->ev, &b->ev_old, &b->dev, &b->res_ev, &b->dwall, &b->cfc_tur, &b->cfv_tur};
+    for (size_t q = 0; q < sizeof sa / sizeof sa[0]; ++q) *sa[q] = (double *)calloc(box, sizeof(double));
+    for (int d = 0; d < 3; ++d) {
+        b->gev[d] = (double *)calloc(box * 3, sizeof(double));
+        b->fc_tur[d] = (double *)calloc(box, sizeof(double));
+        b->fv_tur[d] = (double *)calloc(box, sizeof(double));
+        b->fv1_tur[d] = (double *)calloc(box, sizeof(double));
     }
     return c->nb - 1;
 }
@@ -531,6 +549,7 @@ int orc_geometry(orc_ctx *c)
         c->total_vol = c->total_vol + c->blk[n].cellvol;   /* source/ComputeAllBlockVars.cpp:25-36 */
     }
     for (int n = 0; n < c->nb; ++n) jacobi_convert_coeff(&c->blk[n]);
+    if (c->cfg.turbulence == 200) wall_distance(c);
     c->ng = 1;
     if (c->cfg.multigrid) {
         int ok = 1;
@@ -577,6 +596,7 @@ static void initial_uniform(orc_ctx *c, blk_t *b)
                 double ener = press / (dens * (Gamma - F32(1.0))) + V2 / F32(2.0);
                 cv_t q = {dens, dens * u, dens * v, dens * w, dens * ener};
                 b->cv[x] = q;
+                if (c->cfg.turbulence == 200) b->ev[x] = c->nd.ev_ratio * F32(1.0);
             }
 }
 
@@ -629,18 +649,40 @@ static void initial_interpolate(orc_ctx *c)
                     double ener = press / (Gamma - F32(1.0)) + dens * V2 / F32(2.0);
                     cv_t q = {dens, dens * u, dens * v, dens * w, ener};
                     b->cv[x] = q;
+                    if (c->cfg.turbulence == 200) b->ev[x] = c->nd.ev_ratio * F32(1.0);
                 }
     }
+}
+
+static void initial_level(orc_ctx *m, int type)
+{
+    if (type == 0)
+        for (int n = 0; n < m->nb; ++n) initial_uniform(m, &m->blk[n]);
+    else
+        initial_interpolate(m);
 }
 
 int orc_initial(orc_ctx *c)
 {
     orc_ctx *m = c->ng == 2 ? c->coarse : c;
     if (m->first_in[0] < 0 || m->first_out[0] < 0) return -1;
-    if (c->cfg.initial_type == 0)
-        for (int n = 0; n < m->nb; ++n) initial_uniform(m, &m->blk[n]);
-    else
-        initial_interpolate(m);
+    initial_level(m, c->cfg.initial_type);
+    /* with Spalart-Allmaras the fine level starts from its own field too
+     * (source/Structured_FV3D.cpp:89-95); its state is replaced at Coarse_iter,
+     * its nu~ is not */
+    if (m != c && c->cfg.turbulence == 200) initial_level(c, c->cfg.initial_type);
+    return 0;
+}
+
+/* nu~ of one block from the restart file (source/UserInput.cpp:1134-1153) */
+int orc_set_nut(orc_ctx *c, int block, const double *ev)
+{
+    if (block < 1 || block > c->nb) return -1;
+    blk_t *b = &c->blk[block - 1];
+    int ni = b->IM - 1, nj = b->JM - 1, nk = b->KM - 1;
+    for (int i = 1; i <= ni; ++i)
+        for (int j = 1; j <= nj; ++j)
+            for (int k = 1; k <= nk; ++k) b->ev[C3(b, i, j, k)] = ev[((long)(i - 1) * nj + (j - 1)) * nk + (k - 1)];
     return 0;
 }
 
@@ -736,6 +778,7 @@ static void bc_inlet(orc_ctx *c, blk_t *b, const patch_t *P)
             pg->p = PressBC; pg->t = TempBC; pg->d = DensBC;
             cv_t *cg = &b->cv[g];
             cg->r = DensBC; cg->ru = DensBC * uvel; cg->rv = DensBC * vvel; cg->rw = DensBC * wvel; cg->re = DensBC * ener;
+            if (c->cfg.turbulence == 200) b->ev[g] = c->nd.ev_ratio * F32(1.0);   /* source/BndCondInlet.cpp:104-105 */
             set_dummies(b, P->ks == 1 ? 4 : 5, i, j, *cg, *pg);
         }
 }
@@ -774,6 +817,8 @@ static void bc_outlet_general(orc_ctx *c, blk_t *b, const patch_t *P, int centri
             double ener = pg->p / pg->d / (Gamma - F32(1.0)) + V2_d / F32(2.0);
             cv_t *cg = &b->cv[C3(b, i, j, kv)];
             cg->r = pg->d; cg->ru = pg->d * pg->u; cg->rv = pg->d * pg->v; cg->rw = pg->d * pg->w; cg->re = pg->d * ener;
+            /* source/BndCondOutlet.cpp:180-181, 282-283 */
+            if (c->cfg.turbulence == 200) b->ev[C3(b, i, j, kv)] = b->ev[C3(b, i, j, k)];
             set_dummies(b, P->ks == 1 ? 4 : 5, i, j, *cg, *pg);
         }
 }
@@ -825,12 +870,14 @@ static void bc_outlet_axial(orc_ctx *c, blk_t *b, const patch_t *P)
             pg->t = pg->p / Rgas / cg->r;
             double V2 = pg->u * pg->u + pg->v * pg->v + pg->w * pg->w;
             cg->re = pg->p / (Gamma - F32(1.0)) + pg->d * V2 / F32(2.0);
+            /* source/BndCondOutlet.cpp:102-103 */
+            if (c->cfg.turbulence == 200) b->ev[C3(b, i, j, KM)] = b->ev[C3(b, i, j, KM - 1)];
             set_dummies(b, 5, i, j, *cg, *pg);
         }
 }
 
 /* source/BndCondWall.cpp:20-284 */
-static void bc_wall(blk_t *b, const patch_t *P)
+static void bc_wall(blk_t *b, const patch_t *P, int sa)
 {
     int IM = b->IM, JM = b->JM, KM = b->KM;
     double rot = P->rot;
@@ -866,6 +913,7 @@ static void bc_wall(blk_t *b, const patch_t *P)
             cg->rv = pg->v * cg->r;
             cg->rw = pg->w * cg->r;
             cg->re = ci->re;
+            if (sa) b->ev[C3(b, iv, jv, kv)] = F32(0.0);   /* source/BndCondWall.cpp:104-107 */
             int d = 2 * dir + (lo ? 0 : 1);
             pv_t pd = kmulp(0.5, padd(*pg, *pi));
             cv_t cd = cscale(cadd(*cg, *ci), 0.5);
@@ -953,6 +1001,7 @@ static void shadow_copy(orc_ctx *c, const conn_t *t, int s, int nd, double dr)
                 long xg = shadow_cell(B, fb, 0, t->rng[1 - s], l2, m2);
                 B->cv[xg] = q;
                 B->pv[xg] = w;
+                if (c->cfg.turbulence == 200) B->ev[xg] = A->ev[xs];   /* source/ShadowOutIn.cpp:339-345 */
             }
         }
 }
@@ -988,7 +1037,7 @@ int orc_bc(orc_ctx *c)
                     else bc_outlet_general(c, b, P, c->cfg.turbo_type == 1);
                 } else {
                     P->rot = P->rotate ? b->w_rot : 0.0;
-                    bc_wall(b, P);
+                    bc_wall(b, P, c->cfg.turbulence == 200);
                 }
             }
         }
@@ -1041,10 +1090,13 @@ static double sutherland(const orc_nondim *nd, double temp)
            (temp + (double)(110.4f / 288.15f));
 }
 
-static inline void uvwt(const blk_t *b, int i, int j, int k, double o[4])
+/* u v w T and nu~ of a cell (nu~ is zero without Spalart-Allmaras) */
+#define NGV 5
+static inline void uvwt(const blk_t *b, int i, int j, int k, double o[NGV])
 {
-    const pv_t *w = &b->pv[C3(b, i, j, k)];
-    o[0] = w->u; o[1] = w->v; o[2] = w->w; o[3] = w->t;
+    const long x = C3(b, i, j, k);
+    const pv_t *w = &b->pv[x];
+    o[0] = w->u; o[1] = w->v; o[2] = w->w; o[3] = w->t; o[4] = b->ev[x];
 }
 
 /* Difference of u v w T along tangential direction t at the face of
@@ -1055,7 +1107,7 @@ static inline void uvwt(const blk_t *b, int i, int j, int k, double o[4])
  * (a ghost-box corner, never filled) is left out and the remaining three are
  * averaged.  At the J face j == JM, k == 1 the reference scales M's
  * three-cell sum by 0.25 as well (line 331); that is kept. */
-static void tangential_diff(const blk_t *b, int D, int t, const int x[3], double out[4])
+static void tangential_diff(const blk_t *b, int D, int t, const int x[3], double out[NGV])
 {
     const int N[3] = {b->IM, b->JM, b->KM};
     const int f = D < t ? D : t, s = D < t ? t : D;
@@ -1067,24 +1119,24 @@ static void tangential_diff(const blk_t *b, int D, int t, const int x[3], double
     off[1][1][f] = -1;
     off[1][2][f] = -1; off[1][2][s] = -1;
     off[1][3][s] = -1;
-    double avg[2][4];
+    double avg[2][NGV];
     for (int h = 0; h < 2; ++h) {
-        double sum[4] = {0, 0, 0, 0};
+        double sum[NGV] = {0, 0, 0, 0, 0};
         int n = 0;
         for (int m = 0; m < 4; ++m) {
             int y[3] = {x[0] + off[h][m][0], x[1] + off[h][m][1], x[2] + off[h][m][2]};
             int outD = y[D] < 1 || y[D] > N[D] - 1, outT = y[t] < 1 || y[t] > N[t] - 1;
             if (outD && outT) continue;
-            double v[4];
+            double v[NGV];
             uvwt(b, y[0], y[1], y[2], v);
-            for (int q = 0; q < 4; ++q) sum[q] = n == 0 ? v[q] : sum[q] + v[q];
+            for (int q = 0; q < NGV; ++q) sum[q] = n == 0 ? v[q] : sum[q] + v[q];
             ++n;
         }
         int quirk = h == 1 && D == 1 && t == 2 && x[1] == b->JM && x[2] == 1;
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < NGV; ++q)
             avg[h][q] = n == 4 ? sum[q] / F32(4.0) : quirk ? F32(0.25) * sum[q] / F32(3.0) : sum[q] / F32(3.0);
     }
-    for (int q = 0; q < 4; ++q) out[q] = avg[0][q] - avg[1][q];
+    for (int q = 0; q < NGV; ++q) out[q] = avg[0][q] - avg[1][q];
 }
 
 /* CFluidProps::Gradient (source/Spectral_curl_grad.cpp:85-493): gradients of
@@ -1098,13 +1150,13 @@ static void gradients(blk_t *b)
             for (int j = 1; j <= JM - (D != 1); ++j)
                 for (int k = 1; k <= KM - (D != 2); ++k) {
                     int x[3] = {i, j, k};
-                    double dv[3][4];
+                    double dv[3][NGV];
                     for (int t = 0; t < 3; ++t) {
                         if (t == D) {
-                            double a[4], m[4];
+                            double a[NGV], m[NGV];
                             uvwt(b, i, j, k, a);
                             uvwt(b, i - (D == 0), j - (D == 1), k - (D == 2), m);
-                            for (int q = 0; q < 4; ++q) dv[t][q] = a[q] - m[q];
+                            for (int q = 0; q < NGV; ++q) dv[t][q] = a[q] - m[q];
                         } else {
                             tangential_diff(b, D, t, x, dv[t]);
                         }
@@ -1115,6 +1167,9 @@ static void gradients(blk_t *b)
                     for (int c = 0; c < 3; ++c)
                         for (int q = 0; q < 4; ++q)
                             o[c * 4 + q] = dv[0][q] * g[c] + dv[1][q] * g[3 + c] + dv[2][q] * g[6 + c];
+                    /* nu~ (source/Spectral_curl_grad.cpp:221-223, 352-354, 483-485) */
+                    for (int c = 0; c < 3; ++c)
+                        b->gev[D][xx * 3 + c] = dv[0][4] * g[c] + dv[1][4] * g[3 + c] + dv[2][4] * g[6 + c];
                 }
     }
 }
@@ -1293,6 +1348,156 @@ static int bl_model(orc_ctx *c, blk_t *b)
 
 /* ------------------------------------------------------------ time step */
 /* source/TimeStep.cpp:7-92 */
+/* ------------------------------------------------------ Spalart-Allmaras */
+/* fv1 of the model for nu~ and the laminar kinematic viscosity ev_lam */
+static inline double sa_fv1(double X, double Cv1)
+{
+    return pow(X, F32(3.0)) / (pow(X, F32(3.0)) + Cv1 * Cv1 * Cv1);
+}
+
+/* CSpaceDiscr::SA_Model (source/SA_Model.cpp:9-258): mue_t = fv1 rho nu~ / Re
+ * with the clipping of lines 48-75, the face fluxes of nu~ (first-order upwind
+ * convection, diffusion (1 + c_b2)/sigma (nu + nu~) grad nu~, and the plain
+ * gradient flux of the c_b2 term), the cell source and the viscous sum, and
+ * mue_t of the ghost cells.  The reference reads its face gradients through
+ * aliased pointers (lines 21-32): on I faces all three components are the x
+ * derivative, on K faces the y and z derivatives are swapped; that is kept. */
+static void sa_model(orc_ctx *c, blk_t *b)
+{
+    const double Re = c->nd.re_ref;
+    const double Cb1 = F32(0.1355), Cb2 = F32(0.622), Cv1 = F32(7.1), sigma = (double)(2.0f / 3.0f), kavf = F32(0.41);
+    const double Cw1 = Cb1 / kavf / kavf + (F32(1.0) + Cb2) / sigma;
+    const double Cw2 = F32(0.3), Cw3 = F32(2.0), Ct3 = F32(1.2), Ct4 = F32(0.5);
+    int IM = b->IM, JM = b->JM, KM = b->KM;
+    for (int i = 1; i <= IM - 1; ++i)
+        for (int j = 1; j <= JM - 1; ++j)
+            for (int k = 1; k <= KM - 1; ++k) {
+                long x = C3(b, i, j, k);
+                const pv_t *w = &b->pv[x];
+                double ev_lam = Re * sutherland(&c->nd, w->t) / w->d;
+                double X = b->ev[x] / ev_lam;
+                b->mut[x] = sa_fv1(X, Cv1) * w->d * b->ev[x] / Re;
+            }
+    const double Max = 200.0;
+    for (int i = 1; i <= IM - 1; ++i)
+        for (int j = 1; j <= JM - 1; ++j)
+            for (int k = 1; k <= KM - 1; ++k) {
+                long x = C3(b, i, j, k);
+                const pv_t *w = &b->pv[x];
+                if (b->mut[x] < 0.0) b->mut[x] = 0.0;
+                if (b->ev[x] < 0.0) b->ev[x] = 0.0;
+                double mue_lam = sutherland(&c->nd, w->t);
+                double ev_lam = Re * mue_lam / w->d;
+                if (mue_lam > 0 && b->mut[x] > (Max * mue_lam)) b->mut[x] = Max * mue_lam;
+                if (ev_lam > 0 && b->ev[x] > (Max * ev_lam)) b->ev[x] = Max * ev_lam;
+            }
+    /* faces of the three directions (lines 82-181) */
+    for (int D = 0; D < 3; ++D) {
+        const face_t *Fa = D == 0 ? b->fi : D == 1 ? b->fj : b->fk;
+        for (int i = 1; i <= IM - (D != 0); ++i)
+            for (int j = 1; j <= JM - (D != 1); ++j)
+                for (int k = 1; k <= KM - (D != 2); ++k) {
+                    long x = C3(b, i, j, k), xm = C3(b, i - (D == 0), j - (D == 1), k - (D == 2));
+                    const pv_t *wp = &b->pv[x], *wm = &b->pv[xm];
+                    const face_t *f = &Fa[x];
+                    double densf = F32(0.5) * (wp->d + wm->d);
+                    double tempf = F32(0.5) * (wp->t + wm->t);
+                    double evf = F32(0.5) * (b->ev[x] + b->ev[xm]);
+                    double ev_lam = Re * sutherland(&c->nd, tempf) / densf;
+                    const double *g = b->gev[D] + x * 3;
+                    double gx = g[0], gy = D == 0 ? g[0] : D == 1 ? g[1] : g[2], gz = D == 0 ? g[0] : D == 1 ? g[2] : g[1];
+                    double tx = (F32(1.0) + Cb2) / sigma * (ev_lam + evf) * gx;
+                    double ty = (F32(1.0) + Cb2) / sigma * (ev_lam + evf) * gy;
+                    double tz = (F32(1.0) + Cb2) / sigma * (ev_lam + evf) * gz;
+                    v3_t n = {f->S.x / f->s, f->S.y / f->s, f->S.z / f->s};
+                    double Vn1 = wm->u * n.x + wm->v * n.y + wm->w * n.z;
+                    double Vn2 = wp->u * n.x + wp->v * n.y + wp->w * n.z;
+                    b->fc_tur[D][x] = (F32(0.5) * (Vn1 + fabs(Vn1)) * b->ev[xm] + F32(0.5) * (Vn2 - fabs(Vn2)) * b->ev[x]) * f->s;
+                    b->fv_tur[D][x] = (tx * f->S.x + ty * f->S.y + tz * f->S.z) / Re;
+                    b->fv1_tur[D][x] = gx * f->S.x + gy * f->S.y + gz * f->S.z;
+                }
+    }
+    /* source and viscous sum (lines 184-213) */
+    for (int i = 1; i <= IM - 1; ++i)
+        for (int j = 1; j <= JM - 1; ++j)
+            for (int k = 1; k <= KM - 1; ++k) {
+                long x = C3(b, i, j, k);
+                long xi = C3(b, i + 1, j, k), xj = C3(b, i, j + 1, k), xk = C3(b, i, j, k + 1);
+                const pv_t *w = &b->pv[x];
+                double ev = b->ev[x], dis = b->dwall[x], Om = b->omega[x];
+                double ev_lam = Re * sutherland(&c->nd, w->t) / w->d;
+                double X = ev / ev_lam;
+                double fv1 = sa_fv1(X, Cv1);
+                double fv2 = F32(1.0) - X / (F32(1.0) + X * fv1);
+                double ft2 = Ct3 * exp(F32(0.0) - Ct4 * X * X);
+                double kd = kavf * dis;
+                double sa = Om + ev * fv2 / ((kd * kd) * Re), sb = F32(0.3) * Om;
+                double Sbar = sa > sb ? sa : sb;
+                double ra = ev / (Sbar * kavf * kavf * dis * dis * Re), rb = F32(10.0);
+                double r = ra < rb ? ra : rb;
+                double gg = r + Cw2 * (pow(r, F32(6.0)) - r);
+                double fw = gg * pow((1 + pow(Cw3, F32(6.0))) / (pow(gg, F32(6.0)) + pow(Cw3, F32(6.0))), (double)(1.0f / 6.0f));
+                double ed = ev / dis;
+                double Qc = (Cb1 * (F32(1.0) - ft2) * Sbar * ev -
+                             (Cw1 * fw - Cb1 * ((1 - ft2) * fv2 + ft2) / (kavf * kavf)) * (ed * ed) / Re) * b->cell[x].vol;
+                double sv = Qc + b->fv_tur[0][xi] - b->fv_tur[0][x] + b->fv_tur[1][xj] - b->fv_tur[1][x] + b->fv_tur[2][xk] -
+                            b->fv_tur[2][x];
+                double s1 = b->fv1_tur[0][xi] - b->fv1_tur[0][x] + b->fv1_tur[1][xj] - b->fv1_tur[1][x] + b->fv1_tur[2][xk] -
+                            b->fv1_tur[2][x];
+                b->cfv_tur[x] = sv - Cb2 * (ev_lam + ev) / sigma / Re * s1;
+            }
+    /* mue_t of the ghost cells (lines 215-257) */
+    for (int side = 0; side < 6; ++side) {
+        int d = side / 2, hi = side % 2;
+        int N[3] = {IM, JM, KM};
+        int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+        for (int a = 1; a <= N[ta] - 1; ++a)
+            for (int bb = 1; bb <= N[tb] - 1; ++bb) {
+                int y[3];
+                y[d] = hi ? N[d] : 0; y[ta] = a; y[tb] = bb;
+                long x = C3(b, y[0], y[1], y[2]);
+                const pv_t *w = &b->pv[x];
+                double ev_lam = Re * sutherland(&c->nd, w->t) / w->d;
+                double X = b->ev[x] / ev_lam;
+                b->mut[x] = sa_fv1(X, Cv1) * w->d * b->ev[x] / Re;
+            }
+    }
+}
+
+/* CMesh::WallDistanceSA (source/WallDistanceSA.cpp:9-132): distance from each
+ * cell centroid to the nearest wall-face centroid over all wall patches */
+static void wall_distance(orc_ctx *c)
+{
+    for (int n = 0; n < c->nb; ++n) {
+        blk_t *b = &c->blk[n];
+        for (int i = 1; i <= b->IM - 1; ++i)
+            for (int j = 1; j <= b->JM - 1; ++j)
+                for (int k = 1; k <= b->KM - 1; ++k) {
+                    long x = C3(b, i, j, k);
+                    v3_t cc = b->cell[x].c;
+                    double dmin = 10000.0;
+                    for (int m = 0; m < c->nb; ++m) {
+                        const blk_t *W = &c->blk[m];
+                        for (int q = 0; q < W->npatch; ++q) {
+                            const patch_t *P = &W->patch[q];
+                            if (P->kind != 2) continue;
+                            int dir = (P->face - 1) / 2;
+                            const face_t *F = dir == 0 ? W->fi : dir == 1 ? W->fj : W->fk;
+                            for (int ib = P->is; ib <= P->ie; ++ib)
+                                for (int jb = P->js; jb <= P->je; ++jb)
+                                    for (int kb = P->ks; kb <= P->ke; ++kb) {
+                                        v3_t fcn = F[C3(W, ib, jb, kb)].c;
+                                        double dx = cc.x - fcn.x, dy = cc.y - fcn.y, dz = cc.z - fcn.z;
+                                        double dt = pow(dx * dx + dy * dy + dz * dz, 0.5);
+                                        if (dmin >= dt) dmin = dt;
+                                    }
+                        }
+                    }
+                    b->dwall[x] = dmin;
+                }
+    }
+}
+
 int orc_timestep(orc_ctx *c)
 {
     double Gamma = c->nd.gamma, Pr_lam = c->nd.pr_lam, Pr_tur = c->nd.pr_tur;
@@ -1631,6 +1836,7 @@ static void wall_flux(orc_ctx *c, blk_t *b, const patch_t *P)
                 o->rv = fc.rv * (Vn_a - Vn_e) + fp.p * S.y;
                 o->rw = fc.rw * (Vn_a - Vn_e) + fp.p * S.z;
                 o->re = (fc.re + fp.p) * (Vn_a - Vn_e) + fp.p * Vn_e;
+                if (c->cfg.turbulence == 200) b->fc_tur[dir][xa] = 0;   /* no nu~ convection through a wall */
                 if (P->face % 2 == 1) b->mut[xb] = -b->mut[xa];
                 else b->mut[xa] = -b->mut[xb];
             }
@@ -1673,6 +1879,9 @@ int orc_convective(orc_ctx *c)
                     s = cadd(s, b->face_fc[2][C3(b, i, j, k + 1)]);
                     s = csub(s, b->face_fc[2][x]);
                     b->fc[x] = csub(s, Qc);
+                    if (c->cfg.turbulence == 200)   /* source/ComputeAllBlockVars.cpp:109-114 */
+                        b->cfc_tur[x] = b->fc_tur[0][C3(b, i + 1, j, k)] - b->fc_tur[0][x] + b->fc_tur[1][C3(b, i, j + 1, k)] -
+                                        b->fc_tur[1][x] + b->fc_tur[2][C3(b, i, j, k + 1)] - b->fc_tur[2][x];
                 }
     }
     return 0;
@@ -1688,6 +1897,8 @@ int orc_resflux(orc_ctx *c)
     if (c->cfg.viscous && c->cfg.turbulence == 100)
         for (int n = 0; n < c->nb; ++n)
             if (bl_model(c, &c->blk[n])) return -1;
+    if (c->cfg.viscous && c->cfg.turbulence == 200)
+        for (int n = 0; n < c->nb; ++n) sa_model(c, &c->blk[n]);
     orc_timestep(c);
     orc_convective(c);
     if (c->cfg.viscous)
@@ -1700,6 +1911,7 @@ int orc_resflux(orc_ctx *c)
                 for (int k = 1; k <= b->KM - 1; ++k) {
                     long x = C3(b, i, j, k);
                     b->res[x] = csub(csub(b->fc[x], c->cfg.viscous ? b->fv[x] : zero), zero);
+                    if (c->cfg.turbulence == 200) b->res_ev[x] = b->cfc_tur[x] - b->cfv_tur[x];   /* line 249 */
                 }
     }
     return 0;
@@ -1708,7 +1920,7 @@ int orc_resflux(orc_ctx *c)
 /* ------------------------------------------------------------- LU-SGS */
 /* source/LU_SGS_Marching.cpp:293-419; nb = neighbour cell, fa/fb its two faces
  * (higher index first), as delt_Fn_up/down read them. */
-static cv_t delt_fn(orc_ctx *c, blk_t *b, long nb, const face_t *fa, const face_t *fb)
+static cv_t delt_fn(orc_ctx *c, blk_t *b, long nb, const face_t *fa, const face_t *fb, double *dfe)
 {
     double Gamma = c->nd.gamma;
     v3_t norm;
@@ -1731,12 +1943,18 @@ static cv_t delt_fn(orc_ctx *c, blk_t *b, long nb, const face_t *fa, const face_
     d.rv = (nw.rv * (Vn_new - Vn_e) + press_new * norm.y) - (old.rv * (Vn_old - Vn_e) + press_old * norm.y);
     d.rw = (nw.rw * (Vn_new - Vn_e) + press_new * norm.z) - (old.rw * (Vn_old - Vn_e) + press_old * norm.z);
     d.re = ((nw.re + press_new) * (Vn_new - Vn_e) + press_new * Vn_e) - ((old.re + press_old) * (Vn_old - Vn_e) + press_old * Vn_e);
+    /* the nu~ flux difference, without the grid velocity (lines 352-357) */
+    if (dfe) {
+        double ev_old = b->ev[nb], ev_new = ev_old + b->dev[nb];
+        *dfe = ev_new * Vn_new - ev_old * Vn_old;
+    }
     return d;
 }
 
 static void lusgs_sweeps(orc_ctx *c, blk_t *b)
 {
     int IM = b->IM, JM = b->JM, KM = b->KM;
+    const int sa = c->cfg.turbulence == 200;
     double relax = (float)c->cfg.relax;
     const cv_t zero = {0, 0, 0, 0, 0};
     /* source/LU_SGS_Marching.cpp:61-160 */
@@ -1748,21 +1966,31 @@ static void lusgs_sweeps(orc_ctx *c, blk_t *b)
                 long x = C3(b, i, j, k);
                 double alpha = b->cell[x].vol / b->dt[x] + relax * (b->lamc[0][x] + b->lamc[1][x] + b->lamc[2][x]) +
                                F32(2.0) * (b->lamv[0][x] + b->lamv[1][x] + b->lamv[2][x]);
+                double alpha2 = b->cell[x].vol / b->dt[x] + relax * (b->lamc[0][x] + b->lamc[1][x] + b->lamc[2][x]) +
+                                F32(3.0) * (b->lamv[0][x] + b->lamv[1][x] + b->lamv[2][x]);
                 cv_t A[3], V[3];
+                double Ae[3], Ve[3];
                 for (int d = 0; d < 3; ++d) {
                     int ok = d == 0 ? i != 1 : d == 1 ? j != 1 : k != 1;
-                    if (!ok) { A[d] = zero; V[d] = zero; continue; }
+                    if (!ok) { A[d] = zero; V[d] = zero; Ae[d] = 0.0; Ve[d] = 0.0; continue; }
                     long nb = d == 0 ? C3(b, i - 1, j, k) : d == 1 ? C3(b, i, j - 1, k) : C3(b, i, j, k - 1);
                     const face_t *fa = d == 0 ? &b->fi[x] : d == 1 ? &b->fj[x] : &b->fk[x];
                     const face_t *fb = d == 0 ? &b->fi[nb] : d == 1 ? &b->fj[nb] : &b->fk[nb];
-                    cv_t dF = delt_fn(c, b, nb, fa, fb);
+                    double dfe = 0.0;
+                    cv_t dF = delt_fn(c, b, nb, fa, fb, sa ? &dfe : 0);
                     double s = F32(0.5) * (fb->s + fa->s);
                     A[d] = cscale(cadd(cscale(dF, s), cscale(b->dq[nb], relax * b->lamc[d][nb])), F32(0.5));
                     V[d] = cscale(b->dq[nb], b->lamv[d][nb]);
+                    if (sa) {
+                        Ae[d] = F32(0.5) * (dfe * s + relax * b->lamc[d][nb] * b->dev[nb]);
+                        Ve[d] = b->lamv[d][nb] * b->dev[nb];
+                    }
                 }
                 cv_t s = cadd(cscale(b->res[x], F32(-1.0)), A[0]);
                 s = cadd(s, A[1]); s = cadd(s, A[2]); s = cadd(s, V[0]); s = cadd(s, V[1]); s = cadd(s, V[2]);
                 b->dq[x] = cdiv(s, alpha);
+                if (sa)   /* source/LU_SGS_Marching.cpp:156-157 */
+                    b->dev[x] = (F32(-1.0) * b->res_ev[x] + Ae[0] + Ae[1] + Ae[2] + Ve[0] + Ve[1] + Ve[2]) / alpha2;
             }
     /* source/LU_SGS_Marching.cpp:162-260 */
     for (int plane = IM + JM + KM - 3; plane >= 3; --plane)
@@ -1773,22 +2001,32 @@ static void lusgs_sweeps(orc_ctx *c, blk_t *b)
                 long x = C3(b, i, j, k);
                 double alpha = b->cell[x].vol / b->dt[x] + relax * (b->lamc[0][x] + b->lamc[1][x] + b->lamc[2][x]) +
                                F32(2.0) * (b->lamv[0][x] + b->lamv[1][x] + b->lamv[2][x]);
+                double alpha2 = b->cell[x].vol / b->dt[x] + relax * (b->lamc[0][x] + b->lamc[1][x] + b->lamc[2][x]) +
+                                F32(3.0) * (b->lamv[0][x] + b->lamv[1][x] + b->lamv[2][x]);
                 cv_t A[3], V[3];
+                double Ae[3], Ve[3];
                 for (int d = 0; d < 3; ++d) {
                     int ok = d == 0 ? i != IM - 1 : d == 1 ? j != JM - 1 : k != KM - 1;
-                    if (!ok) { A[d] = zero; V[d] = zero; continue; }
+                    if (!ok) { A[d] = zero; V[d] = zero; Ae[d] = 0.0; Ve[d] = 0.0; continue; }
                     long nb = d == 0 ? C3(b, i + 1, j, k) : d == 1 ? C3(b, i, j + 1, k) : C3(b, i, j, k + 1);
                     long n2 = d == 0 ? C3(b, i + 2, j, k) : d == 1 ? C3(b, i, j + 2, k) : C3(b, i, j, k + 2);
                     const face_t *fa = d == 0 ? &b->fi[n2] : d == 1 ? &b->fj[n2] : &b->fk[n2];
                     const face_t *fb = d == 0 ? &b->fi[nb] : d == 1 ? &b->fj[nb] : &b->fk[nb];
-                    cv_t dF = delt_fn(c, b, nb, fa, fb);
+                    double dfe = 0.0;
+                    cv_t dF = delt_fn(c, b, nb, fa, fb, sa ? &dfe : 0);
                     double s = F32(0.5) * (fb->s + fa->s);
                     A[d] = cscale(csub(cscale(dF, s), cscale(b->dq[nb], relax * b->lamc[d][nb])), F32(0.5));
                     V[d] = cscale(b->dq[nb], b->lamv[d][nb]);
+                    if (sa) {
+                        Ae[d] = F32(0.5) * (dfe * s - relax * b->lamc[d][nb] * b->dev[nb]);
+                        Ve[d] = b->lamv[d][nb] * b->dev[nb];
+                    }
                 }
                 cv_t s = csub(cscale(b->dq[x], alpha), A[0]);
                 s = csub(s, A[1]); s = csub(s, A[2]); s = cadd(s, V[0]); s = cadd(s, V[1]); s = cadd(s, V[2]);
                 b->dq[x] = cdiv(s, alpha);
+                if (sa)   /* source/LU_SGS_Marching.cpp:256-257 */
+                    b->dev[x] = (alpha2 * b->dev[x] - Ae[0] - Ae[1] - Ae[2] + Ve[0] + Ve[1] + Ve[2]) / alpha2;
             }
 }
 
@@ -1867,14 +2105,47 @@ static void limit_vars(orc_ctx *c, blk_t *b)
     if (nneg > 0) b->flag_low_dt = 10;
 }
 
-/* source/LU_SGS_Marching.cpp:10-291 (single grid, no SA) */
+/* CFluidProps::Limit_SA (source/Limit_FlowVars.cpp:122-168): nu~ below 1e-8
+ * is raised to it; above 1000 it becomes the mean of the admissible
+ * neighbours (k outermost), or 1000 when there are none */
+static void limit_sa(blk_t *b)
+{
+    const double SAmin = F32(1.0E-8), LSAmax = F32(1000.0);
+    int IM = b->IM, JM = b->JM, KM = b->KM;
+    for (int i = 1; i <= IM - 1; ++i)
+        for (int j = 1; j <= JM - 1; ++j)
+            for (int k = 1; k <= KM - 1; ++k) {
+                double *e = &b->ev[C3(b, i, j, k)];
+                if (*e < SAmin) *e = SAmin;
+                if (*e > LSAmax) {
+                    double s0 = 0.0;
+                    int kn = 0;
+                    for (int ka = -1; ka <= 1; ++ka)
+                        for (int ja = -1; ja <= 1; ++ja)
+                            for (int ia = -1; ia <= 1; ++ia) {
+                                int k1 = k + ka, j1 = j + ja, i1 = i + ia;
+                                if (i1 > 0 && i1 < IM && j1 > 0 && j1 < JM && k1 > 0 && k1 < KM) {
+                                    double v = b->ev[C3(b, i1, j1, k1)];
+                                    if (v >= SAmin && v <= LSAmax) { s0 = s0 + v; kn = kn + 1; }
+                                }
+                            }
+                    *e = kn == 0 ? LSAmax : s0 / kn;
+                }
+            }
+}
+
+/* source/LU_SGS_Marching.cpp:10-291 (with the nu~ equation of Spalart-Allmaras) */
 int orc_lusgs(orc_ctx *c)
 {
+    const int sa = c->cfg.turbulence == 200;
     for (int n = 0; n < c->nb; ++n) {
         blk_t *b = &c->blk[n];
         for (int i = 1; i <= b->IM - 1; ++i)
             for (int j = 1; j <= b->JM - 1; ++j)
-                for (int k = 1; k <= b->KM - 1; ++k) b->cv_old[C3(b, i, j, k)] = b->cv[C3(b, i, j, k)];
+                for (int k = 1; k <= b->KM - 1; ++k) {
+                    b->cv_old[C3(b, i, j, k)] = b->cv[C3(b, i, j, k)];
+                    if (sa) b->ev_old[C3(b, i, j, k)] = b->ev[C3(b, i, j, k)];
+                }
     }
     if (!c->force) {
         if (orc_resflux(c)) return -1;
@@ -1902,26 +2173,32 @@ int orc_lusgs(orc_ctx *c)
                 for (int k = 1; k <= b->KM - 1; ++k) {
                     long x = C3(b, i, j, k);
                     b->cv[x] = cadd(b->cv[x], b->dq[x]);
+                    if (sa) b->ev[x] = b->ev[x] + b->dev[x];
                 }
     }
     for (int n = 0; n < c->nb; ++n) primitive_vars(c, &c->blk[n]);
     if (c->cfg.limiter)
-        for (int n = 0; n < c->nb; ++n) limit_vars(c, &c->blk[n]);
+        for (int n = 0; n < c->nb; ++n) {
+            limit_vars(c, &c->blk[n]);
+            if (sa) limit_sa(&c->blk[n]);
+        }
     return 0;
 }
 
 /* source/ComputeAllBlockVars.cpp:318-372 */
 int orc_res_calculate(orc_ctx *c, double *res6)
 {
-    double r[5] = {0, 0, 0, 0, 0};
+    double r[5] = {0, 0, 0, 0, 0}, re = 0.0;
     for (int n = 0; n < c->nb; ++n) {
         blk_t *b = &c->blk[n];
         /* the OpenMP reduction sums each block from zero, then adds it to the total */
-        double t[5] = {0, 0, 0, 0, 0};
+        double t[5] = {0, 0, 0, 0, 0}, te = 0.0;
         for (int i = 1; i <= b->IM - 1; ++i)
             for (int j = 1; j <= b->JM - 1; ++j)
                 for (int k = 1; k <= b->KM - 1; ++k) {
                     long x = C3(b, i, j, k);
+                    if (c->cfg.turbulence == 200)
+                        te = te + fabs((b->ev_old[x] - b->ev[x]) / b->ev_old[x]) * b->cell[x].vol;
                     const cv_t *o = &b->cv_old[x], *q = &b->cv[x];
                     cv_t m;
                     m.r = fabs((o->r - q->r) / o->r);
@@ -1933,9 +2210,10 @@ int orc_res_calculate(orc_ctx *c, double *res6)
                     t[0] = t[0] + m.r; t[1] = t[1] + m.ru; t[2] = t[2] + m.rv; t[3] = t[3] + m.rw; t[4] = t[4] + m.re;
                 }
         for (int q = 0; q < 5; ++q) r[q] = r[q] + t[q];
+        re = re + te;
     }
     for (int q = 0; q < 5; ++q) c->res[q] = r[q] / c->total_vol;
-    c->res[5] = 0.0;
+    c->res[5] = c->cfg.turbulence == 200 ? re / c->total_vol : 0.0;
     if (res6) for (int q = 0; q < 6; ++q) res6[q] = c->res[q];
     return 0;
 }
@@ -2224,10 +2502,26 @@ long orc_get(orc_ctx *c, int block, const char *name, double *out)
     else if (!strcmp(name, "lamvK")) s1 = b->lamv[2];
     else if (!strcmp(name, "omega")) s1 = b->omega;
     else if (!strcmp(name, "fv")) c5 = b->fv;
-    else if (!strcmp(name, "mut")) {
+    else if (!strcmp(name, "resev")) s1 = b->res_ev;
+    else if (!strcmp(name, "cfctur")) s1 = b->cfc_tur;
+    else if (!strcmp(name, "cfvtur")) s1 = b->cfv_tur;
+    else if (!strcmp(name, "dev")) s1 = b->dev;
+    else if (!strcmp(name, "dwall")) s1 = b->dwall;
+    else if (!strcmp(name, "mut") || !strcmp(name, "ev")) {
+        const double *src = name[0] == 'm' ? b->mut : b->ev;
         for (int i = 0; i <= IM; ++i)
             for (int j = 0; j <= JM; ++j)
-                for (int k = 0; k <= KM; ++k) out[n++] = b->mut[C3(b, i, j, k)];
+                for (int k = 0; k <= KM; ++k) out[n++] = src[C3(b, i, j, k)];
+        return n;
+    }
+    else if (!strcmp(name, "gevI") || !strcmp(name, "gevJ") || !strcmp(name, "gevK")) {
+        int d = name[3] - 'I';
+        for (int i = 1; i <= IM - (d != 0); ++i)
+            for (int j = 1; j <= JM - (d != 1); ++j)
+                for (int k = 1; k <= KM - (d != 2); ++k) {
+                    memcpy(out + n, b->gev[d] + C3(b, i, j, k) * 3, 3 * sizeof(double));
+                    n += 3;
+                }
         return n;
     }
     else if (!strcmp(name, "fcI") || !strcmp(name, "fcJ") || !strcmp(name, "fcK") || !strcmp(name, "fvI") ||

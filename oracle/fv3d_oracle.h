@@ -33,7 +33,7 @@ typedef struct {
     int recon_var;       /* 100 conservative, 200 primitive */
     int recon_scheme;    /* 100 NND, 200 MUSCL2u, 300 MUSCL2c, 400 MUSCL2o, 500 MUSCL3 */
     int flux_scheme;     /* 200 Steger-Warming, 300 Roe */
-    int turbulence;      /* 0 none (100 BL, 200 SA: not in this library) */
+    int turbulence;      /* 0 none, 100 Baldwin-Lomax, 200 Spalart-Allmaras */
     int limiter;         /* limit flow variables */
     int global_dt;       /* 0 local time step */
     int nthreads;
@@ -92,6 +92,8 @@ int orc_initial(orc_ctx *c);
 /* Restart semantics of source/UserInput.cpp:1118-1159: pv is (ni,nj,nk,6)
  * rho,u,v,w,p,T and ener (ni,nj,nk), cell (i,j,k) at ((i-1)*nj+(j-1))*nk+(k-1). */
 int orc_set_state(orc_ctx *c, int block, const double *pv, const double *ener);
+/* nu~ of a block from the restart file (Spalart-Allmaras), same cell order */
+int orc_set_nut(orc_ctx *c, int block, const double *ev);
 
 int orc_bc(orc_ctx *c);
 int orc_spectral(orc_ctx *c);
@@ -116,7 +118,9 @@ int orc_iterate(orc_ctx *c, int n, double *hist);
  * _pv variants (layers 1..NDummy), vol, fcI fcJ fcK (convective face fluxes),
  * fvI fvJ fvK (viscous face fluxes), gradI gradJ gradK (face gradients: d/dx,
  * d/dy, d/dz of u v w T), metI metJ metK metC (face and cell transformation
- * coefficients, 9 each), SI SJ SK (face vectors). */
+ * coefficients, 9 each), SI SJ SK (face vectors); Spalart-Allmaras: ev (nu~,
+ * full box), resev dev dwall (interior), gevI gevJ gevK (face gradients of
+ * nu~, x y z). */
 long orc_get(orc_ctx *c, int block, const char *name, double *out);
 
 /* Test hooks: the face flux (scheme 200 Steger-Warming, 300 Roe) on face-frame

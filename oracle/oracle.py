@@ -66,6 +66,7 @@ def lib():
         L.orc_coarse.argtypes = [C.c_void_p]
         L.orc_coarse.restype = C.c_void_p
         L.orc_initial.argtypes = [C.c_void_p]
+        L.orc_set_nut.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         _lib = L
     return _lib
 
@@ -131,6 +132,9 @@ class Oracle:
                 pv = np.ascontiguousarray(np.stack([st["rho"], st["u"], st["v"], st["w"], st["p"], st["T"]], -1))
                 e = np.ascontiguousarray(st["rhoE"])
                 L.orc_set_state(self.h, b + 1, pv.ctypes.data, e.ctypes.data)
+                if case.cfg.turbulence == 200:
+                    ev = np.ascontiguousarray(st["nut"], dtype=np.float64)
+                    L.orc_set_nut(self.h, b + 1, ev.ctypes.data)
         elif L.orc_initial(self.h) != 0:
             raise ValueError("start field needs an inlet and an outlet patch")
 
@@ -190,8 +194,12 @@ class Oracle:
             shape = {"I": (im, nj, nk, 12), "J": (ni, jm, nk, 12), "K": (ni, nj, km, 12)}[name[4]]
         elif name in ("metI", "metJ", "metK", "metC"):
             shape = {"I": (im, nj, nk, 9), "J": (ni, jm, nk, 9), "K": (ni, nj, km, 9), "C": (ni, nj, nk, 9)}[name[3]]
-        elif name == "mut":
+        elif name in ("mut", "ev"):
             shape = (im + 1, jm + 1, km + 1)
+        elif name in ("resev", "dev", "dwall", "cfctur", "cfvtur"):
+            shape = (ni, nj, nk)
+        elif name in ("gevI", "gevJ", "gevK"):
+            shape = {"I": (im, nj, nk, 3), "J": (ni, jm, nk, 3), "K": (ni, nj, km, 3)}[name[3]]
         elif name in ("SI", "SJ", "SK"):
             shape = {"SI": (im, nj, nk, 3), "SJ": (ni, jm, nk, 3), "SK": (ni, nj, km, 3)}[name]
         elif name in ("dt", "lamcI", "lamcJ", "lamcK", "lamvI", "lamvJ", "lamvK", "omega", "vol"):

@@ -140,6 +140,26 @@ void dump_block(const std::string &dir, const std::string &tag, CBlocks &B)
                             [](const TEddyVars &t, double *o) { o[0] = t.ev; });
         dump3<TEddyVars, 1>(p + "_resev.npy", B.SpaceDiscr.Res_ev, 1, IM - 1, 1, JM - 1, 1, KM - 1,
                             [](const TEddyVars &t, double *o) { o[0] = t.ev; });
+        dump3<TEddyVars, 1>(p + "_dev.npy", B.TimeDiscr.delt_ev, 1, IM - 1, 1, JM - 1, 1, KM - 1,
+                            [](const TEddyVars &t, double *o) { o[0] = t.ev; });
+        dump3<TEddyVars, 1>(p + "_cfctur.npy", B.SpaceDiscr.Cell_Fc_tur, 1, IM - 1, 1, JM - 1, 1, KM - 1,
+                            [](const TEddyVars &t, double *o) { o[0] = t.ev; });
+        dump3<TEddyVars, 1>(p + "_cfvtur.npy", B.SpaceDiscr.Cell_Fv_tur, 1, IM - 1, 1, JM - 1, 1, KM - 1,
+                            [](const TEddyVars &t, double *o) { o[0] = t.ev; });
+        dump3<TCell, 1>(p + "_dwall.npy", G.Cell, 1, IM - 1, 1, JM - 1, 1, KM - 1,
+                        [](const TCell &t, double *o) { o[0] = t.dis_Wall; });
+        // face gradients of nu~ (x, y, z), as CFluidProps::Gradient stores them
+        REAL ***ex[3] = {F.gradx_I_ev, F.gradx_J_ev, F.gradx_K_ev}, ***ey[3] = {F.grady_I_ev, F.grady_J_ev, F.grady_K_ev},
+             ***ez[3] = {F.gradz_I_ev, F.gradz_J_ev, F.gradz_K_ev};
+        for (int d = 0; d < 3; ++d) {
+            std::vector<double> v;
+            for (int i = 1; i <= IM - (d != 0); ++i)
+                for (int j = 1; j <= JM - (d != 1); ++j)
+                    for (int k = 1; k <= KM - (d != 2); ++k)
+                        v.insert(v.end(), {(double)ex[d][i][j][k], (double)ey[d][i][j][k], (double)ez[d][i][j][k]});
+            write_npy(p + "_gev" + std::string(1, (char)('I' + d)) + ".npy", v,
+                      {IM - (d != 0), JM - (d != 1), KM - (d != 2), 3});
+        }
     }
 }
 

@@ -50,7 +50,13 @@ TINY = {
     "tiny_passage3_bl": (dict(ni=6, nj=6, nk_blocks=(5, 8, 4), seed=8, rpm=6000.0, blade_blocks=(1,),
                               camber=0.1, viscous=True, len_ref=1e-3), 12,
                          {"generator": "annular_passage", "turn": [(2, 1)], "turbulence": 100, "wall_shear": True}),
+    # Spalart-Allmaras (restart nu~ from fv3d.cases.with_sa) on a rotating
+    # three-block passage with a turned middle block
+    "tiny_passage3_sa": (dict(ni=6, nj=6, nk_blocks=(5, 8, 4), seed=11, rpm=6000.0, blade_blocks=(1,),
+                              camber=0.1, viscous=True, len_ref=1e-3), 10,
+                         {"generator": "annular_passage", "turn": [(2, 1)], "turbulence": 200, "wall_shear": True}),
 }
+SA_FIELDS_IT1 = ["dwall", "gevI", "gevJ", "gevK", "cfctur", "cfvtur", "resev", "dev", "ev"]
 # two-grid cycle from CMesh::Initial on the coarse level: name -> (kwargs,
 # iterations, meta); the fixture holds the history and the final fields of
 # both levels
@@ -85,11 +91,13 @@ def gen_tiny(name, kw, n_iter, extra=None):
         arrs = {"history": res["history"]}
         for b in range(1, len(case.blocks) + 1):
             tag = "" if b == 1 else f"b{b}_"
-            for f in FIELDS_IT1 + (NS_FIELDS_IT1 if kw.get("viscous") else []) + (["mut"] if meta.get("turbulence") else []):
+            for f in FIELDS_IT1 + (NS_FIELDS_IT1 if kw.get("viscous") else []) + (["mut"] if meta.get("turbulence") else []) + \
+                    (SA_FIELDS_IT1 if meta.get("turbulence") == 200 else []):
                 arrs["it1_" + tag + f] = np.load(os.path.join(odir, f"it1_b{b}_{f}.npy"))
         for b in range(1, len(case.blocks) + 1):
             tag = "" if b == 1 else f"b{b}_"
-            for f in ("cv", "pv", "res", "dq") + (("mut",) if meta.get("turbulence") else ()):
+            for f in ("cv", "pv", "res", "dq") + (("mut",) if meta.get("turbulence") else ()) + \
+                    (("ev",) if meta.get("turbulence") == 200 else ()):
                 arrs["final_" + tag + f] = np.load(os.path.join(odir, f"final_b{b}_{f}.npy"))
     arrs["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
     np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrs)

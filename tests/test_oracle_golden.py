@@ -5,8 +5,9 @@ synthetic cases; the restatement must reproduce them bit for bit at one
 thread.  Cases cover Roe and Steger-Warming, MUSCL3 / NND / MUSCL2u / MUSCL2o
 reconstruction on primitive and conservative variables, the inlet,
 general-outlet and wall ghost fills, the laminar Navier-Stokes terms, the
-Baldwin-Lomax eddy viscosity, a three-block rotating passage with periodic
-and interior patches, and the two-grid cycle started from CMesh::Initial on
+Baldwin-Lomax eddy viscosity, three-block rotating passages with periodic
+and interior patches (Baldwin-Lomax, and Spalart-Allmaras with the wall
+distance, the nu~ gradients, fluxes and source, and its LU-SGS equation), and the two-grid cycle started from CMesh::Initial on
 the coarse level (uniform start on the bump channel, interpolated start on the
 Baldwin-Lomax passage).
 """
@@ -20,7 +21,9 @@ from fv3d import cases
 import oracle as O
 
 TINY = ["tiny_roe_muscl3_prim", "tiny_sw_nnd_prim", "tiny_roe_muscl3_cons", "tiny_roe_muscl2o_prim",
-        "tiny_sw_muscl2u_cons", "tiny_ns_roe_muscl3_prim", "tiny_bl_roe_muscl3_prim", "tiny_passage3_bl"]
+        "tiny_sw_muscl2u_cons", "tiny_ns_roe_muscl3_prim", "tiny_bl_roe_muscl3_prim", "tiny_passage3_bl",
+        "tiny_passage3_sa"]
+SA_FIELDS = ("dwall", "gevI", "gevJ", "gevK", "cfctur", "cfvtur", "resev", "dev", "ev")
 NS_FIELDS = ("omega", "gradI", "gradJ", "gradK", "fvI", "fvJ", "fvK", "fv", "lamvI", "lamvJ", "lamvK")
 
 
@@ -42,6 +45,8 @@ def test_first_iteration_bitexact(golden_dir, name):
         fields += NS_FIELDS
     if meta.get("turbulence"):
         fields += ("mut",)
+    if meta.get("turbulence") == 200:
+        fields += SA_FIELDS
     for b in range(1, len(case.blocks) + 1):
         tag = "" if b == 1 else f"b{b}_"
         for f in fields:
@@ -65,6 +70,8 @@ def test_history_bitexact(golden_dir, name):
         assert np.array_equal(o.get(b, "pv"), z["final_" + tag + "pv"])
         if meta.get("turbulence"):
             assert np.array_equal(o.get(b, "mut").ravel(), z["final_" + tag + "mut"].ravel())
+        if meta.get("turbulence") == 200:
+            assert np.array_equal(o.get(b, "ev").ravel(), z["final_" + tag + "ev"].ravel())
 
 
 def test_c0_first_iterations(golden_dir):

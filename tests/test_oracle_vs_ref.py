@@ -29,6 +29,9 @@ FIELDS = ["res", "dq", "dt", "lamcI", "lamcJ", "lamcK", "cv", "pv", "fcI", "fcJ"
 # laminar Navier-Stokes: vorticity, face gradients, viscous face and cell
 # fluxes, viscous spectral radii
 NS_FIELDS = ["omega", "gradI", "gradJ", "gradK", "fvI", "fvJ", "fvK", "fv", "lamvI", "lamvJ", "lamvK"]
+# Spalart-Allmaras: wall distance, face gradients of nu~, the cell sums of its
+# convective and viscous fluxes, its residual, increment and value
+SA_FIELDS = ["dwall", "gevI", "gevJ", "gevK", "cfctur", "cfvtur", "resev", "dev", "ev"]
 
 
 def _shifted(case, dx=1.0, dy=0.3):
@@ -100,6 +103,19 @@ def variant(name):
                                                         rpm=8000.0, blade_blocks=(1,), camber=0.15, viscous=True,
                                                         len_ref=1e-3, periodic_from_high=True))
         c.cfg.turbulence = 100
+    elif name == "sa_bump":
+        c = cases.with_sa(cases.with_wall_shear(cases.bump_channel(ni=7, nj=6, nk=10, seed=71, viscous=True,
+                                                                   len_ref=1e-3, max_iter=8)))
+    elif name == "sa_passage_turn1":
+        c = cases.with_sa(cases.with_wall_shear(cases.annular_passage(ni=8, nj=6, nk_blocks=(5, 8, 4), seed=72,
+                                                                      rpm=6000.0, blade_blocks=(1,), camber=0.1,
+                                                                      viscous=True, len_ref=1e-3, max_iter=8)))
+        c = cases.turn_block(c, 1, 1)
+    elif name == "sa_sw_nnd_limiter_centrifugal":
+        c = cases.with_sa(cases.with_wall_shear(cases.bump_channel(ni=6, nj=7, nk=9, seed=73, viscous=True,
+                                                                   len_ref=1e-3, max_iter=8, flux=200,
+                                                                   recon_scheme=100, recon_var=100)))
+        c.cfg.limiter, c.cfg.turbo_type = "Y", "C"
     else:
         raise KeyError(name)
     return c
@@ -108,11 +124,14 @@ def variant(name):
 @pytest.mark.parametrize("name", ["muscl2c_axial", "nnd_cons_sw_centrifugal", "rotating_roe_muscl3",
                                   "ns_sw_nnd_cons", "ns_rotating_roe_muscl2o", "bl_sw_muscl2u_kwall",
                                   "bl_rotating_roe_muscl3", "passage_turn1", "passage_turn2", "passage_turn3",
-                                  "passage_bl_blade"])
+                                  "passage_bl_blade", "sa_bump", "sa_passage_turn1",
+                                  "sa_sw_nnd_limiter_centrifugal"])
 def test_restatement_matches_reference(name):
     case = variant(name)
     n = case.cfg.max_iter
     fields = FIELDS + (NS_FIELDS if case.cfg.equation == "N" else []) + (["mut"] if case.cfg.turbulence else [])
+    if case.cfg.turbulence == 200:
+        fields = fields + SA_FIELDS
     with tempfile.TemporaryDirectory() as td:
         cdir, odir = os.path.join(td, "case"), os.path.join(td, "out")
         case.write(cdir)
@@ -135,6 +154,11 @@ def test_restatement_matches_reference(name):
             if case.cfg.turbulence:
                 want = np.load(os.path.join(odir, f"final_b{b}_mut.npy"))
                 assert np.array_equal(o2.get(b, "mut").ravel(), want.ravel())
+            if case.cfg.turbulence == 200:
+                want = np.load(os.path.join(odir, f"final_b{b}_ev.npy"))
+                assert np.array_equal(o2.get(b, "ev").ravel(), want.ravel())
+        if case.cfg.turbulence == 200:
+            assert o2.res_calculate()[5] == ref["history"][-1, 7], "Res_ev"
 
 
 # ------------------------------------------------------------ two-grid cycle
@@ -168,6 +192,13 @@ def mg_variant(name):
         c = cases.annular_passage(ni=8, nj=6, nk_blocks=(4, 8, 4), seed=36, rpm=4000.0, max_iter=8)
         c.cfg.timestep, c.cfg.turbo_type = "G", "A"
         c.cfg.coarse_iter, c.cfg.initial = 2, "U"
+    elif name == "sa_passage_global_dt_axial":
+        c = cases.with_wall_shear(cases.annular_passage(ni=8, nj=8, nk_blocks=(6, 10, 6), seed=74, rpm=3000.0,
+                                                        blade_blocks=(1,), camber=0.1, viscous=True, len_ref=1e-3,
+                                                        max_iter=10))
+        c.cfg.turbulence = 200
+        c.cfg.coarse_iter, c.cfg.initial, c.cfg.outlet_angle = 3, "I", (0.0, 0.2, 1.0)
+        c.cfg.timestep, c.cfg.turbo_type = "G", "A"
     elif name == "not_coarsenable":
         # 7 cells along i: the patch ranges end on an even node, one level only
         c = cases.bump_channel(ni=7, nj=8, nk=16, seed=35, max_iter=8)
@@ -184,7 +215,8 @@ MG_COARSE = ["cv", "pv", "res", "dq", "dt", "lamcI", "lamcJ", "lamcK"]
 
 
 @pytest.mark.parametrize("name", ["bump_uniform", "bump_interpolate_sw_limiter", "bl_bump", "bl_passage",
-                                  "bl_passage_turn1", "global_dt_axial_passage", "not_coarsenable"])
+                                  "bl_passage_turn1", "global_dt_axial_passage", "sa_passage_global_dt_axial",
+                                  "not_coarsenable"])
 def test_two_grid_matches_reference(name):
     case = mg_variant(name)
     n = case.cfg.max_iter
