@@ -116,6 +116,13 @@ struct fv3d_ctx {
     int *d_axial = nullptr;          // radial-equilibrium outlet patches (indices into d_patches)
     int n_axial = 0;
     unsigned long long *d_dtmin = nullptr;   // per-block minimum time step (global time step)
+    // Spalart-Allmaras: ghost-cell work list (2 (nj nk + ni nk + ni nj) per
+    // block), per-block flags of the nu~ limiter, and whether nu~ was given
+    // (fv3d_nut_upload or fv3d_initial)
+    long *d_gstart = nullptr;
+    long nghost = 0;
+    int *d_sa_hi = nullptr;
+    bool ev_set = false;
     bool owns_stream = true;
     bool mg_coarse = false;          // allocate cv_2h / Res_2h (coarse level)
     int ng = 1;
@@ -191,6 +198,7 @@ static void nondimensionalize(fv3d_ctx *c)
     g.re_ref = c->re_ref;
     g.pr_lam = (float)f.pr_lam;
     g.pr_tur = (float)f.pr_tur;
+    g.ev_ratio = (float)f.ev_ratio;
 }
 
 // source/GeometryVariables.cpp:10-105
@@ -307,8 +315,15 @@ static int setup_block(fv3d_ctx *c, HostBlock &b, int index)
         if ((rc = alloc(&B.lamv, 3 * B.box))) return rc;
         if ((rc = alloc(&B.omega, B.box))) return rc;
     }
-    if (c->cfg.turbulence) {
-        if ((rc = alloc(&B.mut, B.box))) return rc;
+    if (c->cfg.turbulence) if ((rc = alloc(&B.mut, B.box))) return rc;
+    if (c->cfg.turbulence == 200) {
+        if ((rc = alloc(&B.ev, B.box))) return rc;
+        if ((rc = alloc(&B.ev_old, B.box))) return rc;
+        if ((rc = alloc(&B.dev, B.box))) return rc;
+        if ((rc = alloc(&B.res_ev, B.box))) return rc;
+        if ((rc = alloc(&B.dwall, B.box))) return rc;
+    }
+    if (c->cfg.turbulence == 100) {
         unsigned long long *f;
         if ((rc = dalloc(c, &f, B.box))) return rc;
         b.allocs.push_back(f);
@@ -511,6 +526,60 @@ static int build_exchange(fv3d_ctx *c)
     return FV3D_OK;
 }
 
+// Spalart-Allmaras set-up: the ghost-cell work list of k_sa_ghost, the
+// limiter flags and the wall distance of every cell (CMesh::WallDistanceSA,
+// source/WallDistanceSA.cpp:9-132): the centroids of all wall faces of all
+// blocks are listed on the host and k_wall_distance takes the minimum.
+static int setup_sa(fv3d_ctx *c)
+{
+    const int nb = (int)c->blocks.size();
+    int rc;
+    std::vector<long> gst{0};
+    for (const HostBlock &b : c->blocks) {
+        const BlockDev &B = b.dev;
+        gst.push_back(gst.back() + 2L * ((long)B.nj * B.nk + (long)B.ni * B.nk + (long)B.ni * B.nj));
+    }
+    c->nghost = gst.back();
+    if ((rc = dalloc(c, &c->d_gstart, gst.size()))) return rc;
+    HIPCK(c, hipMemcpyAsync(c->d_gstart, gst.data(), gst.size() * sizeof(long), hipMemcpyHostToDevice, c->stream));
+    if ((rc = dalloc(c, &c->d_sa_hi, nb))) return rc;
+    std::vector<std::vector<double>> fcen(nb);   // face centroids of the blocks that hold walls, 3 directions
+    std::vector<double> wf;
+    for (const HostPatch &P : c->patches) {
+        if (P.kind != 2) continue;
+        const int n = P.block - 1, d = (P.face - 1) / 2;
+        HostBlock &b = c->blocks[n];
+        const long box = b.dev.box;
+        if (fcen[n].empty()) {
+            std::vector<double> fS[3], fs[3], fc[3], vol, cc, node;
+            face_vector_volume(b, fS[0], fS[1], fS[2], fs[0], fs[1], fs[2], fc[0], fc[1], fc[2], vol, cc, node);
+            fcen[n].resize(9 * box);
+            for (int q = 0; q < 3; ++q) std::copy(fc[q].begin(), fc[q].end(), fcen[n].begin() + 3 * q * box);
+        }
+        const double *F = fcen[n].data() + 3 * d * box;
+        for (int i = P.rng[0]; i <= P.rng[1]; ++i)
+            for (int j = P.rng[2]; j <= P.rng[3]; ++j)
+                for (int k = P.rng[4]; k <= P.rng[5]; ++k) {
+                    const long x = cidx(b.dev, i, j, k);
+                    wf.push_back(F[x]);
+                    wf.push_back(F[box + x]);
+                    wf.push_back(F[2 * box + x]);
+                }
+    }
+    const int nf = (int)(wf.size() / 3);
+    double *d_wf = nullptr;
+    if (nf > 0) {
+        if ((rc = dalloc(c, &d_wf, wf.size()))) return rc;
+        HIPCK(c, hipMemcpyAsync(d_wf, wf.data(), wf.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    hipLaunchKernelGGL(k_wall_distance, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
+                       c->d_cstart, nb, d_wf, nf);
+    HIPCK(c, hipGetLastError());
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    if (d_wf) HIPCK(c, hipFree(d_wf));
+    return FV3D_OK;
+}
+
 // ------------------------------------------------------------------ API
 extern "C" {
 
@@ -523,7 +592,7 @@ int fv3d_ctx_create(const fv3d_config *cfg, fv3d_ctx **out)
     if (cfg->recon_var != 100 && cfg->recon_var != 200) { *out = c; return fail(c, FV3D_EINVAL, "recon_var %d unsupported", cfg->recon_var); }
     if (cfg->recon_scheme < 100 || cfg->recon_scheme > 500 || cfg->recon_scheme % 100) { *out = c; return fail(c, FV3D_EINVAL, "recon_scheme %d unsupported", cfg->recon_scheme); }
     if (cfg->flux_scheme != 200 && cfg->flux_scheme != 300) { *out = c; return fail(c, FV3D_EINVAL, "flux_scheme %d unsupported", cfg->flux_scheme); }
-    if (cfg->turbulence != 0 && cfg->turbulence != 100) { *out = c; return fail(c, FV3D_EINVAL, "turbulence model %d not in this build", cfg->turbulence); }
+    if (cfg->turbulence != 0 && cfg->turbulence != 100 && cfg->turbulence != 200) { *out = c; return fail(c, FV3D_EINVAL, "turbulence model %d not in this build", cfg->turbulence); }
     if (cfg->turbulence && !cfg->viscous) { *out = c; return fail(c, FV3D_EINVAL, "a turbulence model needs the Navier-Stokes equations"); }
     if (cfg->precision != FV3D_PLAIN_FP64 && cfg->precision != FV3D_DEFECT_FP32) { *out = c; return fail(c, FV3D_EINVAL, "precision mode %d", cfg->precision); }
     c->device = cfg->device;
@@ -555,6 +624,8 @@ int fv3d_ctx_destroy(fv3d_ctx *c)
     if (c->d_mg_tiles) (void)hipFree(c->d_mg_tiles);
     if (c->d_axial) (void)hipFree(c->d_axial);
     if (c->d_dtmin) (void)hipFree(c->d_dtmin);
+    if (c->d_gstart) (void)hipFree(c->d_gstart);
+    if (c->d_sa_hi) (void)hipFree(c->d_sa_hi);
     if (c->d_pack_start) (void)hipFree(c->d_pack_start);
     for (int d = 0; d < 3; ++d) {
         if (c->vdbg.fv[d]) (void)hipFree(c->vdbg.fv[d]);
@@ -968,7 +1039,8 @@ int fv3d_setup(fv3d_ctx *c)
         c->kba_grid = std::min(c->nkba, 4096);
     }
     c->npart = (int)((c->ncell + 255) / 256);
-    if ((rc = dalloc(c, &c->d_partial, 5 * (size_t)c->npart))) return rc;
+    if ((rc = dalloc(c, &c->d_partial, 6 * (size_t)c->npart))) return rc;
+    if (c->cfg.turbulence == 200 && (rc = setup_sa(c))) return rc;
     if ((rc = dalloc(c, &c->d_sums, 2 * 3 * kSecParts))) return rc;
     if ((rc = dalloc(c, &c->d_stage_row, 7))) return rc;
     // device scalars
@@ -1017,6 +1089,26 @@ int fv3d_state_upload(fv3d_ctx *c, int block, const double *pv, const double *en
     HIPCK(c, hipMemcpyAsync(B.Q, Q.data(), Q.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCK(c, hipMemcpyAsync(B.W, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
+    return FV3D_OK;
+}
+
+// nu~ of one block's cells from a restart file (CMesh::ReadRestart's nut
+// field, source/UserInput.cpp:1134-1153)
+int fv3d_nut_upload(fv3d_ctx *c, int block, const double *ev)
+{
+    if (!c || !ev) return FV3D_EINVAL;
+    if (!c->ready) return fail(c, FV3D_ESTATE, "nut_upload before setup");
+    if (c->cfg.turbulence != 200) return fail(c, FV3D_EINVAL, "nu~ exists with the Spalart-Allmaras model only");
+    if (block < 1 || block > (int)c->blocks.size()) return fail(c, FV3D_EINVAL, "block %d", block);
+    const BlockDev &B = c->blocks[block - 1].dev;
+    std::vector<double> E(B.box, 0.0);
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    HIPCK(c, hipMemcpy(E.data(), B.ev, E.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 1; i <= B.ni; ++i)
+        for (int j = 1; j <= B.nj; ++j)
+            for (int k = 1; k <= B.nk; ++k) E[cidx(B, i, j, k)] = ev[((long)(i - 1) * B.nj + (j - 1)) * B.nk + (k - 1)];
+    HIPCK(c, hipMemcpy(B.ev, E.data(), E.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->ev_set = true;
     return FV3D_OK;
 }
 
@@ -1117,7 +1209,17 @@ static int initial_level(fv3d_ctx *m, int type)
                 }
         HIPCK(m, hipMemcpy(B.Q, Q.data(), Q.size() * sizeof(double), hipMemcpyHostToDevice));
         HIPCK(m, hipMemcpy(B.W, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice));
+        if (B.ev) {
+            // nu~ = ev_ratio in the interior (source/Initialization.cpp:40, 71, 130, 166)
+            std::vector<double> E(B.box, 0.0);
+            HIPCK(m, hipMemcpy(E.data(), B.ev, E.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (int i = 1; i <= B.ni; ++i)
+                for (int j = 1; j <= B.nj; ++j)
+                    for (int k = 1; k <= B.nk; ++k) E[cidx(B, i, j, k)] = g.ev_ratio * F32C(1.0);
+            HIPCK(m, hipMemcpy(B.ev, E.data(), E.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
     }
+    if (m->cfg.turbulence == 200) m->ev_set = true;
     return FV3D_OK;
 }
 
@@ -1132,6 +1234,10 @@ int fv3d_initial(fv3d_ctx *c)
         if (m != c) fail(c, rc, "coarse level: %s", m->err.c_str());
         return rc;
     }
+    // with Spalart-Allmaras the fine level starts from its own field too
+    // (source/Structured_FV3D.cpp:89-95); the state interpolated at
+    // Coarse_iter replaces its flow variables, not its nu~
+    if (m != c && c->cfg.turbulence == 200 && (rc = initial_level(c, c->cfg.initial_type))) return rc;
     c->initial_done = true;
     return FV3D_OK;
 }
@@ -1192,11 +1298,31 @@ static int launch_viscous(fv3d_ctx *c)
     return FV3D_OK;
 }
 
+// Spalart-Allmaras in the residual evaluation: the vorticity, the nu~
+// residual (face gradients from nu~ as it stands, fluxes and source from its
+// clipped value), then mue_t and the clipping stored, then mue_t of the ghost
+// cells.  save_old: the LU-SGS step keeps nu~ before the clipping in ev_old.
+static int launch_sa(fv3d_ctx *c, int save_old)
+{
+    const int nb = (int)c->blocks.size();
+    const unsigned gc = (unsigned)((c->ncell + 255) / 256);
+    hipLaunchKernelGGL(k_curl, dim3(gc), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb);
+    hipLaunchKernelGGL((k_sa_faces<kTI, kTJ, kTK>), dim3((unsigned)c->ntiles), dim3(kTI * kTJ * kTK), 0, c->stream,
+                       c->d_blocks, c->d_tiles, c->gas);
+    hipLaunchKernelGGL(k_sa_mut, dim3(gc), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb, c->gas, save_old);
+    hipLaunchKernelGGL(k_sa_ghost, dim3((unsigned)((c->nghost + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
+                       c->d_gstart, nb, c->gas);
+    HIPCK(c, hipGetLastError());
+    return FV3D_OK;
+}
+
 // Turbulence model before the time step (source/ComputeAllBlockVars.cpp:
 // 228-232): the vorticity, then Baldwin-Lomax on all walls at once (one launch
-// per line group size) and the merge of the per-cell minima.
-static int launch_turbulence(fv3d_ctx *c)
+// per line group size) and the merge of the per-cell minima; or
+// Spalart-Allmaras.
+static int launch_turbulence(fv3d_ctx *c, int save_old = 0)
 {
+    if (c->cfg.turbulence == 200) return launch_sa(c, save_old);
     if (c->cfg.turbulence != 100) return FV3D_OK;
     const int nb = (int)c->blocks.size();
     hipLaunchKernelGGL(k_curl, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb);
@@ -1227,6 +1353,26 @@ static int launch_curl(fv3d_ctx *c)
                        (int)c->blocks.size());
     HIPCK(c, hipGetLastError());
     return FV3D_OK;
+}
+
+// The nu~ equation of the LU-SGS sweep in one direction, one launch per
+// hyperplane i + j + k, after the flow sweep of that direction has left its
+// increments in dQ (the reference interleaves the two per cell; the nu~ update
+// of a cell reads the flow increments of its upstream neighbours only, which
+// that flow sweep has finished).
+static void launch_sa_sweep(fv3d_ctx *c, bool fwd)
+{
+    if (c->cfg.turbulence != 200) return;
+    const unsigned g = (unsigned)((c->nplane_threads + 255) / 256);
+    const int nb = (int)c->blocks.size();
+    for (int s = 0; s < c->nplanes; ++s) {
+        if (fwd)
+            hipLaunchKernelGGL(k_sa_plane<true>, dim3(g), dim3(256), 0, c->stream, c->d_blocks, c->d_pstart_plane, nb, s,
+                               c->gas.relax, c->gas);
+        else
+            hipLaunchKernelGGL(k_sa_plane<false>, dim3(g), dim3(256), 0, c->stream, c->d_blocks, c->d_pstart_plane, nb, s,
+                               c->gas.relax, c->gas);
+    }
 }
 
 static int launch_sweeps_kba(fv3d_ctx *c)
@@ -1271,32 +1417,47 @@ static int launch_sweeps_kba(fv3d_ctx *c)
             }
 #undef FV3D_SL_LAUNCH
         };
+        auto unpack = [&]() {
+            if (f32)
+                hipLaunchKernelGGL(k_sweep_unpack<float>, dim3(pg), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb);
+            else
+                hipLaunchKernelGGL(k_sweep_unpack<double>, dim3(pg), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb);
+        };
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[6], c->stream));
         sweep(true);
+        if (c->cfg.turbulence == 200) {
+            // the forward increments in dQ for the nu~ sweep
+            unpack();
+            launch_sa_sweep(c, true);
+        }
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[7], c->stream));
         sweep(false);
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[8], c->stream));
-        if (f32)
-            hipLaunchKernelGGL(k_sweep_unpack<float>, dim3(pg), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb);
-        else
-            hipLaunchKernelGGL(k_sweep_unpack<double>, dim3(pg), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb);
+        unpack();
+        launch_sa_sweep(c, false);
     } else if (c->sweep_mode == 3) {
         hipLaunchKernelGGL((k_lusgs_kba3<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
                            c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_lag);
+        launch_sa_sweep(c, true);
         hipLaunchKernelGGL((k_lusgs_kba3<false, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
                            c->d_blocks, c->d_kba[1], c->nkba, tb, db, err, c->gas.relax, c->gas, c->kba_lag);
+        launch_sa_sweep(c, false);
     } else if (c->sweep_mode == 2) {
         hipLaunchKernelGGL((k_lusgs_kba2<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(kKbaTJ * kKbaTK), 0,
                            c->stream, c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_chunk,
                            c->kba_lag);
+        launch_sa_sweep(c, true);
         hipLaunchKernelGGL((k_lusgs_kba2<false, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(kKbaTJ * kKbaTK), 0,
                            c->stream, c->d_blocks, c->d_kba[1], c->nkba, tb, db, err, c->gas.relax, c->gas, c->kba_chunk,
                            c->kba_lag);
+        launch_sa_sweep(c, false);
     } else {
         hipLaunchKernelGGL((k_lusgs_kba<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(kKbaTJ * kKbaTK), 0,
                            c->stream, c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_chunk);
+        launch_sa_sweep(c, true);
         hipLaunchKernelGGL((k_lusgs_kba<false, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(kKbaTJ * kKbaTK), 0,
                            c->stream, c->d_blocks, c->d_kba[1], c->nkba, tb, db, err, c->gas.relax, c->gas, c->kba_chunk);
+        launch_sa_sweep(c, false);
     }
     HIPCK(c, hipGetLastError());
     return FV3D_OK;
@@ -1310,9 +1471,11 @@ static int launch_sweeps(fv3d_ctx *c)
     for (int s = 0; s < c->nplanes; ++s)
         hipLaunchKernelGGL(k_lusgs_plane<true>, dim3(g), dim3(256), 0, c->stream, c->d_blocks, c->d_pstart_plane, nb, s,
                            c->gas.relax, c->gas);
+    launch_sa_sweep(c, true);
     for (int s = 0; s < c->nplanes; ++s)
         hipLaunchKernelGGL(k_lusgs_plane<false>, dim3(g), dim3(256), 0, c->stream, c->d_blocks, c->d_pstart_plane, nb, s,
                            c->gas.relax, c->gas);
+    launch_sa_sweep(c, false);
     HIPCK(c, hipGetLastError());
     return FV3D_OK;
 }
@@ -1320,11 +1483,20 @@ static int launch_sweeps(fv3d_ctx *c)
 static int launch_update(fv3d_ctx *c)
 {
     int nb = (int)c->blocks.size();
+    const bool sa = c->cfg.turbulence == 200;
+    int *sa_hi = sa && c->cfg.limiter ? c->d_sa_hi : nullptr;
     hipLaunchKernelGGL(k_update, dim3((unsigned)c->npart), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb, c->gas,
-                       c->d_partial, c->d_bad);
+                       c->d_partial, c->d_bad, sa_hi);
     HIPCK(c, hipGetLastError());
     if (c->cfg.limiter) {
         hipLaunchKernelGGL(k_limiter_serial, dim3(1), dim3(64), 0, c->stream, c->d_blocks, nb, c->gas, c->d_bad);
+        if (sa) hipLaunchKernelGGL(k_sa_limit_serial, dim3(1), dim3(64), 0, c->stream, c->d_blocks, nb, c->d_sa_hi);
+        HIPCK(c, hipGetLastError());
+    }
+    if (sa) {
+        // Limit_SA's floor and the Res_ev partials (component 5)
+        hipLaunchKernelGGL(k_sa_res, dim3((unsigned)c->npart), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb,
+                           c->d_partial, c->cfg.limiter ? 1 : 0);
         HIPCK(c, hipGetLastError());
     }
     return FV3D_OK;
@@ -1383,7 +1555,7 @@ static int lusgs_timed(fv3d_ctx *c)
     int rc;
     if (c->timing) HIPCK(c, hipEventRecord(c->ev[0], c->stream));
     if ((rc = launch_bc(c))) return rc;
-    if ((rc = launch_turbulence(c))) return rc;
+    if ((rc = launch_turbulence(c, 1))) return rc;
     if ((rc = launch_spectral_dt(c))) return rc;
     if (c->timing) HIPCK(c, hipEventRecord(c->ev[1], c->stream));
     if ((rc = launch_convective(c))) return rc;
@@ -1408,7 +1580,7 @@ int fv3d_lusgs(fv3d_ctx *c)
 int fv3d_res_calculate(fv3d_ctx *c, double *res6)
 {
     READY(c);
-    std::vector<double> part(5 * (size_t)c->npart);
+    std::vector<double> part(6 * (size_t)c->npart);
     HIPCK(c, hipMemcpyAsync(part.data(), c->d_partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
     for (int q = 0; q < 5; ++q) {
@@ -1416,7 +1588,14 @@ int fv3d_res_calculate(fv3d_ctx *c, double *res6)
         for (int p = 0; p < c->npart; ++p) s += part[(size_t)q * c->npart + p];
         if (res6) res6[q] = s / c->total_vol;
     }
-    if (res6) res6[5] = 0.0;
+    if (res6) {
+        res6[5] = 0.0;
+        if (c->cfg.turbulence == 200) {
+            double s = 0.0;
+            for (int p = 0; p < c->npart; ++p) s += part[(size_t)5 * c->npart + p];
+            res6[5] = s / c->total_vol;
+        }
+    }
     return FV3D_OK;
 }
 
@@ -1552,6 +1731,8 @@ int fv3d_iterate(fv3d_ctx *c, int n, double *hist)
     if (c->ng == 2 && !c->initial_done)
         return fail(c, FV3D_ESTATE, "the two-grid cycle starts from fv3d_initial (reuse_solution = N): from a restart "
                                     "file the reference's cycle divides by coarse time steps it never computed");
+    if (c->cfg.turbulence == 200 && !c->ev_set)
+        return fail(c, FV3D_ESTATE, "Spalart-Allmaras needs nu~: fv3d_nut_upload for every block, or fv3d_initial");
     for (int it = 0; it < n; ++it) {
         const long itn = c->iter_host + 1;
         double *row = c->d_hist + 7L * it;
@@ -1589,6 +1770,8 @@ int fv3d_iterate_graph(fv3d_ctx *c, int n, double *hist)
     if (c->ng == 2 && !c->initial_done)
         return fail(c, FV3D_ESTATE, "the two-grid cycle starts from fv3d_initial (reuse_solution = N): from a restart "
                                     "file the reference's cycle divides by coarse time steps it never computed");
+    if (c->cfg.turbulence == 200 && !c->ev_set)
+        return fail(c, FV3D_ESTATE, "Spalart-Allmaras needs nu~: fv3d_nut_upload for every block, or fv3d_initial");
     // each graph writes its history row into a fixed staging slot that is
     // copied into the history buffer after each replay; with two levels the
     // coarse start-up phase and the cycle have a graph each, and the one
@@ -1725,9 +1908,10 @@ long fv3d_field_download(fv3d_ctx *c, int block, const char *name, double *out)
     std::vector<double> h;
     long cnt = 0;
     std::string nm(name);
-    if (nm == "mut") {
-        if (!B.mut) return fail(c, FV3D_EINVAL, "mut exists with a turbulence model only");
-        if (fetch(B.mut, B.box, h)) return FV3D_EHIP;
+    if (nm == "mut" || nm == "ev") {
+        const double *p = nm == "mut" ? B.mut : B.ev;
+        if (!p) return fail(c, FV3D_EINVAL, "%s exists with %s only", name, nm == "mut" ? "a turbulence model" : "Spalart-Allmaras");
+        if (fetch(p, B.box, h)) return FV3D_EHIP;
         for (int i = 0; i <= B.IM; ++i)
             for (int j = 0; j <= B.JM; ++j)
                 for (int k = 0; k <= B.KM; ++k) out[cnt++] = h[cidx(B, i, j, k)];
@@ -1748,6 +1932,11 @@ long fv3d_field_download(fv3d_ctx *c, int block, const char *name, double *out)
     else if (nm == "dq") { src = B.dQ; C = 5; }
     else if (nm == "dt") { src = B.dt; C = 1; }
     else if (nm == "vol") { src = B.vol; C = 1; }
+    else if (nm == "resev" || nm == "dev" || nm == "dwall") {
+        src = nm == "resev" ? B.res_ev : nm == "dev" ? B.dev : B.dwall;
+        if (!src) return fail(c, FV3D_EINVAL, "%s exists with Spalart-Allmaras only", name);
+        C = 1;
+    }
     else if (nm == "lamcI") { src = B.lamc; C = 1; }
     else if (nm == "lamcJ") { src = B.lamc + B.box; C = 1; }
     else if (nm == "lamcK") { src = B.lamc + 2 * B.box; C = 1; }

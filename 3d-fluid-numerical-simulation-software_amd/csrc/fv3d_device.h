@@ -60,6 +60,14 @@ struct BlockDev {
     // (all ones = not visited; the merge resets it)
     double *mut;
     unsigned long long *blmin;
+    // Spalart-Allmaras (null without it): nu~ over the cell box, its value
+    // before the update, its LU-SGS increment and residual, and the wall
+    // distance of the cells
+    double *ev;
+    double *ev_old;
+    double *dev;
+    double *res_ev;
+    double *dwall;
     double *dt;               // local time step
     double *alpha;            // LU-SGS diagonal Vol/dt + relax*sum(lamc) + 2*sum(lamv)
     const double *swg;        // 15*box static sweep metrics per cell: for d = 0..2,
@@ -106,8 +114,9 @@ struct Gas {
     double c_timestep, relax;
     int recon_var, recon_scheme, flux_scheme;
     int viscous;                  // Navier-Stokes terms on
-    int turbulence;               // 0 laminar, 100 Baldwin-Lomax
+    int turbulence;               // 0 laminar, 100 Baldwin-Lomax, 200 Spalart-Allmaras
     double re_ref, pr_lam, pr_tur;
+    double ev_ratio;              // inlet nu~ (Spalart-Allmaras)
 };
 
 // CFluidProps::SutherLand (include/FluidProps.h:107-112), non-dimensional

@@ -24,6 +24,7 @@ struct PatchDev {
 struct ScalarsDev {
     double cfl, cfl_res;
     double res[5];
+    double res_ev;      // Spalart-Allmaras nu~ residual norm
     double total_vol;
     double dens_ref, vel_ref, temp_ref;
     double nblade_in, nblade_out;
@@ -110,7 +111,8 @@ __global__ void k_sweep_unpack(const BlockDev *blocks, const long *gstart, int n
 constexpr int kPackChunk = 16;   // i-planes per k_sweep_pack / k_sweep_unpack workgroup
 constexpr int kSecParts = 32;    // workgroups per section in k_sections
 __global__ void k_fill_int(int *p, long n, int v);
-__global__ void k_update(const BlockDev *blocks, const long *cstart, int nblock, Gas g, double *partial, int *bad_flag);
+__global__ void k_update(const BlockDev *blocks, const long *cstart, int nblock, Gas g, double *partial, int *bad_flag,
+                         int *sa_hi);
 __global__ void k_limiter_serial(const BlockDev *blocks, int nblock, Gas g, int *bad_flag);
 __global__ void k_sections(const BlockDev *blocks, const int2 *secs, Gas g, double *sums);
 // norms and section integrals of one level (lv) with the CFL ramp and the
@@ -151,5 +153,21 @@ constexpr unsigned long long kBlUnvisited = ~0ull;
 template <int G>
 __global__ void k_bl_lines(const BlockDev *blocks, const PatchDev *patches, const int2 *walls, int nwalls, Gas g);
 __global__ void k_bl_merge(const BlockDev *blocks, const long *cstart, int nblock);
+
+// Spalart-Allmaras (CSpaceDiscr::SA_Model, CMesh::WallDistanceSA,
+// CFluidProps::Limit_SA and the nu~ part of CMesh::LU_SGS_Marching)
+__global__ void k_wall_distance(const BlockDev *blocks, const long *cstart, int nblock, const double *wf, int nf);
+template <int TI, int TJ, int TK>
+__global__ void k_sa_faces(const BlockDev *blocks, const int4 *tiles, Gas g);
+// save_old: keep nu~ as it stands (before the clipping) in ev_old, as the
+// LU-SGS step does before its residual evaluation
+__global__ void k_sa_mut(const BlockDev *blocks, const long *cstart, int nblock, Gas g, int save_old);
+__global__ void k_sa_ghost(const BlockDev *blocks, const long *gstart, int nblock, Gas g);
+template <bool FORWARD>
+__global__ void k_sa_plane(const BlockDev *blocks, const long *pstart, int nblock, int step, double relax, Gas g);
+// Limit_SA: the blocks k_update flagged, serially; then the pointwise floor of
+// the others (clamp set) with the Res_ev partials in component 5
+__global__ void k_sa_limit_serial(const BlockDev *blocks, int nblock, int *hi_flag);
+__global__ void k_sa_res(const BlockDev *blocks, const long *cstart, int nblock, double *partial, int clamp);
 
 }  // namespace fv3d

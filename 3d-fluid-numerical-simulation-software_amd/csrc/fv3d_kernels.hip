@@ -79,6 +79,7 @@ __global__ void k_bc(const BlockDev *__restrict__ blocks, const PatchDev *__rest
         double ener = PressBC / DensBC / (Gamma - F32C(1.0)) + V2 / F32C(2.0);
         pv[0] = DensBC; pv[1] = u; pv[2] = v; pv[3] = w; pv[4] = PressBC; pv[5] = TempBC;
         cv[0] = DensBC; cv[1] = DensBC * u; cv[2] = DensBC * v; cv[3] = DensBC * w; cv[4] = DensBC * ener;
+        if (B.ev) B.ev[xg] = g.ev_ratio * F32C(1.0);   // source/BndCondInlet.cpp:104-105
     } else if (P.kind == 1) {
         // general / centrifugal outlet, K faces only (the radial-equilibrium
         // outlet is k_bc_axial's)
@@ -107,6 +108,7 @@ __global__ void k_bc(const BlockDev *__restrict__ blocks, const PatchDev *__rest
         double V2 = pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3];
         double ener = pv[4] / pv[0] / (Gamma - F32C(1.0)) + V2 / F32C(2.0);
         cv[0] = pv[0]; cv[1] = pv[0] * pv[1]; cv[2] = pv[0] * pv[2]; cv[3] = pv[0] * pv[3]; cv[4] = pv[0] * ener;
+        if (B.ev) B.ev[xg] = B.ev[xi];   // source/BndCondOutlet.cpp:180-181, 282-283
     } else {
         // no-slip wall: mirror the relative velocity about the wall velocity
         const int f = hi ? N : 1;
@@ -129,6 +131,7 @@ __global__ void k_bc(const BlockDev *__restrict__ blocks, const PatchDev *__rest
         cv[4] = B.Q[4 * box + xi];
         for (int c = 0; c < 5; ++c) B.Q[c * box + xg] = cv[c];
         for (int c = 0; c < 6; ++c) B.W[c * box + xg] = pv[c];
+        if (B.ev) B.ev[xg] = F32C(0.0);   // source/BndCondWall.cpp:104-107
         double cd[5], pd[6];
         for (int c = 0; c < 5; ++c) cd[c] = (cv[c] + B.Q[c * box + xi]) * 0.5;
         for (int c = 0; c < 6; ++c) pd[c] = 0.5 * (pv[c] + B.W[c * box + xi]);
@@ -207,6 +210,7 @@ __global__ void __launch_bounds__(256) k_bc_axial(const BlockDev *__restrict__ b
         cv[4] = pv[4] / (Gamma - F32C(1.0)) + pv[0] * V2 / F32C(2.0);
         for (int c = 0; c < 5; ++c) B.Q[c * box + xg] = cv[c];
         for (int c = 0; c < 6; ++c) B.W[c * box + xg] = pv[c];
+        if (B.ev) B.ev[xg] = B.ev[xi];   // source/BndCondOutlet.cpp:102-103
         set_dummies(B, 5, i, j, cv, pv);
     }
 }
@@ -309,6 +313,7 @@ __global__ void k_exchange(const BlockDev *__restrict__ blocks, const ExchDev *_
     if (e.ghost >= 0) {
         for (int c = 0; c < 5; ++c) B.Q[c * B.box + e.ghost] = q[c];
         for (int c = 0; c < 6; ++c) B.W[c * B.box + e.ghost] = w[c];
+        if (B.ev) B.ev[e.ghost] = A.ev[e.src];   // nu~ of the ghost layer only (source/ShadowOutIn.cpp:339-345)
     }
 }
 
@@ -1687,9 +1692,13 @@ template __global__ void k_lusgs_kba2<false, kKbaTJ, kKbaTK>(const BlockDev *, c
 // Q += dQ (source/LU_SGS_Marching.cpp:262-287), Res_Calculate terms
 // (source/ComputeAllBlockVars.cpp:340-357), PrimitiveVars
 // (source/PrimitiveVars.cpp:7-29) and the Limit_Vars trigger test
-// (source/Limit_FlowVars.cpp:28-29).  One partial sum per workgroup.
+// (source/Limit_FlowVars.cpp:28-29).  One partial sum per workgroup.  With
+// Spalart-Allmaras also nu~ += dnu~ (source/LU_SGS_Marching.cpp:271-272) and,
+// when sa_hi is given, the flag of a block holding a value the limiter
+// replaces by its neighbours' mean (above 1000).
 __global__ void __launch_bounds__(256) k_update(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart, int nblock,
-                                                Gas g, double *__restrict__ partial, int *__restrict__ bad_flag)
+                                                Gas g, double *__restrict__ partial, int *__restrict__ bad_flag,
+                                                int *__restrict__ sa_hi)
 {
     __shared__ double red[5][256];
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1720,6 +1729,11 @@ __global__ void __launch_bounds__(256) k_update(const BlockDev *__restrict__ blo
         B.W[x] = d; B.W[box + x] = u; B.W[2 * box + x] = v; B.W[3 * box + x] = w; B.W[4 * box + x] = p; B.W[5 * box + x] = T;
         const double dmin = F32C(1.0E-6), dmax = F32C(1000.0), vmax = F32C(1000.0);
         if (d < dmin || d > dmax || p < dmin || p > dmax || u > vmax || v > vmax || w > vmax) atomicOr(bad_flag + n, 1);
+        if (B.ev) {
+            const double e = B.ev[x] + B.dev[x];
+            B.ev[x] = e;
+            if (sa_hi && e > F32C(1000.0)) atomicOr(sa_hi + n, 1);
+        }
     }
     for (int c = 0; c < 5; ++c) red[c][threadIdx.x] = rm[c];
     __syncthreads();
@@ -1848,19 +1862,22 @@ __global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ par
                                                   ScalarsDev *lv, ScalarsDev *ramp, Gas g, double *__restrict__ hist_row)
 {
     ScalarsDev *sc = lv;
-    __shared__ double red[5][256];
-    double r[5] = {0, 0, 0, 0, 0};
+    // component 5: Res_ev of Spalart-Allmaras (k_sa_res)
+    const int nc = g.turbulence == 200 ? 6 : 5;
+    __shared__ double red[6][256];
+    double r[6] = {0, 0, 0, 0, 0, 0};
     for (int p = threadIdx.x; p < npart; p += blockDim.x)
-        for (int c = 0; c < 5; ++c) r[c] += partial[(long)c * npart + p];
-    for (int c = 0; c < 5; ++c) red[c][threadIdx.x] = r[c];
+        for (int c = 0; c < nc; ++c) r[c] += partial[(long)c * npart + p];
+    for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = r[c];
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w)
-            for (int c = 0; c < 5; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+            for (int c = 0; c < 6; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
     for (int c = 0; c < 5; ++c) sc->res[c] = red[c][0] / sc->total_vol;
+    sc->res_ev = red[5][0] / sc->total_vol;
     const double Gamma = g.gamma, Rgas = g.rgas;
     // section partials in a fixed order: section 0 inlet, 1 outlet; mass, ptm, ttm
     double sums[6] = {0, 0, 0, 0, 0, 0};

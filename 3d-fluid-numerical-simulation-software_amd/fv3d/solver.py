@@ -83,6 +83,9 @@ class GpuSolver:
                 pv = np.ascontiguousarray(np.stack([st["rho"], st["u"], st["v"], st["w"], st["p"], st["T"]], -1))
                 e = np.ascontiguousarray(st["rhoE"])
                 self._check(self.L.fv3d_state_upload(self.h, b + 1, pv.ctypes.data, e.ctypes.data), "state_upload")
+                if case.cfg.turbulence == 200 and "nut" in st:
+                    ev = np.ascontiguousarray(st["nut"], dtype=np.float64)
+                    self._check(self.L.fv3d_nut_upload(self.h, b + 1, ev.ctypes.data), "nut_upload")
         else:
             self._check(self.L.fv3d_initial(self.h), "initial")
 
@@ -151,7 +154,7 @@ class GpuSolver:
         ni, nj, nk = im - 1, jm - 1, km - 1
         if name in ("cv", "pv"):
             shape = (im + 1, jm + 1, km + 1, 5 if name == "cv" else 6)
-        elif name == "mut":
+        elif name in ("mut", "ev"):
             shape = (im + 1, jm + 1, km + 1)
         elif name in ("res", "dq", "fv"):
             shape = (ni, nj, nk, 5)
@@ -159,7 +162,7 @@ class GpuSolver:
             shape = {"I": (im, nj, nk, 5), "J": (ni, jm, nk, 5), "K": (ni, nj, km, 5)}[name[2]]
         elif name in ("gradI", "gradJ", "gradK"):
             shape = {"I": (im, nj, nk, 12), "J": (ni, jm, nk, 12), "K": (ni, nj, km, 12)}[name[4]]
-        elif name in ("dt", "lamcI", "lamcJ", "lamcK", "lamvI", "lamvJ", "lamvK", "omega", "vol"):
+        elif name in ("dt", "lamcI", "lamcJ", "lamcK", "lamvI", "lamvJ", "lamvK", "omega", "vol", "resev", "dev", "dwall"):
             shape = (ni, nj, nk)
         elif name.startswith("d") and name[1] in "IJK":
             c = 6 if name.endswith("_pv") else 5

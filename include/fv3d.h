@@ -52,7 +52,8 @@ typedef struct {
     int recon_var;      /* 100 conservative, 200 primitive */
     int recon_scheme;   /* 100 NND, 200 MUSCL2u, 300 MUSCL2c, 400 MUSCL2o, 500 MUSCL3 */
     int flux_scheme;    /* 200 Steger-Warming, 300 Roe */
-    int turbulence;     /* 0 none (laminar), 100 Baldwin-Lomax (source/BL_Model.cpp) */
+    int turbulence;     /* 0 none (laminar), 100 Baldwin-Lomax (source/BL_Model.cpp),
+                           200 Spalart-Allmaras (source/SA_Model.cpp) */
     int limiter;        /* limit flow variables (source/Limit_FlowVars.cpp:8-119) */
     int global_dt;      /* 0 local time step, 1 global: the minimum over each block (source/TimeStep.cpp:64-86) */
     int device;         /* HIP device ordinal */
@@ -112,12 +113,20 @@ int fv3d_initial(fv3d_ctx *ctx);
  * pv (ni,nj,nk,6) = rho,u,v,w,p,T and ener (ni,nj,nk), i slowest. */
 int fv3d_state_upload(fv3d_ctx *ctx, int block, const double *pv, const double *ener);
 
+/* Restart-file nu~ of Spalart-Allmaras (the nut field CUserInput::ReadSolution
+ * reads, source/UserInput.cpp:1134-1153): ev (ni,nj,nk), i slowest.  With
+ * Spalart-Allmaras the iteration needs nu~ from here (every block) or from
+ * fv3d_initial (ev_ratio everywhere). */
+int fv3d_nut_upload(fv3d_ctx *ctx, int block, const double *ev);
+
 /* Copies a field to the host (synchronises).  Names and shapes:
  *   cv (IM+1,JM+1,KM+1,5), pv (IM+1,JM+1,KM+1,6)  -- cell arrays incl. ghost cells
  *   res dq (ni,nj,nk,5); dt lamcI lamcJ lamcK vol (ni,nj,nk)
  *   fcI (IM,nj,nk,5) fcJ (ni,JM,nk,5) fcK (ni,nj,KM,5)  -- face fluxes
- *   Navier-Stokes: lamvI lamvJ lamvK omega (ni,nj,nk); Baldwin-Lomax: mut
- *   (IM+1,JM+1,KM+1) incl. ghost cells; with debug bit 0 also
+ *   Navier-Stokes: lamvI lamvJ lamvK omega (ni,nj,nk); turbulence models: mut
+ *   (IM+1,JM+1,KM+1) incl. ghost cells; Spalart-Allmaras: ev (IM+1,JM+1,KM+1)
+ *   incl. ghost cells, resev dev dwall (ni,nj,nk) -- nu~ residual, its LU-SGS
+ *   increment and the wall distance; with debug bit 0 also
  *   fvI fvJ fvK (viscous face fluxes, shapes as fcI..), gradI gradJ gradK
  *   (face gradients, 12: d/dx, d/dy, d/dz of u v w T) and fv (ni,nj,nk,5)
  *   dIS_cv dIP_cv (2,nj,nk,5), dJS_cv dJP_cv (ni,2,nk,5), dKS_cv dKP_cv (ni,nj,2,5)
