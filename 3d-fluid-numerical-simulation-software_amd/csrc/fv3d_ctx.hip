@@ -322,6 +322,7 @@ static int setup_block(fv3d_ctx *c, HostBlock &b, int index)
         if ((rc = alloc(&B.dev, B.box))) return rc;
         if ((rc = alloc(&B.res_ev, B.box))) return rc;
         if ((rc = alloc(&B.dwall, B.box))) return rc;
+        if ((rc = alloc(&B.alpha2, B.box))) return rc;
     }
     if (c->cfg.turbulence == 100) {
         unsigned long long *f;
@@ -374,7 +375,7 @@ static int setup_block(fv3d_ctx *c, HostBlock &b, int index)
         b.sweep_rows = ntile * B.nr;
         b.pack_groups = ntile * ((B.ni + kPackChunk - 1) / kPackChunk);
         if ((rc = alloc(&B.srow, b.sweep_rows * kSweepQ * 64))) return rc;
-        if ((rc = alloc(&B.sdq, b.sweep_rows * 5 * 64))) return rc;
+        if ((rc = alloc(&B.sdq, b.sweep_rows * kSweepD * 64))) return rc;
         // the mailboxes of all blocks share one allocation (fv3d_setup), so
         // one fill resets them each iteration
     }
@@ -1359,7 +1360,8 @@ static int launch_curl(fv3d_ctx *c)
 // hyperplane i + j + k, after the flow sweep of that direction has left its
 // increments in dQ (the reference interleaves the two per cell; the nu~ update
 // of a cell reads the flow increments of its upstream neighbours only, which
-// that flow sweep has finished).
+// that flow sweep has finished).  The comparison sweep forms use it; the
+// default row-layout sweep carries nu~ itself.
 static void launch_sa_sweep(fv3d_ctx *c, bool fwd)
 {
     if (c->cfg.turbulence != 200) return;
@@ -1385,6 +1387,7 @@ static int launch_sweeps_kba(fv3d_ctx *c)
     if (c->sweep_mode == 4) {
         const bool f32 = c->cfg.precision == FV3D_DEFECT_FP32;
         const bool visc = c->cfg.viscous;
+        const bool sa = c->cfg.turbulence == 200;   // the nu~ equation inside the sweep
         const int nb = (int)c->blocks.size();
         const unsigned pg = (unsigned)c->pack_groups;
         if (f32) {
@@ -1405,16 +1408,23 @@ static int launch_sweeps_kba(fv3d_ctx *c)
             int *tk = fwd ? tf : tb;
             const KbaTile *tl = c->d_kba[fwd ? 0 : 1];
             dim3 g((unsigned)c->kba_grid), b(256);
-#define FV3D_SL_LAUNCH(F, V, T) \
-    hipLaunchKernelGGL((k_lusgs_sl<F, kKbaTJ, kKbaTK, V, T>), g, b, 0, c->stream, c->d_blocks, tl, c->nkba, tk, err, \
+#define FV3D_SL_LAUNCH(F, V, S, T) \
+    hipLaunchKernelGGL((k_lusgs_sl<F, kKbaTJ, kKbaTK, V, S, T>), g, b, 0, c->stream, c->d_blocks, tl, c->nkba, tk, err, \
                        c->gas.relax, c->gas)
+#define FV3D_SL_FORMS(F, T)                      \
+    do {                                         \
+        if (sa) FV3D_SL_LAUNCH(F, true, true, T);  \
+        else if (visc) FV3D_SL_LAUNCH(F, true, false, T); \
+        else FV3D_SL_LAUNCH(F, false, false, T); \
+    } while (0)
             if (fwd) {
-                if (f32) { if (visc) FV3D_SL_LAUNCH(true, true, float); else FV3D_SL_LAUNCH(true, false, float); }
-                else { if (visc) FV3D_SL_LAUNCH(true, true, double); else FV3D_SL_LAUNCH(true, false, double); }
+                if (f32) FV3D_SL_FORMS(true, float);
+                else FV3D_SL_FORMS(true, double);
             } else {
-                if (f32) { if (visc) FV3D_SL_LAUNCH(false, true, float); else FV3D_SL_LAUNCH(false, false, float); }
-                else { if (visc) FV3D_SL_LAUNCH(false, true, double); else FV3D_SL_LAUNCH(false, false, double); }
+                if (f32) FV3D_SL_FORMS(false, float);
+                else FV3D_SL_FORMS(false, double);
             }
+#undef FV3D_SL_FORMS
 #undef FV3D_SL_LAUNCH
         };
         auto unpack = [&]() {
@@ -1425,16 +1435,10 @@ static int launch_sweeps_kba(fv3d_ctx *c)
         };
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[6], c->stream));
         sweep(true);
-        if (c->cfg.turbulence == 200) {
-            // the forward increments in dQ for the nu~ sweep
-            unpack();
-            launch_sa_sweep(c, true);
-        }
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[7], c->stream));
         sweep(false);
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[8], c->stream));
         unpack();
-        launch_sa_sweep(c, false);
     } else if (c->sweep_mode == 3) {
         hipLaunchKernelGGL((k_lusgs_kba3<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
                            c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_lag);

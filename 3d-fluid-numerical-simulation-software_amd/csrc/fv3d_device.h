@@ -68,6 +68,7 @@ struct BlockDev {
     double *dev;
     double *res_ev;
     double *dwall;
+    double *alpha2;           // nu~ LU-SGS diagonal Vol/dt + relax*sum(lamc) + 3*sum(lamv)
     double *dt;               // local time step
     double *alpha;            // LU-SGS diagonal Vol/dt + relax*sum(lamc) + 2*sum(lamv)
     const double *swg;        // 15*box static sweep metrics per cell: for d = 0..2,
@@ -81,7 +82,7 @@ struct BlockDev {
     // cell's place in the tile), so every cell a wavefront step touches lies
     // in one row and its three upstream neighbours in the row before.
     double *srow;             // per row: kSweepQ quantities x 64 lanes
-    double *sdq;              // per row: dQ, 5 components x 64 lanes
+    double *sdq;              // per row: dQ (5) and dnu~, kSweepD components x 64 lanes
     int ntk;                  // tiles along k
     int nr;                   // rows per tile = ni + kKbaTJ + kKbaTK - 2
     // Edge mailboxes of the row-layout sweeps (forward, backward): per tile

@@ -77,8 +77,12 @@ struct KbaTile {
 constexpr int kKbaTJ = 4, kKbaTK = 16;
 constexpr int kDonePad = 32;   // progress counters one per 128-byte line
 // quantities of a sweep-layout row (BlockDev::srow): Q (5), p, lamc (3),
-// swg (15), R (5), alpha, lamv (3, Navier-Stokes only)
-constexpr int kSQ_Q = 0, kSQ_P = 5, kSQ_LAM = 6, kSQ_SWG = 9, kSQ_R = 24, kSQ_A = 29, kSQ_LV = 30, kSweepQ = 33;
+// swg (15), R (5), alpha, lamv (3, Navier-Stokes only), and with
+// Spalart-Allmaras nu~, its residual and its LU-SGS diagonal alpha2
+constexpr int kSQ_Q = 0, kSQ_P = 5, kSQ_LAM = 6, kSQ_SWG = 9, kSQ_R = 24, kSQ_A = 29, kSQ_LV = 30, kSQ_EV = 33,
+              kSQ_REV = 34, kSQ_A2 = 35, kSweepQ = 36;
+// components of a dQ row (BlockDev::sdq): dQ (5) and dnu~ (Spalart-Allmaras)
+constexpr int kSweepD = 6;
 // what k_sweep_fill writes: the state quantities and/or the static metrics
 constexpr int kGatherState = 1, kGatherMetrics = 2;
 template <bool FWD, int TJ, int TK>
@@ -91,12 +95,14 @@ template <bool FWD, int TJ, int TK>
 __global__ void k_lusgs_kba3(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *done, int *err,
                              double relax, Gas g, int lag);
 // T: double (plain FP64) or float (the FP32 operator of defect correction)
-template <bool FWD, int TJ, int TK, bool VISC, class T>
+// SA: the nu~ equation in the same sweep (Navier-Stokes only)
+template <bool FWD, int TJ, int TK, bool VISC, bool SA, class T>
 __global__ void k_lusgs_sl(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *err, double relax,
                            Gas g);
-// mailbox words per tile row, and the bit pattern of an empty word (a NaN
-// payload no arithmetic produces)
-constexpr int kMboxJ = 5 * kKbaTK, kMboxRow = 5 * (kKbaTK + kKbaTJ);
+// mailbox words per tile row (kSweepD components of the j-edge lanes, then of
+// the k-edge lanes), and the bit pattern of an empty word (a NaN payload no
+// arithmetic produces)
+constexpr int kMboxJ = kSweepD * kKbaTK, kMboxRow = kSweepD * (kKbaTK + kKbaTJ);
 constexpr unsigned long long kMboxEmpty = 0x7ff4dead5eed0001ull;
 constexpr unsigned int kMboxEmpty32 = 0x7fa5eed1u;   // the same for FP32 words
 __global__ void k_fill_u64(unsigned long long *p, long n, unsigned long long v);

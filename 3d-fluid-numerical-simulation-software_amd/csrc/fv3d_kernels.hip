@@ -270,6 +270,9 @@ __global__ void __launch_bounds__(256) k_dt_global(const BlockDev *__restrict__ 
         for (int d = 0; d < 3; ++d) lvd[d] = B.lamv[d * box + x];
     B.alpha[x] = B.vol[x] / dt + g.relax * (B.lamc[x] + B.lamc[box + x] + B.lamc[2 * box + x]) +
                  F32C(2.0) * (lvd[0] + lvd[1] + lvd[2]);
+    if (B.alpha2)
+        B.alpha2[x] = B.vol[x] / dt + g.relax * (B.lamc[x] + B.lamc[box + x] + B.lamc[2 * box + x]) +
+                      F32C(3.0) * (lvd[0] + lvd[1] + lvd[2]);
 }
 
 // ================================================== interior / periodic
@@ -369,6 +372,9 @@ __global__ void k_spectral_dt(const BlockDev *__restrict__ blocks, const long *_
     B.dt[x] = dt;
     // diagonal of the LU-SGS operator (source/LU_SGS_Marching.cpp:76-77)
     B.alpha[x] = B.vol[x] / dt + g.relax * (lam[0] + lam[1] + lam[2]) + F32C(2.0) * (lvd[0] + lvd[1] + lvd[2]);
+    // and of its nu~ equation (source/LU_SGS_Marching.cpp:81-82)
+    if (B.alpha2)
+        B.alpha2[x] = B.vol[x] / dt + g.relax * (lam[0] + lam[1] + lam[2]) + F32C(3.0) * (lvd[0] + lvd[1] + lvd[2]);
 }
 
 // Decrement of the time-step damping counter at the end of CTimeDiscr::TimeStep
@@ -826,8 +832,9 @@ template <class T>
 struct RowPf {
     T q[5], m[5], of[5], vn_old, cl;   // state, metrics, old-state flux terms
     T p, l, lv;
-    T x[5];
+    T x[kSweepD];                      // upstream tile's dQ (and dnu~)
     T r[5], a;
+    T ev, rev, a2;                     // Spalart-Allmaras: neighbour's nu~; own nu~ right-hand side and diagonal
 };
 template <int D>
 struct Dir {
@@ -1092,6 +1099,9 @@ __global__ void __launch_bounds__(256) k_sweep_fill(const BlockDev *__restrict__
         for (int c = 0; c < 5; ++c) dst[(kSQ_R + c) * 64] = valid ? B.R[c * box + x] : 0.0;
         dst[kSQ_A * 64] = valid ? B.alpha[x] : 1.0;
         for (int d = 0; d < 3; ++d) dst[(kSQ_LV + d) * 64] = (valid && B.lamv) ? B.lamv[d * box + x] : 0.0;
+        dst[kSQ_EV * 64] = (valid && B.ev) ? B.ev[x] : 0.0;
+        dst[kSQ_REV * 64] = (valid && B.res_ev) ? B.res_ev[x] : 0.0;
+        dst[kSQ_A2 * 64] = (valid && B.alpha2) ? B.alpha2[x] : 1.0;
     }
     if (what & kGatherMetrics)
         for (int q = 0; q < 15; ++q) dst[(kSQ_SWG + q) * 64] = valid ? B.swg[q * box + x] : 0.0;
@@ -1119,7 +1129,7 @@ __global__ void __launch_bounds__(256) k_sweep_pack(const BlockDev *__restrict__
     const int tj = lane / TK, tk = lane % TK, j = J * TJ + tj + 1, k = K * TK + tk + 1;
     const bool col = j <= B.nj && k <= B.nk;
     const long box = B.box;
-    const int NQ = B.lamv ? 18 : 15;   // lamv rows for Navier-Stokes
+    const int NQ = B.ev ? 21 : B.lamv ? 18 : 15;   // lamv rows for Navier-Stokes, nu~ rows for Spalart-Allmaras
     // the cell-box array and row quantity of each packed quantity
     for (int qi = 0; qi < NQ; ++qi) {
         const double *src;
@@ -1129,7 +1139,10 @@ __global__ void __launch_bounds__(256) k_sweep_pack(const BlockDev *__restrict__
         else if (qi < 9) { src = B.lamc + (qi - 6) * box; rq = kSQ_LAM + qi - 6; }
         else if (qi < 14) { src = B.R + (qi - 9) * box; rq = kSQ_R + qi - 9; }
         else if (qi == 14) { src = B.alpha; rq = kSQ_A; }
-        else { src = B.lamv + (qi - 15) * box; rq = kSQ_LV + qi - 15; }
+        else if (qi < 18) { src = B.lamv + (qi - 15) * box; rq = kSQ_LV + qi - 15; }
+        else if (qi == 18) { src = B.ev; rq = kSQ_EV; }
+        else if (qi == 19) { src = B.res_ev; rq = kSQ_REV; }
+        else { src = B.alpha2; rq = kSQ_A2; }
         for (int ci = w; ci < kPackChunk; ci += 4) {
             const int i = i0 + ci;
             if (col && i <= B.ni) buf[ci][lane] = src[cidx(B, i, j, k)];
@@ -1163,16 +1176,18 @@ __global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict
     const int tj = lane / TK, tk = lane % TK, j = J * TJ + tj + 1, k = K * TK + tk + 1;
     const bool col = j <= B.nj && k <= B.nk;
     const long box = B.box;
-    for (int c = 0; c < 5; ++c) {
+    const int NC = B.ev ? 6 : 5;   // dnu~ with Spalart-Allmaras
+    for (int c = 0; c < NC; ++c) {
         for (int rr = w; rr < kPackChunk + TJ + TK - 2; rr += 4) {
             const int r = i0 - 1 + rr, i = r + 1 - tj - tk;
             if (col && i >= i0 && i < i0 + kPackChunk && i <= B.ni)
-                buf[i - i0][lane] = (double)((const T *)B.sdq)[((long)slot * B.nr + r) * (5 * 64) + c * 64 + lane];
+                buf[i - i0][lane] = (double)((const T *)B.sdq)[((long)slot * B.nr + r) * (kSweepD * 64) + c * 64 + lane];
         }
         __syncthreads();
+        double *dst = c < 5 ? B.dQ + c * box : B.dev;
         for (int ci = w; ci < kPackChunk; ci += 4) {
             const int i = i0 + ci;
-            if (col && i <= B.ni) B.dQ[c * box + cidx(B, i, j, k)] = buf[ci][lane];
+            if (col && i <= B.ni) dst[cidx(B, i, j, k)] = buf[ci][lane];
         }
         __syncthreads();
     }
@@ -1197,24 +1212,33 @@ template __global__ void k_sweep_unpack<float>(const BlockDev *, const long *, i
 // and the consumer reads the words a step ahead and, before using them, waits
 // (bounded) while any word it needs still holds the empty pattern.  Tiles
 // start in ticket order, so every upstream tile is already running.
-template <bool FWD, int TJ, int TK, bool VISC, class T>
+//
+// With SA the nu~ equation of Spalart-Allmaras rides along as a sixth
+// component (source/LU_SGS_Marching.cpp:88-157, 178-257): the direction waves
+// form (nu~ + dnu~) Vn_new - nu~ Vn_old of their neighbour from the same Vn
+// as the flow terms, and the combine wave divides by alpha2.  The reference
+// updates both per cell from the same neighbour increments, so this is its
+// order.
+template <bool FWD, int TJ, int TK, bool VISC, bool SA, class T>
 __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ blocks, const KbaTile *__restrict__ tiles,
                                                   int ntiles, int *__restrict__ ticket, int *__restrict__ err,
                                                   double relax, Gas g)
 {
     static_assert(TJ * TK == 64, "64 cells per tile");
+    static_assert(!SA || VISC, "Spalart-Allmaras runs with the viscous terms");
     // The FP32 operator forms the flux difference without cancellation; the
     // FP64 one follows the reference's expression order.
     constexpr bool kDeltaForm = sizeof(T) == 4;
-    __shared__ T sdq[2][5][64];
-    __shared__ T sA[3][5][64];
-    __shared__ T sV[3][5][64];
-    __shared__ T sR[6][64];
+    constexpr int NC = SA ? 6 : 5;   // increments carried per cell
+    __shared__ T sdq[2][NC][64];
+    __shared__ T sA[3][NC][64];
+    __shared__ T sV[3][NC][64];
+    __shared__ T sR[SA ? 8 : 6][64];   // R, alpha; with SA the nu~ right-hand side and alpha2
     __shared__ int s_q;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     const int tj = lane / TK, tk = lane % TK;
     const T Gamma = (T)g.gamma;
-    constexpr int RQ = kSweepQ * 64, RD = 5 * 64;   // words per row
+    constexpr int RQ = kSweepQ * 64, RD = kSweepD * 64;   // words per row
     for (;;) {
         if (threadIdx.x == 0) s_q = atomicAdd(ticket, 1);
         __syncthreads();
@@ -1270,7 +1294,7 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
             auto issue_cross = [&](RowPf<T> &bn, int step) {
                 if (d == 0) return;
                 const int urow = clampr(FWD ? row_of(step) + TE - 1 : row_of(step) - (TE - 1));
-                for (int c = 0; c < 5; ++c)
+                for (int c = 0; c < NC; ++c)
                     bn.x[c] = __hip_atomic_load(umbox + urow * kMboxRow + mword + c * mstride, __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_AGENT);
             };
@@ -1280,7 +1304,7 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                 if (d == 0) return;
                 auto empty = [&](const T *x) {
                     bool e = false;
-                    for (int c = 0; c < 5; ++c) e = e || (mbox_empty(x[c]));
+                    for (int c = 0; c < NC; ++c) e = e || (mbox_empty(x[c]));
                     return e;
                 };
                 if (!__builtin_amdgcn_ballot_w64(need && empty(b.x))) return;
@@ -1288,7 +1312,7 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                 long spins = 0;
                 while (__builtin_amdgcn_ballot_w64(need && empty(b.x))) {
                     __builtin_amdgcn_s_sleep(1);
-                    for (int c = 0; c < 5; ++c)
+                    for (int c = 0; c < NC; ++c)
                         b.x[c] = __hip_atomic_load(umbox + urow * kMboxRow + mword + c * mstride, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_AGENT);
                     if (++spins > (1L << 22)) { atomicExch(err, 1); break; }
@@ -1306,11 +1330,17 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                 bn.l = src[(kSQ_LAM + d) * 64];
                 if (VISC) bn.lv = src[(kSQ_LV + d) * 64];
                 for (int c = 0; c < 5; ++c) bn.m[c] = src[(kSQ_SWG + d * 5 + c) * 64];
+                if constexpr (SA) bn.ev = src[kSQ_EV * 64];
                 if (d == 0) {
                     const int ro = clampr(rt);
                     for (int c = 0; c < 5; ++c)
                         bn.r[c] = FWD ? rows[ro * RQ + (kSQ_R + c) * 64 + lane] : ((gcT<T>)drows)[ro * RD + c * 64 + lane];
                     bn.a = rows[ro * RQ + kSQ_A * 64 + lane];
+                    if constexpr (SA) {
+                        // forward: Res_ev; backward: the forward dnu~
+                        bn.rev = FWD ? rows[ro * RQ + kSQ_REV * 64 + lane] : ((gcT<T>)drows)[ro * RD + 5 * 64 + lane];
+                        bn.a2 = rows[ro * RQ + kSQ_A2 * 64 + lane];
+                    }
                 }
             };
             // the terms of the flux difference that use only the old state
@@ -1346,13 +1376,14 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                 const int p = d == 0 ? i : d == 1 ? j : k, lim = d == 0 ? ni : d == 1 ? nj : nk;
                 const bool use = valid && (FWD ? p != 1 : p != lim);
                 settle_cross(b, s, use && from_up);
-                T dqn[5];
-                for (int c = 0; c < 5; ++c) {
+                T dqn[NC];
+                for (int c = 0; c < NC; ++c) {
                     const T in_tile = sdq[(s - 1) & 1][c][nlane];
                     dqn[c] = (d != 0 && edge) ? b.x[c] : in_tile;
                 }
                 const T nx = b.m[0], ny = b.m[1], nz = b.m[2], sd = b.m[3], Vn_e = b.m[4];
                 T df[5];
+                T vn_new = T(0), dvn = T(0);   // the neighbour's new normal velocity and its change
                 if constexpr (kDeltaForm) {
                     // F_n(Q+dQ) - F_n(Q) written as sums of products with a
                     // difference factor, so its rounding error scales with dQ:
@@ -1370,6 +1401,8 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                     const T dp = (Gamma - TC(1.0)) *
                                  (dqn[4] - TC(0.5) * ((q[1] * du + q[2] * dv + q[3] * dw) +
                                                       (dqn[1] * u1 + dqn[2] * v1 + dqn[3] * w1)));
+                    vn_new = b.vn_old + dVn;
+                    dvn = dVn;
                     df[0] = dqn[0] * wn + q[0] * dVn;
                     df[1] = dqn[1] * wn + q[1] * dVn + dp * nx;
                     df[2] = dqn[2] * wn + q[2] * dVn + dp * ny;
@@ -1381,6 +1414,7 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                     T dens = nw[0], uvel = nw[1] / nw[0], vvel = nw[2] / nw[0], wvel = nw[3] / nw[0], E = nw[4] / nw[0];
                     T press_new = (Gamma - TC(1.0)) * dens * (E - TC(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel));
                     T Vn_new = uvel * nx + vvel * ny + wvel * nz;
+                    vn_new = Vn_new;
                     df[0] = nw[0] * (Vn_new - Vn_e) - b.of[0];
                     df[1] = (nw[1] * (Vn_new - Vn_e) + press_new * nx) - b.of[1];
                     df[2] = (nw[2] * (Vn_new - Vn_e) + press_new * ny) - b.of[2];
@@ -1395,9 +1429,29 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                     V[c] = use ? v : T(0);
                 }
                 for (int c = 0; c < 5; ++c) { sA[d][c][lane] = A[c]; sV[d][c][lane] = V[c]; }
+                if constexpr (SA) {
+                    // (nu~ + dnu~) Vn_new - nu~ Vn_old (source/LU_SGS_Marching.cpp:351-353); the
+                    // FP32 operator writes it as nu~ dVn + dnu~ Vn_new
+                    const T dev = dqn[5];
+                    T dfe;
+                    if constexpr (kDeltaForm) {
+                        dfe = b.ev * dvn + dev * vn_new;
+                    } else {
+                        (void)dvn;
+                        const T ev_new = b.ev + dev;
+                        dfe = ev_new * vn_new - b.ev * b.vn_old;
+                    }
+                    const T a = FWD ? (dfe * sd + b.cl * dev) * TC(0.5) : (dfe * sd - b.cl * dev) * TC(0.5);
+                    sA[d][5][lane] = use ? a : T(0);
+                    sV[d][5][lane] = use ? b.lv * dev : T(0);
+                }
                 if (d == 0) {
                     for (int c = 0; c < 5; ++c) sR[c][lane] = b.r[c];
                     sR[5][lane] = b.a;
+                    if constexpr (SA) {
+                        sR[6][lane] = b.rev;
+                        sR[7][lane] = b.a2;
+                    }
                 }
                 bar();   // terms of step s are in LDS
                 issue_cross(b1, s + 1);
@@ -1432,7 +1486,7 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                 int i;
                 const bool valid = cell_at(s, i);
                 const int ro = clampr(row_of(s));
-                T outv[5];
+                T outv[NC];
                 bar();
                 if (valid) {
                     const T alpha = sR[5][lane];
@@ -1449,35 +1503,54 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                         outv[c] = v / alpha;
                         sdq[s & 1][c][lane] = outv[c];
                     }
+                    if constexpr (SA) {
+                        // source/LU_SGS_Marching.cpp:156-157, 256-257
+                        const T alpha2 = sR[7][lane];
+                        T v;
+                        if (FWD) {
+                            v = sR[6][lane] * TC(-1.0) + sA[0][5][lane];
+                            v = v + sA[1][5][lane]; v = v + sA[2][5][lane];
+                        } else {
+                            v = sR[6][lane] * alpha2 - sA[0][5][lane];
+                            v = v - sA[1][5][lane]; v = v - sA[2][5][lane];
+                        }
+                        v = v + sV[0][5][lane]; v = v + sV[1][5][lane]; v = v + sV[2][5][lane];
+                        outv[5] = v / alpha2;
+                        sdq[s & 1][5][lane] = outv[5];
+                    }
                 }
                 bar();
                 if (valid) {
                     // hand the edge values to the downstream tiles, then keep dQ
                     if (jedge)
-                        for (int c = 0; c < 5; ++c)
+                        for (int c = 0; c < NC; ++c)
                             __hip_atomic_store(mbox + ro * kMboxRow + tk + c * TK, outv[c], __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_AGENT);
                     if (kedge)
-                        for (int c = 0; c < 5; ++c)
+                        for (int c = 0; c < NC; ++c)
                             __hip_atomic_store(mbox + ro * kMboxRow + kMboxJ + tj + c * TJ, outv[c], __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_AGENT);
-                    for (int c = 0; c < 5; ++c) drows[ro * RD + c * 64 + lane] = outv[c];
+                    for (int c = 0; c < NC; ++c) drows[ro * RD + c * 64 + lane] = outv[c];
                 }
             }
         }
     }
 }
 
-#define FV3D_SL(F, V, T) \
-    template __global__ void k_lusgs_sl<F, kKbaTJ, kKbaTK, V, T>(const BlockDev *, const KbaTile *, int, int *, int *, double, Gas);
-FV3D_SL(true, false, double)
-FV3D_SL(false, false, double)
-FV3D_SL(true, true, double)
-FV3D_SL(false, true, double)
-FV3D_SL(true, false, float)
-FV3D_SL(false, false, float)
-FV3D_SL(true, true, float)
-FV3D_SL(false, true, float)
+#define FV3D_SL(F, V, S, T) \
+    template __global__ void k_lusgs_sl<F, kKbaTJ, kKbaTK, V, S, T>(const BlockDev *, const KbaTile *, int, int *, int *, double, Gas);
+FV3D_SL(true, false, false, double)
+FV3D_SL(false, false, false, double)
+FV3D_SL(true, true, false, double)
+FV3D_SL(false, true, false, double)
+FV3D_SL(true, true, true, double)
+FV3D_SL(false, true, true, double)
+FV3D_SL(true, false, false, float)
+FV3D_SL(false, false, false, float)
+FV3D_SL(true, true, false, float)
+FV3D_SL(false, true, false, float)
+FV3D_SL(true, true, true, float)
+FV3D_SL(false, true, true, float)
 #undef FV3D_SL
 
 __global__ void k_fill_u64(unsigned long long *p, long n, unsigned long long v)

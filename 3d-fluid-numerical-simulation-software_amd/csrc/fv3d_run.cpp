@@ -62,7 +62,11 @@ void print_summary(const fv3d::CaseConfig &cc, const fv3d::BndTable &bnd, const 
         std::printf("%s[%d, %d, %d, %.17g]", b ? ", " : "", dims[b][0], dims[b][1], dims[b][2], checksum(nodes[b]));
     std::printf("],\n \"states\": [");
     for (size_t b = 0; b < st.size(); ++b)
-        std::printf("%s[%.17g, %.17g]", b ? ", " : "", checksum(st[b].pv), checksum(st[b].ener));
+    {
+        std::printf("%s[%.17g, %.17g", b ? ", " : "", checksum(st[b].pv), checksum(st[b].ener));
+        if (!st[b].ev.empty()) std::printf(", %.17g", checksum(st[b].ev));
+        std::printf("]");
+    }
     std::printf("]}\n");
 }
 
@@ -92,7 +96,8 @@ int main(int argc, char **argv)
         std::vector<std::array<int, 3>> dims;
         std::vector<std::vector<double>> nodes = fv3d::ReadGrids(dir + "/" + cc.grid_file, &dims);
         std::vector<fv3d::BlockState> st;
-        if (cc.reuse_solution) st = fv3d::ReadSolution(dir + "/RestartSolution.dat", dims);
+        if (cc.reuse_solution)
+            st = fv3d::ReadSolution(dir + "/RestartSolution.dat", dims, cc.raw.turbulence == 200);
         if (parse_only) {
             print_summary(cc, bnd, dims, nodes, st);
             return 0;

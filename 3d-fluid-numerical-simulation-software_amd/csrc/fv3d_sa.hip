@@ -53,7 +53,10 @@ __device__ static inline double sa_clip(double ev, double temp, double dens, con
 // smallest distance from its centroid to the centroid of any wall face, from
 // 10000.  wf holds the wall-face centroids (x, y, z); each workgroup stages
 // 256 of them at a time in LDS.  FP64 throughout: this is the arithmetic-bound
-// set-up kernel (cells x wall faces distance evaluations).
+// set-up kernel (cells x wall faces distance evaluations).  The minimum is
+// taken over the squared distances and rooted once: the correctly rounded
+// square root is monotone, so sqrt(min(d^2, 10000^2)) is the reference's
+// running minimum of the roots started from 10000.
 __global__ void __launch_bounds__(256) k_wall_distance(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
                                                        int nblock, const double *__restrict__ wf, int nf)
 {
@@ -71,7 +74,7 @@ __global__ void __launch_bounds__(256) k_wall_distance(const BlockDev *__restric
         x = cidx(B, i, j, k);
         cx = B.cc[x]; cy = B.cc[B.box + x]; cz = B.cc[2 * B.box + x];
     }
-    double dmin = 10000.0;
+    double d2min = 10000.0 * 10000.0;
     for (int base = 0; base < nf; base += 256) {
         __syncthreads();
         const int q = base + (int)threadIdx.x;
@@ -84,11 +87,10 @@ __global__ void __launch_bounds__(256) k_wall_distance(const BlockDev *__restric
         const int m = min(256, nf - base);
         for (int e = 0; e < m; ++e) {
             const double dx = cx - sx[e], dy = cy - sy[e], dz = cz - sz[e];
-            const double d = sqrt(dx * dx + dy * dy + dz * dz);
-            if (dmin >= d) dmin = d;
+            d2min = fmin(d2min, dx * dx + dy * dy + dz * dz);
         }
     }
-    if (valid) blocks[n].dwall[x] = dmin;
+    if (valid) blocks[n].dwall[x] = sqrt(d2min);
 }
 
 // ------------------------------------------------------------ nu~ residual
@@ -317,8 +319,7 @@ __global__ void k_sa_plane(const BlockDev *__restrict__ blocks, const long *__re
     const int k = plane - i - j;
     if (k < 1 || k > B.nk) return;
     const long box = B.box, x = cidx(B, i, j, k);
-    const double alpha2 = B.vol[x] / B.dt[x] + relax * (B.lamc[x] + B.lamc[box + x] + B.lamc[2 * box + x]) +
-                          F32C(3.0) * (B.lamv[x] + B.lamv[box + x] + B.lamv[2 * box + x]);
+    const double alpha2 = B.alpha2[x];   // Vol/dt + relax*sum(lamc) + 3*sum(lamv), from k_spectral_dt
     const long st[3] = {B.si, B.sj, 1};
     const int pos[3] = {i, j, k};
     const int lim[3] = {B.ni, B.nj, B.nk};

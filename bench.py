@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--nk", type=int, default=256, help="bump workload: cells along k (streamwise)")
     ap.add_argument("--viscous", action="store_true", help="bump workload: laminar Navier-Stokes")
     ap.add_argument("--single-grid", action="store_true", help="passage workload without the two-grid cycle")
+    ap.add_argument("--turbulence", choices=["bl", "sa"], default="bl",
+                    help="passage workload: Baldwin-Lomax or Spalart-Allmaras")
     ap.add_argument("--coarse-iter", type=int, default=2, help="coarse start-up iterations of the two-grid cycle")
     ap.add_argument("--precision", choices=["fp32", "fp64", "both"], default="both",
                     help="which forms to time (the chosen form is fp32)")
@@ -127,6 +129,9 @@ def workload(a):
                                  camber=0.1, viscous=True, len_ref=1.0, max_iter=a.steps,
                                  z_extent=(-0.06, -0.035, -0.015, 0.035, 0.055, 0.08))
     case.cfg.turbulence = 100
+    if a.turbulence == "sa":
+        # nu~ = ev_ratio (the restart field of the single grid; fv3d_initial's with two levels)
+        case = cases.with_sa(case, amp=0.0)
     levels = "single grid level"
     if not a.single_grid:
         # two-grid cycle from the uniform start field on the coarse level
@@ -138,9 +143,10 @@ def workload(a):
                   f"the uniform start, then fine LU-SGS + forced coarse LU-SGS per iteration)")
     name = ("five-block annular blade passage, 64x64 cross-section, k-blocks 12/40/128/60/24 (1,081,344 cells; "
             "block sizes 49k-524k), hub 0.2 m tip 0.25 m, 22 blades, cambered zero-thickness blade walls, "
-            "periodic pitch and interior patches, Navier-Stokes + Baldwin-Lomax, Roe + MUSCL3 primitive, LU-SGS "
+            "periodic pitch and interior patches, Navier-Stokes + "
+            f"{'Spalart-Allmaras' if a.turbulence == 'sa' else 'Baldwin-Lomax'}, Roe + MUSCL3 primitive, LU-SGS "
             f"local dt CFL 5, {levels}, stationary")
-    return case, name, "passage5" + ("" if a.single_grid else "mg"), True
+    return case, name, "passage5" + ("" if a.single_grid else "mg") + ("sa" if a.turbulence == "sa" else ""), True
 
 
 def time_form(case, precision, a):

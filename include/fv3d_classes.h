@@ -227,13 +227,16 @@ inline std::vector<std::vector<double>> ReadGrids(const std::string &path, std::
 }
 
 // RestartSolution.dat (source/UserInput.cpp:1118-1159): rows of
-// "block i j k p u v w T rho rhoE".  Returned per block as the
-// fv3d_state_upload pair pv (ni,nj,nk,6 = rho,u,v,w,p,T) and ener (ni,nj,nk).
+// "block i j k p u v w T rho rhoE", with Spalart-Allmaras (with_nut) one more
+// column nu~.  Returned per block as the fv3d_state_upload pair pv
+// (ni,nj,nk,6 = rho,u,v,w,p,T) and ener (ni,nj,nk), and ev (ni,nj,nk) for
+// fv3d_nut_upload (empty without with_nut).
 struct BlockState {
-    std::vector<double> pv, ener;
+    std::vector<double> pv, ener, ev;
 };
 
-inline std::vector<BlockState> ReadSolution(const std::string &path, const std::vector<std::array<int, 3>> &dims)
+inline std::vector<BlockState> ReadSolution(const std::string &path, const std::vector<std::array<int, 3>> &dims,
+                                            bool with_nut = false)
 {
     std::ifstream f(path);
     if (!f) throw Error(FV3D_EINVAL, "cannot open " + path);
@@ -242,13 +245,15 @@ inline std::vector<BlockState> ReadSolution(const std::string &path, const std::
         size_t n = (size_t)(dims[b][0] - 1) * (dims[b][1] - 1) * (dims[b][2] - 1);
         out[b].pv.assign(n * 6, 0.0);
         out[b].ener.assign(n, 0.0);
+        if (with_nut) out[b].ev.assign(n, 0.0);
     }
     std::string s;
     while (std::getline(f, s)) {
         std::istringstream is(s);
         int b, i, j, k;
-        double p, u, v, w, T, rho, rhoE;
+        double p, u, v, w, T, rho, rhoE, ev = 0.0;
         if (!(is >> b >> i >> j >> k >> p >> u >> v >> w >> T >> rho >> rhoE)) continue;
+        if (with_nut && !(is >> ev)) throw Error(FV3D_EINVAL, path + ": a row without the nu~ column");
         if (b < 1 || b > (int)dims.size()) throw Error(FV3D_EINVAL, path + ": block id out of range");
         const int nj = dims[b - 1][1] - 1, nk = dims[b - 1][2] - 1, ni = dims[b - 1][0] - 1;
         if (i < 1 || i > ni || j < 1 || j > nj || k < 1 || k > nk) throw Error(FV3D_EINVAL, path + ": cell out of range");
@@ -256,6 +261,7 @@ inline std::vector<BlockState> ReadSolution(const std::string &path, const std::
         double *q = &out[b - 1].pv[x * 6];
         q[0] = rho; q[1] = u; q[2] = v; q[3] = w; q[4] = p; q[5] = T;
         out[b - 1].ener[x] = rhoE;
+        if (with_nut) out[b - 1].ev[x] = ev;
     }
     return out;
 }
@@ -306,6 +312,7 @@ public:
     void Initial(int block, const BlockState &st)
     {
         check(ctx_, fv3d_state_upload(ctx_, block, st.pv.data(), st.ener.data()), "fv3d_state_upload");
+        if (!st.ev.empty()) check(ctx_, fv3d_nut_upload(ctx_, block, st.ev.data()), "fv3d_nut_upload");
     }
     // CMesh::Initial: the uniform or interpolated start field (on the coarse
     // level when two levels run).

@@ -69,6 +69,21 @@ def test_parse_only_matches_written_case(case_dir):
     assert s["states"][0][1] == pytest.approx(checksum(st["rhoE"]), rel=1e-13)
 
 
+def test_parse_only_reads_nu_tilde(tmp_path):
+    """With Spalart-Allmaras the restart rows carry a twelfth column, nu~."""
+    case = cases.with_sa(cases.with_wall_shear(cases.bump_channel(ni=5, nj=4, nk=9, seed=8, viscous=True,
+                                                                  len_ref=1e-3)))
+    d = str(tmp_path / "case")
+    case.write(d)
+    out = subprocess.run([RUN, d, "--parse-only"], check=True, capture_output=True, text=True, timeout=60).stdout
+    s = json.loads(out)
+    assert s["config"]["turbulence"] == 200
+    st = case.states[0]
+    assert len(s["states"][0]) == 3
+    assert s["states"][0][1] == pytest.approx(checksum(st["rhoE"]), rel=1e-13)
+    assert s["states"][0][2] == pytest.approx(checksum(st["nut"]), rel=1e-13)
+
+
 def test_missing_case_reports_error(tmp_path):
     r = subprocess.run([RUN, str(tmp_path / "nope"), "--parse-only"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "cannot open" in r.stderr
