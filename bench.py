@@ -43,14 +43,17 @@ METRIC = "cell-updates/s + time to 1e-6 residual drop, chosen vs plain FP64, 1 G
 
 # Algorithmic HBM bytes per cell of one sweep launch: every array the sweep
 # semantically reads or writes counted once (R|dQ* 5, Q 5, lamc 3, dt 1,
-# vol 1, S 9 -> dQ 5; lamv 3 more for Navier-Stokes), in the operator's word
+# vol 1, S 9 -> dQ 5; lamv 3 more for Navier-Stokes; with Spalart-Allmaras
+# nu~, its residual, the nu~ diagonal and dnu~ 4 more), in the operator's word
 # size (8 B plain FP64, 4 B for the FP32 operator).
 SWEEP_WORDS = {False: 29, True: 32}
+SWEEP_WORDS_SA = 4
 # HBM bytes one forward-sweep launch actually moves, from rocprofv3
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (copy-calibrated); the summaries are
 # under profiles/.  Keyed by (workload, precision).
 MEASURED_FWD_SWEEP_TRAFFIC = {("bump256x64x64", "fp64"): 472.5e6, ("bump256x64x64ns", "fp64"): 506.0e6,
-                              ("passage5", "fp32"): 312.2e6, ("passage5", "fp64"): 568.9e6}
+                              ("passage5", "fp32"): 312.2e6, ("passage5", "fp64"): 568.9e6,
+                              ("passage5mg", "fp32"): 310.1e6}
 HBM_PEAK_GBS = 8000.0
 PREC = {"fp64": 0, "fp32": 1}
 
@@ -224,7 +227,7 @@ def main():
     r = res[chosen]
     # dominant kernel: the forward sweep of the chosen form, timed alone with HIP
     # events on the library's stream (the only stream its kernels run on)
-    words = SWEEP_WORDS[viscous]
+    words = SWEEP_WORDS[viscous] + (SWEEP_WORDS_SA if case.cfg.turbulence == 200 else 0)
     kbytes = words * (4 if chosen == "fp32" else 8)
     kms = r["class_ms"]["forward_sweep"]
     achieved = kbytes * r["cells"] / (kms * 1e-3) / 1e9
