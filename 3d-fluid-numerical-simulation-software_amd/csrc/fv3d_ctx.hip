@@ -32,6 +32,7 @@ struct HostConn {
 };
 
 struct HostBlock {
+    int inst = 0;                // instance (independent passage) of the block
     int IM, JM, KM, nblade;
     double rpm, w_rot, cellvol;
     std::vector<double> nodes;   // (IM, JM, KM, 3) as given
@@ -39,6 +40,14 @@ struct HostBlock {
     std::vector<void *> allocs;
     long sweep_rows = 0;         // rows of the tile-skewed sweep layout
     long pack_groups = 0;        // workgroups of k_sweep_pack / k_sweep_unpack
+};
+
+// An independent passage of a batched context: its blocks are
+// first_block .. first_block + nblock - 1, its outlet static pressure p_out
+// (config units).
+struct HostInst {
+    double p_out;
+    int first_block, nblock;
 };
 
 }  // namespace
@@ -50,6 +59,13 @@ struct fv3d_ctx {
     int device;
     hipStream_t stream = nullptr;
     std::vector<HostBlock> blocks;
+    // instances: declared with fv3d_instance_add, else one holding every block
+    std::vector<HostInst> insts;
+    bool explicit_inst = false;
+    int ninst = 1;
+    std::vector<double> inst_vol;    // total cell volume per instance
+    std::vector<int2> parts_host;    // norm partials of each instance (first, count)
+    int2 *d_parts = nullptr;
     std::vector<HostPatch> patches;
     std::vector<HostConn> conns;
     bool ready = false;
@@ -82,6 +98,7 @@ struct fv3d_ctx {
     int2 *d_secs = nullptr;
     int npatch_dev = 0, ntiles = 0, nplanes = 0;
     long nbc = 0, ncell = 0, nplane_threads = 0;
+    long ncell_real = 0;             // interior cells (ncell also counts the instances' padding)
     double *d_partial = nullptr;
     int npart = 0;
     double *d_sums = nullptr;
@@ -90,7 +107,7 @@ struct fv3d_ctx {
     ScalarsDev *d_sc = nullptr;
     double *d_hist = nullptr;
     long hist_cap = 0;
-    double *d_stage_row = nullptr;   // history row written by the captured graph
+    double *d_stage_row = nullptr;   // history rows (one per instance) written by the captured graph
     double *d_faces[3] = {nullptr, nullptr, nullptr};
     ViscDebug vdbg = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}, nullptr};
     int store_faces = 0;
@@ -108,7 +125,8 @@ struct fv3d_ctx {
     // two-grid cycle: the coarse level is a context of its own on the same
     // stream (coarse is set on the fine level, parent on the coarse one).  The
     // CFL number, its ramp threshold and the iteration count are shared: they
-    // live in the fine level's scalars (ramp_sc); d_cfl points at its CFL.
+    // live in the fine level's scalars (ramp_sc, one per instance), which the
+    // blocks of both levels point at (BlockDev::cfl).
     fv3d_ctx *coarse = nullptr;
     fv3d_ctx *parent = nullptr;
     int4 *d_mg_tiles = nullptr;      // prolongation tiles over the coarse blocks
@@ -129,7 +147,6 @@ struct fv3d_ctx {
     long iter_host = 0;              // iterations run (host copy, picks the cycle phase)
     bool initial_done = false;       // fv3d_initial ran (the two-grid cycle needs it)
     ScalarsDev *ramp_sc = nullptr;
-    double *d_cfl = nullptr;
     // graph
     hipGraphExec_t graph_exec = nullptr;
     hipGraphExec_t graph_coarse = nullptr;   // coarse start-up phase of the two-grid cycle
@@ -199,6 +216,14 @@ static void nondimensionalize(fv3d_ctx *c)
     g.pr_lam = (float)f.pr_lam;
     g.pr_tur = (float)f.pr_tur;
     g.ev_ratio = (float)f.ev_ratio;
+}
+
+// Non-dimensional outlet static pressure of instance w (the config's p_out is
+// instance 1's unless instances were declared), as nondimensionalize forms it.
+static double inst_p_out(const fv3d_ctx *c, int w)
+{
+    const double pout = (float)c->insts[w].p_out;
+    return pout / (c->dens_ref * c->vel_ref * c->vel_ref);
 }
 
 // source/GeometryVariables.cpp:10-105
@@ -530,7 +555,8 @@ static int build_exchange(fv3d_ctx *c)
 // Spalart-Allmaras set-up: the ghost-cell work list of k_sa_ghost, the
 // limiter flags and the wall distance of every cell (CMesh::WallDistanceSA,
 // source/WallDistanceSA.cpp:9-132): the centroids of all wall faces of all
-// blocks are listed on the host and k_wall_distance takes the minimum.
+// blocks of an instance are listed on the host and k_wall_distance takes the
+// minimum over its instance's list.
 static int setup_sa(fv3d_ctx *c)
 {
     const int nb = (int)c->blocks.size();
@@ -546,27 +572,40 @@ static int setup_sa(fv3d_ctx *c)
     if ((rc = dalloc(c, &c->d_sa_hi, nb))) return rc;
     std::vector<std::vector<double>> fcen(nb);   // face centroids of the blocks that hold walls, 3 directions
     std::vector<double> wf;
-    for (const HostPatch &P : c->patches) {
-        if (P.kind != 2) continue;
-        const int n = P.block - 1, d = (P.face - 1) / 2;
-        HostBlock &b = c->blocks[n];
-        const long box = b.dev.box;
-        if (fcen[n].empty()) {
-            std::vector<double> fS[3], fs[3], fc[3], vol, cc, node;
-            face_vector_volume(b, fS[0], fS[1], fS[2], fs[0], fs[1], fs[2], fc[0], fc[1], fc[2], vol, cc, node);
-            fcen[n].resize(9 * box);
-            for (int q = 0; q < 3; ++q) std::copy(fc[q].begin(), fc[q].end(), fcen[n].begin() + 3 * q * box);
+    // the wall faces of each instance, one instance after the other
+    std::vector<int> wf0(c->ninst, 0), nwf(c->ninst, 0);
+    for (int w = 0; w < c->ninst; ++w) {
+        wf0[w] = (int)(wf.size() / 3);
+        for (const HostPatch &P : c->patches) {
+            if (P.kind != 2 || c->blocks[P.block - 1].inst != w) continue;
+            const int n = P.block - 1, d = (P.face - 1) / 2;
+            HostBlock &b = c->blocks[n];
+            const long box = b.dev.box;
+            if (fcen[n].empty()) {
+                std::vector<double> fS[3], fs[3], fc[3], vol, cc, node;
+                face_vector_volume(b, fS[0], fS[1], fS[2], fs[0], fs[1], fs[2], fc[0], fc[1], fc[2], vol, cc, node);
+                fcen[n].resize(9 * box);
+                for (int q = 0; q < 3; ++q) std::copy(fc[q].begin(), fc[q].end(), fcen[n].begin() + 3 * q * box);
+            }
+            const double *F = fcen[n].data() + 3 * d * box;
+            for (int i = P.rng[0]; i <= P.rng[1]; ++i)
+                for (int j = P.rng[2]; j <= P.rng[3]; ++j)
+                    for (int k = P.rng[4]; k <= P.rng[5]; ++k) {
+                        const long x = cidx(b.dev, i, j, k);
+                        wf.push_back(F[x]);
+                        wf.push_back(F[box + x]);
+                        wf.push_back(F[2 * box + x]);
+                    }
         }
-        const double *F = fcen[n].data() + 3 * d * box;
-        for (int i = P.rng[0]; i <= P.rng[1]; ++i)
-            for (int j = P.rng[2]; j <= P.rng[3]; ++j)
-                for (int k = P.rng[4]; k <= P.rng[5]; ++k) {
-                    const long x = cidx(b.dev, i, j, k);
-                    wf.push_back(F[x]);
-                    wf.push_back(F[box + x]);
-                    wf.push_back(F[2 * box + x]);
-                }
+        nwf[w] = (int)(wf.size() / 3) - wf0[w];
     }
+    std::vector<BlockDev> bd;
+    for (HostBlock &b : c->blocks) {
+        b.dev.wf0 = wf0[b.inst];
+        b.dev.nwf = nwf[b.inst];
+        bd.push_back(b.dev);
+    }
+    HIPCK(c, hipMemcpyAsync(c->d_blocks, bd.data(), bd.size() * sizeof(BlockDev), hipMemcpyHostToDevice, c->stream));
     const int nf = (int)(wf.size() / 3);
     double *d_wf = nullptr;
     if (nf > 0) {
@@ -574,7 +613,7 @@ static int setup_sa(fv3d_ctx *c)
         HIPCK(c, hipMemcpyAsync(d_wf, wf.data(), wf.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     }
     hipLaunchKernelGGL(k_wall_distance, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
-                       c->d_cstart, nb, d_wf, nf);
+                       c->d_cstart, nb, d_wf);
     HIPCK(c, hipGetLastError());
     HIPCK(c, hipStreamSynchronize(c->stream));
     if (d_wf) HIPCK(c, hipFree(d_wf));
@@ -626,6 +665,7 @@ int fv3d_ctx_destroy(fv3d_ctx *c)
     if (c->d_axial) (void)hipFree(c->d_axial);
     if (c->d_dtmin) (void)hipFree(c->d_dtmin);
     if (c->d_gstart) (void)hipFree(c->d_gstart);
+    if (c->d_parts) (void)hipFree(c->d_parts);
     if (c->d_sa_hi) (void)hipFree(c->d_sa_hi);
     if (c->d_pack_start) (void)hipFree(c->d_pack_start);
     for (int d = 0; d < 3; ++d) {
@@ -661,8 +701,31 @@ int fv3d_block_add(fv3d_ctx *c, int IM, int JM, int KM, const double *nodes, int
     // source/UserInput.cpp:444 and 1088
     b.w_rot = (double)(2.0f * 3.1415926535897932f) * rpm / F32C(60.0);
     b.w_rot = b.w_rot / (c->vel_ref / c->len_ref);
+    if (c->explicit_inst) {
+        b.inst = (int)c->insts.size() - 1;
+        c->insts.back().nblock++;
+    }
     c->blocks.push_back(std::move(b));
     return (int)c->blocks.size();
+}
+
+int fv3d_instance_add(fv3d_ctx *c, double p_out)
+{
+    if (!c) return FV3D_EINVAL;
+    if (c->ready) return fail(c, FV3D_ESTATE, "instance_add after setup");
+    if (!c->explicit_inst && !c->blocks.empty())
+        return fail(c, FV3D_ESTATE, "instances are declared before their blocks: %d blocks were added first",
+                    (int)c->blocks.size());
+    if (!(p_out > 0.0)) return fail(c, FV3D_EINVAL, "instance outlet pressure %g", p_out);
+    c->explicit_inst = true;
+    c->insts.push_back(HostInst{p_out, (int)c->blocks.size(), 0});
+    return (int)c->insts.size();
+}
+
+int fv3d_instances(const fv3d_ctx *c)
+{
+    if (!c) return FV3D_EINVAL;
+    return c->ready ? c->ninst : (c->explicit_inst ? (int)c->insts.size() : 1);
 }
 
 int fv3d_connection_add(fv3d_ctx *c, int kind, int orient, int block1, int face1, const int *rng1, int block2,
@@ -754,9 +817,16 @@ static void coarse_range(const int *r, int *o)
 // stream and CFL state.
 static int setup_coarse(fv3d_ctx *c)
 {
-    bool ok = true;
-    for (const HostPatch &P : c->patches) ok = ok && range_coarsens(P.rng);
-    for (const HostConn &T : c->conns) ok = ok && range_coarsens(T.rng[0]);
+    // the level count of each instance, as its own run would find it
+    std::vector<char> inst_ok(c->ninst, 1);
+    for (const HostPatch &P : c->patches)
+        if (!range_coarsens(P.rng)) inst_ok[c->blocks[P.block - 1].inst] = 0;
+    for (const HostConn &T : c->conns)
+        if (!range_coarsens(T.rng[0])) inst_ok[c->blocks[T.block[0] - 1].inst] = 0;
+    bool ok = true, any = false;
+    for (char v : inst_ok) { ok = ok && v; any = any || v; }
+    if (any && !ok)
+        return fail(c, FV3D_EINVAL, "two-grid cycle: the instances do not all coarsen (their patch ranges differ)");
     if (!ok) {
         c->ng = 1;
         return FV3D_OK;
@@ -788,6 +858,8 @@ static int setup_coarse(fv3d_ctx *c)
     };
     int rc;
     for (const HostBlock &b : c->blocks) {
+        if (c->explicit_inst && c->insts[b.inst].first_block == (int)(&b - c->blocks.data()))
+            if ((rc = fv3d_instance_add(m, c->insts[b.inst].p_out)) < 0) return bail(rc);
         const int IM = (b.IM + 1) / 2, JM = (b.JM + 1) / 2, KM = (b.KM + 1) / 2;
         std::vector<double> nodes(3 * (size_t)IM * JM * KM);
         for (int i = 1; i <= IM; ++i)
@@ -837,13 +909,24 @@ int fv3d_setup(fv3d_ctx *c)
     if (c->blocks.empty()) return fail(c, FV3D_ESTATE, "no blocks");
     int nb = (int)c->blocks.size();
     int rc;
+    if (!c->explicit_inst) c->insts.assign(1, HostInst{c->cfg.p_out, 0, nb});
+    c->ninst = (int)c->insts.size();
+    for (const HostInst &I : c->insts)
+        if (I.nblock == 0) return fail(c, FV3D_EINVAL, "an instance without blocks");
+    for (const HostConn &T : c->conns)
+        if (c->blocks[T.block[0] - 1].inst != c->blocks[T.block[1] - 1].inst)
+            return fail(c, FV3D_EINVAL, "a connection between blocks %d and %d of different instances", T.block[0],
+                        T.block[1]);
     if ((rc = dalloc(c, &c->d_flags, nb))) return rc;
     if ((rc = dalloc(c, &c->d_bad, nb))) return rc;
     c->total_vol = 0.0;
+    c->inst_vol.assign(c->ninst, 0.0);
     for (int n = 0; n < nb; ++n) {
         if ((rc = setup_block(c, c->blocks[n], n))) return rc;
-        c->total_vol = c->total_vol + c->blocks[n].cellvol;   // source/ComputeAllBlockVars.cpp:25-36
+        // source/ComputeAllBlockVars.cpp:25-36, per instance
+        c->inst_vol[c->blocks[n].inst] = c->inst_vol[c->blocks[n].inst] + c->blocks[n].cellvol;
     }
+    c->total_vol = c->inst_vol[0];
     {
         // mailboxes (forward and backward sets per block) and the pack work list
         long words = 0;
@@ -864,6 +947,11 @@ int fv3d_setup(fv3d_ctx *c)
         if ((rc = dalloc(c, &c->d_pack_start, gst.size()))) return rc;
         HIPCK(c, hipMemcpyAsync(c->d_pack_start, gst.data(), gst.size() * sizeof(long), hipMemcpyHostToDevice, c->stream));
     }
+    // per-instance scalars (filled below); the CFL number each block reads is
+    // its instance's, kept by the fine level
+    if ((rc = dalloc(c, &c->d_sc, c->ninst))) return rc;
+    c->ramp_sc = c->parent ? c->parent->ramp_sc : c->d_sc;
+    for (auto &b : c->blocks) b.dev.cfl = &c->ramp_sc[b.inst].cfl;
     std::vector<BlockDev> bd;
     for (auto &b : c->blocks) bd.push_back(b.dev);
     if ((rc = dalloc(c, &c->d_blocks, nb))) return rc;
@@ -888,11 +976,14 @@ int fv3d_setup(fv3d_ctx *c)
     // patch work list
     std::vector<PatchDev> pd;
     std::vector<long> pst{0};
-    int nin = 0, nout = 0;
-    int2 secs[2] = {{0, 0}, {0, 0}};
+    // inlet and outlet sections of each instance: its first inlet and first
+    // outlet patch (source/Ouput.cpp:9-107)
+    std::vector<int> nin(c->ninst, 0), nout(c->ninst, 0);
+    std::vector<int2> secs(2 * c->ninst, make_int2(0, 0));
     for (const HostPatch &P : c->patches) {
         PatchDev q;
         q.kind = P.kind; q.block = P.block - 1;
+        q.inst = c->blocks[P.block - 1].inst;
         q.dir = (P.face - 1) / 2; q.hi = (P.face - 1) % 2;
         int d = q.dir;
         const int *r = P.rng;
@@ -906,10 +997,13 @@ int fv3d_setup(fv3d_ctx *c)
         q.rot = P.rotate ? c->blocks[P.block - 1].w_rot : 0.0;
         pd.push_back(q);
         pst.push_back(pst.back() + (long)q.na * q.nb);
-        if (P.kind == 0 && nin++ == 0) secs[0] = make_int2(P.block - 1, r[4]);
-        if (P.kind == 1 && nout++ == 0) secs[1] = make_int2(P.block - 1, r[4]);
+        if (P.kind == 0 && nin[q.inst]++ == 0) secs[2 * q.inst] = make_int2(P.block - 1, r[4]);
+        if (P.kind == 1 && nout[q.inst]++ == 0) secs[2 * q.inst + 1] = make_int2(P.block - 1, r[4]);
     }
-    if (nin == 0 || nout == 0) return fail(c, FV3D_EINVAL, "need an inlet and an outlet patch");
+    for (int w = 0; w < c->ninst; ++w)
+        if (nin[w] == 0 || nout[w] == 0)
+            return fail(c, FV3D_EINVAL, c->ninst > 1 ? "instance %d needs an inlet and an outlet patch"
+                                                     : "need an inlet and an outlet patch", w + 1);
     {
         // radial-equilibrium outlets: each covers its block's whole K+ face
         std::vector<int> ax;
@@ -978,15 +1072,27 @@ int fv3d_setup(fv3d_ctx *c)
     HIPCK(c, hipMemcpyAsync(c->d_patches, pd.data(), pd.size() * sizeof(PatchDev), hipMemcpyHostToDevice, c->stream));
     if ((rc = dalloc(c, &c->d_pstart_bc, pst.size()))) return rc;
     HIPCK(c, hipMemcpyAsync(c->d_pstart_bc, pst.data(), pst.size() * sizeof(long), hipMemcpyHostToDevice, c->stream));
-    if ((rc = dalloc(c, &c->d_secs, 2))) return rc;
-    HIPCK(c, hipMemcpyAsync(c->d_secs, secs, sizeof secs, hipMemcpyHostToDevice, c->stream));
+    if ((rc = dalloc(c, &c->d_secs, secs.size()))) return rc;
+    HIPCK(c, hipMemcpyAsync(c->d_secs, secs.data(), secs.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     // cell and plane work lists
     std::vector<long> cst{0}, plst{0};
     std::vector<int4> tiles;
     c->nplanes = 0;
+    c->parts_host.assign(c->ninst, make_int2(0, 0));
+    c->ncell_real = 0;
     for (int n = 0; n < nb; ++n) {
         const BlockDev &B = c->blocks[n].dev;
+        const int w = c->blocks[n].inst;
+        if (n > 0 && c->insts[w].first_block == n) {
+            // an instance starts on a whole 256-cell group, so its norm partials
+            // are those it has when it runs alone: pad the previous block's range
+            cst.back() = (cst.back() + 255) / 256 * 256;
+        }
+        if (c->insts[w].first_block == n) c->parts_host[w].x = (int)(cst.back() / 256);
         cst.push_back(cst.back() + (long)B.ni * B.nj * B.nk);
+        c->ncell_real += (long)B.ni * B.nj * B.nk;
+        if (n == c->insts[w].first_block + c->insts[w].nblock - 1)
+            c->parts_host[w].y = (int)((cst.back() + 255) / 256) - c->parts_host[w].x;
         plst.push_back(plst.back() + (long)B.ni * B.nj);
         c->nplanes = std::max(c->nplanes, B.IM + B.JM + B.KM - 5);
         for (int i0 = 0; i0 < B.ni; i0 += kTI)
@@ -1041,24 +1147,26 @@ int fv3d_setup(fv3d_ctx *c)
     }
     c->npart = (int)((c->ncell + 255) / 256);
     if ((rc = dalloc(c, &c->d_partial, 6 * (size_t)c->npart))) return rc;
+    if ((rc = dalloc(c, &c->d_parts, c->ninst))) return rc;
+    HIPCK(c, hipMemcpyAsync(c->d_parts, c->parts_host.data(), c->ninst * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     if (c->cfg.turbulence == 200 && (rc = setup_sa(c))) return rc;
-    if ((rc = dalloc(c, &c->d_sums, 2 * 3 * kSecParts))) return rc;
-    if ((rc = dalloc(c, &c->d_stage_row, 7))) return rc;
-    // device scalars
-    ScalarsDev sc;
-    memset(&sc, 0, sizeof sc);
-    sc.cfl = (float)c->cfg.cfl;
-    sc.cfl_res = F32C(1.0E-3);   // source/Solver.cpp:21
-    sc.total_vol = c->total_vol;
-    sc.dens_ref = c->dens_ref; sc.vel_ref = c->vel_ref; sc.temp_ref = c->temp_ref;
-    sc.nblade_in = c->blocks[secs[0].x].nblade;
-    sc.nblade_out = c->blocks[secs[1].x].nblade;
-    sc.p_out = c->gas.p_out;
-    if ((rc = dalloc(c, &c->d_sc, 1))) return rc;
-    HIPCK(c, hipMemcpyAsync(c->d_sc, &sc, sizeof sc, hipMemcpyHostToDevice, c->stream));
+    if ((rc = dalloc(c, &c->d_sums, 2 * 3 * kSecParts * (size_t)c->ninst))) return rc;
+    if ((rc = dalloc(c, &c->d_stage_row, 7 * (size_t)c->ninst))) return rc;
+    // device scalars of each instance
+    std::vector<ScalarsDev> scs(c->ninst);
+    for (int w = 0; w < c->ninst; ++w) {
+        ScalarsDev &sc = scs[w];
+        memset(&sc, 0, sizeof sc);
+        sc.cfl = (float)c->cfg.cfl;
+        sc.cfl_res = F32C(1.0E-3);   // source/Solver.cpp:21
+        sc.total_vol = c->inst_vol[w];
+        sc.dens_ref = c->dens_ref; sc.vel_ref = c->vel_ref; sc.temp_ref = c->temp_ref;
+        sc.nblade_in = c->blocks[secs[2 * w].x].nblade;
+        sc.nblade_out = c->blocks[secs[2 * w + 1].x].nblade;
+        sc.p_out = inst_p_out(c, w);
+    }
+    HIPCK(c, hipMemcpyAsync(c->d_sc, scs.data(), scs.size() * sizeof(ScalarsDev), hipMemcpyHostToDevice, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
-    c->ramp_sc = c->parent ? c->parent->ramp_sc : c->d_sc;
-    c->d_cfl = &c->ramp_sc->cfl;
     c->ready = true;
     if (c->cfg.multigrid && (rc = setup_coarse(c))) {
         c->ready = false;
@@ -1120,15 +1228,16 @@ int fv3d_levels(const fv3d_ctx *c) { return c ? c->ng : FV3D_EINVAL; }
 // or CMesh::Initial_Interpolate (74-165), pressure and flow direction linear in
 // z between the first inlet's K face and the first outlet's.  Host code: it
 // runs once, before the first iteration.
-static int initial_level(fv3d_ctx *m, int type)
+static int initial_instance(fv3d_ctx *m, int type, int w)
 {
     const Gas &g = m->gas;
-    const double Gamma = g.gamma, Rgas = g.rgas, Ptin = g.pt_in, Ttin = g.tt_in, Pout = g.p_out;
+    const double Gamma = g.gamma, Rgas = g.rgas, Ptin = g.pt_in, Ttin = g.tt_in, Pout = inst_p_out(m, w);
     const double *din = m->cfg.inlet_dir, *dout = m->cfg.outlet_dir;
     const int nb = (int)m->blocks.size();
     // cell centroids (and K-face centroids) in the reference's operation order
     std::vector<std::vector<double>> cc(nb), fck(nb);
     for (int n = 0; n < nb; ++n) {
+        if (m->blocks[n].inst != w) continue;
         std::vector<double> fS[3], fs[3], fc[3], vol, node;
         face_vector_volume(m->blocks[n], fS[0], fS[1], fS[2], fs[0], fs[1], fs[2], fc[0], fc[1], fc[2], vol, cc[n], node);
         fck[n] = std::move(fc[2]);
@@ -1145,6 +1254,7 @@ static int initial_level(fv3d_ctx *m, int type)
     } else {
         const HostPatch *pin = nullptr, *pout = nullptr;
         for (const HostPatch &P : m->patches) {
+            if (m->blocks[P.block - 1].inst != w) continue;
             if (P.kind == 0 && !pin) pin = &P;
             if (P.kind == 1 && !pout) pout = &P;
         }
@@ -1178,6 +1288,7 @@ static int initial_level(fv3d_ctx *m, int type)
         Vel = Ma * cs;
     }
     for (int n = 0; n < nb; ++n) {
+        if (m->blocks[n].inst != w) continue;
         const BlockDev &B = m->blocks[n].dev;
         std::vector<double> Q(5 * B.box, 0.0), W(6 * B.box, 0.0);
         HIPCK(m, hipMemcpy(Q.data(), B.Q, Q.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -1220,6 +1331,17 @@ static int initial_level(fv3d_ctx *m, int type)
             HIPCK(m, hipMemcpy(B.ev, E.data(), E.size() * sizeof(double), hipMemcpyHostToDevice));
         }
     }
+    return FV3D_OK;
+}
+
+// CMesh::Initial on one level, each instance with its own outlet pressure and
+// its own first inlet and outlet
+static int initial_level(fv3d_ctx *m, int type)
+{
+    for (int w = 0; w < m->ninst; ++w) {
+        int rc = initial_instance(m, type, w);
+        if (rc) return rc;
+    }
     if (m->cfg.turbulence == 200) m->ev_set = true;
     return FV3D_OK;
 }
@@ -1248,10 +1370,10 @@ static int launch_bc(fv3d_ctx *c)
 {
     if (c->nbc > 0)
         hipLaunchKernelGGL(k_bc, dim3((unsigned)((c->nbc + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks, c->d_patches,
-                           c->npatch_dev, c->d_pstart_bc, c->gas, &c->d_sc->p_out);
+                           c->npatch_dev, c->d_pstart_bc, c->gas, c->d_sc);
     if (c->n_axial > 0)
         hipLaunchKernelGGL(k_bc_axial, dim3((unsigned)c->n_axial), dim3(256), 0, c->stream, c->d_blocks, c->d_patches,
-                           c->d_axial, c->gas, &c->d_sc->p_out);
+                           c->d_axial, c->gas, c->d_sc);
     if (c->nper > 0)
         hipLaunchKernelGGL(k_periodic_flip, dim3(1), dim3(64), 0, c->stream, c->d_per, c->nper, c->d_pitch_sign);
     if (c->nexch > 0)
@@ -1264,7 +1386,7 @@ static int launch_bc(fv3d_ctx *c)
 static int launch_spectral_dt(fv3d_ctx *c)
 {
     hipLaunchKernelGGL(k_spectral_dt, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
-                       c->d_cstart, (int)c->blocks.size(), c->gas, c->d_cfl);
+                       c->d_cstart, (int)c->blocks.size(), c->gas);
     HIPCK(c, hipGetLastError());
     if (c->cfg.global_dt) {
         const int nb = (int)c->blocks.size();
@@ -1275,7 +1397,7 @@ static int launch_spectral_dt(fv3d_ctx *c)
         hipLaunchKernelGGL(k_dt_global, dim3(g), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb, c->gas, c->d_dtmin);
         HIPCK(c, hipGetLastError());
     }
-    hipLaunchKernelGGL(k_dt_flag_decrement, dim3(1), dim3(64), 0, c->stream, c->d_blocks, (int)c->blocks.size());
+    hipLaunchKernelGGL(k_dt_flag_decrement, dim3(1), dim3(256), 0, c->stream, c->d_blocks, (int)c->blocks.size());
     HIPCK(c, hipGetLastError());
     return FV3D_OK;
 }
@@ -1508,10 +1630,11 @@ static int launch_update(fv3d_ctx *c)
 
 static int launch_finalize(fv3d_ctx *c, double *hist_row)
 {
-    hipLaunchKernelGGL(k_sections, dim3(2 * kSecParts), dim3(256), 0, c->stream, c->d_blocks, c->d_secs, c->gas, c->d_sums);
+    hipLaunchKernelGGL(k_sections, dim3(2 * kSecParts * c->ninst), dim3(256), 0, c->stream, c->d_blocks, c->d_secs, c->gas,
+                       c->d_sums);
     HIPCK(c, hipGetLastError());
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, c->d_partial, c->npart, c->d_sums, c->d_sc,
-                       c->ramp_sc, c->gas, hist_row);
+    hipLaunchKernelGGL(k_finalize, dim3(c->ninst), dim3(256), 0, c->stream, c->d_partial, c->npart, c->d_parts, c->d_sums,
+                       c->d_sc, c->ramp_sc, c->gas, hist_row);
     HIPCK(c, hipGetLastError());
     return FV3D_OK;
 }
@@ -1581,35 +1704,33 @@ int fv3d_lusgs(fv3d_ctx *c)
     return check_sweep_error(c);
 }
 
-int fv3d_res_calculate(fv3d_ctx *c, double *res6)
+// Res_Calculate of instance w from the norm partials of the last update
+static int instance_norms(fv3d_ctx *c, int w, double *res6)
 {
-    READY(c);
-    std::vector<double> part(6 * (size_t)c->npart);
-    HIPCK(c, hipMemcpyAsync(part.data(), c->d_partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCK(c, hipStreamSynchronize(c->stream));
-    for (int q = 0; q < 5; ++q) {
+    const int p0 = c->parts_host[w].x, np = c->parts_host[w].y;
+    const int nc = c->cfg.turbulence == 200 ? 6 : 5;
+    std::vector<double> part((size_t)np);
+    for (int q = 0; q < 6; ++q) {
+        res6[q] = 0.0;
+        if (q >= nc) continue;
+        HIPCK(c, hipMemcpy(part.data(), c->d_partial + (size_t)q * c->npart + p0, np * sizeof(double),
+                           hipMemcpyDeviceToHost));
         double s = 0.0;
-        for (int p = 0; p < c->npart; ++p) s += part[(size_t)q * c->npart + p];
-        if (res6) res6[q] = s / c->total_vol;
-    }
-    if (res6) {
-        res6[5] = 0.0;
-        if (c->cfg.turbulence == 200) {
-            double s = 0.0;
-            for (int p = 0; p < c->npart; ++p) s += part[(size_t)5 * c->npart + p];
-            res6[5] = s / c->total_vol;
-        }
+        for (int p = 0; p < np; ++p) s += part[p];
+        res6[q] = s / c->inst_vol[w];
     }
     return FV3D_OK;
 }
 
-int fv3d_perf(fv3d_ctx *c, double *out5)
+// COutput::OutputGeneralParameters of instance w (source/Ouput.cpp:282-300)
+static int instance_perf(fv3d_ctx *c, int w, double *out5)
 {
-    READY(c);
-    hipLaunchKernelGGL(k_sections, dim3(2 * kSecParts), dim3(256), 0, c->stream, c->d_blocks, c->d_secs, c->gas, c->d_sums);
+    hipLaunchKernelGGL(k_sections, dim3(2 * kSecParts * c->ninst), dim3(256), 0, c->stream, c->d_blocks, c->d_secs, c->gas,
+                       c->d_sums);
     HIPCK(c, hipGetLastError());
     double part[2 * 3 * kSecParts];
-    HIPCK(c, hipMemcpyAsync(part, c->d_sums, sizeof part, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemcpyAsync(part, c->d_sums + (size_t)w * 2 * 3 * kSecParts, sizeof part, hipMemcpyDeviceToHost,
+                            c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
     // the partials in the order k_finalize adds them
     double s[6] = {0, 0, 0, 0, 0, 0};
@@ -1617,7 +1738,7 @@ int fv3d_perf(fv3d_ctx *c, double *out5)
         for (int p = 0; p < kSecParts; ++p)
             for (int q = 0; q < 3; ++q) s[sec * 3 + q] += part[(sec * kSecParts + p) * 3 + q];
     ScalarsDev sc;
-    HIPCK(c, hipMemcpy(&sc, c->d_sc, sizeof sc, hipMemcpyDeviceToHost));
+    HIPCK(c, hipMemcpy(&sc, c->d_sc + w, sizeof sc, hipMemcpyDeviceToHost));
     const double Gamma = c->gas.gamma, Rgas = c->gas.rgas;
     double mi = s[0] * sc.dens_ref * sc.vel_ref * sc.nblade_in;
     double mo = s[3] * sc.dens_ref * sc.vel_ref * sc.nblade_out;
@@ -1627,9 +1748,42 @@ int fv3d_perf(fv3d_ctx *c, double *out5)
     double PR = pto / pti;
     double L_adk = Gamma / (Gamma - F32C(1.0)) * Rgas * tti * (pow(PR, (Gamma - F32C(1.0)) / Gamma) - F32C(1.0));
     double L_k = Gamma / (Gamma - F32C(1.0)) * Rgas * tti * (tto / tti - F32C(1.0));
-    if (out5) {
-        out5[0] = mi; out5[1] = mo; out5[2] = PR; out5[3] = L_adk / L_k; out5[4] = fabs((mi - mo) / mi);
-    }
+    out5[0] = mi; out5[1] = mo; out5[2] = PR; out5[3] = L_adk / L_k; out5[4] = fabs((mi - mo) / mi);
+    return FV3D_OK;
+}
+
+int fv3d_res_calculate(fv3d_ctx *c, double *res6)
+{
+    READY(c);
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    double r[6];
+    int rc = instance_norms(c, 0, r);
+    if (rc) return rc;
+    if (res6) memcpy(res6, r, sizeof r);
+    return FV3D_OK;
+}
+
+int fv3d_perf(fv3d_ctx *c, double *out5)
+{
+    READY(c);
+    double o[5];
+    int rc = instance_perf(c, 0, o);
+    if (rc) return rc;
+    if (out5) memcpy(out5, o, sizeof o);
+    return FV3D_OK;
+}
+
+int fv3d_instance_results(fv3d_ctx *c, int inst, double *res6, double *perf5)
+{
+    READY(c);
+    if (inst < 1 || inst > c->ninst) return fail(c, FV3D_EINVAL, "instance %d of %d", inst, c->ninst);
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    double r[6], o[5];
+    int rc = instance_norms(c, inst - 1, r);
+    if (!rc) rc = instance_perf(c, inst - 1, o);
+    if (rc) return rc;
+    if (res6) memcpy(res6, r, sizeof r);
+    if (perf5) memcpy(perf5, o, sizeof o);
     return FV3D_OK;
 }
 
@@ -1731,7 +1885,7 @@ int fv3d_iterate(fv3d_ctx *c, int n, double *hist)
     READY(c);
     if (n < 0) return fail(c, FV3D_EINVAL, "n=%d", n);
     int rc;
-    if ((rc = ensure_hist(c, std::max(n, 1)))) return rc;
+    if ((rc = ensure_hist(c, (long)std::max(n, 1) * c->ninst))) return rc;
     if (c->ng == 2 && !c->initial_done)
         return fail(c, FV3D_ESTATE, "the two-grid cycle starts from fv3d_initial (reuse_solution = N): from a restart "
                                     "file the reference's cycle divides by coarse time steps it never computed");
@@ -1739,7 +1893,7 @@ int fv3d_iterate(fv3d_ctx *c, int n, double *hist)
         return fail(c, FV3D_ESTATE, "Spalart-Allmaras needs nu~: fv3d_nut_upload for every block, or fv3d_initial");
     for (int it = 0; it < n; ++it) {
         const long itn = c->iter_host + 1;
-        double *row = c->d_hist + 7L * it;
+        double *row = c->d_hist + 7L * c->ninst * it;
         bool timed = true;
         if (c->ng == 2) {
             const int phase = mg_phase(c, itn);
@@ -1757,11 +1911,15 @@ int fv3d_iterate(fv3d_ctx *c, int n, double *hist)
         }
     }
     if (hist && n > 0)
-        HIPCK(c, hipMemcpyAsync(hist, c->d_hist, 7L * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(c, hipMemcpyAsync(hist, c->d_hist, 7L * n * c->ninst * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
     if ((rc = check_sweep_error(c))) return rc;
     if (c->coarse && (rc = check_sweep_error(c->coarse))) return fail(c, rc, "coarse level: %s", c->coarse->err.c_str());
-    if (hist && n > 0 && !isfinite(hist[7L * (n - 1) + 1])) return fail(c, FV3D_ENUMERIC, "residual is not finite");
+    if (hist && n > 0)
+        for (int w = 0; w < c->ninst; ++w)
+            if (!isfinite(hist[7L * (c->ninst * (n - 1L) + w) + 1]))
+                return fail(c, FV3D_ENUMERIC, c->ninst > 1 ? "instance %d: residual is not finite" : "residual is not finite",
+                            w + 1);
     return FV3D_OK;
 }
 
@@ -1770,7 +1928,7 @@ int fv3d_iterate_graph(fv3d_ctx *c, int n, double *hist)
     READY(c);
     if (n < 0) return fail(c, FV3D_EINVAL, "n=%d", n);
     int rc;
-    if ((rc = ensure_hist(c, std::max(n, 1)))) return rc;
+    if ((rc = ensure_hist(c, (long)std::max(n, 1) * c->ninst))) return rc;
     if (c->ng == 2 && !c->initial_done)
         return fail(c, FV3D_ESTATE, "the two-grid cycle starts from fv3d_initial (reuse_solution = N): from a restart "
                                     "file the reference's cycle divides by coarse time steps it never computed");
@@ -1812,15 +1970,20 @@ int fv3d_iterate_graph(fv3d_ctx *c, int n, double *hist)
             if ((rc = capture(exec, phase))) return rc;
             HIPCK(c, hipGraphLaunch(*exec, c->stream));
         }
-        HIPCK(c, hipMemcpyAsync(c->d_hist + 7L * it, stage_row, 7 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCK(c, hipMemcpyAsync(c->d_hist + 7L * c->ninst * it, stage_row, 7L * c->ninst * sizeof(double),
+                                hipMemcpyDeviceToDevice, c->stream));
         c->iter_host = itn;
     }
     if (hist && n > 0)
-        HIPCK(c, hipMemcpyAsync(hist, c->d_hist, 7L * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(c, hipMemcpyAsync(hist, c->d_hist, 7L * n * c->ninst * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
     if ((rc = check_sweep_error(c))) return rc;
     if (c->coarse && (rc = check_sweep_error(c->coarse))) return fail(c, rc, "coarse level: %s", c->coarse->err.c_str());
-    if (hist && n > 0 && !isfinite(hist[7L * (n - 1) + 1])) return fail(c, FV3D_ENUMERIC, "residual is not finite");
+    if (hist && n > 0)
+        for (int w = 0; w < c->ninst; ++w)
+            if (!isfinite(hist[7L * (c->ninst * (n - 1L) + w) + 1]))
+                return fail(c, FV3D_ENUMERIC, c->ninst > 1 ? "instance %d: residual is not finite" : "residual is not finite",
+                            w + 1);
     return FV3D_OK;
 }
 
@@ -1849,7 +2012,7 @@ int fv3d_timing(fv3d_ctx *c, double *ms, long *counts)
     return FV3D_OK;
 }
 
-long fv3d_cell_count(const fv3d_ctx *c) { return c ? c->ncell : 0; }
+long fv3d_cell_count(const fv3d_ctx *c) { return c ? c->ncell_real : 0; }
 
 int fv3d_set_debug(fv3d_ctx *c, int flags)
 {

@@ -19,13 +19,14 @@
 namespace fv3d {
 
 constexpr int kND = 2;        // Dummy layers for the upwind schemes (source/UserInput.cpp:132-155)
-constexpr int kMaxBlocks = 64;
+constexpr int kMaxBlocks = 4096;
 
 // float literals of the reference widened to double (include/defs.h keeps
 // many constants as float literals even when REAL is double)
 #define F32C(x) ((double)(x##f))
 
 struct BlockDev {
+    int inst;                 // instance (independent passage) the block belongs to
     int IM, JM, KM;           // node counts
     int ni, nj, nk;           // cell counts = IM-1, JM-1, KM-1
     long box;                 // (IM+1)(JM+1)(KM+1)
@@ -69,6 +70,8 @@ struct BlockDev {
     double *res_ev;
     double *dwall;
     double *alpha2;           // nu~ LU-SGS diagonal Vol/dt + relax*sum(lamc) + 3*sum(lamv)
+    int wf0, nwf;             // the wall faces of the block's instance in the wall-distance list
+    const double *cfl;        // CFL number of the block's instance (device scalar)
     double *dt;               // local time step
     double *alpha;            // LU-SGS diagonal Vol/dt + relax*sum(lamc) + 2*sum(lamv)
     const double *swg;        // 15*box static sweep metrics per cell: for d = 0..2,
@@ -99,6 +102,38 @@ struct BlockDev {
 __host__ __device__ inline long cidx(const BlockDev &B, int i, int j, int k)
 {
     return (long)i * B.si + (long)j * B.sj + k;
+}
+
+// The range of work item t in a list of consecutive per-block ranges:
+// the last n < count with start[n] <= t (binary search).
+__device__ inline int find_range(const long *start, int count, long t)
+{
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Cell (i, j, k) of block n for work item t of a cell list (block n owns
+// items cstart[n] .. cstart[n+1]-1, its first ni nj nk items are its cells,
+// k fastest; items past them pad an instance to a whole number of 256-item
+// groups).  False for padding and items past the list.
+__device__ inline bool cell_item(const BlockDev *blocks, const long *cstart, int nblock, long t, int &n, int &i, int &j,
+                                 int &k)
+{
+    if (t >= cstart[nblock]) return false;
+    n = find_range(cstart, nblock, t);
+    const BlockDev &B = blocks[n];
+    long r = t - cstart[n];
+    if (r >= (long)B.ni * B.nj * B.nk) return false;
+    k = (int)(r % B.nk) + 1;
+    r /= B.nk;
+    j = (int)(r % B.nj) + 1;
+    i = (int)(r / B.nj) + 1;
+    return true;
 }
 
 // Dummy slab address of component c (0-4 cv, 5-10 pv) of layer l (1..2) on side s

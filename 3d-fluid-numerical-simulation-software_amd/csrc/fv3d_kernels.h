@@ -17,6 +17,7 @@ struct PatchDev {
     int na, nb;        // tangential extents
     int centrifugal;   // outlet variant
     int axial;         // radial-equilibrium outlet: applied by k_bc_axial, skipped by k_bc
+    int inst;          // instance of the patch's block (its scalars hold the outlet pressure)
     double rot;        // wall rotation rate
 };
 
@@ -33,13 +34,14 @@ struct ScalarsDev {
     double iter;
 };
 
+// sc: the scalars of each instance (outlet pressure)
 __global__ void k_bc(const BlockDev *blocks, const PatchDev *patches, int npatch, const long *pstart, Gas g,
-                     const double *pout_dev);
+                     const ScalarsDev *sc);
 // radial-equilibrium outlet, one workgroup per patch (list: patch indices);
 // the radial lines of a face are at most kAxialMaxI cells long
 constexpr int kAxialMaxI = 2048;
 __global__ void k_bc_axial(const BlockDev *blocks, const PatchDev *patches, const int *list, Gas g,
-                           const double *pout_dev);
+                           const ScalarsDev *sc);
 // global time step (timestep type "G"): per-block minimum of the local steps
 // (dtmin as bits of positive doubles), then dt and the LU-SGS diagonal
 __global__ void k_dt_block_min(const BlockDev *blocks, const long *cstart, int nblock, unsigned long long *dtmin);
@@ -62,7 +64,8 @@ struct PeriodicDev {
 };
 __global__ void k_periodic_flip(PeriodicDev *per, int nper, int *pitch_sign);
 __global__ void k_exchange(const BlockDev *blocks, const ExchDev *el, long n, const PeriodicDev *per);
-__global__ void k_spectral_dt(const BlockDev *blocks, const long *cstart, int nblock, Gas g, const double *cfl_dev);
+// the CFL number of each block's instance: BlockDev::cfl
+__global__ void k_spectral_dt(const BlockDev *blocks, const long *cstart, int nblock, Gas g);
 __global__ void k_dt_flag_decrement(const BlockDev *blocks, int nblock);
 template <int TI, int TJ, int TK>
 __global__ void k_convective(const BlockDev *blocks, const int4 *tiles, Gas g, int store_faces, double *fI, double *fJ,
@@ -122,9 +125,12 @@ __global__ void k_update(const BlockDev *blocks, const long *cstart, int nblock,
 __global__ void k_limiter_serial(const BlockDev *blocks, int nblock, Gas g, int *bad_flag);
 __global__ void k_sections(const BlockDev *blocks, const int2 *secs, Gas g, double *sums);
 // norms and section integrals of one level (lv) with the CFL ramp and the
-// iteration count kept in ramp (the fine level's scalars, shared by the levels)
-__global__ void k_finalize(const double *partial, int npart, const double *sums, ScalarsDev *lv, ScalarsDev *ramp, Gas g,
-                           double *hist_row);
+// iteration count kept in ramp (the fine level's scalars, shared by the
+// levels); one workgroup per instance w: its norm partials are
+// parts[w].x .. parts[w].x + parts[w].y - 1, its scalars lv[w] and ramp[w],
+// its history row hist_row + 7 w
+__global__ void k_finalize(const double *partial, int npart, const int2 *parts, const double *sums, ScalarsDev *lv,
+                           ScalarsDev *ramp, Gas g, double *hist_row);
 // two-grid cycle (CSolver::Prolong / Interpolate, CMesh::CornerTreatment,
 // ComputeForceFunction / AddForceFunction)
 __global__ void k_mg_restrict(const BlockDev *fine, const BlockDev *coarse, const long *cstart, int nblock, Gas g,
@@ -162,7 +168,7 @@ __global__ void k_bl_merge(const BlockDev *blocks, const long *cstart, int nbloc
 
 // Spalart-Allmaras (CSpaceDiscr::SA_Model, CMesh::WallDistanceSA,
 // CFluidProps::Limit_SA and the nu~ part of CMesh::LU_SGS_Marching)
-__global__ void k_wall_distance(const BlockDev *blocks, const long *cstart, int nblock, const double *wf, int nf);
+__global__ void k_wall_distance(const BlockDev *blocks, const long *cstart, int nblock, const double *wf);
 template <int TI, int TJ, int TK>
 __global__ void k_sa_faces(const BlockDev *blocks, const int4 *tiles, Gas g);
 // save_old: keep nu~ as it stands (before the clipping) in ev_old, as the

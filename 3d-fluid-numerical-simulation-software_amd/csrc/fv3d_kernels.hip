@@ -24,12 +24,11 @@ __device__ static void set_dummies(const BlockDev &B, int s, int a, int b, const
 }
 
 __global__ void k_bc(const BlockDev *__restrict__ blocks, const PatchDev *__restrict__ patches, int npatch,
-                     const long *__restrict__ pstart, Gas g, const double *__restrict__ pout_dev)
+                     const long *__restrict__ pstart, Gas g, const ScalarsDev *__restrict__ sc)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= pstart[npatch]) return;
-    int q = 0;
-    while (t >= pstart[q + 1]) ++q;
+    const int q = find_range(pstart, npatch, t);
     const PatchDev P = patches[q];
     const BlockDev &B = blocks[P.block];
     long r = t - pstart[q];
@@ -90,7 +89,7 @@ __global__ void k_bc(const BlockDev *__restrict__ blocks, const PatchDev *__rest
         double nx = B.fS[2][xf] / s_, ny = B.fS[2][box + xf] / s_, nz = B.fS[2][2 * box + xf] / s_;
         double d0 = B.W[xi], u0 = B.W[box + xi], v0 = B.W[2 * box + xi], w0 = B.W[3 * box + xi];
         double p0 = B.W[4 * box + xi], T0 = B.W[5 * box + xi];
-        double Pout = *pout_dev;
+        double Pout = sc[P.inst].p_out;
         double dens0 = d0;
         double c0 = sqrt(Gamma * Rgas * T0);
         double Vn = u0 * nx + v0 * ny + w0 * nz;
@@ -154,7 +153,7 @@ __global__ void k_bc(const BlockDev *__restrict__ blocks, const PatchDev *__rest
 // j line, then the ghost cells and Dummy layers.
 __global__ void __launch_bounds__(256) k_bc_axial(const BlockDev *__restrict__ blocks,
                                                   const PatchDev *__restrict__ patches, const int *__restrict__ list,
-                                                  Gas g, const double *__restrict__ pout_dev)
+                                                  Gas g, const ScalarsDev *__restrict__ sc)
 {
     __shared__ double cu[kAxialMaxI];
     const PatchDev P = patches[list[blockIdx.x]];
@@ -177,7 +176,7 @@ __global__ void __launch_bounds__(256) k_bc_axial(const BlockDev *__restrict__ b
     }
     __syncthreads();
     const int iref = (int)(0.5 * IM);
-    const double Pout = *pout_dev;
+    const double Pout = sc[P.inst].p_out;
     auto rad = [&](int i, int j, int kk) {
         const long xf = cidx(B, i, j, kk);
         return sqrt(fc[xf] * fc[xf] + fc[box + xf] * fc[box + xf]);
@@ -227,19 +226,12 @@ __global__ void __launch_bounds__(256) k_dt_block_min(const BlockDev *__restrict
     __shared__ unsigned long long smin[kSlots];
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long t0 = (long)blockIdx.x * blockDim.x;
-    int n0 = 0;
-    while (n0 + 1 < nblock && t0 >= cstart[n0 + 1]) ++n0;
+    const int n0 = t0 < cstart[nblock] ? find_range(cstart, nblock, t0) : nblock - 1;
     if (threadIdx.x < kSlots) smin[threadIdx.x] = ~0ull;
     __syncthreads();
-    if (t < cstart[nblock]) {
-        int n = n0;
-        while (t >= cstart[n + 1]) ++n;
+    int n, i, j, k;
+    if (cell_item(blocks, cstart, nblock, t, n, i, j, k)) {
         const BlockDev &B = blocks[n];
-        long r = t - cstart[n];
-        const int k = (int)(r % B.nk) + 1;
-        r /= B.nk;
-        const int j = (int)(r % B.nj) + 1;
-        const int i = (int)(r / B.nj) + 1;
         const unsigned long long v = (unsigned long long)__double_as_longlong(B.dt[cidx(B, i, j, k)]);
         if (n - n0 < kSlots) atomicMin(&smin[n - n0], v);
         else atomicMin(dtmin + n, v);
@@ -253,15 +245,9 @@ __global__ void __launch_bounds__(256) k_dt_global(const BlockDev *__restrict__ 
                                                    int nblock, Gas g, const unsigned long long *__restrict__ dtmin)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cstart[nblock]) return;
-    int n = 0;
-    while (t >= cstart[n + 1]) ++n;
+    int n, i, j, k;
+    if (!cell_item(blocks, cstart, nblock, t, n, i, j, k)) return;
     const BlockDev &B = blocks[n];
-    long r = t - cstart[n];
-    const int k = (int)(r % B.nk) + 1;
-    r /= B.nk;
-    const int j = (int)(r % B.nj) + 1;
-    const int i = (int)(r / B.nj) + 1;
     const long box = B.box, x = cidx(B, i, j, k);
     const double dt = __longlong_as_double((long long)dtmin[n]);
     B.dt[x] = dt;
@@ -323,19 +309,12 @@ __global__ void k_exchange(const BlockDev *__restrict__ blocks, const ExchDev *_
 // ======================================================= spectral + dt
 // source/Spectral_curl_grad.cpp:9-42 and source/TimeStep.cpp:7-62, with the
 // viscous spectral radii for Navier-Stokes (TimeStep.cpp:38-49).
-__global__ void k_spectral_dt(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart, int nblock, Gas g,
-                              const double *__restrict__ cfl_dev)
+__global__ void k_spectral_dt(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart, int nblock, Gas g)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cstart[nblock]) return;
-    int n = 0;
-    while (t >= cstart[n + 1]) ++n;
+    int n, i, j, k;
+    if (!cell_item(blocks, cstart, nblock, t, n, i, j, k)) return;
     const BlockDev &B = blocks[n];
-    long r = t - cstart[n];
-    int k = (int)(r % B.nk) + 1;
-    r /= B.nk;
-    int j = (int)(r % B.nj) + 1;
-    int i = (int)(r / B.nj) + 1;
     const long box = B.box, x = cidx(B, i, j, k);
     const double Gamma = g.gamma, Rgas = g.rgas;
     double uvelr = B.W[box + x] + B.w_rot * B.cc[box + x];
@@ -368,7 +347,7 @@ __global__ void k_spectral_dt(const BlockDev *__restrict__ blocks, const long *_
         }
         lv = lvd[0] + lvd[1] + lvd[2];
     }
-    double dt = dt_factor * (*cfl_dev) * B.vol[x] / (lc + g.c_timestep * lv);
+    double dt = dt_factor * (*B.cfl) * B.vol[x] / (lc + g.c_timestep * lv);
     B.dt[x] = dt;
     // diagonal of the LU-SGS operator (source/LU_SGS_Marching.cpp:76-77)
     B.alpha[x] = B.vol[x] / dt + g.relax * (lam[0] + lam[1] + lam[2]) + F32C(2.0) * (lvd[0] + lvd[1] + lvd[2]);
@@ -381,8 +360,7 @@ __global__ void k_spectral_dt(const BlockDev *__restrict__ blocks, const long *_
 // (source/TimeStep.cpp:89-92), after every cell has read it.
 __global__ void k_dt_flag_decrement(const BlockDev *__restrict__ blocks, int nblock)
 {
-    int n = threadIdx.x;
-    if (n < nblock) {
+    for (int n = threadIdx.x; n < nblock; n += blockDim.x) {
         int *f = blocks[n].flag_low_dt;
         if (*f > 0) *f = *f - 1;
     }
@@ -604,8 +582,7 @@ __global__ void k_lusgs_plane(const BlockDev *__restrict__ blocks, const long *_
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= pstart[nblock]) return;
-    int n = 0;
-    while (t >= pstart[n + 1]) ++n;
+    const int n = find_range(pstart, nblock, t);
     const BlockDev &B = blocks[n];
     const long r = t - pstart[n];
     const int i = (int)(r / B.nj) + 1, j = (int)(r % B.nj) + 1;
@@ -1118,8 +1095,7 @@ __global__ void __launch_bounds__(256) k_sweep_pack(const BlockDev *__restrict__
     constexpr int TJ = kKbaTJ, TK = kKbaTK;
     __shared__ double buf[kPackChunk][64];
     // the block of this workgroup: gstart holds the running workgroup counts
-    int block = 0;
-    while (block + 1 < nblock && (long)blockIdx.x >= gstart[block + 1]) ++block;
+    const int block = find_range(gstart, nblock, (long)blockIdx.x);
     const BlockDev &B = blocks[block];
     const long wg = (long)blockIdx.x - gstart[block];
     const int nchunk = (B.ni + kPackChunk - 1) / kPackChunk;
@@ -1165,8 +1141,7 @@ __global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict
     constexpr int TJ = kKbaTJ, TK = kKbaTK;
     __shared__ double buf[kPackChunk][64];
     // the block of this workgroup: gstart holds the running workgroup counts
-    int block = 0;
-    while (block + 1 < nblock && (long)blockIdx.x >= gstart[block + 1]) ++block;
+    const int block = find_range(gstart, nblock, (long)blockIdx.x);
     const BlockDev &B = blocks[block];
     const long wg = (long)blockIdx.x - gstart[block];
     const int nchunk = (B.ni + kPackChunk - 1) / kPackChunk;
@@ -1776,15 +1751,9 @@ __global__ void __launch_bounds__(256) k_update(const BlockDev *__restrict__ blo
     __shared__ double red[5][256];
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double rm[5] = {0, 0, 0, 0, 0};
-    if (t < cstart[nblock]) {
-        int n = 0;
-        while (t >= cstart[n + 1]) ++n;
+    int n, i, j, k;
+    if (cell_item(blocks, cstart, nblock, t, n, i, j, k)) {
         const BlockDev &B = blocks[n];
-        long r = t - cstart[n];
-        int k = (int)(r % B.nk) + 1;
-        r /= B.nk;
-        int j = (int)(r % B.nj) + 1;
-        int i = (int)(r / B.nj) + 1;
         const long box = B.box, x = cidx(B, i, j, k);
         double q[5];
         const double vol = B.vol[x];
@@ -1931,16 +1900,22 @@ __global__ void __launch_bounds__(256) k_sections(const BlockDev *__restrict__ b
 // Residual norm (fixed-order sum of the per-workgroup partials), performance
 // parameters (source/Ouput.cpp:97-107,282-300), CFL ramp for the next iteration
 // (source/Solver.cpp:49-60) and the history row.  One workgroup.
-__global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ partial, int npart, const double *__restrict__ secsum,
-                                                  ScalarsDev *lv, ScalarsDev *ramp, Gas g, double *__restrict__ hist_row)
+__global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ partial, int npart, const int2 *__restrict__ parts,
+                                                  const double *__restrict__ secsum, ScalarsDev *lv, ScalarsDev *ramp, Gas g,
+                                                  double *__restrict__ hist_row)
 {
-    ScalarsDev *sc = lv;
+    const int inst = blockIdx.x;
+    ScalarsDev *sc = lv + inst;
+    ramp += inst;
+    secsum += (long)inst * 2 * kSecParts * 3;
+    if (hist_row) hist_row += 7L * inst;
+    const int p0 = parts[inst].x, np = parts[inst].y;
     // component 5: Res_ev of Spalart-Allmaras (k_sa_res)
     const int nc = g.turbulence == 200 ? 6 : 5;
     __shared__ double red[6][256];
     double r[6] = {0, 0, 0, 0, 0, 0};
-    for (int p = threadIdx.x; p < npart; p += blockDim.x)
-        for (int c = 0; c < nc; ++c) r[c] += partial[(long)c * npart + p];
+    for (int p = threadIdx.x; p < np; p += blockDim.x)
+        for (int c = 0; c < nc; ++c) r[c] += partial[(long)c * npart + p0 + p];
     for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = r[c];
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {

@@ -10,13 +10,6 @@
 
 namespace fv3d {
 
-__device__ static inline int find_block(const long *cstart, int nblock, long t)
-{
-    int n = 0;
-    while (n + 1 < nblock && t >= cstart[n + 1]) ++n;
-    return n;
-}
-
 // W from Q for one interior cell (CFluidProps::PrimitiveVars,
 // source/PrimitiveVars.cpp:7-29) and the limiter's bad-cell test
 // (source/Limit_FlowVars.cpp:26-28)
@@ -45,14 +38,9 @@ __global__ void __launch_bounds__(256) k_mg_restrict(const BlockDev *__restrict_
                                                      int *__restrict__ bad_flag)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cstart[nblock]) return;
-    const int n = find_block(cstart, nblock, t);
+    int n, i, j, k;
+    if (!cell_item(coarse, cstart, nblock, t, n, i, j, k)) return;
     const BlockDev &F = fine[n], &B = coarse[n];
-    long r = t - cstart[n];
-    const int k = (int)(r % B.nk) + 1;
-    r /= B.nk;
-    const int j = (int)(r % B.nj) + 1;
-    const int i = (int)(r / B.nj) + 1;
     double sq[5], sr[5];
     bool first = true;
     for (int a = 0; a < 2; ++a)

@@ -25,14 +25,6 @@ __device__ static inline double sa_fv1(double X, double Cv1)
     return pow(X, F32C(3.0)) / (pow(X, F32C(3.0)) + Cv1 * Cv1 * Cv1);
 }
 
-__device__ static inline void cell_of(const BlockDev &B, long r, int &i, int &j, int &k)
-{
-    k = (int)(r % B.nk) + 1;
-    r /= B.nk;
-    j = (int)(r % B.nj) + 1;
-    i = (int)(r / B.nj) + 1;
-}
-
 // nu~ of an interior cell after the clipping of SA_Model (source/SA_Model.cpp:
 // 64-87): negative values become zero, values above 200 times the laminar
 // kinematic viscosity are capped.  The face fluxes and the source see this
@@ -51,43 +43,58 @@ __device__ static inline double sa_clip(double ev, double temp, double dens, con
 // ----------------------------------------------------------- wall distance
 // CMesh::WallDistanceSA (source/WallDistanceSA.cpp:9-132): for every cell the
 // smallest distance from its centroid to the centroid of any wall face, from
-// 10000.  wf holds the wall-face centroids (x, y, z); each workgroup stages
-// 256 of them at a time in LDS.  FP64 throughout: this is the arithmetic-bound
+// 10000.  wf holds the wall-face centroids (x, y, z) of every instance's
+// walls; a block takes those of its own instance (BlockDev::wf0, nwf); each
+// workgroup stages 256 of them at a time in LDS (the instances of the
+// workgroup's first and last cells bound the range it stages).  FP64 throughout: this is the arithmetic-bound
 // set-up kernel (cells x wall faces distance evaluations).  The minimum is
 // taken over the squared distances and rooted once: the correctly rounded
 // square root is monotone, so sqrt(min(d^2, 10000^2)) is the reference's
 // running minimum of the roots started from 10000.
 __global__ void __launch_bounds__(256) k_wall_distance(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
-                                                       int nblock, const double *__restrict__ wf, int nf)
+                                                       int nblock, const double *__restrict__ wf)
 {
     __shared__ double sx[256], sy[256], sz[256];
+    __shared__ int s_lo, s_hi;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = t < cstart[nblock];
-    int n = 0;
+    int n = 0, i, j, k;
+    const bool valid = cell_item(blocks, cstart, nblock, t, n, i, j, k);
     double cx = 0.0, cy = 0.0, cz = 0.0;
     long x = 0;
+    int f0 = 0, f1 = 0;   // this cell's wall faces
     if (valid) {
-        while (t >= cstart[n + 1]) ++n;
-        int i, j, k;
-        cell_of(blocks[n], t - cstart[n], i, j, k);
         const BlockDev &B = blocks[n];
         x = cidx(B, i, j, k);
         cx = B.cc[x]; cy = B.cc[B.box + x]; cz = B.cc[2 * B.box + x];
+        f0 = B.wf0;
+        f1 = B.wf0 + B.nwf;
     }
+    // the union of the workgroup's face ranges (one range unless the
+    // workgroup spans two instances)
+    if (threadIdx.x == 0) { s_lo = 0x7fffffff; s_hi = 0; }
+    __syncthreads();
+    if (valid && f1 > f0) {
+        atomicMin(&s_lo, f0);
+        atomicMax(&s_hi, f1);
+    }
+    __syncthreads();
+    const int lo = s_lo, hi = s_hi;
     double d2min = 10000.0 * 10000.0;
-    for (int base = 0; base < nf; base += 256) {
+    for (int base = lo; base < hi; base += 256) {
         __syncthreads();
         const int q = base + (int)threadIdx.x;
-        if (q < nf) {
+        if (q < hi) {
             sx[threadIdx.x] = wf[3 * (long)q];
             sy[threadIdx.x] = wf[3 * (long)q + 1];
             sz[threadIdx.x] = wf[3 * (long)q + 2];
         }
         __syncthreads();
-        const int m = min(256, nf - base);
+        const int m = min(256, hi - base);
         for (int e = 0; e < m; ++e) {
+            const int q2 = base + e;
             const double dx = cx - sx[e], dy = cy - sy[e], dz = cz - sz[e];
-            d2min = fmin(d2min, dx * dx + dy * dy + dz * dz);
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (q2 >= f0 && q2 < f1) d2min = fmin(d2min, d2);
         }
     }
     if (valid) blocks[n].dwall[x] = sqrt(d2min);
@@ -234,12 +241,9 @@ __global__ void __launch_bounds__(256) k_sa_mut(const BlockDev *__restrict__ blo
                                                 int nblock, Gas g, int save_old)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cstart[nblock]) return;
-    int n = 0;
-    while (t >= cstart[n + 1]) ++n;
+    int n, i, j, k;
+    if (!cell_item(blocks, cstart, nblock, t, n, i, j, k)) return;
     const BlockDev &B = blocks[n];
-    int i, j, k;
-    cell_of(B, t - cstart[n], i, j, k);
     const long box = B.box, x = cidx(B, i, j, k);
     const SaConst K;
     const double Re = g.re_ref;
@@ -268,8 +272,7 @@ __global__ void __launch_bounds__(256) k_sa_ghost(const BlockDev *__restrict__ b
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= gstart[nblock]) return;
-    int n = 0;
-    while (t >= gstart[n + 1]) ++n;
+    const int n = find_range(gstart, nblock, t);
     const BlockDev &B = blocks[n];
     long r = t - gstart[n];
     const long nf[3] = {(long)B.nj * B.nk, (long)B.ni * B.nk, (long)B.ni * B.nj};
@@ -308,8 +311,7 @@ __global__ void k_sa_plane(const BlockDev *__restrict__ blocks, const long *__re
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= pstart[nblock]) return;
-    int n = 0;
-    while (t >= pstart[n + 1]) ++n;
+    const int n = find_range(pstart, nblock, t);
     const BlockDev &B = blocks[n];
     const long r = t - pstart[n];
     const int i = (int)(r / B.nj) + 1, j = (int)(r % B.nj) + 1;
@@ -403,12 +405,9 @@ __global__ void __launch_bounds__(256) k_sa_res(const BlockDev *__restrict__ blo
     __shared__ double red[256];
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0.0;
-    if (t < cstart[nblock]) {
-        int n = 0;
-        while (t >= cstart[n + 1]) ++n;
+    int n, i, j, k;
+    if (cell_item(blocks, cstart, nblock, t, n, i, j, k)) {
         const BlockDev &B = blocks[n];
-        int i, j, k;
-        cell_of(B, t - cstart[n], i, j, k);
         const long x = cidx(B, i, j, k);
         double e = B.ev[x];
         if (clamp && e < F32C(1.0E-8)) {

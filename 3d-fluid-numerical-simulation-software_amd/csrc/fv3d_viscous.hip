@@ -241,15 +241,9 @@ template __global__ void k_viscous<kTI, kTJ, kTK>(const BlockDev *, const int4 *
 __global__ void k_curl(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart, int nblock)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cstart[nblock]) return;
-    int n = 0;
-    while (t >= cstart[n + 1]) ++n;
+    int n, i, j, k;
+    if (!cell_item(blocks, cstart, nblock, t, n, i, j, k)) return;
     const BlockDev &B = blocks[n];
-    long r = t - cstart[n];
-    const int k = (int)(r % B.nk) + 1;
-    r /= B.nk;
-    const int j = (int)(r % B.nj) + 1;
-    const int i = (int)(r / B.nj) + 1;
     const long box = B.box, x = cidx(B, i, j, k);
     const long st[3] = {B.si, B.sj, 1};
     double d[3][3];   // d/dkesi, d/dyita, d/dzita of u v w
@@ -420,15 +414,9 @@ __global__ void __launch_bounds__(256) k_bl_merge(const BlockDev *__restrict__ b
                                                   int nblock)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cstart[nblock]) return;
-    int n = 0;
-    while (t >= cstart[n + 1]) ++n;
+    int n, i, j, k;
+    if (!cell_item(blocks, cstart, nblock, t, n, i, j, k)) return;
     const BlockDev &B = blocks[n];
-    long r = t - cstart[n];
-    const int k = (int)(r % B.nk) + 1;
-    r /= B.nk;
-    const int j = (int)(r % B.nj) + 1;
-    const int i = (int)(r / B.nj) + 1;
     const long x = cidx(B, i, j, k);
     const unsigned long long v = B.blmin[x];
     if (v != kBlUnvisited) {

@@ -7,6 +7,11 @@ exposes the stages of CMesh::ResFlux and CMesh::LU_SGS_Marching
 and the iteration of CSolver::Solver (source/Solver.cpp:47-135), single grid
 or the two-grid cycle.  Without a restart (``reuse_solution = N``) the start
 field comes from CMesh::Initial on the level the reference starts from.
+
+Given a list of cases, ``GpuSolver`` runs them as a batch: one context, each
+case an instance (fv3d_instance_add) advanced by the same launches.  The cases
+share every config value except the outlet pressure; rotation rates are per
+block anyway.
 """
 from __future__ import annotations
 
@@ -46,46 +51,75 @@ def make_config(cfg: CaseConfig, device: int = 0, precision: int = 0) -> _lib.Co
     return s
 
 
+def _shared_config(cfg: _lib.Config) -> tuple:
+    """The config values the instances of a batch must share (all but p_out)."""
+    out = []
+    for name, _ in _lib.Config._fields_:
+        if name == "p_out":
+            continue
+        v = getattr(cfg, name)
+        out.append(tuple(v) if hasattr(v, "__len__") else v)
+    return tuple(out)
+
+
 class GpuSolver:
-    """One libfv3d context holding one case."""
+    """One libfv3d context holding one case, or a batch of cases."""
 
     def __init__(self, case, device: int = 0, precision: int = 0, debug: int = 0):
         self.L = _lib.load()
-        self.case = case
-        self._cfg = make_config(case.cfg, device, precision)
+        self.batch = isinstance(case, (list, tuple))
+        self.cases = list(case) if self.batch else [case]
+        self.case = self.cases[0]
+        self._cfg = make_config(self.case.cfg, device, precision)
+        ref = _shared_config(self._cfg)
+        for cs in self.cases[1:]:
+            if _shared_config(make_config(cs.cfg, device, precision)) != ref:
+                raise ValueError("the cases of a batch share every config value but the outlet pressure")
+            if cs.cfg.reuse_solution.upper() != self.case.cfg.reuse_solution.upper():
+                raise ValueError("the cases of a batch all start from a restart file, or none does")
         h = C.c_void_p()
         rc = self.L.fv3d_ctx_create(C.byref(self._cfg), C.byref(h))
         self.h = h
         self._check(rc, "ctx_create")
-        for b, xyz in enumerate(case.nodes):
-            bi = case.blocks[b]
-            a = np.ascontiguousarray(xyz, dtype=np.float64)
-            rc = self.L.fv3d_block_add(self.h, bi.im, bi.jm, bi.km, a.ctypes.data, bi.nblade, bi.rpm)
-            self._check(rc, "block_add")
-        for p in case.patches:
-            if p.kind in ("interior", "periodic"):
-                r1 = np.ascontiguousarray(p.rng, dtype=np.int32)
-                r2 = np.ascontiguousarray(p.rng2, dtype=np.int32)
-                self._check(self.L.fv3d_connection_add(self.h, 0 if p.kind == "interior" else 1, p.orient, p.block,
-                                                       p.face, r1.ctypes.data, p.block2, p.face2, r2.ctypes.data),
-                            "connection_add")
-                continue
-            if p.kind not in _KIND:
-                raise NotImplementedError(f"patch kind {p.kind} not in this build")
-            r = np.ascontiguousarray(p.rng, dtype=np.int32)
-            self._check(self.L.fv3d_patch_add(self.h, _KIND[p.kind], p.block, p.face, r.ctypes.data, p.rotate),
-                        "patch_add")
+        self.offsets = []   # global number of each instance's first block, less one
+        nblock = 0
+        for cs in self.cases:
+            if self.batch:
+                self._check(self.L.fv3d_instance_add(self.h, cs.cfg.p_out), "instance_add")
+            off = nblock
+            self.offsets.append(off)
+            for b, xyz in enumerate(cs.nodes):
+                bi = cs.blocks[b]
+                a = np.ascontiguousarray(xyz, dtype=np.float64)
+                rc = self.L.fv3d_block_add(self.h, bi.im, bi.jm, bi.km, a.ctypes.data, bi.nblade, bi.rpm)
+                self._check(rc, "block_add")
+            nblock += len(cs.nodes)
+            for p in cs.patches:
+                if p.kind in ("interior", "periodic"):
+                    r1 = np.ascontiguousarray(p.rng, dtype=np.int32)
+                    r2 = np.ascontiguousarray(p.rng2, dtype=np.int32)
+                    self._check(self.L.fv3d_connection_add(self.h, 0 if p.kind == "interior" else 1, p.orient,
+                                                           p.block + off, p.face, r1.ctypes.data, p.block2 + off,
+                                                           p.face2, r2.ctypes.data), "connection_add")
+                    continue
+                if p.kind not in _KIND:
+                    raise NotImplementedError(f"patch kind {p.kind} not in this build")
+                r = np.ascontiguousarray(p.rng, dtype=np.int32)
+                self._check(self.L.fv3d_patch_add(self.h, _KIND[p.kind], p.block + off, p.face, r.ctypes.data,
+                                                  p.rotate), "patch_add")
         self._check(self.L.fv3d_setup(self.h), "setup")
         if debug:
             self._check(self.L.fv3d_set_debug(self.h, debug), "set_debug")
-        if case.cfg.reuse_solution.upper() == "Y":
-            for b, st in enumerate(case.states):
-                pv = np.ascontiguousarray(np.stack([st["rho"], st["u"], st["v"], st["w"], st["p"], st["T"]], -1))
-                e = np.ascontiguousarray(st["rhoE"])
-                self._check(self.L.fv3d_state_upload(self.h, b + 1, pv.ctypes.data, e.ctypes.data), "state_upload")
-                if case.cfg.turbulence == 200 and "nut" in st:
-                    ev = np.ascontiguousarray(st["nut"], dtype=np.float64)
-                    self._check(self.L.fv3d_nut_upload(self.h, b + 1, ev.ctypes.data), "nut_upload")
+        if self.case.cfg.reuse_solution.upper() == "Y":
+            for cs, off in zip(self.cases, self.offsets):
+                for b, st in enumerate(cs.states):
+                    pv = np.ascontiguousarray(np.stack([st["rho"], st["u"], st["v"], st["w"], st["p"], st["T"]], -1))
+                    e = np.ascontiguousarray(st["rhoE"])
+                    self._check(self.L.fv3d_state_upload(self.h, off + b + 1, pv.ctypes.data, e.ctypes.data),
+                                "state_upload")
+                    if cs.cfg.turbulence == 200 and "nut" in st:
+                        ev = np.ascontiguousarray(st["nut"], dtype=np.float64)
+                        self._check(self.L.fv3d_nut_upload(self.h, off + b + 1, ev.ctypes.data), "nut_upload")
         else:
             self._check(self.L.fv3d_initial(self.h), "initial")
 
@@ -108,10 +142,18 @@ class GpuSolver:
         self._check(getattr(self.L, "fv3d_" + stage)(self.h), stage)
 
     def iterate(self, n: int, graph: bool = False) -> np.ndarray:
-        h = np.zeros((max(n, 1), 7))
+        """History rows (n, 7); for a batch (n, instances, 7)."""
+        k = len(self.cases)
+        h = np.zeros((max(n, 1), k, 7))
         fn = self.L.fv3d_iterate_graph if graph else self.L.fv3d_iterate
         self._check(fn(self.h, n, h.ctypes.data), "iterate")
-        return h[:n]
+        return h[:n] if self.batch else h[:n, 0]
+
+    def instance_results(self, inst: int):
+        """(res6, perf5) of instance inst (1-based) after the last iteration."""
+        r, p = np.zeros(6), np.zeros(5)
+        self._check(self.L.fv3d_instance_results(self.h, inst, r.ctypes.data, p.ctypes.data), "instance_results")
+        return r, p
 
     def res_calculate(self) -> np.ndarray:
         r = np.zeros(6)
@@ -145,9 +187,11 @@ class GpuSolver:
     def n_cells(self) -> int:
         return int(self.L.fv3d_cell_count(self.h))
 
-    def get(self, block: int, name: str, level: int = 1) -> np.ndarray:
-        """A field of one block; level 2 reads the coarse level of the two-grid cycle."""
-        bi = self.case.blocks[block - 1]
+    def get(self, block: int, name: str, level: int = 1, inst: int = 1) -> np.ndarray:
+        """A field of one block (of instance inst); level 2 reads the coarse
+        level of the two-grid cycle."""
+        bi = self.cases[inst - 1].blocks[block - 1]
+        gblock = self.offsets[inst - 1] + block
         im, jm, km = bi.im, bi.jm, bi.km
         if level == 2:
             im, jm, km = (im + 1) // 2, (jm + 1) // 2, (km + 1) // 2
@@ -170,7 +214,7 @@ class GpuSolver:
         else:
             raise KeyError(name)
         out = np.zeros(shape)
-        n = self.L.fv3d_level_field_download(self.h, level, block, name.encode(), out.ctypes.data)
+        n = self.L.fv3d_level_field_download(self.h, level, gblock, name.encode(), out.ctypes.data)
         self._check(n, "field_download " + name)
         if n != out.size:
             raise _lib.Fv3dError(f"field {name}: got {n} values, expected {out.size}")

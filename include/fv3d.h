@@ -101,6 +101,25 @@ int fv3d_setup(fv3d_ctx *ctx);
 /* Grid levels in use after fv3d_setup: 1, or 2 for the two-grid cycle. */
 int fv3d_levels(const fv3d_ctx *ctx);
 
+/* Batching: independent passage instances in one context, advanced together by
+ * every launch (no reference counterpart: the reference runs one case per
+ * process, source/Structured_FV3D.cpp:19-214; each instance here behaves as
+ * that run would, bit for bit).  Declare an instance before its blocks; the
+ * blocks added next, and the patches and connections on them, belong to it.
+ * p_out is the instance's outlet static pressure in config.dat units (the
+ * config's p_out is ignored once instances are declared); every other config
+ * value is shared.  Without a declaration all blocks form one instance.
+ * Returns the instance number (1-based). */
+int fv3d_instance_add(fv3d_ctx *ctx, double p_out);
+
+/* Number of instances (1 unless declared). */
+int fv3d_instances(const fv3d_ctx *ctx);
+
+/* Residual norms (as fv3d_res_calculate) and performance integrals (as
+ * fv3d_perf) of one instance after the last iteration; either pointer may be
+ * NULL.  Synchronises. */
+int fv3d_instance_results(fv3d_ctx *ctx, int inst, double *res6, double *perf5);
+
 /* CMesh::Initial (source/ComputeAllBlockVars.cpp:38-48; Initial_Uniform and
  * Initial_Interpolate, source/Initialization.cpp:17-165): the start field when
  * there is no restart file (reuse_solution = N), on the level the reference
@@ -157,10 +176,12 @@ int fv3d_resflux(fv3d_ctx *ctx);           /* all of CMesh::ResFlux */
  * forward and backward hyperplane sweeps, update, PrimitiveVars, Limit_Vars. */
 int fv3d_lusgs(fv3d_ctx *ctx);
 
-/* CMesh::Res_Calculate (source/ComputeAllBlockVars.cpp:318-372); synchronises. */
+/* CMesh::Res_Calculate (source/ComputeAllBlockVars.cpp:318-372) of the first
+ * instance; synchronises. */
 int fv3d_res_calculate(fv3d_ctx *ctx, double *res6);
 
-/* COutput::OutputGeneralParameters (source/Ouput.cpp:282-300); synchronises.
+/* COutput::OutputGeneralParameters (source/Ouput.cpp:282-300) of the first
+ * instance; synchronises.
  * out5 = mass_in, mass_out, PR, efficiency, mass-flow ratio. */
 int fv3d_perf(fv3d_ctx *ctx, double *out5);
 
@@ -170,7 +191,8 @@ int fv3d_perf(fv3d_ctx *ctx, double *out5);
  * coarse LU-SGS up to Coarse_iter, the coarse-to-fine state interpolation at
  * Coarse_iter, then per iteration fine LU-SGS, fine residual, restriction,
  * forced coarse LU-SGS and the prolonged correction.  hist (may be NULL)
- * receives n rows of: iter res_dens mass_in mass_out PR eff CFL.
+ * receives n rows of: iter res_dens mass_in mass_out PR eff CFL, per instance
+ * (n x instances rows, the instances of an iteration consecutive).
  * Synchronises once at the end. */
 int fv3d_iterate(fv3d_ctx *ctx, int n, double *hist);
 
