@@ -94,9 +94,13 @@ struct fv3d_ctx {
     std::vector<int> bl_launch_n;
     std::vector<int> bl_groups;
     long *d_pstart_bc = nullptr, *d_cstart = nullptr, *d_pstart_plane = nullptr;
-    int4 *d_tiles = nullptr;
+    // residual tiles: one list per tile shape (built on first use), and the
+    // shape each residual kernel uses (0 convective, 1 viscous, 2 nu~)
+    int4 *d_tiles_s[kNumTiles] = {};
+    int ntiles_s[kNumTiles] = {};
+    int tile_sel[3] = {0, 0, 0};
     int2 *d_secs = nullptr;
-    int npatch_dev = 0, ntiles = 0, nplanes = 0;
+    int npatch_dev = 0, nplanes = 0;
     long nbc = 0, ncell = 0, nplane_threads = 0;
     long ncell_real = 0;             // interior cells (ncell also counts the instances' padding)
     double *d_partial = nullptr;
@@ -300,6 +304,26 @@ static int upload(fv3d_ctx *c, HostBlock &b, T **dst, const std::vector<T> &src)
     b.allocs.push_back(*dst);
     HIPCK(c, hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
     return 0;
+}
+
+// The residual tiles of shape q over every block (block, first i, j, k), on
+// the device; built once per shape.
+static int ensure_tiles(fv3d_ctx *c, int q)
+{
+    if (c->d_tiles_s[q]) return FV3D_OK;
+    const int TI = kTileShape[q][0], TJ = kTileShape[q][1], TK = kTileShape[q][2];
+    std::vector<int4> tiles;
+    for (int n = 0; n < (int)c->blocks.size(); ++n) {
+        const BlockDev &B = c->blocks[n].dev;
+        for (int i0 = 0; i0 < B.ni; i0 += TI)
+            for (int j0 = 0; j0 < B.nj; j0 += TJ)
+                for (int k0 = 0; k0 < B.nk; k0 += TK) tiles.push_back(make_int4(n, i0, j0, k0));
+    }
+    int rc = dalloc(c, &c->d_tiles_s[q], tiles.size());
+    if (rc) return rc;
+    HIPCK(c, hipMemcpyAsync(c->d_tiles_s[q], tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    c->ntiles_s[q] = (int)tiles.size();
+    return FV3D_OK;
 }
 
 static int setup_block(fv3d_ctx *c, HostBlock &b, int index)
@@ -675,7 +699,9 @@ int fv3d_ctx_destroy(fv3d_ctx *c)
     if (c->vdbg.fvc) (void)hipFree(c->vdbg.fvc);
     for (HostBlock &b : c->blocks)
         for (void *p : b.allocs) (void)hipFree(p);
-    void *ps[] = {c->d_blocks, c->d_patches, c->d_pstart_bc, c->d_cstart, c->d_pstart_plane, c->d_tiles, c->d_secs,
+    for (int q = 0; q < kNumTiles; ++q)
+        if (c->d_tiles_s[q]) (void)hipFree(c->d_tiles_s[q]);
+    void *ps[] = {c->d_blocks, c->d_patches, c->d_pstart_bc, c->d_cstart, c->d_pstart_plane, c->d_secs,
                   c->d_partial, c->d_sums, c->d_bad, c->d_flags, c->d_sc, c->d_hist, c->d_stage_row, c->d_kba[0], c->d_kba[1], c->d_kba_sync, c->d_faces[0], c->d_faces[1], c->d_faces[2]};
     for (void *p : ps)
         if (p) (void)hipFree(p);
@@ -1076,7 +1102,6 @@ int fv3d_setup(fv3d_ctx *c)
     HIPCK(c, hipMemcpyAsync(c->d_secs, secs.data(), secs.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     // cell and plane work lists
     std::vector<long> cst{0}, plst{0};
-    std::vector<int4> tiles;
     c->nplanes = 0;
     c->parts_host.assign(c->ninst, make_int2(0, 0));
     c->ncell_real = 0;
@@ -1095,19 +1120,14 @@ int fv3d_setup(fv3d_ctx *c)
             c->parts_host[w].y = (int)((cst.back() + 255) / 256) - c->parts_host[w].x;
         plst.push_back(plst.back() + (long)B.ni * B.nj);
         c->nplanes = std::max(c->nplanes, B.IM + B.JM + B.KM - 5);
-        for (int i0 = 0; i0 < B.ni; i0 += kTI)
-            for (int j0 = 0; j0 < B.nj; j0 += kTJ)
-                for (int k0 = 0; k0 < B.nk; k0 += kTK) tiles.push_back(make_int4(n, i0, j0, k0));
     }
     c->ncell = cst.back();
     c->nplane_threads = plst.back();
-    c->ntiles = (int)tiles.size();
     if ((rc = dalloc(c, &c->d_cstart, cst.size()))) return rc;
     HIPCK(c, hipMemcpyAsync(c->d_cstart, cst.data(), cst.size() * sizeof(long), hipMemcpyHostToDevice, c->stream));
     if ((rc = dalloc(c, &c->d_pstart_plane, plst.size()))) return rc;
     HIPCK(c, hipMemcpyAsync(c->d_pstart_plane, plst.data(), plst.size() * sizeof(long), hipMemcpyHostToDevice, c->stream));
-    if ((rc = dalloc(c, &c->d_tiles, tiles.size()))) return rc;
-    HIPCK(c, hipMemcpyAsync(c->d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    if ((rc = ensure_tiles(c, 0))) return rc;
     // wavefront tiles: forward in increasing, backward in decreasing tile
     // diagonal, so every upstream tile holds a smaller ticket
     {
@@ -1404,8 +1424,17 @@ static int launch_spectral_dt(fv3d_ctx *c)
 
 static int launch_convective(fv3d_ctx *c)
 {
-    hipLaunchKernelGGL((k_convective<kTI, kTJ, kTK>), dim3((unsigned)c->ntiles), dim3(kTI * kTJ * kTK), 0, c->stream,
-                       c->d_blocks, c->d_tiles, c->gas, c->store_faces, c->d_faces[0], c->d_faces[1], c->d_faces[2]);
+    const int q = c->tile_sel[0];
+    const dim3 g((unsigned)c->ntiles_s[q]), b(256);
+    switch (q) {
+#define FV3D_CONV(S, A, B, C)                                                                                    \
+    case S:                                                                                                       \
+        hipLaunchKernelGGL((k_convective<A, B, C>), g, b, 0, c->stream, c->d_blocks, c->d_tiles_s[q], c->gas,       \
+                           c->store_faces, c->d_faces[0], c->d_faces[1], c->d_faces[2]);                         \
+        break;
+        FV3D_FOR_TILES(FV3D_CONV)
+#undef FV3D_CONV
+    }
     HIPCK(c, hipGetLastError());
     return FV3D_OK;
 }
@@ -1415,10 +1444,33 @@ static int launch_convective(fv3d_ctx *c)
 static int launch_viscous(fv3d_ctx *c)
 {
     if (!c->cfg.viscous) return FV3D_OK;
-    hipLaunchKernelGGL((k_viscous<kTI, kTJ, kTK>), dim3((unsigned)c->ntiles), dim3(kTI * kTJ * kTK), 0, c->stream,
-                       c->d_blocks, c->d_tiles, c->gas, c->store_faces, c->vdbg);
+    const int q = c->tile_sel[1];
+    const dim3 g((unsigned)c->ntiles_s[q]), b(256);
+    switch (q) {
+#define FV3D_VISC(S, A, B, C)                                                                                    \
+    case S:                                                                                                       \
+        hipLaunchKernelGGL((k_viscous<A, B, C>), g, b, 0, c->stream, c->d_blocks, c->d_tiles_s[q], c->gas,          \
+                           c->store_faces, c->vdbg);                                                             \
+        break;
+        FV3D_FOR_TILES(FV3D_VISC)
+#undef FV3D_VISC
+    }
     HIPCK(c, hipGetLastError());
     return FV3D_OK;
+}
+
+static void launch_sa_faces(fv3d_ctx *c)
+{
+    const int q = c->tile_sel[2];
+    const dim3 g((unsigned)c->ntiles_s[q]), b(256);
+    switch (q) {
+#define FV3D_SAF(S, A, B, C)                                                                                     \
+    case S:                                                                                                       \
+        hipLaunchKernelGGL((k_sa_faces<A, B, C>), g, b, 0, c->stream, c->d_blocks, c->d_tiles_s[q], c->gas);       \
+        break;
+        FV3D_FOR_TILES(FV3D_SAF)
+#undef FV3D_SAF
+    }
 }
 
 // Spalart-Allmaras in the residual evaluation: the vorticity, the nu~
@@ -1430,8 +1482,7 @@ static int launch_sa(fv3d_ctx *c, int save_old)
     const int nb = (int)c->blocks.size();
     const unsigned gc = (unsigned)((c->ncell + 255) / 256);
     hipLaunchKernelGGL(k_curl, dim3(gc), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb);
-    hipLaunchKernelGGL((k_sa_faces<kTI, kTJ, kTK>), dim3((unsigned)c->ntiles), dim3(kTI * kTJ * kTK), 0, c->stream,
-                       c->d_blocks, c->d_tiles, c->gas);
+    launch_sa_faces(c);
     hipLaunchKernelGGL(k_sa_mut, dim3(gc), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb, c->gas, save_old);
     hipLaunchKernelGGL(k_sa_ghost, dim3((unsigned)((c->nghost + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
                        c->d_gstart, nb, c->gas);
@@ -2013,6 +2064,68 @@ int fv3d_timing(fv3d_ctx *c, double *ms, long *counts)
 }
 
 long fv3d_cell_count(const fv3d_ctx *c) { return c ? c->ncell_real : 0; }
+
+// Invalidates the captured graphs after a change of launch configuration.
+static void drop_graphs(fv3d_ctx *c)
+{
+    if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
+    if (c->graph_coarse) { (void)hipGraphExecDestroy(c->graph_coarse); c->graph_coarse = nullptr; }
+}
+
+int fv3d_set_tiles(fv3d_ctx *c, const int *shape3)
+{
+    READY(c);
+    if (!shape3) return FV3D_EINVAL;
+    for (int k = 0; k < 3; ++k) {
+        if (shape3[k] < 0 || shape3[k] >= kNumTiles) return fail(c, FV3D_EINVAL, "tile shape %d", shape3[k]);
+        int rc = ensure_tiles(c, shape3[k]);
+        if (rc) return rc;
+    }
+    for (int k = 0; k < 3; ++k) c->tile_sel[k] = shape3[k];
+    drop_graphs(c);
+    return FV3D_OK;
+}
+
+int fv3d_autotune(fv3d_ctx *c, int reps, int *shape3)
+{
+    READY(c);
+    if (reps < 1) return fail(c, FV3D_EINVAL, "reps=%d", reps);
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    const int saved = c->store_faces;
+    c->store_faces = 0;
+    const bool use[3] = {true, c->cfg.viscous != 0, c->cfg.turbulence == 200};
+    int rc = FV3D_OK;
+    for (int kern = 0; kern < 3 && !rc; ++kern) {
+        if (shape3) shape3[kern] = -1;
+        if (!use[kern]) continue;
+        auto run = [&]() -> int {
+            if (kern == 0) return launch_convective(c);
+            if (kern == 1) return launch_viscous(c);
+            launch_sa_faces(c);
+            HIPCK(c, hipGetLastError());
+            return FV3D_OK;
+        };
+        float best = 0.0f;
+        int bestq = 0;
+        for (int q = 0; q < kNumTiles && !rc; ++q) {
+            if ((rc = ensure_tiles(c, q))) break;
+            c->tile_sel[kern] = q;
+            if ((rc = run())) break;   // warm-up
+            HIPCK(c, hipEventRecord(c->ev[0], c->stream));
+            for (int r = 0; r < reps && !rc; ++r) rc = run();
+            HIPCK(c, hipEventRecord(c->ev[1], c->stream));
+            HIPCK(c, hipEventSynchronize(c->ev[1]));
+            float ms = 0.0f;
+            HIPCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            if (q == 0 || ms < best) { best = ms; bestq = q; }
+        }
+        c->tile_sel[kern] = bestq;
+        if (shape3) shape3[kern] = bestq;
+    }
+    c->store_faces = saved;
+    drop_graphs(c);
+    return rc;
+}
 
 int fv3d_set_debug(fv3d_ctx *c, int flags)
 {

@@ -6,8 +6,14 @@
 
 namespace fv3d {
 
-// convective-residual tile (cells per workgroup = kTI*kTJ*kTK)
+// tile of the residual kernels (k_convective, k_viscous, k_sa_faces; 256
+// cells per workgroup): the shapes they are built for, the default first; the
+// autotuner picks one per kernel (results do not depend on it: every face is
+// evaluated once and every cell sums its faces in a fixed order)
 constexpr int kTI = 4, kTJ = 4, kTK = 16;
+constexpr int kNumTiles = 5;
+constexpr int kTileShape[kNumTiles][3] = {{4, 4, 16}, {2, 8, 16}, {4, 8, 8}, {8, 4, 8}, {8, 8, 4}};
+#define FV3D_FOR_TILES(X) X(0, 4, 4, 16) X(1, 2, 8, 16) X(2, 4, 8, 8) X(3, 8, 4, 8) X(4, 8, 8, 4)
 
 struct PatchDev {
     int kind;          // 0 inlet, 1 outlet, 2 wall

@@ -541,7 +541,10 @@ __global__ void __launch_bounds__(TI *TJ *TK) k_convective(const BlockDev *__res
     }
 }
 
-template __global__ void k_convective<kTI, kTJ, kTK>(const BlockDev *, const int4 *, Gas, int, double *, double *, double *);
+#define FV3D_CONV(S, A, B, C) \
+    template __global__ void k_convective<A, B, C>(const BlockDev *, const int4 *, Gas, int, double *, double *, double *);
+FV3D_FOR_TILES(FV3D_CONV)
+#undef FV3D_CONV
 
 // ================================================================ LU-SGS
 // Flux difference of CTimeDiscr::delt_Fn_up / delt_Fn_down

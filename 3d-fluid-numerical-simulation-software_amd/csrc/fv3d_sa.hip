@@ -230,7 +230,9 @@ __global__ void __launch_bounds__(TI *TJ *TK) k_sa_faces(const BlockDev *__restr
     B.res_ev[x] = cfc - cfv;
 }
 
-template __global__ void k_sa_faces<kTI, kTJ, kTK>(const BlockDev *, const int4 *, Gas);
+#define FV3D_SAF(S, A, B, C) template __global__ void k_sa_faces<A, B, C>(const BlockDev *, const int4 *, Gas);
+FV3D_FOR_TILES(FV3D_SAF)
+#undef FV3D_SAF
 
 // ----------------------------------------------------------- eddy viscosity
 // mue_t = fv1 rho nu~ / Re of the interior cells from nu~ as it stands, then

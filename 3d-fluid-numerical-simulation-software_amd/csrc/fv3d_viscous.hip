@@ -234,7 +234,9 @@ __global__ void __launch_bounds__(TI *TJ *TK) k_viscous(const BlockDev *__restri
     }
 }
 
-template __global__ void k_viscous<kTI, kTJ, kTK>(const BlockDev *, const int4 *, Gas, int, ViscDebug);
+#define FV3D_VISC(S, A, B, C) template __global__ void k_viscous<A, B, C>(const BlockDev *, const int4 *, Gas, int, ViscDebug);
+FV3D_FOR_TILES(FV3D_VISC)
+#undef FV3D_VISC
 
 // ---------------------------------------------------------------- vorticity
 // CFluidProps::Curl (source/Spectral_curl_grad.cpp:44-83), one thread per cell.

@@ -159,6 +159,40 @@ def with_sa(case: Case, amp: float = 0.5) -> Case:
     return case
 
 
+def passage_batch(n: int = 64, seed: int = 3, min_cells: float = 1e5, max_cells: float = 2e6,
+                  turbulence: int = 200, design_rpm: float = 16000.0, design_p_ratio: float = 0.76,
+                  max_iter: int = 500) -> List[Case]:
+    """Independent passages of uneven size for a batched run: cell counts drawn
+    log-uniform in [min_cells, max_cells], each passage the five-block
+    cambered-blade family (64x64 cross-section and k-blocks 12/40/128/60/24 at
+    1.08 M cells, every direction scaled by the cube root of the size ratio,
+    even cell counts), outlet pressure design_p_ratio * Pt times a
+    back-pressure factor drawn uniform in [0.95, 1.25], rotation design_rpm
+    times a factor drawn uniform in [0.7, 1.05].  Navier-Stokes with
+    Spalart-Allmaras (turbulence 200, nu~ = ev_ratio) or Baldwin-Lomax (100)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    base_k = (12, 40, 128, 60, 24)
+    for q in range(n):
+        cells = float(np.exp(rng.uniform(np.log(min_cells), np.log(max_cells))))
+        bp = float(rng.uniform(0.95, 1.25))
+        sp = float(rng.uniform(0.7, 1.05))
+        f = (cells / (64.0 * 64.0 * sum(base_k))) ** (1.0 / 3.0)
+        even = lambda x: max(4, 2 * int(round(x / 2.0)))
+        s = even(64 * f)
+        nk = tuple(even(k * f) for k in base_k)
+        c = annular_passage(ni=s, nj=s, nk_blocks=nk, seed=seed * 1000 + q, rpm=design_rpm * sp, blade_blocks=(2,),
+                            camber=0.1, viscous=True, len_ref=1.0, max_iter=max_iter,
+                            z_extent=(-0.06, -0.035, -0.015, 0.035, 0.055, 0.08))
+        c.cfg.p_out = design_p_ratio * 101325.0 * bp
+        if turbulence == 200:
+            c = with_sa(c, amp=0.0)
+        else:
+            c.cfg.turbulence = turbulence
+        out.append(c)
+    return out
+
+
 def from_meta(meta: Dict) -> Case:
     """The case a fixture's metadata describes: ``kwargs`` for bump_channel (or
     for annular_passage when ``generator`` names it, with ``turn`` a list of

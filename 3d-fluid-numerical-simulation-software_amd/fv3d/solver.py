@@ -149,6 +149,20 @@ class GpuSolver:
         self._check(fn(self.h, n, h.ctypes.data), "iterate")
         return h[:n] if self.batch else h[:n, 0]
 
+    TILE_SHAPES = [(4, 4, 16), (2, 8, 16), (4, 8, 8), (8, 4, 8), (8, 8, 4)]   # fv3d_set_tiles numbering
+
+    def set_tiles(self, shapes):
+        """Tile shape numbers of the convective, viscous and nu~ residual kernels."""
+        a = np.ascontiguousarray(shapes, dtype=np.int32)
+        self._check(self.L.fv3d_set_tiles(self.h, a.ctypes.data), "set_tiles")
+
+    def autotune(self, reps: int = 3):
+        """Times every tile shape of the residual kernels and keeps the fastest;
+        returns the chosen shape numbers (-1 for a kernel not in use)."""
+        a = np.zeros(3, dtype=np.int32)
+        self._check(self.L.fv3d_autotune(self.h, reps, a.ctypes.data), "autotune")
+        return [int(x) for x in a]
+
     def instance_results(self, inst: int):
         """(res6, perf5) of instance inst (1-based) after the last iteration."""
         r, p = np.zeros(6), np.zeros(5)

@@ -63,7 +63,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["passage5", "bump"], default="passage5")
+    ap.add_argument("--workload", choices=["passage5", "bump", "batch"], default="passage5")
+    ap.add_argument("--instances", type=int, default=64, help="batch workload: number of passages")
+    ap.add_argument("--batch-seed", type=int, default=3)
+    ap.add_argument("--batch-rpm", type=float, default=0.0,
+                    help="batch workload: design rotation speed (the synthetic passage is stationary by design)")
+    ap.add_argument("--compare-alone", action="store_true",
+                    help="batch workload: also time every passage alone, one after the other")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default residual tile shapes")
     ap.add_argument("--ni", type=int, default=64, help="bump workload: cells along i (span)")
     ap.add_argument("--nj", type=int, default=64, help="bump workload: cells along j")
     ap.add_argument("--nk", type=int, default=256, help="bump workload: cells along k (streamwise)")
@@ -121,6 +128,14 @@ def cpu_baseline(iters: int):
 
 def workload(a):
     from fv3d import cases
+    if a.workload == "batch":
+        cs = cases.passage_batch(n=a.instances, seed=a.batch_seed, design_rpm=a.batch_rpm, max_iter=a.steps)
+        ncell = sum((bi.im - 1) * (bi.jm - 1) * (bi.km - 1) for c in cs for bi in c.blocks)
+        name = (f"{a.instances} independent five-block blade passages in one batched context ({ncell:,} cells; "
+                f"sizes log-uniform in [0.1M, 2M] cells, seed {a.batch_seed}; back-pressure factor U[0.95, 1.25] "
+                f"on 0.76 Pt; rotation U[0.7, 1.05] x {a.batch_rpm:g} rpm), Navier-Stokes + Spalart-Allmaras, "
+                "Roe + MUSCL3 primitive, LU-SGS local dt CFL 5, single grid level")
+        return cs, name, f"batch{a.instances}s{a.batch_seed}", True
     if a.workload == "bump":
         case = cases.bump_channel(ni=a.ni, nj=a.nj, nk=a.nk, seed=1, max_iter=a.steps, viscous=a.viscous,
                                   len_ref=1e-4 if a.viscous else 1.0)
@@ -154,11 +169,20 @@ def workload(a):
 
 def time_form(case, precision, a):
     """Whole-job throughput of one form and its per-kernel-class device times."""
+    import numpy as np
     from fv3d.solver import GpuSolver
     dbg = {"plane": 2, "kba": 4, "kba2": 8, "kba3": 16, "rows": 0}[a.sweep]
     g = GpuSolver(case, device=0, precision=PREC[precision], debug=dbg)
     graph = not a.no_graph
-    g.iterate(a.warmup, graph=graph)
+    tiles = None
+    if a.no_autotune or a.warmup < 2:
+        g.iterate(a.warmup, graph=graph)
+    else:
+        # residual tile shapes picked on the running state, then one more
+        # warm-up iteration (the graphs are captured again)
+        g.iterate(a.warmup - 1, graph=graph)
+        tiles = [GpuSolver.TILE_SHAPES[q] if q >= 0 else None for q in g.autotune(3)]
+        g.iterate(1, graph=graph)
     g.sync()
     t0 = time.perf_counter()
     hist = g.iterate(a.steps, graph=graph)
@@ -173,8 +197,30 @@ def time_form(case, precision, a):
     g.close()
     classes = ["residual", "lusgs", "update", "bc_spectral_dt", "finalize", "forward_sweep", "backward_sweep",
                "coarse_correction"]
+    last = hist[-1, :, 1] if hist.ndim == 3 else hist[-1:, 1]
     return {"value": ncell * a.steps / secs, "ms_per_step": 1e3 * secs / a.steps, "cells": ncell,
-            "class_ms": {cl: ms[i] for i, cl in enumerate(classes)}, "final_res_dens": float(hist[-1, 1])}
+            "class_ms": {cl: ms[i] for i, cl in enumerate(classes)}, "final_res_dens": float(np.max(last)),
+            "tiles": tiles}
+
+
+def time_alone(cases_, precision, a):
+    """The passages of a batch run one after the other, each in its own
+    context: whole-job throughput over the same iterations."""
+    from fv3d.solver import GpuSolver
+    total_s, cells = 0.0, 0
+    for c in cases_:
+        g = GpuSolver(c, device=0, precision=PREC[precision])
+        g.iterate(a.warmup, graph=True)
+        g.sync()
+        t0 = time.perf_counter()
+        g.iterate(a.steps, graph=True)
+        g.sync()
+        total_s += time.perf_counter() - t0
+        cells += g.n_cells
+        g.close()
+        print(f"alone: passage {cases_.index(c) + 1}/{len(cases_)} timed", file=sys.stderr,
+              flush=True)
+    return {"value": cells * a.steps / total_s, "seconds": total_s}
 
 
 def time_to_drop(case, precision, target, max_iter, skip=0, chunk=500):
@@ -227,12 +273,15 @@ def main():
     r = res[chosen]
     # dominant kernel: the forward sweep of the chosen form, timed alone with HIP
     # events on the library's stream (the only stream its kernels run on)
-    words = SWEEP_WORDS[viscous] + (SWEEP_WORDS_SA if case.cfg.turbulence == 200 else 0)
+    first = case[0] if isinstance(case, list) else case
+    words = SWEEP_WORDS[viscous] + (SWEEP_WORDS_SA if first.cfg.turbulence == 200 else 0)
     kbytes = words * (4 if chosen == "fp32" else 8)
     kms = r["class_ms"]["forward_sweep"]
     achieved = kbytes * r["cells"] / (kms * 1e-3) / 1e9
-    drop = drop_w = None
-    if not a.no_drop:
+    drop = drop_w = alone = None
+    if a.workload == "batch" and a.compare_alone:
+        alone = time_alone(case, chosen, a)
+    if not a.no_drop and a.workload != "batch":
         from fv3d import cases
         c0 = cases.bump_channel(ni=32, nj=32, nk=96, seed=1, max_iter=a.drop_max_iter)
         drop = drop_pair(c0, forms, 1e-6, a.drop_max_iter, 0,
@@ -268,7 +317,11 @@ def main():
                      "bytes_per_cell": kbytes, "class_ms": r["class_ms"]},
         "cpu_baseline": cpu,
         "final_res_dens": r["final_res_dens"],
+        "tiles": r["tiles"],
     }
+    if alone:
+        out["batch_vs_alone"] = {"alone_value": alone["value"], "alone_seconds": alone["seconds"],
+                                 "speedup": r["value"] / alone["value"]}
     print(json.dumps(out))
 
 

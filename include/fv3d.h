@@ -221,6 +221,20 @@ int fv3d_timing(fv3d_ctx *ctx, double *ms, long *counts);
  * set the chunk and extra lag of the wavefront forms. */
 int fv3d_set_debug(fv3d_ctx *ctx, int flags);
 
+/* Tile shapes of the residual kernels (no reference counterpart: the
+ * reference loops over faces).  Each of the three residual kernels -- 0
+ * convective, 1 viscous, 2 Spalart-Allmaras nu~ -- runs with one of the shapes
+ * (cells along i x j x k per workgroup): 0 4x4x16 (default), 1 2x8x16,
+ * 2 4x8x8, 3 8x4x8, 4 8x8x4.  Results do not depend on the shape.
+ * fv3d_set_tiles sets the three shape numbers; fv3d_autotune times every shape
+ * of every kernel in use on the current state (reps launches each, after one
+ * warm-up), keeps the fastest and writes the chosen numbers (-1 for a kernel
+ * not in use) to shape3 (may be NULL).  Both drop the captured graphs; call
+ * them between iterations (the timing runs overwrite the residual, which the
+ * next iteration forms again). */
+int fv3d_set_tiles(fv3d_ctx *ctx, const int *shape3);
+int fv3d_autotune(fv3d_ctx *ctx, int reps, int *shape3);
+
 /* Number of interior cells over all blocks. */
 long fv3d_cell_count(const fv3d_ctx *ctx);
 
