@@ -6,6 +6,7 @@
 // runs on the context's stream without host round trips.
 #include "fv3d.h"
 #include "fv3d_kernels.h"
+#include "fv3d_mlp.h"
 
 #include <math.h>
 #include <stdarg.h>
@@ -145,6 +146,13 @@ struct fv3d_ctx {
     long nghost = 0;
     int *d_sa_hi = nullptr;
     bool ev_set = false;
+    // learned closure: FP32 parameters, Adam moments, gradient, gate bounds,
+    // gate counter, training scratch; closure on = mue_t from the network in
+    // place of the nu~ transport
+    float *d_mlp = nullptr, *d_mlp_m = nullptr, *d_mlp_v = nullptr, *d_mlp_g = nullptr, *d_mlp_bounds = nullptr;
+    unsigned *d_mlp_tripped = nullptr;
+    long mlp_step = 0;
+    bool mlp_closure = false;
     bool owns_stream = true;
     bool mg_coarse = false;          // allocate cv_2h / Res_2h (coarse level)
     int ng = 1;
@@ -576,6 +584,10 @@ static int build_exchange(fv3d_ctx *c)
     return FV3D_OK;
 }
 
+// Whether the nu~ equation is solved (Spalart-Allmaras without the learned
+// closure in its place).
+static bool sa_transport(const fv3d_ctx *c) { return c->cfg.turbulence == 200 && !c->mlp_closure; }
+
 // Spalart-Allmaras set-up: the ghost-cell work list of k_sa_ghost, the
 // limiter flags and the wall distance of every cell (CMesh::WallDistanceSA,
 // source/WallDistanceSA.cpp:9-132): the centroids of all wall faces of all
@@ -690,6 +702,9 @@ int fv3d_ctx_destroy(fv3d_ctx *c)
     if (c->d_dtmin) (void)hipFree(c->d_dtmin);
     if (c->d_gstart) (void)hipFree(c->d_gstart);
     if (c->d_parts) (void)hipFree(c->d_parts);
+    for (void *p : {(void *)c->d_mlp, (void *)c->d_mlp_m, (void *)c->d_mlp_v, (void *)c->d_mlp_g, (void *)c->d_mlp_bounds,
+                    (void *)c->d_mlp_tripped})
+        if (p) (void)hipFree(p);
     if (c->d_sa_hi) (void)hipFree(c->d_sa_hi);
     if (c->d_pack_start) (void)hipFree(c->d_pack_start);
     for (int d = 0; d < 3; ++d) {
@@ -1496,6 +1511,16 @@ static int launch_sa(fv3d_ctx *c, int save_old)
 // Spalart-Allmaras.
 static int launch_turbulence(fv3d_ctx *c, int save_old = 0)
 {
+    if (c->cfg.turbulence == 200 && c->mlp_closure) {
+        // the learned closure in place of the nu~ transport; ghost mue_t as for SA
+        const int nb = (int)c->blocks.size();
+        hipLaunchKernelGGL(k_mlp_closure, dim3((unsigned)((c->ncell + 63) / 64)), dim3(256), 0, c->stream, c->d_blocks,
+                           c->d_cstart, nb, c->gas, c->d_mlp, c->d_mlp_bounds, c->d_mlp_tripped);
+        hipLaunchKernelGGL(k_sa_ghost, dim3((unsigned)((c->nghost + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
+                           c->d_gstart, nb, c->gas);
+        HIPCK(c, hipGetLastError());
+        return FV3D_OK;
+    }
     if (c->cfg.turbulence == 200) return launch_sa(c, save_old);
     if (c->cfg.turbulence != 100) return FV3D_OK;
     const int nb = (int)c->blocks.size();
@@ -1537,7 +1562,7 @@ static int launch_curl(fv3d_ctx *c)
 // default row-layout sweep carries nu~ itself.
 static void launch_sa_sweep(fv3d_ctx *c, bool fwd)
 {
-    if (c->cfg.turbulence != 200) return;
+    if (!sa_transport(c)) return;
     const unsigned g = (unsigned)((c->nplane_threads + 255) / 256);
     const int nb = (int)c->blocks.size();
     for (int s = 0; s < c->nplanes; ++s) {
@@ -1560,7 +1585,7 @@ static int launch_sweeps_kba(fv3d_ctx *c)
     if (c->sweep_mode == 4) {
         const bool f32 = c->cfg.precision == FV3D_DEFECT_FP32;
         const bool visc = c->cfg.viscous;
-        const bool sa = c->cfg.turbulence == 200;   // the nu~ equation inside the sweep
+        const bool sa = sa_transport(c);   // the nu~ equation inside the sweep
         const int nb = (int)c->blocks.size();
         const unsigned pg = (unsigned)c->pack_groups;
         if (f32) {
@@ -1602,9 +1627,11 @@ static int launch_sweeps_kba(fv3d_ctx *c)
         };
         auto unpack = [&]() {
             if (f32)
-                hipLaunchKernelGGL(k_sweep_unpack<float>, dim3(pg), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb);
+                hipLaunchKernelGGL(k_sweep_unpack<float>, dim3(pg), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb,
+                                   sa ? 6 : 5);
             else
-                hipLaunchKernelGGL(k_sweep_unpack<double>, dim3(pg), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb);
+                hipLaunchKernelGGL(k_sweep_unpack<double>, dim3(pg), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb,
+                                   sa ? 6 : 5);
         };
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[6], c->stream));
         sweep(true);
@@ -1660,10 +1687,10 @@ static int launch_sweeps(fv3d_ctx *c)
 static int launch_update(fv3d_ctx *c)
 {
     int nb = (int)c->blocks.size();
-    const bool sa = c->cfg.turbulence == 200;
+    const bool sa = sa_transport(c);
     int *sa_hi = sa && c->cfg.limiter ? c->d_sa_hi : nullptr;
     hipLaunchKernelGGL(k_update, dim3((unsigned)c->npart), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb, c->gas,
-                       c->d_partial, c->d_bad, sa_hi);
+                       c->d_partial, c->d_bad, sa_hi, sa ? 1 : 0);
     HIPCK(c, hipGetLastError());
     if (c->cfg.limiter) {
         hipLaunchKernelGGL(k_limiter_serial, dim3(1), dim3(64), 0, c->stream, c->d_blocks, nb, c->gas, c->d_bad);
@@ -1759,7 +1786,7 @@ int fv3d_lusgs(fv3d_ctx *c)
 static int instance_norms(fv3d_ctx *c, int w, double *res6)
 {
     const int p0 = c->parts_host[w].x, np = c->parts_host[w].y;
-    const int nc = c->cfg.turbulence == 200 ? 6 : 5;
+    const int nc = sa_transport(c) ? 6 : 5;
     std::vector<double> part((size_t)np);
     for (int q = 0; q < 6; ++q) {
         res6[q] = 0.0;
@@ -2125,6 +2152,219 @@ int fv3d_autotune(fv3d_ctx *c, int reps, int *shape3)
     c->store_faces = saved;
     drop_graphs(c);
     return rc;
+}
+
+// ---------------------------------------------------- learned closure
+static int mlp_alloc(fv3d_ctx *c)
+{
+    if (c->d_mlp) return FV3D_OK;
+    int rc;
+    if ((rc = dalloc(c, &c->d_mlp, kMlpParams))) return rc;
+    if ((rc = dalloc(c, &c->d_mlp_m, kMlpParams))) return rc;
+    if ((rc = dalloc(c, &c->d_mlp_v, kMlpParams))) return rc;
+    if ((rc = dalloc(c, &c->d_mlp_g, kMlpParams + 1))) return rc;
+    if ((rc = dalloc(c, &c->d_mlp_bounds, 2 * kMlpIn + 1))) return rc;
+    if ((rc = dalloc(c, &c->d_mlp_tripped, 1))) return rc;
+    return FV3D_OK;
+}
+
+int fv3d_mlp_set(fv3d_ctx *c, const float *params, const float *bounds)
+{
+    READY(c);
+    if (!params) return FV3D_EINVAL;
+    int rc = mlp_alloc(c);
+    if (rc) return rc;
+    std::vector<float> b(2 * kMlpIn + 1);
+    for (int q = 0; q < kMlpIn; ++q) { b[2 * q] = -1e30f; b[2 * q + 1] = 1e30f; }
+    b[2 * kMlpIn] = 1e30f;
+    if (bounds) std::copy(bounds, bounds + 2 * kMlpIn + 1, b.begin());
+    HIPCK(c, hipMemcpyAsync(c->d_mlp, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(c->d_mlp_bounds, b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    // a fresh optimiser state
+    hipLaunchKernelGGL(k_fill_int, dim3((kMlpParams + 255) / 256), dim3(256), 0, c->stream, (int *)c->d_mlp_m,
+                       (long)kMlpParams, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3((kMlpParams + 255) / 256), dim3(256), 0, c->stream, (int *)c->d_mlp_v,
+                       (long)kMlpParams, 0);
+    HIPCK(c, hipGetLastError());
+    c->mlp_step = 0;
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    return FV3D_OK;
+}
+
+int fv3d_mlp_get(fv3d_ctx *c, float *params)
+{
+    READY(c);
+    if (!params) return FV3D_EINVAL;
+    if (!c->d_mlp) return fail(c, FV3D_ESTATE, "no network parameters (fv3d_mlp_set)");
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    HIPCK(c, hipMemcpy(params, c->d_mlp, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+    return FV3D_OK;
+}
+
+long fv3d_mlp_features(fv3d_ctx *c, int block, float *out)
+{
+    READY(c);
+    if (!out) return FV3D_EINVAL;
+    if (block < 1 || block > (int)c->blocks.size()) return fail(c, FV3D_EINVAL, "block %d", block);
+    if (!c->cfg.viscous || !c->blocks[block - 1].dev.dwall)
+        return fail(c, FV3D_EINVAL, "closure features need the Navier-Stokes terms and a wall distance (Spalart-Allmaras)");
+    const BlockDev &B = c->blocks[block - 1].dev;
+    const long ncb = (long)B.ni * B.nj * B.nk;
+    float *d = nullptr;
+    int rc = dalloc(c, &d, (size_t)ncb * kMlpIn);
+    if (rc) return rc;
+    // the block alone: a one-block cell list
+    long *ds = nullptr;
+    const long st[2] = {0, ncb};
+    if ((rc = dalloc(c, &ds, 2))) { (void)hipFree(d); return rc; }
+    HIPCK(c, hipMemcpyAsync(ds, st, sizeof st, hipMemcpyHostToDevice, c->stream));
+    const long fs = 0;
+    long *dfs = nullptr;
+    if ((rc = dalloc(c, &dfs, 1))) { (void)hipFree(d); (void)hipFree(ds); return rc; }
+    HIPCK(c, hipMemcpyAsync(dfs, &fs, sizeof fs, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_mlp_features, dim3((unsigned)((ncb + 255) / 256)), dim3(256), 0, c->stream,
+                       c->d_blocks + (block - 1), ds, 1, c->gas, dfs, d);
+    HIPCK(c, hipGetLastError());
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    HIPCK(c, hipMemcpy(out, d, (size_t)ncb * kMlpIn * sizeof(float), hipMemcpyDeviceToHost));
+    (void)hipFree(d); (void)hipFree(ds); (void)hipFree(dfs);
+    return ncb * kMlpIn;
+}
+
+// Device copies of n samples (features, and targets when t is given).
+static int mlp_upload(fv3d_ctx *c, const float *x, const float *t, long n, float **dx, float **dt)
+{
+    int rc = dalloc(c, dx, (size_t)n * kMlpIn);
+    if (rc) return rc;
+    HIPCK(c, hipMemcpyAsync(*dx, x, (size_t)n * kMlpIn * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (t) {
+        if ((rc = dalloc(c, dt, (size_t)n))) return rc;
+        HIPCK(c, hipMemcpyAsync(*dt, t, (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    }
+    return FV3D_OK;
+}
+
+int fv3d_mlp_forward(fv3d_ctx *c, const float *x, long n, float *y)
+{
+    READY(c);
+    if (!x || !y || n < 1) return FV3D_EINVAL;
+    if (!c->d_mlp) return fail(c, FV3D_ESTATE, "no network parameters (fv3d_mlp_set)");
+    float *dx = nullptr, *dy = nullptr, *unused = nullptr;
+    int rc = mlp_upload(c, x, nullptr, n, &dx, &unused);
+    if (!rc) rc = dalloc(c, &dy, (size_t)n);
+    if (!rc) {
+        hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, c->stream, c->d_mlp, dx, n, dy,
+                           (float *)nullptr);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(y, dy, (size_t)n * sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(c, FV3D_EHIP, "mlp forward: %s", hipGetErrorString(e));
+    }
+    if (dx) (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    return rc;
+}
+
+// One gradient of the mean loss over samples [off, off + m) into d_mlp_g
+// (and the summed squared error into d_mlp_g[kMlpParams]).
+static void mlp_gradient(fv3d_ctx *c, const float *dx, const float *dt, long off, long m, float *dpart, float *dloss)
+{
+    const int nchunk = (int)((m + 63) / 64);
+    hipLaunchKernelGGL(k_mlp_train_chunk, dim3((unsigned)nchunk), dim3(256), 0, c->stream, c->d_mlp, dx + off * kMlpIn,
+                       dt + off, m, 1.0f / (float)m, dpart, dloss);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, c->stream, dpart, dloss, nchunk,
+                       c->d_mlp_g, c->d_mlp_g + kMlpParams);
+}
+
+int fv3d_mlp_grad(fv3d_ctx *c, const float *x, const float *t, long n, float *grad, float *loss)
+{
+    READY(c);
+    if (!x || !t || n < 1) return FV3D_EINVAL;
+    if (!c->d_mlp) return fail(c, FV3D_ESTATE, "no network parameters (fv3d_mlp_set)");
+    float *dx = nullptr, *dt = nullptr, *dpart = nullptr, *dloss = nullptr;
+    const int nchunk = (int)((n + 63) / 64);
+    int rc = mlp_upload(c, x, t, n, &dx, &dt);
+    if (!rc) rc = dalloc(c, &dpart, (size_t)nchunk * kMlpParams);
+    if (!rc) rc = dalloc(c, &dloss, (size_t)nchunk);
+    if (!rc) {
+        mlp_gradient(c, dx, dt, 0, n, dpart, dloss);
+        std::vector<float> g(kMlpParams + 1);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(g.data(), c->d_mlp_g, g.size() * sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(c, FV3D_EHIP, "mlp gradient: %s", hipGetErrorString(e));
+        else {
+            if (grad) std::copy(g.begin(), g.begin() + kMlpParams, grad);
+            if (loss) *loss = g[kMlpParams] / (float)n;
+        }
+    }
+    for (float *p : {dx, dt, dpart, dloss})
+        if (p) (void)hipFree(p);
+    return rc;
+}
+
+int fv3d_mlp_train(fv3d_ctx *c, const float *x, const float *t, long n, int steps, int batch, float lr,
+                   float *loss_hist)
+{
+    READY(c);
+    if (!x || !t || n < 1 || steps < 1 || batch < 64 || batch % 64) return fail(c, FV3D_EINVAL, "training arguments");
+    if (!c->d_mlp) return fail(c, FV3D_ESTATE, "no network parameters (fv3d_mlp_set)");
+    const long m = std::min<long>(batch, n);
+    const int nchunk = (int)((m + 63) / 64);
+    float *dx = nullptr, *dt = nullptr, *dpart = nullptr, *dloss = nullptr, *dhist = nullptr;
+    int rc = mlp_upload(c, x, t, n, &dx, &dt);
+    if (!rc) rc = dalloc(c, &dpart, (size_t)nchunk * kMlpParams);
+    if (!rc) rc = dalloc(c, &dloss, (size_t)nchunk);
+    if (!rc) rc = dalloc(c, &dhist, (size_t)steps);
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int s = 0; s < steps && !rc; ++s) {
+        // consecutive minibatches of the sample order, wrapping (the caller shuffles)
+        const long off = (long)s * m % std::max<long>(1, n - m + 1);
+        mlp_gradient(c, dx, dt, off, m, dpart, dloss);
+        ++c->mlp_step;
+        const float c1 = 1.0f - powf(b1, (float)c->mlp_step), c2 = 1.0f - powf(b2, (float)c->mlp_step);
+        hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, c->stream, c->d_mlp, c->d_mlp_m,
+                           c->d_mlp_v, c->d_mlp_g, lr, b1, b2, eps, c1, c2);
+        HIPCK(c, hipMemcpyAsync(dhist + s, c->d_mlp_g + kMlpParams, sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = fail(c, FV3D_EHIP, "mlp training: %s", hipGetErrorString(e));
+    }
+    if (!rc) {
+        std::vector<float> h(steps);
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(h.data(), dhist, steps * sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(c, FV3D_EHIP, "mlp training: %s", hipGetErrorString(e));
+        else if (loss_hist)
+            for (int s = 0; s < steps; ++s) loss_hist[s] = h[s] / (float)m;
+    }
+    for (float *p : {dx, dt, dpart, dloss, dhist})
+        if (p) (void)hipFree(p);
+    drop_graphs(c);
+    return rc;
+}
+
+int fv3d_mlp_closure(fv3d_ctx *c, int on)
+{
+    READY(c);
+    if (c->cfg.turbulence != 200)
+        return fail(c, FV3D_EINVAL, "the learned closure replaces the Spalart-Allmaras transport (turbulence 200)");
+    if (on && !c->d_mlp) return fail(c, FV3D_ESTATE, "no network parameters (fv3d_mlp_set)");
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    c->mlp_closure = on != 0;
+    drop_graphs(c);
+    return FV3D_OK;
+}
+
+long fv3d_mlp_tripped(fv3d_ctx *c)
+{
+    READY(c);
+    if (!c->d_mlp_tripped) return 0;
+    unsigned v = 0, z = 0;
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    HIPCK(c, hipMemcpy(&v, c->d_mlp_tripped, sizeof v, hipMemcpyDeviceToHost));
+    HIPCK(c, hipMemcpy(c->d_mlp_tripped, &z, sizeof z, hipMemcpyHostToDevice));
+    return (long)v;
 }
 
 int fv3d_set_debug(fv3d_ctx *c, int flags)

@@ -121,13 +121,14 @@ __global__ void k_sweep_fill(const BlockDev *blocks, int block, long nrows, int 
 // one launch over all blocks: workgroups gstart[n] .. gstart[n+1]-1 serve block n
 template <class T>
 __global__ void k_sweep_pack(const BlockDev *blocks, const long *gstart, int nblock);
+// nc: components per cell (5, or 6 with the nu~ increment)
 template <class T>
-__global__ void k_sweep_unpack(const BlockDev *blocks, const long *gstart, int nblock);
+__global__ void k_sweep_unpack(const BlockDev *blocks, const long *gstart, int nblock, int nc);
 constexpr int kPackChunk = 16;   // i-planes per k_sweep_pack / k_sweep_unpack workgroup
 constexpr int kSecParts = 32;    // workgroups per section in k_sections
 __global__ void k_fill_int(int *p, long n, int v);
 __global__ void k_update(const BlockDev *blocks, const long *cstart, int nblock, Gas g, double *partial, int *bad_flag,
-                         int *sa_hi);
+                         int *sa_hi, int sa);
 __global__ void k_limiter_serial(const BlockDev *blocks, int nblock, Gas g, int *bad_flag);
 __global__ void k_sections(const BlockDev *blocks, const int2 *secs, Gas g, double *sums);
 // norms and section integrals of one level (lv) with the CFL ramp and the

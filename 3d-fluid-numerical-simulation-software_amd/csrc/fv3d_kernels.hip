@@ -1139,7 +1139,7 @@ __global__ void __launch_bounds__(256) k_sweep_pack(const BlockDev *__restrict__
 
 template <class T>
 __global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict__ blocks, const long *__restrict__ gstart,
-                                                      int nblock)
+                                                      int nblock, int nc)
 {
     constexpr int TJ = kKbaTJ, TK = kKbaTK;
     __shared__ double buf[kPackChunk][64];
@@ -1154,7 +1154,7 @@ __global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict
     const int tj = lane / TK, tk = lane % TK, j = J * TJ + tj + 1, k = K * TK + tk + 1;
     const bool col = j <= B.nj && k <= B.nk;
     const long box = B.box;
-    const int NC = B.ev ? 6 : 5;   // dnu~ with Spalart-Allmaras
+    const int NC = nc;   // 6: dnu~ too (the nu~ equation in the sweep)
     for (int c = 0; c < NC; ++c) {
         for (int rr = w; rr < kPackChunk + TJ + TK - 2; rr += 4) {
             const int r = i0 - 1 + rr, i = r + 1 - tj - tk;
@@ -1175,8 +1175,8 @@ template __global__ void k_sweep_fill<double>(const BlockDev *, int, long, int);
 template __global__ void k_sweep_fill<float>(const BlockDev *, int, long, int);
 template __global__ void k_sweep_pack<double>(const BlockDev *, const long *, int);
 template __global__ void k_sweep_pack<float>(const BlockDev *, const long *, int);
-template __global__ void k_sweep_unpack<double>(const BlockDev *, const long *, int);
-template __global__ void k_sweep_unpack<float>(const BlockDev *, const long *, int);
+template __global__ void k_sweep_unpack<double>(const BlockDev *, const long *, int, int);
+template __global__ void k_sweep_unpack<float>(const BlockDev *, const long *, int, int);
 
 // Fourth form of the wavefront sweep: the four-wave schedule of k_lusgs_kba3
 // on the tile-skewed layout.  A step touches one row of its tile: the three
@@ -1744,12 +1744,12 @@ template __global__ void k_lusgs_kba2<false, kKbaTJ, kKbaTK>(const BlockDev *, c
 // (source/ComputeAllBlockVars.cpp:340-357), PrimitiveVars
 // (source/PrimitiveVars.cpp:7-29) and the Limit_Vars trigger test
 // (source/Limit_FlowVars.cpp:28-29).  One partial sum per workgroup.  With
-// Spalart-Allmaras also nu~ += dnu~ (source/LU_SGS_Marching.cpp:271-272) and,
-// when sa_hi is given, the flag of a block holding a value the limiter
+// Spalart-Allmaras (sa set) also nu~ += dnu~ (source/LU_SGS_Marching.cpp:271-272)
+// and, when sa_hi is given, the flag of a block holding a value the limiter
 // replaces by its neighbours' mean (above 1000).
 __global__ void __launch_bounds__(256) k_update(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart, int nblock,
                                                 Gas g, double *__restrict__ partial, int *__restrict__ bad_flag,
-                                                int *__restrict__ sa_hi)
+                                                int *__restrict__ sa_hi, int sa)
 {
     __shared__ double red[5][256];
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1774,7 +1774,7 @@ __global__ void __launch_bounds__(256) k_update(const BlockDev *__restrict__ blo
         B.W[x] = d; B.W[box + x] = u; B.W[2 * box + x] = v; B.W[3 * box + x] = w; B.W[4 * box + x] = p; B.W[5 * box + x] = T;
         const double dmin = F32C(1.0E-6), dmax = F32C(1000.0), vmax = F32C(1000.0);
         if (d < dmin || d > dmax || p < dmin || p > dmax || u > vmax || v > vmax || w > vmax) atomicOr(bad_flag + n, 1);
-        if (B.ev) {
+        if (sa) {
             const double e = B.ev[x] + B.dev[x];
             B.ev[x] = e;
             if (sa_hi && e > F32C(1000.0)) atomicOr(sa_hi + n, 1);

@@ -1,0 +1,385 @@
+// Learned eddy-viscosity closure (no reference counterpart): a 6 -> 64 -> 64
+// -> 64 -> 1 perceptron from local non-dimensional flow features to
+// log(1 + mue_t / mue_lam), BF16 weights and activations into the matrix
+// cores (v_mfma_f32_16x16x32_bf16) with FP32 accumulation, tanh hidden layers,
+// softplus output.  Inference runs per cell inside the residual evaluation in
+// place of the Spalart-Allmaras transport; training runs on the GPU with the
+// same forward code and a hand-written backward pass (per-chunk partial
+// gradients, a fixed-order reduction, Adam).
+//
+// Matrix-core lane layout of v_mfma_f32_16x16x32_bf16 (one wave, D = A B + C,
+// A 16 x 32, B 32 x 16): lane l supplies A[l % 16][8 (l / 16) + j] and
+// B[8 (l / 16) + j][l % 16], j = 0..7, and receives D[4 (l / 16) + i][l % 16],
+// i = 0..3.
+#include "fv3d_kernels.h"
+#include "fv3d_mlp.h"
+
+#include <math.h>
+
+namespace fv3d {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kHS = kMlpH + 4;   // LDS row stride of an activation tile (floats)
+
+__device__ inline f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ inline float softplus(float z) { return z > 20.0f ? z : log1pf(expf(z)); }
+__device__ inline float sigmoid(float z) { return 1.0f / (1.0f + expf(-z)); }
+
+// One wave: rows r0 .. r0+15 of the activation tile in (row stride kHS,
+// columns 0..63) times the weight matrix W (row n = output, 64 inputs, FP32
+// in global memory, quantised to BF16 here), plus bias, tanh, into out.
+__device__ inline void dense64(const float *in, int r0, const float *W, const float *b, float *out, int lane)
+{
+    const int row = lane & 15, kb = lane >> 4;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        const int n = nt * 16 + row;
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            bf16x8 a, w;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = 32 * ks + 8 * kb + j;
+                a[j] = (__bf16)in[(r0 + row) * kHS + k];
+                w[j] = (__bf16)W[n * kMlpH + k];
+            }
+            acc = mfma16(a, w, acc);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) out[(r0 + 4 * kb + i) * kHS + n] = tanhf(acc[i] + b[n]);
+    }
+}
+
+// The forward pass of one wave's 16 samples (rows r0..r0+15): x (row stride
+// kMlpXS, features 0..5) -> h0, h1, h2 (tanh activations, FP32 in LDS) -> z
+// (pre-softplus output) for lanes 0..15.
+__device__ inline float mlp_forward16(const float *P, const float *x, float *h0, float *h1, float *h2, int r0,
+                                      int lane)
+{
+    const int row = lane & 15, kb = lane >> 4;
+    // layer 0: K = 6 features in the first 8-wide slice, zero elsewhere
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        const int n = nt * 16 + row;
+        bf16x8 a, w;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const bool in = kb == 0 && j < kMlpIn;
+            a[j] = (__bf16)(in ? x[(r0 + row) * kMlpXS + j] : 0.0f);
+            w[j] = (__bf16)(in ? P[kMlpW0 + n * kMlpIn + j] : 0.0f);
+        }
+        acc = mfma16(a, w, acc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h0[(r0 + 4 * kb + i) * kHS + n] = tanhf(acc[i] + P[kMlpB0 + n]);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    dense64(h0, r0, P + kMlpW1, P + kMlpB1, h1, lane);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    dense64(h1, r0, P + kMlpW2, P + kMlpB2, h2, lane);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    float z = P[kMlpB3];
+    if (lane < 16)
+        for (int k = 0; k < kMlpH; ++k) z += h2[(r0 + lane) * kHS + k] * P[kMlpW3 + k];
+    return z;
+}
+
+// ------------------------------------------------------------ features
+// The six inputs of a cell, from the cell-centred gradients of the solver's
+// own stencil (central differences in index space times the cell
+// coordinate-transformation coefficients, as CFluidProps::Curl forms them):
+//   0 log(1 + d |U| / nu)            wall-distance Reynolds number
+//   1 log(1 + S d^2 / nu)            strain-rate magnitude
+//   2 log(1 + Omega d^2 / nu)        vorticity magnitude
+//   3 (Omega^2 - S^2) / (Omega^2 + S^2)   Q criterion, normalised
+//   4 log(1 + |U| Vol^(1/3) / nu)    cell Reynolds number
+//   5 tanh(d U.grad p / (rho |U|^3)) pressure-gradient parameter
+// with nu = mue_lam / rho (Sutherland, non-dimensional), S = sqrt(2 S_ij S_ij),
+// Omega = sqrt(2 W_ij W_ij), d the wall distance.
+__device__ inline void cell_features(const BlockDev &B, const Gas &g, int i, int j, int k, float f[kMlpIn])
+{
+    const long box = B.box, x = cidx(B, i, j, k);
+    const long st[3] = {B.si, B.sj, 1};
+    double d[3][4];   // d/dkesi, d/dyita, d/dzita of u v w p
+    for (int a = 0; a < 3; ++a)
+        for (int q = 0; q < 4; ++q) d[a][q] = 0.5 * (B.W[(1 + q) * box + x + st[a]] - B.W[(1 + q) * box + x - st[a]]);
+    double gm[9];
+    for (int m = 0; m < 9; ++m) gm[m] = B.cg[m * box + x];
+    double gr[4][3];   // d(u v w p)/d(x y z)
+    for (int q = 0; q < 4; ++q)
+        for (int c = 0; c < 3; ++c) gr[q][c] = d[0][q] * gm[c] + d[1][q] * gm[3 + c] + d[2][q] * gm[6 + c];
+    double s2 = 0.0, w2 = 0.0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            const double sij = 0.5 * (gr[a][b] + gr[b][a]), wij = 0.5 * (gr[a][b] - gr[b][a]);
+            s2 += sij * sij;
+            w2 += wij * wij;
+        }
+    const double S = sqrt(2.0 * s2), Om = sqrt(2.0 * w2);
+    const double rho = B.W[x], u = B.W[box + x], v = B.W[2 * box + x], w = B.W[3 * box + x];
+    const double U = sqrt(u * u + v * v + w * w);
+    const double nu = sutherland(g, B.W[5 * box + x]) / rho;
+    const double dw = B.dwall[x];
+    const double ugp = u * gr[3][0] + v * gr[3][1] + w * gr[3][2];
+    f[0] = (float)log1p(dw * U / nu);
+    f[1] = (float)log1p(S * dw * dw / nu);
+    f[2] = (float)log1p(Om * dw * dw / nu);
+    f[3] = (float)((Om * Om - S * S) / (Om * Om + S * S + 1e-300));
+    f[4] = (float)log1p(U * cbrt(B.vol[x]) / nu);
+    f[5] = (float)tanh(dw * ugp / (rho * U * U * U + 1e-300));
+}
+
+// Features of every interior cell of every block into feat (6 floats per
+// cell, cells in cell-list order without padding: block n at fstart[n]).
+__global__ void __launch_bounds__(256) k_mlp_features(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
+                                                      int nblock, Gas g, const long *__restrict__ fstart,
+                                                      float *__restrict__ feat)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int n, i, j, k;
+    if (!cell_item(blocks, cstart, nblock, t, n, i, j, k)) return;
+    const BlockDev &B = blocks[n];
+    float f[kMlpIn];
+    cell_features(B, g, i, j, k, f);
+    const long r = fstart[n] + ((long)(i - 1) * B.nj + (j - 1)) * B.nk + (k - 1);
+    for (int q = 0; q < kMlpIn; ++q) feat[r * kMlpIn + q] = f[q];
+}
+
+// ----------------------------------------------------------- inference
+// Network output of n samples (features x, 6 per sample) into y (softplus
+// output) and z (pre-activation); 64 samples per workgroup, 16 per wave.
+__global__ void __launch_bounds__(256) k_mlp_forward(const float *__restrict__ P, const float *__restrict__ x, long n,
+                                                     float *__restrict__ y, float *__restrict__ zout)
+{
+    __shared__ float sx[64 * kMlpXS];
+    __shared__ float sh[2][64 * kHS];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const long s0 = (long)blockIdx.x * 64;
+    for (int e = t; e < 64 * kMlpXS; e += 256) {
+        const int r = e / kMlpXS, q = e % kMlpXS;
+        sx[e] = (q < kMlpIn && s0 + r < n) ? x[(s0 + r) * kMlpIn + q] : 0.0f;
+    }
+    __syncthreads();
+    const float z = mlp_forward16(P, sx, sh[0], sh[1], sh[0], wave * 16, lane);
+    const long s = s0 + wave * 16 + lane;
+    if (lane < 16 && s < n) {
+        y[s] = softplus(z);
+        if (zout) zout[s] = z;
+    }
+}
+
+// The closure in the residual evaluation: features, network, gate, mue_t of
+// every interior cell.  A cell whose features leave the training range
+// (bounds: min, max per feature, widened by 10 % of the span) or whose output
+// is not finite or above the largest trained value keeps the Spalart-Allmaras
+// value of its frozen nu~ (fv1 rho nu~ / Re, clipped as SA_Model clips it);
+// tripped counts those cells.
+__global__ void __launch_bounds__(256) k_mlp_closure(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
+                                                     int nblock, Gas g, const float *__restrict__ P,
+                                                     const float *__restrict__ bounds, unsigned *__restrict__ tripped)
+{
+    __shared__ float sx[64 * kMlpXS];
+    __shared__ float sh[2][64 * kHS];
+    __shared__ int sok[64];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const long c0 = (long)blockIdx.x * 64;
+    // one thread per cell: the features and the input half of the gate
+    if (t < 64) {
+        int n, i, j, k;
+        const bool valid = cell_item(blocks, cstart, nblock, c0 + t, n, i, j, k);
+        float f[kMlpIn] = {0, 0, 0, 0, 0, 0};
+        bool ok = valid;
+        if (valid) {
+            cell_features(blocks[n], g, i, j, k, f);
+            for (int q = 0; q < kMlpIn; ++q) {
+                const float lo = bounds[2 * q], hi = bounds[2 * q + 1], pad = 0.1f * (hi - lo);
+                ok = ok && isfinite(f[q]) && f[q] >= lo - pad && f[q] <= hi + pad;
+            }
+        }
+        for (int q = 0; q < kMlpXS; ++q) sx[t * kMlpXS + q] = (q < kMlpIn && ok) ? f[q] : 0.0f;
+        sok[t] = valid ? (ok ? 1 : 2) : 0;
+    }
+    __syncthreads();
+    const float z = mlp_forward16(P, sx, sh[0], sh[1], sh[0], wave * 16, lane);
+    if (lane >= 16) return;
+    const int r = wave * 16 + lane;
+    if (sok[r] == 0) return;
+    int n, i, j, k;
+    cell_item(blocks, cstart, nblock, c0 + r, n, i, j, k);
+    const BlockDev &B = blocks[n];
+    const long box = B.box, x = cidx(B, i, j, k);
+    const float yhat = softplus(z);
+    const double mue_lam = sutherland(g, B.W[5 * box + x]);
+    double mt;
+    if (sok[r] == 1 && isfinite(yhat) && yhat <= bounds[2 * kMlpIn]) {
+        mt = mue_lam * expm1((double)yhat);
+    } else {
+        atomicAdd(tripped, 1u);
+        // Spalart-Allmaras fallback from the frozen nu~ (source/SA_Model.cpp:47-87)
+        const double rho = B.W[x], ev = B.ev[x] < 0.0 ? 0.0 : B.ev[x];
+        const double ev_lam = g.re_ref * mue_lam / rho, X = ev / ev_lam;
+        const double Cv1 = F32C(7.1);
+        const double fv1 = pow(X, F32C(3.0)) / (pow(X, F32C(3.0)) + Cv1 * Cv1 * Cv1);
+        mt = fv1 * rho * ev / g.re_ref;
+        if (mt < 0.0) mt = 0.0;
+    }
+    if (mue_lam > 0 && mt > 200.0 * mue_lam) mt = 200.0 * mue_lam;
+    B.mut[x] = mt;
+}
+
+// ------------------------------------------------------------ training
+// Loss = mean over the n samples of (softplus(z) - t)^2.  One workgroup per
+// chunk of 64 samples: forward (activations kept in LDS), then the backward
+// pass; the chunk's gradient goes to partial[chunk * kMlpParams + p] and its
+// sum of squared errors to loss[chunk].  Matrix products of the backward pass
+// run on the matrix cores with BF16 operands (deltas and activations
+// quantised as the forward pass quantises them), FP32 accumulation.
+__global__ void __launch_bounds__(256) k_mlp_train_chunk(const float *__restrict__ P, const float *__restrict__ x,
+                                                         const float *__restrict__ tgt, long n, float inv_n,
+                                                         float *__restrict__ partial, float *__restrict__ loss)
+{
+    __shared__ float sx[64 * kMlpXS];
+    __shared__ float h[3][64 * kHS];   // activations, overwritten by the deltas
+    __shared__ float dz[64];
+    __shared__ float red[64];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63, row = lane & 15, kb = lane >> 4;
+    const long s0 = (long)blockIdx.x * 64;
+    float *G = partial + (long)blockIdx.x * kMlpParams;
+    for (int e = t; e < 64 * kMlpXS; e += 256) {
+        const int r = e / kMlpXS, q = e % kMlpXS;
+        sx[e] = (q < kMlpIn && s0 + r < n) ? x[(s0 + r) * kMlpIn + q] : 0.0f;
+    }
+    __syncthreads();
+    const float z = mlp_forward16(P, sx, h[0], h[1], h[2], wave * 16, lane);
+    if (lane < 16) {
+        const int r = wave * 16 + lane;
+        const bool in = s0 + r < n;
+        const float e = in ? softplus(z) - tgt[s0 + r] : 0.0f;
+        dz[r] = in ? 2.0f * inv_n * e * sigmoid(z) : 0.0f;
+        red[r] = e * e;
+    }
+    __syncthreads();
+    if (t == 0) {
+        float s = 0.0f;
+        for (int r = 0; r < 64; ++r) s += red[r];
+        loss[blockIdx.x] = s;
+    }
+    // output layer: dW3, db3, then delta2 = dz W3 (1 - h2^2) in place
+    if (t < kMlpH) {
+        float s = 0.0f;
+        for (int r = 0; r < 64; ++r) s += dz[r] * h[2][r * kHS + t];
+        G[kMlpW3 + t] = s;
+    } else if (t == kMlpH) {
+        float s = 0.0f;
+        for (int r = 0; r < 64; ++r) s += dz[r];
+        G[kMlpB3] = s;
+    }
+    __syncthreads();
+    for (int e = t; e < 64 * kMlpH; e += 256) {
+        const int r = e / kMlpH, c = e % kMlpH;
+        const float a = h[2][r * kHS + c];
+        h[2][r * kHS + c] = dz[r] * P[kMlpW3 + c] * (1.0f - a * a);
+    }
+    __syncthreads();
+    // hidden layers L = 2, 1: delta in h[L], activations of the layer below in h[L-1]
+    for (int L = 2; L >= 1; --L) {
+        const float *Wl = P + (L == 2 ? kMlpW2 : kMlpW1);
+        const int oW = L == 2 ? kMlpW2 : kMlpW1, oB = L == 2 ? kMlpB2 : kMlpB1;
+        float *dl = h[L], *al = h[L - 1];
+        // dW[n][k] = sum_s delta[s][n] a[s][k]: output tiles (mt, nt), four per wave
+        for (int tile = wave * 4; tile < wave * 4 + 4; ++tile) {
+            const int mt = tile >> 2, nt = tile & 3;
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int ks = 0; ks < 2; ++ks) {
+                bf16x8 a, b;
+                for (int j = 0; j < 8; ++j) {
+                    const int s = 32 * ks + 8 * kb + j;
+                    a[j] = (__bf16)dl[s * kHS + mt * 16 + row];
+                    b[j] = (__bf16)al[s * kHS + nt * 16 + row];
+                }
+                acc = mfma16(a, b, acc);
+            }
+            for (int i = 0; i < 4; ++i) G[oW + (mt * 16 + 4 * kb + i) * kMlpH + nt * 16 + row] = acc[i];
+        }
+        if (t < kMlpH) {
+            float s = 0.0f;
+            for (int r = 0; r < 64; ++r) s += dl[r * kHS + t];
+            G[oB + t] = s;
+        }
+        __syncthreads();
+        // delta below[s][k] = (sum_n delta[s][n] W[n][k]) (1 - a[s][k]^2), in place of a
+        for (int tile = wave * 4; tile < wave * 4 + 4; ++tile) {
+            const int mt = tile >> 2, nt = tile & 3;
+            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int ks = 0; ks < 2; ++ks) {
+                bf16x8 a, b;
+                for (int j = 0; j < 8; ++j) {
+                    const int nn = 32 * ks + 8 * kb + j;
+                    a[j] = (__bf16)dl[(mt * 16 + row) * kHS + nn];
+                    b[j] = (__bf16)Wl[nn * kMlpH + nt * 16 + row];
+                }
+                acc = mfma16(a, b, acc);
+            }
+            for (int i = 0; i < 4; ++i) {
+                const int s = mt * 16 + 4 * kb + i, c = nt * 16 + row;
+                const float av = al[s * kHS + c];
+                al[s * kHS + c] = acc[i] * (1.0f - av * av);
+            }
+        }
+        __syncthreads();
+    }
+    // input layer: dW0[n][f] = sum_s delta0[s][n] x[s][f], db0
+    for (int o = t; o < kMlpH * kMlpIn; o += 256) {
+        const int nn = o / kMlpIn, f = o % kMlpIn;
+        float s = 0.0f;
+        for (int r = 0; r < 64; ++r) s += h[0][r * kHS + nn] * (float)(__bf16)sx[r * kMlpXS + f];
+        G[kMlpW0 + o] = s;
+    }
+    if (t < kMlpH) {
+        float s = 0.0f;
+        for (int r = 0; r < 64; ++r) s += h[0][r * kHS + t];
+        G[kMlpB0 + t] = s;
+    }
+}
+
+// Fixed-order sum of the chunk gradients (and losses) of one step.
+__global__ void __launch_bounds__(256) k_mlp_reduce(const float *__restrict__ partial, const float *__restrict__ loss,
+                                                    int nchunk, float *__restrict__ grad, float *__restrict__ loss_sum)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < kMlpParams) {
+        float s = 0.0f;
+        for (int c = 0; c < nchunk; ++c) s += partial[(long)c * kMlpParams + p];
+        grad[p] = s;
+    } else if (p == kMlpParams) {
+        float s = 0.0f;
+        for (int c = 0; c < nchunk; ++c) s += loss[c];
+        *loss_sum = s;
+    }
+}
+
+// Adam (Kingma and Ba 2015) on the FP32 master weights.
+__global__ void __launch_bounds__(256) k_mlp_adam(float *__restrict__ P, float *__restrict__ m, float *__restrict__ v,
+                                                  const float *__restrict__ grad, float lr, float b1, float b2, float eps,
+                                                  float c1, float c2)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    const float gr = grad[p];
+    const float mm = b1 * m[p] + (1.0f - b1) * gr, vv = b2 * v[p] + (1.0f - b2) * gr * gr;
+    m[p] = mm;
+    v[p] = vv;
+    P[p] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+}  // namespace fv3d

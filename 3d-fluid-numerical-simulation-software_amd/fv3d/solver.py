@@ -163,6 +163,56 @@ class GpuSolver:
         self._check(self.L.fv3d_autotune(self.h, reps, a.ctypes.data), "autotune")
         return [int(x) for x in a]
 
+    # -- learned closure (fv3d_mlp_*)
+    MLP_PARAMS = 8833
+
+    def mlp_set(self, params, bounds=None):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == self.MLP_PARAMS
+        b = None if bounds is None else np.ascontiguousarray(bounds, dtype=np.float32)
+        self._check(self.L.fv3d_mlp_set(self.h, p.ctypes.data, None if b is None else b.ctypes.data), "mlp_set")
+
+    def mlp_get(self) -> np.ndarray:
+        p = np.zeros(self.MLP_PARAMS, dtype=np.float32)
+        self._check(self.L.fv3d_mlp_get(self.h, p.ctypes.data), "mlp_get")
+        return p
+
+    def mlp_features(self, block: int, inst: int = 1) -> np.ndarray:
+        bi = self.cases[inst - 1].blocks[block - 1]
+        out = np.zeros((bi.im - 1, bi.jm - 1, bi.km - 1, 6), dtype=np.float32)
+        n = self.L.fv3d_mlp_features(self.h, self.offsets[inst - 1] + block, out.ctypes.data)
+        self._check(n, "mlp_features")
+        return out
+
+    def mlp_forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros(x.shape[0], dtype=np.float32)
+        self._check(self.L.fv3d_mlp_forward(self.h, x.ctypes.data, x.shape[0], y.ctypes.data), "mlp_forward")
+        return y
+
+    def mlp_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.MLP_PARAMS, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        self._check(self.L.fv3d_mlp_grad(self.h, x.ctypes.data, t.ctypes.data, x.shape[0], g.ctypes.data,
+                                         loss.ctypes.data), "mlp_grad")
+        return g, float(loss[0])
+
+    def mlp_train(self, x, t, steps: int, batch: int, lr: float) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        h = np.zeros(steps, dtype=np.float32)
+        self._check(self.L.fv3d_mlp_train(self.h, x.ctypes.data, t.ctypes.data, x.shape[0], steps, batch,
+                                          C.c_float(lr), h.ctypes.data), "mlp_train")
+        return h
+
+    def mlp_closure(self, on: bool):
+        self._check(self.L.fv3d_mlp_closure(self.h, int(on)), "mlp_closure")
+
+    def mlp_tripped(self) -> int:
+        return self._check(int(self.L.fv3d_mlp_tripped(self.h)), "mlp_tripped")
+
     def instance_results(self, inst: int):
         """(res6, perf5) of instance inst (1-based) after the last iteration."""
         r, p = np.zeros(6), np.zeros(5)

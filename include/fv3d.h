@@ -235,6 +235,35 @@ int fv3d_set_debug(fv3d_ctx *ctx, int flags);
 int fv3d_set_tiles(fv3d_ctx *ctx, const int *shape3);
 int fv3d_autotune(fv3d_ctx *ctx, int reps, int *shape3);
 
+/* Learned eddy-viscosity closure (no reference counterpart; it takes the
+ * place of the Spalart-Allmaras transport, source/SA_Model.cpp, in a context
+ * with turbulence 200).  A 6 -> 64 -> 64 -> 64 -> 1 network, tanh hidden
+ * layers, softplus output y = log(1 + mue_t / mue_lam), BF16 weights and
+ * activations on the matrix cores with FP32 accumulation.  params: 8833 FP32
+ * values W0[64][6] b0[64] W1[64][64] b1[64] W2[64][64] b2[64] W3[64] b3.
+ * Features per cell (fv3d_mlp_features, 6 floats, cells i-slowest):
+ * log(1 + d|U|/nu), log(1 + S d^2/nu), log(1 + Omega d^2/nu),
+ * (Omega^2 - S^2)/(Omega^2 + S^2), log(1 + |U| Vol^(1/3)/nu),
+ * tanh(d U.grad p / (rho |U|^3)).
+ * fv3d_mlp_set: parameters and the gate bounds (min and max of each feature,
+ * then the largest output; NULL for none), and a fresh Adam state.
+ * fv3d_mlp_forward: outputs y of n samples.  fv3d_mlp_grad: the gradient of
+ * the mean squared error of y against targets t over n samples (and the loss).
+ * fv3d_mlp_train: steps Adam steps (lr) over consecutive minibatches of batch
+ * samples (a multiple of 64), loss_hist[steps] (may be NULL).
+ * fv3d_mlp_closure(on): mue_t from the network each residual evaluation and no
+ * nu~ equation; a cell outside the gate keeps the Spalart-Allmaras value of its
+ * frozen nu~.  fv3d_mlp_tripped: cells that fell back since the last call. */
+int fv3d_mlp_set(fv3d_ctx *ctx, const float *params, const float *bounds);
+int fv3d_mlp_get(fv3d_ctx *ctx, float *params);
+long fv3d_mlp_features(fv3d_ctx *ctx, int block, float *out);
+int fv3d_mlp_forward(fv3d_ctx *ctx, const float *x, long n, float *y);
+int fv3d_mlp_grad(fv3d_ctx *ctx, const float *x, const float *t, long n, float *grad, float *loss);
+int fv3d_mlp_train(fv3d_ctx *ctx, const float *x, const float *t, long n, int steps, int batch, float lr,
+                   float *loss_hist);
+int fv3d_mlp_closure(fv3d_ctx *ctx, int on);
+long fv3d_mlp_tripped(fv3d_ctx *ctx);
+
 /* Number of interior cells over all blocks. */
 long fv3d_cell_count(const fv3d_ctx *ctx);
 
