@@ -1514,7 +1514,7 @@ static int launch_turbulence(fv3d_ctx *c, int save_old = 0)
     if (c->cfg.turbulence == 200 && c->mlp_closure) {
         // the learned closure in place of the nu~ transport; ghost mue_t as for SA
         const int nb = (int)c->blocks.size();
-        hipLaunchKernelGGL(k_mlp_closure, dim3((unsigned)((c->ncell + 63) / 64)), dim3(256), 0, c->stream, c->d_blocks,
+        hipLaunchKernelGGL(k_mlp_closure, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
                            c->d_cstart, nb, c->gas, c->d_mlp, c->d_mlp_bounds, c->d_mlp_tripped);
         hipLaunchKernelGGL(k_sa_ghost, dim3((unsigned)((c->nghost + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
                            c->d_gstart, nb, c->gas);
@@ -2202,7 +2202,9 @@ int fv3d_mlp_get(fv3d_ctx *c, float *params)
     return FV3D_OK;
 }
 
-long fv3d_mlp_features(fv3d_ctx *c, int block, float *out)
+// Per-cell closure data of one block: the features (what = 0, 6 per cell) or
+// the training target (what = 1).
+static long mlp_block_data(fv3d_ctx *c, int block, float *out, int what)
 {
     READY(c);
     if (!out) return FV3D_EINVAL;
@@ -2211,8 +2213,9 @@ long fv3d_mlp_features(fv3d_ctx *c, int block, float *out)
         return fail(c, FV3D_EINVAL, "closure features need the Navier-Stokes terms and a wall distance (Spalart-Allmaras)");
     const BlockDev &B = c->blocks[block - 1].dev;
     const long ncb = (long)B.ni * B.nj * B.nk;
+    const int per = what == 0 ? kMlpIn : 1;
     float *d = nullptr;
-    int rc = dalloc(c, &d, (size_t)ncb * kMlpIn);
+    int rc = dalloc(c, &d, (size_t)ncb * per);
     if (rc) return rc;
     // the block alone: a one-block cell list
     long *ds = nullptr;
@@ -2223,13 +2226,25 @@ long fv3d_mlp_features(fv3d_ctx *c, int block, float *out)
     long *dfs = nullptr;
     if ((rc = dalloc(c, &dfs, 1))) { (void)hipFree(d); (void)hipFree(ds); return rc; }
     HIPCK(c, hipMemcpyAsync(dfs, &fs, sizeof fs, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_mlp_features, dim3((unsigned)((ncb + 255) / 256)), dim3(256), 0, c->stream,
-                       c->d_blocks + (block - 1), ds, 1, c->gas, dfs, d);
+    if (what == 0)
+        hipLaunchKernelGGL(k_mlp_features, dim3((unsigned)((ncb + 255) / 256)), dim3(256), 0, c->stream,
+                           c->d_blocks + (block - 1), ds, 1, c->gas, dfs, d);
+    else
+        hipLaunchKernelGGL(k_mlp_targets, dim3((unsigned)((ncb + 255) / 256)), dim3(256), 0, c->stream,
+                           c->d_blocks + (block - 1), ds, 1, c->gas, dfs, d);
     HIPCK(c, hipGetLastError());
     HIPCK(c, hipStreamSynchronize(c->stream));
-    HIPCK(c, hipMemcpy(out, d, (size_t)ncb * kMlpIn * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCK(c, hipMemcpy(out, d, (size_t)ncb * per * sizeof(float), hipMemcpyDeviceToHost));
     (void)hipFree(d); (void)hipFree(ds); (void)hipFree(dfs);
-    return ncb * kMlpIn;
+    return ncb * per;
+}
+
+long fv3d_mlp_features(fv3d_ctx *c, int block, float *out) { return mlp_block_data(c, block, out, 0); }
+
+long fv3d_mlp_targets(fv3d_ctx *c, int block, float *out)
+{
+    if (c && c->ready && c->cfg.turbulence == 0) return fail(c, FV3D_EINVAL, "targets need an eddy viscosity");
+    return mlp_block_data(c, block, out, 1);
 }
 
 // Device copies of n samples (features, and targets when t is given).

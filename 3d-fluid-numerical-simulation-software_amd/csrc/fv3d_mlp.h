@@ -16,6 +16,8 @@ static_assert(kMlpParams == 8833, "6-64-64-64-1");
 
 __global__ void k_mlp_features(const BlockDev *blocks, const long *cstart, int nblock, Gas g, const long *fstart,
                                float *feat);
+__global__ void k_mlp_targets(const BlockDev *blocks, const long *cstart, int nblock, Gas g, const long *fstart,
+                              float *out);
 __global__ void k_mlp_forward(const float *P, const float *x, long n, float *y, float *zout);
 // bounds: min and max of each feature, then the largest output (2 kMlpIn + 1)
 __global__ void k_mlp_closure(const BlockDev *blocks, const long *cstart, int nblock, Gas g, const float *P,

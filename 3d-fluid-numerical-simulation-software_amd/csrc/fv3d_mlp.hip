@@ -105,38 +105,44 @@ __device__ inline float mlp_forward16(const float *P, const float *x, float *h0,
 //   4 log(1 + |U| Vol^(1/3) / nu)    cell Reynolds number
 //   5 tanh(d U.grad p / (rho |U|^3)) pressure-gradient parameter
 // with nu = mue_lam / rho (Sutherland, non-dimensional), S = sqrt(2 S_ij S_ij),
-// Omega = sqrt(2 W_ij W_ij), d the wall distance.
+// Omega = sqrt(2 W_ij W_ij), d the wall distance.  The differences of the
+// state are taken in FP64 (they cancel), the rest in FP32: the features feed
+// a BF16 network.
 __device__ inline void cell_features(const BlockDev &B, const Gas &g, int i, int j, int k, float f[kMlpIn])
 {
     const long box = B.box, x = cidx(B, i, j, k);
     const long st[3] = {B.si, B.sj, 1};
-    double d[3][4];   // d/dkesi, d/dyita, d/dzita of u v w p
+    float d[3][4];   // d/dkesi, d/dyita, d/dzita of u v w p
     for (int a = 0; a < 3; ++a)
-        for (int q = 0; q < 4; ++q) d[a][q] = 0.5 * (B.W[(1 + q) * box + x + st[a]] - B.W[(1 + q) * box + x - st[a]]);
-    double gm[9];
-    for (int m = 0; m < 9; ++m) gm[m] = B.cg[m * box + x];
-    double gr[4][3];   // d(u v w p)/d(x y z)
+        for (int q = 0; q < 4; ++q)
+            d[a][q] = (float)(0.5 * (B.W[(1 + q) * box + x + st[a]] - B.W[(1 + q) * box + x - st[a]]));
+    float gm[9];
+    for (int m = 0; m < 9; ++m) gm[m] = (float)B.cg[m * box + x];
+    float gr[4][3];   // d(u v w p)/d(x y z)
     for (int q = 0; q < 4; ++q)
         for (int c = 0; c < 3; ++c) gr[q][c] = d[0][q] * gm[c] + d[1][q] * gm[3 + c] + d[2][q] * gm[6 + c];
-    double s2 = 0.0, w2 = 0.0;
+    float s2 = 0.0f, w2 = 0.0f;
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) {
-            const double sij = 0.5 * (gr[a][b] + gr[b][a]), wij = 0.5 * (gr[a][b] - gr[b][a]);
+            const float sij = 0.5f * (gr[a][b] + gr[b][a]), wij = 0.5f * (gr[a][b] - gr[b][a]);
             s2 += sij * sij;
             w2 += wij * wij;
         }
-    const double S = sqrt(2.0 * s2), Om = sqrt(2.0 * w2);
-    const double rho = B.W[x], u = B.W[box + x], v = B.W[2 * box + x], w = B.W[3 * box + x];
-    const double U = sqrt(u * u + v * v + w * w);
-    const double nu = sutherland(g, B.W[5 * box + x]) / rho;
-    const double dw = B.dwall[x];
-    const double ugp = u * gr[3][0] + v * gr[3][1] + w * gr[3][2];
-    f[0] = (float)log1p(dw * U / nu);
-    f[1] = (float)log1p(S * dw * dw / nu);
-    f[2] = (float)log1p(Om * dw * dw / nu);
-    f[3] = (float)((Om * Om - S * S) / (Om * Om + S * S + 1e-300));
-    f[4] = (float)log1p(U * cbrt(B.vol[x]) / nu);
-    f[5] = (float)tanh(dw * ugp / (rho * U * U * U + 1e-300));
+    const float S = sqrtf(2.0f * s2), Om = sqrtf(2.0f * w2);
+    const float rho = (float)B.W[x], u = (float)B.W[box + x], v = (float)B.W[2 * box + x], w = (float)B.W[3 * box + x];
+    const float U = sqrtf(u * u + v * v + w * w);
+    const float T = (float)B.W[5 * box + x];
+    // Sutherland's law (include/FluidProps.h:107-112) in FP32
+    const float mue = 1.0f / (float)g.re_ref * T * sqrtf(T) * (1.0f + 110.4f / 288.15f) / (T + 110.4f / 288.15f);
+    const float nu = mue / rho;
+    const float dw = (float)B.dwall[x];
+    const float ugp = u * gr[3][0] + v * gr[3][1] + w * gr[3][2];
+    f[0] = log1pf(dw * U / nu);
+    f[1] = log1pf(S * dw * dw / nu);
+    f[2] = log1pf(Om * dw * dw / nu);
+    f[3] = (Om * Om - S * S) / (Om * Om + S * S + 1e-30f);
+    f[4] = log1pf(U * cbrtf((float)B.vol[x]) / nu);
+    f[5] = tanhf(dw * ugp / (rho * U * U * U + 1e-30f));
 }
 
 // Features of every interior cell of every block into feat (6 floats per
@@ -153,6 +159,22 @@ __global__ void __launch_bounds__(256) k_mlp_features(const BlockDev *__restrict
     cell_features(B, g, i, j, k, f);
     const long r = fstart[n] + ((long)(i - 1) * B.nj + (j - 1)) * B.nk + (k - 1);
     for (int q = 0; q < kMlpIn; ++q) feat[r * kMlpIn + q] = f[q];
+}
+
+// The training target of every interior cell: log(1 + mue_t / mue_lam) of
+// the current eddy viscosity (a Spalart-Allmaras run's), same order as
+// k_mlp_features.
+__global__ void __launch_bounds__(256) k_mlp_targets(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
+                                                     int nblock, Gas g, const long *__restrict__ fstart,
+                                                     float *__restrict__ out)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int n, i, j, k;
+    if (!cell_item(blocks, cstart, nblock, t, n, i, j, k)) return;
+    const BlockDev &B = blocks[n];
+    const long x = cidx(B, i, j, k);
+    const double mue_lam = sutherland(g, B.W[5 * B.box + x]);
+    out[fstart[n] + ((long)(i - 1) * B.nj + (j - 1)) * B.nk + (k - 1)] = (float)log1p(B.mut[x] / mue_lam);
 }
 
 // ----------------------------------------------------------- inference
@@ -179,7 +201,8 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float *__restrict__ P
 }
 
 // The closure in the residual evaluation: features, network, gate, mue_t of
-// every interior cell.  A cell whose features leave the training range
+// every interior cell; 256 cells per workgroup, each wave runs the network on
+// 64 of them, 16 at a time.  A cell whose features leave the training range
 // (bounds: min, max per feature, widened by 10 % of the span) or whose output
 // is not finite or above the largest trained value keeps the Spalart-Allmaras
 // value of its frozen nu~ (fv1 rho nu~ / Re, clipped as SA_Model clips it);
@@ -188,13 +211,13 @@ __global__ void __launch_bounds__(256) k_mlp_closure(const BlockDev *__restrict_
                                                      int nblock, Gas g, const float *__restrict__ P,
                                                      const float *__restrict__ bounds, unsigned *__restrict__ tripped)
 {
-    __shared__ float sx[64 * kMlpXS];
-    __shared__ float sh[2][64 * kHS];
-    __shared__ int sok[64];
+    __shared__ float sx[256 * kMlpXS];
+    __shared__ float sh[4][2][16 * kHS];   // per wave: activation ping-pong of 16 rows
+    __shared__ int sok[256];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    const long c0 = (long)blockIdx.x * 64;
+    const long c0 = (long)blockIdx.x * 256;
     // one thread per cell: the features and the input half of the gate
-    if (t < 64) {
+    {
         int n, i, j, k;
         const bool valid = cell_item(blocks, cstart, nblock, c0 + t, n, i, j, k);
         float f[kMlpIn] = {0, 0, 0, 0, 0, 0};
@@ -210,31 +233,36 @@ __global__ void __launch_bounds__(256) k_mlp_closure(const BlockDev *__restrict_
         sok[t] = valid ? (ok ? 1 : 2) : 0;
     }
     __syncthreads();
-    const float z = mlp_forward16(P, sx, sh[0], sh[1], sh[0], wave * 16, lane);
+    float zq[4];
+    for (int q = 0; q < 4; ++q)
+        zq[q] = mlp_forward16(P, sx + (wave * 64 + 16 * q) * kMlpXS, sh[wave][0], sh[wave][1], sh[wave][0], 0, lane);
     if (lane >= 16) return;
-    const int r = wave * 16 + lane;
-    if (sok[r] == 0) return;
-    int n, i, j, k;
-    cell_item(blocks, cstart, nblock, c0 + r, n, i, j, k);
-    const BlockDev &B = blocks[n];
-    const long box = B.box, x = cidx(B, i, j, k);
-    const float yhat = softplus(z);
-    const double mue_lam = sutherland(g, B.W[5 * box + x]);
-    double mt;
-    if (sok[r] == 1 && isfinite(yhat) && yhat <= bounds[2 * kMlpIn]) {
-        mt = mue_lam * expm1((double)yhat);
-    } else {
-        atomicAdd(tripped, 1u);
-        // Spalart-Allmaras fallback from the frozen nu~ (source/SA_Model.cpp:47-87)
-        const double rho = B.W[x], ev = B.ev[x] < 0.0 ? 0.0 : B.ev[x];
-        const double ev_lam = g.re_ref * mue_lam / rho, X = ev / ev_lam;
-        const double Cv1 = F32C(7.1);
-        const double fv1 = pow(X, F32C(3.0)) / (pow(X, F32C(3.0)) + Cv1 * Cv1 * Cv1);
-        mt = fv1 * rho * ev / g.re_ref;
-        if (mt < 0.0) mt = 0.0;
+    for (int q = 0; q < 4; ++q) {
+        const float z = zq[q];
+        const int r = wave * 64 + 16 * q + lane;
+        if (sok[r] == 0) continue;
+        int n, i, j, k;
+        cell_item(blocks, cstart, nblock, c0 + r, n, i, j, k);
+        const BlockDev &B = blocks[n];
+        const long box = B.box, x = cidx(B, i, j, k);
+        const float yhat = softplus(z);
+        const double mue_lam = sutherland(g, B.W[5 * box + x]);
+        double mt;
+        if (sok[r] == 1 && isfinite(yhat) && yhat <= bounds[2 * kMlpIn]) {
+            mt = mue_lam * expm1((double)yhat);
+        } else {
+            atomicAdd(tripped, 1u);
+            // Spalart-Allmaras fallback from the frozen nu~ (source/SA_Model.cpp:47-87)
+            const double rho = B.W[x], ev = B.ev[x] < 0.0 ? 0.0 : B.ev[x];
+            const double ev_lam = g.re_ref * mue_lam / rho, X = ev / ev_lam;
+            const double Cv1 = F32C(7.1);
+            const double fv1 = pow(X, F32C(3.0)) / (pow(X, F32C(3.0)) + Cv1 * Cv1 * Cv1);
+            mt = fv1 * rho * ev / g.re_ref;
+            if (mt < 0.0) mt = 0.0;
+        }
+        if (mue_lam > 0 && mt > 200.0 * mue_lam) mt = 200.0 * mue_lam;
+        B.mut[x] = mt;
     }
-    if (mue_lam > 0 && mt > 200.0 * mue_lam) mt = 200.0 * mue_lam;
-    B.mut[x] = mt;
 }
 
 // ------------------------------------------------------------ training

@@ -184,6 +184,13 @@ class GpuSolver:
         self._check(n, "mlp_features")
         return out
 
+    def mlp_targets(self, block: int, inst: int = 1) -> np.ndarray:
+        bi = self.cases[inst - 1].blocks[block - 1]
+        out = np.zeros((bi.im - 1, bi.jm - 1, bi.km - 1), dtype=np.float32)
+        n = self.L.fv3d_mlp_targets(self.h, self.offsets[inst - 1] + block, out.ctypes.data)
+        self._check(n, "mlp_targets")
+        return out
+
     def mlp_forward(self, x) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float32)
         y = np.zeros(x.shape[0], dtype=np.float32)

@@ -63,7 +63,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["passage5", "bump", "batch"], default="passage5")
+    ap.add_argument("--workload", choices=["passage5", "bump", "batch", "closure"], default="passage5")
+    ap.add_argument("--closure-cells", type=float, default=1.3e5, help="closure workload: cells per passage")
+    ap.add_argument("--closure-develop", type=int, default=600, help="closure workload: SA iterations before the split")
+    ap.add_argument("--closure-iters", type=int, default=2400, help="closure workload: iterations after the split")
     ap.add_argument("--instances", type=int, default=64, help="batch workload: number of passages")
     ap.add_argument("--batch-seed", type=int, default=3)
     ap.add_argument("--batch-rpm", type=float, default=0.0,
@@ -257,8 +260,96 @@ def drop_pair(case, forms, target, max_iter, skip, label):
     return d
 
 
+def closure_eval(a):
+    """The learned closure against Spalart-Allmaras: 56 passages of one size
+    (seeded back pressures) develop under Spalart-Allmaras in one batch; the
+    network is trained on the GPU on the fields of 48 of them; the other 8
+    continue from the state at the split once with the closure and once with
+    Spalart-Allmaras, and their mass flow, pressure ratio and efficiency are
+    compared, with the time per iteration of both."""
+    import numpy as np
+    from fv3d import cases, closure
+    from fv3d.solver import GpuSolver
+    cs = cases.passage_batch(n=56, seed=a.batch_seed + 1, min_cells=a.closure_cells, max_cells=a.closure_cells,
+                             design_rpm=0.0)
+    train, held = list(range(48)), list(range(48, 56))
+    g = GpuSolver(cs, precision=1)
+    g.iterate(a.closure_develop, graph=True)
+    g.sync()
+    split = []   # per passage: restart state at the split
+    for q, c in enumerate(cs):
+        st = []
+        for b in range(1, len(c.blocks) + 1):
+            pv = g.get(b, "pv", inst=q + 1)[1:-1, 1:-1, 1:-1]
+            st.append(dict(rho=pv[..., 0], u=pv[..., 1], v=pv[..., 2], w=pv[..., 3], p=pv[..., 4], T=pv[..., 5],
+                           rhoE=g.get(b, "cv", inst=q + 1)[1:-1, 1:-1, 1:-1, 4],
+                           nut=g.get(b, "ev", inst=q + 1)[1:-1, 1:-1, 1:-1]))
+        split.append(st)
+    print(f"closure: {len(cs)} passages developed {a.closure_develop} iterations", file=sys.stderr, flush=True)
+    g.iterate(a.closure_iters, graph=True)   # the training passages run on to a developed state
+    g.sync()
+    xs, ts = [], []
+    for q in train:
+        for b in range(1, len(cs[q].blocks) + 1):
+            xs.append(g.mlp_features(b, inst=q + 1).reshape(-1, closure.N_IN))
+            ts.append(g.mlp_targets(b, inst=q + 1).ravel())
+    x, t = np.concatenate(xs).astype(np.float32), np.concatenate(ts).astype(np.float32)
+    perm = np.random.default_rng(a.batch_seed).permutation(len(x))
+    x, t = x[perm], t[perm]
+    bounds = closure.gate_bounds(x, t)
+    g.mlp_set(closure.init_params(a.batch_seed), bounds)
+    t0 = time.perf_counter()
+    steps = 3000
+    hist = g.mlp_train(x, t, steps, 4096, 2e-3)
+    train_s = time.perf_counter() - t0
+    P = g.mlp_get()
+    g.close()
+    print(f"closure: trained on {len(x):,} cells, loss {hist[0]:.4f} -> {hist[-1]:.5f}", file=sys.stderr, flush=True)
+    heldc = []
+    for q in held:
+        c = cs[q]
+        for st, s2 in zip(c.states, split[q]):
+            st.update(s2)
+        heldc.append(c)
+    res = {}
+    for mode in ("sa", "closure"):
+        h = GpuSolver(heldc, precision=1)
+        if mode == "closure":
+            h.mlp_set(P, bounds)
+            h.mlp_closure(True)
+        h.iterate(5, graph=True)
+        h.sync()
+        t0 = time.perf_counter()
+        hh = h.iterate(a.closure_iters - 5, graph=True)
+        h.sync()
+        secs = time.perf_counter() - t0
+        res[mode] = {"final": hh[-1], "ms_per_iter": 1e3 * secs / (a.closure_iters - 5),
+                     "tripped": h.mlp_tripped() if mode == "closure" else 0, "cells": h.n_cells}
+        h.close()
+    sa, cl = res["sa"]["final"], res["closure"]["final"]
+    rel = lambda col: np.abs(cl[:, col] - sa[:, col]) / np.abs(sa[:, col])
+    out = {
+        "metric": "learned closure vs Spalart-Allmaras on held-out passages",
+        "config": {"workload": f"56 five-block blade passages of {int(a.closure_cells):,} cells (seed {a.batch_seed + 1}), "
+                               f"back-pressure factor U[0.95, 1.25] on 0.76 Pt, stationary; Spalart-Allmaras for "
+                               f"{a.closure_develop} iterations, then {a.closure_iters} more; trained on 48, "
+                               "evaluated on 8 continued from the split", "precision": "FP32 operator, FP64 residual"},
+        "training": {"samples": int(len(x)), "steps": steps, "batch": 4096, "loss_first": float(hist[0]),
+                     "loss_last": float(hist[-1]), "seconds": train_s},
+        "held_out": {"mass_in_rel_err_max": float(rel(2).max()), "mass_in_rel_err_mean": float(rel(2).mean()),
+                     "pr_rel_err_max": float(rel(4).max()), "eff_rel_err_max": float(rel(5).max()),
+                     "gate_fallbacks": int(res["closure"]["tripped"])},
+        "ms_per_iteration": {"sa": res["sa"]["ms_per_iter"], "closure": res["closure"]["ms_per_iter"],
+                             "cells": res["sa"]["cells"]},
+    }
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
+    if a.workload == "closure":
+        closure_eval(a)
+        return
     if a.calibrate_copy:
         calibrate_copy()
         return

@@ -257,6 +257,9 @@ int fv3d_autotune(fv3d_ctx *ctx, int reps, int *shape3);
 int fv3d_mlp_set(fv3d_ctx *ctx, const float *params, const float *bounds);
 int fv3d_mlp_get(fv3d_ctx *ctx, float *params);
 long fv3d_mlp_features(fv3d_ctx *ctx, int block, float *out);
+/* Training targets log(1 + mue_t / mue_lam) of a block's cells from the
+ * current eddy viscosity (ni,nj,nk floats). */
+long fv3d_mlp_targets(fv3d_ctx *ctx, int block, float *out);
 int fv3d_mlp_forward(fv3d_ctx *ctx, const float *x, long n, float *y);
 int fv3d_mlp_grad(fv3d_ctx *ctx, const float *x, const float *t, long n, float *grad, float *loss);
 int fv3d_mlp_train(fv3d_ctx *ctx, const float *x, const float *t, long n, int steps, int batch, float lr,
