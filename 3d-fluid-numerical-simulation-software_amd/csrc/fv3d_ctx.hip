@@ -151,6 +151,7 @@ struct fv3d_ctx {
     // place of the nu~ transport
     float *d_mlp = nullptr, *d_mlp_m = nullptr, *d_mlp_v = nullptr, *d_mlp_g = nullptr, *d_mlp_bounds = nullptr;
     unsigned *d_mlp_tripped = nullptr;
+    float *d_mlp_xc = nullptr;   // closure: 8 floats per cell (features, gate flag)
     long mlp_step = 0;
     bool mlp_closure = false;
     bool owns_stream = true;
@@ -703,7 +704,7 @@ int fv3d_ctx_destroy(fv3d_ctx *c)
     if (c->d_gstart) (void)hipFree(c->d_gstart);
     if (c->d_parts) (void)hipFree(c->d_parts);
     for (void *p : {(void *)c->d_mlp, (void *)c->d_mlp_m, (void *)c->d_mlp_v, (void *)c->d_mlp_g, (void *)c->d_mlp_bounds,
-                    (void *)c->d_mlp_tripped})
+                    (void *)c->d_mlp_tripped, (void *)c->d_mlp_xc})
         if (p) (void)hipFree(p);
     if (c->d_sa_hi) (void)hipFree(c->d_sa_hi);
     if (c->d_pack_start) (void)hipFree(c->d_pack_start);
@@ -1514,8 +1515,13 @@ static int launch_turbulence(fv3d_ctx *c, int save_old = 0)
     if (c->cfg.turbulence == 200 && c->mlp_closure) {
         // the learned closure in place of the nu~ transport; ghost mue_t as for SA
         const int nb = (int)c->blocks.size();
-        hipLaunchKernelGGL(k_mlp_closure, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
-                           c->d_cstart, nb, c->gas, c->d_mlp, c->d_mlp_bounds, c->d_mlp_tripped);
+        const long chunks = (c->ncell + 255) / 256;   // network: 768 workgroups (three per CU), several chunks each
+        hipLaunchKernelGGL(k_mlp_cell_features, dim3((unsigned)chunks), dim3(256), 0, c->stream, c->d_blocks,
+                           c->d_cstart, nb, (long)c->ncell, c->gas, c->d_mlp_bounds, c->d_mlp_xc);
+        hipLaunchKernelGGL(k_mlp_closure, dim3((unsigned)(chunks < 768 ? chunks : 768)), dim3(256), 0, c->stream,
+                           c->d_mlp, (long)c->ncell, c->d_mlp_xc);
+        hipLaunchKernelGGL(k_mlp_mut, dim3((unsigned)chunks), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb,
+                           c->gas, c->d_mlp_bounds, c->d_mlp_xc, c->d_mlp_tripped);
         hipLaunchKernelGGL(k_sa_ghost, dim3((unsigned)((c->nghost + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
                            c->d_gstart, nb, c->gas);
         HIPCK(c, hipGetLastError());
@@ -2269,7 +2275,7 @@ int fv3d_mlp_forward(fv3d_ctx *c, const float *x, long n, float *y)
     int rc = mlp_upload(c, x, nullptr, n, &dx, &unused);
     if (!rc) rc = dalloc(c, &dy, (size_t)n);
     if (!rc) {
-        hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, c->stream, c->d_mlp, dx, n, dy,
+        hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->d_mlp, dx, n, dy,
                            (float *)nullptr);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -2366,6 +2372,10 @@ int fv3d_mlp_closure(fv3d_ctx *c, int on)
         return fail(c, FV3D_EINVAL, "the learned closure replaces the Spalart-Allmaras transport (turbulence 200)");
     if (on && !c->d_mlp) return fail(c, FV3D_ESTATE, "no network parameters (fv3d_mlp_set)");
     HIPCK(c, hipStreamSynchronize(c->stream));
+    if (on && !c->d_mlp_xc) {
+        const int rc = dalloc(c, &c->d_mlp_xc, (size_t)8 * c->ncell);
+        if (rc) return rc;
+    }
     c->mlp_closure = on != 0;
     drop_graphs(c);
     return FV3D_OK;

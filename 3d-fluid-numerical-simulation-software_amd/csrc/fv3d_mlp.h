@@ -20,8 +20,11 @@ __global__ void k_mlp_targets(const BlockDev *blocks, const long *cstart, int nb
                               float *out);
 __global__ void k_mlp_forward(const float *P, const float *x, long n, float *y, float *zout);
 // bounds: min and max of each feature, then the largest output (2 kMlpIn + 1)
-__global__ void k_mlp_closure(const BlockDev *blocks, const long *cstart, int nblock, Gas g, const float *P,
-                              const float *bounds, unsigned *tripped);
+__global__ void k_mlp_cell_features(const BlockDev *blocks, const long *cstart, int nblock, long ncell, Gas g,
+                                    const float *bounds, float *xc);
+__global__ void k_mlp_closure(const float *P, long ncell, float *xc);
+__global__ void k_mlp_mut(const BlockDev *blocks, const long *cstart, int nblock, Gas g, const float *bounds,
+                          const float *xc, unsigned *tripped);
 __global__ void k_mlp_train_chunk(const float *P, const float *x, const float *tgt, long n, float inv_n, float *partial,
                                   float *loss);
 __global__ void k_mlp_reduce(const float *partial, const float *loss, int nchunk, float *grad, float *loss_sum);

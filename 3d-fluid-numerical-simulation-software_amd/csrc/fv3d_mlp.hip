@@ -94,6 +94,117 @@ __device__ inline float mlp_forward16(const float *P, const float *x, float *h0,
     return z;
 }
 
+// ------------------------------------------------- register-resident inference
+// Inference keeps the network in registers: each wave holds the BF16 weight
+// fragments of layers 0-2 and runs the transposed products h^T = W x^T (the
+// weights are the A operand, 16 samples the B operand), so that the output
+// fragment of one layer is, lane for lane, the B operand of the next.  Lane l
+// receives units 16 nt + 4 (l / 16) + i (nt, i = 0..3) of sample l % 16; the
+// next layer's K index is ordered to match: slice ks, position 8 kb + j holds
+// unit 16 (2 ks + j / 4) + 4 kb + j % 4, and the weights are loaded in that
+// order.  The hidden activations never leave the registers.
+struct MlpRegs {
+    bf16x8 w0[4], w1[8], w2[8];   // [nt] and [2 nt + ks]
+};
+
+__device__ inline int k_unit(int ks, int kb, int j) { return 16 * (2 * ks + (j >> 2)) + 4 * kb + (j & 3); }
+
+__device__ inline void mlp_load(const float *__restrict__ P, int lane, MlpRegs &R)
+{
+    const int row = lane & 15, kb = lane >> 4;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            R.w0[nt][j] = (__bf16)((kb == 0 && j < kMlpIn) ? P[kMlpW0 + (16 * nt + row) * kMlpIn + j] : 0.0f);
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int u = k_unit(ks, kb, j);
+                R.w1[2 * nt + ks][j] = (__bf16)P[kMlpW1 + (16 * nt + row) * kMlpH + u];
+                R.w2[2 * nt + ks][j] = (__bf16)P[kMlpW2 + (16 * nt + row) * kMlpH + u];
+            }
+    }
+}
+
+// tanh(x) = 1 - 2 / (1 + e^(2x)): saturates to +-1 without overflow trouble,
+// absolute error near 1e-7, far below the BF16 rounding of the activations.
+__device__ inline float tanh_fast(float x) { return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * x)); }
+
+__device__ inline void dense64_t(const bf16x8 *W, const volatile float *b, float h[4][4], int kb)
+{
+    bf16x8 in[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) in[ks][j] = (__bf16)h[2 * ks + (j >> 2)][j & 3];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = *(const volatile f32x4 *)(b + 16 * nt + 4 * kb);
+        acc = mfma16(W[2 * nt], in[0], acc);
+        acc = mfma16(W[2 * nt + 1], in[1], acc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h[nt][i] = tanh_fast(acc[i]);
+    }
+}
+
+// The small parameters of the network in LDS (biases of layers 0-2, the
+// output weights and bias), staged once per workgroup: read there through a
+// volatile pointer at each use, so the compiler keeps them out of the
+// registers that hold the weight fragments.
+constexpr int kSmallB0 = 0, kSmallB1 = 64, kSmallB2 = 128, kSmallW3 = 192, kSmallB3 = 256, kSmall = 260;
+
+__device__ inline void stage_small(const float *__restrict__ P, float *sp, int t)
+{
+    for (int e = t; e < kSmall; e += blockDim.x) {
+        float v = 0.0f;
+        if (e < kSmallB3) v = P[(e < kSmallB1 ? kMlpB0 : e < kSmallB2 ? kMlpB1 - 64 : e < kSmallW3 ? kMlpB2 - 128
+                                 : kMlpW3 - 192) + e];
+        else if (e == kSmallB3) v = P[kMlpB3];
+        sp[e] = v;
+    }
+}
+
+// The pre-softplus output z of 16 samples; xb is the layer-0 B operand (lanes
+// 0..15: the six features of sample l in positions 0..5, zeros elsewhere).
+// Every lane returns z of sample l % 16.
+__device__ inline float mlp_infer16(const MlpRegs &R, const float *sp, bf16x8 xb, int lane)
+{
+    const int kb = lane >> 4;
+    const volatile float *vs = sp;
+    float h[4][4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = *(const volatile f32x4 *)(vs + kSmallB0 + 16 * nt + 4 * kb);
+        acc = mfma16(R.w0[nt], xb, acc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h[nt][i] = tanh_fast(acc[i]);
+    }
+    dense64_t(R.w1, vs + kSmallB1, h, kb);
+    dense64_t(R.w2, vs + kSmallB2, h, kb);
+    float z = 0.0f;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const f32x4 w = *(const volatile f32x4 *)(vs + kSmallW3 + 16 * nt + 4 * kb);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) z += w[i] * h[nt][i];
+    }
+    z += __shfl_xor(z, 16);
+    z += __shfl_xor(z, 32);
+    return z + vs[kSmallB3];
+}
+
+// Layer-0 operand of samples 16 q .. 16 q + 15 of a wave's 64-row tile sx.
+__device__ inline bf16x8 input_operand(const float *sx, int q, int lane)
+{
+    bf16x8 xb;
+    const int row = lane & 15, kb = lane >> 4;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) xb[j] = (__bf16)((kb == 0 && j < kMlpIn) ? sx[(16 * q + row) * kMlpXS + j] : 0.0f);
+    return xb;
+}
+
 // ------------------------------------------------------------ features
 // The six inputs of a cell, from the cell-centred gradients of the solver's
 // own stencil (central differences in index space times the cell
@@ -179,90 +290,130 @@ __global__ void __launch_bounds__(256) k_mlp_targets(const BlockDev *__restrict_
 
 // ----------------------------------------------------------- inference
 // Network output of n samples (features x, 6 per sample) into y (softplus
-// output) and z (pre-activation); 64 samples per workgroup, 16 per wave.
+// output) and z (pre-activation); 256 samples per workgroup, 64 per wave, the
+// network in registers (mlp_infer16), as the closure evaluates it.
 __global__ void __launch_bounds__(256) k_mlp_forward(const float *__restrict__ P, const float *__restrict__ x, long n,
                                                      float *__restrict__ y, float *__restrict__ zout)
 {
-    __shared__ float sx[64 * kMlpXS];
-    __shared__ float sh[2][64 * kHS];
+    __shared__ float sx[4][64 * kMlpXS];
+    __shared__ __attribute__((aligned(16))) float sp[kSmall];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    const long s0 = (long)blockIdx.x * 64;
-    for (int e = t; e < 64 * kMlpXS; e += 256) {
-        const int r = e / kMlpXS, q = e % kMlpXS;
-        sx[e] = (q < kMlpIn && s0 + r < n) ? x[(s0 + r) * kMlpIn + q] : 0.0f;
-    }
+    const long s = (long)blockIdx.x * 256 + t;
+#pragma unroll
+    for (int q = 0; q < kMlpXS; ++q) sx[wave][lane * kMlpXS + q] = (q < kMlpIn && s < n) ? x[s * kMlpIn + q] : 0.0f;
+    stage_small(P, sp, t);
+    MlpRegs R;
+    mlp_load(P, lane, R);
     __syncthreads();
-    const float z = mlp_forward16(P, sx, sh[0], sh[1], sh[0], wave * 16, lane);
-    const long s = s0 + wave * 16 + lane;
-    if (lane < 16 && s < n) {
-        y[s] = softplus(z);
-        if (zout) zout[s] = z;
+    float zmine = 0.0f;
+#pragma unroll 1
+    for (int q = 0; q < 4; ++q) {
+        const float z = mlp_infer16(R, sp, input_operand(sx[wave], q, lane), lane);
+        if (q == (lane >> 4)) zmine = z;
+    }
+    if (s < n) {
+        y[s] = softplus(zmine);
+        if (zout) zout[s] = zmine;
     }
 }
 
-// The closure in the residual evaluation: features, network, gate, mue_t of
-// every interior cell; 256 cells per workgroup, each wave runs the network on
-// 64 of them, 16 at a time.  A cell whose features leave the training range
-// (bounds: min, max per feature, widened by 10 % of the span) or whose output
-// is not finite or above the largest trained value keeps the Spalart-Allmaras
-// value of its frozen nu~ (fv1 rho nu~ / Re, clipped as SA_Model clips it);
-// tripped counts those cells.
-__global__ void __launch_bounds__(256) k_mlp_closure(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
-                                                     int nblock, Gas g, const float *__restrict__ P,
-                                                     const float *__restrict__ bounds, unsigned *__restrict__ tripped)
+// The closure in the residual evaluation, two passes over the interior cells.
+// First the features and the input half of the gate, one thread per cell, into
+// xc (8 floats per cell of the padded cell numbering: features 0..5, then 1
+// where every feature lies in the training range -- bounds: min, max per
+// feature, widened by 10 % of the span -- else 0).
+__global__ void __launch_bounds__(256) k_mlp_cell_features(const BlockDev *__restrict__ blocks,
+                                                           const long *__restrict__ cstart, int nblock, long ncell, Gas g,
+                                                           const float *__restrict__ bounds, float *__restrict__ xc)
 {
-    __shared__ float sx[256 * kMlpXS];
-    __shared__ float sh[4][2][16 * kHS];   // per wave: activation ping-pong of 16 rows
-    __shared__ int sok[256];
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncell) return;
+    int n, i, j, k;
+    float f[kMlpIn] = {0, 0, 0, 0, 0, 0};
+    bool ok = cell_item(blocks, cstart, nblock, t, n, i, j, k);
+    if (ok) {
+        cell_features(blocks[n], g, i, j, k, f);
+        for (int q = 0; q < kMlpIn; ++q) {
+            const float lo = bounds[2 * q], hi = bounds[2 * q + 1], pad = 0.1f * (hi - lo);
+            ok = ok && isfinite(f[q]) && f[q] >= lo - pad && f[q] <= hi + pad;
+        }
+    }
+    float4 *o = (float4 *)(xc + 8 * t);
+    o[0] = make_float4(f[0], f[1], f[2], f[3]);
+    o[1] = make_float4(f[4], f[5], ok ? 1.0f : 0.0f, 0.0f);
+}
+
+// Then the network: 256-cell chunks per workgroup in a grid-stride loop (the
+// weight fragments are loaded once per wave for several chunks), each wave
+// runs the network on 64 of a chunk's cells, 16 at a time, and leaves the
+// softplus output in slot 7 of the cell's xc row.
+__global__ void __launch_bounds__(256) k_mlp_closure(const float *__restrict__ P, long ncell, float *__restrict__ xc)
+{
+    __shared__ float sx[4][64 * kMlpXS];   // per wave: the features of its 64 cells
+    __shared__ __attribute__((aligned(16))) float sp[kSmall];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    const long c0 = (long)blockIdx.x * 256;
-    // one thread per cell: the features and the input half of the gate
-    {
-        int n, i, j, k;
-        const bool valid = cell_item(blocks, cstart, nblock, c0 + t, n, i, j, k);
-        float f[kMlpIn] = {0, 0, 0, 0, 0, 0};
-        bool ok = valid;
-        if (valid) {
-            cell_features(blocks[n], g, i, j, k, f);
-            for (int q = 0; q < kMlpIn; ++q) {
-                const float lo = bounds[2 * q], hi = bounds[2 * q + 1], pad = 0.1f * (hi - lo);
-                ok = ok && isfinite(f[q]) && f[q] >= lo - pad && f[q] <= hi + pad;
-            }
-        }
-        for (int q = 0; q < kMlpXS; ++q) sx[t * kMlpXS + q] = (q < kMlpIn && ok) ? f[q] : 0.0f;
-        sok[t] = valid ? (ok ? 1 : 2) : 0;
-    }
+    stage_small(P, sp, t);
+    MlpRegs R;
+    mlp_load(P, lane, R);
     __syncthreads();
-    float zq[4];
-    for (int q = 0; q < 4; ++q)
-        zq[q] = mlp_forward16(P, sx + (wave * 64 + 16 * q) * kMlpXS, sh[wave][0], sh[wave][1], sh[wave][0], 0, lane);
-    if (lane >= 16) return;
-    for (int q = 0; q < 4; ++q) {
-        const float z = zq[q];
-        const int r = wave * 64 + 16 * q + lane;
-        if (sok[r] == 0) continue;
-        int n, i, j, k;
-        cell_item(blocks, cstart, nblock, c0 + r, n, i, j, k);
-        const BlockDev &B = blocks[n];
-        const long box = B.box, x = cidx(B, i, j, k);
-        const float yhat = softplus(z);
-        const double mue_lam = sutherland(g, B.W[5 * box + x]);
-        double mt;
-        if (sok[r] == 1 && isfinite(yhat) && yhat <= bounds[2 * kMlpIn]) {
-            mt = mue_lam * expm1((double)yhat);
-        } else {
-            atomicAdd(tripped, 1u);
-            // Spalart-Allmaras fallback from the frozen nu~ (source/SA_Model.cpp:47-87)
-            const double rho = B.W[x], ev = B.ev[x] < 0.0 ? 0.0 : B.ev[x];
-            const double ev_lam = g.re_ref * mue_lam / rho, X = ev / ev_lam;
-            const double Cv1 = F32C(7.1);
-            const double fv1 = pow(X, F32C(3.0)) / (pow(X, F32C(3.0)) + Cv1 * Cv1 * Cv1);
-            mt = fv1 * rho * ev / g.re_ref;
-            if (mt < 0.0) mt = 0.0;
+    // every wave of the workgroup runs the same chunks: the matrix cores need
+    // all 64 lanes, so cells past the end ride along as zeros
+    for (long c0 = (long)blockIdx.x * 256; c0 < ncell; c0 += (long)gridDim.x * 256) {
+        const long c = c0 + t;
+        float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a;
+        if (c < ncell) {
+            a = ((const float4 *)(xc + 8 * c))[0];
+            b = ((const float4 *)(xc + 8 * c))[1];
         }
-        if (mue_lam > 0 && mt > 200.0 * mue_lam) mt = 200.0 * mue_lam;
-        B.mut[x] = mt;
+        const bool ok = b.z != 0.0f;
+        float *r = sx[wave] + lane * kMlpXS;
+        r[0] = ok ? a.x : 0.0f, r[1] = ok ? a.y : 0.0f, r[2] = ok ? a.z : 0.0f, r[3] = ok ? a.w : 0.0f;
+        r[4] = ok ? b.x : 0.0f, r[5] = ok ? b.y : 0.0f, r[6] = 0.0f, r[7] = 0.0f;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        float zmine = 0.0f;
+#pragma unroll 1
+        for (int q = 0; q < 4; ++q) {
+            const float z = mlp_infer16(R, sp, input_operand(sx[wave], q, lane), lane);
+            if (q == (lane >> 4)) zmine = z;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        if (c < ncell) xc[8 * c + 7] = softplus(zmine);
     }
+}
+
+// Last, mue_t of every interior cell from the network output, or, for a cell
+// outside the training range or whose output is not finite or above the
+// largest trained value, the Spalart-Allmaras value of its frozen nu~ (fv1 rho
+// nu~ / Re, clipped as SA_Model clips it); tripped counts those cells.
+__global__ void __launch_bounds__(256) k_mlp_mut(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
+                                                 int nblock, Gas g, const float *__restrict__ bounds,
+                                                 const float *__restrict__ xc, unsigned *__restrict__ tripped)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int n, i, j, k;
+    if (!cell_item(blocks, cstart, nblock, c, n, i, j, k)) return;
+    const BlockDev &B = blocks[n];
+    const long box = B.box, x = cidx(B, i, j, k);
+    const float yhat = xc[8 * c + 7];
+    const bool ok = xc[8 * c + 6] != 0.0f;
+    const double mue_lam = sutherland(g, B.W[5 * box + x]);
+    double mt;
+    if (ok && isfinite(yhat) && yhat <= bounds[2 * kMlpIn]) {
+        mt = mue_lam * expm1((double)yhat);
+    } else {
+        atomicAdd(tripped, 1u);
+        // Spalart-Allmaras fallback from the frozen nu~ (source/SA_Model.cpp:47-87)
+        const double rho = B.W[x], ev = B.ev[x] < 0.0 ? 0.0 : B.ev[x];
+        const double ev_lam = g.re_ref * mue_lam / rho, X = ev / ev_lam;
+        const double Cv1 = F32C(7.1);
+        const double fv1 = pow(X, F32C(3.0)) / (pow(X, F32C(3.0)) + Cv1 * Cv1 * Cv1);
+        mt = fv1 * rho * ev / g.re_ref;
+        if (mt < 0.0) mt = 0.0;
+    }
+    if (mue_lam > 0 && mt > 200.0 * mue_lam) mt = 200.0 * mue_lam;
+    B.mut[x] = mt;
 }
 
 // ------------------------------------------------------------ training
