@@ -69,6 +69,9 @@ struct fv3d_ctx {
     int2 *d_parts = nullptr;
     std::vector<HostPatch> patches;
     std::vector<HostConn> conns;
+    std::vector<HostConn> mixes;     // mixing planes: block, rng of both sides (kind, face, orient unused)
+    MixDev *d_mix = nullptr;
+    int nmix = 0, nmix_lines = 0;
     bool ready = false;
     std::string err;
     // device work lists
@@ -571,6 +574,24 @@ static int build_exchange(fv3d_ctx *c)
     c->nexch = (long)el.size();
     c->nper = (int)per.size();
     int rc;
+    if (!c->mixes.empty()) {
+        std::vector<MixDev> ml;
+        int line = 0;
+        for (const HostConn &T : c->mixes) {
+            MixDev m;
+            m.b1 = T.block[0] - 1; m.b2 = T.block[1] - 1;
+            m.i0 = T.rng[0][0]; m.ni = T.rng[0][1] - T.rng[0][0] + 1;
+            m.j1 = T.rng[0][2]; m.nj1 = T.rng[0][3] - T.rng[0][2] + 1;
+            m.j2 = T.rng[1][2]; m.nj2 = T.rng[1][3] - T.rng[1][2] + 1;
+            m.line0 = line;
+            line += m.ni;
+            ml.push_back(m);
+        }
+        c->nmix = (int)ml.size();
+        c->nmix_lines = line;
+        if ((rc = dalloc(c, &c->d_mix, ml.size()))) return rc;
+        HIPCK(c, hipMemcpyAsync(c->d_mix, ml.data(), ml.size() * sizeof(MixDev), hipMemcpyHostToDevice, c->stream));
+    }
     if (!el.empty()) {
         if ((rc = dalloc(c, &c->d_exch, el.size()))) return rc;
         HIPCK(c, hipMemcpyAsync(c->d_exch, el.data(), el.size() * sizeof(ExchDev), hipMemcpyHostToDevice, c->stream));
@@ -695,6 +716,7 @@ int fv3d_ctx_destroy(fv3d_ctx *c)
     if (c->graph_coarse) (void)hipGraphExecDestroy(c->graph_coarse);
     if (c->d_bl_walls) (void)hipFree(c->d_bl_walls);
     if (c->d_exch) (void)hipFree(c->d_exch);
+    if (c->d_mix) (void)hipFree(c->d_mix);
     if (c->d_per) (void)hipFree(c->d_per);
     if (c->d_pitch_sign) (void)hipFree(c->d_pitch_sign);
     if (c->d_mbox) (void)hipFree(c->d_mbox);
@@ -768,6 +790,34 @@ int fv3d_instances(const fv3d_ctx *c)
 {
     if (!c) return FV3D_EINVAL;
     return c->ready ? c->ninst : (c->explicit_inst ? (int)c->insts.size() : 1);
+}
+
+int fv3d_mixing_plane_add(fv3d_ctx *c, int block1, const int *rng1, int block2, const int *rng2)
+{
+    if (!c || !rng1 || !rng2) return FV3D_EINVAL;
+    if (c->ready) return fail(c, FV3D_ESTATE, "mixing_plane_add after setup");
+    const int blk[2] = {block1, block2};
+    for (int s = 0; s < 2; ++s)
+        if (blk[s] < 1 || blk[s] > (int)c->blocks.size()) return fail(c, FV3D_EINVAL, "mixing plane block %d", blk[s]);
+    const HostBlock &a = c->blocks[block1 - 1], &b = c->blocks[block2 - 1];
+    if (rng1[4] != a.KM || rng1[5] != a.KM || rng2[4] != 1 || rng2[5] != 1)
+        return fail(c, FV3D_EINVAL, "a mixing plane joins block 1's K-high face to block 2's K-low face");
+    if (rng1[0] < 1 || rng1[0] > rng1[1] || rng1[1] > a.IM - 1 || rng1[1] > b.IM - 1)
+        return fail(c, FV3D_EINVAL, "mixing plane radial range %d..%d outside both blocks", rng1[0], rng1[1]);
+    if (rng1[2] < 1 || rng1[2] > rng1[3] || rng1[3] > a.JM - 1 || rng2[2] < 1 || rng2[2] > rng2[3] ||
+        rng2[3] > b.JM - 1)
+        return fail(c, FV3D_EINVAL, "mixing plane pitch-wise range outside its block");
+    if (rng1[3] - rng1[2] + 1 > kMixMaxJ || rng2[3] - rng2[2] + 1 > kMixMaxJ)
+        return fail(c, FV3D_EINVAL, "mixing plane wider than %d cells pitch-wise", kMixMaxJ);
+    if (a.inst != b.inst) return fail(c, FV3D_EINVAL, "a mixing plane between blocks of different instances");
+    HostConn t;
+    t.kind = 2; t.orient = 0;
+    t.block[0] = block1; t.block[1] = block2;
+    t.face[0] = 6; t.face[1] = 5;
+    memcpy(t.rng[0], rng1, sizeof t.rng[0]);
+    memcpy(t.rng[1], rng2, sizeof t.rng[1]);
+    c->mixes.push_back(t);
+    return FV3D_OK;
 }
 
 int fv3d_connection_add(fv3d_ctx *c, int kind, int orient, int block1, int face1, const int *rng1, int block2,
@@ -924,6 +974,12 @@ static int setup_coarse(fv3d_ctx *c)
         coarse_range(T.rng[1], o2);
         if ((rc = fv3d_connection_add(m, T.kind, T.orient, T.block[0], T.face[0], o1, T.block[1], T.face[1], o2)))
             return bail(rc);
+    }
+    for (const HostConn &T : c->mixes) {
+        int o1[6], o2[6];
+        coarse_range(T.rng[0], o1);
+        coarse_range(T.rng[1], o2);
+        if ((rc = fv3d_mixing_plane_add(m, T.block[0], o1, T.block[1], o2))) return bail(rc);
     }
     if ((rc = fv3d_setup(m))) return bail(rc);
     std::vector<int4> tl;
@@ -1415,6 +1471,9 @@ static int launch_bc(fv3d_ctx *c)
     if (c->nexch > 0)
         hipLaunchKernelGGL(k_exchange, dim3((unsigned)((c->nexch + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
                            c->d_exch, c->nexch, c->d_per);
+    if (c->nmix_lines > 0)   // after interior and periodic patches (source/BoundaryCondition.cpp:59-62)
+        hipLaunchKernelGGL(k_mixing_plane, dim3((unsigned)c->nmix_lines), dim3(256), 0, c->stream, c->d_blocks, c->d_mix,
+                           c->nmix, c->gas, c->cfg.turbulence == 200 ? 1 : 0);
     HIPCK(c, hipGetLastError());
     return FV3D_OK;
 }

@@ -27,6 +27,14 @@ struct PatchDev {
     double rot;        // wall rotation rate
 };
 
+// Mixing plane: block b1's K-high face against block b2's K-low face, radial
+// lines i0 .. i0+ni-1 of both, pitch-wise cells j1 .. j1+nj1-1 of b1 and
+// j2 .. j2+nj2-1 of b2; line0 numbers the first line among all planes.
+constexpr int kMixMaxJ = 512;
+struct MixDev {
+    int b1, b2, i0, ni, j1, nj1, j2, nj2, line0;
+};
+
 // Per-run scalars kept on the device so iterations need no host round trip.
 struct ScalarsDev {
     double cfl, cfl_res;
@@ -188,5 +196,7 @@ __global__ void k_sa_plane(const BlockDev *blocks, const long *pstart, int nbloc
 // the others (clamp set) with the Res_ev partials in component 5
 __global__ void k_sa_limit_serial(const BlockDev *blocks, int nblock, int *hi_flag);
 __global__ void k_sa_res(const BlockDev *blocks, const long *cstart, int nblock, double *partial, int clamp);
+
+__global__ void k_mixing_plane(const BlockDev *blocks, const MixDev *mix, int nmix, Gas g, int sa);
 
 }  // namespace fv3d

@@ -1959,4 +1959,127 @@ __global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ par
     if (ramp->cfl > F32C(10.0)) ramp->cfl = F32C(10.0);
 }
 
+// ============================================================ mixing plane
+// CMesh::BndCondMixingPlane (source/BndCondMixingPlane.cpp:287-522), one
+// workgroup per radial line i of a plane.  The threads form the per-face
+// terms of each side's pitch-wise sums into LDS; one thread adds them in j
+// order (the reference's order) and forms the mixed-out state of each side:
+// the flux-averaged mass, momentum, energy (and rho nu~) fluxes, pressure from
+// the quadratic of lines 370 / 432, then velocity, density, temperature and
+// the speed of sound.  The threads then write the characteristic ghost
+// states: block 1's K-high ghosts (and Dummy layers) from the downstream
+// average plus its outgoing waves, block 2's K-low ghosts from the upstream
+// average plus its outgoing wave; nu~ ghosts take the upstream average.
+struct MixAvg {
+    double d, u, v, w, p, c, ev;
+};
+
+__device__ static void mix_terms(const BlockDev &B, int i, int j, int kf, int kc, int sa, double *t)
+{
+    const long xf = cidx(B, i, j, kf), xc = cidx(B, i, j, kc), box = B.box;
+    const double s = B.fs[2][xf];
+    const double nx = B.fS[2][xf] / s, ny = B.fS[2][box + xf] / s, nz = B.fS[2][2 * box + xf] / s;
+    const double u = B.W[box + xc], v = B.W[2 * box + xc], w = B.W[3 * box + xc], p = B.W[4 * box + xc];
+    const double Vn = u * nx + v * ny + w * nz;
+    t[0] = s;
+    t[1] = nx * s;
+    t[2] = ny * s;
+    t[3] = nz * s;
+    t[4] = B.Q[xc] * Vn * s;
+    t[5] = (B.Q[box + xc] * Vn + nx * p) * s;
+    t[6] = (B.Q[2 * box + xc] * Vn + ny * p) * s;
+    t[7] = (B.Q[3 * box + xc] * Vn + nz * p) * s;
+    t[8] = (B.Q[4 * box + xc] + p) * Vn * s;
+    t[9] = sa ? B.Q[xc] * B.ev[xc] * Vn * s : 0.0;
+}
+
+__device__ static MixAvg mix_reduce(const double (*t)[10], int nj, const Gas &g, int sa)
+{
+    const double Gamma = g.gamma, Rgas = g.rgas;
+    double s_sum = 0.0, nsx = 0.0, nsy = 0.0, nsz = 0.0, I[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < nj; ++j) {
+        nsx = nsx + t[j][1];
+        nsy = nsy + t[j][2];
+        nsz = nsz + t[j][3];
+        s_sum = s_sum + t[j][0];
+        for (int m = 1; m <= 6; ++m) I[m] = I[m] + t[j][3 + m];
+    }
+    MixAvg a;
+    for (int m = 1; m <= 5; ++m) I[m] = I[m] / s_sum;
+    a.ev = 0.0;
+    if (sa) {
+        I[6] = I[6] / s_sum;
+        a.ev = I[6] / I[1];
+    }
+    const double nx = nsx / s_sum, ny = nsy / s_sum, nz = nsz / s_sum;
+    const double tA = I[2] * nx + I[3] * ny + I[4] * nz;
+    const double tB = F32C(2.0) * I[1] * I[5] - (I[2] * I[2] + I[3] * I[3] + I[4] * I[4]);
+    const double tC = nx * nx + ny * ny + nz * nz;
+    a.p = (tA / tC + sqrt(tA * tA / tC / tC - (Gamma * Gamma - F32C(1.0)) * tB / tC)) / (Gamma + F32C(1.0));
+    a.u = (I[2] - a.p * nx) / I[1];
+    a.v = (I[3] - a.p * ny) / I[1];
+    a.w = (I[4] - a.p * nz) / I[1];
+    a.d = I[1] / (a.u * nx + a.v * ny + a.w * nz);
+    const double T = a.p / Rgas / a.d;
+    a.c = sqrt(Gamma * Rgas * T);
+    return a;
+}
+
+__device__ static void mix_ghost(const BlockDev &B, int s, int i, int j, int kg, double d, double u, double v,
+                                 double w, double p, double ev, const Gas &g, int sa)
+{
+    const double Gamma = g.gamma, Rgas = g.rgas;
+    const double pv[6] = {d, u, v, w, p, p / Rgas / d};
+    const double V2 = u * u + v * v + w * w;
+    const double cv[5] = {d, d * u, d * v, d * w, p / (Gamma - F32C(1.0)) + d * V2 / F32C(2.0)};
+    const long x = cidx(B, i, j, kg), box = B.box;
+    for (int c = 0; c < 5; ++c) B.Q[c * box + x] = cv[c];
+    for (int c = 0; c < 6; ++c) B.W[c * box + x] = pv[c];
+    if (sa) B.ev[x] = ev;
+    set_dummies(B, s, i, j, cv, pv);
+}
+
+__global__ void __launch_bounds__(256) k_mixing_plane(const BlockDev *__restrict__ blocks,
+                                                      const MixDev *__restrict__ mix, int nmix, Gas g, int sa)
+{
+    __shared__ double t[kMixMaxJ][10];
+    __shared__ MixAvg avg[2];
+    int q = 0;
+    while (q + 1 < nmix && mix[q + 1].line0 <= (int)blockIdx.x) ++q;
+    const MixDev M = mix[q];
+    const int i = M.i0 + ((int)blockIdx.x - M.line0);
+    const BlockDev &U = blocks[M.b1], &D = blocks[M.b2];
+    const int KM = U.KM;
+    for (int side = 0; side < 2; ++side) {
+        const int nj = side ? M.nj2 : M.nj1, j0 = side ? M.j2 : M.j1;
+        for (int r = threadIdx.x; r < nj; r += blockDim.x)
+            mix_terms(side ? D : U, i, j0 + r, side ? 1 : KM, side ? 1 : KM - 1, sa, t[r]);
+        __syncthreads();
+        if (threadIdx.x == 0) avg[side] = mix_reduce(t, nj, g, sa);
+        __syncthreads();
+    }
+    const MixAvg u = avg[0], d = avg[1];
+    for (int r = threadIdx.x; r < M.nj1; r += blockDim.x) {   // lines 460-491
+        const int j = M.j1 + r;
+        const long x = cidx(U, i, j, KM - 1), box = U.box;
+        const double pd = U.W[x], pu = U.W[box + x], pvv = U.W[2 * box + x], pw = U.W[3 * box + x],
+                     pp = U.W[4 * box + x];
+        const double C1 = -u.c * u.c * (pd - u.d) + (pp - u.p);
+        const double C2 = u.d * u.c * (pw - u.w) + (pp - u.p);
+        const double C5 = 0.0;
+        mix_ghost(U, 5, i, j, KM, d.d + (-C1 + (C2 + C5) / F32C(2.0)) / (d.c * d.c), pu, pvv,
+                  d.w + (C2 - C5) / (F32C(2.0) * d.d * d.c), d.p + (C2 + C5) / F32C(2.0), u.ev, g, sa);
+    }
+    for (int r = threadIdx.x; r < M.nj2; r += blockDim.x) {   // lines 494-522
+        const int j = M.j2 + r;
+        const long x = cidx(D, i, j, 1), box = D.box;
+        const double pw = D.W[3 * box + x], pp = D.W[4 * box + x];
+        const double C1 = 0.0, C2 = 0.0, C3 = 0.0, C4 = 0.0;
+        const double C5 = (F32C(0.0) - d.d * d.c * (pw - d.w) + (pp - d.p));
+        mix_ghost(D, 4, i, j, 0, u.d + (-C1 + (C2 + C5) / F32C(2.0)) / (u.c * u.c), u.u + C4 / (u.d * u.c),
+                  u.v + C3 / (u.d * u.c), u.w - C5 / (F32C(2.0) * u.d * u.c), u.p + (C2 + C5) / F32C(2.0), u.ev, g,
+                  sa);
+    }
+}
+
 }  // namespace fv3d

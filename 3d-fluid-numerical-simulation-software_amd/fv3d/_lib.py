@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libfv3d.so")
 # every symbol declared in include/fv3d.h
 EXPORTS = [
     "fv3d_ctx_create", "fv3d_ctx_destroy", "fv3d_last_error", "fv3d_block_add", "fv3d_patch_add",
-    "fv3d_connection_add", "fv3d_setup", "fv3d_levels", "fv3d_instance_add", "fv3d_instances",
+    "fv3d_mixing_plane_add", "fv3d_connection_add", "fv3d_setup", "fv3d_levels", "fv3d_instance_add", "fv3d_instances",
     "fv3d_instance_results", "fv3d_initial",
     "fv3d_state_upload", "fv3d_nut_upload", "fv3d_field_download", "fv3d_level_field_download", "fv3d_bc_apply", "fv3d_spectral", "fv3d_timestep", "fv3d_convective",
     "fv3d_viscous", "fv3d_curl", "fv3d_resflux", "fv3d_lusgs", "fv3d_res_calculate", "fv3d_perf", "fv3d_iterate", "fv3d_iterate_graph", "fv3d_sync",
@@ -59,6 +59,7 @@ def load():
         "fv3d_block_add": (i, [vp, i, i, i, p, i, d]),
         "fv3d_patch_add": (i, [vp, i, i, i, p, i]),
         "fv3d_connection_add": (i, [vp, i, i, i, i, p, i, i, p]),
+        "fv3d_mixing_plane_add": (i, [vp, i, p, i, p]),
         "fv3d_setup": (i, [vp]),
         "fv3d_state_upload": (i, [vp, i, p, p]),
         "fv3d_nut_upload": (i, [vp, i, p]),

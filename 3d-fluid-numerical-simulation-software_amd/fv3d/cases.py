@@ -298,6 +298,21 @@ def annular_passage(ni: int = 8, nj: int = 8, nk_blocks: Sequence[int] = (16,), 
     return Case(cfg, nodes, patches, blocks, states)
 
 
+def with_mixing_plane(case: Case, block1: int, block2: int) -> Case:
+    """The interior patch between block1's K-high face and block2's K-low face
+    becomes a mixing plane (source/BndCondMixingPlane.cpp:287-522): each side
+    then sees the pitch-wise flux average of the other."""
+    for q, p in enumerate(case.patches):
+        if p.kind != "interior":
+            continue
+        if (p.block, p.face, p.block2, p.face2) == (block2, 5, block1, 6):
+            p = dataclasses.replace(p, block=block1, face=6, rng=p.rng2, block2=block2, face2=5, rng2=p.rng)
+        if (p.block, p.face, p.block2, p.face2) == (block1, 6, block2, 5):
+            case.patches[q] = dataclasses.replace(p, kind="mixing", orient=1)
+            return case
+    raise ValueError(f"no interior patch from block {block1} (K high) to block {block2} (K low)")
+
+
 def turn_block(case: Case, block: int, quarter: int) -> Case:
     """Re-indexes block ``block`` (1-based) by a rotation of its (i, j) index
     plane by ``quarter`` x 90 degrees, keeping the grid, the state and every

@@ -95,6 +95,12 @@ class GpuSolver:
                 self._check(rc, "block_add")
             nblock += len(cs.nodes)
             for p in cs.patches:
+                if p.kind == "mixing":
+                    r1 = np.ascontiguousarray(p.rng, dtype=np.int32)
+                    r2 = np.ascontiguousarray(p.rng2, dtype=np.int32)
+                    self._check(self.L.fv3d_mixing_plane_add(self.h, p.block + off, r1.ctypes.data, p.block2 + off,
+                                                             r2.ctypes.data), "mixing_plane_add")
+                    continue
                 if p.kind in ("interior", "periodic"):
                     r1 = np.ascontiguousarray(p.rng, dtype=np.int32)
                     r2 = np.ascontiguousarray(p.rng2, dtype=np.int32)

@@ -80,6 +80,17 @@ int fv3d_block_add(fv3d_ctx *ctx, int IM, int JM, int KM, const double *nodes, i
  * kind 0 inlet, 1 outlet, 2 wall; rng = is ie js je ks ke. */
 int fv3d_patch_add(fv3d_ctx *ctx, int kind, int block, int face, const int *rng, int rotate);
 
+/* Mixing plane between two blade rows (replaces the MixingPlane rows of the
+ * boundary table, source/UserInput.cpp:420-432, applied by
+ * CMesh::BndCondMixingPlane, source/BndCondMixingPlane.cpp:287-522): block1's
+ * K-high face (rng1 = {is, ie, js, je, KM, KM}, cell ranges) against block2's
+ * K-low face (rng2 = {is, ie, js, je, 1, 1}); both sides use rng1's radial
+ * range, at most 512 cells pitch-wise per side.  Per radial line the
+ * flux-averaged state of each side sets the other side's ghost cells through
+ * characteristic relations; nu~ ghosts take the upstream average.  Applied
+ * after the interior and periodic patches, on every grid level. */
+int fv3d_mixing_plane_add(fv3d_ctx *c, int block1, const int *rng1, int block2, const int *rng2);
+
 /* Adds an interior (kind 0) or periodic (kind 1) patch between face1 of block1
  * and face2 of block2 (CUserInput::ReadBndGrid, source/UserInput.cpp:386-417;
  * applied by CMesh::BndCondInterior / BndCondPeriodic,

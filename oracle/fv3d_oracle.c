@@ -83,6 +83,10 @@ struct orc_ctx {
     blk_t blk[64];
     conn_t conn[256];
     int nconn;
+    /* mixing planes (source/UserInput.cpp:420-432): block 1's K-high face
+     * against block 2's K-low face, cell ranges as the table gives them */
+    struct { int blk[2], rng[2][6]; } mix[16];
+    int nmix;
     int first_in[2], first_out[2];   /* block, patch index of BndGeom.Inlet[1] / Outlet[1] */
     int ng;                          /* grid levels in use */
     int force;                       /* coarse level: UseForceFunction (source/LU_SGS_Marching.cpp:49-57) */
@@ -271,6 +275,21 @@ int orc_block_add(orc_ctx *c, int IM, int JM, int KM, const double *nodes, int n
         b->fv1_tur[d] = (double *)calloc(box, sizeof(double));
     }
     return c->nb - 1;
+}
+
+int orc_mixing_plane_add(orc_ctx *c, int block1, const int *rng1, int block2, const int *rng2)
+{
+    if (c->nmix >= 16 || block1 < 1 || block1 > c->nb || block2 < 1 || block2 > c->nb) return -1;
+    const blk_t *A = &c->blk[block1 - 1], *B = &c->blk[block2 - 1];
+    if (rng1[4] != A->KM || rng1[5] != A->KM || rng2[4] != 1 || rng2[5] != 1) return -1;
+    if (rng1[0] < 1 || rng1[1] > A->IM - 1 || rng1[1] > B->IM - 1 || rng1[0] > rng1[1]) return -1;
+    if (rng1[2] < 1 || rng1[3] > A->JM - 1 || rng2[2] < 1 || rng2[3] > B->JM - 1) return -1;
+    c->mix[c->nmix].blk[0] = block1 - 1;
+    c->mix[c->nmix].blk[1] = block2 - 1;
+    memcpy(c->mix[c->nmix].rng[0], rng1, 6 * sizeof(int));
+    memcpy(c->mix[c->nmix].rng[1], rng2, 6 * sizeof(int));
+    ++c->nmix;
+    return 0;
 }
 
 int orc_connection_add(orc_ctx *c, int kind, int orient, int block1, int face1, const int *rng1, int block2, int face2,
@@ -535,6 +554,12 @@ static int build_coarse(orc_ctx *c)
         coarse_range(T->rng[1], o2);
         if (orc_connection_add(m, T->kind, T->orient, T->blk[0] + 1, T->face[0], o1, T->blk[1] + 1, T->face[1], o2))
             return -1;
+    }
+    for (int q = 0; q < c->nmix; ++q) {
+        int o1[6], o2[6];
+        coarse_range(c->mix[q].rng[0], o1);
+        coarse_range(c->mix[q].rng[1], o2);
+        if (orc_mixing_plane_add(m, c->mix[q].blk[0] + 1, o1, c->mix[q].blk[1] + 1, o2)) return -1;
     }
     orc_geometry(m);
     c->coarse = m;
@@ -1023,6 +1048,110 @@ static void bc_connections(orc_ctx *c)
         }
 }
 
+/* CMesh::BndCondMixingPlane (source/BndCondMixingPlane.cpp:287-522): per
+ * radial line i, the area-weighted pitch-wise averages of the mass, momentum,
+ * energy (and rho nu~) fluxes through the K faces on each side, the mixed-out
+ * state that carries them (pressure from the quadratic of lines 370 / 432),
+ * then characteristic ghost states: block 1's K-high ghosts from the
+ * downstream average plus its outgoing waves, block 2's K-low ghosts from the
+ * upstream average plus its outgoing wave. */
+typedef struct { pv_t pv; double c, ev; } mixavg_t;
+
+static mixavg_t mix_average(const orc_ctx *c, const blk_t *b, int i, int js, int je, int kf, int kc)
+{
+    const double Gamma = c->nd.gamma, Rgas = c->nd.rgas;
+    double I[7] = {0, 0, 0, 0, 0, 0, 0}, s_sum = 0.0;
+    v3_t n_sum = {0.0, 0.0, 0.0};
+    for (int j = js; j <= je; ++j) {
+        const face_t *f = &b->fk[C3(b, i, j, kf)];
+        v3_t n = {f->S.x / f->s, f->S.y / f->s, f->S.z / f->s};
+        double ts = f->s;
+        n_sum.x = n_sum.x + n.x * ts;
+        n_sum.y = n_sum.y + n.y * ts;
+        n_sum.z = n_sum.z + n.z * ts;
+        s_sum = s_sum + ts;
+        const pv_t *p = &b->pv[C3(b, i, j, kc)];
+        const cv_t *q = &b->cv[C3(b, i, j, kc)];
+        double Vn = p->u * n.x + p->v * n.y + p->w * n.z;
+        I[1] = I[1] + q->r * Vn * ts;
+        I[2] = I[2] + (q->ru * Vn + n.x * p->p) * ts;
+        I[3] = I[3] + (q->rv * Vn + n.y * p->p) * ts;
+        I[4] = I[4] + (q->rw * Vn + n.z * p->p) * ts;
+        I[5] = I[5] + (q->re + p->p) * Vn * ts;
+        if (c->cfg.turbulence == 200) I[6] = I[6] + q->r * b->ev[C3(b, i, j, kc)] * Vn * ts;
+    }
+    mixavg_t a;
+    for (int m = 1; m <= 5; ++m) I[m] = I[m] / s_sum;
+    a.ev = 0.0;
+    if (c->cfg.turbulence == 200) {
+        I[6] = I[6] / s_sum;
+        a.ev = I[6] / I[1];
+    }
+    v3_t nu = {n_sum.x / s_sum, n_sum.y / s_sum, n_sum.z / s_sum};
+    double tA = I[2] * nu.x + I[3] * nu.y + I[4] * nu.z;
+    double tB = F32(2.0) * I[1] * I[5] - (I[2] * I[2] + I[3] * I[3] + I[4] * I[4]);
+    double tC = nu.x * nu.x + nu.y * nu.y + nu.z * nu.z;
+    a.pv.p = (tA / tC + sqrt(tA * tA / tC / tC - (Gamma * Gamma - F32(1.0)) * tB / tC)) / (Gamma + F32(1.0));
+    a.pv.u = (I[2] - a.pv.p * nu.x) / I[1];
+    a.pv.v = (I[3] - a.pv.p * nu.y) / I[1];
+    a.pv.w = (I[4] - a.pv.p * nu.z) / I[1];
+    a.pv.d = I[1] / (a.pv.u * nu.x + a.pv.v * nu.y + a.pv.w * nu.z);
+    a.pv.t = a.pv.p / Rgas / a.pv.d;
+    a.c = sqrt(Gamma * Rgas * a.pv.t);
+    return a;
+}
+
+static void mix_ghost(const orc_ctx *c, blk_t *b, int side, int i, int j, int kg, pv_t g, double ev)
+{
+    const double Gamma = c->nd.gamma, Rgas = c->nd.rgas;
+    g.t = g.p / Rgas / g.d;
+    double V2 = g.u * g.u + g.v * g.v + g.w * g.w;
+    cv_t q = {g.d, g.d * g.u, g.d * g.v, g.d * g.w, g.p / (Gamma - F32(1.0)) + g.d * V2 / F32(2.0)};
+    long x = C3(b, i, j, kg);
+    b->pv[x] = g;
+    b->cv[x] = q;
+    if (c->cfg.turbulence == 200) b->ev[x] = ev;
+    set_dummies(b, side, i, j, q, g);
+}
+
+static void bc_mixing(orc_ctx *c)
+{
+    for (int q = 0; q < c->nmix; ++q) {
+        blk_t *U = &c->blk[c->mix[q].blk[0]], *D = &c->blk[c->mix[q].blk[1]];
+        const int *r1 = c->mix[q].rng[0], *r2 = c->mix[q].rng[1];
+        const int KM = U->KM;
+        for (int i = r1[0]; i <= r1[1]; ++i) {
+            mixavg_t u = mix_average(c, U, i, r1[2], r1[3], KM, KM - 1);
+            mixavg_t d = mix_average(c, D, i, r2[2], r2[3], 1, 1);
+            for (int j = r1[2]; j <= r1[3]; ++j) {   /* lines 460-491 */
+                const pv_t *p = &U->pv[C3(U, i, j, KM - 1)];
+                double C1 = -u.c * u.c * (p->d - u.pv.d) + (p->p - u.pv.p);
+                double C2 = u.pv.d * u.c * (p->w - u.pv.w) + (p->p - u.pv.p);
+                double C5 = 0.0;
+                pv_t g;
+                g.d = d.pv.d + (-C1 + (C2 + C5) / F32(2.0)) / (d.c * d.c);
+                g.w = d.pv.w + (C2 - C5) / (F32(2.0) * d.pv.d * d.c);
+                g.v = p->v;
+                g.u = p->u;
+                g.p = d.pv.p + (C2 + C5) / F32(2.0);
+                mix_ghost(c, U, 5, i, j, KM, g, u.ev);
+            }
+            for (int j = r2[2]; j <= r2[3]; ++j) {   /* lines 494-522 */
+                const pv_t *p = &D->pv[C3(D, i, j, 1)];
+                double C1 = 0.0, C2 = 0.0, C3 = 0.0, C4 = 0.0;
+                double C5 = (F32(0.0) - d.pv.d * d.c * (p->w - d.pv.w) + (p->p - d.pv.p));
+                pv_t g;
+                g.d = u.pv.d + (-C1 + (C2 + C5) / F32(2.0)) / (u.c * u.c);
+                g.w = u.pv.w - C5 / (F32(2.0) * u.pv.d * u.c);
+                g.v = u.pv.v + C3 / (u.pv.d * u.c);
+                g.u = u.pv.u + C4 / (u.pv.d * u.c);
+                g.p = u.pv.p + (C2 + C5) / F32(2.0);
+                mix_ghost(c, D, 4, i, j, 0, g, u.ev);
+            }
+        }
+    }
+}
+
 int orc_bc(orc_ctx *c)
 {
     for (int pass = 0; pass < 3; ++pass)
@@ -1042,6 +1171,7 @@ int orc_bc(orc_ctx *c)
             }
         }
     bc_connections(c);
+    bc_mixing(c);
     return 0;
 }
 

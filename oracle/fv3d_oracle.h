@@ -78,6 +78,10 @@ int orc_connection_add(orc_ctx *c, int kind, int orient, int block1, int face1, 
  * grid levels as CUserInput::NumberOfMultigrid does (source/UserInput.cpp:
  * 799-1067, at most two) and builds the coarse level: nodes (2i-1, 2j-1,
  * 2k-1) (241-334), patch ranges halved (446-797). */
+/* Mixing plane (source/UserInput.cpp:420-432): block 1's K-high face (range
+ * rng1, cells is..ie, js..je, ks = ke = KM) against block 2's K-low face
+ * (rng2, ks = ke = 1); both sides share the i range of rng1. */
+int orc_mixing_plane_add(orc_ctx *c, int block1, const int *rng1, int block2, const int *rng2);
 int orc_geometry(orc_ctx *c);
 
 /* Grid levels in use (1 or 2) and the coarse level's context (NULL for one). */

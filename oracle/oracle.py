@@ -44,6 +44,7 @@ def lib():
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_block_add.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_double]
         L.orc_patch_add.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int]
+        L.orc_mixing_plane_add.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
         L.orc_connection_add.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
                                          C.c_void_p]
         L.orc_set_state.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
@@ -112,6 +113,12 @@ class Oracle:
             a = np.ascontiguousarray(xyz, dtype=np.float64)
             L.orc_block_add(self.h, bi.im, bi.jm, bi.km, a.ctypes.data, bi.nblade, bi.rpm)
         for p in case.patches:
+            if p.kind == "mixing":
+                r1 = np.ascontiguousarray(p.rng, dtype=np.int32)
+                r2 = np.ascontiguousarray(p.rng2, dtype=np.int32)
+                if L.orc_mixing_plane_add(self.h, p.block, r1.ctypes.data, p.block2, r2.ctypes.data) != 0:
+                    raise ValueError(f"oracle rejected {p}")
+                continue
             if p.kind in ("interior", "periodic"):
                 r1 = np.ascontiguousarray(p.rng, dtype=np.int32)
                 r2 = np.ascontiguousarray(p.rng2, dtype=np.int32)

@@ -116,6 +116,15 @@ def variant(name):
                                                                    len_ref=1e-3, max_iter=8, flux=200,
                                                                    recon_scheme=100, recon_var=100)))
         c.cfg.limiter, c.cfg.turbo_type = "Y", "C"
+    elif name == "mixing_plane_euler":
+        # the interface between the second and third blocks as a mixing plane
+        c = cases.annular_passage(ni=6, nj=8, nk_blocks=(5, 6, 4), max_iter=6, seed=74, rpm=5000.0)
+        c = cases.with_mixing_plane(c, 2, 3)
+    elif name == "mixing_plane_sa":
+        c = cases.with_sa(cases.with_wall_shear(cases.annular_passage(ni=8, nj=6, nk_blocks=(5, 8, 4), seed=75,
+                                                                      rpm=6000.0, blade_blocks=(1,), camber=0.1,
+                                                                      viscous=True, len_ref=1e-3, max_iter=8)))
+        c = cases.with_mixing_plane(c, 1, 2)
     else:
         raise KeyError(name)
     return c
@@ -125,7 +134,7 @@ def variant(name):
                                   "ns_sw_nnd_cons", "ns_rotating_roe_muscl2o", "bl_sw_muscl2u_kwall",
                                   "bl_rotating_roe_muscl3", "passage_turn1", "passage_turn2", "passage_turn3",
                                   "passage_bl_blade", "sa_bump", "sa_passage_turn1",
-                                  "sa_sw_nnd_limiter_centrifugal"])
+                                  "sa_sw_nnd_limiter_centrifugal", "mixing_plane_euler", "mixing_plane_sa"])
 def test_restatement_matches_reference(name):
     case = variant(name)
     n = case.cfg.max_iter
