@@ -208,6 +208,13 @@ def mg_variant(name):
         c.cfg.turbulence = 200
         c.cfg.coarse_iter, c.cfg.initial, c.cfg.outlet_angle = 3, "I", (0.0, 0.2, 1.0)
         c.cfg.timestep, c.cfg.turbo_type = "G", "A"
+    elif name == "bl_passage_mixing_plane":
+        c = cases.with_wall_shear(cases.annular_passage(ni=8, nj=8, nk_blocks=(6, 8, 6), seed=76, rpm=3000.0,
+                                                        blade_blocks=(1,), camber=0.1, viscous=True, len_ref=1e-3,
+                                                        max_iter=8))
+        c.cfg.turbulence = 100
+        c.cfg.coarse_iter, c.cfg.initial, c.cfg.outlet_angle = 2, "I", (0.0, 0.2, 1.0)
+        c = cases.with_mixing_plane(c, 2, 3)
     elif name == "not_coarsenable":
         # 7 cells along i: the patch ranges end on an even node, one level only
         c = cases.bump_channel(ni=7, nj=8, nk=16, seed=35, max_iter=8)
@@ -225,7 +232,7 @@ MG_COARSE = ["cv", "pv", "res", "dq", "dt", "lamcI", "lamcJ", "lamcK"]
 
 @pytest.mark.parametrize("name", ["bump_uniform", "bump_interpolate_sw_limiter", "bl_bump", "bl_passage",
                                   "bl_passage_turn1", "global_dt_axial_passage", "sa_passage_global_dt_axial",
-                                  "not_coarsenable"])
+                                  "not_coarsenable", "bl_passage_mixing_plane"])
 def test_two_grid_matches_reference(name):
     case = mg_variant(name)
     n = case.cfg.max_iter
