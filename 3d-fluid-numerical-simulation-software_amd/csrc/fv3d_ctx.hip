@@ -687,7 +687,6 @@ int fv3d_ctx_create(const fv3d_config *cfg, fv3d_ctx **out)
     *out = nullptr;
     fv3d_ctx *c = new fv3d_ctx();
     c->cfg = *cfg;
-    if (cfg->recon_var != 100 && cfg->recon_var != 200) { *out = c; return fail(c, FV3D_EINVAL, "recon_var %d unsupported", cfg->recon_var); }
     if (cfg->recon_scheme < 100 || cfg->recon_scheme > 500 || cfg->recon_scheme % 100) { *out = c; return fail(c, FV3D_EINVAL, "recon_scheme %d unsupported", cfg->recon_scheme); }
     if (cfg->flux_scheme != 200 && cfg->flux_scheme != 300) { *out = c; return fail(c, FV3D_EINVAL, "flux_scheme %d unsupported", cfg->flux_scheme); }
     if (cfg->turbulence != 0 && cfg->turbulence != 100 && cfg->turbulence != 200) { *out = c; return fail(c, FV3D_EINVAL, "turbulence model %d not in this build", cfg->turbulence); }
@@ -1758,7 +1757,8 @@ static int launch_update(fv3d_ctx *c)
                        c->d_partial, c->d_bad, sa_hi, sa ? 1 : 0);
     HIPCK(c, hipGetLastError());
     if (c->cfg.limiter) {
-        hipLaunchKernelGGL(k_limiter_serial, dim3(1), dim3(64), 0, c->stream, c->d_blocks, nb, c->gas, c->d_bad);
+        hipLaunchKernelGGL(k_limiter_serial, dim3(1), dim3(64), 0, c->stream, c->d_blocks, nb, c->gas, c->d_bad,
+                           c->d_cstart, c->d_partial, c->npart);
         if (sa) hipLaunchKernelGGL(k_sa_limit_serial, dim3(1), dim3(64), 0, c->stream, c->d_blocks, nb, c->d_sa_hi);
         HIPCK(c, hipGetLastError());
     }
@@ -1934,7 +1934,7 @@ static int launch_limiter(fv3d_ctx *c)
 {
     if (!c->cfg.limiter) return FV3D_OK;
     hipLaunchKernelGGL(k_limiter_serial, dim3(1), dim3(64), 0, c->stream, c->d_blocks, (int)c->blocks.size(), c->gas,
-                       c->d_bad);
+                       c->d_bad, c->d_cstart, (double *)nullptr, 0);
     HIPCK(c, hipGetLastError());
     return FV3D_OK;
 }

@@ -259,6 +259,56 @@ __device__ inline void recon1_cons(int scheme, double FF, double F, double B, do
     }
 }
 
+// Characteristic-variable reconstruction (source/Reconstruction.cpp:363-468):
+// the left (S) and right (S1) eigenvector matrices of the face state, the
+// mean of F's and B's primitive variables; the four conservative states go to
+// characteristic variables, are limited component by component as conservative
+// ones are, and come back through S1.  The -1/3, 2/3, 1/6 entries and SQRT(3.0f)
+// are float values in the reference.  qst holds FF, F, B, BB (conservative),
+// wF and wB the primitive F and B.  Positivity is checked by the caller.
+__device__ inline void recon_char(int scheme, double Gamma, double Rgas, const double wF[6], const double wB[6],
+                                  const double qst[4][5], double cvL[5], double cvR[5])
+{
+    const double uvel = F32C(0.5) * (wF[1] + wB[1]), vvel = F32C(0.5) * (wF[2] + wB[2]);
+    const double wvel = F32C(0.5) * (wF[3] + wB[3]), press = F32C(0.5) * (wF[4] + wB[4]);
+    const double dens = F32C(0.5) * (wF[0] + wB[0]);
+    const double V2 = F32C(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel);
+    const double temp = press / dens / Rgas;
+    const double C = sqrt(Gamma * Rgas * temp);
+    const double C2 = C * C;
+    const double sqrt3f = 0x1.bb67aep+0;   // sqrtf(3.0f)
+    const double Med0 = C2 / (Gamma - F32C(1.0));
+    const double Med1 = (Gamma - F32C(1.0)) / C * sqrt3f;
+    const double Med3 = F32C(1.0) / Med0;
+    const double Med4 = F32C(0.5) / C / sqrt3f;
+    const double V_bar = (uvel + vvel + wvel);
+    const double m3 = (double)(-1.0f / 3.0f), t23 = (double)(2.0f / 3.0f), s6 = (double)(1.0f / 6.0f);
+    const double S[5][5] = {
+        {V2 - Med0, -uvel, -vvel, -wvel, F32C(1.0)},
+        {uvel - vvel, F32C(-1.0), F32C(1.0), 0.0, 0.0},
+        {uvel - wvel, F32C(-1.0), 0.0, F32C(1.0), 0.0},
+        {-V_bar - V2 * Med1, F32C(1.0) + Med1 * uvel, F32C(1.0) + Med1 * vvel, F32C(1.0) + Med1 * wvel, -Med1},
+        {-V_bar + V2 * Med1, F32C(1.0) - Med1 * uvel, F32C(1.0) - Med1 * vvel, F32C(1.0) - Med1 * wvel, Med1}};
+    const double S1[5][5] = {
+        {-Med3, 0.0, 0.0, -Med4, Med4},
+        {-Med3 * uvel, m3, m3, s6 - Med4 * uvel, s6 + Med4 * uvel},
+        {-Med3 * vvel, t23, m3, s6 - Med4 * vvel, s6 + Med4 * vvel},
+        {-Med3 * wvel, m3, t23, s6 - Med4 * wvel, s6 + Med4 * wvel},
+        {-Med3 * V2, vvel - V_bar / F32C(3.0), wvel - V_bar / F32C(3.0), V_bar / F32C(6.0) - Med4 * (V2 + Med0),
+         V_bar / F32C(6.0) + Med4 * (V2 + Med0)}};
+    double ch[4][5];
+    for (int o = 0; o < 4; ++o)
+        for (int r = 0; r < 5; ++r)
+            ch[o][r] = S[r][0] * qst[o][0] + S[r][1] * qst[o][1] + S[r][2] * qst[o][2] + S[r][3] * qst[o][3] +
+                       S[r][4] * qst[o][4];
+    double chL[5], chR[5];
+    for (int r = 0; r < 5; ++r) recon1_cons(scheme, ch[0][r], ch[1][r], ch[2][r], ch[3][r], chL[r], chR[r]);
+    for (int r = 0; r < 5; ++r) {
+        cvL[r] = S1[r][0] * chL[0] + S1[r][1] * chL[1] + S1[r][2] * chL[2] + S1[r][3] * chL[3] + S1[r][4] * chL[4];
+        cvR[r] = S1[r][0] * chR[0] + S1[r][1] * chR[1] + S1[r][2] * chR[2] + S1[r][3] * chR[3] + S1[r][4] * chR[4];
+    }
+}
+
 // MIN3 of source/defs.cpp:401-408 keeps the first argument on ties and NaN;
 // fmin differs only for NaN operands, which the positivity fallback catches.
 

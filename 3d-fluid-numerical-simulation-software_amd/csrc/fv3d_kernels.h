@@ -137,7 +137,8 @@ constexpr int kSecParts = 32;    // workgroups per section in k_sections
 __global__ void k_fill_int(int *p, long n, int v);
 __global__ void k_update(const BlockDev *blocks, const long *cstart, int nblock, Gas g, double *partial, int *bad_flag,
                          int *sa_hi, int sa);
-__global__ void k_limiter_serial(const BlockDev *blocks, int nblock, Gas g, int *bad_flag);
+__global__ void k_limiter_serial(const BlockDev *blocks, int nblock, Gas g, int *bad_flag, const long *cstart,
+                                 double *partial, int npart);
 __global__ void k_sections(const BlockDev *blocks, const int2 *secs, Gas g, double *sums);
 // norms and section integrals of one level (lv) with the CFL ramp and the
 // iteration count kept in ramp (the fine level's scalars, shared by the

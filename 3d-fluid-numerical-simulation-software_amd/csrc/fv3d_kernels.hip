@@ -402,9 +402,12 @@ __device__ static void face_flux(const BlockDev &B, const Gas &g, int d, int i, 
             return;
         }
     }
-    // four-cell stencil along d; cells outside [1, N-1] come from the Dummy layers
-    const bool prim = g.recon_var == 200;
-    double st[4][5];
+    // four-cell stencil along d; cells outside [1, N-1] come from the Dummy
+    // layers.  Characteristic reconstruction (any variable code other than
+    // 100 and 200, source/UserInput.cpp:117-130) takes the conservative
+    // stencil and the primitive variables of F and B.
+    const bool prim = g.recon_var == 200, chr = g.recon_var != 100 && g.recon_var != 200;
+    double st[4][5], wFB[2][6];
     const long dstride = d == 0 ? B.si : d == 1 ? B.sj : 1;
     const double *src = prim ? B.W : B.Q;
     for (int o = 0; o < 4; ++o) {
@@ -412,17 +415,27 @@ __device__ static void face_flux(const BlockDev &B, const Gas &g, int d, int i, 
         if (pc >= 1 && pc <= N - 1) {
             long xc = xf + (long)(o - 2) * dstride;
             for (int c = 0; c < 5; ++c) st[o][c] = src[c * box + xc];
+            if (chr && (o == 1 || o == 2))
+                for (int c = 0; c < 6; ++c) wFB[o - 1][c] = B.W[c * box + xc];
         } else {
             int s = 2 * d + (pc >= N ? 1 : 0);
             int layer = pc < 1 ? 1 - pc : pc - (N - 1);
             int ta = d == 0 ? j : i, tb = d == 2 ? j : k;
             int c0 = prim ? 5 : 0;
             for (int c = 0; c < 5; ++c) st[o][c] = *dummy_ptr(B, s, layer, c0 + c, ta, tb);
+            if (chr && (o == 1 || o == 2))
+                for (int c = 0; c < 6; ++c) wFB[o - 1][c] = *dummy_ptr(B, s, layer, 5 + c, ta, tb);
         }
     }
     double L[5], R[5], cvL[5], cvR[5];
     const double lim_zero = F32C(1E-20);
-    if (prim) {
+    if (chr) {
+        recon_char(g.recon_scheme, Gamma, g.rgas, wFB[0], wFB[1], st, cvL, cvR);
+        double pL = (Gamma - F32C(1.0)) * (cvL[4] - F32C(0.5) * (cvL[1] * cvL[1] + cvL[2] * cvL[2] + cvL[3] * cvL[3]) / cvL[0]);
+        double pR = (Gamma - F32C(1.0)) * (cvR[4] - F32C(0.5) * (cvR[1] * cvR[1] + cvR[2] * cvR[2] + cvR[3] * cvR[3]) / cvR[0]);
+        if (cvL[0] <= lim_zero || pL <= lim_zero) for (int c = 0; c < 5; ++c) cvL[c] = st[1][c];
+        if (cvR[0] <= lim_zero || pR <= lim_zero) for (int c = 0; c < 5; ++c) cvR[c] = st[2][c];
+    } else if (prim) {
         for (int c = 0; c < 5; ++c) recon1_prim(g.recon_scheme, st[0][c], st[1][c], st[2][c], st[3][c], L[c], R[c]);
         if (L[0] <= lim_zero || L[4] <= lim_zero) for (int c = 0; c < 5; ++c) L[c] = st[1][c];
         if (R[0] <= lim_zero || R[4] <= lim_zero) for (int c = 0; c < 5; ++c) R[c] = st[2][c];
@@ -1803,7 +1816,15 @@ __device__ static bool bad_cell(const BlockDev &B, long x)
            B.W[3 * box + x] > vmax;
 }
 
-__global__ void k_limiter_serial(const BlockDev *__restrict__ blocks, int nblock, Gas g, int *__restrict__ bad_flag)
+// Res_Calculate runs after Limit_Vars in the reference
+// (source/LU_SGS_Marching.cpp:287-290, source/Solver.cpp:92-93), but k_update
+// formed the norm partials from the state before the repair.  With partial
+// given, each repaired cell's |(q_old - q)/q_old| Vol terms are moved from the
+// pre-repair value to the repaired one in its 256-cell group's partial
+// (q_old = the pre-repair value minus dQ).  Without it (the limiter after a
+// prolongation, which no norm follows) nothing is patched.
+__global__ void k_limiter_serial(const BlockDev *__restrict__ blocks, int nblock, Gas g, int *__restrict__ bad_flag,
+                                 const long *__restrict__ cstart, double *__restrict__ partial, int npart)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const double Gamma = g.gamma, Rgas = g.rgas;
@@ -1851,8 +1872,18 @@ __global__ void k_limiter_serial(const BlockDev *__restrict__ blocks, int nblock
                     }
                     B.W[x] = d; B.W[box + x] = u; B.W[2 * box + x] = v; B.W[3 * box + x] = w; B.W[4 * box + x] = p;
                     B.W[5 * box + x] = p / d / Rgas;
-                    B.Q[x] = d; B.Q[box + x] = d * u; B.Q[2 * box + x] = d * v; B.Q[3 * box + x] = d * w;
-                    B.Q[4 * box + x] = p / (Gamma - F32C(1.0)) + d * (u * u + v * v + w * w) / F32C(2.0);
+                    double qn[5];
+                    qn[0] = d; qn[1] = d * u; qn[2] = d * v; qn[3] = d * w;
+                    qn[4] = p / (Gamma - F32C(1.0)) + d * (u * u + v * v + w * w) / F32C(2.0);
+                    if (partial) {
+                        const long t = cstart[n] + ((long)(i - 1) * B.nj + (j - 1)) * B.nk + (k - 1);
+                        const double vol = B.vol[x];
+                        for (int c = 0; c < 5; ++c) {
+                            const double qp = B.Q[c * box + x], o = qp - B.dQ[c * box + x];
+                            partial[(long)c * npart + t / 256] += fabs((o - qn[c]) / o) * vol - fabs((o - qp) / o) * vol;
+                        }
+                    }
+                    for (int c = 0; c < 5; ++c) B.Q[c * box + x] = qn[c];
                 }
         if (nneg > 0) *B.flag_low_dt = 10;
     }

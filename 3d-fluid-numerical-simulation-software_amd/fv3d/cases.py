@@ -159,6 +159,28 @@ def with_sa(case: Case, amp: float = 0.5) -> Case:
     return case
 
 
+def with_cell_values(case: Case, block: int, index, rho=None, p=None, u=None, nut=None) -> Case:
+    """Overwrites restart values of block ``block`` (1-based) at ``index`` (a
+    numpy index into the (ni, nj, nk) cell arrays, 0-based cells): density,
+    pressure, x velocity and nu~ where given, with T and rhoE made consistent.
+    Used to start a run from states outside the limiter's admissible range
+    (source/Limit_FlowVars.cpp:16-20, 124) so that its repair branches run."""
+    c = nondim_constants(case.cfg)
+    g, r = c["gamma"], c["rgas"]
+    st = case.states[block - 1]
+    if rho is not None:
+        st["rho"][index] = rho
+    if p is not None:
+        st["p"][index] = p
+    if u is not None:
+        st["u"][index] = u
+    if nut is not None:
+        st["nut"][index] = nut
+    st["T"] = st["p"] / (st["rho"] * r)
+    st["rhoE"] = st["p"] / (g - 1.0) + st["rho"] * (st["u"] ** 2 + st["v"] ** 2 + st["w"] ** 2) / 2.0
+    return case
+
+
 def passage_batch(n: int = 64, seed: int = 3, min_cells: float = 1e5, max_cells: float = 2e6,
                   turbulence: int = 200, design_rpm: float = 16000.0, design_p_ratio: float = 0.76,
                   max_iter: int = 500) -> List[Case]:
@@ -225,7 +247,7 @@ def annular_passage(ni: int = 8, nj: int = 8, nk_blocks: Sequence[int] = (16,), 
                     beta: float = 1.5, cfl: float = 5.0, max_iter: int = 500, flux: int = 300,
                     recon_scheme: int = 500, recon_var: int = 200, p_ratio: float = 0.95,
                     viscous: bool = False, len_ref: float = 1.0, rotate_walls: bool = True,
-                    periodic_from_high: bool = False, stagger: float = 0.0) -> Case:
+                    periodic_from_high: bool = False, stagger: float = 0.0, profile: str = "arc") -> Case:
     """Annular-sector blade passage: i radial (hub I- to tip I+, tanh-clustered),
     j pitchwise over one blade pitch 2*pi/nblade, k axial.  The passage is cut
     along k into blocks of ``nk_blocks`` cells joined by interior patches
@@ -241,7 +263,12 @@ def annular_passage(ni: int = 8, nj: int = 8, nk_blocks: Sequence[int] = (16,), 
     a relative inflow at that angle.  ``periodic_from_high`` lists the
     periodic pair from J+ to J- (its second side then lies at the smaller
     angle, which makes the reference flip the block's pitch angle every time
-    it applies the condition, source/BndCondInteriorPeriodic.cpp:54-59)."""
+    it applies the condition, source/BndCondInteriorPeriodic.cpp:54-59).
+    ``profile`` "arc" makes the blade a circular arc through both ends (equal
+    and opposite edge angles); "turn" makes it a turning vane, the parabola
+    theta = camber * pitch * t^2, which meets an axial inflow at zero
+    incidence and turns it; the blocks downstream of it are shifted by the
+    vane's pitchwise offset so the interfaces stay conforming."""
     nb = len(nk_blocks)
     if not z_extent:
         z_extent = [float(b) for b in range(nb + 1)]
@@ -256,11 +283,15 @@ def annular_passage(ni: int = 8, nj: int = 8, nk_blocks: Sequence[int] = (16,), 
         im, jm, km = ni + 1, nj + 1, nk + 1
         z = np.linspace(z_extent[b], z_extent[b + 1], km)
         t = (z - z_extent[b]) / (z_extent[b + 1] - z_extent[b])
-        if b in blade_blocks and camber != 0.0:
+        if b in blade_blocks and camber != 0.0 and profile == "turn":
+            th0 = pitch * camber * t * t
+        elif b in blade_blocks and camber != 0.0:
             # circular arc through (0, 0) and (1, 0) with mid-height camber
             h = camber
             rad = (0.25 + h * h) / (2.0 * h)
             th0 = pitch * ((h - rad) + np.sqrt(np.maximum(rad * rad - (t - 0.5) ** 2, 0.0)))
+        elif profile == "turn" and any(q < b for q in blade_blocks):
+            th0 = np.full(km, pitch * camber)
         else:
             th0 = np.zeros(km)
         zmid = 0.5 * (z_extent[0] + z_extent[-1])

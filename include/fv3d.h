@@ -49,7 +49,7 @@ typedef struct {
     double cfl, c_timestep, relax;
     int turbo_type;     /* 0 axial, 1 centrifugal, 2 general outlet */
     int viscous;        /* 0 Euler, 1 Navier-Stokes */
-    int recon_var;      /* 100 conservative, 200 primitive */
+    int recon_var;      /* 100 conservative, 200 primitive, any other value characteristic (source/UserInput.cpp:117-130) */
     int recon_scheme;   /* 100 NND, 200 MUSCL2u, 300 MUSCL2c, 400 MUSCL2o, 500 MUSCL3 */
     int flux_scheme;    /* 200 Steger-Warming, 300 Roe */
     int turbulence;     /* 0 none (laminar), 100 Baldwin-Lomax (source/BL_Model.cpp),

@@ -1669,6 +1669,22 @@ int orc_timestep(orc_ctx *c)
     return 0;
 }
 
+/* Branch counters of the repair paths (reconstruction positivity fallback,
+ * Limit_Vars, Limit_SA), read and cleared by orc_stats; tests use them to show
+ * that a case really takes the branch it is meant to pin. */
+static long g_stats[ORC_NSTATS];
+
+static void stat_add(int which)
+{
+#pragma omp atomic
+    g_stats[which]++;
+}
+
+void orc_stats(long *out)
+{
+    for (int n = 0; n < ORC_NSTATS; ++n) { out[n] = g_stats[n]; g_stats[n] = 0; }
+}
+
 /* ------------------------------------------------------- reconstruction */
 /* source/Reconstruction.cpp:70-265 */
 static void recon_prim(int scheme, double Gamma, pv_t FF, pv_t F, pv_t B, pv_t BB, cv_t *cvL, cv_t *cvR)
@@ -1718,18 +1734,21 @@ static void recon_prim(int scheme, double Gamma, pv_t FF, pv_t F, pv_t B, pv_t B
         L = padd(F, pmul(pdiv(SL, 4.0), padd(pmul(ksubp(1.0, pdiv(SL, 3.0)), dF), pmul(paddk(pdiv(SL, 3.0), 1.0), dB))));
         R = psub(B, pmul(pdiv(SR, 4.0), padd(pmul(ksubp(1.0, pdiv(SR, 3.0)), dR), pmul(paddk(pdiv(SR, 3.0), 1.0), dB))));
     }
-    if (L.d <= lim_zero || L.p <= lim_zero) L = F;
-    if (R.d <= lim_zero || R.p <= lim_zero) R = B;
+    if (L.d <= lim_zero || L.p <= lim_zero) { L = F; stat_add(ORC_STAT_RECON_FALLBACK); }
+    if (R.d <= lim_zero || R.p <= lim_zero) { R = B; stat_add(ORC_STAT_RECON_FALLBACK); }
     cvL->r = L.d; cvL->ru = L.d * L.u; cvL->rv = L.d * L.v; cvL->rw = L.d * L.w;
     cvL->re = L.p / (Gamma - F32(1.0)) + L.d * (L.u * L.u + L.v * L.v + L.w * L.w) / F32(2.0);
     cvR->r = R.d; cvR->ru = R.d * R.u; cvR->rv = R.d * R.v; cvR->rw = R.d * R.w;
     cvR->re = R.p / (Gamma - F32(1.0)) + R.d * (R.u * R.u + R.v * R.v + R.w * R.w) / F32(2.0);
 }
 
-/* source/Reconstruction.cpp:7-68 */
-static void recon_cons(int scheme, double Gamma, cv_t FF, cv_t F, cv_t B, cv_t BB, cv_t *cvL, cv_t *cvR)
+/* The limited left/right states of one five-component variable set, before
+ * the positivity test: the scheme part of ConservationReconstruct
+ * (source/Reconstruction.cpp:15-59), which CharacterReconstruct repeats on
+ * characteristic variables (source/Reconstruction.cpp:407-459). */
+static void recon_cons_core(int scheme, cv_t FF, cv_t F, cv_t B, cv_t BB, cv_t *cvL, cv_t *cvR)
 {
-    const double eps = F32(1E-6), eta = F32(0.05), k2c = F32(1.0), k2u = F32(-1.0), lim_zero = F32(1E-20);
+    const double eps = F32(1E-6), eta = F32(0.05), k2c = F32(1.0), k2u = F32(-1.0);
     cv_t L = {0}, R = {0};
     if (scheme == 100) {
         L = cadd(F, cscale(cminmod(csub(B, F), csub(F, FF)), F32(0.5)));
@@ -1772,11 +1791,88 @@ static void recon_cons(int scheme, double Gamma, cv_t FF, cv_t F, cv_t B, cv_t B
         L = cadd(F, cmul(cdiv(SL, F32(4.0)), cadd(cmul(ksubc(F32(1.0), cdiv(SL, F32(3.0))), dF), cmul(caddk(cdiv(SL, F32(3.0)), F32(1.0)), dB))));
         R = csub(B, cmul(cdiv(SR, F32(4.0)), cadd(cmul(ksubc(F32(1.0), cdiv(SR, F32(3.0))), dR), cmul(caddk(cdiv(SR, F32(3.0)), F32(1.0)), dB))));
     }
-    double pL = (Gamma - F32(1.0)) * (L.re - F32(0.5) * (L.ru * L.ru + L.rv * L.rv + L.rw * L.rw) / L.r);
-    double pR = (Gamma - F32(1.0)) * (R.re - F32(0.5) * (R.ru * R.ru + R.rv * R.rv + R.rw * R.rw) / R.r);
-    if (L.r <= lim_zero || pL <= lim_zero) L = F;
-    if (R.r <= lim_zero || pR <= lim_zero) R = B;
     *cvL = L; *cvR = R;
+}
+
+/* First-order fallback of a conservative face state whose density or
+ * pressure is not positive (source/Reconstruction.cpp:61-67, 461-467). */
+static void cons_positivity(double Gamma, cv_t F, cv_t B, cv_t *L, cv_t *R)
+{
+    const double lim_zero = F32(1E-20);
+    double pL = (Gamma - F32(1.0)) * (L->re - F32(0.5) * (L->ru * L->ru + L->rv * L->rv + L->rw * L->rw) / L->r);
+    double pR = (Gamma - F32(1.0)) * (R->re - F32(0.5) * (R->ru * R->ru + R->rv * R->rv + R->rw * R->rw) / R->r);
+    if (L->r <= lim_zero || pL <= lim_zero) { *L = F; stat_add(ORC_STAT_RECON_FALLBACK); }
+    if (R->r <= lim_zero || pR <= lim_zero) { *R = B; stat_add(ORC_STAT_RECON_FALLBACK); }
+}
+
+/* source/Reconstruction.cpp:7-68 */
+static void recon_cons(int scheme, double Gamma, cv_t FF, cv_t F, cv_t B, cv_t BB, cv_t *cvL, cv_t *cvR)
+{
+    recon_cons_core(scheme, FF, F, B, BB, cvL, cvR);
+    cons_positivity(Gamma, F, B, cvL, cvR);
+}
+
+/* REAL[6][6] * TConsVars of source/defs.cpp:359-370, rows 1..5 summed left
+ * to right */
+static cv_t mat5(const double A[6][6], cv_t b)
+{
+    cv_t c;
+    c.r = A[1][1] * b.r + A[1][2] * b.ru + A[1][3] * b.rv + A[1][4] * b.rw + A[1][5] * b.re;
+    c.ru = A[2][1] * b.r + A[2][2] * b.ru + A[2][3] * b.rv + A[2][4] * b.rw + A[2][5] * b.re;
+    c.rv = A[3][1] * b.r + A[3][2] * b.ru + A[3][3] * b.rv + A[3][4] * b.rw + A[3][5] * b.re;
+    c.rw = A[4][1] * b.r + A[4][2] * b.ru + A[4][3] * b.rv + A[4][4] * b.rw + A[4][5] * b.re;
+    c.re = A[5][1] * b.r + A[5][2] * b.ru + A[5][3] * b.rv + A[5][4] * b.rw + A[5][5] * b.re;
+    return c;
+}
+
+/* CSpaceDiscr::CharacterReconstruct (source/Reconstruction.cpp:363-468,
+ * reconstruction variable code 300, and any code other than 100 and 200 per
+ * source/UserInput.cpp:117-130): the left and right eigenvector matrices S, S1
+ * of the face state averaged from the primitive variables of F and B, the
+ * four conservative states taken to characteristic variables by S, limited
+ * as ConservationReconstruct limits conservative ones, and back through S1.
+ * The -1/3, 2/3 and 1/6 entries are float quotients in the reference, and
+ * its SQRT(3.0f) is the float root. */
+static void recon_char(int scheme, double Gamma, double Rgas, pv_t wF, pv_t wB, cv_t FF, cv_t F, cv_t B, cv_t BB,
+                       cv_t *cvL, cv_t *cvR)
+{
+    double S[6][6] = {{0}}, S1[6][6] = {{0}};
+    double fd = F32(0.5) * (wF.d + wB.d), uvel = F32(0.5) * (wF.u + wB.u), vvel = F32(0.5) * (wF.v + wB.v);
+    double wvel = F32(0.5) * (wF.w + wB.w), press = F32(0.5) * (wF.p + wB.p);
+    double V2 = F32(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel);
+    double temp = press / fd / Rgas;
+    double C = sqrt(Gamma * Rgas * temp);
+    double C2 = C * C;
+    double Med0 = C2 / (Gamma - F32(1.0));
+    /* SQRT(3.0f) is C++'s float overload of sqrt: the float root of 3 */
+    const double sqrt3f = (double)sqrtf(3.0f);
+    double Med1 = (Gamma - F32(1.0)) / C * sqrt3f;
+    double Med3 = F32(1.0) / Med0;
+    double Med4 = F32(0.5) / C / sqrt3f;
+    double V_bar = (uvel + vvel + wvel);
+    const double m_third = (double)(-1.0f / 3.0f), two_thirds = (double)(2.0f / 3.0f);
+    const double sixth = (double)(1.0f / 6.0f);
+    S[1][1] = V2 - Med0; S[1][2] = -uvel; S[1][3] = -vvel; S[1][4] = -wvel; S[1][5] = F32(1.0);
+    S[2][1] = uvel - vvel; S[2][2] = F32(-1.0); S[2][3] = F32(1.0); S[2][4] = 0.0; S[2][5] = 0.0;
+    S[3][1] = uvel - wvel; S[3][2] = F32(-1.0); S[3][3] = 0.0; S[3][4] = F32(1.0); S[3][5] = 0.0;
+    S[4][1] = -V_bar - V2 * Med1; S[4][2] = F32(1.0) + Med1 * uvel; S[4][3] = F32(1.0) + Med1 * vvel;
+    S[4][4] = F32(1.0) + Med1 * wvel; S[4][5] = -Med1;
+    S[5][1] = -V_bar + V2 * Med1; S[5][2] = F32(1.0) - Med1 * uvel; S[5][3] = F32(1.0) - Med1 * vvel;
+    S[5][4] = F32(1.0) - Med1 * wvel; S[5][5] = Med1;
+    S1[1][1] = -Med3; S1[1][2] = 0.0; S1[1][3] = 0.0; S1[1][4] = -Med4; S1[1][5] = Med4;
+    S1[2][1] = -Med3 * uvel; S1[2][2] = m_third; S1[2][3] = m_third;
+    S1[2][4] = sixth - Med4 * uvel; S1[2][5] = sixth + Med4 * uvel;
+    S1[3][1] = -Med3 * vvel; S1[3][2] = two_thirds; S1[3][3] = m_third;
+    S1[3][4] = sixth - Med4 * vvel; S1[3][5] = sixth + Med4 * vvel;
+    S1[4][1] = -Med3 * wvel; S1[4][2] = m_third; S1[4][3] = two_thirds;
+    S1[4][4] = sixth - Med4 * wvel; S1[4][5] = sixth + Med4 * wvel;
+    S1[5][1] = -Med3 * V2; S1[5][2] = vvel - V_bar / F32(3.0); S1[5][3] = wvel - V_bar / F32(3.0);
+    S1[5][4] = V_bar / F32(6.0) - Med4 * (V2 + Med0); S1[5][5] = V_bar / F32(6.0) + Med4 * (V2 + Med0);
+    cv_t chL, chR;
+    recon_cons_core(scheme, mat5(S, FF), mat5(S, F), mat5(S, B), mat5(S, BB), &chL, &chR);
+    *cvL = mat5(S1, chL);
+    *cvR = mat5(S1, chR);
+    cons_positivity(Gamma, F, B, cvL, cvR);
 }
 
 /* --------------------------------------------------------- face fluxes */
@@ -1921,7 +2017,8 @@ static void face_flux(orc_ctx *c, blk_t *b, int dir, int i, int j, int k)
     stencil(b, dir, i, j, k, 0, &qB, &wB);
     stencil(b, dir, i, j, k, 1, &qBB, &wBB);
     if (c->cfg.recon_var == 100) recon_cons(c->cfg.recon_scheme, Gamma, qFF, qF, qB, qBB, &cvL, &cvR);
-    else recon_prim(c->cfg.recon_scheme, Gamma, wFF, wF, wB, wBB, &cvL, &cvR);
+    else if (c->cfg.recon_var == 200) recon_prim(c->cfg.recon_scheme, Gamma, wFF, wF, wB, wBB, &cvL, &cvR);
+    else recon_char(c->cfg.recon_scheme, Gamma, c->nd.rgas, wF, wB, qFF, qF, qB, qBB, &cvL, &cvR);
     const face_t *f;
     v3_t diag;
     v3_t *N = b->node;
@@ -2212,7 +2309,9 @@ static void limit_vars(orc_ctx *c, blk_t *b)
                                 }
                             }
                         }
+                stat_add(ORC_STAT_LIMIT_VARS);
                 if (kn == 0) {
+                    stat_add(ORC_STAT_LIMIT_VARS_KN0);
                     const pv_t *q = &b->pv[C3(b, i1, j1, k1)];
                     d = q->d; u = q->u; v = q->v; ww = q->w; p = q->p;
                     if (d < dmin) d = dmin;
@@ -2232,7 +2331,7 @@ static void limit_vars(orc_ctx *c, blk_t *b)
                 q->r = d; q->ru = d * u; q->rv = d * v; q->rw = d * ww;
                 q->re = p / (Gamma - F32(1.0)) + d * (u * u + v * v + ww * ww) / F32(2.0);
             }
-    if (nneg > 0) b->flag_low_dt = 10;
+    if (nneg > 0) { b->flag_low_dt = 10; stat_add(ORC_STAT_LOW_DT); }
 }
 
 /* CFluidProps::Limit_SA (source/Limit_FlowVars.cpp:122-168): nu~ below 1e-8
@@ -2259,6 +2358,8 @@ static void limit_sa(blk_t *b)
                                     if (v >= SAmin && v <= LSAmax) { s0 = s0 + v; kn = kn + 1; }
                                 }
                             }
+                    stat_add(ORC_STAT_LIMIT_SA);
+                    if (kn == 0) stat_add(ORC_STAT_LIMIT_SA_KN0);
                     *e = kn == 0 ? LSAmax : s0 / kn;
                 }
             }
@@ -2783,6 +2884,18 @@ int orc_test_recon(int var, int scheme, double gamma, const double *ff, const do
         cv_t a = {ff[0], ff[1], ff[2], ff[3], ff[4]}, c = {f[0], f[1], f[2], f[3], f[4]};
         cv_t d = {b[0], b[1], b[2], b[3], b[4]}, e = {bb[0], bb[1], bb[2], bb[3], bb[4]};
         recon_cons(scheme, gamma, a, c, d, e, &L, &R);
+    } else if (var == 300) {
+        /* primitive inputs (rho u v w p T); the conservative states are formed
+         * from them, and Rgas is the non-dimensional 1/gamma */
+        const double *in[4] = {ff, f, b, bb};
+        pv_t w[4];
+        cv_t q[4];
+        for (int n = 0; n < 4; ++n) {
+            w[n] = (pv_t){in[n][0], in[n][1], in[n][2], in[n][3], in[n][4], in[n][5]};
+            q[n] = (cv_t){w[n].d, w[n].d * w[n].u, w[n].d * w[n].v, w[n].d * w[n].w,
+                          w[n].p / (gamma - F32(1.0)) + w[n].d * (w[n].u * w[n].u + w[n].v * w[n].v + w[n].w * w[n].w) / F32(2.0)};
+        }
+        recon_char(scheme, gamma, 1.0 / gamma, w[1], w[2], q[0], q[1], q[2], q[3], &L, &R);
     } else {
         return -1;
     }

@@ -127,6 +127,21 @@ int orc_iterate(orc_ctx *c, int n, double *hist);
  * nu~, x y z). */
 long orc_get(orc_ctx *c, int block, const char *name, double *out);
 
+/* Counters of the repair branches since the last call (then cleared):
+ * [0] face states replaced by the first-order fallback (density or pressure
+ * not positive after reconstruction), [1] cells Limit_Vars replaced, [2] of
+ * those, cells with no admissible neighbour (the last-neighbour clamp),
+ * [3] nu~ values Limit_SA replaced (above 1000), [4] of those, with no
+ * admissible neighbour, [5] blocks whose Flag_low_dt Limit_Vars set. */
+#define ORC_NSTATS 6
+#define ORC_STAT_RECON_FALLBACK 0
+#define ORC_STAT_LIMIT_VARS 1
+#define ORC_STAT_LIMIT_VARS_KN0 2
+#define ORC_STAT_LIMIT_SA 3
+#define ORC_STAT_LIMIT_SA_KN0 4
+#define ORC_STAT_LOW_DT 5
+void orc_stats(long *out);
+
 /* Test hooks: the face flux (scheme 200 Steger-Warming, 300 Roe) on face-frame
  * states, and the reconstruction (var 100 conservative / 200 primitive) of one
  * face from its four-cell stencil FF, F | B, BB.  Also orc_get "SI" "SJ" "SK":

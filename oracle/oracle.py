@@ -68,6 +68,7 @@ def lib():
         L.orc_coarse.restype = C.c_void_p
         L.orc_initial.argtypes = [C.c_void_p]
         L.orc_set_nut.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_stats.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -221,6 +222,19 @@ class Oracle:
         if n != out.size:
             raise RuntimeError(f"orc_get({name}) returned {n}, expected {out.size}")
         return out
+
+
+STAT_NAMES = ("recon_fallback", "limit_vars", "limit_vars_kn0", "limit_sa", "limit_sa_kn0", "low_dt")
+
+
+def stats() -> Dict[str, int]:
+    """Repair-branch counters of the restatement since the last call (then
+    cleared): first-order reconstruction fallbacks, Limit_Vars replacements
+    (and those with no admissible neighbour), Limit_SA replacements (and those
+    with none), and blocks whose time-step damping Limit_Vars switched on."""
+    out = (C.c_long * len(STAT_NAMES))()
+    lib().orc_stats(out)
+    return dict(zip(STAT_NAMES, list(out)))
 
 
 def run_reference(case_dir: str, n_iter: int, out_dir: str, threads: int = 1, dump_first: bool = False,

@@ -192,3 +192,56 @@ def test_reconstruction_mirror_symmetry(var, scheme):
         l, r = O.point_recon(var, scheme, GAMMA, *st)
         lm, rm = O.point_recon(var, scheme, GAMMA, *st[::-1])
         assert np.array_equal(r, lm) and np.array_equal(l, rm)
+
+
+# Characteristic reconstruction (source/Reconstruction.cpp:363-468): S takes
+# conservative states to characteristic variables of the face state and S1
+# takes them back.  S1 S is the identity up to the reference's float
+# quotients (-1/3, 2/3, 1/6 in S1), so a mistyped entry of either matrix
+# breaks constant and linear data; the limiter is the conservative one.
+@pytest.mark.parametrize("scheme", [100, 200, 300, 400, 500])
+def test_characteristic_reconstruction_constant_data(scheme):
+    w = prim(1.1, 0.3, -0.2, 0.4, 0.8)
+    l, r = O.point_recon(300, scheme, GAMMA, w, w, w, w)
+    q = cons(*w[:5])
+    np.testing.assert_allclose(l, q, rtol=1e-7, atol=1e-8)
+    np.testing.assert_allclose(r, q, rtol=1e-7, atol=1e-8)
+
+
+@pytest.mark.parametrize("scheme", [100, 200, 300, 500])
+def test_characteristic_reconstruction_linear_data(scheme):
+    # conservative states linear in the stencil position stay linear in the
+    # characteristic variables of one frozen face state: the midpoint again
+    base = cons(1.0, 0.2, -0.1, 0.3, 0.9)
+    slope = np.array([0.05, 0.06, -0.02, 0.04, 0.1])
+    qs = [base + slope * t for t in (-1.0, 0.0, 1.0, 2.0)]
+    st = []
+    for q in qs:
+        rho, u, v, w = q[0], q[1] / q[0], q[2] / q[0], q[3] / q[0]
+        p = (GAMMA - 1.0) * (q[4] - 0.5 * rho * (u * u + v * v + w * w))
+        st.append(prim(rho, u, v, w, p))
+    l, r = O.point_recon(300, scheme, GAMMA, *st)
+    mid = 0.5 * (qs[1] + qs[2])
+    np.testing.assert_allclose(l, mid, rtol=1e-7)
+    np.testing.assert_allclose(r, mid, rtol=1e-7)
+
+
+@pytest.mark.parametrize("scheme", [100, 200, 300, 500])
+def test_characteristic_reconstruction_mirror_symmetry(scheme):
+    # the face state is the mean of F and B, the same for the mirrored stencil
+    rng = np.random.default_rng(29 + scheme)
+    for _ in range(20):
+        st = [prim(rng.uniform(0.8, 1.2), *rng.uniform(-0.5, 0.5, 3), rng.uniform(0.7, 1.1)) for _ in range(4)]
+        l, r = O.point_recon(300, scheme, GAMMA, *st)
+        lm, rm = O.point_recon(300, scheme, GAMMA, *st[::-1])
+        assert np.array_equal(r, lm) and np.array_equal(l, rm)
+
+
+def test_characteristic_differs_from_conservative():
+    # an entropy wave (density jump at constant p and u) next to a velocity
+    # kink: limiting characteristic fields separately gives other states
+    st = [prim(1.0, 0.3, 0.0, 0.0, 0.8), prim(1.0, 0.35, 0.0, 0.0, 0.8), prim(1.4, 0.2, 0.0, 0.0, 0.8),
+          prim(1.45, 0.3, 0.0, 0.0, 0.8)]
+    l3, _ = O.point_recon(300, 500, GAMMA, *st)
+    l1, _ = O.point_recon(100, 500, GAMMA, *[cons(*w[:5]) for w in st])
+    assert np.max(np.abs(l3 - l1)) > 1e-4

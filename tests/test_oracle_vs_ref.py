@@ -125,16 +125,71 @@ def variant(name):
                                                                       rpm=6000.0, blade_blocks=(1,), camber=0.1,
                                                                       viscous=True, len_ref=1e-3, max_iter=8)))
         c = cases.with_mixing_plane(c, 1, 2)
+    elif name == "limiter_low_pressure":
+        # two neighbouring cells at p = 1e-7: MUSCL3 reconstructs a negative
+        # pressure next to them (first-order fallback), Limit_Vars replaces
+        # them by the mean of their admissible neighbours, and the time-step
+        # damping (Flag_low_dt = 10, decremented per TimeStep) runs twice
+        c = cases.bump_channel(ni=6, nj=5, nk=10, seed=41, max_iter=12)
+        c.cfg.limiter = "Y"
+        c = cases.with_cell_values(c, 1, (2, 2, slice(4, 6)), p=1e-7)
+    elif name == "limiter_dense_corner":
+        # a 2x2x2 block corner at rho = 1e4: the corner cell has no
+        # admissible neighbour (the last-neighbour clamp of kn == 0)
+        c = cases.bump_channel(ni=6, nj=5, nk=10, seed=41, max_iter=12)
+        c.cfg.limiter = "Y"
+        c = cases.with_cell_values(c, 1, (slice(0, 2), slice(0, 2), slice(0, 2)), rho=1e4)
+    elif name == "limiter_sa_high":
+        # nu~ above 1000: Limit_SA's neighbour mean, and at the block's first
+        # cell (all neighbours above 1000) its 1000 clamp
+        c = cases.with_sa(cases.with_wall_shear(cases.bump_channel(ni=7, nj=6, nk=10, seed=71, viscous=True,
+                                                                   len_ref=1e-3, max_iter=8)))
+        c.cfg.limiter = "Y"
+        c = cases.with_cell_values(c, 1, (slice(0, 2), slice(0, 2), slice(0, 2)), nut=1e5)
+        c = cases.with_cell_values(c, 1, (0, 0, 0), nut=3000.0)
+        c = cases.with_cell_values(c, 1, (4, 3, 6), nut=1e5)
+    elif name == "char_muscl3_roe":
+        c = _shifted(cases.bump_channel(ni=6, nj=5, nk=10, seed=42, recon_var=300, max_iter=12))
+    elif name == "char_nnd_sw_ns":
+        c = cases.with_wall_shear(cases.bump_channel(ni=7, nj=6, nk=9, seed=43, flux=200, recon_scheme=100,
+                                                     recon_var=300, max_iter=12, viscous=True, len_ref=1e-4))
+    elif name == "char_muscl2u_low_pressure":
+        # characteristic MUSCL2u next to a low-pressure pair: the conservative
+        # positivity fallback of CharacterReconstruct, then Limit_Vars
+        c = cases.bump_channel(ni=6, nj=5, nk=10, seed=44, recon_scheme=200, recon_var=300, max_iter=12)
+        c.cfg.limiter = "Y"
+        c = cases.with_cell_values(c, 1, (2, 2, slice(4, 6)), p=1e-7)
+    elif name == "char_muscl2o_rotating_passage":
+        c = cases.annular_passage(ni=6, nj=8, nk_blocks=(5, 6, 4), max_iter=6, seed=45, rpm=5000.0)
+        c.cfg.recon_scheme, c.cfg.recon_var = 400, 300
+        cases.turn_block(c, 2, 1)
+    elif name == "char_muscl2c_bl":
+        c = cases.with_wall_shear(cases.annular_passage(ni=7, nj=6, nk_blocks=(4, 7, 5), max_iter=6, seed=46,
+                                                        rpm=8000.0, blade_blocks=(1,), camber=0.15, viscous=True,
+                                                        len_ref=1e-3))
+        c.cfg.turbulence, c.cfg.recon_scheme, c.cfg.recon_var, c.cfg.flux_scheme = 100, 300, 300, 200
     else:
         raise KeyError(name)
     return c
+
+
+# repair branches each case must take in the restatement (oracle counters)
+MUST_FIRE = {
+    "limiter_low_pressure": ("recon_fallback", "limit_vars", "low_dt"),
+    "limiter_dense_corner": ("recon_fallback", "limit_vars", "limit_vars_kn0", "low_dt"),
+    "limiter_sa_high": ("limit_sa", "limit_sa_kn0"),
+    "char_muscl2u_low_pressure": ("recon_fallback", "limit_vars", "low_dt"),
+}
 
 
 @pytest.mark.parametrize("name", ["muscl2c_axial", "nnd_cons_sw_centrifugal", "rotating_roe_muscl3",
                                   "ns_sw_nnd_cons", "ns_rotating_roe_muscl2o", "bl_sw_muscl2u_kwall",
                                   "bl_rotating_roe_muscl3", "passage_turn1", "passage_turn2", "passage_turn3",
                                   "passage_bl_blade", "sa_bump", "sa_passage_turn1",
-                                  "sa_sw_nnd_limiter_centrifugal", "mixing_plane_euler", "mixing_plane_sa"])
+                                  "sa_sw_nnd_limiter_centrifugal", "mixing_plane_euler", "mixing_plane_sa",
+                                  "limiter_low_pressure", "limiter_dense_corner", "limiter_sa_high",
+                                  "char_muscl3_roe", "char_nnd_sw_ns", "char_muscl2u_low_pressure",
+                                  "char_muscl2o_rotating_passage", "char_muscl2c_bl"])
 def test_restatement_matches_reference(name):
     case = variant(name)
     n = case.cfg.max_iter
@@ -153,7 +208,11 @@ def test_restatement_matches_reference(name):
                 got = o.get(b, f).reshape(want.shape)
                 assert np.array_equal(got, want), f"{name} block {b}: {f}, max diff {np.max(np.abs(got - want))}"
         o2 = O.Oracle(case)
+        O.stats()
         h = o2.iterate(n)
+        fired = O.stats()
+        for branch in MUST_FIRE.get(name, ()):
+            assert fired[branch] > 0, f"{name}: the {branch} branch never ran ({fired})"
         # (the rotating case reverses the outlet mass flow, which makes the
         # mass-averaged pressure ratio NaN on both sides)
         assert np.array_equal(h[:, :7], ref["history"][:, :7], equal_nan=True), name
