@@ -103,6 +103,11 @@ struct fv3d_ctx {
     int4 *d_tiles_s[kNumTiles] = {};
     int ntiles_s[kNumTiles] = {};
     int tile_sel[3] = {0, 0, 0};
+    // streaming convective residual: column tiles per configuration (built on
+    // first use); tile_sel[0] >= kNumTiles selects configuration
+    // tile_sel[0] - kNumTiles of it (face capture always runs the per-tile kernel)
+    int4 *d_stiles[kNumStream] = {};
+    int nstiles[kNumStream] = {};
     int2 *d_secs = nullptr;
     int npatch_dev = 0, nplanes = 0;
     long nbc = 0, ncell = 0, nplane_threads = 0;
@@ -331,10 +336,41 @@ static int ensure_tiles(fv3d_ctx *c, int q)
             for (int j0 = 0; j0 < B.nj; j0 += TJ)
                 for (int k0 = 0; k0 < B.nk; k0 += TK) tiles.push_back(make_int4(n, i0, j0, k0));
     }
-    int rc = dalloc(c, &c->d_tiles_s[q], tiles.size());
+    // XCD order: the hardware hands workgroup w to XCD w mod 8, so entry w is
+    // tile (w mod 8) * nper + w / 8 and each XCD works through one contiguous
+    // eighth of the list (neighbouring tiles share their halos through that
+    // XCD's L2); entries past the list have block -1 and do nothing
+    const size_t n = tiles.size(), nper = (n + 7) / 8;
+    std::vector<int4> xt(8 * nper, make_int4(-1, 0, 0, 0));
+    for (size_t w = 0; w < xt.size(); ++w) {
+        const size_t idx = (w % 8) * nper + w / 8;
+        if (idx < n) xt[w] = tiles[idx];
+    }
+    int rc = dalloc(c, &c->d_tiles_s[q], xt.size());
     if (rc) return rc;
-    HIPCK(c, hipMemcpyAsync(c->d_tiles_s[q], tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
-    c->ntiles_s[q] = (int)tiles.size();
+    HIPCK(c, hipMemcpyAsync(c->d_tiles_s[q], xt.data(), xt.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    c->ntiles_s[q] = (int)xt.size();
+    return FV3D_OK;
+}
+
+// The column tiles of streaming configuration q (block, first i of the chunk,
+// first j, first k), in the order the XCD mapping of k_conv_stream assumes.
+static int ensure_stream_tiles(fv3d_ctx *c, int q)
+{
+    if (c->d_stiles[q]) return FV3D_OK;
+    const int TJ = kStreamShape[q][0], TK = kStreamShape[q][1], CI = kStreamShape[q][2];
+    std::vector<int4> tiles;
+    for (int n = 0; n < (int)c->blocks.size(); ++n) {
+        const BlockDev &B = c->blocks[n].dev;
+        for (int i0 = 0; i0 < B.ni; i0 += CI)
+            for (int j0 = 0; j0 < B.nj; j0 += TJ)
+                for (int k0 = 0; k0 < B.nk; k0 += TK) tiles.push_back(make_int4(n, i0, j0, k0));
+    }
+    int rc = dalloc(c, &c->d_stiles[q], tiles.size());
+    if (rc) return rc;
+    HIPCK(c, hipMemcpyAsync(c->d_stiles[q], tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice,
+                            c->stream));
+    c->nstiles[q] = (int)tiles.size();
     return FV3D_OK;
 }
 
@@ -738,6 +774,8 @@ int fv3d_ctx_destroy(fv3d_ctx *c)
         for (void *p : b.allocs) (void)hipFree(p);
     for (int q = 0; q < kNumTiles; ++q)
         if (c->d_tiles_s[q]) (void)hipFree(c->d_tiles_s[q]);
+    for (int q = 0; q < kNumStream; ++q)
+        if (c->d_stiles[q]) (void)hipFree(c->d_stiles[q]);
     void *ps[] = {c->d_blocks, c->d_patches, c->d_pstart_bc, c->d_cstart, c->d_pstart_plane, c->d_secs,
                   c->d_partial, c->d_sums, c->d_bad, c->d_flags, c->d_sc, c->d_hist, c->d_stage_row, c->d_kba[0], c->d_kba[1], c->d_kba_sync, c->d_faces[0], c->d_faces[1], c->d_faces[2]};
     for (void *p : ps)
@@ -1498,7 +1536,34 @@ static int launch_spectral_dt(fv3d_ctx *c)
 
 static int launch_convective(fv3d_ctx *c)
 {
-    const int q = c->tile_sel[0];
+    if (!c->store_faces && c->tile_sel[0] >= kNumTiles) {
+        const int q = c->tile_sel[0] - kNumTiles;
+        int rc = ensure_stream_tiles(c, q);
+        if (rc) return rc;
+        const int nt = c->nstiles[q], ci = kStreamShape[q][2];
+        const int mode = c->cfg.recon_var == 200 ? 0 : c->cfg.recon_var == 100 ? 1 : 2;
+        const dim3 g((unsigned)(8 * ((nt + 7) / 8)));
+        switch (q < 2 || q == 4 ? 0 : 1) {
+#define FV3D_STREAM(S, TJ, TK)                                                                                     \
+    case S:                                                                                                         \
+        if (mode == 0)                                                                                              \
+            hipLaunchKernelGGL((k_conv_stream<TJ, TK, 0>), g, dim3(TJ * TK), 0, c->stream, c->d_blocks,             \
+                               c->d_stiles[q], nt, ci, c->gas);                                                     \
+        else if (mode == 1)                                                                                         \
+            hipLaunchKernelGGL((k_conv_stream<TJ, TK, 1>), g, dim3(TJ * TK), 0, c->stream, c->d_blocks,             \
+                               c->d_stiles[q], nt, ci, c->gas);                                                     \
+        else                                                                                                        \
+            hipLaunchKernelGGL((k_conv_stream<TJ, TK, 2>), g, dim3(TJ * TK), 0, c->stream, c->d_blocks,             \
+                               c->d_stiles[q], nt, ci, c->gas);                                                     \
+        break;
+            FV3D_FOR_STREAM_TILES(FV3D_STREAM)
+#undef FV3D_STREAM
+        }
+        HIPCK(c, hipGetLastError());
+        return FV3D_OK;
+    }
+    const int q = c->tile_sel[0] < kNumTiles ? c->tile_sel[0] : 0;
+    if (!c->d_tiles_s[q]) return fail(c, FV3D_ESTATE, "convective tiles of shape %d not built", q);
     const dim3 g((unsigned)c->ntiles_s[q]), b(256);
     switch (q) {
 #define FV3D_CONV(S, A, B, C)                                                                                    \
@@ -2169,8 +2234,10 @@ int fv3d_set_tiles(fv3d_ctx *c, const int *shape3)
     READY(c);
     if (!shape3) return FV3D_EINVAL;
     for (int k = 0; k < 3; ++k) {
-        if (shape3[k] < 0 || shape3[k] >= kNumTiles) return fail(c, FV3D_EINVAL, "tile shape %d", shape3[k]);
-        int rc = ensure_tiles(c, shape3[k]);
+        if (shape3[k] < 0 || shape3[k] >= (k == 0 ? kNumConv : kNumTiles))
+            return fail(c, FV3D_EINVAL, "tile shape %d", shape3[k]);
+        int rc = ensure_tiles(c, shape3[k] < kNumTiles ? shape3[k] : 0);
+        if (!rc && k == 0 && shape3[k] >= kNumTiles) rc = ensure_stream_tiles(c, shape3[k] - kNumTiles);
         if (rc) return rc;
     }
     for (int k = 0; k < 3; ++k) c->tile_sel[k] = shape3[k];
@@ -2199,8 +2266,10 @@ int fv3d_autotune(fv3d_ctx *c, int reps, int *shape3)
         };
         float best = 0.0f;
         int bestq = 0;
-        for (int q = 0; q < kNumTiles && !rc; ++q) {
-            if ((rc = ensure_tiles(c, q))) break;
+        for (int q = 0; q < (kern == 0 ? kNumConv : kNumTiles) && !rc; ++q) {
+            if (q >= kNumTiles) {
+                if ((rc = ensure_stream_tiles(c, q - kNumTiles))) break;
+            } else if ((rc = ensure_tiles(c, q))) break;
             c->tile_sel[kern] = q;
             if ((rc = run())) break;   // warm-up
             HIPCK(c, hipEventRecord(c->ev[0], c->stream));

@@ -376,4 +376,98 @@ __device__ inline void sw_half(double Gamma, const double Q[5], double que, doub
     f[4] = pre * ((Gamma - F32C(1.0)) * l1 * (u * u + v * v + w * w) + F32C(0.5) * l4 * A + F32C(0.5) * l5 * Bq + W2);
 }
 
+// ------------------------------------------------- face flux from a stencil
+// The interior face of CSpaceDiscr::Upwind_Convective
+// (source/Upwind_Convective.cpp:5-360) from its four-cell stencil FF, F, B, BB
+// (st: primitive rho u v w p for recon_var 200, conservative otherwise) and, for
+// characteristic reconstruction, the primitive F and B (wF, wB); then the
+// face frame of Coord_Rotate (source/Flux_Split_Scheme.cpp:728-776) from the
+// face vector S, its area s, the node diagonal r and the grid velocity of the
+// face centroid (fcx, fcy) times w_rot, the Roe or Steger-Warming flux, and
+// the flux rotated back and times the area.
+__device__ inline void face_core(const Gas &g, const double st[4][5], const double wF[6], const double wB[6],
+                                 double Sx, double Sy, double Sz, double s_, double fcx, double fcy, double w_rot,
+                                 double rx, double ry, double rz, double out[5])
+{
+    const double Gamma = g.gamma;
+    const double lim_zero = F32C(1E-20);
+    double L[5], R[5], cvL[5], cvR[5];
+    if (g.recon_var != 100 && g.recon_var != 200) {
+        recon_char(g.recon_scheme, Gamma, g.rgas, wF, wB, st, cvL, cvR);
+        double pL = (Gamma - F32C(1.0)) * (cvL[4] - F32C(0.5) * (cvL[1] * cvL[1] + cvL[2] * cvL[2] + cvL[3] * cvL[3]) / cvL[0]);
+        double pR = (Gamma - F32C(1.0)) * (cvR[4] - F32C(0.5) * (cvR[1] * cvR[1] + cvR[2] * cvR[2] + cvR[3] * cvR[3]) / cvR[0]);
+        if (cvL[0] <= lim_zero || pL <= lim_zero) for (int c = 0; c < 5; ++c) cvL[c] = st[1][c];
+        if (cvR[0] <= lim_zero || pR <= lim_zero) for (int c = 0; c < 5; ++c) cvR[c] = st[2][c];
+    } else if (g.recon_var == 200) {
+        for (int c = 0; c < 5; ++c) recon1_prim(g.recon_scheme, st[0][c], st[1][c], st[2][c], st[3][c], L[c], R[c]);
+        if (L[0] <= lim_zero || L[4] <= lim_zero) for (int c = 0; c < 5; ++c) L[c] = st[1][c];
+        if (R[0] <= lim_zero || R[4] <= lim_zero) for (int c = 0; c < 5; ++c) R[c] = st[2][c];
+        cvL[0] = L[0]; cvL[1] = L[0] * L[1]; cvL[2] = L[0] * L[2]; cvL[3] = L[0] * L[3];
+        cvL[4] = L[4] / (Gamma - F32C(1.0)) + L[0] * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]) / F32C(2.0);
+        cvR[0] = R[0]; cvR[1] = R[0] * R[1]; cvR[2] = R[0] * R[2]; cvR[3] = R[0] * R[3];
+        cvR[4] = R[4] / (Gamma - F32C(1.0)) + R[0] * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3]) / F32C(2.0);
+    } else {
+        for (int c = 0; c < 5; ++c) recon1_cons(g.recon_scheme, st[0][c], st[1][c], st[2][c], st[3][c], cvL[c], cvR[c]);
+        double pL = (Gamma - F32C(1.0)) * (cvL[4] - F32C(0.5) * (cvL[1] * cvL[1] + cvL[2] * cvL[2] + cvL[3] * cvL[3]) / cvL[0]);
+        double pR = (Gamma - F32C(1.0)) * (cvR[4] - F32C(0.5) * (cvR[1] * cvR[1] + cvR[2] * cvR[2] + cvR[3] * cvR[3]) / cvR[0]);
+        if (cvL[0] <= lim_zero || pL <= lim_zero) for (int c = 0; c < 5; ++c) cvL[c] = st[1][c];
+        if (cvR[0] <= lim_zero || pR <= lim_zero) for (int c = 0; c < 5; ++c) cvR[c] = st[2][c];
+    }
+    double nx = Sx / s_, ny = Sy / s_, nz = Sz / s_;
+    double rl = sqrt(rx * rx + ry * ry + rz * rz);
+    double t1x = rx / rl, t1y = ry / rl, t1z = rz / rl;
+    double qx = ny * t1z - nz * t1y, qy = nz * t1x - nx * t1z, qz = nx * t1y - ny * t1x;
+    double ql = sqrt(qx * qx + qy * qy + qz * qz);
+    double t2x = qx / ql, t2y = qy / ql, t2z = qz / ql;
+    double fue = F32C(-1.0) * fcy * w_rot, fve = fcx * w_rot;
+    double que = fue * nx + fve * ny + F32C(0.0) * nz;
+    double QL[5], QR[5];
+    QL[0] = cvL[0]; QL[4] = cvL[4];
+    QL[1] = cvL[1] * nx + cvL[2] * ny + cvL[3] * nz;
+    QL[2] = cvL[1] * t1x + cvL[2] * t1y + cvL[3] * t1z;
+    QL[3] = cvL[1] * t2x + cvL[2] * t2y + cvL[3] * t2z;
+    QR[0] = cvR[0]; QR[4] = cvR[4];
+    QR[1] = cvR[1] * nx + cvR[2] * ny + cvR[3] * nz;
+    QR[2] = cvR[1] * t1x + cvR[2] * t1y + cvR[3] * t1z;
+    QR[3] = cvR[1] * t2x + cvR[2] * t2y + cvR[3] * t2z;
+    double Fc[5];
+    if (g.flux_scheme == 200) {
+        double fp_[5], fm_[5];
+        sw_half(Gamma, QL, que, 1.0, fp_);
+        sw_half(Gamma, QR, que, -1.0, fm_);
+        for (int c = 0; c < 5; ++c) Fc[c] = fp_[c] + fm_[c];
+    } else {
+        roe_flux(Gamma, QL, QR, que, Fc);
+    }
+    out[0] = Fc[0] * s_;
+    out[1] = (Fc[1] * nx + Fc[2] * t1x + Fc[3] * t2x) * s_;
+    out[2] = (Fc[1] * ny + Fc[2] * t1y + Fc[3] * t2y) * s_;
+    out[3] = (Fc[1] * nz + Fc[2] * t1z + Fc[3] * t2z) * s_;
+    out[4] = Fc[4] * s_;
+}
+
+// The WallFlux override of a wall face (source/WallFlux.cpp:70-175): the flux
+// of the state averaged from the primitive variables of the two cells either
+// side (a = the cell the face's index names, b = the one below it), with the
+// wall's own rotation rate rot, on face vector S and face centroid (fcx, fcy).
+__device__ inline void wall_face_core(double Gamma, const double wa[5], const double wb[5], double Sx, double Sy,
+                                      double Sz, double fcx, double fcy, double rot, double out[5])
+{
+    double fd = F32C(0.5) * (wa[0] + wb[0]);
+    double fu = F32C(0.5) * (wa[1] + wb[1]);
+    double fv = F32C(0.5) * (wa[2] + wb[2]);
+    double fw = F32C(0.5) * (wa[3] + wb[3]);
+    double fp = F32C(0.5) * (wa[4] + wb[4]);
+    double fue = F32C(-1.0) * fcy * rot, fve = fcx * rot;
+    double c1 = fd * fu, c2 = fd * fv, c3 = fd * fw;
+    double Vn_a = fu * Sx + fv * Sy + fw * Sz;
+    double Vn_e = fue * Sx + fve * Sy + F32C(0.0) * Sz;
+    double c4 = fp / (Gamma - F32C(1.0)) + fd * (fu * fu + fv * fv + fw * fw) / F32C(2.0);
+    out[0] = fd * (Vn_a - Vn_e);
+    out[1] = c1 * (Vn_a - Vn_e) + fp * Sx;
+    out[2] = c2 * (Vn_a - Vn_e) + fp * Sy;
+    out[3] = c3 * (Vn_a - Vn_e) + fp * Sz;
+    out[4] = (c4 + fp) * (Vn_a - Vn_e) + fp * Vn_e;
+}
+
 }  // namespace fv3d

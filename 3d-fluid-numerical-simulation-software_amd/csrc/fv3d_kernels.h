@@ -15,6 +15,16 @@ constexpr int kNumTiles = 5;
 constexpr int kTileShape[kNumTiles][3] = {{4, 4, 16}, {2, 8, 16}, {4, 8, 8}, {8, 4, 8}, {8, 8, 4}};
 #define FV3D_FOR_TILES(X) X(0, 4, 4, 16) X(1, 2, 8, 16) X(2, 4, 8, 8) X(3, 8, 4, 8) X(4, 8, 8, 4)
 
+// streaming convective residual (k_conv_stream): column tiles TJ x TK marching
+// along i in chunks of ci cells.  The convective kernel's configuration number
+// q counts the per-tile shapes first (q < kNumTiles, k_convective), then these
+// (q - kNumTiles, k_conv_stream); the autotuner times all of them, and results
+// do not depend on the choice.
+constexpr int kNumStream = 5;
+constexpr int kStreamShape[kNumStream][3] = {{16, 16, 4}, {16, 16, 2}, {8, 32, 4}, {8, 32, 2}, {16, 16, 8}};
+constexpr int kNumConv = kNumTiles + kNumStream;
+#define FV3D_FOR_STREAM_TILES(X) X(0, 16, 16) X(1, 8, 32)
+
 struct PatchDev {
     int kind;          // 0 inlet, 1 outlet, 2 wall
     int block;         // 0-based block index
@@ -137,6 +147,8 @@ constexpr int kSecParts = 32;    // workgroups per section in k_sections
 __global__ void k_fill_int(int *p, long n, int v);
 __global__ void k_update(const BlockDev *blocks, const long *cstart, int nblock, Gas g, double *partial, int *bad_flag,
                          int *sa_hi, int sa);
+template <int TJ, int TK, int MODE>
+__global__ void k_conv_stream(const BlockDev *blocks, const int4 *tiles, int ntiles, int ci, Gas g);
 __global__ void k_limiter_serial(const BlockDev *blocks, int nblock, Gas g, int *bad_flag, const long *cstart,
                                  double *partial, int npart);
 __global__ void k_sections(const BlockDev *blocks, const int2 *secs, Gas g, double *sums);

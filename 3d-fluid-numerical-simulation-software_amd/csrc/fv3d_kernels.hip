@@ -372,7 +372,6 @@ __global__ void k_dt_flag_decrement(const BlockDev *__restrict__ blocks, int nbl
 __device__ static void face_flux(const BlockDev &B, const Gas &g, int d, int i, int j, int k, double out[5])
 {
     const long box = B.box;
-    const double Gamma = g.gamma;
     const int N = d == 0 ? B.IM : d == 1 ? B.JM : B.KM;
     const int p = d == 0 ? i : d == 1 ? j : k;
     const long xf = cidx(B, i, j, k);
@@ -381,24 +380,11 @@ __device__ static void face_flux(const BlockDev &B, const Gas &g, int d, int i, 
         const int ta = d == 0 ? j : i, tb = d == 2 ? j : k;
         const long tix = (long)ta * B.Db[s] + tb;
         if (B.wmask[s] != nullptr && B.wmask[s][tix]) {
-            const double rot = B.wrot[s][tix];
             const long xb = xf - (d == 0 ? B.si : d == 1 ? B.sj : 1);
-            double Sx = B.fS[d][xf], Sy = B.fS[d][box + xf], Sz = B.fS[d][2 * box + xf];
-            double fd = F32C(0.5) * (B.W[xf] + B.W[xb]);
-            double fu = F32C(0.5) * (B.W[box + xf] + B.W[box + xb]);
-            double fv = F32C(0.5) * (B.W[2 * box + xf] + B.W[2 * box + xb]);
-            double fw = F32C(0.5) * (B.W[3 * box + xf] + B.W[3 * box + xb]);
-            double fp = F32C(0.5) * (B.W[4 * box + xf] + B.W[4 * box + xb]);
-            double fue = F32C(-1.0) * B.fc[d][box + xf] * rot, fve = B.fc[d][xf] * rot;
-            double c1 = fd * fu, c2 = fd * fv, c3 = fd * fw;
-            double Vn_a = fu * Sx + fv * Sy + fw * Sz;
-            double Vn_e = fue * Sx + fve * Sy + F32C(0.0) * Sz;
-            double c4 = fp / (Gamma - F32C(1.0)) + fd * (fu * fu + fv * fv + fw * fw) / F32C(2.0);
-            out[0] = fd * (Vn_a - Vn_e);
-            out[1] = c1 * (Vn_a - Vn_e) + fp * Sx;
-            out[2] = c2 * (Vn_a - Vn_e) + fp * Sy;
-            out[3] = c3 * (Vn_a - Vn_e) + fp * Sz;
-            out[4] = (c4 + fp) * (Vn_a - Vn_e) + fp * Vn_e;
+            double wa[5], wb[5];
+            for (int c = 0; c < 5; ++c) { wa[c] = B.W[c * box + xf]; wb[c] = B.W[c * box + xb]; }
+            wall_face_core(g.gamma, wa, wb, B.fS[d][xf], B.fS[d][box + xf], B.fS[d][2 * box + xf], B.fc[d][xf],
+                           B.fc[d][box + xf], B.wrot[s][tix], out);
             return;
         }
     }
@@ -427,64 +413,13 @@ __device__ static void face_flux(const BlockDev &B, const Gas &g, int d, int i, 
                 for (int c = 0; c < 6; ++c) wFB[o - 1][c] = *dummy_ptr(B, s, layer, 5 + c, ta, tb);
         }
     }
-    double L[5], R[5], cvL[5], cvR[5];
-    const double lim_zero = F32C(1E-20);
-    if (chr) {
-        recon_char(g.recon_scheme, Gamma, g.rgas, wFB[0], wFB[1], st, cvL, cvR);
-        double pL = (Gamma - F32C(1.0)) * (cvL[4] - F32C(0.5) * (cvL[1] * cvL[1] + cvL[2] * cvL[2] + cvL[3] * cvL[3]) / cvL[0]);
-        double pR = (Gamma - F32C(1.0)) * (cvR[4] - F32C(0.5) * (cvR[1] * cvR[1] + cvR[2] * cvR[2] + cvR[3] * cvR[3]) / cvR[0]);
-        if (cvL[0] <= lim_zero || pL <= lim_zero) for (int c = 0; c < 5; ++c) cvL[c] = st[1][c];
-        if (cvR[0] <= lim_zero || pR <= lim_zero) for (int c = 0; c < 5; ++c) cvR[c] = st[2][c];
-    } else if (prim) {
-        for (int c = 0; c < 5; ++c) recon1_prim(g.recon_scheme, st[0][c], st[1][c], st[2][c], st[3][c], L[c], R[c]);
-        if (L[0] <= lim_zero || L[4] <= lim_zero) for (int c = 0; c < 5; ++c) L[c] = st[1][c];
-        if (R[0] <= lim_zero || R[4] <= lim_zero) for (int c = 0; c < 5; ++c) R[c] = st[2][c];
-        cvL[0] = L[0]; cvL[1] = L[0] * L[1]; cvL[2] = L[0] * L[2]; cvL[3] = L[0] * L[3];
-        cvL[4] = L[4] / (Gamma - F32C(1.0)) + L[0] * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]) / F32C(2.0);
-        cvR[0] = R[0]; cvR[1] = R[0] * R[1]; cvR[2] = R[0] * R[2]; cvR[3] = R[0] * R[3];
-        cvR[4] = R[4] / (Gamma - F32C(1.0)) + R[0] * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3]) / F32C(2.0);
-    } else {
-        for (int c = 0; c < 5; ++c) recon1_cons(g.recon_scheme, st[0][c], st[1][c], st[2][c], st[3][c], cvL[c], cvR[c]);
-        double pL = (Gamma - F32C(1.0)) * (cvL[4] - F32C(0.5) * (cvL[1] * cvL[1] + cvL[2] * cvL[2] + cvL[3] * cvL[3]) / cvL[0]);
-        double pR = (Gamma - F32C(1.0)) * (cvR[4] - F32C(0.5) * (cvR[1] * cvR[1] + cvR[2] * cvR[2] + cvR[3] * cvR[3]) / cvR[0]);
-        if (cvL[0] <= lim_zero || pL <= lim_zero) for (int c = 0; c < 5; ++c) cvL[c] = st[1][c];
-        if (cvR[0] <= lim_zero || pR <= lim_zero) for (int c = 0; c < 5; ++c) cvR[c] = st[2][c];
-    }
-    // face frame (source/Flux_Split_Scheme.cpp:728-776)
-    const double s_ = B.fs[d][xf];
-    double nx = B.fS[d][xf] / s_, ny = B.fS[d][box + xf] / s_, nz = B.fS[d][2 * box + xf] / s_;
     const long xd = d == 0 ? cidx(B, i, j + 1, k + 1) : d == 1 ? cidx(B, i + 1, j, k + 1) : cidx(B, i + 1, j + 1, k);
-    double rx = B.node[xd] - B.node[xf], ry = B.node[box + xd] - B.node[box + xf], rz = B.node[2 * box + xd] - B.node[2 * box + xf];
-    double rl = sqrt(rx * rx + ry * ry + rz * rz);
-    double t1x = rx / rl, t1y = ry / rl, t1z = rz / rl;
-    double qx = ny * t1z - nz * t1y, qy = nz * t1x - nx * t1z, qz = nx * t1y - ny * t1x;
-    double ql = sqrt(qx * qx + qy * qy + qz * qz);
-    double t2x = qx / ql, t2y = qy / ql, t2z = qz / ql;
-    double fue = F32C(-1.0) * B.fc[d][box + xf] * B.w_rot, fve = B.fc[d][xf] * B.w_rot;
-    double que = fue * nx + fve * ny + F32C(0.0) * nz;
-    double QL[5], QR[5];
-    QL[0] = cvL[0]; QL[4] = cvL[4];
-    QL[1] = cvL[1] * nx + cvL[2] * ny + cvL[3] * nz;
-    QL[2] = cvL[1] * t1x + cvL[2] * t1y + cvL[3] * t1z;
-    QL[3] = cvL[1] * t2x + cvL[2] * t2y + cvL[3] * t2z;
-    QR[0] = cvR[0]; QR[4] = cvR[4];
-    QR[1] = cvR[1] * nx + cvR[2] * ny + cvR[3] * nz;
-    QR[2] = cvR[1] * t1x + cvR[2] * t1y + cvR[3] * t1z;
-    QR[3] = cvR[1] * t2x + cvR[2] * t2y + cvR[3] * t2z;
-    double Fc[5];
-    if (g.flux_scheme == 200) {
-        double fp_[5], fm_[5];
-        sw_half(Gamma, QL, que, 1.0, fp_);
-        sw_half(Gamma, QR, que, -1.0, fm_);
-        for (int c = 0; c < 5; ++c) Fc[c] = fp_[c] + fm_[c];
-    } else {
-        roe_flux(Gamma, QL, QR, que, Fc);
-    }
-    out[0] = Fc[0] * s_;
-    out[1] = (Fc[1] * nx + Fc[2] * t1x + Fc[3] * t2x) * s_;
-    out[2] = (Fc[1] * ny + Fc[2] * t1y + Fc[3] * t2y) * s_;
-    out[3] = (Fc[1] * nz + Fc[2] * t1z + Fc[3] * t2z) * s_;
-    out[4] = Fc[4] * s_;
+    // a non-rotating block skips the face centroid (its grid velocity is zero)
+    double fcx = 0.0, fcy = 0.0;
+    if (B.w_rot != 0.0) { fcx = B.fc[d][xf]; fcy = B.fc[d][box + xf]; }
+    face_core(g, st, wFB[0], wFB[1], B.fS[d][xf], B.fS[d][box + xf], B.fS[d][2 * box + xf], B.fs[d][xf], fcx, fcy,
+              B.w_rot, B.node[xd] - B.node[xf], B.node[box + xd] - B.node[box + xf],
+              B.node[2 * box + xd] - B.node[2 * box + xf], out);
 }
 
 // Tile of TI x TJ x TK cells per workgroup; every face of the tile is computed
@@ -499,6 +434,7 @@ __global__ void __launch_bounds__(TI *TJ *TK) k_convective(const BlockDev *__res
     __shared__ double sJ[5][TI][TJ + 1][TK];
     __shared__ double sK[5][TI][TJ][TK + 1];
     const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0) return;   // padding of the XCD-ordered tile list
     const BlockDev &B = blocks[tl.x];
     const int t = threadIdx.x;
     const int tk = t % TK, tj = (t / TK) % TJ, ti = t / (TK * TJ);

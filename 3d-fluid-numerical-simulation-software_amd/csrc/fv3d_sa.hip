@@ -123,6 +123,7 @@ __global__ void __launch_bounds__(TI *TJ *TK) k_sa_faces(const BlockDev *__restr
     __shared__ double sw[7 * NH];
     __shared__ double sf[3 * TL::NF];   // convective, viscous, plain gradient flux
     const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0) return;   // padding of the XCD-ordered tile list
     const BlockDev &B = blocks[tl.x];
     const long box = B.box;
     const int t = threadIdx.x;

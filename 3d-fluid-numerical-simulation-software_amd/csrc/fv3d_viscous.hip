@@ -169,6 +169,7 @@ __global__ void __launch_bounds__(TI *TJ *TK) k_viscous(const BlockDev *__restri
     __shared__ double sw[4 * NH];
     __shared__ double sf[5 * TL::NF];
     const int4 tl = tiles[blockIdx.x];
+    if (tl.x < 0) return;   // padding of the XCD-ordered tile list
     const BlockDev &B = blocks[tl.x];
     const long box = B.box;
     const int t = threadIdx.x;

@@ -184,7 +184,8 @@ def time_form(case, precision, a):
         # residual tile shapes picked on the running state, then one more
         # warm-up iteration (the graphs are captured again)
         g.iterate(a.warmup - 1, graph=graph)
-        tiles = [GpuSolver.TILE_SHAPES[q] if q >= 0 else None for q in g.autotune(3)]
+        sel = g.autotune(3)
+        tiles = [GpuSolver.conv_shape(sel[0])] + [GpuSolver.TILE_SHAPES[q] if q >= 0 else None for q in sel[1:]]
         g.iterate(1, graph=graph)
     g.sync()
     t0 = time.perf_counter()
