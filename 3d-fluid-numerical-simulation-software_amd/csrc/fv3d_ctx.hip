@@ -164,7 +164,9 @@ struct fv3d_ctx {
     bool mlp_closure = false;
     bool owns_stream = true;
     bool mg_coarse = false;          // allocate cv_2h / Res_2h (coarse level)
-    int ng = 1;
+    bool mg_fas = false;             // a level of the N-level V-cycle: allocate the forcing too
+    int ng = 1;                      // 2 when a coarser level exists
+    int nlev = 1;                    // levels from this one down
     long iter_host = 0;              // iterations run (host copy, picks the cycle phase)
     bool initial_done = false;       // fv3d_initial ran (the two-grid cycle needs it)
     ScalarsDev *ramp_sc = nullptr;
@@ -464,6 +466,7 @@ static int setup_block(fv3d_ctx *c, HostBlock &b, int index)
         if ((rc = alloc(&B.cv2h, 5 * B.box))) return rc;
         if ((rc = alloc(&B.r2h, 5 * B.box))) return rc;
     }
+    if (c->mg_fas && (rc = alloc(&B.fasP, 5 * B.box))) return rc;
     {
         // tile-skewed sweep rows (k_sweep_fill at set-up, k_sweep_pack per iteration)
         B.ntk = (B.nk + kKbaTK - 1) / kKbaTK;
@@ -962,17 +965,29 @@ static int setup_coarse(fv3d_ctx *c)
     }
     // coarse cell (I,J,K) covers fine cells 2I-1..2I: every block needs an even
     // number of cells along each direction (patches covering its faces ensure
-    // it; a block whose faces they do not cover is rejected here)
+    // it; a block whose faces they do not cover is rejected here, except below
+    // the first coarse level, where the V-cycle just stops)
     for (size_t n = 0; n < c->blocks.size(); ++n) {
         const HostBlock &b = c->blocks[n];
-        if ((b.IM - 1) % 2 || (b.JM - 1) % 2 || (b.KM - 1) % 2)
+        const bool tiny = (b.IM - 1) / 2 < 3 || (b.JM - 1) / 2 < 3 || (b.KM - 1) / 2 < 3;
+        if ((b.IM - 1) % 2 || (b.JM - 1) % 2 || (b.KM - 1) % 2 || (tiny && c->mg_coarse)) {
+            if (c->mg_coarse) {
+                c->ng = 1;
+                return FV3D_OK;
+            }
             return fail(c, FV3D_EINVAL, "two-grid cycle: block %d has %dx%dx%d cells, not even along every direction",
                         (int)n + 1, b.IM - 1, b.JM - 1, b.KM - 1);
+        }
     }
+    // levels still wanted below this one: the reference's cycle stops at the
+    // first coarse level; the V-cycle goes on while cfg.mg_levels allows
+    const int want = c->cfg.mg_levels >= 3 ? c->cfg.mg_levels : 2;
     fv3d_ctx *m = new fv3d_ctx();
     m->cfg = c->cfg;
-    m->cfg.multigrid = 0;
+    m->cfg.mg_levels = want - 1;
+    m->cfg.multigrid = want - 1 >= 2 ? 1 : 0;
     m->cfg.turbulence = 0;
+    m->mg_fas = c->mg_fas || want >= 3;
     m->device = c->device;
     m->stream = c->stream;
     m->owns_stream = false;
@@ -1034,6 +1049,7 @@ static int setup_coarse(fv3d_ctx *c)
     c->n_mg_tiles = (int)tl.size();
     c->coarse = m;
     c->ng = 2;
+    c->nlev = 1 + m->nlev;
     return FV3D_OK;
 }
 
@@ -1350,7 +1366,7 @@ int fv3d_nut_upload(fv3d_ctx *c, int block, const double *ev)
     return FV3D_OK;
 }
 
-int fv3d_levels(const fv3d_ctx *c) { return c ? c->ng : FV3D_EINVAL; }
+int fv3d_levels(const fv3d_ctx *c) { return c ? c->nlev : FV3D_EINVAL; }
 
 // CMesh::Initial (source/ComputeAllBlockVars.cpp:38-48) on one level: the
 // uniform field of CBlocks::Initial_Uniform (source/Initialization.cpp:17-72)
@@ -2014,6 +2030,51 @@ static int launch_limiter(fv3d_ctx *c)
 // prolonged correction; the fine level reports.
 enum { kMgCoarse = 0, kMgSwitch = 1, kMgCycle = 2 };
 
+// The residual of one level (CMesh::ResFlux, source/ComputeAllBlockVars.cpp:
+// 223-255) into B.R, with lambda, dt and the LU-SGS diagonals of that state.
+static int level_resflux(fv3d_ctx *c)
+{
+    int rc;
+    if ((rc = launch_bc(c))) return rc;
+    if ((rc = launch_turbulence(c))) return rc;
+    if ((rc = launch_spectral_dt(c))) return rc;
+    if ((rc = launch_convective(c))) return rc;
+    return launch_viscous(c);
+}
+
+// Level m of the N-level V-cycle below its finer level f, whose B.R holds its
+// total residual: restriction (state and residual), the residual of the
+// restricted state and the forcing, one forced LU-SGS step, then (above the
+// coarsest level) the total residual restricted to the next level down, its
+// correction prolonged back, and the limiter.  The first coarse level of the
+// reference's two-grid cycle reuses lambda and dt of its start-up phase
+// instead (mg_iteration); every level below it is this one.
+static int fas_level(fv3d_ctx *f, fv3d_ctx *m)
+{
+    int rc;
+    const dim3 g((unsigned)((m->ncell + 255) / 256)), b(256);
+    const int nb = (int)m->blocks.size();
+    hipLaunchKernelGGL(k_mg_restrict, g, b, 0, f->stream, f->d_blocks, m->d_blocks, m->d_cstart, nb, m->gas, m->d_bad);
+    HIPCK(f, hipGetLastError());
+    if ((rc = launch_limiter(m))) return rc;
+    if ((rc = level_resflux(m))) return rc;
+    hipLaunchKernelGGL(k_fas<true>, g, b, 0, f->stream, m->d_blocks, m->d_cstart, nb);
+    HIPCK(f, hipGetLastError());
+    if ((rc = launch_sweeps(m))) return rc;
+    if ((rc = launch_update(m))) return rc;
+    if (m->coarse) {
+        if ((rc = level_resflux(m))) return rc;
+        hipLaunchKernelGGL(k_fas<false>, g, b, 0, f->stream, m->d_blocks, m->d_cstart, nb);
+        HIPCK(f, hipGetLastError());
+        if ((rc = fas_level(m, m->coarse))) return rc;
+        hipLaunchKernelGGL(k_mg_prolong<false>, dim3((unsigned)m->n_mg_tiles), b, 0, f->stream, m->d_blocks,
+                           m->coarse->d_blocks, m->d_mg_tiles, m->gas, m->d_bad);
+        HIPCK(f, hipGetLastError());
+        if ((rc = launch_limiter(m))) return rc;
+    }
+    return FV3D_OK;
+}
+
 static int mg_iteration(fv3d_ctx *c, int phase, double *row)
 {
     fv3d_ctx *m = c->coarse;
@@ -2036,12 +2097,17 @@ static int mg_iteration(fv3d_ctx *c, int phase, double *row)
     if ((rc = launch_spectral_dt(c))) return rc;
     if ((rc = launch_convective(c))) return rc;
     if ((rc = launch_viscous(c))) return rc;
-    hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)((m->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
-                       m->d_blocks, m->d_cstart, nb, m->gas, m->d_bad);
-    HIPCK(c, hipGetLastError());
-    if ((rc = launch_limiter(m))) return rc;
-    if ((rc = launch_sweeps(m))) return rc;
-    if ((rc = launch_update(m))) return rc;
+    if (m->coarse) {
+        // N-level V-cycle: the levels below take the full approximation storage form
+        if ((rc = fas_level(c, m))) return rc;
+    } else {
+        hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)((m->ncell + 255) / 256)), dim3(256), 0, c->stream,
+                           c->d_blocks, m->d_blocks, m->d_cstart, nb, m->gas, m->d_bad);
+        HIPCK(c, hipGetLastError());
+        if ((rc = launch_limiter(m))) return rc;
+        if ((rc = launch_sweeps(m))) return rc;
+        if ((rc = launch_update(m))) return rc;
+    }
     hipLaunchKernelGGL(k_mg_prolong<false>, dim3((unsigned)c->n_mg_tiles), dim3(256), 0, c->stream, c->d_blocks,
                        m->d_blocks, c->d_mg_tiles, c->gas, c->d_bad);
     HIPCK(c, hipGetLastError());

@@ -97,6 +97,9 @@ struct BlockDev {
     // cv_2h and the restricted residual Res_2h, 5*box each
     double *cv2h;
     double *r2h;
+    // levels below the first coarse one of the N-level V-cycle: the forcing
+    // P = Res_2h - R(Q_2h) of the full approximation storage scheme, 5*box
+    double *fasP;
 };
 
 __host__ __device__ inline long cidx(const BlockDev &B, int i, int j, int k)

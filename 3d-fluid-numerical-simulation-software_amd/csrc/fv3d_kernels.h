@@ -168,6 +168,8 @@ __global__ void k_mg_corner(const BlockDev *coarse);
 constexpr int kMgTI = 2, kMgTJ = 4, kMgTK = 8;
 template <bool STATE>
 __global__ void k_mg_prolong(const BlockDev *fine, const BlockDev *coarse, const int4 *tiles, Gas g, int *bad_flag);
+template <bool FORCE>
+__global__ void k_fas(const BlockDev *blocks, const long *cstart, int nblock);
 
 // laminar Navier-Stokes terms (fv3d_viscous.hip)
 // coordinate-transformation coefficients of every face and cell of one block (set-up)

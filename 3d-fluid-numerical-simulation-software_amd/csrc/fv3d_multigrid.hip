@@ -190,6 +190,42 @@ __global__ void __launch_bounds__(256) k_mg_prolong(const BlockDev *__restrict__
     }
 }
 
+// N-level V-cycle, full approximation storage on the levels below the first
+// coarse one (the reference has two levels at most, source/UserInput.cpp:
+// 1057-1058, and its coarse level takes the restricted fine residual as its
+// right-hand side, source/ComputeAllBlockVars.cpp:269-309).  On level l, after
+// restriction (cv_2h = Q_l, Res_2h = the restricted total residual of level
+// l-1) and a residual evaluation R(Q_l):
+//   FORCE: P = Res_2h - R(Q_l), R = Res_2h  (the right-hand side of the sweep
+//          is the restricted residual, as on the reference's coarse level, but
+//          with lambda and dt of the current state);
+//   ADD:   R = R(Q_l after the sweep) + P  (the total residual restricted to
+//          level l+1).
+// At a converged fine state every forcing cancels its residual and every
+// correction Q_l - cv_2h vanishes, so the fixed point is the fine level's.
+template <bool FORCE>
+__global__ void __launch_bounds__(256) k_fas(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
+                                             int nblock)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int n, i, j, k;
+    if (!cell_item(blocks, cstart, nblock, t, n, i, j, k)) return;
+    const BlockDev &B = blocks[n];
+    const long x = cidx(B, i, j, k), box = B.box;
+    for (int c = 0; c < 5; ++c) {
+        const long y = c * box + x;
+        if (FORCE) {
+            B.fasP[y] = B.r2h[y] - B.R[y];
+            B.R[y] = B.r2h[y];
+        } else {
+            B.R[y] = B.R[y] + B.fasP[y];
+        }
+    }
+}
+
+template __global__ void k_fas<true>(const BlockDev *, const long *, int);
+template __global__ void k_fas<false>(const BlockDev *, const long *, int);
+
 template __global__ void k_mg_prolong<true>(const BlockDev *, const BlockDev *, const int4 *, Gas, int *);
 template __global__ void k_mg_prolong<false>(const BlockDev *, const BlockDev *, const int4 *, Gas, int *);
 

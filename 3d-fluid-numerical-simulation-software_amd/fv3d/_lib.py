@@ -33,7 +33,7 @@ class Config(C.Structure):
                 ("flux_scheme", C.c_int), ("turbulence", C.c_int), ("limiter", C.c_int), ("global_dt", C.c_int),
                 ("device", C.c_int), ("precision", C.c_int),
                 ("multigrid", C.c_int), ("coarse_iter", C.c_int), ("initial_type", C.c_int),
-                ("p21ratio", C.c_double), ("outlet_dir", C.c_double * 3)]
+                ("p21ratio", C.c_double), ("outlet_dir", C.c_double * 3), ("mg_levels", C.c_int)]
 
 
 class Fv3dError(RuntimeError):

@@ -215,6 +215,31 @@ def passage_batch(n: int = 64, seed: int = 3, min_cells: float = 1e5, max_cells:
     return out
 
 
+def vane_passage(ni: int = 88, nk_blocks: Sequence[int] = (24, 48, 144, 72, 40), seed: int = 1,
+                 turbulence: int = 200, levels: int = 0, coarse_iter: int = 2, p_ratio: float = 0.95,
+                 camber: float = 0.1, max_iter: int = 500) -> Case:
+    """The learned-closure passage: the five-block annular passage (hub 0.2 m,
+    tip 0.25 m, 22 blades) with a turning vane in the middle block (the
+    parabola theta = camber * pitch * t^2, which meets the axial inflow at
+    zero incidence), ni x ni cells in cross-section and nk_blocks along the
+    axis (88 x 88 x (24, 48, 144, 72, 40) = 2,540,032 cells, every size a
+    multiple of 8 so the grid halves three times), Navier-Stokes with
+    Spalart-Allmaras (nu~ = ev_ratio at the start) or Baldwin-Lomax, Roe +
+    MUSCL3 on primitive variables, outlet p_ratio * Pt, stationary; multigrid
+    from the uniform field on the coarse level with ``levels`` grid levels
+    (0: the reference's two-grid cycle)."""
+    c = annular_passage(ni=ni, nj=ni, nk_blocks=nk_blocks, seed=seed, rpm=0.0, blade_blocks=(2,), camber=camber,
+                        viscous=True, len_ref=1.0, max_iter=max_iter, profile="turn", p_ratio=p_ratio,
+                        z_extent=(-0.06, -0.035, -0.015, 0.035, 0.055, 0.08))
+    if turbulence == 200:
+        c = with_sa(c, amp=0.0)
+    else:
+        c.cfg.turbulence = turbulence
+    c.cfg.use_multigrid, c.cfg.reuse_solution, c.cfg.initial = "Y", "N", "U"
+    c.cfg.coarse_iter, c.cfg.mg_levels = coarse_iter, levels
+    return c
+
+
 def from_meta(meta: Dict) -> Case:
     """The case a fixture's metadata describes: ``kwargs`` for bump_channel (or
     for annular_passage when ``generator`` names it, with ``turn`` a list of

@@ -85,6 +85,9 @@ class Config:
     flux_scheme: int = 300         # 200 Steger-Warming / 300 Roe
     time_scheme: int = 500         # 500 LU-SGS
     turbulence: int = 0            # 0 none / 100 Baldwin-Lomax / 200 Spalart-Allmaras
+    # not in config.dat: grid levels with use_multigrid (0/2 the reference's
+    # two-grid cycle, 3-5 the N-level V-cycle of this library)
+    mg_levels: int = 0
 
     def lines(self) -> List[str]:
         a = self.inlet_angle

@@ -48,6 +48,7 @@ def make_config(cfg: CaseConfig, device: int = 0, precision: int = 0) -> _lib.Co
     s.p21ratio = cfg.p21_ratio
     for c in range(3):
         s.outlet_dir[c] = cfg.outlet_angle[c]
+    s.mg_levels = cfg.mg_levels
     return s
 
 

@@ -64,6 +64,12 @@ typedef struct {
     int initial_type;   /* start field of fv3d_initial: 0 uniform, 1 interpolated along z */
     double p21ratio;    /* P21ratio: outlet/inlet static pressure of the interpolated start */
     double outlet_dir[3];   /* OutletAirAngle: flow direction at the outlet of the interpolated start */
+    /* Grid levels with multigrid (no config.dat counterpart): 0 or 2 runs the
+     * reference's two-grid cycle, capped at two levels by
+     * source/UserInput.cpp:1057-1058; 3 to 5 run an N-level V-cycle (full
+     * approximation storage below the first coarse level) over as many of
+     * them as every block and patch range can be halved to. */
+    int mg_levels;
 } fv3d_config;
 
 /* Context life cycle. */
@@ -109,7 +115,8 @@ int fv3d_connection_add(fv3d_ctx *ctx, int kind, int orient, int block1, int fac
  * source/UserInput.cpp:241-334 and 446-797). */
 int fv3d_setup(fv3d_ctx *ctx);
 
-/* Grid levels in use after fv3d_setup: 1, or 2 for the two-grid cycle. */
+/* Grid levels in use after fv3d_setup: 1, 2 for the two-grid cycle, or up to
+ * cfg.mg_levels for the N-level V-cycle. */
 int fv3d_levels(const fv3d_ctx *ctx);
 
 /* Batching: independent passage instances in one context, advanced together by
