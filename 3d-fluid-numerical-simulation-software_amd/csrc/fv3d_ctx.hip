@@ -162,6 +162,7 @@ struct fv3d_ctx {
     float *d_mlp_xc = nullptr;   // closure: 8 floats per cell (features, gate flag)
     long mlp_step = 0;
     bool mlp_closure = false;
+    double mlp_relax = 1.0;          // under-relaxation of the closure's mue_t
     bool owns_stream = true;
     bool mg_coarse = false;          // allocate cv_2h / Res_2h (coarse level)
     bool mg_fas = false;             // a level of the N-level V-cycle: allocate the forcing too
@@ -175,7 +176,7 @@ struct fv3d_ctx {
     hipGraphExec_t graph_coarse = nullptr;   // coarse start-up phase of the two-grid cycle
     // timing
     int timing = 0;
-    hipEvent_t ev[10];
+    hipEvent_t ev[14];
     double t_ms[FV3D_TIMING_SLOTS] = {};
     long t_n[FV3D_TIMING_SLOTS] = {};
 };
@@ -736,7 +737,7 @@ int fv3d_ctx_create(const fv3d_config *cfg, fv3d_ctx **out)
     if (e != hipSuccess) { *out = c; return fail(c, FV3D_EHIP, "hipSetDevice(%d): %s", c->device, hipGetErrorString(e)); }
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { *out = c; return fail(c, FV3D_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
-    for (int n = 0; n < 10; ++n) (void)hipEventCreate(&c->ev[n]);
+    for (int n = 0; n < 14; ++n) (void)hipEventCreate(&c->ev[n]);
     nondimensionalize(c);
     *out = c;
     return FV3D_OK;
@@ -784,7 +785,7 @@ int fv3d_ctx_destroy(fv3d_ctx *c)
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (c->stream) {
-        for (int n = 0; n < 10; ++n) (void)hipEventDestroy(c->ev[n]);
+        for (int n = 0; n < 14; ++n) (void)hipEventDestroy(c->ev[n]);
         if (c->owns_stream) (void)hipStreamDestroy(c->stream);
     }
     delete c;
@@ -993,7 +994,7 @@ static int setup_coarse(fv3d_ctx *c)
     m->owns_stream = false;
     m->parent = c;
     m->mg_coarse = true;
-    for (int n = 0; n < 10; ++n) (void)hipEventCreate(&m->ev[n]);
+    for (int n = 0; n < 14; ++n) (void)hipEventCreate(&m->ev[n]);
     nondimensionalize(m);
     auto bail = [&](int rc) {
         int code = fail(c, rc, "coarse level: %s", m->err.c_str());
@@ -1660,7 +1661,7 @@ static int launch_turbulence(fv3d_ctx *c, int save_old = 0)
         hipLaunchKernelGGL(k_mlp_closure, dim3((unsigned)(chunks < 768 ? chunks : 768)), dim3(256), 0, c->stream,
                            c->d_mlp, (long)c->ncell, c->d_mlp_xc);
         hipLaunchKernelGGL(k_mlp_mut, dim3((unsigned)chunks), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb,
-                           c->gas, c->d_mlp_bounds, c->d_mlp_xc, c->d_mlp_tripped);
+                           c->gas, c->d_mlp_bounds, c->d_mlp_xc, c->d_mlp_tripped, c->mlp_relax);
         hipLaunchKernelGGL(k_sa_ghost, dim3((unsigned)((c->nghost + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
                            c->d_gstart, nb, c->gas);
         HIPCK(c, hipGetLastError());
@@ -1906,10 +1907,13 @@ static int lusgs_timed(fv3d_ctx *c)
     int rc;
     if (c->timing) HIPCK(c, hipEventRecord(c->ev[0], c->stream));
     if ((rc = launch_bc(c))) return rc;
+    if (c->timing) HIPCK(c, hipEventRecord(c->ev[10], c->stream));
     if ((rc = launch_turbulence(c, 1))) return rc;
+    if (c->timing) HIPCK(c, hipEventRecord(c->ev[11], c->stream));
     if ((rc = launch_spectral_dt(c))) return rc;
     if (c->timing) HIPCK(c, hipEventRecord(c->ev[1], c->stream));
     if ((rc = launch_convective(c))) return rc;
+    if (c->timing) HIPCK(c, hipEventRecord(c->ev[12], c->stream));
     if ((rc = launch_viscous(c))) return rc;
     if (c->timing) HIPCK(c, hipEventRecord(c->ev[2], c->stream));
     if ((rc = launch_sweeps(c))) return rc;
@@ -2137,8 +2141,13 @@ static int accumulate_timing(fv3d_ctx *c)
 {
     HIPCK(c, hipEventSynchronize(c->ev[5]));
     float ms;
-    const int pairs[4][2] = {{2, 3}, {3, 4}, {0, 1}, {4, 5}};   // convective, sweeps, update, bc+spectral... finalize
-    // class 0 residual, 1 sweeps, 2 update, 3 bc+spectral+dt, 4 finalize
+    // slots: 0 residual (convective + viscous), 1 LU-SGS operator (pack, sweeps,
+    // unpack), 2 update, 3 ghost fill + turbulence + spectral radii and dt,
+    // 4 finalize, 5/6 forward/backward sweep kernels, 7 the cycle's coarse
+    // correction (second fine residual, transfers, coarse levels), 8 convective
+    // residual kernel, 9 viscous (and nu~) residual kernels, 10 ghost fill and
+    // patch exchange, 11 turbulence model (vorticity, Baldwin-Lomax,
+    // Spalart-Allmaras or the learned closure)
     HIPCK(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[2])); c->t_ms[0] += ms; c->t_n[0]++;
     HIPCK(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3])); c->t_ms[1] += ms; c->t_n[1]++;
     HIPCK(c, hipEventElapsedTime(&ms, c->ev[3], c->ev[4])); c->t_ms[2] += ms; c->t_n[2]++;
@@ -2150,7 +2159,10 @@ static int accumulate_timing(fv3d_ctx *c)
         HIPCK(c, hipEventElapsedTime(&ms, c->ev[6], c->ev[7])); c->t_ms[5] += ms; c->t_n[5]++;
         HIPCK(c, hipEventElapsedTime(&ms, c->ev[7], c->ev[8])); c->t_ms[6] += ms; c->t_n[6]++;
     }
-    (void)pairs;
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[12])); c->t_ms[8] += ms; c->t_n[8]++;
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev[12], c->ev[2])); c->t_ms[9] += ms; c->t_n[9]++;
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[10])); c->t_ms[10] += ms; c->t_n[10]++;
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev[10], c->ev[11])); c->t_ms[11] += ms; c->t_n[11]++;
     return FV3D_OK;
 }
 
@@ -2571,6 +2583,15 @@ int fv3d_mlp_closure(fv3d_ctx *c, int on)
         if (rc) return rc;
     }
     c->mlp_closure = on != 0;
+    drop_graphs(c);
+    return FV3D_OK;
+}
+
+int fv3d_mlp_relax(fv3d_ctx *c, double omega)
+{
+    READY(c);
+    if (!(omega > 0.0 && omega <= 1.0)) return fail(c, FV3D_EINVAL, "closure relaxation %g not in (0, 1]", omega);
+    c->mlp_relax = omega;
     drop_graphs(c);
     return FV3D_OK;
 }

@@ -389,7 +389,8 @@ __global__ void __launch_bounds__(256) k_mlp_closure(const float *__restrict__ P
 // nu~ / Re, clipped as SA_Model clips it); tripped counts those cells.
 __global__ void __launch_bounds__(256) k_mlp_mut(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
                                                  int nblock, Gas g, const float *__restrict__ bounds,
-                                                 const float *__restrict__ xc, unsigned *__restrict__ tripped)
+                                                 const float *__restrict__ xc, unsigned *__restrict__ tripped,
+                                                 double relax)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int n, i, j, k;
@@ -413,6 +414,8 @@ __global__ void __launch_bounds__(256) k_mlp_mut(const BlockDev *__restrict__ bl
         if (mt < 0.0) mt = 0.0;
     }
     if (mue_lam > 0 && mt > 200.0 * mue_lam) mt = 200.0 * mue_lam;
+    // under-relaxation of the algebraic closure: the fixed point is unchanged
+    if (relax != 1.0) mt = B.mut[x] + relax * (mt - B.mut[x]);
     B.mut[x] = mt;
 }
 

@@ -20,7 +20,7 @@ EXPORTS = [
     "fv3d_viscous", "fv3d_curl", "fv3d_resflux", "fv3d_lusgs", "fv3d_res_calculate", "fv3d_perf", "fv3d_iterate", "fv3d_iterate_graph", "fv3d_sync",
     "fv3d_set_timing", "fv3d_timing", "fv3d_set_debug", "fv3d_set_tiles", "fv3d_autotune", "fv3d_cell_count",
     "fv3d_mlp_set", "fv3d_mlp_get", "fv3d_mlp_features", "fv3d_mlp_targets", "fv3d_mlp_forward", "fv3d_mlp_grad", "fv3d_mlp_train",
-    "fv3d_mlp_closure", "fv3d_mlp_tripped",
+    "fv3d_mlp_closure", "fv3d_mlp_relax", "fv3d_mlp_tripped",
 ]
 
 
@@ -86,6 +86,7 @@ def load():
         "fv3d_mlp_grad": (i, [vp, p, p, l, p, p]),
         "fv3d_mlp_train": (i, [vp, p, p, l, i, i, C.c_float, p]),
         "fv3d_mlp_closure": (i, [vp, i]),
+        "fv3d_mlp_relax": (i, [vp, C.c_double]),
         "fv3d_mlp_tripped": (l, [vp]),
         "fv3d_cell_count": (l, [vp]),
     }

@@ -233,6 +233,9 @@ class GpuSolver:
     def mlp_closure(self, on: bool):
         self._check(self.L.fv3d_mlp_closure(self.h, int(on)), "mlp_closure")
 
+    def mlp_relax(self, omega: float):
+        self._check(self.L.fv3d_mlp_relax(self.h, float(omega)), "mlp_relax")
+
     def mlp_tripped(self) -> int:
         return self._check(int(self.L.fv3d_mlp_tripped(self.h)), "mlp_tripped")
 
@@ -258,7 +261,7 @@ class GpuSolver:
     def set_timing(self, on: bool):
         self._check(self.L.fv3d_set_timing(self.h, int(on)), "set_timing")
 
-    TIMING_SLOTS = 8   # FV3D_TIMING_SLOTS
+    TIMING_SLOTS = 12   # FV3D_TIMING_SLOTS
 
     @property
     def levels(self) -> int:

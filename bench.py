@@ -1,96 +1,123 @@
-"""Throughput and time to a 1e-6 residual drop of the LU-SGS iteration on one
-MI355X, the chosen form (FP32 defect correction: FP32 LU-SGS operator over an
-FP64 residual) against plain FP64.
+"""Throughput and time to a 1e-6 residual drop of the solver on one MI355X:
+the chosen form against the plain FP64 drop-in.  Prints one JSON line.
 
-A step is one full iteration of the solver (ghost fill incl. interior/periodic
-exchange, vorticity and Baldwin-Lomax, spectral radii, local time step,
-MUSCL + Roe residual, viscous flux, forward and backward hyperplane sweeps,
-update, residual norm, section integrals and the CFL ramp; with the two-grid
-cycle also the second fine residual, restriction, the coarse LU-SGS and the
-prolonged correction) over the whole synthetic grid, with the state resident
-in HBM.  Prints one JSON line.
+Default workload (the learned-closure passage): a five-block annular passage
+(hub 0.2 m, tip 0.25 m, 22 blades) with a turning vane in the middle block,
+88 x 88 cells in cross-section and 24/48/144/72/40 along the axis
+(2,540,032 cells), Navier-Stokes with Spalart-Allmaras, Roe + MUSCL3 on
+primitive variables, LU-SGS with local time step at CFL 5, multigrid from the
+uniform field on the coarse level, stationary, outlet at 0.95 Pt.
 
-Default workload: a five-block annular blade passage (64 x 64 cross-section,
-k-blocks of 12/40/128/60/24 cells, 1,081,344 cells), cambered blade walls,
-hub and tip walls, periodic pitch, interior patches between the blocks,
-Navier-Stokes with Baldwin-Lomax, Roe + MUSCL3 on primitive variables, local
-time step CFL 5, two-grid cycle started from the uniform field on the coarse
-level.  ``value`` is the chosen form's cell-updates/s on it (fine cells per
-cycle iteration).  The time to a 1e-6 residual drop is measured on the
-96x32x32 bump channel, which converges in both forms within seconds; the
-passage itself is timed to a 1e-4 drop (its Baldwin-Lomax solution settles
-into a limit cycle near 5e-5 of the first residual).
+* plain FP64: the reference's method on it, FP64 throughout, its two-grid
+  cycle (the reference's maximum), Spalart-Allmaras transport;
+* chosen: FP32 defect correction (FP32 LU-SGS operator over the FP64
+  residual, update and norms), the four-level V-cycle, and all blocks of
+  every stage in one launch captured in a HIP graph with autotuned residual
+  tiles.  ``--closure`` adds the learned eddy-viscosity closure in place of
+  the nu~ transport (the Spalart-Allmaras value as the gated fallback, the
+  network of fv3d/closure_vane.npz written by ``--workload closure-train``);
+  it is not in the default line because the BF16 network leaves the residual
+  a floor near a 1e-6 drop (see the closure-train results).
+
+A step is one fine-level iteration (with multigrid: fine LU-SGS, the second
+fine residual, the coarse levels and the prolonged correction) over the whole
+grid with the state resident in HBM; ``value`` is the chosen form's
+cell-updates/s (fine cells x iterations / s).  ``time_to_1e-6_drop`` runs both
+forms from the uniform start until Res.dens (the reference's density residual,
+source/ComputeAllBlockVars.cpp:340-363) has fallen to 1e-6 of its first
+fine-level value and compares mass flow, pressure ratio and efficiency
+(source/Ouput.cpp:282-300) there.
+
+Other workloads: ``c2`` (64 independent passages of 0.1-2M cells batched in
+one context), ``passage5`` (the 1.08M-cell Baldwin-Lomax passage),
+``bump`` (the single-block bump channel), ``closure-train`` (the closure's
+training and held-out evaluation; writes the network).
 
 ``cpu_baseline`` times the reference solver itself (compiled from its sources
-into oracle/_ref) on the host, single-threaded: with more than one OpenMP
-thread the reference's face loops race on a shared variable
-(source/Upwind_Convective.cpp:7,29) and give wrong fluxes.
+into oracle/_ref) on the host, single-threaded, on a 64,512-cell member of
+the same passage family: with more than one OpenMP thread the reference's face
+loops race on a shared variable (source/Upwind_Convective.cpp:7,29).
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import platform
 import sys
 import tempfile
 import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, os.path.join(REPO, "3d-fluid-numerical-simulation-software_amd"))
+PKG = os.path.join(REPO, "3d-fluid-numerical-simulation-software_amd")
+sys.path.insert(0, PKG)
 sys.path.insert(0, os.path.join(REPO, "oracle"))
 
 METRIC = "cell-updates/s + time to 1e-6 residual drop, chosen vs plain FP64, 1 GPU"
-
-# Algorithmic HBM bytes per cell of one sweep launch: every array the sweep
-# semantically reads or writes counted once (R|dQ* 5, Q 5, lamc 3, dt 1,
-# vol 1, S 9 -> dQ 5; lamv 3 more for Navier-Stokes; with Spalart-Allmaras
-# nu~, its residual, the nu~ diagonal and dnu~ 4 more), in the operator's word
-# size (8 B plain FP64, 4 B for the FP32 operator).
-SWEEP_WORDS = {False: 29, True: 32}
-SWEEP_WORDS_SA = 4
-# HBM bytes one forward-sweep launch actually moves, from rocprofv3
-# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (copy-calibrated); the summaries are
-# under profiles/.  Keyed by (workload, precision).
-MEASURED_FWD_SWEEP_TRAFFIC = {("bump256x64x64", "fp64"): 472.5e6, ("bump256x64x64ns", "fp64"): 506.0e6,
-                              ("passage5", "fp32"): 312.2e6, ("passage5", "fp64"): 568.9e6,
-                              ("passage5mg", "fp32"): 310.1e6}
 HBM_PEAK_GBS = 8000.0
 PREC = {"fp64": 0, "fp32": 1}
+CLOSURE_WEIGHTS = os.path.join(PKG, "fv3d", "closure_vane.npz")
+PROFILES = os.path.join(REPO, "profiles")
+
+# Timing slots of fv3d_timing (include/fv3d.h)
+SLOTS = ["residual", "lusgs", "update", "bc_spectral_dt", "finalize", "forward_sweep", "backward_sweep",
+         "coarse_correction", "convective", "viscous", "bc_exchange", "turbulence"]
+
+
+def kernel_bytes(precision: str, sa_transport: bool):
+    """Algorithmic HBM bytes per cell of one launch: every array a kernel
+    semantically reads or writes counted once (derived quantities such as W,
+    |S| and unit normals count as recomputed), FP64 words of 8 B, the sweep
+    operator's words in its own precision."""
+    w = 4 if precision == "fp32" else 8
+    nu = 4 if sa_transport else 0    # nu~, its residual, its diagonal, dnu~
+    return {
+        # Q 5, S_I,J,K 9, nodes 3 -> R 5
+        "convective": 22 * 8,
+        # Q 5, mue_t 1, face coefficients 27, S 9 -> R 5 read and written
+        "viscous": 52 * 8,
+        # R 5, Q 5, lambda_c 3, dt 1, Vol 1, S 9, lambda_v 3 -> dQ* 5 (+ nu~)
+        "forward_sweep": (32 + nu) * w,
+        # dQ* 5, Q 5, lambda_c 3, dt 1, Vol 1, S 9, lambda_v 3 -> dQ 5 (+ nu~)
+        "backward_sweep": (32 + nu) * w,
+    }
+
+
+KERNEL_NAMES = {"convective": "k_convective (MUSCL + Roe/Steger-Warming + wall flux + source -> R)",
+                "viscous": "k_viscous (face gradients + viscous flux, and the nu~ residual k_sa_faces)",
+                "forward_sweep": "k_lusgs_sl<forward> (LU-SGS forward wavefront sweep)",
+                "backward_sweep": "k_lusgs_sl<backward> (LU-SGS backward wavefront sweep)"}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["passage5", "bump", "batch", "closure"], default="passage5")
-    ap.add_argument("--closure-cells", type=float, default=1.3e5, help="closure workload: cells per passage")
-    ap.add_argument("--closure-develop", type=int, default=600, help="closure workload: SA iterations before the split")
-    ap.add_argument("--closure-iters", type=int, default=2400, help="closure workload: iterations after the split")
-    ap.add_argument("--instances", type=int, default=64, help="batch workload: number of passages")
+    ap.add_argument("--workload", choices=["c3", "c2", "passage5", "bump", "closure-train"], default="c3")
+    ap.add_argument("--levels", type=int, default=4, help="grid levels of the chosen form's V-cycle")
+    ap.add_argument("--closure", action="store_true",
+                    help="chosen form with the learned closure (fv3d/closure_vane.npz) in place of the nu~ transport")
+    ap.add_argument("--instances", type=int, default=64, help="c2: number of passages")
     ap.add_argument("--batch-seed", type=int, default=3)
     ap.add_argument("--batch-rpm", type=float, default=0.0,
-                    help="batch workload: design rotation speed (the synthetic passage is stationary by design)")
-    ap.add_argument("--compare-alone", action="store_true",
-                    help="batch workload: also time every passage alone, one after the other")
+                    help="c2: design rotation speed (the synthetic passage is stationary by design)")
+    ap.add_argument("--compare-alone", action="store_true", help="c2: also time every passage alone")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default residual tile shapes")
-    ap.add_argument("--ni", type=int, default=64, help="bump workload: cells along i (span)")
-    ap.add_argument("--nj", type=int, default=64, help="bump workload: cells along j")
-    ap.add_argument("--nk", type=int, default=256, help="bump workload: cells along k (streamwise)")
-    ap.add_argument("--viscous", action="store_true", help="bump workload: laminar Navier-Stokes")
-    ap.add_argument("--single-grid", action="store_true", help="passage workload without the two-grid cycle")
-    ap.add_argument("--turbulence", choices=["bl", "sa"], default="bl",
-                    help="passage workload: Baldwin-Lomax or Spalart-Allmaras")
-    ap.add_argument("--coarse-iter", type=int, default=2, help="coarse start-up iterations of the two-grid cycle")
-    ap.add_argument("--precision", choices=["fp32", "fp64", "both"], default="both",
-                    help="which forms to time (the chosen form is fp32)")
+    ap.add_argument("--ni", type=int, default=64, help="bump: cells along i (span)")
+    ap.add_argument("--nj", type=int, default=64, help="bump: cells along j")
+    ap.add_argument("--nk", type=int, default=256, help="bump: cells along k (streamwise)")
+    ap.add_argument("--viscous", action="store_true", help="bump: laminar Navier-Stokes")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--sweep", choices=["rows", "kba3", "kba2", "kba", "plane"], default="rows")
     ap.add_argument("--no-drop", action="store_true", help="skip the time-to-1e-6-drop runs")
-    ap.add_argument("--drop-max-iter", type=int, default=40000)
-    ap.add_argument("--passage-drop-max-iter", type=int, default=20000)
+    ap.add_argument("--drop-max-iter", type=int, default=20000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--cpu-iters", type=int, default=8)
+    ap.add_argument("--train-instances", type=int, default=56, help="closure-train: passages (48 train + 8 held out)")
+    ap.add_argument("--train-develop", type=int, default=2500, help="closure-train: V-cycle iterations per passage")
+    ap.add_argument("--closure-out", default=None, help="closure-train: where to write the network")
+    ap.add_argument("--no-heldout", action="store_true", help="closure-train: train only")
+    ap.add_argument("--closure-relax", type=float, default=0.2, help="under-relaxation of the closure's mue_t")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run three plain 1 GiB device copies and exit (scales the HBM byte counters)")
     return ap.parse_args()
@@ -111,71 +138,63 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_per_launch": n * 8, "launches": 3}))
 
 
+def host_cpu():
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
 def cpu_baseline(iters: int):
-    """Reference solver on the C0 bump channel (96x32x32 cells), one thread."""
+    """The compiled reference solver, one thread, on a 64,512-cell member of the
+    vane-passage family (24 x 24 x 112 cells, Spalart-Allmaras, two-grid)."""
     import oracle as O
     from fv3d import cases
     if not os.path.exists(O.REF_BIN):
         return None
-    case = cases.bump_channel(ni=32, nj=32, nk=96, seed=1, max_iter=iters)
+    case = cases.vane_passage(ni=24, nk_blocks=(8, 16, 48, 24, 16), levels=0, max_iter=iters)
     with tempfile.TemporaryDirectory() as td:
         cdir, odir = os.path.join(td, "case"), os.path.join(td, "out")
         case.write(cdir)
         r = O.run_reference(cdir, iters, odir, threads=1, dump=False, timeout=600)
     t = r["timing"]
+    model, ncpu = host_cpu()
     return {"value": t["cells"] * t["iterations"] / t["loop_seconds"], "unit": "cell-updates/s", "cores": 1,
             "kind": "reference",
-            "sample": f"reference solver (oracle/_ref, -O2, 1 thread) on the 96x32x32 bump channel, "
-                      f"{t['iterations']} LU-SGS iterations, {t['loop_seconds']:.2f} s"}
+            "sample": f"reference solver (oracle/_ref, g++ -O2, 1 thread) on a 64,512-cell vane passage of the "
+                      f"same family (24x24 cross-section, Spalart-Allmaras, two-grid cycle), {t['iterations']} "
+                      f"iterations in {t['loop_seconds']:.2f} s; host {model}, {ncpu} logical CPUs visible"}
 
 
-def workload(a):
-    from fv3d import cases
-    if a.workload == "batch":
-        cs = cases.passage_batch(n=a.instances, seed=a.batch_seed, design_rpm=a.batch_rpm, max_iter=a.steps)
-        ncell = sum((bi.im - 1) * (bi.jm - 1) * (bi.km - 1) for c in cs for bi in c.blocks)
-        name = (f"{a.instances} independent five-block blade passages in one batched context ({ncell:,} cells; "
-                f"sizes log-uniform in [0.1M, 2M] cells, seed {a.batch_seed}; back-pressure factor U[0.95, 1.25] "
-                f"on 0.76 Pt; rotation U[0.7, 1.05] x {a.batch_rpm:g} rpm), Navier-Stokes + Spalart-Allmaras, "
-                "Roe + MUSCL3 primitive, LU-SGS local dt CFL 5, single grid level")
-        return cs, name, f"batch{a.instances}s{a.batch_seed}", True
-    if a.workload == "bump":
-        case = cases.bump_channel(ni=a.ni, nj=a.nj, nk=a.nk, seed=1, max_iter=a.steps, viscous=a.viscous,
-                                  len_ref=1e-4 if a.viscous else 1.0)
-        name = f"{'laminar Navier-Stokes' if a.viscous else 'inviscid'} bump channel {a.nk}x{a.nj}x{a.ni} cells " \
-               f"(single block), Roe + MUSCL3 primitive, LU-SGS local dt CFL 5"
-        key = f"bump{a.nk}x{a.nj}x{a.ni}" + ("ns" if a.viscous else "")
-        return case, name, key, a.viscous
-    case = cases.annular_passage(ni=64, nj=64, nk_blocks=(12, 40, 128, 60, 24), seed=1, rpm=0.0, blade_blocks=(2,),
-                                 camber=0.1, viscous=True, len_ref=1.0, max_iter=a.steps,
-                                 z_extent=(-0.06, -0.035, -0.015, 0.035, 0.055, 0.08))
-    case.cfg.turbulence = 100
-    if a.turbulence == "sa":
-        # nu~ = ev_ratio (the restart field of the single grid; fv3d_initial's with two levels)
-        case = cases.with_sa(case, amp=0.0)
-    levels = "single grid level"
-    if not a.single_grid:
-        # two-grid cycle from the uniform start field on the coarse level
-        case.cfg.use_multigrid, case.cfg.reuse_solution, case.cfg.initial = "Y", "N", "U"
-        case.cfg.coarse_iter = a.coarse_iter
-        if a.warmup <= a.coarse_iter:
-            raise SystemExit("--warmup must exceed --coarse-iter so that only cycle iterations are timed")
-        levels = (f"two-grid cycle (coarse level 135,168 cells; {a.coarse_iter} coarse start-up iterations from "
-                  f"the uniform start, then fine LU-SGS + forced coarse LU-SGS per iteration)")
-    name = ("five-block annular blade passage, 64x64 cross-section, k-blocks 12/40/128/60/24 (1,081,344 cells; "
-            "block sizes 49k-524k), hub 0.2 m tip 0.25 m, 22 blades, cambered zero-thickness blade walls, "
-            "periodic pitch and interior patches, Navier-Stokes + "
-            f"{'Spalart-Allmaras' if a.turbulence == 'sa' else 'Baldwin-Lomax'}, Roe + MUSCL3 primitive, LU-SGS "
-            f"local dt CFL 5, {levels}, stationary")
-    return case, name, "passage5" + ("" if a.single_grid else "mg") + ("sa" if a.turbulence == "sa" else ""), True
-
-
-def time_form(case, precision, a):
-    """Whole-job throughput of one form and its per-kernel-class device times."""
+def load_closure(relax: float = 1.0):
     import numpy as np
+    if not os.path.exists(CLOSURE_WEIGHTS):
+        return None
+    z = np.load(CLOSURE_WEIGHTS)
+    return {"params": z["params"].astype(np.float32), "bounds": z["bounds"].astype(np.float32),
+            "meta": json.loads(bytes(z["meta"]).decode()) if "meta" in z else {}, "relax": relax}
+
+
+def solver(case, precision, closure=None):
     from fv3d.solver import GpuSolver
-    dbg = {"plane": 2, "kba": 4, "kba2": 8, "kba3": 16, "rows": 0}[a.sweep]
-    g = GpuSolver(case, device=0, precision=PREC[precision], debug=dbg)
+    g = GpuSolver(case, device=0, precision=PREC[precision])
+    if closure is not None:
+        g.mlp_set(closure["params"], closure["bounds"])
+        g.mlp_closure(True)
+        g.mlp_relax(closure.get("relax", 1.0))
+    return g
+
+
+def time_form(case, precision, a, closure=None):
+    """Whole-job throughput of one form and its per-kernel device times."""
+    from fv3d.solver import GpuSolver
+    g = solver(case, precision, closure)
     graph = not a.no_graph
     tiles = None
     if a.no_autotune or a.warmup < 2:
@@ -189,201 +208,114 @@ def time_form(case, precision, a):
         g.iterate(1, graph=graph)
     g.sync()
     t0 = time.perf_counter()
-    hist = g.iterate(a.steps, graph=graph)
+    g.iterate(a.steps, graph=graph)
     g.sync()
     secs = time.perf_counter() - t0
     ncell = g.n_cells
-    # per-kernel device times from HIP events on the library's stream
+    # per-kernel device times from HIP events on the library's stream (the
+    # only stream its kernels run on), in a second, eager pass
     g.set_timing(True)
-    g.iterate(max(5, min(a.steps, 20)), graph=False)
+    g.iterate(max(5, min(a.steps, 10)), graph=False)
     ms, _ = g.timing()
     g.set_timing(False)
     g.close()
-    classes = ["residual", "lusgs", "update", "bc_spectral_dt", "finalize", "forward_sweep", "backward_sweep",
-               "coarse_correction"]
-    last = hist[-1, :, 1] if hist.ndim == 3 else hist[-1:, 1]
     return {"value": ncell * a.steps / secs, "ms_per_step": 1e3 * secs / a.steps, "cells": ncell,
-            "class_ms": {cl: ms[i] for i, cl in enumerate(classes)}, "final_res_dens": float(np.max(last)),
-            "tiles": tiles}
+            "class_ms": {cl: float(ms[i]) for i, cl in enumerate(SLOTS)}, "tiles": tiles}
 
 
-def time_alone(cases_, precision, a):
-    """The passages of a batch run one after the other, each in its own
-    context: whole-job throughput over the same iterations."""
-    from fv3d.solver import GpuSolver
-    total_s, cells = 0.0, 0
-    for c in cases_:
-        g = GpuSolver(c, device=0, precision=PREC[precision])
-        g.iterate(a.warmup, graph=True)
-        g.sync()
-        t0 = time.perf_counter()
-        g.iterate(a.steps, graph=True)
-        g.sync()
-        total_s += time.perf_counter() - t0
-        cells += g.n_cells
-        g.close()
-        print(f"alone: passage {cases_.index(c) + 1}/{len(cases_)} timed", file=sys.stderr,
-              flush=True)
-    return {"value": cells * a.steps / total_s, "seconds": total_s}
-
-
-def time_to_drop(case, precision, target, max_iter, skip=0, chunk=500):
-    """Wall time and iterations until the density residual falls to ``target``
-    times the first fine-level residual (iteration skip+1; the two-grid cycle's
-    first ``skip`` iterations run on the coarse level), iterations replayed
-    from HIP graphs; the host reads the history once per chunk."""
+def time_to_drop(case, precision, target, max_iter, closure=None, chunk=100):
+    """Wall time and iterations until Res.dens falls to ``target`` times the
+    first fine-level value (iteration coarse_iter + 1 with multigrid), with
+    graph replay; the host reads the history once per chunk.  Returns the
+    history row of the iteration that reached it."""
     import numpy as np
-    from fv3d.solver import GpuSolver
-    g = GpuSolver(case, device=0, precision=PREC[precision])
+    skip = case.cfg.coarse_iter if case.cfg.use_multigrid.upper() == "Y" else 0
+    g = solver(case, precision, closure)
     h = g.iterate(skip + 1, graph=True)   # captures the graphs; the last residual is the reference level
     r0 = float(h[-1, 1])
     g.sync()
     t0 = time.perf_counter()
-    it, hit = skip + 1, None
+    it, hit, row = skip + 1, None, None
     while it < max_iter and hit is None:
         h = g.iterate(chunk, graph=True)
         k = np.nonzero(h[:, 1] <= target * r0)[0]
         if len(k):
             hit = it + int(k[0]) + 1
+            row = h[k[0]]
         it += chunk
     g.sync()
     secs = time.perf_counter() - t0
+    tripped = g.mlp_tripped() if closure is not None else None
     g.close()
-    return {"iterations": hit, "seconds": secs if hit else None, "res0": r0,
+    val = lambda col: float(row[col]) if row is not None else None
+    return {"iterations": hit, "seconds": secs if hit else None, "res0": r0, "mass_in": val(2), "mass_out": val(3),
+            "pressure_ratio": val(4), "efficiency": val(5), "gate_fallbacks": tripped,
             "note": None if hit else f"not reached in {max_iter} iterations"}
 
 
-def drop_pair(case, forms, target, max_iter, skip, label):
-    d = {p: time_to_drop(case, p, target, max_iter, skip) for p in forms}
-    d["workload"] = label
-    if "fp32" in d and "fp64" in d and d["fp32"]["seconds"] and d["fp64"]["seconds"]:
-        d["speedup_chosen_vs_plain"] = d["fp64"]["seconds"] / d["fp32"]["seconds"]
-    return d
+def traffic_from_profiles(kernel: str, wkey: str):
+    """HBM bytes per launch of a kernel on a workload, from the 'traffic:'
+    lines of the rocprofv3 summaries in profiles/ (FETCH_SIZE and WRITE_SIZE,
+    copy-calibrated)."""
+    if not os.path.isdir(PROFILES):
+        return None
+    for name in sorted(os.listdir(PROFILES)):
+        if not name.endswith(".md"):
+            continue
+        with open(os.path.join(PROFILES, name)) as f:
+            for line in f:
+                # traffic: <kernel> <workload> <bytes>
+                p = line.strip().strip("`").split()
+                if len(p) == 4 and p[0] == "traffic:" and p[1] == kernel and p[2] == wkey:
+                    return float(p[3])
+    return None
 
 
-def closure_eval(a):
-    """The learned closure against Spalart-Allmaras: 56 passages of one size
-    (seeded back pressures) develop under Spalart-Allmaras in one batch; the
-    network is trained on the GPU on the fields of 48 of them; the other 8
-    continue from the state at the split once with the closure and once with
-    Spalart-Allmaras, and their mass flow, pressure ratio and efficiency are
-    compared, with the time per iteration of both."""
-    import numpy as np
-    from fv3d import cases, closure
-    from fv3d.solver import GpuSolver
-    cs = cases.passage_batch(n=56, seed=a.batch_seed + 1, min_cells=a.closure_cells, max_cells=a.closure_cells,
-                             design_rpm=0.0)
-    train, held = list(range(48)), list(range(48, 56))
-    g = GpuSolver(cs, precision=1)
-    g.iterate(a.closure_develop, graph=True)
-    g.sync()
-    split = []   # per passage: restart state at the split
-    for q, c in enumerate(cs):
-        st = []
-        for b in range(1, len(c.blocks) + 1):
-            pv = g.get(b, "pv", inst=q + 1)[1:-1, 1:-1, 1:-1]
-            st.append(dict(rho=pv[..., 0], u=pv[..., 1], v=pv[..., 2], w=pv[..., 3], p=pv[..., 4], T=pv[..., 5],
-                           rhoE=g.get(b, "cv", inst=q + 1)[1:-1, 1:-1, 1:-1, 4],
-                           nut=g.get(b, "ev", inst=q + 1)[1:-1, 1:-1, 1:-1]))
-        split.append(st)
-    print(f"closure: {len(cs)} passages developed {a.closure_develop} iterations", file=sys.stderr, flush=True)
-    g.iterate(a.closure_iters, graph=True)   # the training passages run on to a developed state
-    g.sync()
-    xs, ts = [], []
-    for q in train:
-        for b in range(1, len(cs[q].blocks) + 1):
-            xs.append(g.mlp_features(b, inst=q + 1).reshape(-1, closure.N_IN))
-            ts.append(g.mlp_targets(b, inst=q + 1).ravel())
-    x, t = np.concatenate(xs).astype(np.float32), np.concatenate(ts).astype(np.float32)
-    perm = np.random.default_rng(a.batch_seed).permutation(len(x))
-    x, t = x[perm], t[perm]
-    bounds = closure.gate_bounds(x, t)
-    g.mlp_set(closure.init_params(a.batch_seed), bounds)
-    t0 = time.perf_counter()
-    steps = 3000
-    hist = g.mlp_train(x, t, steps, 4096, 2e-3)
-    train_s = time.perf_counter() - t0
-    P = g.mlp_get()
-    g.close()
-    print(f"closure: trained on {len(x):,} cells, loss {hist[0]:.4f} -> {hist[-1]:.5f}", file=sys.stderr, flush=True)
-    heldc = []
-    for q in held:
-        c = cs[q]
-        for st, s2 in zip(c.states, split[q]):
-            st.update(s2)
-        heldc.append(c)
-    res = {}
-    for mode in ("sa", "closure"):
-        h = GpuSolver(heldc, precision=1)
-        if mode == "closure":
-            h.mlp_set(P, bounds)
-            h.mlp_closure(True)
-        h.iterate(5, graph=True)
-        h.sync()
-        t0 = time.perf_counter()
-        hh = h.iterate(a.closure_iters - 5, graph=True)
-        h.sync()
-        secs = time.perf_counter() - t0
-        res[mode] = {"final": hh[-1], "ms_per_iter": 1e3 * secs / (a.closure_iters - 5),
-                     "tripped": h.mlp_tripped() if mode == "closure" else 0, "cells": h.n_cells}
-        h.close()
-    sa, cl = res["sa"]["final"], res["closure"]["final"]
-    rel = lambda col: np.abs(cl[:, col] - sa[:, col]) / np.abs(sa[:, col])
-    out = {
-        "metric": "learned closure vs Spalart-Allmaras on held-out passages",
-        "config": {"workload": f"56 five-block blade passages of {int(a.closure_cells):,} cells (seed {a.batch_seed + 1}), "
-                               f"back-pressure factor U[0.95, 1.25] on 0.76 Pt, stationary; Spalart-Allmaras for "
-                               f"{a.closure_develop} iterations, then {a.closure_iters} more; trained on 48, "
-                               "evaluated on 8 continued from the split", "precision": "FP32 operator, FP64 residual"},
-        "training": {"samples": int(len(x)), "steps": steps, "batch": 4096, "loss_first": float(hist[0]),
-                     "loss_last": float(hist[-1]), "seconds": train_s},
-        "held_out": {"mass_in_rel_err_max": float(rel(2).max()), "mass_in_rel_err_mean": float(rel(2).mean()),
-                     "pr_rel_err_max": float(rel(4).max()), "eff_rel_err_max": float(rel(5).max()),
-                     "gate_fallbacks": int(res["closure"]["tripped"])},
-        "ms_per_iteration": {"sa": res["sa"]["ms_per_iter"], "closure": res["closure"]["ms_per_iter"],
-                             "cells": res["sa"]["cells"]},
-    }
-    print(json.dumps(out))
+def roofline(r, precision, sa_transport, wkey):
+    byt = kernel_bytes(precision, sa_transport)
+    ks = {}
+    for k, b in byt.items():
+        ms = r["class_ms"].get(k, 0.0)
+        if ms > 0:
+            ach = b * r["cells"] / (ms * 1e-3) / 1e9
+            ks[k] = {"kernel": KERNEL_NAMES[k], "ms": ms, "bytes_per_cell": b, "achieved": ach,
+                     "frac": ach / HBM_PEAK_GBS}
+    dom = max(ks, key=lambda k: ks[k]["ms"])
+    d = ks[dom]
+    return {"bound": "hbm", "kernel": d["kernel"], "achieved": d["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": d["frac"], "traffic": traffic_from_profiles(dom, wkey), "kernel_ms_avg": d["ms"],
+            "bytes_per_cell": d["bytes_per_cell"], "dominant_by": "per-launch device time in this run",
+            "kernels": ks, "class_ms": r["class_ms"]}
 
 
-def main():
-    a = parse()
-    if a.workload == "closure":
-        closure_eval(a)
-        return
-    if a.calibrate_copy:
-        calibrate_copy()
-        return
-    if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
-                          "n_gpus": a.gpus}))
-        sys.exit(2)
-    case, wname, wkey, viscous = workload(a)
-    forms = ["fp32", "fp64"] if a.precision == "both" else [a.precision]
-    res = {p: time_form(case, p, a) for p in forms}
-    chosen = "fp32" if "fp32" in res else "fp64"
-    r = res[chosen]
-    # dominant kernel: the forward sweep of the chosen form, timed alone with HIP
-    # events on the library's stream (the only stream its kernels run on)
-    first = case[0] if isinstance(case, list) else case
-    words = SWEEP_WORDS[viscous] + (SWEEP_WORDS_SA if first.cfg.turbulence == 200 else 0)
-    kbytes = words * (4 if chosen == "fp32" else 8)
-    kms = r["class_ms"]["forward_sweep"]
-    achieved = kbytes * r["cells"] / (kms * 1e-3) / 1e9
-    drop = drop_w = alone = None
-    if a.workload == "batch" and a.compare_alone:
-        alone = time_alone(case, chosen, a)
-    if not a.no_drop and a.workload != "batch":
-        from fv3d import cases
-        c0 = cases.bump_channel(ni=32, nj=32, nk=96, seed=1, max_iter=a.drop_max_iter)
-        drop = drop_pair(c0, forms, 1e-6, a.drop_max_iter, 0,
-                         "C0 bump channel 96x32x32 (98,304 cells), inviscid, Roe + MUSCL3, CFL 5 ramp, single grid")
-        if a.workload == "passage5":
-            # the Baldwin-Lomax passage settles into a limit cycle near 5e-5 of
-            # its first residual in both forms, so its drop is timed to 1e-4
-            skip = 0 if a.single_grid else a.coarse_iter
-            drop_w = drop_pair(case, forms, 1e-4, a.passage_drop_max_iter, skip, wname)
+def c3_main(a):
+    from fv3d import cases
+    closure = load_closure(a.closure_relax) if a.closure else None
+    plain = cases.vane_passage(levels=0, max_iter=a.drop_max_iter)
+    chosen = cases.vane_passage(levels=a.levels, max_iter=a.drop_max_iter)
+    cells = plain.n_cells
+    wname = (f"vane passage 88x88x(24,48,144,72,40) ({cells:,} cells; blocks 0.19M-1.12M), hub 0.2 m tip 0.25 m, "
+             "22 blades, turning vane theta = 0.1 pitch t^2, Navier-Stokes + Spalart-Allmaras, Roe + MUSCL3 "
+             "primitive, LU-SGS local dt CFL 5, outlet 0.95 Pt, stationary")
+    r = time_form(chosen, "fp32", a, closure)
+    print(f"bench: chosen form timed, {r['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    p = time_form(plain, "fp64", a)
+    print(f"bench: plain FP64 timed, {p['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    drop = None
+    if not a.no_drop:
+        dc = time_to_drop(chosen, "fp32", 1e-6, a.drop_max_iter, closure)
+        print(f"bench: chosen form drop {dc['iterations']} iterations, {dc['seconds']} s", file=sys.stderr, flush=True)
+        dp = time_to_drop(plain, "fp64", 1e-6, a.drop_max_iter)
+        print(f"bench: plain FP64 drop {dp['iterations']} iterations, {dp['seconds']} s", file=sys.stderr, flush=True)
+        drop = {"chosen": dc, "plain_fp64": dp, "workload": "the headline workload, from the uniform start"}
+        if dc["seconds"] and dp["seconds"]:
+            drop["speedup_chosen_vs_plain"] = dp["seconds"] / dc["seconds"]
+            for q in ("mass_in", "pressure_ratio", "efficiency"):
+                drop[q + "_rel_err"] = abs(dc[q] - dp[q]) / abs(dp[q])
     cpu = None if a.no_cpu_baseline else cpu_baseline(a.cpu_iters)
+    chosen_desc = (f"FP32 defect correction + {a.levels}-level V-cycle + one launch per stage over all blocks in a "
+                   "HIP graph, autotuned tiles" + (" + learned closure (Spalart-Allmaras fallback)" if closure else
+                                                  " + Spalart-Allmaras transport (no trained closure present)"))
     out = {
         "metric": METRIC,
         "value": r["value"],
@@ -395,26 +327,179 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 residual/update, f32 LU-SGS operator" if chosen == "fp32" else "f64",
+        "dtype": "f64 residual/update/norms, f32 LU-SGS operator and coarse corrections",
         "data": "synthetic",
-        "config": {"workload": wname, "cells": r["cells"], "hip_graph": not a.no_graph, "sweep": a.sweep,
-                   "chosen_form": "FP32 defect correction" if chosen == "fp32" else "plain FP64"},
-        "plain_fp64": ({"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
-                        "class_ms": res["fp64"]["class_ms"]} if "fp64" in res else None),
+        "config": {"workload": wname, "cells": cells, "chosen_form": chosen_desc,
+                   "plain_form": "plain FP64, the reference's two-grid cycle, Spalart-Allmaras transport",
+                   "hip_graph": not a.no_graph, "tiles": r["tiles"],
+                   "closure": closure["meta"] if closure else None},
+        "plain_fp64": {"value": p["value"], "ms_per_step": p["ms_per_step"], "class_ms": p["class_ms"]},
         "time_to_1e-6_drop": drop,
-        "time_to_1e-4_drop_workload": drop_w,
-        "roofline": {"bound": "hbm", "kernel": f"k_lusgs_sl<forward, {chosen}> (LU-SGS forward sweep)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": MEASURED_FWD_SWEEP_TRAFFIC.get((wkey, chosen)), "kernel_ms_avg": kms,
-                     "bytes_per_cell": kbytes, "class_ms": r["class_ms"]},
+        "roofline": roofline(r, "fp32", closure is None, "c3"),
+        "roofline_plain_fp64": roofline(p, "fp64", True, "c3"),
         "cpu_baseline": cpu,
-        "final_res_dens": r["final_res_dens"],
-        "tiles": r["tiles"],
     }
-    if alone:
-        out["batch_vs_alone"] = {"alone_value": alone["value"], "alone_seconds": alone["seconds"],
-                                 "speedup": r["value"] / alone["value"]}
     print(json.dumps(out))
+
+
+def c2_main(a):
+    from fv3d import cases
+    cs = cases.passage_batch(n=a.instances, seed=a.batch_seed, design_rpm=a.batch_rpm, max_iter=a.steps)
+    ncell = sum(c.n_cells for c in cs)
+    wname = (f"{a.instances} independent five-block blade passages in one batched context ({ncell:,} cells; "
+             f"sizes log-uniform in [0.1M, 2M] cells, seed {a.batch_seed}; back-pressure factor U[0.95, 1.25] "
+             f"on 0.76 Pt; rotation U[0.7, 1.05] x {a.batch_rpm:g} rpm), Navier-Stokes + Spalart-Allmaras, "
+             "Roe + MUSCL3 primitive, LU-SGS local dt CFL 5, single grid level")
+    r = time_form(cs, "fp32", a)
+    out = {"metric": "cell-updates/s of the batched passages, FP32 LU-SGS over FP64 residual", "value": r["value"],
+           "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64 residual/update, f32 LU-SGS operator", "data": "synthetic",
+           "config": {"workload": wname, "cells": ncell, "tiles": r["tiles"]},
+           "roofline": roofline(r, "fp32", True, f"c2_{a.instances}s{a.batch_seed}")}
+    if a.compare_alone:
+        total, cells = 0.0, 0
+        for q, c in enumerate(cs):
+            g = solver(c, "fp32")
+            g.iterate(a.warmup, graph=True)
+            g.sync()
+            t0 = time.perf_counter()
+            g.iterate(a.steps, graph=True)
+            g.sync()
+            total += time.perf_counter() - t0
+            cells += g.n_cells
+            g.close()
+            print(f"alone: passage {q + 1}/{len(cs)} timed", file=sys.stderr, flush=True)
+        alone = cells * a.steps / total
+        out["batch_vs_alone"] = {"alone_value": alone, "speedup": r["value"] / alone}
+    print(json.dumps(out))
+
+
+def legacy_main(a):
+    """The 1.08M-cell Baldwin-Lomax passage (two-grid) or the bump channel,
+    FP32 defect correction against plain FP64."""
+    from fv3d import cases
+    if a.workload == "bump":
+        case = cases.bump_channel(ni=a.ni, nj=a.nj, nk=a.nk, seed=1, max_iter=a.steps, viscous=a.viscous,
+                                  len_ref=1e-4 if a.viscous else 1.0)
+        wname = f"{'laminar' if a.viscous else 'inviscid'} bump channel {a.nk}x{a.nj}x{a.ni} cells"
+        wkey = f"bump{a.nk}x{a.nj}x{a.ni}" + ("ns" if a.viscous else "")
+    else:
+        case = cases.annular_passage(ni=64, nj=64, nk_blocks=(12, 40, 128, 60, 24), seed=1, rpm=0.0,
+                                     blade_blocks=(2,), camber=0.1, viscous=True, len_ref=1.0, max_iter=a.steps,
+                                     z_extent=(-0.06, -0.035, -0.015, 0.035, 0.055, 0.08))
+        case.cfg.turbulence = 100
+        case.cfg.use_multigrid, case.cfg.reuse_solution, case.cfg.initial, case.cfg.coarse_iter = "Y", "N", "U", 2
+        wname = ("five-block annular blade passage 64x64x(12,40,128,60,24) (1,081,344 cells), circular-arc blade, "
+                 "Navier-Stokes + Baldwin-Lomax, two-grid cycle, stationary")
+        wkey = "passage5mg"
+    res = {pr: time_form(case, pr, a) for pr in ("fp32", "fp64")}
+    r = res["fp32"]
+    out = {"metric": METRIC, "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64 residual/update, f32 LU-SGS operator", "data": "synthetic",
+           "config": {"workload": wname, "cells": r["cells"], "tiles": r["tiles"]},
+           "plain_fp64": {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"]},
+           "roofline": roofline(r, "fp32", False, wkey)}
+    print(json.dumps(out))
+
+
+def closure_train(a):
+    """The closure's training set and held-out evaluation on the vane family:
+    ``train_instances`` passages of 40 x 40 x (16, 24, 64, 32, 16) cells
+    (243,200 each) at back-pressure ratios drawn uniform in [0.93, 0.97] of Pt
+    (which sets the through-flow speed), developed with Spalart-Allmaras in one
+    batched context on the three-level V-cycle; the network is trained on the
+    GPU on the mue_t fields of all but the last 8; the last 8 are then run
+    from the uniform start to a 1e-6 drop once with the closure and once with
+    Spalart-Allmaras, and mass flow, pressure ratio and efficiency are
+    compared.  Writes the network to fv3d/closure_vane.npz."""
+    import numpy as np
+    from fv3d import cases, closure
+    rng = np.random.default_rng(a.batch_seed)
+    n = a.train_instances
+    prs = rng.uniform(0.93, 0.97, n)
+
+    def fam(q, levels=3):
+        return cases.vane_passage(ni=40, nk_blocks=(16, 24, 64, 32, 16), seed=100 + q, levels=levels,
+                                  p_ratio=float(prs[q]), max_iter=100000)
+
+    train, held = list(range(n - 8)), list(range(n - 8, n))
+    t0 = time.perf_counter()
+    g = solver([fam(q) for q in train], "fp32")
+    g.iterate(a.train_develop, graph=True)
+    g.sync()
+    dev_s = time.perf_counter() - t0
+    print(f"closure: developed {len(train)} passages in {dev_s:.1f} s", file=sys.stderr, flush=True)
+    xs, ts = [], []
+    for q in range(len(train)):
+        for b in range(1, 6):
+            xs.append(g.mlp_features(b, inst=q + 1).reshape(-1, closure.N_IN))
+            ts.append(g.mlp_targets(b, inst=q + 1).ravel())
+    x, t = np.concatenate(xs).astype(np.float32), np.concatenate(ts).astype(np.float32)
+    perm = np.random.default_rng(a.batch_seed + 1).permutation(len(x))
+    x, t = x[perm], t[perm]
+    bounds = closure.gate_bounds(x, t)
+    g.mlp_set(closure.init_params(a.batch_seed), bounds)
+    t1 = time.perf_counter()
+    steps = 4000
+    hist = g.mlp_train(x, t, steps, 4096, 2e-3)
+    train_s = time.perf_counter() - t1
+    P = g.mlp_get()
+    g.close()
+    print(f"closure: trained on {len(x):,} cells in {train_s:.1f} s, loss {hist[0]:.4f} -> {hist[-1]:.5f}",
+          file=sys.stderr, flush=True)
+    model = {"params": P, "bounds": bounds, "meta": {}, "relax": a.closure_relax}
+    if a.no_heldout:
+        np.savez(a.closure_out or CLOSURE_WEIGHTS, params=P, bounds=bounds)
+        print(json.dumps({"training_loss": [float(hist[0]), float(hist[-1])], "samples": int(len(x))}))
+        return
+    res = {}
+    for mode in ("sa", "closure"):
+        rows = []
+        for q in held:
+            d = time_to_drop(fam(q), "fp32", 1e-6, 12000, model if mode == "closure" else None)
+            rows.append(d)
+            print(f"closure: held-out {q} {mode}: {d['iterations']} iterations", file=sys.stderr, flush=True)
+        res[mode] = rows
+    rel = lambda k: [abs(c[k] - s[k]) / abs(s[k]) if c[k] is not None and s[k] is not None else None
+                     for c, s in zip(res["closure"], res["sa"])]
+    meta = {"family": "vane passage 40x40x(16,24,64,32,16), 243,200 cells, back pressure U[0.93, 0.97] Pt, "
+                      f"seed {a.batch_seed}", "train_passages": len(train), "samples": int(len(x)), "steps": steps,
+            "batch": 4096, "lr": 2e-3, "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
+            "develop_iterations": a.train_develop}
+    np.savez(a.closure_out or CLOSURE_WEIGHTS, params=P, bounds=bounds,
+             meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+    out = {"metric": "learned closure vs Spalart-Allmaras on held-out passages (converged to a 1e-6 drop)",
+           "training": meta, "develop_seconds": dev_s, "train_seconds": train_s,
+           "held_out": {"pressure_ratios": [float(prs[q]) for q in held],
+                        "mass_in_rel_err": rel("mass_in"), "pr_rel_err": rel("pressure_ratio"),
+                        "eff_rel_err": rel("efficiency"),
+                        "iterations_sa": [d["iterations"] for d in res["sa"]],
+                        "iterations_closure": [d["iterations"] for d in res["closure"]],
+                        "seconds_sa": [d["seconds"] for d in res["sa"]],
+                        "seconds_closure": [d["seconds"] for d in res["closure"]],
+                        "gate_fallbacks": [d["gate_fallbacks"] for d in res["closure"]]}}
+    print(json.dumps(out))
+
+
+def main():
+    a = parse()
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    if a.workload == "c3":
+        c3_main(a)
+    elif a.workload == "c2":
+        c2_main(a)
+    elif a.workload == "closure-train":
+        closure_train(a)
+    else:
+        legacy_main(a)
 
 
 if __name__ == "__main__":

@@ -224,10 +224,13 @@ int fv3d_sync(fv3d_ctx *ctx);
  * scatter of the default sweep form), 2 update, 3 ghost fill + spectral + time
  * step, 4 norms/integrals, 5 forward sweep kernel, 6 backward sweep kernel
  * (5 and 6 for the default form only), 7 coarse-grid correction (second fine
- * residual, restriction, coarse LU-SGS, prolongation; two-grid cycle only).
+ * residual, restriction, coarse LU-SGS, prolongation; multigrid only),
+ * 8 convective residual kernel, 9 viscous and nu~ residual kernels, 10 ghost
+ * fill and patch exchange, 11 turbulence model (vorticity, Baldwin-Lomax,
+ * Spalart-Allmaras or the learned closure).
  * ms and counts hold FV3D_TIMING_SLOTS entries.  Filled by fv3d_iterate when
  * timing is on (fine-level iterations only). */
-#define FV3D_TIMING_SLOTS 8
+#define FV3D_TIMING_SLOTS 12
 int fv3d_set_timing(fv3d_ctx *ctx, int on);
 int fv3d_timing(fv3d_ctx *ctx, double *ms, long *counts);
 
@@ -271,7 +274,11 @@ int fv3d_autotune(fv3d_ctx *ctx, int reps, int *shape3);
  * samples (a multiple of 64), loss_hist[steps] (may be NULL).
  * fv3d_mlp_closure(on): mue_t from the network each residual evaluation and no
  * nu~ equation; a cell outside the gate keeps the Spalart-Allmaras value of its
- * frozen nu~.  fv3d_mlp_tripped: cells that fell back since the last call. */
+ * frozen nu~.  fv3d_mlp_relax(omega): mue_t moves by omega (0 < omega <= 1,
+ * default 1) of the way from its previous value to the closure's each
+ * evaluation (no reference counterpart: the closure is algebraic, and damping
+ * its coupling to the flow leaves the converged state unchanged).
+ * fv3d_mlp_tripped: cells that fell back since the last call. */
 int fv3d_mlp_set(fv3d_ctx *ctx, const float *params, const float *bounds);
 int fv3d_mlp_get(fv3d_ctx *ctx, float *params);
 long fv3d_mlp_features(fv3d_ctx *ctx, int block, float *out);
@@ -283,6 +290,7 @@ int fv3d_mlp_grad(fv3d_ctx *ctx, const float *x, const float *t, long n, float *
 int fv3d_mlp_train(fv3d_ctx *ctx, const float *x, const float *t, long n, int steps, int batch, float lr,
                    float *loss_hist);
 int fv3d_mlp_closure(fv3d_ctx *ctx, int on);
+int fv3d_mlp_relax(fv3d_ctx *ctx, double omega);
 long fv3d_mlp_tripped(fv3d_ctx *ctx);
 
 /* Number of interior cells over all blocks. */
