@@ -110,6 +110,7 @@ def parse():
     ap.add_argument("--viscous", action="store_true", help="bump: laminar Navier-Stokes")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-drop", action="store_true", help="skip the time-to-1e-6-drop runs")
+    ap.add_argument("--no-plain", action="store_true", help="c3: time the chosen form only")
     ap.add_argument("--drop-max-iter", type=int, default=20000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=8)
@@ -299,16 +300,20 @@ def c3_main(a):
              "primitive, LU-SGS local dt CFL 5, outlet 0.95 Pt, stationary")
     r = time_form(chosen, "fp32", a, closure)
     print(f"bench: chosen form timed, {r['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
-    p = time_form(plain, "fp64", a)
-    print(f"bench: plain FP64 timed, {p['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    p = None
+    if not a.no_plain:
+        p = time_form(plain, "fp64", a)
+        print(f"bench: plain FP64 timed, {p['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     drop = None
     if not a.no_drop:
         dc = time_to_drop(chosen, "fp32", 1e-6, a.drop_max_iter, closure)
         print(f"bench: chosen form drop {dc['iterations']} iterations, {dc['seconds']} s", file=sys.stderr, flush=True)
-        dp = time_to_drop(plain, "fp64", 1e-6, a.drop_max_iter)
-        print(f"bench: plain FP64 drop {dp['iterations']} iterations, {dp['seconds']} s", file=sys.stderr, flush=True)
+        dp = None
+        if not a.no_plain:
+            dp = time_to_drop(plain, "fp64", 1e-6, a.drop_max_iter)
+            print(f"bench: plain FP64 drop {dp['iterations']} iterations, {dp['seconds']} s", file=sys.stderr, flush=True)
         drop = {"chosen": dc, "plain_fp64": dp, "workload": "the headline workload, from the uniform start"}
-        if dc["seconds"] and dp["seconds"]:
+        if dp and dc["seconds"] and dp["seconds"]:
             drop["speedup_chosen_vs_plain"] = dp["seconds"] / dc["seconds"]
             for q in ("mass_in", "pressure_ratio", "efficiency"):
                 drop[q + "_rel_err"] = abs(dc[q] - dp[q]) / abs(dp[q])
@@ -333,10 +338,10 @@ def c3_main(a):
                    "plain_form": "plain FP64, the reference's two-grid cycle, Spalart-Allmaras transport",
                    "hip_graph": not a.no_graph, "tiles": r["tiles"],
                    "closure": closure["meta"] if closure else None},
-        "plain_fp64": {"value": p["value"], "ms_per_step": p["ms_per_step"], "class_ms": p["class_ms"]},
+        "plain_fp64": {"value": p["value"], "ms_per_step": p["ms_per_step"], "class_ms": p["class_ms"]} if p else None,
         "time_to_1e-6_drop": drop,
         "roofline": roofline(r, "fp32", closure is None, "c3"),
-        "roofline_plain_fp64": roofline(p, "fp64", True, "c3"),
+        "roofline_plain_fp64": roofline(p, "fp64", True, "c3") if p else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -458,7 +463,7 @@ def closure_train(a):
     for mode in ("sa", "closure"):
         rows = []
         for q in held:
-            d = time_to_drop(fam(q), "fp32", 1e-6, 12000, model if mode == "closure" else None)
+            d = time_to_drop(fam(q), "fp32", 1e-6, 6000, model if mode == "closure" else None)
             rows.append(d)
             print(f"closure: held-out {q} {mode}: {d['iterations']} iterations", file=sys.stderr, flush=True)
         res[mode] = rows
