@@ -1532,7 +1532,10 @@ static int launch_bc(fv3d_ctx *c)
     return FV3D_OK;
 }
 
-static int launch_spectral_dt(fv3d_ctx *c)
+// CFluidProps::SpectralRadii and CTimeDiscr::TimeStep (fused): lambda_c,
+// lambda_v, dt and the LU-SGS diagonals; with decrement the Flag_low_dt
+// countdown at the end of TimeStep (source/TimeStep.cpp:89-92) runs too.
+static int launch_spectral_dt(fv3d_ctx *c, bool decrement = true)
 {
     hipLaunchKernelGGL(k_spectral_dt, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
                        c->d_cstart, (int)c->blocks.size(), c->gas);
@@ -1546,6 +1549,7 @@ static int launch_spectral_dt(fv3d_ctx *c)
         hipLaunchKernelGGL(k_dt_global, dim3(g), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb, c->gas, c->d_dtmin);
         HIPCK(c, hipGetLastError());
     }
+    if (!decrement) return FV3D_OK;
     hipLaunchKernelGGL(k_dt_flag_decrement, dim3(1), dim3(256), 0, c->stream, c->d_blocks, (int)c->blocks.size());
     HIPCK(c, hipGetLastError());
     return FV3D_OK;
@@ -1871,11 +1875,15 @@ static int launch_finalize(fv3d_ctx *c, double *hist_row)
     } while (0)
 
 int fv3d_bc_apply(fv3d_ctx *c) { READY(c); return launch_bc(c); }
-int fv3d_spectral(fv3d_ctx *c) { READY(c); return launch_spectral_dt(c); }
-int fv3d_timestep(fv3d_ctx *c) { READY(c); return FV3D_OK; }   // fused into fv3d_spectral
+// SpectralRadii and TimeStep share one kernel: both calls form lambda and dt
+// (idempotent), and only fv3d_timestep counts the low-dt damping down, once,
+// as the reference's TimeStep does.
+int fv3d_spectral(fv3d_ctx *c) { READY(c); return launch_spectral_dt(c, false); }
+int fv3d_timestep(fv3d_ctx *c) { READY(c); return launch_spectral_dt(c, true); }
 int fv3d_convective(fv3d_ctx *c) { READY(c); return launch_convective(c); }
 int fv3d_viscous(fv3d_ctx *c) { READY(c); return launch_viscous(c); }
 int fv3d_curl(fv3d_ctx *c) { READY(c); return launch_curl(c); }
+int fv3d_turbulence(fv3d_ctx *c) { READY(c); return launch_turbulence(c); }
 
 int fv3d_resflux(fv3d_ctx *c)
 {
@@ -2125,6 +2133,7 @@ static int mg_phase(const fv3d_ctx *c, long it)
     return it < c->cfg.coarse_iter ? kMgCoarse : it == c->cfg.coarse_iter ? kMgSwitch : kMgCycle;
 }
 
+
 static int ensure_hist(fv3d_ctx *c, long n)
 {
     if (n <= c->hist_cap) return FV3D_OK;
@@ -2135,6 +2144,95 @@ static int ensure_hist(fv3d_ctx *c, long n)
     if (rc) return rc;
     c->hist_cap = n;
     return FV3D_OK;
+}
+
+// ------------------------------------------------- two-grid cycle, by stage
+// The stages of one CSolver::Solver iteration (source/Solver.cpp:61-103) as
+// separate calls, for a caller that drives the schedule itself; the same
+// kernels as fv3d_iterate, so the same bits.
+#define TWO_LEVEL(c)                                                                                          \
+    do {                                                                                                      \
+        READY(c);                                                                                             \
+        if ((c)->ng != 2 || (c)->coarse->coarse)                                                              \
+            return fail((c), FV3D_ESTATE, "the stage calls drive the two-grid cycle (two levels in use: %d)", \
+                        (c)->nlev);                                                                           \
+    } while (0)
+
+int fv3d_mg_coarse_lusgs(fv3d_ctx *c, int forced)
+{
+    TWO_LEVEL(c);
+    fv3d_ctx *m = c->coarse;
+    int rc;
+    if (!forced) return lusgs_timed(m);
+    if ((rc = launch_sweeps(m))) return rc;
+    return launch_update(m);
+}
+
+int fv3d_mg_restrict(fv3d_ctx *c)
+{
+    TWO_LEVEL(c);
+    fv3d_ctx *m = c->coarse;
+    hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)((m->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
+                       m->d_blocks, m->d_cstart, (int)c->blocks.size(), m->gas, m->d_bad);
+    HIPCK(c, hipGetLastError());
+    return launch_limiter(m);
+}
+
+int fv3d_mg_correct(fv3d_ctx *c)
+{
+    TWO_LEVEL(c);
+    fv3d_ctx *m = c->coarse;
+    hipLaunchKernelGGL(k_mg_prolong<false>, dim3((unsigned)c->n_mg_tiles), dim3(256), 0, c->stream, c->d_blocks,
+                       m->d_blocks, c->d_mg_tiles, c->gas, c->d_bad);
+    HIPCK(c, hipGetLastError());
+    return launch_limiter(c);
+}
+
+int fv3d_mg_interpolate_state(fv3d_ctx *c)
+{
+    TWO_LEVEL(c);
+    fv3d_ctx *m = c->coarse;
+    hipLaunchKernelGGL(k_mg_corner, dim3((unsigned)c->blocks.size()), dim3(256), 0, c->stream, m->d_blocks);
+    hipLaunchKernelGGL(k_mg_prolong<true>, dim3((unsigned)c->n_mg_tiles), dim3(256), 0, c->stream, c->d_blocks,
+                       m->d_blocks, c->d_mg_tiles, c->gas, c->d_bad);
+    HIPCK(c, hipGetLastError());
+    return launch_limiter(c);
+}
+
+int fv3d_iteration_end(fv3d_ctx *c, double *row7)
+{
+    READY(c);
+    int rc;
+    if ((rc = ensure_hist(c, c->ninst))) return rc;
+    const long itn = c->iter_host + 1;
+    // the coarse start-up iterations report the coarse level (source/Solver.cpp:71-87)
+    fv3d_ctx *r = c->ng == 2 && mg_phase(c, itn) != kMgCycle ? c->coarse : c;
+    if ((rc = launch_finalize(r, c->d_hist))) return rc;
+    c->iter_host = itn;
+    if (row7) HIPCK(c, hipMemcpyAsync(row7, c->d_hist, 7L * c->ninst * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    if ((rc = check_sweep_error(c))) return rc;
+    if (c->coarse && (rc = check_sweep_error(c->coarse))) return fail(c, rc, "coarse level: %s", c->coarse->err.c_str());
+    return FV3D_OK;
+}
+
+int fv3d_primitive_vars(fv3d_ctx *c)
+{
+    READY(c);
+    hipLaunchKernelGGL(k_prim_all, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
+                       c->d_cstart, (int)c->blocks.size(), c->gas, c->d_bad);
+    HIPCK(c, hipGetLastError());
+    return FV3D_OK;
+}
+
+int fv3d_limit_vars(fv3d_ctx *c)
+{
+    READY(c);
+    if (!c->cfg.limiter) return FV3D_OK;
+    // every block is scanned (the flags of the iteration's kernels are not relied on)
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, c->stream, c->d_bad, (int)c->blocks.size(), 1);
+    HIPCK(c, hipGetLastError());
+    return launch_limiter(c);
 }
 
 static int accumulate_timing(fv3d_ctx *c)
@@ -2651,6 +2749,24 @@ long fv3d_level_field_download(fv3d_ctx *c, int level, int block, const char *na
     long r = fv3d_field_download(c->coarse, block, name, out);
     if (r < 0) fail(c, (int)r, "coarse level: %s", c->coarse->err.c_str());
     return r;
+}
+
+int fv3d_wall_distance_upload(fv3d_ctx *c, int block, const double *d)
+{
+    READY(c);
+    if (!d) return FV3D_EINVAL;
+    if (block < 1 || block > (int)c->blocks.size()) return fail(c, FV3D_EINVAL, "block %d", block);
+    const BlockDev &B = c->blocks[block - 1].dev;
+    if (!B.dwall) return fail(c, FV3D_EINVAL, "the wall distance exists with Spalart-Allmaras only");
+    std::vector<double> h((size_t)B.box);
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    HIPCK(c, hipMemcpy(h.data(), B.dwall, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    long q = 0;
+    for (int i = 1; i <= B.ni; ++i)
+        for (int j = 1; j <= B.nj; ++j)
+            for (int k = 1; k <= B.nk; ++k) h[cidx(B, i, j, k)] = d[q++];
+    HIPCK(c, hipMemcpy(B.dwall, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    return FV3D_OK;
 }
 
 long fv3d_field_download(fv3d_ctx *c, int block, const char *name, double *out)

@@ -133,6 +133,8 @@ constexpr int kMboxJ = kSweepD * kKbaTK, kMboxRow = kSweepD * (kKbaTK + kKbaTJ);
 constexpr unsigned long long kMboxEmpty = 0x7ff4dead5eed0001ull;
 constexpr unsigned int kMboxEmpty32 = 0x7fa5eed1u;   // the same for FP32 words
 __global__ void k_fill_u64(unsigned long long *p, long n, unsigned long long v);
+__global__ void k_fill_i32(int *p, int n, int v);
+__global__ void k_prim_all(const BlockDev *blocks, const long *cstart, int nblock, Gas g, int *bad_flag);
 // row storage in T (double, or float for the FP32 operator); unpack widens dQ to double
 template <class T>
 __global__ void k_sweep_fill(const BlockDev *blocks, int block, long nrows, int what);

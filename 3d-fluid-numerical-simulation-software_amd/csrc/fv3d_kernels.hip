@@ -1486,6 +1486,11 @@ __global__ void k_fill_u64(unsigned long long *p, long n, unsigned long long v)
     if (t < n) p[t] = v;
 }
 
+__global__ void k_fill_i32(int *p, int n, int v)
+{
+    for (int e = threadIdx.x; e < n; e += blockDim.x) p[e] = v;
+}
+
 __global__ void k_fill_int(int *p, long n, int v)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -26,6 +26,21 @@ __device__ static inline void prim_vars(const BlockDev &B, long x, const double 
     if (d < dmin || d > dmax || p < dmin || p > dmax || u > vmax || v > vmax || w > vmax) atomicOr(bad, 1);
 }
 
+// CFluidProps::PrimitiveVars over every interior cell (source/PrimitiveVars.cpp:
+// 7-29), with the limiter's bad-cell flags.
+__global__ void __launch_bounds__(256) k_prim_all(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
+                                                  int nblock, Gas g, int *__restrict__ bad_flag)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int n, i, j, k;
+    if (!cell_item(blocks, cstart, nblock, t, n, i, j, k)) return;
+    const BlockDev &B = blocks[n];
+    const long x = cidx(B, i, j, k);
+    double q[5];
+    for (int c = 0; c < 5; ++c) q[c] = B.Q[c * B.box + x];
+    prim_vars(B, x, q, g, bad_flag + n);
+}
+
 // CSolver::Prolong (fine to coarse, source/Prolong_Interpolate.cpp:7-50): the
 // volume-weighted state cv_2h = sum(Q V) / V_H, which becomes the coarse
 // state, and Res_2h = sum(R) of the eight fine cells, in the reference's order

@@ -75,18 +75,21 @@ void print_summary(const fv3d::CaseConfig &cc, const fv3d::BndTable &bnd, const 
 int main(int argc, char **argv)
 {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: %s <case_dir> [iterations] [--history FILE] [--eager] [--device N] [--fp32] [--parse-only]\n",
+        std::fprintf(stderr, "usage: %s <case_dir> [iterations] [--history FILE] [--eager] [--by-stage] [--device N] [--fp32] "
+                             "[--no-output] [--parse-only]\n",
                      argv[0]);
         return 2;
     }
     std::string dir = argv[1], history;
     int n_iter = -1, device = 0;
-    bool eager = false, parse_only = false, fp32 = false;
+    bool eager = false, parse_only = false, fp32 = false, by_stage = false, no_output = false;
     for (int a = 2; a < argc; ++a) {
         if (!std::strcmp(argv[a], "--history") && a + 1 < argc) history = argv[++a];
         else if (!std::strcmp(argv[a], "--eager")) eager = true;
         else if (!std::strcmp(argv[a], "--parse-only")) parse_only = true;
         else if (!std::strcmp(argv[a], "--fp32")) fp32 = true;
+        else if (!std::strcmp(argv[a], "--by-stage")) by_stage = true;
+        else if (!std::strcmp(argv[a], "--no-output")) no_output = true;
         else if (!std::strcmp(argv[a], "--device") && a + 1 < argc) device = std::atoi(argv[++a]);
         else n_iter = std::atoi(argv[a]);
     }
@@ -106,6 +109,12 @@ int main(int argc, char **argv)
         if (fp32) cc.raw.precision = FV3D_DEFECT_FP32;
         fv3d::CMesh mesh(cc.raw);
         fv3d::LoadCase(mesh, dir, bnd, nodes, dims, st);
+        const bool sa = cc.raw.turbulence == 200;
+        if (sa && cc.reuse_wall_distance) {
+            // CUserInput::ReadWallDistance in place of the computed distances
+            std::vector<std::vector<double>> wd = fv3d::ReadWallDistance(dir + "/WallDistance.dat", dims);
+            for (size_t b = 0; b < wd.size(); ++b) mesh.SetWallDistance((int)b + 1, wd[b]);
+        }
         fv3d::CSolver solver(mesh, !eager);
         if (n_iter < 0) n_iter = cc.max_iter;
         FILE *hf = history.empty() ? nullptr : std::fopen(history.c_str(), "w");
@@ -115,9 +124,18 @@ int main(int argc, char **argv)
         const int chunk = 50;
         int done = 0;
         bool converged = false;
+        double last_res = 0.0;
         while (done < n_iter && !converged) {
             int m = n_iter - done < chunk ? n_iter - done : chunk;
-            std::vector<fv3d::HistoryRow> h = solver.Run(m);
+            // near the stop test the iterations run one at a time, so the state
+            // ends where the reference's loop would break
+            const bool near = cc.conv_tol > 0.0 && last_res > 0.0 && last_res <= 100.0 * cc.conv_tol;
+            if (near || by_stage) m = 1;
+            std::vector<fv3d::HistoryRow> h;
+            if (by_stage)
+                h.push_back(solver.SolverByStage(done + 1, mesh.Levels() == 2 ? cc.coarse_iter : 0));
+            else
+                h = solver.Run(m);
             size_t last = 0;
             for (size_t q = 0; q < h.size() && !converged; ++q) {
                 const fv3d::HistoryRow &r = h[q];
@@ -130,10 +148,16 @@ int main(int argc, char **argv)
             }
             done += m;
             const fv3d::HistoryRow &r = h[last];
+            last_res = r[1];
             std::printf("iter %6d  res %.6e  mass in %.6e out %.6e  PR %.6f  eff %.6f  CFL %.3f\n", (int)r[0], r[1],
                         r[2], r[3], r[4], r[5], r[6]);
         }
         if (hf) std::fclose(hf);
+        if (!no_output) {
+            // COutput::OutputRestartSolution and the wall distances, in the case directory
+            fv3d::WriteSolution(dir + "/RestartSolution.dat", mesh, dims, sa);
+            if (sa && !cc.reuse_wall_distance) fv3d::WriteWallDistance(dir + "/WallDistance.dat", mesh, dims);
+        }
         return 0;
     } catch (const fv3d::Error &e) {
         std::fprintf(stderr, "fv3d_run: %s (status %d)\n", e.what(), e.status);

@@ -21,6 +21,8 @@ EXPORTS = [
     "fv3d_set_timing", "fv3d_timing", "fv3d_set_debug", "fv3d_set_tiles", "fv3d_autotune", "fv3d_cell_count",
     "fv3d_mlp_set", "fv3d_mlp_get", "fv3d_mlp_features", "fv3d_mlp_targets", "fv3d_mlp_forward", "fv3d_mlp_grad", "fv3d_mlp_train",
     "fv3d_mlp_closure", "fv3d_mlp_relax", "fv3d_mlp_tripped",
+    "fv3d_primitive_vars", "fv3d_limit_vars", "fv3d_mg_coarse_lusgs", "fv3d_mg_interpolate_state", "fv3d_mg_restrict",
+    "fv3d_mg_correct", "fv3d_iteration_end", "fv3d_turbulence", "fv3d_wall_distance_upload",
 ]
 
 
@@ -87,6 +89,15 @@ def load():
         "fv3d_mlp_train": (i, [vp, p, p, l, i, i, C.c_float, p]),
         "fv3d_mlp_closure": (i, [vp, i]),
         "fv3d_mlp_relax": (i, [vp, C.c_double]),
+        "fv3d_primitive_vars": (i, [vp]),
+        "fv3d_limit_vars": (i, [vp]),
+        "fv3d_mg_coarse_lusgs": (i, [vp, i]),
+        "fv3d_mg_interpolate_state": (i, [vp]),
+        "fv3d_mg_restrict": (i, [vp]),
+        "fv3d_mg_correct": (i, [vp]),
+        "fv3d_iteration_end": (i, [vp, p]),
+        "fv3d_turbulence": (i, [vp]),
+        "fv3d_wall_distance_upload": (i, [vp, i, p]),
         "fv3d_mlp_tripped": (l, [vp]),
         "fv3d_cell_count": (l, [vp]),
     }

@@ -171,13 +171,23 @@ int fv3d_nut_upload(fv3d_ctx *ctx, int block, const double *ev);
  * Returns the number of doubles written. */
 long fv3d_field_download(fv3d_ctx *ctx, int block, const char *name, double *out);
 
+/* CUserInput::ReadWallDistance (source/UserInput.cpp:1094-1116, used with
+ * ReuseWallDistance = Y): replaces one block's cell wall distances (ni*nj*nk,
+ * k fastest, as fv3d_field_download's "dwall") with the caller's, after
+ * fv3d_setup computed them (Spalart-Allmaras only). */
+int fv3d_wall_distance_upload(fv3d_ctx *ctx, int block, const double *d);
+
 /* The same for a grid level: 1 fine, 2 coarse (two-grid cycle). */
 long fv3d_level_field_download(fv3d_ctx *ctx, int level, int block, const char *name, double *out);
 
 /* Stages of CMesh::ResFlux (source/ComputeAllBlockVars.cpp:223-255). */
 int fv3d_bc_apply(fv3d_ctx *ctx);          /* CMesh::BoundaryCondition */
-int fv3d_spectral(fv3d_ctx *ctx);          /* CFluidProps::SpectralRadii */
-int fv3d_timestep(fv3d_ctx *ctx);          /* CTimeDiscr::TimeStep */
+/* CFluidProps::SpectralRadii (source/Spectral_curl_grad.cpp:9-42) and
+ * CTimeDiscr::TimeStep (source/TimeStep.cpp:7-92) share one kernel: either
+ * call forms lambda_c, lambda_v, dt and the LU-SGS diagonals; fv3d_timestep
+ * also counts the low-dt damping down once, as TimeStep does. */
+int fv3d_spectral(fv3d_ctx *ctx);
+int fv3d_timestep(fv3d_ctx *ctx);
 int fv3d_convective(fv3d_ctx *ctx);        /* CMesh::ConvectiveFlux incl. WallFlux, source term and Res */
 /* CMesh::ViscousFlux (source/ComputeAllBlockVars.cpp:131-160) with
  * CFluidProps::Gradient (source/Spectral_curl_grad.cpp:85-493): subtracts the
@@ -188,11 +198,47 @@ int fv3d_viscous(fv3d_ctx *ctx);
  * magnitude "omega".  Only turbulence models read it, so fv3d_resflux does not
  * evaluate it for laminar flow. */
 int fv3d_curl(fv3d_ctx *ctx);
+/* The turbulence stage of ResFlux (source/ComputeAllBlockVars.cpp:228-232):
+ * CBlocks::BL_Model (source/BL_Model.cpp:8-329) with its vorticity, or
+ * CSpaceDiscr::SA_Model (source/SA_Model.cpp:9-258: vorticity, the nu~
+ * residual, mue_t with its clipping, ghost mue_t), or the learned closure when
+ * it is on; nothing for laminar flow.  fv3d_resflux runs it after the ghost
+ * fill. */
+int fv3d_turbulence(fv3d_ctx *ctx);
 int fv3d_resflux(fv3d_ctx *ctx);           /* all of CMesh::ResFlux */
 
 /* CMesh::LU_SGS_Marching (source/LU_SGS_Marching.cpp:10-291): save, ResFlux,
  * forward and backward hyperplane sweeps, update, PrimitiveVars, Limit_Vars. */
 int fv3d_lusgs(fv3d_ctx *ctx);
+
+/* CFluidProps::PrimitiveVars (source/PrimitiveVars.cpp:7-29) and
+ * CFluidProps::Limit_Vars (source/Limit_FlowVars.cpp:8-119, every block
+ * scanned; with the limiter off in the config it does nothing) as calls of
+ * their own; fv3d_lusgs runs both at its end. */
+int fv3d_primitive_vars(fv3d_ctx *ctx);
+int fv3d_limit_vars(fv3d_ctx *ctx);
+
+/* The two-grid cycle by stage (CSolver::Solver, source/Solver.cpp:61-103; two
+ * levels in use).  A start-up iteration is fv3d_mg_coarse_lusgs(ctx, 0) (the
+ * coarse level's own LU-SGS step), followed at Coarse_iter by
+ * fv3d_mg_interpolate_state (CornerTreatment, Interpolate of the state,
+ * PrimitiveVars, Limit_Vars); a cycle iteration is fv3d_lusgs, fv3d_resflux,
+ * fv3d_mg_restrict (CSolver::Prolong with ComputeForceFunction /
+ * AddForceFunction, source/Prolong_Interpolate.cpp:7-50 and
+ * source/ComputeAllBlockVars.cpp:269-309, then the coarse PrimitiveVars and
+ * Limit_Vars), fv3d_mg_coarse_lusgs(ctx, 1) (the forced coarse LU-SGS on the
+ * coarse lambda and dt), fv3d_mg_correct (ComputeDeltConsVars, Interpolate of
+ * the correction, PrimitiveVars, Limit_Vars, source/Prolong_Interpolate.cpp:
+ * 52-176); each iteration ends with fv3d_iteration_end, which forms
+ * Res_Calculate, the CFL ramp and the section integrals of the level that
+ * reports and returns its history row (iter, Res.dens, mass in, mass out, PR,
+ * efficiency, CFL; one row per instance).  The same kernels as fv3d_iterate,
+ * so the same bits. */
+int fv3d_mg_coarse_lusgs(fv3d_ctx *ctx, int forced);
+int fv3d_mg_interpolate_state(fv3d_ctx *ctx);
+int fv3d_mg_restrict(fv3d_ctx *ctx);
+int fv3d_mg_correct(fv3d_ctx *ctx);
+int fv3d_iteration_end(fv3d_ctx *ctx, double *row7);
 
 /* CMesh::Res_Calculate (source/ComputeAllBlockVars.cpp:318-372) of the first
  * instance; synchronises. */

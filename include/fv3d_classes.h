@@ -72,7 +72,7 @@ struct CaseConfig {
     std::string grid_file, bc_file;
     int max_iter = 0, coarse_iter = 0, n_threads = 1;
     double conv_tol = 0.0;
-    bool multigrid = false, reuse_solution = false;
+    bool multigrid = false, reuse_solution = false, reuse_wall_distance = false;
     int time_scheme = 0;
 };
 
@@ -106,9 +106,15 @@ inline CaseConfig ReadConfig(const std::string &path)
         std::istringstream s(v(27));
         s >> r.outlet_dir[0] >> r.outlet_dir[1] >> r.outlet_dir[2];
     }
+    // UseFlowMassImposed (source/UserInput.cpp:79-80): the outlet pressure
+    // rescaled towards a target mass flow during the run
+    // (source/Structured_FV3D.cpp:186-195) is not in this build
+    if (flag(28))
+        throw Error(FV3D_EINVAL, path + ": UseFlowMassImposed = Y (outlet pressure driven to a mass flow) is not "
+                                        "supported; set it to N");
     c.max_iter = inum(33); c.coarse_iter = inum(34);
     c.conv_tol = num(36); c.n_threads = inum(37);
-    c.multigrid = flag(38); c.reuse_solution = flag(39);
+    c.multigrid = flag(38); c.reuse_solution = flag(39); c.reuse_wall_distance = flag(40);
     r.multigrid = c.multigrid;
     r.coarse_iter = c.coarse_iter;
     r.cfl = num(44); r.c_timestep = num(45); r.relax = num(46);
@@ -231,6 +237,12 @@ inline std::vector<std::vector<double>> ReadGrids(const std::string &path, std::
 // column nu~.  Returned per block as the fv3d_state_upload pair pv
 // (ni,nj,nk,6 = rho,u,v,w,p,T) and ener (ni,nj,nk), and ev (ni,nj,nk) for
 // fv3d_nut_upload (empty without with_nut).
+// The five conservative components of a residual norm, as the reference's
+// TConsVars (include/defs.h:58-64) names them.
+struct TConsVars {
+    double dens = 0.0, xmom = 0.0, ymom = 0.0, zmom = 0.0, ener = 0.0;
+};
+
 struct BlockState {
     std::vector<double> pv, ener, ev;
 };
@@ -304,6 +316,13 @@ public:
         check(ctx_, fv3d_connection_add(ctx_, kind, orient, block1, face1, rng1, block2, face2, rng2),
               "fv3d_connection_add");
     }
+    // Mixing plane (CUserInput::ReadBndGrid's MixingPlane rows,
+    // source/UserInput.cpp:420-432): block1's K-high face against block2's
+    // K-low face (source/BndCondMixingPlane.cpp:287-522).
+    void AddMixingPlane(int block1, const int rng1[6], int block2, const int rng2[6])
+    {
+        check(ctx_, fv3d_mixing_plane_add(ctx_, block1, rng1, block2, rng2), "fv3d_mixing_plane_add");
+    }
     // CMesh::Geometry: metrics and device set-up, and with UseMultigrid the
     // coarse level when the patch ranges allow it.
     void Geometry() { check(ctx_, fv3d_setup(ctx_), "fv3d_setup"); }
@@ -329,6 +348,27 @@ public:
         check(ctx_, fv3d_res_calculate(ctx_, r.data()), "fv3d_res_calculate");
         return r;
     }
+    // The reference's signature: Res_Calculate(TConsVars &Res, REAL &Res_ev)
+    // (include/Mesh.h:34).
+    void Res_Calculate(TConsVars &res, double &res_ev)
+    {
+        const std::array<double, 6> r = Res_Calculate();
+        res.dens = r[0]; res.xmom = r[1]; res.ymom = r[2]; res.zmom = r[3]; res.ener = r[4];
+        res_ev = r[5];
+    }
+    // CFluidProps::PrimitiveVars and Limit_Vars of every block
+    // (source/PrimitiveVars.cpp:7-29, source/Limit_FlowVars.cpp:8-119; the
+    // latter does nothing with the limiter off in config.dat).
+    void PrimitiveVars() { check(ctx_, fv3d_primitive_vars(ctx_), "fv3d_primitive_vars"); }
+    void Limit_Vars() { check(ctx_, fv3d_limit_vars(ctx_), "fv3d_limit_vars"); }
+    // CMesh::WallDistanceSA (source/WallDistanceSA.cpp:9-132): computed on the
+    // device at Geometry(); the distances of one block's cells.
+    std::vector<double> WallDistanceSA(int block, size_t ncell) { return Field(block, "dwall", ncell); }
+    // CUserInput::ReadWallDistance: distances of a block read from WallDistance.dat.
+    void SetWallDistance(int block, const std::vector<double> &d)
+    {
+        check(ctx_, fv3d_wall_distance_upload(ctx_, block, d.data()), "fv3d_wall_distance_upload");
+    }
     // Host mirror of a device field, for output (names as fv3d_field_download).
     std::vector<double> Field(int block, const char *name, size_t count)
     {
@@ -348,6 +388,10 @@ private:
 class CSpaceDiscr {
 public:
     explicit CSpaceDiscr(CMesh &m) : m_(m) {}
+    // CSpaceDiscr::SA_Model / CBlocks::BL_Model: the turbulence stage of ResFlux
+    // (after BoundaryCondition).  CSpaceDiscr::WallFlux runs inside
+    // ConvectiveFlux (wall faces take the face-averaged flux there).
+    void SA_Model() { check(m_.ctx(), fv3d_turbulence(m_.ctx()), "fv3d_turbulence"); }
     // CMesh::ConvectiveFlux incl. WallFlux, the source term and Res = Fc.
     void ConvectiveFlux() { check(m_.ctx(), fv3d_convective(m_.ctx()), "fv3d_convective"); }
     // CMesh::ViscousFlux with the face gradients: Res -= Fv (after ConvectiveFlux;
@@ -397,6 +441,39 @@ public:
         check(m_.ctx(), rc, "fv3d_iterate");
         return r;
     }
+    // The stages of one iteration of CSolver::Solver (source/Solver.cpp:61-103)
+    // as the reference's private methods: the two-grid cycle's Prolong
+    // (restriction with the forcing), the forced coarse Time_Marching,
+    // ComputeDeltConsVars + Interpolate (the correction), and the state
+    // Interpolate at Coarse_iter.
+    void Prolong() { check(m_.ctx(), fv3d_mg_restrict(m_.ctx()), "fv3d_mg_restrict"); }
+    void CoarseTime_Marching(bool forced)
+    {
+        check(m_.ctx(), fv3d_mg_coarse_lusgs(m_.ctx(), forced ? 1 : 0), "fv3d_mg_coarse_lusgs");
+    }
+    void Interpolate() { check(m_.ctx(), fv3d_mg_correct(m_.ctx()), "fv3d_mg_correct"); }
+    void InterpolateState() { check(m_.ctx(), fv3d_mg_interpolate_state(m_.ctx()), "fv3d_mg_interpolate_state"); }
+    // One iteration driven stage by stage (single grid or two-grid cycle);
+    // the same bits as Solver().  iter is the iteration number (1-based).
+    HistoryRow SolverByStage(long iter, int coarse_iter)
+    {
+        fv3d_ctx *c = m_.ctx();
+        if (m_.Levels() == 1) {
+            m_.Time_Marching();
+        } else if (iter <= coarse_iter) {
+            CoarseTime_Marching(false);
+            if (iter == coarse_iter) InterpolateState();
+        } else {
+            m_.Time_Marching();
+            m_.ResFlux();
+            Prolong();
+            CoarseTime_Marching(true);
+            Interpolate();
+        }
+        HistoryRow r{};
+        check(c, fv3d_iteration_end(c, r.data()), "fv3d_iteration_end");
+        return r;
+    }
     // n iterations in one call (one synchronisation at the end).
     std::vector<HistoryRow> Run(int n)
     {
@@ -425,8 +502,9 @@ inline void LoadCase(CMesh &mesh, const std::string &dir, const BndTable &bnd,
         mesh.AddBlock(dims[b][0], dims[b][1], dims[b][2], nodes[b], nblade, rpm);
     }
     for (const PatchRow &p : bnd.patches) {
-        if (p.kind == 5) throw Error(FV3D_EINVAL, "mixing-plane patches not in this build");
-        if (p.kind == 3 || p.kind == 4)
+        if (p.kind == 5)
+            mesh.AddMixingPlane(p.block, p.rng, p.block2, p.rng2);
+        else if (p.kind == 3 || p.kind == 4)
             mesh.AddConnection(p.kind == 3 ? 1 : 0, p.orient, p.block, p.face, p.rng, p.block2, p.face2, p.rng2);
         else
             mesh.AddPatch(p.kind, p.block, p.face, p.rng, p.rotate);
@@ -435,6 +513,82 @@ inline void LoadCase(CMesh &mesh, const std::string &dir, const BndTable &bnd,
     if (states.empty())
         mesh.Initial();
     for (size_t b = 0; b < states.size(); ++b) mesh.Initial((int)b + 1, states[b]);
+}
+
+// COutput::OutputRestartSolution (source/Ouput.cpp:227-259): rows of
+// "block i j k p u v w T rho rhoE" (and nu~ with Spalart-Allmaras), k outermost,
+// i innermost, in the solver's non-dimensional units; written with 17
+// significant digits (the reference's stream default is 6), so ReadSolution
+// gives back the same bits.
+inline void WriteSolution(const std::string &path, CMesh &mesh, const std::vector<std::array<int, 3>> &dims,
+                          bool with_nut)
+{
+    std::ofstream f(path);
+    if (!f) throw Error(FV3D_EINVAL, "cannot write " + path);
+    f.precision(17);
+    for (size_t b = 0; b < dims.size(); ++b) {
+        const int IM = dims[b][0], JM = dims[b][1], KM = dims[b][2];
+        const size_t box = (size_t)(IM + 1) * (JM + 1) * (KM + 1);
+        std::vector<double> pv = mesh.Field((int)b + 1, "pv", 6 * box), cv = mesh.Field((int)b + 1, "cv", 5 * box);
+        std::vector<double> ev;
+        if (with_nut) ev = mesh.Field((int)b + 1, "ev", box);
+        for (int k = 1; k <= KM - 1; ++k)
+            for (int j = 1; j <= JM - 1; ++j)
+                for (int i = 1; i <= IM - 1; ++i) {
+                    const size_t x = ((size_t)i * (JM + 1) + j) * (KM + 1) + k;
+                    const double *w = &pv[6 * x];
+                    f << b + 1 << " " << i << " " << j << " " << k << " " << w[4] << " " << w[1] << " " << w[2] << " "
+                      << w[3] << " " << w[5] << " " << w[0] << " " << cv[5 * x + 4];
+                    if (with_nut) f << " " << ev[x];
+                    f << "\n";
+                }
+    }
+    if (!f) throw Error(FV3D_EINVAL, "error writing " + path);
+}
+
+// WallDistance.dat (source/WallDistanceSA.cpp:162-185, read back by
+// CUserInput::ReadWallDistance, source/UserInput.cpp:1094-1116): per block its
+// number, its cell counts, then "i j k d" rows, k outermost.  Distances are
+// returned per block with k fastest (the fv3d_wall_distance_upload order).
+inline std::vector<std::vector<double>> ReadWallDistance(const std::string &path,
+                                                         const std::vector<std::array<int, 3>> &dims)
+{
+    std::ifstream f(path);
+    if (!f) throw Error(FV3D_EINVAL, "cannot open " + path);
+    std::vector<std::vector<double>> out(dims.size());
+    for (size_t b = 0; b < dims.size(); ++b) {
+        const int ni = dims[b][0] - 1, nj = dims[b][1] - 1, nk = dims[b][2] - 1;
+        int id, a, c, d;
+        if (!(f >> id >> a >> c >> d)) throw Error(FV3D_EINVAL, path + ": missing block header");
+        if (a != ni || c != nj || d != nk) throw Error(FV3D_EINVAL, path + ": block cell counts differ from the grid");
+        out[b].assign((size_t)ni * nj * nk, 0.0);
+        for (int k = 1; k <= nk; ++k)
+            for (int j = 1; j <= nj; ++j)
+                for (int i = 1; i <= ni; ++i) {
+                    int ii, jj, kk;
+                    double v;
+                    if (!(f >> ii >> jj >> kk >> v)) throw Error(FV3D_EINVAL, path + ": short wall-distance list");
+                    out[b][((size_t)(i - 1) * nj + (j - 1)) * nk + (k - 1)] = v;
+                }
+    }
+    return out;
+}
+
+inline void WriteWallDistance(const std::string &path, CMesh &mesh, const std::vector<std::array<int, 3>> &dims)
+{
+    std::ofstream f(path);
+    if (!f) throw Error(FV3D_EINVAL, "cannot write " + path);
+    f.precision(17);
+    for (size_t b = 0; b < dims.size(); ++b) {
+        const int ni = dims[b][0] - 1, nj = dims[b][1] - 1, nk = dims[b][2] - 1;
+        std::vector<double> d = mesh.Field((int)b + 1, "dwall", (size_t)ni * nj * nk);
+        f << b + 1 << "\n" << ni << " " << nj << " " << nk << "\n";
+        for (int k = 1; k <= nk; ++k)
+            for (int j = 1; j <= nj; ++j)
+                for (int i = 1; i <= ni; ++i)
+                    f << i << " " << j << " " << k << " " << d[((size_t)(i - 1) * nj + (j - 1)) * nk + (k - 1)] << "\n";
+    }
+    if (!f) throw Error(FV3D_EINVAL, "error writing " + path);
 }
 
 }  // namespace fv3d

@@ -1,0 +1,171 @@
+"""The C ABI's stage calls against the solver loop: one two-grid iteration
+driven stage by stage (CSolver::Solver's order, source/Solver.cpp:61-103) is
+bit for bit fv3d_iterate's, through the coarse start-up phase, the switch at
+Coarse_iter and the cycle; SpectralRadii + TimeStep as separate calls match
+the oracle's; PrimitiveVars and Limit_Vars as calls of their own repair a
+restart state the way the oracle's iteration does."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from fv3d import cases
+from fv3d.solver import GpuSolver
+import oracle as O
+from test_oracle_vs_ref import variant
+
+pytestmark = pytest.mark.gpu
+
+
+def _two_grid(max_iter=10):
+    c = cases.with_wall_shear(cases.annular_passage(ni=8, nj=8, nk_blocks=(6, 10, 6), seed=34, rpm=3000.0,
+                                                    blade_blocks=(1,), camber=0.1, viscous=True, len_ref=1e-3,
+                                                    max_iter=max_iter))
+    c.cfg.turbulence = 100
+    c.cfg.use_multigrid, c.cfg.reuse_solution, c.cfg.initial, c.cfg.coarse_iter = "Y", "N", "I", 3
+    return c
+
+
+def _check(g, rc, what):
+    assert rc == 0, f"{what}: {g.L.fv3d_last_error(g.h).decode()}"
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_two_grid_by_stage_equals_iterate(precision):
+    case = _two_grid()
+    n = 9
+    a = GpuSolver(case, precision=precision)
+    ha = a.iterate(n)
+    b = GpuSolver(case, precision=precision)
+    L = b.L
+    rows = []
+    for it in range(1, n + 1):
+        if it <= case.cfg.coarse_iter:
+            _check(b, L.fv3d_mg_coarse_lusgs(b.h, 0), "coarse lusgs")
+            if it == case.cfg.coarse_iter:
+                _check(b, L.fv3d_mg_interpolate_state(b.h), "interpolate state")
+        else:
+            _check(b, L.fv3d_lusgs(b.h), "lusgs")
+            _check(b, L.fv3d_resflux(b.h), "resflux")
+            _check(b, L.fv3d_mg_restrict(b.h), "restrict")
+            _check(b, L.fv3d_mg_coarse_lusgs(b.h, 1), "forced coarse lusgs")
+            _check(b, L.fv3d_mg_correct(b.h), "correct")
+        row = np.zeros(7)
+        _check(b, L.fv3d_iteration_end(b.h, row.ctypes.data), "iteration end")
+        rows.append(row)
+    np.testing.assert_array_equal(np.array(rows), ha)
+    for blk in range(1, len(case.blocks) + 1):
+        np.testing.assert_array_equal(a.get(blk, "cv"), b.get(blk, "cv"))
+        np.testing.assert_array_equal(a.get(blk, "cv", level=2), b.get(blk, "cv", level=2))
+    a.close()
+    b.close()
+
+
+def test_stage_calls_need_two_levels():
+    g = GpuSolver(cases.bump_channel(ni=6, nj=6, nk=8, seed=1, max_iter=2))
+    assert g.L.fv3d_mg_restrict(g.h) < 0
+    assert "two-grid" in g.L.fv3d_last_error(g.h).decode()
+    g.close()
+
+
+def test_spectral_and_timestep_calls():
+    case = cases.with_wall_shear(cases.bump_channel(ni=7, nj=6, nk=9, seed=24, viscous=True, len_ref=1e-4, max_iter=2))
+    o = O.Oracle(case)
+    for s in ("bc", "spectral", "timestep"):
+        o.call(s)
+    g = GpuSolver(case)
+    for s in ("bc_apply", "spectral", "timestep"):
+        g.call(s)
+    for f in ("dt", "lamcI", "lamcJ", "lamcK", "lamvI", "lamvJ", "lamvK"):
+        a, w = g.get(1, f), o.get(1, f)
+        assert np.max(np.abs(a - w)) <= 1e-12 * np.max(np.abs(w)), f
+    g.close()
+
+
+def test_primitive_vars_and_limit_vars_calls():
+    # the dense-corner restart: one LU-SGS step of the iteration against the
+    # same step with its own PrimitiveVars / Limit_Vars calls repeated after it
+    # (idempotent on a limited state), and a restart state repaired directly
+    case = variant("limiter_dense_corner")
+    a = GpuSolver(case)
+    a.call("lusgs")
+    cv = a.get(1, "cv").copy()
+    a.call("primitive_vars")
+    a.call("limit_vars")
+    np.testing.assert_array_equal(a.get(1, "cv"), cv)
+    a.close()
+    g = GpuSolver(case)
+    g.call("primitive_vars")
+    g.call("limit_vars")
+    pv = g.get(1, "pv")[1:-1, 1:-1, 1:-1]
+    assert np.all(pv[..., 0] <= 1000.0) and np.all(pv[..., 0] >= 1e-6)
+    st = case.states[0]
+    np.testing.assert_array_equal(pv[3:, 3:, 3:, 0], st["rho"][3:, 3:, 3:])
+    g.close()
+
+
+import os
+import subprocess
+
+RUN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                   "3d-fluid-numerical-simulation-software_amd", "lib", "fv3d_run")
+
+
+def _run(d, n, *extra):
+    hist = os.path.join(d, "history.txt")
+    subprocess.run([RUN, d, str(n), "--history", hist] + list(extra), check=True, timeout=120,
+                   stdout=subprocess.DEVNULL)
+    return np.loadtxt(hist, ndmin=2)
+
+
+def test_runner_by_stage_and_mixing_plane_rows(tmp_path):
+    # a two-grid case with a mixing-plane row in its boundary table, run by
+    # the C++ layer stage by stage, through the solver loop, and from Python
+    case = cases.with_mixing_plane(_two_grid(), 2, 3)
+    d = str(tmp_path / "case")
+    case.write(d)
+    h_loop = _run(d, 8, "--no-output")
+    h_stage = _run(d, 8, "--by-stage", "--no-output")
+    np.testing.assert_array_equal(h_stage, h_loop)
+    g = GpuSolver(case)
+    np.testing.assert_array_equal(g.iterate(8, graph=True), h_loop)
+    g.close()
+
+
+def test_runner_writes_restart_and_wall_distance(tmp_path):
+    # RestartSolution.dat at full precision equals the state, and a rerun that
+    # reads WallDistance.dat (ReuseWallDistance = Y) has the same history
+    case = cases.with_sa(cases.with_wall_shear(cases.annular_passage(ni=8, nj=6, nk_blocks=(5, 8, 4), seed=72,
+                                                                     rpm=6000.0, blade_blocks=(1,), camber=0.1,
+                                                                     viscous=True, len_ref=1e-3, max_iter=6)))
+    d = str(tmp_path / "case")
+    case.write(d)
+    h1 = _run(d, 6)
+    g = GpuSolver(case)
+    g.iterate(6, graph=True)
+    rows = np.loadtxt(os.path.join(d, "RestartSolution.dat"), ndmin=2)
+    for b, bi in enumerate(case.blocks, start=1):
+        r = rows[rows[:, 0] == b]
+        ni, nj, nk = bi.im - 1, bi.jm - 1, bi.km - 1
+        # file order: k outermost, i innermost
+        idx = (r[:, 1].astype(int) - 1, r[:, 2].astype(int) - 1, r[:, 3].astype(int) - 1)
+        pv = g.get(b, "pv")[1:-1, 1:-1, 1:-1]
+        cv = g.get(b, "cv")[1:-1, 1:-1, 1:-1]
+        ev = g.get(b, "ev")[1:-1, 1:-1, 1:-1]
+        for col, want in ((4, pv[..., 4]), (5, pv[..., 1]), (6, pv[..., 2]), (7, pv[..., 3]), (8, pv[..., 5]),
+                          (9, pv[..., 0]), (10, cv[..., 4]), (11, ev)):
+            np.testing.assert_array_equal(r[:, col], want[idx])
+        assert len(r) == ni * nj * nk
+    g.close()
+    assert os.path.exists(os.path.join(d, "WallDistance.dat"))
+    # rerun from the original start with the distances read back
+    case.write(d)
+    with open(os.path.join(d, "config.dat")) as f:
+        lines = f.read().split("\n")
+    lines[39] = "Y" + lines[39][1:]
+    with open(os.path.join(d, "config.dat"), "w") as f:
+        f.write("\n".join(lines))
+    wd = os.path.join(d, "WallDistance.dat")
+    assert os.path.exists(wd)
+    h2 = _run(d, 6, "--no-output")
+    np.testing.assert_array_equal(h2, h1)

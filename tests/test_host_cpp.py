@@ -87,3 +87,27 @@ def test_parse_only_reads_nu_tilde(tmp_path):
 def test_missing_case_reports_error(tmp_path):
     r = subprocess.run([RUN, str(tmp_path / "nope"), "--parse-only"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+def test_mass_flow_imposition_is_refused(tmp_path):
+    # UseFlowMassImposed = Y rescales the outlet pressure during the run
+    # (source/Structured_FV3D.cpp:186-195); the runner refuses it instead of
+    # running the case with a fixed outlet pressure
+    case = cases.bump_channel(ni=5, nj=4, nk=9, seed=7)
+    case.cfg.mass_imposed = "Y"
+    d = str(tmp_path / "case")
+    case.write(d)
+    r = subprocess.run([RUN, d, "--parse-only"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "UseFlowMassImposed" in r.stderr
+
+
+def test_parse_only_reads_mixing_plane_rows(tmp_path):
+    case = cases.with_mixing_plane(cases.annular_passage(ni=6, nj=8, nk_blocks=(5, 6, 4), max_iter=6, seed=74,
+                                                         rpm=5000.0), 2, 3)
+    d = str(tmp_path / "case")
+    case.write(d)
+    s = json.loads(subprocess.run([RUN, d, "--parse-only"], check=True, capture_output=True, text=True,
+                                  timeout=60).stdout)
+    mp = [p for p in case.patches if p.kind == "mixing"]
+    assert len(mp) == 1
+    assert any(row[0] == 5 for row in s["patches"]), s["patches"]
