@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -718,6 +719,37 @@ static int setup_sa(fv3d_ctx *c)
     return FV3D_OK;
 }
 
+// ------------------------------------------- scratch of fv3d_mlp_grad_noalloc
+// The partial gradients and losses of up to kNoAllocChunks 64-sample chunks
+// (8833 + 1 floats each) and the loss sum live in a file-scope device array,
+// so fv3d_mlp_grad_noalloc allocates nothing.  Every GPU holds its own copy
+// (9 MB) until the process ends; host code uses only the per-device address
+// hipGetSymbolAddress returns, taken once per device at context creation.
+constexpr int kNoAllocChunks = 256;   // 16,384 samples
+constexpr long kNoAllocFloats = (long)kNoAllocChunks * (kMlpParams + 1) + 1;
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+constexpr int kNoAllocDevices = 64;
+static float *g_scratch_ptr[kNoAllocDevices];
+static std::mutex g_scratch_mu;
+
+static int noalloc_init(fv3d_ctx *c)
+{
+    if (c->device < 0 || c->device >= kNoAllocDevices)
+        return fail(c, FV3D_EINVAL, "device %d beyond the %d the library tracks", c->device, kNoAllocDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[c->device]) return FV3D_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(c, FV3D_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[c->device] = (float *)p;
+    return FV3D_OK;
+}
+
+__global__ void k_noalloc_loss(const float *__restrict__ sum, long n, float *__restrict__ loss)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *loss = *sum / (float)n;
+}
+
 // ------------------------------------------------------------------ API
 extern "C" {
 
@@ -735,6 +767,7 @@ int fv3d_ctx_create(const fv3d_config *cfg, fv3d_ctx **out)
     c->device = cfg->device;
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) { *out = c; return fail(c, FV3D_EHIP, "hipSetDevice(%d): %s", c->device, hipGetErrorString(e)); }
+    if (int rc = noalloc_init(c)) { *out = c; return rc; }
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { *out = c; return fail(c, FV3D_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
     for (int n = 0; n < 14; ++n) (void)hipEventCreate(&c->ev[n]);
@@ -2627,6 +2660,29 @@ int fv3d_mlp_grad(fv3d_ctx *c, const float *x, const float *t, long n, float *gr
     for (float *p : {dx, dt, dpart, dloss})
         if (p) (void)hipFree(p);
     return rc;
+}
+
+int fv3d_mlp_grad_noalloc(fv3d_ctx *c, const float *x_dev, const float *t_dev, long n, float *grad_dev,
+                          float *loss_dev)
+{
+    READY(c);
+    if (!x_dev || !t_dev || !grad_dev || !loss_dev || n < 1) return fail(c, FV3D_EINVAL, "mlp_grad_noalloc arguments");
+    if (n > 64L * kNoAllocChunks)
+        return fail(c, FV3D_EINVAL, "mlp_grad_noalloc: %ld samples, at most %d", n, 64 * kNoAllocChunks);
+    if (!c->d_mlp) return fail(c, FV3D_ESTATE, "no network parameters (fv3d_mlp_set)");
+    float *s = g_scratch_ptr[c->device];
+    if (!s) return fail(c, FV3D_ESTATE, "scratch address of device %d not taken", c->device);
+    // every chunk's partial gradient and loss is written before it is read, so
+    // nothing in the scratch needs clearing between calls
+    float *part = s, *loss = s + (long)kNoAllocChunks * kMlpParams, *sum = s + kNoAllocFloats - 1;
+    const int nchunk = (int)((n + 63) / 64);
+    hipLaunchKernelGGL(k_mlp_train_chunk, dim3((unsigned)nchunk), dim3(256), 0, c->stream, c->d_mlp, x_dev, t_dev, n,
+                       1.0f / (float)n, part, loss);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, c->stream, part, loss, nchunk,
+                       grad_dev, sum);
+    hipLaunchKernelGGL(k_noalloc_loss, dim3(1), dim3(64), 0, c->stream, sum, n, loss_dev);
+    HIPCK(c, hipGetLastError());
+    return FV3D_OK;
 }
 
 int fv3d_mlp_train(fv3d_ctx *c, const float *x, const float *t, long n, int steps, int batch, float lr,

@@ -19,7 +19,7 @@ EXPORTS = [
     "fv3d_state_upload", "fv3d_nut_upload", "fv3d_field_download", "fv3d_level_field_download", "fv3d_bc_apply", "fv3d_spectral", "fv3d_timestep", "fv3d_convective",
     "fv3d_viscous", "fv3d_curl", "fv3d_resflux", "fv3d_lusgs", "fv3d_res_calculate", "fv3d_perf", "fv3d_iterate", "fv3d_iterate_graph", "fv3d_sync",
     "fv3d_set_timing", "fv3d_timing", "fv3d_set_debug", "fv3d_set_tiles", "fv3d_autotune", "fv3d_cell_count",
-    "fv3d_mlp_set", "fv3d_mlp_get", "fv3d_mlp_features", "fv3d_mlp_targets", "fv3d_mlp_forward", "fv3d_mlp_grad", "fv3d_mlp_train",
+    "fv3d_mlp_set", "fv3d_mlp_get", "fv3d_mlp_features", "fv3d_mlp_targets", "fv3d_mlp_forward", "fv3d_mlp_grad", "fv3d_mlp_grad_noalloc", "fv3d_mlp_train",
     "fv3d_mlp_closure", "fv3d_mlp_relax", "fv3d_mlp_tripped",
     "fv3d_primitive_vars", "fv3d_limit_vars", "fv3d_mg_coarse_lusgs", "fv3d_mg_interpolate_state", "fv3d_mg_restrict",
     "fv3d_mg_correct", "fv3d_iteration_end", "fv3d_turbulence", "fv3d_wall_distance_upload",
@@ -86,6 +86,7 @@ def load():
         "fv3d_mlp_targets": (l, [vp, i, p]),
         "fv3d_mlp_forward": (i, [vp, p, l, p]),
         "fv3d_mlp_grad": (i, [vp, p, p, l, p, p]),
+        "fv3d_mlp_grad_noalloc": (i, [vp, vp, vp, l, vp, vp]),
         "fv3d_mlp_train": (i, [vp, p, p, l, i, i, C.c_float, p]),
         "fv3d_mlp_closure": (i, [vp, i]),
         "fv3d_mlp_relax": (i, [vp, C.c_double]),

@@ -333,6 +333,17 @@ long fv3d_mlp_features(fv3d_ctx *ctx, int block, float *out);
 long fv3d_mlp_targets(fv3d_ctx *ctx, int block, float *out);
 int fv3d_mlp_forward(fv3d_ctx *ctx, const float *x, long n, float *y);
 int fv3d_mlp_grad(fv3d_ctx *ctx, const float *x, const float *t, long n, float *grad, float *loss);
+/* No counterpart in the reference.  fv3d_mlp_grad for callers that cannot
+ * allocate (code under HIP graph capture, training or serving loops that
+ * forbid hipMalloc): x_dev (n x 6), t_dev (n), grad_dev (8833) and loss_dev
+ * (1, the mean loss) are device buffers the caller owns; the partial
+ * gradients go to a static device scratch array of the library (9 MB per
+ * GPU, held until the process ends), so n is at most 16,384 samples and
+ * larger n is rejected with FV3D_EINVAL.  Nothing is allocated or freed.
+ * The call is asynchronous on the context's stream (fv3d_sync waits); calls
+ * that overlap on one GPU, from any context, race on the shared scratch. */
+int fv3d_mlp_grad_noalloc(fv3d_ctx *ctx, const float *x_dev, const float *t_dev, long n, float *grad_dev,
+                          float *loss_dev);
 int fv3d_mlp_train(fv3d_ctx *ctx, const float *x, const float *t, long n, int steps, int batch, float lr,
                    float *loss_hist);
 int fv3d_mlp_closure(fv3d_ctx *ctx, int on);
