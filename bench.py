@@ -84,7 +84,7 @@ def kernel_bytes(precision: str, sa_transport: bool):
 
 
 KERNEL_NAMES = {"convective": "k_convective (MUSCL + Roe/Steger-Warming + wall flux + source -> R)",
-                "viscous": "k_viscous (face gradients + viscous flux, and the nu~ residual k_sa_faces)",
+                "viscous": "k_viscous (face gradients + laminar and eddy viscous flux -> R)",
                 "forward_sweep": "k_lusgs_sl<forward> (LU-SGS forward wavefront sweep)",
                 "backward_sweep": "k_lusgs_sl<backward> (LU-SGS backward wavefront sweep)"}
 
@@ -104,6 +104,9 @@ def parse():
                     help="c2: design rotation speed (the synthetic passage is stationary by design)")
     ap.add_argument("--compare-alone", action="store_true", help="c2: also time every passage alone")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default residual tile shapes")
+    ap.add_argument("--tiles", default=None,
+                    help="fixed configuration numbers C,V,S of the convective, viscous and nu~ residual kernels "
+                         "in place of the autotuner (GpuSolver.TILE_SHAPES / STREAM_SHAPES numbering)")
     ap.add_argument("--ni", type=int, default=64, help="bump: cells along i (span)")
     ap.add_argument("--nj", type=int, default=64, help="bump: cells along j")
     ap.add_argument("--nk", type=int, default=256, help="bump: cells along k (streamwise)")
@@ -198,7 +201,12 @@ def time_form(case, precision, a, closure=None):
     g = solver(case, precision, closure)
     graph = not a.no_graph
     tiles = None
-    if a.no_autotune or a.warmup < 2:
+    if a.tiles:
+        sel = [int(x) for x in a.tiles.split(",")]
+        g.set_tiles(sel)
+        tiles = [GpuSolver.conv_shape(sel[0])] + [GpuSolver.TILE_SHAPES[q] for q in sel[1:]]
+        g.iterate(a.warmup, graph=graph)
+    elif a.no_autotune or a.warmup < 2:
         g.iterate(a.warmup, graph=graph)
     else:
         # residual tile shapes picked on the running state, then one more
@@ -280,7 +288,7 @@ def roofline(r, precision, sa_transport, wkey):
         if ms > 0:
             ach = b * r["cells"] / (ms * 1e-3) / 1e9
             ks[k] = {"kernel": KERNEL_NAMES[k], "ms": ms, "bytes_per_cell": b, "achieved": ach,
-                     "frac": ach / HBM_PEAK_GBS}
+                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic_from_profiles(k, wkey)}
     dom = max(ks, key=lambda k: ks[k]["ms"])
     d = ks[dom]
     return {"bound": "hbm", "kernel": d["kernel"], "achieved": d["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -341,7 +349,7 @@ def c3_main(a):
         "plain_fp64": {"value": p["value"], "ms_per_step": p["ms_per_step"], "class_ms": p["class_ms"]} if p else None,
         "time_to_1e-6_drop": drop,
         "roofline": roofline(r, "fp32", closure is None, "c3"),
-        "roofline_plain_fp64": roofline(p, "fp64", True, "c3") if p else None,
+        "roofline_plain_fp64": roofline(p, "fp64", True, "c3_fp64") if p else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
