@@ -13,11 +13,12 @@ uniform field on the coarse level, stationary, outlet at 0.95 Pt.
 * chosen: FP32 defect correction (FP32 LU-SGS operator over the FP64
   residual, update and norms), the four-level V-cycle, and all blocks of
   every stage in one launch captured in a HIP graph with autotuned residual
-  tiles.  ``--closure`` adds the learned eddy-viscosity closure in place of
-  the nu~ transport (the Spalart-Allmaras value as the gated fallback, the
-  network of fv3d/closure_vane.npz written by ``--workload closure-train``);
-  it is not in the default line because the BF16 network leaves the residual
-  a floor near a 1e-6 drop (see the closure-train results).
+  tiles, and the learned eddy-viscosity closure in place of the nu~ transport
+  (the network of fv3d/closure_vane.npz written by ``--workload
+  closure-train``, its mue_t under-relaxed by 0.2, the Spalart-Allmaras value
+  of the frozen nu~ as the gated fallback).  ``--no-closure`` runs the chosen
+  form with the Spalart-Allmaras transport instead; by default the drop runs
+  include that variant too.
 
 A step is one fine-level iteration (with multigrid: fine LU-SGS, the second
 fine residual, the coarse levels and the prolonged correction) over the whole
@@ -37,6 +38,9 @@ training and held-out evaluation; writes the network).
 into oracle/_ref) on the host, single-threaded, on a 64,512-cell member of
 the same passage family: with more than one OpenMP thread the reference's face
 loops race on a shared variable (source/Upwind_Convective.cpp:7,29).
+``cpu_baseline_all_cores`` times the C restatement of the same solver (the
+test oracle, bit-identical to the reference at one thread) on every OpenMP
+thread on the same sample.
 """
 from __future__ import annotations
 
@@ -96,8 +100,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", choices=["c3", "c2", "passage5", "bump", "closure-train"], default="c3")
     ap.add_argument("--levels", type=int, default=4, help="grid levels of the chosen form's V-cycle")
-    ap.add_argument("--closure", action="store_true",
-                    help="chosen form with the learned closure (fv3d/closure_vane.npz) in place of the nu~ transport")
+    ap.add_argument("--no-closure", action="store_true",
+                    help="chosen form with the Spalart-Allmaras transport instead of the learned closure "
+                         "(fv3d/closure_vane.npz)")
+    ap.add_argument("--no-chosen-sa", action="store_true",
+                    help="skip the chosen form's Spalart-Allmaras drop run beside the closure's")
     ap.add_argument("--instances", type=int, default=64, help="c2: number of passages")
     ap.add_argument("--batch-seed", type=int, default=3)
     ap.add_argument("--batch-rpm", type=float, default=0.0,
@@ -116,7 +123,7 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="c3: time the chosen form only")
     ap.add_argument("--drop-max-iter", type=int, default=20000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=8)
+    ap.add_argument("--cpu-iters", type=int, default=24)
     ap.add_argument("--train-instances", type=int, default=56, help="closure-train: passages (48 train + 8 held out)")
     ap.add_argument("--train-develop", type=int, default=2500, help="closure-train: V-cycle iterations per passage")
     ap.add_argument("--closure-out", default=None, help="closure-train: where to write the network")
@@ -174,6 +181,27 @@ def cpu_baseline(iters: int):
             "sample": f"reference solver (oracle/_ref, g++ -O2, 1 thread) on a 64,512-cell vane passage of the "
                       f"same family (24x24 cross-section, Spalart-Allmaras, two-grid cycle), {t['iterations']} "
                       f"iterations in {t['loop_seconds']:.2f} s; host {model}, {ncpu} logical CPUs visible"}
+
+
+def cpu_all_cores(iters: int):
+    """The C restatement (oracle/, the checker of the GPU path) on every thread
+    OpenMP is given, on the same 64,512-cell sample: its cell and face loops
+    and the cells of each LU-SGS hyperplane run in parallel, with the same bits
+    as one thread.  The reference itself cannot run multi-threaded (its
+    convective face loops share one flag variable across OpenMP threads)."""
+    import oracle as O
+    from fv3d import cases
+    case = cases.vane_passage(ni=24, nk_blocks=(8, 16, 48, 24, 16), levels=0, max_iter=iters)
+    threads = O.set_threads(0)
+    o = O.Oracle(case)
+    t0 = time.perf_counter()
+    o.iterate(iters)
+    secs = time.perf_counter() - t0
+    del o
+    model, ncpu = host_cpu()
+    return {"value": case.n_cells * iters / secs, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"C restatement (oracle/fv3d_oracle.c, gcc -O2 -fopenmp, {threads} threads) on the same "
+                      f"64,512-cell vane passage, {iters} iterations in {secs:.2f} s; host {model}"}
 
 
 def load_closure(relax: float = 1.0):
@@ -255,10 +283,15 @@ def time_to_drop(case, precision, target, max_iter, closure=None, chunk=100):
     g.sync()
     secs = time.perf_counter() - t0
     tripped = g.mlp_tripped() if closure is not None else None
+    # every cycle iteration evaluates the closure on every fine cell twice (the
+    # iteration's residual and the one the restriction takes); the coarse
+    # start-up iterations do not evaluate it
+    evals = 2 * g.n_cells * (it - skip) if closure is not None else None
     g.close()
     val = lambda col: float(row[col]) if row is not None else None
     return {"iterations": hit, "seconds": secs if hit else None, "res0": r0, "mass_in": val(2), "mass_out": val(3),
-            "pressure_ratio": val(4), "efficiency": val(5), "gate_fallbacks": tripped,
+            "pressure_ratio": val(4), "efficiency": val(5), "gate_fallbacks": tripped, "closure_evaluations": evals,
+            "gate_fallback_fraction": tripped / evals if evals else None,
             "note": None if hit else f"not reached in {max_iter} iterations"}
 
 
@@ -280,6 +313,28 @@ def traffic_from_profiles(kernel: str, wkey: str):
     return None
 
 
+def valu_from_profiles(kernel: str, wkey: str):
+    """VALU wave instructions and FP64 flop per launch of a kernel on a
+    workload, from the 'valu:' lines of the rocprofv3 summaries in profiles/."""
+    if not os.path.isdir(PROFILES):
+        return None
+    for name in sorted(os.listdir(PROFILES)):
+        if not name.endswith(".md"):
+            continue
+        with open(os.path.join(PROFILES, name)) as f:
+            for line in f:
+                # valu: <kernel> <workload> <wave instructions> <fp64 flop>
+                p = line.strip().split()
+                if len(p) == 5 and p[0] == "valu:" and p[1] == kernel and p[2] == wkey:
+                    return float(p[3]), float(p[4])
+    return None
+
+
+# one VALU wave instruction per SIMD per 4 cycles: 256 CUs x 64 lanes x 2.4 GHz
+VALU_LANE_OPS = 256 * 64 * 2.4e9
+FP64_PEAK_TFLOPS = 78.6
+
+
 def roofline(r, precision, sa_transport, wkey):
     byt = kernel_bytes(precision, sa_transport)
     ks = {}
@@ -289,17 +344,26 @@ def roofline(r, precision, sa_transport, wkey):
             ach = b * r["cells"] / (ms * 1e-3) / 1e9
             ks[k] = {"kernel": KERNEL_NAMES[k], "ms": ms, "bytes_per_cell": b, "achieved": ach,
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic_from_profiles(k, wkey)}
+            v = valu_from_profiles(k, wkey)
+            if v:
+                # issue time of the counted VALU instructions at the full rate
+                # (transcendentals not weighted here) against the measured time
+                ks[k]["valu"] = {"wave_instructions": v[0], "issue_floor_ms": 1e3 * v[0] * 64 / VALU_LANE_OPS,
+                                 "issue_frac": (v[0] * 64 / VALU_LANE_OPS) / (ms * 1e-3),
+                                 "fp64_tflops": v[1] / (ms * 1e-3) / 1e12,
+                                 "fp64_frac": v[1] / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
     dom = max(ks, key=lambda k: ks[k]["ms"])
     d = ks[dom]
     return {"bound": "hbm", "kernel": d["kernel"], "achieved": d["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": d["frac"], "traffic": traffic_from_profiles(dom, wkey), "kernel_ms_avg": d["ms"],
             "bytes_per_cell": d["bytes_per_cell"], "dominant_by": "per-launch device time in this run",
+            "valu": d.get("valu"),
             "kernels": ks, "class_ms": r["class_ms"]}
 
 
 def c3_main(a):
     from fv3d import cases
-    closure = load_closure(a.closure_relax) if a.closure else None
+    closure = None if a.no_closure else load_closure(a.closure_relax)
     plain = cases.vane_passage(levels=0, max_iter=a.drop_max_iter)
     chosen = cases.vane_passage(levels=a.levels, max_iter=a.drop_max_iter)
     cells = plain.n_cells
@@ -316,16 +380,29 @@ def c3_main(a):
     if not a.no_drop:
         dc = time_to_drop(chosen, "fp32", 1e-6, a.drop_max_iter, closure)
         print(f"bench: chosen form drop {dc['iterations']} iterations, {dc['seconds']} s", file=sys.stderr, flush=True)
+        ds = None
+        if closure is not None and not a.no_chosen_sa:
+            ds = time_to_drop(chosen, "fp32", 1e-6, a.drop_max_iter)
+            print(f"bench: chosen form with Spalart-Allmaras drop {ds['iterations']} iterations, {ds['seconds']} s",
+                  file=sys.stderr, flush=True)
         dp = None
         if not a.no_plain:
             dp = time_to_drop(plain, "fp64", 1e-6, a.drop_max_iter)
             print(f"bench: plain FP64 drop {dp['iterations']} iterations, {dp['seconds']} s", file=sys.stderr, flush=True)
-        drop = {"chosen": dc, "plain_fp64": dp, "workload": "the headline workload, from the uniform start"}
+        drop = {"chosen": dc, "chosen_spalart_allmaras": ds, "plain_fp64": dp,
+                "workload": "the headline workload, from the uniform start"}
         if dp and dc["seconds"] and dp["seconds"]:
             drop["speedup_chosen_vs_plain"] = dp["seconds"] / dc["seconds"]
-            for q in ("mass_in", "pressure_ratio", "efficiency"):
+            # the vane is stationary: its 'efficiency' divides by a near-zero
+            # work term and is not a meaningful comparison, so it is not checked
+            for q in ("mass_in", "pressure_ratio"):
                 drop[q + "_rel_err"] = abs(dc[q] - dp[q]) / abs(dp[q])
+        if dp and ds and ds["seconds"] and dp["seconds"]:
+            drop["speedup_chosen_spalart_allmaras_vs_plain"] = dp["seconds"] / ds["seconds"]
+            for q in ("mass_in", "pressure_ratio"):
+                drop["chosen_spalart_allmaras_" + q + "_rel_err"] = abs(ds[q] - dp[q]) / abs(dp[q])
     cpu = None if a.no_cpu_baseline else cpu_baseline(a.cpu_iters)
+    cpu_all = None if a.no_cpu_baseline else cpu_all_cores(a.cpu_iters)
     chosen_desc = (f"FP32 defect correction + {a.levels}-level V-cycle + one launch per stage over all blocks in a "
                    "HIP graph, autotuned tiles" + (" + learned closure (Spalart-Allmaras fallback)" if closure else
                                                   " + Spalart-Allmaras transport (no trained closure present)"))
@@ -348,9 +425,10 @@ def c3_main(a):
                    "closure": closure["meta"] if closure else None},
         "plain_fp64": {"value": p["value"], "ms_per_step": p["ms_per_step"], "class_ms": p["class_ms"]} if p else None,
         "time_to_1e-6_drop": drop,
-        "roofline": roofline(r, "fp32", closure is None, "c3"),
+        "roofline": roofline(r, "fp32", closure is None, "c3" if closure is None else "c3_closure"),
         "roofline_plain_fp64": roofline(p, "fp64", True, "c3_fp64") if p else None,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
     }
     print(json.dumps(out))
 

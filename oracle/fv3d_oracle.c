@@ -13,10 +13,16 @@
 #include "fv3d_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
 #define ND 2
+/* Cell and face loops run on OpenMP threads for blocks of more than this many
+ * cells; every iteration writes only its own cell or face, so the results do
+ * not depend on the thread count. */
+#define PAR_CELLS 4096L
+#define PAR(b) ((long)(b)->IM * (b)->JM * (b)->KM > PAR_CELLS)
 #define F32(x) ((double)(x##f))
 
 typedef struct { double r, ru, rv, rw, re; } cv_t;
@@ -1183,6 +1189,8 @@ int orc_spectral(orc_ctx *c)
     for (int n = 0; n < c->nb; ++n) {
         blk_t *b = &c->blk[n];
         double w_rot = b->w_rot;
+        const int par = PAR(b);
+        #pragma omp parallel for schedule(static) if (par)
         for (int i = 1; i <= b->IM - 1; ++i)
             for (int j = 1; j <= b->JM - 1; ++j)
                 for (int k = 1; k <= b->KM - 1; ++k) {
@@ -1274,8 +1282,10 @@ static void tangential_diff(const blk_t *b, int D, int t, const int x[3], double
 static void gradients(blk_t *b)
 {
     int IM = b->IM, JM = b->JM, KM = b->KM;
+    const int par = PAR(b);
     for (int D = 0; D < 3; ++D) {
         const face_t *F = D == 0 ? b->fi : D == 1 ? b->fj : b->fk;
+        #pragma omp parallel for schedule(static) if (par)
         for (int i = 1; i <= IM - (D != 0); ++i)
             for (int j = 1; j <= JM - (D != 1); ++j)
                 for (int k = 1; k <= KM - (D != 2); ++k) {
@@ -1307,6 +1317,8 @@ static void gradients(blk_t *b)
 /* CFluidProps::Curl (source/Spectral_curl_grad.cpp:44-83) */
 static void curl(blk_t *b)
 {
+    const int par = PAR(b);
+    #pragma omp parallel for schedule(static) if (par)
     for (int k = 1; k <= b->KM - 1; ++k)
         for (int j = 1; j <= b->JM - 1; ++j)
             for (int i = 1; i <= b->IM - 1; ++i) {
@@ -1336,8 +1348,10 @@ static void viscous_flux(orc_ctx *c, blk_t *b)
     int IM = b->IM, JM = b->JM, KM = b->KM;
     double Gamma = c->nd.gamma, Rgas = c->nd.rgas, Pr_lam = c->nd.pr_lam, Pr_tur = c->nd.pr_tur;
     const double c43 = (double)(4.0f / 3.0f), c23 = (double)(2.0f / 3.0f);
+    const int par = PAR(b);
     for (int D = 0; D < 3; ++D) {
         const face_t *F = D == 0 ? b->fi : D == 1 ? b->fj : b->fk;
+        #pragma omp parallel for schedule(static) if (par)
         for (int i = 1; i <= IM - (D != 0); ++i)
             for (int j = 1; j <= JM - (D != 1); ++j)
                 for (int k = 1; k <= KM - (D != 2); ++k) {
@@ -1368,6 +1382,7 @@ static void viscous_flux(orc_ctx *c, blk_t *b)
                     b->face_fv[D][x] = fv;
                 }
     }
+    #pragma omp parallel for schedule(static) if (par)
     for (int i = 1; i <= IM - 1; ++i)
         for (int j = 1; j <= JM - 1; ++j)
             for (int k = 1; k <= KM - 1; ++k) {
@@ -1499,6 +1514,8 @@ static void sa_model(orc_ctx *c, blk_t *b)
     const double Cw1 = Cb1 / kavf / kavf + (F32(1.0) + Cb2) / sigma;
     const double Cw2 = F32(0.3), Cw3 = F32(2.0), Ct3 = F32(1.2), Ct4 = F32(0.5);
     int IM = b->IM, JM = b->JM, KM = b->KM;
+    const int par = PAR(b);
+    #pragma omp parallel for schedule(static) if (par)
     for (int i = 1; i <= IM - 1; ++i)
         for (int j = 1; j <= JM - 1; ++j)
             for (int k = 1; k <= KM - 1; ++k) {
@@ -1509,6 +1526,7 @@ static void sa_model(orc_ctx *c, blk_t *b)
                 b->mut[x] = sa_fv1(X, Cv1) * w->d * b->ev[x] / Re;
             }
     const double Max = 200.0;
+    #pragma omp parallel for schedule(static) if (par)
     for (int i = 1; i <= IM - 1; ++i)
         for (int j = 1; j <= JM - 1; ++j)
             for (int k = 1; k <= KM - 1; ++k) {
@@ -1524,6 +1542,7 @@ static void sa_model(orc_ctx *c, blk_t *b)
     /* faces of the three directions (lines 82-181) */
     for (int D = 0; D < 3; ++D) {
         const face_t *Fa = D == 0 ? b->fi : D == 1 ? b->fj : b->fk;
+        #pragma omp parallel for schedule(static) if (par)
         for (int i = 1; i <= IM - (D != 0); ++i)
             for (int j = 1; j <= JM - (D != 1); ++j)
                 for (int k = 1; k <= KM - (D != 2); ++k) {
@@ -1548,6 +1567,7 @@ static void sa_model(orc_ctx *c, blk_t *b)
                 }
     }
     /* source and viscous sum (lines 184-213) */
+    #pragma omp parallel for schedule(static) if (par)
     for (int i = 1; i <= IM - 1; ++i)
         for (int j = 1; j <= JM - 1; ++j)
             for (int k = 1; k <= KM - 1; ++k) {
@@ -1600,6 +1620,8 @@ static void wall_distance(orc_ctx *c)
 {
     for (int n = 0; n < c->nb; ++n) {
         blk_t *b = &c->blk[n];
+        const int par = PAR(b);
+        #pragma omp parallel for schedule(static) if (par)
         for (int i = 1; i <= b->IM - 1; ++i)
             for (int j = 1; j <= b->JM - 1; ++j)
                 for (int k = 1; k <= b->KM - 1; ++k) {
@@ -1634,6 +1656,8 @@ int orc_timestep(orc_ctx *c)
     for (int n = 0; n < c->nb; ++n) {
         blk_t *b = &c->blk[n];
         double dt_factor = b->flag_low_dt == 0 ? F32(1.0) : (double)(1.0f / (1.0f + b->flag_low_dt));
+        const int par = PAR(b);
+        #pragma omp parallel for schedule(static) if (par)
         for (int i = 1; i <= b->IM - 1; ++i)
             for (int j = 1; j <= b->JM - 1; ++j)
                 for (int k = 1; k <= b->KM - 1; ++k) {
@@ -1678,6 +1702,12 @@ static void stat_add(int which)
 {
 #pragma omp atomic
     g_stats[which]++;
+}
+
+int orc_set_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
 }
 
 void orc_stats(long *out)
@@ -2075,12 +2105,16 @@ int orc_convective(orc_ctx *c)
     for (int n = 0; n < c->nb; ++n) {
         blk_t *b = &c->blk[n];
         int IM = b->IM, JM = b->JM, KM = b->KM;
+        const int par = PAR(b);
+        #pragma omp parallel for schedule(static) if (par)
         for (int i = 1; i <= IM; ++i)
             for (int j = 1; j <= JM - 1; ++j)
                 for (int k = 1; k <= KM - 1; ++k) face_flux(c, b, 0, i, j, k);
+        #pragma omp parallel for schedule(static) if (par)
         for (int i = 1; i <= IM - 1; ++i)
             for (int j = 1; j <= JM; ++j)
                 for (int k = 1; k <= KM - 1; ++k) face_flux(c, b, 1, i, j, k);
+        #pragma omp parallel for schedule(static) if (par)
         for (int i = 1; i <= IM - 1; ++i)
             for (int j = 1; j <= JM - 1; ++j)
                 for (int k = 1; k <= KM; ++k) face_flux(c, b, 2, i, j, k);
@@ -2093,6 +2127,8 @@ int orc_convective(orc_ctx *c)
     for (int n = 0; n < c->nb; ++n) {
         blk_t *b = &c->blk[n];
         double w_rot = b->w_rot;
+        const int par = PAR(b);
+        #pragma omp parallel for schedule(static) if (par)
         for (int i = 1; i <= b->IM - 1; ++i)
             for (int j = 1; j <= b->JM - 1; ++j)
                 for (int k = 1; k <= b->KM - 1; ++k) {
@@ -2133,6 +2169,8 @@ int orc_resflux(orc_ctx *c)
     const cv_t zero = {0, 0, 0, 0, 0};
     for (int n = 0; n < c->nb; ++n) {
         blk_t *b = &c->blk[n];
+        const int par = PAR(b);
+        #pragma omp parallel for schedule(static) if (par)
         for (int i = 1; i <= b->IM - 1; ++i)
             for (int j = 1; j <= b->JM - 1; ++j)
                 for (int k = 1; k <= b->KM - 1; ++k) {
@@ -2184,8 +2222,11 @@ static void lusgs_sweeps(orc_ctx *c, blk_t *b)
     const int sa = c->cfg.turbulence == 200;
     double relax = (float)c->cfg.relax;
     const cv_t zero = {0, 0, 0, 0, 0};
-    /* source/LU_SGS_Marching.cpp:61-160 */
+    /* source/LU_SGS_Marching.cpp:61-160; the cells of one hyperplane depend
+     * only on the previous one */
+    const int par = PAR(b);
     for (int plane = 3; plane <= IM + JM + KM - 3; ++plane)
+        #pragma omp parallel for schedule(static) if (par)
         for (int i = 1; i <= IM - 1; ++i)
             for (int j = 1; j <= JM - 1; ++j) {
                 int k = plane - i - j;
@@ -2221,6 +2262,7 @@ static void lusgs_sweeps(orc_ctx *c, blk_t *b)
             }
     /* source/LU_SGS_Marching.cpp:162-260 */
     for (int plane = IM + JM + KM - 3; plane >= 3; --plane)
+        #pragma omp parallel for schedule(static) if (par)
         for (int i = IM - 1; i >= 1; --i)
             for (int j = JM - 1; j >= 1; --j) {
                 int k = plane - i - j;
@@ -2261,6 +2303,8 @@ static void lusgs_sweeps(orc_ctx *c, blk_t *b)
 static void primitive_vars(orc_ctx *c, blk_t *b)
 {
     double Gamma = c->nd.gamma, Rgas = c->nd.rgas;
+    const int par = PAR(b);
+    #pragma omp parallel for schedule(static) if (par)
     for (int i = 1; i < b->IM; ++i)
         for (int j = 1; j < b->JM; ++j)
             for (int k = 1; k < b->KM; ++k) {
@@ -2399,6 +2443,8 @@ int orc_lusgs(orc_ctx *c)
     for (int n = 0; n < c->nb; ++n) {
         blk_t *b = &c->blk[n];
         lusgs_sweeps(c, b);
+        const int par = PAR(b);
+        #pragma omp parallel for schedule(static) if (par)
         for (int i = 1; i <= b->IM - 1; ++i)
             for (int j = 1; j <= b->JM - 1; ++j)
                 for (int k = 1; k <= b->KM - 1; ++k) {

@@ -142,6 +142,11 @@ long orc_get(orc_ctx *c, int block, const char *name, double *out);
 #define ORC_STAT_LOW_DT 5
 void orc_stats(long *out);
 
+/* OpenMP threads of the cell and face loops (n > 0 sets them); returns the
+ * count in use.  Results do not depend on it: every parallel loop writes only
+ * its own cells or faces, and every sum keeps the serial order. */
+int orc_set_threads(int n);
+
 /* Test hooks: the face flux (scheme 200 Steger-Warming, 300 Roe) on face-frame
  * states, and the reconstruction (var 100 conservative / 200 primitive) of one
  * face from its four-cell stencil FF, F | B, BB.  Also orc_get "SI" "SJ" "SK":

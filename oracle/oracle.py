@@ -69,6 +69,7 @@ def lib():
         L.orc_initial.argtypes = [C.c_void_p]
         L.orc_set_nut.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_stats.argtypes = [C.c_void_p]
+        L.orc_set_threads.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -222,6 +223,12 @@ class Oracle:
         if n != out.size:
             raise RuntimeError(f"orc_get({name}) returned {n}, expected {out.size}")
         return out
+
+
+def set_threads(n: int = 0) -> int:
+    """OpenMP threads of the restatement's cell and face loops (n > 0 sets
+    them); returns the count in use.  Results are the same for any count."""
+    return int(lib().orc_set_threads(n))
 
 
 STAT_NAMES = ("recon_fallback", "limit_vars", "limit_vars_kn0", "limit_sa", "limit_sa_kn0", "low_dt")
