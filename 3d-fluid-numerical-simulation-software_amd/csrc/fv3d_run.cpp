@@ -1,18 +1,26 @@
 // fv3d_run: runs a reference case directory on the GPU through the C++ class
 // layer, in place of the reference's executable (source/Structured_FV3D.cpp:19-214).
 //
-//   fv3d_run <case_dir> [iterations] [--history FILE] [--eager] [--device N] [--fp32]
+//   fv3d_run <case_dir> [iterations] [--history FILE] [--eager] [--by-stage] [--device N] [--fp32]
+//            [--no-output]
 //   fv3d_run <case_dir> --parse-only
 //
 // The case directory holds config.dat, the grid and boundary-definition files
-// it names, and RestartSolution.dat when reuse_solution = Y (else the start
-// field is CMesh::Initial's, on the coarse level with UseMultigrid).  Without an iteration
-// count the config's max_iter is used.  The history ends at the first
-// iteration whose density residual is at or below the config's convergence
-// tolerance with inlet and outlet mass flows within 0.3 %, as the reference's
-// main loop does (the device may have run up to the end of that chunk).  --parse-only reads the case files and
-// prints a JSON summary without touching the GPU.  --fp32 runs the LU-SGS
-// operator in FP32 over the FP64 residual (defect correction; same fixed point).
+// it names, RestartSolution.dat when reuse_solution = Y (else the start field
+// is CMesh::Initial's, on the coarse level with UseMultigrid), and
+// WallDistance.dat when ReuseWallDistance = Y with Spalart-Allmaras.  Without
+// an iteration count the config's max_iter is used.  The run stops at the
+// first iteration whose density residual is at or below the config's
+// convergence tolerance with inlet and outlet mass flows within 0.3 %, as the
+// reference's main loop does: it iterates in chunks while the residual is
+// above 100 x the tolerance and one iteration at a time below, so the final
+// state is that iteration's.  It then writes RestartSolution.dat (and, with
+// Spalart-Allmaras, WallDistance.dat) into the case directory unless
+// --no-output.  --by-stage drives each iteration through the stage calls of
+// the two-grid cycle instead of one fused call; --eager skips graph replay;
+// --parse-only reads the case files and prints a JSON summary without
+// touching the GPU.  --fp32 runs the LU-SGS operator in FP32 over the FP64
+// residual (defect correction; same fixed point).
 #include <cinttypes>
 #include <cmath>
 #include <cstdio>

@@ -240,6 +240,66 @@ def vane_passage(ni: int = 88, nk_blocks: Sequence[int] = (24, 48, 144, 72, 40),
     return c
 
 
+def rotor_stator_stage(ni: int = 128, nj_rotor: int = 128, nj_stator: int = 96,
+                       rotor_nk: Sequence[int] = (24, 40, 64, 120, 64, 40, 24),
+                       stator_nk: Sequence[int] = (24, 48, 72, 144, 96, 64, 32),
+                       nblade_rotor: int = 22, nblade_stator: int = 30, rpm: float = 3000.0,
+                       camber_rotor: float = 0.08, camber_stator: float = -0.06, p_ratio: float = 0.95,
+                       seed: int = 1, turbulence: int = 200, levels: int = 0, coarse_iter: int = 2,
+                       max_iter: int = 500) -> Case:
+    """A rotor row and a stator row joined by a mixing plane: two annular
+    passages (hub 0.2 m, tip 0.25 m, the same radial cells) of nblade_rotor and
+    nblade_stator blades, nj_rotor and nj_stator cells pitchwise, cut along k
+    into the blocks of rotor_nk and stator_nk cells (128 x (128, 96) x
+    (376, 480) = 12,058,624 cells in 14 blocks by default; every size a
+    multiple of 8, so the grid halves three times).  The rotor spins at rpm
+    with its hub; its blades (the middle block) follow the relative flow of
+    the axial inflow at zero incidence on every radius (a helix at -omega/V_x,
+    V_x from the isentropic outlet Mach number) with a turning vane of
+    camber_rotor pitches on top; the stator's vane (its middle block) turns by
+    camber_stator pitches.  Blocks 1..len(rotor_nk) are the rotor row, the
+    rest the stator row; the rotor's last block (K high) meets the stator's
+    first (K low) at the mixing plane (source/BndCondMixingPlane.cpp:287-522).
+    Navier-Stokes with Spalart-Allmaras, Roe + MUSCL3 on primitive variables,
+    LU-SGS with local dt at CFL 5, inlet total conditions, outlet at
+    p_ratio * Pt, multigrid from the uniform field (``levels`` grid levels, 0:
+    the reference's two-grid cycle)."""
+    nr, ns = len(rotor_nk), len(stator_nk)
+    zr = tuple(np.linspace(-0.2, 0.0, nr + 1))
+    zs = tuple(np.linspace(0.0, 0.26, ns + 1))
+    # axial velocity of the isentropic flow at the outlet pressure ratio
+    g = 1.4
+    mach = np.sqrt(2.0 / (g - 1.0) * (p_ratio ** (-(g - 1.0) / g) - 1.0))
+    temp = 288.15 / (1.0 + 0.5 * (g - 1.0) * mach * mach)
+    vx = mach * np.sqrt(g * 287.0 * temp)
+    omega = rpm * 2.0 * np.pi / 60.0
+    kw = dict(ni=ni, seed=seed, viscous=True, len_ref=1.0, max_iter=max_iter, profile="turn", p_ratio=p_ratio)
+    R = annular_passage(nj=nj_rotor, nk_blocks=rotor_nk, nblade=nblade_rotor, rpm=rpm, z_extent=zr,
+                        blade_blocks=(nr // 2,), camber=camber_rotor, helix=-omega / vx, **kw)
+    S = annular_passage(nj=nj_stator, nk_blocks=stator_nk, nblade=nblade_stator, rpm=0.0, z_extent=zs,
+                        blade_blocks=(ns // 2,), camber=camber_stator, **kw)
+    nodes = R.nodes + S.nodes
+    blocks = R.blocks + [BlockInfo(bi.bid + nr, bi.im, bi.jm, bi.km, bi.nblade, bi.rpm) for bi in S.blocks]
+    states = R.states + S.states
+    patches = [p for p in R.patches if p.kind != "outlet"]
+    for p in S.patches:
+        if p.kind != "inlet":
+            patches.append(dataclasses.replace(p, block=p.block + nr, block2=p.block2 + nr if p.block2 else 0))
+    km1 = R.nodes[nr - 1].shape[2]
+    patches.append(Patch("mixing", nr, 6, (1, ni, 1, nj_rotor, km1, km1), orient=1, block2=nr + 1, face2=5,
+                         rng2=(1, ni, 1, nj_stator, 1, 1)))
+    kind_order = {"inlet": 0, "outlet": 1, "wall": 2, "periodic": 3, "interior": 4, "mixing": 5}
+    patches.sort(key=lambda p: kind_order[p.kind])
+    c = Case(R.cfg, nodes, patches, blocks, states)
+    if turbulence == 200:
+        c = with_sa(c, amp=0.0)
+    else:
+        c.cfg.turbulence = turbulence
+    c.cfg.use_multigrid, c.cfg.reuse_solution, c.cfg.initial = "Y", "N", "U"
+    c.cfg.coarse_iter, c.cfg.mg_levels = coarse_iter, levels
+    return c
+
+
 def from_meta(meta: Dict) -> Case:
     """The case a fixture's metadata describes: ``kwargs`` for bump_channel (or
     for annular_passage when ``generator`` names it, with ``turn`` a list of
@@ -272,7 +332,8 @@ def annular_passage(ni: int = 8, nj: int = 8, nk_blocks: Sequence[int] = (16,), 
                     beta: float = 1.5, cfl: float = 5.0, max_iter: int = 500, flux: int = 300,
                     recon_scheme: int = 500, recon_var: int = 200, p_ratio: float = 0.95,
                     viscous: bool = False, len_ref: float = 1.0, rotate_walls: bool = True,
-                    periodic_from_high: bool = False, stagger: float = 0.0, profile: str = "arc") -> Case:
+                    periodic_from_high: bool = False, stagger: float = 0.0, profile: str = "arc",
+                    helix: float = 0.0) -> Case:
     """Annular-sector blade passage: i radial (hub I- to tip I+, tanh-clustered),
     j pitchwise over one blade pitch 2*pi/nblade, k axial.  The passage is cut
     along k into blocks of ``nk_blocks`` cells joined by interior patches
@@ -293,7 +354,11 @@ def annular_passage(ni: int = 8, nj: int = 8, nk_blocks: Sequence[int] = (16,), 
     and opposite edge angles); "turn" makes it a turning vane, the parabola
     theta = camber * pitch * t^2, which meets an axial inflow at zero
     incidence and turns it; the blocks downstream of it are shifted by the
-    vane's pitchwise offset so the interfaces stay conforming."""
+    vane's pitchwise offset so the interfaces stay conforming.  ``helix``
+    (radians per metre) turns every surface of constant j by helix * z about
+    the axis, the same angle at every radius: a rotor row whose blade meets the
+    relative flow of an axial inflow V_x at rotation omega at zero incidence
+    on every radius has helix = -omega / V_x."""
     nb = len(nk_blocks)
     if not z_extent:
         z_extent = [float(b) for b in range(nb + 1)]
@@ -321,7 +386,7 @@ def annular_passage(ni: int = 8, nj: int = 8, nk_blocks: Sequence[int] = (16,), 
             th0 = np.zeros(km)
         zmid = 0.5 * (z_extent[0] + z_extent[-1])
         shear = np.tan(np.radians(stagger)) * (z[None, None, :] - zmid) / r[:, None, None]
-        th = th0[None, None, :] + pitch * np.linspace(0.0, 1.0, jm)[None, :, None] + shear
+        th = th0[None, None, :] + pitch * np.linspace(0.0, 1.0, jm)[None, :, None] + shear + helix * z[None, None, :]
         xyz = np.empty((im, jm, km, 3))
         rr = r[:, None, None]
         xyz[..., 0] = rr * np.cos(th)

@@ -98,7 +98,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["c3", "c2", "passage5", "bump", "closure-train"], default="c3")
+    ap.add_argument("--workload", choices=["c3", "c2", "c4", "passage5", "bump", "closure-train"], default="c3")
     ap.add_argument("--levels", type=int, default=4, help="grid levels of the chosen form's V-cycle")
     ap.add_argument("--no-closure", action="store_true",
                     help="chosen form with the Spalart-Allmaras transport instead of the learned closure "
@@ -123,6 +123,8 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="c3: time the chosen form only")
     ap.add_argument("--drop-max-iter", type=int, default=20000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-converged", action="store_true",
+                    help="skip the converged-field error of both forms on C0 against the reference state")
     ap.add_argument("--cpu-iters", type=int, default=24)
     ap.add_argument("--train-instances", type=int, default=56, help="closure-train: passages (48 train + 8 held out)")
     ap.add_argument("--train-develop", type=int, default=2500, help="closure-train: V-cycle iterations per passage")
@@ -162,14 +164,68 @@ def host_cpu():
     return model, os.cpu_count()
 
 
-def cpu_baseline(iters: int):
-    """The compiled reference solver, one thread, on a 64,512-cell member of the
-    vane-passage family (24 x 24 x 112 cells, Spalart-Allmaras, two-grid)."""
-    import oracle as O
+CONVERGED_C0 = os.path.join(REPO, "tests", "golden", "c0_converged.npz")
+
+
+def converged_field_error():
+    """Max and L2 error (relative to each field's largest magnitude) of every
+    converged conservative field of both GPU forms on C0 (the 96x32x32 bump
+    channel, single grid) against the reference's FP64 converged state
+    (tests/golden/c0_converged.npz, written by the C restatement, which is the
+    compiled reference bit for bit), after the same number of iterations."""
+    import numpy as np
     from fv3d import cases
+    from fv3d.solver import GpuSolver
+    if not os.path.exists(CONVERGED_C0):
+        return None
+    z = np.load(CONVERGED_C0)
+    meta = json.loads(bytes(z["meta"]).decode())
+    n = int(meta["iterations"])
+    ref = z["cv"]
+    out = {"workload": "C0 bump channel 96x32x32, Euler, Roe + MUSCL3 primitive, LU-SGS local dt CFL 5, single grid",
+           "iterations": n, "reference_drop": meta["drop"]}
+    for form, prec in (("plain_fp64", "fp64"), ("chosen_fp32_operator", "fp32")):
+        case = cases.bump_channel(max_iter=n + 10, **meta["kwargs"])
+        g = GpuSolver(case, device=0, precision=PREC[prec])
+        done = 0
+        while done < n:
+            k = min(5000, n - done)
+            g.iterate(k, graph=True)
+            done += k
+        cv = g.get(1, "cv")[1:-1, 1:-1, 1:-1]
+        g.close()
+        errs = {}
+        for q, name in enumerate(("rho", "rho_u", "rho_v", "rho_w", "rho_E")):
+            scale = float(np.abs(ref[..., q]).max())
+            d = cv[..., q] - ref[..., q]
+            errs[name] = {"max": float(np.abs(d).max()) / scale, "l2": float(np.sqrt(np.mean(d * d))) / scale}
+        out[form] = errs
+        print(f"bench: converged C0 {form}: max error {max(e['max'] for e in errs.values()):.2e}",
+              file=sys.stderr, flush=True)
+    return out
+
+
+def cpu_sample(wkey: str, iters: int):
+    """A bounded member of the workload's family for the CPU legs, and its
+    description: the vane passage at a 24 x 24 cross-section (64,512 cells), or
+    the stage at 24 x (24, 18) cells (56,736 cells), Spalart-Allmaras, the
+    reference's two-grid cycle."""
+    from fv3d import cases
+    if wkey == "c4":
+        c = cases.rotor_stator_stage(ni=24, nj_rotor=24, nj_stator=18, rotor_nk=(4, 6, 8, 16, 8, 6, 4),
+                                     stator_nk=(4, 6, 10, 18, 12, 8, 4), max_iter=iters)
+        return c, f"{c.n_cells:,}-cell rotor + stator stage of the same family (24 x (24, 18) cross-section)"
+    c = cases.vane_passage(ni=24, nk_blocks=(8, 16, 48, 24, 16), levels=0, max_iter=iters)
+    return c, f"{c.n_cells:,}-cell vane passage of the same family (24x24 cross-section)"
+
+
+def cpu_baseline(iters: int, wkey: str = "c3"):
+    """The compiled reference solver, one thread, on the workload's CPU sample
+    (Spalart-Allmaras, two-grid)."""
+    import oracle as O
     if not os.path.exists(O.REF_BIN):
         return None
-    case = cases.vane_passage(ni=24, nk_blocks=(8, 16, 48, 24, 16), levels=0, max_iter=iters)
+    case, desc = cpu_sample(wkey, iters)
     with tempfile.TemporaryDirectory() as td:
         cdir, odir = os.path.join(td, "case"), os.path.join(td, "out")
         case.write(cdir)
@@ -178,20 +234,19 @@ def cpu_baseline(iters: int):
     model, ncpu = host_cpu()
     return {"value": t["cells"] * t["iterations"] / t["loop_seconds"], "unit": "cell-updates/s", "cores": 1,
             "kind": "reference",
-            "sample": f"reference solver (oracle/_ref, g++ -O2, 1 thread) on a 64,512-cell vane passage of the "
-                      f"same family (24x24 cross-section, Spalart-Allmaras, two-grid cycle), {t['iterations']} "
-                      f"iterations in {t['loop_seconds']:.2f} s; host {model}, {ncpu} logical CPUs visible"}
+            "sample": f"reference solver (oracle/_ref, g++ -O2, 1 thread) on a {desc}, Spalart-Allmaras, "
+                      f"two-grid cycle, {t['iterations']} iterations in {t['loop_seconds']:.2f} s; host {model}, "
+                      f"{ncpu} logical CPUs visible"}
 
 
-def cpu_all_cores(iters: int):
+def cpu_all_cores(iters: int, wkey: str = "c3"):
     """The C restatement (oracle/, the checker of the GPU path) on every thread
     OpenMP is given, on the same 64,512-cell sample: its cell and face loops
     and the cells of each LU-SGS hyperplane run in parallel, with the same bits
     as one thread.  The reference itself cannot run multi-threaded (its
     convective face loops share one flag variable across OpenMP threads)."""
     import oracle as O
-    from fv3d import cases
-    case = cases.vane_passage(ni=24, nk_blocks=(8, 16, 48, 24, 16), levels=0, max_iter=iters)
+    case, desc = cpu_sample(wkey, iters)
     threads = O.set_threads(0)
     o = O.Oracle(case)
     t0 = time.perf_counter()
@@ -201,7 +256,7 @@ def cpu_all_cores(iters: int):
     model, ncpu = host_cpu()
     return {"value": case.n_cells * iters / secs, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"C restatement (oracle/fv3d_oracle.c, gcc -O2 -fopenmp, {threads} threads) on the same "
-                      f"64,512-cell vane passage, {iters} iterations in {secs:.2f} s; host {model}"}
+                      f"{desc}, {iters} iterations in {secs:.2f} s; host {model}"}
 
 
 def load_closure(relax: float = 1.0):
@@ -363,13 +418,28 @@ def roofline(r, precision, sa_transport, wkey):
 
 def c3_main(a):
     from fv3d import cases
-    closure = None if a.no_closure else load_closure(a.closure_relax)
     plain = cases.vane_passage(levels=0, max_iter=a.drop_max_iter)
     chosen = cases.vane_passage(levels=a.levels, max_iter=a.drop_max_iter)
+    wname = (f"vane passage 88x88x(24,48,144,72,40) ({plain.n_cells:,} cells; blocks 0.19M-1.12M), hub 0.2 m "
+             "tip 0.25 m, 22 blades, turning vane theta = 0.1 pitch t^2, Navier-Stokes + Spalart-Allmaras, "
+             "Roe + MUSCL3 primitive, LU-SGS local dt CFL 5, outlet 0.95 Pt, stationary")
+    passage_main(a, plain, chosen, wname, "c3")
+
+
+def c4_main(a):
+    from fv3d import cases
+    plain = cases.rotor_stator_stage(levels=0, max_iter=a.drop_max_iter)
+    chosen = cases.rotor_stator_stage(levels=a.levels, max_iter=a.drop_max_iter)
+    wname = (f"rotor + stator stage, 14 blocks ({plain.n_cells:,} cells; blocks 0.4M-2.0M): rotor 22 blades "
+             "128x128x(24,40,64,120,64,40,24) at 3000 rpm, blades on the relative-flow helix, stator 30 blades "
+             "128x96x(24,48,72,144,96,64,32), mixing plane between the rows, hub 0.2 m tip 0.25 m, "
+             "Navier-Stokes + Spalart-Allmaras, Roe + MUSCL3 primitive, LU-SGS local dt CFL 5, outlet 0.95 Pt")
+    passage_main(a, plain, chosen, wname, "c4")
+
+
+def passage_main(a, plain, chosen, wname, wkey):
+    closure = None if a.no_closure else load_closure(a.closure_relax)
     cells = plain.n_cells
-    wname = (f"vane passage 88x88x(24,48,144,72,40) ({cells:,} cells; blocks 0.19M-1.12M), hub 0.2 m tip 0.25 m, "
-             "22 blades, turning vane theta = 0.1 pitch t^2, Navier-Stokes + Spalart-Allmaras, Roe + MUSCL3 "
-             "primitive, LU-SGS local dt CFL 5, outlet 0.95 Pt, stationary")
     r = time_form(chosen, "fp32", a, closure)
     print(f"bench: chosen form timed, {r['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     p = None
@@ -401,8 +471,9 @@ def c3_main(a):
             drop["speedup_chosen_spalart_allmaras_vs_plain"] = dp["seconds"] / ds["seconds"]
             for q in ("mass_in", "pressure_ratio"):
                 drop["chosen_spalart_allmaras_" + q + "_rel_err"] = abs(ds[q] - dp[q]) / abs(dp[q])
-    cpu = None if a.no_cpu_baseline else cpu_baseline(a.cpu_iters)
-    cpu_all = None if a.no_cpu_baseline else cpu_all_cores(a.cpu_iters)
+    conv = None if (a.no_converged or a.no_drop) else converged_field_error()
+    cpu = None if a.no_cpu_baseline else cpu_baseline(a.cpu_iters, wkey)
+    cpu_all = None if a.no_cpu_baseline else cpu_all_cores(a.cpu_iters, wkey)
     chosen_desc = (f"FP32 defect correction + {a.levels}-level V-cycle + one launch per stage over all blocks in a "
                    "HIP graph, autotuned tiles" + (" + learned closure (Spalart-Allmaras fallback)" if closure else
                                                   " + Spalart-Allmaras transport (no trained closure present)"))
@@ -425,8 +496,9 @@ def c3_main(a):
                    "closure": closure["meta"] if closure else None},
         "plain_fp64": {"value": p["value"], "ms_per_step": p["ms_per_step"], "class_ms": p["class_ms"]} if p else None,
         "time_to_1e-6_drop": drop,
-        "roofline": roofline(r, "fp32", closure is None, "c3" if closure is None else "c3_closure"),
-        "roofline_plain_fp64": roofline(p, "fp64", True, "c3_fp64") if p else None,
+        "converged_field_error": conv,
+        "roofline": roofline(r, "fp32", closure is None, wkey if closure is None else wkey + "_closure"),
+        "roofline_plain_fp64": roofline(p, "fp64", True, wkey + "_fp64") if p else None,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
     }
@@ -580,11 +652,13 @@ def main():
         calibrate_copy()
         return
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
+        print(json.dumps({"error": "one GPU only: --gpus must be 1 (the multi-GPU sharding is not implemented)",
                           "n_gpus": a.gpus}))
         sys.exit(2)
     if a.workload == "c3":
         c3_main(a)
+    elif a.workload == "c4":
+        c4_main(a)
     elif a.workload == "c2":
         c2_main(a)
     elif a.workload == "closure-train":

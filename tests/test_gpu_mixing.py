@@ -38,7 +38,21 @@ def mg_case():
     return cases.with_mixing_plane(c, 2, 3)
 
 
-@pytest.mark.parametrize("make", [euler_case, sa_case], ids=["euler", "sa"])
+def stage_case():
+    # rotor (8 cells pitchwise, 22 blades, 3000 rpm) and stator (5 cells, 30
+    # blades, stationary): the two sides of the mixing plane differ in cell
+    # count, pitch and rotation
+    c = cases.rotor_stator_stage(ni=6, nj_rotor=8, nj_stator=5, rotor_nk=(4, 6, 4), stator_nk=(3, 7, 4), max_iter=8)
+    c.cfg.use_multigrid = "N"
+    return c
+
+
+def stage_mg_case():
+    return cases.rotor_stator_stage(ni=6, nj_rotor=8, nj_stator=6, rotor_nk=(4, 6, 4), stator_nk=(4, 8, 4),
+                                    coarse_iter=2, max_iter=8)
+
+
+@pytest.mark.parametrize("make", [euler_case, sa_case, stage_case], ids=["euler", "sa", "stage"])
 def test_ghosts_after_bc(make):
     case = make()
     print(f"case mixing plane bc {make.__name__}", flush=True)
@@ -51,7 +65,8 @@ def test_ghosts_after_bc(make):
             np.testing.assert_allclose(g.get(b, f), o.get(b, f), rtol=1e-13, atol=1e-15, err_msg=f"block {b} {f}")
 
 
-@pytest.mark.parametrize("make", [euler_case, sa_case, mg_case], ids=["euler", "sa", "two-grid"])
+@pytest.mark.parametrize("make", [euler_case, sa_case, mg_case, stage_case, stage_mg_case],
+                         ids=["euler", "sa", "two-grid", "stage", "stage-two-grid"])
 def test_history_matches_oracle(make):
     case = make()
     n = case.cfg.max_iter

@@ -44,7 +44,9 @@ def test_forward_matches_restatement():
     yr, _, _ = M.forward(P, x, True)
     err = np.abs(y - yr) / (1 + np.abs(yr))
     print(f"  median {np.median(err):.2e} p99 {np.quantile(err, 0.99):.2e} max {err.max():.2e}")
-    assert np.quantile(err, 0.99) <= 1e-3 and err.max() <= 3e-2
+    # measured: p99 9.7e-8, max 8.8e-5 (a BF16 rounding of an activation that
+    # falls the other way on the two sides); bars ten times above
+    assert np.quantile(err, 0.99) <= 1e-6 and err.max() <= 1e-3
 
 
 def test_gradient_matches_restatement():
@@ -63,7 +65,7 @@ def test_gradient_matches_restatement():
         a, b = gg[o:o + s], gr[o:o + s]
         rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
         print(f"  {name}: relative L2 error {rel:.2e}")
-        assert rel <= 2e-2, name
+        assert rel <= 5e-4, name   # measured: at most 4.9e-5 (W0)
 
 
 def test_training_reduces_the_loss():

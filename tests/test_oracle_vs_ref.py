@@ -148,6 +148,12 @@ def variant(name):
         c = cases.with_cell_values(c, 1, (slice(0, 2), slice(0, 2), slice(0, 2)), nut=1e5)
         c = cases.with_cell_values(c, 1, (0, 0, 0), nut=3000.0)
         c = cases.with_cell_values(c, 1, (4, 3, 6), nut=1e5)
+    elif name == "stage_mixing_plane":
+        # single grid: rotor and stator rows of different pitchwise cell
+        # counts, blade counts and speeds at a mixing plane, Spalart-Allmaras
+        c = cases.rotor_stator_stage(ni=6, nj_rotor=8, nj_stator=5, rotor_nk=(4, 6, 4), stator_nk=(3, 7, 4),
+                                     max_iter=8)
+        c.cfg.use_multigrid = "N"
     elif name == "char_muscl3_roe":
         c = _shifted(cases.bump_channel(ni=6, nj=5, nk=10, seed=42, recon_var=300, max_iter=12))
     elif name == "char_nnd_sw_ns":
@@ -189,7 +195,7 @@ MUST_FIRE = {
                                   "sa_sw_nnd_limiter_centrifugal", "mixing_plane_euler", "mixing_plane_sa",
                                   "limiter_low_pressure", "limiter_dense_corner", "limiter_sa_high",
                                   "char_muscl3_roe", "char_nnd_sw_ns", "char_muscl2u_low_pressure",
-                                  "char_muscl2o_rotating_passage", "char_muscl2c_bl"])
+                                  "char_muscl2o_rotating_passage", "char_muscl2c_bl", "stage_mixing_plane"])
 def test_restatement_matches_reference(name):
     case = variant(name)
     n = case.cfg.max_iter
@@ -274,6 +280,12 @@ def mg_variant(name):
         c.cfg.turbulence = 100
         c.cfg.coarse_iter, c.cfg.initial, c.cfg.outlet_angle = 2, "I", (0.0, 0.2, 1.0)
         c = cases.with_mixing_plane(c, 2, 3)
+    elif name == "stage_mixing_plane_sa":
+        # a rotor row (8 cells pitchwise, 22 blades, 3000 rpm) and a stator row
+        # (6 cells, 30 blades, stationary) joined by a mixing plane whose two
+        # sides differ in cell count, pitch and rotation
+        c = cases.rotor_stator_stage(ni=6, nj_rotor=8, nj_stator=6, rotor_nk=(4, 6, 4), stator_nk=(4, 8, 4),
+                                     coarse_iter=2, max_iter=8)
     elif name == "not_coarsenable":
         # 7 cells along i: the patch ranges end on an even node, one level only
         c = cases.bump_channel(ni=7, nj=8, nk=16, seed=35, max_iter=8)
@@ -291,7 +303,7 @@ MG_COARSE = ["cv", "pv", "res", "dq", "dt", "lamcI", "lamcJ", "lamcK"]
 
 @pytest.mark.parametrize("name", ["bump_uniform", "bump_interpolate_sw_limiter", "bl_bump", "bl_passage",
                                   "bl_passage_turn1", "global_dt_axial_passage", "sa_passage_global_dt_axial",
-                                  "not_coarsenable", "bl_passage_mixing_plane"])
+                                  "not_coarsenable", "bl_passage_mixing_plane", "stage_mixing_plane_sa"])
 def test_two_grid_matches_reference(name):
     case = mg_variant(name)
     n = case.cfg.max_iter
