@@ -159,7 +159,7 @@ struct fv3d_ctx {
     // gate counter, training scratch; closure on = mue_t from the network in
     // place of the nu~ transport
     float *d_mlp = nullptr, *d_mlp_m = nullptr, *d_mlp_v = nullptr, *d_mlp_g = nullptr, *d_mlp_bounds = nullptr;
-    unsigned *d_mlp_tripped = nullptr;
+    unsigned long long *d_mlp_tripped = nullptr;
     float *d_mlp_xc = nullptr;   // closure: 8 floats per cell (features, gate flag)
     long mlp_step = 0;
     bool mlp_closure = false;
@@ -1670,12 +1670,14 @@ static void launch_sa_faces(fv3d_ctx *c)
 // residual (face gradients from nu~ as it stands, fluxes and source from its
 // clipped value), then mue_t and the clipping stored, then mue_t of the ghost
 // cells.  save_old: the LU-SGS step keeps nu~ before the clipping in ev_old.
-static int launch_sa(fv3d_ctx *c, int save_old)
+static int launch_sa(fv3d_ctx *c, int save_old, bool mut_only)
 {
     const int nb = (int)c->blocks.size();
     const unsigned gc = (unsigned)((c->ncell + 255) / 256);
-    hipLaunchKernelGGL(k_curl, dim3(gc), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb);
-    launch_sa_faces(c);
+    if (!mut_only) {
+        hipLaunchKernelGGL(k_curl, dim3(gc), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb);
+        launch_sa_faces(c);
+    }
     hipLaunchKernelGGL(k_sa_mut, dim3(gc), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb, c->gas, save_old);
     hipLaunchKernelGGL(k_sa_ghost, dim3((unsigned)((c->nghost + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
                        c->d_gstart, nb, c->gas);
@@ -1686,8 +1688,12 @@ static int launch_sa(fv3d_ctx *c, int save_old)
 // Turbulence model before the time step (source/ComputeAllBlockVars.cpp:
 // 228-232): the vorticity, then Baldwin-Lomax on all walls at once (one launch
 // per line group size) and the merge of the per-cell minima; or
-// Spalart-Allmaras.
-static int launch_turbulence(fv3d_ctx *c, int save_old = 0)
+// Spalart-Allmaras.  mut_only: Spalart-Allmaras forms mue_t (with the
+// clipping of nu~ and mue_t and the ghost values) but not the nu~ residual
+// and the vorticity it needs -- for the two-grid cycle's second fine
+// residual, whose nu~ residual nothing reads before the next iteration's
+// first residual forms it again.
+static int launch_turbulence(fv3d_ctx *c, int save_old = 0, bool mut_only = false)
 {
     if (c->cfg.turbulence == 200 && c->mlp_closure) {
         // the learned closure in place of the nu~ transport; ghost mue_t as for SA
@@ -1704,7 +1710,7 @@ static int launch_turbulence(fv3d_ctx *c, int save_old = 0)
         HIPCK(c, hipGetLastError());
         return FV3D_OK;
     }
-    if (c->cfg.turbulence == 200) return launch_sa(c, save_old);
+    if (c->cfg.turbulence == 200) return launch_sa(c, save_old, mut_only);
     if (c->cfg.turbulence != 100) return FV3D_OK;
     const int nb = (int)c->blocks.size();
     hipLaunchKernelGGL(k_curl, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks, c->d_cstart, nb);
@@ -2137,8 +2143,10 @@ static int mg_iteration(fv3d_ctx *c, int phase, double *row)
         return launch_finalize(m, row);
     }
     if ((rc = lusgs_timed(c))) return rc;
+    // the fine residual at the new state, for the restriction (the coarse
+    // levels carry no turbulence model, so the nu~ residual is not needed)
     if ((rc = launch_bc(c))) return rc;
-    if ((rc = launch_turbulence(c))) return rc;
+    if ((rc = launch_turbulence(c, 0, true))) return rc;
     if ((rc = launch_spectral_dt(c))) return rc;
     if ((rc = launch_convective(c))) return rc;
     if ((rc = launch_viscous(c))) return rc;
@@ -2754,7 +2762,7 @@ long fv3d_mlp_tripped(fv3d_ctx *c)
 {
     READY(c);
     if (!c->d_mlp_tripped) return 0;
-    unsigned v = 0, z = 0;
+    unsigned long long v = 0, z = 0;
     HIPCK(c, hipStreamSynchronize(c->stream));
     HIPCK(c, hipMemcpy(&v, c->d_mlp_tripped, sizeof v, hipMemcpyDeviceToHost));
     HIPCK(c, hipMemcpy(c->d_mlp_tripped, &z, sizeof z, hipMemcpyHostToDevice));

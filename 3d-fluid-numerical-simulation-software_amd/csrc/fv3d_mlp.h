@@ -24,7 +24,7 @@ __global__ void k_mlp_cell_features(const BlockDev *blocks, const long *cstart, 
                                     const float *bounds, float *xc);
 __global__ void k_mlp_closure(const float *P, long ncell, float *xc);
 __global__ void k_mlp_mut(const BlockDev *blocks, const long *cstart, int nblock, Gas g, const float *bounds,
-                          const float *xc, unsigned *tripped, double relax);
+                          const float *xc, unsigned long long *tripped, double relax);
 __global__ void k_mlp_train_chunk(const float *P, const float *x, const float *tgt, long n, float inv_n, float *partial,
                                   float *loss);
 __global__ void k_mlp_reduce(const float *partial, const float *loss, int nchunk, float *grad, float *loss_sum);

@@ -389,34 +389,40 @@ __global__ void __launch_bounds__(256) k_mlp_closure(const float *__restrict__ P
 // nu~ / Re, clipped as SA_Model clips it); tripped counts those cells.
 __global__ void __launch_bounds__(256) k_mlp_mut(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
                                                  int nblock, Gas g, const float *__restrict__ bounds,
-                                                 const float *__restrict__ xc, unsigned *__restrict__ tripped,
-                                                 double relax)
+                                                 const float *__restrict__ xc,
+                                                 unsigned long long *__restrict__ tripped, double relax)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int n, i, j, k;
-    if (!cell_item(blocks, cstart, nblock, c, n, i, j, k)) return;
-    const BlockDev &B = blocks[n];
-    const long box = B.box, x = cidx(B, i, j, k);
-    const float yhat = xc[8 * c + 7];
-    const bool ok = xc[8 * c + 6] != 0.0f;
-    const double mue_lam = sutherland(g, B.W[5 * box + x]);
-    double mt;
-    if (ok && isfinite(yhat) && yhat <= bounds[2 * kMlpIn]) {
-        mt = mue_lam * expm1((double)yhat);
-    } else {
-        atomicAdd(tripped, 1u);
-        // Spalart-Allmaras fallback from the frozen nu~ (source/SA_Model.cpp:47-87)
-        const double rho = B.W[x], ev = B.ev[x] < 0.0 ? 0.0 : B.ev[x];
-        const double ev_lam = g.re_ref * mue_lam / rho, X = ev / ev_lam;
-        const double Cv1 = F32C(7.1);
-        const double fv1 = pow(X, F32C(3.0)) / (pow(X, F32C(3.0)) + Cv1 * Cv1 * Cv1);
-        mt = fv1 * rho * ev / g.re_ref;
-        if (mt < 0.0) mt = 0.0;
+    bool trip = false;
+    if (cell_item(blocks, cstart, nblock, c, n, i, j, k)) {
+        const BlockDev &B = blocks[n];
+        const long box = B.box, x = cidx(B, i, j, k);
+        const float yhat = xc[8 * c + 7];
+        const bool ok = xc[8 * c + 6] != 0.0f;
+        const double mue_lam = sutherland(g, B.W[5 * box + x]);
+        double mt;
+        if (ok && isfinite(yhat) && yhat <= bounds[2 * kMlpIn]) {
+            mt = mue_lam * expm1((double)yhat);
+        } else {
+            trip = true;
+            // Spalart-Allmaras fallback from the frozen nu~ (source/SA_Model.cpp:47-87)
+            const double rho = B.W[x], ev = B.ev[x] < 0.0 ? 0.0 : B.ev[x];
+            const double ev_lam = g.re_ref * mue_lam / rho, X = ev / ev_lam;
+            const double Cv1 = F32C(7.1);
+            const double fv1 = pow(X, F32C(3.0)) / (pow(X, F32C(3.0)) + Cv1 * Cv1 * Cv1);
+            mt = fv1 * rho * ev / g.re_ref;
+            if (mt < 0.0) mt = 0.0;
+        }
+        if (mue_lam > 0 && mt > 200.0 * mue_lam) mt = 200.0 * mue_lam;
+        // under-relaxation of the algebraic closure: the fixed point is unchanged
+        if (relax != 1.0) mt = B.mut[x] + relax * (mt - B.mut[x]);
+        B.mut[x] = mt;
     }
-    if (mue_lam > 0 && mt > 200.0 * mue_lam) mt = 200.0 * mue_lam;
-    // under-relaxation of the algebraic closure: the fixed point is unchanged
-    if (relax != 1.0) mt = B.mut[x] + relax * (mt - B.mut[x]);
-    B.mut[x] = mt;
+    // one atomic per wave for the fallback count (every lane reaches this
+    // point: the grid is whole 256-thread workgroups and no lane returns)
+    const unsigned long long m = __ballot(trip);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(tripped, (unsigned long long)__popcll(m));
 }
 
 // ------------------------------------------------------------ training

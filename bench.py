@@ -120,7 +120,9 @@ def parse():
     ap.add_argument("--viscous", action="store_true", help="bump: laminar Navier-Stokes")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-drop", action="store_true", help="skip the time-to-1e-6-drop runs")
-    ap.add_argument("--no-plain", action="store_true", help="c3: time the chosen form only")
+    ap.add_argument("--no-plain", action="store_true", help="c3/c4: time the chosen form only")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="c3/c4: time the plain FP64 form and its drop only (a long plain run in a call of its own)")
     ap.add_argument("--drop-max-iter", type=int, default=20000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-converged", action="store_true",
@@ -440,6 +442,21 @@ def c4_main(a):
 def passage_main(a, plain, chosen, wname, wkey):
     closure = None if a.no_closure else load_closure(a.closure_relax)
     cells = plain.n_cells
+    if a.plain_only:
+        p = time_form(plain, "fp64", a)
+        print(f"bench: plain FP64 timed, {p['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+        dp = None if a.no_drop else time_to_drop(plain, "fp64", 1e-6, a.drop_max_iter)
+        if dp:
+            print(f"bench: plain FP64 drop {dp['iterations']} iterations, {dp['seconds']} s", file=sys.stderr, flush=True)
+        print(json.dumps({"metric": METRIC, "value": p["value"], "unit": "cell-updates/s", "n_gpus": 1,
+                          "steps": a.steps, "warmup": a.warmup, "ms_per_step": p["ms_per_step"],
+                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                          "data": "synthetic",
+                          "config": {"workload": wname, "cells": cells, "form": "plain FP64 only, the reference's "
+                                     "two-grid cycle, Spalart-Allmaras transport"},
+                          "time_to_1e-6_drop": {"plain_fp64": dp},
+                          "roofline_plain_fp64": roofline(p, "fp64", True, wkey + "_fp64")}))
+        return
     r = time_form(chosen, "fp32", a, closure)
     print(f"bench: chosen form timed, {r['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     p = None

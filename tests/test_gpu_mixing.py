@@ -48,7 +48,7 @@ def stage_case():
 
 
 def stage_mg_case():
-    return cases.rotor_stator_stage(ni=6, nj_rotor=8, nj_stator=6, rotor_nk=(4, 6, 4), stator_nk=(4, 8, 4),
+    return cases.rotor_stator_stage(ni=6, nj_rotor=8, nj_stator=6, rotor_nk=(6, 8, 6), stator_nk=(6, 10, 6),
                                     coarse_iter=2, max_iter=8)
 
 

@@ -9,6 +9,11 @@
   (the shipped network, mue_t under-relaxed by 0.2): both reach a 1e-6 drop
   of Res.dens, and the closure's mass flow and pressure ratio there are within
   0.1 % of Spalart-Allmaras'; the gate's fallback count is reported.
+* The rotor + stator stage (14 blocks, 12.06M cells, a mixing plane between
+  rows of different blade counts, pitchwise cell counts and speeds): the
+  plain two-grid FP64 form and the four-level FP32-operator form run with
+  finite histories, and the four-level form's graph replay is bit for bit its
+  eager execution.
 """
 import os
 
@@ -80,3 +85,24 @@ def test_c3_closure_against_spalart_allmaras_full_size():
         rel = abs(cl[col] - sa[col]) / abs(sa[col])
         print(f"{name}: closure {cl[col]:.8g}, Spalart-Allmaras {sa[col]:.8g}, relative difference {rel:.2e}", flush=True)
         assert rel <= 1e-3
+
+
+def test_c4_stage_full_size():
+    case = cases.rotor_stator_stage(levels=4, max_iter=100)
+    plain = cases.rotor_stator_stage(levels=0, max_iter=100)
+    print(f"rotor + stator stage, {case.n_cells:,} cells, {len(case.blocks)} blocks", flush=True)
+    assert len(case.blocks) == 14 and case.n_cells > 12_000_000
+    g = GpuSolver(plain, precision=0)
+    assert g.levels == 2
+    h = g.iterate(10, graph=True)
+    assert np.all(np.isfinite(h[:, 1:6]))
+    g.close()
+    a = GpuSolver(case, precision=1)
+    assert a.levels == 4
+    ha = a.iterate(8)
+    a.close()
+    b = GpuSolver(case, precision=1)
+    hb = np.concatenate([b.iterate(3, graph=True), b.iterate(5, graph=True)])
+    assert np.all(np.isfinite(hb[:, 1:6]))
+    np.testing.assert_array_equal(ha, hb)
+    b.close()

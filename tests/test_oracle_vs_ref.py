@@ -284,7 +284,7 @@ def mg_variant(name):
         # a rotor row (8 cells pitchwise, 22 blades, 3000 rpm) and a stator row
         # (6 cells, 30 blades, stationary) joined by a mixing plane whose two
         # sides differ in cell count, pitch and rotation
-        c = cases.rotor_stator_stage(ni=6, nj_rotor=8, nj_stator=6, rotor_nk=(4, 6, 4), stator_nk=(4, 8, 4),
+        c = cases.rotor_stator_stage(ni=6, nj_rotor=8, nj_stator=6, rotor_nk=(6, 8, 6), stator_nk=(6, 10, 6),
                                      coarse_iter=2, max_iter=8)
     elif name == "not_coarsenable":
         # 7 cells along i: the patch ranges end on an even node, one level only
