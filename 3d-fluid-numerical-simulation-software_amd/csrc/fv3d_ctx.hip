@@ -131,10 +131,8 @@ struct fv3d_ctx {
     KbaTile *d_kba[2] = {nullptr, nullptr};
     int nkba = 0;
     int *d_kba_sync = nullptr;
-    int kba_chunk = 1;
-    int kba_lag = 0;
-    int sweep_mode = 4;   // 0 one launch per hyperplane, 1..4 wavefront forms (3: four waves per tile,
-                          // 4: the same on the tile-skewed row layout)
+    int sweep_mode = 4;   // 0 one launch per hyperplane (the reference's schedule, a cross-check),
+                          // 4 the persistent wavefront on the tile-skewed row layout
     int kba_grid = 0;
     // two-grid cycle: the coarse level is a context of its own on the same
     // stream (coarse is set on the fine level, parent on the coarse one).  The
@@ -1770,7 +1768,6 @@ static int launch_sweeps_kba(fv3d_ctx *c)
     long nsync = 64 + 2L * c->nkba * kDonePad - 32;
     hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((nsync + 255) / 256)), dim3(256), 0, c->stream, c->d_kba_sync + 32, nsync, 0);
     int *err = c->d_kba_sync, *tf = c->d_kba_sync + 32, *tb = c->d_kba_sync + 48;
-    int *df = c->d_kba_sync + 64, *db = df + (long)c->nkba * kDonePad;
     if (c->sweep_mode == 4) {
         const bool f32 = c->cfg.precision == FV3D_DEFECT_FP32;
         const bool visc = c->cfg.viscous;
@@ -1828,29 +1825,6 @@ static int launch_sweeps_kba(fv3d_ctx *c)
         sweep(false);
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[8], c->stream));
         unpack();
-    } else if (c->sweep_mode == 3) {
-        hipLaunchKernelGGL((k_lusgs_kba3<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
-                           c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_lag);
-        launch_sa_sweep(c, true);
-        hipLaunchKernelGGL((k_lusgs_kba3<false, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(256), 0, c->stream,
-                           c->d_blocks, c->d_kba[1], c->nkba, tb, db, err, c->gas.relax, c->gas, c->kba_lag);
-        launch_sa_sweep(c, false);
-    } else if (c->sweep_mode == 2) {
-        hipLaunchKernelGGL((k_lusgs_kba2<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(kKbaTJ * kKbaTK), 0,
-                           c->stream, c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_chunk,
-                           c->kba_lag);
-        launch_sa_sweep(c, true);
-        hipLaunchKernelGGL((k_lusgs_kba2<false, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(kKbaTJ * kKbaTK), 0,
-                           c->stream, c->d_blocks, c->d_kba[1], c->nkba, tb, db, err, c->gas.relax, c->gas, c->kba_chunk,
-                           c->kba_lag);
-        launch_sa_sweep(c, false);
-    } else {
-        hipLaunchKernelGGL((k_lusgs_kba<true, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(kKbaTJ * kKbaTK), 0,
-                           c->stream, c->d_blocks, c->d_kba[0], c->nkba, tf, df, err, c->gas.relax, c->gas, c->kba_chunk);
-        launch_sa_sweep(c, true);
-        hipLaunchKernelGGL((k_lusgs_kba<false, kKbaTJ, kKbaTK>), dim3((unsigned)c->kba_grid), dim3(kKbaTJ * kKbaTK), 0,
-                           c->stream, c->d_blocks, c->d_kba[1], c->nkba, tb, db, err, c->gas.relax, c->gas, c->kba_chunk);
-        launch_sa_sweep(c, false);
     }
     HIPCK(c, hipGetLastError());
     return FV3D_OK;
@@ -2773,17 +2747,13 @@ int fv3d_set_debug(fv3d_ctx *c, int flags)
 {
     READY(c);
     c->store_faces = flags & 1;
-    c->sweep_mode = (flags & 2) ? 0 : (flags & 4) ? 1 : (flags & 8) ? 2 : (flags & 16) ? 3 : 4;
-    if (c->cfg.viscous && c->sweep_mode >= 1 && c->sweep_mode <= 3) {
-        c->sweep_mode = 4;
-        return fail(c, FV3D_EINVAL, "the viscous terms run with the default sweep form or the per-hyperplane form");
-    }
+    if (flags & ~3) return fail(c, FV3D_EINVAL, "debug flags 0x%x: only bits 0 (face capture) and 1 (one launch "
+                                                 "per hyperplane) exist", flags);
+    c->sweep_mode = (flags & 2) ? 0 : 4;
     if (c->cfg.precision == FV3D_DEFECT_FP32 && c->sweep_mode != 4) {
         c->sweep_mode = 4;
         return fail(c, FV3D_EINVAL, "the FP32 operator runs with the default sweep form only");
     }
-    if ((flags >> 8) & 0xff) c->kba_chunk = (flags >> 8) & 0xff;
-    c->kba_lag = (flags >> 16) & 0xff;
     if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
     if (c->store_faces && !c->d_faces[0]) {
         long mx = 0;

@@ -112,15 +112,6 @@ constexpr int kSQ_Q = 0, kSQ_P = 5, kSQ_LAM = 6, kSQ_SWG = 9, kSQ_R = 24, kSQ_A 
 constexpr int kSweepD = 6;
 // what k_sweep_fill writes: the state quantities and/or the static metrics
 constexpr int kGatherState = 1, kGatherMetrics = 2;
-template <bool FWD, int TJ, int TK>
-__global__ void k_lusgs_kba(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *done, int *err,
-                            double relax, Gas g, int chunk);
-template <bool FWD, int TJ, int TK>
-__global__ void k_lusgs_kba2(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *done, int *err,
-                             double relax, Gas g, int chunk, int lag);
-template <bool FWD, int TJ, int TK>
-__global__ void k_lusgs_kba3(const BlockDev *blocks, const KbaTile *tiles, int ntiles, int *ticket, int *done, int *err,
-                             double relax, Gas g, int lag);
 // T: double (plain FP64) or float (the FP32 operator of defect correction)
 // SA: the nu~ equation in the same sweep (Navier-Stokes only)
 template <bool FWD, int TJ, int TK, bool VISC, bool SA, class T>

@@ -587,173 +587,14 @@ __global__ void k_lusgs_plane(const BlockDev *__restrict__ blocks, const long *_
 template __global__ void k_lusgs_plane<true>(const BlockDev *, const long *, int, int, double, Gas);
 template __global__ void k_lusgs_plane<false>(const BlockDev *, const long *, int, int, double, Gas);
 
-// Persistent wavefront form of the same sweeps.  The march axis is i; the
-// (j,k) cross-section is cut into TJ x TK tiles, one wave per tile.  Inside a
-// tile thread (tj,tk) handles cell i = s - tj - tk + 1 at step s (mirrored for
-// the backward sweep), so every step depends only on the previous one: the
-// i-neighbour is the thread's own last result, the j/k-neighbours come from LDS
-// or, at the tile edge, from the upstream tile through global memory.  Tiles
-// are handed out by a ticket counter in topological order, so a tile waits
-// only on tiles already held by running waves; progress counters are published
-// with agent-scope release stores and polled with bounded spins.
-template <bool FWD, int TJ, int TK>
-__global__ void __launch_bounds__(TJ *TK) k_lusgs_kba(const BlockDev *__restrict__ blocks, const KbaTile *__restrict__ tiles,
-                                                      int ntiles, int *__restrict__ ticket, int *__restrict__ done,
-                                                      int *__restrict__ err, double relax, Gas g, int chunk)
-{
-    static_assert(TJ * TK == 64, "one wave per tile");
-    __shared__ double sdq[2][5][TJ][TK];
-    const int t = threadIdx.x;
-    const int tj = t / TK, tk = t % TK;
-    const double Gamma = g.gamma;
-    for (;;) {
-        int q = 0;
-        if (t == 0) q = atomicAdd(ticket, 1);
-        q = __shfl(q, 0);
-        if (q >= ntiles) return;
-        const KbaTile tl = tiles[q];
-        const BlockDev &B = blocks[tl.block];
-        const long box = B.box;
-        // mirrored lane position for the backward sweep
-        const int lj = FWD ? tj : TJ - 1 - tj, lk = FWD ? tk : TK - 1 - tk;
-        const int j = tl.j0 + tj + 1, k = tl.k0 + tk + 1;
-        const bool col = j <= B.nj && k <= B.nk;
-        const int nsteps = B.ni + TJ + TK - 2;
-        int seen_j = tl.dep_j < 0 ? 1 << 30 : 0, seen_k = tl.dep_k < 0 ? 1 << 30 : 0;
-        double prev[5] = {0, 0, 0, 0, 0};
-        for (int s = 0; s < nsteps; ++s) {
-            const int need_j = min(s + TJ, nsteps), need_k = min(s + TK, nsteps);
-            if (seen_j < need_j || seen_k < need_k) {
-                long spins = 0;
-                while (seen_j < need_j) {
-                    seen_j = __hip_atomic_load(done + tl.dep_j * kDonePad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (seen_j >= need_j) break;
-                    __builtin_amdgcn_s_sleep(1);
-                    if (++spins > (1L << 22)) { if (t == 0) atomicExch(err, 1); seen_j = 1 << 30; }
-                }
-                while (seen_k < need_k) {
-                    seen_k = __hip_atomic_load(done + tl.dep_k * kDonePad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (seen_k >= need_k) break;
-                    __builtin_amdgcn_s_sleep(1);
-                    if (++spins > (1L << 22)) { if (t == 0) atomicExch(err, 1); seen_k = 1 << 30; }
-                }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            }
-            const int li = s - lj - lk;            // 0-based position along the march axis
-            const int i = FWD ? li + 1 : B.ni - li;
-            const bool valid = col && li >= 0 && li < B.ni;
-            double out[5];
-            if (valid) {
-                const long x = cidx(B, i, j, k);
-                double alpha = B.vol[x] / B.dt[x] + relax * (B.lamc[x] + B.lamc[box + x] + B.lamc[2 * box + x]) +
-                               F32C(2.0) * (0.0 + 0.0 + 0.0);
-                const int pos[3] = {i, j, k};
-                const int lim[3] = {B.ni, B.nj, B.nk};
-                const long st[3] = {B.si, B.sj, 1};
-                double A[3][5], V[3][5];
-                for (int d = 0; d < 3; ++d) {
-                    const bool ok = FWD ? pos[d] != 1 : pos[d] != lim[d];
-                    if (!ok) {
-                        for (int c = 0; c < 5; ++c) { A[d][c] = 0.0; V[d][c] = 0.0; }
-                        continue;
-                    }
-                    const long nb = FWD ? x - st[d] : x + st[d];
-                    double dqn[5];
-                    if (d == 0) {
-                        for (int c = 0; c < 5; ++c) dqn[c] = prev[c];
-                    } else if (d == 1 ? tj != (FWD ? 0 : TJ - 1) : tk != (FWD ? 0 : TK - 1)) {
-                        const int pj = d == 1 ? tj + (FWD ? -1 : 1) : tj, pk = d == 2 ? tk + (FWD ? -1 : 1) : tk;
-                        for (int c = 0; c < 5; ++c) dqn[c] = sdq[(s - 1) & 1][c][pj][pk];
-                    } else {
-                        for (int c = 0; c < 5; ++c) dqn[c] = B.dQ[c * box + nb];
-                    }
-                    // delt_Fn at the neighbour with its increment dqn
-                    const long fa = nb + st[d], fb = nb;
-                    const double *S = B.fS[d], *sa = B.fs[d];
-                    double nx = (S[fa] / sa[fa] + S[fb] / sa[fb]) / F32C(2.0);
-                    double ny = (S[box + fa] / sa[fa] + S[box + fb] / sa[fb]) / F32C(2.0);
-                    double nz = (S[2 * box + fa] / sa[fa] + S[2 * box + fb] / sa[fb]) / F32C(2.0);
-                    double uvel_e = F32C(-1.0) * B.cc[box + nb] * B.w_rot, vvel_e = B.cc[nb] * B.w_rot;
-                    double o[5], nw[5];
-                    for (int c = 0; c < 5; ++c) { o[c] = B.Q[c * box + nb]; nw[c] = o[c] + dqn[c]; }
-                    double dens = nw[0], uvel = nw[1] / nw[0], vvel = nw[2] / nw[0], wvel = nw[3] / nw[0], E = nw[4] / nw[0];
-                    double press_new = (Gamma - F32C(1.0)) * dens * (E - F32C(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel));
-                    double press_old = B.W[4 * box + nb];
-                    double Vn_old = o[1] / o[0] * nx + o[2] / o[0] * ny + o[3] / o[0] * nz;
-                    double Vn_new = uvel * nx + vvel * ny + wvel * nz;
-                    double Vn_e = uvel_e * nx + vvel_e * ny + F32C(0.0) * nz;
-                    double df[5];
-                    df[0] = nw[0] * (Vn_new - Vn_e) - o[0] * (Vn_old - Vn_e);
-                    df[1] = (nw[1] * (Vn_new - Vn_e) + press_new * nx) - (o[1] * (Vn_old - Vn_e) + press_old * nx);
-                    df[2] = (nw[2] * (Vn_new - Vn_e) + press_new * ny) - (o[2] * (Vn_old - Vn_e) + press_old * ny);
-                    df[3] = (nw[3] * (Vn_new - Vn_e) + press_new * nz) - (o[3] * (Vn_old - Vn_e) + press_old * nz);
-                    df[4] = ((nw[4] + press_new) * (Vn_new - Vn_e) + press_new * Vn_e) -
-                            ((o[4] + press_old) * (Vn_old - Vn_e) + press_old * Vn_e);
-                    double sd = F32C(0.5) * (sa[fb] + sa[fa]);
-                    double rl = relax * B.lamc[d * box + nb];
-                    for (int c = 0; c < 5; ++c) {
-                        A[d][c] = FWD ? (df[c] * sd + dqn[c] * rl) * F32C(0.5) : (df[c] * sd - dqn[c] * rl) * F32C(0.5);
-                        V[d][c] = dqn[c] * 0.0;
-                    }
-                }
-                for (int c = 0; c < 5; ++c) {
-                    double v;
-                    if (FWD) {
-                        v = B.R[c * box + x] * F32C(-1.0) + A[0][c];
-                        v = v + A[1][c]; v = v + A[2][c];
-                    } else {
-                        v = B.dQ[c * box + x] * alpha - A[0][c];
-                        v = v - A[1][c]; v = v - A[2][c];
-                    }
-                    v = v + V[0][c]; v = v + V[1][c]; v = v + V[2][c];
-                    out[c] = v / alpha;
-                    B.dQ[c * box + x] = out[c];
-                    sdq[s & 1][c][tj][tk] = out[c];
-                    prev[c] = out[c];
-                }
-            }
-            __syncthreads();
-            if ((s + 1) % chunk == 0 || s + 1 == nsteps) {
-                if (t == 0) __hip_atomic_store(done + q * kDonePad, s + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-    }
-}
-
-template __global__ void k_lusgs_kba<true, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
-template __global__ void k_lusgs_kba<false, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
-
-// Third form of the wavefront sweep: four waves per tile of TJ x TK = 64
-// cells.  Waves 0, 1, 2 each own one neighbour direction (i, j, k): one step
-// ahead they load the neighbour's state and metrics (wave 0 also the cell's own
-// right-hand side and diagonal), then compute the flux difference and the
-// off-diagonal terms of the step's 64 cells and leave them in LDS.  Wave 3
-// combines them into dQ, keeps dQ in LDS for the next step and stores it.
-// Wave 3 issues no loads, so before it stores step s it waits only for the
-// stores of step s-1 and then publishes s completed steps.
-//
-// Each wave kind runs its own straight-line loop (the direction loops are
-// specialised per direction at compile time) and every load is issued
-// unconditionally with indices clamped into the block: the compiler's
-// memory-counter waits are then exact, so a load issued one step ahead is
-// waited for only when its value is used.  The loop is unrolled by two over
-// two prefetch buffers so that no prefetched value is copied in flight.
-typedef const double __attribute__((address_space(1))) *gcd_t;
-typedef double __attribute__((address_space(1))) *gd_t;
-typedef int __attribute__((address_space(1))) *gi_t;
 // the row-layout sweep in the arithmetic type T
 template <class T>
 using gcT = const T __attribute__((address_space(1))) *;
 template <class T>
 using gT = T __attribute__((address_space(1))) *;
 #define TC(x) ((T)(x##f))
-#define GLD(p, idx) (((gcd_t)(p))[idx])
 
 namespace {
-struct SweepPf {
-    double q[5], p, l, m[5], x[5], r[5], a;
-    int pend;
-};
 // row-layout sweep: per step, the neighbour's static data with the terms of
 // the flux difference that do not depend on dQ, the upstream dQ, and (wave 0)
 // the cell's right-hand side and diagonal
@@ -783,222 +624,6 @@ __device__ inline T *mbox_base(const BlockDev &B, bool fwd)
 }
 }  // namespace
 
-template <bool FWD, int TJ, int TK>
-__global__ void __launch_bounds__(256) k_lusgs_kba3(const BlockDev *__restrict__ blocks, const KbaTile *__restrict__ tiles,
-                                                    int ntiles, int *__restrict__ ticket, int *__restrict__ done,
-                                                    int *__restrict__ err, double relax, Gas g, int lag)
-{
-    static_assert(TJ * TK == 64, "64 cells per tile");
-    __shared__ double sdq[2][5][64];
-    __shared__ double sA[3][5][64];
-    __shared__ double sV[3][5][64];
-    __shared__ double sR[6][64];
-    __shared__ int s_q;
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
-    const int tj = lane / TK, tk = lane % TK;
-    const double Gamma = g.gamma;
-    for (;;) {
-        if (threadIdx.x == 0) s_q = atomicAdd(ticket, 1);
-        __syncthreads();
-        const int q = __builtin_amdgcn_readfirstlane(s_q);
-        __syncthreads();
-        if (q >= ntiles) return;
-        const KbaTile tl = tiles[q];
-        const BlockDev &B = blocks[tl.block];
-        const int ni = B.ni, nj = B.nj, nk = B.nk, si = B.si, sj = B.sj, box = (int)B.box;
-        const int lj = FWD ? tj : TJ - 1 - tj, lk = FWD ? tk : TK - 1 - tk;
-        const int j = tl.j0 + tj + 1, k = tl.k0 + tk + 1;
-        const bool col = j <= nj && k <= nk;
-        const int jk = min(j, nj) * sj + min(k, nk);
-        const int nsteps = ni + TJ + TK - 2;
-        constexpr int NB = kSweepAhead + 1;   // prefetch buffers = unrolling of the step loop
-        const int send = (nsteps + kSweepAhead + NB - 1) / NB * NB - kSweepAhead;
-        // cell of this lane at a step, clamped into the block when the lane is idle
-        auto cell_at = [&](int step, int &i, int &x) -> bool {
-            const int l2 = step - lj - lk;
-            const int l2c = min(max(l2, 0), ni - 1);
-            i = FWD ? l2c + 1 : ni - l2c;
-            x = i * si + jk;
-            return col && l2 >= 0 && l2 < ni;
-        };
-        auto bar = []() {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-            __builtin_amdgcn_s_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-        };
-        auto dir_loop = [&](auto dc) {
-            constexpr int d = decltype(dc)::value;
-            constexpr int T = d == 1 ? TJ : TK;
-            const gcd_t Qg = (gcd_t)B.Q, Wg = (gcd_t)B.W, Lg = (gcd_t)B.lamc, Mg = (gcd_t)B.swg, Ag = (gcd_t)B.alpha;
-            const gcd_t Rg = (gcd_t)(FWD ? (const double *)B.R : (const double *)B.dQ);
-            const gd_t Dg = (gd_t)B.dQ;
-            const int st = d == 0 ? si : d == 1 ? sj : 1;
-            const int dep = d == 1 ? tl.dep_j : d == 2 ? tl.dep_k : -1;
-            const bool edge = d == 1 ? tj == (FWD ? 0 : TJ - 1) : d == 2 ? tk == (FWD ? 0 : TK - 1) : false;
-            const gi_t dep_ctr = (gi_t)(done + max(dep, 0) * kDonePad);
-            int seen = dep < 0 ? 1 << 30 : 0;
-            // wait for the upstream tile, then load everything `step` needs: the
-            // counter first, so that waiting for it later never waits for the
-            // data loads issued after it
-            auto issue = [&](SweepPf &bn, int step, int pend) {
-                if (d != 0) {
-                    // every path consumes the early look, and not before this point
-                    // (the empty asm keeps the compiler from hoisting the read)
-                    asm volatile("" : "+v"(pend));
-                    seen = max(seen, pend);
-                    if (dep >= 0) {
-                        const int need = min(step + T + lag, nsteps);
-                        long spins = 0;
-                        while (seen < need) {
-                            seen = __hip_atomic_load(dep_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            if (seen >= need) break;
-                            __builtin_amdgcn_s_sleep(1);
-                            if (++spins > (1L << 22)) { atomicExch(err, 1); seen = 1 << 30; }
-                        }
-                    }
-                }
-                int i, x;
-                cell_at(step, i, x);
-                const int nb = FWD ? x - st : x + st;
-                if (d != 0) {
-                    // early look at the upstream counter, consumed kSweepAhead steps later
-                    bn.pend = __hip_atomic_load(dep_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    // upstream dQ for the edge lanes (the other lanes' values go unused)
-                    for (int c = 0; c < 5; ++c)
-                        bn.x[c] = __hip_atomic_load(Dg + c * box + nb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                } else {
-                    bn.pend = 0;
-                }
-                for (int c = 0; c < 5; ++c) bn.q[c] = Qg[c * box + nb];
-                bn.p = Wg[4 * box + nb];
-                bn.l = Lg[d * box + nb];
-                for (int c = 0; c < 5; ++c) bn.m[c] = Mg[(d * 5 + c) * box + nb];
-                if (d == 0) {
-                    for (int c = 0; c < 5; ++c) bn.r[c] = Rg[c * box + x];
-                    bn.a = Ag[x];
-                }
-            };
-            auto work = [&](const SweepPf &b, int s) {
-                int i, x;
-                const bool valid = cell_at(s, i, x);
-                const int p = d == 0 ? i : d == 1 ? j : k, lim = d == 0 ? ni : d == 1 ? nj : nk;
-                const bool use = valid && (FWD ? p != 1 : p != lim);
-                // Branch-free: every lane evaluates the flux difference on its
-                // (clamped) data and idle lanes select zero, so each prefetched
-                // register is read on every path.
-                const int pl = (d == 0 || edge) ? lane
-                                                : d == 1 ? lane + (FWD ? -TK : TK) : lane + (FWD ? -1 : 1);
-                double dqn[5];
-                for (int c = 0; c < 5; ++c) {
-                    const double in_tile = sdq[(s - 1) & 1][c][pl];
-                    dqn[c] = (d != 0 && edge) ? b.x[c] : in_tile;
-                }
-                const double nx = b.m[0], ny = b.m[1], nz = b.m[2], sd = b.m[3], Vn_e = b.m[4];
-                const double *o = b.q;
-                const double cL = relax * b.l;
-                double nw[5];
-                for (int c = 0; c < 5; ++c) nw[c] = o[c] + dqn[c];
-                double dens = nw[0], uvel = nw[1] / nw[0], vvel = nw[2] / nw[0], wvel = nw[3] / nw[0], E = nw[4] / nw[0];
-                double press_new = (Gamma - F32C(1.0)) * dens * (E - F32C(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel));
-                double press_old = b.p;
-                double Vn_old = o[1] / o[0] * nx + o[2] / o[0] * ny + o[3] / o[0] * nz;
-                double Vn_new = uvel * nx + vvel * ny + wvel * nz;
-                double df[5];
-                df[0] = nw[0] * (Vn_new - Vn_e) - o[0] * (Vn_old - Vn_e);
-                df[1] = (nw[1] * (Vn_new - Vn_e) + press_new * nx) - (o[1] * (Vn_old - Vn_e) + press_old * nx);
-                df[2] = (nw[2] * (Vn_new - Vn_e) + press_new * ny) - (o[2] * (Vn_old - Vn_e) + press_old * ny);
-                df[3] = (nw[3] * (Vn_new - Vn_e) + press_new * nz) - (o[3] * (Vn_old - Vn_e) + press_old * nz);
-                df[4] = ((nw[4] + press_new) * (Vn_new - Vn_e) + press_new * Vn_e) -
-                        ((o[4] + press_old) * (Vn_old - Vn_e) + press_old * Vn_e);
-                double A[5], V[5];
-                for (int c = 0; c < 5; ++c) {
-                    const double a = FWD ? (df[c] * sd + dqn[c] * cL) * F32C(0.5) : (df[c] * sd - dqn[c] * cL) * F32C(0.5);
-                    const double v = dqn[c] * 0.0;   // lamv * dQ, lamv = 0 on the inviscid path
-                    A[c] = use ? a : 0.0;
-                    V[c] = use ? v : 0.0;
-                }
-                for (int c = 0; c < 5; ++c) { sA[d][c][lane] = A[c]; sV[d][c][lane] = V[c]; }
-                if (d == 0) {
-                    for (int c = 0; c < 5; ++c) sR[c][lane] = b.r[c];
-                    sR[5][lane] = b.a;
-                }
-                bar();   // terms of step s are in LDS
-                bar();   // dQ of step s is in LDS
-            };
-            // kSweepAhead steps in flight: step t's data is loaded while step
-            // t - kSweepAhead is computed; the loop is unrolled over the buffers
-            // The loop runs the steps [-kSweepAhead, send): the idle steps at
-            // either end (every lane invalid, the combine wave runs them too)
-            // let no load precede the loop and make the step count a multiple
-            // of the unrolling, so every step runs the same code.
-            SweepPf b[NB] = {};
-            // nothing in flight on entry (loads left over from the previous
-            // tile would otherwise make the compiler wait inside the loop)
-            __builtin_amdgcn_s_waitcnt(0);
-            for (int s0 = -kSweepAhead; s0 < send; s0 += NB) {
-#pragma unroll
-                for (int u = 0; u < NB; ++u) {
-                    const int s = s0 + u;
-                    // the counter look taken kSweepAhead steps ago: waiting for it
-                    // never waits for a load issued after it
-                    issue(b[(u + kSweepAhead) % NB], s + kSweepAhead, b[u].pend);
-                    work(b[u], s);
-                }
-            }
-        };
-        if (wave == 0) {
-            dir_loop(Dir<0>());
-        } else if (wave == 1) {
-            dir_loop(Dir<1>());
-        } else if (wave == 2) {
-            dir_loop(Dir<2>());
-        } else {
-            // ---- combine wave (source/LU_SGS_Marching.cpp:155, 255)
-            const gd_t Dg = (gd_t)B.dQ;
-            const bool pub = (tj == (FWD ? TJ - 1 : 0)) || (tk == (FWD ? TK - 1 : 0));
-            const gi_t my_ctr = (gi_t)(done + q * kDonePad);
-            for (int s = -kSweepAhead; s < send; ++s) {
-                int i, x;
-                const bool valid = cell_at(s, i, x);
-                double outv[5];
-                bar();
-                if (valid) {
-                    const double alpha = sR[5][lane];
-                    for (int c = 0; c < 5; ++c) {
-                        double v;
-                        if (FWD) {
-                            v = sR[c][lane] * F32C(-1.0) + sA[0][c][lane];
-                            v = v + sA[1][c][lane]; v = v + sA[2][c][lane];
-                        } else {
-                            v = sR[c][lane] * alpha - sA[0][c][lane];
-                            v = v - sA[1][c][lane]; v = v - sA[2][c][lane];
-                        }
-                        v = v + sV[0][c][lane]; v = v + sV[1][c][lane]; v = v + sV[2][c][lane];
-                        outv[c] = v / alpha;
-                        sdq[s & 1][c][lane] = outv[c];
-                    }
-                }
-                bar();
-                // the stores of step s-1 have completed: publish s finished steps
-                __builtin_amdgcn_s_waitcnt(0);
-                if (s > 0 && lane == 0) __hip_atomic_store(my_ctr, min(s, nsteps), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (valid) {
-                    for (int c = 0; c < 5; ++c) {
-                        if (pub) __hip_atomic_store(Dg + c * box + x, outv[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        else Dg[c * box + x] = outv[c];
-                    }
-                }
-            }
-            __builtin_amdgcn_s_waitcnt(0);
-            if (lane == 0) __hip_atomic_store(my_ctr, nsteps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-}
-
-template __global__ void k_lusgs_kba3<true, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
-template __global__ void k_lusgs_kba3<false, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int);
-
-// Clears the ticket, error flag and progress counters of one sweep.
 // ---------------------------------------------------------------------------
 // Tile-skewed sweep layout (BlockDev::srow / sdq).
 //
@@ -1127,8 +752,17 @@ template __global__ void k_sweep_pack<float>(const BlockDev *, const long *, int
 template __global__ void k_sweep_unpack<double>(const BlockDev *, const long *, int, int);
 template __global__ void k_sweep_unpack<float>(const BlockDev *, const long *, int, int);
 
-// Fourth form of the wavefront sweep: the four-wave schedule of k_lusgs_kba3
-// on the tile-skewed layout.  A step touches one row of its tile: the three
+// The persistent wavefront form of the sweeps, on the tile-skewed layout.  The
+// (j,k) cross-section is cut into 4 x 16 tiles, one workgroup of four waves
+// per tile; tiles are handed out by a ticket counter in topological order, so
+// a tile waits only on tiles already running.  Waves 0, 1, 2 each own one
+// neighbour direction (i, j, k): a step ahead they load the neighbour's
+// state and metrics, then form the flux difference and off-diagonal terms of
+// the step's 64 cells in LDS; wave 3 combines them into dQ, keeps dQ in LDS for
+// the next step and stores it.  Every load is issued unconditionally with
+// clamped indices, so the compiler's memory-counter waits are exact.  The
+// per-hyperplane kernel k_lusgs_plane above is the reference's own schedule
+// and gives the same bits.  A step touches one row of its tile: the three
 // direction waves read their neighbours' state and metrics from the previous
 // row (a lane shift) or, on the tile's upstream edge, from the upstream tile's
 // row, so every load is a contiguous run of one row.  dQ goes to the tile's
@@ -1496,202 +1130,6 @@ __global__ void k_fill_int(int *p, long n, int v)
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = v;
 }
-
-// Second form of the wavefront sweep: static sweep metrics and the diagonal
-// come precomputed, the next step's inputs are prefetched into registers while
-// the current step computes, and tile-edge increments travel through
-// agent-coherent atomic stores/loads followed by a progress counter, so no
-// cache-wide write-back or invalidate is needed per step.
-struct SweepIn {
-    double own[6];       // R (forward) or dQ* (backward) x5, alpha
-    double q[3][5];      // neighbour Q per direction
-    double pl[3][2];     // neighbour p, relax*lamc
-    double gm[3][5];     // neighbour averaged normal (3), area, grid normal velocity
-};
-
-template <bool FWD>
-__device__ inline void sweep_load(const BlockDev &B, long x, int i, int j, int k, double relax, SweepIn &in)
-{
-    const long box = B.box;
-    for (int c = 0; c < 5; ++c) in.own[c] = FWD ? B.R[c * box + x] : B.dQ[c * box + x];
-    in.own[5] = B.alpha[x];
-    const int pos[3] = {i, j, k};
-    const int lim[3] = {B.ni, B.nj, B.nk};
-    const long st[3] = {B.si, B.sj, 1};
-    for (int d = 0; d < 3; ++d) {
-        const bool ok = FWD ? pos[d] != 1 : pos[d] != lim[d];
-        const long nb = ok ? (FWD ? x - st[d] : x + st[d]) : x;
-        for (int c = 0; c < 5; ++c) in.q[d][c] = B.Q[c * box + nb];
-        in.pl[d][0] = B.W[4 * box + nb];
-        in.pl[d][1] = relax * B.lamc[d * box + nb];
-        for (int c = 0; c < 5; ++c) in.gm[d][c] = B.swg[(d * 5 + c) * box + nb];
-    }
-}
-
-template <bool FWD, int TJ, int TK>
-__global__ void __launch_bounds__(TJ *TK) k_lusgs_kba2(const BlockDev *__restrict__ blocks, const KbaTile *__restrict__ tiles,
-                                                       int ntiles, int *__restrict__ ticket, int *__restrict__ done,
-                                                       int *__restrict__ err, double relax, Gas g, int chunk, int lag)
-{
-    static_assert(TJ * TK == 64, "one wave per tile");
-    __shared__ double sdq[2][5][TJ][TK];
-    const int t = threadIdx.x;
-    const int tj = t / TK, tk = t % TK;
-    const double Gamma = g.gamma;
-    for (;;) {
-        int q = 0;
-        if (t == 0) q = atomicAdd(ticket, 1);
-        q = __shfl(q, 0);
-        if (q >= ntiles) return;
-        const KbaTile tl = tiles[q];
-        const BlockDev &B = blocks[tl.block];
-        const long box = B.box;
-        const int lj = FWD ? tj : TJ - 1 - tj, lk = FWD ? tk : TK - 1 - tk;
-        const int j = tl.j0 + tj + 1, k = tl.k0 + tk + 1;
-        const bool col = j <= B.nj && k <= B.nk;
-        const int nsteps = B.ni + TJ + TK - 2;
-        const bool edge_j = tj == (FWD ? 0 : TJ - 1), edge_k = tk == (FWD ? 0 : TK - 1);
-        const bool xj = edge_j && tl.dep_j >= 0, xk = edge_k && tl.dep_k >= 0;
-        const bool pub = (tj == (FWD ? TJ - 1 : 0)) || (tk == (FWD ? TK - 1 : 0));
-        const long sj = B.sj;
-        int seen_j = tl.dep_j < 0 ? 1 << 30 : 0, seen_k = tl.dep_k < 0 ? 1 << 30 : 0;
-        int pend_j = seen_j, pend_k = seen_k;
-        double prev[5] = {0, 0, 0, 0, 0};
-        double xnj[5] = {0, 0, 0, 0, 0}, xnk[5] = {0, 0, 0, 0, 0};
-        SweepIn nxt;
-        // waits until the upstream tiles have published the steps that step
-        // `st` of this tile reads, plus `lag` further steps of slack
-        auto ensure = [&](int st) {
-            const int need_j = min(st + TJ + lag, nsteps), need_k = min(st + TK + lag, nsteps);
-            seen_j = max(seen_j, pend_j);
-            seen_k = max(seen_k, pend_k);
-            long spins = 0;
-            while (seen_j < need_j) {
-                seen_j = __hip_atomic_load(done + tl.dep_j * kDonePad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (seen_j >= need_j) break;
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > (1L << 22)) { if (t == 0) atomicExch(err, 1); seen_j = 1 << 30; }
-            }
-            while (seen_k < need_k) {
-                seen_k = __hip_atomic_load(done + tl.dep_k * kDonePad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (seen_k >= need_k) break;
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > (1L << 22)) { if (t == 0) atomicExch(err, 1); seen_k = 1 << 30; }
-            }
-        };
-        // tile-edge increments of the upstream tiles read by step `st`
-        auto load_cross = [&](int st) {
-            const int l2 = st - lj - lk, i2 = FWD ? l2 + 1 : B.ni - l2;
-            if (col && l2 >= 0 && l2 < B.ni) {
-                const long x2 = cidx(B, i2, j, k);
-                if (xj)
-                    for (int c = 0; c < 5; ++c)
-                        xnj[c] = __hip_atomic_load(B.dQ + c * box + (FWD ? x2 - sj : x2 + sj), __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-                if (xk)
-                    for (int c = 0; c < 5; ++c)
-                        xnk[c] = __hip_atomic_load(B.dQ + c * box + (FWD ? x2 - 1 : x2 + 1), __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-            }
-        };
-        int li = -lj - lk;
-        {
-            const int i = FWD ? li + 1 : B.ni - li;
-            if (col && li >= 0 && li < B.ni) sweep_load<FWD>(B, cidx(B, i, j, k), i, j, k, relax, nxt);
-        }
-        ensure(0);
-        load_cross(0);
-        for (int s = 0; s < nsteps; ++s, ++li) {
-            const int i = FWD ? li + 1 : B.ni - li;
-            const bool valid = col && li >= 0 && li < B.ni;
-            const SweepIn cur = nxt;
-            double xcj[5], xck[5];
-            for (int c = 0; c < 5; ++c) { xcj[c] = xnj[c]; xck[c] = xnk[c]; }
-            // next step: availability, tile-edge increments, static inputs, and a non-blocking poll
-            if (s + 1 < nsteps) {
-                ensure(s + 1);
-                load_cross(s + 1);
-                const int li2 = li + 1, i2 = FWD ? li2 + 1 : B.ni - li2;
-                if (col && li2 >= 0 && li2 < B.ni) sweep_load<FWD>(B, cidx(B, i2, j, k), i2, j, k, relax, nxt);
-                if (seen_j < nsteps) pend_j = __hip_atomic_load(done + tl.dep_j * kDonePad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (seen_k < nsteps) pend_k = __hip_atomic_load(done + tl.dep_k * kDonePad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            if (valid) {
-                const long x = cidx(B, i, j, k);
-                const int pos[3] = {i, j, k};
-                const int lim[3] = {B.ni, B.nj, B.nk};
-                const double alpha = cur.own[5];
-                double A[3][5], V[3][5];
-                for (int d = 0; d < 3; ++d) {
-                    const bool act = FWD ? pos[d] != 1 : pos[d] != lim[d];
-                    if (!act) {
-                        for (int c = 0; c < 5; ++c) { A[d][c] = 0.0; V[d][c] = 0.0; }
-                        continue;
-                    }
-                    double dqn[5];
-                    if (d == 0) {
-                        for (int c = 0; c < 5; ++c) dqn[c] = prev[c];
-                    } else if (d == 1 ? !edge_j : !edge_k) {
-                        const int pj = d == 1 ? tj + (FWD ? -1 : 1) : tj, pk = d == 2 ? tk + (FWD ? -1 : 1) : tk;
-                        for (int c = 0; c < 5; ++c) dqn[c] = sdq[(s - 1) & 1][c][pj][pk];
-                    } else {
-                        for (int c = 0; c < 5; ++c) dqn[c] = d == 1 ? xcj[c] : xck[c];
-                    }
-                    const double *o = cur.q[d];
-                    const double nx = cur.gm[d][0], ny = cur.gm[d][1], nz = cur.gm[d][2], sd = cur.gm[d][3];
-                    const double Vn_e = cur.gm[d][4];
-                    double nw[5];
-                    for (int c = 0; c < 5; ++c) nw[c] = o[c] + dqn[c];
-                    double dens = nw[0], uvel = nw[1] / nw[0], vvel = nw[2] / nw[0], wvel = nw[3] / nw[0], E = nw[4] / nw[0];
-                    double press_new = (Gamma - F32C(1.0)) * dens * (E - F32C(0.5) * (uvel * uvel + vvel * vvel + wvel * wvel));
-                    double press_old = cur.pl[d][0];
-                    double Vn_old = o[1] / o[0] * nx + o[2] / o[0] * ny + o[3] / o[0] * nz;
-                    double Vn_new = uvel * nx + vvel * ny + wvel * nz;
-                    double df[5];
-                    df[0] = nw[0] * (Vn_new - Vn_e) - o[0] * (Vn_old - Vn_e);
-                    df[1] = (nw[1] * (Vn_new - Vn_e) + press_new * nx) - (o[1] * (Vn_old - Vn_e) + press_old * nx);
-                    df[2] = (nw[2] * (Vn_new - Vn_e) + press_new * ny) - (o[2] * (Vn_old - Vn_e) + press_old * ny);
-                    df[3] = (nw[3] * (Vn_new - Vn_e) + press_new * nz) - (o[3] * (Vn_old - Vn_e) + press_old * nz);
-                    df[4] = ((nw[4] + press_new) * (Vn_new - Vn_e) + press_new * Vn_e) -
-                            ((o[4] + press_old) * (Vn_old - Vn_e) + press_old * Vn_e);
-                    const double rl = cur.pl[d][1];
-                    for (int c = 0; c < 5; ++c) {
-                        A[d][c] = FWD ? (df[c] * sd + dqn[c] * rl) * F32C(0.5) : (df[c] * sd - dqn[c] * rl) * F32C(0.5);
-                        V[d][c] = dqn[c] * 0.0;   // lamv * dQ, lamv = 0 on the inviscid path
-                    }
-                }
-                for (int c = 0; c < 5; ++c) {
-                    double v;
-                    if (FWD) {
-                        v = cur.own[c] * F32C(-1.0) + A[0][c];
-                        v = v + A[1][c]; v = v + A[2][c];
-                    } else {
-                        v = cur.own[c] * alpha - A[0][c];
-                        v = v - A[1][c]; v = v - A[2][c];
-                    }
-                    v = v + V[0][c]; v = v + V[1][c]; v = v + V[2][c];
-                    const double out = v / alpha;
-                    if (pub) __hip_atomic_store(B.dQ + c * box + x, out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    else B.dQ[c * box + x] = out;
-                    sdq[s & 1][c][tj][tk] = out;
-                    prev[c] = out;
-                }
-            }
-            // neighbours in this wave read the increment from LDS next step
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-            __builtin_amdgcn_s_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-            if ((s + 1) % chunk == 0 || s + 1 == nsteps) {
-                // every edge store of the published steps has completed
-                __builtin_amdgcn_s_waitcnt(0);
-                if (t == 0) __hip_atomic_store(done + q * kDonePad, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-    }
-}
-
-template __global__ void k_lusgs_kba2<true, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int, int);
-template __global__ void k_lusgs_kba2<false, kKbaTJ, kKbaTK>(const BlockDev *, const KbaTile *, int, int *, int *, int *, double, Gas, int, int);
 
 // ===================================================== update + norms
 // Q += dQ (source/LU_SGS_Marching.cpp:262-287), Res_Calculate terms

@@ -132,6 +132,7 @@ def parse():
     ap.add_argument("--train-develop", type=int, default=2500, help="closure-train: V-cycle iterations per passage")
     ap.add_argument("--closure-out", default=None, help="closure-train: where to write the network")
     ap.add_argument("--no-heldout", action="store_true", help="closure-train: train only")
+    ap.add_argument("--heldout-max-iter", type=int, default=6000, help="closure-train: iteration cap per held-out run")
     ap.add_argument("--closure-relax", type=float, default=0.2, help="under-relaxation of the closure's mue_t")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run three plain 1 GiB device copies and exit (scales the HBM byte counters)")
@@ -334,6 +335,7 @@ def time_to_drop(case, precision, target, max_iter, closure=None, chunk=25):
     g.sync()
     t0 = time.perf_counter()
     it, hit, row = skip + 1, None, None
+    last = t0
     while it < max_iter and hit is None:
         h = g.iterate(chunk, graph=True)
         k = np.nonzero(h[:, 1] <= target * r0)[0]
@@ -341,6 +343,10 @@ def time_to_drop(case, precision, target, max_iter, closure=None, chunk=25):
             hit = it + int(k[0]) + 1
             row = h[k[0]]
         it += chunk
+        if time.perf_counter() - last > 60.0:   # a progress line a minute on long runs
+            last = time.perf_counter()
+            print(f"bench: drop run at {it} iterations, Res.dens / first {h[-1, 1] / r0:.3e}", file=sys.stderr,
+                  flush=True)
     g.sync()
     secs = time.perf_counter() - t0
     tripped = g.mlp_tripped() if closure is not None else None
@@ -436,9 +442,10 @@ def c4_main(a):
     from fv3d import cases
     plain = cases.rotor_stator_stage(levels=0, max_iter=a.drop_max_iter)
     chosen = cases.rotor_stator_stage(levels=a.levels, max_iter=a.drop_max_iter)
-    wname = (f"rotor + stator stage, 14 blocks ({plain.n_cells:,} cells; blocks 0.4M-2.0M): rotor 22 blades "
-             "128x128x(24,40,64,120,64,40,24) at 3000 rpm, blades on the relative-flow helix, stator 30 blades "
-             "128x96x(24,48,72,144,96,64,32), mixing plane between the rows, hub 0.2 m tip 0.25 m, "
+    wname = (f"rotor + stator stage, 14 blocks ({plain.n_cells:,} cells; blocks 0.29M-1.97M): rotor 22 blades "
+             "128x128x(24,40,64,120,64,40,24) at 3000 rpm, blades on the relative-flow helix turning 0.04 pitch, "
+             "stator 30 blades 128x96x(24,48,72,144,96,64,32) turning -0.03 pitch, mixing plane between the rows, "
+             "hub 0.2 m tip 0.25 m, "
              "Navier-Stokes + Spalart-Allmaras, Roe + MUSCL3 primitive, LU-SGS local dt CFL 5, outlet 0.95 Pt")
     passage_main(a, plain, chosen, wname, "c4")
 
@@ -588,39 +595,56 @@ def legacy_main(a):
     print(json.dumps(out))
 
 
+# the closure's training family: the vane passage at three resolutions (cells
+# across, blocks along the axis; every size a multiple of 8)
+CLOSURE_RES = {40: (16, 24, 64, 32, 16), 64: (16, 32, 104, 48, 32), 88: (24, 48, 144, 72, 40)}
+
+
 def closure_train(a):
     """The closure's training set and held-out evaluation on the vane family:
-    ``train_instances`` passages of 40 x 40 x (16, 24, 64, 32, 16) cells
-    (243,200 each) at back-pressure ratios drawn uniform in [0.93, 0.97] of Pt
-    (which sets the through-flow speed), developed with Spalart-Allmaras in one
-    batched context on the three-level V-cycle; the network is trained on the
-    GPU on the mue_t fields of all but the last 8; the last 8 are then run
-    from the uniform start to a 1e-6 drop once with the closure and once with
-    Spalart-Allmaras, and mass flow, pressure ratio and efficiency are
-    compared.  Writes the network to fv3d/closure_vane.npz."""
+    ``train_instances - 8`` training passages, each at one of three
+    resolutions (40, 64 or 88 cells across, 0.24M / 0.96M / 2.54M cells,
+    drawn in turn) and a back-pressure ratio drawn uniform in [0.93, 0.97] of
+    Pt (which sets the through-flow speed), developed with Spalart-Allmaras in
+    one batched context on the three-level V-cycle; the network is trained on
+    the GPU on their mue_t fields; 8 held-out passages at the headline
+    resolution (88 cells across, 2,540,032 cells) and back pressures of their
+    own then run from the uniform start to a 1e-6 drop once with the closure
+    and once with Spalart-Allmaras, in the chosen form, and mass flow and
+    pressure ratio are compared, with the gate's fallback fraction.  Writes
+    the network to fv3d/closure_vane.npz (or --closure-out)."""
     import numpy as np
     from fv3d import cases, closure
     rng = np.random.default_rng(a.batch_seed)
     n = a.train_instances
     prs = rng.uniform(0.93, 0.97, n)
+    res_cycle = (40, 64, 88)
 
-    def fam(q, levels=3):
-        return cases.vane_passage(ni=40, nk_blocks=(16, 24, 64, 32, 16), seed=100 + q, levels=levels,
+    def fam(q, ni, levels=3):
+        return cases.vane_passage(ni=ni, nk_blocks=CLOSURE_RES[ni], seed=100 + q, levels=levels,
                                   p_ratio=float(prs[q]), max_iter=100000)
 
     train, held = list(range(n - 8)), list(range(n - 8, n))
+    tcases = [fam(q, res_cycle[q % 3]) for q in train]
     t0 = time.perf_counter()
-    g = solver([fam(q) for q in train], "fp32")
-    g.iterate(a.train_develop, graph=True)
+    g = solver(tcases, "fp32")
+    done = 0
+    while done < a.train_develop:   # in chunks, with a progress line each
+        k = min(250, a.train_develop - done)
+        g.iterate(k, graph=True)
+        done += k
+        print(f"closure: developing, {done} iterations ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
     g.sync()
     dev_s = time.perf_counter() - t0
-    print(f"closure: developed {len(train)} passages in {dev_s:.1f} s", file=sys.stderr, flush=True)
+    ncell = sum(c.n_cells for c in tcases)
+    print(f"closure: developed {len(train)} passages ({ncell:,} cells) in {dev_s:.1f} s", file=sys.stderr, flush=True)
     xs, ts = [], []
     for q in range(len(train)):
         for b in range(1, 6):
             xs.append(g.mlp_features(b, inst=q + 1).reshape(-1, closure.N_IN))
             ts.append(g.mlp_targets(b, inst=q + 1).ravel())
     x, t = np.concatenate(xs).astype(np.float32), np.concatenate(ts).astype(np.float32)
+    del xs, ts
     perm = np.random.default_rng(a.batch_seed + 1).permutation(len(x))
     x, t = x[perm], t[perm]
     bounds = closure.gate_bounds(x, t)
@@ -633,37 +657,36 @@ def closure_train(a):
     g.close()
     print(f"closure: trained on {len(x):,} cells in {train_s:.1f} s, loss {hist[0]:.4f} -> {hist[-1]:.5f}",
           file=sys.stderr, flush=True)
-    model = {"params": P, "bounds": bounds, "meta": {}, "relax": a.closure_relax}
+    meta = {"family": "vane passage at 40, 64 and 88 cells across (0.24M, 0.96M, 2.54M cells) in turn, back "
+                      f"pressure U[0.93, 0.97] Pt, seed {a.batch_seed}", "train_passages": len(train),
+            "train_cells": ncell, "samples": int(len(x)), "steps": steps, "batch": 4096, "lr": 2e-3,
+            "loss_first": float(hist[0]), "loss_last": float(hist[-1]), "develop_iterations": a.train_develop}
+    out_path = a.closure_out or CLOSURE_WEIGHTS
+    np.savez(out_path, params=P, bounds=bounds, meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
     if a.no_heldout:
-        np.savez(a.closure_out or CLOSURE_WEIGHTS, params=P, bounds=bounds)
-        print(json.dumps({"training_loss": [float(hist[0]), float(hist[-1])], "samples": int(len(x))}))
+        print(json.dumps({"training": meta}))
         return
+    model = {"params": P, "bounds": bounds, "meta": meta, "relax": a.closure_relax}
     res = {}
     for mode in ("sa", "closure"):
         rows = []
         for q in held:
-            d = time_to_drop(fam(q), "fp32", 1e-6, 6000, model if mode == "closure" else None)
+            d = time_to_drop(fam(q, 88, levels=a.levels), "fp32", 1e-6, a.heldout_max_iter,
+                             model if mode == "closure" else None)
             rows.append(d)
             print(f"closure: held-out {q} {mode}: {d['iterations']} iterations", file=sys.stderr, flush=True)
         res[mode] = rows
     rel = lambda k: [abs(c[k] - s[k]) / abs(s[k]) if c[k] is not None and s[k] is not None else None
                      for c, s in zip(res["closure"], res["sa"])]
-    meta = {"family": "vane passage 40x40x(16,24,64,32,16), 243,200 cells, back pressure U[0.93, 0.97] Pt, "
-                      f"seed {a.batch_seed}", "train_passages": len(train), "samples": int(len(x)), "steps": steps,
-            "batch": 4096, "lr": 2e-3, "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
-            "develop_iterations": a.train_develop}
-    np.savez(a.closure_out or CLOSURE_WEIGHTS, params=P, bounds=bounds,
-             meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
     out = {"metric": "learned closure vs Spalart-Allmaras on held-out passages (converged to a 1e-6 drop)",
            "training": meta, "develop_seconds": dev_s, "train_seconds": train_s,
-           "held_out": {"pressure_ratios": [float(prs[q]) for q in held],
+           "held_out": {"cells": fam(held[0], 88).n_cells, "pressure_ratios": [float(prs[q]) for q in held],
                         "mass_in_rel_err": rel("mass_in"), "pr_rel_err": rel("pressure_ratio"),
-                        "eff_rel_err": rel("efficiency"),
                         "iterations_sa": [d["iterations"] for d in res["sa"]],
                         "iterations_closure": [d["iterations"] for d in res["closure"]],
                         "seconds_sa": [d["seconds"] for d in res["sa"]],
                         "seconds_closure": [d["seconds"] for d in res["closure"]],
-                        "gate_fallbacks": [d["gate_fallbacks"] for d in res["closure"]]}}
+                        "gate_fallback_fraction": [d["gate_fallback_fraction"] for d in res["closure"]]}}
     print(json.dumps(out))
 
 

@@ -283,9 +283,9 @@ int fv3d_timing(fv3d_ctx *ctx, double *ms, long *counts);
 /* Debug flags: bit 0 keeps the face fluxes of the last residual evaluation
  * so fv3d_field_download can return fcI/fcJ/fcK, and for Navier-Stokes the
  * viscous fvI/fvJ/fvK, gradI/gradJ/gradK and fv (single-block meshes).
- * Bits 1-4 select another LU-SGS sweep form for comparison (1: one launch per
- * hyperplane, 2-4: earlier wavefront forms, Euler only); bits 8-15 and 16-23
- * set the chunk and extra lag of the wavefront forms. */
+ * Bit 1 runs the LU-SGS sweeps as one launch per hyperplane (the reference's
+ * schedule, source/LU_SGS_Marching.cpp:59-260) instead of the persistent
+ * wavefront, as a cross-check (plain FP64 only).  Other bits are refused. */
 int fv3d_set_debug(fv3d_ctx *ctx, int flags);
 
 /* Tile shapes of the residual kernels (no reference counterpart: the
