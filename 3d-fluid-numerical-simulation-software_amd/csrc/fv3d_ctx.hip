@@ -1686,13 +1686,16 @@ static int launch_sa(fv3d_ctx *c, int save_old, bool mut_only)
 // Turbulence model before the time step (source/ComputeAllBlockVars.cpp:
 // 228-232): the vorticity, then Baldwin-Lomax on all walls at once (one launch
 // per line group size) and the merge of the per-cell minima; or
-// Spalart-Allmaras.  mut_only: Spalart-Allmaras forms mue_t (with the
-// clipping of nu~ and mue_t and the ghost values) but not the nu~ residual
-// and the vorticity it needs -- for the two-grid cycle's second fine
-// residual, whose nu~ residual nothing reads before the next iteration's
-// first residual forms it again.
+// Spalart-Allmaras.  mut_only (the cycle's second fine residual, the one the
+// restriction takes): Spalart-Allmaras forms mue_t (with the clipping of nu~
+// and mue_t and the ghost values) but not the nu~ residual and the vorticity
+// it needs, which nothing reads before the next iteration's first residual
+// forms them again; the learned closure keeps the mue_t of the iteration's
+// first evaluation (it has no reference counterpart; lagging the coarse-grid
+// correction's mue_t by half an iteration leaves the fixed point unchanged).
 static int launch_turbulence(fv3d_ctx *c, int save_old = 0, bool mut_only = false)
 {
+    if (c->cfg.turbulence == 200 && c->mlp_closure && mut_only) return FV3D_OK;
     if (c->cfg.turbulence == 200 && c->mlp_closure) {
         // the learned closure in place of the nu~ transport; ghost mue_t as for SA
         const int nb = (int)c->blocks.size();

@@ -350,10 +350,10 @@ def time_to_drop(case, precision, target, max_iter, closure=None, chunk=25):
     g.sync()
     secs = time.perf_counter() - t0
     tripped = g.mlp_tripped() if closure is not None else None
-    # every cycle iteration evaluates the closure on every fine cell twice (the
-    # iteration's residual and the one the restriction takes); the coarse
-    # start-up iterations do not evaluate it
-    evals = 2 * g.n_cells * (it - skip) if closure is not None else None
+    # every cycle iteration evaluates the closure once on every fine cell (its
+    # first residual; the one the restriction takes keeps that mue_t); the
+    # coarse start-up iterations do not evaluate it
+    evals = g.n_cells * (it - skip) if closure is not None else None
     g.close()
     val = lambda col: float(row[col]) if row is not None else None
     return {"iterations": hit, "seconds": secs if hit else None, "res0": r0, "mass_in": val(2), "mass_out": val(3),
