@@ -132,6 +132,20 @@ def test_runner_by_stage_and_mixing_plane_rows(tmp_path):
     g.close()
 
 
+def test_runner_stage_with_non_matching_mixing_plane(tmp_path):
+    # the rotor + stator stage (22 against 30 blades, 8 against 6 cells
+    # pitchwise, rotating against stationary): the boundary table's
+    # MixingPlane row read by the C++ layer gives the Python path's history
+    case = cases.rotor_stator_stage(ni=6, nj_rotor=8, nj_stator=6, rotor_nk=(6, 8, 6), stator_nk=(6, 10, 6),
+                                    coarse_iter=2, max_iter=8)
+    d = str(tmp_path / "case")
+    case.write(d)
+    h_run = _run(d, 8, "--no-output")
+    g = GpuSolver(case)
+    np.testing.assert_array_equal(g.iterate(8, graph=True), h_run)
+    g.close()
+
+
 def test_runner_writes_restart_and_wall_distance(tmp_path):
     # RestartSolution.dat at full precision equals the state, and a rerun that
     # reads WallDistance.dat (ReuseWallDistance = Y) has the same history

@@ -111,3 +111,17 @@ def test_parse_only_reads_mixing_plane_rows(tmp_path):
     mp = [p for p in case.patches if p.kind == "mixing"]
     assert len(mp) == 1
     assert any(row[0] == 5 for row in s["patches"]), s["patches"]
+
+
+def test_parse_only_reads_non_matching_stage(tmp_path):
+    # the rotor + stator stage: 14 blocks, two blade counts and speeds, one
+    # MixingPlane row joining 128- and 96-cell pitchwise ranges
+    case = cases.rotor_stator_stage(ni=8, nj_rotor=16, nj_stator=12, rotor_nk=(4, 4, 4, 8, 4, 4, 4),
+                                    stator_nk=(4, 4, 4, 8, 4, 4, 4), max_iter=4)
+    d = str(tmp_path / "case")
+    case.write(d)
+    s = json.loads(subprocess.run([RUN, d, "--parse-only"], check=True, capture_output=True, text=True,
+                                  timeout=60).stdout)
+    assert len(s["blocks"]) == 14
+    rows = [row for row in s["patches"] if row[0] == 5]
+    assert len(rows) == 1, s["patches"]
