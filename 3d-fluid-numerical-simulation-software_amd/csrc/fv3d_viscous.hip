@@ -167,7 +167,11 @@ __global__ void __launch_bounds__(TI *TJ *TK) k_viscous(const BlockDev *__restri
     using TL = VTile<TI, TJ, TK>;
     constexpr int NT = TI * TJ * TK, NH = TL::NH;
     __shared__ double sw[4 * NH];
-    __shared__ double sf[5 * TL::NF];
+    // momentum and energy components of the face fluxes; the continuity
+    // component of the viscous flux is zero on every face (F1 = 0,
+    // source/ViscousFlux.cpp:150-160), so it takes no LDS and three
+    // workgroups fit on a CU instead of two
+    __shared__ double sf[4 * TL::NF];
     const int4 tl = tiles[blockIdx.x];
     if (tl.x < 0) return;   // padding of the XCD-ordered tile list
     const BlockDev &B = blocks[tl.x];
@@ -208,7 +212,7 @@ __global__ void __launch_bounds__(TI *TJ *TK) k_viscous(const BlockDev *__restri
         else if (D == 1) face_fv<1, TL>(sw, l, i, j, k, B, g, o, gs);
         else face_fv<2, TL>(sw, l, i, j, k, B, g, o, gs);
 #pragma unroll
-        for (int q = 0; q < 5; ++q) sf[q * TL::NF + f] = o[q];
+        for (int q = 1; q < 5; ++q) sf[(q - 1) * TL::NF + f] = o[q];
         if (store)
             for (int q = 0; q < 5; ++q) dbg.fv[D][q * box + x] = o[q];
     }
@@ -222,9 +226,11 @@ __global__ void __launch_bounds__(TI *TJ *TK) k_viscous(const BlockDev *__restri
     const int fJ0 = TL::NFI + (ti * (TJ + 1) + tj) * TK + tk, fJ1 = TL::NFI + (ti * (TJ + 1) + tj + 1) * TK + tk;
     const int fK0 = TL::NFI + TL::NFJ + (ti * TJ + tj) * (TK + 1) + tk, fK1 = fK0 + 1;
     const long x = cidx(B, i, j, k);
+    // density: R - 0 - 0 leaves R as the convective kernel wrote it
+    if (store) dbg.fvc[x] = 0.0;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
-        const double *F = sf + q * TL::NF;
+    for (int q = 1; q < 5; ++q) {
+        const double *F = sf + (q - 1) * TL::NF;
         double s = F[fI1] - F[fI0];
         s = s + F[fJ1];
         s = s - F[fJ0];

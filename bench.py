@@ -78,8 +78,9 @@ def kernel_bytes(precision: str, sa_transport: bool):
     return {
         # Q 5, S_I,J,K 9, nodes 3 -> R 5
         "convective": 22 * 8,
-        # Q 5, mue_t 1, face coefficients 27, S 9 -> R 5 read and written
-        "viscous": 52 * 8,
+        # Q 5, mue_t 1, face coefficients 27, S 9 -> R 4 read and written (the
+        # continuity component of the viscous flux is zero, R's density stays)
+        "viscous": 50 * 8,
         # R 5, Q 5, lambda_c 3, dt 1, Vol 1, S 9, lambda_v 3 -> dQ* 5 (+ nu~)
         "forward_sweep": (32 + nu) * w,
         # dQ* 5, Q 5, lambda_c 3, dt 1, Vol 1, S 9, lambda_v 3 -> dQ 5 (+ nu~)
