@@ -426,8 +426,10 @@ __device__ static void face_flux(const BlockDev &B, const Gas &g, int d, int i, 
 // once into LDS, then each cell sums its six faces and subtracts the rotation
 // source (source/ComputeAllBlockVars.cpp:85-118, 237-254).  Optionally stores
 // the face fluxes for inspection.
+// Three waves per SIMD: the register budget (168) holds the face code with a
+// few spills, and the third wave hides more latency than the spills cost.
 template <int TI, int TJ, int TK>
-__global__ void __launch_bounds__(TI *TJ *TK) k_convective(const BlockDev *__restrict__ blocks, const int4 *__restrict__ tiles,
+__global__ void __launch_bounds__(TI *TJ *TK) __attribute__((amdgpu_waves_per_eu(3))) k_convective(const BlockDev *__restrict__ blocks, const int4 *__restrict__ tiles,
                                                            Gas g, int store_faces, double *fI0, double *fJ0, double *fK0)
 {
     __shared__ double sI[5][TI + 1][TJ][TK];
@@ -442,23 +444,23 @@ __global__ void __launch_bounds__(TI *TJ *TK) k_convective(const BlockDev *__res
     const bool valid = i <= B.ni && j <= B.nj && k <= B.nk;
     double f[5];
     if (valid) {
-        face_flux(B, g, 0, i, j, k, f);
-        for (int c = 0; c < 5; ++c) sI[c][ti][tj][tk] = f[c];
-        face_flux(B, g, 1, i, j, k, f);
-        for (int c = 0; c < 5; ++c) sJ[c][ti][tj][tk] = f[c];
-        face_flux(B, g, 2, i, j, k, f);
-        for (int c = 0; c < 5; ++c) sK[c][ti][tj][tk] = f[c];
-        if (ti == TI - 1 || i == B.ni) {
-            face_flux(B, g, 0, i + 1, j, k, f);
-            for (int c = 0; c < 5; ++c) sI[c][ti + 1][tj][tk] = f[c];
-        }
-        if (tj == TJ - 1 || j == B.nj) {
-            face_flux(B, g, 1, i, j + 1, k, f);
-            for (int c = 0; c < 5; ++c) sJ[c][ti][tj + 1][tk] = f[c];
-        }
-        if (tk == TK - 1 || k == B.nk) {
-            face_flux(B, g, 2, i, j, k + 1, f);
-            for (int c = 0; c < 5; ++c) sK[c][ti][tj][tk + 1] = f[c];
+        // the cell's three low faces, then the high faces on the tile's upper
+        // edges, all through one copy of the face code: the loop stays rolled
+        // because six inlined copies of it (every scheme and reconstruction)
+        // would not fit the instruction cache
+#pragma nounroll
+        for (int e = 0; e < 6; ++e) {
+            const int d = e < 3 ? e : e - 3, hi = e < 3 ? 0 : 1;
+            if (hi && !(d == 0 ? (ti == TI - 1 || i == B.ni) : d == 1 ? (tj == TJ - 1 || j == B.nj)
+                                                                    : (tk == TK - 1 || k == B.nk)))
+                continue;
+            face_flux(B, g, d, i + (d == 0 ? hi : 0), j + (d == 1 ? hi : 0), k + (d == 2 ? hi : 0), f);
+            if (d == 0)
+                for (int c = 0; c < 5; ++c) sI[c][ti + hi][tj][tk] = f[c];
+            else if (d == 1)
+                for (int c = 0; c < 5; ++c) sJ[c][ti][tj + hi][tk] = f[c];
+            else
+                for (int c = 0; c < 5; ++c) sK[c][ti][tj][tk + hi] = f[c];
         }
     }
     __syncthreads();
@@ -610,8 +612,16 @@ template <int D>
 struct Dir {
     static constexpr int value = D;
 };
-constexpr int kSweepAhead = 1;   // prefetch distance of the direction waves, in steps
-constexpr int kRowS0 = -2;       // first (idle) step of the row-layout sweep loops
+// A direction wave issues the static loads of a step (the neighbour's state and
+// metrics, its own right-hand side) kStaticAhead steps before the step and
+// forms their early terms one step before it, so a load has kStaticAhead - 1
+// steps to arrive; it cycles through kRing row buffers.
+#ifndef FV3D_STATIC_AHEAD
+#define FV3D_STATIC_AHEAD 3
+#endif
+constexpr int kStaticAhead = FV3D_STATIC_AHEAD;
+constexpr int kRing = kStaticAhead + 1;
+constexpr int kRowS0 = -kStaticAhead;   // first (idle) step of the row-layout sweep loops
 // whether a mailbox word still holds the empty pattern
 __device__ inline bool mbox_empty(double x) { return __builtin_bit_cast(unsigned long long, x) == kMboxEmpty; }
 __device__ inline bool mbox_empty(float x) { return __builtin_bit_cast(unsigned int, x) == kMboxEmpty32; }
@@ -661,6 +671,32 @@ __global__ void __launch_bounds__(256) k_sweep_fill(const BlockDev *__restrict__
         for (int q = 0; q < 15; ++q) dst[(kSQ_SWG + q) * 64] = valid ? B.swg[q * box + x] : 0.0;
 }
 
+namespace {
+// Work item of workgroup b in a grid of n.  The hardware hands workgroup b to
+// XCD b mod 8, so XCD x runs b = x, x + 8, ...; XCD x takes the x-th run of
+// consecutive work items, in the order it runs them.  Neighbouring items share
+// cache lines at their edges (a 16-cell run of k starts at any offset within a
+// 128 B line), which are then fetched into, and written back from, one L2.
+__device__ inline long xcd_item(long b, long n)
+{
+    const long x = b % 8;
+    long start = 0;
+    for (long y = 0; y < x; ++y) start += (n - y + 7) / 8;   // items of the XCDs before x
+    return start + b / 8;
+}
+// (tile, chunk) of pack/unpack work item wg of a block: k tiles fastest, then
+// chunks of i, then j tiles, so consecutive items are neighbours along k (they
+// share the cell-box lines at their k edges) and items ntk apart neighbours
+// along i (they share the partly written rows at their chunk edges)
+__device__ inline void pack_item(const BlockDev &B, long wg, int nchunk, int &slot, int &i0)
+{
+    const int K = (int)(wg % B.ntk);
+    const long rest = wg / B.ntk;
+    i0 = (int)(rest % nchunk) * kPackChunk + 1;
+    slot = (int)(rest / nchunk) * B.ntk + K;
+}
+}  // namespace
+
 // One workgroup per (tile, chunk of kPackChunk i-planes).  The chunk's cells are
 // read plane by plane along k (coalesced) into LDS, then written out row by
 // row: the rows a chunk touches are partly written (the lanes whose cell lies
@@ -672,11 +708,13 @@ __global__ void __launch_bounds__(256) k_sweep_pack(const BlockDev *__restrict__
     constexpr int TJ = kKbaTJ, TK = kKbaTK;
     __shared__ double buf[kPackChunk][64];
     // the block of this workgroup: gstart holds the running workgroup counts
-    const int block = find_range(gstart, nblock, (long)blockIdx.x);
+    const long item = xcd_item(blockIdx.x, gridDim.x);
+    const int block = find_range(gstart, nblock, item);
     const BlockDev &B = blocks[block];
-    const long wg = (long)blockIdx.x - gstart[block];
+    const long wg = item - gstart[block];
     const int nchunk = (B.ni + kPackChunk - 1) / kPackChunk;
-    const int slot = (int)(wg / nchunk), i0 = (int)(wg % nchunk) * kPackChunk + 1;
+    int slot, i0;
+    pack_item(B, wg, nchunk, slot, i0);
     const int J = slot / B.ntk, K = slot % B.ntk;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int tj = lane / TK, tk = lane % TK, j = J * TJ + tj + 1, k = K * TK + tk + 1;
@@ -718,11 +756,13 @@ __global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict
     constexpr int TJ = kKbaTJ, TK = kKbaTK;
     __shared__ double buf[kPackChunk][64];
     // the block of this workgroup: gstart holds the running workgroup counts
-    const int block = find_range(gstart, nblock, (long)blockIdx.x);
+    const long item = xcd_item(blockIdx.x, gridDim.x);
+    const int block = find_range(gstart, nblock, item);
     const BlockDev &B = blocks[block];
-    const long wg = (long)blockIdx.x - gstart[block];
+    const long wg = item - gstart[block];
     const int nchunk = (B.ni + kPackChunk - 1) / kPackChunk;
-    const int slot = (int)(wg / nchunk), i0 = (int)(wg % nchunk) * kPackChunk + 1;
+    int slot, i0;
+    pack_item(B, wg, nchunk, slot, i0);
     const int J = slot / B.ntk, K = slot % B.ntk;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int tj = lane / TK, tk = lane % TK, j = J * TJ + tj + 1, k = K * TK + tk + 1;
@@ -816,8 +856,8 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
         const int j = tl.j0 + tj + 1, k = tl.k0 + tk + 1;
         const bool col = j <= nj && k <= nk;
         const int nsteps = nr;
-        // steps run: [kRowS0, send), a whole number of three-step unrollings
-        const int send = (nsteps - kRowS0 + 2) / 3 * 3 + kRowS0;
+        // steps run: [kRowS0, send), a whole number of kRing-step unrollings
+        const int send = (nsteps - kRowS0 + kRing - 1) / kRing * kRing + kRowS0;
         auto clampr = [&](int r) { return min(max(r, 0), nr - 1); };
         // own row of a step, and whether this lane's cell is in the block
         auto row_of = [&](int step) { return FWD ? step : nr - 1 - step; };
@@ -931,7 +971,7 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
             // step s: the dQ-dependent part of the flux difference (critical
             // path), then, while the combine wave works, the loads and early
             // terms of the next steps
-            auto work = [&](RowPf<T> &b, RowPf<T> &b1, RowPf<T> &b2, int s) {
+            auto work = [&](RowPf<T> &b, RowPf<T> &b1, RowPf<T> &bn, int s) {
                 int i;
                 const bool valid = cell_at(s, i);
                 const int p = d == 0 ? i : d == 1 ? j : k, lim = d == 0 ? ni : d == 1 ? nj : nk;
@@ -1016,19 +1056,21 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                 }
                 bar();   // terms of step s are in LDS
                 issue_cross(b1, s + 1);
-                issue_static(b2, s + 2);
+                issue_static(bn, s + kStaticAhead);
                 precompute(b1);
                 bar();   // dQ of step s is in LDS
             };
-            // b[u] serves step s0+u: static data issued two steps ahead, the
-            // upstream dQ one step ahead.  The loop starts two idle steps early
-            // (every lane invalid) so that no load precedes it and every step
-            // runs the same code; nothing is in flight on entry.
-            RowPf<T> b[3] = {};
+            // b[u] serves step s0+u: static data issued kStaticAhead steps
+            // ahead (into the buffer the previous step has finished with), the
+            // upstream dQ one step ahead.  The loop starts kStaticAhead idle
+            // steps early (every lane invalid) so that no load precedes it and
+            // every step runs the same code; nothing is in flight on entry.
+            RowPf<T> b[kRing] = {};
             __builtin_amdgcn_s_waitcnt(0);
-            for (int s0 = kRowS0; s0 < send; s0 += 3) {
+            for (int s0 = kRowS0; s0 < send; s0 += kRing) {
 #pragma unroll
-                for (int u = 0; u < 3; ++u) work(b[u], b[(u + 1) % 3], b[(u + 2) % 3], s0 + u);
+                for (int u = 0; u < kRing; ++u)
+                    work(b[u], b[(u + 1) % kRing], b[(u + kStaticAhead) % kRing], s0 + u);
             }
         };
         if (wave == 0) {
@@ -1327,7 +1369,24 @@ __global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ par
     const int nc = g.turbulence == 200 ? 6 : 5;
     __shared__ double red[6][256];
     double r[6] = {0, 0, 0, 0, 0, 0};
-    for (int p = threadIdx.x; p < np; p += blockDim.x)
+    // each thread adds partials p = t, t + 256, ... in that order; eight of
+    // them are loaded before the first is added, so a thread waits on memory
+    // once per eight rather than once per partial (d_partial holds six
+    // components, so the sixth is always in bounds)
+    int p = threadIdx.x;
+    for (; p + 7 * 256 < np; p += 8 * 256) {
+        double v[8][6];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) v[u][c] = partial[(long)c * npart + p0 + p + u * 256];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                if (c < nc) r[c] += v[u][c];
+    }
+    for (; p < np; p += 256)
         for (int c = 0; c < nc; ++c) r[c] += partial[(long)c * npart + p0 + p];
     for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = r[c];
     __syncthreads();
