@@ -41,7 +41,7 @@ struct HostBlock {
     BlockDev dev;                // device pointers + dims
     std::vector<void *> allocs;
     long sweep_rows = 0;         // rows of the tile-skewed sweep layout
-    long pack_groups = 0;        // workgroups of k_sweep_pack / k_sweep_unpack
+    long pack_groups = 0;        // (tile, chunk) items of k_sweep_pack / k_sweep_unpack
 };
 
 // An independent passage of a batched context: its blocks are
@@ -88,7 +88,7 @@ struct fv3d_ctx {
     std::vector<PatchDev> pd_host;   // the device patch list, host copy
     double *d_mbox = nullptr;         // sweep mailboxes of all blocks
     long mbox_words = 0;
-    long *d_pack_start = nullptr;     // running pack workgroup counts per block
+    long *d_pack_start = nullptr;     // running pack (tile, chunk) item counts per block
     long pack_groups = 0;
     // Baldwin-Lomax: one launch per group size G (16, 32 or 64 cell groups per
     // line, by line length) over every wall that needs it; d_bl_walls holds
@@ -1545,14 +1545,15 @@ int fv3d_initial(fv3d_ctx *c)
 // ------------------------------------------------------------- stages
 static int launch_bc(fv3d_ctx *c)
 {
+    // the periodic signs are updated by k_bc's first thread when it runs
     if (c->nbc > 0)
         hipLaunchKernelGGL(k_bc, dim3((unsigned)((c->nbc + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks, c->d_patches,
-                           c->npatch_dev, c->d_pstart_bc, c->gas, c->d_sc);
+                           c->npatch_dev, c->d_pstart_bc, c->gas, c->d_sc, c->d_per, c->nper, c->d_pitch_sign);
+    else if (c->nper > 0)
+        hipLaunchKernelGGL(k_periodic_flip, dim3(1), dim3(64), 0, c->stream, c->d_per, c->nper, c->d_pitch_sign);
     if (c->n_axial > 0)
         hipLaunchKernelGGL(k_bc_axial, dim3((unsigned)c->n_axial), dim3(256), 0, c->stream, c->d_blocks, c->d_patches,
                            c->d_axial, c->gas, c->d_sc);
-    if (c->nper > 0)
-        hipLaunchKernelGGL(k_periodic_flip, dim3(1), dim3(64), 0, c->stream, c->d_per, c->nper, c->d_pitch_sign);
     if (c->nexch > 0)
         hipLaunchKernelGGL(k_exchange, dim3((unsigned)((c->nexch + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
                            c->d_exch, c->nexch, c->d_per);
@@ -1776,18 +1777,26 @@ static int launch_sweeps_kba(fv3d_ctx *c)
         const bool visc = c->cfg.viscous;
         const bool sa = sa_transport(c);   // the nu~ equation inside the sweep
         const int nb = (int)c->blocks.size();
-        const unsigned pg = (unsigned)c->pack_groups;
+        // rows packed per cell, increments per cell, and the split of each
+        // (tile, chunk) item over workgroups: about 8192 workgroups per launch,
+        // so that a coarse level's few tiles still spread over the chip
+        const int nq = sa ? 21 : visc ? 18 : 15, ncomp = sa ? 6 : 5;
+        const long per = c->pack_groups > 0 ? std::max(1L, (8192 + c->pack_groups - 1) / c->pack_groups) : 1;
+        const int qs = (int)std::min<long>(nq, per), cs = (int)std::min<long>(ncomp, per);
+        const unsigned pgq = (unsigned)(c->pack_groups * qs), pgc = (unsigned)(c->pack_groups * cs);
         if (f32) {
             // FP32 words: fill every 32-bit word of the shared allocation
             const long nw = 2 * c->mbox_words;
             hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, c->stream, (int *)c->d_mbox, nw,
                                (int)kMboxEmpty32);
-            hipLaunchKernelGGL(k_sweep_pack<float>, dim3(pg), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb);
+            hipLaunchKernelGGL(k_sweep_pack<float>, dim3(pgq), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb, nq,
+                               qs);
         } else {
             const long nw = c->mbox_words;
             hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, c->stream,
                                (unsigned long long *)c->d_mbox, nw, kMboxEmpty);
-            hipLaunchKernelGGL(k_sweep_pack<double>, dim3(pg), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb);
+            hipLaunchKernelGGL(k_sweep_pack<double>, dim3(pgq), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb, nq,
+                               qs);
         }
         // forward and backward sweeps in the operator's precision, with or
         // without the viscous terms
@@ -1816,11 +1825,11 @@ static int launch_sweeps_kba(fv3d_ctx *c)
         };
         auto unpack = [&]() {
             if (f32)
-                hipLaunchKernelGGL(k_sweep_unpack<float>, dim3(pg), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb,
-                                   sa ? 6 : 5);
+                hipLaunchKernelGGL(k_sweep_unpack<float>, dim3(pgc), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb,
+                                   ncomp, cs);
             else
-                hipLaunchKernelGGL(k_sweep_unpack<double>, dim3(pg), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb,
-                                   sa ? 6 : 5);
+                hipLaunchKernelGGL(k_sweep_unpack<double>, dim3(pgc), dim3(256), 0, c->stream, c->d_blocks, c->d_pack_start, nb,
+                                   ncomp, cs);
         };
         if (c->timing) HIPCK(c, hipEventRecord(c->ev[6], c->stream));
         sweep(true);

@@ -58,9 +58,11 @@ struct ScalarsDev {
     double iter;
 };
 
-// sc: the scalars of each instance (outlet pressure)
+struct PeriodicDev;
+// sc: the scalars of each instance (outlet pressure); with nper > 0 thread 0
+// first does the work of k_periodic_flip
 __global__ void k_bc(const BlockDev *blocks, const PatchDev *patches, int npatch, const long *pstart, Gas g,
-                     const ScalarsDev *sc);
+                     const ScalarsDev *sc, PeriodicDev *per, int nper, int *pitch_sign);
 // radial-equilibrium outlet, one workgroup per patch (list: patch indices);
 // the radial lines of a face are at most kAxialMaxI cells long
 constexpr int kAxialMaxI = 2048;
@@ -129,12 +131,16 @@ __global__ void k_prim_all(const BlockDev *blocks, const long *cstart, int nbloc
 // row storage in T (double, or float for the FP32 operator); unpack widens dQ to double
 template <class T>
 __global__ void k_sweep_fill(const BlockDev *blocks, int block, long nrows, int what);
-// one launch over all blocks: workgroups gstart[n] .. gstart[n+1]-1 serve block n
+// one launch over all blocks: (tile, chunk) items gstart[n] .. gstart[n+1]-1
+// serve block n, each split over qsplit workgroups (a grid of
+// gstart[nblock] * qsplit) that take equal shares of the nq quantities (15;
+// 18 with lamv; 21 with the nu~ rows, when nu~ is carried in the sweep)
 template <class T>
-__global__ void k_sweep_pack(const BlockDev *blocks, const long *gstart, int nblock);
-// nc: components per cell (5, or 6 with the nu~ increment)
+__global__ void k_sweep_pack(const BlockDev *blocks, const long *gstart, int nblock, int nq, int qsplit);
+// nc: components per cell (5, or 6 with the nu~ increment), split over csplit
+// workgroups per item
 template <class T>
-__global__ void k_sweep_unpack(const BlockDev *blocks, const long *gstart, int nblock, int nc);
+__global__ void k_sweep_unpack(const BlockDev *blocks, const long *gstart, int nblock, int nc, int csplit);
 constexpr int kPackChunk = 16;   // i-planes per k_sweep_pack / k_sweep_unpack workgroup
 constexpr int kSecParts = 32;    // workgroups per section in k_sections
 __global__ void k_fill_int(int *p, long n, int v);

@@ -23,10 +23,25 @@ __device__ static void set_dummies(const BlockDev &B, int s, int a, int b, const
     }
 }
 
+// The pitch-angle sign of each periodic patch's block, updated in table order
+// (source/BndCondInteriorPeriodic.cpp:54-59); one thread, so patches that share
+// a block see each other's flips in order.
+__device__ static void periodic_flip(PeriodicDev *__restrict__ per, int nper, int *__restrict__ pitch_sign)
+{
+    for (int p = 0; p < nper; ++p) {
+        if (per[p].flip) pitch_sign[per[p].block] = -pitch_sign[per[p].block];
+        per[p].sign = pitch_sign[per[p].block];
+    }
+}
+
+// Thread 0 also updates the periodic signs (nper > 0) for the exchange that
+// follows: no ghost fill here reads them, so this saves a launch.
 __global__ void k_bc(const BlockDev *__restrict__ blocks, const PatchDev *__restrict__ patches, int npatch,
-                     const long *__restrict__ pstart, Gas g, const ScalarsDev *__restrict__ sc)
+                     const long *__restrict__ pstart, Gas g, const ScalarsDev *__restrict__ sc,
+                     PeriodicDev *__restrict__ per, int nper, int *__restrict__ pitch_sign)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0 && nper > 0) periodic_flip(per, nper, pitch_sign);
     if (t >= pstart[npatch]) return;
     const int q = find_range(pstart, npatch, t);
     const PatchDev P = patches[q];
@@ -262,16 +277,10 @@ __global__ void __launch_bounds__(256) k_dt_global(const BlockDev *__restrict__ 
 }
 
 // ================================================== interior / periodic
-// The pitch-angle sign of each periodic patch's block, updated in table order
-// (source/BndCondInteriorPeriodic.cpp:54-59); one thread, so patches that share
-// a block see each other's flips in order.
 __global__ void k_periodic_flip(PeriodicDev *__restrict__ per, int nper, int *__restrict__ pitch_sign)
 {
     if (threadIdx.x != 0) return;
-    for (int p = 0; p < nper; ++p) {
-        if (per[p].flip) pitch_sign[per[p].block] = -pitch_sign[per[p].block];
-        per[p].sign = pitch_sign[per[p].block];
-    }
+    periodic_flip(per, nper, pitch_sign);
 }
 
 // One transfer of source/ShadowOutIn.cpp: Trans rotates the momentum and the
@@ -622,6 +631,10 @@ struct Dir {
 constexpr int kStaticAhead = FV3D_STATIC_AHEAD;
 constexpr int kRing = kStaticAhead + 1;
 constexpr int kRowS0 = -kStaticAhead;   // first (idle) step of the row-layout sweep loops
+// 1/x for the FP32 operator: the hardware reciprocal (within 1 ulp), one
+// instruction where a correctly rounded division takes about ten; the FP32
+// operator only preconditions the FP64 residual, whose fixed point it keeps
+__device__ inline float rcp_op(float x) { return __builtin_amdgcn_rcpf(x); }
 // whether a mailbox word still holds the empty pattern
 __device__ inline bool mbox_empty(double x) { return __builtin_bit_cast(unsigned long long, x) == kMboxEmpty; }
 __device__ inline bool mbox_empty(float x) { return __builtin_bit_cast(unsigned int, x) == kMboxEmpty32; }
@@ -697,21 +710,24 @@ __device__ inline void pack_item(const BlockDev &B, long wg, int nchunk, int &sl
 }
 }  // namespace
 
-// One workgroup per (tile, chunk of kPackChunk i-planes).  The chunk's cells are
-// read plane by plane along k (coalesced) into LDS, then written out row by
-// row: the rows a chunk touches are partly written (the lanes whose cell lies
-// in the chunk), each as one contiguous run.
+// One workgroup per (tile, chunk of kPackChunk i-planes, group of quantities):
+// the quantities are split into qsplit groups so that a small grid still
+// spreads over the chip.  The chunk's cells are read plane by plane along k
+// (coalesced) into LDS, then written out row by row: the rows a chunk touches
+// are partly written (the lanes whose cell lies in the chunk), each as one
+// contiguous run.
 template <class T>
 __global__ void __launch_bounds__(256) k_sweep_pack(const BlockDev *__restrict__ blocks, const long *__restrict__ gstart,
-                                                    int nblock)
+                                                    int nblock, int nq, int qsplit)
 {
     constexpr int TJ = kKbaTJ, TK = kKbaTK;
     __shared__ double buf[kPackChunk][64];
-    // the block of this workgroup: gstart holds the running workgroup counts
+    // the block of this workgroup: gstart holds the running (tile, chunk) counts
     const long item = xcd_item(blockIdx.x, gridDim.x);
-    const int block = find_range(gstart, nblock, item);
+    const int qg = (int)(item % qsplit);
+    const int block = find_range(gstart, nblock, item / qsplit);
     const BlockDev &B = blocks[block];
-    const long wg = item - gstart[block];
+    const long wg = item / qsplit - gstart[block];
     const int nchunk = (B.ni + kPackChunk - 1) / kPackChunk;
     int slot, i0;
     pack_item(B, wg, nchunk, slot, i0);
@@ -720,9 +736,9 @@ __global__ void __launch_bounds__(256) k_sweep_pack(const BlockDev *__restrict__
     const int tj = lane / TK, tk = lane % TK, j = J * TJ + tj + 1, k = K * TK + tk + 1;
     const bool col = j <= B.nj && k <= B.nk;
     const long box = B.box;
-    const int NQ = B.ev ? 21 : B.lamv ? 18 : 15;   // lamv rows for Navier-Stokes, nu~ rows for Spalart-Allmaras
+    const int NQ = nq;   // 15; 18 with the lamv rows of Navier-Stokes; 21 with the nu~ rows of Spalart-Allmaras
     // the cell-box array and row quantity of each packed quantity
-    for (int qi = 0; qi < NQ; ++qi) {
+    for (int qi = qg * NQ / qsplit; qi < (qg + 1) * NQ / qsplit; ++qi) {
         const double *src;
         int rq;
         if (qi < 5) { src = B.Q + qi * box; rq = kSQ_Q + qi; }
@@ -749,17 +765,20 @@ __global__ void __launch_bounds__(256) k_sweep_pack(const BlockDev *__restrict__
     }
 }
 
+// The reverse of k_sweep_pack for dQ (and dnu~), components split into
+// csplit groups the same way.
 template <class T>
 __global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict__ blocks, const long *__restrict__ gstart,
-                                                      int nblock, int nc)
+                                                      int nblock, int nc, int csplit)
 {
     constexpr int TJ = kKbaTJ, TK = kKbaTK;
     __shared__ double buf[kPackChunk][64];
-    // the block of this workgroup: gstart holds the running workgroup counts
+    // the block of this workgroup: gstart holds the running (tile, chunk) counts
     const long item = xcd_item(blockIdx.x, gridDim.x);
-    const int block = find_range(gstart, nblock, item);
+    const int cg = (int)(item % csplit);
+    const int block = find_range(gstart, nblock, item / csplit);
     const BlockDev &B = blocks[block];
-    const long wg = item - gstart[block];
+    const long wg = item / csplit - gstart[block];
     const int nchunk = (B.ni + kPackChunk - 1) / kPackChunk;
     int slot, i0;
     pack_item(B, wg, nchunk, slot, i0);
@@ -769,7 +788,7 @@ __global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict
     const bool col = j <= B.nj && k <= B.nk;
     const long box = B.box;
     const int NC = nc;   // 6: dnu~ too (the nu~ equation in the sweep)
-    for (int c = 0; c < NC; ++c) {
+    for (int c = cg * NC / csplit; c < (cg + 1) * NC / csplit; ++c) {
         for (int rr = w; rr < kPackChunk + TJ + TK - 2; rr += 4) {
             const int r = i0 - 1 + rr, i = r + 1 - tj - tk;
             if (col && i >= i0 && i < i0 + kPackChunk && i <= B.ni)
@@ -787,10 +806,10 @@ __global__ void __launch_bounds__(256) k_sweep_unpack(const BlockDev *__restrict
 
 template __global__ void k_sweep_fill<double>(const BlockDev *, int, long, int);
 template __global__ void k_sweep_fill<float>(const BlockDev *, int, long, int);
-template __global__ void k_sweep_pack<double>(const BlockDev *, const long *, int);
-template __global__ void k_sweep_pack<float>(const BlockDev *, const long *, int);
-template __global__ void k_sweep_unpack<double>(const BlockDev *, const long *, int, int);
-template __global__ void k_sweep_unpack<float>(const BlockDev *, const long *, int, int);
+template __global__ void k_sweep_pack<double>(const BlockDev *, const long *, int, int, int);
+template __global__ void k_sweep_pack<float>(const BlockDev *, const long *, int, int, int);
+template __global__ void k_sweep_unpack<double>(const BlockDev *, const long *, int, int, int);
+template __global__ void k_sweep_unpack<float>(const BlockDev *, const long *, int, int, int);
 
 // The persistent wavefront form of the sweeps, on the tile-skewed layout.  The
 // (j,k) cross-section is cut into 4 x 16 tiles, one workgroup of four waves
@@ -950,16 +969,19 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                 const T *o = b.q;
                 const T nx = b.m[0], ny = b.m[1], nz = b.m[2], Vn_e = b.m[4];
                 const T press_old = b.p;
-                const T Vn_old = o[1] / o[0] * nx + o[2] / o[0] * ny + o[3] / o[0] * nz;
-                b.vn_old = Vn_old;
                 if constexpr (kDeltaForm) {
                     // old velocity and total enthalpy per volume for the
-                    // cancellation-free difference below
-                    b.of[0] = o[1] / o[0];
-                    b.of[1] = o[2] / o[0];
-                    b.of[2] = o[3] / o[0];
+                    // cancellation-free difference below (one reciprocal of
+                    // the density: this operator only preconditions)
+                    const T r0 = rcp_op(o[0]);
+                    b.of[0] = o[1] * r0;
+                    b.of[1] = o[2] * r0;
+                    b.of[2] = o[3] * r0;
                     b.of[3] = o[4] + press_old;
+                    b.vn_old = b.of[0] * nx + b.of[1] * ny + b.of[2] * nz;
                 } else {
+                    const T Vn_old = o[1] / o[0] * nx + o[2] / o[0] * ny + o[3] / o[0] * nz;
+                    b.vn_old = Vn_old;
                     b.of[0] = o[0] * (Vn_old - Vn_e);
                     b.of[1] = o[1] * (Vn_old - Vn_e) + press_old * nx;
                     b.of[2] = o[2] * (Vn_old - Vn_e) + press_old * ny;
@@ -993,9 +1015,9 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                     //   d(rho w) = d rho w' + rho dVn,  d(m w + p n) = dm w' + m dVn + dp n,
                     //   d((E+p) w + p Vn_e) = (dE + dp) w' + (E+p) dVn + dp Vn_e
                     const T *q = b.q;
-                    const T rho1 = q[0] + dqn[0];
-                    const T du = (dqn[1] - b.of[0] * dqn[0]) / rho1, dv = (dqn[2] - b.of[1] * dqn[0]) / rho1,
-                            dw = (dqn[3] - b.of[2] * dqn[0]) / rho1;
+                    const T r1 = rcp_op(q[0] + dqn[0]);
+                    const T du = (dqn[1] - b.of[0] * dqn[0]) * r1, dv = (dqn[2] - b.of[1] * dqn[0]) * r1,
+                            dw = (dqn[3] - b.of[2] * dqn[0]) * r1;
                     const T u1 = b.of[0] + du, v1 = b.of[1] + dv, w1 = b.of[2] + dw;
                     const T dVn = du * nx + dv * ny + dw * nz;
                     const T wn = (b.vn_old + dVn) - Vn_e;
@@ -1093,6 +1115,8 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                 bar();
                 if (valid) {
                     const T alpha = sR[5][lane];
+                    T ra = T(0);
+                    if constexpr (kDeltaForm) ra = rcp_op(alpha);
                     for (int c = 0; c < 5; ++c) {
                         T v;
                         if (FWD) {
@@ -1103,7 +1127,8 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                             v = v - sA[1][c][lane]; v = v - sA[2][c][lane];
                         }
                         v = v + sV[0][c][lane]; v = v + sV[1][c][lane]; v = v + sV[2][c][lane];
-                        outv[c] = v / alpha;
+                        if constexpr (kDeltaForm) outv[c] = v * ra;
+                        else outv[c] = v / alpha;
                         sdq[s & 1][c][lane] = outv[c];
                     }
                     if constexpr (SA) {
@@ -1118,7 +1143,8 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                             v = v - sA[1][5][lane]; v = v - sA[2][5][lane];
                         }
                         v = v + sV[0][5][lane]; v = v + sV[1][5][lane]; v = v + sV[2][5][lane];
-                        outv[5] = v / alpha2;
+                        if constexpr (kDeltaForm) outv[5] = v * rcp_op(alpha2);
+                        else outv[5] = v / alpha2;
                         sdq[s & 1][5][lane] = outv[5];
                     }
                 }
