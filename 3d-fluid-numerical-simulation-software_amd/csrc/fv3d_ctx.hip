@@ -109,6 +109,10 @@ struct fv3d_ctx {
     // tile_sel[0] - kNumTiles of it (face capture always runs the per-tile kernel)
     int4 *d_stiles[kNumStream] = {};
     int nstiles[kNumStream] = {};
+    // face-parallel convective form (configuration kConvFaces, built on first
+    // use): running face counts per (block, direction), and their total
+    long *d_fstart = nullptr;
+    long nfaces = 0;
     int2 *d_secs = nullptr;
     int npatch_dev = 0, nplanes = 0;
     long nbc = 0, ncell = 0, nplane_threads = 0;
@@ -373,6 +377,36 @@ static int ensure_stream_tiles(fv3d_ctx *c, int q)
     HIPCK(c, hipMemcpyAsync(c->d_stiles[q], tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice,
                             c->stream));
     c->nstiles[q] = (int)tiles.size();
+    return FV3D_OK;
+}
+
+// The face-parallel convective form: per block the face-flux array
+// (15 box-sized slabs, direction-major) and the face work list; the block
+// table on the device is uploaded again with the new pointers.  Allocates, so
+// it runs at set-up or from fv3d_set_tiles / fv3d_autotune, never inside a
+// captured iteration.
+static int ensure_face_form(fv3d_ctx *c)
+{
+    if (c->d_fstart) return FV3D_OK;
+    std::vector<long> fst{0};
+    for (HostBlock &b : c->blocks) {
+        BlockDev &B = b.dev;
+        int rc = dalloc(c, &B.fcv, 15 * (size_t)B.box);
+        if (rc) return rc;
+        b.allocs.push_back(B.fcv);
+        fst.push_back(fst.back() + (long)(B.ni + 1) * B.nj * B.nk);
+        fst.push_back(fst.back() + (long)B.ni * (B.nj + 1) * B.nk);
+        fst.push_back(fst.back() + (long)B.ni * B.nj * (B.nk + 1));
+    }
+    int rc = dalloc(c, &c->d_fstart, fst.size());
+    if (rc) return rc;
+    HIPCK(c, hipMemcpyAsync(c->d_fstart, fst.data(), fst.size() * sizeof(long), hipMemcpyHostToDevice, c->stream));
+    c->nfaces = fst.back();
+    if (c->d_blocks) {
+        std::vector<BlockDev> bd;
+        for (HostBlock &b : c->blocks) bd.push_back(b.dev);
+        HIPCK(c, hipMemcpyAsync(c->d_blocks, bd.data(), bd.size() * sizeof(BlockDev), hipMemcpyHostToDevice, c->stream));
+    }
     return FV3D_OK;
 }
 
@@ -800,6 +834,7 @@ int fv3d_ctx_destroy(fv3d_ctx *c)
         if (p) (void)hipFree(p);
     if (c->d_sa_hi) (void)hipFree(c->d_sa_hi);
     if (c->d_pack_start) (void)hipFree(c->d_pack_start);
+    if (c->d_fstart) (void)hipFree(c->d_fstart);
     for (int d = 0; d < 3; ++d) {
         if (c->vdbg.fv[d]) (void)hipFree(c->vdbg.fv[d]);
         if (c->vdbg.grad[d]) (void)hipFree(c->vdbg.grad[d]);
@@ -1066,6 +1101,11 @@ static int setup_coarse(fv3d_ctx *c)
         if ((rc = fv3d_mixing_plane_add(m, T.block[0], o1, T.block[1], o2))) return bail(rc);
     }
     if ((rc = fv3d_setup(m))) return bail(rc);
+    // a small coarse level spreads its convective residual one face per thread
+    if (m->ncell <= kFaceFormCells) {
+        if ((rc = ensure_face_form(m))) return bail(rc);
+        m->tile_sel[0] = kConvFaces;
+    }
     std::vector<int4> tl;
     for (int n = 0; n < (int)m->blocks.size(); ++n) {
         const BlockDev &B = m->blocks[n].dev;
@@ -1589,6 +1629,16 @@ static int launch_spectral_dt(fv3d_ctx *c, bool decrement = true)
 
 static int launch_convective(fv3d_ctx *c)
 {
+    if (!c->store_faces && c->tile_sel[0] == kConvFaces) {
+        if (!c->d_fstart) return fail(c, FV3D_ESTATE, "face-parallel convective form not built");
+        const int nb = (int)c->blocks.size();
+        hipLaunchKernelGGL(k_conv_faces, dim3((unsigned)((c->nfaces + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
+                           c->d_fstart, 3 * nb, c->gas);
+        hipLaunchKernelGGL(k_conv_sum, dim3((unsigned)((c->ncell + 255) / 256)), dim3(256), 0, c->stream, c->d_blocks,
+                           c->d_cstart, nb);
+        HIPCK(c, hipGetLastError());
+        return FV3D_OK;
+    }
     if (!c->store_faces && c->tile_sel[0] >= kNumTiles) {
         const int q = c->tile_sel[0] - kNumTiles;
         int rc = ensure_stream_tiles(c, q);
@@ -2440,7 +2490,8 @@ int fv3d_set_tiles(fv3d_ctx *c, const int *shape3)
         if (shape3[k] < 0 || shape3[k] >= (k == 0 ? kNumConv : kNumTiles))
             return fail(c, FV3D_EINVAL, "tile shape %d", shape3[k]);
         int rc = ensure_tiles(c, shape3[k] < kNumTiles ? shape3[k] : 0);
-        if (!rc && k == 0 && shape3[k] >= kNumTiles) rc = ensure_stream_tiles(c, shape3[k] - kNumTiles);
+        if (!rc && k == 0 && shape3[k] == kConvFaces) rc = ensure_face_form(c);
+        else if (!rc && k == 0 && shape3[k] >= kNumTiles) rc = ensure_stream_tiles(c, shape3[k] - kNumTiles);
         if (rc) return rc;
     }
     for (int k = 0; k < 3; ++k) c->tile_sel[k] = shape3[k];
@@ -2470,7 +2521,9 @@ int fv3d_autotune(fv3d_ctx *c, int reps, int *shape3)
         float best = 0.0f;
         int bestq = 0;
         for (int q = 0; q < (kern == 0 ? kNumConv : kNumTiles) && !rc; ++q) {
-            if (q >= kNumTiles) {
+            if (q == kConvFaces) {
+                if ((rc = ensure_face_form(c))) break;
+            } else if (q >= kNumTiles) {
                 if ((rc = ensure_stream_tiles(c, q - kNumTiles))) break;
             } else if ((rc = ensure_tiles(c, q))) break;
             c->tile_sel[kern] = q;

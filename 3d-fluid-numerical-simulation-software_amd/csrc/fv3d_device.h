@@ -78,6 +78,8 @@ struct BlockDev {
                               // averaged unit normal (3), averaged area, grid normal velocity
     double *R;                // 5*box residual
     double *dQ;               // 5*box LU-SGS increment
+    double *fcv;              // 15*box convective face fluxes (direction, component) of the face-parallel
+                              // form, face (d, i, j, k) at cell (i, j, k); null unless that form is built
     int *flag_low_dt;         // time-step damping counter (source/Limit_FlowVars.cpp:117)
     // Tile-skewed sweep layout.  The (j,k) plane is cut into tiles of
     // kKbaTJ x kKbaTK columns; tile (J,K) is slot J*ntk+K and owns nr rows.

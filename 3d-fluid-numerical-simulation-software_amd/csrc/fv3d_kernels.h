@@ -22,8 +22,18 @@ constexpr int kTileShape[kNumTiles][3] = {{4, 4, 16}, {2, 8, 16}, {4, 8, 8}, {8,
 // do not depend on the choice.
 constexpr int kNumStream = 5;
 constexpr int kStreamShape[kNumStream][3] = {{16, 16, 4}, {16, 16, 2}, {8, 32, 4}, {8, 32, 2}, {16, 16, 8}};
-constexpr int kNumConv = kNumTiles + kNumStream;
 #define FV3D_FOR_STREAM_TILES(X) X(0, 16, 16) X(1, 8, 32)
+// face-parallel convective residual (k_conv_faces, then k_conv_sum): one thread
+// per face, configuration kConvFaces after the streaming ones.  Coarse
+// multigrid levels of at most kFaceFormCells cells take it by default: their
+// tiles are too few to fill the chip, and a tile workgroup evaluates five or
+// six faces per thread in turn.
+constexpr int kConvFaces = kNumTiles + kNumStream;
+constexpr int kNumConv = kConvFaces + 1;   // convective configurations
+#ifndef FV3D_FACE_FORM_CELLS
+#define FV3D_FACE_FORM_CELLS 1000000
+#endif
+constexpr long kFaceFormCells = FV3D_FACE_FORM_CELLS;
 
 struct PatchDev {
     int kind;          // 0 inlet, 1 outlet, 2 wall
@@ -96,6 +106,9 @@ __global__ void k_dt_flag_decrement(const BlockDev *blocks, int nblock);
 template <int TI, int TJ, int TK>
 __global__ void k_convective(const BlockDev *blocks, const int4 *tiles, Gas g, int store_faces, double *fI, double *fJ,
                              double *fK);
+// fstart: running face counts per (block, direction), 3 * nblock entries + 1
+__global__ void k_conv_faces(const BlockDev *blocks, const long *fstart, int nlist, Gas g);
+__global__ void k_conv_sum(const BlockDev *blocks, const long *cstart, int nblock);
 template <bool FORWARD>
 __global__ void k_lusgs_plane(const BlockDev *blocks, const long *pstart, int nblock, int step, double relax, Gas g);
 // one tile of the persistent wavefront sweep; dep_*: ticket of the upstream

@@ -506,6 +506,51 @@ __global__ void __launch_bounds__(TI *TJ *TK) __attribute__((amdgpu_waves_per_eu
 FV3D_FOR_TILES(FV3D_CONV)
 #undef FV3D_CONV
 
+// Face-parallel form of the same residual: every face of every block once, one
+// thread per face (faces of a block and direction i-slowest, k fastest), into
+// B.fcv; k_conv_sum then forms each cell's residual from its six faces and the
+// rotation source in k_convective's order, so the two forms agree bit for bit.
+__global__ void __launch_bounds__(256) k_conv_faces(const BlockDev *__restrict__ blocks, const long *__restrict__ fstart,
+                                                    int nlist, Gas g)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= fstart[nlist]) return;
+    const int q = find_range(fstart, nlist, t);
+    const int d = q % 3;
+    const BlockDev &B = blocks[q / 3];
+    const long r = t - fstart[q];
+    const int nj = B.nj + (d == 1 ? 1 : 0), nk = B.nk + (d == 2 ? 1 : 0);
+    const int i = (int)(r / ((long)nj * nk)) + 1, j = (int)(r / nk % nj) + 1, k = (int)(r % nk) + 1;
+    double f[5];
+    face_flux(B, g, d, i, j, k, f);
+    const long x = cidx(B, i, j, k);
+    for (int c = 0; c < 5; ++c) B.fcv[(long)(d * 5 + c) * B.box + x] = f[c];
+}
+
+__global__ void __launch_bounds__(256) k_conv_sum(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
+                                                  int nblock)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int n, i, j, k;
+    if (!cell_item(blocks, cstart, nblock, t, n, i, j, k)) return;
+    const BlockDev &B = blocks[n];
+    const long box = B.box, x = cidx(B, i, j, k);
+    const double w_rot = B.w_rot;
+    const double vol = B.vol[x];
+    const double dens = B.W[x], uvel = B.W[box + x], vvel = B.W[2 * box + x];
+    double Qc[5] = {0.0, dens * (w_rot * vvel) * vol, F32C(-1.0) * dens * (w_rot * uvel) * vol, 0.0, 0.0};
+    for (int c = 0; c < 5; ++c) {
+        const double *FI = B.fcv + (long)c * box, *FJ = B.fcv + (long)(5 + c) * box, *FK = B.fcv + (long)(10 + c) * box;
+        double s = FI[x + B.si] - FI[x];
+        s = s + FJ[x + B.sj];
+        s = s - FJ[x];
+        s = s + FK[x + 1];
+        s = s - FK[x];
+        s = s - Qc[c];
+        B.R[c * box + x] = (s - 0.0) - 0.0;
+    }
+}
+
 // ================================================================ LU-SGS
 // Flux difference of CTimeDiscr::delt_Fn_up / delt_Fn_down
 // (source/LU_SGS_Marching.cpp:293-419) at neighbour cell nb whose faces along

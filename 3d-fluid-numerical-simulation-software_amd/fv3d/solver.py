@@ -160,15 +160,19 @@ class GpuSolver:
     # the convective residual's streaming configurations, numbered after
     # TILE_SHAPES (5 + n): column tile (TJ, TK) and i-chunk length
     STREAM_SHAPES = [(16, 16, 4), (16, 16, 2), (8, 32, 4), (8, 32, 2), (16, 16, 8)]
+    CONV_FACES = 10   # the convective residual one face per thread (the default on small coarse levels)
 
     @classmethod
     def conv_shape(cls, q: int):
-        """Name of convective configuration q: a tile shape or ("stream", TJ, TK, ci)."""
+        """Name of convective configuration q: a tile shape, ("stream", TJ, TK, ci) or ("faces",)."""
+        if q == cls.CONV_FACES:
+            return ("faces",)
         return cls.TILE_SHAPES[q] if q < len(cls.TILE_SHAPES) else ("stream",) + cls.STREAM_SHAPES[q - 5]
 
     def set_tiles(self, shapes):
         """Configuration numbers of the convective (TILE_SHAPES, then
-        STREAM_SHAPES from 5 on), viscous and nu~ (TILE_SHAPES) residual kernels."""
+        STREAM_SHAPES from 5 on, then CONV_FACES), viscous and nu~ (TILE_SHAPES)
+        residual kernels."""
         a = np.ascontiguousarray(shapes, dtype=np.int32)
         self._check(self.L.fv3d_set_tiles(self.h, a.ctypes.data), "set_tiles")
 

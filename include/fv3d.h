@@ -294,8 +294,10 @@ int fv3d_set_debug(fv3d_ctx *ctx, int flags);
  * (cells along i x j x k per workgroup): 0 4x4x16 (default), 1 2x8x16,
  * 2 4x8x8, 3 8x4x8, 4 8x8x4.  The convective kernel also takes 5-9, a
  * column of (j x k) 16x16 or 8x32 cells streamed along i in chunks of 4, 2,
- * 4, 2 and 8 i-planes (5 16x16/4, 6 16x16/2, 7 8x32/4, 8 8x32/2, 9 16x16/8).
- * Results do not depend on the shape.
+ * 4, 2 and 8 i-planes (5 16x16/4, 6 16x16/2, 7 8x32/4, 8 8x32/2, 9 16x16/8),
+ * and 10, one thread per face (face fluxes to memory, then one thread per
+ * cell sums them), which coarse multigrid levels of at most 1,000,000 cells
+ * take by default.  Results do not depend on the shape.
  * fv3d_set_tiles sets the three shape numbers; fv3d_autotune times every shape
  * of every kernel in use on the current state (reps launches each, after one
  * warm-up), keeps the fastest and writes the chosen numbers (-1 for a kernel

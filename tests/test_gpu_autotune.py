@@ -41,8 +41,10 @@ def test_autotune_keeps_the_bits():
     hr = ref.iterate(6)
     h1 = g.iterate(2, graph=True)
     chosen = g.autotune(2)
-    print(f"  chosen {[GpuSolver.TILE_SHAPES[s] for s in chosen]}")
-    assert all(0 <= s < len(GpuSolver.TILE_SHAPES) for s in chosen)
+    print(f"  chosen {[GpuSolver.conv_shape(chosen[0])] + [GpuSolver.TILE_SHAPES[s] for s in chosen[1:]]}")
+    # the convective kernel may take any of its configurations, the others a tile shape
+    assert 0 <= chosen[0] <= GpuSolver.CONV_FACES
+    assert all(0 <= s < len(GpuSolver.TILE_SHAPES) for s in chosen[1:])
     h2 = g.iterate(4, graph=True)
     np.testing.assert_array_equal(hr, np.concatenate([h1, h2]))
     for b in range(1, len(case.blocks) + 1):

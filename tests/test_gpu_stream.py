@@ -1,5 +1,6 @@
 """The streaming convective residual (k_conv_stream, convective configurations
-5-9) against the per-tile kernel (k_convective, configurations 0-4): both evaluate every
+5-9) and the face-parallel one (k_conv_faces + k_conv_sum, configuration 10)
+against the per-tile kernel (k_convective, configurations 0-4): all evaluate every
 face once with the same face code and sum each cell's six faces in the same
 order, so the residual must agree bit for bit, for every reconstruction
 variable, both flux schemes, rotating blocks with rotating walls, partial
@@ -73,5 +74,25 @@ def test_stream_configurations(q):
         g.call("convective")
     for blk in range(1, len(case.blocks) + 1):
         np.testing.assert_array_equal(a.get(blk, "res"), b.get(blk, "res"))
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("name", list(VARIANTS))
+def test_face_form_equals_tiled(name):
+    case = VARIANTS[name]
+    print(f"case {name}, face-parallel form", flush=True)
+    a = GpuSolver(case)
+    a.set_tiles((GpuSolver.CONV_FACES, 0, 0))
+    b = GpuSolver(case)
+    for g in (a, b):
+        g.call("bc_apply")
+        g.call("convective")
+    for blk in range(1, len(case.blocks) + 1):
+        np.testing.assert_array_equal(a.get(blk, "res"), b.get(blk, "res"), err_msg=f"{name} block {blk}")
+    ha, hb = a.iterate(3), b.iterate(3)
+    np.testing.assert_array_equal(ha, hb)
+    for blk in range(1, len(case.blocks) + 1):
+        np.testing.assert_array_equal(a.get(blk, "cv"), b.get(blk, "cv"))
     a.close()
     b.close()
