@@ -130,7 +130,18 @@ __device__ inline bool cell_item(const BlockDev *blocks, const long *cstart, int
                                  int &k)
 {
     if (t >= cstart[nblock]) return false;
-    n = find_range(cstart, nblock, t);
+    // The wave's lanes hold consecutive items: the block of the first active
+    // lane's item is searched once for the wave (a uniform search, scalar
+    // loads), and each lane steps on from it over the block ends below its
+    // own item (none for most waves).
+    const long t0 = ((long)__builtin_amdgcn_readfirstlane((int)(t >> 32)) << 32) |
+                    (unsigned)__builtin_amdgcn_readfirstlane((int)t);
+    if (t >= t0) {
+        n = find_range(cstart, nblock, t0);
+        while (n + 1 < nblock && cstart[n + 1] <= t) ++n;
+    } else {
+        n = find_range(cstart, nblock, t);
+    }
     const BlockDev &B = blocks[n];
     long r = t - cstart[n];
     if (r >= (long)B.ni * B.nj * B.nk) return false;
