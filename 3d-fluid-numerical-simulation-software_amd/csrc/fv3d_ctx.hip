@@ -2554,7 +2554,7 @@ static int mlp_alloc(fv3d_ctx *c)
     if ((rc = dalloc(c, &c->d_mlp_v, kMlpParams))) return rc;
     if ((rc = dalloc(c, &c->d_mlp_g, kMlpParams + 1))) return rc;
     if ((rc = dalloc(c, &c->d_mlp_bounds, 2 * kMlpIn + 1))) return rc;
-    if ((rc = dalloc(c, &c->d_mlp_tripped, 1))) return rc;
+    if ((rc = dalloc(c, &c->d_mlp_tripped, (size_t)kTripSlots * kTripStride))) return rc;
     return FV3D_OK;
 }
 
@@ -2801,11 +2801,14 @@ long fv3d_mlp_tripped(fv3d_ctx *c)
 {
     READY(c);
     if (!c->d_mlp_tripped) return 0;
-    unsigned long long v = 0, z = 0;
+    // the sum of the counter slots, which are then cleared
+    std::vector<unsigned long long> v((size_t)kTripSlots * kTripStride), z(v.size(), 0);
     HIPCK(c, hipStreamSynchronize(c->stream));
-    HIPCK(c, hipMemcpy(&v, c->d_mlp_tripped, sizeof v, hipMemcpyDeviceToHost));
-    HIPCK(c, hipMemcpy(c->d_mlp_tripped, &z, sizeof z, hipMemcpyHostToDevice));
-    return (long)v;
+    HIPCK(c, hipMemcpy(v.data(), c->d_mlp_tripped, v.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    HIPCK(c, hipMemcpy(c->d_mlp_tripped, z.data(), z.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+    unsigned long long sum = 0;
+    for (int q = 0; q < kTripSlots; ++q) sum += v[(size_t)q * kTripStride];
+    return (long)sum;
 }
 
 int fv3d_set_debug(fv3d_ctx *c, int flags)

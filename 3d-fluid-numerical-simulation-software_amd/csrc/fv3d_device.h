@@ -152,6 +152,33 @@ __device__ inline bool cell_item(const BlockDev *blocks, const long *cstart, int
     return true;
 }
 
+// f(B, n, i, j, k) for the cell of work item t, as cell_item finds it (nothing
+// for padding and items past the list).  When every active lane of the wave
+// lies in one block (most waves), the block index is wave-uniform and the
+// block table entry is read through scalar loads, all issued at once; a wave
+// across a block end takes the per-lane form.  f is compiled for both.
+template <class F>
+__device__ inline void for_cell_item(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart, int nblock,
+                                     long t, F &&f)
+{
+    const long t0 = ((long)__builtin_amdgcn_readfirstlane((int)(t >> 32)) << 32) |
+                    (unsigned)__builtin_amdgcn_readfirstlane((int)t);
+    const int n0 = find_range(cstart, nblock, t0);
+    const long s0 = cstart[n0], e0 = cstart[n0 + 1];
+    if (__builtin_amdgcn_ballot_w64(t < s0 || t >= e0) == 0) {
+        const BlockDev &B = blocks[n0];
+        long r = t - s0;
+        if (r >= (long)B.ni * B.nj * B.nk) return;   // padding of the instance
+        const int k = (int)(r % B.nk) + 1;
+        r /= B.nk;
+        const int j = (int)(r % B.nj) + 1, i = (int)(r / B.nj) + 1;
+        f(B, n0, i, j, k);
+    } else {
+        int n, i, j, k;
+        if (cell_item(blocks, cstart, nblock, t, n, i, j, k)) f(blocks[n], n, i, j, k);
+    }
+}
+
 // Dummy slab address of component c (0-4 cv, 5-10 pv) of layer l (1..2) on side s
 // at tangential indices (a, b): I sides (j,k), J sides (i,k), K sides (i,j).
 __device__ inline double *dummy_ptr(const BlockDev &B, int s, int l, int c, int a, int b)

@@ -320,49 +320,48 @@ __global__ void k_exchange(const BlockDev *__restrict__ blocks, const ExchDev *_
 // viscous spectral radii for Navier-Stokes (TimeStep.cpp:38-49).
 __global__ void k_spectral_dt(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart, int nblock, Gas g)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int n, i, j, k;
-    if (!cell_item(blocks, cstart, nblock, t, n, i, j, k)) return;
-    const BlockDev &B = blocks[n];
-    const long box = B.box, x = cidx(B, i, j, k);
-    const double Gamma = g.gamma, Rgas = g.rgas;
-    double uvelr = B.W[box + x] + B.w_rot * B.cc[box + x];
-    double vvelr = B.W[2 * box + x] - B.w_rot * B.cc[x];
-    double wvelr = B.W[3 * box + x];
-    double csoun = sqrt(Gamma * Rgas * B.W[5 * box + x]);
-    double lam[3], eq[3];
-    const long xn[3] = {x + B.si, x + B.sj, x + 1};
-    for (int d = 0; d < 3; ++d) {
-        const double *S = B.fS[d];
-        double Sx = (S[x] + S[xn[d]]) * F32C(0.5);
-        double Sy = (S[box + x] + S[box + xn[d]]) * F32C(0.5);
-        double Sz = (S[2 * box + x] + S[2 * box + xn[d]]) * F32C(0.5);
-        eq[d] = Sx * Sx + Sy * Sy + Sz * Sz;
-        double CS = csoun * sqrt(eq[d]);
-        lam[d] = fabs(uvelr * Sx + vvelr * Sy + wvelr * Sz) + CS;
-        B.lamc[d * box + x] = lam[d];
-    }
-    int flag = *B.flag_low_dt;
-    double dt_factor = flag == 0 ? F32C(1.0) : (double)(1.0f / (1.0f + (float)flag));
-    double lc = lam[0] + lam[1] + lam[2];
-    double lv = 0.0, lvd[3] = {0.0, 0.0, 0.0};
-    if (g.viscous) {
-        const double mue_lam = sutherland(g, B.W[5 * box + x]), mue_tur = B.mut ? B.mut[x] : 0.0, dens = B.W[x];
-        const double a = (double)(4.0f / 3.0f) / dens, b = Gamma / dens;
-        const double m = a > b ? a : b;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for_cell_item(blocks, cstart, nblock, t, [&](const BlockDev &B, int, int i, int j, int k) {
+        const long box = B.box, x = cidx(B, i, j, k);
+        const double Gamma = g.gamma, Rgas = g.rgas;
+        double uvelr = B.W[box + x] + B.w_rot * B.cc[box + x];
+        double vvelr = B.W[2 * box + x] - B.w_rot * B.cc[x];
+        double wvelr = B.W[3 * box + x];
+        double csoun = sqrt(Gamma * Rgas * B.W[5 * box + x]);
+        double lam[3], eq[3];
+        const long xn[3] = {x + B.si, x + B.sj, x + 1};
         for (int d = 0; d < 3; ++d) {
-            lvd[d] = m * (mue_lam / g.pr_lam + mue_tur / g.pr_tur) * eq[d] / B.vol[x];
-            B.lamv[d * box + x] = lvd[d];
+            const double *S = B.fS[d];
+            double Sx = (S[x] + S[xn[d]]) * F32C(0.5);
+            double Sy = (S[box + x] + S[box + xn[d]]) * F32C(0.5);
+            double Sz = (S[2 * box + x] + S[2 * box + xn[d]]) * F32C(0.5);
+            eq[d] = Sx * Sx + Sy * Sy + Sz * Sz;
+            double CS = csoun * sqrt(eq[d]);
+            lam[d] = fabs(uvelr * Sx + vvelr * Sy + wvelr * Sz) + CS;
+            B.lamc[d * box + x] = lam[d];
         }
-        lv = lvd[0] + lvd[1] + lvd[2];
-    }
-    double dt = dt_factor * (*B.cfl) * B.vol[x] / (lc + g.c_timestep * lv);
-    B.dt[x] = dt;
-    // diagonal of the LU-SGS operator (source/LU_SGS_Marching.cpp:76-77)
-    B.alpha[x] = B.vol[x] / dt + g.relax * (lam[0] + lam[1] + lam[2]) + F32C(2.0) * (lvd[0] + lvd[1] + lvd[2]);
-    // and of its nu~ equation (source/LU_SGS_Marching.cpp:81-82)
-    if (B.alpha2)
-        B.alpha2[x] = B.vol[x] / dt + g.relax * (lam[0] + lam[1] + lam[2]) + F32C(3.0) * (lvd[0] + lvd[1] + lvd[2]);
+        int flag = *B.flag_low_dt;
+        double dt_factor = flag == 0 ? F32C(1.0) : (double)(1.0f / (1.0f + (float)flag));
+        double lc = lam[0] + lam[1] + lam[2];
+        double lv = 0.0, lvd[3] = {0.0, 0.0, 0.0};
+        if (g.viscous) {
+            const double mue_lam = sutherland(g, B.W[5 * box + x]), mue_tur = B.mut ? B.mut[x] : 0.0, dens = B.W[x];
+            const double a = (double)(4.0f / 3.0f) / dens, b = Gamma / dens;
+            const double m = a > b ? a : b;
+            for (int d = 0; d < 3; ++d) {
+                lvd[d] = m * (mue_lam / g.pr_lam + mue_tur / g.pr_tur) * eq[d] / B.vol[x];
+                B.lamv[d * box + x] = lvd[d];
+            }
+            lv = lvd[0] + lvd[1] + lvd[2];
+        }
+        double dt = dt_factor * (*B.cfl) * B.vol[x] / (lc + g.c_timestep * lv);
+        B.dt[x] = dt;
+        // diagonal of the LU-SGS operator (source/LU_SGS_Marching.cpp:76-77)
+        B.alpha[x] = B.vol[x] / dt + g.relax * (lam[0] + lam[1] + lam[2]) + F32C(2.0) * (lvd[0] + lvd[1] + lvd[2]);
+        // and of its nu~ equation (source/LU_SGS_Marching.cpp:81-82)
+        if (B.alpha2)
+            B.alpha2[x] = B.vol[x] / dt + g.relax * (lam[0] + lam[1] + lam[2]) + F32C(3.0) * (lvd[0] + lvd[1] + lvd[2]);
+    });
 }
 
 // Decrement of the time-step damping counter at the end of CTimeDiscr::TimeStep
@@ -675,6 +674,16 @@ struct Dir {
 #endif
 constexpr int kStaticAhead = FV3D_STATIC_AHEAD;
 constexpr int kRing = kStaticAhead + 1;
+// The upstream tile's dQ for a step is read kCrossAhead steps before the step
+// (the word may still be empty then; the step re-reads it until it is not).
+// Reading it earlier than one step ahead finds it empty more often than it
+// hides latency: measured on the 2.54M-cell vane passage, two steps ahead
+// gained 3-7 % on the fine FP32 sweeps and lost as much on FP64 and coarse ones.
+#ifndef FV3D_CROSS_AHEAD
+#define FV3D_CROSS_AHEAD 1
+#endif
+constexpr int kCrossAhead = FV3D_CROSS_AHEAD;
+static_assert(kCrossAhead >= 1 && kCrossAhead <= kStaticAhead, "the cross read lands in a free row buffer");
 constexpr int kRowS0 = -kStaticAhead;   // first (idle) step of the row-layout sweep loops
 // 1/x for the FP32 operator: the hardware reciprocal (within 1 ulp), one
 // instruction where a correctly rounded division takes about ten; the FP32
@@ -1038,7 +1047,7 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
             // step s: the dQ-dependent part of the flux difference (critical
             // path), then, while the combine wave works, the loads and early
             // terms of the next steps
-            auto work = [&](RowPf<T> &b, RowPf<T> &b1, RowPf<T> &bn, int s) {
+            auto work = [&](RowPf<T> &b, RowPf<T> &b1, RowPf<T> &bc, RowPf<T> &bn, int s) {
                 int i;
                 const bool valid = cell_at(s, i);
                 const int p = d == 0 ? i : d == 1 ? j : k, lim = d == 0 ? ni : d == 1 ? nj : nk;
@@ -1122,14 +1131,14 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
                     }
                 }
                 bar();   // terms of step s are in LDS
-                issue_cross(b1, s + 1);
+                issue_cross(bc, s + kCrossAhead);
                 issue_static(bn, s + kStaticAhead);
                 precompute(b1);
                 bar();   // dQ of step s is in LDS
             };
             // b[u] serves step s0+u: static data issued kStaticAhead steps
             // ahead (into the buffer the previous step has finished with), the
-            // upstream dQ one step ahead.  The loop starts kStaticAhead idle
+            // upstream dQ kCrossAhead steps ahead.  The loop starts kStaticAhead idle
             // steps early (every lane invalid) so that no load precedes it and
             // every step runs the same code; nothing is in flight on entry.
             RowPf<T> b[kRing] = {};
@@ -1137,7 +1146,7 @@ __global__ void __launch_bounds__(256) k_lusgs_sl(const BlockDev *__restrict__ b
             for (int s0 = kRowS0; s0 < send; s0 += kRing) {
 #pragma unroll
                 for (int u = 0; u < kRing; ++u)
-                    work(b[u], b[(u + 1) % kRing], b[(u + kStaticAhead) % kRing], s0 + u);
+                    work(b[u], b[(u + 1) % kRing], b[(u + kCrossAhead) % kRing], b[(u + kStaticAhead) % kRing], s0 + u);
             }
         };
         if (wave == 0) {
@@ -1259,9 +1268,7 @@ __global__ void __launch_bounds__(256) k_update(const BlockDev *__restrict__ blo
     __shared__ double red[5][256];
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double rm[5] = {0, 0, 0, 0, 0};
-    int n, i, j, k;
-    if (cell_item(blocks, cstart, nblock, t, n, i, j, k)) {
-        const BlockDev &B = blocks[n];
+    for_cell_item(blocks, cstart, nblock, t, [&](const BlockDev &B, int n, int i, int j, int k) {
         const long box = B.box, x = cidx(B, i, j, k);
         double q[5];
         const double vol = B.vol[x];
@@ -1284,7 +1291,7 @@ __global__ void __launch_bounds__(256) k_update(const BlockDev *__restrict__ blo
             B.ev[x] = e;
             if (sa_hi && e > F32C(1000.0)) atomicOr(sa_hi + n, 1);
         }
-    }
+    });
     for (int c = 0; c < 5; ++c) red[c][threadIdx.x] = rm[c];
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {

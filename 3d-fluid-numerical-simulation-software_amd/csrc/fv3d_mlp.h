@@ -9,6 +9,10 @@ namespace fv3d {
 // 6 -> 64 -> 64 -> 64 -> 1; FP32 parameters in one array:
 // W0[64][6] b0[64] W1[64][64] b1[64] W2[64][64] b2[64] W3[64] b3
 constexpr int kMlpIn = 6, kMlpH = 64, kMlpXS = 8;   // kMlpXS: LDS row stride of the inputs
+// the fallback count of k_mlp_mut: kTripSlots counters kTripStride words
+// apart (one cache line each), workgroup w adding to slot w mod kTripSlots, so
+// that the atomics of a launch do not queue on one address; the reader sums them
+constexpr int kTripSlots = 64, kTripStride = 16;
 constexpr int kMlpW0 = 0, kMlpB0 = kMlpW0 + kMlpH * kMlpIn, kMlpW1 = kMlpB0 + kMlpH, kMlpB1 = kMlpW1 + kMlpH * kMlpH,
               kMlpW2 = kMlpB1 + kMlpH, kMlpB2 = kMlpW2 + kMlpH * kMlpH, kMlpW3 = kMlpB2 + kMlpH, kMlpB3 = kMlpW3 + kMlpH,
               kMlpParams = kMlpB3 + 1;

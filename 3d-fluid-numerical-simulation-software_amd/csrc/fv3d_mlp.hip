@@ -328,19 +328,52 @@ __global__ void __launch_bounds__(256) k_mlp_cell_features(const BlockDev *__res
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ncell) return;
-    int n, i, j, k;
     float f[kMlpIn] = {0, 0, 0, 0, 0, 0};
-    bool ok = cell_item(blocks, cstart, nblock, t, n, i, j, k);
-    if (ok) {
-        cell_features(blocks[n], g, i, j, k, f);
+    bool ok = false;
+    for_cell_item(blocks, cstart, nblock, t, [&](const BlockDev &B, int, int i, int j, int k) {
+        cell_features(B, g, i, j, k, f);
+        ok = true;
         for (int q = 0; q < kMlpIn; ++q) {
             const float lo = bounds[2 * q], hi = bounds[2 * q + 1], pad = 0.1f * (hi - lo);
             ok = ok && isfinite(f[q]) && f[q] >= lo - pad && f[q] <= hi + pad;
         }
-    }
+    });
     float4 *o = (float4 *)(xc + 8 * t);
     o[0] = make_float4(f[0], f[1], f[2], f[3]);
     o[1] = make_float4(f[4], f[5], ok ? 1.0f : 0.0f, 0.0f);
+}
+
+// mue_t of cell (i, j, k) of block B from the network output yhat, or, for a
+// cell outside the training range (ok false) or whose output is not finite or
+// above the largest trained value, the Spalart-Allmaras value of its frozen
+// nu~ (fv1 rho nu~ / Re, clipped as SA_Model clips it); true when the
+// fallback was taken.
+__device__ static bool mut_cell(const BlockDev &B, int i, int j, int k, const Gas &g, const float *__restrict__ bounds,
+                                float yhat, bool ok, double relax)
+{
+    const long box = B.box, x = cidx(B, i, j, k);
+    // both loads before any arithmetic, so a wave waits on memory once for them
+    const double temp = B.W[5 * box + x], mut_old = B.mut[x];
+    const double mue_lam = sutherland(g, temp);
+    double mt;
+    bool trip = false;
+    if (ok && isfinite(yhat) && yhat <= bounds[2 * kMlpIn]) {
+        mt = mue_lam * expm1((double)yhat);
+    } else {
+        trip = true;
+        // Spalart-Allmaras fallback from the frozen nu~ (source/SA_Model.cpp:47-87)
+        const double rho = B.W[x], ev = B.ev[x] < 0.0 ? 0.0 : B.ev[x];
+        const double ev_lam = g.re_ref * mue_lam / rho, X = ev / ev_lam;
+        const double Cv1 = F32C(7.1);
+        const double fv1 = pow(X, F32C(3.0)) / (pow(X, F32C(3.0)) + Cv1 * Cv1 * Cv1);
+        mt = fv1 * rho * ev / g.re_ref;
+        if (mt < 0.0) mt = 0.0;
+    }
+    if (mue_lam > 0 && mt > 200.0 * mue_lam) mt = 200.0 * mue_lam;
+    // under-relaxation of the algebraic closure: the fixed point is unchanged
+    if (relax != 1.0) mt = mut_old + relax * (mt - mut_old);
+    B.mut[x] = mt;
+    return trip;
 }
 
 // Then the network: 256-cell chunks per workgroup in a grid-stride loop (the
@@ -383,46 +416,28 @@ __global__ void __launch_bounds__(256) k_mlp_closure(const float *__restrict__ P
     }
 }
 
-// Last, mue_t of every interior cell from the network output, or, for a cell
-// outside the training range or whose output is not finite or above the
-// largest trained value, the Spalart-Allmaras value of its frozen nu~ (fv1 rho
-// nu~ / Re, clipped as SA_Model clips it); tripped counts those cells.
+// mue_t of every interior cell from the network outputs in xc; tripped counts
+// the fallbacks.
 __global__ void __launch_bounds__(256) k_mlp_mut(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart,
                                                  int nblock, Gas g, const float *__restrict__ bounds,
                                                  const float *__restrict__ xc,
                                                  unsigned long long *__restrict__ tripped, double relax)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int n, i, j, k;
+    // the gate flag and network output, loaded unconditionally (an index past
+    // the list reads item 0, unused) before the block search, which they do
+    // not depend on, so the wave waits on memory once for them and the block
+    const float2 v = ((const float2 *)(xc + 8 * (c < cstart[nblock] ? c : 0)))[3];
     bool trip = false;
-    if (cell_item(blocks, cstart, nblock, c, n, i, j, k)) {
-        const BlockDev &B = blocks[n];
-        const long box = B.box, x = cidx(B, i, j, k);
-        const float yhat = xc[8 * c + 7];
-        const bool ok = xc[8 * c + 6] != 0.0f;
-        const double mue_lam = sutherland(g, B.W[5 * box + x]);
-        double mt;
-        if (ok && isfinite(yhat) && yhat <= bounds[2 * kMlpIn]) {
-            mt = mue_lam * expm1((double)yhat);
-        } else {
-            trip = true;
-            // Spalart-Allmaras fallback from the frozen nu~ (source/SA_Model.cpp:47-87)
-            const double rho = B.W[x], ev = B.ev[x] < 0.0 ? 0.0 : B.ev[x];
-            const double ev_lam = g.re_ref * mue_lam / rho, X = ev / ev_lam;
-            const double Cv1 = F32C(7.1);
-            const double fv1 = pow(X, F32C(3.0)) / (pow(X, F32C(3.0)) + Cv1 * Cv1 * Cv1);
-            mt = fv1 * rho * ev / g.re_ref;
-            if (mt < 0.0) mt = 0.0;
-        }
-        if (mue_lam > 0 && mt > 200.0 * mue_lam) mt = 200.0 * mue_lam;
-        // under-relaxation of the algebraic closure: the fixed point is unchanged
-        if (relax != 1.0) mt = B.mut[x] + relax * (mt - B.mut[x]);
-        B.mut[x] = mt;
-    }
+    for_cell_item(blocks, cstart, nblock, c, [&](const BlockDev &B, int, int i, int j, int k) {
+        trip = mut_cell(B, i, j, k, g, bounds, v.y, v.x != 0.0f, relax);
+    });
     // one atomic per wave for the fallback count (every lane reaches this
-    // point: the grid is whole 256-thread workgroups and no lane returns)
+    // point: the grid is whole 256-thread workgroups and no lane returns),
+    // into the workgroup's slot
     const unsigned long long m = __ballot(trip);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(tripped, (unsigned long long)__popcll(m));
+    if ((threadIdx.x & 63) == 0 && m)
+        atomicAdd(tripped + (blockIdx.x % kTripSlots) * kTripStride, (unsigned long long)__popcll(m));
 }
 
 // ------------------------------------------------------------ training

@@ -249,24 +249,23 @@ FV3D_FOR_TILES(FV3D_VISC)
 // CFluidProps::Curl (source/Spectral_curl_grad.cpp:44-83), one thread per cell.
 __global__ void k_curl(const BlockDev *__restrict__ blocks, const long *__restrict__ cstart, int nblock)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int n, i, j, k;
-    if (!cell_item(blocks, cstart, nblock, t, n, i, j, k)) return;
-    const BlockDev &B = blocks[n];
-    const long box = B.box, x = cidx(B, i, j, k);
-    const long st[3] = {B.si, B.sj, 1};
-    double d[3][3];   // d/dkesi, d/dyita, d/dzita of u v w
-    for (int a = 0; a < 3; ++a)
-        for (int q = 0; q < 3; ++q) d[a][q] = F32C(0.5) * (B.W[(1 + q) * box + x + st[a]] - B.W[(1 + q) * box + x - st[a]]);
-    double gm[9];
-    for (int m = 0; m < 9; ++m) gm[m] = B.cg[m * box + x];
-    const double v_x = d[0][1] * gm[0] + d[1][1] * gm[3] + d[2][1] * gm[6];
-    const double w_x = d[0][2] * gm[0] + d[1][2] * gm[3] + d[2][2] * gm[6];
-    const double u_y = d[0][0] * gm[1] + d[1][0] * gm[4] + d[2][0] * gm[7];
-    const double w_y = d[0][2] * gm[1] + d[1][2] * gm[4] + d[2][2] * gm[7];
-    const double u_z = d[0][0] * gm[2] + d[1][0] * gm[5] + d[2][0] * gm[8];
-    const double v_z = d[0][1] * gm[2] + d[1][1] * gm[5] + d[2][1] * gm[8];
-    B.omega[x] = sqrt(F32C(0.5) * ((w_y - v_z) * (w_y - v_z) + (u_z - w_x) * (u_z - w_x) + (v_x - u_y) * (v_x - u_y)));
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for_cell_item(blocks, cstart, nblock, t, [&](const BlockDev &B, int, int i, int j, int k) {
+        const long box = B.box, x = cidx(B, i, j, k);
+        const long st[3] = {B.si, B.sj, 1};
+        double d[3][3];   // d/dkesi, d/dyita, d/dzita of u v w
+        for (int a = 0; a < 3; ++a)
+            for (int q = 0; q < 3; ++q) d[a][q] = F32C(0.5) * (B.W[(1 + q) * box + x + st[a]] - B.W[(1 + q) * box + x - st[a]]);
+        double gm[9];
+        for (int m = 0; m < 9; ++m) gm[m] = B.cg[m * box + x];
+        const double v_x = d[0][1] * gm[0] + d[1][1] * gm[3] + d[2][1] * gm[6];
+        const double w_x = d[0][2] * gm[0] + d[1][2] * gm[3] + d[2][2] * gm[6];
+        const double u_y = d[0][0] * gm[1] + d[1][0] * gm[4] + d[2][0] * gm[7];
+        const double w_y = d[0][2] * gm[1] + d[1][2] * gm[4] + d[2][2] * gm[7];
+        const double u_z = d[0][0] * gm[2] + d[1][0] * gm[5] + d[2][0] * gm[8];
+        const double v_z = d[0][1] * gm[2] + d[1][1] * gm[5] + d[2][1] * gm[8];
+        B.omega[x] = sqrt(F32C(0.5) * ((w_y - v_z) * (w_y - v_z) + (u_z - w_x) * (u_z - w_x) + (v_x - u_y) * (v_x - u_y)));
+    });
 }
 
 // ------------------------------------------------------------- Baldwin-Lomax
