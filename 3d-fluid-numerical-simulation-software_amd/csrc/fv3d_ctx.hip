@@ -2522,6 +2522,9 @@ int fv3d_autotune(fv3d_ctx *c, int reps, int *shape3)
         int bestq = 0;
         for (int q = 0; q < (kern == 0 ? kNumConv : kNumTiles) && !rc; ++q) {
             if (q == kConvFaces) {
+                // the face-parallel form is for small grids (as on the coarse
+                // levels); on a large one it writes and reads back every face
+                if (c->ncell > kFaceFormCells) continue;
                 if ((rc = ensure_face_form(c))) break;
             } else if (q >= kNumTiles) {
                 if ((rc = ensure_stream_tiles(c, q - kNumTiles))) break;

@@ -12,8 +12,10 @@ uniform field on the coarse level, stationary, outlet at 0.95 Pt.
   cycle (the reference's maximum), Spalart-Allmaras transport;
 * chosen: FP32 defect correction (FP32 LU-SGS operator over the FP64
   residual, update and norms), the four-level V-cycle, and all blocks of
-  every stage in one launch captured in a HIP graph with autotuned residual
-  tiles, and the learned eddy-viscosity closure in place of the nu~ transport
+  every stage in one launch captured in a HIP graph with the residual tiles
+  the autotuner picks on this workload (fixed, so that the counter profile
+  applies; ``--autotune`` picks them per run), and the learned eddy-viscosity
+  closure in place of the nu~ transport
   (the network of fv3d/closure_vane.npz written by ``--workload
   closure-train``, its mue_t under-relaxed by 0.2, the Spalart-Allmaras value
   of the frozen nu~ as the gated fallback).  ``--no-closure`` runs the chosen
@@ -112,6 +114,9 @@ def parse():
                     help="c2: design rotation speed (the synthetic passage is stationary by design)")
     ap.add_argument("--compare-alone", action="store_true", help="c2: also time every passage alone")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default residual tile shapes")
+    ap.add_argument("--autotune", action="store_true",
+                    help="vane passage: pick the residual tile shapes on the running state instead of the fixed "
+                         "configuration the counter profile measured")
     ap.add_argument("--tiles", default=None,
                     help="fixed configuration numbers C,V,S of the convective, viscous and nu~ residual kernels "
                          "in place of the autotuner (GpuSolver.TILE_SHAPES / STREAM_SHAPES numbering)")
@@ -323,14 +328,17 @@ def time_form(case, precision, a, closure=None):
             "class_ms": {cl: float(ms[i]) for i, cl in enumerate(SLOTS)}, "tiles": tiles}
 
 
-def time_to_drop(case, precision, target, max_iter, closure=None, chunk=25):
+def time_to_drop(case, precision, target, max_iter, closure=None, chunk=25, tiles=None):
     """Wall time and iterations until Res.dens falls to ``target`` times the
     first fine-level value (iteration coarse_iter + 1 with multigrid), with
     graph replay; the host reads the history once per chunk.  Returns the
-    history row of the iteration that reached it."""
+    history row of the iteration that reached it.  tiles: "C,V,S" residual
+    configurations (None: the defaults)."""
     import numpy as np
     skip = case.cfg.coarse_iter if case.cfg.use_multigrid.upper() == "Y" else 0
     g = solver(case, precision, closure)
+    if tiles:
+        g.set_tiles([int(x) for x in tiles.split(",")])
     h = g.iterate(skip + 1, graph=True)   # captures the graphs; the last residual is the reference level
     r0 = float(h[-1, 1])
     g.sync()
@@ -451,13 +459,21 @@ def c4_main(a):
     passage_main(a, plain, chosen, wname, "c4")
 
 
+# Residual configurations of the vane passage: what the autotuner picks on it
+# (convective 8x4x8, viscous 4x4x16, nu~ 4x8x8), fixed so that every run times
+# the kernels the counter profile in profiles/lusgs_vcycle_vane_c3.md measured
+FIXED_TILES = {"c3": "3,0,2"}
+
+
 def passage_main(a, plain, chosen, wname, wkey):
+    if a.tiles is None and not a.autotune and wkey in FIXED_TILES:
+        a.tiles = FIXED_TILES[wkey]
     closure = None if a.no_closure else load_closure(a.closure_relax)
     cells = plain.n_cells
     if a.plain_only:
         p = time_form(plain, "fp64", a)
         print(f"bench: plain FP64 timed, {p['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
-        dp = None if a.no_drop else time_to_drop(plain, "fp64", 1e-6, a.drop_max_iter)
+        dp = None if a.no_drop else time_to_drop(plain, "fp64", 1e-6, a.drop_max_iter, tiles=a.tiles)
         if dp:
             print(f"bench: plain FP64 drop {dp['iterations']} iterations, {dp['seconds']} s", file=sys.stderr, flush=True)
         print(json.dumps({"metric": METRIC, "value": p["value"], "unit": "cell-updates/s", "n_gpus": 1,
@@ -477,16 +493,16 @@ def passage_main(a, plain, chosen, wname, wkey):
         print(f"bench: plain FP64 timed, {p['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     drop = None
     if not a.no_drop:
-        dc = time_to_drop(chosen, "fp32", 1e-6, a.drop_max_iter, closure)
+        dc = time_to_drop(chosen, "fp32", 1e-6, a.drop_max_iter, closure, tiles=a.tiles)
         print(f"bench: chosen form drop {dc['iterations']} iterations, {dc['seconds']} s", file=sys.stderr, flush=True)
         ds = None
         if closure is not None and not a.no_chosen_sa:
-            ds = time_to_drop(chosen, "fp32", 1e-6, a.drop_max_iter)
+            ds = time_to_drop(chosen, "fp32", 1e-6, a.drop_max_iter, tiles=a.tiles)
             print(f"bench: chosen form with Spalart-Allmaras drop {ds['iterations']} iterations, {ds['seconds']} s",
                   file=sys.stderr, flush=True)
         dp = None
         if not a.no_plain:
-            dp = time_to_drop(plain, "fp64", 1e-6, a.drop_max_iter)
+            dp = time_to_drop(plain, "fp64", 1e-6, a.drop_max_iter, tiles=a.tiles)
             print(f"bench: plain FP64 drop {dp['iterations']} iterations, {dp['seconds']} s", file=sys.stderr, flush=True)
         drop = {"chosen": dc, "chosen_spalart_allmaras": ds, "plain_fp64": dp,
                 "workload": "the headline workload, from the uniform start"}
@@ -503,9 +519,13 @@ def passage_main(a, plain, chosen, wname, wkey):
     conv = None if (a.no_converged or a.no_drop) else converged_field_error()
     cpu = None if a.no_cpu_baseline else cpu_baseline(a.cpu_iters, wkey)
     cpu_all = None if a.no_cpu_baseline else cpu_all_cores(a.cpu_iters, wkey)
+    tile_desc = ("autotuned tiles" if a.tiles is None else
+                 f"tiles {a.tiles} (what the autotuner picks on this workload)" if wkey in FIXED_TILES and
+                 a.tiles == FIXED_TILES[wkey] else f"tiles {a.tiles}")
     chosen_desc = (f"FP32 defect correction + {a.levels}-level V-cycle + one launch per stage over all blocks in a "
-                   "HIP graph, autotuned tiles" + (" + learned closure (Spalart-Allmaras fallback)" if closure else
-                                                  " + Spalart-Allmaras transport (no trained closure present)"))
+                   f"HIP graph, {tile_desc}, face-parallel convective residual on levels of at most 1M cells" +
+                   (" + learned closure (Spalart-Allmaras fallback)" if closure else
+                    " + Spalart-Allmaras transport (no trained closure present)"))
     out = {
         "metric": METRIC,
         "value": r["value"],

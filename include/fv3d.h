@@ -297,7 +297,8 @@ int fv3d_set_debug(fv3d_ctx *ctx, int flags);
  * 4, 2 and 8 i-planes (5 16x16/4, 6 16x16/2, 7 8x32/4, 8 8x32/2, 9 16x16/8),
  * and 10, one thread per face (face fluxes to memory, then one thread per
  * cell sums them), which coarse multigrid levels of at most 1,000,000 cells
- * take by default.  Results do not depend on the shape.
+ * take by default (fv3d_autotune tries it on such grids only).  Results do
+ * not depend on the shape.
  * fv3d_set_tiles sets the three shape numbers; fv3d_autotune times every shape
  * of every kernel in use on the current state (reps launches each, after one
  * warm-up), keeps the fastest and writes the chosen numbers (-1 for a kernel
