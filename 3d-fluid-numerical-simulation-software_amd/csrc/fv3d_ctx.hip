@@ -1627,6 +1627,21 @@ static int launch_spectral_dt(fv3d_ctx *c, bool decrement = true)
     return FV3D_OK;
 }
 
+// The TimeStep of a residual evaluation that only a restriction reads, in the
+// FP32 defect-correction form: lambda, dt and the LU-SGS diagonals it would form
+// are formed again by the level's next evaluation before anything reads them
+// (the restriction takes Q, R and the volumes), so only the Flag_low_dt
+// countdown at its end runs (source/TimeStep.cpp:89-92).  The plain FP64 path
+// forms them, as the reference's ResFlux does, so that they can be read after
+// an iteration.
+static int launch_restriction_timestep(fv3d_ctx *c)
+{
+    if (c->cfg.precision != FV3D_DEFECT_FP32) return launch_spectral_dt(c);
+    hipLaunchKernelGGL(k_dt_flag_decrement, dim3(1), dim3(256), 0, c->stream, c->d_blocks, (int)c->blocks.size());
+    HIPCK(c, hipGetLastError());
+    return FV3D_OK;
+}
+
 static int launch_convective(fv3d_ctx *c)
 {
     if (!c->store_faces && c->tile_sel[0] == kConvFaces) {
@@ -2119,12 +2134,14 @@ enum { kMgCoarse = 0, kMgSwitch = 1, kMgCycle = 2 };
 
 // The residual of one level (CMesh::ResFlux, source/ComputeAllBlockVars.cpp:
 // 223-255) into B.R, with lambda, dt and the LU-SGS diagonals of that state.
-static int level_resflux(fv3d_ctx *c)
+// restriction_only: the evaluation whose residual only the restriction to the
+// level below reads (launch_restriction_timestep)
+static int level_resflux(fv3d_ctx *c, bool restriction_only = false)
 {
     int rc;
     if ((rc = launch_bc(c))) return rc;
     if ((rc = launch_turbulence(c))) return rc;
-    if ((rc = launch_spectral_dt(c))) return rc;
+    if ((rc = restriction_only ? launch_restriction_timestep(c) : launch_spectral_dt(c))) return rc;
     if ((rc = launch_convective(c))) return rc;
     return launch_viscous(c);
 }
@@ -2150,7 +2167,7 @@ static int fas_level(fv3d_ctx *f, fv3d_ctx *m)
     if ((rc = launch_sweeps(m))) return rc;
     if ((rc = launch_update(m))) return rc;
     if (m->coarse) {
-        if ((rc = level_resflux(m))) return rc;
+        if ((rc = level_resflux(m, true))) return rc;
         hipLaunchKernelGGL(k_fas<false>, g, b, 0, f->stream, m->d_blocks, m->d_cstart, nb);
         HIPCK(f, hipGetLastError());
         if ((rc = fas_level(m, m->coarse))) return rc;
@@ -2183,7 +2200,7 @@ static int mg_iteration(fv3d_ctx *c, int phase, double *row)
     // levels carry no turbulence model, so the nu~ residual is not needed)
     if ((rc = launch_bc(c))) return rc;
     if ((rc = launch_turbulence(c, 0, true))) return rc;
-    if ((rc = launch_spectral_dt(c))) return rc;
+    if ((rc = launch_restriction_timestep(c))) return rc;
     if ((rc = launch_convective(c))) return rc;
     if ((rc = launch_viscous(c))) return rc;
     if (m->coarse) {
