@@ -36,7 +36,10 @@ enum fv3d_precision {
     FV3D_PLAIN_FP64 = 0,   /* every kernel in FP64 */
     FV3D_DEFECT_FP32 = 1   /* FP64 residual/update/norms, FP32 LU-SGS operator (both
                               sweeps, default sweep form); converges to the FP64
-                              fixed point */
+                              fixed point.  The cycle's residual evaluations that
+                              only a restriction reads skip lambda, dt and the
+                              diagonals, so after an iteration those fields are the
+                              iteration's first evaluation's */
 };
 
 /* Raw config.dat values, as parsed by CUserInput::ReadConfig
